@@ -1,0 +1,9 @@
+"""MI355X-native CM1 dynamical-core large step (gfx950 HIP kernels, C ABI).
+
+Operator modules mirror the reference's GM1 entry points; every one runs in
+``libgm1hip.so`` and raises when that library is missing.
+"""
+
+from . import advection5  # noqa: F401  (registers C-ABI signatures)
+
+__all__ = ["advection5"]
