@@ -118,7 +118,7 @@ GM1_DEV float recon5(float s1, float s2, float s3, float s4, float s5,
 
 // CM1's lateral family word: wbc | ebc<<4 | sbc<<8 | nbc<<12 with each side
 // 1 periodic, 2 open-radiative, 3/4 rigid.  Zero is a tile-local launch.
-GM1_DEV unsigned side_of(unsigned obc, unsigned side) {
+__host__ __device__ inline unsigned side_of(unsigned obc, unsigned side) {
   return (obc >> (4u * side)) & 15u;
 }
 

@@ -129,7 +129,41 @@ def advection5():
     _save("advection5", **out)
 
 
-GENERATORS = {"advection5": advection5}
+def momentum5():
+    """cpu_terrain_open_momentum_advection5_inplace for three lateral families."""
+    _ref()
+    from cm1core.cuda_momentum_advection5 import cpu_terrain_open_momentum_advection5_inplace
+
+    out = {}
+    cases = (((10, 9, 8, 21), (2, 2, 2, 2)), ((8, 11, 9, 22), (1, 1, 1, 1)), ((9, 8, 10, 23), (1, 1, 2, 3)))
+    for case, ((ni, nj, nk, seed), sides) in enumerate(cases):
+        rng = np.random.default_rng(seed)
+        sh = _shapes(ni, nj, nk)
+        m = seeded_metrics(rng, ni, nj, nk)
+        a = {
+            "u3d": (3.0 * rng.standard_normal(sh["u"])).astype(F),
+            "v3d": (3.0 * rng.standard_normal(sh["v"])).astype(F),
+            "w3d": (1.0 * rng.standard_normal(sh["w"])).astype(F),
+            "uten": (0.01 * rng.standard_normal(sh["u"])).astype(F),
+            "vten": (0.01 * rng.standard_normal(sh["v"])).astype(F),
+            "wten": (0.01 * rng.standard_normal(sh["w"])).astype(F),
+            "rru": np.zeros(sh["u"], F), "rrv": np.zeros(sh["v"], F), "rrw": np.zeros(sh["w"], F),
+            "uh": np.ones(ni + 2 * NG, F), "vh": np.ones(nj + 2 * NG, F),
+        }
+        divx = np.zeros((nk, nj, ni), F)
+        out.update({f"c{case}_in_{k}": v.copy() for k, v in a.items()})
+        out.update({f"c{case}_metric_{k}": v for k, v in m.items() if k in (
+            "rho0_3d", "rf0", "rr0", "rrf0", "c1", "c2", "gz", "gzu", "gzv", "rgzu", "rgzv", "dzdx", "dzdy")})
+        cpu_terrain_open_momentum_advection5_inplace(
+            a, m, divx, ni=ni, nj=nj, nk=nk, nrk=2, dx=250.0, dy=200.0, dz=150.0, lateral_sides=sides)
+        out.update({f"c{case}_out_{k}": a[k].copy() for k in ("rru", "rrv", "rrw", "uten", "vten", "wten")})
+        out[f"c{case}_out_divx"] = divx
+        out[f"c{case}_dims"] = np.array([ni, nj, nk], np.int64)
+        out[f"c{case}_sides"] = np.array(sides, np.int64)
+    _save("momentum5", **out)
+
+
+GENERATORS = {"advection5": advection5, "momentum5": momentum5}
 
 
 if __name__ == "__main__":
