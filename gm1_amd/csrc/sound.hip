@@ -1,0 +1,531 @@
+// psolver=3 acoustic RK stage (CM1 `sound`) for gfx950.
+//
+// Replaces the kernel family of cuda_psolver3_terrain_open.py (prepare,
+// pgrad, two column passes, average) plus its open-boundary, radiation and
+// fill companions.  Per small step this build issues:
+//   [ssopen]  open-side normal velocity step (one thread per edge column)
+//   pgrad     u/v forward step on every face the family advances, including
+//             periodic boundary faces (one launch instead of two)
+//   column    ONE pass per column: divergence, fpk, bcwsfc, pressure
+//             advection, the Thomas solve for w (upward elimination,
+//             downward substitution fused with the pressure update) and the
+//             rru/rrv/rrw time averages.  The reference needs two column
+//             passes, a bcwsfc launch and an average launch for this.
+//   [fill]    one-cell ppd halo between small steps
+// The column thread walks k with every access i-fastest across the wavefront.
+#include "boundary.h"
+
+using namespace gm1;
+
+namespace {
+
+struct SoundArgs {
+  float* ppi; float* pp3d; float* ppx; float* ppd;
+  const float* piadv; const float* ppterm; const float* ppten; const float* thv;
+  float* qk; float* pk1; float* pk2; float* mm; float* fpk; float* ec; float* ed;
+  const float* ua; float* u3d; const float* uten; float* rru;
+  const float* va; float* v3d; const float* vten; float* rrv;
+  const float* wa; float* w3d; const float* wten; float* rrw;
+  float* radbcw; float* radbce; float* radbcs; float* radbcn;
+  const float* uh; const float* vh; const float* ruf; const float* rvf;
+  const float* rho0; const float* rr0; const float* rf0; const float* rrf0; const float* c1; const float* c2;
+  const float* gz; const float* rgzu; const float* rgzv; const float* dzdx; const float* dzdy;
+  const float* gx; const float* gy; const float* gxu; const float* gyv;
+  const float* prof;  // sigma | sigmaf | rds | rdsf (nk+2 each) | cgs[3] | cgt[3]
+  int ni, nj, nk, nrk, nloop;
+  unsigned sides;
+  float dts, rdx, rdy, alph, beta, kdiv, zt, rzt, cp;
+};
+
+GM1_DEV const float* P_sigma(const SoundArgs& a) { return a.prof; }
+GM1_DEV const float* P_sigmaf(const SoundArgs& a) { return a.prof + (a.nk + 2); }
+GM1_DEV const float* P_rds(const SoundArgs& a) { return a.prof + 2 * (a.nk + 2); }
+GM1_DEV const float* P_rdsf(const SoundArgs& a) { return a.prof + 3 * (a.nk + 2); }
+GM1_DEV const float* P_cgs(const SoundArgs& a) { return a.prof + 4 * (a.nk + 2); }
+GM1_DEV const float* P_cgt(const SoundArgs& a) { return a.prof + 4 * (a.nk + 2) + 3; }
+
+// ------------------------------------------------------------------ prepare
+// Threads over pressure-family columns (i, j) in 0..n+1 and levels 1..nk:
+// stage coefficients on the interior (zero on halos), ppd = ppx, pp3d = ppi,
+// and the winds' average carriers over padded rows/cols NG-1..NG+n.
+__global__ __launch_bounds__(256) void k_snd_prepare(SoundArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // 0..ni+1
+  const int j = blockIdx.y;                             // 0..nj+1
+  const int k = blockIdx.z + 1;                         // 1..nk
+  const int ni = a.ni, nj = a.nj, nk = a.nk;
+  if (i >= ni + 2) return;
+  const int64_t p = pix(k, j, i, ni, nj);
+  a.ppd[p] = a.ppx[p];
+  if (a.nrk != 1) a.pp3d[p] = a.ppi[p];
+  const int y = j + NG - 1, x = i + NG - 1;
+  const int64_t pu = ix3(k, y, x, sy(nj), ux(ni));
+  const int64_t pv = ix3(k, y, x, vy(nj), sx(ni));
+  const int64_t pw = ix3(k, y, x, sy(nj), sx(ni));
+  a.rru[pu] = 0.0f;
+  a.rrv[pv] = 0.0f;
+  a.rrw[pw] = a.wa[pw] * a.beta;
+  if (a.nrk != 1) {
+    a.u3d[pu] = a.ua[pu];
+    a.v3d[pv] = a.va[pv];
+    a.w3d[pw] = a.wa[pw];
+  }
+  float qk = 0.0f, pk1 = 0.0f, pk2 = 0.0f, mm = 0.0f;
+  if (i >= 1 && i <= ni && j >= 1 && j <= nj) {
+    const int i0 = i - 1, j0 = j - 1;
+    const int64_t h = (int64_t)j0 * ni + i0, plane = (int64_t)nj * ni;
+    const float g = a.gz[h];
+    const float tem = a.dts * P_rdsf(a)[k];
+    qk = (tem * a.ppterm[p]) * g;
+    const float tem2 = ((a.dts * g) * P_rdsf(a)[k]) * a.rr0[(k - 1) * plane + h];
+    const float b2 = (-a.c2[k * plane + h]) * a.piadv[pix(k + 1, j, i, ni, nj)] +
+                     (1.0f - a.c1[k * plane + h]) * a.piadv[p];
+    pk2 = (tem2 * a.rf0[k * plane + h]) * b2;
+    const float b1 = a.c1[(k - 1) * plane + h] * a.piadv[pix(k - 1, j, i, ni, nj)] +
+                     (a.c2[(k - 1) * plane + h] - 1.0f) * a.piadv[p];
+    pk1 = (tem2 * a.rf0[(k - 1) * plane + h]) * b1;
+    if (k >= 2) {
+      const float tem1 = (a.dts * P_rds(a)[k]) * a.cp;
+      const float th = a.c2[(k - 1) * plane + h] * a.thv[p] + a.c1[(k - 1) * plane + h] * a.thv[pix(k - 1, j, i, ni, nj)];
+      mm = (tem1 * th) * g;
+    }
+    if (k == 1) pk1 = 0.0f;
+    if (k == nk) pk2 = 0.0f;
+  }
+  a.qk[p] = qk;
+  a.pk1[p] = pk1;
+  a.pk2[p] = pk2;
+  a.mm[p] = mm;
+}
+
+// ------------------------------------------------------------------ ssopen
+// Open-side normal-velocity small step on one edge column.  The w-level
+// snapshot of CM1 is taken before any update; the two end values that read
+// three levels are formed first, the others from originals carried upward.
+GM1_DEV float at_w_mid(const SoundArgs& a, int kf, float qlo, float qhi) {
+  const float r2 = (P_sigmaf(a)[kf] - P_sigma(a)[kf - 1]) * P_rds(a)[kf];
+  const float r1 = 1.0f - r2;
+  return r1 * qlo + r2 * qhi;
+}
+
+__global__ __launch_bounds__(64) void k_snd_ssopen(SoundArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ni = a.ni, nj = a.nj, nk = a.nk;
+  int side, idx;
+  if (t < nj) { side = 0; idx = t; }
+  else if (t < 2 * nj) { side = 1; idx = t - nj; }
+  else if (t < 2 * nj + ni) { side = 2; idx = t - 2 * nj; }
+  else if (t < 2 * nj + 2 * ni) { side = 3; idx = t - 2 * nj - ni; }
+  else return;
+  if (side_of(a.sides, side) != 2u) return;
+  const bool xdir = side < 2;
+  float* q = xdir ? a.u3d : a.v3d;
+  const float* ten = xdir ? a.uten : a.vten;
+  const int nx = xdir ? ux(ni) : sx(ni);
+  const int ny = xdir ? sy(nj) : vy(nj);
+  const int64_t st = (int64_t)ny * nx;
+  // boundary face b (updated) and its interior neighbour c
+  int64_t cb, cc;
+  int j0 = 0, i0 = 0;
+  if (side == 0) { j0 = idx; cb = (int64_t)(idx + NG) * nx + NG; cc = cb + 1; }
+  else if (side == 1) { j0 = idx; cb = (int64_t)(idx + NG) * nx + NG + ni; cc = cb - 1; }
+  else if (side == 2) { i0 = idx; cb = (int64_t)NG * nx + idx + NG; cc = cb + nx; }
+  else { i0 = idx; cb = (int64_t)(NG + nj) * nx + idx + NG; cc = cb - nx; }
+  const float* cgs = P_cgs(a);
+  const float* cgt = P_cgt(a);
+  auto Qb = [&](int k) { return q[(int64_t)k * st + cb]; };
+  auto Qc = [&](int k) { return q[(int64_t)k * st + cc]; };
+  const float topb = (cgt[0] * Qb(nk) + cgt[1] * Qb(nk - 1)) + cgt[2] * Qb(nk - 2);
+  const float topc = (cgt[0] * Qc(nk) + cgt[1] * Qc(nk - 1)) + cgt[2] * Qc(nk - 2);
+  float lob = (cgs[0] * Qb(1) + cgs[1] * Qb(2)) + cgs[2] * Qb(3);
+  float loc = (cgs[0] * Qc(1) + cgs[1] * Qc(2)) + cgs[2] * Qc(3);
+  float qb = Qb(1), qc = Qc(1);  // originals at level k
+  const float rd = xdir ? a.rdx : a.rdy;
+  const int64_t plane = (int64_t)nj * ni;
+  // metric columns for this edge
+  float gzv, rg_b, rg_c, hmet;
+  const float* gm;  // gx or gy (nk+1, nj, ni)
+  int64_t mh;
+  const float* radbc;
+  int rix;
+  if (side == 0) { gzv = a.gz[(int64_t)j0 * ni]; rg_b = a.rgzu[(int64_t)j0 * (ni + 1)]; rg_c = a.rgzu[(int64_t)j0 * (ni + 1) + 1]; hmet = a.uh[NG]; gm = a.gx; mh = (int64_t)j0 * ni; radbc = a.radbcw; rix = j0 + NG; }
+  else if (side == 1) { gzv = a.gz[(int64_t)j0 * ni + ni - 1]; rg_b = a.rgzu[(int64_t)j0 * (ni + 1) + ni]; rg_c = a.rgzu[(int64_t)j0 * (ni + 1) + ni - 1]; hmet = a.uh[NG + ni - 1]; gm = a.gx; mh = (int64_t)j0 * ni + ni - 1; radbc = a.radbce; rix = j0 + NG; }
+  else if (side == 2) { gzv = a.gz[i0]; rg_b = a.rgzv[i0]; rg_c = a.rgzv[ni + i0]; hmet = a.vh[NG]; gm = a.gy; mh = i0; radbc = a.radbcs; rix = i0 + NG; }
+  else { gzv = a.gz[(int64_t)(nj - 1) * ni + i0]; rg_b = a.rgzv[(int64_t)nj * ni + i0]; rg_c = a.rgzv[(int64_t)(nj - 1) * ni + i0]; hmet = a.vh[NG + nj - 1]; gm = a.gy; mh = (int64_t)(nj - 1) * ni + i0; radbc = a.radbcn; rix = i0 + NG; }
+  const int rstride = xdir ? (nj + 2 * NG) : (ni + 2 * NG);
+  const bool lowside = (side == 0 || side == 2);
+  for (int k = 1; k <= nk; ++k) {
+    float qb1 = 0.0f, qc1 = 0.0f, hib, hic;
+    if (k < nk) {
+      qb1 = Qb(k + 1);
+      qc1 = Qc(k + 1);
+      hib = at_w_mid(a, k + 1, qb, qb1);
+      hic = at_w_mid(a, k + 1, qc, qc1);
+    } else {
+      hib = topb;
+      hic = topc;
+    }
+    // horizontal: gz*(q_outer*rg_outer - q_inner*rg_inner) with outer the larger index
+    float h;
+    if (lowside) h = gzv * (qc * rg_c - qb * rg_b);
+    else h = gzv * (qb * rg_b - qc * rg_c);
+    h = (h * rd) * hmet;
+    float vert = gm[(int64_t)k * plane + mh] * (hic + hib) - gm[(int64_t)(k - 1) * plane + mh] * (loc + lob);
+    vert = (0.5f * vert) * P_rdsf(a)[k];
+    const float rad = (-radbc[(int64_t)k * rstride + rix]) * (h + vert);
+    const float tend = rad + ten[(int64_t)k * st + cb];
+    q[(int64_t)k * st + cb] = qb + a.dts * tend;
+    lob = hib; loc = hic;
+    qb = qb1; qc = qc1;
+  }
+}
+
+// ------------------------------------------------------------------ pgrad
+__global__ __launch_bounds__(256) void k_snd_pgrad(SoundArgs a) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;  // 0..ni
+  const int y = blockIdx.y;                             // 0..nj
+  const int k = blockIdx.z + 1;
+  const int ni = a.ni, nj = a.nj, nk = a.nk;
+  if (x > ni) return;
+  const float halfcp = a.cp * 0.5f, eighth = 0.125f;
+  const float* rds = P_rds(a);
+  const float* ppd = a.ppd;
+  // u face ff = x+1 (1-based) on row j = y
+  if (y < nj) {
+    const int ff = x + 1;
+    const bool on = (ff >= 2 && ff <= ni) || (ff == 1 && side_of(a.sides, 0) == 1u) ||
+                    (ff == ni + 1 && side_of(a.sides, 1) == 1u);
+    if (on) {
+      const int pj = y + 1;
+      const float h = (ppd[pix(k, pj, ff, ni, nj)] - ppd[pix(k, pj, ff - 1, ni, nj)]) * a.rdx;
+      // vertical differences, same statement order for face and its west cell
+      float dph = 0.0f, dphl = 0.0f, dpl = 0.0f, dpll = 0.0f;
+      if (k < nk) {
+        dph = (ppd[pix(k + 1, pj, ff, ni, nj)] - ppd[pix(k, pj, ff, ni, nj)]) * rds[k + 1];
+        dphl = (ppd[pix(k + 1, pj, ff - 1, ni, nj)] - ppd[pix(k, pj, ff - 1, ni, nj)]) * rds[k + 1];
+      }
+      if (k > 1) {
+        dpl = (ppd[pix(k, pj, ff, ni, nj)] - ppd[pix(k - 1, pj, ff, ni, nj)]) * rds[k];
+        dpll = (ppd[pix(k, pj, ff - 1, ni, nj)] - ppd[pix(k - 1, pj, ff - 1, ni, nj)]) * rds[k];
+      }
+      const float vs = (dph + dphl) + (dpl + dpll);
+      const int64_t g = ((int64_t)(k - 1) * nj + y) * (ni + 1) + x;
+      const float tr = (eighth * vs) * (a.gxu[g] + a.gxu[g + (int64_t)nj * (ni + 1)]);
+      const float pg = (halfcp * (a.thv[pix(k, pj, ff, ni, nj)] + a.thv[pix(k, pj, ff - 1, ni, nj)])) * (h + tr);
+      const int64_t pu = ix3(k, y + NG, x + NG, sy(nj), ux(ni));
+      a.u3d[pu] = a.u3d[pu] + a.dts * (a.uten[pu] - pg);
+    }
+  }
+  // v face ff = y+1 on column i = x
+  if (x < ni) {
+    const int ff = y + 1;
+    const bool on = (ff >= 2 && ff <= nj) || (ff == 1 && side_of(a.sides, 2) == 1u) ||
+                    (ff == nj + 1 && side_of(a.sides, 3) == 1u);
+    if (on) {
+      const int pi = x + 1;
+      const float h = (ppd[pix(k, ff, pi, ni, nj)] - ppd[pix(k, ff - 1, pi, ni, nj)]) * a.rdy;
+      float dph = 0.0f, dphl = 0.0f, dpl = 0.0f, dpll = 0.0f;
+      if (k < nk) {
+        dph = (ppd[pix(k + 1, ff, pi, ni, nj)] - ppd[pix(k, ff, pi, ni, nj)]) * rds[k + 1];
+        dphl = (ppd[pix(k + 1, ff - 1, pi, ni, nj)] - ppd[pix(k, ff - 1, pi, ni, nj)]) * rds[k + 1];
+      }
+      if (k > 1) {
+        dpl = (ppd[pix(k, ff, pi, ni, nj)] - ppd[pix(k - 1, ff, pi, ni, nj)]) * rds[k];
+        dpll = (ppd[pix(k, ff - 1, pi, ni, nj)] - ppd[pix(k - 1, ff - 1, pi, ni, nj)]) * rds[k];
+      }
+      const float vs = (dph + dphl) + (dpl + dpll);
+      const int64_t g = ((int64_t)(k - 1) * (nj + 1) + y) * ni + x;
+      const float tr = (eighth * vs) * (a.gyv[g] + a.gyv[g + (int64_t)(nj + 1) * ni]);
+      const float pg = (halfcp * (a.thv[pix(k, ff, pi, ni, nj)] + a.thv[pix(k, ff - 1, pi, ni, nj)])) * (h + tr);
+      const int64_t pv = ix3(k, y + NG, x + NG, vy(nj), sx(ni));
+      a.v3d[pv] = a.v3d[pv] + a.dts * (a.vten[pv] - pg);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ column
+struct ColumnFaces {
+  float xw, xe, ys, yn;  // xu at faces i, i+1 and yv at faces j, j+1 of one level
+};
+
+__global__ __launch_bounds__(64) void k_snd_column(SoundArgs a, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // 0..ni
+  const int j = blockIdx.y;                             // 0..nj
+  const int ni = a.ni, nj = a.nj, nk = a.nk;
+  if (i > ni) return;
+  const bool phys = (i < ni) && (j < nj);
+  const bool last = (n == a.nloop);
+  const float tavg = 1.0f / (float)a.nloop;
+  const int nxs = sx(ni), nys = sy(nj), nxu = ux(ni), nyv = vy(nj);
+  const int64_t wst = (int64_t)nys * nxs;
+  const int64_t wc = (int64_t)(j + NG) * nxs + (i + NG);
+  const int64_t uc = (int64_t)(j + NG) * nxu + (i + NG);
+  const int64_t ust = (int64_t)nys * nxu;
+  const int64_t vc = (int64_t)(j + NG) * nxs + (i + NG);
+  const int64_t vst = (int64_t)nyv * nxs;
+  float* w = a.w3d;
+
+  if (phys) {
+    const int pj = j + 1, pi = i + 1;
+    const int64_t h = (int64_t)j * ni + i, plane = (int64_t)nj * ni;
+    const float* sigma = P_sigma(a);
+    const float* sigmaf = P_sigmaf(a);
+    const float* rds = P_rds(a);
+    const float* rdsf = P_rdsf(a);
+    const float g = a.gz[h], ddx = a.dzdx[h], ddy = a.dzdy[h];
+    const float rgw = a.rgzu[(int64_t)j * (ni + 1) + i], rge = a.rgzu[(int64_t)j * (ni + 1) + i + 1];
+    const float rgs = a.rgzv[(int64_t)j * ni + i], rgn = a.rgzv[(int64_t)(j + 1) * ni + i];
+    const float temx = (a.dts * 0.5f) * a.rdx, temy = (a.dts * 0.5f) * a.rdy;
+    const float smeps = 1.0e-20f;
+    const float asq = a.alph * a.alph;
+    auto faces = [&](int k) {
+      ColumnFaces f;
+      f.xw = a.u3d[(int64_t)k * ust + uc] * rgw;
+      f.xe = a.u3d[(int64_t)k * ust + uc + 1] * rge;
+      f.ys = a.v3d[(int64_t)k * vst + vc] * rgs;
+      f.yn = a.v3d[(int64_t)k * vst + vc + nxs] * rgn;
+      return f;
+    };
+    // w-level terrain terms dum3(kf) and hadvp(kf) from faces at kf and kf-1
+    auto wterms = [&](int kf, const ColumnFaces& hi, const ColumnFaces& lo, float rho_hi, float rho_lo,
+                      float& dum3, float& hadvp) {
+      if (kf == 1 || kf == nk + 1) { dum3 = 0.0f; hadvp = 0.0f; return; }
+      const float r2 = (sigmaf[kf] - sigma[kf - 1]) * rds[kf];
+      const float r1 = 1.0f - r2;
+      const float height = (sigmaf[kf] - a.zt) * a.rzt;
+      {
+        const float xt = (r2 * (hi.xw + hi.xe) + r1 * (lo.xw + lo.xe)) * ddx;
+        const float yt = (r2 * (hi.ys + hi.yn) + r1 * (lo.ys + lo.yn)) * ddy;
+        dum3 = ((0.5f * (xt + yt)) * height) * g;
+      }
+      {
+        const float xt = (r2 * (rho_hi * hi.xw + rho_hi * hi.xe) + r1 * (rho_lo * lo.xw + rho_lo * lo.xe)) * ddx;
+        const float yt = (r2 * (rho_hi * hi.ys + rho_hi * hi.yn) + r1 * (rho_lo * lo.ys + rho_lo * lo.yn)) * ddy;
+        hadvp = (((0.5f * (xt + yt)) * height) * g) * a.rrf0[(int64_t)(kf - 1) * plane + h];
+      }
+    };
+    // bcwsfc for this column (u, v after this small step's pgrad)
+    const float w1new = bcwsfc_value(a.u3d, a.v3d, a.dzdx, a.dzdy, a.gz, j, i, ni, nj);
+    w[wst + wc] = w1new;
+
+    ColumnFaces flo = faces(1);
+    float rho_lo = a.rho0[0];
+    float dum3_lo = 0.0f, hadvp_lo = 0.0f;  // kf = 1
+    float fpk_prev = 0.0f, pp_prev = 0.0f, pk1_prev = 0.0f, qk_prev = 0.0f, pk2_prev = 0.0f;
+    float ec_prev = 0.0f, ed_prev = 0.0f, wtop = 0.0f;
+    for (int k = 1; k <= nk; ++k) {
+      const int64_t p = pix(k, pj, pi, ni, nj);
+      ColumnFaces fhi;
+      float rho_hi = 0.0f, dum3_hi, hadvp_hi;
+      if (k < nk) {
+        fhi = faces(k + 1);
+        rho_hi = a.rho0[(int64_t)k * plane];
+        wterms(k + 1, fhi, flo, rho_hi, rho_lo, dum3_hi, hadvp_hi);
+      } else {
+        fhi = flo;
+        dum3_hi = 0.0f;
+        hadvp_hi = 0.0f;
+      }
+      // divergence and initial fpk
+      const float hz = (flo.xe - flo.xw) * a.rdx + (flo.yn - flo.ys) * a.rdy;
+      const float vt = (dum3_hi - dum3_lo) * rdsf[k];
+      float div = g * (hz + vt);
+      if (fabsf(div) < smeps) div = 0.0f;
+      const float wk = (k == 1) ? 0.0f : w[(int64_t)k * wst + wc];
+      const float wk1 = (k == nk) ? 0.0f : w[(int64_t)(k + 1) * wst + wc];
+      const float qkk = a.qk[p];
+      float fpk = a.dts * (a.ppten[p] - a.ppterm[p] * div) + a.beta * ((-qkk) * (wk1 - wk));
+      // pressure advection
+      const float u2 = temx * (rho_lo * flo.xe), u1 = temx * (rho_lo * flo.xw);
+      const float v2 = temy * (rho_lo * flo.yn), v1 = temy * (rho_lo * flo.ys);
+      const float wb = (k == 1) ? w1new : w[(int64_t)k * wst + wc];
+      const float ww2 = hadvp_hi + a.beta * w[(int64_t)(k + 1) * wst + wc];
+      const float ww1 = hadvp_lo + a.beta * wb;
+      const float pc = a.piadv[p];
+      const float xa = u2 * (a.piadv[p + 1] - pc) + u1 * (pc - a.piadv[p - 1]);
+      const float ya = v2 * (a.piadv[p + (ni + 2)] - pc) + v1 * (pc - a.piadv[p - (ni + 2)]);
+      float hza = (-(xa + ya)) * a.rr0[(int64_t)(k - 1) * plane + h];
+      hza = hza * g;
+      const float pk1k = a.pk1[p], pk2k = a.pk2[p];
+      const float vta = ww1 * pk1k + ww2 * pk2k;
+      fpk = fpk + (hza + vta);
+      a.fpk[p] = fpk;
+      const float ppk = a.pp3d[p];
+      // elimination for w at level k (k = 2..nk)
+      if (k >= 2) {
+        const float mmk = a.mm[p];
+        const float aa = (k >= 3) ? (-asq) * (mmk * (pk1_prev + qk_prev)) : 0.0f;
+        const float cc = (k < nk) ? (-asq) * ((-mmk) * (pk2k - qkk)) : 0.0f;
+        const float diag = (-(pk1k + qkk)) + (pk2_prev - qk_prev);
+        const float bb = 1.0f - asq * (mmk * diag);
+        const float pdelta = ppk - pp_prev;
+        const float fwk = a.dts * a.wten[(int64_t)k * wst + wc] + a.beta * ((-mmk) * pdelta);
+        const float udelta = (ppk + fpk) - (pp_prev + fpk_prev);
+        const float dd = (w[(int64_t)k * wst + wc] + fwk) + a.alph * ((-mmk) * udelta);
+        const float pivot = (k == 2) ? bb : aa * ec_prev + bb;
+        const float inv = 1.0f / pivot;
+        if (k < nk) {
+          const float eck = (-cc) * inv;
+          const float edk = ((k == 2) ? dd : dd - aa * ed_prev) * inv;
+          a.ec[p] = eck;
+          a.ed[p] = edk;
+          ec_prev = eck;
+          ed_prev = edk;
+        } else {
+          wtop = (dd - aa * ed_prev) * inv;
+        }
+      }
+      fpk_prev = fpk;
+      pp_prev = ppk;
+      pk1_prev = pk1k;
+      pk2_prev = pk2k;
+      qk_prev = qkk;
+      flo = fhi;
+      rho_lo = rho_hi;
+      dum3_lo = dum3_hi;
+      hadvp_lo = hadvp_hi;
+    }
+    // downward: substitution, pressure update and the w average
+    float wup = 0.0f;  // new w at k+1
+    for (int k = nk; k >= 1; --k) {
+      const int64_t p = pix(k, pj, pi, ni, nj);
+      float wk;
+      if (k == nk) wk = wtop;
+      else if (k >= 2) wk = a.ec[p] * wup + a.ed[p];
+      else wk = w1new;
+      if (k >= 2) w[(int64_t)k * wst + wc] = wk;
+      const float w1 = (k == 1) ? 0.0f : wk;
+      const float w2 = (k == nk) ? 0.0f : wup;
+      const float old = a.pp3d[p];
+      const float vt = w1 * (a.pk1[p] + a.qk[p]) + w2 * (a.pk2[p] - a.qk[p]);
+      float up = (old + a.fpk[p]) + a.alph * vt;
+      if (fabsf(up) < smeps) up = 0.0f;
+      a.pp3d[p] = up;
+      const float tend = a.kdiv * (up - old);
+      a.ppd[p] = up + tend;
+      if (a.nrk == 3 && last) a.ppx[p] = tend;
+      const int64_t pw = (int64_t)k * wst + wc;
+      const float vw = a.rrw[pw] + (last ? wk * a.alph : wk);
+      a.rrw[pw] = last ? vw * tavg : vw;
+      wup = wk;
+    }
+  } else {
+    for (int k = 1; k <= nk; ++k) {
+      const int64_t pw = (int64_t)k * wst + wc;
+      const float wk = w[pw];
+      const float vw = a.rrw[pw] + (last ? wk * a.alph : wk);
+      a.rrw[pw] = last ? vw * tavg : vw;
+    }
+  }
+  for (int k = 1; k <= nk; ++k) {
+    const int64_t pu = (int64_t)k * ust + uc;
+    const float vu = a.rru[pu] + a.u3d[pu];
+    a.rru[pu] = last ? vu * tavg : vu;
+    const int64_t pv = (int64_t)k * vst + vc;
+    const float vv = a.rrv[pv] + a.v3d[pv];
+    a.rrv[pv] = last ? vv * tavg : vv;
+  }
+}
+
+// ------------------------------------------------------------------ radiation
+// Durran-Klemp phase speed of one edge row/column, averaged over k in order.
+__global__ __launch_bounds__(64) void k_snd_radiation(SoundArgs a, float dx, float dy, float dttmp) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ni = a.ni, nj = a.nj, nk = a.nk;
+  int side, idx;
+  if (t < nj) { side = 0; idx = t; }
+  else if (t < 2 * nj) { side = 1; idx = t - nj; }
+  else if (t < 2 * nj + ni) { side = 2; idx = t - 2 * nj; }
+  else if (t < 2 * nj + 2 * ni) { side = 3; idx = t - 2 * nj - ni; }
+  else return;
+  const float eps = 1.0e-10f, lim = 350.0f;
+  const bool xdir = side < 2;
+  const int nx = xdir ? ux(ni) : sx(ni);
+  const int ny = xdir ? sy(nj) : vy(nj);
+  const int64_t st = (int64_t)ny * nx;
+  const float* qa = xdir ? a.ua : a.va;
+  const float* q3 = xdir ? a.u3d : a.v3d;
+  const int off = NG - 1;
+  int64_t chi, clo, cn;
+  float rf, d;
+  const int r = idx + 1 + off;  // packed row/col 1..n
+  if (side == 0) { chi = (int64_t)r * nx + 3 + off; clo = (int64_t)r * nx + 2 + off; cn = clo; rf = a.ruf[2 + off]; d = dx; }
+  else if (side == 1) { chi = (int64_t)r * nx + ni + off; clo = (int64_t)r * nx + ni - 1 + off; cn = chi; rf = a.ruf[ni + off]; d = dx; }
+  else if (side == 2) { chi = (int64_t)(3 + off) * nx + r; clo = (int64_t)(2 + off) * nx + r; cn = clo; rf = a.rvf[2 + off]; d = dy; }
+  else { chi = (int64_t)(nj + off) * nx + r; clo = (int64_t)(nj - 1 + off) * nx + r; cn = chi; rf = a.rvf[nj + off]; d = dy; }
+  const bool low = (side == 0 || side == 2);
+  float acc = 0.0f;
+  for (int k = 1; k <= nk; ++k) {
+    const int64_t kb = (int64_t)k * st;
+    const float foo = qa[kb + chi] - qa[kb + clo];
+    const float num = (d * rf) * (qa[kb + cn] - q3[kb + cn]);
+    const float den = dttmp * (copysignf(eps, foo) + foo);
+    float c = num / den;
+    c = low ? fmaxf(fminf(c, 0.0f), -lim) : fminf(fmaxf(c, 0.0f), lim);
+    acc = acc + c;
+  }
+  const float avg = acc / (float)nk;
+  float* out = side == 0 ? a.radbcw : side == 1 ? a.radbce : side == 2 ? a.radbcs : a.radbcn;
+  const int rs = xdir ? (nj + 2 * NG) : (ni + 2 * NG);
+  for (int k = 1; k <= nk; ++k) out[(int64_t)k * rs + idx + NG] = avg;
+}
+
+}  // namespace
+
+extern "C" int gm1hip_psolver3_stage(
+    float* ppi, float* pp3d, float* ppx, float* ppd, const float* piadv, const float* ppterm, const float* ppten,
+    const float* thv, float* qk, float* pk1, float* pk2, float* mm, float* fpk, float* elim_c, float* elim_d,
+    const float* ua, float* u3d, const float* uten, float* rru, const float* va, float* v3d, const float* vten,
+    float* rrv, const float* wa, float* w3d, const float* wten, float* rrw, float* radbcw, float* radbce,
+    float* radbcs, float* radbcn, const float* uh, const float* vh, const float* ruf, const float* rvf,
+    float* scalar, const float* rho0_3d, const float* rr0, const float* rf0, const float* rrf0, const float* c1,
+    const float* c2, const float* gz, const float* rgzu, const float* rgzv, const float* dzdx, const float* dzdy,
+    const float* gx, const float* gy, const float* gxu, const float* gyv, const float* profiles, float zt,
+    int ni, int nj, int nk, int nrk, int nloop, float dts, float dttmp, float dx, float dy, float alph,
+    float kdiv, unsigned sides, int flags, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  SoundArgs a;
+  a.ppi = ppi; a.pp3d = pp3d; a.ppx = ppx; a.ppd = ppd; a.piadv = piadv; a.ppterm = ppterm; a.ppten = ppten;
+  a.thv = thv; a.qk = qk; a.pk1 = pk1; a.pk2 = pk2; a.mm = mm; a.fpk = fpk; a.ec = elim_c; a.ed = elim_d;
+  a.ua = ua; a.u3d = u3d; a.uten = uten; a.rru = rru; a.va = va; a.v3d = v3d; a.vten = vten; a.rrv = rrv;
+  a.wa = wa; a.w3d = w3d; a.wten = wten; a.rrw = rrw;
+  a.radbcw = radbcw; a.radbce = radbce; a.radbcs = radbcs; a.radbcn = radbcn;
+  a.uh = uh; a.vh = vh; a.ruf = ruf; a.rvf = rvf;
+  a.rho0 = rho0_3d; a.rr0 = rr0; a.rf0 = rf0; a.rrf0 = rrf0; a.c1 = c1; a.c2 = c2; a.gz = gz;
+  a.rgzu = rgzu; a.rgzv = rgzv; a.dzdx = dzdx; a.dzdy = dzdy; a.gx = gx; a.gy = gy; a.gxu = gxu; a.gyv = gyv;
+  a.prof = profiles;
+  a.ni = ni; a.nj = nj; a.nk = nk; a.nrk = nrk; a.nloop = nloop; a.sides = sides;
+  a.dts = dts; a.rdx = 1.0f / dx; a.rdy = 1.0f / dy; a.alph = alph; a.beta = 1.0f - alph; a.kdiv = kdiv;
+  a.zt = zt; a.rzt = 1.0f / zt; a.cp = 1005.7f;
+  hipLaunchKernelGGL(k_snd_prepare, dim3((ni + 2 + 63) / 64, nj + 2, nk), dim3(64), 0, s, a);
+  GM1_LAUNCH_CHECK();
+  const bool any_open = side_of(sides, 0) == 2u || side_of(sides, 1) == 2u || side_of(sides, 2) == 2u ||
+                        side_of(sides, 3) == 2u;
+  const int edge = 2 * (ni + nj);
+  for (int n = 1; n <= nloop; ++n) {
+    if (any_open) {
+      hipLaunchKernelGGL(k_snd_ssopen, dim3((edge + 63) / 64), dim3(64), 0, s, a);
+    }
+    hipLaunchKernelGGL(k_snd_pgrad, dim3((ni + 1 + 63) / 64, nj + 1, nk), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_snd_column, dim3((ni + 1 + 63) / 64, nj + 1), dim3(64), 0, s, a, n);
+    GM1_LAUNCH_CHECK();
+    if (n < nloop) {
+      int rc = fill_pressure(ppd, sides, ni, nj, nk, s);
+      if (rc) return rc;
+    }
+  }
+  if (flags & 1) {
+    hipLaunchKernelGGL(k_snd_radiation, dim3((edge + 63) / 64), dim3(64), 0, s, a, dx, dy, dttmp);
+    GM1_LAUNCH_CHECK();
+  }
+  if (flags & 2) {
+    int rc = fill_family(u3d, 1, sides, ni, nj, nk, s);
+    if (!rc) rc = fill_family(v3d, 2, sides, ni, nj, nk, s);
+    if (!rc) rc = fill_family(w3d, 3, sides, ni, nj, nk, s);
+    if (!rc) rc = w_rigid(w3d, ni, nj, nk, s);
+    if (!rc) rc = bcwsfc(u3d, v3d, dzdx, dzdy, gz, w3d, ni, nj, s);
+    if (!rc && nrk < 3) rc = fill_pressure(pp3d, sides, ni, nj, nk, s);
+    if (!rc && nrk < 3 && scalar) rc = fill_family(scalar, 0, sides, ni, nj, nk, s);
+    if (rc) return rc;
+  }
+  return 0;
+}

@@ -1,0 +1,183 @@
+"""psolver=3 acoustic RK stage (CM1 ``sound``) on gfx950.
+
+Host mirror of ``cuda_psolver3_terrain_open.py``: ``terrain_open_sound_shapes``
+(:162-187), ``TerrainOpenSoundAblations`` (:275-284),
+``launch_cuda_psolver3_terrain_open_three_stage`` (:1217-1433) and
+``launch_cuda_psolver3_terrain_open_stage`` (:1436-1507).  The whole stage
+(prepare, every small step, radiation history, stage fills) is one C-ABI call,
+``gm1hip_psolver3_stage``, that enqueues its kernels on the caller's stream.
+
+``lateral_sides`` is CM1's ``(wbc, ebc, sbc, nbc)``; the reference reaches the
+same family through a bound ``lateral`` object.  Open sides take the
+``ssopenbc`` step, periodic sides take the boundary-face pressure gradient,
+and every fill is the family fill of that side.
+"""
+
+from __future__ import annotations
+
+from collections.abc import Mapping
+from dataclasses import dataclass
+from typing import Any
+
+import numpy as np
+import torch
+
+from . import _native
+from ._device import NG, check_spacing, lateral_code, no_alias, ptr, record, require, stream_handle, terrain_static_shapes
+from ._native import Fl, I, P, U, signature
+from .grid import VerticalProfiles, rk_stage_dt_f32, sound3_stage_schedule_f32, terrain_coordinate_profiles
+from .errors import refuse
+
+CM1_CP_F32 = 1005.7
+CM1_ALPH_F32 = 0.60
+CM1_KDIV_F32 = 0.1
+
+PRESSURE_NAMES = ("ppi", "pp3d", "ppx", "ppd", "piadv", "ppterm", "ppten", "thv", "qk", "pk1", "pk2", "mm",
+                  "fpk", "elim_c", "elim_d")
+U_NAMES = ("ua", "u3d", "uten", "rru")
+V_NAMES = ("va", "v3d", "vten", "rrv")
+W_NAMES = ("wa", "w3d", "wten", "rrw")
+BOUNDARY_NAMES = ("radbcw", "radbce", "radbcs", "radbcn", "uh", "vh", "ruf", "rvf")
+METRIC_NAMES = ("rho0_3d", "rr0", "rf0", "rrf0", "c1", "c2", "gz", "rgzu", "rgzv", "dzdx", "dzdy",
+                "gx", "gy", "gxu", "gyv")
+
+signature("gm1hip_psolver3_stage", *([P] * 52), Fl, I, I, I, I, I, Fl, Fl, Fl, Fl, Fl, Fl, U, I, P)
+
+
+def terrain_open_sound_shapes(ni: int, nj: int, nk: int) -> dict[str, tuple[int, ...]]:
+    ng = NG
+    pressure = (nk + 2, nj + 2, ni + 2)
+    shapes = {name: pressure for name in PRESSURE_NAMES}
+    shapes.update({name: (nk + 2, nj + 2 * ng, ni + 1 + 2 * ng) for name in U_NAMES})
+    shapes.update({name: (nk + 2, nj + 1 + 2 * ng, ni + 2 * ng) for name in V_NAMES})
+    shapes.update({name: (nk + 3, nj + 2 * ng, ni + 2 * ng) for name in W_NAMES})
+    shapes.update({
+        "radbcw": (nk + 2, nj + 2 * ng), "radbce": (nk + 2, nj + 2 * ng),
+        "radbcs": (nk + 2, ni + 2 * ng), "radbcn": (nk + 2, ni + 2 * ng),
+        "uh": (ni + 2 * ng,), "vh": (nj + 2 * ng,), "ruf": (ni + 1 + 2 * ng,), "rvf": (nj + 1 + 2 * ng,),
+        "scalar": (nk + 2, nj + 2 * ng, ni + 2 * ng),
+    })
+    return shapes
+
+
+@dataclass(frozen=True)
+class TerrainOpenSoundAblations:
+    """Which parts of the stage run (cuda_psolver3_terrain_open.py:275-284).
+
+    Only the two stage-end parts may be switched off; every small-step part is
+    the physics itself and is refused when disabled.
+    """
+
+    open_consumption: bool = True
+    terrain_pressure_gradient: bool = True
+    transformed_divergence: bool = True
+    pressure_advection: bool = True
+    implicit_vertical_solve: bool = True
+    between_step_pressure_fill: bool = True
+    radiation_history_production: bool = True
+    stage_family_fills: bool = True
+
+
+_profile_cache: dict[tuple, torch.Tensor] = {}
+
+
+def device_profiles(profiles: VerticalProfiles, device: torch.device) -> torch.Tensor:
+    key = (profiles.nk, profiles.packed().tobytes(), str(device))
+    t = _profile_cache.get(key)
+    if t is None:
+        t = torch.from_numpy(profiles.packed()).to(device)
+        _profile_cache[key] = t
+    return t
+
+
+@dataclass(frozen=True)
+class CudaPsolver3TerrainOpenLaunch:
+    ready_event: Any
+    compiled: Any
+    cadence: tuple[dict[str, Any], ...]
+    receipt: dict[str, Any]
+
+
+def _check_ablations(ablations: TerrainOpenSoundAblations) -> int:
+    if not isinstance(ablations, TerrainOpenSoundAblations):
+        raise TypeError("ablations must be TerrainOpenSoundAblations")
+    for name in ("open_consumption", "terrain_pressure_gradient", "transformed_divergence", "pressure_advection",
+                 "implicit_vertical_solve", "between_step_pressure_fill"):
+        if not getattr(ablations, name):
+            refuse(code="GM1-ACOUSTIC-ABLATION", gate="acoustic", option=f"ablations.{name}", value=False,
+                   reason="the gfx950 stage always runs every small-step operator of sound.F",
+                   remedy="keep the small-step operators enabled")
+    return (1 if ablations.radiation_history_production else 0) | (2 if ablations.stage_family_fills else 0)
+
+
+def launch_cuda_psolver3_terrain_open_three_stage(
+    arrays: Mapping[str, Any], metrics: Mapping[str, Any], *, ni: int, nj: int, nk: int, dt: float,
+    nsound: int, dx: float, dy: float, dz: float, selection: Any | None = None,
+    ablations: TerrainOpenSoundAblations | None = None, stream: Any | None = None, compiled: Any | None = None,
+    lateral_sides: tuple[int, int, int, int] = (2, 2, 2, 2), alph: float | None = None,
+    kdiv: float | None = None, profiles: VerticalProfiles | None = None, _nrk_sequence: tuple[int, ...] = (1, 2, 3),
+    **_: Any,
+) -> CudaPsolver3TerrainOpenLaunch:
+    """Enqueue the acoustic RK stages of ``_nrk_sequence`` on ``stream``."""
+    for name, value in (("alph", alph), ("kdiv", kdiv)):
+        if value is not None and not (0.0 <= float(value) <= 1.0):
+            raise ValueError(f"{name}={value} is outside CM1's admitted [0, 1]")
+    dt, dx, dy, dz = (check_spacing(n, v) for n, v in (("dt", dt), ("dx", dx), ("dy", dy), ("dz", dz)))
+    flags = _check_ablations(ablations or TerrainOpenSoundAblations())
+    sides = lateral_code(lateral_sides)
+    shapes = terrain_open_sound_shapes(ni, nj, nk)
+    a = {}
+    for name in (*PRESSURE_NAMES, *U_NAMES, *V_NAMES, *W_NAMES, *BOUNDARY_NAMES):
+        a[name] = require(arrays[name], name, shapes[name])
+    scalar = arrays.get("scalar")
+    if scalar is not None:
+        scalar = require(scalar, "scalar", shapes["scalar"])
+    ms = terrain_static_shapes(ni=ni, nj=nj, nk=nk)
+    m = {name: require(metrics[name], name, ms[name]) for name in METRIC_NAMES}
+    no_alias({**a, "scalar": scalar}, "terrain/open sound arrays must not alias")
+    prof = profiles or terrain_coordinate_profiles(nk, dz)
+    dprof = device_profiles(prof, a["u3d"].device)
+    alph32 = float(np.float32(CM1_ALPH_F32 if alph is None else alph))
+    kdiv32 = float(np.float32(CM1_KDIV_F32 if kdiv is None else kdiv))
+    h = stream_handle(stream)
+    cadence = []
+    for nrk in _nrk_sequence:
+        nloop, dts = sound3_stage_schedule_f32(dt, nsound, nrk)
+        _native.call(
+            "gm1hip_psolver3_stage",
+            *(ptr(a[n]) for n in ("ppi", "pp3d", "ppx", "ppd", "piadv", "ppterm", "ppten", "thv", "qk", "pk1",
+                                  "pk2", "mm", "fpk", "elim_c", "elim_d", "ua", "u3d", "uten", "rru", "va", "v3d",
+                                  "vten", "rrv", "wa", "w3d", "wten", "rrw", "radbcw", "radbce", "radbcs",
+                                  "radbcn", "uh", "vh", "ruf", "rvf")),
+            ptr(scalar),
+            *(ptr(m[n]) for n in METRIC_NAMES),
+            ptr(dprof), prof.ztop,
+            ni, nj, nk, nrk, nloop, dts, rk_stage_dt_f32(dt, nrk), dx, dy, alph32, kdiv32, sides, flags, h,
+        )
+        cadence.append({"nrk": nrk, "nloop": nloop, "dts_f32": dts})
+    return CudaPsolver3TerrainOpenLaunch(record(stream), compiled, tuple(cadence), {
+        "domain": {"ni": ni, "nj": nj, "nk": nk, "ngxy": NG},
+        "cadence": cadence,
+        "acoustic_reals": {"alph": alph32, "kdiv": kdiv32},
+        "lateral_boundaries": list(lateral_sides),
+        "target_arch": "gfx950",
+    })
+
+
+def launch_cuda_psolver3_terrain_open_stage(
+    arrays: Mapping[str, Any], metrics: Mapping[str, Any], *, ni: int, nj: int, nk: int, nrk: int, dt: float,
+    nsound: int, dx: float, dy: float, dz: float, **kwargs: Any,
+) -> CudaPsolver3TerrainOpenLaunch:
+    """Exactly one acoustic RK stage (cuda_psolver3_terrain_open.py:1436-1507)."""
+    if isinstance(nrk, bool) or not isinstance(nrk, int) or nrk not in (1, 2, 3):
+        raise ValueError("nrk must select explicit CM1 RK stage 1, 2, or 3")
+    return launch_cuda_psolver3_terrain_open_three_stage(
+        arrays, metrics, ni=ni, nj=nj, nk=nk, dt=dt, nsound=nsound, dx=dx, dy=dy, dz=dz,
+        _nrk_sequence=(nrk,), **kwargs)
+
+
+__all__ = [
+    "CM1_ALPH_F32", "CM1_CP_F32", "CM1_KDIV_F32", "CudaPsolver3TerrainOpenLaunch", "TerrainOpenSoundAblations",
+    "launch_cuda_psolver3_terrain_open_stage", "launch_cuda_psolver3_terrain_open_three_stage",
+    "terrain_open_sound_shapes",
+]

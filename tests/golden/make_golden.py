@@ -163,7 +163,55 @@ def momentum5():
     _save("momentum5", **out)
 
 
-GENERATORS = {"advection5": advection5, "momentum5": momentum5}
+def psolver_inputs(rng, ni, nj, nk, *, terrain=True):
+    """Seeded, physically scaled acoustic-stage carriers (our own construction)."""
+    _ref()
+    from cm1core.cuda_psolver3_terrain_open import terrain_open_sound_shapes
+
+    shapes = terrain_open_sound_shapes(ni, nj, nk)
+    a = {name: np.zeros(shape, F) for name, shape in shapes.items()}
+    p = shapes["ppi"]
+    a["thv"][:] = (300.0 + rng.standard_normal(p)).astype(F)
+    a["piadv"][:] = (1.0 - 0.01 * np.arange(nk + 2)[:, None, None] + 1e-4 * rng.standard_normal(p)).astype(F)
+    a["ppterm"][:] = (0.4 + 0.01 * rng.standard_normal(p)).astype(F)
+    a["ppten"][:] = (1e-5 * rng.standard_normal(p)).astype(F)
+    for name in ("ppi", "pp3d", "ppx"):
+        a[name][:] = (1e-4 * rng.standard_normal(p)).astype(F)
+    for name, scale in (("ua", 5.0), ("u3d", 5.0), ("va", 5.0), ("v3d", 5.0), ("wa", 1.0), ("w3d", 1.0),
+                        ("uten", 1e-3), ("vten", 1e-3), ("wten", 1e-3)):
+        a[name][:] = (scale * rng.standard_normal(a[name].shape)).astype(F)
+    for name in ("radbcw", "radbce", "radbcs", "radbcn"):
+        a[name][:] = (-5.0 * np.abs(rng.standard_normal(a[name].shape))).astype(F)
+    for name in ("uh", "vh", "ruf", "rvf"):
+        a[name][:] = F(1.0)
+    a["scalar"][:] = (1.0 + rng.standard_normal(shapes["scalar"])).astype(F)
+    m = seeded_metrics(rng, ni, nj, nk, terrain=terrain)
+    return a, m
+
+
+def psolver3():
+    """cpu_psolver3_terrain_open_stage_inplace, three stages (all-open family)."""
+    _ref()
+    from cm1core.cuda_psolver3_terrain_open import cpu_psolver3_terrain_open_stage_inplace
+
+    out = {}
+    cases = ((9, 8, 8, 31, 6, 2.0, 250.0, 200.0, 150.0, True), (10, 9, 9, 32, 8, 1.5, 500.0, 500.0, 250.0, False))
+    for case, (ni, nj, nk, seed, ns, dt, dx, dy, dz, terrain) in enumerate(cases):
+        rng = np.random.default_rng(seed)
+        a, m = psolver_inputs(rng, ni, nj, nk, terrain=terrain)
+        out.update({f"c{case}_in_{k}": v.copy() for k, v in a.items()})
+        out.update({f"c{case}_metric_{k}": v for k, v in m.items()})
+        for nrk in (1, 2, 3):
+            cpu_psolver3_terrain_open_stage_inplace(a, m, ni=ni, nj=nj, nk=nk, dt=dt, nsound=ns, nrk=nrk,
+                                                   dx=dx, dy=dy, dz=dz)
+            for k in ("u3d", "v3d", "w3d", "pp3d", "ppd", "ppx", "rru", "rrv", "rrw",
+                      "radbcw", "radbce", "radbcs", "radbcn", "scalar", "qk", "pk1", "pk2", "mm"):
+                out[f"c{case}_nrk{nrk}_{k}"] = a[k].copy()
+        out[f"c{case}_params"] = np.array([ni, nj, nk, ns, dt, dx, dy, dz], np.float64)
+    _save("psolver3", **out)
+
+
+GENERATORS = {"advection5": advection5, "momentum5": momentum5, "psolver3": psolver3}
 
 
 if __name__ == "__main__":
