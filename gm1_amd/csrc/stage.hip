@@ -1,0 +1,400 @@
+// Per-RK-stage bridge kernels of the CM1 large step (gfx950):
+// eqtset=2 stage terms, buoyancy, post-acoustic getdiv/getdivx, scalar
+// packing and the A-anchored RK compose, held-tendency seeding, the held
+// compose of solve1 and the solve3 pressure/density finish.
+//
+// Replaces cuda_moist_stage_bridge.py, the compose/pack kernels of
+// cuda_scalar_moist_rk.py, cuda_momentum_rk.py's seed and the forecast
+// kernels of cuda_gm1_forecast.py.  All are single-pass, i-fastest
+// elementwise or column kernels; getdivx's vertical mass flux and divx share
+// one column pass.
+#include "boundary.h"
+
+using namespace gm1;
+
+namespace {
+
+constexpr float RD = 287.04f, RV = 461.5f, CP = 1005.7f, CPV = 1870.0f;
+constexpr float CPL = 4190.0f, CPI = 2106.0f, GRAV = 9.81f;
+
+// ----------------------------------------------------------- stage terms
+// q3d: (numq, nk, nj, ni) species in the order qv, qc, qr, qi, qs, qg.
+__global__ __launch_bounds__(256) void k_stage_terms(
+    const float* __restrict__ th0, const float* __restrict__ qv0, const float* __restrict__ pi0,
+    const float* __restrict__ qc0, const float* __restrict__ qi0, const float* __restrict__ th3d,
+    const float* __restrict__ q3d, const float* __restrict__ pp3d, float* __restrict__ thv,
+    float* __restrict__ ppterm, float* __restrict__ dum8, float* __restrict__ thterm, int ni, int nj, int nk,
+    int numq) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i >= ni) return;
+  const float cv = CP - RD, cvv = CPV - RV, rovcp = RD / CP, reps = RV / RD, repsm1 = reps - 1.0f;
+  const int64_t n = (int64_t)ni * nj * nk;
+  const int64_t z = ((int64_t)k * nj + j) * ni + i;
+  // species beyond numq are exact zeros (a dry state, or warm rain without ice)
+  const float q0 = numq >= 1 ? q3d[z] : 0.0f;
+  const float q1 = numq >= 2 ? q3d[n + z] : 0.0f, q2 = numq >= 3 ? q3d[2 * n + z] : 0.0f;
+  const float q3 = numq >= 4 ? q3d[3 * n + z] : 0.0f, q4 = numq >= 5 ? q3d[4 * n + z] : 0.0f;
+  const float q5 = numq >= 6 ? q3d[5 * n + z] : 0.0f;
+  const float qv = fmaxf(q0, 0.0f);
+  const float liq = q1 + q2;
+  const float ice = (q3 + q4) + q5;
+  const float qli = fmaxf(liq + ice, 0.0f);
+  const float cpli = CPL * fmaxf(liq, 0.0f) + CPI * fmaxf(ice, 0.0f);
+  const float bt = th0[k], bq = qv0[k], bp = pi0[k];
+  const float rth = 1.0f / bt, theta = bt + th3d[z];
+  const float bc = qc0 ? qc0[k] : 0.0f, bi = qi0 ? qi0[k] : 0.0f;
+  float b = th3d[z] * rth + repsm1 * (qv - bq);
+  b = b - ((qli - bc) - bi);
+  dum8[ix3(k + 1, j + NG, i + NG, sy(nj), sx(ni))] = GRAV * b;
+  const int64_t p = pix(k + 1, j + 1, i + 1, ni, nj);
+  thv[p] = (theta * (1.0f + reps * qv)) / ((1.0f + qv) + qli);
+  const float cpm = (CP + CPV * qv) + cpli;
+  const float cvm = 1.0f / ((cv + cvv * qv) + cpli);
+  thterm[z] = (theta * ((RD + RV * qv) - rovcp * cpm)) * cvm;
+  const float pi = bp + pp3d[p];
+  ppterm[p] = ((pi * rovcp) * cpm) * cvm;
+}
+
+__global__ __launch_bounds__(256) void k_piadv(const float* __restrict__ pi0, const float* __restrict__ pp3d,
+                                               float* __restrict__ piadv, int ni, int nj, int nk) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z + 1;
+  if (i >= ni + 2) return;
+  const int64_t p = pix(k, j, i, ni, nj);
+  piadv[p] = pi0[k - 1] + pp3d[p];
+}
+
+// ----------------------------------------------------------- buoyancy
+struct BuoyArgs {
+  const float* dum8; const float* c1; const float* c2; const float* gxu; const float* gyv;
+  float* uten; float* vten; float* wten; int ni, nj, nk;
+};
+
+GM1_DEV float d8(const BuoyArgs& a, int k, int j, int i) { return a.dum8[ix3(k, j + NG, i + NG, sy(a.nj), sx(a.ni))]; }
+GM1_DEV float bfw(const BuoyArgs& a, int kf, int j, int i) {
+  if (kf == 1 || kf == a.nk + 1) return 0.0f;
+  const int64_t p = ((int64_t)(kf - 1) * a.nj + j) * a.ni + i;
+  return a.c1[p] * d8(a, kf - 1, j, i) + a.c2[p] * d8(a, kf, j, i);
+}
+
+__global__ __launch_bounds__(256) void k_buoyancy(BuoyArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z + 1;
+  const int ni = a.ni, nj = a.nj, nk = a.nk;
+  if (i >= ni) return;
+  if (k >= 2) {
+    const int64_t d = ix3(k, j + NG, i + NG, sy(nj), sx(ni));
+    a.wten[d] = a.wten[d] + bfw(a, k, j, i);
+  }
+  if (i >= 1) {  // u face i between cells i-1 and i
+    float t = (bfw(a, k + 1, j, i) + bfw(a, k + 1, j, i - 1)) + (bfw(a, k, j, i) + bfw(a, k, j, i - 1));
+    t = 0.125f * t;
+    t = t - 0.25f * (d8(a, k, j, i) + d8(a, k, j, i - 1));
+    t = t * (a.gxu[((int64_t)(k - 1) * nj + j) * (ni + 1) + i] + a.gxu[((int64_t)k * nj + j) * (ni + 1) + i]);
+    const int64_t d = ix3(k, j + NG, i + NG, sy(nj), ux(ni));
+    a.uten[d] = a.uten[d] + t;
+  }
+  if (j >= 1) {  // v face j between cells j-1 and j
+    float t = (bfw(a, k + 1, j, i) + bfw(a, k + 1, j - 1, i)) + (bfw(a, k, j, i) + bfw(a, k, j - 1, i));
+    t = 0.125f * t;
+    t = t - 0.25f * (d8(a, k, j, i) + d8(a, k, j - 1, i));
+    t = t * (a.gyv[((int64_t)(k - 1) * (nj + 1) + j) * ni + i] + a.gyv[((int64_t)k * (nj + 1) + j) * ni + i]);
+    const int64_t d = ix3(k, j + NG, i + NG, vy(nj), sx(ni));
+    a.vten[d] = a.vten[d] + t;
+  }
+}
+
+// ----------------------------------------------------------- getdiv
+struct DivArgs {
+  float* rru; float* rrv; float* rrw;
+  const float* rho0; const float* rf0; const float* rgzu; const float* rgzv; const float* dzdx; const float* dzdy;
+  const float* gz; const float* thterm; float* div; float* theta_div; float* divx;
+  int ni, nj, nk; float rdx, rdy, dz;
+};
+
+// one column: w-level terrain terms from the averaged winds, then div per level
+__global__ __launch_bounds__(64) void k_getdiv(DivArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  const int ni = a.ni, nj = a.nj, nk = a.nk;
+  if (i >= ni) return;
+  const int nxu = ux(ni), nyu = sy(nj), nxs = sx(ni), nyv = vy(nj);
+  const int64_t h = (int64_t)j * ni + i;
+  const float rgw = a.rgzu[(int64_t)j * (ni + 1) + i], rge = a.rgzu[(int64_t)j * (ni + 1) + i + 1];
+  const float rgs = a.rgzv[h], rgn = a.rgzv[h + ni];
+  const float dz = a.dz, rds = 1.0f / dz, rdsf = 1.0f / dz, zt = (float)nk * dz, rzt = 1.0f / zt;
+  const float ddx = a.dzdx[h], ddy = a.dzdy[h], g = a.gz[h];
+  auto xw = [&](int k) { return a.rru[ix3(k, j + NG, i + NG, nyu, nxu)] * rgw; };
+  auto xe = [&](int k) { return a.rru[ix3(k, j + NG, i + NG + 1, nyu, nxu)] * rge; };
+  auto ys = [&](int k) { return a.rrv[ix3(k, j + NG, i + NG, nyv, nxs)] * rgs; };
+  auto yn = [&](int k) { return a.rrv[ix3(k, j + NG + 1, i + NG, nyv, nxs)] * rgn; };
+  auto wt = [&](int kf) {
+    if (kf == 1 || kf == nk + 1) return 0.0f;
+    const float sf = (float)(kf - 1) * dz, sm = ((float)(kf - 1) - 0.5f) * dz;
+    float r2 = (sf - sm) * rds;
+    const float r1 = 0.5f * (1.0f - r2);
+    r2 = 0.5f * r2;
+    const float ux_ = r2 * (xw(kf) + xe(kf)) + r1 * (xw(kf - 1) + xe(kf - 1));
+    const float vy_ = r2 * (ys(kf) + yn(kf)) + r1 * (ys(kf - 1) + yn(kf - 1));
+    float t = ux_ * ddx + vy_ * ddy;
+    t = ((t * (sf - zt)) * g) * rzt;
+    return a.rrw[ix3(kf, j + NG, i + NG, nyu, nxs)] + t;
+  };
+  float wlo = wt(1);
+  for (int k = 1; k <= nk; ++k) {
+    const float whi = wt(k + 1);
+    float d = (xe(k) - xw(k)) * a.rdx + (yn(k) - ys(k)) * a.rdy;
+    d = d + (whi - wlo) * rdsf;
+    if (fabsf(d) < 1.0e-20f) d = 0.0f;
+    const int64_t z = (int64_t)(k - 1) * nj * ni + h;
+    a.div[z] = d;
+    a.theta_div[z] = d * a.thterm[z];
+    wlo = whi;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_getdivx_h(DivArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // 0..ni
+  const int j = blockIdx.y;                             // 0..nj
+  const int k = blockIdx.z + 1;
+  const int ni = a.ni, nj = a.nj;
+  if (i > ni) return;
+  const int64_t plane = (int64_t)(k - 1) * nj * ni;
+  if (j < nj) {
+    const int l = (i == 0) ? 0 : i - 1, r = (i == ni) ? ni - 1 : i;
+    const float den = 0.5f * (a.rho0[plane + (int64_t)j * ni + l] + a.rho0[plane + (int64_t)j * ni + r]);
+    const int64_t p = ix3(k, j + NG, i + NG, sy(nj), ux(ni));
+    a.rru[p] = (den * a.rru[p]) * a.rgzu[(int64_t)j * (ni + 1) + i];
+  }
+  if (i < ni) {
+    const int s = (j == 0) ? 0 : j - 1, n = (j == nj) ? nj - 1 : j;
+    const float den = 0.5f * (a.rho0[plane + (int64_t)s * ni + i] + a.rho0[plane + (int64_t)n * ni + i]);
+    const int64_t p = ix3(k, j + NG, i + NG, vy(nj), sx(ni));
+    a.rrv[p] = (den * a.rrv[p]) * a.rgzv[(int64_t)j * ni + i];
+  }
+}
+
+__global__ __launch_bounds__(64) void k_getdivx_v(DivArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  const int ni = a.ni, nj = a.nj, nk = a.nk;
+  if (i >= ni) return;
+  const int nxu = ux(ni), nyu = sy(nj), nxs = sx(ni), nyv = vy(nj);
+  const int x = i + NG, y = j + NG;
+  const int64_t h = (int64_t)j * ni + i;
+  const float dz = a.dz, rds = 1.0f / dz, rdsf = 1.0f / dz, zt = (float)nk * dz, rzt = 1.0f / zt;
+  const float ddx = a.dzdx[h], ddy = a.dzdy[h], g = a.gz[h];
+  const int64_t wst = (int64_t)nyu * nxs, wc = (int64_t)y * nxs + x;
+  a.rrw[wst + wc] = 0.0f;
+  float below = 0.0f;
+  for (int kf = 2; kf <= nk + 1; ++kf) {
+    float val = 0.0f;
+    if (kf <= nk) {
+      const float sf = (float)(kf - 1) * dz, sm = ((float)(kf - 1) - 0.5f) * dz;
+      float r2 = (sf - sm) * rds;
+      const float r1 = 0.5f * (1.0f - r2);
+      r2 = 0.5f * r2;
+      const float uxv = r2 * (a.rru[ix3(kf, y, x, nyu, nxu)] + a.rru[ix3(kf, y, x + 1, nyu, nxu)]) +
+                        r1 * (a.rru[ix3(kf - 1, y, x, nyu, nxu)] + a.rru[ix3(kf - 1, y, x + 1, nyu, nxu)]);
+      const float vyv = r2 * (a.rrv[ix3(kf, y, x, nyv, nxs)] + a.rrv[ix3(kf, y + 1, x, nyv, nxs)]) +
+                        r1 * (a.rrv[ix3(kf - 1, y, x, nyv, nxs)] + a.rrv[ix3(kf - 1, y + 1, x, nyv, nxs)]);
+      float t = uxv * ddx + vyv * ddy;
+      t = ((t * (sf - zt)) * g) * rzt;
+      val = a.rf0[(int64_t)(kf - 1) * nj * ni + h] * a.rrw[(int64_t)kf * wst + wc] + t;
+    }
+    a.rrw[(int64_t)kf * wst + wc] = val;
+    const int k = kf - 1;
+    float d = (a.rru[ix3(k, y, x + 1, nyu, nxu)] - a.rru[ix3(k, y, x, nyu, nxu)]) * a.rdx +
+              (a.rrv[ix3(k, y + 1, x, nyv, nxs)] - a.rrv[ix3(k, y, x, nyv, nxs)]) * a.rdy;
+    d = d + (val - below) * rdsf;
+    a.divx[(int64_t)(k - 1) * nj * ni + h] = (fabsf(d) < 1.0e-20f) ? 0.0f : d;
+    below = val;
+  }
+}
+
+// ----------------------------------------------------------- scalar RK
+__global__ __launch_bounds__(256) void k_pack(const float* __restrict__ th0, const float* __restrict__ src,
+                                              float* __restrict__ padded, int ni, int nj, int nk, int theta) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i >= ni) return;
+  const int64_t z = ((int64_t)k * nj + j) * ni + i;
+  const float v = theta ? (th0[k] - 300.0f) + src[z] : src[z];
+  padded[ix3(k + 1, j + NG, i + NG, sy(nj), sx(ni))] = v;
+}
+
+__global__ __launch_bounds__(256) void k_compose(const float* __restrict__ a0, const float* __restrict__ held,
+                                                 const float* __restrict__ divterm, const float* __restrict__ adv,
+                                                 float* __restrict__ out, int64_t n, float dttmp) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const float seed = divterm ? held[q] - divterm[q] : held[q];
+  const float v = a0[q] + dttmp * (seed + adv[q]);
+  out[q] = (fabsf(v) < 1.0e-30f) ? 0.0f : v;
+}
+
+// ----------------------------------------------------------- held seed
+__global__ __launch_bounds__(256) void k_seed_held(const float* __restrict__ u1, const float* __restrict__ v1,
+                                                   const float* __restrict__ w1, float* __restrict__ ut,
+                                                   float* __restrict__ vt, float* __restrict__ wt, int ni, int nj,
+                                                   int nk) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // 0..ni
+  const int j = blockIdx.y;                             // 0..nj
+  const int k = blockIdx.z + 1;
+  if (i > ni) return;
+  const int x = i + NG, y = j + NG;
+  const int64_t pu = ix3(k, y, x, sy(nj), ux(ni)), pv = ix3(k, y, x, vy(nj), sx(ni)), pw = ix3(k, y, x, sy(nj), sx(ni));
+  ut[pu] = u1[pu];
+  vt[pv] = v1[pv];
+  wt[pw] = w1[pw];
+}
+
+// ----------------------------------------------------------- solve1 held compose
+__global__ __launch_bounds__(256) void k_held_compose(
+    const float* __restrict__ th0, const float* __restrict__ pi0, const float* __restrict__ rho,
+    const float* __restrict__ ppi, const float* __restrict__ tha, const float* __restrict__ qa,
+    float* __restrict__ thheld, float* __restrict__ qheld, float* __restrict__ ppten,
+    const float* __restrict__ qpten, const float* __restrict__ qtten, const float* __restrict__ qvten,
+    const float* __restrict__ qcten, int ni, int nj, int nk, float dt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i >= ni) return;
+  const float rp00 = 1.0f / 100000.0f, rddcv = RD / (CP - RD), tem = 0.0001f * 0.0001f, qsmall = 1.0e-14f;
+  const int64_t n = (int64_t)ni * nj * nk, z = ((int64_t)k * nj + j) * ni + i;
+  const int64_t p = pix(k + 1, j + 1, i + 1, ni, nj);
+  const float ht = thheld[z], qvt = qheld[z];
+  if (fabsf(dt * ht) > tem || fabsf(dt * qvt) > qsmall) {
+    const float thnew = tha[z] + dt * ht, qvnew = qa[z] + dt * qvt;
+    float base = (rho[z] * (th0[k] + thnew)) * (RD + RV * qvnew);
+    base = base * rp00;
+    const float pinew = powf(base, rddcv) - pi0[k];
+    ppten[p] = ((pinew - ppi[p]) / dt) + qpten[z];
+  } else {
+    ppten[p] = qpten[z];
+  }
+  thheld[z] = ht + qtten[z];
+  qheld[z] = qvt + qvten[z];
+  qheld[n + z] = qheld[n + z] + qcten[z];
+}
+
+// ----------------------------------------------------------- solve3
+__global__ __launch_bounds__(256) void k_solve3_pressure(float* __restrict__ ppx, const float* __restrict__ pp3d,
+                                                         float* __restrict__ ppi, int ni, int nj) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z + 1;
+  if (i >= ni) return;
+  const int64_t p = pix(k, j + 1, i + 1, ni, nj);
+  ppx[p] = pp3d[p] + ppx[p];
+  ppi[p] = pp3d[p];
+}
+
+__global__ __launch_bounds__(256) void k_solve3_density(const float* __restrict__ rho, const float* __restrict__ c1,
+                                                        const float* __restrict__ c2, float* __restrict__ rr,
+                                                        float* __restrict__ rf, int ni, int nj, int nk) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, f = blockIdx.z;  // 0..nk
+  if (i >= ni) return;
+  const int64_t plane = (int64_t)nj * ni, h = (int64_t)j * ni + i;
+  if (f < nk) rr[f * plane + h] = 1.0f / rho[f * plane + h];
+  float v;
+  if (f == 0) v = (1.875f * rho[h] + -1.25f * rho[plane + h]) + 0.375f * rho[2 * plane + h];
+  else if (f == nk) v = (1.875f * rho[(nk - 1) * plane + h] + -1.25f * rho[(nk - 2) * plane + h]) + 0.375f * rho[(nk - 3) * plane + h];
+  else v = c1[f * plane + h] * rho[(f - 1) * plane + h] + c2[f * plane + h] * rho[f * plane + h];
+  rf[f * plane + h] = v;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gm1hip_stage_terms(const float* th0, const float* qv0, const float* pi0, const float* qc0, const float* qi0,
+                       const float* th3d, const float* q3d, int numq, const float* pp3d, float* thv, float* ppterm,
+                       float* piadv, float* dum8, float* thterm, int ni, int nj, int nk, unsigned sides,
+                       void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_stage_terms, dim3((ni + 63) / 64, nj, nk), dim3(64), 0, s, th0, qv0, pi0, qc0, qi0, th3d, q3d,
+                     pp3d, thv, ppterm, dum8, thterm, ni, nj, nk, numq);
+  hipLaunchKernelGGL(k_piadv, dim3((ni + 2 + 63) / 64, nj + 2, nk), dim3(64), 0, s, pi0, pp3d, piadv, ni, nj, nk);
+  GM1_LAUNCH_CHECK();
+  int rc = fill_pressure(thv, sides, ni, nj, nk, s);
+  if (!rc) rc = fill_family(dum8, 0, sides, ni, nj, nk, s);
+  return rc;
+}
+
+int gm1hip_buoyancy(const float* dum8, const float* c1, const float* c2, const float* gxu, const float* gyv,
+                    float* uten, float* vten, float* wten, int ni, int nj, int nk, void* stream) {
+  BuoyArgs a{dum8, c1, c2, gxu, gyv, uten, vten, wten, ni, nj, nk};
+  hipLaunchKernelGGL(k_buoyancy, dim3((ni + 63) / 64, nj, nk), dim3(64), 0, (hipStream_t)stream, a);
+  GM1_LAUNCH_CHECK();
+  return 0;
+}
+
+int gm1hip_post_acoustic(float* rru, float* rrv, float* rrw, const float* rho0_3d, const float* rf0,
+                         const float* rgzu, const float* rgzv, const float* dzdx, const float* dzdy, const float* gz,
+                         const float* thterm, float* divergence, float* theta_div, float* divx, int ni, int nj,
+                         int nk, float dx, float dy, float dz, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  DivArgs a{rru, rrv, rrw, rho0_3d, rf0, rgzu, rgzv, dzdx, dzdy, gz, thterm, divergence, theta_div, divx,
+            ni, nj, nk, 1.0f / dx, 1.0f / dy, dz};
+  hipLaunchKernelGGL(k_getdiv, dim3((ni + 63) / 64, nj), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(k_getdivx_h, dim3((ni + 1 + 63) / 64, nj + 1, nk), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(k_getdivx_v, dim3((ni + 63) / 64, nj), dim3(64), 0, s, a);
+  GM1_LAUNCH_CHECK();
+  return 0;
+}
+
+int gm1hip_pack_scalar(const float* th0, const float* src, float* padded, int ni, int nj, int nk, int theta,
+                       unsigned sides, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_pack, dim3((ni + 63) / 64, nj, nk), dim3(64), 0, s, th0, src, padded, ni, nj, nk, theta);
+  GM1_LAUNCH_CHECK();
+  return fill_family(padded, 0, sides, ni, nj, nk, s);
+}
+
+int gm1hip_rk_compose(const float* a0, const float* held, const float* divterm, const float* adv, float* out,
+                      int64_t n, float dttmp, void* stream) {
+  hipLaunchKernelGGL(k_compose, dim3(gm1_blocks(n, 256)), dim3(256), 0, (hipStream_t)stream, a0, held, divterm, adv,
+                     out, n, dttmp);
+  GM1_LAUNCH_CHECK();
+  return 0;
+}
+
+int gm1hip_seed_held_momentum(const float* uten1, const float* vten1, const float* wten1, float* uten, float* vten,
+                              float* wten, int ni, int nj, int nk, void* stream) {
+  hipLaunchKernelGGL(k_seed_held, dim3((ni + 1 + 63) / 64, nj + 1, nk), dim3(64), 0, (hipStream_t)stream, uten1,
+                     vten1, wten1, uten, vten, wten, ni, nj, nk);
+  GM1_LAUNCH_CHECK();
+  return 0;
+}
+
+int gm1hip_held_compose(const float* th0, const float* pi0, const float* rho, const float* ppi, const float* tha,
+                        const float* qa, float* thheld, float* qheld, float* ppten, const float* qpten,
+                        const float* qtten, const float* qvten, const float* qcten, int ni, int nj, int nk, float dt,
+                        void* stream) {
+  hipLaunchKernelGGL(k_held_compose, dim3((ni + 63) / 64, nj, nk), dim3(64), 0, (hipStream_t)stream, th0, pi0, rho,
+                     ppi, tha, qa, thheld, qheld, ppten, qpten, qtten, qvten, qcten, ni, nj, nk, dt);
+  GM1_LAUNCH_CHECK();
+  return 0;
+}
+
+int gm1hip_solve3_pressure(float* ppx, float* pp3d, float* ppi, int ni, int nj, int nk, unsigned sides,
+                           void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_solve3_pressure, dim3((ni + 63) / 64, nj, nk), dim3(64), 0, s, ppx, pp3d, ppi, ni, nj);
+  GM1_LAUNCH_CHECK();
+  int rc = fill_pressure(ppx, sides, ni, nj, nk, s);
+  if (!rc) rc = fill_pressure(pp3d, sides, ni, nj, nk, s);
+  if (!rc) rc = fill_pressure(ppi, sides, ni, nj, nk, s);
+  return rc;
+}
+
+int gm1hip_solve3_density(const float* rho, const float* c1, const float* c2, float* rr, float* rf, int ni, int nj,
+                          int nk, void* stream) {
+  hipLaunchKernelGGL(k_solve3_density, dim3((ni + 63) / 64, nj, nk + 1), dim3(64), 0, (hipStream_t)stream, rho, c1,
+                     c2, rr, rf, ni, nj, nk);
+  GM1_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // extern "C"

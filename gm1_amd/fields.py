@@ -1,0 +1,228 @@
+"""Synthetic initial states for the benchmark and parity configurations (host side).
+
+The reference builds its analytic storm environments in Rust
+(``rust/cm1-state/src/storm_initialization.rs:40-330``, a Weisman-Klemp
+hydrostatic fixed point) and perturbs them with a cos^2 warm bubble
+(``initial_perturbation.rs:448-467``) and seeded noise
+(``ensemble_branch.py:116-161``).  This module writes its own NumPy
+generator of the same kind of state:
+
+* potential temperature theta(z) = 300 + 43 (z/12 km)^1.25 K below 12 km,
+  isothermal at 213 K above (the Weisman-Klemp form); surface pressure
+  1000 hPa; Exner function integrated hydrostatically in binary64;
+* moist soundings: relative humidity 0.9 below 1.2 km falling linearly to
+  0.25 at 12 km, surface qv capped at 14 g/kg, saturation from Bolton's
+  (1980) vapour-pressure fit, virtual potential temperature in the balance;
+* winds: the configuration's analytic profile;
+* bubble: amplitude * cos^2(pi*beta/2) for beta < 1 with beta the scaled
+  distance from the centre;
+* noise: splitmix64 (Steele, Lea & Flood 2014) uniform in [-a, a) in k, j, i
+  order, the stream the reference's ensemble branch uses.
+
+Everything is computed in binary64 and rounded once to binary32.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .grid import VerticalProfiles, terrain_coordinate_profiles
+
+F = np.float32
+NG = 3
+G = 9.81
+RD = 287.04
+RV = 461.5
+CP = 1005.7
+P00 = 1.0e5
+EPS = RD / RV
+
+_GOLDEN = 0x9E3779B97F4A7C15
+_MIX1 = 0xBF58476D1CE4E5B9
+_MIX2 = 0x94D049BB133111EB
+
+
+def splitmix64_words(seed: int, count: int) -> np.ndarray:
+    """``count`` consecutive splitmix64 outputs after ``seed`` (uint64)."""
+    if seed < 0 or seed >= 1 << 64 or count < 1:
+        raise ValueError("seed must be a uint64 and count >= 1")
+    idx = np.arange(1, count + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + idx * np.uint64(_GOLDEN)
+        z = z ^ (z >> np.uint64(30))
+        z = z * np.uint64(_MIX1)
+        z = z ^ (z >> np.uint64(27))
+        z = z * np.uint64(_MIX2)
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform_noise(seed: int, amplitude: float, shape: tuple[int, ...]) -> np.ndarray:
+    """amplitude*(2u-1) with u the top 53 bits of each splitmix64 word."""
+    n = int(np.prod(shape))
+    u = (splitmix64_words(seed, n) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+    return (float(amplitude) * (2.0 * u - 1.0)).astype(np.float32).reshape(shape)
+
+
+@dataclass
+class CaseConfig:
+    """One benchmark/parity case: grid, time step, family and environment."""
+
+    name: str
+    ni: int
+    nj: int
+    nk: int
+    dx: float
+    dy: float
+    dz: float
+    dt: float
+    nsound: int
+    lateral_sides: tuple[int, int, int, int]
+    moist: bool
+    wind: str  # "linear15" or "hodograph"
+    bubbles: tuple[tuple[float, float, float, float], ...]  # (amp K, x m, y m, z m)
+    bubble_radius: tuple[float, float] = (10000.0, 1400.0)
+    noise: float = 0.05
+    seed: int = 1
+    stretched: tuple[float, float, float] | None = None  # (dz_low, z_low, dz_top)
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def cells(self) -> int:
+        return self.ni * self.nj * self.nk
+
+
+def dry_c1(ni=64, nj=64, nk=40) -> CaseConfig:
+    """64x64x40, dx 1 km, dz 500 m, periodic, dt 6 s, 6 substeps, dry."""
+    return CaseConfig("dry-periodic-64x64x40", ni, nj, nk, 1000.0, 1000.0, 500.0, 6.0, 6, (1, 1, 1, 1), False,
+                      "linear15", ((2.0, ni * 500.0, nj * 500.0, 1400.0),), noise=0.05, seed=1)
+
+
+def moist_c2(ni=320, nj=320, nk=80) -> CaseConfig:
+    """320x320x80, dx=dz=250 m, open-radiative, dt 1.5 s, 8 substeps, Kessler."""
+    return CaseConfig("moist-kessler-320x320x80", ni, nj, nk, 250.0, 250.0, 250.0, 1.5, 8, (2, 2, 2, 2), True,
+                      "hodograph", ((3.0, ni * 125.0, nj * 125.0, 1400.0),), noise=0.1, seed=2)
+
+
+def stretched_levels(nk: int, dz_low: float, z_low: float, dz_top: float, z_top: float) -> tuple[np.ndarray, np.ndarray]:
+    """w-level heights: dz_low up to z_low, then geometric growth to dz_top at z_top."""
+    n_low = int(round(z_low / dz_low))
+    n_up = nk - n_low
+    # solve for the ratio r with dz_low*r^n_up ~ dz_top and matching the depth
+    r = (dz_top / dz_low) ** (1.0 / max(n_up, 1))
+    dzs = [dz_low] * n_low + [dz_low * r ** (m + 1) for m in range(n_up)]
+    zf = np.concatenate([[0.0], np.cumsum(dzs)])
+    sigmaf = np.zeros(nk + 2, np.float64)
+    sigmaf[1 : nk + 2] = zf
+    sigma = np.zeros(nk + 2, np.float64)
+    sigma[1 : nk + 1] = 0.5 * (zf[:-1] + zf[1:])
+    return sigma, sigmaf
+
+
+def profiles_for(cfg: CaseConfig) -> VerticalProfiles:
+    if cfg.stretched is None:
+        return terrain_coordinate_profiles(cfg.nk, cfg.dz)
+    dz_low, z_low, dz_top = cfg.stretched
+    sigma, sigmaf = stretched_levels(cfg.nk, dz_low, z_low, dz_top, 18000.0)
+    return terrain_coordinate_profiles(cfg.nk, cfg.dz, sigma=sigma.astype(F), sigmaf=sigmaf.astype(F))
+
+
+def _theta_env(z: np.ndarray) -> np.ndarray:
+    ztr, thtr, ttr = 12000.0, 343.0, 213.0
+    th = np.where(z <= ztr, 300.0 + 43.0 * (np.maximum(z, 0.0) / ztr) ** 1.25, thtr * np.exp(G / (CP * ttr) * (z - ztr)))
+    return th
+
+
+def _rh_env(z: np.ndarray) -> np.ndarray:
+    return np.where(z <= 1200.0, 0.9, np.where(z >= 12000.0, 0.25, 0.9 - 0.65 * (z - 1200.0) / 10800.0))
+
+
+def base_state(cfg: CaseConfig, prof: VerticalProfiles) -> dict[str, np.ndarray]:
+    """Hydrostatic column at scalar and w levels (binary64 then binary32)."""
+    nk = cfg.nk
+    zs = prof.sigma[1 : nk + 1].astype(np.float64)
+    zf = prof.sigmaf[1 : nk + 2].astype(np.float64)
+    th = _theta_env(zs)
+    qv = np.zeros(nk)
+    # integrate Exner from the surface (pi=1 at 1000 hPa) at scalar levels
+    pi = np.empty(nk)
+    for it in range(20):
+        thv = th * (1.0 + qv / EPS) / (1.0 + qv)
+        pi[0] = 1.0 - G / CP * zs[0] / thv[0]
+        for k in range(1, nk):
+            pi[k] = pi[k - 1] - G / CP * (zs[k] - zs[k - 1]) * 2.0 / (thv[k - 1] + thv[k])
+        if not cfg.moist:
+            break
+        t = th * pi
+        p = P00 * pi ** (CP / RD)
+        es = 611.2 * np.exp(17.67 * (t - 273.15) / (t - 29.65))
+        qvs = EPS * es / np.maximum(p - es, 1.0)
+        qv = np.minimum(_rh_env(zs) * qvs, 0.014)
+    thv = th * (1.0 + qv / EPS) / (1.0 + qv)
+    p = P00 * pi ** (CP / RD)
+    rho = p / (RD * thv * pi)
+    # w-level density by linear interpolation in height, extrapolated at the ends
+    rhof = np.interp(zf, zs, rho)
+    rhof[0] = rho[0] + (rho[0] - rho[1]) * (zs[0] - zf[0]) / (zs[1] - zs[0])
+    rhof[-1] = rho[-1] + (rho[-1] - rho[-2]) * (zf[-1] - zs[-1]) / (zs[-1] - zs[-2])
+    c2 = np.zeros(nk + 1)
+    c2[1:nk] = (zf[1:nk] - zs[:-1]) / (zs[1:] - zs[:-1])
+    c1 = 1.0 - c2
+    return {"th0": th.astype(F), "pi0": pi.astype(F), "qv0": qv.astype(F), "rho0": rho.astype(F),
+            "rf0": rhof.astype(F), "c1": c1.astype(F), "c2": c2.astype(F), "zs": zs, "zf": zf}
+
+
+def metrics_for(cfg: CaseConfig, base: dict[str, np.ndarray]) -> dict[str, np.ndarray]:
+    """Flat-terrain metric carriers in the reference's terrain_static_shapes layout."""
+    ni, nj, nk = cfg.ni, cfg.nj, cfg.nk
+    m: dict[str, np.ndarray] = {}
+    col = lambda v: np.broadcast_to(np.asarray(v, np.float32)[:, None, None], (len(v), nj, ni)).copy()  # noqa: E731
+    m["rho0_3d"] = col(base["rho0"])
+    m["rr0"] = (F(1.0) / m["rho0_3d"]).astype(F)
+    m["rf0"] = col(base["rf0"])
+    m["rrf0"] = (F(1.0) / m["rf0"]).astype(F)
+    m["c1"] = col(base["c1"])
+    m["c2"] = col(base["c2"])
+    m["zh"] = col(base["zs"].astype(F))
+    m["zf"] = col(base["zf"].astype(F))
+    for name, shape in (("mh", (nk, nj, ni)), ("rmh", (nk, nj, ni)), ("mf", (nk + 1, nj, ni)), ("rmf", (nk + 1, nj, ni)),
+                        ("gz", (nj, ni)), ("rgz", (nj, ni)), ("gzu", (nj, ni + 1)), ("rgzu", (nj, ni + 1)),
+                        ("gzv", (nj + 1, ni)), ("rgzv", (nj + 1, ni))):
+        m[name] = np.ones(shape, np.float32)
+    for name, shape in (("terrain_msl", (nj, ni)), ("dzdx", (nj, ni)), ("dzdy", (nj, ni)), ("gx", (nk + 1, nj, ni)),
+                        ("gxu", (nk + 1, nj, ni + 1)), ("gy", (nk + 1, nj, ni)), ("gyv", (nk + 1, nj + 1, ni))):
+        m[name] = np.zeros(shape, np.float32)
+    return m
+
+
+def wind_profile(cfg: CaseConfig, z: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    if cfg.wind == "linear15":
+        return np.minimum(z / 6000.0, 1.0) * 15.0, np.zeros_like(z)
+    # quarter circle of radius 10 m/s below 2 km, straight shear to 30 m/s at 6 km
+    ang = np.clip(z / 2000.0, 0.0, 1.0) * (0.5 * math.pi)
+    u = np.where(z < 2000.0, 10.0 - 10.0 * np.cos(ang), 10.0 + 20.0 * np.clip((z - 2000.0) / 4000.0, 0.0, 1.0))
+    v = np.where(z < 2000.0, 10.0 * np.sin(ang), 10.0)
+    return u, v
+
+
+def initial_fields(cfg: CaseConfig, prof: VerticalProfiles, base: dict[str, np.ndarray]) -> dict[str, np.ndarray]:
+    """theta', u, v, w, pressure carriers and moisture of the initial state."""
+    ni, nj, nk = cfg.ni, cfg.nj, cfg.nk
+    zs = base["zs"]
+    x = (np.arange(ni) + 0.5) * cfg.dx
+    y = (np.arange(nj) + 0.5) * cfg.dy
+    th3d = np.zeros((nk, nj, ni), np.float64)
+    xr, zr = cfg.bubble_radius
+    for amp, xc, yc, zc in cfg.bubbles:
+        beta = np.sqrt(((x[None, None, :] - xc) / xr) ** 2 + ((y[None, :, None] - yc) / xr) ** 2 + ((zs[:, None, None] - zc) / zr) ** 2)
+        th3d += np.where(beta < 1.0, amp * np.cos(0.5 * math.pi * beta) ** 2, 0.0)
+    th3d = th3d.astype(F) + uniform_noise(cfg.seed, cfg.noise, (nk, nj, ni))
+    u, v = wind_profile(cfg, zs)
+    out = {"th3d": th3d.astype(F), "u_profile": u.astype(F), "v_profile": v.astype(F)}
+    if cfg.moist:
+        q = np.zeros((3, nk, nj, ni), np.float32)
+        q[0] = np.broadcast_to(base["qv0"][:, None, None], (nk, nj, ni))
+        out["q3d"] = q
+    return out

@@ -1,0 +1,218 @@
+"""The composed dry large step: oracle invariants on the CPU, bit parity on the GPU.
+
+The composition (cuda_gm1_forecast.py:1241-1627) has no CPU form in the
+reference; each operator is pinned in its own test file, and the
+composition is pinned here by a state at rest staying exactly at rest,
+periodic translation equivariance, and the stage operators' closed forms.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from helpers import F, NG, assert_bits_equal, seeded_metrics, shapes, to_dev, to_host
+from gm1_amd.fields import CaseConfig
+from gm1_amd.large_step import host_initial_state
+from oracle import large_step as ol
+from oracle import sound as osound
+from oracle import stage as ost
+
+PROGNOSTIC = ("u3d", "v3d", "w3d", "th3d", "pp3d", "ppx", "ppi", "ppd")
+
+
+def dry_case(ni, nj, nk, sides, *, amp=2.0, noise=0.05, wind="linear15", seed=3):
+    return CaseConfig(f"dry-{ni}x{nj}x{nk}", ni, nj, nk, 1000.0, 1000.0, 500.0, 6.0, 6, tuple(sides), False, wind,
+                      ((amp, ni * 450.0, nj * 520.0, 1400.0),), bubble_radius=(ni * 300.0, 1400.0), noise=noise,
+                      seed=seed)
+
+
+def oracle_run(host, nsteps):
+    cfg = host["cfg"]
+    a = {k: v.copy() for k, v in host["arrays"].items()}
+    prof = osound.terrain_coordinate_profiles(cfg.nk, cfg.dz)
+    c = {"ni": cfg.ni, "nj": cfg.nj, "nk": cfg.nk, "dx": cfg.dx, "dy": cfg.dy, "dz": cfg.dz, "dt": cfg.dt,
+         "nsound": cfg.nsound, "lateral_sides": cfg.lateral_sides}
+    out = []
+    for _ in range(nsteps):
+        ol.large_step(a, host["metrics"], host["base"], prof, c, zt=F(cfg.nk * cfg.dz))
+        out.append({k: a[k].copy() for k in PROGNOSTIC})
+    return out
+
+
+def test_state_at_rest_stays_exactly_at_rest():
+    cfg = dry_case(12, 10, 10, (1, 1, 1, 1), amp=0.0, noise=0.0)
+    host = host_initial_state(cfg)
+    for name in ("u3d", "ua", "v3d", "va"):
+        host["arrays"][name][...] = F(0.0)
+    (s,) = oracle_run(host, 1)
+    for name in PROGNOSTIC:
+        assert not np.any(s[name]), name
+
+
+def test_periodic_large_step_is_translation_equivariant():
+    ni, nj, nk = 16, 12, 10
+    cfg = dry_case(ni, nj, nk, (1, 1, 1, 1))
+    h0 = host_initial_state(cfg)
+    h1 = host_initial_state(cfg)
+    di, dj = 5, 3
+    for name in ("th3d", "tha"):
+        h1["arrays"][name][...] = np.roll(h0["arrays"][name], (dj, di), axis=(1, 2))
+    (s0,) = oracle_run(h0, 1)
+    (s1,) = oracle_run(h1, 1)
+    interior = {"u3d": (NG, NG + nj, NG, NG + ni), "v3d": (NG, NG + nj, NG, NG + ni), "w3d": (NG, NG + nj, NG, NG + ni),
+                "pp3d": (1, nj + 1, 1, ni + 1)}
+    for name, (y0, y1, x0, x1) in interior.items():
+        want = np.roll(s0[name][:, y0:y1, x0:x1], (dj, di), axis=(1, 2))
+        assert_bits_equal(s1[name][:, y0:y1, x0:x1], want, name)
+    assert_bits_equal(s1["th3d"], np.roll(s0["th3d"], (dj, di), axis=(1, 2)), "th3d")
+
+
+def test_bubble_rises_and_fields_stay_bounded():
+    cfg = dry_case(16, 12, 12, (1, 1, 1, 1), noise=0.0)
+    host = host_initial_state(cfg)
+    s = oracle_run(host, 2)[-1]
+    w = s["w3d"][2 : cfg.nk + 1, NG : NG + cfg.nj, NG : NG + cfg.ni]
+    assert np.isfinite(w).all() and w.max() > 1e-3 and np.abs(w).max() < 5.0
+    assert np.abs(s["pp3d"]).max() < 1e-3
+
+
+def test_compose_clamps_only_after_the_update():
+    """A-anchored compose (cuda_scalar_moist_rk.py:336-341); tests/test_cuda_scalar_moist_rk.py:119-183 shape."""
+    tha = np.array([1.0, 1e-31, 0.0, 2.0], F)
+    held = np.array([0.0, 0.0, 1e-31, -1.0], F)
+    div = np.array([0.0, 0.0, 0.0, 0.0], F)
+    adv = np.array([0.0, 0.0, 0.0, 0.5], F)
+    out = ost.compose_theta(tha, held, div, adv, F(2.0))
+    np.testing.assert_array_equal(out, np.array([1.0, 0.0, 0.0, 1.0], F))
+
+
+def test_density_finish_closed_forms():
+    rho = np.linspace(1.2, 0.4, 8, dtype=F)[:, None, None] * np.ones((8, 2, 3), F)
+    c2 = np.full((9, 2, 3), F(0.5))
+    c1 = (F(1.0) - c2).astype(F)
+    rr, rf = ost.solve3_density(rho, c1, c2)
+    np.testing.assert_array_equal(rr, F(1.0) / rho)
+    # three-point extrapolation is exact for a linear column
+    np.testing.assert_allclose(rf[0], 1.2 + 0.5 * (1.2 - (1.2 - 0.8 / 7)), rtol=2e-6)
+    np.testing.assert_allclose(rf[1:8], 0.5 * (rho[:-1] + rho[1:]), rtol=1e-7)
+
+
+def _stage_inputs(rng, ni, nj, nk):
+    sh = shapes(ni, nj, nk)
+    m = seeded_metrics(rng, ni, nj, nk)
+    a = {}
+    for n in ("u3d", "uten", "rru", "uten1"):
+        a[n] = (2 * rng.standard_normal(sh["u"])).astype(F)
+    for n in ("v3d", "vten", "rrv", "vten1"):
+        a[n] = (2 * rng.standard_normal(sh["v"])).astype(F)
+    for n in ("w3d", "wten", "rrw", "wten1"):
+        a[n] = rng.standard_normal(sh["w"]).astype(F)
+    for n in ("thv", "ppterm", "piadv", "pp3d", "ppx", "ppi"):
+        a[n] = (1e-3 * rng.standard_normal(sh["p"])).astype(F)
+    a["dum8"] = np.zeros(sh["s"], F)
+    a["thterm"] = np.zeros((nk, nj, ni), F)
+    base = {"th0": np.linspace(300, 340, nk).astype(F), "qv0": np.linspace(0.014, 0.001, nk).astype(F),
+            "pi0": np.linspace(1.0, 0.7, nk).astype(F)}
+    th3d = rng.standard_normal((nk, nj, ni)).astype(F)
+    q3d = np.abs(1e-3 * rng.standard_normal((3, nk, nj, ni))).astype(F)
+    return a, m, base, th3d, q3d
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sides", [(1, 1, 1, 1), (2, 2, 2, 2), (3, 4, 2, 2)])
+@pytest.mark.parametrize("moist", [False, True])
+def test_gpu_stage_terms_and_buoyancy(device, sides, moist):
+    import torch
+
+    from gm1_amd import _native
+    from gm1_amd._device import lateral_code, ptr, stream_handle
+
+    ni, nj, nk = 19, 13, 9
+    rng = np.random.default_rng(11 + moist)
+    a, m, base, th3d, q3d = _stage_inputs(rng, ni, nj, nk)
+    q = q3d if moist else None
+    ha = {k: v.copy() for k, v in a.items()}
+    qfull = None
+    if moist:
+        qfull = np.zeros((6, nk, nj, ni), F)
+        qfull[:3] = q3d
+    ost.stage_terms(ha, base, th3d, qfull, ni=ni, nj=nj, nk=nk, sides=sides)
+    ost.buoyancy(ha, m, ni=ni, nj=nj, nk=nk)
+    da, dm, db = to_dev(a, device), to_dev(m, device), to_dev(base, device)
+    dth, dq = to_dev(th3d, device), (to_dev(q, device) if moist else None)
+    h = stream_handle(None)
+    _native.call("gm1hip_stage_terms", ptr(db["th0"]), ptr(db["qv0"]), ptr(db["pi0"]), None, None, ptr(dth),
+                 ptr(dq), 3 if moist else 0, ptr(da["pp3d"]), ptr(da["thv"]), ptr(da["ppterm"]), ptr(da["piadv"]),
+                 ptr(da["dum8"]), ptr(da["thterm"]), ni, nj, nk, lateral_code(sides), h)
+    _native.call("gm1hip_buoyancy", ptr(da["dum8"]), ptr(dm["c1"]), ptr(dm["c2"]), ptr(dm["gxu"]), ptr(dm["gyv"]),
+                 ptr(da["uten"]), ptr(da["vten"]), ptr(da["wten"]), ni, nj, nk, h)
+    torch.cuda.synchronize()
+    got = to_host(da)
+    for name in ("thv", "ppterm", "thterm", "dum8", "uten", "vten", "wten"):
+        assert_bits_equal(got[name], ha[name], name)
+    ks = slice(1, nk + 1)
+    assert_bits_equal(got["piadv"][ks], ha["piadv"][ks], "piadv")
+
+
+@pytest.mark.gpu
+def test_gpu_post_acoustic_and_compose(device):
+    import torch
+
+    from gm1_amd import _native
+    from gm1_amd._device import ptr, stream_handle
+
+    ni, nj, nk = 21, 17, 10
+    rng = np.random.default_rng(5)
+    a, m, base, th3d, q3d = _stage_inputs(rng, ni, nj, nk)
+    a["thterm"] = rng.standard_normal((nk, nj, ni)).astype(F)
+    ha = {k: v.copy() for k, v in a.items()}
+    div, tdiv, divx = (np.zeros((nk, nj, ni), F) for _ in range(3))
+    ost.getdiv(ha, m, div, tdiv, ha["thterm"], ni=ni, nj=nj, nk=nk, dx=250.0, dy=300.0, dz=200.0)
+    ost.getdivx(ha, m, divx, ni=ni, nj=nj, nk=nk, dx=250.0, dy=300.0, dz=200.0)
+    da, dm = to_dev(a, device), to_dev(m, device)
+    dd = {n: torch.zeros((nk, nj, ni), dtype=torch.float32, device=device) for n in ("div", "tdiv", "divx")}
+    h = stream_handle(None)
+    _native.call("gm1hip_post_acoustic", ptr(da["rru"]), ptr(da["rrv"]), ptr(da["rrw"]), ptr(dm["rho0_3d"]),
+                 ptr(dm["rf0"]), ptr(dm["rgzu"]), ptr(dm["rgzv"]), ptr(dm["dzdx"]), ptr(dm["dzdy"]), ptr(dm["gz"]),
+                 ptr(da["thterm"]), ptr(dd["div"]), ptr(dd["tdiv"]), ptr(dd["divx"]), ni, nj, nk, 250.0, 300.0,
+                 200.0, h)
+    # compose: A + dttmp*((held - theta_div) + adv), |x| < 1e-30 -> 0
+    tha = rng.standard_normal((nk, nj, ni)).astype(F)
+    tha.flat[::7] = F(1e-31)
+    held = (1e-3 * rng.standard_normal((nk, nj, ni))).astype(F)
+    adv = (1e-3 * rng.standard_normal((nk, nj, ni))).astype(F)
+    held.flat[::7] = F(0.0)
+    adv.flat[::7] = F(0.0)
+    tdiv_zeroed = tdiv.copy()
+    tdiv_zeroed.flat[::7] = F(0.0)
+    want = ost.compose_theta(tha, held, tdiv_zeroed, adv, F(2.0))
+    dtha, dheld, dadv, dtd = (to_dev(x, device) for x in (tha, held, adv, tdiv_zeroed))
+    out = torch.empty_like(dtha)
+    _native.call("gm1hip_rk_compose", ptr(dtha), ptr(dheld), ptr(dtd), ptr(dadv), ptr(out), tha.size, 2.0, h)
+    torch.cuda.synchronize()
+    assert_bits_equal(to_host(dd["div"]), div, "divergence")
+    assert_bits_equal(to_host(dd["tdiv"]), tdiv, "theta_div")
+    assert_bits_equal(to_host(dd["divx"]), divx, "divx")
+    got = to_host(da)
+    for name in ("rru", "rrv", "rrw"):
+        assert_bits_equal(got[name], ha[name], name)
+    assert_bits_equal(to_host(out), want, "compose")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sides", [(1, 1, 1, 1), (2, 2, 2, 2), (1, 1, 2, 2)])
+def test_gpu_dry_large_step_bit_exact(device, sides):
+    import torch
+
+    from gm1_amd.large_step import create_gm1_large_step_state, launch_cuda_gm1_large_step
+
+    cfg = dry_case(24, 20, 12, sides)
+    host = host_initial_state(cfg)
+    want = oracle_run(host, 2)
+    st = create_gm1_large_step_state(host, device)
+    for step in range(2):
+        launch_cuda_gm1_large_step(st)
+        torch.cuda.synchronize()
+        for name in PROGNOSTIC:
+            assert_bits_equal(to_host(st.a[name]), want[step][name], f"step {step + 1} {name}")
