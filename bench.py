@@ -1,0 +1,217 @@
+#!/usr/bin/env python3
+"""Throughput of the CM1 dynamical-core large step on one MI355X.
+
+Prints ONE JSON line: grid-cell RK3 large-steps/s over K timed steps of the
+320x320x80 open-radiative warm-bubble case (dx = dz = 250 m, dt = 1.5 s,
+8 acoustic substeps: 3 + 4 + 8 = 15 small steps per large step), with all
+fields resident in HBM before the timed region.
+
+``roofline`` is computed for the dominant kernel from its algorithmic bytes
+per launch and its average launch duration, both measured live here with
+HIP event pairs recorded on the stream the kernel runs on.  ``traffic`` is
+read from the committed rocprofv3 counter summary for the same kernel and
+workload when one exists.  ``cpu_baseline`` times the NumPy restatement of
+the same step (the oracle, one thread) on a bounded sub-domain sample.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+from dataclasses import replace
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+WORD = 4
+
+# Algorithmic words per physical cell for one launch of each timed kernel
+# (each input read once, each output written once; halos, staggering and
+# 1-D/2-D metric columns are not counted).
+KERNEL_WORDS = {
+    # reads u, v, w, wten, pp3d, ppten, ppterm, piadv, qk, pk1, pk2, mm, rr0,
+    # rrf0, rru, rrv, rrw; writes w, pp3d, ppd, rru, rrv, rrw
+    "k_snd_column": 23,
+    # reads u, v, uten, vten, ppd, thv, gxu, gyv; writes u, v
+    "k_snd_pgrad": 10,
+    # reads scalar, rru, rrv, rrw, divx, rr0; writes tendency
+    "k_scalar_adv5": 7,
+}
+
+
+def parse_args(argv=None):
+    p = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--ni", type=int, default=320)
+    p.add_argument("--nj", type=int, default=320)
+    p.add_argument("--nk", type=int, default=80)
+    p.add_argument("--kernel", default="k_snd_column", choices=sorted(KERNEL_WORDS))
+    p.add_argument("--cpu-sample", type=int, default=128, help="horizontal size of the oracle's sub-domain sample")
+    p.add_argument("--cpu-steps", type=int, default=2)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--calibration-copy-mib", type=int, default=0,
+                   help="issue one plain device copy of this size before the steps (HBM counter calibration)")
+    return p.parse_args(argv)
+
+
+def workload(ni, nj, nk):
+    from gm1_amd.fields import moist_c2
+
+    # the moist configuration's grid, sounding, winds, bubble and boundaries;
+    # this step carries no water species yet
+    return replace(moist_c2(ni, nj, nk), moist=False, name=f"dry-open-{ni}x{nj}x{nk}")
+
+
+def traffic_from_profiles(kernel: str, label: str):
+    """HBM bytes per launch from profiles/<kernel>_<label>_hbm.md (``traffic_bytes_per_launch: N``)."""
+    path = ROOT / "profiles" / f"{kernel}_{label}_hbm.md"
+    if not path.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", path.read_text())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(cfg, sample: int, steps: int):
+    """Time the oracle (NumPy, one thread) on a sample x sample x nk sub-domain of the same case."""
+    import numpy as np
+
+    from gm1_amd.large_step import host_initial_state
+    from oracle import large_step as ol
+    from oracle import sound as osound
+
+    sub = replace(cfg, ni=sample, nj=sample)
+    host = host_initial_state(sub)
+    a = host["arrays"]
+    prof = osound.terrain_coordinate_profiles(sub.nk, sub.dz)
+    c = {"ni": sub.ni, "nj": sub.nj, "nk": sub.nk, "dx": sub.dx, "dy": sub.dy, "dz": sub.dz, "dt": sub.dt,
+         "nsound": sub.nsound, "lateral_sides": sub.lateral_sides}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ol.large_step(a, host["metrics"], host["base"], prof, c, zt=np.float32(sub.nk * sub.dz))
+    elapsed = time.perf_counter() - t0
+    return {
+        "value": sub.cells * steps / elapsed,
+        "unit": "grid-cell large-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{steps} large steps of a {sample}x{sample}x{sub.nk} sub-domain of the same case "
+                  f"({elapsed:.1f} s, NumPy oracle, one thread)",
+    }
+
+
+def main(argv=None) -> int:
+    args = parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              "(the large step is sharded across GPUs only by design, not implemented)", file=sys.stderr)
+        return 2
+    import torch
+
+    from gm1_amd import timing
+    from gm1_amd.large_step import create_gm1_large_step_state, host_initial_state, launch_cuda_gm1_large_step
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        return 2
+    cfg = workload(args.ni, args.nj, args.nk)
+    host = host_initial_state(cfg)
+    stream = torch.cuda.Stream()
+    st = create_gm1_large_step_state(host, "cuda:0", stream=stream)
+    del host
+    torch.cuda.synchronize()
+    print(f"[bench] state resident: {cfg.name}, {cfg.cells} cells", file=sys.stderr, flush=True)
+
+    if args.calibration_copy_mib > 0:
+        n = args.calibration_copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] calibration copy: {n * 4} bytes read, {n * 4} bytes written", file=sys.stderr, flush=True)
+        del src, dst
+
+    for _ in range(args.warmup):
+        launch_cuda_gm1_large_step(st)
+    torch.cuda.synchronize()
+    print(f"[bench] {args.warmup} warmup steps done", file=sys.stderr, flush=True)
+
+    small_steps = {6: 11, 8: 15, 10: 19}.get(cfg.nsound, 3 * cfg.nsound)
+    per_step = {"k_snd_column": small_steps, "k_snd_pgrad": small_steps, "k_scalar_adv5": 3}[args.kernel]
+    timing.arm(args.kernel, per_step * args.steps + 8)
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for _ in range(args.steps):
+        launch_cuda_gm1_large_step(st)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    elapsed_ms = start.elapsed_time(stop)
+    kernel_ms, launches = timing.collect()
+    print(f"[bench] {args.steps} timed steps: {elapsed_ms:.2f} ms", file=sys.stderr, flush=True)
+
+    w = st.a["w3d"]
+    if not torch.isfinite(w).all():
+        print("[bench] non-finite w after the timed steps", file=sys.stderr)
+        return 1
+
+    ms_per_step = elapsed_ms / args.steps
+    value = cfg.cells * args.steps / (elapsed_ms * 1e-3)
+    avg_launch_s = kernel_ms * 1e-3 / max(launches, 1)
+    algo_bytes = KERNEL_WORDS[args.kernel] * WORD * cfg.cells
+    achieved = algo_bytes / avg_launch_s / 1e9
+    label = f"{cfg.ni}x{cfg.nj}x{cfg.nk}"
+    result = {
+        "metric": "grid-cell RK3 large-steps/s on 1 GPU",
+        "value": value,
+        "unit": "grid-cell large-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic warm-bubble fields (analytic sounding, splitmix64 noise)",
+        "config": {
+            "workload": f"CM1 large step (WENO5/5th-order advection + psolver=3 acoustic), {label} cells, "
+                        f"dx=dy=dz={cfg.dx:g} m, open-radiative laterals, dt={cfg.dt:g} s, "
+                        f"{cfg.nsound} substeps ({small_steps} small steps/step), dry (no water species yet)",
+            "cells": cfg.cells,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": args.kernel,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic_from_profiles(args.kernel, label),
+            "algorithmic_bytes_per_launch": algo_bytes,
+            "avg_launch_us": avg_launch_s * 1e6,
+            "launches": launches,
+            "kernel_share_of_step": kernel_ms / elapsed_ms,
+        },
+    }
+    if not args.no_cpu_baseline:
+        print("[bench] timing the CPU baseline sample", file=sys.stderr, flush=True)
+        result["cpu_baseline"] = cpu_baseline(cfg, args.cpu_sample, args.cpu_steps)
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

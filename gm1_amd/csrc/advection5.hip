@@ -10,6 +10,7 @@
 //   * i is the fastest index of every access, so a wavefront's 64 lanes read
 //     one contiguous 256-byte row segment.
 #include "cm1_common.h"
+#include "timing.h"
 
 using namespace gm1;
 
@@ -316,8 +317,8 @@ int gm1hip_scalar_advection5(const float* scalar, const float* rru, const float*
   chunks = (nk + a.kchunk - 1) / a.kchunk;
   dim3 grid((ni + TX - 1) / TX, (nj + TY - 1) / TY, chunks);
   hipStream_t s = (hipStream_t)stream;
-  if (use_weno) hipLaunchKernelGGL(k_scalar_adv5<true>, grid, dim3(NT), 0, s, a);
-  else hipLaunchKernelGGL(k_scalar_adv5<false>, grid, dim3(NT), 0, s, a);
+  if (use_weno) GM1_TIMED(TK_SCALAR_ADV5, s, hipLaunchKernelGGL(k_scalar_adv5<true>, grid, dim3(NT), 0, s, a));
+  else GM1_TIMED(TK_SCALAR_ADV5, s, hipLaunchKernelGGL(k_scalar_adv5<false>, grid, dim3(NT), 0, s, a));
   GM1_LAUNCH_CHECK();
   return 0;
 }

@@ -10,6 +10,7 @@
 //                      registers (one launch instead of two).
 //   * k_mom_adv{u,v,w}: one thread per output word, i fastest.
 #include "cm1_common.h"
+#include "timing.h"
 
 using namespace gm1;
 
@@ -473,11 +474,11 @@ int gm1hip_momentum_advection5(
   const bool ps = side_of(obc, 2) == 1u, pn = side_of(obc, 3) == 1u;
   const int f0 = pw ? 0 : 1, nf = ni - 1 + (pw ? 1 : 0) + (pe ? 1 : 0);
   const int g0 = ps ? 0 : 1, nr = nj - 1 + (ps ? 1 : 0) + (pn ? 1 : 0);
-  hipLaunchKernelGGL(k_mom_advu, dim3((nf + 63) / 64, nj, nk), blk, 0, s, a, f0, nf);
+  GM1_TIMED(TK_MOM_ADVU, s, hipLaunchKernelGGL(k_mom_advu, dim3((nf + 63) / 64, nj, nk), blk, 0, s, a, f0, nf));
   GM1_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_mom_advv, dim3((ni + 63) / 64, nr, nk), blk, 0, s, a, g0, nr);
+  GM1_TIMED(TK_MOM_ADVV, s, hipLaunchKernelGGL(k_mom_advv, dim3((ni + 63) / 64, nr, nk), blk, 0, s, a, g0, nr));
   GM1_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_mom_advw, dim3((ni + 63) / 64, nj, nk - 1), blk, 0, s, a);
+  GM1_TIMED(TK_MOM_ADVW, s, hipLaunchKernelGGL(k_mom_advw, dim3((ni + 63) / 64, nj, nk - 1), blk, 0, s, a));
   GM1_LAUNCH_CHECK();
   return 0;
 }
@@ -499,11 +500,11 @@ int gm1hip_momentum_transport5(
   const bool ps = side_of(obc, 2) == 1u, pn = side_of(obc, 3) == 1u;
   const int f0 = pw ? 0 : 1, nf = ni - 1 + (pw ? 1 : 0) + (pe ? 1 : 0);
   const int g0 = ps ? 0 : 1, nr = nj - 1 + (ps ? 1 : 0) + (pn ? 1 : 0);
-  hipLaunchKernelGGL(k_mom_advu, dim3((nf + 63) / 64, nj, nk), blk, 0, s, a, f0, nf);
+  GM1_TIMED(TK_MOM_ADVU, s, hipLaunchKernelGGL(k_mom_advu, dim3((nf + 63) / 64, nj, nk), blk, 0, s, a, f0, nf));
   GM1_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_mom_advv, dim3((ni + 63) / 64, nr, nk), blk, 0, s, a, g0, nr);
+  GM1_TIMED(TK_MOM_ADVV, s, hipLaunchKernelGGL(k_mom_advv, dim3((ni + 63) / 64, nr, nk), blk, 0, s, a, g0, nr));
   GM1_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_mom_advw, dim3((ni + 63) / 64, nj, nk - 1), blk, 0, s, a);
+  GM1_TIMED(TK_MOM_ADVW, s, hipLaunchKernelGGL(k_mom_advw, dim3((ni + 63) / 64, nj, nk - 1), blk, 0, s, a));
   GM1_LAUNCH_CHECK();
   return 0;
 }

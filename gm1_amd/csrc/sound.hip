@@ -14,6 +14,7 @@
 //   [fill]    one-cell ppd halo between small steps
 // The column thread walks k with every access i-fastest across the wavefront.
 #include "boundary.h"
+#include "timing.h"
 
 using namespace gm1;
 
@@ -505,8 +506,10 @@ extern "C" int gm1hip_psolver3_stage(
     if (any_open) {
       hipLaunchKernelGGL(k_snd_ssopen, dim3((edge + 63) / 64), dim3(64), 0, s, a);
     }
-    hipLaunchKernelGGL(k_snd_pgrad, dim3((ni + 1 + 63) / 64, nj + 1, nk), dim3(64), 0, s, a);
-    hipLaunchKernelGGL(k_snd_column, dim3((ni + 1 + 63) / 64, nj + 1), dim3(64), 0, s, a, n);
+    GM1_TIMED(TK_ACOUSTIC_PGRAD, s,
+              hipLaunchKernelGGL(k_snd_pgrad, dim3((ni + 1 + 63) / 64, nj + 1, nk), dim3(64), 0, s, a));
+    GM1_TIMED(TK_ACOUSTIC_COLUMN, s,
+              hipLaunchKernelGGL(k_snd_column, dim3((ni + 1 + 63) / 64, nj + 1), dim3(64), 0, s, a, n));
     GM1_LAUNCH_CHECK();
     if (n < nloop) {
       int rc = fill_pressure(ppd, sides, ni, nj, nk, s);

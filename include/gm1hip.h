@@ -1,0 +1,162 @@
+// C ABI of libgm1hip.so: the CM1 dynamical-core large step on gfx950.
+//
+// Every entry point enqueues its kernels on `stream` (a hipStream_t passed
+// as void*; NULL is the default stream), allocates nothing, never
+// synchronises, and returns a hipError_t code (0 on success).  Arrays are
+// caller-owned float32 device pointers in CM1's (z, y, x) C-order layout:
+//
+//   u family   (nk+2, nj+6, ni+7)      v family   (nk+2, nj+7, ni+6)
+//   w family   (nk+3, nj+6, ni+6)      scalars    (nk+2, nj+6, ni+6)
+//   pressure   (nk+2, nj+2, ni+2)      cells      (nk, nj, ni)
+//   metrics    terrain_static_shapes (cuda_terrain_open_boundary.py:429-460)
+//
+// Padded index i (0-based) holds CM1's Fortran column i-2 (ngxy = 3).
+// `obc`/`sides` packs CM1's lateral families (wbc, ebc, sbc, nbc) as
+// wbc | ebc<<4 | sbc<<8 | nbc<<12 with 1 periodic, 2 open-radiative,
+// 3/4 rigid; 0 (scalar advection only) is a tile-local launch.
+//
+// Arithmetic: IEEE binary32 with no contraction, IEEE denormals and
+// correctly rounded division and square root; WENO weights in binary64.
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+// ----------------------------------------------------------------- advection
+// Replaces launch_cuda_terrain_mass_flux5 (cuda_advection5.py:849-945):
+// rru, rrv, rrw mass fluxes and divx (|d| < 1e-20 -> 0).
+int gm1hip_terrain_mass_flux5(const float* u3d, const float* v3d, const float* w3d, float* rru, float* rrv,
+                              float* rrw, const float* rho0_3d, const float* rf0, const float* rgzu,
+                              const float* rgzv, const float* dzdx, const float* dzdy, const float* gz, float* divx,
+                              int ni, int nj, int nk, float dx, float dy, float dz, void* stream);
+
+// Replaces launch_cuda_scalar_advection5 (cuda_advection5.py:948-1049):
+// 5th-order (use_weno=0) or WENO5 (use_weno=1) face fluxes and the scalar
+// tendency in one LDS-tiled pass.  flux_x/y/z may be NULL (fluxes then stay
+// on chip); otherwise they receive the face fluxes as the reference's
+// scratch does.
+int gm1hip_scalar_advection5(const float* scalar, const float* rru, const float* rrv, const float* rrw,
+                             const float* c1, const float* c2, const float* rr0, const float* gz, const float* divx,
+                             float* tendency, float* flux_x, float* flux_y, float* flux_z, int ni, int nj, int nk,
+                             int use_weno, double weps, float dx, float dy, float dz, unsigned obc, void* stream);
+
+// Replaces launch_cuda_terrain_open_momentum_advection5
+// (cuda_momentum_advection5.py:1173-1330): mass fluxes, divx and the
+// advu/advv/advw increments added into uten/vten/wten.  With
+// stop_after_divergence=1 only the mass fluxes and divx are produced (the
+// reference's rrw_boundary_hook / post_divergence_hook seam); the caller
+// then finishes with gm1hip_momentum_transport5.
+int gm1hip_momentum_advection5(const float* u3d, const float* v3d, const float* w3d, float* uten, float* vten,
+                               float* wten, float* rru, float* rrv, float* rrw, const float* uh, const float* vh,
+                               const float* rho0_3d, const float* rf0, const float* rrf0, const float* c2,
+                               const float* gz, const float* gzu, const float* gzv, const float* rgzu,
+                               const float* rgzv, const float* dzdx, const float* dzdy, float* divx, int ni, int nj,
+                               int nk, float dx, float dy, float dz, unsigned obc, int stop_after_divergence,
+                               void* stream);
+
+// Second half of the above (cuda_momentum_advection5.py:871-1011,
+// _cpu_momentum_transport5) from given mass fluxes and divx.
+int gm1hip_momentum_transport5(const float* u3d, const float* v3d, const float* w3d, float* uten, float* vten,
+                               float* wten, const float* rru, const float* rrv, const float* rrw, const float* uh,
+                               const float* vh, const float* rho0_3d, const float* rrf0, const float* c2,
+                               const float* gz, const float* gzu, const float* gzv, const float* divx, int ni, int nj,
+                               int nk, float dx, float dy, float dz, unsigned obc, void* stream);
+
+// ----------------------------------------------------------------- acoustic
+// Replaces launch_cuda_psolver3_terrain_open_stage
+// (cuda_psolver3_terrain_open.py:1436-1507) for one RK stage `nrk` with
+// `nloop` small steps of length `dts`: coefficient preparation, per small
+// step [ssopen on open sides], the forward u/v step (periodic boundary faces
+// included, cuda_boundary_families.py:1640), one fused column pass
+// (divergence, fpk, bcwsfc, pressure advection, Thomas solve for w, pressure
+// update and filter, time averages), the ppd halo; then the radiation
+// history (flags & 1) and the stage-end family fills (flags & 2).
+// `profiles` packs sigma | sigmaf | rds | rdsf (nk+2 each) | cgs[3] | cgt[3].
+// fpk, elim_c and elim_d are the column solve's scratch (pressure shape).
+int gm1hip_psolver3_stage(
+    float* ppi, float* pp3d, float* ppx, float* ppd, const float* piadv, const float* ppterm, const float* ppten,
+    const float* thv, float* qk, float* pk1, float* pk2, float* mm, float* fpk, float* elim_c, float* elim_d,
+    const float* ua, float* u3d, const float* uten, float* rru, const float* va, float* v3d, const float* vten,
+    float* rrv, const float* wa, float* w3d, const float* wten, float* rrw, float* radbcw, float* radbce,
+    float* radbcs, float* radbcn, const float* uh, const float* vh, const float* ruf, const float* rvf,
+    float* scalar, const float* rho0_3d, const float* rr0, const float* rf0, const float* rrf0, const float* c1,
+    const float* c2, const float* gz, const float* rgzu, const float* rgzv, const float* dzdx, const float* dzdy,
+    const float* gx, const float* gy, const float* gxu, const float* gyv, const float* profiles, float zt,
+    int ni, int nj, int nk, int nrk, int nloop, float dts, float dttmp, float dx, float dy, float alph,
+    float kdiv, unsigned sides, int flags, void* stream);
+
+// ----------------------------------------------------------------- boundaries
+// Family halo fill of one padded field (cuda_boundary_families.py:887-940,
+// cpu_family_we_inplace / cpu_family_sn_inplace); kind 0 scalar, 1 u, 2 v, 3 w.
+int gm1hip_fill_family(float* a, int kind, unsigned sides, int ni, int nj, int nk, void* stream);
+// One-cell pressure halo (cuda_boundary_families.py:941-964, cpu_pressure_we/sn_inplace).
+int gm1hip_fill_pressure(float* p, unsigned sides, int ni, int nj, int nk, void* stream);
+// Rigid floor and lid, w = 0 at k = 1 and nk+1 (cuda_sound3_open_boundary.py:315, flag 1).
+int gm1hip_w_rigid(float* w, int ni, int nj, int nk, void* stream);
+// w(k=1) from the terrain slope (cuda_terrain_open_boundary.py:641-665).
+int gm1hip_bcwsfc(const float* u3d, const float* v3d, const float* dzdx, const float* dzdy, const float* gz,
+                  float* w3d, int ni, int nj, void* stream);
+// Stage-entry w(k=1) halo in W, E, S, N order (cuda_momentum_rk.py:274-290).
+int gm1hip_bc2d_open(float* w3d, int ni, int nj, void* stream);
+
+// ----------------------------------------------------------------- RK stage bridge
+// eqtset=2 stage terms (cuda_moist_stage_bridge.py:378-430 and
+// cpu_eqtset2_cell_f32 :62-128): thv, ppterm, piadv (pressure family),
+// dum8 (padded scalar, g*b), thterm (cells); thv and dum8 halos filled.
+// q3d is (numq, nk, nj, ni) in the order qv, qc, qr, qi, qs, qg (numq 0..6;
+// missing species are zero); qc0/qi0 may be NULL (zero profiles).
+int gm1hip_stage_terms(const float* th0, const float* qv0, const float* pi0, const float* qc0, const float* qi0,
+                       const float* th3d, const float* q3d, int numq, const float* pp3d, float* thv, float* ppterm,
+                       float* piadv, float* dum8, float* thterm, int ni, int nj, int nk, unsigned sides,
+                       void* stream);
+// Buoyancy added into wten/uten/vten (cuda_moist_stage_bridge.py:433-453).
+int gm1hip_buoyancy(const float* dum8, const float* c1, const float* c2, const float* gxu, const float* gyv,
+                    float* uten, float* vten, float* wten, int ni, int nj, int nk, void* stream);
+// Post-acoustic bridge (cuda_moist_stage_bridge.py:456-520): divergence of
+// the time-averaged winds, theta_div = div*thterm, then rru/rrv/rrw rebuilt
+// in place as mass fluxes and divx.
+int gm1hip_post_acoustic(float* rru, float* rrv, float* rrw, const float* rho0_3d, const float* rf0,
+                         const float* rgzu, const float* rgzv, const float* dzdx, const float* dzdy, const float* gz,
+                         const float* thterm, float* divergence, float* theta_div, float* divx, int ni, int nj,
+                         int nk, float dx, float dy, float dz, void* stream);
+// Scalar packing into the padded carrier, theta as (th0 - 300) + th3d when
+// theta != 0 (cuda_scalar_moist_rk.py:255-280), then its family halo.
+int gm1hip_pack_scalar(const float* th0, const float* src, float* padded, int ni, int nj, int nk, int theta,
+                       unsigned sides, void* stream);
+// A-anchored RK compose out = a0 + dttmp*((held - divterm) + adv) with
+// |out| < 1e-30 -> 0 (cuda_scalar_moist_rk.py:283-349); divterm may be NULL.
+int gm1hip_rk_compose(const float* a0, const float* held, const float* divterm, const float* adv, float* out,
+                      int64_t n, float dttmp, void* stream);
+// Held momentum tendencies copied over i=1:ni+1, j=1:nj+1, k=1:nk
+// (cuda_momentum_rk.py:242-271).
+int gm1hip_seed_held_momentum(const float* uten1, const float* vten1, const float* wten1, float* uten, float* vten,
+                              float* wten, int ni, int nj, int nk, void* stream);
+// solve1 held compose: ppten from held theta/qv and the microphysics held
+// rates (kernel gm1_predict_and_compose_held_v1, cuda_gm1_forecast.py:555-566).
+int gm1hip_held_compose(const float* th0, const float* pi0, const float* rho, const float* ppi, const float* tha,
+                        const float* qa, float* thheld, float* qheld, float* ppten, const float* qpten,
+                        const float* qtten, const float* qvten, const float* qcten, int ni, int nj, int nk, float dt,
+                        void* stream);
+// solve3 pressure: ppx = pp3d + ppx, ppi = pp3d and their halos
+// (cuda_gm1_forecast.py:1414-1435, _finish_solve3).
+int gm1hip_solve3_pressure(float* ppx, float* pp3d, float* ppi, int ni, int nj, int nk, unsigned sides,
+                           void* stream);
+// solve3 density: rr = 1/rho, rf by c1/c2 with 3-point end extrapolation
+// (kernel gm1_solve3_density_v1, cuda_gm1_forecast.py:568-577).
+int gm1hip_solve3_density(const float* rho, const float* c1, const float* c2, float* rr, float* rf, int ni, int nj,
+                          int nk, void* stream);
+
+// ----------------------------------------------------------------- measurement
+// No counterpart in the reference: live per-kernel GPU time for throughput
+// reports.  arm() creates `capacity` event pairs (before a timed region) and
+// brackets every launch of kernel id 1 k_snd_column, 2 k_snd_pgrad,
+// 3 k_scalar_adv5, 4/5/6 k_mom_advu/advv/advw; 0 disarms.  collect() waits,
+// sums and disarms; it fails if more launches happened than were armed.
+int gm1hip_kernel_timer_arm(int kernel, int capacity);
+int gm1hip_kernel_timer_collect(double* total_ms, int* launches);
+
+#ifdef __cplusplus
+}
+#endif

@@ -211,7 +211,84 @@ def psolver3():
     _save("psolver3", **out)
 
 
-GENERATORS = {"advection5": advection5, "momentum5": momentum5, "psolver3": psolver3}
+def stage():
+    """eqtset=2 cell terms, theta packing, A-anchored compose, held seed and bc2d."""
+    _ref()
+    from cm1core.cuda_moist_stage_bridge import cpu_eqtset2_cell_f32
+    from cm1core.cuda_momentum_rk import cpu_bc2d_open_w_surface_inplace, cpu_seed_momentum_rk_tendencies_inplace
+    from cm1core.cuda_scalar_moist_rk import cpu_compose_scalar_moist_rk_stage_inplace, cpu_theta_transport_scalar_inplace
+
+    rng = np.random.default_rng(41)
+    out = {}
+    # eqtset=2 terms on single cells: profiles, perturbations and six species
+    # (some negative, some exactly zero) drawn per cell
+    n = 240
+    cells = {
+        "th0": rng.uniform(295.0, 350.0, n), "qv0": rng.uniform(0.0, 0.015, n), "pi0": rng.uniform(0.6, 1.0, n),
+        "qc0": np.where(rng.random(n) < 0.3, rng.uniform(0.0, 1e-3, n), 0.0),
+        "qi0": np.where(rng.random(n) < 0.3, rng.uniform(0.0, 1e-4, n), 0.0),
+        "th3d": rng.normal(0.0, 2.0, n), "pp3d": rng.normal(0.0, 1e-3, n),
+    }
+    for q in ("qv", "qc", "qr", "qi", "qs", "qg"):
+        v = rng.uniform(-2e-4, 3e-3 if q in ("qv", "qc", "qr") else 5e-4, n)
+        v[rng.random(n) < 0.25] = 0.0
+        cells[q] = v
+    cells = {k: v.astype(F) for k, v in cells.items()}
+    res = {k: np.zeros(n, F) for k in ("buoyancy", "theta_rho", "theta_term", "pressure_term")}
+    for c in range(n):
+        r = cpu_eqtset2_cell_f32(**{k: float(v[c]) for k, v in cells.items()})
+        for k in res:
+            res[k][c] = r[k]
+    out.update({f"eq_in_{k}": v for k, v in cells.items()})
+    out.update({f"eq_out_{k}": v for k, v in res.items()})
+    # theta packing
+    ni, nj, nk = 11, 9, 8
+    th0 = rng.uniform(299.0, 345.0, nk).astype(F)
+    th3d = rng.normal(0.0, 2.0, (nk, nj, ni)).astype(F)
+    padded = np.zeros((nk + 2, nj + 2 * NG, ni + 2 * NG), F)
+    cpu_theta_transport_scalar_inplace(th0, th3d, padded, ni=ni, nj=nj, nk=nk)
+    out.update({"pack_th0": th0, "pack_th3d": th3d, "pack_padded": padded})
+    # A-anchored compose, six species, three stages at dt 1.5 s
+    shape = (nk, nj, ni)
+    comp = {"theta_a": rng.normal(0.0, 2.0, shape), "theta_held": rng.normal(0.0, 1e-3, shape),
+            "theta_divergence_term": rng.normal(0.0, 1e-3, shape), "theta_advection": rng.normal(0.0, 1e-2, shape),
+            "moisture_a": np.abs(rng.normal(0.0, 1e-3, (6,) + shape)),
+            "moisture_held": rng.normal(0.0, 1e-7, (6,) + shape),
+            "moisture_advection": rng.normal(0.0, 1e-6, (6,) + shape)}
+    comp = {k: v.astype(F) for k, v in comp.items()}
+    comp["theta_a"].flat[::13] = F(1e-31)
+    comp["theta_held"].flat[::13] = F(0.0)
+    comp["theta_divergence_term"].flat[::13] = F(0.0)
+    comp["theta_advection"].flat[::13] = F(0.0)
+    comp["moisture_a"].flat[::11] = F(0.0)
+    comp["moisture_held"].flat[::11] = F(0.0)
+    comp["moisture_advection"].flat[::11] = F(0.0)
+    out.update({f"comp_in_{k}": v for k, v in comp.items()})
+    for nrk in (1, 2, 3):
+        th = np.zeros(shape, F)
+        q = np.zeros((6,) + shape, F)
+        cpu_compose_scalar_moist_rk_stage_inplace(
+            comp["theta_a"], comp["moisture_a"], comp["theta_held"], comp["moisture_held"],
+            comp["theta_divergence_term"], comp["theta_advection"], comp["moisture_advection"], th, q,
+            nrk=nrk, dt=1.5)
+        out[f"comp_nrk{nrk}_theta"] = th
+        out[f"comp_nrk{nrk}_moisture"] = q
+    # held seed and bc2d on the padded momentum family
+    sh = _shapes(ni, nj, nk)
+    held = {n: rng.normal(0.0, 1.0, sh[f]).astype(F) for n, f in (("uten1", "u"), ("vten1", "v"), ("wten1", "w"))}
+    tend = {n: rng.normal(0.0, 1.0, sh[f]).astype(F) for n, f in (("uten", "u"), ("vten", "v"), ("wten", "w"))}
+    out.update({f"seed_in_{k}": v.copy() for k, v in {**held, **tend}.items()})
+    cpu_seed_momentum_rk_tendencies_inplace(held, tend, ni=ni, nj=nj, nk=nk)
+    out.update({f"seed_out_{k}": v for k, v in tend.items()})
+    w = rng.normal(0.0, 1.0, sh["w"]).astype(F)
+    out["bc2d_in_w"] = w.copy()
+    cpu_bc2d_open_w_surface_inplace(w, ni=ni, nj=nj, nk=nk)
+    out["bc2d_out_w"] = w
+    out["dims"] = np.array([ni, nj, nk], np.int64)
+    _save("stage", **out)
+
+
+GENERATORS = {"advection5": advection5, "momentum5": momentum5, "psolver3": psolver3, "stage": stage}
 
 
 if __name__ == "__main__":

@@ -216,3 +216,64 @@ def test_gpu_dry_large_step_bit_exact(device, sides):
         torch.cuda.synchronize()
         for name in PROGNOSTIC:
             assert_bits_equal(to_host(st.a[name]), want[step][name], f"step {step + 1} {name}")
+
+
+# ---------------------------------------------------------------- golden vectors
+def test_eqtset2_terms_match_reference_cells():
+    """cpu_eqtset2_cell_f32 (cuda_moist_stage_bridge.py:62-128) on 240 seeded cells."""
+    from helpers import golden
+
+    g = golden("stage")
+    n = g["eq_in_th0"].size
+    base = {"th0": g["eq_in_th0"], "qv0": g["eq_in_qv0"], "pi0": g["eq_in_pi0"], "qc0": g["eq_in_qc0"],
+            "qi0": g["eq_in_qi0"]}
+    a = {"thv": np.zeros((n + 2, 3, 3), F), "ppterm": np.zeros((n + 2, 3, 3), F),
+         "piadv": np.zeros((n + 2, 3, 3), F), "pp3d": np.zeros((n + 2, 3, 3), F),
+         "dum8": np.zeros((n + 2, 1 + 2 * NG, 1 + 2 * NG), F), "thterm": np.zeros((n, 1, 1), F)}
+    a["pp3d"][1 : n + 1, 1, 1] = g["eq_in_pp3d"]
+    q = np.stack([g[f"eq_in_{s}"] for s in ("qv", "qc", "qr", "qi", "qs", "qg")])[:, :, None, None]
+    ost.stage_terms(a, base, g["eq_in_th3d"][:, None, None], q, ni=1, nj=1, nk=n, sides=(2, 2, 2, 2))
+    assert_bits_equal(a["dum8"][1 : n + 1, NG, NG], g["eq_out_buoyancy"], "buoyancy")
+    assert_bits_equal(a["thv"][1 : n + 1, 1, 1], g["eq_out_theta_rho"], "theta_rho")
+    assert_bits_equal(a["thterm"][:, 0, 0], g["eq_out_theta_term"], "theta_term")
+    assert_bits_equal(a["ppterm"][1 : n + 1, 1, 1], g["eq_out_pressure_term"], "pressure_term")
+
+
+def test_theta_packing_and_compose_match_reference():
+    """cpu_theta_transport_scalar_inplace and cpu_compose_scalar_moist_rk_stage_inplace
+    (cuda_scalar_moist_rk.py:255-349) on seeded carriers, all three stages."""
+    from helpers import golden
+
+    g = golden("stage")
+    ni, nj, nk = (int(x) for x in g["dims"])
+    padded = np.zeros((nk + 2, nj + 2 * NG, ni + 2 * NG), F)
+    ost.pack_theta(g["pack_th0"], g["pack_th3d"], padded, ni=ni, nj=nj, nk=nk)
+    assert_bits_equal(padded, g["pack_padded"], "theta packing")
+    for nrk in (1, 2, 3):
+        dttmp = F(F(1.5) / F(4 - nrk))
+        th = ost.compose_theta(g["comp_in_theta_a"], g["comp_in_theta_held"], g["comp_in_theta_divergence_term"],
+                               g["comp_in_theta_advection"], dttmp)
+        q = ost.compose_moisture(g["comp_in_moisture_a"], g["comp_in_moisture_held"],
+                                 g["comp_in_moisture_advection"], dttmp)
+        assert_bits_equal(th, g[f"comp_nrk{nrk}_theta"], f"theta nrk={nrk}")
+        assert_bits_equal(q, g[f"comp_nrk{nrk}_moisture"], f"moisture nrk={nrk}")
+    # the reference zeroes only after the update: the seeded 1e-31 anchors clamp
+    assert not np.any(g["comp_nrk3_theta"].flat[::13])
+
+
+def test_held_seed_and_bc2d_match_reference():
+    """cpu_seed_momentum_rk_tendencies_inplace and cpu_bc2d_open_w_surface_inplace
+    (cuda_momentum_rk.py:242-290)."""
+    from oracle import boundary as obc
+    from helpers import golden
+
+    g = golden("stage")
+    ni, nj, nk = (int(x) for x in g["dims"])
+    held = [g[f"seed_in_{n}"] for n in ("uten1", "vten1", "wten1")]
+    tend = [g[f"seed_in_{n}"].copy() for n in ("uten", "vten", "wten")]
+    ost.seed_held(held, tend, ni=ni, nj=nj, nk=nk)
+    for name, t in zip(("uten", "vten", "wten"), tend):
+        assert_bits_equal(t, g[f"seed_out_{name}"], name)
+    w = g["bc2d_in_w"].copy()
+    obc.bc2d_open_w_surface(w, ni=ni, nj=nj)
+    assert_bits_equal(w, g["bc2d_out_w"], "bc2d")
