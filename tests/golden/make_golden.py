@@ -288,7 +288,121 @@ def stage():
     _save("stage", **out)
 
 
-GENERATORS = {"advection5": advection5, "momentum5": momentum5, "psolver3": psolver3, "stage": stage}
+def pdef():
+    """cpu_pdef_direction_inplace for x, y and z on seeded rows with activated negatives."""
+    _ref()
+    from cm1core.cuda_scalar_moist_rk import cpu_pdef_direction_inplace
+
+    rng = np.random.default_rng(53)
+    ni, nj, nk = 12, 10, 9
+    sh = _shapes(ni, nj, nk)
+    out = {"dims": np.array([ni, nj, nk], np.int64)}
+    qa = np.abs(rng.normal(0.0, 1e-3, (nk, nj, ni))).astype(F)
+    qa[rng.random(qa.shape) < 0.2] = F(0.0)
+    s = np.abs(rng.normal(0.0, 1e-3, sh["s"])).astype(F)
+    rho0 = (1.1 + 0.05 * rng.standard_normal((nk, nj, ni))).astype(F)
+    gz = (1.0 + 0.02 * rng.standard_normal((nj, ni))).astype(F)
+    rgz = (F(1.0) / gz).astype(F)
+    out.update({"qa": qa, "s": s, "rho0": rho0, "gz": gz, "rgz": rgz})
+    for axis, fshape, spacing in (("x", sh["u"], 250.0), ("y", sh["v"], 300.0), ("z", sh["w"], 200.0)):
+        flux = (3.0 * rng.standard_normal(fshape)).astype(F)
+        # directional tendencies large enough that about a fifth of the candidates go negative
+        directional = (rng.normal(0.0, 1.0, (nk, nj, ni)) * 4e-4).astype(F)
+        candidate = np.zeros((nk, nj, ni), F)
+        out[f"{axis}_flux"] = flux
+        out[f"{axis}_in_directional"] = directional.copy()
+        cpu_pdef_direction_inplace(qa, s, rho0, gz, rgz, flux, directional, candidate, axis=axis, dt=1.5,
+                                   spacing=spacing)
+        out[f"{axis}_out_directional"] = directional
+        out[f"{axis}_out_candidate"] = candidate
+        out[f"{axis}_spacing"] = np.array(spacing)
+    _save("pdef", **out)
+
+
+def kessler_state(rng, ni, nj, nk, *, dz):
+    """A seeded moist column field: some cells supersaturated, cloudy, raining or slightly negative."""
+    z = (np.arange(nk) + 0.5) * dz
+    th0 = (300.0 + 0.004 * z)[:, None, None] * np.ones((nk, nj, ni))
+    pi0 = (1.0 - 9.81 / 1005.7 * z / 300.0)[:, None, None] * np.ones((nk, nj, ni))
+    a = {"th0": th0.astype(F), "pi0": pi0.astype(F)}
+    a["th3d"] = rng.normal(0.0, 1.0, (nk, nj, ni)).astype(F)
+    a["pp3d"] = rng.normal(0.0, 1e-4, (nk, nj, ni)).astype(F)
+    pi = (a["pi0"] + a["pp3d"]).astype(np.float64)
+    prs = 1.0e5 * pi ** (1005.7 / 287.04)
+    t = (a["th0"] + a["th3d"]).astype(np.float64) * pi
+    es = 611.2 * np.exp(17.67 * (t - 273.15) / (t - 29.65))
+    qvs = 0.622 * es / (prs - es)
+    a["qv"] = (qvs * rng.uniform(0.7, 1.06, t.shape)).astype(F)
+    qc = np.where(rng.random(t.shape) < 0.5, rng.uniform(0.0, 2e-3, t.shape), 0.0)
+    qc[rng.random(t.shape) < 0.05] = -1e-6
+    qc[rng.random(t.shape) < 0.05] = 5e-15
+    qr = np.where(rng.random(t.shape) < 0.5, rng.uniform(0.0, 3e-3, t.shape), 0.0)
+    qr[rng.random(t.shape) < 0.05] = -2e-6
+    qr[rng.random(t.shape) < 0.05] = 5e-9
+    a["qc"], a["qr"] = qc.astype(F), qr.astype(F)
+    a["prs"] = prs.astype(F)
+    a["rho"] = (prs / (287.04 * (1.0 + a["qv"] / 0.622) * t)).astype(F)
+    a["rr"] = a["rho"].copy()
+    a["ruh"] = np.ones(ni, F)
+    a["rvh"] = np.ones(nj, F)
+    a["rmh"] = np.ones((nk, nj, ni), F)
+    a["mh"] = np.ones((nk, nj, ni), F)
+    a["fall_velocity"] = np.zeros((nk, nj, ni), F)
+    a["rain"] = np.zeros((1, nj, ni), F)
+    a["prate"] = np.zeros((nj, ni), F)
+    a["asq"] = np.zeros(3, np.float64)
+    a["qbudget"] = np.zeros(10, np.float64)
+    return a
+
+
+def kessler():
+    """cpu_gm1_kessler_driver_inplace (eqtset 1 and 2, one and several fall substeps),
+    cpu_gm1_kessler_large_step_inplace (eqtset 2 with held rates) and cpu_apmasscon_inplace."""
+    _ref()
+    from cm1core.cuda_apmasscon import cpu_apmasscon_inplace, cpu_initial_dry_mass
+    from cm1core.cuda_gm1_kessler_driver import Gm1KesslerDriverSelection, cpu_gm1_kessler_driver_inplace
+    from cm1core.cuda_gm1_kessler_large_step import cpu_gm1_kessler_large_step_inplace
+
+    out = {}
+    ni, nj, nk = 7, 5, 12
+    out["dims"] = np.array([ni, nj, nk], np.int64)
+    cases = ((0, 1.5, 250.0, 2), (1, 20.0, 100.0, 2), (2, 1.5, 250.0, 1))
+    for case, dt, dz, eqt in cases:
+        rng = np.random.default_rng(70 + case)
+        a = kessler_state(rng, ni, nj, nk, dz=dz)
+        out.update({f"d{case}_in_{k}": v.copy() for k, v in a.items()})
+        res = cpu_gm1_kessler_driver_inplace(a, ni=ni, nj=nj, nk=nk, dx=250.0, dy=250.0, dz=dz, dt=dt,
+                                             selection=Gm1KesslerDriverSelection(equation_set=eqt))
+        out.update({f"d{case}_out_{k}": v for k, v in a.items()})
+        out[f"d{case}_iterations"] = res["satadj"]["iterations"]
+        out[f"d{case}_nfall"] = np.array(res["max_fall_substeps"])
+        out[f"d{case}_params"] = np.array([dt, dz, eqt], np.float64)
+    # the large-step transition with held rates from a previous step
+    rng = np.random.default_rng(80)
+    a = kessler_state(rng, ni, nj, nk, dz=250.0)
+    held = {n: rng.normal(0.0, s, (nk, nj, ni)).astype(F)
+            for n, s in (("qpten", 1e-6), ("qtten", 1e-3), ("qvten", 1e-7), ("qcten", 1e-7))}
+    out.update({f"t_in_{k}": v.copy() for k, v in a.items()})
+    out.update({f"t_in_{k}": v.copy() for k, v in held.items()})
+    cpu_gm1_kessler_large_step_inplace(a, held, ni=ni, nj=nj, nk=nk, dx=250.0, dy=250.0, dz=250.0, dt=1.5,
+                                       selection=Gm1KesslerDriverSelection(equation_set=2))
+    out.update({f"t_out_{k}": v for k, v in a.items()})
+    out.update({f"t_out_{k}": v for k, v in held.items()})
+    # dry-mass ledger and correction
+    rng = np.random.default_rng(81)
+    a = kessler_state(rng, ni, nj, nk, dz=250.0)
+    m = {k: a[k] for k in ("rho", "ruh", "rvh", "rmh", "pi0", "th0", "th3d", "qv", "pp3d", "prs")}
+    mass1 = cpu_initial_dry_mass(m, ni=ni, nj=nj, nk=nk) * (1.0 + 1e-5)
+    out.update({f"m_in_{k}": v.copy() for k, v in m.items()})
+    r = cpu_apmasscon_inplace(m, ni=ni, nj=nj, nk=nk, mass1=mass1)
+    out.update({f"m_out_{k}": v for k, v in m.items()})
+    out["m_mass1"] = np.array(mass1)
+    out["m_result"] = np.array([r.mass2_before_f64, r.pressure_sum_before_f64, r.correction_f32, r.mass_after_f64])
+    _save("kessler", **out)
+
+
+GENERATORS = {"advection5": advection5, "momentum5": momentum5, "psolver3": psolver3, "stage": stage, "pdef": pdef,
+              "kessler": kessler}
 
 
 if __name__ == "__main__":
