@@ -55,7 +55,7 @@ def parse_args(argv=None):
     p.add_argument("--nj", type=int, default=320)
     p.add_argument("--nk", type=int, default=80)
     p.add_argument("--kernel", default="k_snd_column", choices=sorted(KERNEL_WORDS))
-    p.add_argument("--cpu-sample", type=int, default=128, help="horizontal size of the oracle's sub-domain sample")
+    p.add_argument("--cpu-sample", type=int, default=96, help="horizontal size of the oracle's sub-domain sample")
     p.add_argument("--cpu-steps", type=int, default=2)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--calibration-copy-mib", type=int, default=0,
@@ -66,9 +66,7 @@ def parse_args(argv=None):
 def workload(ni, nj, nk):
     from gm1_amd.fields import moist_c2
 
-    # the moist configuration's grid, sounding, winds, bubble and boundaries;
-    # this step carries no water species yet
-    return replace(moist_c2(ni, nj, nk), moist=False, name=f"dry-open-{ni}x{nj}x{nk}")
+    return moist_c2(ni, nj, nk)
 
 
 def traffic_from_profiles(kernel: str, label: str):
@@ -88,15 +86,18 @@ def cpu_baseline(cfg, sample: int, steps: int):
     from oracle import large_step as ol
     from oracle import sound as osound
 
+    from oracle import kessler as ok
+
     sub = replace(cfg, ni=sample, nj=sample)
     host = host_initial_state(sub)
     a = host["arrays"]
     prof = osound.terrain_coordinate_profiles(sub.nk, sub.dz)
     c = {"ni": sub.ni, "nj": sub.nj, "nk": sub.nk, "dx": sub.dx, "dy": sub.dy, "dz": sub.dz, "dt": sub.dt,
          "nsound": sub.nsound, "lateral_sides": sub.lateral_sides}
+    mass1 = ok.dry_mass(a["rho"], a["ruh"], a["rvh"], a["rmh"]) if "rho" in a else None
     t0 = time.perf_counter()
     for _ in range(steps):
-        ol.large_step(a, host["metrics"], host["base"], prof, c, zt=np.float32(sub.nk * sub.dz))
+        ol.large_step(a, host["metrics"], host["base"], prof, c, zt=np.float32(sub.nk * sub.dz), mass1=mass1)
     elapsed = time.perf_counter() - t0
     return {
         "value": sub.cells * steps / elapsed,
@@ -185,9 +186,10 @@ def main(argv=None) -> int:
         "dtype": "f32",
         "data": "synthetic warm-bubble fields (analytic sounding, splitmix64 noise)",
         "config": {
-            "workload": f"CM1 large step (WENO5/5th-order advection + psolver=3 acoustic), {label} cells, "
+            "workload": f"CM1 large step, moist Kessler warm bubble (WENO5/5th-order advection, psolver=3 "
+                        f"acoustic, pdef moisture, Kessler ptype 1 eqtset 2, dry-mass correction), {label} cells, "
                         f"dx=dy=dz={cfg.dx:g} m, open-radiative laterals, dt={cfg.dt:g} s, "
-                        f"{cfg.nsound} substeps ({small_steps} small steps/step), dry (no water species yet)",
+                        f"{cfg.nsound} substeps ({small_steps} small steps/step)",
             "cells": cfg.cells,
         },
         "roofline": {

@@ -23,12 +23,15 @@ U = ctypes.c_uint
 Fl = ctypes.c_float
 D = ctypes.c_double
 
-# name -> argument types; every entry point returns int (hipError_t).
+# name -> argument types; entry points return int (hipError_t) unless a
+# query registers another result type.
 SIGNATURES: dict[str, tuple] = {}
+RESTYPES: dict[str, object] = {}
 
 
-def signature(name: str, *argtypes) -> None:
+def signature(name: str, *argtypes, restype=ctypes.c_int) -> None:
     SIGNATURES[name] = argtypes
+    RESTYPES[name] = restype
 
 
 def lib() -> ctypes.CDLL:
@@ -46,7 +49,7 @@ def lib() -> ctypes.CDLL:
             for name, argtypes in SIGNATURES.items():
                 fn = getattr(handle, name)
                 fn.argtypes = list(argtypes)
-                fn.restype = ctypes.c_int
+                fn.restype = RESTYPES.get(name, ctypes.c_int)
             _lib = handle
     return _lib
 
@@ -54,6 +57,11 @@ def lib() -> ctypes.CDLL:
 def call(name: str, *args) -> None:
     fn = getattr(lib(), name)
     check(fn(*args), name)
+
+
+def query(name: str, *args):
+    """Call an entry point that returns a value rather than a status."""
+    return getattr(lib(), name)(*args)
 
 
 def check(code: int, name: str) -> None:

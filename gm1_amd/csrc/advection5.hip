@@ -104,26 +104,38 @@ __global__ __launch_bounds__(256) void k_tmf_divergence(
 // ---------------------------------------------------------------------------
 // scalar face fluxes and tendency (cuda_advection5.py:440-579, :661-669)
 // ---------------------------------------------------------------------------
-template <bool WENO>
+// max |s| over a five-point stencil: the WENO gate of the positive-definite
+// fluxes (flux = 0 where every stencil value is <= weps).
+__device__ __forceinline__ float max5(float a, float b, float c, float d, float e) {
+  return fmaxf(fmaxf(fabsf(a), fabsf(b)), fmaxf(fabsf(c), fmaxf(fabsf(d), fabsf(e))));
+}
+
+template <bool WENO, bool GATE>
+__device__ __forceinline__ float recon_gated(float m, float s1, float s2, float s3, float s4, float s5, double weps) {
+  if (GATE && !((double)max5(s1, s2, s3, s4, s5) > weps)) return 0.0f;
+  return m * recon5(s1, s2, s3, s4, s5, WENO, weps);
+}
+
+template <bool WENO, bool GATE>
 __device__ __forceinline__ float xflux(const float* __restrict__ s, const float* __restrict__ rru,
                                        int k, int y, int x, int nys, int nxs, int nxu, double weps) {
   float m = rru[ix3(k, y, x, nys, nxu)];
   const float* row = s + ix3(k, y, 0, nys, nxs);
-  float r = (m >= 0.0f) ? recon5(row[x - 3], row[x - 2], row[x - 1], row[x], row[x + 1], WENO, weps)
-                        : recon5(row[x + 2], row[x + 1], row[x], row[x - 1], row[x - 2], WENO, weps);
-  return m * r;
+  return (m >= 0.0f) ? recon_gated<WENO, GATE>(m, row[x - 3], row[x - 2], row[x - 1], row[x], row[x + 1], weps)
+                     : recon_gated<WENO, GATE>(m, row[x + 2], row[x + 1], row[x], row[x - 1], row[x - 2], weps);
 }
 
-template <bool WENO>
+template <bool WENO, bool GATE>
 __device__ __forceinline__ float yflux(const float* __restrict__ s, const float* __restrict__ rrv,
                                        int k, int y, int x, int nys, int nxs, int nyv, double weps) {
   float m = rrv[ix3(k, y, x, nyv, nxs)];
   const float* col = s + ix3(k, 0, x, nys, nxs);
   const int64_t st = nxs;
-  float r = (m >= 0.0f)
-                ? recon5(col[(y - 3) * st], col[(y - 2) * st], col[(y - 1) * st], col[y * st], col[(y + 1) * st], WENO, weps)
-                : recon5(col[(y + 2) * st], col[(y + 1) * st], col[y * st], col[(y - 1) * st], col[(y - 2) * st], WENO, weps);
-  return m * r;
+  return (m >= 0.0f)
+             ? recon_gated<WENO, GATE>(m, col[(y - 3) * st], col[(y - 2) * st], col[(y - 1) * st], col[y * st],
+                                       col[(y + 1) * st], weps)
+             : recon_gated<WENO, GATE>(m, col[(y + 2) * st], col[(y + 1) * st], col[y * st], col[(y - 1) * st],
+                                       col[(y - 2) * st], weps);
 }
 
 // z-face flux at face k (2..nk) of one column with the lower-order closures.
@@ -155,6 +167,40 @@ __device__ __forceinline__ float zflux(const float* __restrict__ col, int64_t st
   return m * r;
 }
 
+// z-face flux of the positive-definite chain (cm1_rk_pdef_vertical_flux_v1):
+// the lower closures of vadv_weno5 / vadv_lwr_weno_35(_b) and, with
+// pdefweno, the magnitude gate (a gated face stores +0, never m*0).
+__device__ __forceinline__ float zflux_pdef(const float* __restrict__ col, int64_t st,
+                                            const float* __restrict__ rrw, int64_t wp, float cc1, float cc2,
+                                            int k, int nk, double weps, bool pdefweno) {
+  const float m = rrw[wp];
+  auto S = [&](int level) { return col[(int64_t)level * st]; };
+  const float wf = (float)weps;
+  float r;
+  if (k == 2) {
+    r = (m >= 0.0f) ? cc1 * S(k - 1) + cc2 * S(k) : upstrpd(S(k + 1), S(k), S(k - 1), wf);
+  } else if (k == nk) {
+    r = (m > 0.0f) ? upstrpd(S(k - 2), S(k - 1), S(k), wf) : cc1 * S(k - 1) + cc2 * S(k);
+  } else {
+    const bool low = (k == 3 && (pdefweno ? m > 0.0f : m >= 0.0f)) || (k == nk - 1 && m < 0.0f);
+    if (low) {
+      r = (k == 3) ? upstrpd(S(k - 2), S(k - 1), S(k), wf) : upstrpd(S(k + 1), S(k), S(k - 1), wf);
+    } else {
+      float s1, s2, s3, s4, s5;
+      if (m >= 0.0f) { s1 = S(k - 3); s2 = S(k - 2); s3 = S(k - 1); s4 = S(k); s5 = S(k + 1); }
+      else { s1 = S(k + 2); s2 = S(k + 1); s3 = S(k); s4 = S(k - 1); s5 = S(k - 2); }
+      if (pdefweno && !((double)max5(s1, s2, s3, s4, s5) > weps)) return 0.0f;
+      r = weno5(s1, s2, s3, s4, s5, weps);
+    }
+  }
+  return m * r;
+}
+
+// Output modes of the fused tile kernel.
+constexpr int MODE_TENDENCY = 0;  // tend = advection tendency
+constexpr int MODE_COMPOSE = 1;   // out = a0 + dttmp*((held - divterm) + tendency), |out|<1e-30 -> 0
+constexpr int MODE_PDEF = 2;      // advx/advy/advz = directional tendencies of the pdef chain
+
 struct ScalarAdvArgs {
   const float* s;
   const float* rru;
@@ -169,14 +215,26 @@ struct ScalarAdvArgs {
   float* fx;  // optional flux outputs (nullptr: not written)
   float* fy;
   float* fz;
+  // MODE_COMPOSE
+  const float* a0;
+  const float* held;
+  const float* divterm;  // may be null
+  float* out;
+  float dttmp;
+  // MODE_PDEF
+  float* advx;
+  float* advy;
+  float* advz;
+  int pdefweno;
   int ni, nj, nk, kchunk;
   float rdx, rdy, rdsf;
   double weps;
   unsigned obc;
 };
 
-template <bool WENO>
+template <bool WENO, int MODE>
 __global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
+  constexpr bool GATE = (MODE == MODE_PDEF);
   __shared__ float lfx[TY][TX + 1];
   __shared__ float lfy[TY + 1][TX];
   const int ni = a.ni, nj = a.nj, nk = a.nk;
@@ -197,17 +255,21 @@ __global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
   const int64_t mst = (int64_t)nj * ni;
 
   // z-face flux at the bottom face of the first level of the chunk.
+  auto zf = [&](int kf) {
+    const int64_t wp = (int64_t)kf * st + wcol;
+    const float cc1 = a.c1[(int64_t)(kf - 1) * mst + mcol], cc2 = a.c2[(int64_t)(kf - 1) * mst + mcol];
+    if (MODE == MODE_PDEF) return zflux_pdef(col, st, a.rrw, wp, cc1, cc2, kf, nk, a.weps, a.pdefweno != 0);
+    return zflux<WENO>(col, st, a.rrw, wp, cc1, cc2, kf, nk, a.weps);
+  };
   float fzb = 0.0f;
-  if (live && kb >= 2)
-    fzb = zflux<WENO>(col, st, a.rrw, kb * st + wcol, a.c1[(kb - 1) * mst + mcol],
-                      a.c2[(kb - 1) * mst + mcol], kb, nk, a.weps);
+  if (live && kb >= 2) fzb = zf(kb);
   for (int k = kb; k <= ke; ++k) {
     // x-face fluxes: faces i0..i0+TX of the TY rows.
     for (int slot = threadIdx.x; slot < TY * (TX + 1); slot += NT) {
       const int r = slot / (TX + 1), t = slot % (TX + 1);
       const int f = i0 + t, jj = j0 + r;
       if (f <= ni && jj < nj) {
-        float v = xflux<WENO>(a.s, a.rru, k, jj + NG, f + NG, nys, nxs, nxu, a.weps);
+        float v = xflux<WENO, GATE>(a.s, a.rru, k, jj + NG, f + NG, nys, nxs, nxu, a.weps);
         lfx[r][t] = v;
         if (a.fx && (t < TX || f == ni)) a.fx[ix3(k, jj + NG, f + NG, nys, nxu)] = v;
       }
@@ -217,15 +279,13 @@ __global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
       const int r = slot / TX, t = slot % TX;
       const int ii = i0 + t, g = j0 + r;
       if (ii < ni && g <= nj) {
-        float v = yflux<WENO>(a.s, a.rrv, k, g + NG, ii + NG, nys, nxs, nyv, a.weps);
+        float v = yflux<WENO, GATE>(a.s, a.rrv, k, g + NG, ii + NG, nys, nxs, nyv, a.weps);
         lfy[r][t] = v;
         if (a.fy && (r < TY || g == nj)) a.fy[ix3(k, g + NG, ii + NG, nyv, nxs)] = v;
       }
     }
     float fzt = 0.0f;
-    if (live && k + 1 <= nk)
-      fzt = zflux<WENO>(col, st, a.rrw, (int64_t)(k + 1) * st + wcol, a.c1[(int64_t)k * mst + mcol],
-                        a.c2[(int64_t)k * mst + mcol], k + 1, nk, a.weps);
+    if (live && k + 1 <= nk) fzt = zf(k + 1);
     __syncthreads();
     if (live && a.fz) {
       if (k == 1) a.fz[1 * st + wcol] = 0.0f;
@@ -261,14 +321,217 @@ __global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
       if (k == 1) az = (-fzt) * a.rdsf;
       else if (k == nk) az = fzb * a.rdsf;
       else az = (fzb - fzt) * a.rdsf;
-      float total = (ax + ay) + az;
       const int64_t q = (int64_t)(k - 1) * mst + mcol;
-      total = total + sv * a.divx[q];
-      a.tend[q] = (total * a.rr0[q]) * a.gz[mcol];
+      if (MODE == MODE_PDEF) {
+        a.advx[q] = ax;
+        a.advy[q] = ay;
+        a.advz[q] = az;
+      } else {
+        float total = (ax + ay) + az;
+        total = total + sv * a.divx[q];
+        const float adv = (total * a.rr0[q]) * a.gz[mcol];
+        if (MODE == MODE_TENDENCY) {
+          a.tend[q] = adv;
+        } else {
+          const float seed = a.divterm ? a.held[q] - a.divterm[q] : a.held[q];
+          const float v = a.a0[q] + a.dttmp * (seed + adv);
+          a.out[q] = (fabsf(v) < 1.0e-30f) ? 0.0f : v;
+        }
+      }
     }
     fzb = fzt;
     __syncthreads();
   }
+}
+
+
+// ---------------------------------------------------------------------------
+// positive-definite repair of the final-stage moisture advection
+// (cm1_rk_pdef_candidate_{x,y,z}_v1 / correct_{x,y,z}_v1 / total_v1, pdef.F)
+// fused with the A-anchored compose.  A 64x8 tile marches up a column chunk;
+// per level the x candidates of the tile's rows (+-2 cells, the row extended
+// past the domain by the lateral family) and the y candidates of its columns
+// (+-2 rows) are formed once into LDS; the z candidates of each column slide
+// through a five-level register window (zero beyond the column).
+// ---------------------------------------------------------------------------
+constexpr int RX = 64;
+constexpr int RY = 8;
+constexpr int RT = RX * RY;
+
+struct PdefArgs {
+  const float* s;  // padded scalar (current values)
+  const float* rru; const float* rrv; const float* rrw;
+  const float* rho0; const float* rr0; const float* gz; const float* rgz;
+  const float* divx;
+  const float* qa; const float* held;
+  const float* advx; const float* advy; const float* advz;
+  float* out;
+  int ni, nj, nk, kchunk;
+  float dt, rdt, rdx, rdy, rdz;
+  unsigned obc;
+};
+
+// bc.F's halo cell c (-2..n+1) of a row of n cells as a physical index.
+GM1_DEV int family_cell(int c, int n, unsigned lo, unsigned hi) {
+  if (c < 0) return lo == 1u ? c + n : (lo == 2u ? 0 : -c - 1);
+  if (c >= n) return hi == 1u ? c - n : (hi == 2u ? n - 1 : 2 * n - 1 - c);
+  return c;
+}
+
+GM1_DEV float cand_x(const PdefArgs& a, int k, int j, int i) {
+  const int nxs = sx(a.ni), nys = sy(a.nj), nxu = ux(a.ni);
+  const int64_t q = ((int64_t)(k - 1) * a.nj + j) * a.ni + i;
+  const int x = i + NG, y = j + NG;
+  const float uw = a.rru[ix3(k, y, x, nys, nxu)], ue = a.rru[ix3(k, y, x + 1, nys, nxu)];
+  const float r0a = a.rho0[q] * a.qa[q];
+  if ((i == 0 && side_of(a.obc, 0) == 2u && uw >= 0.0f) || (i + 1 == a.ni && side_of(a.obc, 1) == 2u && ue <= 0.0f))
+    return r0a;
+  const float d = ue - uw;
+  return r0a + (a.dt * (a.advx[q] + (a.s[ix3(k, y, x, nys, nxs)] * d) * a.rdx)) * a.gz[(int64_t)j * a.ni + i];
+}
+
+GM1_DEV float cand_y(const PdefArgs& a, int k, int j, int i) {
+  const int nxs = sx(a.ni), nys = sy(a.nj), nyv = vy(a.nj);
+  const int64_t q = ((int64_t)(k - 1) * a.nj + j) * a.ni + i;
+  const int x = i + NG, y = j + NG;
+  const float vs = a.rrv[ix3(k, y, x, nyv, nxs)], vn = a.rrv[ix3(k, y + 1, x, nyv, nxs)];
+  const float r0a = a.rho0[q] * a.qa[q];
+  if ((j == 0 && side_of(a.obc, 2) == 2u && vs >= 0.0f) || (j + 1 == a.nj && side_of(a.obc, 3) == 2u && vn <= 0.0f))
+    return r0a;
+  const float d = vn - vs;
+  return r0a + (a.dt * (a.advy[q] + (a.s[ix3(k, y, x, nys, nxs)] * d) * a.rdy)) * a.gz[(int64_t)j * a.ni + i];
+}
+
+GM1_DEV float cand_z(const PdefArgs& a, int k, int j, int i) {
+  if (k < 1 || k > a.nk) return 0.0f;
+  const int nxs = sx(a.ni), nys = sy(a.nj);
+  const int64_t q = ((int64_t)(k - 1) * a.nj + j) * a.ni + i;
+  const int x = i + NG, y = j + NG;
+  const float d = a.rrw[ix3(k + 1, y, x, nys, nxs)] - a.rrw[ix3(k, y, x, nys, nxs)];
+  const float r0a = a.rho0[q] * a.qa[q];
+  return r0a + (a.dt * (a.advz[q] + (a.s[ix3(k, y, x, nys, nxs)] * d) * a.rdz)) * a.gz[(int64_t)j * a.ni + i];
+}
+
+// ratio of the donor repair at the middle of three consecutive values
+GM1_DEV float donor_ratio(float lo, float c, float hi) {
+  if (c >= 0.0f) return 0.0f;
+  const float sum = fmaxf(0.0f, lo) + fmaxf(0.0f, hi);
+  return (sum > 1.0e-30f) ? fmaxf(c, -sum) / sum : 0.0f;
+}
+
+// mass change and flag of the centre of five consecutive values v[0..4]
+GM1_DEV void row_mass(const float v[5], float& mass, bool& flag) {
+  mass = 0.0f;
+  flag = false;
+  const float c = v[2];
+  if (c < 0.0f) {
+    const float r = donor_ratio(v[1], c, v[3]);
+    if (r != 0.0f) {
+      const float sum = fmaxf(0.0f, v[1]) + fmaxf(0.0f, v[3]);
+      mass = 0.0f - sum * r;
+      flag = true;
+    }
+  } else if (c > 1.0e-30f) {
+    const float rl = donor_ratio(v[0], v[1], v[2]), rr = donor_ratio(v[2], v[3], v[4]);
+    if (rl != 0.0f) { mass = mass + c * rl; flag = true; }
+    if (rr != 0.0f) { mass = mass + c * rr; flag = true; }
+  }
+}
+
+__global__ __launch_bounds__(RT) void k_pdef_repair(PdefArgs a) {
+  __shared__ float lcx[RY][RX + 4];
+  __shared__ float lcy[RY + 4][RX];
+  const int ni = a.ni, nj = a.nj, nk = a.nk;
+  const int tx = threadIdx.x % RX, ty = threadIdx.x / RX;
+  const int i0 = blockIdx.x * RX, j0 = blockIdx.y * RY;
+  const int i = i0 + tx, j = j0 + ty;
+  const bool live = (i < ni) && (j < nj);
+  const int kb = 1 + blockIdx.z * a.kchunk;
+  const int ke = min(nk, kb + a.kchunk - 1);
+  const unsigned wbc = side_of(a.obc, 0), ebc = side_of(a.obc, 1), sbc = side_of(a.obc, 2), nbc = side_of(a.obc, 3);
+  const int nxs = sx(ni), nys = sy(nj), nxu = ux(ni), nyv = vy(nj);
+  const int x = i + NG, y = j + NG;
+  const int64_t mcol = (int64_t)j * ni + i, mst = (int64_t)nj * ni;
+  float cz[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};  // candidates at k-2..k+2
+  if (live)
+    for (int d = 0; d < 4; ++d) cz[d + 1] = cand_z(a, kb - 2 + d, j, i);
+  for (int k = kb; k <= ke; ++k) {
+    for (int slot = threadIdx.x; slot < RY * (RX + 4); slot += RT) {
+      const int r = slot / (RX + 4), t = slot % (RX + 4);
+      const int jj = j0 + r, c = i0 - 2 + t;
+      if (jj < nj && c <= ni + 1 && c >= -2 && (c < i0 + RX + 2)) lcx[r][t] = cand_x(a, k, jj, family_cell(c, ni, wbc, ebc));
+    }
+    for (int slot = threadIdx.x; slot < (RY + 4) * RX; slot += RT) {
+      const int r = slot / RX, t = slot % RX;
+      const int ii = i0 + t, c = j0 - 2 + r;
+      if (ii < ni && c <= nj + 1 && c >= -2) lcy[r][t] = cand_y(a, k, family_cell(c, nj, sbc, nbc), ii);
+    }
+    for (int d = 0; d < 4; ++d) cz[d] = cz[d + 1];
+    cz[4] = live ? cand_z(a, k + 2, j, i) : 0.0f;
+    __syncthreads();
+    if (live) {
+      const int64_t q = (int64_t)(k - 1) * mst + mcol;
+      const float sv = a.s[ix3(k, y, x, nys, nxs)];
+      const float r0a = a.rho0[q] * a.qa[q];
+      const float rg = a.rgz[mcol];
+      float ax = a.advx[q], ay = a.advy[q], az = a.advz[q];
+      float v[5], mass;
+      bool flag;
+      for (int d = 0; d < 5; ++d) v[d] = lcx[ty][tx + d];
+      row_mass(v, mass, flag);
+      if (flag) {
+        const float du = a.rru[ix3(k, y, x + 1, nys, nxu)] - a.rru[ix3(k, y, x, nys, nxu)];
+        ax = (((v[2] + mass) - r0a) * a.rdt) * rg - (sv * du) * a.rdx;
+      }
+      for (int d = 0; d < 5; ++d) v[d] = lcy[ty + d][tx];
+      row_mass(v, mass, flag);
+      if (flag) {
+        const float dv = a.rrv[ix3(k, y + 1, x, nyv, nxs)] - a.rrv[ix3(k, y, x, nyv, nxs)];
+        ay = (((v[2] + mass) - r0a) * a.rdt) * rg - (sv * dv) * a.rdy;
+      }
+      row_mass(cz, mass, flag);
+      if (flag) {
+        const float dw = a.rrw[ix3(k + 1, y, x, nys, nxs)] - a.rrw[ix3(k, y, x, nys, nxs)];
+        az = (((cz[2] + mass) - r0a) * a.rdt) * rg - (sv * dw) * a.rdz;
+      }
+      float t = (ax + ay) + az;
+      t = t + sv * a.divx[q];
+      const float adv = (t * a.rr0[q]) * a.gz[mcol];
+      const float o = a.qa[q] + a.dt * (a.held[q] + adv);
+      a.out[q] = (fabsf(o) < 1.0e-30f) ? 0.0f : o;
+    }
+    __syncthreads();
+  }
+}
+
+int chunks_for(int tiles, int nk, int* kchunk) {
+  // enough column chunks to put >= 2048 tiles in flight on 256 CUs
+  int chunks = (2048 + tiles - 1) / tiles;
+  if (chunks > nk) chunks = nk;
+  if (chunks < 1) chunks = 1;
+  *kchunk = (nk + chunks - 1) / chunks;
+  return (nk + *kchunk - 1) / *kchunk;
+}
+
+int launch_scalar_tile(ScalarAdvArgs& a, int use_weno, int mode, hipStream_t s) {
+  const int tiles = ((a.ni + TX - 1) / TX) * ((a.nj + TY - 1) / TY);
+  const int chunks = chunks_for(tiles, a.nk, &a.kchunk);
+  dim3 grid((a.ni + TX - 1) / TX, (a.nj + TY - 1) / TY, chunks);
+  if (mode == MODE_TENDENCY) {
+    if (use_weno)
+      GM1_TIMED(TK_SCALAR_ADV5, s, hipLaunchKernelGGL((k_scalar_adv5<true, MODE_TENDENCY>), grid, dim3(NT), 0, s, a));
+    else
+      GM1_TIMED(TK_SCALAR_ADV5, s, hipLaunchKernelGGL((k_scalar_adv5<false, MODE_TENDENCY>), grid, dim3(NT), 0, s, a));
+  } else if (mode == MODE_COMPOSE) {
+    if (use_weno)
+      GM1_TIMED(TK_SCALAR_ADV5, s, hipLaunchKernelGGL((k_scalar_adv5<true, MODE_COMPOSE>), grid, dim3(NT), 0, s, a));
+    else
+      GM1_TIMED(TK_SCALAR_ADV5, s, hipLaunchKernelGGL((k_scalar_adv5<false, MODE_COMPOSE>), grid, dim3(NT), 0, s, a));
+  } else {
+    GM1_TIMED(TK_SCALAR_ADV5, s, hipLaunchKernelGGL((k_scalar_adv5<true, MODE_PDEF>), grid, dim3(NT), 0, s, a));
+  }
+  GM1_LAUNCH_CHECK();
+  return 0;
 }
 
 }  // namespace
@@ -301,24 +564,60 @@ int gm1hip_scalar_advection5(const float* scalar, const float* rru, const float*
                              float* tendency, float* flux_x, float* flux_y, float* flux_z,
                              int ni, int nj, int nk, int use_weno, double weps, float dx,
                              float dy, float dz, unsigned obc, void* stream) {
-  ScalarAdvArgs a;
+  ScalarAdvArgs a{};
   a.s = scalar; a.rru = rru; a.rrv = rrv; a.rrw = rrw; a.c1 = c1; a.c2 = c2;
   a.divx = divx; a.rr0 = rr0; a.gz = gz; a.tend = tendency;
   a.fx = flux_x; a.fy = flux_y; a.fz = flux_z;
   a.ni = ni; a.nj = nj; a.nk = nk;
   a.rdx = 1.0f / dx; a.rdy = 1.0f / dy; a.rdsf = 1.0f / dz;
   a.weps = weps; a.obc = obc;
-  const int tiles = ((ni + TX - 1) / TX) * ((nj + TY - 1) / TY);
-  // enough column chunks to put >= 2048 tiles in flight on 256 CUs
-  int chunks = (2048 + tiles - 1) / tiles;
-  if (chunks > nk) chunks = nk;
-  if (chunks < 1) chunks = 1;
-  a.kchunk = (nk + chunks - 1) / chunks;
-  chunks = (nk + a.kchunk - 1) / a.kchunk;
-  dim3 grid((ni + TX - 1) / TX, (nj + TY - 1) / TY, chunks);
+  return launch_scalar_tile(a, use_weno, MODE_TENDENCY, (hipStream_t)stream);
+}
+
+int gm1hip_scalar_rk_stage(const float* scalar, const float* rru, const float* rrv, const float* rrw,
+                           const float* c1, const float* c2, const float* rr0, const float* gz, const float* divx,
+                           const float* a0, const float* held, const float* divterm, float* out, int ni, int nj,
+                           int nk, int use_weno, double weps, float dx, float dy, float dz, unsigned obc,
+                           float dttmp, void* stream) {
+  if (obc == 0u) return (int)hipErrorInvalidValue;
+  ScalarAdvArgs a{};
+  a.s = scalar; a.rru = rru; a.rrv = rrv; a.rrw = rrw; a.c1 = c1; a.c2 = c2;
+  a.divx = divx; a.rr0 = rr0; a.gz = gz;
+  a.a0 = a0; a.held = held; a.divterm = divterm; a.out = out; a.dttmp = dttmp;
+  a.ni = ni; a.nj = nj; a.nk = nk;
+  a.rdx = 1.0f / dx; a.rdy = 1.0f / dy; a.rdsf = 1.0f / dz;
+  a.weps = weps; a.obc = obc;
+  return launch_scalar_tile(a, use_weno, MODE_COMPOSE, (hipStream_t)stream);
+}
+
+int gm1hip_pdef_moisture_stage(const float* scalar, const float* rru, const float* rrv, const float* rrw,
+                               const float* c1, const float* c2, const float* rho0_3d, const float* rr0,
+                               const float* gz, const float* rgz, const float* divx, const float* qa,
+                               const float* held, float* out, float* advx, float* advy, float* advz, int ni,
+                               int nj, int nk, int pdefweno, double weps, float dx, float dy, float dz,
+                               unsigned obc, float dttmp, void* stream) {
+  if (obc == 0u || nk < 4) return (int)hipErrorInvalidValue;
   hipStream_t s = (hipStream_t)stream;
-  if (use_weno) GM1_TIMED(TK_SCALAR_ADV5, s, hipLaunchKernelGGL(k_scalar_adv5<true>, grid, dim3(NT), 0, s, a));
-  else GM1_TIMED(TK_SCALAR_ADV5, s, hipLaunchKernelGGL(k_scalar_adv5<false>, grid, dim3(NT), 0, s, a));
+  ScalarAdvArgs a{};
+  a.s = scalar; a.rru = rru; a.rrv = rrv; a.rrw = rrw; a.c1 = c1; a.c2 = c2;
+  a.divx = divx; a.rr0 = rr0; a.gz = gz;
+  a.advx = advx; a.advy = advy; a.advz = advz; a.pdefweno = pdefweno;
+  a.ni = ni; a.nj = nj; a.nk = nk;
+  a.rdx = 1.0f / dx; a.rdy = 1.0f / dy; a.rdsf = 1.0f / dz;
+  a.weps = weps; a.obc = obc;
+  int rc = launch_scalar_tile(a, 1, MODE_PDEF, s);
+  if (rc) return rc;
+  PdefArgs p{};
+  p.s = scalar; p.rru = rru; p.rrv = rrv; p.rrw = rrw;
+  p.rho0 = rho0_3d; p.rr0 = rr0; p.gz = gz; p.rgz = rgz; p.divx = divx;
+  p.qa = qa; p.held = held; p.advx = advx; p.advy = advy; p.advz = advz; p.out = out;
+  p.ni = ni; p.nj = nj; p.nk = nk;
+  p.dt = dttmp; p.rdt = 1.0f / dttmp; p.rdx = 1.0f / dx; p.rdy = 1.0f / dy; p.rdz = 1.0f / dz;
+  p.obc = obc;
+  const int tiles = ((ni + RX - 1) / RX) * ((nj + RY - 1) / RY);
+  const int chunks = chunks_for(tiles, nk, &p.kchunk);
+  dim3 grid((ni + RX - 1) / RX, (nj + RY - 1) / RY, chunks);
+  hipLaunchKernelGGL(k_pdef_repair, grid, dim3(RT), 0, s, p);
   GM1_LAUNCH_CHECK();
   return 0;
 }

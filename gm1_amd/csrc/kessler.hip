@@ -1,0 +1,603 @@
+// Kessler warm rain (CM1 ptype=1, efall=0) and its large-step transition on gfx950.
+//
+// Replaces cuda_gm1_kessler_driver.py / cuda_gm1_kessler_local.py /
+// cuda_gm1_kessler_large_step.py / cuda_apmasscon.py.  Design:
+//   * k_kes_column: ONE thread per column does everything that is serial in
+//     k: pdefq of qv, qc, qr (binary64 column sums in k order, so the column
+//     rescale and its budget are the reference's words), the fall speed, and
+//     the flux-form fallout in nfall substeps with a carried flux.  (The
+//     reference runs pre-pdefq and fallout one thread per j-ROW.)
+//   * k_kes_cell: ONE thread per cell for autoconversion, accretion, rain
+//     evaporation and the saturation adjustment (eqtset 2 iterates to
+//     |dT| <= 2e-4 K, at most 100 times); the eqtset=2 held recapture of the
+//     large-step transition is fused in.  Per-wave binary64 partial budgets
+//     are formed with a fixed shuffle tree.
+//   * k_kes_budgets: one block folds the column and wave partials into asq
+//     and qbudget in a fixed order (rows in i order, then j order, as the
+//     reference; levels in k order).
+// Transcendentals are binary64 exp/pow rounded once to binary32.  Budgets and
+// status words stay on the device; nothing synchronises with the host.
+#include "cm1_common.h"
+
+using namespace gm1;
+
+namespace {
+
+constexpr float RD = 287.04f, CP = 1005.7f, CPV = 1870.0f, RV = 461.5f, XLV = 2501000.0f, CPL = 4190.0f;
+
+struct KesConst {
+  float cv, cvv, rovcp, eps, reps, lv2, lv1, rp00, p00, cpdcv, rddcv, cpdrd;
+};
+
+GM1_DEV KesConst kes_const() {
+  KesConst c;
+  c.cv = CP - RD;
+  c.cvv = CPV - RV;
+  c.rovcp = RD / CP;
+  c.eps = RD / RV;
+  c.reps = RV / RD;
+  c.lv2 = CPL - CPV;
+  c.lv1 = XLV + c.lv2 * 273.15f;
+  c.rp00 = 1.0f / 1.0e5f;
+  c.p00 = 1.0e5f;
+  c.cpdcv = CP / c.cv;
+  c.rddcv = RD / c.cv;
+  c.cpdrd = CP / RD;
+  return c;
+}
+
+GM1_DEV float exp_r(float x) { return (float)exp((double)x); }
+GM1_DEV float pow_r(float b, float e) { return (float)pow((double)b, (double)e); }
+
+GM1_DEV float sat_qv(const KesConst& c, float t, float prs_cap, float pden) {
+  float esl = 611.2f * exp_r((17.67f * (t - 273.15f)) / (t - 29.65f));
+  esl = fminf(esl, prs_cap * 0.5f);
+  return (c.eps * esl) / (pden - esl);
+}
+
+struct KesArgs {
+  const float* pi0; const float* th0; const float* ruh; const float* rvh; const float* rmh; const float* mh;
+  float* rho; float* rr; float* pp3d; float* th3d; float* prs; float* qv; float* qc; float* qr; float* vt;
+  float* rain; float* prate;
+  double* colbud;   // [5][nj][ni]: pdefq qv, qc, qr, fallout, surface energy
+  double* wavebud;  // [5][nk][nj][nseg]: autoconversion, accretion, evaporation, condensation, cloud evaporation
+  int* wavestat;    // [nk][nj][nseg][2]: max satadj iterations, failures
+  // held rates of the large-step transition (eqtset 2; null when not recapturing)
+  float* qpten; float* qtten; float* qvten; float* qcten;
+  int ni, nj, nk, nseg, eqtset;
+  float dx, dy, dz, dt;
+};
+
+// ------------------------------------------------------------------ column phase
+__global__ __launch_bounds__(64) void k_kes_column(KesArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  const int ni = a.ni, nj = a.nj, nk = a.nk;
+  if (i >= ni) return;
+  const KesConst c = kes_const();
+  const int64_t plane = (int64_t)nj * ni, h = (int64_t)j * ni + i;
+  const float ru = a.ruh[i], rv = a.rvh[j];
+  float* qs[3] = {a.qv, a.qc, a.qr};
+  const float rmax[3] = {0.0f, 1.0e-14f, 1.0e-14f};
+  // temperature at the lowest level before any phase (pdefq leaves theta, pi alone)
+  const float t0 = (a.th0[h] + a.th3d[h]) * (a.pi0[h] + a.pp3d[h]);
+  for (int sp = 0; sp < 3; ++sp) {
+    float* q = qs[sp];
+    double t1 = 0.0, t2 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int k = 0; k < nk; ++k) {
+      const int64_t p = k * plane + h;
+      const float r = a.rho[p];
+      float v = q[p];
+      t1 += (double)(r * v);
+      a1 += (double)((((r * v) * ru) * rv) * a.rmh[p]);
+      if (v < rmax[sp]) {
+        v = 0.0f;
+        q[p] = v;
+      }
+      t2 += (double)(r * v);
+    }
+    double t3 = (t1 + 1.0e-20) / (t2 + 1.0e-20);
+    if (t3 < 0.0) t3 = 1.0;
+    for (int k = 0; k < nk; ++k) {
+      const int64_t p = k * plane + h;
+      const float v = (float)(t3 * (double)q[p]);
+      q[p] = v;
+      a2 += (double)((((a.rho[p] * v) * ru) * rv) * a.rmh[p]);
+    }
+    a.colbud[(int64_t)sp * plane + h] = a2 - a1;
+  }
+  // fall speed
+  for (int k = 0; k < nk; ++k) {
+    const int64_t p = k * plane + h;
+    const float r = a.rho[p], q = a.qr[p];
+    float v = 0.0f;
+    if (q > 1.0e-12f) v = (14.34f * pow_r(r * q, 0.1346f)) * sqrtf(1.15f / r);
+    a.vt[p] = v;
+  }
+  {  // surface energy flux of falling rain
+    float term = a.dt * a.vt[h];
+    term = term * a.rr[h];
+    term = term * a.qr[h];
+    term = term * (CPL * t0 - c.lv1);
+    term = term * ru;
+    term = term * rv;
+    a.colbud[4 * plane + h] = (double)term;
+  }
+  // fallout
+  const float rdz = 1.0f / a.dz;
+  float crmax = 0.0f;
+  for (int k = 0; k < nk; ++k) {
+    const int64_t p = k * plane + h;
+    crmax = fmaxf(crmax, ((a.vt[p] * a.dt) * rdz) * a.mh[p]);
+  }
+  int nfall = (int)(crmax + 1.0f);
+  if (nfall < 1) nfall = 1;
+  for (int k = nk - 2; k >= 0; --k) {
+    const int64_t p = k * plane + h;
+    if (a.qr[p] < 1.0e-8f) a.vt[p] = a.vt[p + plane];
+  }
+  const float dtfall = a.dt / (float)nfall;
+  double bud = 0.0;
+  for (int sub = 0; sub < nfall; ++sub) {
+    float rqk = (a.rho[h] * a.vt[h]) * fmaxf(0.0f, a.qr[h]);
+    const float rq0 = rqk;
+    for (int k = 0; k < nk; ++k) {
+      const int64_t p = k * plane + h;
+      const float rqk1 = (k + 1 < nk) ? (a.rho[p + plane] * a.vt[p + plane]) * fmaxf(0.0f, a.qr[p + plane]) : 0.0f;
+      const float delta = (((dtfall * (rqk1 - rqk)) * rdz) * a.mh[p]) / a.rho[p];
+      a.qr[p] = a.qr[p] + delta;
+      rqk = rqk1;
+    }
+    a.rain[h] = a.rain[h] + (0.1f * dtfall) * rq0;
+    a.prate[h] = rq0;
+    float term = dtfall * a.rr[h];
+    term = term * a.vt[h];
+    term = term * fmaxf(0.0f, a.qr[h]);
+    term = term * ru;
+    term = term * rv;
+    bud += (double)term;
+  }
+  a.colbud[3 * plane + h] = bud;
+}
+
+// ------------------------------------------------------------------ cell phase
+GM1_DEV double wave_sum(double v) {
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+GM1_DEV int wave_max(int v) {
+  for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+__global__ __launch_bounds__(64) void k_kes_cell(KesArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  const int ni = a.ni, nj = a.nj;
+  const bool live = i < ni;
+  double bud[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  int iters = 0, fail = 0;
+  if (live) {
+    const KesConst c = kes_const();
+    const int64_t p = ((int64_t)k * nj + j) * ni + i;
+    const float pi0 = a.pi0[p], th0 = a.th0[p];
+    float pp = a.pp3d[p], th = a.th3d[p], prs = a.prs[p], rho = a.rho[p];
+    const float rr = a.rr[p];
+    float qv = a.qv[p], qc = a.qc[p], qr = a.qr[p];
+    const float dt = a.dt;
+    // ---- local conversion (cuda_gm1_kessler_local.py:290-416)
+    float pisum = pi0 + pp;
+    const float tmp = (th0 + th) * pisum;
+    float ar = 0.0f, cr = 0.0f, er = 0.0f;
+    if (qc > 0.0f) {
+      ar = fmaxf(0.001f * (qc - 0.001f), 0.0f);
+      ar = ar * dt;
+    }
+    if (qc > 0.0f && qr > 0.0f) cr = ((2.2f * qc) * pow_r(qr, 0.875f)) * dt;
+    if (qr > 0.0f) {
+      const float qvs = sat_qv(c, tmp, prs, prs);
+      if (qv < qvs) {
+        const float rhoqr = rho * qr;
+        const float vent = 1.6f + 30.3922f * pow_r(rhoqr, 0.2046f);
+        const float deficit = 1.0f - qv / qvs;
+        const float num = (vent * deficit) * pow_r(rhoqr, 0.525f);
+        const float den = (2.03e4f + 9.584e6f / (qvs * prs)) * rho;
+        er = fminf((num / den) * dt, qr);
+        if (qv + er > qvs) er = qvs - qv;
+      }
+    }
+    if (ar + cr > qc) {
+      const float term1 = ar + cr;
+      ar = (qc * ar) / term1;
+      cr = (qc * cr) / term1;
+    }
+    qv = qv + er;
+    qc = qc - (ar + cr);
+    qr = qr + ((ar + cr) - er);
+    if (er > 1.0e-7f) {
+      if (a.eqtset == 2) {
+        const float cpml = (CP + CPV * qv) + CPL * (qc + qr);
+        const float cvml = (c.cv + c.cvv * qv) + CPL * (qc + qr);
+        const float rm = RD + RV * qv;
+        const float latent = c.lv1 - c.lv2 * tmp;
+        const float first = latent / ((c.cpdcv * cvml) * pisum);
+        const float second = ((th0 + th) * (RV / cvml)) * (1.0f - (c.rovcp * cpml) / rm);
+        th = th - er * (first - second);
+        const float base = ((rho * rm) * (th0 + th)) * c.rp00;
+        pp = pow_r(base, c.rddcv) - pi0;
+        pisum = pi0 + pp;
+        prs = c.p00 * pow_r(pisum, c.cpdrd);
+      } else {
+        const float latent = c.lv1 - c.lv2 * tmp;
+        th = th - er * (latent / (CP * pisum));
+        const float den = ((RD * (th0 + th)) * pisum) * (1.0f + qv * c.reps);
+        rho = prs / den;
+      }
+    }
+    const float tem = (a.ruh[i] * a.rvh[j]) * a.rmh[p];
+    bud[0] = (double)((rr * ar) * tem);
+    bud[1] = (double)((rr * cr) * tem);
+    bud[2] = (double)((rr * er) * tem);
+    // ---- saturation adjustment (cuda_gm1_kessler_driver.py:337-428)
+    const float oqv = qv, oqc = qc;
+    float qvn = qv, qcn = qc;
+    float tnew = (th0 + th) * (pi0 + pp);
+    float qvs = sat_qv(c, tnew, prs, prs);
+    if (a.eqtset == 2) {
+      if (oqc > 1.0e-12f || oqv > qvs) {
+        const float ql = fmaxf(oqc + qr, 0.0f);
+        const float cvml = (c.cv + c.cvv * qvn) + CPL * ql;
+        const float lhv = c.lv1 - c.lv2 * tnew;
+        const float t1 = (lhv - RV * tnew) / cvml;
+        const float d1 = (t1 * 17.67f) * 243.5f;
+        const float converge = 2.0f * 0.0001f;
+        float tlast = tnew, pnew = prs;
+        int n = 0;
+        while (true) {
+          n += 1;
+          const float sq = (tnew - 29.65f) * (tnew - 29.65f);
+          const float denom = 1.0f + (d1 * qvs) / sq;
+          float dqv = (qvs - qvn) / denom;
+          dqv = fminf(dqv, qcn);
+          if (qvn + dqv < 1.0e-20f) dqv = 1.0e-20f - qvn;
+          qvn = qvn + dqv;
+          qcn = qcn - dqv;
+          tnew = tnew - dqv * t1;
+          pnew = (rho * (RD + RV * qvn)) * tnew;
+          const bool conv = fabs((double)(tnew - tlast)) <= (double)converge;
+          if (!conv) {
+            tlast = tnew;
+            qvs = sat_qv(c, tnew, prs, pnew);
+          }
+          if (n == 100) {
+            fail = conv ? 0 : 1;
+            break;
+          }
+          if (conv) break;
+        }
+        iters = n;
+        prs = pnew;
+        pp = pow_r(pnew * c.rp00, c.rovcp) - pi0;
+        th = tnew / (pi0 + pp) - th0;
+        qv = qvn;
+        qc = qcn;
+      }
+    } else {
+      const float lhv = c.lv1 - c.lv2 * tnew;
+      const float sq = (tnew - 29.65f) * (tnew - 29.65f);
+      const float num = ((lhv * qvs) * 17.67f) * 243.5f;
+      float dqv = (qvs - qvn) / (1.0f + num / (CP * sq));
+      if (qvn + dqv < 1.0e-20f) dqv = 1.0e-20f - qvn;
+      dqv = fminf(dqv, fmaxf(0.0f, qcn));
+      qvn = qvn + dqv;
+      qcn = qcn - dqv;
+      iters = 1;
+      th = th - dqv * (lhv / (CP * (pi0 + pp)));
+      qv = qvn;
+      qc = qcn;
+      const float den = ((RD * (th0 + th)) * (pi0 + pp)) * (1.0f + qvn * c.reps);
+      rho = prs / den;
+    }
+    const float cond = fmaxf(qcn - oqc, 0.0f), evac = fmaxf(qvn - oqv, 0.0f);
+    bud[3] = (double)((rr * cond) * tem);
+    bud[4] = (double)((rr * evac) * tem);
+    a.pp3d[p] = pp;
+    a.th3d[p] = th;
+    a.prs[p] = prs;
+    a.rho[p] = rho;
+    a.qv[p] = qv;
+    a.qc[p] = qc;
+    a.qr[p] = qr;
+    if (a.qpten) {  // eqtset=2 held recapture (cuda_gm1_kessler_large_step.py:281-285)
+      a.qpten[p] = (pp - a.qpten[p]) / dt;
+      a.qtten[p] = (th - a.qtten[p]) / dt;
+      a.qvten[p] = (qv - a.qvten[p]) / dt;
+      a.qcten[p] = (qc - a.qcten[p]) / dt;
+    }
+  }
+  // per-wave budget partials and status (fixed shuffle tree: deterministic)
+  const int64_t slot = ((int64_t)k * nj + j) * a.nseg + blockIdx.x;
+  const int64_t stride = (int64_t)a.nk * nj * a.nseg;
+  for (int b = 0; b < 5; ++b) {
+    const double v = wave_sum(bud[b]);
+    if (threadIdx.x == 0) a.wavebud[b * stride + slot] = v;
+  }
+  const int mi = wave_max(iters), mf = wave_max(fail);
+  if (threadIdx.x == 0) {
+    a.wavestat[2 * slot] = mi;
+    a.wavestat[2 * slot + 1] = mf;
+  }
+}
+
+// ------------------------------------------------------------------ budgets
+// One block: rows of the column budgets (i order), then rows in j order into
+// asq/qbudget; wave partials per level in (j, segment) order, then levels in
+// k order; status maxima.
+__global__ __launch_bounds__(256) void k_kes_budgets(const double* __restrict__ colbud,
+                                                     const double* __restrict__ wavebud,
+                                                     const int* __restrict__ wavestat, double* __restrict__ rowscr,
+                                                     double* __restrict__ asq, double* __restrict__ qbudget,
+                                                     int* __restrict__ status, int ni, int nj, int nk, int nseg,
+                                                     float dx, float dy, float dz) {
+  __shared__ int smax[256];
+  __shared__ int sfail[256];
+  const int t = threadIdx.x;
+  const int64_t plane = (int64_t)nj * ni;
+  {  // status: strided partial maxima, then a tree in LDS
+    int mi = 0, mf = 0;
+    const int64_t n = (int64_t)nk * nj * nseg;
+    for (int64_t q = t; q < n; q += blockDim.x) {
+      mi = max(mi, wavestat[2 * q]);
+      mf += wavestat[2 * q + 1];
+    }
+    smax[t] = mi;
+    sfail[t] = mf;
+  }
+  // column budgets: 5 x nj row sums in i order
+  for (int r = t; r < 5 * nj; r += blockDim.x) {
+    const int b = r / nj, j = r % nj;
+    double s = 0.0;
+    for (int i = 0; i < ni; ++i) s += colbud[b * plane + (int64_t)j * ni + i];
+    rowscr[r] = s;
+  }
+  // level budgets: 5 x nk sums over (j, segment)
+  const int64_t lstride = (int64_t)nj * nseg;
+  for (int r = t; r < 5 * nk; r += blockDim.x) {
+    const int b = r / nk, k = r % nk;
+    double s = 0.0;
+    const double* w = wavebud + (int64_t)b * nk * lstride + (int64_t)k * lstride;
+    for (int64_t q = 0; q < lstride; ++q) s += w[q];
+    rowscr[5 * nj + r] = s;
+  }
+  __syncthreads();
+  const double volume = (double)((dx * dy) * dz);
+  const double area = (double)(dx * dy);
+  if (t < 3) {  // pdefq asq[t]
+    double acc = asq[t];
+    for (int j = 0; j < nj; ++j) acc += rowscr[t * nj + j] * volume;
+    asq[t] = acc;
+  } else if (t == 3) {  // fallout -> qbudget[5]
+    double acc = qbudget[5];
+    for (int j = 0; j < nj; ++j) acc += rowscr[3 * nj + j] * area;
+    qbudget[5] = acc;
+  } else if (t == 4) {  // surface energy -> qbudget[6]
+    double acc = qbudget[6];
+    for (int j = 0; j < nj; ++j) acc += (rowscr[4 * nj + j] * (double)dx) * (double)dy;
+    qbudget[6] = acc;
+  } else if (t >= 8 && t < 13) {  // level budgets -> qbudget[2,3,4,0,1]
+    const int b = t - 8;
+    const int slot = (b < 3) ? 2 + b : b - 3;
+    double acc = qbudget[slot];
+    for (int k = 0; k < nk; ++k) acc = acc + rowscr[5 * nj + b * nk + k] * volume;
+    qbudget[slot] = acc;
+  } else if (t == 16) {
+    int mi = 0, mf = 0;
+    for (int q = 0; q < (int)blockDim.x; ++q) {
+      mi = max(mi, smax[q]);
+      mf += sfail[q];
+    }
+    status[0] = max(status[0], mi);
+    status[1] = status[1] + mf;
+  }
+}
+
+// ------------------------------------------------------------------ transition
+// eqtset=2 held remove/snapshot, then the pre-physics EOS; pp3d is taken from
+// the dynamics' pressure family (one-cell halo) into the mass array.
+__global__ __launch_bounds__(256) void k_kes_transition_pre(
+    const float* __restrict__ pp_family, float* __restrict__ pp3d, float* __restrict__ th3d, float* __restrict__ qv,
+    float* __restrict__ qc, float* __restrict__ qpten, float* __restrict__ qtten, float* __restrict__ qvten,
+    float* __restrict__ qcten, const float* __restrict__ pi0, const float* __restrict__ th0, float* __restrict__ prs,
+    float* __restrict__ rho, float* __restrict__ rr, int ni, int nj, int nk, float dt, int eqtset, int copy_rr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i >= ni) return;
+  const int64_t p = ((int64_t)k * nj + j) * ni + i;
+  float pp = pp_family ? pp_family[pix(k + 1, j + 1, i + 1, ni, nj)] : pp3d[p];
+  float th = th3d[p], v = qv[p], cc = qc[p];
+  if (eqtset == 2) {
+    pp = pp - qpten[p] * dt;
+    qpten[p] = pp;
+    th = th - qtten[p] * dt;
+    qtten[p] = th;
+    v = v - qvten[p] * dt;
+    qvten[p] = v;
+    cc = cc - qcten[p] * dt;
+    qcten[p] = cc;
+  }
+  pp3d[p] = pp;
+  th3d[p] = th;
+  qv[p] = v;
+  qc[p] = cc;
+  const float work = pi0[p] + pp;
+  const float pr = pow_r(work, CP / RD) * 1.0e5f;
+  prs[p] = pr;
+  float r = (th0[p] + th) * work;
+  const float gas = fmaxf(v, 0.0f) * RV + RD;
+  r = pr / (r * gas);
+  rho[p] = r;
+  if (copy_rr) rr[p] = r;
+}
+
+// dry-air mass ledger (cuda_apmasscon.py): per-wave partials of
+// rho*ruh*rvh*rmh and pi0+pp3d, a fixed-order fold, the Exner correction.
+__global__ __launch_bounds__(64) void k_mass_partials(const float* __restrict__ rho, const float* __restrict__ ruh,
+                                                      const float* __restrict__ rvh, const float* __restrict__ rmh,
+                                                      const float* __restrict__ pi0, const float* __restrict__ pp3d,
+                                                      double* __restrict__ part, int ni, int nj, int nk, int nseg,
+                                                      int with_pressure) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  double d = 0.0, pr = 0.0;
+  if (i < ni) {
+    const int64_t p = ((int64_t)k * nj + j) * ni + i;
+    d = (double)(((rho[p] * ruh[i]) * rvh[j]) * rmh[p]);
+    if (with_pressure) pr = (double)(pi0[p] + pp3d[p]);
+  }
+  d = wave_sum(d);
+  pr = wave_sum(pr);
+  const int64_t slot = ((int64_t)k * nj + j) * nseg + blockIdx.x;
+  if (threadIdx.x == 0) {
+    part[2 * slot] = d;
+    part[2 * slot + 1] = pr;
+  }
+}
+
+// ledger[0] = dry mass (levels in k order); with mass1 > 0 also the f32
+// Exner correction into corr[0].
+__global__ __launch_bounds__(256) void k_mass_fold(const double* __restrict__ part, double* __restrict__ levels,
+                                                   double* __restrict__ ledger, float* __restrict__ corr, int nk,
+                                                   int nj, int nseg, double mass1, int ncells) {
+  const int t = threadIdx.x;
+  const int64_t lstride = (int64_t)nj * nseg;
+  for (int r = t; r < 2 * nk; r += blockDim.x) {
+    const int which = r / nk, k = r % nk;
+    double s = 0.0;
+    for (int64_t q = 0; q < lstride; ++q) s += part[2 * ((int64_t)k * lstride + q) + which];
+    levels[r] = s;
+  }
+  __syncthreads();
+  if (t == 0) {
+    double m2 = 0.0, ps = 0.0;
+    for (int k = 0; k < nk; ++k) m2 += levels[k];
+    for (int k = 0; k < nk; ++k) ps += levels[nk + k];
+    ledger[0] = m2;
+    ledger[1] = ps;
+    if (mass1 > 0.0) {
+      const double e = (double)(RD / (CP - RD));
+      double c = pow(mass1 / m2, e) - 1.0;
+      c = c * ps;
+      c = c / (double)ncells;
+      corr[0] = (float)c;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mass_apply(const float* __restrict__ corr, float* __restrict__ pp3d,
+                                                    const float* __restrict__ pi0, const float* __restrict__ th0,
+                                                    const float* __restrict__ th3d, const float* __restrict__ qv,
+                                                    float* __restrict__ prs, float* __restrict__ rho,
+                                                    float* __restrict__ rr, int64_t n) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const float pp = pp3d[p] + corr[0];
+  pp3d[p] = pp;
+  const float pi = pi0[p] + pp;
+  const float pr = 1.0e5f * pow_r(pi, CP / RD);
+  const float theta = th0[p] + th3d[p];
+  const float gas = RD + fmaxf(0.0f, qv[p]) * RV;
+  const float r = pr / ((theta * pi) * gas);
+  prs[p] = pr;
+  rho[p] = r;
+  if (rr) rr[p] = r;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Scratch the driver needs (bytes), for ni x nj x nk.
+int64_t gm1hip_kessler_scratch_bytes(int ni, int nj, int nk) {
+  const int64_t nseg = (ni + 63) / 64;
+  const int64_t col = 5LL * nj * ni * 8;
+  const int64_t wave = 5LL * nk * nj * nseg * 8;
+  const int64_t stat = 2LL * nk * nj * nseg * 4;
+  const int64_t rows = (5LL * nj + 5LL * nk) * 8;
+  return col + wave + stat + rows + 4 * 256;
+}
+
+int gm1hip_kessler_driver(const float* ruh, const float* rvh, const float* rmh, const float* mh, const float* pi0,
+                          const float* th0, float* rho, float* rr, float* pp3d, float* th3d, float* prs, float* qv,
+                          float* qc, float* qr, float* fall_velocity, float* rain, float* prate, double* asq,
+                          double* qbudget, int* status, float* qpten, float* qtten, float* qvten, float* qcten,
+                          void* scratch, int64_t scratch_bytes, int ni, int nj, int nk, float dx, float dy, float dz,
+                          float dt, int eqtset, void* stream) {
+  if (eqtset != 1 && eqtset != 2) return (int)hipErrorInvalidValue;
+  if (scratch_bytes < gm1hip_kessler_scratch_bytes(ni, nj, nk)) return (int)hipErrorInvalidValue;
+  hipStream_t s = (hipStream_t)stream;
+  KesArgs a{};
+  a.pi0 = pi0; a.th0 = th0; a.ruh = ruh; a.rvh = rvh; a.rmh = rmh; a.mh = mh;
+  a.rho = rho; a.rr = rr; a.pp3d = pp3d; a.th3d = th3d; a.prs = prs; a.qv = qv; a.qc = qc; a.qr = qr;
+  a.vt = fall_velocity; a.rain = rain; a.prate = prate;
+  a.qpten = qpten; a.qtten = qtten; a.qvten = qvten; a.qcten = qcten;
+  a.ni = ni; a.nj = nj; a.nk = nk; a.nseg = (ni + 63) / 64; a.eqtset = eqtset;
+  a.dx = dx; a.dy = dy; a.dz = dz; a.dt = dt;
+  char* base = (char*)scratch;
+  a.colbud = (double*)base;
+  base += 5LL * nj * ni * 8;
+  a.wavebud = (double*)base;
+  base += 5LL * nk * nj * a.nseg * 8;
+  double* rows = (double*)base;
+  base += (5LL * nj + 5LL * nk) * 8;
+  a.wavestat = (int*)base;
+  hipLaunchKernelGGL(k_kes_column, dim3((ni + 63) / 64, nj), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(k_kes_cell, dim3(a.nseg, nj, nk), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(k_kes_budgets, dim3(1), dim3(256), 0, s, a.colbud, a.wavebud, a.wavestat, rows, asq, qbudget,
+                     status, ni, nj, nk, a.nseg, dx, dy, dz);
+  GM1_LAUNCH_CHECK();
+  return 0;
+}
+
+int gm1hip_kessler_transition_pre(const float* pp_family, float* pp3d, float* th3d, float* qv, float* qc, float* qpten,
+                                  float* qtten, float* qvten, float* qcten, const float* pi0, const float* th0,
+                                  float* prs, float* rho, float* rr, int ni, int nj, int nk, float dt, int eqtset,
+                                  int copy_rr, void* stream) {
+  hipLaunchKernelGGL(k_kes_transition_pre, dim3((ni + 255) / 256, nj, nk), dim3(256), 0, (hipStream_t)stream,
+                     pp_family, pp3d, th3d, qv, qc, qpten, qtten, qvten, qcten, pi0, th0, prs, rho, rr, ni, nj, nk,
+                     dt, eqtset, copy_rr);
+  GM1_LAUNCH_CHECK();
+  return 0;
+}
+
+int64_t gm1hip_dry_mass_scratch_bytes(int ni, int nj, int nk) {
+  const int64_t nseg = (ni + 63) / 64;
+  return 2LL * nk * nj * nseg * 8 + 2LL * nk * 8 + 64;
+}
+
+// Dry-air mass ledger; with mass1 > 0 also the apmasscon correction applied
+// to pp3d/prs/rho (and rr when given).  ledger[0] = mass before, ledger[1] =
+// sum(pi0+pp3d) before, corr[0] = the f32 correction.
+int gm1hip_apmasscon(const float* ruh, const float* rvh, const float* rmh, const float* pi0, const float* th0,
+                     const float* th3d, const float* qv, float* pp3d, float* prs, float* rho, float* rr,
+                     double* ledger, float* corr, double mass1, void* scratch, int64_t scratch_bytes, int ni, int nj,
+                     int nk, void* stream) {
+  if (scratch_bytes < gm1hip_dry_mass_scratch_bytes(ni, nj, nk)) return (int)hipErrorInvalidValue;
+  hipStream_t s = (hipStream_t)stream;
+  const int nseg = (ni + 63) / 64;
+  double* part = (double*)scratch;
+  double* levels = part + 2LL * nk * nj * nseg;
+  hipLaunchKernelGGL(k_mass_partials, dim3(nseg, nj, nk), dim3(64), 0, s, rho, ruh, rvh, rmh, pi0, pp3d, part, ni,
+                     nj, nk, nseg, mass1 > 0.0 ? 1 : 0);
+  hipLaunchKernelGGL(k_mass_fold, dim3(1), dim3(256), 0, s, part, levels, ledger, corr, nk, nj, nseg, mass1,
+                     ni * nj * nk);
+  if (mass1 > 0.0) {
+    const int64_t n = (int64_t)ni * nj * nk;
+    hipLaunchKernelGGL(k_mass_apply, dim3(gm1_blocks(n, 256)), dim3(256), 0, s, corr, pp3d, pi0, th0, th3d, qv, prs,
+                       rho, rr, n);
+  }
+  GM1_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // extern "C"

@@ -279,14 +279,24 @@ __global__ __launch_bounds__(256) void k_held_compose(
 }
 
 // ----------------------------------------------------------- solve3
-__global__ __launch_bounds__(256) void k_solve3_pressure(float* __restrict__ ppx, const float* __restrict__ pp3d,
-                                                         float* __restrict__ ppi, int ni, int nj) {
+// With `mass` (the microphysics' (nk,nj,ni) pp3d) the new pp3d comes from it
+// and is written back into the pressure family.
+__global__ __launch_bounds__(256) void k_solve3_pressure(float* __restrict__ ppx, float* __restrict__ pp3d,
+                                                         float* __restrict__ ppi, const float* __restrict__ mass,
+                                                         int ni, int nj) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y, k = blockIdx.z + 1;
   if (i >= ni) return;
   const int64_t p = pix(k, j + 1, i + 1, ni, nj);
-  ppx[p] = pp3d[p] + ppx[p];
-  ppi[p] = pp3d[p];
+  float v;
+  if (mass) {
+    v = mass[((int64_t)(k - 1) * nj + j) * ni + i];
+    pp3d[p] = v;
+  } else {
+    v = pp3d[p];
+  }
+  ppx[p] = v + ppx[p];
+  ppi[p] = v;
 }
 
 __global__ __launch_bounds__(256) void k_solve3_density(const float* __restrict__ rho, const float* __restrict__ c1,
@@ -378,10 +388,11 @@ int gm1hip_held_compose(const float* th0, const float* pi0, const float* rho, co
   return 0;
 }
 
-int gm1hip_solve3_pressure(float* ppx, float* pp3d, float* ppi, int ni, int nj, int nk, unsigned sides,
-                           void* stream) {
+int gm1hip_solve3_pressure(float* ppx, float* pp3d, float* ppi, const float* pp3d_mass, int ni, int nj, int nk,
+                           unsigned sides, void* stream) {
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_solve3_pressure, dim3((ni + 63) / 64, nj, nk), dim3(64), 0, s, ppx, pp3d, ppi, ni, nj);
+  hipLaunchKernelGGL(k_solve3_pressure, dim3((ni + 63) / 64, nj, nk), dim3(64), 0, s, ppx, pp3d, ppi, pp3d_mass, ni,
+                     nj);
   GM1_LAUNCH_CHECK();
   int rc = fill_pressure(ppx, sides, ni, nj, nk, s);
   if (!rc) rc = fill_pressure(pp3d, sides, ni, nj, nk, s);

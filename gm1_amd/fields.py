@@ -224,5 +224,33 @@ def initial_fields(cfg: CaseConfig, prof: VerticalProfiles, base: dict[str, np.n
     if cfg.moist:
         q = np.zeros((3, nk, nj, ni), np.float32)
         q[0] = np.broadcast_to(base["qv0"][:, None, None], (nk, nj, ni))
+        rh = cfg.extra.get("bubble_rh")
+        if rh is not None:
+            # raise the bubble's vapour to rh times saturation at its temperature (Bolton 1980)
+            pi = base["pi0"].astype(np.float64)[:, None, None]
+            t = (base["th0"].astype(np.float64)[:, None, None] + th3d.astype(np.float64)) * pi
+            p = P00 * pi ** (CP / RD)
+            es = 611.2 * np.exp(17.67 * (t - 273.15) / (t - 29.65))
+            qvs = EPS * es / (p - es)
+            inside = np.zeros((nk, nj, ni), bool)
+            for amp, xc, yc, zc in cfg.bubbles:
+                beta = np.sqrt(((x[None, None, :] - xc) / xr) ** 2 + ((y[None, :, None] - yc) / xr) ** 2
+                               + ((zs[:, None, None] - zc) / zr) ** 2)
+                inside |= beta < 1.0
+            q[0] = np.where(inside, np.maximum(q[0], (rh * qvs).astype(F)), q[0])
         out["q3d"] = q
     return out
+
+
+def initial_thermodynamics(base: dict[str, np.ndarray], th3d: np.ndarray, qv: np.ndarray | None) -> dict[str, np.ndarray]:
+    """Pressure and density of the initial state (pi' = 0) by the moist equation of state.
+
+    prs = p00 pi0^(cp/rd); rho = prs / ((th0 + th') pi0 (rd + rv max(qv, 0))), in binary32.
+    """
+    nk, nj, ni = th3d.shape
+    pi0 = np.broadcast_to(base["pi0"][:, None, None], th3d.shape).astype(F)
+    th0 = np.broadcast_to(base["th0"][:, None, None], th3d.shape).astype(F)
+    prs = (np.power(pi0.astype(np.float64), float(F(CP) / F(RD))) * P00).astype(F)
+    vap = np.zeros_like(th3d) if qv is None else np.maximum(qv, F(0.0))
+    rho = prs / (((th0 + th3d) * pi0) * (F(RD) + F(RV) * vap))
+    return {"prs": prs.astype(F), "rho": rho.astype(F), "pi0_3d": pi0.copy(), "th0_3d": th0.copy()}

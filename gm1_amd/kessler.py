@@ -1,0 +1,199 @@
+"""Kessler warm rain and its large-step transition on gfx950.
+
+Host mirror of ``cuda_gm1_kessler_driver.py`` (``Gm1KesslerDriverSelection``
+:100-160, ``gm1_kessler_driver_shapes`` :180-212,
+``create_cuda_gm1_kessler_driver_workspace`` :806,
+``launch_cuda_gm1_kessler_driver`` :907-1087), of the transition
+``CudaGm1KesslerLargeStepTransition`` (cuda_gm1_kessler_large_step.py:295-560)
+and of the dry-air mass correction (cuda_apmasscon.py).  The device work is
+three C-ABI calls; budgets (asq, qbudget) and the satadj status words stay in
+device buffers.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Any, Mapping
+
+import torch
+
+from . import _native
+from ._device import ptr, stream_handle
+from ._native import D, Fl, I, P, signature
+from .errors import refuse
+
+I64 = ctypes.c_int64
+signature("gm1hip_kessler_scratch_bytes", I, I, I, restype=I64)
+signature("gm1hip_dry_mass_scratch_bytes", I, I, I, restype=I64)
+signature("gm1hip_kessler_driver", *([P] * 25), I64, I, I, I, Fl, Fl, Fl, Fl, I, P)
+signature("gm1hip_kessler_transition_pre", *([P] * 14), I, I, I, Fl, I, I, P)
+signature("gm1hip_apmasscon", *([P] * 13), D, P, I64, I, I, I, P)
+
+
+@dataclass(frozen=True)
+class Gm1KesslerDriverSelection:
+    """The sealed native branch (cuda_gm1_kessler_driver.py:100-160); efall=1 is not built."""
+
+    ptype: int = 1
+    equation_set: int = 1
+    testcase: int = 0
+    pdscheme: int = 1
+    efall: int = 0
+    axisymmetric: int = 0
+    moving_domain: int = 0
+    rain_bins: int = 1
+    nrk: int = 4
+    input_phase: str = "pre-pdefq"
+    output_phase: str = "post-satadj"
+
+
+DEFAULT_GM1_KESSLER_DRIVER_SELECTION = Gm1KesslerDriverSelection()
+
+
+def validate_selection(selection: Gm1KesslerDriverSelection) -> None:
+    if not isinstance(selection, Gm1KesslerDriverSelection):
+        raise TypeError("selection must be a Gm1KesslerDriverSelection")
+    fixed = {"ptype": 1, "testcase": 0, "pdscheme": 1, "axisymmetric": 0, "moving_domain": 0, "rain_bins": 1,
+             "nrk": 4, "input_phase": "pre-pdefq", "output_phase": "post-satadj"}
+    for name, wanted in fixed.items():
+        value = getattr(selection, name)
+        if type(value) is not type(wanted) or value != wanted:
+            refuse(code="GM1-KESSLER-DRIVER-SELECTION", gate="physics-seam", option=name, value=value,
+                   reason="the complete Kessler operator is sealed to one exact native branch",
+                   remedy=f"use {name}={wanted!r}")
+    if selection.efall != 0:
+        refuse(code="GM1-KESSLER-DRIVER-EFALL", gate="physics-seam", option="efall", value=selection.efall,
+               reason="the gfx950 driver implements the efall=0 branch only",
+               remedy="use efall=0")
+    if type(selection.equation_set) is not int or selection.equation_set not in (1, 2):
+        refuse(code="GM1-KESSLER-DRIVER-EQTSET", gate="physics-seam", option="equation_set",
+               value=selection.equation_set,
+               reason="the authority has exactly traditional and energy-conserving branches",
+               remedy="use equation_set=1 or equation_set=2")
+
+
+def gm1_kessler_driver_shapes(ni: int, nj: int, nk: int) -> dict[str, tuple[int, ...]]:
+    mass = (nk, nj, ni)
+    out = {n: mass for n in ("rmh", "mh", "pi0", "th0", "rho", "rr", "pp3d", "th3d", "prs", "qv", "qc", "qr",
+                             "fall_velocity")}
+    out.update({"ruh": (ni,), "rvh": (nj,), "rain": (1, nj, ni), "prate": (nj, ni), "asq": (3,), "qbudget": (10,)})
+    return out
+
+
+@dataclass
+class Gm1KesslerDriverWorkspace:
+    scratch: torch.Tensor
+    status: torch.Tensor  # int32 [max satadj iterations, non-converged cells]
+    ni: int
+    nj: int
+    nk: int
+
+
+def create_cuda_gm1_kessler_driver_workspace(ni: int, nj: int, nk: int, device: Any = "cuda") -> Gm1KesslerDriverWorkspace:
+    nbytes = int(_native.query("gm1hip_kessler_scratch_bytes", ni, nj, nk))
+    scratch = torch.empty(nbytes, dtype=torch.uint8, device=device)
+    status = torch.zeros(2, dtype=torch.int32, device=device)
+    return Gm1KesslerDriverWorkspace(scratch, status, ni, nj, nk)
+
+
+def _check(arrays: Mapping[str, Any], ni: int, nj: int, nk: int) -> dict[str, torch.Tensor]:
+    out = {}
+    for name, shape in gm1_kessler_driver_shapes(ni, nj, nk).items():
+        t = arrays[name]
+        dtype = torch.float64 if name in ("asq", "qbudget") else torch.float32
+        if not isinstance(t, torch.Tensor) or t.dtype != dtype or tuple(t.shape) != shape or not t.is_contiguous():
+            raise ValueError(f"{name} must be a contiguous {dtype} tensor of shape {shape}")
+        if not t.is_cuda:
+            raise ValueError(f"{name} must be on the HIP device")
+        out[name] = t
+    return out
+
+
+def launch_cuda_gm1_kessler_driver(
+    arrays: Mapping[str, Any], *, ni: int, nj: int, nk: int, dx: float, dy: float, dz: float, dt: float,
+    selection: Gm1KesslerDriverSelection = DEFAULT_GM1_KESSLER_DRIVER_SELECTION, stream: Any | None = None,
+    workspace: Gm1KesslerDriverWorkspace | None = None, held: Mapping[str, Any] | None = None, **_: Any,
+) -> Gm1KesslerDriverWorkspace:
+    """Every ptype=1 phase (pdefq, fall speed, fallout, local, satadj) in place.
+
+    With ``held`` (qpten/qtten/qvten/qcten holding the snapshot of the
+    eqtset=2 transition) the held rates are recaptured in the same pass.
+    """
+    validate_selection(selection)
+    a = _check(arrays, ni, nj, nk)
+    if workspace is None:
+        workspace = create_cuda_gm1_kessler_driver_workspace(ni, nj, nk, a["rho"].device)
+    h = held or {}
+    _native.call(
+        "gm1hip_kessler_driver",
+        *(ptr(a[n]) for n in ("ruh", "rvh", "rmh", "mh", "pi0", "th0", "rho", "rr", "pp3d", "th3d", "prs", "qv",
+                              "qc", "qr", "fall_velocity", "rain", "prate", "asq", "qbudget")),
+        ptr(workspace.status), ptr(h.get("qpten")), ptr(h.get("qtten")), ptr(h.get("qvten")), ptr(h.get("qcten")),
+        ptr(workspace.scratch), workspace.scratch.numel(), ni, nj, nk, float(dx), float(dy), float(dz), float(dt),
+        selection.equation_set, stream_handle(stream))
+    return workspace
+
+
+@dataclass
+class DryMassWorkspace:
+    scratch: torch.Tensor
+    ledger: torch.Tensor  # float64 [mass before, sum(pi0+pp3d) before]
+    correction: torch.Tensor  # float32 [1]
+
+
+def create_dry_mass_workspace(ni: int, nj: int, nk: int, device: Any = "cuda") -> DryMassWorkspace:
+    nbytes = int(_native.query("gm1hip_dry_mass_scratch_bytes", ni, nj, nk))
+    return DryMassWorkspace(torch.empty(nbytes, dtype=torch.uint8, device=device),
+                            torch.zeros(2, dtype=torch.float64, device=device),
+                            torch.zeros(1, dtype=torch.float32, device=device))
+
+
+def launch_cuda_apmasscon(arrays: Mapping[str, Any], *, ni: int, nj: int, nk: int, mass1: float,
+                          workspace: DryMassWorkspace, stream: Any | None = None) -> DryMassWorkspace:
+    """Exner correction restoring the dry-air mass ledger (cuda_apmasscon.py:256-327).
+
+    ``mass1 <= 0`` only measures the ledger into ``workspace.ledger``.
+    """
+    a = arrays
+    _native.call("gm1hip_apmasscon", *(ptr(a.get(n)) for n in ("ruh", "rvh", "rmh", "pi0", "th0", "th3d", "qv",
+                                                                  "pp3d", "prs", "rho", "rr")),
+                 ptr(workspace.ledger), ptr(workspace.correction), float(mass1), ptr(workspace.scratch),
+                 workspace.scratch.numel(), ni, nj, nk, stream_handle(stream))
+    return workspace
+
+
+def launch_cuda_gm1_kessler_transition(
+    arrays: Mapping[str, Any], held: Mapping[str, Any], *, ni: int, nj: int, nk: int, dx: float, dy: float,
+    dz: float, dt: float, selection: Gm1KesslerDriverSelection, workspace: Gm1KesslerDriverWorkspace,
+    pressure_family: Any | None = None, mass1: float | None = None, dry_mass: DryMassWorkspace | None = None,
+    stream: Any | None = None,
+) -> None:
+    """cpu_gm1_kessler_large_step_inplace on the device (cuda_gm1_kessler_large_step.py:183-292).
+
+    eqtset=2 held remove/snapshot and the pre-physics EOS (pp3d taken from
+    ``pressure_family`` when given), the dry-mass hook when ``mass1`` is
+    given, ``rr <- rho``, the driver, and the held recapture.
+    """
+    validate_selection(selection)
+    a = _check(arrays, ni, nj, nk)
+    eqt = selection.equation_set
+    hook = mass1 is not None
+    _native.call("gm1hip_kessler_transition_pre", ptr(pressure_family), ptr(a["pp3d"]), ptr(a["th3d"]),
+                 ptr(a["qv"]), ptr(a["qc"]), ptr(held["qpten"]), ptr(held["qtten"]), ptr(held["qvten"]),
+                 ptr(held["qcten"]), ptr(a["pi0"]), ptr(a["th0"]), ptr(a["prs"]), ptr(a["rho"]), ptr(a["rr"]), ni,
+                 nj, nk, float(dt), eqt, int(not hook), stream_handle(stream))
+    if hook:
+        if dry_mass is None:
+            raise ValueError("the dry-mass hook needs a DryMassWorkspace")
+        launch_cuda_apmasscon(a, ni=ni, nj=nj, nk=nk, mass1=float(mass1), workspace=dry_mass, stream=stream)
+    launch_cuda_gm1_kessler_driver(a, ni=ni, nj=nj, nk=nk, dx=dx, dy=dy, dz=dz, dt=dt, selection=selection,
+                                   stream=stream, workspace=workspace, held=held if eqt == 2 else None)
+
+
+__all__ = [
+    "DEFAULT_GM1_KESSLER_DRIVER_SELECTION", "DryMassWorkspace", "Gm1KesslerDriverSelection",
+    "Gm1KesslerDriverWorkspace", "create_cuda_gm1_kessler_driver_workspace", "create_dry_mass_workspace",
+    "gm1_kessler_driver_shapes", "launch_cuda_apmasscon", "launch_cuda_gm1_kessler_driver",
+    "launch_cuda_gm1_kessler_transition",
+]

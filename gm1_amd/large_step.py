@@ -6,11 +6,13 @@ driver ``launch_cuda_gm1_rk_stage`` (:1241-1345), in the same source order
 (``gm1_runtime.large_step_schedule``, ``gm1_runtime.py:1969-2007``):
 
   solve1   held carriers zeroed (no SGS/TKE closure on this path), held
-           compose of ppten and the microphysics held rates
+           compose of ppten and the microphysics held rates (moist)
   nrk=1..3 eqtset=2 stage terms -> held momentum seed (+ bcwsfc, bc2d for
            nrk>=2) -> momentum advection -> buoyancy -> acoustic stage ->
            post-acoustic getdiv/getdivx -> theta and moisture RK stage
-  finish   microphysics (Kessler, moist runs) -> solve3 (ppx, ppi, halos,
+           (moisture at nrk=3 through the positive-definite chain)
+  finish   Kessler transition (held remove, EOS, dry-mass correction,
+           driver, held recapture; moist) -> solve3 (ppx, ppi, halos,
            A <- current copies, density)
 
 Every operator is one C-ABI call into ``libgm1hip.so``; the whole step is
@@ -30,25 +32,37 @@ import torch
 from . import _native
 from ._device import NG, lateral_code, ptr, stream_handle
 from ._native import Fl, I, P, U, signature
-from .advection5 import launch_cuda_scalar_advection5
-from .fields import CaseConfig, base_state, initial_fields, metrics_for, profiles_for
+from .fields import CaseConfig, base_state, initial_fields, initial_thermodynamics, metrics_for, profiles_for
 from .grid import VerticalProfiles
+from .kessler import (
+    Gm1KesslerDriverSelection,
+    create_cuda_gm1_kessler_driver_workspace,
+    create_dry_mass_workspace,
+    launch_cuda_apmasscon,
+    launch_cuda_gm1_kessler_transition,
+)
 from .momentum_advection5 import launch_cuda_terrain_open_momentum_advection5
 from .psolver3 import TerrainOpenSoundAblations, launch_cuda_psolver3_terrain_open_stage
+from .scalar_moist_rk import (
+    launch_pdef_moisture,
+    launch_scalar_rk_compose,
+    moisture_weps_f64,
+    pack_scalar,
+    theta_weps_f64,
+)
 
 signature("gm1hip_stage_terms", P, P, P, P, P, P, P, I, P, P, P, P, P, P, I, I, I, U, P)
 signature("gm1hip_buoyancy", P, P, P, P, P, P, P, P, I, I, I, P)
 signature("gm1hip_post_acoustic", *([P] * 14), I, I, I, Fl, Fl, Fl, P)
-signature("gm1hip_pack_scalar", P, P, P, I, I, I, I, U, P)
 signature("gm1hip_rk_compose", P, P, P, P, P, ctypes.c_int64, Fl, P)
 signature("gm1hip_seed_held_momentum", P, P, P, P, P, P, I, I, I, P)
 signature("gm1hip_held_compose", *([P] * 13), I, I, I, Fl, P)
-signature("gm1hip_solve3_pressure", P, P, P, I, I, I, U, P)
+signature("gm1hip_solve3_pressure", P, P, P, P, I, I, I, U, P)
 signature("gm1hip_solve3_density", P, P, P, P, P, I, I, I, P)
 
-CM1_EPSILON_F32 = 1.0e-18
-THETA_WEPS = float(np.float32(10.0) * np.float32(CM1_EPSILON_F32))
-MOISTURE_WEPS = float(np.float32(1.0e-2) * np.float32(CM1_EPSILON_F32))
+THETA_WEPS = theta_weps_f64()
+MOISTURE_WEPS = moisture_weps_f64(0)
+KESSLER_HELD = ("qpten", "qtten", "qvten", "qcten")
 
 
 def padded_shapes(ni: int, nj: int, nk: int) -> dict[str, tuple[int, ...]]:
@@ -74,12 +88,12 @@ def host_initial_state(cfg: CaseConfig) -> dict[str, Any]:
         a[name] = np.zeros(sh["v"], np.float32)
     for name in ("w3d", "wa", "wten", "wten1", "rrw"):
         a[name] = np.zeros(sh["w"], np.float32)
-    for name in ("dum8", "theta_padded", "moisture_padded"):
+    for name in ("dum8", "theta_padded"):
         a[name] = np.zeros(sh["s"], np.float32)
     for name in ("pp3d", "ppi", "ppx", "ppd", "piadv", "ppterm", "ppten", "thv", "qk", "pk1", "pk2", "mm", "fpk",
                  "elim_c", "elim_d"):
         a[name] = np.zeros(sh["p"], np.float32)
-    for name in ("th3d", "tha", "thten1", "thterm", "divergence", "theta_div", "divx", "advection"):
+    for name in ("th3d", "tha", "thten1", "thterm", "divergence", "theta_div", "divx"):
         a[name] = np.zeros(sh["cell"], np.float32)
     a["radbcw"] = np.zeros((nk + 2, nj + 2 * NG), np.float32)
     a["radbce"] = np.zeros((nk + 2, nj + 2 * NG), np.float32)
@@ -101,6 +115,22 @@ def host_initial_state(cfg: CaseConfig) -> dict[str, Any]:
         a["q3d"] = init["q3d"].copy()
         a["qa"] = init["q3d"].copy()
         a["qten"] = np.zeros_like(a["q3d"])
+        a["moisture_padded"] = np.zeros(sh["s"], np.float32)
+        for name in ("advx", "advy", "advz", "pp_mass", "fall_velocity") + KESSLER_HELD:
+            a[name] = np.zeros(sh["cell"], np.float32)
+        thermo = initial_thermodynamics(base, init["th3d"], init["q3d"][0])
+        a["prs"], a["rho"] = thermo["prs"], thermo["rho"]
+        a["rr"] = (np.float32(1.0) / thermo["rho"]).astype(np.float32)
+        a["rf"] = np.zeros((nk + 1, nj, ni), np.float32)
+        a["pi0_3d"], a["th0_3d"] = thermo["pi0_3d"], thermo["th0_3d"]
+        a["ruh"] = np.ones(ni, np.float32)
+        a["rvh"] = np.ones(nj, np.float32)
+        a["rmh"] = np.ones(sh["cell"], np.float32)
+        a["mh"] = np.ones(sh["cell"], np.float32)
+        a["rain"] = np.zeros((1, nj, ni), np.float32)
+        a["prate"] = np.zeros((nj, ni), np.float32)
+        a["asq"] = np.zeros(3, np.float64)
+        a["qbudget"] = np.zeros(10, np.float64)
     base_d = {"th0": base["th0"], "pi0": base["pi0"], "qv0": base["qv0"]}
     return {"cfg": cfg, "prof": prof, "base": base_d, "metrics": met, "arrays": a, "numq": numq}
 
@@ -115,21 +145,51 @@ class Gm1LargeStepState:
     numq: int
     stream: Any | None = None
     step_index: int = 0
+    selection: Gm1KesslerDriverSelection | None = None
+    kessler_workspace: Any = None
+    dry_mass: Any = None
+    mass1: float | None = None
     receipts: list = field(default_factory=list)
 
 
 def create_gm1_large_step_state(host: dict[str, Any], device: torch.device | str = "cuda",
-                                stream: Any | None = None) -> Gm1LargeStepState:
+                                stream: Any | None = None, *, dry_mass_correction: bool = True,
+                                equation_set: int = 2) -> Gm1LargeStepState:
+    """Upload the state; for a moist run also the Kessler workspaces and the dry-mass ledger.
+
+    The ledger ``mass1`` is measured once here (one host read, outside any
+    step), as the reference persists it at initialisation.
+    """
     dev = torch.device(device)
     t = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)  # noqa: E731
-    return Gm1LargeStepState(
+    st = Gm1LargeStepState(
         host["cfg"], host["prof"], {k: t(v) for k, v in host["base"].items()},
         {k: t(v) for k, v in host["metrics"].items()}, {k: t(v) for k, v in host["arrays"].items()},
         host["numq"], stream)
+    if st.numq:
+        cfg = st.cfg
+        st.selection = Gm1KesslerDriverSelection(equation_set=equation_set)
+        st.kessler_workspace = create_cuda_gm1_kessler_driver_workspace(cfg.ni, cfg.nj, cfg.nk, dev)
+        if dry_mass_correction:
+            st.dry_mass = create_dry_mass_workspace(cfg.ni, cfg.nj, cfg.nk, dev)
+            launch_cuda_apmasscon({"rho": st.a["rho"], "ruh": st.a["ruh"], "rvh": st.a["rvh"], "rmh": st.a["rmh"]},
+                                  ni=cfg.ni, nj=cfg.nj, nk=cfg.nk, mass1=0.0, workspace=st.dry_mass, stream=stream)
+            torch.cuda.synchronize(dev)
+            st.mass1 = float(st.dry_mass.ledger[0].item())
+    return st
 
 
 def state_to_host(st: Gm1LargeStepState) -> dict[str, np.ndarray]:
     return {k: v.detach().cpu().numpy() for k, v in st.a.items()}
+
+
+def _kessler_arrays(st: Gm1LargeStepState) -> dict[str, torch.Tensor]:
+    a = st.a
+    return {"ruh": a["ruh"], "rvh": a["rvh"], "rmh": a["rmh"], "mh": a["mh"], "pi0": a["pi0_3d"],
+            "th0": a["th0_3d"], "rho": a["rho"], "rr": a["rr"], "pp3d": a["pp_mass"], "th3d": a["th3d"],
+            "prs": a["prs"], "qv": a["q3d"][0], "qc": a["q3d"][1], "qr": a["q3d"][2],
+            "fall_velocity": a["fall_velocity"], "rain": a["rain"], "prate": a["prate"], "asq": a["asq"],
+            "qbudget": a["qbudget"]}
 
 
 def _rk_stage(st: Gm1LargeStepState, nrk: int) -> None:
@@ -160,18 +220,23 @@ def _rk_stage(st: Gm1LargeStepState, nrk: int) -> None:
                  ptr(m["rf0"]), ptr(m["rgzu"]), ptr(m["rgzv"]), ptr(m["dzdx"]), ptr(m["dzdy"]), ptr(m["gz"]),
                  ptr(a["thterm"]), ptr(a["divergence"]), ptr(a["theta_div"]), ptr(a["divx"]), ni, nj, nk,
                  cfg.dx, cfg.dy, cfg.dz, h)
-    dttmp = float(np.float32(np.float32(cfg.dt) / np.float32(4 - nrk)))
-    n = ni * nj * nk
-    # theta: (th0 - 300) + th3d, family halos, advection, A-anchored compose
-    _native.call("gm1hip_pack_scalar", ptr(b["th0"]), ptr(a["th3d"]), ptr(a["theta_padded"]), ni, nj, nk, 1, code, h)
     mass = {"rru": a["rru"], "rrv": a["rrv"], "rrw": a["rrw"]}
-    launch_cuda_scalar_advection5(
-        a["theta_padded"], mass, m, a["divx"], a["advection"], None, ni=ni, nj=nj, nk=nk, nrk=nrk, dx=cfg.dx,
-        dy=cfg.dy, dz=cfg.dz, weps=THETA_WEPS, stream=st.stream, lateral_sides=sides)
-    _native.call("gm1hip_rk_compose", ptr(a["tha"]), ptr(a["thten1"]), ptr(a["theta_div"]), ptr(a["advection"]),
-                 ptr(a["th3d"]), n, dttmp, h)
-    if st.numq:
-        raise NotImplementedError("moisture transport is composed by the moist large step")
+    grid = dict(ni=ni, nj=nj, nk=nk, dt=cfg.dt, dx=cfg.dx, dy=cfg.dy, dz=cfg.dz, lateral_sides=sides,
+                stream=st.stream)
+    # theta: (th0 - 300) + th3d with family halos, then advection + A-anchored compose in one pass
+    pack_scalar(b["th0"], a["th3d"], a["theta_padded"], ni=ni, nj=nj, nk=nk, theta=True, lateral_sides=sides,
+                stream=st.stream)
+    launch_scalar_rk_compose(a["theta_padded"], mass, m, a["divx"], a["tha"], a["thten1"], a["theta_div"],
+                             a["th3d"], nrk=nrk, weps=THETA_WEPS, **grid)
+    for sp in range(st.numq):
+        pack_scalar(b["th0"], q3d[sp], a["moisture_padded"], ni=ni, nj=nj, nk=nk, theta=False,
+                    lateral_sides=sides, stream=st.stream)
+        if nrk < 3:
+            launch_scalar_rk_compose(a["moisture_padded"], mass, m, a["divx"], a["qa"][sp], a["qten"][sp], None,
+                                     q3d[sp], nrk=nrk, weps=moisture_weps_f64(sp), **grid)
+        else:
+            launch_pdef_moisture(a["moisture_padded"], mass, m, a["divx"], a["qa"][sp], a["qten"][sp], q3d[sp],
+                                 {"advx": a["advx"], "advy": a["advy"], "advz": a["advz"]}, species=sp, **grid)
 
 
 def launch_cuda_gm1_large_step(st: Gm1LargeStepState) -> dict[str, Any]:
@@ -183,16 +248,42 @@ def launch_cuda_gm1_large_step(st: Gm1LargeStepState) -> dict[str, Any]:
     with torch.cuda.stream(st.stream) if st.stream is not None else _nullcontext():
         for name in ("uten1", "vten1", "wten1", "thten1", "ppten"):
             a[name].zero_()
+        if st.numq:
+            a["qten"].zero_()
+    if st.numq:
+        b = st.base
+        _native.call("gm1hip_held_compose", ptr(b["th0"]), ptr(b["pi0"]), ptr(a["rho"]), ptr(a["ppi"]),
+                     ptr(a["tha"]), ptr(a["qa"]), ptr(a["thten1"]), ptr(a["qten"]), ptr(a["ppten"]),
+                     ptr(a["qpten"]), ptr(a["qtten"]), ptr(a["qvten"]), ptr(a["qcten"]), ni, nj, nk, cfg.dt, h)
     for nrk in (1, 2, 3):
         _rk_stage(st, nrk)
-    _native.call("gm1hip_solve3_pressure", ptr(a["ppx"]), ptr(a["pp3d"]), ptr(a["ppi"]), ni, nj, nk, code, h)
+    if st.numq:
+        launch_cuda_gm1_kessler_transition(
+            _kessler_arrays(st), {n: a[n] for n in KESSLER_HELD}, ni=ni, nj=nj, nk=nk, dx=cfg.dx, dy=cfg.dy,
+            dz=cfg.dz, dt=cfg.dt, selection=st.selection, workspace=st.kessler_workspace,
+            pressure_family=a["pp3d"], mass1=st.mass1, dry_mass=st.dry_mass, stream=st.stream)
+    _native.call("gm1hip_solve3_pressure", ptr(a["ppx"]), ptr(a["pp3d"]), ptr(a["ppi"]),
+                 ptr(a["pp_mass"]) if st.numq else None, ni, nj, nk, code, h)
     with torch.cuda.stream(st.stream) if st.stream is not None else _nullcontext():
         a["ua"].copy_(a["u3d"])
         a["va"].copy_(a["v3d"])
         a["wa"].copy_(a["w3d"])
         a["tha"].copy_(a["th3d"])
+        if st.numq:
+            a["qa"].copy_(a["q3d"])
+    if st.numq:
+        m = st.metrics
+        _native.call("gm1hip_solve3_density", ptr(a["rho"]), ptr(m["c1"]), ptr(m["c2"]), ptr(a["rr"]), ptr(a["rf"]),
+                     ni, nj, nk, h)
     st.step_index += 1
-    return {"step": st.step_index, "cells": cfg.cells, "lateral_sides": list(cfg.lateral_sides)}
+    return {"step": st.step_index, "cells": cfg.cells, "lateral_sides": list(cfg.lateral_sides),
+            "moist": bool(st.numq)}
+
+
+def kessler_status(st: Gm1LargeStepState) -> dict[str, int]:
+    """Max satadj iterations and non-converged cells so far (reads the device; synchronises)."""
+    s = st.kessler_workspace.status.detach().cpu().tolist()
+    return {"max_satadj_iterations": int(s[0]), "non_converged_cells": int(s[1])}
 
 
 class _nullcontext:

@@ -42,6 +42,29 @@ int gm1hip_scalar_advection5(const float* scalar, const float* rru, const float*
                              float* tendency, float* flux_x, float* flux_y, float* flux_z, int ni, int nj, int nk,
                              int use_weno, double weps, float dx, float dy, float dz, unsigned obc, void* stream);
 
+// One scalar's RK stage of cuda_scalar_moist_rk.py (:1041-1499) with the
+// advection fused into the A-anchored compose (:283-349):
+// out = a0 + dttmp*((held - divterm) + advection), |out| < 1e-30 -> 0;
+// divterm may be NULL (moisture rows).  WENO5 when use_weno.  obc must name
+// the lateral families (non-zero).
+int gm1hip_scalar_rk_stage(const float* scalar, const float* rru, const float* rrv, const float* rrw,
+                           const float* c1, const float* c2, const float* rr0, const float* gz, const float* divx,
+                           const float* a0, const float* held, const float* divterm, float* out, int ni, int nj,
+                           int nk, int use_weno, double weps, float dx, float dy, float dz, unsigned obc,
+                           float dttmp, void* stream);
+
+// Final-stage moisture row (cuda_scalar_moist_rk.py:1376-1499, pdef.F):
+// gated WENO5 fluxes (pdefweno: the vertical gate, off for qv), directional
+// tendencies into advx/advy/advz (cell-shaped scratch), the donor repair in
+// x, y and z (cm1_rk_pdef_candidate_*/correct_*), the total and the compose
+// out = qa + dttmp*(held + advection).
+int gm1hip_pdef_moisture_stage(const float* scalar, const float* rru, const float* rrv, const float* rrw,
+                               const float* c1, const float* c2, const float* rho0_3d, const float* rr0,
+                               const float* gz, const float* rgz, const float* divx, const float* qa,
+                               const float* held, float* out, float* advx, float* advy, float* advz, int ni,
+                               int nj, int nk, int pdefweno, double weps, float dx, float dy, float dz,
+                               unsigned obc, float dttmp, void* stream);
+
 // Replaces launch_cuda_terrain_open_momentum_advection5
 // (cuda_momentum_advection5.py:1173-1330): mass fluxes, divx and the
 // advu/advv/advw increments added into uten/vten/wten.  With
@@ -140,13 +163,58 @@ int gm1hip_held_compose(const float* th0, const float* pi0, const float* rho, co
                         const float* qtten, const float* qvten, const float* qcten, int ni, int nj, int nk, float dt,
                         void* stream);
 // solve3 pressure: ppx = pp3d + ppx, ppi = pp3d and their halos
-// (cuda_gm1_forecast.py:1414-1435, _finish_solve3).
-int gm1hip_solve3_pressure(float* ppx, float* pp3d, float* ppi, int ni, int nj, int nk, unsigned sides,
-                           void* stream);
+// (cuda_gm1_forecast.py:1414-1435, _finish_solve3).  With pp3d_mass (the
+// microphysics' (nk,nj,ni) pp3d, may be NULL) the new pp3d comes from it and
+// is copied into the pressure family first.
+int gm1hip_solve3_pressure(float* ppx, float* pp3d, float* ppi, const float* pp3d_mass, int ni, int nj, int nk,
+                           unsigned sides, void* stream);
 // solve3 density: rr = 1/rho, rf by c1/c2 with 3-point end extrapolation
 // (kernel gm1_solve3_density_v1, cuda_gm1_forecast.py:568-577).
 int gm1hip_solve3_density(const float* rho, const float* c1, const float* c2, float* rr, float* rf, int ni, int nj,
                           int nk, void* stream);
+
+// ----------------------------------------------------------------- Kessler
+// Device scratch (bytes) gm1hip_kessler_driver needs at ni x nj x nk.
+int64_t gm1hip_kessler_scratch_bytes(int ni, int nj, int nk);
+
+// Replaces launch_cuda_gm1_kessler_driver (cuda_gm1_kessler_driver.py:907-1087,
+// ptype 1, efall 0, eqtset 1 or 2): pdefq of qv/qc/qr, fall speed, fallout in
+// nfall substeps, autoconversion/accretion/evaporation
+// (cuda_gm1_kessler_local.py:251-426) and saturation adjustment (:337-428).
+// Mass arrays are (nk,nj,ni); rain (1,nj,ni), prate (nj,ni); asq (3) and
+// qbudget (10) are binary64 and accumulate.  status[0] keeps the largest
+// satadj iteration count, status[1] counts cells that did not converge in
+// 100 iterations.  With qpten..qcten non-NULL (holding the eqtset=2
+// snapshot) the held rates are recaptured, (state - snapshot)/dt.
+int gm1hip_kessler_driver(const float* ruh, const float* rvh, const float* rmh, const float* mh, const float* pi0,
+                          const float* th0, float* rho, float* rr, float* pp3d, float* th3d, float* prs, float* qv,
+                          float* qc, float* qr, float* fall_velocity, float* rain, float* prate, double* asq,
+                          double* qbudget, int* status, float* qpten, float* qtten, float* qvten, float* qcten,
+                          void* scratch, int64_t scratch_bytes, int ni, int nj, int nk, float dx, float dy, float dz,
+                          float dt, int eqtset, void* stream);
+
+// First half of cpu_gm1_kessler_large_step_inplace
+// (cuda_gm1_kessler_large_step.py:238-268): eqtset=2 held remove/snapshot
+// (state -= rate*dt, rate <- state) and the pre-physics equation of state.
+// pp_family (pressure family, may be NULL) supplies pp3d; rr <- rho when
+// copy_rr (no dry-mass hook follows).
+int gm1hip_kessler_transition_pre(const float* pp_family, float* pp3d, float* th3d, float* qv, float* qc,
+                                  float* qpten, float* qtten, float* qvten, float* qcten, const float* pi0,
+                                  const float* th0, float* prs, float* rho, float* rr, int ni, int nj, int nk,
+                                  float dt, int eqtset, int copy_rr, void* stream);
+
+// Device scratch (bytes) gm1hip_apmasscon needs.
+int64_t gm1hip_dry_mass_scratch_bytes(int ni, int nj, int nk);
+
+// Dry-air mass ledger and, with mass1 > 0, the Exner correction of
+// cpu_apmasscon_inplace (cuda_apmasscon.py:256-327) applied to
+// pp3d/prs/rho (rr <- rho when rr is non-NULL).  ledger[0] = mass before,
+// ledger[1] = sum(pi0 + pp3d) before (binary64, fixed reduction order),
+// corr[0] = the binary32 correction.
+int gm1hip_apmasscon(const float* ruh, const float* rvh, const float* rmh, const float* pi0, const float* th0,
+                     const float* th3d, const float* qv, float* pp3d, float* prs, float* rho, float* rr,
+                     double* ledger, float* corr, double mass1, void* scratch, int64_t scratch_bytes, int ni, int nj,
+                     int nk, void* stream);
 
 // ----------------------------------------------------------------- measurement
 // No counterpart in the reference: live per-kernel GPU time for throughput

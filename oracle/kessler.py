@@ -49,12 +49,43 @@ CPDRD = CP / RD
 ZERO = F(0.0)
 
 
+# Transcendental mode.  "numpy": NumPy's float32 exp/power, exactly what the
+# reference's CPU transcription calls (pins the golden vectors).  "rounded":
+# binary64 exp/pow rounded once to binary32, the GPU kernels' formulation; the
+# two differ by at most one unit in the last place per call.
+_MODE = {"value": "numpy"}
+
+
+class transcendentals:
+    """Context manager selecting the transcendental mode ("numpy" or "rounded")."""
+
+    def __init__(self, mode: str):
+        if mode not in ("numpy", "rounded"):
+            raise ValueError(mode)
+        self.mode = mode
+
+    def __enter__(self):
+        self.prev = _MODE["value"]
+        _MODE["value"] = self.mode
+        return self
+
+    def __exit__(self, *exc):
+        _MODE["value"] = self.prev
+        return False
+
+
 def _exp(x):
-    return np.exp(x.astype(F)).astype(F)
+    x = np.asarray(x, F)
+    if _MODE["value"] == "rounded":
+        return np.exp(x.astype(f64)).astype(F)
+    return np.exp(x).astype(F)
 
 
 def _pow(b, e):
-    return np.power(b.astype(F), F(e)).astype(F)
+    b = np.asarray(b, F)
+    if _MODE["value"] == "rounded":
+        return np.power(b.astype(f64), f64(F(e))).astype(F)
+    return np.power(b, F(e)).astype(F)
 
 
 def saturation_qv(tnew, prs_cap, pdenom):
@@ -328,7 +359,7 @@ HELD = (("pp3d", "qpten"), ("th3d", "qtten"), ("qv", "qvten"), ("qc", "qcten"))
 def eos(a):
     """prs = p00 (pi0+pp3d)^(cp/rd); rho = prs/((th0+th3d)(pi0+pp3d)(rd + max(qv,0) rv))."""
     work = a["pi0"] + a["pp3d"]
-    a["prs"][...] = np.power(work, CPDRD).astype(F) * P00
+    a["prs"][...] = _pow(work, CPDRD) * P00
     rho = (a["th0"] + a["th3d"]) * work
     gas = np.maximum(a["qv"], ZERO) * RV + RD
     a["rho"][...] = a["prs"] / (rho * gas)
@@ -384,11 +415,16 @@ def apmasscon(a, mass1):
         psum += acc
     corr = (np.power(f64(mass1) / f64(mass2), f64(RDDCV)) - 1.0) * psum / f64(ni * nj * nk)
     corr32 = F(corr)
-    a["pp3d"][...] = a["pp3d"] + corr32
+    apply_mass_correction(a, corr32)
+    return {"mass2": mass2, "pressure_sum": psum, "correction": float(corr32)}
+
+
+def apply_mass_correction(a, corr32):
+    """pp3d += correction, then prs and rho by the equation of state (cuda_apmasscon.py:300-316)."""
+    a["pp3d"][...] = a["pp3d"] + F(corr32)
     pi = a["pi0"] + a["pp3d"]
-    prs = P00 * np.power(pi, CPDRD).astype(F)
+    prs = P00 * _pow(pi, CPDRD)
     theta = a["th0"] + a["th3d"]
     gas = RD + np.maximum(ZERO, a["qv"]) * RV
     a["prs"][...] = prs
     a["rho"][...] = prs / ((theta * pi) * gas)
-    return {"mass2": mass2, "pressure_sum": psum, "correction": float(corr32)}

@@ -18,7 +18,7 @@ HEADER = ROOT / "include" / "gm1hip.h"
 
 def declared_symbols() -> list[str]:
     text = HEADER.read_text()
-    return sorted(set(re.findall(r"^int\s+(gm1hip_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^(?:int|int64_t)\s+(gm1hip_\w+)\s*\(", text, flags=re.M)))
 
 
 def test_header_declares_every_registered_signature():

@@ -277,3 +277,47 @@ def test_held_seed_and_bc2d_match_reference():
     w = g["bc2d_in_w"].copy()
     obc.bc2d_open_w_surface(w, ni=ni, nj=nj)
     assert_bits_equal(w, g["bc2d_out_w"], "bc2d")
+
+
+def moist_case(ni, nj, nk, sides, *, seed=5):
+    return CaseConfig(f"moist-{ni}x{nj}x{nk}", ni, nj, nk, 250.0, 250.0, 250.0, 1.5, 8, tuple(sides), True,
+                      "hodograph", ((3.0, ni * 120.0, nj * 130.0, 1000.0),), bubble_radius=(ni * 80.0, 1000.0),
+                      noise=0.1, seed=seed, extra={"bubble_rh": 1.03})
+
+
+MOIST_FIELDS = PROGNOSTIC + ("q3d", "rho", "prs", "rr", "rf", "rain", "prate", "qpten", "qtten", "qvten", "qcten",
+                             "fall_velocity")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sides", [(2, 2, 2, 2), (1, 1, 1, 1)])
+def test_gpu_moist_large_step_bit_exact(device, sides):
+    """Two moist large steps (Kessler, pdef moisture, dry-mass correction) against the oracle.
+
+    The dry-mass correction is the one reduction whose order differs (a fixed
+    tree on the device); the oracle applies the device's binary32 correction
+    after checking its ledger agrees to 1e-13.
+    """
+    import torch
+
+    from gm1_amd.large_step import create_gm1_large_step_state, kessler_status, launch_cuda_gm1_large_step
+
+    cfg = moist_case(24, 20, 16, sides)
+    host = host_initial_state(cfg)
+    st = create_gm1_large_step_state(host, device)
+    a = {k: v.copy() for k, v in host["arrays"].items()}
+    prof = osound.terrain_coordinate_profiles(cfg.nk, cfg.dz)
+    c = {"ni": cfg.ni, "nj": cfg.nj, "nk": cfg.nk, "dx": cfg.dx, "dy": cfg.dy, "dz": cfg.dz, "dt": cfg.dt,
+         "nsound": cfg.nsound, "lateral_sides": cfg.lateral_sides}
+    for step in range(2):
+        launch_cuda_gm1_large_step(st)
+        torch.cuda.synchronize()
+        corr = float(st.dry_mass.correction.cpu().numpy()[0])
+        ol.large_step(a, host["metrics"], host["base"], prof, c, zt=F(cfg.nk * cfg.dz), mass_correction=corr)
+        for name in MOIST_FIELDS:
+            assert_bits_equal(to_host(st.a[name]), a[name], f"step {step + 1} {name}")
+        np.testing.assert_allclose(to_host(st.a["qbudget"]), a["qbudget"], rtol=1e-11, atol=1e-25)
+        np.testing.assert_allclose(to_host(st.a["asq"]), a["asq"], rtol=1e-11, atol=1e-25)
+    # the bubble condensed
+    assert a["q3d"][1].max() > 1e-5
+    assert kessler_status(st)["non_converged_cells"] == 0

@@ -57,3 +57,94 @@ def test_family_index_maps_halos_like_the_fills():
     np.testing.assert_array_equal(op._family_index(n, 1, 1), [4, 5, 0, 1, 2, 3, 4, 5, 0, 1])
     np.testing.assert_array_equal(op._family_index(n, 2, 2), [0, 0, 0, 1, 2, 3, 4, 5, 5, 5])
     np.testing.assert_array_equal(op._family_index(n, 3, 4), [1, 0, 0, 1, 2, 3, 4, 5, 5, 4])
+
+
+# ---------------------------------------------------------------- GPU parity
+def _moist_inputs(rng, ni, nj, nk):
+    from oracle import boundary as obc
+    from oracle.advection5 import cpu_terrain_mass_flux5_inplace
+
+    sh = shapes(ni, nj, nk)
+    m = seeded_metrics(rng, ni, nj, nk)
+    a = {"u3d": (4 * rng.standard_normal(sh["u"])).astype(F), "v3d": (4 * rng.standard_normal(sh["v"])).astype(F),
+         "w3d": (2 * rng.standard_normal(sh["w"])).astype(F), "rru": np.zeros(sh["u"], F),
+         "rrv": np.zeros(sh["v"], F), "rrw": np.zeros(sh["w"], F)}
+    divx = np.zeros((nk, nj, ni), F)
+    cpu_terrain_mass_flux5_inplace(a, m, divx, ni=ni, nj=nj, nk=nk, dx=250.0, dy=300.0, dz=200.0)
+    return a, m, divx
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sides", [(2, 2, 2, 2), (1, 1, 1, 1), (3, 4, 2, 1)])
+@pytest.mark.parametrize("species", [0, 1])
+def test_gpu_pdef_moisture_stage_bit_exact(device, sides, species):
+    import torch
+
+    from gm1_amd.scalar_moist_rk import launch_pdef_moisture
+    from oracle import boundary as obc
+    from oracle import stage as ost
+
+    if sides == (3, 4, 2, 1):
+        sides = (3, 4, 2, 2)
+    ni, nj, nk = 37, 21, 12
+    rng = np.random.default_rng(100 + species)
+    a, m, divx = _moist_inputs(rng, ni, nj, nk)
+    sh = shapes(ni, nj, nk)
+    q = np.abs(rng.normal(0.0, 1e-3, (nk, nj, ni))).astype(F)
+    q[rng.random(q.shape) < 0.3] = F(0.0)
+    q[rng.random(q.shape) < 0.05] = F(1e-25)  # under the WENO gate
+    qa = np.abs(q + rng.normal(0.0, 2e-4, q.shape)).astype(F)
+    qa[rng.random(q.shape) < 0.3] = F(0.0)
+    held = rng.normal(0.0, 1e-7, q.shape).astype(F)
+    s = np.zeros(sh["s"], F)
+    s[1 : nk + 1, NG : NG + nj, NG : NG + ni] = q
+    obc.fill(s, obc.KIND_SCALAR, sides, ni=ni, nj=nj, nk=nk)
+    weps = float(F(F(1e-2) * F(1e-18)))
+    dt = 1.5
+    adv = op.pdef_advection(qa, s, a, m, divx, ni=ni, nj=nj, nk=nk, dt=F(dt), dx=250.0, dy=300.0, dz=200.0,
+                            weps=weps, pdefweno=species != 0, sides=sides)
+    want = ost.compose_moisture(qa, held, adv, F(dt))
+    d = lambda x: to_dev(x, device)  # noqa: E731
+    out = torch.zeros((nk, nj, ni), dtype=torch.float32, device=device)
+    scratch = {n: torch.zeros((nk, nj, ni), dtype=torch.float32, device=device) for n in ("advx", "advy", "advz")}
+    launch_pdef_moisture(d(s), {k: d(a[k]) for k in ("rru", "rrv", "rrw")}, d(m), d(divx), d(qa), d(held), out,
+                         scratch, species=species, ni=ni, nj=nj, nk=nk, dt=dt, dx=250.0, dy=300.0, dz=200.0,
+                         lateral_sides=sides)
+    torch.cuda.synchronize()
+    assert_bits_equal(to_host(out), want, f"pdef moisture species={species} sides={sides}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nrk", [1, 3])
+def test_gpu_fused_scalar_compose_bit_exact(device, nrk):
+    import torch
+
+    from gm1_amd.scalar_moist_rk import launch_scalar_rk_compose
+    from oracle import boundary as obc
+    from oracle import stage as ost
+    from oracle.advection5 import cpu_scalar_advection5_inplace
+
+    ni, nj, nk, sides = 29, 18, 11, (2, 2, 1, 1)
+    rng = np.random.default_rng(7 + nrk)
+    a, m, divx = _moist_inputs(rng, ni, nj, nk)
+    sh = shapes(ni, nj, nk)
+    s = np.zeros(sh["s"], F)
+    s[1 : nk + 1, NG : NG + nj, NG : NG + ni] = rng.normal(0.0, 2.0, (nk, nj, ni)).astype(F)
+    obc.fill(s, obc.KIND_SCALAR, sides, ni=ni, nj=nj, nk=nk)
+    tha = rng.normal(0.0, 2.0, (nk, nj, ni)).astype(F)
+    held = rng.normal(0.0, 1e-3, (nk, nj, ni)).astype(F)
+    divterm = rng.normal(0.0, 1e-3, (nk, nj, ni)).astype(F)
+    adv = np.zeros((nk, nj, ni), F)
+    fl = {"x": np.zeros(sh["u"], F), "y": np.zeros(sh["v"], F), "z": np.zeros(sh["w"], F)}
+    weps = float(F(F(10.0) * F(1e-18)))
+    cpu_scalar_advection5_inplace(s, a, m, divx, adv, fl, ni=ni, nj=nj, nk=nk, nrk=nrk, dx=250.0, dy=300.0,
+                                  dz=200.0, weps=weps, lateral_sides=sides)
+    dttmp = F(F(1.5) / F(4 - nrk))
+    want = ost.compose_theta(tha, held, divterm, adv, dttmp)
+    d = lambda x: to_dev(x, device)  # noqa: E731
+    out = torch.zeros((nk, nj, ni), dtype=torch.float32, device=device)
+    launch_scalar_rk_compose(d(s), {k: d(a[k]) for k in ("rru", "rrv", "rrw")}, d(m), d(divx), d(tha), d(held),
+                             d(divterm), out, ni=ni, nj=nj, nk=nk, nrk=nrk, dt=1.5, dx=250.0, dy=300.0, dz=200.0,
+                             weps=weps, lateral_sides=sides)
+    torch.cuda.synchronize()
+    assert_bits_equal(to_host(out), want, f"fused compose nrk={nrk}")
