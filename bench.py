@@ -58,6 +58,9 @@ def parse_args(argv=None):
     p.add_argument("--cpu-sample", type=int, default=96, help="horizontal size of the oracle's sub-domain sample")
     p.add_argument("--cpu-steps", type=int, default=2)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--graph", action="store_true",
+                   help="time replays of the step captured into a HIP graph (the kernel timer then runs in an "
+                        "eager pass of the same length)")
     p.add_argument("--calibration-copy-mib", type=int, default=0,
                    help="issue one plain device copy of this size before the steps (HBM counter calibration)")
     return p.parse_args(argv)
@@ -147,19 +150,44 @@ def main(argv=None) -> int:
     print(f"[bench] {args.warmup} warmup steps done", file=sys.stderr, flush=True)
 
     small_steps = {6: 11, 8: 15, 10: 19}.get(cfg.nsound, 3 * cfg.nsound)
-    per_step = {"k_snd_column": small_steps, "k_snd_pgrad": small_steps, "k_scalar_adv5": 3}[args.kernel]
-    timing.arm(args.kernel, per_step * args.steps + 8)
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    start.record(stream)
-    for _ in range(args.steps):
-        launch_cuda_gm1_large_step(st)
-    stop.record(stream)
-    torch.cuda.synchronize()
-    elapsed_ms = start.elapsed_time(stop)
-    kernel_ms, launches = timing.collect()
-    print(f"[bench] {args.steps} timed steps: {elapsed_ms:.2f} ms", file=sys.stderr, flush=True)
+    per_step = {"k_snd_column": small_steps, "k_snd_pgrad": small_steps, "k_scalar_adv5": 3 * (1 + st.numq)}[args.kernel]
+
+    def eager_pass(timed: bool):
+        if timed:
+            timing.arm(args.kernel, per_step * args.steps + 8)
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        start.record(stream)
+        for _ in range(args.steps):
+            launch_cuda_gm1_large_step(st)
+        stop.record(stream)
+        host_s = time.perf_counter() - t0
+        torch.cuda.synchronize()
+        kern = timing.collect() if timed else (0.0, 0)
+        return start.elapsed_time(stop), host_s, kern
+
+    elapsed_ms, host_s, (kernel_ms, launches) = eager_pass(True)
+    print(f"[bench] {args.steps} eager steps: {elapsed_ms:.2f} ms on the GPU, {host_s * 1e3:.2f} ms to enqueue",
+          file=sys.stderr, flush=True)
+    mode = "eager"
+    if args.graph:
+        from gm1_amd.graph import Gm1LargeStepGraph
+
+        g = Gm1LargeStepGraph(st, steps=1, stream=stream)
+        g.replay()
+        torch.cuda.synchronize()
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for _ in range(args.steps):
+            g.replay()
+        stop.record(stream)
+        torch.cuda.synchronize()
+        elapsed_ms = start.elapsed_time(stop)
+        mode = "hip-graph replay"
+        print(f"[bench] {args.steps} graph replays: {elapsed_ms:.2f} ms", file=sys.stderr, flush=True)
 
     w = st.a["w3d"]
     if not torch.isfinite(w).all():
@@ -191,6 +219,7 @@ def main(argv=None) -> int:
                         f"dx=dy=dz={cfg.dx:g} m, open-radiative laterals, dt={cfg.dt:g} s, "
                         f"{cfg.nsound} substeps ({small_steps} small steps/step)",
             "cells": cfg.cells,
+            "launch": mode,
         },
         "roofline": {
             "bound": "hbm",

@@ -51,7 +51,10 @@ def stream_handle(stream: Any | None) -> int:
     return int(s.cuda_stream)
 
 
-def record(stream: Any | None) -> torch.cuda.Event:
+def record(stream: Any | None) -> torch.cuda.Event | None:
+    """A ready event recorded after the launches; None while the stream is being captured."""
+    if torch.cuda.is_current_stream_capturing():
+        return None
     event = torch.cuda.Event()
     event.record(stream if stream is not None else torch.cuda.current_stream())
     return event

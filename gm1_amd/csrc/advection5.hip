@@ -17,7 +17,7 @@ using namespace gm1;
 namespace {
 
 constexpr int TX = 64;
-constexpr int TY = 4;
+constexpr int TY = 8;
 constexpr int NT = TX * TY;
 
 // ---------------------------------------------------------------------------
@@ -226,7 +226,7 @@ struct ScalarAdvArgs {
   float* advy;
   float* advz;
   int pdefweno;
-  int ni, nj, nk, kchunk;
+  int ni, nj, nk, kchunk, nchunks;
   float rdx, rdy, rdsf;
   double weps;
   unsigned obc;
@@ -239,9 +239,13 @@ __global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
   __shared__ float lfy[TY + 1][TX];
   const int ni = a.ni, nj = a.nj, nk = a.nk;
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-  const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+  // tiles ordered i fastest, then j, then column chunk; each XCD sweeps a
+  // contiguous range so the y-halo rows a tile shares stay in one L2
+  const int gxt = (ni + TX - 1) / TX, gyt = (nj + TY - 1) / TY;
+  const int tile = xcd_tile((int)blockIdx.x, gxt * gyt * a.nchunks);
+  const int i0 = (tile % gxt) * TX, j0 = ((tile / gxt) % gyt) * TY;
   const int i = i0 + tx, j = j0 + ty;
-  const int kb = 1 + blockIdx.z * a.kchunk;
+  const int kb = 1 + (tile / (gxt * gyt)) * a.kchunk;
   const int ke = min(nk, kb + a.kchunk - 1);
   const int nxs = sx(ni), nys = sy(nj), nxu = ux(ni), nyv = vy(nj);
   const bool live = (i < ni) && (j < nj);
@@ -366,7 +370,7 @@ struct PdefArgs {
   const float* qa; const float* held;
   const float* advx; const float* advy; const float* advz;
   float* out;
-  int ni, nj, nk, kchunk;
+  int ni, nj, nk, kchunk, nchunks;
   float dt, rdt, rdx, rdy, rdz;
   unsigned obc;
 };
@@ -443,10 +447,12 @@ __global__ __launch_bounds__(RT) void k_pdef_repair(PdefArgs a) {
   __shared__ float lcy[RY + 4][RX];
   const int ni = a.ni, nj = a.nj, nk = a.nk;
   const int tx = threadIdx.x % RX, ty = threadIdx.x / RX;
-  const int i0 = blockIdx.x * RX, j0 = blockIdx.y * RY;
+  const int gxt = (ni + RX - 1) / RX, gyt = (nj + RY - 1) / RY;
+  const int tile = xcd_tile((int)blockIdx.x, gxt * gyt * a.nchunks);
+  const int i0 = (tile % gxt) * RX, j0 = ((tile / gxt) % gyt) * RY;
   const int i = i0 + tx, j = j0 + ty;
   const bool live = (i < ni) && (j < nj);
-  const int kb = 1 + blockIdx.z * a.kchunk;
+  const int kb = 1 + (tile / (gxt * gyt)) * a.kchunk;
   const int ke = min(nk, kb + a.kchunk - 1);
   const unsigned wbc = side_of(a.obc, 0), ebc = side_of(a.obc, 1), sbc = side_of(a.obc, 2), nbc = side_of(a.obc, 3);
   const int nxs = sx(ni), nys = sy(nj), nxu = ux(ni), nyv = vy(nj);
@@ -516,7 +522,8 @@ int chunks_for(int tiles, int nk, int* kchunk) {
 int launch_scalar_tile(ScalarAdvArgs& a, int use_weno, int mode, hipStream_t s) {
   const int tiles = ((a.ni + TX - 1) / TX) * ((a.nj + TY - 1) / TY);
   const int chunks = chunks_for(tiles, a.nk, &a.kchunk);
-  dim3 grid((a.ni + TX - 1) / TX, (a.nj + TY - 1) / TY, chunks);
+  a.nchunks = chunks;
+  dim3 grid(tiles * chunks);
   if (mode == MODE_TENDENCY) {
     if (use_weno)
       GM1_TIMED(TK_SCALAR_ADV5, s, hipLaunchKernelGGL((k_scalar_adv5<true, MODE_TENDENCY>), grid, dim3(NT), 0, s, a));
@@ -616,8 +623,8 @@ int gm1hip_pdef_moisture_stage(const float* scalar, const float* rru, const floa
   p.obc = obc;
   const int tiles = ((ni + RX - 1) / RX) * ((nj + RY - 1) / RY);
   const int chunks = chunks_for(tiles, nk, &p.kchunk);
-  dim3 grid((ni + RX - 1) / RX, (nj + RY - 1) / RY, chunks);
-  hipLaunchKernelGGL(k_pdef_repair, grid, dim3(RT), 0, s, p);
+  p.nchunks = chunks;
+  hipLaunchKernelGGL(k_pdef_repair, dim3(tiles * chunks), dim3(RT), 0, s, p);
   GM1_LAUNCH_CHECK();
   return 0;
 }

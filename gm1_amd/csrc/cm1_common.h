@@ -35,6 +35,17 @@ GM1_DEV int sy(int nj) { return nj + 2 * NG; }      // scalar / w / u rows
 GM1_DEV int ux(int ni) { return ni + 1 + 2 * NG; }  // u row length
 GM1_DEV int vy(int nj) { return nj + 1 + 2 * NG; }  // v rows
 
+// Tile index for linear workgroup id `linear` of `n`, chosen so that each of
+// the eight XCDs -- which receive workgroups round-robin in dispatch order --
+// works on one contiguous range of tiles, keeping neighbouring rows in the
+// same L2.
+GM1_DEV int xcd_tile(int linear, int n) {
+  constexpr int X = 8;
+  const int per = n / X, rem = n % X;
+  const int x = linear % X, r = linear / X;
+  return (x < rem) ? x * (per + 1) + r : rem * (per + 1) + (x - rem) * per + r;
+}
+
 // Pressure family (nk+2, nj+2, ni+2).
 GM1_DEV int64_t pix(int k, int j, int i, int ni, int nj) {
   return ((int64_t)k * (nj + 2) + j) * (int64_t)(ni + 2) + i;

@@ -69,6 +69,11 @@ struct KesArgs {
 };
 
 // ------------------------------------------------------------------ column phase
+// Column loops load KB levels at a time (independent loads in flight) and
+// then run the level arithmetic in k order, so every binary64 sum and every
+// carried value is the sequential one.
+constexpr int KB = 8;
+
 __global__ __launch_bounds__(64) void k_kes_column(KesArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
@@ -76,48 +81,84 @@ __global__ __launch_bounds__(64) void k_kes_column(KesArgs a) {
   if (i >= ni) return;
   const KesConst c = kes_const();
   const int64_t plane = (int64_t)nj * ni, h = (int64_t)j * ni + i;
+  const float* __restrict__ rho = a.rho;
+  const float* __restrict__ rmh = a.rmh;
+  const float* __restrict__ mh = a.mh;
+  float* __restrict__ vt = a.vt;
+  float* __restrict__ qr = a.qr;
   const float ru = a.ruh[i], rv = a.rvh[j];
   float* qs[3] = {a.qv, a.qc, a.qr};
   const float rmax[3] = {0.0f, 1.0e-14f, 1.0e-14f};
   // temperature at the lowest level before any phase (pdefq leaves theta, pi alone)
   const float t0 = (a.th0[h] + a.th3d[h]) * (a.pi0[h] + a.pp3d[h]);
   for (int sp = 0; sp < 3; ++sp) {
-    float* q = qs[sp];
+    float* __restrict__ q = qs[sp];
     double t1 = 0.0, t2 = 0.0, a1 = 0.0, a2 = 0.0;
-    for (int k = 0; k < nk; ++k) {
-      const int64_t p = k * plane + h;
-      const float r = a.rho[p];
-      float v = q[p];
-      t1 += (double)(r * v);
-      a1 += (double)((((r * v) * ru) * rv) * a.rmh[p]);
-      if (v < rmax[sp]) {
-        v = 0.0f;
-        q[p] = v;
-      }
-      t2 += (double)(r * v);
+    for (int k0 = 0; k0 < nk; k0 += KB) {
+      float r[KB], v[KB], m[KB];
+#pragma unroll
+      for (int u = 0; u < KB; ++u)
+        if (k0 + u < nk) {
+          const int64_t p = (k0 + u) * plane + h;
+          r[u] = rho[p];
+          v[u] = q[p];
+          m[u] = rmh[p];
+        }
+#pragma unroll
+      for (int u = 0; u < KB; ++u)
+        if (k0 + u < nk) {
+          t1 += (double)(r[u] * v[u]);
+          a1 += (double)((((r[u] * v[u]) * ru) * rv) * m[u]);
+          if (v[u] < rmax[sp]) {
+            v[u] = 0.0f;
+            q[(k0 + u) * plane + h] = 0.0f;
+          }
+          t2 += (double)(r[u] * v[u]);
+        }
     }
     double t3 = (t1 + 1.0e-20) / (t2 + 1.0e-20);
     if (t3 < 0.0) t3 = 1.0;
-    for (int k = 0; k < nk; ++k) {
-      const int64_t p = k * plane + h;
-      const float v = (float)(t3 * (double)q[p]);
-      q[p] = v;
-      a2 += (double)((((a.rho[p] * v) * ru) * rv) * a.rmh[p]);
+    for (int k0 = 0; k0 < nk; k0 += KB) {
+      float r[KB], v[KB], m[KB];
+#pragma unroll
+      for (int u = 0; u < KB; ++u)
+        if (k0 + u < nk) {
+          const int64_t p = (k0 + u) * plane + h;
+          r[u] = rho[p];
+          v[u] = q[p];
+          m[u] = rmh[p];
+        }
+#pragma unroll
+      for (int u = 0; u < KB; ++u)
+        if (k0 + u < nk) {
+          const float nv = (float)(t3 * (double)v[u]);
+          q[(k0 + u) * plane + h] = nv;
+          a2 += (double)((((r[u] * nv) * ru) * rv) * m[u]);
+        }
     }
     a.colbud[(int64_t)sp * plane + h] = a2 - a1;
   }
   // fall speed
-  for (int k = 0; k < nk; ++k) {
-    const int64_t p = k * plane + h;
-    const float r = a.rho[p], q = a.qr[p];
-    float v = 0.0f;
-    if (q > 1.0e-12f) v = (14.34f * pow_r(r * q, 0.1346f)) * sqrtf(1.15f / r);
-    a.vt[p] = v;
+  for (int k0 = 0; k0 < nk; k0 += KB) {
+    float r[KB], q[KB];
+#pragma unroll
+    for (int u = 0; u < KB; ++u)
+      if (k0 + u < nk) {
+        r[u] = rho[(k0 + u) * plane + h];
+        q[u] = qr[(k0 + u) * plane + h];
+      }
+#pragma unroll
+    for (int u = 0; u < KB; ++u)
+      if (k0 + u < nk) {
+        float v = 0.0f;
+        if (q[u] > 1.0e-12f) v = (14.34f * pow_r(r[u] * q[u], 0.1346f)) * sqrtf(1.15f / r[u]);
+        vt[(k0 + u) * plane + h] = v;
+      }
   }
   {  // surface energy flux of falling rain
-    float term = a.dt * a.vt[h];
+    float term = a.dt * vt[h];
     term = term * a.rr[h];
-    term = term * a.qr[h];
+    term = term * qr[h];
     term = term * (CPL * t0 - c.lv1);
     term = term * ru;
     term = term * rv;
@@ -126,33 +167,72 @@ __global__ __launch_bounds__(64) void k_kes_column(KesArgs a) {
   // fallout
   const float rdz = 1.0f / a.dz;
   float crmax = 0.0f;
-  for (int k = 0; k < nk; ++k) {
-    const int64_t p = k * plane + h;
-    crmax = fmaxf(crmax, ((a.vt[p] * a.dt) * rdz) * a.mh[p]);
+  for (int k0 = 0; k0 < nk; k0 += KB) {
+    float v[KB], m[KB];
+#pragma unroll
+    for (int u = 0; u < KB; ++u)
+      if (k0 + u < nk) {
+        v[u] = vt[(k0 + u) * plane + h];
+        m[u] = mh[(k0 + u) * plane + h];
+      }
+#pragma unroll
+    for (int u = 0; u < KB; ++u)
+      if (k0 + u < nk) crmax = fmaxf(crmax, ((v[u] * a.dt) * rdz) * m[u]);
   }
   int nfall = (int)(crmax + 1.0f);
   if (nfall < 1) nfall = 1;
-  for (int k = nk - 2; k >= 0; --k) {
-    const int64_t p = k * plane + h;
-    if (a.qr[p] < 1.0e-8f) a.vt[p] = a.vt[p + plane];
+  {  // levels with little rain take the speed of the level above, top down
+    float above = vt[(int64_t)(nk - 1) * plane + h];
+    for (int k1 = nk - 2; k1 >= 0; k1 -= KB) {
+      float q[KB], v[KB];
+#pragma unroll
+      for (int u = 0; u < KB; ++u)
+        if (k1 - u >= 0) {
+          q[u] = qr[(k1 - u) * plane + h];
+          v[u] = vt[(k1 - u) * plane + h];
+        }
+#pragma unroll
+      for (int u = 0; u < KB; ++u)
+        if (k1 - u >= 0) {
+          if (q[u] < 1.0e-8f) {
+            v[u] = above;
+            vt[(k1 - u) * plane + h] = above;
+          }
+          above = v[u];
+        }
+    }
   }
   const float dtfall = a.dt / (float)nfall;
   double bud = 0.0;
   for (int sub = 0; sub < nfall; ++sub) {
-    float rqk = (a.rho[h] * a.vt[h]) * fmaxf(0.0f, a.qr[h]);
+    float rqk = (rho[h] * vt[h]) * fmaxf(0.0f, qr[h]);
     const float rq0 = rqk;
-    for (int k = 0; k < nk; ++k) {
-      const int64_t p = k * plane + h;
-      const float rqk1 = (k + 1 < nk) ? (a.rho[p + plane] * a.vt[p + plane]) * fmaxf(0.0f, a.qr[p + plane]) : 0.0f;
-      const float delta = (((dtfall * (rqk1 - rqk)) * rdz) * a.mh[p]) / a.rho[p];
-      a.qr[p] = a.qr[p] + delta;
-      rqk = rqk1;
+    for (int k0 = 0; k0 < nk; k0 += KB) {
+      // old qr of levels k0..k0+KB (the update of level k reads only level k+1's old value)
+      float r[KB + 1], v[KB + 1], q[KB + 1], m[KB];
+#pragma unroll
+      for (int u = 0; u <= KB; ++u)
+        if (k0 + u < nk) {
+          const int64_t p = (k0 + u) * plane + h;
+          r[u] = rho[p];
+          v[u] = vt[p];
+          q[u] = qr[p];
+          if (u < KB) m[u] = mh[p];
+        }
+#pragma unroll
+      for (int u = 0; u < KB; ++u)
+        if (k0 + u < nk) {
+          const float rqk1 = (k0 + u + 1 < nk) ? (r[u + 1] * v[u + 1]) * fmaxf(0.0f, q[u + 1]) : 0.0f;
+          const float delta = (((dtfall * (rqk1 - rqk)) * rdz) * m[u]) / r[u];
+          qr[(k0 + u) * plane + h] = q[u] + delta;
+          rqk = rqk1;
+        }
     }
     a.rain[h] = a.rain[h] + (0.1f * dtfall) * rq0;
     a.prate[h] = rq0;
     float term = dtfall * a.rr[h];
-    term = term * a.vt[h];
-    term = term * fmaxf(0.0f, a.qr[h]);
+    term = term * vt[h];
+    term = term * fmaxf(0.0f, qr[h]);
     term = term * ru;
     term = term * rv;
     bud += (double)term;
@@ -331,20 +411,48 @@ __global__ __launch_bounds__(64) void k_kes_cell(KesArgs a) {
 }
 
 // ------------------------------------------------------------------ budgets
-// One block: rows of the column budgets (i order), then rows in j order into
-// asq/qbudget; wave partials per level in (j, segment) order, then levels in
-// k order; status maxima.
-__global__ __launch_bounds__(256) void k_kes_budgets(const double* __restrict__ colbud,
-                                                     const double* __restrict__ wavebud,
-                                                     const int* __restrict__ wavestat, double* __restrict__ rowscr,
-                                                     double* __restrict__ asq, double* __restrict__ qbudget,
-                                                     int* __restrict__ status, int ni, int nj, int nk, int nseg,
-                                                     float dx, float dy, float dz) {
+// Sum of n doubles in index order, eight loads in flight at a time (the
+// additions stay in order, so the result is the sequential sum).
+GM1_DEV double ordered_sum(const double* __restrict__ v, int64_t n, int64_t stride) {
+  double s = 0.0;
+  int64_t q = 0;
+  for (; q + 8 <= n; q += 8) {
+    double b[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) b[u] = v[(q + u) * stride];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += b[u];
+  }
+  for (; q < n; ++q) s += v[q * stride];
+  return s;
+}
+
+// rows: colbud rows (5 x nj, i order) and level sums of the wave partials
+// (5 x nk, (j, segment) order), one thread each.
+__global__ __launch_bounds__(256) void k_kes_rows(const double* __restrict__ colbud, const double* __restrict__ wavebud,
+                                                  double* __restrict__ rowscr, int ni, int nj, int nk, int nseg) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nrow = 5 * nj, nlev = 5 * nk;
+  if (r < nrow) {
+    const int b = r / nj, j = r % nj;
+    rowscr[r] = ordered_sum(colbud + (int64_t)b * nj * ni + (int64_t)j * ni, ni, 1);
+  } else if (r < nrow + nlev) {
+    const int q = r - nrow, b = q / nk, k = q % nk;
+    const int64_t lstride = (int64_t)nj * nseg;
+    rowscr[r] = ordered_sum(wavebud + (int64_t)b * nk * lstride + (int64_t)k * lstride, lstride, 1);
+  }
+}
+
+// The folds into asq/qbudget in the reference's order (j rows, k levels) and
+// the satadj status maxima.
+__global__ __launch_bounds__(256) void k_kes_budgets(const double* __restrict__ rowscr,
+                                                     const int* __restrict__ wavestat, double* __restrict__ asq,
+                                                     double* __restrict__ qbudget, int* __restrict__ status, int nj,
+                                                     int nk, int nseg, float dx, float dy, float dz) {
   __shared__ int smax[256];
   __shared__ int sfail[256];
   const int t = threadIdx.x;
-  const int64_t plane = (int64_t)nj * ni;
-  {  // status: strided partial maxima, then a tree in LDS
+  {
     int mi = 0, mf = 0;
     const int64_t n = (int64_t)nk * nj * nseg;
     for (int64_t q = t; q < n; q += blockDim.x) {
@@ -353,22 +461,6 @@ __global__ __launch_bounds__(256) void k_kes_budgets(const double* __restrict__ 
     }
     smax[t] = mi;
     sfail[t] = mf;
-  }
-  // column budgets: 5 x nj row sums in i order
-  for (int r = t; r < 5 * nj; r += blockDim.x) {
-    const int b = r / nj, j = r % nj;
-    double s = 0.0;
-    for (int i = 0; i < ni; ++i) s += colbud[b * plane + (int64_t)j * ni + i];
-    rowscr[r] = s;
-  }
-  // level budgets: 5 x nk sums over (j, segment)
-  const int64_t lstride = (int64_t)nj * nseg;
-  for (int r = t; r < 5 * nk; r += blockDim.x) {
-    const int b = r / nk, k = r % nk;
-    double s = 0.0;
-    const double* w = wavebud + (int64_t)b * nk * lstride + (int64_t)k * lstride;
-    for (int64_t q = 0; q < lstride; ++q) s += w[q];
-    rowscr[5 * nj + r] = s;
   }
   __syncthreads();
   const double volume = (double)((dx * dy) * dz);
@@ -466,18 +558,18 @@ __global__ __launch_bounds__(64) void k_mass_partials(const float* __restrict__ 
 
 // ledger[0] = dry mass (levels in k order); with mass1 > 0 also the f32
 // Exner correction into corr[0].
-__global__ __launch_bounds__(256) void k_mass_fold(const double* __restrict__ part, double* __restrict__ levels,
-                                                   double* __restrict__ ledger, float* __restrict__ corr, int nk,
-                                                   int nj, int nseg, double mass1, int ncells) {
-  const int t = threadIdx.x;
+__global__ __launch_bounds__(256) void k_mass_levels(const double* __restrict__ part, double* __restrict__ levels,
+                                                     int nk, int nj, int nseg) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= 2 * nk) return;
+  const int which = r / nk, k = r % nk;
   const int64_t lstride = (int64_t)nj * nseg;
-  for (int r = t; r < 2 * nk; r += blockDim.x) {
-    const int which = r / nk, k = r % nk;
-    double s = 0.0;
-    for (int64_t q = 0; q < lstride; ++q) s += part[2 * ((int64_t)k * lstride + q) + which];
-    levels[r] = s;
-  }
-  __syncthreads();
+  levels[r] = ordered_sum(part + 2 * (int64_t)k * lstride + which, lstride, 2);
+}
+
+__global__ __launch_bounds__(64) void k_mass_fold(const double* __restrict__ levels, double* __restrict__ ledger,
+                                                  float* __restrict__ corr, int nk, double mass1, int ncells) {
+  const int t = threadIdx.x;
   if (t == 0) {
     double m2 = 0.0, ps = 0.0;
     for (int k = 0; k < nk; ++k) m2 += levels[k];
@@ -553,8 +645,11 @@ int gm1hip_kessler_driver(const float* ruh, const float* rvh, const float* rmh, 
   a.wavestat = (int*)base;
   hipLaunchKernelGGL(k_kes_column, dim3((ni + 63) / 64, nj), dim3(64), 0, s, a);
   hipLaunchKernelGGL(k_kes_cell, dim3(a.nseg, nj, nk), dim3(64), 0, s, a);
-  hipLaunchKernelGGL(k_kes_budgets, dim3(1), dim3(256), 0, s, a.colbud, a.wavebud, a.wavestat, rows, asq, qbudget,
-                     status, ni, nj, nk, a.nseg, dx, dy, dz);
+  const int nsum = 5 * nj + 5 * nk;
+  hipLaunchKernelGGL(k_kes_rows, dim3((nsum + 255) / 256), dim3(256), 0, s, a.colbud, a.wavebud, rows, ni, nj, nk,
+                     a.nseg);
+  hipLaunchKernelGGL(k_kes_budgets, dim3(1), dim3(256), 0, s, rows, a.wavestat, asq, qbudget, status, nj, nk, a.nseg,
+                     dx, dy, dz);
   GM1_LAUNCH_CHECK();
   return 0;
 }
@@ -589,8 +684,8 @@ int gm1hip_apmasscon(const float* ruh, const float* rvh, const float* rmh, const
   double* levels = part + 2LL * nk * nj * nseg;
   hipLaunchKernelGGL(k_mass_partials, dim3(nseg, nj, nk), dim3(64), 0, s, rho, ruh, rvh, rmh, pi0, pp3d, part, ni,
                      nj, nk, nseg, mass1 > 0.0 ? 1 : 0);
-  hipLaunchKernelGGL(k_mass_fold, dim3(1), dim3(256), 0, s, part, levels, ledger, corr, nk, nj, nseg, mass1,
-                     ni * nj * nk);
+  hipLaunchKernelGGL(k_mass_levels, dim3((2 * nk + 255) / 256), dim3(256), 0, s, part, levels, nk, nj, nseg);
+  hipLaunchKernelGGL(k_mass_fold, dim3(1), dim3(64), 0, s, levels, ledger, corr, nk, mass1, ni * nj * nk);
   if (mass1 > 0.0) {
     const int64_t n = (int64_t)ni * nj * nk;
     hipLaunchKernelGGL(k_mass_apply, dim3(gm1_blocks(n, 256)), dim3(256), 0, s, corr, pp3d, pi0, th0, th3d, qv, prs,

@@ -180,13 +180,100 @@ __global__ __launch_bounds__(64) void k_snd_ssopen(SoundArgs a) {
   }
 }
 
-// ------------------------------------------------------------------ pgrad
-__global__ __launch_bounds__(256) void k_snd_pgrad(SoundArgs a) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;  // 0..ni
-  const int y = blockIdx.y;                             // 0..nj
-  const int k = blockIdx.z + 1;
+// k-parallel form of the same step: one workgroup per edge column snapshots
+// the column's original boundary and interior values into LDS, then every
+// level is updated at once (each level reads only original values).
+constexpr int SS_T = 64;
+constexpr int SS_MAXK = 256;
+
+__global__ __launch_bounds__(SS_T) void k_snd_ssopen2(SoundArgs a) {
+  __shared__ float sb[SS_MAXK + 2];
+  __shared__ float sc[SS_MAXK + 2];
+  const int t = blockIdx.x;
   const int ni = a.ni, nj = a.nj, nk = a.nk;
-  if (x > ni) return;
+  int side, idx;
+  if (t < nj) { side = 0; idx = t; }
+  else if (t < 2 * nj) { side = 1; idx = t - nj; }
+  else if (t < 2 * nj + ni) { side = 2; idx = t - 2 * nj; }
+  else { side = 3; idx = t - 2 * nj - ni; }
+  if (side_of(a.sides, side) != 2u) return;  // uniform over the workgroup
+  const bool xdir = side < 2;
+  float* q = xdir ? a.u3d : a.v3d;
+  const float* ten = xdir ? a.uten : a.vten;
+  const int nx = xdir ? ux(ni) : sx(ni);
+  const int ny = xdir ? sy(nj) : vy(nj);
+  const int64_t st = (int64_t)ny * nx;
+  int64_t cb, cc;
+  int j0 = 0, i0 = 0;
+  if (side == 0) { j0 = idx; cb = (int64_t)(idx + NG) * nx + NG; cc = cb + 1; }
+  else if (side == 1) { j0 = idx; cb = (int64_t)(idx + NG) * nx + NG + ni; cc = cb - 1; }
+  else if (side == 2) { i0 = idx; cb = (int64_t)NG * nx + idx + NG; cc = cb + nx; }
+  else { i0 = idx; cb = (int64_t)(NG + nj) * nx + idx + NG; cc = cb - nx; }
+  for (int k = 1 + (int)threadIdx.x; k <= nk; k += SS_T) {
+    sb[k] = q[(int64_t)k * st + cb];
+    sc[k] = q[(int64_t)k * st + cc];
+  }
+  __syncthreads();
+  const float* cgs = P_cgs(a);
+  const float* cgt = P_cgt(a);
+  const float topb = (cgt[0] * sb[nk] + cgt[1] * sb[nk - 1]) + cgt[2] * sb[nk - 2];
+  const float topc = (cgt[0] * sc[nk] + cgt[1] * sc[nk - 1]) + cgt[2] * sc[nk - 2];
+  const float botb = (cgs[0] * sb[1] + cgs[1] * sb[2]) + cgs[2] * sb[3];
+  const float botc = (cgs[0] * sc[1] + cgs[1] * sc[2]) + cgs[2] * sc[3];
+  const float rd = xdir ? a.rdx : a.rdy;
+  const int64_t plane = (int64_t)nj * ni;
+  float gzv, rg_b, rg_c, hmet;
+  const float* gm;
+  int64_t mh;
+  const float* radbc;
+  int rix;
+  if (side == 0) { gzv = a.gz[(int64_t)j0 * ni]; rg_b = a.rgzu[(int64_t)j0 * (ni + 1)]; rg_c = a.rgzu[(int64_t)j0 * (ni + 1) + 1]; hmet = a.uh[NG]; gm = a.gx; mh = (int64_t)j0 * ni; radbc = a.radbcw; rix = j0 + NG; }
+  else if (side == 1) { gzv = a.gz[(int64_t)j0 * ni + ni - 1]; rg_b = a.rgzu[(int64_t)j0 * (ni + 1) + ni]; rg_c = a.rgzu[(int64_t)j0 * (ni + 1) + ni - 1]; hmet = a.uh[NG + ni - 1]; gm = a.gx; mh = (int64_t)j0 * ni + ni - 1; radbc = a.radbce; rix = j0 + NG; }
+  else if (side == 2) { gzv = a.gz[i0]; rg_b = a.rgzv[i0]; rg_c = a.rgzv[ni + i0]; hmet = a.vh[NG]; gm = a.gy; mh = i0; radbc = a.radbcs; rix = i0 + NG; }
+  else { gzv = a.gz[(int64_t)(nj - 1) * ni + i0]; rg_b = a.rgzv[(int64_t)nj * ni + i0]; rg_c = a.rgzv[(int64_t)(nj - 1) * ni + i0]; hmet = a.vh[NG + nj - 1]; gm = a.gy; mh = (int64_t)(nj - 1) * ni + i0; radbc = a.radbcn; rix = i0 + NG; }
+  const int rstride = xdir ? (nj + 2 * NG) : (ni + 2 * NG);
+  const bool lowside = (side == 0 || side == 2);
+  for (int k = 1 + (int)threadIdx.x; k <= nk; k += SS_T) {
+    const float qb = sb[k], qc = sc[k];
+    float hib, hic, lob, loc;
+    if (k < nk) {
+      hib = at_w_mid(a, k + 1, qb, sb[k + 1]);
+      hic = at_w_mid(a, k + 1, qc, sc[k + 1]);
+    } else {
+      hib = topb;
+      hic = topc;
+    }
+    if (k == 1) {
+      lob = botb;
+      loc = botc;
+    } else {
+      lob = at_w_mid(a, k, sb[k - 1], qb);
+      loc = at_w_mid(a, k, sc[k - 1], qc);
+    }
+    float h;
+    if (lowside) h = gzv * (qc * rg_c - qb * rg_b);
+    else h = gzv * (qb * rg_b - qc * rg_c);
+    h = (h * rd) * hmet;
+    float vert = gm[(int64_t)k * plane + mh] * (hic + hib) - gm[(int64_t)(k - 1) * plane + mh] * (loc + lob);
+    vert = (0.5f * vert) * P_rdsf(a)[k];
+    const float rad = (-radbc[(int64_t)k * rstride + rix]) * (h + vert);
+    const float tend = rad + ten[(int64_t)k * st + cb];
+    q[(int64_t)k * st + cb] = qb + a.dts * tend;
+  }
+}
+
+// ------------------------------------------------------------------ pgrad
+constexpr int PG_TX = 64, PG_TY = 4;
+
+__global__ __launch_bounds__(PG_TX * PG_TY) void k_snd_pgrad(SoundArgs a) {
+  const int ni = a.ni, nj = a.nj, nk = a.nk;
+  const int gx = (ni + 1 + PG_TX - 1) / PG_TX, gy = (nj + 1 + PG_TY - 1) / PG_TY;
+  const int tile = xcd_tile((int)blockIdx.x, gx * gy * nk);
+  const int k = tile / (gx * gy) + 1;
+  const int rem = tile % (gx * gy);
+  const int x = (rem % gx) * PG_TX + (int)threadIdx.x % PG_TX;  // 0..ni
+  const int y = (rem / gx) * PG_TY + (int)threadIdx.x / PG_TX;  // 0..nj
+  if (x > ni || y > nj) return;
   const float halfcp = a.cp * 0.5f, eighth = 0.125f;
   const float* rds = P_rds(a);
   const float* ppd = a.ppd;
@@ -427,6 +514,235 @@ __global__ __launch_bounds__(64) void k_snd_column(SoundArgs a, int n) {
   }
 }
 
+// ------------------------------------------------------------------ column (LDS-staged)
+// The same statements as k_snd_column, scheduled for the CDNA memory system:
+// a workgroup owns C2W columns of one row and all nk levels.  Phase A
+// (k-parallel, C2S level slots of contiguous chunks) forms fpk and the
+// per-level terms of the tridiagonal system into LDS; phase B runs only the
+// Thomas recurrence, one lane per column, out of LDS; phase C (k-parallel)
+// does the pressure update, the filter and the time averages.  fpk and the
+// elimination coefficients never reach HBM.  Workgroups are mapped so each
+// XCD sweeps a contiguous band of rows.
+constexpr int C2W = 32;
+constexpr int C2T = 512;
+constexpr int C2S = C2T / C2W;
+constexpr int C2PLANES = 6;  // fpk, pp, s1 = pk1+qk, s2 = pk2-qk, mm -> ec, dd -> ed -> w
+
+int column2_lds_bytes(int nk) { return C2PLANES * (nk + 1) * C2W * (int)sizeof(float); }
+
+__global__ __launch_bounds__(C2T) void k_snd_column2(SoundArgs a, int n) {
+  extern __shared__ float lds[];
+  __shared__ float lw1[C2W];
+  const int ni = a.ni, nj = a.nj, nk = a.nk;
+  const int gx = (ni + 1 + C2W - 1) / C2W;
+  const int tile = xcd_tile((int)blockIdx.x, gx * (nj + 1));
+  const int j = tile / gx;
+  const int c = threadIdx.x % C2W, slot = threadIdx.x / C2W;
+  const int i = (tile % gx) * C2W + c;
+  const bool inside = (i <= ni);
+  const bool phys = (i < ni) && (j < nj);
+  const bool last = (n == a.nloop);
+  const float tavg = 1.0f / (float)a.nloop;
+  const int nxs = sx(ni), nys = sy(nj), nxu = ux(ni), nyv = vy(nj);
+  const int64_t wst = (int64_t)nys * nxs;
+  const int64_t wc = (int64_t)(j + NG) * nxs + (i + NG);
+  const int64_t uc = (int64_t)(j + NG) * nxu + (i + NG);
+  const int64_t ust = (int64_t)nys * nxu;
+  const int64_t vc = (int64_t)(j + NG) * nxs + (i + NG);
+  const int64_t vst = (int64_t)nyv * nxs;
+  float* w = a.w3d;
+  const int L = (nk + 1) * C2W;
+  float* Lfpk = lds;
+  float* Lpp = lds + L;
+  float* Ls1 = lds + 2 * L;
+  float* Ls2 = lds + 3 * L;
+  float* Lmm = lds + 4 * L;
+  float* Ldd = lds + 5 * L;
+  auto at = [&](int k) { return k * C2W + c; };
+  const int chunk = (nk + C2S - 1) / C2S;
+  const int kb = 1 + slot * chunk, ke = min(nk, kb + chunk - 1);
+  const float asq = a.alph * a.alph;
+  const float smeps = 1.0e-20f;
+
+  // ---------------- phase A: fpk and the per-level terms
+  if (phys && kb <= ke) {
+    const int pj = j + 1, pi = i + 1;
+    const int64_t h = (int64_t)j * ni + i, plane = (int64_t)nj * ni;
+    const float* sigma = P_sigma(a);
+    const float* sigmaf = P_sigmaf(a);
+    const float* rds = P_rds(a);
+    const float* rdsf = P_rdsf(a);
+    const float g = a.gz[h], ddx = a.dzdx[h], ddy = a.dzdy[h];
+    const float rgw = a.rgzu[(int64_t)j * (ni + 1) + i], rge = a.rgzu[(int64_t)j * (ni + 1) + i + 1];
+    const float rgs = a.rgzv[(int64_t)j * ni + i], rgn = a.rgzv[(int64_t)(j + 1) * ni + i];
+    const float temx = (a.dts * 0.5f) * a.rdx, temy = (a.dts * 0.5f) * a.rdy;
+    auto faces = [&](int k) {
+      ColumnFaces f;
+      f.xw = a.u3d[(int64_t)k * ust + uc] * rgw;
+      f.xe = a.u3d[(int64_t)k * ust + uc + 1] * rge;
+      f.ys = a.v3d[(int64_t)k * vst + vc] * rgs;
+      f.yn = a.v3d[(int64_t)k * vst + vc + nxs] * rgn;
+      return f;
+    };
+    auto wterms = [&](int kf, const ColumnFaces& hi, const ColumnFaces& lo, float rho_hi, float rho_lo,
+                      float& dum3, float& hadvp) {
+      if (kf == 1 || kf == nk + 1) { dum3 = 0.0f; hadvp = 0.0f; return; }
+      const float r2 = (sigmaf[kf] - sigma[kf - 1]) * rds[kf];
+      const float r1 = 1.0f - r2;
+      const float height = (sigmaf[kf] - a.zt) * a.rzt;
+      {
+        const float xt = (r2 * (hi.xw + hi.xe) + r1 * (lo.xw + lo.xe)) * ddx;
+        const float yt = (r2 * (hi.ys + hi.yn) + r1 * (lo.ys + lo.yn)) * ddy;
+        dum3 = ((0.5f * (xt + yt)) * height) * g;
+      }
+      {
+        const float xt = (r2 * (rho_hi * hi.xw + rho_hi * hi.xe) + r1 * (rho_lo * lo.xw + rho_lo * lo.xe)) * ddx;
+        const float yt = (r2 * (rho_hi * hi.ys + rho_hi * hi.yn) + r1 * (rho_lo * lo.ys + rho_lo * lo.yn)) * ddy;
+        hadvp = (((0.5f * (xt + yt)) * height) * g) * a.rrf0[(int64_t)(kf - 1) * plane + h];
+      }
+    };
+    float w1new = 0.0f;
+    if (kb == 1) {
+      w1new = bcwsfc_value(a.u3d, a.v3d, a.dzdx, a.dzdy, a.gz, j, i, ni, nj);
+      w[wst + wc] = w1new;
+      lw1[c] = w1new;
+    }
+    ColumnFaces flo = faces(kb);
+    float rho_lo = a.rho0[(int64_t)(kb - 1) * plane];
+    float dum3_lo = 0.0f, hadvp_lo = 0.0f;
+    if (kb >= 2) wterms(kb, flo, faces(kb - 1), rho_lo, a.rho0[(int64_t)(kb - 2) * plane], dum3_lo, hadvp_lo);
+    for (int k = kb; k <= ke; ++k) {
+      const int64_t p = pix(k, pj, pi, ni, nj);
+      ColumnFaces fhi;
+      float rho_hi = 0.0f, dum3_hi, hadvp_hi;
+      if (k < nk) {
+        fhi = faces(k + 1);
+        rho_hi = a.rho0[(int64_t)k * plane];
+        wterms(k + 1, fhi, flo, rho_hi, rho_lo, dum3_hi, hadvp_hi);
+      } else {
+        fhi = flo;
+        dum3_hi = 0.0f;
+        hadvp_hi = 0.0f;
+      }
+      const float hz = (flo.xe - flo.xw) * a.rdx + (flo.yn - flo.ys) * a.rdy;
+      const float vt = (dum3_hi - dum3_lo) * rdsf[k];
+      float div = g * (hz + vt);
+      if (fabsf(div) < smeps) div = 0.0f;
+      const float wk = (k == 1) ? 0.0f : w[(int64_t)k * wst + wc];
+      const float wk1 = (k == nk) ? 0.0f : w[(int64_t)(k + 1) * wst + wc];
+      const float qkk = a.qk[p];
+      float fpk = a.dts * (a.ppten[p] - a.ppterm[p] * div) + a.beta * ((-qkk) * (wk1 - wk));
+      const float u2 = temx * (rho_lo * flo.xe), u1 = temx * (rho_lo * flo.xw);
+      const float v2 = temy * (rho_lo * flo.yn), v1 = temy * (rho_lo * flo.ys);
+      const float wb = (k == 1) ? w1new : w[(int64_t)k * wst + wc];
+      const float ww2 = hadvp_hi + a.beta * w[(int64_t)(k + 1) * wst + wc];
+      const float ww1 = hadvp_lo + a.beta * wb;
+      const float pc = a.piadv[p];
+      const float xa = u2 * (a.piadv[p + 1] - pc) + u1 * (pc - a.piadv[p - 1]);
+      const float ya = v2 * (a.piadv[p + (ni + 2)] - pc) + v1 * (pc - a.piadv[p - (ni + 2)]);
+      float hza = (-(xa + ya)) * a.rr0[(int64_t)(k - 1) * plane + h];
+      hza = hza * g;
+      const float pk1k = a.pk1[p], pk2k = a.pk2[p];
+      const float vta = ww1 * pk1k + ww2 * pk2k;
+      fpk = fpk + (hza + vta);
+      Lfpk[at(k)] = fpk;
+      Lpp[at(k)] = a.pp3d[p];
+      Ls1[at(k)] = pk1k + qkk;
+      Ls2[at(k)] = pk2k - qkk;
+      Lmm[at(k)] = a.mm[p];
+      flo = fhi;
+      rho_lo = rho_hi;
+      dum3_lo = dum3_hi;
+      hadvp_lo = hadvp_hi;
+    }
+  }
+  __syncthreads();
+  // ---------------- phase A': right-hand sides dd(k), k = 2..nk
+  if (phys) {
+    for (int k = max(kb, 2); k <= ke; ++k) {
+      const float mmk = Lmm[at(k)];
+      const float ppk = Lpp[at(k)], ppm = Lpp[at(k - 1)];
+      const float pdelta = ppk - ppm;
+      const float fwk = a.dts * a.wten[(int64_t)k * wst + wc] + a.beta * ((-mmk) * pdelta);
+      const float udelta = (ppk + Lfpk[at(k)]) - (ppm + Lfpk[at(k - 1)]);
+      Ldd[at(k)] = (w[(int64_t)k * wst + wc] + fwk) + a.alph * ((-mmk) * udelta);
+    }
+  }
+  __syncthreads();
+  // ---------------- phase B: Thomas recurrence, one lane per column
+  if (phys && slot == 0) {
+    float ec_prev = 0.0f, ed_prev = 0.0f, wtop = 0.0f;
+    for (int k = 2; k <= nk; ++k) {
+      const float mmk = Lmm[at(k)];
+      const float s1k = Ls1[at(k)], s2k = Ls2[at(k)], s1m = Ls1[at(k - 1)], s2m = Ls2[at(k - 1)];
+      const float dd = Ldd[at(k)];
+      const float aa = (k >= 3) ? (-asq) * (mmk * s1m) : 0.0f;
+      const float cc = (k < nk) ? (-asq) * ((-mmk) * s2k) : 0.0f;
+      const float diag = (-s1k) + s2m;
+      const float bb = 1.0f - asq * (mmk * diag);
+      const float pivot = (k == 2) ? bb : aa * ec_prev + bb;
+      const float inv = 1.0f / pivot;
+      if (k < nk) {
+        const float eck = (-cc) * inv;
+        const float edk = ((k == 2) ? dd : dd - aa * ed_prev) * inv;
+        Lmm[at(k)] = eck;
+        Ldd[at(k)] = edk;
+        ec_prev = eck;
+        ed_prev = edk;
+      } else {
+        wtop = (dd - aa * ed_prev) * inv;
+      }
+    }
+    float wup = 0.0f;
+    for (int k = nk; k >= 2; --k) {
+      const float wk = (k == nk) ? wtop : Lmm[at(k)] * wup + Ldd[at(k)];
+      Ldd[at(k)] = wk;
+      wup = wk;
+    }
+  }
+  __syncthreads();
+  // ---------------- phase C: pressure update, filter, time averages
+  if (phys) {
+    const int pj = j + 1, pi = i + 1;
+    for (int k = kb; k <= ke; ++k) {
+      const int64_t p = pix(k, pj, pi, ni, nj);
+      const float wk = (k == 1) ? lw1[c] : Ldd[at(k)];
+      const float wup = (k == nk) ? 0.0f : Ldd[at(k + 1)];
+      if (k >= 2) w[(int64_t)k * wst + wc] = wk;
+      const float w1 = (k == 1) ? 0.0f : wk;
+      const float w2 = (k == nk) ? 0.0f : wup;
+      const float old = Lpp[at(k)];
+      const float vt = w1 * Ls1[at(k)] + w2 * Ls2[at(k)];
+      float up = (old + Lfpk[at(k)]) + a.alph * vt;
+      if (fabsf(up) < smeps) up = 0.0f;
+      a.pp3d[p] = up;
+      const float tend = a.kdiv * (up - old);
+      a.ppd[p] = up + tend;
+      if (a.nrk == 3 && last) a.ppx[p] = tend;
+      const int64_t pw = (int64_t)k * wst + wc;
+      const float vw = a.rrw[pw] + (last ? wk * a.alph : wk);
+      a.rrw[pw] = last ? vw * tavg : vw;
+    }
+  } else if (inside) {
+    for (int k = kb; k <= ke; ++k) {
+      const int64_t pw = (int64_t)k * wst + wc;
+      const float wk = w[pw];
+      const float vw = a.rrw[pw] + (last ? wk * a.alph : wk);
+      a.rrw[pw] = last ? vw * tavg : vw;
+    }
+  }
+  if (inside) {
+    for (int k = kb; k <= ke; ++k) {
+      const int64_t pu = (int64_t)k * ust + uc;
+      const float vu = a.rru[pu] + a.u3d[pu];
+      a.rru[pu] = last ? vu * tavg : vu;
+      const int64_t pv = (int64_t)k * vst + vc;
+      const float vv = a.rrv[pv] + a.v3d[pv];
+      a.rrv[pv] = last ? vv * tavg : vv;
+    }
+  }
+}
+
 // ------------------------------------------------------------------ radiation
 // Durran-Klemp phase speed of one edge row/column, averaged over k in order.
 __global__ __launch_bounds__(64) void k_snd_radiation(SoundArgs a, float dx, float dy, float dttmp) {
@@ -502,14 +818,24 @@ extern "C" int gm1hip_psolver3_stage(
   const bool any_open = side_of(sides, 0) == 2u || side_of(sides, 1) == 2u || side_of(sides, 2) == 2u ||
                         side_of(sides, 3) == 2u;
   const int edge = 2 * (ni + nj);
+  // the LDS-staged column pass unless the caller asks for the register one
+  // (flags & 4) or the column does not fit the 64 KiB workgroup LDS window
+  const bool staged = !(flags & 4) && column2_lds_bytes(nk) + (int)(C2W * sizeof(float)) <= 65536;
   for (int n = 1; n <= nloop; ++n) {
     if (any_open) {
-      hipLaunchKernelGGL(k_snd_ssopen, dim3((edge + 63) / 64), dim3(64), 0, s, a);
+      if (nk <= SS_MAXK) hipLaunchKernelGGL(k_snd_ssopen2, dim3(edge), dim3(SS_T), 0, s, a);
+      else hipLaunchKernelGGL(k_snd_ssopen, dim3((edge + 63) / 64), dim3(64), 0, s, a);
     }
     GM1_TIMED(TK_ACOUSTIC_PGRAD, s,
-              hipLaunchKernelGGL(k_snd_pgrad, dim3((ni + 1 + 63) / 64, nj + 1, nk), dim3(64), 0, s, a));
-    GM1_TIMED(TK_ACOUSTIC_COLUMN, s,
-              hipLaunchKernelGGL(k_snd_column, dim3((ni + 1 + 63) / 64, nj + 1), dim3(64), 0, s, a, n));
+              hipLaunchKernelGGL(k_snd_pgrad, dim3(((ni + 1 + PG_TX - 1) / PG_TX) * ((nj + 1 + PG_TY - 1) / PG_TY) * nk),
+                                 dim3(PG_TX * PG_TY), 0, s, a));
+    if (staged)
+      GM1_TIMED(TK_ACOUSTIC_COLUMN, s,
+                hipLaunchKernelGGL(k_snd_column2, dim3(((ni + 1 + C2W - 1) / C2W) * (nj + 1)), dim3(C2T),
+                                   column2_lds_bytes(nk), s, a, n));
+    else
+      GM1_TIMED(TK_ACOUSTIC_COLUMN, s,
+                hipLaunchKernelGGL(k_snd_column, dim3((ni + 1 + 63) / 64, nj + 1), dim3(64), 0, s, a, n));
     GM1_LAUNCH_CHECK();
     if (n < nloop) {
       int rc = fill_pressure(ppd, sides, ni, nj, nk, s);

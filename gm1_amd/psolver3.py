@@ -116,14 +116,21 @@ def launch_cuda_psolver3_terrain_open_three_stage(
     ablations: TerrainOpenSoundAblations | None = None, stream: Any | None = None, compiled: Any | None = None,
     lateral_sides: tuple[int, int, int, int] = (2, 2, 2, 2), alph: float | None = None,
     kdiv: float | None = None, profiles: VerticalProfiles | None = None, _nrk_sequence: tuple[int, ...] = (1, 2, 3),
-    **_: Any,
+    column_pass: str = "staged", **_: Any,
 ) -> CudaPsolver3TerrainOpenLaunch:
-    """Enqueue the acoustic RK stages of ``_nrk_sequence`` on ``stream``."""
+    """Enqueue the acoustic RK stages of ``_nrk_sequence`` on ``stream``.
+
+    ``column_pass`` selects the vertical pass: "staged" (LDS-staged, k-parallel
+    outside the Thomas recurrence) or "register" (one thread walks each
+    column); both produce the same words.
+    """
+    if column_pass not in ("staged", "register"):
+        raise ValueError("column_pass must be 'staged' or 'register'")
     for name, value in (("alph", alph), ("kdiv", kdiv)):
         if value is not None and not (0.0 <= float(value) <= 1.0):
             raise ValueError(f"{name}={value} is outside CM1's admitted [0, 1]")
     dt, dx, dy, dz = (check_spacing(n, v) for n, v in (("dt", dt), ("dx", dx), ("dy", dy), ("dz", dz)))
-    flags = _check_ablations(ablations or TerrainOpenSoundAblations())
+    flags = _check_ablations(ablations or TerrainOpenSoundAblations()) | (4 if column_pass == "register" else 0)
     sides = lateral_code(lateral_sides)
     shapes = terrain_open_sound_shapes(ni, nj, nk)
     a = {}

@@ -321,3 +321,32 @@ def test_gpu_moist_large_step_bit_exact(device, sides):
     # the bubble condensed
     assert a["q3d"][1].max() > 1e-5
     assert kessler_status(st)["non_converged_cells"] == 0
+
+
+@pytest.mark.gpu
+def test_gpu_graph_replays_match_eager_steps(device):
+    """A captured moist large step replayed three times equals three eager steps, replay by replay.
+
+    Eager work and host synchronisations run between the replays.
+    """
+    import torch
+
+    from gm1_amd.graph import Gm1LargeStepGraph
+    from gm1_amd.large_step import create_gm1_large_step_state, launch_cuda_gm1_large_step
+
+    cfg = moist_case(24, 20, 16, (2, 2, 2, 2), seed=9)
+    host = host_initial_state(cfg)
+    eager = create_gm1_large_step_state(host, device)
+    graphed = create_gm1_large_step_state(host, device, stream=torch.cuda.Stream())
+    g = Gm1LargeStepGraph(graphed, steps=1)
+    for step in range(3):
+        launch_cuda_gm1_large_step(eager)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        scratch = torch.ones(1 << 16, device=device).cumsum(0)  # eager work between replays
+        torch.cuda.synchronize()
+        assert float(scratch[-1]) == float(1 << 16)
+        for name in MOIST_FIELDS:
+            assert_bits_equal(to_host(graphed.a[name]), to_host(eager.a[name]), f"replay {step + 1} {name}")
+    assert graphed.step_index == eager.step_index == 3

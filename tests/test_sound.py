@@ -186,14 +186,15 @@ def test_periodic_stage_is_translation_equivariant_and_keeps_duplicate_faces():
 
 # ------------------------------------------------------------------ GPU parity
 
-def _gpu_stage(device, a, m, *, ni, nj, nk, dt, ns, dx, dy, dz, sides, nrks=(1, 2, 3), flags=None):
+def _gpu_stage(device, a, m, *, ni, nj, nk, dt, ns, dx, dy, dz, sides, nrks=(1, 2, 3), flags=None,
+               column_pass="staged"):
     from gm1_amd.psolver3 import TerrainOpenSoundAblations, launch_cuda_psolver3_terrain_open_stage
 
     da, dm = to_dev(a, device), to_dev(m, device)
     abl = TerrainOpenSoundAblations() if flags is None else flags
     for nrk in nrks:
         launch_cuda_psolver3_terrain_open_stage(da, dm, ni=ni, nj=nj, nk=nk, nrk=nrk, dt=dt, nsound=ns, dx=dx, dy=dy,
-                                                dz=dz, lateral_sides=sides, ablations=abl)
+                                                dz=dz, lateral_sides=sides, ablations=abl, column_pass=column_pass)
     return to_host(da)
 
 
@@ -222,3 +223,17 @@ def test_gpu_acoustic_stage_bit_exact(device, sides, dims):
                                        dz=150.0, lateral_sides=sides)
     for k in OUT:
         assert_bits_equal(got[k], want[k], f"{k} sides={sides}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sides", [(2, 2, 2, 2), (1, 1, 1, 1)])
+def test_gpu_staged_and_register_column_passes_agree(device, sides):
+    """The LDS-staged column pass and the one-thread-per-column pass give the same words."""
+    ni, nj, nk = 67, 33, 40
+    a, m = _acoustic_case(np.random.default_rng(17), ni, nj, nk, terrain=True)
+    got = {}
+    for mode in ("staged", "register"):
+        got[mode] = _gpu_stage(device, {k: v.copy() for k, v in a.items()}, m, ni=ni, nj=nj, nk=nk, dt=2.0, ns=8,
+                               dx=250.0, dy=200.0, dz=150.0, sides=sides, column_pass=mode)
+    for k in OUT:
+        assert_bits_equal(got["staged"][k], got["register"][k], f"{k} sides={sides}")
