@@ -427,10 +427,14 @@ GM1_DEV double ordered_sum(const double* __restrict__ v, int64_t n, int64_t stri
   return s;
 }
 
-// rows: colbud rows (5 x nj, i order) and level sums of the wave partials
-// (5 x nk, (j, segment) order), one thread each.
+// rows: colbud rows (5 x nj, i order), level sums of the wave partials
+// (5 x nk, (j, segment) order) and STAT_PARTS strided partial maxima of the
+// satadj status words, one thread each.
+constexpr int STAT_PARTS = 2048;
+
 __global__ __launch_bounds__(256) void k_kes_rows(const double* __restrict__ colbud, const double* __restrict__ wavebud,
-                                                  double* __restrict__ rowscr, int ni, int nj, int nk, int nseg) {
+                                                  const int* __restrict__ wavestat, double* __restrict__ rowscr,
+                                                  int* __restrict__ statscr, int ni, int nj, int nk, int nseg) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int nrow = 5 * nj, nlev = 5 * nk;
   if (r < nrow) {
@@ -440,24 +444,33 @@ __global__ __launch_bounds__(256) void k_kes_rows(const double* __restrict__ col
     const int q = r - nrow, b = q / nk, k = q % nk;
     const int64_t lstride = (int64_t)nj * nseg;
     rowscr[r] = ordered_sum(wavebud + (int64_t)b * nk * lstride + (int64_t)k * lstride, lstride, 1);
+  } else if (r < nrow + nlev + STAT_PARTS) {
+    const int part = r - nrow - nlev;
+    const int64_t n = (int64_t)nk * nj * nseg;
+    int mi = 0, mf = 0;
+    for (int64_t q = part; q < n; q += STAT_PARTS) {
+      mi = max(mi, wavestat[2 * q]);
+      mf += wavestat[2 * q + 1];
+    }
+    statscr[2 * part] = mi;
+    statscr[2 * part + 1] = mf;
   }
 }
 
 // The folds into asq/qbudget in the reference's order (j rows, k levels) and
 // the satadj status maxima.
 __global__ __launch_bounds__(256) void k_kes_budgets(const double* __restrict__ rowscr,
-                                                     const int* __restrict__ wavestat, double* __restrict__ asq,
+                                                     const int* __restrict__ statscr, double* __restrict__ asq,
                                                      double* __restrict__ qbudget, int* __restrict__ status, int nj,
-                                                     int nk, int nseg, float dx, float dy, float dz) {
+                                                     int nk, float dx, float dy, float dz) {
   __shared__ int smax[256];
   __shared__ int sfail[256];
   const int t = threadIdx.x;
   {
     int mi = 0, mf = 0;
-    const int64_t n = (int64_t)nk * nj * nseg;
-    for (int64_t q = t; q < n; q += blockDim.x) {
-      mi = max(mi, wavestat[2 * q]);
-      mf += wavestat[2 * q + 1];
+    for (int q = t; q < STAT_PARTS; q += blockDim.x) {
+      mi = max(mi, statscr[2 * q]);
+      mf += statscr[2 * q + 1];
     }
     smax[t] = mi;
     sfail[t] = mf;
@@ -616,7 +629,7 @@ int64_t gm1hip_kessler_scratch_bytes(int ni, int nj, int nk) {
   const int64_t wave = 5LL * nk * nj * nseg * 8;
   const int64_t stat = 2LL * nk * nj * nseg * 4;
   const int64_t rows = (5LL * nj + 5LL * nk) * 8;
-  return col + wave + stat + rows + 4 * 256;
+  return col + wave + stat + rows + 8LL * 2048 + 64;
 }
 
 int gm1hip_kessler_driver(const float* ruh, const float* rvh, const float* rmh, const float* mh, const float* pi0,
@@ -645,11 +658,12 @@ int gm1hip_kessler_driver(const float* ruh, const float* rvh, const float* rmh, 
   a.wavestat = (int*)base;
   hipLaunchKernelGGL(k_kes_column, dim3((ni + 63) / 64, nj), dim3(64), 0, s, a);
   hipLaunchKernelGGL(k_kes_cell, dim3(a.nseg, nj, nk), dim3(64), 0, s, a);
-  const int nsum = 5 * nj + 5 * nk;
-  hipLaunchKernelGGL(k_kes_rows, dim3((nsum + 255) / 256), dim3(256), 0, s, a.colbud, a.wavebud, rows, ni, nj, nk,
-                     a.nseg);
-  hipLaunchKernelGGL(k_kes_budgets, dim3(1), dim3(256), 0, s, rows, a.wavestat, asq, qbudget, status, nj, nk, a.nseg,
-                     dx, dy, dz);
+  int* statscr = a.wavestat + 2LL * nk * nj * a.nseg;
+  const int nsum = 5 * nj + 5 * nk + STAT_PARTS;
+  hipLaunchKernelGGL(k_kes_rows, dim3((nsum + 255) / 256), dim3(256), 0, s, a.colbud, a.wavebud, a.wavestat, rows,
+                     statscr, ni, nj, nk, a.nseg);
+  hipLaunchKernelGGL(k_kes_budgets, dim3(1), dim3(256), 0, s, rows, statscr, asq, qbudget, status, nj, nk, dx, dy,
+                     dz);
   GM1_LAUNCH_CHECK();
   return 0;
 }

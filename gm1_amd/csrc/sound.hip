@@ -418,7 +418,8 @@ __global__ __launch_bounds__(64) void k_snd_column(SoundArgs a, int n) {
       const float vt = (dum3_hi - dum3_lo) * rdsf[k];
       float div = g * (hz + vt);
       if (fabsf(div) < smeps) div = 0.0f;
-      const float wk = (k == 1) ? 0.0f : w[(int64_t)k * wst + wc];
+      const float wraw = w[(int64_t)k * wst + wc];
+      const float wk = (k == 1) ? 0.0f : wraw;
       const float wk1 = (k == nk) ? 0.0f : w[(int64_t)(k + 1) * wst + wc];
       const float qkk = a.qk[p];
       float fpk = a.dts * (a.ppten[p] - a.ppterm[p] * div) + a.beta * ((-qkk) * (wk1 - wk));
@@ -523,14 +524,13 @@ __global__ __launch_bounds__(64) void k_snd_column(SoundArgs a, int n) {
 // does the pressure update, the filter and the time averages.  fpk and the
 // elimination coefficients never reach HBM.  Workgroups are mapped so each
 // XCD sweeps a contiguous band of rows.
-constexpr int C2W = 32;
-constexpr int C2T = 512;
-constexpr int C2S = C2T / C2W;
 constexpr int C2PLANES = 6;  // fpk, pp, s1 = pk1+qk, s2 = pk2-qk, mm -> ec, dd -> ed -> w
 
-int column2_lds_bytes(int nk) { return C2PLANES * (nk + 1) * C2W * (int)sizeof(float); }
+int column2_lds_bytes(int nk, int cw) { return C2PLANES * (nk + 1) * cw * (int)sizeof(float); }
 
+template <int C2W, int C2T>
 __global__ __launch_bounds__(C2T) void k_snd_column2(SoundArgs a, int n) {
+  constexpr int C2S = C2T / C2W;
   extern __shared__ float lds[];
   __shared__ float lw1[C2W];
   const int ni = a.ni, nj = a.nj, nk = a.nk;
@@ -628,7 +628,8 @@ __global__ __launch_bounds__(C2T) void k_snd_column2(SoundArgs a, int n) {
       const float vt = (dum3_hi - dum3_lo) * rdsf[k];
       float div = g * (hz + vt);
       if (fabsf(div) < smeps) div = 0.0f;
-      const float wk = (k == 1) ? 0.0f : w[(int64_t)k * wst + wc];
+      const float wraw = w[(int64_t)k * wst + wc];
+      const float wk = (k == 1) ? 0.0f : wraw;
       const float wk1 = (k == nk) ? 0.0f : w[(int64_t)(k + 1) * wst + wc];
       const float qkk = a.qk[p];
       float fpk = a.dts * (a.ppten[p] - a.ppterm[p] * div) + a.beta * ((-qkk) * (wk1 - wk));
@@ -650,6 +651,14 @@ __global__ __launch_bounds__(C2T) void k_snd_column2(SoundArgs a, int n) {
       Ls1[at(k)] = pk1k + qkk;
       Ls2[at(k)] = pk2k - qkk;
       Lmm[at(k)] = a.mm[p];
+      Ldd[at(k)] = wraw;  // w(k) before the solve, read back by phase A'
+      {  // time averages of this column's west u face and south v face
+        const int64_t pu = (int64_t)k * ust + uc, pv = (int64_t)k * vst + vc;
+        const float au = a.rru[pu] + a.u3d[pu];
+        a.rru[pu] = last ? au * tavg : au;
+        const float av = a.rrv[pv] + a.v3d[pv];
+        a.rrv[pv] = last ? av * tavg : av;
+      }
       flo = fhi;
       rho_lo = rho_hi;
       dum3_lo = dum3_hi;
@@ -659,23 +668,45 @@ __global__ __launch_bounds__(C2T) void k_snd_column2(SoundArgs a, int n) {
   __syncthreads();
   // ---------------- phase A': right-hand sides dd(k), k = 2..nk
   if (phys) {
-    for (int k = max(kb, 2); k <= ke; ++k) {
-      const float mmk = Lmm[at(k)];
-      const float ppk = Lpp[at(k)], ppm = Lpp[at(k - 1)];
-      const float pdelta = ppk - ppm;
-      const float fwk = a.dts * a.wten[(int64_t)k * wst + wc] + a.beta * ((-mmk) * pdelta);
-      const float udelta = (ppk + Lfpk[at(k)]) - (ppm + Lfpk[at(k - 1)]);
-      Ldd[at(k)] = (w[(int64_t)k * wst + wc] + fwk) + a.alph * ((-mmk) * udelta);
+    const float* __restrict__ wten = a.wten;
+    constexpr int B = 4;
+    for (int k0 = max(kb, 2); k0 <= ke; k0 += B) {
+      float wt[B], wo[B];
+#pragma unroll
+      for (int u = 0; u < B; ++u)
+        if (k0 + u <= ke) {
+          wt[u] = wten[(int64_t)(k0 + u) * wst + wc];
+          wo[u] = Ldd[at(k0 + u)];
+        }
+#pragma unroll
+      for (int u = 0; u < B; ++u)
+        if (k0 + u <= ke) {
+          const int k = k0 + u;
+          const float mmk = Lmm[at(k)];
+          const float ppk = Lpp[at(k)], ppm = Lpp[at(k - 1)];
+          const float pdelta = ppk - ppm;
+          const float fwk = a.dts * wt[u] + a.beta * ((-mmk) * pdelta);
+          const float udelta = (ppk + Lfpk[at(k)]) - (ppm + Lfpk[at(k - 1)]);
+          Ldd[at(k)] = (wo[u] + fwk) + a.alph * ((-mmk) * udelta);
+        }
     }
   }
   __syncthreads();
   // ---------------- phase B: Thomas recurrence, one lane per column
+  // (the next level's LDS words are fetched while the current level's
+  // division runs; s1/s2 of the level below are carried)
   if (phys && slot == 0) {
     float ec_prev = 0.0f, ed_prev = 0.0f, wtop = 0.0f;
+    float s1m = Ls1[at(1)], s2m = Ls2[at(1)];
+    float mmn = Lmm[at(2)], s1n = Ls1[at(2)], s2n = Ls2[at(2)], ddn = Ldd[at(2)];
     for (int k = 2; k <= nk; ++k) {
-      const float mmk = Lmm[at(k)];
-      const float s1k = Ls1[at(k)], s2k = Ls2[at(k)], s1m = Ls1[at(k - 1)], s2m = Ls2[at(k - 1)];
-      const float dd = Ldd[at(k)];
+      const float mmk = mmn, s1k = s1n, s2k = s2n, dd = ddn;
+      if (k < nk) {
+        mmn = Lmm[at(k + 1)];
+        s1n = Ls1[at(k + 1)];
+        s2n = Ls2[at(k + 1)];
+        ddn = Ldd[at(k + 1)];
+      }
       const float aa = (k >= 3) ? (-asq) * (mmk * s1m) : 0.0f;
       const float cc = (k < nk) ? (-asq) * ((-mmk) * s2k) : 0.0f;
       const float diag = (-s1k) + s2m;
@@ -692,10 +723,19 @@ __global__ __launch_bounds__(C2T) void k_snd_column2(SoundArgs a, int n) {
       } else {
         wtop = (dd - aa * ed_prev) * inv;
       }
+      s1m = s1k;
+      s2m = s2k;
     }
-    float wup = 0.0f;
-    for (int k = nk; k >= 2; --k) {
-      const float wk = (k == nk) ? wtop : Lmm[at(k)] * wup + Ldd[at(k)];
+    float wup = wtop;
+    Ldd[at(nk)] = wtop;
+    float ecn = Lmm[at(nk - 1)], edn = Ldd[at(nk - 1)];
+    for (int k = nk - 1; k >= 2; --k) {
+      const float eck = ecn, edk = edn;
+      if (k > 2) {
+        ecn = Lmm[at(k - 1)];
+        edn = Ldd[at(k - 1)];
+      }
+      const float wk = eck * wup + edk;
       Ldd[at(k)] = wk;
       wup = wk;
     }
@@ -731,14 +771,32 @@ __global__ __launch_bounds__(C2T) void k_snd_column2(SoundArgs a, int n) {
       a.rrw[pw] = last ? vw * tavg : vw;
     }
   }
-  if (inside) {
-    for (int k = kb; k <= ke; ++k) {
-      const int64_t pu = (int64_t)k * ust + uc;
-      const float vu = a.rru[pu] + a.u3d[pu];
-      a.rru[pu] = last ? vu * tavg : vu;
-      const int64_t pv = (int64_t)k * vst + vc;
-      const float vv = a.rrv[pv] + a.v3d[pv];
-      a.rrv[pv] = last ? vv * tavg : vv;
+  if (inside && !phys) {  // faces of the east column and north row
+    float* __restrict__ rru = a.rru;
+    float* __restrict__ rrv = a.rrv;
+    const float* __restrict__ u3 = a.u3d;
+    const float* __restrict__ v3 = a.v3d;
+    constexpr int B = 4;
+    for (int k0 = kb; k0 <= ke; k0 += B) {
+      float ru[B], uu[B], rv[B], vv[B];
+#pragma unroll
+      for (int u = 0; u < B; ++u)
+        if (k0 + u <= ke) {
+          const int64_t pu = (int64_t)(k0 + u) * ust + uc, pv = (int64_t)(k0 + u) * vst + vc;
+          ru[u] = rru[pu];
+          uu[u] = u3[pu];
+          rv[u] = rrv[pv];
+          vv[u] = v3[pv];
+        }
+#pragma unroll
+      for (int u = 0; u < B; ++u)
+        if (k0 + u <= ke) {
+          const int64_t pu = (int64_t)(k0 + u) * ust + uc, pv = (int64_t)(k0 + u) * vst + vc;
+          const float au = ru[u] + uu[u];
+          rru[pu] = last ? au * tavg : au;
+          const float av = rv[u] + vv[u];
+          rrv[pv] = last ? av * tavg : av;
+        }
     }
   }
 }
@@ -820,7 +878,9 @@ extern "C" int gm1hip_psolver3_stage(
   const int edge = 2 * (ni + nj);
   // the LDS-staged column pass unless the caller asks for the register one
   // (flags & 4) or the column does not fit the 64 KiB workgroup LDS window
-  const bool staged = !(flags & 4) && column2_lds_bytes(nk) + (int)(C2W * sizeof(float)) <= 65536;
+  const bool narrow = (flags & 8) != 0;
+  const int cw = narrow ? 16 : 32;
+  const bool staged = !(flags & 4) && column2_lds_bytes(nk, cw) + (int)(cw * sizeof(float)) <= 65536;
   for (int n = 1; n <= nloop; ++n) {
     if (any_open) {
       if (nk <= SS_MAXK) hipLaunchKernelGGL(k_snd_ssopen2, dim3(edge), dim3(SS_T), 0, s, a);
@@ -829,10 +889,14 @@ extern "C" int gm1hip_psolver3_stage(
     GM1_TIMED(TK_ACOUSTIC_PGRAD, s,
               hipLaunchKernelGGL(k_snd_pgrad, dim3(((ni + 1 + PG_TX - 1) / PG_TX) * ((nj + 1 + PG_TY - 1) / PG_TY) * nk),
                                  dim3(PG_TX * PG_TY), 0, s, a));
-    if (staged)
+    if (staged && narrow)
       GM1_TIMED(TK_ACOUSTIC_COLUMN, s,
-                hipLaunchKernelGGL(k_snd_column2, dim3(((ni + 1 + C2W - 1) / C2W) * (nj + 1)), dim3(C2T),
-                                   column2_lds_bytes(nk), s, a, n));
+                hipLaunchKernelGGL((k_snd_column2<16, 256>), dim3(((ni + 1 + 15) / 16) * (nj + 1)), dim3(256),
+                                   column2_lds_bytes(nk, 16), s, a, n));
+    else if (staged)
+      GM1_TIMED(TK_ACOUSTIC_COLUMN, s,
+                hipLaunchKernelGGL((k_snd_column2<32, 512>), dim3(((ni + 1 + 31) / 32) * (nj + 1)), dim3(512),
+                                   column2_lds_bytes(nk, 32), s, a, n));
     else
       GM1_TIMED(TK_ACOUSTIC_COLUMN, s,
                 hipLaunchKernelGGL(k_snd_column, dim3((ni + 1 + 63) / 64, nj + 1), dim3(64), 0, s, a, n));

@@ -149,6 +149,7 @@ class Gm1LargeStepState:
     kessler_workspace: Any = None
     dry_mass: Any = None
     mass1: float | None = None
+    column_pass: str = "staged"
     receipts: list = field(default_factory=list)
 
 
@@ -215,7 +216,8 @@ def _rk_stage(st: Gm1LargeStepState, nrk: int) -> None:
                  ptr(a["uten"]), ptr(a["vten"]), ptr(a["wten"]), ni, nj, nk, h)
     launch_cuda_psolver3_terrain_open_stage(
         a, m, ni=ni, nj=nj, nk=nk, nrk=nrk, dt=cfg.dt, nsound=cfg.nsound, dx=cfg.dx, dy=cfg.dy, dz=cfg.dz,
-        stream=st.stream, lateral_sides=sides, profiles=st.prof, ablations=TerrainOpenSoundAblations())
+        stream=st.stream, lateral_sides=sides, profiles=st.prof, ablations=TerrainOpenSoundAblations(),
+        column_pass=st.column_pass)
     _native.call("gm1hip_post_acoustic", ptr(a["rru"]), ptr(a["rrv"]), ptr(a["rrw"]), ptr(m["rho0_3d"]),
                  ptr(m["rf0"]), ptr(m["rgzu"]), ptr(m["rgzv"]), ptr(m["dzdx"]), ptr(m["dzdy"]), ptr(m["gz"]),
                  ptr(a["thterm"]), ptr(a["divergence"]), ptr(a["theta_div"]), ptr(a["divx"]), ni, nj, nk,

@@ -124,13 +124,13 @@ def launch_cuda_psolver3_terrain_open_three_stage(
     outside the Thomas recurrence) or "register" (one thread walks each
     column); both produce the same words.
     """
-    if column_pass not in ("staged", "register"):
-        raise ValueError("column_pass must be 'staged' or 'register'")
+    if column_pass not in ("staged", "staged16", "register"):
+        raise ValueError("column_pass must be 'staged', 'staged16' or 'register'")
     for name, value in (("alph", alph), ("kdiv", kdiv)):
         if value is not None and not (0.0 <= float(value) <= 1.0):
             raise ValueError(f"{name}={value} is outside CM1's admitted [0, 1]")
     dt, dx, dy, dz = (check_spacing(n, v) for n, v in (("dt", dt), ("dx", dx), ("dy", dy), ("dz", dz)))
-    flags = _check_ablations(ablations or TerrainOpenSoundAblations()) | (4 if column_pass == "register" else 0)
+    flags = _check_ablations(ablations or TerrainOpenSoundAblations()) | {"staged": 0, "register": 4, "staged16": 8}[column_pass]
     sides = lateral_code(lateral_sides)
     shapes = terrain_open_sound_shapes(ni, nj, nk)
     a = {}
