@@ -110,6 +110,18 @@ int gm1hip_psolver3_stage(
     int ni, int nj, int nk, int nrk, int nloop, float dts, float dttmp, float dx, float dy, float alph,
     float kdiv, unsigned sides, int flags, void* stream);
 
+// Replaces one stage of launch_cuda_sound3_periodic_three_stage
+// (cuda_sound3_periodic.py:1216-1460; CPU form cpu_sound3_periodic_stage_inplace
+// :672-1064): the flat, uniform, doubly periodic stage on 1-cell halos.
+// `arrays` is a host table of 47 device pointers in the order of
+// cuda_sound3_periodic.ALL_ARRAY_NAMES (:59-94): mh rr0 rf0 c1 c2 mf piadv
+// ppterm thv ppten ppi pp3d ppx ppd qk pk1 pk2 mm fpk divh hadvp elim_c elim_d
+// tri_aa tri_bb tri_cc tri_dd tri_pivot tri_residual | wten wa w3d rrw | ua u3d
+// uten rru | va v3d vten rrv | dtu0 dtv0 uf vf uh vh, shaped per array_shapes
+// (:449-491).  Every elimination and tridiagonal diagnostic is written.
+int gm1hip_sound3_periodic_stage(void* const* arrays, int ni, int nj, int nk, int nrk, int nloop, float dts,
+                                 float rdx, float rdy, float rdz, float cp, float alph, float kdiv, void* stream);
+
 // ----------------------------------------------------------------- boundaries
 // Family halo fill of one padded field (cuda_boundary_families.py:887-940,
 // cpu_family_we_inplace / cpu_family_sn_inplace); kind 0 scalar, 1 u, 2 v, 3 w.

@@ -401,8 +401,95 @@ def kessler():
     _save("kessler", **out)
 
 
+def periodic_inputs(rng, ni, nj, nk, *, smooth):
+    """Inputs of the periodic acoustic stage (our own construction), halos periodic."""
+    _ref()
+    from cm1core.cuda_sound3_periodic import (array_shapes, default_periodic_face_weights,
+                                              fill_periodic_scalar_inplace, fill_periodic_uv_inplace)
+
+    shapes = array_shapes(ni, nj, nk)
+    a = {name: np.zeros(shape, F) for name, shape in shapes.items()}
+    a["dtu0"], a["dtv0"] = default_periodic_face_weights(ni, nj)
+    for name in ("uf", "vf", "uh", "vh"):
+        a[name].fill(F(1.0))
+    z = (np.arange(nk + 2, dtype=np.float64) / (nk + 1))[:, None, None]
+    xs = 2.0 * np.pi * np.arange(ni) / ni
+    ys = 2.0 * np.pi * np.arange(nj) / nj
+    h = (0.7 * np.sin(xs)[None, :] + 0.4 * np.cos(ys)[:, None])[None] if smooth else rng.uniform(-1, 1, (1, nj, ni))
+    inner = (slice(None), slice(1, nj + 1), slice(1, ni + 1))
+
+    def put(name, value):
+        a[name][inner] = np.broadcast_to(value, (nk + 2, nj, ni)).astype(F)
+
+    put("mh", 1.0 if smooth else 1.0 + 0.25 * np.sin(3.0 * z) + 0.02 * h)
+    put("mf", 1.0 if smooth else 1.0 + 0.2 * np.cos(2.0 * z) - 0.02 * h)
+    put("rr0", 1.0 + 0.01 * z)
+    put("rf0", 0.99 + 0.012 * z)
+    put("c1", 0.45 if smooth else 0.45 + 0.05 * z)
+    put("c2", 0.55 if smooth else 0.55 - 0.05 * z)
+    put("piadv", 1.0 + 0.0003 * z + 0.00001 * h)
+    put("ppterm", 0.014 + 0.0002 * z)
+    put("thv", 299.0 + 1.8 * z + 0.03 * h)
+    put("ppten", 2.0e-6 * h + 1.0e-6 * z)
+    pressure = 2.0e-4 * h + 4.0e-5 * z
+    put("ppi", pressure)
+    put("ppx", 0.2 * pressure + 1.0e-6 * z)
+    for name in ("mh", "rr0", "rf0", "c1", "c2", "mf", "piadv", "ppterm", "thv", "ppten", "ppi", "ppx"):
+        fill_periodic_scalar_inplace(a[name], ni=ni, nj=nj, nk=nk)
+    zk = (np.arange(1, nk + 1) / nk)[:, None, None]
+    hw = h if smooth else rng.uniform(-1, 1, (1, nj, ni))
+    a["wa"][1 : nk + 1] = (0.003 * hw * np.sin(np.pi * zk)).astype(F)
+    a["wten"][1 : nk + 1] = np.broadcast_to(2.0e-5 * hw, (nk, nj, ni)).astype(F)
+    xf = 2.0 * np.pi * np.arange(ni + 1) / ni
+    yc = 2.0 * np.pi * np.arange(nj) / nj
+    a["ua"][1 : nk + 1, 1 : nj + 1, :] = np.broadcast_to(4.0 + 0.15 * np.sin(xf)[None, :] * np.cos(yc)[:, None],
+                                                         (nk, nj, ni + 1)).astype(F)
+    a["uten"][1 : nk + 1, 1 : nj + 1, :] = np.broadcast_to(1.0e-4 * np.cos(xf)[None, :] * np.cos(yc)[:, None],
+                                                           (nk, nj, ni + 1)).astype(F)
+    yf = 2.0 * np.pi * np.arange(nj + 1) / nj
+    xc = 2.0 * np.pi * np.arange(ni) / ni
+    a["va"][1 : nk + 1, :, 1 : ni + 1] = np.broadcast_to(-1.5 + 0.12 * np.cos(xc)[None, :] * np.sin(yf)[:, None],
+                                                         (nk, nj + 1, ni)).astype(F)
+    a["vten"][1 : nk + 1, :, 1 : ni + 1] = np.broadcast_to(-8.0e-5 * np.sin(xc)[None, :] * np.cos(yf)[:, None],
+                                                           (nk, nj + 1, ni)).astype(F)
+    if not smooth:
+        a["uten"][1 : nk + 1, 1 : nj + 1, :] += rng.normal(0, 2e-5, (nk, nj, ni + 1)).astype(F)
+        a["uten"][1 : nk + 1, 1 : nj + 1, ni] = a["uten"][1 : nk + 1, 1 : nj + 1, 0]
+        a["vten"][1 : nk + 1, :, 1 : ni + 1] += rng.normal(0, 2e-5, (nk, nj + 1, ni)).astype(F)
+        a["vten"][1 : nk + 1, nj, 1 : ni + 1] = a["vten"][1 : nk + 1, 0, 1 : ni + 1]
+    fill_periodic_uv_inplace(a["ua"], a["va"], ni=ni, nj=nj, nk=nk)
+    for dst, src in (("u3d", "ua"), ("v3d", "va"), ("w3d", "wa"), ("pp3d", "ppi"), ("ppd", "ppx")):
+        np.copyto(a[dst], a[src])
+    return a
+
+
+def sound3_periodic():
+    _ref()
+    from cm1core.cuda_sound3_periodic import (COMPARE_OUTPUT_NAMES, cpu_sound3_periodic_three_stage_inplace,
+                                              validate_periodic_contract)
+
+    out = {}
+    cases = (("a", 5, 4, 7, True, dict(dt=2.0, nsound=6, rdx=1.0 / 1000.0, rdy=1.0 / 1000.0, rdz=1.0 / 500.0)),
+             ("b", 9, 7, 10, False, dict(dt=3.0, nsound=6, rdx=1.0 / 500.0, rdy=1.0 / 700.0, rdz=1.0 / 250.0)),
+             ("c", 6, 5, 8, False, dict(dt=1.5, nsound=8, rdx=1.0 / 250.0, rdy=1.0 / 250.0, rdz=1.0 / 250.0,
+                                         alph=0.55, kdiv=0.05)))
+    for tag, ni, nj, nk, smooth, params in cases:
+        rng = np.random.default_rng(300 + ord(tag))
+        a = periodic_inputs(rng, ni, nj, nk, smooth=smooth)
+        validate_periodic_contract(a, ni=ni, nj=nj, nk=nk)
+        out.update({f"{tag}_in_{k}": v.copy() for k, v in a.items()})
+        result = cpu_sound3_periodic_three_stage_inplace(a, ni=ni, nj=nj, nk=nk, **params)
+        out.update({f"{tag}_out_{k}": a[k] for k in COMPARE_OUTPUT_NAMES})
+        out[f"{tag}_dims"] = np.array([ni, nj, nk])
+        out[f"{tag}_params"] = np.array([params["dt"], params["nsound"], params["rdx"], params["rdy"], params["rdz"],
+                                         params.get("alph", 0.60), params.get("kdiv", 0.10)])
+        out[f"{tag}_cadence"] = np.array([[s.nrk, s.nloop, s.ppd_boundary_fills, s.stage_pressure_boundary_fills]
+                                          for s in result.cadence])
+    _save("sound3_periodic", **out)
+
+
 GENERATORS = {"advection5": advection5, "momentum5": momentum5, "psolver3": psolver3, "stage": stage, "pdef": pdef,
-              "kessler": kessler}
+              "kessler": kessler, "sound3_periodic": sound3_periodic}
 
 
 if __name__ == "__main__":
