@@ -227,9 +227,14 @@ struct ScalarAdvArgs {
   float* advz;
   int pdefweno;
   int ni, nj, nk, kchunk, nchunks;
-  float rdx, rdy, rdsf;
+  float rdx, rdy, rdsf;  // rdsf: 1/dz, or rdz of the metric reference on a flat stretched column
   double weps;
   unsigned obc;
+  // Flat stretched column (terrain_flag=.false., adv.F:458-478, 555-572,
+  // 646-656): mh (nk+2, CM1-indexed) or null.  With it the horizontal
+  // differences are -(fe-fw)*rdx, the vertical ((d*rdz)*mh(k)) and the
+  // tendency is scaled by rr0(1,1,k) alone.
+  const float* mh;
 };
 
 template <bool WENO, int MODE>
@@ -312,19 +317,25 @@ __global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
         if (ie) fe = fw;
         if (js) fs = fn;
         if (jn) fn = fs;
-        ax = (fw - fe) * a.rdx;
-        ay = (fs - fn) * a.rdy;
+        ax = a.mh ? (-(fe - fw)) * a.rdx : (fw - fe) * a.rdx;
+        ay = a.mh ? (-(fn - fs)) * a.rdy : (fs - fn) * a.rdy;
         if (iw) ax = ax - (sv * (ue - uw)) * a.rdx;
         if (ie) ax = ax - (sv * (ue - uw)) * a.rdx;
         if (js) ay = ay - (sv * (vn - vs)) * a.rdy;
         if (jn) ay = ay - (sv * (vn - vs)) * a.rdy;
       } else {
-        ax = (fw - fe) * a.rdx;
-        ay = (fs - fn) * a.rdy;
+        ax = a.mh ? (-(fe - fw)) * a.rdx : (fw - fe) * a.rdx;
+        ay = a.mh ? (-(fn - fs)) * a.rdy : (fs - fn) * a.rdy;
       }
-      if (k == 1) az = (-fzt) * a.rdsf;
-      else if (k == nk) az = fzb * a.rdsf;
-      else az = (fzb - fzt) * a.rdsf;
+      if (a.mh) {
+        if (k == 1) az = ((-fzt) * a.rdsf) * a.mh[k];
+        else if (k == nk) az = (fzb * a.rdsf) * a.mh[k];
+        else az = ((-(fzt - fzb)) * a.rdsf) * a.mh[k];
+      } else {
+        if (k == 1) az = (-fzt) * a.rdsf;
+        else if (k == nk) az = fzb * a.rdsf;
+        else az = (fzb - fzt) * a.rdsf;
+      }
       const int64_t q = (int64_t)(k - 1) * mst + mcol;
       if (MODE == MODE_PDEF) {
         a.advx[q] = ax;
@@ -333,7 +344,7 @@ __global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
       } else {
         float total = (ax + ay) + az;
         total = total + sv * a.divx[q];
-        const float adv = (total * a.rr0[q]) * a.gz[mcol];
+        const float adv = a.mh ? total * a.rr0[(int64_t)(k - 1) * mst] : (total * a.rr0[q]) * a.gz[mcol];
         if (MODE == MODE_TENDENCY) {
           a.tend[q] = adv;
         } else {
@@ -373,6 +384,11 @@ struct PdefArgs {
   int ni, nj, nk, kchunk, nchunks;
   float dt, rdt, rdx, rdy, rdz;
   unsigned obc;
+  // Flat stretched column (pdef.F:37-47, 263-275, 477-486, 548-557 with
+  // terrain_flag=.false.): mh (nk+2) or null; candidates rho0(1,1,k)*s0 +
+  // dt*(adv + div) without gz, corrections without rgz, vertical
+  // ((s*d)*rdz)*mh(k), the total scaled by rr0(1,1,k).
+  const float* mh;
 };
 
 // bc.F's halo cell c (-2..n+1) of a row of n cells as a physical index.
@@ -387,10 +403,11 @@ GM1_DEV float cand_x(const PdefArgs& a, int k, int j, int i) {
   const int64_t q = ((int64_t)(k - 1) * a.nj + j) * a.ni + i;
   const int x = i + NG, y = j + NG;
   const float uw = a.rru[ix3(k, y, x, nys, nxu)], ue = a.rru[ix3(k, y, x + 1, nys, nxu)];
-  const float r0a = a.rho0[q] * a.qa[q];
+  const float r0a = (a.mh ? a.rho0[(int64_t)(k - 1) * a.nj * a.ni] : a.rho0[q]) * a.qa[q];
   if ((i == 0 && side_of(a.obc, 0) == 2u && uw >= 0.0f) || (i + 1 == a.ni && side_of(a.obc, 1) == 2u && ue <= 0.0f))
     return r0a;
   const float d = ue - uw;
+  if (a.mh) return r0a + a.dt * (a.advx[q] + (a.s[ix3(k, y, x, nys, nxs)] * d) * a.rdx);
   return r0a + (a.dt * (a.advx[q] + (a.s[ix3(k, y, x, nys, nxs)] * d) * a.rdx)) * a.gz[(int64_t)j * a.ni + i];
 }
 
@@ -399,10 +416,11 @@ GM1_DEV float cand_y(const PdefArgs& a, int k, int j, int i) {
   const int64_t q = ((int64_t)(k - 1) * a.nj + j) * a.ni + i;
   const int x = i + NG, y = j + NG;
   const float vs = a.rrv[ix3(k, y, x, nyv, nxs)], vn = a.rrv[ix3(k, y + 1, x, nyv, nxs)];
-  const float r0a = a.rho0[q] * a.qa[q];
+  const float r0a = (a.mh ? a.rho0[(int64_t)(k - 1) * a.nj * a.ni] : a.rho0[q]) * a.qa[q];
   if ((j == 0 && side_of(a.obc, 2) == 2u && vs >= 0.0f) || (j + 1 == a.nj && side_of(a.obc, 3) == 2u && vn <= 0.0f))
     return r0a;
   const float d = vn - vs;
+  if (a.mh) return r0a + a.dt * (a.advy[q] + (a.s[ix3(k, y, x, nys, nxs)] * d) * a.rdy);
   return r0a + (a.dt * (a.advy[q] + (a.s[ix3(k, y, x, nys, nxs)] * d) * a.rdy)) * a.gz[(int64_t)j * a.ni + i];
 }
 
@@ -412,6 +430,10 @@ GM1_DEV float cand_z(const PdefArgs& a, int k, int j, int i) {
   const int64_t q = ((int64_t)(k - 1) * a.nj + j) * a.ni + i;
   const int x = i + NG, y = j + NG;
   const float d = a.rrw[ix3(k + 1, y, x, nys, nxs)] - a.rrw[ix3(k, y, x, nys, nxs)];
+  if (a.mh) {
+    const float r0c = a.rho0[(int64_t)(k - 1) * a.nj * a.ni] * a.qa[q];
+    return r0c + a.dt * (a.advz[q] + ((a.s[ix3(k, y, x, nys, nxs)] * d) * a.rdz) * a.mh[k]);
+  }
   const float r0a = a.rho0[q] * a.qa[q];
   return r0a + (a.dt * (a.advz[q] + (a.s[ix3(k, y, x, nys, nxs)] * d) * a.rdz)) * a.gz[(int64_t)j * a.ni + i];
 }
@@ -478,8 +500,9 @@ __global__ __launch_bounds__(RT) void k_pdef_repair(PdefArgs a) {
     if (live) {
       const int64_t q = (int64_t)(k - 1) * mst + mcol;
       const float sv = a.s[ix3(k, y, x, nys, nxs)];
-      const float r0a = a.rho0[q] * a.qa[q];
-      const float rg = a.rgz[mcol];
+      const bool flat = a.mh != nullptr;
+      const float r0a = (flat ? a.rho0[(int64_t)(k - 1) * mst] : a.rho0[q]) * a.qa[q];
+      const float rg = flat ? 1.0f : a.rgz[mcol];
       float ax = a.advx[q], ay = a.advy[q], az = a.advz[q];
       float v[5], mass;
       bool flag;
@@ -487,22 +510,25 @@ __global__ __launch_bounds__(RT) void k_pdef_repair(PdefArgs a) {
       row_mass(v, mass, flag);
       if (flag) {
         const float du = a.rru[ix3(k, y, x + 1, nys, nxu)] - a.rru[ix3(k, y, x, nys, nxu)];
-        ax = (((v[2] + mass) - r0a) * a.rdt) * rg - (sv * du) * a.rdx;
+        const float first = flat ? ((v[2] + mass) - r0a) * a.rdt : (((v[2] + mass) - r0a) * a.rdt) * rg;
+        ax = first - (sv * du) * a.rdx;
       }
       for (int d = 0; d < 5; ++d) v[d] = lcy[ty + d][tx];
       row_mass(v, mass, flag);
       if (flag) {
         const float dv = a.rrv[ix3(k, y + 1, x, nyv, nxs)] - a.rrv[ix3(k, y, x, nyv, nxs)];
-        ay = (((v[2] + mass) - r0a) * a.rdt) * rg - (sv * dv) * a.rdy;
+        const float first = flat ? ((v[2] + mass) - r0a) * a.rdt : (((v[2] + mass) - r0a) * a.rdt) * rg;
+        ay = first - (sv * dv) * a.rdy;
       }
       row_mass(cz, mass, flag);
       if (flag) {
         const float dw = a.rrw[ix3(k + 1, y, x, nys, nxs)] - a.rrw[ix3(k, y, x, nys, nxs)];
-        az = (((cz[2] + mass) - r0a) * a.rdt) * rg - (sv * dw) * a.rdz;
+        if (flat) az = ((cz[2] + mass) - r0a) * a.rdt - ((sv * dw) * a.rdz) * a.mh[k];
+        else az = (((cz[2] + mass) - r0a) * a.rdt) * rg - (sv * dw) * a.rdz;
       }
       float t = (ax + ay) + az;
       t = t + sv * a.divx[q];
-      const float adv = (t * a.rr0[q]) * a.gz[mcol];
+      const float adv = flat ? t * a.rr0[(int64_t)(k - 1) * mst] : (t * a.rr0[q]) * a.gz[mcol];
       const float o = a.qa[q] + a.dt * (a.held[q] + adv);
       a.out[q] = (fabsf(o) < 1.0e-30f) ? 0.0f : o;
     }
@@ -597,29 +623,44 @@ int gm1hip_scalar_rk_stage(const float* scalar, const float* rru, const float* r
   return launch_scalar_tile(a, use_weno, MODE_COMPOSE, (hipStream_t)stream);
 }
 
-int gm1hip_pdef_moisture_stage(const float* scalar, const float* rru, const float* rrv, const float* rrw,
-                               const float* c1, const float* c2, const float* rho0_3d, const float* rr0,
-                               const float* gz, const float* rgz, const float* divx, const float* qa,
-                               const float* held, float* out, float* advx, float* advy, float* advz, int ni,
-                               int nj, int nk, int pdefweno, double weps, float dx, float dy, float dz,
-                               unsigned obc, float dttmp, void* stream) {
+int gm1hip_scalar_rk_stage_flat(const float* scalar, const float* rru, const float* rrv, const float* rrw,
+                                const float* c1, const float* c2, const float* rr0, const float* flat_column,
+                                const float* divx, const float* a0, const float* held, const float* divterm, float* out,
+                                int ni, int nj, int nk, int use_weno, double weps, float dx, float dy, float rdz,
+                                unsigned obc, float dttmp, void* stream) {
+  if (obc == 0u || !flat_column) return (int)hipErrorInvalidValue;
+  ScalarAdvArgs a{};
+  a.s = scalar; a.rru = rru; a.rrv = rrv; a.rrw = rrw; a.c1 = c1; a.c2 = c2;
+  a.divx = divx; a.rr0 = rr0; a.gz = nullptr; a.mh = flat_column;
+  a.a0 = a0; a.held = held; a.divterm = divterm; a.out = out; a.dttmp = dttmp;
+  a.ni = ni; a.nj = nj; a.nk = nk;
+  a.rdx = 1.0f / dx; a.rdy = 1.0f / dy; a.rdsf = rdz;
+  a.weps = weps; a.obc = obc;
+  return launch_scalar_tile(a, use_weno, MODE_COMPOSE, (hipStream_t)stream);
+}
+
+static int pdef_stage(const float* scalar, const float* rru, const float* rrv, const float* rrw, const float* c1,
+                      const float* c2, const float* rho0_3d, const float* rr0, const float* gz, const float* rgz,
+                      const float* mh, const float* divx, const float* qa, const float* held, float* out,
+                      float* advx, float* advy, float* advz, int ni, int nj, int nk, int pdefweno, double weps,
+                      float dx, float dy, float rdzv, unsigned obc, float dttmp, void* stream) {
   if (obc == 0u || nk < 4) return (int)hipErrorInvalidValue;
   hipStream_t s = (hipStream_t)stream;
   ScalarAdvArgs a{};
   a.s = scalar; a.rru = rru; a.rrv = rrv; a.rrw = rrw; a.c1 = c1; a.c2 = c2;
-  a.divx = divx; a.rr0 = rr0; a.gz = gz;
+  a.divx = divx; a.rr0 = rr0; a.gz = gz; a.mh = mh;
   a.advx = advx; a.advy = advy; a.advz = advz; a.pdefweno = pdefweno;
   a.ni = ni; a.nj = nj; a.nk = nk;
-  a.rdx = 1.0f / dx; a.rdy = 1.0f / dy; a.rdsf = 1.0f / dz;
+  a.rdx = 1.0f / dx; a.rdy = 1.0f / dy; a.rdsf = rdzv;
   a.weps = weps; a.obc = obc;
   int rc = launch_scalar_tile(a, 1, MODE_PDEF, s);
   if (rc) return rc;
   PdefArgs p{};
   p.s = scalar; p.rru = rru; p.rrv = rrv; p.rrw = rrw;
-  p.rho0 = rho0_3d; p.rr0 = rr0; p.gz = gz; p.rgz = rgz; p.divx = divx;
+  p.rho0 = rho0_3d; p.rr0 = rr0; p.gz = gz; p.rgz = rgz; p.divx = divx; p.mh = mh;
   p.qa = qa; p.held = held; p.advx = advx; p.advy = advy; p.advz = advz; p.out = out;
   p.ni = ni; p.nj = nj; p.nk = nk;
-  p.dt = dttmp; p.rdt = 1.0f / dttmp; p.rdx = 1.0f / dx; p.rdy = 1.0f / dy; p.rdz = 1.0f / dz;
+  p.dt = dttmp; p.rdt = 1.0f / dttmp; p.rdx = 1.0f / dx; p.rdy = 1.0f / dy; p.rdz = rdzv;
   p.obc = obc;
   const int tiles = ((ni + RX - 1) / RX) * ((nj + RY - 1) / RY);
   const int chunks = chunks_for(tiles, nk, &p.kchunk);
@@ -627,6 +668,27 @@ int gm1hip_pdef_moisture_stage(const float* scalar, const float* rru, const floa
   hipLaunchKernelGGL(k_pdef_repair, dim3(tiles * chunks), dim3(RT), 0, s, p);
   GM1_LAUNCH_CHECK();
   return 0;
+}
+
+int gm1hip_pdef_moisture_stage(const float* scalar, const float* rru, const float* rrv, const float* rrw,
+                               const float* c1, const float* c2, const float* rho0_3d, const float* rr0,
+                               const float* gz, const float* rgz, const float* divx, const float* qa,
+                               const float* held, float* out, float* advx, float* advy, float* advz, int ni,
+                               int nj, int nk, int pdefweno, double weps, float dx, float dy, float dz,
+                               unsigned obc, float dttmp, void* stream) {
+  return pdef_stage(scalar, rru, rrv, rrw, c1, c2, rho0_3d, rr0, gz, rgz, nullptr, divx, qa, held, out, advx, advy,
+                    advz, ni, nj, nk, pdefweno, weps, dx, dy, 1.0f / dz, obc, dttmp, stream);
+}
+
+int gm1hip_pdef_moisture_stage_flat(const float* scalar, const float* rru, const float* rrv, const float* rrw,
+                                    const float* c1, const float* c2, const float* rho0_3d, const float* rr0,
+                                    const float* flat_column, const float* divx, const float* qa, const float* held,
+                                    float* out, float* advx, float* advy, float* advz, int ni, int nj, int nk,
+                                    int pdefweno, double weps, float dx, float dy, float rdz, unsigned obc,
+                                    float dttmp, void* stream) {
+  if (!flat_column) return (int)hipErrorInvalidValue;
+  return pdef_stage(scalar, rru, rrv, rrw, c1, c2, rho0_3d, rr0, nullptr, nullptr, flat_column, divx, qa, held, out,
+                    advx, advy, advz, ni, nj, nk, pdefweno, weps, dx, dy, rdz, obc, dttmp, stream);
 }
 
 }  // extern "C"

@@ -38,6 +38,11 @@ struct MomArgs {
   int ni, nj, nk;
   float rdx, rdy, rdz;
   unsigned obc;
+  // Flat stretched column (terrain_flag=.false.): mh (nk+2) then mf (nk+3),
+  // CM1-indexed; null on the terrain-coordinate family.  rr0 is the base
+  // state's reciprocal density, read at the first column.
+  const float* flat;
+  const float* rr0;
 };
 
 GM1_DEV bool per(unsigned obc, unsigned s) { return side_of(obc, s) == 1u; }
@@ -115,6 +120,52 @@ __global__ __launch_bounds__(256) void k_mom_mass_v(
               (rrv[ix3(k, y + 1, x, nyv, nxs)] - rrv[ix3(k, y, x, nyv, nxs)]) * rdy;
     d = d + (val - below) * rdz;
     divx[(int64_t)(k - 1) * nj * ni + h] = (fabsf(d) < 1.0e-20f) ? 0.0f : d;
+    below = val;
+  }
+}
+
+// Flat stretched column (solve2.F:493-520, 605-617 with terrain_flag=.false.):
+// rru = rho0(1,1,k)*u, rrv = rho0(1,1,k)*v, rrw = rf0(1,1,k)*w (zero at both
+// lids), divx's vertical term ((d*rdz)*mh(k)) and smeps 1e-30.
+__global__ __launch_bounds__(256) void k_mom_mass_h_flat(
+    const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ rho0,
+    float* __restrict__ rru, float* __restrict__ rrv, int ni, int nj, int nk) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  const int k0 = blockIdx.z, k = k0 + 1;
+  const float rc = rho0[(int64_t)k0 * nj * ni];
+  if (i <= ni && j < nj) {
+    const int64_t p = ix3(k, j + NG, i + NG, sy(nj), ux(ni));
+    rru[p] = rc * u[p];
+  }
+  if (i < ni && j <= nj) {
+    const int64_t p = ix3(k, j + NG, i + NG, vy(nj), sx(ni));
+    rrv[p] = rc * v[p];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mom_mass_v_flat(
+    const float* __restrict__ w, const float* __restrict__ rf0, const float* __restrict__ rru,
+    const float* __restrict__ rrv, const float* __restrict__ mh, float* __restrict__ rrw,
+    float* __restrict__ divx, int ni, int nj, int nk, float rdx, float rdy, float rdz) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= ni) return;
+  const int nxu = ux(ni), nyu = sy(nj), nxs = sx(ni), nyv = vy(nj);
+  const int x = i + NG, y = j + NG;
+  const int64_t h = (int64_t)j * ni + i;
+  const int64_t wst = (int64_t)nyu * nxs;
+  const int64_t wcol = (int64_t)y * nxs + x;
+  rrw[wst + wcol] = 0.0f;
+  float below = 0.0f;
+  for (int kf = 2; kf <= nk + 1; ++kf) {
+    const float val = (kf == nk + 1) ? 0.0f : rf0[(int64_t)(kf - 1) * nj * ni] * w[(int64_t)kf * wst + wcol];
+    rrw[(int64_t)kf * wst + wcol] = val;
+    const int k = kf - 1;
+    float d = (rru[ix3(k, y, x + 1, nyu, nxu)] - rru[ix3(k, y, x, nyu, nxu)]) * rdx +
+              (rrv[ix3(k, y + 1, x, nyv, nxs)] - rrv[ix3(k, y, x, nyv, nxs)]) * rdy;
+    d = d + ((val - below) * rdz) * mh[k];
+    divx[(int64_t)(k - 1) * nj * ni + h] = (fabsf(d) < 1.0e-30f) ? 0.0f : d;
     below = val;
   }
 }
@@ -218,7 +269,7 @@ __global__ __launch_bounds__(256) void k_mom_advu(MomArgs a, int f0, int nf) {
     const float m = R0[east + NG] + R0[west + NG];
     if (m >= 0.0f) {
       const float c = (R1[west + NG] - R0[west + NG]) + (R1[east + NG] - R0[east + NG]);
-      ay = ay - (uc * 0.5f) * (c * (a.rdy * a.vh[y]));
+      ay = a.flat ? ay - (((uc * 0.5f) * c) * a.rdy) * a.vh[y] : ay - (uc * 0.5f) * (c * (a.rdy * a.vh[y]));
     }
   } else if (j == nj - 1 && opn(a.obc, 3)) {
     const float* RN = a.rrv + ix3(k, NG + nj, 0, nyv, nxv);
@@ -227,17 +278,19 @@ __global__ __launch_bounds__(256) void k_mom_advu(MomArgs a, int f0, int nf) {
     const float m = RN[east + NG] + RN[west + NG];
     if (m <= 0.0f) {
       const float c = (R1[west + NG] - R0[west + NG]) + (R1[east + NG] - R0[east + NG]);
-      ay = ay - (uc * 0.5f) * (c * (a.rdy * a.vh[y]));
+      ay = a.flat ? ay - (((uc * 0.5f) * c) * a.rdy) * a.vh[y] : ay - (uc * 0.5f) * (c * (a.rdy * a.vh[y]));
     }
   }
   float az;
   if (k == 1) az = (-uvz<0>(a, 2, j, f)) * a.rdz;
   else if (k == nk) az = uvz<0>(a, nk, j, f) * a.rdz;
   else az = (-(uvz<0>(a, k + 1, j, f) - uvz<0>(a, k, j, f))) * a.rdz;
+  if (a.flat) az = az * a.flat[k];
   float tot = (ax + ay) + az;
   const int64_t row = ((int64_t)k0 * nj + j) * ni;
   tot = tot + uc * (0.5f * (a.divx[row + east] + a.divx[row + west]));
-  const float sc = a.gzu[(int64_t)j * (ni + 1) + f] / (0.5f * (a.rho0[row + west] + a.rho0[row + east]));
+  const float sc = a.flat ? a.rr0[(int64_t)k0 * nj * ni]
+                          : a.gzu[(int64_t)j * (ni + 1) + f] / (0.5f * (a.rho0[row + west] + a.rho0[row + east]));
   const int64_t p = ix3(k, y, x, nyu, nxu);
   a.uten[p] = a.uten[p] + tot * sc;
 }
@@ -298,24 +351,26 @@ __global__ __launch_bounds__(256) void k_mom_advv(MomArgs a, int g0, int nr) {
     const float m = Rn[NG] + Rs[NG];
     if (m >= 0.0f) {
       const float c = (Rs[NG + 1] - Rs[NG]) + (Rn[NG + 1] - Rn[NG]);
-      ax = ax - (vc * 0.5f) * (c * (a.rdx * a.uh[NG]));
+      ax = a.flat ? ax - (((vc * 0.5f) * c) * a.rdx) * a.uh[NG] : ax - (vc * 0.5f) * (c * (a.rdx * a.uh[NG]));
     }
   } else if (i == ni - 1 && opn(a.obc, 1)) {
     const float m = Rn[NG + ni] + Rs[NG + ni];
     if (m <= 0.0f) {
       const float c = (Rs[x + 1] - Rs[x]) + (Rn[x + 1] - Rn[x]);
-      ax = ax - (vc * 0.5f) * (c * (a.rdx * a.uh[x]));
+      ax = a.flat ? ax - (((vc * 0.5f) * c) * a.rdx) * a.uh[x] : ax - (vc * 0.5f) * (c * (a.rdx * a.uh[x]));
     }
   }
   float az;
   if (k == 1) az = (-uvz<1>(a, 2, g, i)) * a.rdz;
   else if (k == nk) az = uvz<1>(a, nk, g, i) * a.rdz;
   else az = (-(uvz<1>(a, k + 1, g, i) - uvz<1>(a, k, g, i))) * a.rdz;
+  if (a.flat) az = az * a.flat[k];
   float tot = (ax + ay) + az;
   const int64_t plane = (int64_t)k0 * nj * ni;
   tot = tot + vc * (0.5f * (a.divx[plane + (int64_t)north * ni + i] + a.divx[plane + (int64_t)south * ni + i]));
-  const float sc = a.gzv[(int64_t)g * ni + i] /
-                   (0.5f * (a.rho0[plane + (int64_t)south * ni + i] + a.rho0[plane + (int64_t)north * ni + i]));
+  const float sc = a.flat ? a.rr0[plane]
+                          : a.gzv[(int64_t)g * ni + i] /
+                                (0.5f * (a.rho0[plane + (int64_t)south * ni + i] + a.rho0[plane + (int64_t)north * ni + i]));
   const int64_t p = ix3(k, y, x, nyv, nxv);
   a.vten[p] = a.vten[p] + tot * sc;
 }
@@ -398,8 +453,15 @@ __global__ __launch_bounds__(256) void k_mom_advw(MomArgs a) {
   if (i >= ni) return;
   const int nxu = ux(ni), nyu = sy(nj), nxs = sx(ni), nys = sy(nj), nyv = vy(nj);
   const int y = j + NG, x = i + NG;
-  float ax = (-(wfx(a, kf, j, i + 1) - wfx(a, kf, j, i))) * (a.rdx * a.uh[x]);
-  float ay = (-(wfy(a, kf, j + 1, i) - wfy(a, kf, j, i))) * (a.rdy * a.vh[y]);
+  const bool flat = a.flat != nullptr;
+  float ax, ay;
+  if (flat) {
+    ax = ((-(wfx(a, kf, j, i + 1) - wfx(a, kf, j, i))) * a.rdx) * a.uh[x];
+    ay = ((-(wfy(a, kf, j + 1, i) - wfy(a, kf, j, i))) * a.rdy) * a.vh[y];
+  } else {
+    ax = (-(wfx(a, kf, j, i + 1) - wfx(a, kf, j, i))) * (a.rdx * a.uh[x]);
+    ay = (-(wfy(a, kf, j + 1, i) - wfy(a, kf, j, i))) * (a.rdy * a.vh[y]);
+  }
   const int64_t h = (int64_t)j * ni + i;
   const float c2 = a.c2[(int64_t)(kf - 1) * nj * ni + h];
   const float c1 = 1.0f - c2;
@@ -410,7 +472,7 @@ __global__ __launch_bounds__(256) void k_mom_advw(MomArgs a) {
     const float m = c2 * Rt[NG] + c1 * Rb[NG];
     if (m >= 0.0f) {
       const float c = c1 * (Rb[NG + 1] - Rb[NG]) + c2 * (Rt[NG + 1] - Rt[NG]);
-      ax = ax - wc0 * (c * (a.rdx * a.uh[x]));
+      ax = flat ? ax - ((wc0 * c) * a.rdx) * a.uh[x] : ax - wc0 * (c * (a.rdx * a.uh[x]));
     }
   } else if (i == ni - 1 && opn(a.obc, 1)) {
     const float* Rt = a.rru + ix3(kf, y, 0, nyu, nxu);
@@ -418,7 +480,7 @@ __global__ __launch_bounds__(256) void k_mom_advw(MomArgs a) {
     const float m = c2 * Rt[NG + ni] + c1 * Rb[NG + ni];
     if (m <= 0.0f) {
       const float c = c1 * (Rb[x + 1] - Rb[x]) + c2 * (Rt[x + 1] - Rt[x]);
-      ax = ax - wc0 * (c * (a.rdx * a.uh[x]));
+      ax = flat ? ax - ((wc0 * c) * a.rdx) * a.uh[x] : ax - wc0 * (c * (a.rdx * a.uh[x]));
     }
   }
   if (j == 0 && opn(a.obc, 2)) {
@@ -426,20 +488,21 @@ __global__ __launch_bounds__(256) void k_mom_advw(MomArgs a) {
     if (m >= 0.0f) {
       const float c = c1 * (a.rrv[ix3(kf - 1, NG + 1, x, nyv, nxs)] - a.rrv[ix3(kf - 1, NG, x, nyv, nxs)]) +
                       c2 * (a.rrv[ix3(kf, NG + 1, x, nyv, nxs)] - a.rrv[ix3(kf, NG, x, nyv, nxs)]);
-      ay = ay - wc0 * (c * (a.rdy * a.vh[y]));
+      ay = flat ? ay - ((wc0 * c) * a.rdy) * a.vh[y] : ay - wc0 * (c * (a.rdy * a.vh[y]));
     }
   } else if (j == nj - 1 && opn(a.obc, 3)) {
     const float m = c2 * a.rrv[ix3(kf, NG + nj, x, nyv, nxs)] + c1 * a.rrv[ix3(kf - 1, NG + nj, x, nyv, nxs)];
     if (m <= 0.0f) {
       const float c = c1 * (a.rrv[ix3(kf - 1, y + 1, x, nyv, nxs)] - a.rrv[ix3(kf - 1, y, x, nyv, nxs)]) +
                       c2 * (a.rrv[ix3(kf, y + 1, x, nyv, nxs)] - a.rrv[ix3(kf, y, x, nyv, nxs)]);
-      ay = ay - wc0 * (c * (a.rdy * a.vh[y]));
+      ay = flat ? ay - ((wc0 * c) * a.rdy) * a.vh[y] : ay - wc0 * (c * (a.rdy * a.vh[y]));
     }
   }
-  const float az = (-(wz(a, kf, j, i) - wz(a, kf - 1, j, i))) * a.rdz;
+  float az = (-(wz(a, kf, j, i) - wz(a, kf - 1, j, i))) * a.rdz;
+  if (flat) az = az * a.flat[(a.nk + 2) + kf];
   float tot = (ax + ay) + az;
   tot = tot + wc0 * (c2 * a.divx[(int64_t)(kf - 1) * nj * ni + h] + c1 * a.divx[(int64_t)(kf - 2) * nj * ni + h]);
-  const float sc = a.rrf0[(int64_t)(kf - 1) * nj * ni + h] * a.gz[h];
+  const float sc = flat ? a.rrf0[(int64_t)(kf - 1) * nj * ni] : a.rrf0[(int64_t)(kf - 1) * nj * ni + h] * a.gz[h];
   const int64_t p = ix3(kf, y, x, nys, nxs);
   a.wten[p] = a.wten[p] + tot * sc;
 }
@@ -470,6 +533,7 @@ int gm1hip_momentum_advection5(
   a.rho0 = rho0_3d; a.c2 = c2; a.gzu = gzu; a.gzv = gzv; a.rrf0 = rrf0; a.gz = gz; a.uh = uh; a.vh = vh;
   a.uten = uten; a.vten = vten; a.wten = wten; a.ni = ni; a.nj = nj; a.nk = nk;
   a.rdx = rdx; a.rdy = rdy; a.rdz = rdz; a.obc = obc;
+  a.flat = nullptr; a.rr0 = nullptr;
   const bool pw = side_of(obc, 0) == 1u, pe = side_of(obc, 1) == 1u;
   const bool ps = side_of(obc, 2) == 1u, pn = side_of(obc, 3) == 1u;
   const int f0 = pw ? 0 : 1, nf = ni - 1 + (pw ? 1 : 0) + (pe ? 1 : 0);
@@ -495,7 +559,43 @@ int gm1hip_momentum_transport5(
   a.rho0 = rho0_3d; a.c2 = c2; a.gzu = gzu; a.gzv = gzv; a.rrf0 = rrf0; a.gz = gz; a.uh = uh; a.vh = vh;
   a.uten = uten; a.vten = vten; a.wten = wten; a.ni = ni; a.nj = nj; a.nk = nk;
   a.rdx = 1.0f / dx; a.rdy = 1.0f / dy; a.rdz = 1.0f / dz; a.obc = obc;
+  a.flat = nullptr; a.rr0 = nullptr;
   dim3 blk(64);
+  const bool pw = side_of(obc, 0) == 1u, pe = side_of(obc, 1) == 1u;
+  const bool ps = side_of(obc, 2) == 1u, pn = side_of(obc, 3) == 1u;
+  const int f0 = pw ? 0 : 1, nf = ni - 1 + (pw ? 1 : 0) + (pe ? 1 : 0);
+  const int g0 = ps ? 0 : 1, nr = nj - 1 + (ps ? 1 : 0) + (pn ? 1 : 0);
+  GM1_TIMED(TK_MOM_ADVU, s, hipLaunchKernelGGL(k_mom_advu, dim3((nf + 63) / 64, nj, nk), blk, 0, s, a, f0, nf));
+  GM1_LAUNCH_CHECK();
+  GM1_TIMED(TK_MOM_ADVV, s, hipLaunchKernelGGL(k_mom_advv, dim3((ni + 63) / 64, nr, nk), blk, 0, s, a, g0, nr));
+  GM1_LAUNCH_CHECK();
+  GM1_TIMED(TK_MOM_ADVW, s, hipLaunchKernelGGL(k_mom_advw, dim3((ni + 63) / 64, nj, nk - 1), blk, 0, s, a));
+  GM1_LAUNCH_CHECK();
+  return 0;
+}
+
+// The flat stretched column's momentum advection (launch_cuda_gm1_child_stretched_momentum5
+// with a flat column, cuda_gm1_child_stretched_momentum.py:655-757).
+int gm1hip_momentum_advection5_flat(
+    const float* u3d, const float* v3d, const float* w3d, float* uten, float* vten, float* wten,
+    float* rru, float* rrv, float* rrw, const float* uh, const float* vh, const float* rho0_3d,
+    const float* rf0, const float* rr0, const float* rrf0, const float* c2, const float* flat_column,
+    float* divx, int ni, int nj, int nk, float dx, float dy, float rdz, unsigned obc, void* stream) {
+  if (!flat_column) return (int)hipErrorInvalidValue;
+  hipStream_t s = (hipStream_t)stream;
+  const float rdx = 1.0f / dx, rdy = 1.0f / dy;
+  dim3 blk(64);
+  hipLaunchKernelGGL(k_mom_mass_h_flat, dim3((ni + 1 + 63) / 64, nj + 1, nk), blk, 0, s, u3d, v3d, rho0_3d, rru,
+                     rrv, ni, nj, nk);
+  GM1_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_mom_mass_v_flat, dim3((ni + 63) / 64, nj), blk, 0, s, w3d, rf0, rru, rrv, flat_column, rrw,
+                     divx, ni, nj, nk, rdx, rdy, rdz);
+  GM1_LAUNCH_CHECK();
+  MomArgs a;
+  a.u = u3d; a.v = v3d; a.w = w3d; a.rru = rru; a.rrv = rrv; a.rrw = rrw; a.divx = divx;
+  a.rho0 = rho0_3d; a.c2 = c2; a.gzu = nullptr; a.gzv = nullptr; a.rrf0 = rrf0; a.gz = nullptr; a.uh = uh;
+  a.vh = vh; a.uten = uten; a.vten = vten; a.wten = wten; a.ni = ni; a.nj = nj; a.nk = nk;
+  a.rdx = rdx; a.rdy = rdy; a.rdz = rdz; a.obc = obc; a.flat = flat_column; a.rr0 = rr0;
   const bool pw = side_of(obc, 0) == 1u, pe = side_of(obc, 1) == 1u;
   const bool ps = side_of(obc, 2) == 1u, pn = side_of(obc, 3) == 1u;
   const int f0 = pw ? 0 : 1, nf = ni - 1 + (pw ? 1 : 0) + (pe ? 1 : 0);

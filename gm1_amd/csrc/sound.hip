@@ -36,6 +36,11 @@ struct SoundArgs {
   int ni, nj, nk, nrk, nloop;
   unsigned sides;
   float dts, rdx, rdy, alph, beta, kdiv, zt, rzt, cp;
+  // Flat stretched column (sound.F with terrain_flag=.false.): mh (nk+2)
+  // then mf (nk+3), CM1-indexed, and rdz = 1/dz of the metric reference;
+  // null on the terrain-coordinate family.
+  const float* flat;
+  float rdzf;
 };
 
 GM1_DEV const float* P_sigma(const SoundArgs& a) { return a.prof; }
@@ -71,7 +76,23 @@ __global__ __launch_bounds__(256) void k_snd_prepare(SoundArgs a) {
     a.w3d[pw] = a.wa[pw];
   }
   float qk = 0.0f, pk1 = 0.0f, pk2 = 0.0f, mm = 0.0f;
-  if (i >= 1 && i <= ni && j >= 1 && j <= nj) {
+  if (a.flat && i >= 1 && i <= ni && j >= 1 && j <= nj) {
+    // sound.F:119-143: ((dts*rdz)*mh(k)) and the base state at CM1's (1,1,k)
+    const int64_t plane = (int64_t)nj * ni, c0 = (int64_t)(k - 1) * plane, c1i = (int64_t)k * plane;
+    const float mhk = a.flat[k], mfk = a.flat[(nk + 2) + k];
+    const float dr = a.dts * a.rdzf;
+    const float tem = dr * mhk;
+    const float r2 = ((dr * mhk) * a.rr0[c0]) * a.rf0[c1i];
+    const float r1 = ((dr * mhk) * a.rr0[c0]) * a.rf0[c0];
+    const float tem1 = (dr * a.cp) * mfk;
+    const float c1a = a.c1[c0], c1b = a.c1[c1i], c2a = a.c2[c0], c2b = a.c2[c1i];
+    qk = tem * a.ppterm[p];
+    pk2 = r2 * ((-c2b) * a.piadv[pix(k + 1, j, i, ni, nj)] + (1.0f - c1b) * a.piadv[p]);
+    pk1 = r1 * (c1a * a.piadv[pix(k - 1, j, i, ni, nj)] + (c2a - 1.0f) * a.piadv[p]);
+    mm = tem1 * (c2a * a.thv[p] + c1a * a.thv[pix(k - 1, j, i, ni, nj)]);
+    if (k == 1) pk1 = 0.0f;
+    if (k == nk) pk2 = 0.0f;
+  } else if (i >= 1 && i <= ni && j >= 1 && j <= nj) {
     const int i0 = i - 1, j0 = j - 1;
     const int64_t h = (int64_t)j0 * ni + i0, plane = (int64_t)nj * ni;
     const float g = a.gz[h];
@@ -330,6 +351,84 @@ __global__ __launch_bounds__(PG_TX * PG_TY) void k_snd_pgrad(SoundArgs a) {
   }
 }
 
+// ------------------------------------------------------------------ flat stretched column
+// bc.F:2410-2419 and its three twins on a flat column: the open boundary
+// face steps u(1) += dts*((((-radbcw*(u(2)-u(1)))*rdx)*uh(1)) + uten(1)) with
+// no reconstruction at w points.  Only boundary faces are written; the
+// interior faces read are never written here.
+__global__ __launch_bounds__(256) void k_snd_ssopen_flat(SoundArgs a) {
+  const int ni = a.ni, nj = a.nj, nk = a.nk;
+  const int64_t we = (int64_t)nk * nj, sn = (int64_t)nk * ni;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < we) {
+    const int j0 = (int)(q % nj), k = (int)(q / nj) + 1, y = j0 + NG;
+    const int nxu = ux(ni), nyu = sy(nj);
+    if (side_of(a.sides, 0) == 2u) {
+      const int64_t b = ix3(k, y, NG, nyu, nxu);
+      const float t = ((((-a.radbcw[(int64_t)k * (nj + 2 * NG) + y]) * (a.u3d[b + 1] - a.u3d[b])) * a.rdx) * a.uh[NG]) +
+                      a.uten[b];
+      a.u3d[b] = a.u3d[b] + a.dts * t;
+    }
+    if (side_of(a.sides, 1) == 2u) {
+      const int64_t b = ix3(k, y, ni + NG, nyu, nxu);
+      const float t =
+          ((((-a.radbce[(int64_t)k * (nj + 2 * NG) + y]) * (a.u3d[b] - a.u3d[b - 1])) * a.rdx) * a.uh[ni + NG - 1]) +
+          a.uten[b];
+      a.u3d[b] = a.u3d[b] + a.dts * t;
+    }
+    return;
+  }
+  const int64_t r = q - we;
+  if (r < sn) {
+    const int i0 = (int)(r % ni), k = (int)(r / ni) + 1, x = i0 + NG;
+    const int nxv = sx(ni), nyv = vy(nj);
+    if (side_of(a.sides, 2) == 2u) {
+      const int64_t b = ix3(k, NG, x, nyv, nxv);
+      const float t = ((((-a.radbcs[(int64_t)k * (ni + 2 * NG) + x]) * (a.v3d[b + nxv] - a.v3d[b])) * a.rdy) * a.vh[NG]) +
+                      a.vten[b];
+      a.v3d[b] = a.v3d[b] + a.dts * t;
+    }
+    if (side_of(a.sides, 3) == 2u) {
+      const int64_t b = ix3(k, nj + NG, x, nyv, nxv);
+      const float t =
+          ((((-a.radbcn[(int64_t)k * (ni + 2 * NG) + x]) * (a.v3d[b] - a.v3d[b - nxv])) * a.rdy) * a.vh[nj + NG - 1]) +
+          a.vten[b];
+      a.v3d[b] = a.v3d[b] + a.dts * t;
+    }
+  }
+}
+
+// sound.F:318-342 on a flat column: every face i = 1..ni+1, j = 1..nj+1 takes
+// u += (dts*dtu)*(uten - ((tem1*dpp)*(thv(i)+thv(i-1)))), dtu = 0 on an open
+// boundary face and on the row outside the grid (param.F:9182-9219).
+__global__ __launch_bounds__(PG_TX * PG_TY) void k_snd_pgrad_flat(SoundArgs a) {
+  const int ni = a.ni, nj = a.nj, nk = a.nk;
+  const int gx = (ni + 1 + PG_TX - 1) / PG_TX, gy = (nj + 1 + PG_TY - 1) / PG_TY;
+  const int tile = xcd_tile((int)blockIdx.x, gx * gy * nk);
+  const int k = tile / (gx * gy) + 1;
+  const int rem = tile % (gx * gy);
+  const int x = (rem % gx) * PG_TX + (int)threadIdx.x % PG_TX;  // i - 1, 0..ni
+  const int y = (rem / gx) * PG_TY + (int)threadIdx.x / PG_TX;  // j - 1, 0..nj
+  if (x > ni || y > nj) return;
+  const int i = x + 1, j = y + 1;
+  const float tem1 = (a.rdx * a.cp) * 0.5f, tem2 = (a.rdy * a.cp) * 0.5f;
+  const bool pw = side_of(a.sides, 0) == 1u, pe = side_of(a.sides, 1) == 1u;
+  const bool ps = side_of(a.sides, 2) == 1u, pn = side_of(a.sides, 3) == 1u;
+  const float wu = (j <= nj && (i > 1 || pw) && (i < ni + 1 || pe)) ? 1.0f : 0.0f;
+  const float wv = (i <= ni && (j > 1 || ps) && (j < nj + 1 || pn)) ? 1.0f : 0.0f;
+  const float* ppd = a.ppd;
+  const float* thv = a.thv;
+  const int64_t pc = pix(k, j, i, ni, nj);
+  float pu = tem1 * (ppd[pc] - ppd[pc - 1]);
+  pu = pu * (thv[pc] + thv[pc - 1]);
+  const int64_t xu0 = ix3(k, y + NG, x + NG, sy(nj), ux(ni));
+  a.u3d[xu0] = a.u3d[xu0] + (a.dts * wu) * (a.uten[xu0] - pu);
+  float pv = tem2 * (ppd[pc] - ppd[pc - (ni + 2)]);
+  pv = pv * (thv[pc] + thv[pc - (ni + 2)]);
+  const int64_t xv0 = ix3(k, y + NG, x + NG, vy(nj), sx(ni));
+  a.v3d[xv0] = a.v3d[xv0] + (a.dts * wv) * (a.vten[xv0] - pv);
+}
+
 // ------------------------------------------------------------------ column
 struct ColumnFaces {
   float xw, xe, ys, yn;  // xu at faces i, i+1 and yv at faces j, j+1 of one level
@@ -528,7 +627,7 @@ constexpr int C2PLANES = 6;  // fpk, pp, s1 = pk1+qk, s2 = pk2-qk, mm -> ec, dd 
 
 int column2_lds_bytes(int nk, int cw) { return C2PLANES * (nk + 1) * cw * (int)sizeof(float); }
 
-template <int C2W, int C2T>
+template <int C2W, int C2T, bool FLAT>
 __global__ __launch_bounds__(C2T) void k_snd_column2(SoundArgs a, int n) {
   constexpr int C2S = C2T / C2W;
   extern __shared__ float lds[];
@@ -562,10 +661,47 @@ __global__ __launch_bounds__(C2T) void k_snd_column2(SoundArgs a, int n) {
   const int chunk = (nk + C2S - 1) / C2S;
   const int kb = 1 + slot * chunk, ke = min(nk, kb + chunk - 1);
   const float asq = a.alph * a.alph;
-  const float smeps = 1.0e-20f;
+  const float smeps = FLAT ? 1.0e-30f : 1.0e-20f;
 
+  // ---------------- phase A (flat column, sound.F:491-509): fpk from the
+  // horizontal divergence, the flat pressure advection and w as it is
+  if (FLAT && phys && kb <= ke) {
+    const int pj = j + 1, pi = i + 1;
+    const float temx = (a.dts * 0.5f) * a.rdx, temy = (a.dts * 0.5f) * a.rdy;
+    if (kb == 1) lw1[c] = w[wst + wc];
+    for (int k = kb; k <= ke; ++k) {
+      const int64_t p = pix(k, pj, pi, ni, nj);
+      const float uw = a.u3d[(int64_t)k * ust + uc], ue = a.u3d[(int64_t)k * ust + uc + 1];
+      const float vs = a.v3d[(int64_t)k * vst + vc], vn = a.v3d[(int64_t)k * vst + vc + nxs];
+      float div = (ue - uw) * a.rdx + (vn - vs) * a.rdy;
+      if (fabsf(div) < smeps) div = 0.0f;
+      const float pc = a.piadv[p];
+      float hx = uw * (pc - a.piadv[p - 1]) + ue * (a.piadv[p + 1] - pc);
+      hx = (-hx) * temx;
+      float hy = vs * (pc - a.piadv[p - (ni + 2)]) + vn * (a.piadv[p + (ni + 2)] - pc);
+      hy = (-hy) * temy;
+      const float hp = hx + hy;
+      const float wraw = w[(int64_t)k * wst + wc];
+      const float qkk = a.qk[p], pk1k = a.pk1[p], pk2k = a.pk2[p];
+      const float s1 = pk1k + qkk, s2 = pk2k - qkk;
+      const float vert = wraw * s1 + w[(int64_t)(k + 1) * wst + wc] * s2;
+      Lfpk[at(k)] = (a.dts * (a.ppten[p] - a.ppterm[p] * div) + hp) + a.beta * vert;
+      Lpp[at(k)] = a.pp3d[p];
+      Ls1[at(k)] = s1;
+      Ls2[at(k)] = s2;
+      Lmm[at(k)] = a.mm[p];
+      Ldd[at(k)] = wraw;
+      {
+        const int64_t pu = (int64_t)k * ust + uc, pv = (int64_t)k * vst + vc;
+        const float au = a.rru[pu] + uw;
+        a.rru[pu] = last ? au * tavg : au;
+        const float av = a.rrv[pv] + vs;
+        a.rrv[pv] = last ? av * tavg : av;
+      }
+    }
+  }
   // ---------------- phase A: fpk and the per-level terms
-  if (phys && kb <= ke) {
+  if (!FLAT && phys && kb <= ke) {
     const int pj = j + 1, pi = i + 1;
     const int64_t h = (int64_t)j * ni + i, plane = (int64_t)nj * ni;
     const float* sigma = P_sigma(a);
@@ -747,10 +883,10 @@ __global__ __launch_bounds__(C2T) void k_snd_column2(SoundArgs a, int n) {
     for (int k = kb; k <= ke; ++k) {
       const int64_t p = pix(k, pj, pi, ni, nj);
       const float wk = (k == 1) ? lw1[c] : Ldd[at(k)];
-      const float wup = (k == nk) ? 0.0f : Ldd[at(k + 1)];
+      const float wup = (k == nk) ? (FLAT ? w[(int64_t)(nk + 1) * wst + wc] : 0.0f) : Ldd[at(k + 1)];
       if (k >= 2) w[(int64_t)k * wst + wc] = wk;
-      const float w1 = (k == 1) ? 0.0f : wk;
-      const float w2 = (k == nk) ? 0.0f : wup;
+      const float w1 = (k == 1 && !FLAT) ? 0.0f : wk;
+      const float w2 = (k == nk && !FLAT) ? 0.0f : wup;
       const float old = Lpp[at(k)];
       const float vt = w1 * Ls1[at(k)] + w2 * Ls2[at(k)];
       float up = (old + Lfpk[at(k)]) + a.alph * vt;
@@ -846,7 +982,7 @@ __global__ __launch_bounds__(64) void k_snd_radiation(SoundArgs a, float dx, flo
 
 }  // namespace
 
-extern "C" int gm1hip_psolver3_stage(
+static int psolver3_stage(
     float* ppi, float* pp3d, float* ppx, float* ppd, const float* piadv, const float* ppterm, const float* ppten,
     const float* thv, float* qk, float* pk1, float* pk2, float* mm, float* fpk, float* elim_c, float* elim_d,
     const float* ua, float* u3d, const float* uten, float* rru, const float* va, float* v3d, const float* vten,
@@ -855,8 +991,8 @@ extern "C" int gm1hip_psolver3_stage(
     float* scalar, const float* rho0_3d, const float* rr0, const float* rf0, const float* rrf0, const float* c1,
     const float* c2, const float* gz, const float* rgzu, const float* rgzv, const float* dzdx, const float* dzdy,
     const float* gx, const float* gy, const float* gxu, const float* gyv, const float* profiles, float zt,
-    int ni, int nj, int nk, int nrk, int nloop, float dts, float dttmp, float dx, float dy, float alph,
-    float kdiv, unsigned sides, int flags, void* stream) {
+    const float* flat, float rdz, int ni, int nj, int nk, int nrk, int nloop, float dts, float dttmp, float dx,
+    float dy, float alph, float kdiv, unsigned sides, int flags, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   SoundArgs a;
   a.ppi = ppi; a.pp3d = pp3d; a.ppx = ppx; a.ppd = ppd; a.piadv = piadv; a.ppterm = ppterm; a.ppten = ppten;
@@ -871,6 +1007,7 @@ extern "C" int gm1hip_psolver3_stage(
   a.ni = ni; a.nj = nj; a.nk = nk; a.nrk = nrk; a.nloop = nloop; a.sides = sides;
   a.dts = dts; a.rdx = 1.0f / dx; a.rdy = 1.0f / dy; a.alph = alph; a.beta = 1.0f - alph; a.kdiv = kdiv;
   a.zt = zt; a.rzt = 1.0f / zt; a.cp = 1005.7f;
+  a.flat = flat; a.rdzf = rdz;
   hipLaunchKernelGGL(k_snd_prepare, dim3((ni + 2 + 63) / 64, nj + 2, nk), dim3(64), 0, s, a);
   GM1_LAUNCH_CHECK();
   const bool any_open = side_of(sides, 0) == 2u || side_of(sides, 1) == 2u || side_of(sides, 2) == 2u ||
@@ -881,7 +1018,27 @@ extern "C" int gm1hip_psolver3_stage(
   const bool narrow = (flags & 8) != 0;
   const int cw = narrow ? 16 : 32;
   const bool staged = !(flags & 4) && column2_lds_bytes(nk, cw) + (int)(cw * sizeof(float)) <= 65536;
-  for (int n = 1; n <= nloop; ++n) {
+  if (flat) {
+    // the flat column runs the staged pass only (k_snd_column is the terrain text)
+    if (!staged || narrow) return (int)hipErrorInvalidValue;
+    for (int n = 1; n <= nloop; ++n) {
+      if (any_open)
+        hipLaunchKernelGGL(k_snd_ssopen_flat, dim3(gm1_blocks((int64_t)nk * (ni + nj), 256)), dim3(256), 0, s, a);
+      GM1_TIMED(TK_ACOUSTIC_PGRAD, s,
+                hipLaunchKernelGGL(k_snd_pgrad_flat,
+                                   dim3(((ni + 1 + PG_TX - 1) / PG_TX) * ((nj + 1 + PG_TY - 1) / PG_TY) * nk),
+                                   dim3(PG_TX * PG_TY), 0, s, a));
+      GM1_TIMED(TK_ACOUSTIC_COLUMN, s,
+                hipLaunchKernelGGL((k_snd_column2<32, 512, true>), dim3(((ni + 1 + 31) / 32) * (nj + 1)), dim3(512),
+                                   column2_lds_bytes(nk, 32), s, a, n));
+      GM1_LAUNCH_CHECK();
+      if (n < nloop) {
+        int rc = fill_pressure(ppd, sides, ni, nj, nk, s);
+        if (rc) return rc;
+      }
+    }
+  }
+  for (int n = 1; n <= nloop && !flat; ++n) {
     if (any_open) {
       if (nk <= SS_MAXK) hipLaunchKernelGGL(k_snd_ssopen2, dim3(edge), dim3(SS_T), 0, s, a);
       else hipLaunchKernelGGL(k_snd_ssopen, dim3((edge + 63) / 64), dim3(64), 0, s, a);
@@ -891,11 +1048,11 @@ extern "C" int gm1hip_psolver3_stage(
                                  dim3(PG_TX * PG_TY), 0, s, a));
     if (staged && narrow)
       GM1_TIMED(TK_ACOUSTIC_COLUMN, s,
-                hipLaunchKernelGGL((k_snd_column2<16, 256>), dim3(((ni + 1 + 15) / 16) * (nj + 1)), dim3(256),
+                hipLaunchKernelGGL((k_snd_column2<16, 256, false>), dim3(((ni + 1 + 15) / 16) * (nj + 1)), dim3(256),
                                    column2_lds_bytes(nk, 16), s, a, n));
     else if (staged)
       GM1_TIMED(TK_ACOUSTIC_COLUMN, s,
-                hipLaunchKernelGGL((k_snd_column2<32, 512>), dim3(((ni + 1 + 31) / 32) * (nj + 1)), dim3(512),
+                hipLaunchKernelGGL((k_snd_column2<32, 512, false>), dim3(((ni + 1 + 31) / 32) * (nj + 1)), dim3(512),
                                    column2_lds_bytes(nk, 32), s, a, n));
     else
       GM1_TIMED(TK_ACOUSTIC_COLUMN, s,
@@ -915,10 +1072,47 @@ extern "C" int gm1hip_psolver3_stage(
     if (!rc) rc = fill_family(v3d, 2, sides, ni, nj, nk, s);
     if (!rc) rc = fill_family(w3d, 3, sides, ni, nj, nk, s);
     if (!rc) rc = w_rigid(w3d, ni, nj, nk, s);
-    if (!rc) rc = bcwsfc(u3d, v3d, dzdx, dzdy, gz, w3d, ni, nj, s);
+    if (!rc && !flat) rc = bcwsfc(u3d, v3d, dzdx, dzdy, gz, w3d, ni, nj, s);
     if (!rc && nrk < 3) rc = fill_pressure(pp3d, sides, ni, nj, nk, s);
     if (!rc && nrk < 3 && scalar) rc = fill_family(scalar, 0, sides, ni, nj, nk, s);
     if (rc) return rc;
   }
   return 0;
+}
+
+extern "C" int gm1hip_psolver3_stage(
+    float* ppi, float* pp3d, float* ppx, float* ppd, const float* piadv, const float* ppterm, const float* ppten,
+    const float* thv, float* qk, float* pk1, float* pk2, float* mm, float* fpk, float* elim_c, float* elim_d,
+    const float* ua, float* u3d, const float* uten, float* rru, const float* va, float* v3d, const float* vten,
+    float* rrv, const float* wa, float* w3d, const float* wten, float* rrw, float* radbcw, float* radbce,
+    float* radbcs, float* radbcn, const float* uh, const float* vh, const float* ruf, const float* rvf,
+    float* scalar, const float* rho0_3d, const float* rr0, const float* rf0, const float* rrf0, const float* c1,
+    const float* c2, const float* gz, const float* rgzu, const float* rgzv, const float* dzdx, const float* dzdy,
+    const float* gx, const float* gy, const float* gxu, const float* gyv, const float* profiles, float zt,
+    int ni, int nj, int nk, int nrk, int nloop, float dts, float dttmp, float dx, float dy, float alph,
+    float kdiv, unsigned sides, int flags, void* stream) {
+  return psolver3_stage(ppi, pp3d, ppx, ppd, piadv, ppterm, ppten, thv, qk, pk1, pk2, mm, fpk, elim_c, elim_d, ua, u3d,
+                        uten, rru, va, v3d, vten, rrv, wa, w3d, wten, rrw, radbcw, radbce, radbcs, radbcn, uh, vh, ruf,
+                        rvf, scalar, rho0_3d, rr0, rf0, rrf0, c1, c2, gz, rgzu, rgzv, dzdx, dzdy, gx, gy, gxu, gyv,
+                        profiles, zt, nullptr, 0.0f, ni, nj, nk, nrk, nloop, dts, dttmp, dx, dy, alph, kdiv, sides,
+                        flags, stream);
+}
+
+// The flat stretched column's acoustic stage (launch_cuda_gm1_flat_acoustic_stage with the all-open
+// Durran-Klemp family, cuda_gm1_flat_lateral_acoustic.py:117; cuda_gm1_child_psolver3_open.py:1618-1792).
+extern "C" int gm1hip_psolver3_stage_flat(
+    float* ppi, float* pp3d, float* ppx, float* ppd, const float* piadv, const float* ppterm, const float* ppten,
+    const float* thv, float* qk, float* pk1, float* pk2, float* mm, float* fpk, float* elim_c, float* elim_d,
+    const float* ua, float* u3d, const float* uten, float* rru, const float* va, float* v3d, const float* vten,
+    float* rrv, const float* wa, float* w3d, const float* wten, float* rrw, float* radbcw, float* radbce,
+    float* radbcs, float* radbcn, const float* uh, const float* vh, const float* ruf, const float* rvf,
+    float* scalar, const float* rho0_3d, const float* rr0, const float* rf0, const float* c1, const float* c2,
+    const float* flat_column, float rdz, const float* profiles, int ni, int nj, int nk, int nrk, int nloop,
+    float dts, float dttmp, float dx, float dy, float alph, float kdiv, unsigned sides, int flags, void* stream) {
+  if (!flat_column || !profiles) return (int)hipErrorInvalidValue;
+  return psolver3_stage(ppi, pp3d, ppx, ppd, piadv, ppterm, ppten, thv, qk, pk1, pk2, mm, fpk, elim_c, elim_d, ua, u3d,
+                        uten, rru, va, v3d, vten, rrv, wa, w3d, wten, rrw, radbcw, radbce, radbcs, radbcn, uh, vh, ruf,
+                        rvf, scalar, rho0_3d, rr0, rf0, nullptr, c1, c2, nullptr, nullptr, nullptr, nullptr, nullptr,
+                        nullptr, nullptr, nullptr, nullptr, profiles, 1.0f, flat_column, rdz, ni, nj, nk, nrk, nloop,
+                        dts, dttmp, dx, dy, alph, kdiv, sides, flags, stream);
 }

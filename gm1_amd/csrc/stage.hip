@@ -213,6 +213,92 @@ __global__ __launch_bounds__(64) void k_getdivx_v(DivArgs a) {
   }
 }
 
+// ----------------------------------------------------------- flat stretched column
+// misclibs.F's flat getdiv/getdivx and solve2.F:897-906's flat buoyancy
+// (terrain_flag=.false.): ((d*rdz)*mh(k)) vertically, the base state at
+// CM1's (1,1,k), smeps 1e-30, and no horizontal buoyancy term.
+__global__ __launch_bounds__(256) void k_buoyancy_flat(const float* __restrict__ dum8, const float* __restrict__ c1,
+                                                       const float* __restrict__ c2, float* __restrict__ wten, int ni,
+                                                       int nj, int nk) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, kf = blockIdx.z + 2;
+  if (i >= ni) return;
+  const int64_t p = ((int64_t)(kf - 1) * nj + j) * ni + i;
+  const int nxs = sx(ni), nys = sy(nj);
+  const float add = c1[p] * dum8[ix3(kf - 1, j + NG, i + NG, nys, nxs)] + c2[p] * dum8[ix3(kf, j + NG, i + NG, nys, nxs)];
+  const int64_t d = ix3(kf, j + NG, i + NG, nys, nxs);
+  wten[d] = wten[d] + add;
+}
+
+struct FlatDivArgs {
+  float* rru; float* rrv; float* rrw;
+  const float* rho0; const float* rf0; const float* mh; const float* thterm;
+  float* div; float* theta_div; float* divx;
+  int ni, nj, nk; float rdx, rdy, rdz;
+};
+
+__global__ __launch_bounds__(64) void k_getdiv_flat(FlatDivArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  const int ni = a.ni, nj = a.nj, nk = a.nk;
+  if (i >= ni) return;
+  const int nxu = ux(ni), nyu = sy(nj), nxs = sx(ni), nyv = vy(nj);
+  const int x = i + NG, y = j + NG;
+  const int64_t h = (int64_t)j * ni + i;
+  float wlo = a.rrw[ix3(1, y, x, nyu, nxs)];
+  for (int k = 1; k <= nk; ++k) {
+    const float whi = a.rrw[ix3(k + 1, y, x, nyu, nxs)];
+    float d = (a.rru[ix3(k, y, x + 1, nyu, nxu)] - a.rru[ix3(k, y, x, nyu, nxu)]) * a.rdx +
+              (a.rrv[ix3(k, y + 1, x, nyv, nxs)] - a.rrv[ix3(k, y, x, nyv, nxs)]) * a.rdy;
+    d = d + ((whi - wlo) * a.rdz) * a.mh[k];
+    if (fabsf(d) < 1.0e-30f) d = 0.0f;
+    const int64_t z = (int64_t)(k - 1) * nj * ni + h;
+    a.div[z] = d;
+    a.theta_div[z] = d * a.thterm[z];
+    wlo = whi;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_getdivx_h_flat(FlatDivArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // 0..ni
+  const int j = blockIdx.y;                             // 0..nj
+  const int k = blockIdx.z + 1;
+  const int ni = a.ni, nj = a.nj;
+  if (i > ni) return;
+  const float rc = a.rho0[(int64_t)(k - 1) * nj * ni];
+  if (j < nj) {
+    const int64_t p = ix3(k, j + NG, i + NG, sy(nj), ux(ni));
+    a.rru[p] = a.rru[p] * rc;
+  }
+  if (i < ni) {
+    const int64_t p = ix3(k, j + NG, i + NG, vy(nj), sx(ni));
+    a.rrv[p] = a.rrv[p] * rc;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_getdivx_v_flat(FlatDivArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  const int ni = a.ni, nj = a.nj, nk = a.nk;
+  if (i >= ni) return;
+  const int nxu = ux(ni), nyu = sy(nj), nxs = sx(ni), nyv = vy(nj);
+  const int x = i + NG, y = j + NG;
+  const int64_t h = (int64_t)j * ni + i;
+  const int64_t wst = (int64_t)nyu * nxs, wc = (int64_t)y * nxs + x;
+  a.rrw[wst + wc] = 0.0f;
+  float below = 0.0f;
+  for (int kf = 2; kf <= nk + 1; ++kf) {
+    const float val = (kf <= nk) ? a.rrw[(int64_t)kf * wst + wc] * a.rf0[(int64_t)(kf - 1) * nj * ni] : 0.0f;
+    a.rrw[(int64_t)kf * wst + wc] = val;
+    const int k = kf - 1;
+    float d = (a.rru[ix3(k, y, x + 1, nyu, nxu)] - a.rru[ix3(k, y, x, nyu, nxu)]) * a.rdx +
+              (a.rrv[ix3(k, y + 1, x, nyv, nxs)] - a.rrv[ix3(k, y, x, nyv, nxs)]) * a.rdy;
+    d = d + ((val - below) * a.rdz) * a.mh[k];
+    a.divx[(int64_t)(k - 1) * nj * ni + h] = (fabsf(d) < 1.0e-30f) ? 0.0f : d;
+    below = val;
+  }
+}
+
 // ----------------------------------------------------------- scalar RK
 __global__ __launch_bounds__(256) void k_pack(const float* __restrict__ th0, const float* __restrict__ src,
                                               float* __restrict__ padded, int ni, int nj, int nk, int theta) {
@@ -350,6 +436,29 @@ int gm1hip_post_acoustic(float* rru, float* rrv, float* rrw, const float* rho0_3
   hipLaunchKernelGGL(k_getdiv, dim3((ni + 63) / 64, nj), dim3(64), 0, s, a);
   hipLaunchKernelGGL(k_getdivx_h, dim3((ni + 1 + 63) / 64, nj + 1, nk), dim3(64), 0, s, a);
   hipLaunchKernelGGL(k_getdivx_v, dim3((ni + 63) / 64, nj), dim3(64), 0, s, a);
+  GM1_LAUNCH_CHECK();
+  return 0;
+}
+
+int gm1hip_buoyancy_flat(const float* dum8, const float* c1, const float* c2, float* wten, int ni, int nj, int nk,
+                         void* stream) {
+  if (nk < 2) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_buoyancy_flat, dim3((ni + 63) / 64, nj, nk - 1), dim3(64), 0, (hipStream_t)stream, dum8, c1, c2,
+                     wten, ni, nj, nk);
+  GM1_LAUNCH_CHECK();
+  return 0;
+}
+
+int gm1hip_post_acoustic_flat(float* rru, float* rrv, float* rrw, const float* rho0_3d, const float* rf0,
+                              const float* flat_column, const float* thterm, float* divergence, float* theta_div,
+                              float* divx, int ni, int nj, int nk, float dx, float dy, float rdz, void* stream) {
+  if (!flat_column) return (int)hipErrorInvalidValue;
+  hipStream_t s = (hipStream_t)stream;
+  FlatDivArgs a{rru, rrv, rrw, rho0_3d, rf0, flat_column, thterm, divergence, theta_div, divx,
+                ni, nj, nk, 1.0f / dx, 1.0f / dy, rdz};
+  hipLaunchKernelGGL(k_getdiv_flat, dim3((ni + 63) / 64, nj), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(k_getdivx_h_flat, dim3((ni + 1 + 63) / 64, nj + 1, nk), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(k_getdivx_v_flat, dim3((ni + 63) / 64, nj), dim3(64), 0, s, a);
   GM1_LAUNCH_CHECK();
   return 0;
 }

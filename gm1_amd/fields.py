@@ -29,7 +29,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .grid import VerticalProfiles, terrain_coordinate_profiles
+from .grid import VerticalLevels, VerticalProfiles, build_vertical_levels, terrain_coordinate_profiles
 
 F = np.float32
 NG = 3
@@ -86,12 +86,21 @@ class CaseConfig:
     bubble_radius: tuple[float, float] = (10000.0, 1400.0)
     noise: float = 0.05
     seed: int = 1
-    stretched: tuple[float, float, float] | None = None  # (dz_low, z_low, dz_top)
+    # CM1 stretch_z=1 parameters (ztop, str_bot, str_top, dz_bot, dz_top); None is the uniform column of
+    # spacing dz.  ``dz`` stays the metric reference of mh = dz/(zf(k+1)-zf(k)).
+    stretched: tuple[float, float, float, float, float] | None = None
     extra: dict = field(default_factory=dict)
 
     @property
     def cells(self) -> int:
         return self.ni * self.nj * self.nk
+
+    def levels(self) -> VerticalLevels:
+        if self.stretched is None:
+            return build_vertical_levels(stretch_z=0, nk=self.nk, dz=self.dz)
+        ztop, str_bot, str_top, dz_bot, dz_top = self.stretched
+        return build_vertical_levels(stretch_z=1, nk=self.nk, dz=self.dz, ztop=ztop, str_bot=str_bot,
+                                     str_top=str_top, dz_bot=dz_bot, dz_top=dz_top)
 
 
 def dry_c1(ni=64, nj=64, nk=40) -> CaseConfig:
@@ -106,27 +115,26 @@ def moist_c2(ni=320, nj=320, nk=80) -> CaseConfig:
                       "hodograph", ((3.0, ni * 125.0, nj * 125.0, 1400.0),), noise=0.1, seed=2)
 
 
-def stretched_levels(nk: int, dz_low: float, z_low: float, dz_top: float, z_top: float) -> tuple[np.ndarray, np.ndarray]:
-    """w-level heights: dz_low up to z_low, then geometric growth to dz_top at z_top."""
-    n_low = int(round(z_low / dz_low))
-    n_up = nk - n_low
-    # solve for the ratio r with dz_low*r^n_up ~ dz_top and matching the depth
-    r = (dz_top / dz_low) ** (1.0 / max(n_up, 1))
-    dzs = [dz_low] * n_low + [dz_low * r ** (m + 1) for m in range(n_up)]
-    zf = np.concatenate([[0.0], np.cumsum(dzs)])
-    sigmaf = np.zeros(nk + 2, np.float64)
-    sigmaf[1 : nk + 2] = zf
-    sigma = np.zeros(nk + 2, np.float64)
-    sigma[1 : nk + 1] = 0.5 * (zf[:-1] + zf[1:])
-    return sigma, sigmaf
+def large_stretched_c3(ni=640, nj=768, nk=80) -> CaseConfig:
+    """640x768x80, dx 100 m, stretch_z=1 column 20 m -> 280 m, two bubbles 8 km apart, dt 0.6 s, 10 substeps.
+
+    CM1's stretch_z=1 coordinate with dz_bot 20 m up to 200 m, a quadratic
+    stretch to dz_top 280 m at 2450 m and 280 m layers to a 17.85 km top: the
+    feasible stretch_z=1 column of 80 levels nearest to "20 m at the ground,
+    280 m aloft, top near 18 km" (20 m up to 500 m and an 18 km top do not fit
+    80 levels with 280 m aloft).
+    """
+    x0, y0 = ni * 50.0, nj * 50.0
+    return CaseConfig("moist-kessler-stretched-640x768x80", ni, nj, nk, 100.0, 100.0, 100.0, 0.6, 10, (2, 2, 2, 2),
+                      True, "hodograph", ((3.0, x0 - 4000.0, y0, 1400.0), (3.0, x0 + 4000.0, y0, 1400.0)),
+                      bubble_radius=(4000.0, 1400.0), noise=0.1, seed=3,
+                      stretched=(17850.0, 200.0, 2450.0, 20.0, 280.0))
 
 
 def profiles_for(cfg: CaseConfig) -> VerticalProfiles:
     if cfg.stretched is None:
         return terrain_coordinate_profiles(cfg.nk, cfg.dz)
-    dz_low, z_low, dz_top = cfg.stretched
-    sigma, sigmaf = stretched_levels(cfg.nk, dz_low, z_low, dz_top, 18000.0)
-    return terrain_coordinate_profiles(cfg.nk, cfg.dz, sigma=sigma.astype(F), sigmaf=sigmaf.astype(F))
+    return cfg.levels().profiles()
 
 
 def _theta_env(z: np.ndarray) -> np.ndarray:
@@ -191,6 +199,10 @@ def metrics_for(cfg: CaseConfig, base: dict[str, np.ndarray]) -> dict[str, np.nd
                         ("gz", (nj, ni)), ("rgz", (nj, ni)), ("gzu", (nj, ni + 1)), ("rgzu", (nj, ni + 1)),
                         ("gzv", (nj + 1, ni)), ("rgzv", (nj + 1, ni))):
         m[name] = np.ones(shape, np.float32)
+    if cfg.stretched is not None:
+        levels = cfg.levels()
+        for name, lo, hi in (("mh", 1, nk + 1), ("rmh", 1, nk + 1), ("mf", 1, nk + 2), ("rmf", 1, nk + 2)):
+            m[name] = col(getattr(levels, name)[lo:hi])
     for name, shape in (("terrain_msl", (nj, ni)), ("dzdx", (nj, ni)), ("dzdy", (nj, ni)), ("gx", (nk + 1, nj, ni)),
                         ("gxu", (nk + 1, nj, ni + 1)), ("gy", (nk + 1, nj, ni)), ("gyv", (nk + 1, nj + 1, ni))):
         m[name] = np.zeros(shape, np.float32)

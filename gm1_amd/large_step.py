@@ -33,6 +33,10 @@ from . import _native
 from ._device import NG, lateral_code, ptr, stream_handle
 from ._native import Fl, I, P, U, signature
 from .fields import CaseConfig, base_state, initial_fields, initial_thermodynamics, metrics_for, profiles_for
+from .stretched import (flat_stretched_column, launch_cuda_gm1_child_stretched_momentum5,
+                        launch_cuda_gm1_child_stretched_pdef_moisture, launch_cuda_gm1_child_stretched_post_acoustic,
+                        launch_cuda_gm1_flat_acoustic_stage, launch_cuda_gm1_flat_stretched_buoyancy,
+                        launch_flat_scalar_rk_compose)
 from .grid import VerticalProfiles
 from .kessler import (
     Gm1KesslerDriverSelection,
@@ -125,14 +129,15 @@ def host_initial_state(cfg: CaseConfig) -> dict[str, Any]:
         a["pi0_3d"], a["th0_3d"] = thermo["pi0_3d"], thermo["th0_3d"]
         a["ruh"] = np.ones(ni, np.float32)
         a["rvh"] = np.ones(nj, np.float32)
-        a["rmh"] = np.ones(sh["cell"], np.float32)
-        a["mh"] = np.ones(sh["cell"], np.float32)
+        a["rmh"] = met["rmh"].copy()
+        a["mh"] = met["mh"].copy()
         a["rain"] = np.zeros((1, nj, ni), np.float32)
         a["prate"] = np.zeros((nj, ni), np.float32)
         a["asq"] = np.zeros(3, np.float64)
         a["qbudget"] = np.zeros(10, np.float64)
     base_d = {"th0": base["th0"], "pi0": base["pi0"], "qv0": base["qv0"]}
-    return {"cfg": cfg, "prof": prof, "base": base_d, "metrics": met, "arrays": a, "numq": numq}
+    return {"cfg": cfg, "prof": prof, "base": base_d, "metrics": met, "arrays": a, "numq": numq,
+            "levels": cfg.levels() if cfg.stretched is not None else None}
 
 
 @dataclass
@@ -151,6 +156,8 @@ class Gm1LargeStepState:
     mass1: float | None = None
     column_pass: str = "staged"
     receipts: list = field(default_factory=list)
+    # a stretched (flat) column runs the terrain_flag=.false. family
+    flat: Any = None
 
 
 def create_gm1_large_step_state(host: dict[str, Any], device: torch.device | str = "cuda",
@@ -167,6 +174,8 @@ def create_gm1_large_step_state(host: dict[str, Any], device: torch.device | str
         host["cfg"], host["prof"], {k: t(v) for k, v in host["base"].items()},
         {k: t(v) for k, v in host["metrics"].items()}, {k: t(v) for k, v in host["arrays"].items()},
         host["numq"], stream)
+    if host.get("levels") is not None:
+        st.flat = flat_stretched_column(host["levels"], dev)
     if st.numq:
         cfg = st.cfg
         st.selection = Gm1KesslerDriverSelection(equation_set=equation_set)
@@ -205,6 +214,9 @@ def _rk_stage(st: Gm1LargeStepState, nrk: int) -> None:
                  ptr(a["dum8"]), ptr(a["thterm"]), ni, nj, nk, code, h)
     _native.call("gm1hip_seed_held_momentum", ptr(a["uten1"]), ptr(a["vten1"]), ptr(a["wten1"]), ptr(a["uten"]),
                  ptr(a["vten"]), ptr(a["wten"]), ni, nj, nk, h)
+    if st.flat is not None:
+        _flat_rk_stage(st, nrk)
+        return
     if nrk >= 2:
         _native.call("gm1hip_bcwsfc", ptr(a["u3d"]), ptr(a["v3d"]), ptr(m["dzdx"]), ptr(m["dzdy"]), ptr(m["gz"]),
                      ptr(a["w3d"]), ni, nj, h)
@@ -239,6 +251,43 @@ def _rk_stage(st: Gm1LargeStepState, nrk: int) -> None:
         else:
             launch_pdef_moisture(a["moisture_padded"], mass, m, a["divx"], a["qa"][sp], a["qten"][sp], q3d[sp],
                                  {"advx": a["advx"], "advy": a["advy"], "advz": a["advz"]}, species=sp, **grid)
+
+
+def _flat_rk_stage(st: Gm1LargeStepState, nrk: int) -> None:
+    """The rest of one RK stage on a flat stretched column (cuda_cartesian_dynamics.py:1070-1262).
+
+    No bcwsfc/bc2d (terrain_flag statements), then the flat momentum,
+    buoyancy, acoustic stage, getdiv/getdivx, theta and moisture texts.
+    """
+    cfg, a, m, b = st.cfg, st.a, st.metrics, st.base
+    ni, nj, nk = cfg.ni, cfg.nj, cfg.nk
+    sides = cfg.lateral_sides
+    q3d = a.get("q3d")
+    launch_cuda_gm1_child_stretched_momentum5(a, m, st.flat, ni=ni, nj=nj, nk=nk, nrk=nrk, dx=cfg.dx, dy=cfg.dy,
+                                              divx=a["divx"], stream=st.stream, lateral_sides=sides)
+    launch_cuda_gm1_flat_stretched_buoyancy(a, m, ni=ni, nj=nj, nk=nk, stream=st.stream)
+    launch_cuda_gm1_flat_acoustic_stage(a, m, ni=ni, nj=nj, nk=nk, nrk=nrk, dt=cfg.dt, nsound=cfg.nsound, dx=cfg.dx,
+                                        dy=cfg.dy, vertical=st.flat, stream=st.stream, lateral_sides=sides)
+    launch_cuda_gm1_child_stretched_post_acoustic(a, m, st.flat, ni=ni, nj=nj, nk=nk, dx=cfg.dx, dy=cfg.dy,
+                                                  thterm=a["thterm"], divergence=a["divergence"],
+                                                  theta_divergence_term=a["theta_div"], divx=a["divx"],
+                                                  stream=st.stream)
+    mass = {"rru": a["rru"], "rrv": a["rrv"], "rrw": a["rrw"]}
+    grid = dict(ni=ni, nj=nj, nk=nk, dt=cfg.dt, dx=cfg.dx, dy=cfg.dy, lateral_sides=sides, stream=st.stream)
+    pack_scalar(b["th0"], a["th3d"], a["theta_padded"], ni=ni, nj=nj, nk=nk, theta=True, lateral_sides=sides,
+                stream=st.stream)
+    launch_flat_scalar_rk_compose(a["theta_padded"], mass, m, st.flat, a["divx"], a["tha"], a["thten1"],
+                                  a["theta_div"], a["th3d"], nrk=nrk, weps=THETA_WEPS, **grid)
+    for sp in range(st.numq):
+        pack_scalar(b["th0"], q3d[sp], a["moisture_padded"], ni=ni, nj=nj, nk=nk, theta=False,
+                    lateral_sides=sides, stream=st.stream)
+        if nrk < 3:
+            launch_flat_scalar_rk_compose(a["moisture_padded"], mass, m, st.flat, a["divx"], a["qa"][sp],
+                                          a["qten"][sp], None, q3d[sp], nrk=nrk, weps=moisture_weps_f64(sp), **grid)
+        else:
+            launch_cuda_gm1_child_stretched_pdef_moisture(
+                a["moisture_padded"], mass, m, st.flat, a["divx"], a["qa"][sp], a["qten"][sp], q3d[sp],
+                {"advx": a["advx"], "advy": a["advy"], "advz": a["advz"]}, species=sp, **grid)
 
 
 def launch_cuda_gm1_large_step(st: Gm1LargeStepState) -> dict[str, Any]:

@@ -228,6 +228,70 @@ int gm1hip_apmasscon(const float* ruh, const float* rvh, const float* rmh, const
                      double* ledger, float* corr, double mass1, void* scratch, int64_t scratch_bytes, int ni, int nj,
                      int nk, void* stream);
 
+// ----------------------------------------------------------------- flat stretched column
+// The stretched Cartesian lane (cuda_cartesian_dynamics.py:1070-1262): CM1's
+// terrain_flag=.false. statements on a stretch_z column.  `flat_column` is a
+// device buffer of mh (nk+2) then mf (nk+3), CM1-indexed
+// (cm1_vertical_grid.py:616-652); rdz = 1/dz of the metric reference
+// (param.F:6327).  The base state is read at the first physical column.
+
+// Replaces launch_cuda_gm1_child_stretched_momentum5 on a flat column
+// (cuda_gm1_child_stretched_momentum.py:430-757; texts :215-318): rru, rrv,
+// rrw, divx (smeps 1e-30) and the advu/advv/advw increments.
+int gm1hip_momentum_advection5_flat(const float* u3d, const float* v3d, const float* w3d, float* uten, float* vten,
+                                    float* wten, float* rru, float* rrv, float* rrw, const float* uh,
+                                    const float* vh, const float* rho0_3d, const float* rf0, const float* rr0,
+                                    const float* rrf0, const float* c2, const float* flat_column, float* divx,
+                                    int ni, int nj, int nk, float dx, float dy, float rdz, unsigned obc,
+                                    void* stream);
+
+// Replaces launch_cuda_gm1_flat_stretched_buoyancy
+// (cuda_gm1_child_stretched_moist.py:240-285): wten += c1*dum8(k-1) + c2*dum8(k).
+int gm1hip_buoyancy_flat(const float* dum8, const float* c1, const float* c2, float* wten, int ni, int nj, int nk,
+                         void* stream);
+
+// Replaces launch_cuda_gm1_child_stretched_post_acoustic on a flat column
+// (cuda_gm1_child_stretched_moist.py:286-372; misclibs.F's flat getdiv/getdivx).
+int gm1hip_post_acoustic_flat(float* rru, float* rrv, float* rrw, const float* rho0_3d, const float* rf0,
+                              const float* flat_column, const float* thterm, float* divergence, float* theta_div,
+                              float* divx, int ni, int nj, int nk, float dx, float dy, float rdz, void* stream);
+
+// The flat column's scalar advection fused with the A-anchored compose
+// (cuda_gm1_child_stretched_scalar.py:262-352, adv.F:458-478, 555-572,
+// 646-656), as gm1hip_scalar_rk_stage for the terrain family.
+int gm1hip_scalar_rk_stage_flat(const float* scalar, const float* rru, const float* rrv, const float* rrw,
+                                const float* c1, const float* c2, const float* rr0, const float* flat_column,
+                                const float* divx, const float* a0, const float* held, const float* divterm,
+                                float* out, int ni, int nj, int nk, int use_weno, double weps, float dx, float dy,
+                                float rdz, unsigned obc, float dttmp, void* stream);
+
+// The flat column's final-stage positive-definite moisture chain
+// (launch_cuda_gm1_child_stretched_pdef_moisture, cuda_gm1_child_stretched_scalar.py:851;
+// texts :355-448), as gm1hip_pdef_moisture_stage for the terrain family.
+int gm1hip_pdef_moisture_stage_flat(const float* scalar, const float* rru, const float* rrv, const float* rrw,
+                                    const float* c1, const float* c2, const float* rho0_3d, const float* rr0,
+                                    const float* flat_column, const float* divx, const float* qa, const float* held,
+                                    float* out, float* advx, float* advy, float* advz, int ni, int nj, int nk,
+                                    int pdefweno, double weps, float dx, float dy, float rdz, unsigned obc,
+                                    float dttmp, void* stream);
+
+// Replaces launch_cuda_gm1_flat_acoustic_stage for the all-open Durran-Klemp
+// family (cuda_gm1_flat_lateral_acoustic.py:117; cuda_gm1_child_psolver3_open.py
+// :340-535, 1618-1792): flat coefficients, flat ssopenbc, the pressure
+// gradient on every live face, the flat fpk, the Thomas solve, the update
+// with w(k), w(k+1) as they are, the time averages, and (flags & 2) the stage
+// fills without bcwsfc.  `profiles` is the packed terrain profile buffer
+// (unused by the flat statements; kept for the shared argument block).
+int gm1hip_psolver3_stage_flat(
+    float* ppi, float* pp3d, float* ppx, float* ppd, const float* piadv, const float* ppterm, const float* ppten,
+    const float* thv, float* qk, float* pk1, float* pk2, float* mm, float* fpk, float* elim_c, float* elim_d,
+    const float* ua, float* u3d, const float* uten, float* rru, const float* va, float* v3d, const float* vten,
+    float* rrv, const float* wa, float* w3d, const float* wten, float* rrw, float* radbcw, float* radbce,
+    float* radbcs, float* radbcn, const float* uh, const float* vh, const float* ruf, const float* rvf,
+    float* scalar, const float* rho0_3d, const float* rr0, const float* rf0, const float* c1, const float* c2,
+    const float* flat_column, float rdz, const float* profiles, int ni, int nj, int nk, int nrk, int nloop,
+    float dts, float dttmp, float dx, float dy, float alph, float kdiv, unsigned sides, int flags, void* stream);
+
 // ----------------------------------------------------------------- measurement
 // No counterpart in the reference: live per-kernel GPU time for throughput
 // reports.  arm() creates `capacity` event pairs (before a timed region) and

@@ -132,9 +132,14 @@ def _zface_flux(scalar, rrw, c1, c2, k, ni, nj, nk, use_weno, weps):
 
 def cpu_scalar_advection5_inplace(
     scalar, arrays, metrics, divx, tendency, fluxes, *, ni, nj, nk, nrk, dx, dy, dz,
-    weps, wflag=True, lateral_sides=None,
+    weps, wflag=True, lateral_sides=None, flat=None,
 ):
     """cuda_advection5.py:440-579; ``lateral_sides`` adds the whole-domain form.
+
+    ``flat`` (mh, rdz of a flat stretched column) selects the
+    terrain_flag=.false. tendency of cuda_gm1_child_stretched_scalar.py
+    :262-352 (adv.F:458-478, 555-572, 646-656): -(fe-fw)*rdx,
+    ((d*rdz)*mh(k)) vertically and the scale rr0(1,1,k).
 
     With ``lateral_sides=None`` only the tile-local interior
     ``3 <= i < ni-3, 3 <= j < nj-3`` is written (the reference CPU
@@ -187,8 +192,12 @@ def cpu_scalar_advection5_inplace(
         fe = np.where(ie, fw, fe)
         fs = np.where(js, fn, fs)
         fn = np.where(jn, fs, fn)
-    ax = (fw - fe) * rdx
-    ay = (fs - fn) * rdy
+    if flat is not None:
+        ax = (-(fe - fw)) * rdx
+        ay = (-(fn - fs)) * rdy
+    else:
+        ax = (fw - fe) * rdx
+        ay = (fs - fn) * rdy
     if obc:
         cx = (sv * (ue - uw)) * rdx
         cy = (sv * (vn - vs)) * rdy
@@ -197,12 +206,22 @@ def cpu_scalar_advection5_inplace(
         ay = np.where(js, ay - cy, ay)
         ay = np.where(jn, ay - cy, ay)
     az = np.empty(shape, np.float32)
-    az[0] = (-fz[2, ys, xs]) * rdsf
-    az[nk - 1] = fz[nk, ys, xs] * rdsf
-    az[1 : nk - 1] = (fz[2:nk, ys, xs] - fz[3 : nk + 1, ys, xs]) * rdsf
+    if flat is not None:
+        rdz, mh = F(flat.rdz), flat.mh[1 : nk + 1][:, None, None]
+        az[0] = (-fz[2, ys, xs]) * rdz
+        az[nk - 1] = fz[nk, ys, xs] * rdz
+        az[1 : nk - 1] = (-(fz[3 : nk + 1, ys, xs] - fz[2:nk, ys, xs])) * rdz
+        az = az * mh
+    else:
+        az[0] = (-fz[2, ys, xs]) * rdsf
+        az[nk - 1] = fz[nk, ys, xs] * rdsf
+        az[1 : nk - 1] = (fz[2:nk, ys, xs] - fz[3 : nk + 1, ys, xs]) * rdsf
     total = (ax + ay) + az
     total = total + sv * divx
-    value = (total * rr0) * gz[None]
+    if flat is not None:
+        value = total * rr0[:, 0, 0][:, None, None]
+    else:
+        value = (total * rr0) * gz[None]
     if obc:
         tendency[...] = value
     else:

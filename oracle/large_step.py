@@ -33,7 +33,7 @@ from .advection5 import cpu_scalar_advection5_inplace
 from .common import F, NG
 from .momentum5 import cpu_terrain_open_momentum_advection5_inplace
 from .pdef import pdef_advection
-from .sound import cpu_psolver3_stage_inplace
+from .sound import cpu_flat_stage_inplace, cpu_psolver3_stage_inplace
 
 THETA_WEPS = float(F(10.0) * F(1.0e-18))
 MOISTURE_WEPS = float(F(1.0e-2) * F(1.0e-18))
@@ -54,30 +54,43 @@ def _species6(q3d):
     return q
 
 
-def rk_stage(a, m, base, prof, cfg, nrk, *, zt):
+def rk_stage(a, m, base, prof, cfg, nrk, *, zt, flat=None):
+    """One RK stage; ``flat`` (mh, mf, rdz) runs the flat stretched column's operators.
+
+    The flat column is cuda_cartesian_dynamics.py's stretched lane
+    (:1070-1100, 1150-1192, 1213-1224, 1242-1262): no bcwsfc/bc2d, the flat
+    momentum, buoyancy, acoustic stage, getdiv/getdivx and scalar texts.
+    """
     ni, nj, nk = cfg["ni"], cfg["nj"], cfg["nk"]
     dx, dy, dz, dt = cfg["dx"], cfg["dy"], cfg["dz"], cfg["dt"]
     sides = tuple(cfg["lateral_sides"])
     q3d = a.get("q3d")
     st.stage_terms(a, base, a["th3d"], _species6(q3d), ni=ni, nj=nj, nk=nk, sides=sides)
     st.seed_held((a["uten1"], a["vten1"], a["wten1"]), (a["uten"], a["vten"], a["wten"]), ni=ni, nj=nj, nk=nk)
-    if nrk >= 2:
+    if nrk >= 2 and flat is None:
         bc.bcwsfc(a, m, ni=ni, nj=nj, nk=nk)
         bc.bc2d_open_w_surface(a["w3d"], ni=ni, nj=nj)
     cpu_terrain_open_momentum_advection5_inplace(a, m, a["divx"], ni=ni, nj=nj, nk=nk, nrk=nrk, dx=dx, dy=dy, dz=dz,
-                                                 lateral_sides=sides)
-    st.buoyancy(a, m, ni=ni, nj=nj, nk=nk)
-    cpu_psolver3_stage_inplace(a, m, ni=ni, nj=nj, nk=nk, dt=dt, nsound=cfg["nsound"], nrk=nrk, dx=dx, dy=dy, dz=dz,
-                               lateral_sides=sides, prof=prof, zt=zt)
-    st.getdiv(a, m, a["divergence"], a["theta_div"], a["thterm"], ni=ni, nj=nj, nk=nk, dx=dx, dy=dy, dz=dz)
-    st.getdivx(a, m, a["divx"], ni=ni, nj=nj, nk=nk, dx=dx, dy=dy, dz=dz)
+                                                 lateral_sides=sides, flat=flat)
+    if flat is None:
+        st.buoyancy(a, m, ni=ni, nj=nj, nk=nk)
+        cpu_psolver3_stage_inplace(a, m, ni=ni, nj=nj, nk=nk, dt=dt, nsound=cfg["nsound"], nrk=nrk, dx=dx, dy=dy,
+                                   dz=dz, lateral_sides=sides, prof=prof, zt=zt)
+        st.getdiv(a, m, a["divergence"], a["theta_div"], a["thterm"], ni=ni, nj=nj, nk=nk, dx=dx, dy=dy, dz=dz)
+        st.getdivx(a, m, a["divx"], ni=ni, nj=nj, nk=nk, dx=dx, dy=dy, dz=dz)
+    else:
+        st.buoyancy_flat(a, m, ni=ni, nj=nj, nk=nk)
+        cpu_flat_stage_inplace(a, m, flat, ni=ni, nj=nj, nk=nk, dt=dt, nsound=cfg["nsound"], nrk=nrk, dx=dx, dy=dy,
+                               lateral_sides=sides)
+        st.getdiv_flat(a, a["divergence"], a["theta_div"], a["thterm"], flat, ni=ni, nj=nj, nk=nk, dx=dx, dy=dy)
+        st.getdivx_flat(a, m, a["divx"], flat, ni=ni, nj=nj, nk=nk, dx=dx, dy=dy)
     dttmp = F(F(dt) / F(4 - nrk))
     padded = np.zeros((nk + 2, nj + 2 * NG, ni + 2 * NG), np.float32)
     adv = np.zeros((nk, nj, ni), np.float32)
     st.pack_theta(base["th0"], a["th3d"], padded, ni=ni, nj=nj, nk=nk)
     bc.fill(padded, bc.KIND_SCALAR, sides, ni=ni, nj=nj, nk=nk)
     cpu_scalar_advection5_inplace(padded, a, m, a["divx"], adv, _flux_scratch(ni, nj, nk), ni=ni, nj=nj, nk=nk,
-                                  nrk=nrk, dx=dx, dy=dy, dz=dz, weps=THETA_WEPS, lateral_sides=sides)
+                                  nrk=nrk, dx=dx, dy=dy, dz=dz, weps=THETA_WEPS, lateral_sides=sides, flat=flat)
     a["th3d"][...] = st.compose_theta(a["tha"], a["thten1"], a["theta_div"], adv, dttmp)
     if q3d is None:
         return
@@ -89,10 +102,11 @@ def rk_stage(a, m, base, prof, cfg, nrk, *, zt):
         if nrk < 3:
             adv = np.zeros((nk, nj, ni), np.float32)
             cpu_scalar_advection5_inplace(padded, a, m, a["divx"], adv, _flux_scratch(ni, nj, nk), ni=ni, nj=nj,
-                                          nk=nk, nrk=nrk, dx=dx, dy=dy, dz=dz, weps=weps, lateral_sides=sides)
+                                          nk=nk, nrk=nrk, dx=dx, dy=dy, dz=dz, weps=weps, lateral_sides=sides,
+                                          flat=flat)
         else:
             adv = pdef_advection(a["qa"][sp], padded, a, m, a["divx"], ni=ni, nj=nj, nk=nk, dt=dttmp, dx=dx, dy=dy,
-                                 dz=dz, weps=weps, pdefweno=sp != 0, sides=sides)
+                                 dz=dz, weps=weps, pdefweno=sp != 0, sides=sides, flat=flat)
         q3d[sp] = st.compose_moisture(a["qa"][sp], a["qten"][sp], adv, dttmp)
 
 
@@ -104,7 +118,8 @@ def kessler_arrays(a):
             "qbudget": a["qbudget"]}
 
 
-def large_step(a, m, base, prof, cfg, *, zt, mass_correction=None, mass1=None, equation_set=2, budgets=True):
+def large_step(a, m, base, prof, cfg, *, zt, mass_correction=None, mass1=None, equation_set=2, budgets=True,
+               flat=None):
     """Advance the state ``a`` by one large step in place.
 
     The dry-mass hook: ``mass_correction`` applies a given binary32 Exner
@@ -123,7 +138,7 @@ def large_step(a, m, base, prof, cfg, *, zt, mass_correction=None, mass1=None, e
         st.predict_and_compose_held(base, a["rho"], ppi, a["tha"], a["qa"], a["thten1"], a["qten"], a["ppten"],
                                     rates, cfg["dt"], ni=ni, nj=nj, nk=nk)
     for nrk in (1, 2, 3):
-        rk_stage(a, m, base, prof, cfg, nrk, zt=zt)
+        rk_stage(a, m, base, prof, cfg, nrk, zt=zt, flat=flat)
     if moist:
         a["pp_mass"][...] = a["pp3d"][1 : nk + 1, 1 : nj + 1, 1 : ni + 1]
         hook = None

@@ -68,9 +68,18 @@ class Lateral:
         return out
 
 
-def mass_flux_and_divergence(a, m, divx, *, ni, nj, nk, dx, dy, dz):
-    """cuda_momentum_advection5.py:787-863 (every face, column and cell)."""
+def mass_flux_and_divergence(a, m, divx, *, ni, nj, nk, dx, dy, dz, flat=None):
+    """cuda_momentum_advection5.py:787-863 (every face, column and cell).
+
+    ``flat`` (a flat stretched column: ``mh``, ``mf``, ``rdz``) selects the
+    terrain_flag=.false. statements of cuda_gm1_child_stretched_momentum.py
+    :215-262 (solve2.F:493-520, 605-617): rru = rho0(1,1,k)*u, rrv likewise,
+    rrw = rf0(1,1,k)*w, and divx's vertical term ((d*rdz)*mh(k)) with
+    smeps 1e-30.
+    """
     ng = NG
+    if flat is not None:
+        return _flat_mass_flux_and_divergence(a, m, divx, ni=ni, nj=nj, nk=nk, dx=dx, dy=dy, flat=flat)
     rho0 = m["rho0_3d"]
     ks = slice(1, nk + 1)
     f = np.arange(ni + 1)
@@ -113,6 +122,28 @@ def mass_flux_and_divergence(a, m, divx, *, ni, nj, nk, dx, dy, dz):
         rrv[ks, ng + 1 : ng + nj + 1, xs] - rrv[ks, ys, xs]) * rdy
     v = v + (rrw[2 : nk + 2, ys, xs] - rrw[1 : nk + 1, ys, xs]) * rdz
     divx[...] = np.where(np.abs(v) < F(1.0e-20), ZERO, v)
+    return rdx, rdy, rdz
+
+
+def _flat_mass_flux_and_divergence(a, m, divx, *, ni, nj, nk, dx, dy, flat):
+    ng = NG
+    ks = slice(1, nk + 1)
+    rho_c = m["rho0_3d"][:, 0, 0]
+    a["rru"][ks, ng : ng + nj, ng : ng + ni + 1] = rho_c[:, None, None] * a["u3d"][ks, ng : ng + nj, ng : ng + ni + 1]
+    a["rrv"][ks, ng : ng + nj + 1, ng : ng + ni] = rho_c[:, None, None] * a["v3d"][ks, ng : ng + nj + 1, ng : ng + ni]
+    ys, xs = slice(ng, ng + nj), slice(ng, ng + ni)
+    a["rrw"][1, ys, xs] = ZERO
+    a["rrw"][nk + 1, ys, xs] = ZERO
+    for k in range(2, nk + 1):
+        a["rrw"][k, ys, xs] = m["rf0"][k - 1, 0, 0] * a["w3d"][k, ys, xs]
+    rdx = F(1.0) / F(dx)
+    rdy = F(1.0) / F(dy)
+    rdz = F(flat.rdz)
+    rru, rrv, rrw = a["rru"], a["rrv"], a["rrw"]
+    v = (rru[ks, ys, ng + 1 : ng + ni + 1] - rru[ks, ys, xs]) * rdx + (
+        rrv[ks, ng + 1 : ng + nj + 1, xs] - rrv[ks, ys, xs]) * rdy
+    v = v + ((rrw[2 : nk + 2, ys, xs] - rrw[1 : nk + 1, ys, xs]) * rdz) * flat.mh[1 : nk + 1][:, None, None]
+    divx[...] = np.where(np.abs(v) < F(1.0e-30), ZERO, v)
     return rdx, rdy, rdz
 
 
@@ -162,8 +193,16 @@ def _wz(w, rrw, k, nk, ys, xs):
     return mass * r
 
 
-def momentum_transport5(a, m, divx, *, ni, nj, nk, rdx, rdy, rdz, lateral_sides=(2, 2, 2, 2)):
-    """advu/advv/advw increments (cuda_momentum_advection5.py:871-1011)."""
+def momentum_transport5(a, m, divx, *, ni, nj, nk, rdx, rdy, rdz, lateral_sides=(2, 2, 2, 2), flat=None):
+    """advu/advv/advw increments (cuda_momentum_advection5.py:871-1011).
+
+    ``flat`` selects the flat stretched column's statements
+    (cuda_gm1_child_stretched_momentum.py:264-318, adv.F:1380-1485,
+    2321-2416, 3299-3356): vertical differences ((d*rdz)*mh(k)) for u and v
+    and ((d*rdz)*mf(k)) for w, tendencies scaled by rr0(1,1,k) / rrf0(1,1,k),
+    the open-side corrections grouped left to right (((q*0.5)*c)*rd)*h, and
+    w's horizontal differences ((d*rdx)*uh).
+    """
     ng = NG
     lat = Lateral(ni, nj, lateral_sides)
     u, v, w = a["u3d"], a["v3d"], a["w3d"]
@@ -226,13 +265,15 @@ def momentum_transport5(a, m, divx, *, ni, nj, nk, rdx, rdy, rdz, lateral_sides=
         if lat.open(2):
             mass = rrv[k, ng, east + ng] + rrv[k, ng, west + ng]
             corr = (rrv[k, ng + 1, west + ng] - rrv[k, ng, west + ng]) + (rrv[k, ng + 1, east + ng] - rrv[k, ng, east + ng])
-            upd = advy[0] - (uc[0] * HALF) * (corr * (rdy * vh[ng]))
+            upd = (advy[0] - (((uc[0] * HALF) * corr) * rdy) * vh[ng] if flat is not None
+                   else advy[0] - (uc[0] * HALF) * (corr * (rdy * vh[ng])))
             advy[0] = np.where(mass >= ZERO, upd, advy[0])
         if lat.open(3) and not (nj - 1 == 0 and lat.open(2)):
             y = nj - 1 + ng
             mass = rrv[k, ng + nj, east + ng] + rrv[k, ng + nj, west + ng]
             corr = (rrv[k, y + 1, west + ng] - rrv[k, y, west + ng]) + (rrv[k, y + 1, east + ng] - rrv[k, y, east + ng])
-            upd = advy[nj - 1] - (uc[nj - 1] * HALF) * (corr * (rdy * vh[y]))
+            upd = (advy[nj - 1] - (((uc[nj - 1] * HALF) * corr) * rdy) * vh[y] if flat is not None
+                   else advy[nj - 1] - (uc[nj - 1] * HALF) * (corr * (rdy * vh[y])))
             advy[nj - 1] = np.where(mass <= ZERO, upd, advy[nj - 1])
         if k == 1:
             advz = (-xz[2]) * rdz
@@ -240,9 +281,14 @@ def momentum_transport5(a, m, divx, *, ni, nj, nk, rdx, rdy, rdz, lateral_sides=
             advz = xz[nk] * rdz
         else:
             advz = (-(xz[k + 1] - xz[k])) * rdz
+        if flat is not None:
+            advz = advz * flat.mh[k]
         total = (advx + advy) + advz
         total = total + uc * (HALF * (divx[k0][:, east] + divx[k0][:, west]))
-        scale = gzu[:, faces] / (HALF * (rho0[k0][:, west] + rho0[k0][:, east]))
+        if flat is not None:
+            scale = m["rr0"][k0, 0, 0]
+        else:
+            scale = gzu[:, faces] / (HALF * (rho0[k0][:, west] + rho0[k0][:, east]))
         a["uten"][k][ng : ng + nj, faces + ng] = a["uten"][k][ng : ng + nj, faces + ng] + total * scale
 
     # ---------------- advv: rows first_row..last_row, columns 0..ni-1
@@ -299,13 +345,15 @@ def momentum_transport5(a, m, divx, *, ni, nj, nk, rdx, rdy, rdz, lateral_sides=
         if lat.open(0):
             mass = rru[k][north + ng, ng] + rru[k][south + ng, ng]
             corr = (rru[k][south + ng, ng + 1] - rru[k][south + ng, ng]) + (rru[k][north + ng, ng + 1] - rru[k][north + ng, ng])
-            upd = advx[:, 0] - (vc[:, 0] * HALF) * (corr * (rdx * uh[ng]))
+            upd = (advx[:, 0] - (((vc[:, 0] * HALF) * corr) * rdx) * uh[ng] if flat is not None
+                   else advx[:, 0] - (vc[:, 0] * HALF) * (corr * (rdx * uh[ng])))
             advx[:, 0] = np.where(mass >= ZERO, upd, advx[:, 0])
         if lat.open(1) and not (ni - 1 == 0 and lat.open(0)):
             x = ni - 1 + ng
             mass = rru[k][north + ng, ng + ni] + rru[k][south + ng, ng + ni]
             corr = (rru[k][south + ng, x + 1] - rru[k][south + ng, x]) + (rru[k][north + ng, x + 1] - rru[k][north + ng, x])
-            upd = advx[:, ni - 1] - (vc[:, ni - 1] * HALF) * (corr * (rdx * uh[x]))
+            upd = (advx[:, ni - 1] - (((vc[:, ni - 1] * HALF) * corr) * rdx) * uh[x] if flat is not None
+                   else advx[:, ni - 1] - (vc[:, ni - 1] * HALF) * (corr * (rdx * uh[x])))
             advx[:, ni - 1] = np.where(mass <= ZERO, upd, advx[:, ni - 1])
         if k == 1:
             advz = (-yz[2]) * rdz
@@ -313,9 +361,14 @@ def momentum_transport5(a, m, divx, *, ni, nj, nk, rdx, rdy, rdz, lateral_sides=
             advz = yz[nk] * rdz
         else:
             advz = (-(yz[k + 1] - yz[k])) * rdz
+        if flat is not None:
+            advz = advz * flat.mh[k]
         total = (advx + advy) + advz
         total = total + vc * (HALF * (divx[k0][north] + divx[k0][south]))
-        scale = gzv[rows] / (HALF * (rho0[k0][south] + rho0[k0][north]))
+        if flat is not None:
+            scale = m["rr0"][k0, 0, 0]
+        else:
+            scale = gzv[rows] / (HALF * (rho0[k0][south] + rho0[k0][north]))
         a["vten"][k][rows + ng, ng : ng + ni] = a["vten"][k][rows + ng, ng : ng + ni] + total * scale
 
     # ---------------- advw: w levels 2..nk, every cell
@@ -371,47 +424,58 @@ def momentum_transport5(a, m, divx, *, ni, nj, nk, rdx, rdy, rdz, lateral_sides=
     wz = {k: _wz(w, rrw, k, nk, ys, xs) for k in range(1, nk + 1)}
     for kf in range(2, nk + 1):
         fx = w_fx(kf)
-        advx = (-(fx[:, 1:] - fx[:, :-1])) * (rdx * uh[ng : ng + ni][None, :])
         fy = w_fy(kf)
-        advy = (-(fy[1:] - fy[:-1])) * (rdy * vh[ng : ng + nj][:, None])
+        if flat is not None:
+            advx = ((-(fx[:, 1:] - fx[:, :-1])) * rdx) * uh[ng : ng + ni][None, :]
+            advy = ((-(fy[1:] - fy[:-1])) * rdy) * vh[ng : ng + nj][:, None]
+        else:
+            advx = (-(fx[:, 1:] - fx[:, :-1])) * (rdx * uh[ng : ng + ni][None, :])
+            advy = (-(fy[1:] - fy[:-1])) * (rdy * vh[ng : ng + nj][:, None])
         cc2 = c2[kf - 1]
         cc1 = F(1.0) - cc2
         wc = w[kf, ys, xs]
         if lat.open(0):
             mass = cc2[:, 0] * rru[kf, ys, ng] + cc1[:, 0] * rru[kf - 1, ys, ng]
             corr = cc1[:, 0] * (rru[kf - 1, ys, ng + 1] - rru[kf - 1, ys, ng]) + cc2[:, 0] * (rru[kf, ys, ng + 1] - rru[kf, ys, ng])
-            upd = advx[:, 0] - wc[:, 0] * (corr * (rdx * uh[ng]))
+            upd = (advx[:, 0] - ((wc[:, 0] * corr) * rdx) * uh[ng] if flat is not None
+                   else advx[:, 0] - wc[:, 0] * (corr * (rdx * uh[ng])))
             advx[:, 0] = np.where(mass >= ZERO, upd, advx[:, 0])
         if lat.open(1) and not (ni - 1 == 0 and lat.open(0)):
             x = ni - 1 + ng
             mass = cc2[:, ni - 1] * rru[kf, ys, ng + ni] + cc1[:, ni - 1] * rru[kf - 1, ys, ng + ni]
             corr = cc1[:, ni - 1] * (rru[kf - 1, ys, x + 1] - rru[kf - 1, ys, x]) + cc2[:, ni - 1] * (rru[kf, ys, x + 1] - rru[kf, ys, x])
-            upd = advx[:, ni - 1] - wc[:, ni - 1] * (corr * (rdx * uh[x]))
+            upd = (advx[:, ni - 1] - ((wc[:, ni - 1] * corr) * rdx) * uh[x] if flat is not None
+                   else advx[:, ni - 1] - wc[:, ni - 1] * (corr * (rdx * uh[x])))
             advx[:, ni - 1] = np.where(mass <= ZERO, upd, advx[:, ni - 1])
         if lat.open(2):
             mass = cc2[0] * rrv[kf, ng, xs] + cc1[0] * rrv[kf - 1, ng, xs]
             corr = cc1[0] * (rrv[kf - 1, ng + 1, xs] - rrv[kf - 1, ng, xs]) + cc2[0] * (rrv[kf, ng + 1, xs] - rrv[kf, ng, xs])
-            upd = advy[0] - wc[0] * (corr * (rdy * vh[ng]))
+            upd = (advy[0] - ((wc[0] * corr) * rdy) * vh[ng] if flat is not None
+                   else advy[0] - wc[0] * (corr * (rdy * vh[ng])))
             advy[0] = np.where(mass >= ZERO, upd, advy[0])
         if lat.open(3) and not (nj - 1 == 0 and lat.open(2)):
             y = nj - 1 + ng
             mass = cc2[nj - 1] * rrv[kf, ng + nj, xs] + cc1[nj - 1] * rrv[kf - 1, ng + nj, xs]
             corr = cc1[nj - 1] * (rrv[kf - 1, y + 1, xs] - rrv[kf - 1, y, xs]) + cc2[nj - 1] * (rrv[kf, y + 1, xs] - rrv[kf, y, xs])
-            upd = advy[nj - 1] - wc[nj - 1] * (corr * (rdy * vh[y]))
+            upd = (advy[nj - 1] - ((wc[nj - 1] * corr) * rdy) * vh[y] if flat is not None
+                   else advy[nj - 1] - wc[nj - 1] * (corr * (rdy * vh[y])))
             advy[nj - 1] = np.where(mass <= ZERO, upd, advy[nj - 1])
         advz = (-(wz[kf] - wz[kf - 1])) * rdz
+        if flat is not None:
+            advz = advz * flat.mf[kf]
         total = (advx + advy) + advz
         total = total + wc * (cc2 * divx[kf - 1] + cc1 * divx[kf - 2])
-        scale = rrf0[kf - 1] * gz
+        scale = rrf0[kf - 1, 0, 0] if flat is not None else rrf0[kf - 1] * gz
         a["wten"][kf, ys, xs] = a["wten"][kf, ys, xs] + total * scale
 
 
 def cpu_terrain_open_momentum_advection5_inplace(
-    arrays, metrics, divx, *, ni, nj, nk, nrk, dx, dy, dz, lateral_sides=(2, 2, 2, 2),
+    arrays, metrics, divx, *, ni, nj, nk, nrk, dx, dy, dz, lateral_sides=(2, 2, 2, 2), flat=None,
 ):
-    """cuda_momentum_advection5.py:753-868 then :871-1011."""
+    """cuda_momentum_advection5.py:753-868 then :871-1011 (``flat``: the flat stretched column's texts)."""
     if nrk not in (1, 2, 3):
         raise ValueError("nrk must be an integer in 1..3")
-    rdx, rdy, rdz = mass_flux_and_divergence(arrays, metrics, divx, ni=ni, nj=nj, nk=nk, dx=dx, dy=dy, dz=dz)
+    rdx, rdy, rdz = mass_flux_and_divergence(arrays, metrics, divx, ni=ni, nj=nj, nk=nk, dx=dx, dy=dy, dz=dz,
+                                             flat=flat)
     momentum_transport5(arrays, metrics, divx, ni=ni, nj=nj, nk=nk, rdx=rdx, rdy=rdy, rdz=rdz,
-                        lateral_sides=lateral_sides)
+                        lateral_sides=lateral_sides, flat=flat)

@@ -91,8 +91,11 @@ def pdef_fluxes(s, rru, rrv, rrw, c1, c2, *, ni, nj, nk, weps, pdefweno):
     return fx, fy, fz
 
 
-def directional(fx, fy, fz, s, rru, rrv, *, ni, nj, nk, dx, dy, dz, sides):
-    """ax, ay, az (cells) with the open-side inflow edits (cm1_rk_pdef_directional_v1)."""
+def directional(fx, fy, fz, s, rru, rrv, *, ni, nj, nk, dx, dy, dz, sides, flat=None):
+    """ax, ay, az (cells) with the open-side inflow edits (cm1_rk_pdef_directional_v1).
+
+    ``flat``: -(fe-fw)*rdx and ((d*rdz)*mh(k)) (cuda_gm1_child_stretched_scalar.py:355-378).
+    """
     ng = NG
     rdx, rdy, rdz = F(1.0) / F(dx), F(1.0) / F(dy), F(1.0) / F(dz)
     K, J, I = slice(1, nk + 1), slice(ng, ng + nj), slice(ng, ng + ni)
@@ -111,8 +114,12 @@ def directional(fx, fy, fz, s, rru, rrv, *, ni, nj, nk, dx, dy, dz, sides):
     fe = np.where(ie, fw, fe)
     fs = np.where(js, fn, fs)
     fn = np.where(jn, fs, fn)
-    ax = (fw - fe) * rdx
-    ay = (fs - fn) * rdy
+    if flat is not None:
+        ax = (-(fe - fw)) * rdx
+        ay = (-(fn - fs)) * rdy
+    else:
+        ax = (fw - fe) * rdx
+        ay = (fs - fn) * rdy
     sv = s[K, J, I]
     tx = (sv * (ue - uw)) * rdx
     ty = (sv * (vn - vs)) * rdy
@@ -121,6 +128,12 @@ def directional(fx, fy, fz, s, rru, rrv, *, ni, nj, nk, dx, dy, dz, sides):
     ay = np.where(js, ay - ty, ay)
     ay = np.where(jn, ay - ty, ay)
     az = np.empty((nk, nj, ni), np.float32)
+    if flat is not None:
+        rdz = F(flat.rdz)
+        az[0] = (-fz[2, J, I]) * rdz
+        az[nk - 1] = fz[nk, J, I] * rdz
+        az[1 : nk - 1] = (-(fz[3 : nk + 1, J, I] - fz[2:nk, J, I])) * rdz
+        return ax, ay, az * flat.mh[1 : nk + 1][:, None, None]
     az[0] = (-fz[2, J, I]) * rdz
     az[nk - 1] = fz[nk, J, I] * rdz
     if nk > 2:
@@ -182,8 +195,13 @@ def _row_mass(values, ext):
     return mass, flag
 
 
-def repair_direction(qa, s, rho0, gz, rgz, flux, adv, *, axis, ni, nj, nk, dt, spacing, sides):
-    """Candidate + repair of one direction; returns the corrected directional tendency."""
+def repair_direction(qa, s, rho0, gz, rgz, flux, adv, *, axis, ni, nj, nk, dt, spacing, sides, flat=None):
+    """Candidate + repair of one direction; returns the corrected directional tendency.
+
+    ``flat``: the flat column's candidate rho0(1,1,k)*s0 + dt*(adv + div) and
+    correction ((dum+mass) - rho0(1,1,k)*s0)*rdt with no gz/rgz, and the
+    vertical ((s*d)*rdz)*mh(k) (cuda_gm1_child_stretched_scalar.py:380-448).
+    """
     ng = NG
     dt = F(dt)
     rd = F(1.0) / F(spacing)
@@ -197,9 +215,17 @@ def repair_direction(qa, s, rho0, gz, rgz, flux, adv, *, axis, ni, nj, nk, dt, s
     else:
         lo_f, hi_f = flux[1 : nk + 1, J, I], flux[2 : nk + 2, J, I]
     d = hi_f - lo_f
-    div = (sv * d) * rd
-    r0a = rho0 * qa
-    cand = r0a + (dt * (adv + div)) * gz
+    if flat is not None:
+        if axis == "z":
+            div = ((sv * d) * F(flat.rdz)) * flat.mh[1 : nk + 1][:, None, None]
+        else:
+            div = (sv * d) * rd
+        r0a = rho0[:, 0, 0][:, None, None] * qa
+        cand = r0a + dt * (adv + div)
+    else:
+        div = (sv * d) * rd
+        r0a = rho0 * qa
+        cand = r0a + (dt * (adv + div)) * gz
     if axis != "z":
         w, e, so, no = sides
         if axis == "x":
@@ -217,25 +243,29 @@ def repair_direction(qa, s, rho0, gz, rgz, flux, adv, *, axis, ni, nj, nk, dt, s
     else:
         mt, ft = _row_mass(np.moveaxis(cand, 0, -1), None)
         mass, flag = np.moveaxis(mt, -1, 0), np.moveaxis(ft, -1, 0)
-    first = (((cand + mass) - r0a) * rdt) * rgz
+    first = ((cand + mass) - r0a) * rdt
+    if flat is None:
+        first = first * rgz
     return np.where(flag, first - div, adv), cand
 
 
-def total(s, ax, ay, az, divx, rr0, gz, *, ni, nj, nk):
+def total(s, ax, ay, az, divx, rr0, gz, *, ni, nj, nk, flat=None):
     ng = NG
     sv = s[1 : nk + 1, ng : ng + nj, ng : ng + ni]
     t = ((ax + ay) + az) + sv * divx
+    if flat is not None:
+        return t * rr0[:, 0, 0][:, None, None]
     return (t * rr0) * gz
 
 
-def pdef_advection(qa, s, a, m, divx, *, ni, nj, nk, dt, dx, dy, dz, weps, pdefweno, sides):
-    """The moisture row's final-stage advection (cells) from the whole chain."""
+def pdef_advection(qa, s, a, m, divx, *, ni, nj, nk, dt, dx, dy, dz, weps, pdefweno, sides, flat=None):
+    """The moisture row's final-stage advection (cells) from the whole chain (``flat``: flat stretched column)."""
     fx, fy, fz = pdef_fluxes(s, a["rru"], a["rrv"], a["rrw"], m["c1"], m["c2"], ni=ni, nj=nj, nk=nk, weps=weps,
                              pdefweno=pdefweno)
     ax, ay, az = directional(fx, fy, fz, s, a["rru"], a["rrv"], ni=ni, nj=nj, nk=nk, dx=dx, dy=dy, dz=dz,
-                             sides=sides)
-    common = dict(ni=ni, nj=nj, nk=nk, dt=dt, sides=sides)
+                             sides=sides, flat=flat)
+    common = dict(ni=ni, nj=nj, nk=nk, dt=dt, sides=sides, flat=flat)
     ax, _ = repair_direction(qa, s, m["rho0_3d"], m["gz"], m["rgz"], a["rru"], ax, axis="x", spacing=dx, **common)
     ay, _ = repair_direction(qa, s, m["rho0_3d"], m["gz"], m["rgz"], a["rrv"], ay, axis="y", spacing=dy, **common)
     az, _ = repair_direction(qa, s, m["rho0_3d"], m["gz"], m["rgz"], a["rrw"], az, axis="z", spacing=dz, **common)
-    return total(s, ax, ay, az, divx, m["rr0"], m["gz"], ni=ni, nj=nj, nk=nk)
+    return total(s, ax, ay, az, divx, m["rr0"], m["gz"], ni=ni, nj=nj, nk=nk, flat=flat)

@@ -355,3 +355,193 @@ def cpu_psolver3_stage_inplace(
             if "scalar" in a:
                 bc.fill(a["scalar"], bc.KIND_SCALAR, sides, ni=ni, nj=nj, nk=nk)
     return nloop, float(dts)
+
+
+# ----------------------------------------------------------------------------- flat stretched column
+# sound.F with terrain_flag=.false. on a stretched column
+# (cuda_gm1_child_psolver3_open.py:340-535 and _enqueue_flat_stage :1618-1792):
+# the terrain stage's order less u/v reconstruction at w points and bcwsfc,
+# with flat coefficients on mh/mf/rdz, the flat pressure gradient on every
+# live face, the flat fpk (sound.F:491-509), the update with w(k), w(k+1) as
+# they are, and the flat ssopenbc (bc.F:2410-2419 and its twins).
+FLAT_SMEPS = F(1.0e-30)
+
+
+def flat_prepare_coefficients(a, m, flat, *, ni, nj, nk, dts):
+    """qk, pk1, pk2, mm from mh/mf/rdz and the base state at CM1's (1,1,k)."""
+    cp = F(CP)
+    for name in ("qk", "pk1", "pk2", "mm", "fpk"):
+        a[name].fill(ZERO)
+    J, I = slice(1, nj + 1), slice(1, ni + 1)
+    rdz = F(flat.rdz)
+    pia, thv = a["piadv"], a["thv"]
+    for k in range(1, nk + 1):
+        dr = dts * rdz
+        tem = dr * flat.mh[k]
+        r2 = ((dr * flat.mh[k]) * m["rr0"][k - 1, 0, 0]) * m["rf0"][k, 0, 0]
+        r1 = ((dr * flat.mh[k]) * m["rr0"][k - 1, 0, 0]) * m["rf0"][k - 1, 0, 0]
+        tem1 = (dr * cp) * flat.mf[k]
+        c1a, c1b, c2a, c2b = m["c1"][k - 1, 0, 0], m["c1"][k, 0, 0], m["c2"][k - 1, 0, 0], m["c2"][k, 0, 0]
+        a["qk"][k, J, I] = tem * a["ppterm"][k, J, I]
+        a["pk2"][k, J, I] = r2 * ((-c2b) * pia[k + 1, J, I] + (F(1.0) - c1b) * pia[k, J, I])
+        a["pk1"][k, J, I] = r1 * (c1a * pia[k - 1, J, I] + (c2a - F(1.0)) * pia[k, J, I])
+        a["mm"][k, J, I] = tem1 * (c2a * thv[k, J, I] + c1a * thv[k - 1, J, I])
+        if k == 1:
+            a["pk1"][k, J, I] = ZERO
+        if k == nk:
+            a["pk2"][k, J, I] = ZERO
+
+
+def flat_ssopen(a, sides, *, ni, nj, nk, dts, rdx, rdy):
+    """Open-side normal velocity: u(1) += dts*((((-radbcw*(u(2)-u(1)))*rdx)*uh(1)) + uten(1))."""
+    ng = NG
+    u, v = a["u3d"], a["v3d"]
+    K, Y, X = slice(1, nk + 1), slice(ng, ng + nj), slice(ng, ng + ni)
+    if sides[0] == 2:
+        t = ((((-a["radbcw"][K, Y]) * (u[K, Y, ng + 1] - u[K, Y, ng])) * rdx) * a["uh"][ng]) + a["uten"][K, Y, ng]
+        u[K, Y, ng] = u[K, Y, ng] + dts * t
+    if sides[1] == 2:
+        xe = ni + ng
+        t = ((((-a["radbce"][K, Y]) * (u[K, Y, xe] - u[K, Y, xe - 1])) * rdx) * a["uh"][ni + ng - 1]) + a["uten"][K, Y, xe]
+        u[K, Y, xe] = u[K, Y, xe] + dts * t
+    if sides[2] == 2:
+        t = ((((-a["radbcs"][K, X]) * (v[K, ng + 1, X] - v[K, ng, X])) * rdy) * a["vh"][ng]) + a["vten"][K, ng, X]
+        v[K, ng, X] = v[K, ng, X] + dts * t
+    if sides[3] == 2:
+        yn = nj + ng
+        t = ((((-a["radbcn"][K, X]) * (v[K, yn, X] - v[K, yn - 1, X])) * rdy) * a["vh"][nj + ng - 1]) + a["vten"][K, yn, X]
+        v[K, yn, X] = v[K, yn, X] + dts * t
+
+
+def flat_pressure_gradient(a, sides, *, ni, nj, nk, dts, rdx, rdy):
+    """Every face i = 1..ni+1, j = 1..nj+1 with CM1's face weights (zero on open boundary faces)."""
+    ng = NG
+    cp = F(CP)
+    tem1 = (rdx * cp) * HALF
+    tem2 = (rdy * cp) * HALF
+    ppd, thv = a["ppd"], a["thv"]
+    pw, pe, ps, pn = (s == 1 for s in sides)
+    ii = np.arange(1, ni + 2)[None, :]
+    jj = np.arange(1, nj + 2)[:, None]
+    wu = ((jj <= nj) & ((ii > 1) | pw) & ((ii < ni + 1) | pe)).astype(np.float32)
+    wv = ((ii <= ni) & ((jj > 1) | ps) & ((jj < nj + 1) | pn)).astype(np.float32)
+    Jp, Ip = slice(1, nj + 2), slice(1, ni + 2)
+    for k in range(1, nk + 1):
+        pu = tem1 * (ppd[k, Jp, Ip] - ppd[k, Jp, 0 : ni + 1])
+        pu = pu * (thv[k, Jp, Ip] + thv[k, Jp, 0 : ni + 1])
+        sl = (k, slice(ng, ng + nj + 1), slice(ng, ng + ni + 1))
+        a["u3d"][sl] = a["u3d"][sl] + (dts * wu) * (a["uten"][sl] - pu)
+        pv = tem2 * (ppd[k, Jp, Ip] - ppd[k, 0 : nj + 1, Ip])
+        pv = pv * (thv[k, Jp, Ip] + thv[k, 0 : nj + 1, Ip])
+        a["v3d"][sl] = a["v3d"][sl] + (dts * wv) * (a["vten"][sl] - pv)
+
+
+def flat_small_step(a, m, flat, sides, *, ni, nj, nk, dts, rdx, rdy, nrk, n, nloop, alph, beta, kdiv):
+    ng = NG
+    ys, xs = slice(ng, ng + nj), slice(ng, ng + ni)
+    J, I = slice(1, nj + 1), slice(1, ni + 1)
+    flat_ssopen(a, sides, ni=ni, nj=nj, nk=nk, dts=dts, rdx=rdx, rdy=rdy)
+    flat_pressure_gradient(a, sides, ni=ni, nj=nj, nk=nk, dts=dts, rdx=rdx, rdy=rdy)
+    u, v, w, pia = a["u3d"], a["v3d"], a["w3d"], a["piadv"]
+    temx = (dts * HALF) * rdx
+    temy = (dts * HALF) * rdy
+    P = lambda name, k: a[name][k, J, I]  # noqa: E731
+    for k in range(1, nk + 1):
+        div = (u[k, ys, ng + 1 : ng + ni + 1] - u[k, ys, xs]) * rdx + (v[k, ng + 1 : ng + nj + 1, xs] - v[k, ys, xs]) * rdy
+        div = np.where(np.abs(div) < FLAT_SMEPS, ZERO, div)
+        pc = pia[k, J, I]
+        hx = u[k, ys, xs] * (pc - pia[k, J, 0:ni]) + u[k, ys, ng + 1 : ng + ni + 1] * (pia[k, J, 2 : ni + 2] - pc)
+        hx = (-hx) * temx
+        hy = v[k, ys, xs] * (pc - pia[k, 0:nj, I]) + v[k, ng + 1 : ng + nj + 1, xs] * (pia[k, 2 : nj + 2, I] - pc)
+        hy = (-hy) * temy
+        hp = hx + hy
+        vert = w[k, ys, xs] * (P("pk1", k) + P("qk", k)) + w[k + 1, ys, xs] * (P("pk2", k) - P("qk", k))
+        a["fpk"][k, J, I] = (dts * (P("ppten", k) - P("ppterm", k) * div) + hp) + beta * vert
+    asq = alph * alph
+    mm, pk1, pk2, qk, fpk, pp3d = a["mm"], a["pk1"], a["pk2"], a["qk"], a["fpk"], a["pp3d"]
+    ec = np.zeros((nk + 2, nj, ni), np.float32)
+    ed = np.zeros((nk + 2, nj, ni), np.float32)
+    for k in range(2, nk + 1):
+        zero = np.zeros((nj, ni), np.float32)
+        aa = (-asq) * (mm[k, J, I] * (pk1[k - 1, J, I] + qk[k - 1, J, I])) if k >= 3 else zero
+        cc = (-asq) * ((-mm[k, J, I]) * (pk2[k, J, I] - qk[k, J, I])) if k < nk else zero
+        diag = (-(pk1[k, J, I] + qk[k, J, I])) + (pk2[k - 1, J, I] - qk[k - 1, J, I])
+        bb = F(1.0) - asq * (mm[k, J, I] * diag)
+        pdelta = pp3d[k, J, I] - pp3d[k - 1, J, I]
+        fwk = dts * a["wten"][k, ys, xs] + beta * ((-mm[k, J, I]) * pdelta)
+        udelta = (pp3d[k, J, I] + fpk[k, J, I]) - (pp3d[k - 1, J, I] + fpk[k - 1, J, I])
+        dd = (w[k, ys, xs] + fwk) + alph * ((-mm[k, J, I]) * udelta)
+        pivot = bb if k == 2 else aa * ec[k - 1] + bb
+        inv = F(1.0) / pivot
+        if k < nk:
+            ec[k] = (-cc) * inv
+            ed[k] = (dd if k == 2 else dd - aa * ed[k - 1]) * inv
+        else:
+            w[k, ys, xs] = (dd - aa * ed[k - 1]) * inv
+    for k in range(nk - 1, 1, -1):
+        w[k, ys, xs] = ec[k] * w[k + 1, ys, xs] + ed[k]
+    for k in range(1, nk + 1):
+        old = pp3d[k, J, I].copy()
+        vt = w[k, ys, xs] * (pk1[k, J, I] + qk[k, J, I]) + w[k + 1, ys, xs] * (pk2[k, J, I] - qk[k, J, I])
+        up = (old + fpk[k, J, I]) + alph * vt
+        up = np.where(np.abs(up) < FLAT_SMEPS, ZERO, up)
+        pp3d[k, J, I] = up
+        tend = kdiv * (up - old)
+        a["ppd"][k, J, I] = up + tend
+        if nrk == 3 and n == nloop:
+            a["ppx"][k, J, I] = tend
+    if n < nloop:
+        bc.bcp(a["ppd"], sides, ni=ni, nj=nj, nk=nk)
+    tavg = F(1.0) / F(nloop)
+    yy, xx = slice(ng, ng + nj + 1), slice(ng, ng + ni + 1)
+    ks = slice(1, nk + 1)
+    vu = a["rru"][ks, yy, xx] + a["u3d"][ks, yy, xx]
+    vv = a["rrv"][ks, yy, xx] + a["v3d"][ks, yy, xx]
+    addw = w[ks, yy, xx] if n < nloop else w[ks, yy, xx] * alph
+    vw = a["rrw"][ks, yy, xx] + addw
+    if n < nloop:
+        a["rru"][ks, yy, xx], a["rrv"][ks, yy, xx], a["rrw"][ks, yy, xx] = vu, vv, vw
+    else:
+        a["rru"][ks, yy, xx], a["rrv"][ks, yy, xx], a["rrw"][ks, yy, xx] = vu * tavg, vv * tavg, vw * tavg
+
+
+def cpu_flat_stage_inplace(a, m, flat, *, ni, nj, nk, dt, nsound, nrk, dx, dy, lateral_sides=(2, 2, 2, 2),
+                           alph=None, kdiv=None, radiation=True, stage_family_fills=True):
+    """One acoustic RK stage on a flat stretched column (``flat``: mh, mf, rdz)."""
+    ng = NG
+    sides = tuple(lateral_sides)
+    nloop, dts_v = sound3_stage_schedule_f32(dt, nsound, nrk)
+    dts = F(dts_v)
+    alph = F(ALPH if alph is None else alph)
+    kdiv = F(KDIV if kdiv is None else kdiv)
+    beta = F(1.0) - alph
+    rdx = F(1.0) / F(dx)
+    rdy = F(1.0) / F(dy)
+    flat_prepare_coefficients(a, m, flat, ni=ni, nj=nj, nk=nk, dts=dts)
+    a["ppd"][1 : nk + 1] = a["ppx"][1 : nk + 1]
+    if nrk != 1:
+        a["pp3d"][1 : nk + 1] = a["ppi"][1 : nk + 1]
+    ks = slice(1, nk + 1)
+    yy, xx = slice(ng - 1, ng + nj + 1), slice(ng - 1, ng + ni + 1)
+    a["rru"][ks, yy, xx] = ZERO
+    a["rrv"][ks, yy, xx] = ZERO
+    a["rrw"][ks, yy, xx] = a["wa"][ks, yy, xx] * beta
+    if nrk != 1:
+        a["u3d"][ks, yy, xx] = a["ua"][ks, yy, xx]
+        a["v3d"][ks, yy, xx] = a["va"][ks, yy, xx]
+        a["w3d"][ks, yy, xx] = a["wa"][ks, yy, xx]
+    for n in range(1, nloop + 1):
+        flat_small_step(a, m, flat, sides, ni=ni, nj=nj, nk=nk, dts=dts, rdx=rdx, rdy=rdy, nrk=nrk, n=n,
+                        nloop=nloop, alph=alph, beta=beta, kdiv=kdiv)
+    if radiation:
+        bc.radiation4(a, ni=ni, nj=nj, nk=nk, dx=dx, dy=dy, dttmp=rk_stage_dt_f32(dt, nrk))
+    if stage_family_fills:
+        bc.fill(a["u3d"], bc.KIND_U, sides, ni=ni, nj=nj, nk=nk)
+        bc.fill(a["v3d"], bc.KIND_V, sides, ni=ni, nj=nj, nk=nk)
+        bc.fill(a["w3d"], bc.KIND_W, sides, ni=ni, nj=nj, nk=nk)
+        bc.bcw_rigid(a["w3d"], ni=ni, nj=nj, nk=nk)
+        if nrk < 3:
+            bc.bcp(a["pp3d"], sides, ni=ni, nj=nj, nk=nk)
+            if "scalar" in a:
+                bc.fill(a["scalar"], bc.KIND_SCALAR, sides, ni=ni, nj=nj, nk=nk)
+    return nloop, float(dts)

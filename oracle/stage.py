@@ -256,3 +256,51 @@ def solve3_density(rho, c1, c2):
     for f in range(1, nk):
         rf[f] = c1[f] * rho[f - 1] + c2[f] * rho[f]
     return rr, rf
+
+
+# ----------------------------------------------------------------------------- flat stretched column
+# The terrain_flag=.false. texts of cuda_gm1_child_stretched_moist.py:155-190
+# (misclibs.F:127-138, 241-270, 324-336; solve2.F:897-906).  ``flat`` carries
+# the column's mh (nk+2), mf (nk+3) and rdz = 1/dz of the metric reference;
+# the base state is read at the first physical column, CM1's (1,1,k).
+FLAT_SMEPS = F(1.0e-30)
+
+
+def getdiv_flat(a, div, theta_div, thterm, flat, *, ni, nj, nk, dx, dy):
+    """((du*rdx) + (dv*rdy)) + ((dw*rdz)*mh(k)) of the stage averages, theta_div = div*thterm."""
+    ys, xs = slice(NG, NG + nj), slice(NG, NG + ni)
+    rdx, rdy, rdz = F(1.0) / F(dx), F(1.0) / F(dy), F(flat.rdz)
+    u, v, w = a["rru"], a["rrv"], a["rrw"]
+    for k in range(1, nk + 1):
+        d = (u[k, ys, NG + 1 : NG + ni + 1] - u[k, ys, xs]) * rdx + (v[k, NG + 1 : NG + nj + 1, xs] - v[k, ys, xs]) * rdy
+        d = d + ((w[k + 1, ys, xs] - w[k, ys, xs]) * rdz) * flat.mh[k]
+        d = np.where(np.abs(d) < FLAT_SMEPS, ZERO, d)
+        div[k - 1] = d
+        theta_div[k - 1] = d * thterm[k - 1]
+
+
+def getdivx_flat(a, m, divx, flat, *, ni, nj, nk, dx, dy):
+    """rru, rrv <- (avg)*rho0(1,1,k), rrw <- (avg)*rf0(1,1,k) (0 at both lids), then divx."""
+    ks = slice(1, nk + 1)
+    ys, xs = slice(NG, NG + nj), slice(NG, NG + ni)
+    rho_c = m["rho0_3d"][:, 0, 0][:, None, None]
+    a["rru"][ks, ys, NG : NG + ni + 1] = a["rru"][ks, ys, NG : NG + ni + 1] * rho_c
+    a["rrv"][ks, NG : NG + nj + 1, xs] = a["rrv"][ks, NG : NG + nj + 1, xs] * rho_c
+    a["rrw"][1, ys, xs] = ZERO
+    a["rrw"][nk + 1, ys, xs] = ZERO
+    for kf in range(2, nk + 1):
+        a["rrw"][kf, ys, xs] = a["rrw"][kf, ys, xs] * m["rf0"][kf - 1, 0, 0]
+    rdx, rdy, rdz = F(1.0) / F(dx), F(1.0) / F(dy), F(flat.rdz)
+    rru, rrv, rrw = a["rru"], a["rrv"], a["rrw"]
+    d = (rru[ks, ys, NG + 1 : NG + ni + 1] - rru[ks, ys, xs]) * rdx + (rrv[ks, NG + 1 : NG + nj + 1, xs] - rrv[ks, ys, xs]) * rdy
+    d = d + ((rrw[2 : nk + 2, ys, xs] - rrw[1 : nk + 1, ys, xs]) * rdz) * flat.mh[1 : nk + 1][:, None, None]
+    divx[...] = np.where(np.abs(d) < FLAT_SMEPS, ZERO, d)
+
+
+def buoyancy_flat(a, m, *, ni, nj, nk):
+    """wten(k) += c1(k)*dum8(k-1) + c2(k)*dum8(k) for k = 2..nk; no horizontal term on a flat column."""
+    d8 = a["dum8"]
+    ys, xs = slice(NG, NG + nj), slice(NG, NG + ni)
+    for kf in range(2, nk + 1):
+        add = m["c1"][kf - 1] * d8[kf - 1, ys, xs] + m["c2"][kf - 1] * d8[kf, ys, xs]
+        a["wten"][kf, ys, xs] = a["wten"][kf, ys, xs] + add
