@@ -51,9 +51,12 @@ def parse_args(argv=None):
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--ni", type=int, default=320)
-    p.add_argument("--nj", type=int, default=320)
-    p.add_argument("--nk", type=int, default=80)
+    p.add_argument("--workload", choices=("moist-open", "stretched"), default="moist-open",
+                   help="moist-open: 320x320x80 uniform 250 m (the headline case); stretched: 640x768x80 on a "
+                        "stretch_z=1 column (20 m -> 280 m), dx 100 m, two bubbles")
+    p.add_argument("--ni", type=int, default=None)
+    p.add_argument("--nj", type=int, default=None)
+    p.add_argument("--nk", type=int, default=None)
     p.add_argument("--kernel", default="k_snd_column", choices=sorted(KERNEL_WORDS))
     p.add_argument("--cpu-sample", type=int, default=96, help="horizontal size of the oracle's sub-domain sample")
     p.add_argument("--cpu-steps", type=int, default=2)
@@ -66,10 +69,12 @@ def parse_args(argv=None):
     return p.parse_args(argv)
 
 
-def workload(ni, nj, nk):
-    from gm1_amd.fields import moist_c2
+def workload(name, ni, nj, nk):
+    from gm1_amd.fields import large_stretched_c3, moist_c2
 
-    return moist_c2(ni, nj, nk)
+    if name == "stretched":
+        return large_stretched_c3(ni or 640, nj or 768, nk or 80)
+    return moist_c2(ni or 320, nj or 320, nk or 80)
 
 
 def traffic_from_profiles(kernel: str, label: str):
@@ -94,13 +99,15 @@ def cpu_baseline(cfg, sample: int, steps: int):
     sub = replace(cfg, ni=sample, nj=sample)
     host = host_initial_state(sub)
     a = host["arrays"]
-    prof = osound.terrain_coordinate_profiles(sub.nk, sub.dz)
+    levels = host["levels"]
+    prof = host["prof"] if levels is not None else osound.terrain_coordinate_profiles(sub.nk, sub.dz)
+    zt = np.float32(levels.ztop) if levels is not None else np.float32(sub.nk * sub.dz)
     c = {"ni": sub.ni, "nj": sub.nj, "nk": sub.nk, "dx": sub.dx, "dy": sub.dy, "dz": sub.dz, "dt": sub.dt,
          "nsound": sub.nsound, "lateral_sides": sub.lateral_sides}
     mass1 = ok.dry_mass(a["rho"], a["ruh"], a["rvh"], a["rmh"]) if "rho" in a else None
     t0 = time.perf_counter()
     for _ in range(steps):
-        ol.large_step(a, host["metrics"], host["base"], prof, c, zt=np.float32(sub.nk * sub.dz), mass1=mass1)
+        ol.large_step(a, host["metrics"], host["base"], prof, c, zt=zt, mass1=mass1, flat=levels)
     elapsed = time.perf_counter() - t0
     return {
         "value": sub.cells * steps / elapsed,
@@ -126,7 +133,7 @@ def main(argv=None) -> int:
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)
         return 2
-    cfg = workload(args.ni, args.nj, args.nk)
+    cfg = workload(args.workload, args.ni, args.nj, args.nk)
     host = host_initial_state(cfg)
     stream = torch.cuda.Stream()
     st = create_gm1_large_step_state(host, "cuda:0", stream=stream)
@@ -216,7 +223,10 @@ def main(argv=None) -> int:
         "config": {
             "workload": f"CM1 large step, moist Kessler warm bubble (WENO5/5th-order advection, psolver=3 "
                         f"acoustic, pdef moisture, Kessler ptype 1 eqtset 2, dry-mass correction), {label} cells, "
-                        f"dx=dy=dz={cfg.dx:g} m, open-radiative laterals, dt={cfg.dt:g} s, "
+                        + (f"dx=dy=dz={cfg.dx:g} m" if cfg.stretched is None else
+                           f"dx=dy={cfg.dx:g} m, stretch_z=1 column {cfg.stretched[3]:g} m -> {cfg.stretched[4]:g} m, "
+                           f"top {cfg.stretched[0]:g} m")
+                        + f", open-radiative laterals, dt={cfg.dt:g} s, "
                         f"{cfg.nsound} substeps ({small_steps} small steps/step)",
             "cells": cfg.cells,
             "launch": mode,
