@@ -64,6 +64,8 @@ def parse_args(argv=None):
     p.add_argument("--graph", action="store_true",
                    help="time replays of the step captured into a HIP graph (the kernel timer then runs in an "
                         "eager pass of the same length)")
+    p.add_argument("--column-pass", choices=("staged", "staged16", "register"), default="staged",
+                   help="the acoustic column pass (all give the same words)")
     p.add_argument("--calibration-copy-mib", type=int, default=0,
                    help="issue one plain device copy of this size before the steps (HBM counter calibration)")
     return p.parse_args(argv)
@@ -137,6 +139,7 @@ def main(argv=None) -> int:
     host = host_initial_state(cfg)
     stream = torch.cuda.Stream()
     st = create_gm1_large_step_state(host, "cuda:0", stream=stream)
+    st.column_pass = args.column_pass
     del host
     torch.cuda.synchronize()
     print(f"[bench] state resident: {cfg.name}, {cfg.cells} cells", file=sys.stderr, flush=True)
@@ -230,6 +233,7 @@ def main(argv=None) -> int:
                         f"{cfg.nsound} substeps ({small_steps} small steps/step)",
             "cells": cfg.cells,
             "launch": mode,
+            "column_pass": args.column_pass,
         },
         "roofline": {
             "bound": "hbm",

@@ -623,12 +623,13 @@ __global__ __launch_bounds__(64) void k_snd_column(SoundArgs a, int n) {
 // does the pressure update, the filter and the time averages.  fpk and the
 // elimination coefficients never reach HBM.  Workgroups are mapped so each
 // XCD sweeps a contiguous band of rows.
-constexpr int C2PLANES = 6;  // fpk, pp, s1 = pk1+qk, s2 = pk2-qk, mm -> ec, dd -> ed -> w
+// LDS planes: fpk, pp, s1 = pk1+qk, s2 = pk2-qk, mm -> ec, dd -> ed -> w.
+constexpr int C2PLANES = 6;
 
 int column2_lds_bytes(int nk, int cw) { return C2PLANES * (nk + 1) * cw * (int)sizeof(float); }
 
 template <int C2W, int C2T, bool FLAT>
-__global__ __launch_bounds__(C2T) void k_snd_column2(SoundArgs a, int n) {
+__device__ __forceinline__ void column2_body(const SoundArgs& a, int n) {
   constexpr int C2S = C2T / C2W;
   extern __shared__ float lds[];
   __shared__ float lw1[C2W];
@@ -935,6 +936,11 @@ __global__ __launch_bounds__(C2T) void k_snd_column2(SoundArgs a, int n) {
         }
     }
   }
+}
+
+template <int C2W, int C2T, bool FLAT>
+__global__ __launch_bounds__(C2T) void k_snd_column2(SoundArgs a, int n) {
+  column2_body<C2W, C2T, FLAT>(a, n);
 }
 
 // ------------------------------------------------------------------ radiation

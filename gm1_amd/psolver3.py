@@ -121,8 +121,9 @@ def launch_cuda_psolver3_terrain_open_three_stage(
     """Enqueue the acoustic RK stages of ``_nrk_sequence`` on ``stream``.
 
     ``column_pass`` selects the vertical pass: "staged" (LDS-staged, k-parallel
-    outside the Thomas recurrence) or "register" (one thread walks each
-    column); both produce the same words.
+    outside the Thomas recurrence, 32 columns per workgroup), "staged16" (16
+    columns) or "register" (one thread walks each column); all produce the
+    same words.
     """
     if column_pass not in ("staged", "staged16", "register"):
         raise ValueError("column_pass must be 'staged', 'staged16' or 'register'")
