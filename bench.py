@@ -51,9 +51,14 @@ def parse_args(argv=None):
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", choices=("moist-open", "stretched"), default="moist-open",
+    p.add_argument("--workload", choices=("moist-open", "stretched", "ensemble"), default="moist-open",
                    help="moist-open: 320x320x80 uniform 250 m (the headline case); stretched: 640x768x80 on a "
-                        "stretch_z=1 column (20 m -> 280 m), dx 100 m, two bubbles")
+                        "stretch_z=1 column (20 m -> 280 m), dx 100 m, two bubbles; ensemble: 24 perturbed "
+                        "members on uneven grids, batched (graph replays over streams) against plain "
+                        "(sequential eager steps)")
+    p.add_argument("--members", type=int, default=24)
+    p.add_argument("--ensemble-seed", type=int, default=2024)
+    p.add_argument("--streams", type=int, default=4)
     p.add_argument("--ni", type=int, default=None)
     p.add_argument("--nj", type=int, default=None)
     p.add_argument("--nk", type=int, default=None)
@@ -121,6 +126,92 @@ def cpu_baseline(cfg, sample: int, steps: int):
     }
 
 
+def run_ensemble(args) -> int:
+    """Batched ensemble (graph replays over streams) against the plain sequential form, one JSON line."""
+    import torch
+
+    from gm1_amd import timing
+    from gm1_amd.ensemble import Gm1Ensemble, create_ensemble_states, sequential_steps
+    from gm1_amd.fields import ensemble_members
+
+    members = ensemble_members(args.ensemble_seed, args.members)
+    states = create_ensemble_states(members, "cuda:0")
+    cells = sum(m.cells for m in members)
+    torch.cuda.synchronize()
+    print(f"[bench] {len(members)} members resident, {cells} cells", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        sequential_steps(states, 1)
+    torch.cuda.synchronize()
+    small_steps = 15
+    timing.arm(args.kernel, len(members) * small_steps * args.steps + 8)
+    t0 = time.perf_counter()
+    sequential_steps(states, args.steps)
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t0
+    kernel_ms, launches = timing.collect()
+    print(f"[bench] plain: {args.steps} steps of every member in {plain_s * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    ens = Gm1Ensemble(states, streams=args.streams)
+    ens.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ens.step()
+    torch.cuda.synchronize()
+    batched_s = time.perf_counter() - t0
+    print(f"[bench] batched: {args.steps} steps of every member in {batched_s * 1e3:.1f} ms", file=sys.stderr,
+          flush=True)
+    for st in states:
+        if not torch.isfinite(st.a["w3d"]).all():
+            print(f"[bench] non-finite w in {st.cfg.name}", file=sys.stderr)
+            return 1
+    avg_launch_s = kernel_ms * 1e-3 / max(launches, 1)
+    algo_bytes = KERNEL_WORDS[args.kernel] * WORD * cells / len(members)
+    achieved = algo_bytes / avg_launch_s / 1e9
+    value = cells * args.steps / batched_s
+    result = {
+        "metric": "grid-cell RK3 large-steps/s on 1 GPU, batched ensemble",
+        "value": value,
+        "unit": "grid-cell large-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": batched_s * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic warm-bubble fields (analytic sounding, splitmix64 draws and noise)",
+        "config": {
+            "workload": f"{len(members)} perturbed moist Kessler warm bubbles on uneven grids (ni, nj in "
+                        f"{{128,192,256,320}}, nk in {{64,80}}, dx=dy=dz=250 m, dt=1.5 s, 8 substeps), "
+                        f"{cells} cells in total; batched = one HIP-graph replay per member per step over "
+                        f"{args.streams} streams, plain = eager steps member after member",
+            "members": len(members),
+            "cells": cells,
+            "plain_value": cells * args.steps / plain_s,
+            "plain_ms_per_step": plain_s * 1e3 / args.steps,
+            "batched_over_plain": plain_s / batched_s,
+            "member_steps_per_s": len(members) * args.steps / batched_s,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": args.kernel,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": algo_bytes,
+            "avg_launch_us": avg_launch_s * 1e6,
+            "launches": launches,
+            "measured_in": "the plain pass (mean member)",
+        },
+        "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 def main(argv=None) -> int:
     args = parse_args(argv)
     if args.gpus != 1:
@@ -135,6 +226,8 @@ def main(argv=None) -> int:
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)
         return 2
+    if args.workload == "ensemble":
+        return run_ensemble(args)
     cfg = workload(args.workload, args.ni, args.nj, args.nk)
     host = host_initial_state(cfg)
     stream = torch.cuda.Stream()

@@ -131,6 +131,35 @@ def large_stretched_c3(ni=640, nj=768, nk=80) -> CaseConfig:
                       stretched=(17850.0, 200.0, 2450.0, 20.0, 280.0))
 
 
+ENSEMBLE_SIZES = (128, 192, 256, 320)
+ENSEMBLE_LEVELS = (64, 80)
+
+
+def ensemble_members(seed: int = 2024, count: int = 24) -> list[CaseConfig]:
+    """Perturbed moist warm bubbles on uneven grids, every draw from ``seed`` (splitmix64).
+
+    Per member: ni, nj from {128, 192, 256, 320}, nk from {64, 80}, dx = dy = dz
+    = 250 m, open-radiative laterals, dt 1.5 s with 8 substeps; bubble
+    amplitude U(1.5, 3.5) K, centre offset U(-5, 5) km in x and y from the
+    domain centre, the moisture profile scaled by U(0.9, 1.1), noise
+    U(-0.1, 0.1) K seeded per member.
+    """
+    words = splitmix64_words(seed, 8 * count).reshape(count, 8)
+    unit = (words >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+    out = []
+    for m in range(count):
+        w, u = words[m], unit[m]
+        ni, nj = ENSEMBLE_SIZES[int(w[0] % 4)], ENSEMBLE_SIZES[int(w[1] % 4)]
+        nk = ENSEMBLE_LEVELS[int(w[2] % 2)]
+        amp = 1.5 + 2.0 * u[3]
+        x0 = ni * 125.0 + (2.0 * u[4] - 1.0) * 5000.0
+        y0 = nj * 125.0 + (2.0 * u[5] - 1.0) * 5000.0
+        out.append(CaseConfig(f"member-{m:02d}-{ni}x{nj}x{nk}", ni, nj, nk, 250.0, 250.0, 250.0, 1.5, 8,
+                              (2, 2, 2, 2), True, "hodograph", ((amp, x0, y0, 1400.0),), noise=0.1,
+                              seed=seed * 1000 + m, extra={"qv_scale": 0.9 + 0.2 * u[6]}))
+    return out
+
+
 def profiles_for(cfg: CaseConfig) -> VerticalProfiles:
     if cfg.stretched is None:
         return terrain_coordinate_profiles(cfg.nk, cfg.dz)
@@ -167,7 +196,7 @@ def base_state(cfg: CaseConfig, prof: VerticalProfiles) -> dict[str, np.ndarray]
         p = P00 * pi ** (CP / RD)
         es = 611.2 * np.exp(17.67 * (t - 273.15) / (t - 29.65))
         qvs = EPS * es / np.maximum(p - es, 1.0)
-        qv = np.minimum(_rh_env(zs) * qvs, 0.014)
+        qv = np.minimum(_rh_env(zs) * qvs, 0.014) * float(cfg.extra.get("qv_scale", 1.0))
     thv = th * (1.0 + qv / EPS) / (1.0 + qv)
     p = P00 * pi ** (CP / RD)
     rho = p / (RD * thv * pi)

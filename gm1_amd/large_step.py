@@ -46,7 +46,7 @@ from .kessler import (
     launch_cuda_gm1_kessler_transition,
 )
 from .momentum_advection5 import launch_cuda_terrain_open_momentum_advection5
-from .psolver3 import TerrainOpenSoundAblations, launch_cuda_psolver3_terrain_open_stage
+from .psolver3 import TerrainOpenSoundAblations, device_profiles, launch_cuda_psolver3_terrain_open_stage
 from .scalar_moist_rk import (
     launch_pdef_moisture,
     launch_scalar_rk_compose,
@@ -176,6 +176,11 @@ def create_gm1_large_step_state(host: dict[str, Any], device: torch.device | str
         host["numq"], stream)
     if host.get("levels") is not None:
         st.flat = flat_stretched_column(host["levels"], dev)
+    # device copies of the vertical profiles are made here, never inside a step
+    # (a step may be captured into a graph, where host-to-device copies are not allowed)
+    device_profiles(st.prof, dev)
+    if st.flat is not None:
+        device_profiles(st.flat.levels.profiles(), dev)
     if st.numq:
         cfg = st.cfg
         st.selection = Gm1KesslerDriverSelection(equation_set=equation_set)
