@@ -626,16 +626,51 @@ __global__ __launch_bounds__(64) void k_snd_column(SoundArgs a, int n) {
 // LDS planes: fpk, pp, s1 = pk1+qk, s2 = pk2-qk, mm -> ec, dd -> ed -> w.
 constexpr int C2PLANES = 6;
 
-int column2_lds_bytes(int nk, int cw) { return C2PLANES * (nk + 1) * cw * (int)sizeof(float); }
+// the pipelined pass keeps wten in a seventh plane
+int column2_lds_bytes(int nk, int cw, bool pipe = false) {
+  return (C2PLANES + (pipe ? 1 : 0)) * (nk + 1) * cw * (int)sizeof(float);
+}
 
-template <int C2W, int C2T, bool FLAT>
-__device__ __forceinline__ void column2_body(const SoundArgs& a, int n) {
+// rru(k) += u(k) and rrv(k) += v(k) on one column's west u face and south v
+// face for levels kb..ke (divided by nloop on the last small step); the words
+// of four levels are requested before any is written.
+GM1_DEV void average_uv_faces(const SoundArgs& a, int kb, int ke, int64_t ust, int64_t uc, int64_t vst, int64_t vc,
+                              bool last, float tavg) {
+  float* __restrict__ rru = a.rru;
+  float* __restrict__ rrv = a.rrv;
+  const float* __restrict__ u3 = a.u3d;
+  const float* __restrict__ v3 = a.v3d;
+  constexpr int B = 4;
+  for (int k0 = kb; k0 <= ke; k0 += B) {
+    float ru[B], uu[B], rv[B], vv[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+      if (k0 + u <= ke) {
+        const int64_t pu = (int64_t)(k0 + u) * ust + uc, pv = (int64_t)(k0 + u) * vst + vc;
+        ru[u] = rru[pu];
+        uu[u] = u3[pu];
+        rv[u] = rrv[pv];
+        vv[u] = v3[pv];
+      }
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+      if (k0 + u <= ke) {
+        const int64_t pu = (int64_t)(k0 + u) * ust + uc, pv = (int64_t)(k0 + u) * vst + vc;
+        const float au = ru[u] + uu[u];
+        rru[pu] = last ? au * tavg : au;
+        const float av = rv[u] + vv[u];
+        rrv[pv] = last ? av * tavg : av;
+      }
+  }
+}
+
+template <int C2W, int C2T, bool FLAT, bool PIPE>
+__device__ __forceinline__ void column2_body(const SoundArgs& a, int n, int tile) {
   constexpr int C2S = C2T / C2W;
   extern __shared__ float lds[];
   __shared__ float lw1[C2W];
   const int ni = a.ni, nj = a.nj, nk = a.nk;
   const int gx = (ni + 1 + C2W - 1) / C2W;
-  const int tile = xcd_tile((int)blockIdx.x, gx * (nj + 1));
   const int j = tile / gx;
   const int c = threadIdx.x % C2W, slot = threadIdx.x / C2W;
   const int i = (tile % gx) * C2W + c;
@@ -658,6 +693,7 @@ __device__ __forceinline__ void column2_body(const SoundArgs& a, int n) {
   float* Ls2 = lds + 3 * L;
   float* Lmm = lds + 4 * L;
   float* Ldd = lds + 5 * L;
+  float* Lwt = lds + 6 * L;  // wten (pipelined pass only)
   auto at = [&](int k) { return k * C2W + c; };
   const int chunk = (nk + C2S - 1) / C2S;
   const int kb = 1 + slot * chunk, ke = min(nk, kb + chunk - 1);
@@ -670,34 +706,85 @@ __device__ __forceinline__ void column2_body(const SoundArgs& a, int n) {
     const int pj = j + 1, pi = i + 1;
     const float temx = (a.dts * 0.5f) * a.rdx, temy = (a.dts * 0.5f) * a.rdy;
     if (kb == 1) lw1[c] = w[wst + wc];
-    for (int k = kb; k <= ke; ++k) {
-      const int64_t p = pix(k, pj, pi, ni, nj);
-      const float uw = a.u3d[(int64_t)k * ust + uc], ue = a.u3d[(int64_t)k * ust + uc + 1];
-      const float vs = a.v3d[(int64_t)k * vst + vc], vn = a.v3d[(int64_t)k * vst + vc + nxs];
-      float div = (ue - uw) * a.rdx + (vn - vs) * a.rdy;
-      if (fabsf(div) < smeps) div = 0.0f;
-      const float pc = a.piadv[p];
-      float hx = uw * (pc - a.piadv[p - 1]) + ue * (a.piadv[p + 1] - pc);
-      hx = (-hx) * temx;
-      float hy = vs * (pc - a.piadv[p - (ni + 2)]) + vn * (a.piadv[p + (ni + 2)] - pc);
-      hy = (-hy) * temy;
-      const float hp = hx + hy;
-      const float wraw = w[(int64_t)k * wst + wc];
-      const float qkk = a.qk[p], pk1k = a.pk1[p], pk2k = a.pk2[p];
-      const float s1 = pk1k + qkk, s2 = pk2k - qkk;
-      const float vert = wraw * s1 + w[(int64_t)(k + 1) * wst + wc] * s2;
-      Lfpk[at(k)] = (a.dts * (a.ppten[p] - a.ppterm[p] * div) + hp) + a.beta * vert;
-      Lpp[at(k)] = a.pp3d[p];
-      Ls1[at(k)] = s1;
-      Ls2[at(k)] = s2;
-      Lmm[at(k)] = a.mm[p];
-      Ldd[at(k)] = wraw;
-      {
-        const int64_t pu = (int64_t)k * ust + uc, pv = (int64_t)k * vst + vc;
-        const float au = a.rru[pu] + uw;
-        a.rru[pu] = last ? au * tavg : au;
-        const float av = a.rrv[pv] + vs;
-        a.rrv[pv] = last ? av * tavg : av;
+    if constexpr (PIPE) {
+      // level k+1's words are requested before level k is formed
+      struct In { float uw, ue, vs, vn, pc, pe, pw, pn, ps, wup, wt, qk, pk1, pk2, ppten, ppterm, pp, mm; };
+      auto load = [&](int k) {
+        const int64_t p = pix(k, pj, pi, ni, nj);
+        In v;
+        v.uw = a.u3d[(int64_t)k * ust + uc];
+        v.ue = a.u3d[(int64_t)k * ust + uc + 1];
+        v.vs = a.v3d[(int64_t)k * vst + vc];
+        v.vn = a.v3d[(int64_t)k * vst + vc + nxs];
+        v.pc = a.piadv[p];
+        v.pe = a.piadv[p + 1];
+        v.pw = a.piadv[p - 1];
+        v.pn = a.piadv[p + (ni + 2)];
+        v.ps = a.piadv[p - (ni + 2)];
+        v.wup = w[(int64_t)(k + 1) * wst + wc];
+        v.wt = a.wten[(int64_t)k * wst + wc];
+        v.qk = a.qk[p];
+        v.pk1 = a.pk1[p];
+        v.pk2 = a.pk2[p];
+        v.ppten = a.ppten[p];
+        v.ppterm = a.ppterm[p];
+        v.pp = a.pp3d[p];
+        v.mm = a.mm[p];
+        return v;
+      };
+      float wraw = w[(int64_t)kb * wst + wc];
+      In nx = load(kb);
+      for (int k = kb; k <= ke; ++k) {
+        const In cu = nx;
+        nx = load(min(k + 1, ke));
+        float div = (cu.ue - cu.uw) * a.rdx + (cu.vn - cu.vs) * a.rdy;
+        if (fabsf(div) < smeps) div = 0.0f;
+        float hx = cu.uw * (cu.pc - cu.pw) + cu.ue * (cu.pe - cu.pc);
+        hx = (-hx) * temx;
+        float hy = cu.vs * (cu.pc - cu.ps) + cu.vn * (cu.pn - cu.pc);
+        hy = (-hy) * temy;
+        const float hp = hx + hy;
+        const float s1 = cu.pk1 + cu.qk, s2 = cu.pk2 - cu.qk;
+        const float vert = wraw * s1 + cu.wup * s2;
+        Lfpk[at(k)] = (a.dts * (cu.ppten - cu.ppterm * div) + hp) + a.beta * vert;
+        Lpp[at(k)] = cu.pp;
+        Ls1[at(k)] = s1;
+        Ls2[at(k)] = s2;
+        Lmm[at(k)] = cu.mm;
+        Ldd[at(k)] = wraw;
+        Lwt[at(k)] = cu.wt;
+        wraw = cu.wup;
+      }
+    } else {
+      for (int k = kb; k <= ke; ++k) {
+        const int64_t p = pix(k, pj, pi, ni, nj);
+        const float uw = a.u3d[(int64_t)k * ust + uc], ue = a.u3d[(int64_t)k * ust + uc + 1];
+        const float vs = a.v3d[(int64_t)k * vst + vc], vn = a.v3d[(int64_t)k * vst + vc + nxs];
+        float div = (ue - uw) * a.rdx + (vn - vs) * a.rdy;
+        if (fabsf(div) < smeps) div = 0.0f;
+        const float pc = a.piadv[p];
+        float hx = uw * (pc - a.piadv[p - 1]) + ue * (a.piadv[p + 1] - pc);
+        hx = (-hx) * temx;
+        float hy = vs * (pc - a.piadv[p - (ni + 2)]) + vn * (a.piadv[p + (ni + 2)] - pc);
+        hy = (-hy) * temy;
+        const float hp = hx + hy;
+        const float wraw = w[(int64_t)k * wst + wc];
+        const float qkk = a.qk[p], pk1k = a.pk1[p], pk2k = a.pk2[p];
+        const float s1 = pk1k + qkk, s2 = pk2k - qkk;
+        const float vert = wraw * s1 + w[(int64_t)(k + 1) * wst + wc] * s2;
+        Lfpk[at(k)] = (a.dts * (a.ppten[p] - a.ppterm[p] * div) + hp) + a.beta * vert;
+        Lpp[at(k)] = a.pp3d[p];
+        Ls1[at(k)] = s1;
+        Ls2[at(k)] = s2;
+        Lmm[at(k)] = a.mm[p];
+        Ldd[at(k)] = wraw;
+        {
+          const int64_t pu = (int64_t)k * ust + uc, pv = (int64_t)k * vst + vc;
+          const float au = a.rru[pu] + uw;
+          a.rru[pu] = last ? au * tavg : au;
+          const float av = a.rrv[pv] + vs;
+          a.rrv[pv] = last ? av * tavg : av;
+        }
       }
     }
   }
@@ -748,58 +835,152 @@ __device__ __forceinline__ void column2_body(const SoundArgs& a, int n) {
     float rho_lo = a.rho0[(int64_t)(kb - 1) * plane];
     float dum3_lo = 0.0f, hadvp_lo = 0.0f;
     if (kb >= 2) wterms(kb, flo, faces(kb - 1), rho_lo, a.rho0[(int64_t)(kb - 2) * plane], dum3_lo, hadvp_lo);
-    for (int k = kb; k <= ke; ++k) {
-      const int64_t p = pix(k, pj, pi, ni, nj);
-      ColumnFaces fhi;
-      float rho_hi = 0.0f, dum3_hi, hadvp_hi;
-      if (k < nk) {
-        fhi = faces(k + 1);
-        rho_hi = a.rho0[(int64_t)k * plane];
-        wterms(k + 1, fhi, flo, rho_hi, rho_lo, dum3_hi, hadvp_hi);
-      } else {
-        fhi = flo;
-        dum3_hi = 0.0f;
-        hadvp_hi = 0.0f;
+    if constexpr (PIPE) {
+      // level k+1's words (faces of level k+1 included) are requested before
+      // level k is formed
+      const float* rdsf_ = rdsf;
+      struct In {
+        ColumnFaces f1;
+        float rho1, rrf1, sgf1, sg0, rds1, rdsfk, wup, wt, qk, ppten, ppterm, pc, pe, pw, pn, ps, rr0, pk1, pk2, pp,
+            mm;
+      };
+      auto load = [&](int k) {
+        const int64_t p = pix(k, pj, pi, ni, nj);
+        In v;
+        v.f1 = faces(k + 1);  // level nk+1 exists in the wind arrays; unused at k = nk
+        v.rho1 = a.rho0[(int64_t)min(k, nk - 1) * plane];
+        v.rrf1 = a.rrf0[(int64_t)k * plane + h];
+        v.sgf1 = sigmaf[k + 1];
+        v.sg0 = sigma[k];
+        v.rds1 = rds[k + 1];
+        v.rdsfk = rdsf_[k];
+        v.wup = w[(int64_t)(k + 1) * wst + wc];
+        v.wt = a.wten[(int64_t)k * wst + wc];
+        v.qk = a.qk[p];
+        v.ppten = a.ppten[p];
+        v.ppterm = a.ppterm[p];
+        v.pc = a.piadv[p];
+        v.pe = a.piadv[p + 1];
+        v.pw = a.piadv[p - 1];
+        v.pn = a.piadv[p + (ni + 2)];
+        v.ps = a.piadv[p - (ni + 2)];
+        v.rr0 = a.rr0[(int64_t)(k - 1) * plane + h];
+        v.pk1 = a.pk1[p];
+        v.pk2 = a.pk2[p];
+        v.pp = a.pp3d[p];
+        v.mm = a.mm[p];
+        return v;
+      };
+      float wraw = (kb == 1) ? w1new : w[(int64_t)kb * wst + wc];
+      In nx = load(kb);
+      for (int k = kb; k <= ke; ++k) {
+        const In cu = nx;
+        nx = load(min(k + 1, ke));
+        ColumnFaces fhi = flo;
+        float rho_hi = 0.0f, dum3_hi = 0.0f, hadvp_hi = 0.0f;
+        if (k < nk) {  // w level k+1 is interior
+          fhi = cu.f1;
+          rho_hi = cu.rho1;
+          const float r2 = (cu.sgf1 - cu.sg0) * cu.rds1;
+          const float r1 = 1.0f - r2;
+          const float height = (cu.sgf1 - a.zt) * a.rzt;
+          {
+            const float xt = (r2 * (fhi.xw + fhi.xe) + r1 * (flo.xw + flo.xe)) * ddx;
+            const float yt = (r2 * (fhi.ys + fhi.yn) + r1 * (flo.ys + flo.yn)) * ddy;
+            dum3_hi = ((0.5f * (xt + yt)) * height) * g;
+          }
+          {
+            const float xt =
+                (r2 * (rho_hi * fhi.xw + rho_hi * fhi.xe) + r1 * (rho_lo * flo.xw + rho_lo * flo.xe)) * ddx;
+            const float yt =
+                (r2 * (rho_hi * fhi.ys + rho_hi * fhi.yn) + r1 * (rho_lo * flo.ys + rho_lo * flo.yn)) * ddy;
+            hadvp_hi = (((0.5f * (xt + yt)) * height) * g) * cu.rrf1;
+          }
+        }
+        const float hz = (flo.xe - flo.xw) * a.rdx + (flo.yn - flo.ys) * a.rdy;
+        const float vt = (dum3_hi - dum3_lo) * cu.rdsfk;
+        float div = g * (hz + vt);
+        if (fabsf(div) < smeps) div = 0.0f;
+        const float wk = (k == 1) ? 0.0f : wraw;
+        const float wk1 = (k == nk) ? 0.0f : cu.wup;
+        float fpk = a.dts * (cu.ppten - cu.ppterm * div) + a.beta * ((-cu.qk) * (wk1 - wk));
+        const float u2 = temx * (rho_lo * flo.xe), u1 = temx * (rho_lo * flo.xw);
+        const float v2 = temy * (rho_lo * flo.yn), v1 = temy * (rho_lo * flo.ys);
+        const float ww2 = hadvp_hi + a.beta * cu.wup;
+        const float ww1 = hadvp_lo + a.beta * wraw;
+        const float xa = u2 * (cu.pe - cu.pc) + u1 * (cu.pc - cu.pw);
+        const float ya = v2 * (cu.pn - cu.pc) + v1 * (cu.pc - cu.ps);
+        float hza = (-(xa + ya)) * cu.rr0;
+        hza = hza * g;
+        const float vta = ww1 * cu.pk1 + ww2 * cu.pk2;
+        fpk = fpk + (hza + vta);
+        Lfpk[at(k)] = fpk;
+        Lpp[at(k)] = cu.pp;
+        Ls1[at(k)] = cu.pk1 + cu.qk;
+        Ls2[at(k)] = cu.pk2 - cu.qk;
+        Lmm[at(k)] = cu.mm;
+        Ldd[at(k)] = wraw;
+        Lwt[at(k)] = cu.wt;
+        flo = fhi;
+        rho_lo = rho_hi;
+        dum3_lo = dum3_hi;
+        hadvp_lo = hadvp_hi;
+        wraw = cu.wup;
       }
-      const float hz = (flo.xe - flo.xw) * a.rdx + (flo.yn - flo.ys) * a.rdy;
-      const float vt = (dum3_hi - dum3_lo) * rdsf[k];
-      float div = g * (hz + vt);
-      if (fabsf(div) < smeps) div = 0.0f;
-      const float wraw = w[(int64_t)k * wst + wc];
-      const float wk = (k == 1) ? 0.0f : wraw;
-      const float wk1 = (k == nk) ? 0.0f : w[(int64_t)(k + 1) * wst + wc];
-      const float qkk = a.qk[p];
-      float fpk = a.dts * (a.ppten[p] - a.ppterm[p] * div) + a.beta * ((-qkk) * (wk1 - wk));
-      const float u2 = temx * (rho_lo * flo.xe), u1 = temx * (rho_lo * flo.xw);
-      const float v2 = temy * (rho_lo * flo.yn), v1 = temy * (rho_lo * flo.ys);
-      const float wb = (k == 1) ? w1new : w[(int64_t)k * wst + wc];
-      const float ww2 = hadvp_hi + a.beta * w[(int64_t)(k + 1) * wst + wc];
-      const float ww1 = hadvp_lo + a.beta * wb;
-      const float pc = a.piadv[p];
-      const float xa = u2 * (a.piadv[p + 1] - pc) + u1 * (pc - a.piadv[p - 1]);
-      const float ya = v2 * (a.piadv[p + (ni + 2)] - pc) + v1 * (pc - a.piadv[p - (ni + 2)]);
-      float hza = (-(xa + ya)) * a.rr0[(int64_t)(k - 1) * plane + h];
-      hza = hza * g;
-      const float pk1k = a.pk1[p], pk2k = a.pk2[p];
-      const float vta = ww1 * pk1k + ww2 * pk2k;
-      fpk = fpk + (hza + vta);
-      Lfpk[at(k)] = fpk;
-      Lpp[at(k)] = a.pp3d[p];
-      Ls1[at(k)] = pk1k + qkk;
-      Ls2[at(k)] = pk2k - qkk;
-      Lmm[at(k)] = a.mm[p];
-      Ldd[at(k)] = wraw;  // w(k) before the solve, read back by phase A'
-      {  // time averages of this column's west u face and south v face
-        const int64_t pu = (int64_t)k * ust + uc, pv = (int64_t)k * vst + vc;
-        const float au = a.rru[pu] + a.u3d[pu];
-        a.rru[pu] = last ? au * tavg : au;
-        const float av = a.rrv[pv] + a.v3d[pv];
-        a.rrv[pv] = last ? av * tavg : av;
+    } else {
+      for (int k = kb; k <= ke; ++k) {
+        const int64_t p = pix(k, pj, pi, ni, nj);
+        ColumnFaces fhi;
+        float rho_hi = 0.0f, dum3_hi, hadvp_hi;
+        if (k < nk) {
+          fhi = faces(k + 1);
+          rho_hi = a.rho0[(int64_t)k * plane];
+          wterms(k + 1, fhi, flo, rho_hi, rho_lo, dum3_hi, hadvp_hi);
+        } else {
+          fhi = flo;
+          dum3_hi = 0.0f;
+          hadvp_hi = 0.0f;
+        }
+        const float hz = (flo.xe - flo.xw) * a.rdx + (flo.yn - flo.ys) * a.rdy;
+        const float vt = (dum3_hi - dum3_lo) * rdsf[k];
+        float div = g * (hz + vt);
+        if (fabsf(div) < smeps) div = 0.0f;
+        const float wraw = w[(int64_t)k * wst + wc];
+        const float wk = (k == 1) ? 0.0f : wraw;
+        const float wk1 = (k == nk) ? 0.0f : w[(int64_t)(k + 1) * wst + wc];
+        const float qkk = a.qk[p];
+        float fpk = a.dts * (a.ppten[p] - a.ppterm[p] * div) + a.beta * ((-qkk) * (wk1 - wk));
+        const float u2 = temx * (rho_lo * flo.xe), u1 = temx * (rho_lo * flo.xw);
+        const float v2 = temy * (rho_lo * flo.yn), v1 = temy * (rho_lo * flo.ys);
+        const float wb = (k == 1) ? w1new : w[(int64_t)k * wst + wc];
+        const float ww2 = hadvp_hi + a.beta * w[(int64_t)(k + 1) * wst + wc];
+        const float ww1 = hadvp_lo + a.beta * wb;
+        const float pc = a.piadv[p];
+        const float xa = u2 * (a.piadv[p + 1] - pc) + u1 * (pc - a.piadv[p - 1]);
+        const float ya = v2 * (a.piadv[p + (ni + 2)] - pc) + v1 * (pc - a.piadv[p - (ni + 2)]);
+        float hza = (-(xa + ya)) * a.rr0[(int64_t)(k - 1) * plane + h];
+        hza = hza * g;
+        const float pk1k = a.pk1[p], pk2k = a.pk2[p];
+        const float vta = ww1 * pk1k + ww2 * pk2k;
+        fpk = fpk + (hza + vta);
+        Lfpk[at(k)] = fpk;
+        Lpp[at(k)] = a.pp3d[p];
+        Ls1[at(k)] = pk1k + qkk;
+        Ls2[at(k)] = pk2k - qkk;
+        Lmm[at(k)] = a.mm[p];
+        Ldd[at(k)] = wraw;  // w(k) before the solve, read back by phase A'
+        {  // time averages of this column's west u face and south v face
+          const int64_t pu = (int64_t)k * ust + uc, pv = (int64_t)k * vst + vc;
+          const float au = a.rru[pu] + a.u3d[pu];
+          a.rru[pu] = last ? au * tavg : au;
+          const float av = a.rrv[pv] + a.v3d[pv];
+          a.rrv[pv] = last ? av * tavg : av;
+        }
+        flo = fhi;
+        rho_lo = rho_hi;
+        dum3_lo = dum3_hi;
+        hadvp_lo = hadvp_hi;
       }
-      flo = fhi;
-      rho_lo = rho_hi;
-      dum3_lo = dum3_hi;
-      hadvp_lo = hadvp_hi;
     }
   }
   __syncthreads();
@@ -812,7 +993,7 @@ __device__ __forceinline__ void column2_body(const SoundArgs& a, int n) {
 #pragma unroll
       for (int u = 0; u < B; ++u)
         if (k0 + u <= ke) {
-          wt[u] = wten[(int64_t)(k0 + u) * wst + wc];
+          wt[u] = PIPE ? Lwt[at(k0 + u)] : wten[(int64_t)(k0 + u) * wst + wc];
           wo[u] = Ldd[at(k0 + u)];
         }
 #pragma unroll
@@ -877,9 +1058,51 @@ __device__ __forceinline__ void column2_body(const SoundArgs& a, int n) {
       wup = wk;
     }
   }
+  if constexpr (PIPE) {
+    // while lane group 0 runs the recurrence, the other level slots add the
+    // u and v faces of every column of the tile into their time averages
+    if (inside && slot >= 1) {
+      const int ch2 = (nk + C2S - 2) / (C2S - 1);
+      const int kb2 = 1 + (slot - 1) * ch2, ke2 = min(nk, kb2 + ch2 - 1);
+      average_uv_faces(a, kb2, ke2, ust, uc, vst, vc, last, tavg);
+    }
+  }
   __syncthreads();
   // ---------------- phase C: pressure update, filter, time averages
-  if (phys) {
+  if (PIPE && phys) {
+    // the rrw words of a batch of levels are requested before any of them is
+    // updated (rrw is written by no other statement of this phase)
+    const int pj = j + 1, pi = i + 1;
+    const float wlid = FLAT ? w[(int64_t)(nk + 1) * wst + wc] : 0.0f;
+    constexpr int B = 8;
+    for (int k0 = kb; k0 <= ke; k0 += B) {
+      float rw[B];
+#pragma unroll
+      for (int u = 0; u < B; ++u)
+        if (k0 + u <= ke) rw[u] = a.rrw[(int64_t)(k0 + u) * wst + wc];
+#pragma unroll
+      for (int u = 0; u < B; ++u)
+        if (k0 + u <= ke) {
+          const int k = k0 + u;
+          const int64_t p = pix(k, pj, pi, ni, nj);
+          const float wk = (k == 1) ? lw1[c] : Ldd[at(k)];
+          const float wup = (k == nk) ? wlid : Ldd[at(k + 1)];
+          if (k >= 2) w[(int64_t)k * wst + wc] = wk;
+          const float w1 = (k == 1 && !FLAT) ? 0.0f : wk;
+          const float w2 = (k == nk && !FLAT) ? 0.0f : wup;
+          const float old = Lpp[at(k)];
+          const float vt = w1 * Ls1[at(k)] + w2 * Ls2[at(k)];
+          float up = (old + Lfpk[at(k)]) + a.alph * vt;
+          if (fabsf(up) < smeps) up = 0.0f;
+          a.pp3d[p] = up;
+          const float tend = a.kdiv * (up - old);
+          a.ppd[p] = up + tend;
+          if (a.nrk == 3 && last) a.ppx[p] = tend;
+          const float vw = rw[u] + (last ? wk * a.alph : wk);
+          a.rrw[(int64_t)k * wst + wc] = last ? vw * tavg : vw;
+        }
+    }
+  } else if (phys) {
     const int pj = j + 1, pi = i + 1;
     for (int k = kb; k <= ke; ++k) {
       const int64_t p = pix(k, pj, pi, ni, nj);
@@ -908,39 +1131,14 @@ __device__ __forceinline__ void column2_body(const SoundArgs& a, int n) {
       a.rrw[pw] = last ? vw * tavg : vw;
     }
   }
-  if (inside && !phys) {  // faces of the east column and north row
-    float* __restrict__ rru = a.rru;
-    float* __restrict__ rrv = a.rrv;
-    const float* __restrict__ u3 = a.u3d;
-    const float* __restrict__ v3 = a.v3d;
-    constexpr int B = 4;
-    for (int k0 = kb; k0 <= ke; k0 += B) {
-      float ru[B], uu[B], rv[B], vv[B];
-#pragma unroll
-      for (int u = 0; u < B; ++u)
-        if (k0 + u <= ke) {
-          const int64_t pu = (int64_t)(k0 + u) * ust + uc, pv = (int64_t)(k0 + u) * vst + vc;
-          ru[u] = rru[pu];
-          uu[u] = u3[pu];
-          rv[u] = rrv[pv];
-          vv[u] = v3[pv];
-        }
-#pragma unroll
-      for (int u = 0; u < B; ++u)
-        if (k0 + u <= ke) {
-          const int64_t pu = (int64_t)(k0 + u) * ust + uc, pv = (int64_t)(k0 + u) * vst + vc;
-          const float au = ru[u] + uu[u];
-          rru[pu] = last ? au * tavg : au;
-          const float av = rv[u] + vv[u];
-          rrv[pv] = last ? av * tavg : av;
-        }
-    }
-  }
+  if (!PIPE && inside && !phys)  // faces of the east column and north row
+    average_uv_faces(a, kb, ke, ust, uc, vst, vc, last, tavg);
 }
 
-template <int C2W, int C2T, bool FLAT>
+template <int C2W, int C2T, bool FLAT, bool PIPE>
 __global__ __launch_bounds__(C2T) void k_snd_column2(SoundArgs a, int n) {
-  column2_body<C2W, C2T, FLAT>(a, n);
+  const int ntiles = ((a.ni + 1 + C2W - 1) / C2W) * (a.nj + 1);
+  column2_body<C2W, C2T, FLAT, PIPE>(a, n, xcd_tile((int)blockIdx.x, ntiles));
 }
 
 // ------------------------------------------------------------------ radiation
@@ -988,6 +1186,31 @@ __global__ __launch_bounds__(64) void k_snd_radiation(SoundArgs a, float dx, flo
 
 }  // namespace
 
+// LDS a workgroup may allocate on the current device (160 KiB on gfx950).
+static int lds_limit() {
+  static int cached[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 65536;
+  if (!cached[dev]) {
+    int v = 0;
+    cached[dev] = (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+                      ? v : 65536;
+  }
+  return cached[dev];
+}
+
+template <bool FLAT>
+static void launch_column2(bool narrow, bool pipe, const SoundArgs& a, int n, hipStream_t s) {
+  const int ni = a.ni, nj = a.nj, nk = a.nk;
+  const int cw = narrow ? 16 : 32;
+  const dim3 grid(((ni + cw - 1 + 1) / cw) * (nj + 1));
+  const int lds = column2_lds_bytes(nk, cw, pipe);
+  if (narrow && pipe) hipLaunchKernelGGL((k_snd_column2<16, 256, FLAT, true>), grid, dim3(256), lds, s, a, n);
+  else if (narrow) hipLaunchKernelGGL((k_snd_column2<16, 256, FLAT, false>), grid, dim3(256), lds, s, a, n);
+  else if (pipe) hipLaunchKernelGGL((k_snd_column2<32, 512, FLAT, true>), grid, dim3(512), lds, s, a, n);
+  else hipLaunchKernelGGL((k_snd_column2<32, 512, FLAT, false>), grid, dim3(512), lds, s, a, n);
+}
+
 static int psolver3_stage(
     float* ppi, float* pp3d, float* ppx, float* ppd, const float* piadv, const float* ppterm, const float* ppten,
     const float* thv, float* qk, float* pk1, float* pk2, float* mm, float* fpk, float* elim_c, float* elim_d,
@@ -1020,10 +1243,14 @@ static int psolver3_stage(
                         side_of(sides, 3) == 2u;
   const int edge = 2 * (ni + nj);
   // the LDS-staged column pass unless the caller asks for the register one
-  // (flags & 4) or the column does not fit the 64 KiB workgroup LDS window
+  // (flags & 4) or the column does not fit the workgroup LDS limit
   const bool narrow = (flags & 8) != 0;
+  // flags & 16: the staged pass with its loads issued a level (phase A) or a
+  // batch of levels (phase C) ahead, wten staged through LDS with the other
+  // phase-A words, and the u/v time averages run beside the recurrence
+  const bool pipe = (flags & 16) != 0;
   const int cw = narrow ? 16 : 32;
-  const bool staged = !(flags & 4) && column2_lds_bytes(nk, cw) + (int)(cw * sizeof(float)) <= 65536;
+  const bool staged = !(flags & 4) && column2_lds_bytes(nk, cw, pipe) + (int)(cw * sizeof(float)) <= lds_limit();
   if (flat) {
     // the flat column runs the staged pass only (k_snd_column is the terrain text)
     if (!staged || narrow) return (int)hipErrorInvalidValue;
@@ -1034,9 +1261,7 @@ static int psolver3_stage(
                 hipLaunchKernelGGL(k_snd_pgrad_flat,
                                    dim3(((ni + 1 + PG_TX - 1) / PG_TX) * ((nj + 1 + PG_TY - 1) / PG_TY) * nk),
                                    dim3(PG_TX * PG_TY), 0, s, a));
-      GM1_TIMED(TK_ACOUSTIC_COLUMN, s,
-                hipLaunchKernelGGL((k_snd_column2<32, 512, true>), dim3(((ni + 1 + 31) / 32) * (nj + 1)), dim3(512),
-                                   column2_lds_bytes(nk, 32), s, a, n));
+      GM1_TIMED(TK_ACOUSTIC_COLUMN, s, launch_column2<true>(false, pipe, a, n, s));
       GM1_LAUNCH_CHECK();
       if (n < nloop) {
         int rc = fill_pressure(ppd, sides, ni, nj, nk, s);
@@ -1052,14 +1277,8 @@ static int psolver3_stage(
     GM1_TIMED(TK_ACOUSTIC_PGRAD, s,
               hipLaunchKernelGGL(k_snd_pgrad, dim3(((ni + 1 + PG_TX - 1) / PG_TX) * ((nj + 1 + PG_TY - 1) / PG_TY) * nk),
                                  dim3(PG_TX * PG_TY), 0, s, a));
-    if (staged && narrow)
-      GM1_TIMED(TK_ACOUSTIC_COLUMN, s,
-                hipLaunchKernelGGL((k_snd_column2<16, 256, false>), dim3(((ni + 1 + 15) / 16) * (nj + 1)), dim3(256),
-                                   column2_lds_bytes(nk, 16), s, a, n));
-    else if (staged)
-      GM1_TIMED(TK_ACOUSTIC_COLUMN, s,
-                hipLaunchKernelGGL((k_snd_column2<32, 512, false>), dim3(((ni + 1 + 31) / 32) * (nj + 1)), dim3(512),
-                                   column2_lds_bytes(nk, 32), s, a, n));
+    if (staged)
+      GM1_TIMED(TK_ACOUSTIC_COLUMN, s, launch_column2<false>(narrow, pipe, a, n, s));
     else
       GM1_TIMED(TK_ACOUSTIC_COLUMN, s,
                 hipLaunchKernelGGL(k_snd_column, dim3((ni + 1 + 63) / 64, nj + 1), dim3(64), 0, s, a, n));

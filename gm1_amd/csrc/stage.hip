@@ -78,30 +78,84 @@ GM1_DEV float bfw(const BuoyArgs& a, int kf, int j, int i) {
   return a.c1[p] * d8(a, kf - 1, j, i) + a.c2[p] * d8(a, kf, j, i);
 }
 
-__global__ __launch_bounds__(256) void k_buoyancy(BuoyArgs a) {
+// wten(k) += b(k) at w levels 2..nk, and the terrain-horizontal parts on the
+// interior u and v faces.  Each thread marches one column through a chunk of
+// BUOY_KC levels, carrying dum8 and the w-level buoyancy of its own cell and
+// of its west and south neighbours in registers: b at w level k+1 is formed
+// once and reused as level k's on the next iteration, so a cell's b is formed
+// once per chunk instead of six times.
+constexpr int BUOY_KC = 8;
+
+__global__ __launch_bounds__(64) void k_buoyancy_col(BuoyArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z + 1;
+  const int j = blockIdx.y;
   const int ni = a.ni, nj = a.nj, nk = a.nk;
-  if (i >= ni) return;
-  if (k >= 2) {
-    const int64_t d = ix3(k, j + NG, i + NG, sy(nj), sx(ni));
-    a.wten[d] = a.wten[d] + bfw(a, k, j, i);
+  const int k0 = (int)blockIdx.z * BUOY_KC + 1;
+  if (i >= ni || k0 > nk) return;
+  const int k1 = min(k0 + BUOY_KC - 1, nk);
+  const bool fu = i >= 1, fv = j >= 1;
+  const float* __restrict__ c1 = a.c1;
+  const float* __restrict__ c2 = a.c2;
+  const int64_t plane = (int64_t)nj * ni;
+  const int64_t cp = (int64_t)j * ni + i;       // c1/c2 column
+  const int64_t wp = fu ? cp - 1 : cp;          // west neighbour (unused when !fu)
+  const int64_t sp = fv ? cp - ni : cp;         // south neighbour (unused when !fv)
+  const int64_t sst = (int64_t)sy(nj) * sx(ni);
+  const float* __restrict__ dm = a.dum8;
+  const int64_t dc = ix3(0, j + NG, i + NG, sy(nj), sx(ni));
+  const int64_t dw = fu ? dc - 1 : dc;
+  const int64_t ds = fv ? dc - sx(ni) : dc;
+  // w level k0: zero at the surface (k0 == 1), else b(k0) from dum8(k0-1), dum8(k0).
+  float cur_c = dm[dc + k0 * sst], cur_w = dm[dw + k0 * sst], cur_s = dm[ds + k0 * sst];
+  float lo_c = 0.0f, lo_w = 0.0f, lo_s = 0.0f;
+  if (k0 > 1) {
+    const int64_t q = (int64_t)(k0 - 1) * plane;
+    const int64_t m = (int64_t)(k0 - 1) * sst;
+    lo_c = c1[q + cp] * dm[dc + m] + c2[q + cp] * cur_c;
+    lo_w = c1[q + wp] * dm[dw + m] + c2[q + wp] * cur_w;
+    lo_s = c1[q + sp] * dm[ds + m] + c2[q + sp] * cur_s;
   }
-  if (i >= 1) {  // u face i between cells i-1 and i
-    float t = (bfw(a, k + 1, j, i) + bfw(a, k + 1, j, i - 1)) + (bfw(a, k, j, i) + bfw(a, k, j, i - 1));
-    t = 0.125f * t;
-    t = t - 0.25f * (d8(a, k, j, i) + d8(a, k, j, i - 1));
-    t = t * (a.gxu[((int64_t)(k - 1) * nj + j) * (ni + 1) + i] + a.gxu[((int64_t)k * nj + j) * (ni + 1) + i]);
-    const int64_t d = ix3(k, j + NG, i + NG, sy(nj), ux(ni));
-    a.uten[d] = a.uten[d] + t;
-  }
-  if (j >= 1) {  // v face j between cells j-1 and j
-    float t = (bfw(a, k + 1, j, i) + bfw(a, k + 1, j - 1, i)) + (bfw(a, k, j, i) + bfw(a, k, j - 1, i));
-    t = 0.125f * t;
-    t = t - 0.25f * (d8(a, k, j, i) + d8(a, k, j - 1, i));
-    t = t * (a.gyv[((int64_t)(k - 1) * (nj + 1) + j) * ni + i] + a.gyv[((int64_t)k * (nj + 1) + j) * ni + i]);
-    const int64_t d = ix3(k, j + NG, i + NG, vy(nj), sx(ni));
-    a.vten[d] = a.vten[d] + t;
+#pragma unroll
+  for (int kk = 0; kk < BUOY_KC; ++kk) {
+    const int k = k0 + kk;
+    if (k > k1) break;
+    float hi_c = 0.0f, hi_w = 0.0f, hi_s = 0.0f, nxt_c = 0.0f, nxt_w = 0.0f, nxt_s = 0.0f;
+    if (k < nk) {  // w level k+1 is interior; level nk+1 (the lid) carries no buoyancy
+      const int64_t q = (int64_t)k * plane;
+      const int64_t m = (int64_t)(k + 1) * sst;
+      nxt_c = dm[dc + m];
+      nxt_w = dm[dw + m];
+      nxt_s = dm[ds + m];
+      hi_c = c1[q + cp] * cur_c + c2[q + cp] * nxt_c;
+      hi_w = c1[q + wp] * cur_w + c2[q + wp] * nxt_w;
+      hi_s = c1[q + sp] * cur_s + c2[q + sp] * nxt_s;
+    }
+    if (k >= 2) {
+      const int64_t d = ix3(k, j + NG, i + NG, sy(nj), sx(ni));
+      a.wten[d] = a.wten[d] + lo_c;
+    }
+    if (fu) {
+      float t = (hi_c + hi_w) + (lo_c + lo_w);
+      t = 0.125f * t;
+      t = t - 0.25f * (cur_c + cur_w);
+      t = t * (a.gxu[((int64_t)(k - 1) * nj + j) * (ni + 1) + i] + a.gxu[((int64_t)k * nj + j) * (ni + 1) + i]);
+      const int64_t d = ix3(k, j + NG, i + NG, sy(nj), ux(ni));
+      a.uten[d] = a.uten[d] + t;
+    }
+    if (fv) {
+      float t = (hi_c + hi_s) + (lo_c + lo_s);
+      t = 0.125f * t;
+      t = t - 0.25f * (cur_c + cur_s);
+      t = t * (a.gyv[((int64_t)(k - 1) * (nj + 1) + j) * ni + i] + a.gyv[((int64_t)k * (nj + 1) + j) * ni + i]);
+      const int64_t d = ix3(k, j + NG, i + NG, vy(nj), sx(ni));
+      a.vten[d] = a.vten[d] + t;
+    }
+    lo_c = hi_c;
+    lo_w = hi_w;
+    lo_s = hi_s;
+    cur_c = nxt_c;
+    cur_w = nxt_w;
+    cur_s = nxt_s;
   }
 }
 
@@ -421,7 +475,8 @@ int gm1hip_stage_terms(const float* th0, const float* qv0, const float* pi0, con
 int gm1hip_buoyancy(const float* dum8, const float* c1, const float* c2, const float* gxu, const float* gyv,
                     float* uten, float* vten, float* wten, int ni, int nj, int nk, void* stream) {
   BuoyArgs a{dum8, c1, c2, gxu, gyv, uten, vten, wten, ni, nj, nk};
-  hipLaunchKernelGGL(k_buoyancy, dim3((ni + 63) / 64, nj, nk), dim3(64), 0, (hipStream_t)stream, a);
+  hipLaunchKernelGGL(k_buoyancy_col, dim3((ni + 63) / 64, nj, (nk + BUOY_KC - 1) / BUOY_KC), dim3(64), 0,
+                     (hipStream_t)stream, a);
   GM1_LAUNCH_CHECK();
   return 0;
 }

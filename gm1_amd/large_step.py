@@ -272,7 +272,8 @@ def _flat_rk_stage(st: Gm1LargeStepState, nrk: int) -> None:
                                               divx=a["divx"], stream=st.stream, lateral_sides=sides)
     launch_cuda_gm1_flat_stretched_buoyancy(a, m, ni=ni, nj=nj, nk=nk, stream=st.stream)
     launch_cuda_gm1_flat_acoustic_stage(a, m, ni=ni, nj=nj, nk=nk, nrk=nrk, dt=cfg.dt, nsound=cfg.nsound, dx=cfg.dx,
-                                        dy=cfg.dy, vertical=st.flat, stream=st.stream, lateral_sides=sides)
+                                        dy=cfg.dy, vertical=st.flat, stream=st.stream, lateral_sides=sides,
+                                        column_pass=st.column_pass)
     launch_cuda_gm1_child_stretched_post_acoustic(a, m, st.flat, ni=ni, nj=nj, nk=nk, dx=cfg.dx, dy=cfg.dy,
                                                   thterm=a["thterm"], divergence=a["divergence"],
                                                   theta_divergence_term=a["theta_div"], divx=a["divx"],

@@ -98,6 +98,13 @@ class CudaPsolver3TerrainOpenLaunch:
     receipt: dict[str, Any]
 
 
+# column pass name -> the stage call's flag bits: "staged" is 32 columns x 512
+# threads per workgroup, "staged16" 16 x 256; the "pipelined" forms issue each
+# level's loads a level ahead and run the u/v time averages beside the
+# recurrence; "register" walks each column in one thread
+COLUMN_PASSES = {"staged": 0, "pipelined": 16, "staged16": 8, "pipelined16": 24, "register": 4}
+
+
 def _check_ablations(ablations: TerrainOpenSoundAblations) -> int:
     if not isinstance(ablations, TerrainOpenSoundAblations):
         raise TypeError("ablations must be TerrainOpenSoundAblations")
@@ -121,17 +128,18 @@ def launch_cuda_psolver3_terrain_open_three_stage(
     """Enqueue the acoustic RK stages of ``_nrk_sequence`` on ``stream``.
 
     ``column_pass`` selects the vertical pass: "staged" (LDS-staged, k-parallel
-    outside the Thomas recurrence, 32 columns per workgroup), "staged16" (16
-    columns) or "register" (one thread walks each column); all produce the
-    same words.
+    outside the Thomas recurrence, 32 columns x 512 threads per workgroup),
+    "staged16" (16 x 256), "pipelined"/"pipelined16" (the same with loads
+    issued a level ahead) or "register" (one thread walks each column); all
+    produce the same words.
     """
-    if column_pass not in ("staged", "staged16", "register"):
-        raise ValueError("column_pass must be 'staged', 'staged16' or 'register'")
+    if column_pass not in COLUMN_PASSES:
+        raise ValueError(f"column_pass must be one of {COLUMN_PASSES}")
     for name, value in (("alph", alph), ("kdiv", kdiv)):
         if value is not None and not (0.0 <= float(value) <= 1.0):
             raise ValueError(f"{name}={value} is outside CM1's admitted [0, 1]")
     dt, dx, dy, dz = (check_spacing(n, v) for n, v in (("dt", dt), ("dx", dx), ("dy", dy), ("dz", dz)))
-    flags = _check_ablations(ablations or TerrainOpenSoundAblations()) | {"staged": 0, "register": 4, "staged16": 8}[column_pass]
+    flags = _check_ablations(ablations or TerrainOpenSoundAblations()) | COLUMN_PASSES[column_pass]
     sides = lateral_code(lateral_sides)
     shapes = terrain_open_sound_shapes(ni, nj, nk)
     a = {}

@@ -30,8 +30,8 @@ from ._device import NG, lateral_code, ptr, record, require, shapes, stream_hand
 from ._native import D, Fl, I, P, U, signature
 from .errors import refuse
 from .grid import VerticalLevels, rk_stage_dt_f32, sound3_stage_schedule_f32
-from .psolver3 import (CM1_ALPH_F32, CM1_KDIV_F32, CudaPsolver3TerrainOpenLaunch, TerrainOpenSoundAblations,
-                       device_profiles, terrain_open_sound_shapes)
+from .psolver3 import (CM1_ALPH_F32, CM1_KDIV_F32, COLUMN_PASSES, CudaPsolver3TerrainOpenLaunch,
+                       TerrainOpenSoundAblations, device_profiles, terrain_open_sound_shapes)
 from .scalar_moist_rk import CM1_NRKMAX, moisture_weps_f64, scalar_moist_rk_stage_timestep_f32
 
 signature("gm1hip_momentum_advection5_flat", *([P] * 18), I, I, I, Fl, Fl, Fl, U, P)
@@ -189,7 +189,7 @@ def launch_cuda_gm1_flat_acoustic_stage(
     arrays: Mapping[str, Any], metrics: Mapping[str, Any], *, ni: int, nj: int, nk: int, nrk: int, dt: float,
     nsound: int, dx: float, dy: float, vertical: FlatStretchedColumn, ablations: TerrainOpenSoundAblations | None = None,
     stream: Any | None = None, lateral_sides: tuple[int, int, int, int] = (2, 2, 2, 2), alph: float | None = None,
-    kdiv: float | None = None, lateral: Any | None = None, **_: Any,
+    kdiv: float | None = None, lateral: Any | None = None, column_pass: str = "staged", **_: Any,
 ) -> CudaPsolver3TerrainOpenLaunch:
     """One acoustic RK stage on a flat stretched column (sound.F with terrain_flag=.false.).
 
@@ -211,6 +211,9 @@ def launch_cuda_gm1_flat_acoustic_stage(
                    reason="the gfx950 stage always runs every small-step operator of sound.F",
                    remedy="keep the small-step operators enabled")
     flags = (1 if abl.radiation_history_production else 0) | (2 if abl.stage_family_fills else 0)
+    if column_pass not in ("staged", "pipelined"):
+        raise ValueError(f"the flat column runs the 32-column staged pass ('staged' or 'pipelined'), not {column_pass!r}")
+    flags |= COLUMN_PASSES[column_pass]
     shp = terrain_open_sound_shapes(ni, nj, nk)
     names = ("ppi", "pp3d", "ppx", "ppd", "piadv", "ppterm", "ppten", "thv", "qk", "pk1", "pk2", "mm", "fpk",
              "elim_c", "elim_d", "ua", "u3d", "uten", "rru", "va", "v3d", "vten", "rrv", "wa", "w3d", "wten", "rrw",

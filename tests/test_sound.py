@@ -228,14 +228,16 @@ def test_gpu_acoustic_stage_bit_exact(device, sides, dims):
 @pytest.mark.gpu
 @pytest.mark.parametrize("sides", [(2, 2, 2, 2), (1, 1, 1, 1)])
 def test_gpu_staged_and_register_column_passes_agree(device, sides):
-    """The LDS-staged column passes (32 and 16 columns per workgroup) and the
+    """The LDS-staged column passes (every workgroup shape) and the
     one-thread-per-column pass give the same words."""
     ni, nj, nk = 67, 33, 40
     a, m = _acoustic_case(np.random.default_rng(17), ni, nj, nk, terrain=True)
     got = {}
-    for mode in ("staged", "staged16", "register"):
+    for mode in ("staged", "pipelined", "staged16", "pipelined16", "register"):
         got[mode] = _gpu_stage(device, {k: v.copy() for k, v in a.items()}, m, ni=ni, nj=nj, nk=nk, dt=2.0, ns=8,
                                dx=250.0, dy=200.0, dz=150.0, sides=sides, column_pass=mode)
     for k in OUT:
         assert_bits_equal(got["staged"][k], got["register"][k], f"{k} sides={sides}")
         assert_bits_equal(got["staged16"][k], got["register"][k], f"{k} sides={sides} (16 columns)")
+        for mode in ("pipelined", "pipelined16"):
+            assert_bits_equal(got[mode][k], got["register"][k], f"{k} sides={sides} ({mode})")

@@ -290,8 +290,9 @@ def test_gpu_flat_buoyancy_post_acoustic_and_acoustic_stage_match_oracle(device)
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("sides", [(2, 2, 2, 2), (1, 1, 1, 1)])
-def test_gpu_stretched_moist_large_step_bit_exact(device, sides):
+@pytest.mark.parametrize("sides,column_pass", [((2, 2, 2, 2), "staged"), ((1, 1, 1, 1), "staged"),
+                                               ((2, 2, 2, 2), "pipelined")])
+def test_gpu_stretched_moist_large_step_bit_exact(device, sides, column_pass):
     """Two moist large steps on a stretched column against the oracle (device dry-mass correction injected)."""
     import torch
 
@@ -300,10 +301,11 @@ def test_gpu_stretched_moist_large_step_bit_exact(device, sides):
     cfg = stretched_case(24, 20, 16, sides)
     host = host_initial_state(cfg)
     st = create_gm1_large_step_state(host, device)
+    st.column_pass = column_pass
     assert st.flat is not None
     a = {k: v.copy() for k, v in host["arrays"].items()}
     lv = host["levels"]
-    print(f"stretched moist large step {cfg.ni}x{cfg.nj}x{cfg.nk} sides {sides}", flush=True)
+    print(f"stretched moist large step {cfg.ni}x{cfg.nj}x{cfg.nk} sides {sides} {column_pass}", flush=True)
     for step in range(2):
         launch_cuda_gm1_large_step(st)
         torch.cuda.synchronize()
