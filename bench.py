@@ -70,7 +70,7 @@ def parse_args(argv=None):
                    help="time replays of the step captured into a HIP graph (the kernel timer then runs in an "
                         "eager pass of the same length)")
     p.add_argument("--column-pass", choices=("staged", "pipelined", "staged16", "pipelined16", "register"),
-                   default="staged",
+                   default="pipelined",
                    help="the acoustic column pass (all give the same words)")
     p.add_argument("--calibration-copy-mib", type=int, default=0,
                    help="issue one plain device copy of this size before the steps (HBM counter calibration)")

@@ -138,13 +138,29 @@ __device__ __forceinline__ float yflux(const float* __restrict__ s, const float*
                                        col[(y - 2) * st], weps);
 }
 
+// x- and y-face fluxes from a level plane staged in LDS: lp[r][c] holds the
+// padded scalar at row j0+NG-3+r, column i0+NG-3+c of the tile's level.
+constexpr int LPX = TX + 7;  // TX + 6 halo columns, +1 against bank aliasing
+template <bool WENO, bool GATE>
+__device__ __forceinline__ float xflux_lds(const float (*lp)[LPX], float m, int r, int t, double weps) {
+  const float* row = lp[r + 3] + t;  // row[3] is the cell east of face t
+  return (m >= 0.0f) ? recon_gated<WENO, GATE>(m, row[0], row[1], row[2], row[3], row[4], weps)
+                     : recon_gated<WENO, GATE>(m, row[5], row[4], row[3], row[2], row[1], weps);
+}
+
+template <bool WENO, bool GATE>
+__device__ __forceinline__ float yflux_lds(const float (*lp)[LPX], float m, int r, int t, double weps) {
+  const int c = t + 3;  // lp[r + 3][c] is the cell north of face r
+  return (m >= 0.0f) ? recon_gated<WENO, GATE>(m, lp[r][c], lp[r + 1][c], lp[r + 2][c], lp[r + 3][c], lp[r + 4][c],
+                                               weps)
+                     : recon_gated<WENO, GATE>(m, lp[r + 5][c], lp[r + 4][c], lp[r + 3][c], lp[r + 2][c],
+                                               lp[r + 1][c], weps);
+}
+
 // z-face flux at face k (2..nk) of one column with the lower-order closures.
-template <bool WENO>
-__device__ __forceinline__ float zflux(const float* __restrict__ col, int64_t st,
-                                       const float* __restrict__ rrw, int64_t wp,
-                                       float cc1, float cc2, int k, int nk, double weps) {
-  const float m = rrw[wp];
-  auto S = [&](int level) { return col[(int64_t)level * st]; };
+// S(level) reads the column's scalar at a level.
+template <bool WENO, class Get>
+__device__ __forceinline__ float zflux_g(Get S, float m, float cc1, float cc2, int k, int nk, double weps) {
   const float wf = (float)weps;
   float r;
   if (k == 2) {
@@ -167,14 +183,19 @@ __device__ __forceinline__ float zflux(const float* __restrict__ col, int64_t st
   return m * r;
 }
 
+template <bool WENO>
+__device__ __forceinline__ float zflux(const float* __restrict__ col, int64_t st,
+                                       const float* __restrict__ rrw, int64_t wp,
+                                       float cc1, float cc2, int k, int nk, double weps) {
+  return zflux_g<WENO>([&](int level) { return col[(int64_t)level * st]; }, rrw[wp], cc1, cc2, k, nk, weps);
+}
+
 // z-face flux of the positive-definite chain (cm1_rk_pdef_vertical_flux_v1):
 // the lower closures of vadv_weno5 / vadv_lwr_weno_35(_b) and, with
 // pdefweno, the magnitude gate (a gated face stores +0, never m*0).
-__device__ __forceinline__ float zflux_pdef(const float* __restrict__ col, int64_t st,
-                                            const float* __restrict__ rrw, int64_t wp, float cc1, float cc2,
-                                            int k, int nk, double weps, bool pdefweno) {
-  const float m = rrw[wp];
-  auto S = [&](int level) { return col[(int64_t)level * st]; };
+template <class Get>
+__device__ __forceinline__ float zflux_pdef_g(Get S, float m, float cc1, float cc2, int k, int nk, double weps,
+                                              bool pdefweno) {
   const float wf = (float)weps;
   float r;
   if (k == 2) {
@@ -194,6 +215,13 @@ __device__ __forceinline__ float zflux_pdef(const float* __restrict__ col, int64
     }
   }
   return m * r;
+}
+
+__device__ __forceinline__ float zflux_pdef(const float* __restrict__ col, int64_t st,
+                                            const float* __restrict__ rrw, int64_t wp, float cc1, float cc2,
+                                            int k, int nk, double weps, bool pdefweno) {
+  return zflux_pdef_g([&](int level) { return col[(int64_t)level * st]; }, rrw[wp], cc1, cc2, k, nk, weps,
+                      pdefweno);
 }
 
 // Output modes of the fused tile kernel.
@@ -242,6 +270,7 @@ __global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
   constexpr bool GATE = (MODE == MODE_PDEF);
   __shared__ float lfx[TY][TX + 1];
   __shared__ float lfy[TY + 1][TX];
+  __shared__ float lp[TY + 6][LPX];  // the tile's level plane with a 3-cell halo
   const int ni = a.ni, nj = a.nj, nk = a.nk;
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   // tiles ordered i fastest, then j, then column chunk; each XCD sweeps a
@@ -263,22 +292,46 @@ __global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
   const int64_t mcol = (int64_t)j * ni + i;
   const int64_t mst = (int64_t)nj * ni;
 
-  // z-face flux at the bottom face of the first level of the chunk.
-  auto zf = [&](int kf) {
+  // The column's scalar S(k-3..k+3) at level k rides in registers: win[q]
+  // holds level k-3+q (0 outside the padded column; those words are never
+  // read by a face's stencil, which stays within levels 1..nk).
+  float win[7];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    const int lev = kb - 3 + q;
+    win[q] = (live && lev >= 0 && lev <= nk + 1) ? col[(int64_t)lev * st] : 0.0f;
+  }
+  // z-face flux at face kf from the window b, b[0] = S(kf-3).
+  auto zf = [&](int kf, const float* b) {
     const int64_t wp = (int64_t)kf * st + wcol;
     const float cc1 = a.c1[(int64_t)(kf - 1) * mst + mcol], cc2 = a.c2[(int64_t)(kf - 1) * mst + mcol];
-    if (MODE == MODE_PDEF) return zflux_pdef(col, st, a.rrw, wp, cc1, cc2, kf, nk, a.weps, a.pdefweno != 0);
-    return zflux<WENO>(col, st, a.rrw, wp, cc1, cc2, kf, nk, a.weps);
+    auto S = [&](int level) { return b[level - (kf - 3)]; };
+    if (MODE == MODE_PDEF) return zflux_pdef_g(S, a.rrw[wp], cc1, cc2, kf, nk, a.weps, a.pdefweno != 0);
+    return zflux_g<WENO>(S, a.rrw[wp], cc1, cc2, kf, nk, a.weps);
   };
   float fzb = 0.0f;
-  if (live && kb >= 2) fzb = zf(kb);
+  if (live && kb >= 2) fzb = zf(kb, win);  // S(kb-3..)
   for (int k = kb; k <= ke; ++k) {
+    const int lnext = k + 4;  // the level entering the window after this one
+    const float snext = (live && lnext <= nk + 1) ? col[(int64_t)lnext * st] : 0.0f;
+    // the level plane: rows j0-3..j0+TY+2, columns i0-3..i0+TX+2 (padded
+    // indices j0..j0+TY+5, i0..i0+TX+5), clipped to the padded array
+    {
+      const float* plane = a.s + (int64_t)k * st;
+      for (int slot = threadIdx.x; slot < (TY + 6) * (TX + 6); slot += NT) {
+        const int r = slot / (TX + 6), c = slot % (TX + 6);
+        const int yy = j0 + r, xx = i0 + c;
+        if (yy < nys && xx < nxs) lp[r][c] = plane[(int64_t)yy * nxs + xx];
+      }
+    }
+    __syncthreads();
     // x-face fluxes: faces i0..i0+TX of the TY rows.
     for (int slot = threadIdx.x; slot < TY * (TX + 1); slot += NT) {
       const int r = slot / (TX + 1), t = slot % (TX + 1);
       const int f = i0 + t, jj = j0 + r;
       if (f <= ni && jj < nj) {
-        float v = xflux<WENO, GATE>(a.s, a.rru, k, jj + NG, f + NG, nys, nxs, nxu, a.weps);
+        const float m = a.rru[ix3(k, jj + NG, f + NG, nys, nxu)];
+        float v = xflux_lds<WENO, GATE>(lp, m, r, t, a.weps);
         lfx[r][t] = v;
         if (a.fx && (t < TX || f == ni)) a.fx[ix3(k, jj + NG, f + NG, nys, nxu)] = v;
       }
@@ -288,13 +341,14 @@ __global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
       const int r = slot / TX, t = slot % TX;
       const int ii = i0 + t, g = j0 + r;
       if (ii < ni && g <= nj) {
-        float v = yflux<WENO, GATE>(a.s, a.rrv, k, g + NG, ii + NG, nys, nxs, nyv, a.weps);
+        const float m = a.rrv[ix3(k, g + NG, ii + NG, nyv, nxs)];
+        float v = yflux_lds<WENO, GATE>(lp, m, r, t, a.weps);
         lfy[r][t] = v;
         if (a.fy && (r < TY || g == nj)) a.fy[ix3(k, g + NG, ii + NG, nyv, nxs)] = v;
       }
     }
     float fzt = 0.0f;
-    if (live && k + 1 <= nk) fzt = zf(k + 1);
+    if (live && k + 1 <= nk) fzt = zf(k + 1, win + 1);  // S(k-2..)
     __syncthreads();
     if (live && a.fz) {
       if (k == 1) a.fz[1 * st + wcol] = 0.0f;
@@ -304,7 +358,7 @@ __global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
     if (owned) {
       float fw = lfx[ty][tx], fe = lfx[ty][tx + 1];
       float fs = lfy[ty][tx], fn = lfy[ty + 1][tx];
-      const float sv = col[(int64_t)k * st];
+      const float sv = lp[ty + 3][tx + 3];
       float ax, ay, az;
       if (obc != 0u) {
         const float uw = a.rru[ix3(k, y, x, nys, nxu)], ue = a.rru[ix3(k, y, x + 1, nys, nxu)];
@@ -355,6 +409,9 @@ __global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
       }
     }
     fzb = fzt;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) win[q] = win[q + 1];
+    win[6] = snext;
     __syncthreads();
   }
 }

@@ -1248,9 +1248,11 @@ static int psolver3_stage(
   // flags & 16: the staged pass with its loads issued a level (phase A) or a
   // batch of levels (phase C) ahead, wten staged through LDS with the other
   // phase-A words, and the u/v time averages run beside the recurrence
-  const bool pipe = (flags & 16) != 0;
   const int cw = narrow ? 16 : 32;
-  const bool staged = !(flags & 4) && column2_lds_bytes(nk, cw, pipe) + (int)(cw * sizeof(float)) <= lds_limit();
+  const int lds_max = lds_limit() - (int)(cw * sizeof(float));  // less the static w(1) row
+  // (a column too tall for the seventh plane runs the six-plane staged pass)
+  const bool pipe = (flags & 16) != 0 && column2_lds_bytes(nk, cw, true) <= lds_max;
+  const bool staged = !(flags & 4) && column2_lds_bytes(nk, cw, pipe) <= lds_max;
   if (flat) {
     // the flat column runs the staged pass only (k_snd_column is the terrain text)
     if (!staged || narrow) return (int)hipErrorInvalidValue;

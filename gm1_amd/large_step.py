@@ -154,7 +154,7 @@ class Gm1LargeStepState:
     kessler_workspace: Any = None
     dry_mass: Any = None
     mass1: float | None = None
-    column_pass: str = "staged"
+    column_pass: str = "pipelined"
     receipts: list = field(default_factory=list)
     # a stretched (flat) column runs the terrain_flag=.false. family
     flat: Any = None

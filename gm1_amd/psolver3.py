@@ -123,7 +123,7 @@ def launch_cuda_psolver3_terrain_open_three_stage(
     ablations: TerrainOpenSoundAblations | None = None, stream: Any | None = None, compiled: Any | None = None,
     lateral_sides: tuple[int, int, int, int] = (2, 2, 2, 2), alph: float | None = None,
     kdiv: float | None = None, profiles: VerticalProfiles | None = None, _nrk_sequence: tuple[int, ...] = (1, 2, 3),
-    column_pass: str = "staged", **_: Any,
+    column_pass: str = "pipelined", **_: Any,
 ) -> CudaPsolver3TerrainOpenLaunch:
     """Enqueue the acoustic RK stages of ``_nrk_sequence`` on ``stream``.
 

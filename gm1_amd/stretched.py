@@ -189,7 +189,7 @@ def launch_cuda_gm1_flat_acoustic_stage(
     arrays: Mapping[str, Any], metrics: Mapping[str, Any], *, ni: int, nj: int, nk: int, nrk: int, dt: float,
     nsound: int, dx: float, dy: float, vertical: FlatStretchedColumn, ablations: TerrainOpenSoundAblations | None = None,
     stream: Any | None = None, lateral_sides: tuple[int, int, int, int] = (2, 2, 2, 2), alph: float | None = None,
-    kdiv: float | None = None, lateral: Any | None = None, column_pass: str = "staged", **_: Any,
+    kdiv: float | None = None, lateral: Any | None = None, column_pass: str = "pipelined", **_: Any,
 ) -> CudaPsolver3TerrainOpenLaunch:
     """One acoustic RK stage on a flat stretched column (sound.F with terrain_flag=.false.).
 
