@@ -205,7 +205,7 @@ def cpu_scalar_advection5_inplace(
         ax = np.where(ie, ax - cx, ax)
         ay = np.where(js, ay - cy, ay)
         ay = np.where(jn, ay - cy, ay)
-    az = np.empty(shape, np.float32)
+    az = np.empty(shape, F)
     if flat is not None:
         rdz, mh = F(flat.rdz), flat.mh[1 : nk + 1][:, None, None]
         az[0] = (-fz[2, ys, xs]) * rdz

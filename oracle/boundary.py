@@ -236,7 +236,7 @@ def radiation4(a, *, ni, nj, nk, dx, dy, dttmp, limit=350.0):
     def speed(foo_hi, foo_lo, qa_n, q3_n, rf_n, d32, west):
         foo = foo_hi - foo_lo
         num = (d32 * rf_n) * (qa_n - q3_n)
-        signed = np.copysign(eps, foo).astype(np.float32)
+        signed = np.copysign(eps, foo).astype(F)
         den = dt32 * (signed + foo)
         with np.errstate(all="ignore"):
             c = num / den
@@ -255,7 +255,7 @@ def radiation4(a, *, ni, nj, nk, dx, dy, dttmp, limit=350.0):
                a["rvf"][nj + off], dy32, False)
 
     def kavg(c):
-        acc = np.zeros(c.shape[1:], np.float32)
+        acc = np.zeros(c.shape[1:], F)
         for k in range(nk):
             acc = acc + c[k]
         return acc / F(nk)

@@ -8,9 +8,14 @@ scalars padded by ``ng = 3`` lateral cells (``cuda_sound3_open_boundary.py:36``
 
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
-F = np.float32
+# The restatement's arithmetic type: binary32 (the reference's), or binary64
+# when GM1_ORACLE_BINARY64=1 is set at import -- the same statements in double
+# precision, the yardstick for the binary32 rounding error of a result.
+F = np.float64 if os.environ.get("GM1_ORACLE_BINARY64") == "1" else np.float32
 NG = 3
 
 # Constant weights, each the binary64 quotient rounded once to binary32
@@ -103,13 +108,13 @@ def weno5(s1, s2, s3, s4, s5, weps: float):
     with np.errstate(all="ignore"):
         num = (w1 * p1.astype(np.float64) + w2 * p2.astype(np.float64)) + w3 * p3.astype(np.float64)
         out = num / ((w1 + w2) + w3)
-    return out.astype(np.float32)
+    return out.astype(F)
 
 
 def upstrpd(s1, s2, s3, weps: float):
     """Positive-definite lower closure (cuda_advection5.py:244-253)."""
     dd = s2 - s1
-    se = np.copysign(F(abs(F(weps))), dd).astype(np.float32)
+    se = np.copysign(F(abs(F(weps))), dd).astype(F)
     with np.errstate(all="ignore"):
         rr = (s3 - s2) / (se + dd)
     cand = C1_6 + C2_6 * rr

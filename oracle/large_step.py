@@ -41,15 +41,15 @@ KESSLER_HELD = ("qpten", "qtten", "qvten", "qcten")
 
 
 def _flux_scratch(ni, nj, nk):
-    return {"x": np.zeros((nk + 2, nj + 2 * NG, ni + 1 + 2 * NG), np.float32),
-            "y": np.zeros((nk + 2, nj + 1 + 2 * NG, ni + 2 * NG), np.float32),
-            "z": np.zeros((nk + 3, nj + 2 * NG, ni + 2 * NG), np.float32)}
+    return {"x": np.zeros((nk + 2, nj + 2 * NG, ni + 1 + 2 * NG), F),
+            "y": np.zeros((nk + 2, nj + 1 + 2 * NG, ni + 2 * NG), F),
+            "z": np.zeros((nk + 3, nj + 2 * NG, ni + 2 * NG), F)}
 
 
 def _species6(q3d):
     if q3d is None:
         return None
-    q = np.zeros((6,) + q3d.shape[1:], np.float32)
+    q = np.zeros((6,) + q3d.shape[1:], F)
     q[: q3d.shape[0]] = q3d
     return q
 
@@ -85,8 +85,8 @@ def rk_stage(a, m, base, prof, cfg, nrk, *, zt, flat=None):
         st.getdiv_flat(a, a["divergence"], a["theta_div"], a["thterm"], flat, ni=ni, nj=nj, nk=nk, dx=dx, dy=dy)
         st.getdivx_flat(a, m, a["divx"], flat, ni=ni, nj=nj, nk=nk, dx=dx, dy=dy)
     dttmp = F(F(dt) / F(4 - nrk))
-    padded = np.zeros((nk + 2, nj + 2 * NG, ni + 2 * NG), np.float32)
-    adv = np.zeros((nk, nj, ni), np.float32)
+    padded = np.zeros((nk + 2, nj + 2 * NG, ni + 2 * NG), F)
+    adv = np.zeros((nk, nj, ni), F)
     st.pack_theta(base["th0"], a["th3d"], padded, ni=ni, nj=nj, nk=nk)
     bc.fill(padded, bc.KIND_SCALAR, sides, ni=ni, nj=nj, nk=nk)
     cpu_scalar_advection5_inplace(padded, a, m, a["divx"], adv, _flux_scratch(ni, nj, nk), ni=ni, nj=nj, nk=nk,
@@ -95,12 +95,12 @@ def rk_stage(a, m, base, prof, cfg, nrk, *, zt, flat=None):
     if q3d is None:
         return
     for sp in range(q3d.shape[0]):
-        padded = np.zeros((nk + 2, nj + 2 * NG, ni + 2 * NG), np.float32)
+        padded = np.zeros((nk + 2, nj + 2 * NG, ni + 2 * NG), F)
         st.pack_scalar(q3d[sp], padded, ni=ni, nj=nj, nk=nk)
         bc.fill(padded, bc.KIND_SCALAR, sides, ni=ni, nj=nj, nk=nk)
         weps = float(F(F(1.0e-2) * F(1.0e-18)))
         if nrk < 3:
-            adv = np.zeros((nk, nj, ni), np.float32)
+            adv = np.zeros((nk, nj, ni), F)
             cpu_scalar_advection5_inplace(padded, a, m, a["divx"], adv, _flux_scratch(ni, nj, nk), ni=ni, nj=nj,
                                           nk=nk, nrk=nrk, dx=dx, dy=dy, dz=dz, weps=weps, lateral_sides=sides,
                                           flat=flat)

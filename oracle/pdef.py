@@ -42,9 +42,9 @@ def _gate(stencil, weps):
 def pdef_fluxes(s, rru, rrv, rrw, c1, c2, *, ni, nj, nk, weps, pdefweno):
     """fx (u shape), fy (v shape), fz (w shape) with only the written faces set."""
     ng = NG
-    fx = np.zeros(rru.shape, np.float32)
-    fy = np.zeros(rrv.shape, np.float32)
-    fz = np.zeros(rrw.shape, np.float32)
+    fx = np.zeros(rru.shape, F)
+    fy = np.zeros(rrv.shape, F)
+    fz = np.zeros(rrw.shape, F)
     S = s[1 : nk + 1, ng : ng + nj, :]
     x = np.arange(ni + 1) + ng
     m = rru[1 : nk + 1, ng : ng + nj, ng : ng + ni + 1]
@@ -127,7 +127,7 @@ def directional(fx, fy, fz, s, rru, rrv, *, ni, nj, nk, dx, dy, dz, sides, flat=
     ax = np.where(ie, ax - tx, ax)
     ay = np.where(js, ay - ty, ay)
     ay = np.where(jn, ay - ty, ay)
-    az = np.empty((nk, nj, ni), np.float32)
+    az = np.empty((nk, nj, ni), F)
     if flat is not None:
         rdz = F(flat.rdz)
         az[0] = (-fz[2, J, I]) * rdz
@@ -159,7 +159,7 @@ def _row_mass(values, ext):
     """
     n = values.shape[-1]
     if ext is None:
-        pad = np.zeros(values.shape[:-1] + (2,), np.float32)
+        pad = np.zeros(values.shape[:-1] + (2,), F)
         v = np.concatenate([pad, values, pad], axis=-1)
     else:
         v = values[..., ext]
@@ -174,14 +174,14 @@ def _row_mass(values, ext):
         s = f1 + f2
         with np.errstate(all="ignore"):
             r = np.where(s > SMEPS, np.maximum(cv, -s) / s, ZERO)
-        return np.where(cv >= ZERO, ZERO, r).astype(np.float32), s
+        return np.where(cv >= ZERO, ZERO, r).astype(F), s
 
     c = np.arange(n)
     vc = at(c)
     r0, s0 = ratio(c)
     rl, _ = ratio(c - 1)
     rr, _ = ratio(c + 1)
-    mass = np.zeros(values.shape, np.float32)
+    mass = np.zeros(values.shape, F)
     flag = np.zeros(values.shape, bool)
     neg = (vc < ZERO) & (r0 != ZERO)
     mass = np.where(neg, ZERO - s0 * r0, mass)

@@ -95,7 +95,7 @@ def _extrapolation_weights(sigma, sigmaf, nk, top):
     x0 = float(F(F(x0n) / den))
     alpha = [float(F(F(v) / den)) for v in nodes]
     d = _fornberg_delta(alpha, x0)
-    return np.array([F(d[2][1][nu]) for nu in range(3)], dtype=np.float32)
+    return np.array([F(d[2][1][nu]) for nu in range(3)], dtype=F)
 
 
 def terrain_coordinate_profiles(nk, dz, *, sigma=None, sigmaf=None):
@@ -105,18 +105,18 @@ def terrain_coordinate_profiles(nk, dz, *, sigma=None, sigmaf=None):
     stretched; otherwise it is uniform with layer thickness ``dz``.
     """
     dz32 = F(dz)
-    s = np.zeros(nk + 2, np.float32)
-    sf = np.zeros(nk + 2, np.float32)
+    s = np.zeros(nk + 2, F)
+    sf = np.zeros(nk + 2, F)
     if sigma is None:
         for k in range(1, nk + 1):
             s[k] = (F(k) - F(0.5)) * dz32
         for k in range(1, nk + 2):
             sf[k] = F(k - 1) * dz32
     else:
-        s[1 : nk + 1] = np.asarray(sigma, np.float32)[1 : nk + 1]
-        sf[1 : nk + 2] = np.asarray(sigmaf, np.float32)[1 : nk + 2]
-    rds = np.zeros(nk + 2, np.float32)
-    rdsf = np.zeros(nk + 2, np.float32)
+        s[1 : nk + 1] = np.asarray(sigma, F)[1 : nk + 1]
+        sf[1 : nk + 2] = np.asarray(sigmaf, F)[1 : nk + 2]
+    rds = np.zeros(nk + 2, F)
+    rdsf = np.zeros(nk + 2, F)
     for k in range(2, nk + 1):
         rds[k] = F(1.0) / (s[k] - s[k - 1])
     for k in range(1, nk + 1):
@@ -182,7 +182,7 @@ def terrain_pressure_gradient(a, m, prof, *, ni, nj, nk, dts, rdx, rdy):
 
 def _terrain_w(xu, yv, m, prof, *, ni, nj, nk, zt, weight=None):
     """w-level terrain term from face velocities (dum3 :508-540 / hadvp :628-648)."""
-    out = np.zeros((nk + 2, nj, ni), np.float32)
+    out = np.zeros((nk + 2, nj, ni), F)
     rzt = F(1.0) / zt
     for kf in range(2, nk + 1):
         r2 = (prof["sigmaf"][kf] - prof["sigma"][kf - 1]) * prof["rds"][kf]
@@ -202,6 +202,37 @@ def _terrain_w(xu, yv, m, prof, *, ni, nj, nk, zt, weight=None):
     return out
 
 
+# Arithmetic type of the Thomas elimination and back substitution: the
+# restatement's own (F) unless TRIDIAGONAL_BINARY64 is set, a checker knob that
+# runs only that step in binary64 (w rounded to F when stored) to measure how
+# much of a binary32 result's error the elimination contributes.
+TRIDIAGONAL_BINARY64 = False
+
+
+def _elimination_type():
+    return np.float64 if TRIDIAGONAL_BINARY64 else F
+
+
+def _eliminate_row(k, nk, aa, bb, cc, dd, ec, ed, wt, T):
+    """Forward elimination of row k (k = 2..nk): pivot aa*ec(k-1)+bb; the top row solves w(nk)."""
+    aa, bb, cc, dd = (x.astype(T) for x in (aa, bb, cc, dd))
+    pivot = bb if k == 2 else aa * ec[k - 1] + bb
+    inv = T(1.0) / pivot
+    if k < nk:
+        ec[k] = (-cc) * inv
+        ed[k] = (dd if k == 2 else dd - aa * ed[k - 1]) * inv
+    else:
+        wt[k] = (dd - aa * ed[k - 1]) * inv
+
+
+def _back_substitute(nk, ec, ed, wt, w, ys, xs):
+    """w(k) = ec(k)*w(k+1) + ed(k) for k = nk-1..2, then w(2..nk) stored in the restatement's type."""
+    for k in range(nk - 1, 1, -1):
+        wt[k] = ec[k] * wt[k + 1] + ed[k]
+    for k in range(2, nk + 1):
+        w[k, ys, xs] = wt[k].astype(w.dtype)
+
+
 def acoustic_small_step(a, m, prof, sides, *, ni, nj, nk, dts, rdx, rdy, dx, dy, zt, nrk, n, nloop,
                         alph, beta, kdiv, tridiagonal=None):
     """One small step of the stage loop (cuda_psolver3_terrain_open.py:838-863).
@@ -216,8 +247,8 @@ def acoustic_small_step(a, m, prof, sides, *, ni, nj, nk, dts, rdx, rdy, dx, dy,
     terrain_pressure_gradient(a, m, prof, ni=ni, nj=nj, nk=nk, dts=dts, rdx=rdx, rdy=rdy)
     bc.periodic_face_pgrad(a, m, prof, sides, ni=ni, nj=nj, nk=nk, dts=dts, rdx=rdx, rdy=rdy, cp=CP)
     # transformed velocities xu, yv (:482-507)
-    xu = np.zeros((nk + 2, nj, ni + 1), np.float32)
-    yv = np.zeros((nk + 2, nj + 1, ni), np.float32)
+    xu = np.zeros((nk + 2, nj, ni + 1), F)
+    yv = np.zeros((nk + 2, nj + 1, ni), F)
     xu[1 : nk + 1] = a["u3d"][1 : nk + 1, ys, ng : ng + ni + 1] * m["rgzu"][None]
     yv[1 : nk + 1] = a["v3d"][1 : nk + 1, ng : ng + nj + 1, xs] * m["rgzv"][None]
     dum3 = _terrain_w(xu, yv, m, prof, ni=ni, nj=nj, nk=nk, zt=zt)
@@ -260,11 +291,13 @@ def acoustic_small_step(a, m, prof, sides, *, ni, nj, nk, dts, rdx, rdy, dx, dy,
     # implicit vertical solve and pressure update (:678-745)
     asq = alph * alph
     mm, pk1, pk2, qk, fpk, pp3d = a["mm"], a["pk1"], a["pk2"], a["qk"], a["fpk"], a["pp3d"]
-    ec = np.zeros((nk + 2, nj, ni), np.float32)
-    ed = np.zeros((nk + 2, nj, ni), np.float32)
+    T = _elimination_type()
+    ec = np.zeros((nk + 2, nj, ni), T)
+    ed = np.zeros((nk + 2, nj, ni), T)
+    wt = np.zeros((nk + 2, nj, ni), T)
     for k in range(2, nk + 1):
-        aa = (-asq) * (mm[k, J, I] * (pk1[k - 1, J, I] + qk[k - 1, J, I])) if k >= 3 else np.zeros((nj, ni), np.float32)
-        cc = (-asq) * ((-mm[k, J, I]) * (pk2[k, J, I] - qk[k, J, I])) if k < nk else np.zeros((nj, ni), np.float32)
+        aa = (-asq) * (mm[k, J, I] * (pk1[k - 1, J, I] + qk[k - 1, J, I])) if k >= 3 else np.zeros((nj, ni), F)
+        cc = (-asq) * ((-mm[k, J, I]) * (pk2[k, J, I] - qk[k, J, I])) if k < nk else np.zeros((nj, ni), F)
         diag = (-(pk1[k, J, I] + qk[k, J, I])) + (pk2[k - 1, J, I] - qk[k - 1, J, I])
         bb = F(1.0) - asq * (mm[k, J, I] * diag)
         pdelta = pp3d[k, J, I] - pp3d[k - 1, J, I]
@@ -273,15 +306,8 @@ def acoustic_small_step(a, m, prof, sides, *, ni, nj, nk, dts, rdx, rdy, dx, dy,
         dd = (w[k, ys, xs] + fwk) + alph * ((-mm[k, J, I]) * udelta)
         if tridiagonal is not None:
             tridiagonal.setdefault("rows", []).append((k, aa, bb, cc, dd))
-        pivot = bb if k == 2 else aa * ec[k - 1] + bb
-        inv = F(1.0) / pivot
-        if k < nk:
-            ec[k] = (-cc) * inv
-            ed[k] = (dd if k == 2 else dd - aa * ed[k - 1]) * inv
-        else:
-            w[k, ys, xs] = (dd - aa * ed[k - 1]) * inv
-    for k in range(nk - 1, 1, -1):
-        w[k, ys, xs] = ec[k] * w[k + 1, ys, xs] + ed[k]
+        _eliminate_row(k, nk, aa, bb, cc, dd, ec, ed, wt, T)
+    _back_substitute(nk, ec, ed, wt, w, ys, xs)
     for k in range(1, nk + 1):
         w1 = ZERO if k == 1 else w[k, ys, xs]
         w2 = ZERO if k == nk else w[k + 1, ys, xs]
@@ -423,8 +449,8 @@ def flat_pressure_gradient(a, sides, *, ni, nj, nk, dts, rdx, rdy):
     pw, pe, ps, pn = (s == 1 for s in sides)
     ii = np.arange(1, ni + 2)[None, :]
     jj = np.arange(1, nj + 2)[:, None]
-    wu = ((jj <= nj) & ((ii > 1) | pw) & ((ii < ni + 1) | pe)).astype(np.float32)
-    wv = ((ii <= ni) & ((jj > 1) | ps) & ((jj < nj + 1) | pn)).astype(np.float32)
+    wu = ((jj <= nj) & ((ii > 1) | pw) & ((ii < ni + 1) | pe)).astype(F)
+    wv = ((ii <= ni) & ((jj > 1) | ps) & ((jj < nj + 1) | pn)).astype(F)
     Jp, Ip = slice(1, nj + 2), slice(1, ni + 2)
     for k in range(1, nk + 1):
         pu = tem1 * (ppd[k, Jp, Ip] - ppd[k, Jp, 0 : ni + 1])
@@ -459,10 +485,12 @@ def flat_small_step(a, m, flat, sides, *, ni, nj, nk, dts, rdx, rdy, nrk, n, nlo
         a["fpk"][k, J, I] = (dts * (P("ppten", k) - P("ppterm", k) * div) + hp) + beta * vert
     asq = alph * alph
     mm, pk1, pk2, qk, fpk, pp3d = a["mm"], a["pk1"], a["pk2"], a["qk"], a["fpk"], a["pp3d"]
-    ec = np.zeros((nk + 2, nj, ni), np.float32)
-    ed = np.zeros((nk + 2, nj, ni), np.float32)
+    T = _elimination_type()
+    ec = np.zeros((nk + 2, nj, ni), T)
+    ed = np.zeros((nk + 2, nj, ni), T)
+    wt = np.zeros((nk + 2, nj, ni), T)
     for k in range(2, nk + 1):
-        zero = np.zeros((nj, ni), np.float32)
+        zero = np.zeros((nj, ni), F)
         aa = (-asq) * (mm[k, J, I] * (pk1[k - 1, J, I] + qk[k - 1, J, I])) if k >= 3 else zero
         cc = (-asq) * ((-mm[k, J, I]) * (pk2[k, J, I] - qk[k, J, I])) if k < nk else zero
         diag = (-(pk1[k, J, I] + qk[k, J, I])) + (pk2[k - 1, J, I] - qk[k - 1, J, I])
@@ -471,15 +499,8 @@ def flat_small_step(a, m, flat, sides, *, ni, nj, nk, dts, rdx, rdy, nrk, n, nlo
         fwk = dts * a["wten"][k, ys, xs] + beta * ((-mm[k, J, I]) * pdelta)
         udelta = (pp3d[k, J, I] + fpk[k, J, I]) - (pp3d[k - 1, J, I] + fpk[k - 1, J, I])
         dd = (w[k, ys, xs] + fwk) + alph * ((-mm[k, J, I]) * udelta)
-        pivot = bb if k == 2 else aa * ec[k - 1] + bb
-        inv = F(1.0) / pivot
-        if k < nk:
-            ec[k] = (-cc) * inv
-            ed[k] = (dd if k == 2 else dd - aa * ed[k - 1]) * inv
-        else:
-            w[k, ys, xs] = (dd - aa * ed[k - 1]) * inv
-    for k in range(nk - 1, 1, -1):
-        w[k, ys, xs] = ec[k] * w[k + 1, ys, xs] + ed[k]
+        _eliminate_row(k, nk, aa, bb, cc, dd, ec, ed, wt, T)
+    _back_substitute(nk, ec, ed, wt, w, ys, xs)
     for k in range(1, nk + 1):
         old = pp3d[k, J, I].copy()
         vt = w[k, ys, xs] * (pk1[k, J, I] + qk[k, J, I]) + w[k + 1, ys, xs] * (pk2[k, J, I] - qk[k, J, I])

@@ -47,7 +47,7 @@ def stage_terms(a, base, th3d, q3d, *, ni, nj, nk, sides):
     repsm1 = reps - F(1.0)
     shape = (nk, nj, ni)
     if q3d is None:
-        q = np.zeros((6, nk, nj, ni), np.float32)
+        q = np.zeros((6, nk, nj, ni), F)
     else:
         q = q3d
     qv = np.maximum(q[0], ZERO)
@@ -58,8 +58,8 @@ def stage_terms(a, base, th3d, q3d, *, ni, nj, nk, sides):
     th0 = base["th0"][:, None, None]
     qv0 = base["qv0"][:, None, None]
     pi0 = base["pi0"][:, None, None]
-    qc0 = base.get("qc0", np.zeros(nk, np.float32))[:, None, None]
-    qi0 = base.get("qi0", np.zeros(nk, np.float32))[:, None, None]
+    qc0 = base.get("qc0", np.zeros(nk, F))[:, None, None]
+    qi0 = base.get("qi0", np.zeros(nk, F))[:, None, None]
     rth = F(1.0) / th0
     theta = th0 + th3d
     b = th3d * rth + repsm1 * (qv - qv0)
@@ -89,7 +89,7 @@ def buoyancy(a, m, *, ni, nj, nk):
 
     def bf(kf):  # (nj, ni) buoyancy at w level kf, zero at the ends
         if kf == 1 or kf == nk + 1:
-            return np.zeros((nj, ni), np.float32)
+            return np.zeros((nj, ni), F)
         return c1[kf - 1] * d8[kf - 1, ys, xs] + c2[kf - 1] * d8[kf, ys, xs]
 
     for k in range(1, nk + 1):
@@ -128,7 +128,7 @@ def getdiv(a, m, div, theta_div, thterm, *, ni, nj, nk, dx, dy, dz):
     u, v, w = a["rru"], a["rrv"], a["rrw"]
     xu = u[:, ys, NG : NG + ni + 1] * m["rgzu"][None]
     yv = v[:, NG : NG + nj + 1, xs] * m["rgzv"][None]
-    wt = np.zeros((nk + 2, nj, ni), np.float32)
+    wt = np.zeros((nk + 2, nj, ni), F)
     for kf in range(2, nk + 1):
         sf, r2, r1 = _uniform_weights(kf, dz)
         ux = r2 * (xu[kf, :, :ni] + xu[kf, :, 1:]) + r1 * (xu[kf - 1, :, :ni] + xu[kf - 1, :, 1:])
@@ -227,7 +227,7 @@ def predict_and_compose_held(base, rho, ppi, tha, qa, thheld, qheld, ppten, rate
         qvnew = qa[0] + dt * qvt
         b = (rho * (base["th0"][:, None, None] + thnew)) * (rd + rv * qvnew)
         b = b * rp00
-        pinew = np.power(b, rddcv).astype(np.float32) - base["pi0"][:, None, None]
+        pinew = np.power(b, rddcv).astype(F) - base["pi0"][:, None, None]
         val = ((pinew - ppi) / dt) + rates["qpten"]
         out = np.where(cond, val, out)
     ppten[1 : nk + 1, J, I] = out
@@ -250,7 +250,7 @@ def solve3_density(rho, c1, c2):
     """rr = 1/rho, rf by c1/c2 interpolation with 3-point end extrapolation (gm1_solve3_density_v1)."""
     nk = rho.shape[0]
     rr = F(1.0) / rho
-    rf = np.empty((nk + 1,) + rho.shape[1:], np.float32)
+    rf = np.empty((nk + 1,) + rho.shape[1:], F)
     rf[0] = (F(1.875) * rho[0] + F(-1.25) * rho[1]) + F(0.375) * rho[2]
     rf[nk] = (F(1.875) * rho[nk - 1] + F(-1.25) * rho[nk - 2]) + F(0.375) * rho[nk - 3]
     for f in range(1, nk):
