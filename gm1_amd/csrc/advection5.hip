@@ -265,7 +265,9 @@ struct ScalarAdvArgs {
   const float* mh;
 };
 
-template <bool WENO, int MODE>
+// FLAT: the flat stretched column's text (a.mh set), fixed at compile time so
+// the terrain instance carries none of its branches.
+template <bool WENO, int MODE, bool FLAT>
 __global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
   constexpr bool GATE = (MODE == MODE_PDEF);
   __shared__ float lfx[TY][TX + 1];
@@ -371,17 +373,17 @@ __global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
         if (ie) fe = fw;
         if (js) fs = fn;
         if (jn) fn = fs;
-        ax = a.mh ? (-(fe - fw)) * a.rdx : (fw - fe) * a.rdx;
-        ay = a.mh ? (-(fn - fs)) * a.rdy : (fs - fn) * a.rdy;
+        ax = FLAT ? (-(fe - fw)) * a.rdx : (fw - fe) * a.rdx;
+        ay = FLAT ? (-(fn - fs)) * a.rdy : (fs - fn) * a.rdy;
         if (iw) ax = ax - (sv * (ue - uw)) * a.rdx;
         if (ie) ax = ax - (sv * (ue - uw)) * a.rdx;
         if (js) ay = ay - (sv * (vn - vs)) * a.rdy;
         if (jn) ay = ay - (sv * (vn - vs)) * a.rdy;
       } else {
-        ax = a.mh ? (-(fe - fw)) * a.rdx : (fw - fe) * a.rdx;
-        ay = a.mh ? (-(fn - fs)) * a.rdy : (fs - fn) * a.rdy;
+        ax = FLAT ? (-(fe - fw)) * a.rdx : (fw - fe) * a.rdx;
+        ay = FLAT ? (-(fn - fs)) * a.rdy : (fs - fn) * a.rdy;
       }
-      if (a.mh) {
+      if constexpr (FLAT) {
         if (k == 1) az = ((-fzt) * a.rdsf) * a.mh[k];
         else if (k == nk) az = (fzb * a.rdsf) * a.mh[k];
         else az = ((-(fzt - fzb)) * a.rdsf) * a.mh[k];
@@ -398,7 +400,7 @@ __global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
       } else {
         float total = (ax + ay) + az;
         total = total + sv * a.divx[q];
-        const float adv = a.mh ? total * a.rr0[(int64_t)(k - 1) * mst] : (total * a.rr0[q]) * a.gz[mcol];
+        const float adv = FLAT ? total * a.rr0[(int64_t)(k - 1) * mst] : (total * a.rr0[q]) * a.gz[mcol];
         if (MODE == MODE_TENDENCY) {
           a.tend[q] = adv;
         } else {
@@ -455,39 +457,42 @@ GM1_DEV int family_cell(int c, int n, unsigned lo, unsigned hi) {
   return c;
 }
 
+template <bool FLAT>
 GM1_DEV float cand_x(const PdefArgs& a, int k, int j, int i) {
   const int nxs = sx(a.ni), nys = sy(a.nj), nxu = ux(a.ni);
   const int64_t q = ((int64_t)(k - 1) * a.nj + j) * a.ni + i;
   const int x = i + NG, y = j + NG;
   const float uw = a.rru[ix3(k, y, x, nys, nxu)], ue = a.rru[ix3(k, y, x + 1, nys, nxu)];
-  const float r0a = (a.mh ? a.rho0[(int64_t)(k - 1) * a.nj * a.ni] : a.rho0[q]) * a.qa[q];
+  const float r0a = (FLAT ? a.rho0[(int64_t)(k - 1) * a.nj * a.ni] : a.rho0[q]) * a.qa[q];
   if ((i == 0 && side_of(a.obc, 0) == 2u && uw >= 0.0f) || (i + 1 == a.ni && side_of(a.obc, 1) == 2u && ue <= 0.0f))
     return r0a;
   const float d = ue - uw;
-  if (a.mh) return r0a + a.dt * (a.advx[q] + (a.s[ix3(k, y, x, nys, nxs)] * d) * a.rdx);
+  if constexpr (FLAT) return r0a + a.dt * (a.advx[q] + (a.s[ix3(k, y, x, nys, nxs)] * d) * a.rdx);
   return r0a + (a.dt * (a.advx[q] + (a.s[ix3(k, y, x, nys, nxs)] * d) * a.rdx)) * a.gz[(int64_t)j * a.ni + i];
 }
 
+template <bool FLAT>
 GM1_DEV float cand_y(const PdefArgs& a, int k, int j, int i) {
   const int nxs = sx(a.ni), nys = sy(a.nj), nyv = vy(a.nj);
   const int64_t q = ((int64_t)(k - 1) * a.nj + j) * a.ni + i;
   const int x = i + NG, y = j + NG;
   const float vs = a.rrv[ix3(k, y, x, nyv, nxs)], vn = a.rrv[ix3(k, y + 1, x, nyv, nxs)];
-  const float r0a = (a.mh ? a.rho0[(int64_t)(k - 1) * a.nj * a.ni] : a.rho0[q]) * a.qa[q];
+  const float r0a = (FLAT ? a.rho0[(int64_t)(k - 1) * a.nj * a.ni] : a.rho0[q]) * a.qa[q];
   if ((j == 0 && side_of(a.obc, 2) == 2u && vs >= 0.0f) || (j + 1 == a.nj && side_of(a.obc, 3) == 2u && vn <= 0.0f))
     return r0a;
   const float d = vn - vs;
-  if (a.mh) return r0a + a.dt * (a.advy[q] + (a.s[ix3(k, y, x, nys, nxs)] * d) * a.rdy);
+  if constexpr (FLAT) return r0a + a.dt * (a.advy[q] + (a.s[ix3(k, y, x, nys, nxs)] * d) * a.rdy);
   return r0a + (a.dt * (a.advy[q] + (a.s[ix3(k, y, x, nys, nxs)] * d) * a.rdy)) * a.gz[(int64_t)j * a.ni + i];
 }
 
+template <bool FLAT>
 GM1_DEV float cand_z(const PdefArgs& a, int k, int j, int i) {
   if (k < 1 || k > a.nk) return 0.0f;
   const int nxs = sx(a.ni), nys = sy(a.nj);
   const int64_t q = ((int64_t)(k - 1) * a.nj + j) * a.ni + i;
   const int x = i + NG, y = j + NG;
   const float d = a.rrw[ix3(k + 1, y, x, nys, nxs)] - a.rrw[ix3(k, y, x, nys, nxs)];
-  if (a.mh) {
+  if constexpr (FLAT) {
     const float r0c = a.rho0[(int64_t)(k - 1) * a.nj * a.ni] * a.qa[q];
     return r0c + a.dt * (a.advz[q] + ((a.s[ix3(k, y, x, nys, nxs)] * d) * a.rdz) * a.mh[k]);
   }
@@ -521,6 +526,7 @@ GM1_DEV void row_mass(const float v[5], float& mass, bool& flag) {
   }
 }
 
+template <bool FLAT>
 __global__ __launch_bounds__(RT) void k_pdef_repair(PdefArgs a) {
   __shared__ float lcx[RY][RX + 4];
   __shared__ float lcy[RY + 4][RX];
@@ -539,25 +545,25 @@ __global__ __launch_bounds__(RT) void k_pdef_repair(PdefArgs a) {
   const int64_t mcol = (int64_t)j * ni + i, mst = (int64_t)nj * ni;
   float cz[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};  // candidates at k-2..k+2
   if (live)
-    for (int d = 0; d < 4; ++d) cz[d + 1] = cand_z(a, kb - 2 + d, j, i);
+    for (int d = 0; d < 4; ++d) cz[d + 1] = cand_z<FLAT>(a, kb - 2 + d, j, i);
   for (int k = kb; k <= ke; ++k) {
     for (int slot = threadIdx.x; slot < RY * (RX + 4); slot += RT) {
       const int r = slot / (RX + 4), t = slot % (RX + 4);
       const int jj = j0 + r, c = i0 - 2 + t;
-      if (jj < nj && c <= ni + 1 && c >= -2 && (c < i0 + RX + 2)) lcx[r][t] = cand_x(a, k, jj, family_cell(c, ni, wbc, ebc));
+      if (jj < nj && c <= ni + 1 && c >= -2 && (c < i0 + RX + 2)) lcx[r][t] = cand_x<FLAT>(a, k, jj, family_cell(c, ni, wbc, ebc));
     }
     for (int slot = threadIdx.x; slot < (RY + 4) * RX; slot += RT) {
       const int r = slot / RX, t = slot % RX;
       const int ii = i0 + t, c = j0 - 2 + r;
-      if (ii < ni && c <= nj + 1 && c >= -2) lcy[r][t] = cand_y(a, k, family_cell(c, nj, sbc, nbc), ii);
+      if (ii < ni && c <= nj + 1 && c >= -2) lcy[r][t] = cand_y<FLAT>(a, k, family_cell(c, nj, sbc, nbc), ii);
     }
     for (int d = 0; d < 4; ++d) cz[d] = cz[d + 1];
-    cz[4] = live ? cand_z(a, k + 2, j, i) : 0.0f;
+    cz[4] = live ? cand_z<FLAT>(a, k + 2, j, i) : 0.0f;
     __syncthreads();
     if (live) {
       const int64_t q = (int64_t)(k - 1) * mst + mcol;
       const float sv = a.s[ix3(k, y, x, nys, nxs)];
-      const bool flat = a.mh != nullptr;
+      constexpr bool flat = FLAT;
       const float r0a = (flat ? a.rho0[(int64_t)(k - 1) * mst] : a.rho0[q]) * a.qa[q];
       const float rg = flat ? 1.0f : a.rgz[mcol];
       float ax = a.advx[q], ay = a.advy[q], az = a.advz[q];
@@ -602,24 +608,34 @@ int chunks_for(int tiles, int nk, int* kchunk) {
   return (nk + *kchunk - 1) / *kchunk;
 }
 
+template <bool FLAT>
+void launch_scalar_variant(const ScalarAdvArgs& a, int use_weno, int mode, dim3 grid, hipStream_t s) {
+  if (mode == MODE_TENDENCY) {
+    if (use_weno)
+      GM1_TIMED(TK_SCALAR_ADV5, s,
+                hipLaunchKernelGGL((k_scalar_adv5<true, MODE_TENDENCY, FLAT>), grid, dim3(NT), 0, s, a));
+    else
+      GM1_TIMED(TK_SCALAR_ADV5, s,
+                hipLaunchKernelGGL((k_scalar_adv5<false, MODE_TENDENCY, FLAT>), grid, dim3(NT), 0, s, a));
+  } else if (mode == MODE_COMPOSE) {
+    if (use_weno)
+      GM1_TIMED(TK_SCALAR_ADV5, s,
+                hipLaunchKernelGGL((k_scalar_adv5<true, MODE_COMPOSE, FLAT>), grid, dim3(NT), 0, s, a));
+    else
+      GM1_TIMED(TK_SCALAR_ADV5, s,
+                hipLaunchKernelGGL((k_scalar_adv5<false, MODE_COMPOSE, FLAT>), grid, dim3(NT), 0, s, a));
+  } else {
+    GM1_TIMED(TK_SCALAR_ADV5, s, hipLaunchKernelGGL((k_scalar_adv5<true, MODE_PDEF, FLAT>), grid, dim3(NT), 0, s, a));
+  }
+}
+
 int launch_scalar_tile(ScalarAdvArgs& a, int use_weno, int mode, hipStream_t s) {
   const int tiles = ((a.ni + TX - 1) / TX) * ((a.nj + TY - 1) / TY);
   const int chunks = chunks_for(tiles, a.nk, &a.kchunk);
   a.nchunks = chunks;
   dim3 grid(tiles * chunks);
-  if (mode == MODE_TENDENCY) {
-    if (use_weno)
-      GM1_TIMED(TK_SCALAR_ADV5, s, hipLaunchKernelGGL((k_scalar_adv5<true, MODE_TENDENCY>), grid, dim3(NT), 0, s, a));
-    else
-      GM1_TIMED(TK_SCALAR_ADV5, s, hipLaunchKernelGGL((k_scalar_adv5<false, MODE_TENDENCY>), grid, dim3(NT), 0, s, a));
-  } else if (mode == MODE_COMPOSE) {
-    if (use_weno)
-      GM1_TIMED(TK_SCALAR_ADV5, s, hipLaunchKernelGGL((k_scalar_adv5<true, MODE_COMPOSE>), grid, dim3(NT), 0, s, a));
-    else
-      GM1_TIMED(TK_SCALAR_ADV5, s, hipLaunchKernelGGL((k_scalar_adv5<false, MODE_COMPOSE>), grid, dim3(NT), 0, s, a));
-  } else {
-    GM1_TIMED(TK_SCALAR_ADV5, s, hipLaunchKernelGGL((k_scalar_adv5<true, MODE_PDEF>), grid, dim3(NT), 0, s, a));
-  }
+  if (a.mh) launch_scalar_variant<true>(a, use_weno, mode, grid, s);
+  else launch_scalar_variant<false>(a, use_weno, mode, grid, s);
   GM1_LAUNCH_CHECK();
   return 0;
 }
@@ -722,7 +738,8 @@ static int pdef_stage(const float* scalar, const float* rru, const float* rrv, c
   const int tiles = ((ni + RX - 1) / RX) * ((nj + RY - 1) / RY);
   const int chunks = chunks_for(tiles, nk, &p.kchunk);
   p.nchunks = chunks;
-  hipLaunchKernelGGL(k_pdef_repair, dim3(tiles * chunks), dim3(RT), 0, s, p);
+  if (p.mh) hipLaunchKernelGGL(k_pdef_repair<true>, dim3(tiles * chunks), dim3(RT), 0, s, p);
+  else hipLaunchKernelGGL(k_pdef_repair<false>, dim3(tiles * chunks), dim3(RT), 0, s, p);
   GM1_LAUNCH_CHECK();
   return 0;
 }

@@ -250,6 +250,7 @@ GM1_DEV float uvz(const MomArgs& a, int kf, int j, int i) {
   return m * r;
 }
 
+template <bool FLAT>
 __global__ __launch_bounds__(256) void k_mom_advu(MomArgs a, int f0, int nf) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y, k0 = blockIdx.z, k = k0 + 1;
@@ -269,7 +270,7 @@ __global__ __launch_bounds__(256) void k_mom_advu(MomArgs a, int f0, int nf) {
     const float m = R0[east + NG] + R0[west + NG];
     if (m >= 0.0f) {
       const float c = (R1[west + NG] - R0[west + NG]) + (R1[east + NG] - R0[east + NG]);
-      ay = a.flat ? ay - (((uc * 0.5f) * c) * a.rdy) * a.vh[y] : ay - (uc * 0.5f) * (c * (a.rdy * a.vh[y]));
+      ay = FLAT ? ay - (((uc * 0.5f) * c) * a.rdy) * a.vh[y] : ay - (uc * 0.5f) * (c * (a.rdy * a.vh[y]));
     }
   } else if (j == nj - 1 && opn(a.obc, 3)) {
     const float* RN = a.rrv + ix3(k, NG + nj, 0, nyv, nxv);
@@ -278,18 +279,18 @@ __global__ __launch_bounds__(256) void k_mom_advu(MomArgs a, int f0, int nf) {
     const float m = RN[east + NG] + RN[west + NG];
     if (m <= 0.0f) {
       const float c = (R1[west + NG] - R0[west + NG]) + (R1[east + NG] - R0[east + NG]);
-      ay = a.flat ? ay - (((uc * 0.5f) * c) * a.rdy) * a.vh[y] : ay - (uc * 0.5f) * (c * (a.rdy * a.vh[y]));
+      ay = FLAT ? ay - (((uc * 0.5f) * c) * a.rdy) * a.vh[y] : ay - (uc * 0.5f) * (c * (a.rdy * a.vh[y]));
     }
   }
   float az;
   if (k == 1) az = (-uvz<0>(a, 2, j, f)) * a.rdz;
   else if (k == nk) az = uvz<0>(a, nk, j, f) * a.rdz;
   else az = (-(uvz<0>(a, k + 1, j, f) - uvz<0>(a, k, j, f))) * a.rdz;
-  if (a.flat) az = az * a.flat[k];
+  if constexpr (FLAT) az = az * a.flat[k];
   float tot = (ax + ay) + az;
   const int64_t row = ((int64_t)k0 * nj + j) * ni;
   tot = tot + uc * (0.5f * (a.divx[row + east] + a.divx[row + west]));
-  const float sc = a.flat ? a.rr0[(int64_t)k0 * nj * ni]
+  const float sc = FLAT ? a.rr0[(int64_t)k0 * nj * ni]
                           : a.gzu[(int64_t)j * (ni + 1) + f] / (0.5f * (a.rho0[row + west] + a.rho0[row + east]));
   const int64_t p = ix3(k, y, x, nyu, nxu);
   a.uten[p] = a.uten[p] + tot * sc;
@@ -332,6 +333,7 @@ GM1_DEV float vfy(const MomArgs& a, int k, int p, int i) {
   return m * r;
 }
 
+template <bool FLAT>
 __global__ __launch_bounds__(256) void k_mom_advv(MomArgs a, int g0, int nr) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int r = blockIdx.y, k0 = blockIdx.z, k = k0 + 1;
@@ -351,24 +353,24 @@ __global__ __launch_bounds__(256) void k_mom_advv(MomArgs a, int g0, int nr) {
     const float m = Rn[NG] + Rs[NG];
     if (m >= 0.0f) {
       const float c = (Rs[NG + 1] - Rs[NG]) + (Rn[NG + 1] - Rn[NG]);
-      ax = a.flat ? ax - (((vc * 0.5f) * c) * a.rdx) * a.uh[NG] : ax - (vc * 0.5f) * (c * (a.rdx * a.uh[NG]));
+      ax = FLAT ? ax - (((vc * 0.5f) * c) * a.rdx) * a.uh[NG] : ax - (vc * 0.5f) * (c * (a.rdx * a.uh[NG]));
     }
   } else if (i == ni - 1 && opn(a.obc, 1)) {
     const float m = Rn[NG + ni] + Rs[NG + ni];
     if (m <= 0.0f) {
       const float c = (Rs[x + 1] - Rs[x]) + (Rn[x + 1] - Rn[x]);
-      ax = a.flat ? ax - (((vc * 0.5f) * c) * a.rdx) * a.uh[x] : ax - (vc * 0.5f) * (c * (a.rdx * a.uh[x]));
+      ax = FLAT ? ax - (((vc * 0.5f) * c) * a.rdx) * a.uh[x] : ax - (vc * 0.5f) * (c * (a.rdx * a.uh[x]));
     }
   }
   float az;
   if (k == 1) az = (-uvz<1>(a, 2, g, i)) * a.rdz;
   else if (k == nk) az = uvz<1>(a, nk, g, i) * a.rdz;
   else az = (-(uvz<1>(a, k + 1, g, i) - uvz<1>(a, k, g, i))) * a.rdz;
-  if (a.flat) az = az * a.flat[k];
+  if constexpr (FLAT) az = az * a.flat[k];
   float tot = (ax + ay) + az;
   const int64_t plane = (int64_t)k0 * nj * ni;
   tot = tot + vc * (0.5f * (a.divx[plane + (int64_t)north * ni + i] + a.divx[plane + (int64_t)south * ni + i]));
-  const float sc = a.flat ? a.rr0[plane]
+  const float sc = FLAT ? a.rr0[plane]
                           : a.gzv[(int64_t)g * ni + i] /
                                 (0.5f * (a.rho0[plane + (int64_t)south * ni + i] + a.rho0[plane + (int64_t)north * ni + i]));
   const int64_t p = ix3(k, y, x, nyv, nxv);
@@ -446,6 +448,7 @@ GM1_DEV float wz(const MomArgs& a, int k, int j, int i) {
   return m * r;
 }
 
+template <bool FLAT>
 __global__ __launch_bounds__(256) void k_mom_advw(MomArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y, kf = blockIdx.z + 2;
@@ -453,7 +456,7 @@ __global__ __launch_bounds__(256) void k_mom_advw(MomArgs a) {
   if (i >= ni) return;
   const int nxu = ux(ni), nyu = sy(nj), nxs = sx(ni), nys = sy(nj), nyv = vy(nj);
   const int y = j + NG, x = i + NG;
-  const bool flat = a.flat != nullptr;
+  constexpr bool flat = FLAT;
   float ax, ay;
   if (flat) {
     ax = ((-(wfx(a, kf, j, i + 1) - wfx(a, kf, j, i))) * a.rdx) * a.uh[x];
@@ -538,11 +541,11 @@ int gm1hip_momentum_advection5(
   const bool ps = side_of(obc, 2) == 1u, pn = side_of(obc, 3) == 1u;
   const int f0 = pw ? 0 : 1, nf = ni - 1 + (pw ? 1 : 0) + (pe ? 1 : 0);
   const int g0 = ps ? 0 : 1, nr = nj - 1 + (ps ? 1 : 0) + (pn ? 1 : 0);
-  GM1_TIMED(TK_MOM_ADVU, s, hipLaunchKernelGGL(k_mom_advu, dim3((nf + 63) / 64, nj, nk), blk, 0, s, a, f0, nf));
+  GM1_TIMED(TK_MOM_ADVU, s, hipLaunchKernelGGL(k_mom_advu<false>, dim3((nf + 63) / 64, nj, nk), blk, 0, s, a, f0, nf));
   GM1_LAUNCH_CHECK();
-  GM1_TIMED(TK_MOM_ADVV, s, hipLaunchKernelGGL(k_mom_advv, dim3((ni + 63) / 64, nr, nk), blk, 0, s, a, g0, nr));
+  GM1_TIMED(TK_MOM_ADVV, s, hipLaunchKernelGGL(k_mom_advv<false>, dim3((ni + 63) / 64, nr, nk), blk, 0, s, a, g0, nr));
   GM1_LAUNCH_CHECK();
-  GM1_TIMED(TK_MOM_ADVW, s, hipLaunchKernelGGL(k_mom_advw, dim3((ni + 63) / 64, nj, nk - 1), blk, 0, s, a));
+  GM1_TIMED(TK_MOM_ADVW, s, hipLaunchKernelGGL(k_mom_advw<false>, dim3((ni + 63) / 64, nj, nk - 1), blk, 0, s, a));
   GM1_LAUNCH_CHECK();
   return 0;
 }
@@ -565,11 +568,11 @@ int gm1hip_momentum_transport5(
   const bool ps = side_of(obc, 2) == 1u, pn = side_of(obc, 3) == 1u;
   const int f0 = pw ? 0 : 1, nf = ni - 1 + (pw ? 1 : 0) + (pe ? 1 : 0);
   const int g0 = ps ? 0 : 1, nr = nj - 1 + (ps ? 1 : 0) + (pn ? 1 : 0);
-  GM1_TIMED(TK_MOM_ADVU, s, hipLaunchKernelGGL(k_mom_advu, dim3((nf + 63) / 64, nj, nk), blk, 0, s, a, f0, nf));
+  GM1_TIMED(TK_MOM_ADVU, s, hipLaunchKernelGGL(k_mom_advu<false>, dim3((nf + 63) / 64, nj, nk), blk, 0, s, a, f0, nf));
   GM1_LAUNCH_CHECK();
-  GM1_TIMED(TK_MOM_ADVV, s, hipLaunchKernelGGL(k_mom_advv, dim3((ni + 63) / 64, nr, nk), blk, 0, s, a, g0, nr));
+  GM1_TIMED(TK_MOM_ADVV, s, hipLaunchKernelGGL(k_mom_advv<false>, dim3((ni + 63) / 64, nr, nk), blk, 0, s, a, g0, nr));
   GM1_LAUNCH_CHECK();
-  GM1_TIMED(TK_MOM_ADVW, s, hipLaunchKernelGGL(k_mom_advw, dim3((ni + 63) / 64, nj, nk - 1), blk, 0, s, a));
+  GM1_TIMED(TK_MOM_ADVW, s, hipLaunchKernelGGL(k_mom_advw<false>, dim3((ni + 63) / 64, nj, nk - 1), blk, 0, s, a));
   GM1_LAUNCH_CHECK();
   return 0;
 }
@@ -600,11 +603,11 @@ int gm1hip_momentum_advection5_flat(
   const bool ps = side_of(obc, 2) == 1u, pn = side_of(obc, 3) == 1u;
   const int f0 = pw ? 0 : 1, nf = ni - 1 + (pw ? 1 : 0) + (pe ? 1 : 0);
   const int g0 = ps ? 0 : 1, nr = nj - 1 + (ps ? 1 : 0) + (pn ? 1 : 0);
-  GM1_TIMED(TK_MOM_ADVU, s, hipLaunchKernelGGL(k_mom_advu, dim3((nf + 63) / 64, nj, nk), blk, 0, s, a, f0, nf));
+  GM1_TIMED(TK_MOM_ADVU, s, hipLaunchKernelGGL(k_mom_advu<true>, dim3((nf + 63) / 64, nj, nk), blk, 0, s, a, f0, nf));
   GM1_LAUNCH_CHECK();
-  GM1_TIMED(TK_MOM_ADVV, s, hipLaunchKernelGGL(k_mom_advv, dim3((ni + 63) / 64, nr, nk), blk, 0, s, a, g0, nr));
+  GM1_TIMED(TK_MOM_ADVV, s, hipLaunchKernelGGL(k_mom_advv<true>, dim3((ni + 63) / 64, nr, nk), blk, 0, s, a, g0, nr));
   GM1_LAUNCH_CHECK();
-  GM1_TIMED(TK_MOM_ADVW, s, hipLaunchKernelGGL(k_mom_advw, dim3((ni + 63) / 64, nj, nk - 1), blk, 0, s, a));
+  GM1_TIMED(TK_MOM_ADVW, s, hipLaunchKernelGGL(k_mom_advw<true>, dim3((ni + 63) / 64, nj, nk - 1), blk, 0, s, a));
   GM1_LAUNCH_CHECK();
   return 0;
 }
