@@ -263,7 +263,30 @@ struct ScalarAdvArgs {
   // differences are -(fe-fw)*rdx, the vertical ((d*rdz)*mh(k)) and the
   // tendency is scaled by rr0(1,1,k) alone.
   const float* mh;
+  // Unpadded source (nk, nj, ni) read in place of the padded scalar s, or
+  // null; th0 (nk) turns it into the packed theta (th0 - 300) + theta'.
+  const float* src;
+  const float* th0;
 };
+
+// bc.F's halo cell c (-3..n+2) of a row of n cells as a physical index.
+GM1_DEV int family_cell(int c, int n, unsigned lo, unsigned hi) {
+  if (c < 0) return lo == 1u ? c + n : (lo == 2u ? 0 : -c - 1);
+  if (c >= n) return hi == 1u ? c - n : (hi == 2u ? n - 1 : 2 * n - 1 - c);
+  return c;
+}
+
+// The padded scalar's word (k, yy, xx) formed from an unpadded source: the
+// lateral family's image of a halo cell (periodic wrap, open clamp, rigid
+// mirror -- the cells bcs writes and the face stencils read) and, for theta,
+// the pack (th0 - 300) + theta' rounded once.
+GM1_DEV float packed_word(const float* __restrict__ src, const float* __restrict__ th0, int k, int yy, int xx,
+                          int ni, int nj, unsigned obc) {
+  const int j = family_cell(yy - NG, nj, side_of(obc, 2), side_of(obc, 3));
+  const int i = family_cell(xx - NG, ni, side_of(obc, 0), side_of(obc, 1));
+  const float v = src[((int64_t)(k - 1) * nj + j) * ni + i];
+  return th0 ? (th0[k - 1] - 300.0f) + v : v;
+}
 
 // FLAT: the flat stretched column's text (a.mh set), fixed at compile time so
 // the terrain instance carries none of its branches.
@@ -298,11 +321,12 @@ __global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
   // holds level k-3+q (0 outside the padded column; those words are never
   // read by a face's stencil, which stays within levels 1..nk).
   float win[7];
+  auto colword = [&](int lev) {  // the column's word at a level, 0 where no stencil reads
+    if (a.src) return (lev >= 1 && lev <= nk) ? packed_word(a.src, a.th0, lev, y, x, ni, nj, obc) : 0.0f;
+    return (lev >= 0 && lev <= nk + 1) ? col[(int64_t)lev * st] : 0.0f;
+  };
 #pragma unroll
-  for (int q = 0; q < 7; ++q) {
-    const int lev = kb - 3 + q;
-    win[q] = (live && lev >= 0 && lev <= nk + 1) ? col[(int64_t)lev * st] : 0.0f;
-  }
+  for (int q = 0; q < 7; ++q) win[q] = live ? colword(kb - 3 + q) : 0.0f;
   // z-face flux at face kf from the window b, b[0] = S(kf-3).
   auto zf = [&](int kf, const float* b) {
     const int64_t wp = (int64_t)kf * st + wcol;
@@ -315,7 +339,7 @@ __global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
   if (live && kb >= 2) fzb = zf(kb, win);  // S(kb-3..)
   for (int k = kb; k <= ke; ++k) {
     const int lnext = k + 4;  // the level entering the window after this one
-    const float snext = (live && lnext <= nk + 1) ? col[(int64_t)lnext * st] : 0.0f;
+    const float snext = live ? colword(lnext) : 0.0f;
     // the level plane: rows j0-3..j0+TY+2, columns i0-3..i0+TX+2 (padded
     // indices j0..j0+TY+5, i0..i0+TX+5), clipped to the padded array
     {
@@ -323,7 +347,8 @@ __global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
       for (int slot = threadIdx.x; slot < (TY + 6) * (TX + 6); slot += NT) {
         const int r = slot / (TX + 6), c = slot % (TX + 6);
         const int yy = j0 + r, xx = i0 + c;
-        if (yy < nys && xx < nxs) lp[r][c] = plane[(int64_t)yy * nxs + xx];
+        if (yy < nys && xx < nxs)
+          lp[r][c] = a.src ? packed_word(a.src, a.th0, k, yy, xx, ni, nj, obc) : plane[(int64_t)yy * nxs + xx];
       }
     }
     __syncthreads();
@@ -448,14 +473,15 @@ struct PdefArgs {
   // dt*(adv + div) without gz, corrections without rgz, vertical
   // ((s*d)*rdz)*mh(k), the total scaled by rr0(1,1,k).
   const float* mh;
+  const float* src;  // unpadded source (nk, nj, ni) in place of s, or null
 };
 
-// bc.F's halo cell c (-2..n+1) of a row of n cells as a physical index.
-GM1_DEV int family_cell(int c, int n, unsigned lo, unsigned hi) {
-  if (c < 0) return lo == 1u ? c + n : (lo == 2u ? 0 : -c - 1);
-  if (c >= n) return hi == 1u ? c - n : (hi == 2u ? n - 1 : 2 * n - 1 - c);
-  return c;
+// The scalar at an interior cell (k, y, x padded indices).
+GM1_DEV float pdef_word(const PdefArgs& a, int k, int y, int x) {
+  if (a.src) return a.src[((int64_t)(k - 1) * a.nj + (y - NG)) * a.ni + (x - NG)];
+  return a.s[ix3(k, y, x, sy(a.nj), sx(a.ni))];
 }
+
 
 template <bool FLAT>
 GM1_DEV float cand_x(const PdefArgs& a, int k, int j, int i) {
@@ -467,8 +493,8 @@ GM1_DEV float cand_x(const PdefArgs& a, int k, int j, int i) {
   if ((i == 0 && side_of(a.obc, 0) == 2u && uw >= 0.0f) || (i + 1 == a.ni && side_of(a.obc, 1) == 2u && ue <= 0.0f))
     return r0a;
   const float d = ue - uw;
-  if constexpr (FLAT) return r0a + a.dt * (a.advx[q] + (a.s[ix3(k, y, x, nys, nxs)] * d) * a.rdx);
-  return r0a + (a.dt * (a.advx[q] + (a.s[ix3(k, y, x, nys, nxs)] * d) * a.rdx)) * a.gz[(int64_t)j * a.ni + i];
+  if constexpr (FLAT) return r0a + a.dt * (a.advx[q] + (pdef_word(a, k, y, x) * d) * a.rdx);
+  return r0a + (a.dt * (a.advx[q] + (pdef_word(a, k, y, x) * d) * a.rdx)) * a.gz[(int64_t)j * a.ni + i];
 }
 
 template <bool FLAT>
@@ -481,8 +507,8 @@ GM1_DEV float cand_y(const PdefArgs& a, int k, int j, int i) {
   if ((j == 0 && side_of(a.obc, 2) == 2u && vs >= 0.0f) || (j + 1 == a.nj && side_of(a.obc, 3) == 2u && vn <= 0.0f))
     return r0a;
   const float d = vn - vs;
-  if constexpr (FLAT) return r0a + a.dt * (a.advy[q] + (a.s[ix3(k, y, x, nys, nxs)] * d) * a.rdy);
-  return r0a + (a.dt * (a.advy[q] + (a.s[ix3(k, y, x, nys, nxs)] * d) * a.rdy)) * a.gz[(int64_t)j * a.ni + i];
+  if constexpr (FLAT) return r0a + a.dt * (a.advy[q] + (pdef_word(a, k, y, x) * d) * a.rdy);
+  return r0a + (a.dt * (a.advy[q] + (pdef_word(a, k, y, x) * d) * a.rdy)) * a.gz[(int64_t)j * a.ni + i];
 }
 
 template <bool FLAT>
@@ -494,10 +520,10 @@ GM1_DEV float cand_z(const PdefArgs& a, int k, int j, int i) {
   const float d = a.rrw[ix3(k + 1, y, x, nys, nxs)] - a.rrw[ix3(k, y, x, nys, nxs)];
   if constexpr (FLAT) {
     const float r0c = a.rho0[(int64_t)(k - 1) * a.nj * a.ni] * a.qa[q];
-    return r0c + a.dt * (a.advz[q] + ((a.s[ix3(k, y, x, nys, nxs)] * d) * a.rdz) * a.mh[k]);
+    return r0c + a.dt * (a.advz[q] + ((pdef_word(a, k, y, x) * d) * a.rdz) * a.mh[k]);
   }
   const float r0a = a.rho0[q] * a.qa[q];
-  return r0a + (a.dt * (a.advz[q] + (a.s[ix3(k, y, x, nys, nxs)] * d) * a.rdz)) * a.gz[(int64_t)j * a.ni + i];
+  return r0a + (a.dt * (a.advz[q] + (pdef_word(a, k, y, x) * d) * a.rdz)) * a.gz[(int64_t)j * a.ni + i];
 }
 
 // ratio of the donor repair at the middle of three consecutive values
@@ -562,7 +588,7 @@ __global__ __launch_bounds__(RT) void k_pdef_repair(PdefArgs a) {
     __syncthreads();
     if (live) {
       const int64_t q = (int64_t)(k - 1) * mst + mcol;
-      const float sv = a.s[ix3(k, y, x, nys, nxs)];
+      const float sv = pdef_word(a, k, y, x);
       constexpr bool flat = FLAT;
       const float r0a = (flat ? a.rho0[(int64_t)(k - 1) * mst] : a.rho0[q]) * a.qa[q];
       const float rg = flat ? 1.0f : a.rgz[mcol];
@@ -712,15 +738,15 @@ int gm1hip_scalar_rk_stage_flat(const float* scalar, const float* rru, const flo
   return launch_scalar_tile(a, use_weno, MODE_COMPOSE, (hipStream_t)stream);
 }
 
-static int pdef_stage(const float* scalar, const float* rru, const float* rrv, const float* rrw, const float* c1,
-                      const float* c2, const float* rho0_3d, const float* rr0, const float* gz, const float* rgz,
-                      const float* mh, const float* divx, const float* qa, const float* held, float* out,
+static int pdef_stage(const float* scalar, const float* src, const float* rru, const float* rrv, const float* rrw,
+                      const float* c1, const float* c2, const float* rho0_3d, const float* rr0, const float* gz,
+                      const float* rgz, const float* mh, const float* divx, const float* qa, const float* held, float* out,
                       float* advx, float* advy, float* advz, int ni, int nj, int nk, int pdefweno, double weps,
                       float dx, float dy, float rdzv, unsigned obc, float dttmp, void* stream) {
   if (obc == 0u || nk < 4) return (int)hipErrorInvalidValue;
   hipStream_t s = (hipStream_t)stream;
   ScalarAdvArgs a{};
-  a.s = scalar; a.rru = rru; a.rrv = rrv; a.rrw = rrw; a.c1 = c1; a.c2 = c2;
+  a.s = scalar; a.src = src; a.rru = rru; a.rrv = rrv; a.rrw = rrw; a.c1 = c1; a.c2 = c2;
   a.divx = divx; a.rr0 = rr0; a.gz = gz; a.mh = mh;
   a.advx = advx; a.advy = advy; a.advz = advz; a.pdefweno = pdefweno;
   a.ni = ni; a.nj = nj; a.nk = nk;
@@ -729,7 +755,7 @@ static int pdef_stage(const float* scalar, const float* rru, const float* rrv, c
   int rc = launch_scalar_tile(a, 1, MODE_PDEF, s);
   if (rc) return rc;
   PdefArgs p{};
-  p.s = scalar; p.rru = rru; p.rrv = rrv; p.rrw = rrw;
+  p.s = scalar; p.src = src; p.rru = rru; p.rrv = rrv; p.rrw = rrw;
   p.rho0 = rho0_3d; p.rr0 = rr0; p.gz = gz; p.rgz = rgz; p.divx = divx; p.mh = mh;
   p.qa = qa; p.held = held; p.advx = advx; p.advy = advy; p.advz = advz; p.out = out;
   p.ni = ni; p.nj = nj; p.nk = nk;
@@ -750,8 +776,8 @@ int gm1hip_pdef_moisture_stage(const float* scalar, const float* rru, const floa
                                const float* held, float* out, float* advx, float* advy, float* advz, int ni,
                                int nj, int nk, int pdefweno, double weps, float dx, float dy, float dz,
                                unsigned obc, float dttmp, void* stream) {
-  return pdef_stage(scalar, rru, rrv, rrw, c1, c2, rho0_3d, rr0, gz, rgz, nullptr, divx, qa, held, out, advx, advy,
-                    advz, ni, nj, nk, pdefweno, weps, dx, dy, 1.0f / dz, obc, dttmp, stream);
+  return pdef_stage(scalar, nullptr, rru, rrv, rrw, c1, c2, rho0_3d, rr0, gz, rgz, nullptr, divx, qa, held, out,
+                    advx, advy, advz, ni, nj, nk, pdefweno, weps, dx, dy, 1.0f / dz, obc, dttmp, stream);
 }
 
 int gm1hip_pdef_moisture_stage_flat(const float* scalar, const float* rru, const float* rrv, const float* rrw,
@@ -761,7 +787,40 @@ int gm1hip_pdef_moisture_stage_flat(const float* scalar, const float* rru, const
                                     int pdefweno, double weps, float dx, float dy, float rdz, unsigned obc,
                                     float dttmp, void* stream) {
   if (!flat_column) return (int)hipErrorInvalidValue;
-  return pdef_stage(scalar, rru, rrv, rrw, c1, c2, rho0_3d, rr0, nullptr, nullptr, flat_column, divx, qa, held, out,
+  return pdef_stage(scalar, nullptr, rru, rrv, rrw, c1, c2, rho0_3d, rr0, nullptr, nullptr, flat_column, divx, qa,
+                    held, out, advx, advy, advz, ni, nj, nk, pdefweno, weps, dx, dy, rdz, obc, dttmp, stream);
+}
+
+// The two calls above reading the scalar from its unpadded carrier (theta'
+// with th0, or one moisture species) with the pack and the lateral family
+// fill folded into the tiles' loads.  Terrain columns pass gz (and rgz) and
+// rdz = 1/dz; flat stretched columns pass flat_column and the metric rdz.
+int gm1hip_scalar_rk_stage_src(const float* src, const float* th0, const float* rru, const float* rrv,
+                               const float* rrw, const float* c1, const float* c2, const float* rr0,
+                               const float* gz, const float* flat_column, const float* divx, const float* a0,
+                               const float* held, const float* divterm, float* out, int ni, int nj, int nk,
+                               int use_weno, double weps, float dx, float dy, float rdz, unsigned obc, float dttmp,
+                               void* stream) {
+  if (obc == 0u || !src || (gz == nullptr) == (flat_column == nullptr)) return (int)hipErrorInvalidValue;
+  ScalarAdvArgs a{};
+  a.src = src; a.th0 = th0; a.rru = rru; a.rrv = rrv; a.rrw = rrw; a.c1 = c1; a.c2 = c2;
+  a.divx = divx; a.rr0 = rr0; a.gz = gz; a.mh = flat_column;
+  a.a0 = a0; a.held = held; a.divterm = divterm; a.out = out; a.dttmp = dttmp;
+  a.ni = ni; a.nj = nj; a.nk = nk;
+  a.rdx = 1.0f / dx; a.rdy = 1.0f / dy; a.rdsf = rdz;
+  a.weps = weps; a.obc = obc;
+  return launch_scalar_tile(a, use_weno, MODE_COMPOSE, (hipStream_t)stream);
+}
+
+int gm1hip_pdef_moisture_stage_src(const float* src, const float* rru, const float* rrv, const float* rrw,
+                                   const float* c1, const float* c2, const float* rho0_3d, const float* rr0,
+                                   const float* gz, const float* rgz, const float* flat_column, const float* divx,
+                                   const float* qa, const float* held, float* out, float* advx, float* advy,
+                                   float* advz, int ni, int nj, int nk, int pdefweno, double weps, float dx, float dy,
+                                   float rdz, unsigned obc, float dttmp, void* stream) {
+  if (!src || (gz == nullptr) == (flat_column == nullptr) || (gz != nullptr) != (rgz != nullptr))
+    return (int)hipErrorInvalidValue;
+  return pdef_stage(nullptr, src, rru, rrv, rrw, c1, c2, rho0_3d, rr0, gz, rgz, flat_column, divx, qa, held, out,
                     advx, advy, advz, ni, nj, nk, pdefweno, weps, dx, dy, rdz, obc, dttmp, stream);
 }
 

@@ -34,9 +34,8 @@ from ._device import NG, lateral_code, ptr, stream_handle
 from ._native import Fl, I, P, U, signature
 from .fields import CaseConfig, base_state, initial_fields, initial_thermodynamics, metrics_for, profiles_for
 from .stretched import (flat_stretched_column, launch_cuda_gm1_child_stretched_momentum5,
-                        launch_cuda_gm1_child_stretched_pdef_moisture, launch_cuda_gm1_child_stretched_post_acoustic,
-                        launch_cuda_gm1_flat_acoustic_stage, launch_cuda_gm1_flat_stretched_buoyancy,
-                        launch_flat_scalar_rk_compose)
+                        launch_cuda_gm1_child_stretched_post_acoustic, launch_cuda_gm1_flat_acoustic_stage,
+                        launch_cuda_gm1_flat_stretched_buoyancy)
 from .grid import VerticalProfiles
 from .kessler import (
     Gm1KesslerDriverSelection,
@@ -47,13 +46,7 @@ from .kessler import (
 )
 from .momentum_advection5 import launch_cuda_terrain_open_momentum_advection5
 from .psolver3 import TerrainOpenSoundAblations, device_profiles, launch_cuda_psolver3_terrain_open_stage
-from .scalar_moist_rk import (
-    launch_pdef_moisture,
-    launch_scalar_rk_compose,
-    moisture_weps_f64,
-    pack_scalar,
-    theta_weps_f64,
-)
+from .scalar_moist_rk import launch_scalar_stage_from_carrier, moisture_weps_f64, theta_weps_f64
 
 signature("gm1hip_stage_terms", P, P, P, P, P, P, P, I, P, P, P, P, P, P, I, I, I, U, P)
 signature("gm1hip_buoyancy", P, P, P, P, P, P, P, P, I, I, I, P)
@@ -92,12 +85,11 @@ def host_initial_state(cfg: CaseConfig) -> dict[str, Any]:
         a[name] = np.zeros(sh["v"], np.float32)
     for name in ("w3d", "wa", "wten", "wten1", "rrw"):
         a[name] = np.zeros(sh["w"], np.float32)
-    for name in ("dum8", "theta_padded"):
-        a[name] = np.zeros(sh["s"], np.float32)
+    a["dum8"] = np.zeros(sh["s"], np.float32)
     for name in ("pp3d", "ppi", "ppx", "ppd", "piadv", "ppterm", "ppten", "thv", "qk", "pk1", "pk2", "mm", "fpk",
                  "elim_c", "elim_d"):
         a[name] = np.zeros(sh["p"], np.float32)
-    for name in ("th3d", "tha", "thten1", "thterm", "divergence", "theta_div", "divx"):
+    for name in ("th3d", "tha", "thten1", "thterm", "divergence", "theta_div", "divx", "scalar_next"):
         a[name] = np.zeros(sh["cell"], np.float32)
     a["radbcw"] = np.zeros((nk + 2, nj + 2 * NG), np.float32)
     a["radbce"] = np.zeros((nk + 2, nj + 2 * NG), np.float32)
@@ -119,7 +111,6 @@ def host_initial_state(cfg: CaseConfig) -> dict[str, Any]:
         a["q3d"] = init["q3d"].copy()
         a["qa"] = init["q3d"].copy()
         a["qten"] = np.zeros_like(a["q3d"])
-        a["moisture_padded"] = np.zeros(sh["s"], np.float32)
         for name in ("advx", "advy", "advz", "pp_mass", "fall_velocity") + KESSLER_HELD:
             a[name] = np.zeros(sh["cell"], np.float32)
         thermo = initial_thermodynamics(base, init["th3d"], init["q3d"][0])
@@ -240,22 +231,10 @@ def _rk_stage(st: Gm1LargeStepState, nrk: int) -> None:
                  ptr(a["thterm"]), ptr(a["divergence"]), ptr(a["theta_div"]), ptr(a["divx"]), ni, nj, nk,
                  cfg.dx, cfg.dy, cfg.dz, h)
     mass = {"rru": a["rru"], "rrv": a["rrv"], "rrw": a["rrw"]}
-    grid = dict(ni=ni, nj=nj, nk=nk, dt=cfg.dt, dx=cfg.dx, dy=cfg.dy, dz=cfg.dz, lateral_sides=sides,
-                stream=st.stream)
-    # theta: (th0 - 300) + th3d with family halos, then advection + A-anchored compose in one pass
-    pack_scalar(b["th0"], a["th3d"], a["theta_padded"], ni=ni, nj=nj, nk=nk, theta=True, lateral_sides=sides,
-                stream=st.stream)
-    launch_scalar_rk_compose(a["theta_padded"], mass, m, a["divx"], a["tha"], a["thten1"], a["theta_div"],
-                             a["th3d"], nrk=nrk, weps=THETA_WEPS, **grid)
-    for sp in range(st.numq):
-        pack_scalar(b["th0"], q3d[sp], a["moisture_padded"], ni=ni, nj=nj, nk=nk, theta=False,
-                    lateral_sides=sides, stream=st.stream)
-        if nrk < 3:
-            launch_scalar_rk_compose(a["moisture_padded"], mass, m, a["divx"], a["qa"][sp], a["qten"][sp], None,
-                                     q3d[sp], nrk=nrk, weps=moisture_weps_f64(sp), **grid)
-        else:
-            launch_pdef_moisture(a["moisture_padded"], mass, m, a["divx"], a["qa"][sp], a["qten"][sp], q3d[sp],
-                                 {"advx": a["advx"], "advy": a["advy"], "advz": a["advz"]}, species=sp, **grid)
+    # theta' and each moisture species advected straight from their carriers (the packed
+    # (th0 - 300) + theta' and the lateral-family halos formed in the tiles' loads) and
+    # composed A-anchored in one pass
+    _scalar_stages(st, nrk, mass, flat_column=None, rdz=float(np.float32(1.0) / np.float32(cfg.dz)))
 
 
 def _flat_rk_stage(st: Gm1LargeStepState, nrk: int) -> None:
@@ -264,10 +243,9 @@ def _flat_rk_stage(st: Gm1LargeStepState, nrk: int) -> None:
     No bcwsfc/bc2d (terrain_flag statements), then the flat momentum,
     buoyancy, acoustic stage, getdiv/getdivx, theta and moisture texts.
     """
-    cfg, a, m, b = st.cfg, st.a, st.metrics, st.base
+    cfg, a, m = st.cfg, st.a, st.metrics
     ni, nj, nk = cfg.ni, cfg.nj, cfg.nk
     sides = cfg.lateral_sides
-    q3d = a.get("q3d")
     launch_cuda_gm1_child_stretched_momentum5(a, m, st.flat, ni=ni, nj=nj, nk=nk, nrk=nrk, dx=cfg.dx, dy=cfg.dy,
                                               divx=a["divx"], stream=st.stream, lateral_sides=sides)
     launch_cuda_gm1_flat_stretched_buoyancy(a, m, ni=ni, nj=nj, nk=nk, stream=st.stream)
@@ -279,21 +257,27 @@ def _flat_rk_stage(st: Gm1LargeStepState, nrk: int) -> None:
                                                   theta_divergence_term=a["theta_div"], divx=a["divx"],
                                                   stream=st.stream)
     mass = {"rru": a["rru"], "rrv": a["rrv"], "rrw": a["rrw"]}
-    grid = dict(ni=ni, nj=nj, nk=nk, dt=cfg.dt, dx=cfg.dx, dy=cfg.dy, lateral_sides=sides, stream=st.stream)
-    pack_scalar(b["th0"], a["th3d"], a["theta_padded"], ni=ni, nj=nj, nk=nk, theta=True, lateral_sides=sides,
-                stream=st.stream)
-    launch_flat_scalar_rk_compose(a["theta_padded"], mass, m, st.flat, a["divx"], a["tha"], a["thten1"],
-                                  a["theta_div"], a["th3d"], nrk=nrk, weps=THETA_WEPS, **grid)
+    _scalar_stages(st, nrk, mass, flat_column=st.flat.column, rdz=float(st.flat.rdz))
+
+
+def _scalar_stages(st: Gm1LargeStepState, nrk: int, mass: dict[str, Any], *, flat_column: Any | None,
+                   rdz: float) -> None:
+    """Theta' then each moisture species (the final stage's rows through the pdef chain)."""
+    cfg, a, m, b = st.cfg, st.a, st.metrics, st.base
+    q3d = a.get("q3d")
+    grid = dict(ni=cfg.ni, nj=cfg.nj, nk=cfg.nk, nrk=nrk, dt=cfg.dt, dx=cfg.dx, dy=cfg.dy, rdz=rdz,
+                lateral_sides=cfg.lateral_sides, flat_column=flat_column, stream=st.stream)
+    launch_scalar_stage_from_carrier(a["th3d"], b["th0"], mass, m, a["divx"], a["tha"], a["thten1"],
+                                     a["theta_div"], a["scalar_next"], weps=THETA_WEPS, **grid)
     for sp in range(st.numq):
-        pack_scalar(b["th0"], q3d[sp], a["moisture_padded"], ni=ni, nj=nj, nk=nk, theta=False,
-                    lateral_sides=sides, stream=st.stream)
         if nrk < 3:
-            launch_flat_scalar_rk_compose(a["moisture_padded"], mass, m, st.flat, a["divx"], a["qa"][sp],
-                                          a["qten"][sp], None, q3d[sp], nrk=nrk, weps=moisture_weps_f64(sp), **grid)
+            launch_scalar_stage_from_carrier(q3d[sp], None, mass, m, a["divx"], a["qa"][sp], a["qten"][sp], None,
+                                             a["scalar_next"], weps=moisture_weps_f64(sp), **grid)
         else:
-            launch_cuda_gm1_child_stretched_pdef_moisture(
-                a["moisture_padded"], mass, m, st.flat, a["divx"], a["qa"][sp], a["qten"][sp], q3d[sp],
-                {"advx": a["advx"], "advy": a["advy"], "advz": a["advz"]}, species=sp, **grid)
+            launch_scalar_stage_from_carrier(q3d[sp], None, mass, m, a["divx"], a["qa"][sp], a["qten"][sp], None,
+                                             a["scalar_next"], pdef_species=sp,
+                                             pdef_scratch={"advx": a["advx"], "advy": a["advy"],
+                                                           "advz": a["advz"]}, **grid)
 
 
 def launch_cuda_gm1_large_step(st: Gm1LargeStepState) -> dict[str, Any]:

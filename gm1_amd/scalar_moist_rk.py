@@ -25,6 +25,8 @@ from ._native import D, Fl, I, P, U, signature
 signature("gm1hip_scalar_rk_stage", *([P] * 13), I, I, I, I, D, Fl, Fl, Fl, U, Fl, P)
 signature("gm1hip_pdef_moisture_stage", *([P] * 17), I, I, I, I, D, Fl, Fl, Fl, U, Fl, P)
 signature("gm1hip_pack_scalar", P, P, P, I, I, I, I, U, P)
+signature("gm1hip_scalar_rk_stage_src", *([P] * 15), I, I, I, I, D, Fl, Fl, Fl, U, Fl, P)
+signature("gm1hip_pdef_moisture_stage_src", *([P] * 18), I, I, I, I, D, Fl, Fl, Fl, U, Fl, P)
 
 CM1_NRKMAX = 3
 CM1_TH0R_F32 = 300.0
@@ -114,7 +116,71 @@ def launch_pdef_moisture(
                  stream_handle(stream))
 
 
+def launch_scalar_stage_from_carrier(
+    carrier: Any, th0: Any | None, mass_flux: Mapping[str, Any], metrics: Mapping[str, Any], divx: Any, a0: Any,
+    held: Any, divterm: Any | None, scratch: Any, *, ni: int, nj: int, nk: int, nrk: int, dt: float, dx: float,
+    dy: float, rdz: float, lateral_sides, flat_column: Any | None = None, pdef_species: int | None = None,
+    pdef_scratch: Mapping[str, Any] | None = None, weps: float | None = None, stream: Any | None = None,
+) -> None:
+    """One scalar's RK stage read straight from its unpadded carrier (nk, nj, ni), result back in it.
+
+    The pack into a padded carrier and its lateral-family fill
+    (cuda_scalar_moist_rk.py:255-280, 1041-1160) are folded into the tiles'
+    loads (``gm1hip_scalar_rk_stage_src`` / ``gm1hip_pdef_moisture_stage_src``);
+    ``th0`` makes the carrier theta' (packed as (th0 - 300) + theta').  The
+    kernel writes ``scratch`` (cell-shaped) -- a tile reads its neighbours'
+    halo words of the carrier, so the carrier is updated by a copy after it.
+    Terrain columns take ``metrics["gz"]``/``["rgz"]`` and ``rdz = 1/dz``; flat
+    stretched columns ``flat_column`` and the metric rdz.  ``pdef_species``
+    selects the final-stage positive-definite moisture chain.
+    """
+    cell = (nk, nj, ni)
+    sh = shapes(ni, nj, nk)
+    for name, t in (("carrier", carrier), ("divx", divx), ("a0", a0), ("held", held), ("scratch", scratch)):
+        require(t, name, cell)
+    if divterm is not None:
+        require(divterm, "divterm", cell)
+    if carrier.data_ptr() == scratch.data_ptr():
+        raise ValueError("scratch must not alias the carrier")
+    rru, rrv, rrw = (require(mass_flux["rru"], "rru", sh["u"]), require(mass_flux["rrv"], "rrv", sh["v"]),
+                     require(mass_flux["rrw"], "rrw", sh["w"]))
+    terrain = flat_column is None
+    gz = metrics["gz"] if terrain else None
+    obc = lateral_code(lateral_sides)
+    dttmp = scalar_moist_rk_stage_timestep_f32(dt, nrk)
+    h = stream_handle(stream)
+    if pdef_species is None:
+        use_weno = int(nrk == CM1_NRKMAX)
+        _native.call("gm1hip_scalar_rk_stage_src", ptr(carrier), ptr(th0), ptr(rru), ptr(rrv), ptr(rrw),
+                     ptr(metrics["c1"]), ptr(metrics["c2"]), ptr(metrics["rr0"]), ptr(gz), ptr(flat_column),
+                     ptr(divx), ptr(a0), ptr(held), ptr(divterm), ptr(scratch), ni, nj, nk, use_weno, float(weps),
+                     float(dx), float(dy), float(rdz), obc, dttmp, h)
+    else:
+        if nrk != CM1_NRKMAX or th0 is not None:
+            raise ValueError("the positive-definite chain is the final stage's moisture row")
+        rgz = metrics["rgz"] if terrain else None
+        ps = pdef_scratch or {}
+        for name in ("advx", "advy", "advz"):
+            require(ps[name], name, cell)
+        _native.call("gm1hip_pdef_moisture_stage_src", ptr(carrier), ptr(rru), ptr(rrv), ptr(rrw),
+                     ptr(metrics["c1"]), ptr(metrics["c2"]), ptr(metrics["rho0_3d"]), ptr(metrics["rr0"]), ptr(gz),
+                     ptr(rgz), ptr(flat_column), ptr(divx), ptr(a0), ptr(held), ptr(scratch), ptr(ps["advx"]),
+                     ptr(ps["advy"]), ptr(ps["advz"]), ni, nj, nk, int(pdef_species != 0),
+                     moisture_weps_f64(pdef_species), float(dx), float(dy), float(rdz), obc, dttmp, h)
+    with torch.cuda.stream(stream) if stream is not None else _NoStream():
+        carrier.copy_(scratch)
+
+
+class _NoStream:
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        return False
+
+
 __all__ = [
     "CM1_QMAG_F32", "CM1_SCALAR_MOIST_SPECIES", "launch_pdef_moisture", "launch_scalar_rk_compose",
+    "launch_scalar_stage_from_carrier",
     "moisture_weps_f64", "pack_scalar", "scalar_moist_rk_stage_timestep_f32", "theta_weps_f64",
 ]

@@ -275,6 +275,30 @@ int gm1hip_pdef_moisture_stage_flat(const float* scalar, const float* rru, const
                                     int pdefweno, double weps, float dx, float dy, float rdz, unsigned obc,
                                     float dttmp, void* stream);
 
+// gm1hip_scalar_rk_stage (terrain: gz, rdz = 1/dz) and
+// gm1hip_scalar_rk_stage_flat (flat_column, metric rdz) reading the scalar
+// from its unpadded carrier `src` (nk, nj, ni) -- theta' with th0 (nk) for the
+// packed theta (th0 - 300) + theta' of cuda_scalar_moist_rk.py:255-280, or a
+// moisture species with th0 = NULL -- with the pack and the lateral family
+// fill (cuda_boundary_families.py:887-938) folded into the loads.  Exactly one
+// of gz and flat_column is non-NULL.  The large step's own call: the
+// reference packs into a padded carrier first (cuda_scalar_moist_rk.py:1041-1160).
+int gm1hip_scalar_rk_stage_src(const float* src, const float* th0, const float* rru, const float* rrv,
+                               const float* rrw, const float* c1, const float* c2, const float* rr0,
+                               const float* gz, const float* flat_column, const float* divx, const float* a0,
+                               const float* held, const float* divterm, float* out, int ni, int nj, int nk,
+                               int use_weno, double weps, float dx, float dy, float rdz, unsigned obc, float dttmp,
+                               void* stream);
+
+// gm1hip_pdef_moisture_stage / _flat on an unpadded moisture carrier `src`,
+// as gm1hip_scalar_rk_stage_src (terrain: gz and rgz; flat: flat_column).
+int gm1hip_pdef_moisture_stage_src(const float* src, const float* rru, const float* rrv, const float* rrw,
+                                   const float* c1, const float* c2, const float* rho0_3d, const float* rr0,
+                                   const float* gz, const float* rgz, const float* flat_column, const float* divx,
+                                   const float* qa, const float* held, float* out, float* advx, float* advy,
+                                   float* advz, int ni, int nj, int nk, int pdefweno, double weps, float dx, float dy,
+                                   float rdz, unsigned obc, float dttmp, void* stream);
+
 // Replaces launch_cuda_gm1_flat_acoustic_stage for the all-open Durran-Klemp
 // family (cuda_gm1_flat_lateral_acoustic.py:117; cuda_gm1_child_psolver3_open.py
 // :340-535, 1618-1792): flat coefficients, flat ssopenbc, the pressure

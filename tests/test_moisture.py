@@ -148,3 +148,67 @@ def test_gpu_fused_scalar_compose_bit_exact(device, nrk):
                              weps=weps, lateral_sides=sides)
     torch.cuda.synchronize()
     assert_bits_equal(to_host(out), want, f"fused compose nrk={nrk}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sides", [(3, 4, 2, 2), (1, 1, 1, 1), (2, 2, 4, 3)])
+@pytest.mark.parametrize("row", ["theta-nrk1", "theta-nrk3", "qv-pdef", "qc-pdef"])
+def test_gpu_stage_from_unpadded_carrier_bit_exact(device, sides, row):
+    """The carrier entry points (pack and lateral fill formed in the tiles' loads) against the oracle's
+    pack + fill + advection + compose, on rigid, periodic, open and mixed sides."""
+    import torch
+
+    from gm1_amd.scalar_moist_rk import launch_scalar_stage_from_carrier
+    from oracle import boundary as obc
+    from oracle import stage as ost
+    from oracle.advection5 import cpu_scalar_advection5_inplace
+
+    ni, nj, nk = 33, 19, 12
+    rng = np.random.default_rng(300 + len(row) + sum(sides))
+    a, m, divx = _moist_inputs(rng, ni, nj, nk)
+    sh = shapes(ni, nj, nk)
+    cell = (nk, nj, ni)
+    dt, dx, dy, dz = 1.5, 250.0, 300.0, 200.0
+    s = np.zeros(sh["s"], F)
+    d = lambda x: to_dev(x, device)  # noqa: E731
+    mass = {k: d(a[k]) for k in ("rru", "rrv", "rrw")}
+    scratch = torch.zeros(cell, dtype=torch.float32, device=device)
+    rdz = float(F(1.0) / F(dz))
+    if row.startswith("theta"):
+        nrk = 1 if row.endswith("1") else 3
+        th0 = (F(300.0) + np.linspace(0.0, 40.0, nk)).astype(F)
+        th = rng.normal(0.0, 2.0, cell).astype(F)
+        ost.pack_theta(th0, th, s, ni=ni, nj=nj, nk=nk)
+        obc.fill(s, obc.KIND_SCALAR, sides, ni=ni, nj=nj, nk=nk)
+        tha = rng.normal(0.0, 2.0, cell).astype(F)
+        held = rng.normal(0.0, 1e-3, cell).astype(F)
+        divterm = rng.normal(0.0, 1e-3, cell).astype(F)
+        adv = np.zeros(cell, F)
+        fl = {"x": np.zeros(sh["u"], F), "y": np.zeros(sh["v"], F), "z": np.zeros(sh["w"], F)}
+        weps = float(F(F(10.0) * F(1e-18)))
+        cpu_scalar_advection5_inplace(s, a, m, divx, adv, fl, ni=ni, nj=nj, nk=nk, nrk=nrk, dx=dx, dy=dy, dz=dz,
+                                      weps=weps, lateral_sides=sides)
+        want = ost.compose_theta(tha, held, divterm, adv, F(F(dt) / F(4 - nrk)))
+        carrier = d(th)
+        launch_scalar_stage_from_carrier(carrier, d(th0), mass, d(m), d(divx), d(tha), d(held), d(divterm), scratch,
+                                         ni=ni, nj=nj, nk=nk, nrk=nrk, dt=dt, dx=dx, dy=dy, rdz=rdz,
+                                         lateral_sides=sides, weps=weps)
+    else:
+        species = 0 if row.startswith("qv") else 1
+        q = np.abs(rng.normal(0.0, 1e-3, cell)).astype(F)
+        q[rng.random(cell) < 0.3] = F(0.0)
+        qa = np.abs(q + rng.normal(0.0, 2e-4, cell)).astype(F)
+        held = rng.normal(0.0, 1e-7, cell).astype(F)
+        s[1 : nk + 1, NG : NG + nj, NG : NG + ni] = q
+        obc.fill(s, obc.KIND_SCALAR, sides, ni=ni, nj=nj, nk=nk)
+        weps = float(F(F(1e-2) * F(1e-18)))
+        adv = op.pdef_advection(qa, s, a, m, divx, ni=ni, nj=nj, nk=nk, dt=F(dt), dx=dx, dy=dy, dz=dz, weps=weps,
+                                pdefweno=species != 0, sides=sides)
+        want = ost.compose_moisture(qa, held, adv, F(dt))
+        carrier = d(q)
+        ps = {n: torch.zeros(cell, dtype=torch.float32, device=device) for n in ("advx", "advy", "advz")}
+        launch_scalar_stage_from_carrier(carrier, None, mass, d(m), d(divx), d(qa), d(held), None, scratch,
+                                         ni=ni, nj=nj, nk=nk, nrk=3, dt=dt, dx=dx, dy=dy, rdz=rdz,
+                                         lateral_sides=sides, pdef_species=species, pdef_scratch=ps)
+    torch.cuda.synchronize()
+    assert_bits_equal(to_host(carrier), want, f"{row} sides={sides}")
