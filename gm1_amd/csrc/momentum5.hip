@@ -53,26 +53,34 @@ GM1_DEV int wf(int i, int n, bool lo, bool hi) { return (lo && i < 0) ? i + n : 
 GM1_DEV int clampi(int a, int n) { return a < 0 ? 0 : (a >= n ? n - 1 : a); }
 
 // ------------------------------------------------------------- mass fluxes
+// Levels per thread of the mass-flux kernels (loads of MKB levels in flight).
+constexpr int MKB = 8;
+
 __global__ __launch_bounds__(256) void k_mom_mass_h(
     const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ rho0,
     const float* __restrict__ rgzu, const float* __restrict__ rgzv, float* __restrict__ rru,
     float* __restrict__ rrv, int ni, int nj, int nk) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;  // 0..ni (face or cell)
   const int j = blockIdx.y;                             // 0..nj (row or face)
-  const int k0 = blockIdx.z, k = k0 + 1;
   const float half = 0.5f;
-  const int64_t plane = (int64_t)k0 * nj * ni;
-  if (i <= ni && j < nj) {
-    const int l = (i == 0) ? 0 : i - 1, r = (i == ni) ? ni - 1 : i;
-    float d = rho0[plane + (int64_t)j * ni + l] + rho0[plane + (int64_t)j * ni + r];
-    const int64_t p = ix3(k, j + NG, i + NG, sy(nj), ux(ni));
-    rru[p] = (half * d) * (u[p] * rgzu[(int64_t)j * (ni + 1) + i]);
-  }
-  if (i < ni && j <= nj) {
-    const int b = (j == 0) ? 0 : j - 1, t = (j == nj) ? nj - 1 : j;
-    float d = rho0[plane + (int64_t)b * ni + i] + rho0[plane + (int64_t)t * ni + i];
-    const int64_t p = ix3(k, j + NG, i + NG, vy(nj), sx(ni));
-    rrv[p] = (half * d) * (v[p] * rgzv[(int64_t)j * ni + i]);
+  // MKB levels per thread, the loads of all of them in flight together (past
+  // the top the level is clamped and its words are stored again)
+#pragma unroll
+  for (int uu = 0; uu < MKB; ++uu) {
+    const int k0 = min((int)blockIdx.z * MKB + uu, nk - 1), k = k0 + 1;  // clamped: the top level again
+    const int64_t plane = (int64_t)k0 * nj * ni;
+    if (i <= ni && j < nj) {
+      const int l = (i == 0) ? 0 : i - 1, r = (i == ni) ? ni - 1 : i;
+      float d = rho0[plane + (int64_t)j * ni + l] + rho0[plane + (int64_t)j * ni + r];
+      const int64_t p = ix3(k, j + NG, i + NG, sy(nj), ux(ni));
+      rru[p] = (half * d) * (u[p] * rgzu[(int64_t)j * (ni + 1) + i]);
+    }
+    if (i < ni && j <= nj) {
+      const int b = (j == 0) ? 0 : j - 1, t = (j == nj) ? nj - 1 : j;
+      float d = rho0[plane + (int64_t)b * ni + i] + rho0[plane + (int64_t)t * ni + i];
+      const int64_t p = ix3(k, j + NG, i + NG, vy(nj), sx(ni));
+      rrv[p] = (half * d) * (v[p] * rgzv[(int64_t)j * ni + i]);
+    }
   }
 }
 
@@ -92,27 +100,32 @@ __global__ __launch_bounds__(256) void k_mom_mass_v(
   const float ddx = dzdx[h], ddy = dzdy[h], g = gz[h];
   const int64_t wst = (int64_t)nyu * nxs;
   const int64_t wcol = (int64_t)y * nxs + x;
-  rrw[wst + wcol] = 0.0f;  // kf = 1
-  float below = 0.0f;
-  for (int kf = 2; kf <= nk + 1; ++kf) {
-    float val;
-    if (kf == nk + 1) {
-      val = 0.0f;
-    } else {
-      const float sf = (float)(kf - 1) * dz, sm = (float)((double)kf - 1.5) * dz;
-      float r2 = (sf - sm) * rdz;
-      const float r1 = half * (1.0f - r2);
-      r2 = half * r2;
-      float uxv = r2 * (rru[ix3(kf, y, x, nyu, nxu)] + rru[ix3(kf, y, x + 1, nyu, nxu)]) +
-                  r1 * (rru[ix3(kf - 1, y, x, nyu, nxu)] + rru[ix3(kf - 1, y, x + 1, nyu, nxu)]);
-      float vyv = r2 * (rrv[ix3(kf, y, x, nyv, nxs)] + rrv[ix3(kf, y + 1, x, nyv, nxs)]) +
-                  r1 * (rrv[ix3(kf - 1, y, x, nyv, nxs)] + rrv[ix3(kf - 1, y + 1, x, nyv, nxs)]);
-      float t = uxv * ddx + vyv * ddy;
-      t = t * (sf - zt);
-      t = t * g;
-      t = t * rzt;
-      val = rf0[(int64_t)(kf - 1) * nj * ni + h] * w[(int64_t)kf * wst + wcol] + t;
-    }
+  // rrw at w level kf (0 at kf = 1 and nk + 1)
+  auto rrw_at = [&](int kf) {
+    if (kf <= 1 || kf > nk) return 0.0f;
+    const float sf = (float)(kf - 1) * dz, sm = (float)((double)kf - 1.5) * dz;
+    float r2 = (sf - sm) * rdz;
+    const float r1 = half * (1.0f - r2);
+    r2 = half * r2;
+    float uxv = r2 * (rru[ix3(kf, y, x, nyu, nxu)] + rru[ix3(kf, y, x + 1, nyu, nxu)]) +
+                r1 * (rru[ix3(kf - 1, y, x, nyu, nxu)] + rru[ix3(kf - 1, y, x + 1, nyu, nxu)]);
+    float vyv = r2 * (rrv[ix3(kf, y, x, nyv, nxs)] + rrv[ix3(kf, y + 1, x, nyv, nxs)]) +
+                r1 * (rrv[ix3(kf - 1, y, x, nyv, nxs)] + rrv[ix3(kf - 1, y + 1, x, nyv, nxs)]);
+    float t = uxv * ddx + vyv * ddy;
+    t = t * (sf - zt);
+    t = t * g;
+    t = t * rzt;
+    return rf0[(int64_t)(kf - 1) * nj * ni + h] * w[(int64_t)kf * wst + wcol] + t;
+  };
+  // this thread's chunk: w levels kf0 .. kf0+MKB-1 of 2..nk+1 (rrw(1) = 0 by the first chunk)
+  const int kf0 = 2 + (int)blockIdx.z * MKB;
+  if (kf0 == 2) rrw[wst + wcol] = 0.0f;
+  float below = rrw_at(kf0 - 1);
+#pragma unroll
+  for (int q = 0; q < MKB; ++q) {
+    // clamped past w level nk+1: the top pair is formed and stored again
+    const int kf = min(kf0 + q, nk + 1);
+    const float val = rrw_at(kf);
     rrw[(int64_t)kf * wst + wcol] = val;
     // divx at scalar level k = kf-1 from the rrw pair (k, k+1)
     const int k = kf - 1;
@@ -120,7 +133,7 @@ __global__ __launch_bounds__(256) void k_mom_mass_v(
               (rrv[ix3(k, y + 1, x, nyv, nxs)] - rrv[ix3(k, y, x, nyv, nxs)]) * rdy;
     d = d + (val - below) * rdz;
     divx[(int64_t)(k - 1) * nj * ni + h] = (fabsf(d) < 1.0e-20f) ? 0.0f : d;
-    below = val;
+    below = (kf <= nk) ? val : below;  // the top pair keeps its lower word for the clamped repeats
   }
 }
 
@@ -524,11 +537,11 @@ int gm1hip_momentum_advection5(
   hipStream_t s = (hipStream_t)stream;
   const float rdx = 1.0f / dx, rdy = 1.0f / dy, rdz = 1.0f / dz;
   dim3 blk(64);
-  hipLaunchKernelGGL(k_mom_mass_h, dim3((ni + 1 + 63) / 64, nj + 1, nk), blk, 0, s, u3d, v3d, rho0_3d,
-                     rgzu, rgzv, rru, rrv, ni, nj, nk);
+  hipLaunchKernelGGL(k_mom_mass_h, dim3((ni + 1 + 63) / 64, nj + 1, (nk + MKB - 1) / MKB), blk, 0, s, u3d, v3d,
+                     rho0_3d, rgzu, rgzv, rru, rrv, ni, nj, nk);
   GM1_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_mom_mass_v, dim3((ni + 63) / 64, nj), blk, 0, s, w3d, rf0, dzdx, dzdy, gz, rru,
-                     rrv, rrw, divx, ni, nj, nk, dz, rdx, rdy);
+  hipLaunchKernelGGL(k_mom_mass_v, dim3((ni + 63) / 64, nj, (nk + MKB - 1) / MKB), blk, 0, s, w3d, rf0, dzdx, dzdy,
+                     gz, rru, rrv, rrw, divx, ni, nj, nk, dz, rdx, rdy);
   GM1_LAUNCH_CHECK();
   if (stop_after_divergence) return 0;
   MomArgs a;
