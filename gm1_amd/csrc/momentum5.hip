@@ -145,15 +145,18 @@ __global__ __launch_bounds__(256) void k_mom_mass_h_flat(
     float* __restrict__ rru, float* __restrict__ rrv, int ni, int nj, int nk) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
-  const int k0 = blockIdx.z, k = k0 + 1;
-  const float rc = rho0[(int64_t)k0 * nj * ni];
-  if (i <= ni && j < nj) {
-    const int64_t p = ix3(k, j + NG, i + NG, sy(nj), ux(ni));
-    rru[p] = rc * u[p];
-  }
-  if (i < ni && j <= nj) {
-    const int64_t p = ix3(k, j + NG, i + NG, vy(nj), sx(ni));
-    rrv[p] = rc * v[p];
+#pragma unroll
+  for (int uu = 0; uu < MKB; ++uu) {
+    const int k0 = min((int)blockIdx.z * MKB + uu, nk - 1), k = k0 + 1;  // clamped: the top level again
+    const float rc = rho0[(int64_t)k0 * nj * ni];
+    if (i <= ni && j < nj) {
+      const int64_t p = ix3(k, j + NG, i + NG, sy(nj), ux(ni));
+      rru[p] = rc * u[p];
+    }
+    if (i < ni && j <= nj) {
+      const int64_t p = ix3(k, j + NG, i + NG, vy(nj), sx(ni));
+      rrv[p] = rc * v[p];
+    }
   }
 }
 
@@ -169,17 +172,24 @@ __global__ __launch_bounds__(256) void k_mom_mass_v_flat(
   const int64_t h = (int64_t)j * ni + i;
   const int64_t wst = (int64_t)nyu * nxs;
   const int64_t wcol = (int64_t)y * nxs + x;
-  rrw[wst + wcol] = 0.0f;
-  float below = 0.0f;
-  for (int kf = 2; kf <= nk + 1; ++kf) {
-    const float val = (kf == nk + 1) ? 0.0f : rf0[(int64_t)(kf - 1) * nj * ni] * w[(int64_t)kf * wst + wcol];
+  auto rrw_at = [&](int kf) {
+    return (kf <= 1 || kf > nk) ? 0.0f : rf0[(int64_t)(kf - 1) * nj * ni] * w[(int64_t)kf * wst + wcol];
+  };
+  // this thread's chunk: w levels kf0 .. kf0+MKB-1 of 2..nk+1 (rrw(1) = 0 by the first chunk)
+  const int kf0 = 2 + (int)blockIdx.z * MKB;
+  if (kf0 == 2) rrw[wst + wcol] = 0.0f;
+  float below = rrw_at(kf0 - 1);
+#pragma unroll
+  for (int q = 0; q < MKB; ++q) {
+    const int kf = min(kf0 + q, nk + 1);  // clamped past w level nk+1: the top pair again
+    const float val = rrw_at(kf);
     rrw[(int64_t)kf * wst + wcol] = val;
     const int k = kf - 1;
     float d = (rru[ix3(k, y, x + 1, nyu, nxu)] - rru[ix3(k, y, x, nyu, nxu)]) * rdx +
               (rrv[ix3(k, y + 1, x, nyv, nxs)] - rrv[ix3(k, y, x, nyv, nxs)]) * rdy;
     d = d + ((val - below) * rdz) * mh[k];
     divx[(int64_t)(k - 1) * nj * ni + h] = (fabsf(d) < 1.0e-30f) ? 0.0f : d;
-    below = val;
+    below = (kf <= nk) ? val : below;  // the top pair keeps its lower word for the clamped repeats
   }
 }
 
@@ -601,11 +611,11 @@ int gm1hip_momentum_advection5_flat(
   hipStream_t s = (hipStream_t)stream;
   const float rdx = 1.0f / dx, rdy = 1.0f / dy;
   dim3 blk(64);
-  hipLaunchKernelGGL(k_mom_mass_h_flat, dim3((ni + 1 + 63) / 64, nj + 1, nk), blk, 0, s, u3d, v3d, rho0_3d, rru,
-                     rrv, ni, nj, nk);
+  hipLaunchKernelGGL(k_mom_mass_h_flat, dim3((ni + 1 + 63) / 64, nj + 1, (nk + MKB - 1) / MKB), blk, 0, s, u3d, v3d,
+                     rho0_3d, rru, rrv, ni, nj, nk);
   GM1_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_mom_mass_v_flat, dim3((ni + 63) / 64, nj), blk, 0, s, w3d, rf0, rru, rrv, flat_column, rrw,
-                     divx, ni, nj, nk, rdx, rdy, rdz);
+  hipLaunchKernelGGL(k_mom_mass_v_flat, dim3((ni + 63) / 64, nj, (nk + MKB - 1) / MKB), blk, 0, s, w3d, rf0, rru, rrv,
+                     flat_column, rrw, divx, ni, nj, nk, rdx, rdy, rdz);
   GM1_LAUNCH_CHECK();
   MomArgs a;
   a.u = u3d; a.v = v3d; a.w = w3d; a.rru = rru; a.rrv = rrv; a.rrw = rrw; a.divx = divx;

@@ -31,43 +31,34 @@ __global__ __launch_bounds__(256) void k_stage_terms(
     float* __restrict__ ppterm, float* __restrict__ dum8, float* __restrict__ thterm, int ni, int nj, int nk,
     int numq) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y;
+  const int j = blockIdx.y, k = blockIdx.z;
   if (i >= ni) return;
   const float cv = CP - RD, cvv = CPV - RV, rovcp = RD / CP, reps = RV / RD, repsm1 = reps - 1.0f;
   const int64_t n = (int64_t)ni * nj * nk;
-  // KB levels per thread: the levels' loads are independent of the stores
-  // (restrict), so all of them are in flight before the first store
-#pragma unroll
-  for (int u = 0; u < KB; ++u) {
-    // past the column's top the level is clamped: this thread forms and
-    // stores its top level's words again (no branch splits the batch)
-    const int k = min((int)blockIdx.z * KB + u, nk - 1);
-    const int64_t z = ((int64_t)k * nj + j) * ni + i;
-    // species beyond numq are exact zeros (a dry state, or warm rain without ice)
-    const float q0 = numq >= 1 ? q3d[z] : 0.0f;
-    const float q1 = numq >= 2 ? q3d[n + z] : 0.0f, q2 = numq >= 3 ? q3d[2 * n + z] : 0.0f;
-    const float q3 = numq >= 4 ? q3d[3 * n + z] : 0.0f, q4 = numq >= 5 ? q3d[4 * n + z] : 0.0f;
-    const float q5 = numq >= 6 ? q3d[5 * n + z] : 0.0f;
-    const float qv = fmaxf(q0, 0.0f);
-    const float liq = q1 + q2;
-    const float ice = (q3 + q4) + q5;
-    const float qli = fmaxf(liq + ice, 0.0f);
-    const float cpli = CPL * fmaxf(liq, 0.0f) + CPI * fmaxf(ice, 0.0f);
-    const float bt = th0[k], bq = qv0[k], bp = pi0[k];
-    const float rth = 1.0f / bt, theta = bt + th3d[z];
-    const float bc = qc0 ? qc0[k] : 0.0f, bi = qi0 ? qi0[k] : 0.0f;
-    float b = th3d[z] * rth + repsm1 * (qv - bq);
-    b = b - ((qli - bc) - bi);
-    const int64_t p = pix(k + 1, j + 1, i + 1, ni, nj);
-    const float pi = bp + pp3d[p];
-    const float tv = (theta * (1.0f + reps * qv)) / ((1.0f + qv) + qli);
-    const float cpm = (CP + CPV * qv) + cpli;
-    const float cvm = 1.0f / ((cv + cvv * qv) + cpli);
-    dum8[ix3(k + 1, j + NG, i + NG, sy(nj), sx(ni))] = GRAV * b;
-    thv[p] = tv;
-    thterm[z] = (theta * ((RD + RV * qv) - rovcp * cpm)) * cvm;
-    ppterm[p] = ((pi * rovcp) * cpm) * cvm;
-  }
+  const int64_t z = ((int64_t)k * nj + j) * ni + i;
+  // species beyond numq are exact zeros (a dry state, or warm rain without ice)
+  const float q0 = numq >= 1 ? q3d[z] : 0.0f;
+  const float q1 = numq >= 2 ? q3d[n + z] : 0.0f, q2 = numq >= 3 ? q3d[2 * n + z] : 0.0f;
+  const float q3 = numq >= 4 ? q3d[3 * n + z] : 0.0f, q4 = numq >= 5 ? q3d[4 * n + z] : 0.0f;
+  const float q5 = numq >= 6 ? q3d[5 * n + z] : 0.0f;
+  const float qv = fmaxf(q0, 0.0f);
+  const float liq = q1 + q2;
+  const float ice = (q3 + q4) + q5;
+  const float qli = fmaxf(liq + ice, 0.0f);
+  const float cpli = CPL * fmaxf(liq, 0.0f) + CPI * fmaxf(ice, 0.0f);
+  const float bt = th0[k], bq = qv0[k], bp = pi0[k];
+  const float rth = 1.0f / bt, theta = bt + th3d[z];
+  const float bc = qc0 ? qc0[k] : 0.0f, bi = qi0 ? qi0[k] : 0.0f;
+  float b = th3d[z] * rth + repsm1 * (qv - bq);
+  b = b - ((qli - bc) - bi);
+  dum8[ix3(k + 1, j + NG, i + NG, sy(nj), sx(ni))] = GRAV * b;
+  const int64_t p = pix(k + 1, j + 1, i + 1, ni, nj);
+  thv[p] = (theta * (1.0f + reps * qv)) / ((1.0f + qv) + qli);
+  const float cpm = (CP + CPV * qv) + cpli;
+  const float cvm = 1.0f / ((cv + cvv * qv) + cpli);
+  thterm[z] = (theta * ((RD + RV * qv) - rovcp * cpm)) * cvm;
+  const float pi = bp + pp3d[p];
+  ppterm[p] = ((pi * rovcp) * cpm) * cvm;
 }
 
 __global__ __launch_bounds__(256) void k_piadv(const float* __restrict__ pi0, const float* __restrict__ pp3d,
@@ -313,6 +304,12 @@ __global__ __launch_bounds__(64) void k_getdivx_v(DivArgs a) {
 constexpr int GX_SLOTS = 8;
 constexpr int GX_MAXL = 16;
 
+// Level slots pay only while one thread per column leaves the chip short of
+// waves (fewer than 2^18 columns: 256 CUs x 16 waves x 64 lanes).
+bool slotted_columns(int ni, int nj, int nk) {
+  return (int64_t)ni * nj < (int64_t)1 << 18 && nk <= GX_SLOTS * GX_MAXL;
+}
+
 __global__ __launch_bounds__(64 * GX_SLOTS) void k_getdivx_v_slots(DivArgs a) {
   const int c = threadIdx.x % 64, slot = threadIdx.x / 64;
   const int i = blockIdx.x * 64 + c;
@@ -374,13 +371,24 @@ __global__ __launch_bounds__(256) void k_buoyancy_flat(const float* __restrict__
                                                        const float* __restrict__ c2, float* __restrict__ wten, int ni,
                                                        int nj, int nk) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, kf = blockIdx.z + 2;
+  const int j = blockIdx.y;
   if (i >= ni) return;
-  const int64_t p = ((int64_t)(kf - 1) * nj + j) * ni + i;
   const int nxs = sx(ni), nys = sy(nj);
-  const float add = c1[p] * dum8[ix3(kf - 1, j + NG, i + NG, nys, nxs)] + c2[p] * dum8[ix3(kf, j + NG, i + NG, nys, nxs)];
-  const int64_t d = ix3(kf, j + NG, i + NG, nys, nxs);
-  wten[d] = wten[d] + add;
+  // KB w levels (2..nk) per thread, all read before the first (in-place) write
+  float wt[KB], ad[KB];
+#pragma unroll
+  for (int u = 0; u < KB; ++u) {
+    const int kf = min(2 + (int)blockIdx.z * KB + u, nk);
+    const int64_t p = ((int64_t)(kf - 1) * nj + j) * ni + i;
+    ad[u] = c1[p] * dum8[ix3(kf - 1, j + NG, i + NG, nys, nxs)] + c2[p] * dum8[ix3(kf, j + NG, i + NG, nys, nxs)];
+    wt[u] = wten[ix3(kf, j + NG, i + NG, nys, nxs)];
+  }
+#pragma unroll
+  for (int u = 0; u < KB; ++u) {
+    const int kf = 2 + (int)blockIdx.z * KB + u;
+    if (kf > nk) break;
+    wten[ix3(kf, j + NG, i + NG, nys, nxs)] = wt[u] + ad[u];
+  }
 }
 
 struct FlatDivArgs {
@@ -398,8 +406,12 @@ __global__ __launch_bounds__(64) void k_getdiv_flat(FlatDivArgs a) {
   const int nxu = ux(ni), nyu = sy(nj), nxs = sx(ni), nyv = vy(nj);
   const int x = i + NG, y = j + NG;
   const int64_t h = (int64_t)j * ni + i;
-  float wlo = a.rrw[ix3(1, y, x, nyu, nxs)];
-  for (int k = 1; k <= nk; ++k) {
+  // this thread's chunk of KB levels (clamped past the top: the top level again)
+  const int kb = 1 + (int)blockIdx.z * KB;
+  float wlo = a.rrw[ix3(kb, y, x, nyu, nxs)];
+#pragma unroll
+  for (int q = 0; q < KB; ++q) {
+    const int k = min(kb + q, nk);
     const float whi = a.rrw[ix3(k + 1, y, x, nyu, nxs)];
     float d = (a.rru[ix3(k, y, x + 1, nyu, nxu)] - a.rru[ix3(k, y, x, nyu, nxu)]) * a.rdx +
               (a.rrv[ix3(k, y + 1, x, nyv, nxs)] - a.rrv[ix3(k, y, x, nyv, nxs)]) * a.rdy;
@@ -408,24 +420,32 @@ __global__ __launch_bounds__(64) void k_getdiv_flat(FlatDivArgs a) {
     const int64_t z = (int64_t)(k - 1) * nj * ni + h;
     a.div[z] = d;
     a.theta_div[z] = d * a.thterm[z];
-    wlo = whi;
+    wlo = (k < nk) ? whi : wlo;  // the top level keeps its lower word for the clamped repeats
   }
 }
 
 __global__ __launch_bounds__(256) void k_getdivx_h_flat(FlatDivArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;  // 0..ni
   const int j = blockIdx.y;                             // 0..nj
-  const int k = blockIdx.z + 1;
-  const int ni = a.ni, nj = a.nj;
+  const int ni = a.ni, nj = a.nj, nk = a.nk;
   if (i > ni) return;
-  const float rc = a.rho0[(int64_t)(k - 1) * nj * ni];
-  if (j < nj) {
-    const int64_t p = ix3(k, j + NG, i + NG, sy(nj), ux(ni));
-    a.rru[p] = a.rru[p] * rc;
+  float* __restrict__ rru = a.rru;
+  float* __restrict__ rrv = a.rrv;
+  // KB levels per thread: all faces read first (clamped past the top), then rewritten once
+  float ru[KB], rv[KB];
+#pragma unroll
+  for (int u = 0; u < KB; ++u) {
+    const int k = min((int)blockIdx.z * KB + u, nk - 1) + 1;
+    ru[u] = (j < nj) ? rru[ix3(k, j + NG, i + NG, sy(nj), ux(ni))] : 0.0f;
+    rv[u] = (i < ni) ? rrv[ix3(k, j + NG, i + NG, vy(nj), sx(ni))] : 0.0f;
   }
-  if (i < ni) {
-    const int64_t p = ix3(k, j + NG, i + NG, vy(nj), sx(ni));
-    a.rrv[p] = a.rrv[p] * rc;
+#pragma unroll
+  for (int u = 0; u < KB; ++u) {
+    if ((int)blockIdx.z * KB + u >= nk) break;
+    const int k = (int)blockIdx.z * KB + u + 1;
+    const float rc = a.rho0[(int64_t)(k - 1) * nj * ni];
+    if (j < nj) rru[ix3(k, j + NG, i + NG, sy(nj), ux(ni))] = ru[u] * rc;
+    if (i < ni) rrv[ix3(k, j + NG, i + NG, vy(nj), sx(ni))] = rv[u] * rc;
   }
 }
 
@@ -442,6 +462,46 @@ __global__ __launch_bounds__(64) void k_getdivx_v_flat(FlatDivArgs a) {
   float below = 0.0f;
   for (int kf = 2; kf <= nk + 1; ++kf) {
     const float val = (kf <= nk) ? a.rrw[(int64_t)kf * wst + wc] * a.rf0[(int64_t)(kf - 1) * nj * ni] : 0.0f;
+    a.rrw[(int64_t)kf * wst + wc] = val;
+    const int k = kf - 1;
+    float d = (a.rru[ix3(k, y, x + 1, nyu, nxu)] - a.rru[ix3(k, y, x, nyu, nxu)]) * a.rdx +
+              (a.rrv[ix3(k, y + 1, x, nyv, nxs)] - a.rrv[ix3(k, y, x, nyv, nxs)]) * a.rdy;
+    d = d + ((val - below) * a.rdz) * a.mh[k];
+    a.divx[(int64_t)(k - 1) * nj * ni + h] = (fabsf(d) < 1.0e-30f) ? 0.0f : d;
+    below = val;
+  }
+}
+
+// k_getdivx_v_flat over GX_SLOTS level slots of one workgroup (raw words read
+// before the barrier, rewritten after it), as k_getdivx_v_slots.
+__global__ __launch_bounds__(64 * GX_SLOTS) void k_getdivx_v_flat_slots(FlatDivArgs a) {
+  const int c = threadIdx.x % 64, slot = threadIdx.x / 64;
+  const int i = blockIdx.x * 64 + c;
+  const int j = blockIdx.y;
+  const int ni = a.ni, nj = a.nj, nk = a.nk;
+  const bool live = i < ni;
+  const int L = (nk + GX_SLOTS - 1) / GX_SLOTS;
+  const int kf0 = 2 + slot * L, kf1 = min(kf0 + L - 1, nk + 1);
+  const int nxu = ux(ni), nyu = sy(nj), nxs = sx(ni), nyv = vy(nj);
+  const int x = i + NG, y = j + NG;
+  const int64_t h = (int64_t)j * ni + i;
+  const int64_t wst = (int64_t)nyu * nxs, wc = (int64_t)y * nxs + x;
+  float raw[GX_MAXL + 1];  // raw[q]: averaged w at level kf0 - 1 + q
+#pragma unroll
+  for (int q = 0; q <= GX_MAXL; ++q) {
+    const int kf = kf0 - 1 + q;
+    raw[q] = (live && kf <= kf1 && kf >= 2 && kf <= nk) ? a.rrw[(int64_t)kf * wst + wc] : 0.0f;
+  }
+  __syncthreads();
+  if (!live || kf0 > kf1) return;
+  auto val_at = [&](int kf, float w) { return (kf >= 2 && kf <= nk) ? w * a.rf0[(int64_t)(kf - 1) * nj * ni] : 0.0f; };
+  if (kf0 == 2) a.rrw[wst + wc] = 0.0f;
+  float below = val_at(kf0 - 1, raw[0]);
+#pragma unroll
+  for (int q = 1; q <= GX_MAXL; ++q) {
+    const int kf = kf0 - 1 + q;
+    if (kf > kf1) break;
+    const float val = val_at(kf, raw[q]);
     a.rrw[(int64_t)kf * wst + wc] = val;
     const int k = kf - 1;
     float d = (a.rru[ix3(k, y, x + 1, nyu, nxu)] - a.rru[ix3(k, y, x, nyu, nxu)]) * a.rdx +
@@ -480,17 +540,13 @@ __global__ __launch_bounds__(256) void k_seed_held(const float* __restrict__ u1,
                                                    int nk) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;  // 0..ni
   const int j = blockIdx.y;                             // 0..nj
+  const int k = blockIdx.z + 1;
   if (i > ni) return;
   const int x = i + NG, y = j + NG;
-#pragma unroll
-  for (int u = 0; u < KB; ++u) {
-    const int k = min((int)blockIdx.z * KB + u, nk - 1) + 1;  // clamped: the top level again
-    const int64_t pu = ix3(k, y, x, sy(nj), ux(ni)), pv = ix3(k, y, x, vy(nj), sx(ni));
-    const int64_t pw = ix3(k, y, x, sy(nj), sx(ni));
-    ut[pu] = u1[pu];
-    vt[pv] = v1[pv];
-    wt[pw] = w1[pw];
-  }
+  const int64_t pu = ix3(k, y, x, sy(nj), ux(ni)), pv = ix3(k, y, x, vy(nj), sx(ni)), pw = ix3(k, y, x, sy(nj), sx(ni));
+  ut[pu] = u1[pu];
+  vt[pv] = v1[pv];
+  wt[pw] = w1[pw];
 }
 
 // ----------------------------------------------------------- solve1 held compose
@@ -566,7 +622,7 @@ int gm1hip_stage_terms(const float* th0, const float* qv0, const float* pi0, con
                        float* piadv, float* dum8, float* thterm, int ni, int nj, int nk, unsigned sides,
                        void* stream) {
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_stage_terms, dim3((ni + 63) / 64, nj, (nk + KB - 1) / KB), dim3(64), 0, s, th0, qv0, pi0,
+  hipLaunchKernelGGL(k_stage_terms, dim3((ni + 63) / 64, nj, nk), dim3(64), 0, s, th0, qv0, pi0,
                      qc0, qi0, th3d, q3d,
                      pp3d, thv, ppterm, dum8, thterm, ni, nj, nk, numq);
   hipLaunchKernelGGL(k_piadv, dim3((ni + 2 + 63) / 64, nj + 2, (nk + KB - 1) / KB), dim3(64), 0, s, pi0, pp3d,
@@ -595,7 +651,7 @@ int gm1hip_post_acoustic(float* rru, float* rrv, float* rrw, const float* rho0_3
             ni, nj, nk, 1.0f / dx, 1.0f / dy, dz};
   hipLaunchKernelGGL(k_getdiv, dim3((ni + 63) / 64, nj, (nk + KB - 1) / KB), dim3(64), 0, s, a);
   hipLaunchKernelGGL(k_getdivx_h, dim3((ni + 1 + 63) / 64, nj + 1, (nk + KB - 1) / KB), dim3(64), 0, s, a);
-  if (nk <= GX_SLOTS * GX_MAXL)
+  if (slotted_columns(ni, nj, nk))
     hipLaunchKernelGGL(k_getdivx_v_slots, dim3((ni + 63) / 64, nj), dim3(64 * GX_SLOTS), 0, s, a);
   else
     hipLaunchKernelGGL(k_getdivx_v, dim3((ni + 63) / 64, nj), dim3(64), 0, s, a);
@@ -606,8 +662,8 @@ int gm1hip_post_acoustic(float* rru, float* rrv, float* rrw, const float* rho0_3
 int gm1hip_buoyancy_flat(const float* dum8, const float* c1, const float* c2, float* wten, int ni, int nj, int nk,
                          void* stream) {
   if (nk < 2) return (int)hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_buoyancy_flat, dim3((ni + 63) / 64, nj, nk - 1), dim3(64), 0, (hipStream_t)stream, dum8, c1, c2,
-                     wten, ni, nj, nk);
+  hipLaunchKernelGGL(k_buoyancy_flat, dim3((ni + 63) / 64, nj, (nk - 1 + KB - 1) / KB), dim3(64), 0,
+                     (hipStream_t)stream, dum8, c1, c2, wten, ni, nj, nk);
   GM1_LAUNCH_CHECK();
   return 0;
 }
@@ -619,9 +675,12 @@ int gm1hip_post_acoustic_flat(float* rru, float* rrv, float* rrw, const float* r
   hipStream_t s = (hipStream_t)stream;
   FlatDivArgs a{rru, rrv, rrw, rho0_3d, rf0, flat_column, thterm, divergence, theta_div, divx,
                 ni, nj, nk, 1.0f / dx, 1.0f / dy, rdz};
-  hipLaunchKernelGGL(k_getdiv_flat, dim3((ni + 63) / 64, nj), dim3(64), 0, s, a);
-  hipLaunchKernelGGL(k_getdivx_h_flat, dim3((ni + 1 + 63) / 64, nj + 1, nk), dim3(64), 0, s, a);
-  hipLaunchKernelGGL(k_getdivx_v_flat, dim3((ni + 63) / 64, nj), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(k_getdiv_flat, dim3((ni + 63) / 64, nj, (nk + KB - 1) / KB), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(k_getdivx_h_flat, dim3((ni + 1 + 63) / 64, nj + 1, (nk + KB - 1) / KB), dim3(64), 0, s, a);
+  if (slotted_columns(ni, nj, nk))
+    hipLaunchKernelGGL(k_getdivx_v_flat_slots, dim3((ni + 63) / 64, nj), dim3(64 * GX_SLOTS), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_getdivx_v_flat, dim3((ni + 63) / 64, nj), dim3(64), 0, s, a);
   GM1_LAUNCH_CHECK();
   return 0;
 }
@@ -644,8 +703,8 @@ int gm1hip_rk_compose(const float* a0, const float* held, const float* divterm, 
 
 int gm1hip_seed_held_momentum(const float* uten1, const float* vten1, const float* wten1, float* uten, float* vten,
                               float* wten, int ni, int nj, int nk, void* stream) {
-  hipLaunchKernelGGL(k_seed_held, dim3((ni + 1 + 63) / 64, nj + 1, (nk + KB - 1) / KB), dim3(64), 0,
-                     (hipStream_t)stream, uten1, vten1, wten1, uten, vten, wten, ni, nj, nk);
+  hipLaunchKernelGGL(k_seed_held, dim3((ni + 1 + 63) / 64, nj + 1, nk), dim3(64), 0, (hipStream_t)stream, uten1,
+                     vten1, wten1, uten, vten, wten, ni, nj, nk);
   GM1_LAUNCH_CHECK();
   return 0;
 }
