@@ -1143,15 +1143,21 @@ __global__ __launch_bounds__(C2T) void k_snd_column2(SoundArgs a, int n) {
 
 // ------------------------------------------------------------------ radiation
 // Durran-Klemp phase speed of one edge row/column, averaged over k in order.
-__global__ __launch_bounds__(64) void k_snd_radiation(SoundArgs a, float dx, float dy, float dttmp) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+// One workgroup per edge position: its lanes form the levels' phase speeds
+// (a division each) into LDS, then one lane adds them in level order.
+constexpr int RAD_T = 64;
+constexpr int RAD_MAXK = 1024;
+
+__global__ __launch_bounds__(RAD_T) void k_snd_radiation(SoundArgs a, float dx, float dy, float dttmp) {
+  __shared__ float cs[RAD_MAXK];
+  __shared__ float avg_s;
+  const int t = blockIdx.x;
   const int ni = a.ni, nj = a.nj, nk = a.nk;
   int side, idx;
   if (t < nj) { side = 0; idx = t; }
   else if (t < 2 * nj) { side = 1; idx = t - nj; }
   else if (t < 2 * nj + ni) { side = 2; idx = t - 2 * nj; }
-  else if (t < 2 * nj + 2 * ni) { side = 3; idx = t - 2 * nj - ni; }
-  else return;
+  else { side = 3; idx = t - 2 * nj - ni; }
   const float eps = 1.0e-10f, lim = 350.0f;
   const bool xdir = side < 2;
   const int nx = xdir ? ux(ni) : sx(ni);
@@ -1168,20 +1174,25 @@ __global__ __launch_bounds__(64) void k_snd_radiation(SoundArgs a, float dx, flo
   else if (side == 2) { chi = (int64_t)(3 + off) * nx + r; clo = (int64_t)(2 + off) * nx + r; cn = clo; rf = a.rvf[2 + off]; d = dy; }
   else { chi = (int64_t)(nj + off) * nx + r; clo = (int64_t)(nj - 1 + off) * nx + r; cn = chi; rf = a.rvf[nj + off]; d = dy; }
   const bool low = (side == 0 || side == 2);
-  float acc = 0.0f;
-  for (int k = 1; k <= nk; ++k) {
+  for (int k = 1 + (int)threadIdx.x; k <= nk; k += RAD_T) {
     const int64_t kb = (int64_t)k * st;
     const float foo = qa[kb + chi] - qa[kb + clo];
     const float num = (d * rf) * (qa[kb + cn] - q3[kb + cn]);
     const float den = dttmp * (copysignf(eps, foo) + foo);
     float c = num / den;
-    c = low ? fmaxf(fminf(c, 0.0f), -lim) : fminf(fmaxf(c, 0.0f), lim);
-    acc = acc + c;
+    cs[k - 1] = low ? fmaxf(fminf(c, 0.0f), -lim) : fminf(fmaxf(c, 0.0f), lim);
   }
-  const float avg = acc / (float)nk;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float acc = 0.0f;
+    for (int k = 0; k < nk; ++k) acc = acc + cs[k];
+    avg_s = acc / (float)nk;
+  }
+  __syncthreads();
+  const float avg = avg_s;
   float* out = side == 0 ? a.radbcw : side == 1 ? a.radbce : side == 2 ? a.radbcs : a.radbcn;
   const int rs = xdir ? (nj + 2 * NG) : (ni + 2 * NG);
-  for (int k = 1; k <= nk; ++k) out[(int64_t)k * rs + idx + NG] = avg;
+  for (int k = 1 + (int)threadIdx.x; k <= nk; k += RAD_T) out[(int64_t)k * rs + idx + NG] = avg;
 }
 
 }  // namespace
@@ -1291,7 +1302,8 @@ static int psolver3_stage(
     }
   }
   if (flags & 1) {
-    hipLaunchKernelGGL(k_snd_radiation, dim3((edge + 63) / 64), dim3(64), 0, s, a, dx, dy, dttmp);
+    if (nk > RAD_MAXK) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_snd_radiation, dim3(edge), dim3(RAD_T), 0, s, a, dx, dy, dttmp);
     GM1_LAUNCH_CHECK();
   }
   if (flags & 2) {
