@@ -193,45 +193,9 @@ __global__ __launch_bounds__(256) void k_mom_mass_v_flat(
   }
 }
 
-// ------------------------------------------------------------- advu
-GM1_DEV float ufx(const MomArgs& a, int k, int j, int f) {
-  const bool pw = per(a.obc, 0), pe = per(a.obc, 1);
-  const int nxu = ux(a.ni), nyu = sy(a.nj);
-  const int y = j + NG, x = f + NG;
-  const float* R = a.rru + ix3(k, y, 0, nyu, nxu);
-  const float m = 0.5f * (R[wf(f, a.ni, pw, pe) + NG] + R[wf(f + 1, a.ni, pw, pe) + NG]);
-  const float* U = a.u + ix3(k, y, 0, nyu, nxu);
-  const float r = (m >= 0.0f) ? flx5(U[x - 2], U[x - 1], U[x], U[x + 1], U[x + 2])
-                              : flx5(U[x + 3], U[x + 2], U[x + 1], U[x], U[x - 1]);
-  return m * r;
-}
-
-GM1_DEV float ufy_base(const MomArgs& a, int k, int p, int f) {
-  const bool pw = per(a.obc, 0), pe = per(a.obc, 1);
-  const int nxu = ux(a.ni), nyu = sy(a.nj), nxv = sx(a.ni), nyv = vy(a.nj);
-  const int y = p + NG, x = f + NG;
-  const float* R = a.rrv + ix3(k, y, 0, nyv, nxv);
-  const float m = 0.5f * (R[wc(f, a.ni, pw, pe) + NG] + R[wc(f - 1, a.ni, pw, pe) + NG]);
-  const float* U = a.u + ix3(k, 0, x, nyu, nxu);
-  auto q = [&](int yy) { return U[(int64_t)yy * nxu]; };
-  const float r = (m >= 0.0f) ? flx5(q(y - 3), q(y - 2), q(y - 1), q(y), q(y + 1))
-                              : flx5(q(y + 2), q(y + 1), q(y), q(y - 1), q(y - 2));
-  return m * r;
-}
-
-GM1_DEV float ufy(const MomArgs& a, int k, int p, int f) {
-  const bool pw = per(a.obc, 0), pe = per(a.obc, 1);
-  const int nxv = sx(a.ni), nyv = vy(a.nj);
-  const float* R = a.rrv + ix3(k, p + NG, 0, nyv, nxv);
-  const float mt = R[wc(f, a.ni, pw, pe) + NG] + R[wc(f - 1, a.ni, pw, pe) + NG];
-  if (p == 0 && opn(a.obc, 2) && mt >= 0.0f) return ufy_base(a, k, 1, f);
-  if (p == a.nj && opn(a.obc, 3) && mt <= 0.0f) return ufy_base(a, k, a.nj - 1, f);
-  return ufy_base(a, k, p, f);
-}
-
 // vertical flux of u (stag 0) or v (stag 1) at w level kf of column (j, i)
-template <int STAG>
-GM1_DEV float uvz(const MomArgs& a, int kf, int j, int i) {
+template <int STAG, class Get>
+GM1_DEV float uvz_g(const MomArgs& a, int kf, int j, int i, Get Q) {
   const int ni = a.ni, nj = a.nj, nk = a.nk;
   const int nxs = sx(ni), nys = sy(nj);
   float m, c2h;
@@ -250,12 +214,6 @@ GM1_DEV float uvz(const MomArgs& a, int kf, int j, int i) {
     c2h = 0.5f * (c2k[(int64_t)clampi(j - 1, nj) * ni + ic] + c2k[(int64_t)clampi(j, nj) * ni + ic]);
   }
   const float c1h = 1.0f - c2h;
-  const float* q = (STAG == 0) ? a.u : a.v;
-  const int nxq = (STAG == 0) ? ux(ni) : sx(ni);
-  const int nyq = (STAG == 0) ? sy(nj) : vy(nj);
-  const int64_t st = (int64_t)nyq * nxq;
-  const float* col = q + (int64_t)(j + NG) * nxq + (i + NG);
-  auto Q = [&](int lev) { return col[(int64_t)lev * st]; };
   float r;
   if (kf == 2) {
     r = (m >= 0.0f) ? c1h * Q(1) + c2h * Q(2) : flx3(Q(3), Q(2), Q(1));
@@ -273,131 +231,297 @@ GM1_DEV float uvz(const MomArgs& a, int kf, int j, int i) {
   return m * r;
 }
 
+// ... with the column read from the wind array
+template <int STAG>
+GM1_DEV float uvz(const MomArgs& a, int kf, int j, int i) {
+  const int ni = a.ni, nj = a.nj;
+  const float* q = (STAG == 0) ? a.u : a.v;
+  const int nxq = (STAG == 0) ? ux(ni) : sx(ni);
+  const int nyq = (STAG == 0) ? sy(nj) : vy(nj);
+  const int64_t st = (int64_t)nyq * nxq;
+  const float* col = q + (int64_t)(j + NG) * nxq + (i + NG);
+  return uvz_g<STAG>(a, kf, j, i, [&](int lev) { return col[(int64_t)lev * st]; });
+}
+
+
+// ------------------------------------------------------------- tiled advu / advv
+// advu / advv (cuda_momentum_advection5.py:1059-1064; the transport of
+// _cpu_momentum_transport5, :753-1012) as a MTX x MTY tile marching a chunk
+// of levels: per level the tile's wind plane (3-cell halo) is staged in LDS,
+// every x and y flux point of the tile is formed once from it into LDS (the
+// open fallback rows included), and the vertical flux comes from a 7-level
+// register window of the thread's own column, the lower face's flux carried
+// from the level below.
+constexpr int MTX = 64, MTY = 8, MNT = MTX * MTY, MLX = MTX + 7;
+
+int mom_chunks(int tiles, int nk, int* kchunk) {
+  int chunks = (2048 + tiles - 1) / tiles;  // >= 2048 tiles in flight
+  if (chunks > nk) chunks = nk;
+  if (chunks < 1) chunks = 1;
+  *kchunk = (nk + chunks - 1) / chunks;
+  return (nk + *kchunk - 1) / *kchunk;
+}
+
+// u-momentum: faces f0 .. f0+nf-1 (tile columns), rows 0..nj-1 (tile rows)
 template <bool FLAT>
-__global__ __launch_bounds__(256) void k_mom_advu(MomArgs a, int f0, int nf) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k0 = blockIdx.z, k = k0 + 1;
-  if (t >= nf) return;
-  const int f = f0 + t;
+__global__ __launch_bounds__(MNT) void k_mom_advu_tile(MomArgs a, int f0, int nf, int kchunk, int nchunks) {
+  __shared__ float lp[MTY + 6][MLX];   // u: LDS row r = padded row ja + r, col c = padded x fa + c
+  __shared__ float lfx[MTY][MTX + 1];  // x flux at centre fa - 1 + t
+  __shared__ float lfy[MTY + 1][MTX];  // y flux at corner row ja + r of face fa + t
   const int ni = a.ni, nj = a.nj, nk = a.nk;
+  const int tx = threadIdx.x % MTX, ty = threadIdx.x / MTX;
+  const int gx = (nf + MTX - 1) / MTX, gy = (nj + MTY - 1) / MTY;
+  const int tile = xcd_tile((int)blockIdx.x, gx * gy * nchunks);
+  const int fa = f0 + (tile % gx) * MTX, ja = ((tile / gx) % gy) * MTY;
+  const int kb = 1 + (tile / (gx * gy)) * kchunk, ke = min(nk, kb + kchunk - 1);
+  const int fend = f0 + nf;  // one past the last face
   const bool pw = per(a.obc, 0), pe = per(a.obc, 1);
-  const int west = wc(f - 1, ni, pw, pe), east = wc(f, ni, pw, pe);
   const int nxu = ux(ni), nyu = sy(nj), nxv = sx(ni), nyv = vy(nj);
+  const int64_t ust = (int64_t)nyu * nxu;
+  const int f = fa + tx, j = ja + ty;
+  const bool live = f < fend && j < nj;
   const int y = j + NG, x = f + NG;
-  float ax = (-(ufx(a, k, j, f) - ufx(a, k, j, f - 1))) * a.rdx;
-  float ay = (-(ufy(a, k, j + 1, f) - ufy(a, k, j, f))) * a.rdy;
-  const float uc = a.u[ix3(k, y, x, nyu, nxu)];
-  if (j == 0 && opn(a.obc, 2)) {
-    const float* R0 = a.rrv + ix3(k, NG, 0, nyv, nxv);
-    const float* R1 = a.rrv + ix3(k, NG + 1, 0, nyv, nxv);
-    const float m = R0[east + NG] + R0[west + NG];
-    if (m >= 0.0f) {
-      const float c = (R1[west + NG] - R0[west + NG]) + (R1[east + NG] - R0[east + NG]);
-      ay = FLAT ? ay - (((uc * 0.5f) * c) * a.rdy) * a.vh[y] : ay - (uc * 0.5f) * (c * (a.rdy * a.vh[y]));
-    }
-  } else if (j == nj - 1 && opn(a.obc, 3)) {
-    const float* RN = a.rrv + ix3(k, NG + nj, 0, nyv, nxv);
-    const float* R0 = a.rrv + ix3(k, y, 0, nyv, nxv);
-    const float* R1 = a.rrv + ix3(k, y + 1, 0, nyv, nxv);
-    const float m = RN[east + NG] + RN[west + NG];
-    if (m <= 0.0f) {
-      const float c = (R1[west + NG] - R0[west + NG]) + (R1[east + NG] - R0[east + NG]);
-      ay = FLAT ? ay - (((uc * 0.5f) * c) * a.rdy) * a.vh[y] : ay - (uc * 0.5f) * (c * (a.rdy * a.vh[y]));
-    }
+  const float* col = a.u + (int64_t)y * nxu + x;
+  float win[7];  // u(k-3 .. k+3) of this face's column
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    const int lev = kb - 3 + q;
+    win[q] = (live && lev >= 0 && lev <= nk + 1) ? col[(int64_t)lev * ust] : 0.0f;
   }
-  float az;
-  if (k == 1) az = (-uvz<0>(a, 2, j, f)) * a.rdz;
-  else if (k == nk) az = uvz<0>(a, nk, j, f) * a.rdz;
-  else az = (-(uvz<0>(a, k + 1, j, f) - uvz<0>(a, k, j, f))) * a.rdz;
-  if constexpr (FLAT) az = az * a.flat[k];
-  float tot = (ax + ay) + az;
-  const int64_t row = ((int64_t)k0 * nj + j) * ni;
-  tot = tot + uc * (0.5f * (a.divx[row + east] + a.divx[row + west]));
-  const float sc = FLAT ? a.rr0[(int64_t)k0 * nj * ni]
-                          : a.gzu[(int64_t)j * (ni + 1) + f] / (0.5f * (a.rho0[row + west] + a.rho0[row + east]));
-  const int64_t p = ix3(k, y, x, nyu, nxu);
-  a.uten[p] = a.uten[p] + tot * sc;
+  auto zf = [&](int kf, const float* b) {  // b[0] = u(kf - 3)
+    return uvz_g<0>(a, kf, j, f, [&](int lev) { return b[lev - (kf - 3)]; });
+  };
+  float zlo = (live && kb >= 2) ? zf(kb, win) : 0.0f;
+  for (int k = kb; k <= ke; ++k) {
+    const int lnext = k + 4;
+    const float unext = (live && lnext <= nk + 1) ? col[(int64_t)lnext * ust] : 0.0f;
+    {
+      const float* plane = a.u + (int64_t)k * ust;
+      for (int sl = threadIdx.x; sl < (MTY + 6) * (MTX + 6); sl += MNT) {
+        const int r = sl / (MTX + 6), c = sl % (MTX + 6);
+        const int yy = ja + r, xx = fa + c;
+        if (yy < nyu && xx < nxu) lp[r][c] = plane[(int64_t)yy * nxu + xx];
+      }
+    }
+    __syncthreads();
+    // x fluxes of u at centres cc = fa-1 .. fa+MTX-1 of the tile's rows: the
+    // mean of the two rru faces times u's fifth-order upwind value
+    for (int sl = threadIdx.x; sl < MTY * (MTX + 1); sl += MNT) {
+      const int r = sl / (MTX + 1), t = sl % (MTX + 1);
+      const int cc = fa - 1 + t, jj = ja + r;
+      if (jj < nj && cc < fend) {
+        const float* R = a.rru + ix3(k, jj + NG, 0, nyu, nxu);
+        const float m = 0.5f * (R[wf(cc, ni, pw, pe) + NG] + R[wf(cc + 1, ni, pw, pe) + NG]);
+        const float* L = lp[r + 3] + t;  // L[c] = u at padded x (cc + NG) - 2 + c
+        const float rv = (m >= 0.0f) ? flx5(L[0], L[1], L[2], L[3], L[4]) : flx5(L[5], L[4], L[3], L[2], L[1]);
+        lfx[r][t] = m * rv;
+      }
+    }
+    // y fluxes of u at corner rows p = ja .. ja+MTY of the tile's faces; on an
+    // open south/north side an inflowing edge corner takes the next row's flux
+    for (int sl = threadIdx.x; sl < (MTY + 1) * MTX; sl += MNT) {
+      const int r = sl / MTX, t = sl % MTX;
+      const int p = ja + r, ff = fa + t;
+      if (p <= nj && ff < fend) {
+        const float* R = a.rrv + ix3(k, p + NG, 0, nyv, nxv);
+        const float mt = R[wc(ff, ni, pw, pe) + NG] + R[wc(ff - 1, ni, pw, pe) + NG];
+        int pb = p;
+        if (p == 0 && opn(a.obc, 2) && mt >= 0.0f) pb = 1;
+        else if (p == nj && opn(a.obc, 3) && mt <= 0.0f) pb = nj - 1;
+        const float* Rb = a.rrv + ix3(k, pb + NG, 0, nyv, nxv);
+        const float m = 0.5f * (Rb[wc(ff, ni, pw, pe) + NG] + Rb[wc(ff - 1, ni, pw, pe) + NG]);
+        const int r0 = pb - ja;  // LDS row of u at padded row pb + NG - 3
+        const int c = t + 3;
+        const float rv = (m >= 0.0f)
+                             ? flx5(lp[r0][c], lp[r0 + 1][c], lp[r0 + 2][c], lp[r0 + 3][c], lp[r0 + 4][c])
+                             : flx5(lp[r0 + 5][c], lp[r0 + 4][c], lp[r0 + 3][c], lp[r0 + 2][c], lp[r0 + 1][c]);
+        lfy[r][t] = m * rv;
+      }
+    }
+    float zhi = 0.0f;
+    if (live && k + 1 <= nk) zhi = zf(k + 1, win + 1);
+    __syncthreads();
+    if (live) {
+      const int west = wc(f - 1, ni, pw, pe), east = wc(f, ni, pw, pe);
+      float ax = (-(lfx[ty][tx + 1] - lfx[ty][tx])) * a.rdx;
+      float ay = (-(lfy[ty + 1][tx] - lfy[ty][tx])) * a.rdy;
+      const float uc = lp[ty + 3][tx + 3];
+      if (j == 0 && opn(a.obc, 2)) {
+        const float* R0 = a.rrv + ix3(k, NG, 0, nyv, nxv);
+        const float* R1 = a.rrv + ix3(k, NG + 1, 0, nyv, nxv);
+        const float m = R0[east + NG] + R0[west + NG];
+        if (m >= 0.0f) {
+          const float c = (R1[west + NG] - R0[west + NG]) + (R1[east + NG] - R0[east + NG]);
+          ay = FLAT ? ay - (((uc * 0.5f) * c) * a.rdy) * a.vh[y] : ay - (uc * 0.5f) * (c * (a.rdy * a.vh[y]));
+        }
+      } else if (j == nj - 1 && opn(a.obc, 3)) {
+        const float* RN = a.rrv + ix3(k, NG + nj, 0, nyv, nxv);
+        const float* R0 = a.rrv + ix3(k, y, 0, nyv, nxv);
+        const float* R1 = a.rrv + ix3(k, y + 1, 0, nyv, nxv);
+        const float m = RN[east + NG] + RN[west + NG];
+        if (m <= 0.0f) {
+          const float c = (R1[west + NG] - R0[west + NG]) + (R1[east + NG] - R0[east + NG]);
+          ay = FLAT ? ay - (((uc * 0.5f) * c) * a.rdy) * a.vh[y] : ay - (uc * 0.5f) * (c * (a.rdy * a.vh[y]));
+        }
+      }
+      float az;
+      if (k == 1) az = (-zhi) * a.rdz;
+      else if (k == nk) az = zlo * a.rdz;
+      else az = (-(zhi - zlo)) * a.rdz;
+      if constexpr (FLAT) az = az * a.flat[k];
+      float tot = (ax + ay) + az;
+      const int k0 = k - 1;
+      const int64_t row = ((int64_t)k0 * nj + j) * ni;
+      tot = tot + uc * (0.5f * (a.divx[row + east] + a.divx[row + west]));
+      const float sc = FLAT ? a.rr0[(int64_t)k0 * nj * ni]
+                            : a.gzu[(int64_t)j * (ni + 1) + f] / (0.5f * (a.rho0[row + west] + a.rho0[row + east]));
+      const int64_t pidx = ix3(k, y, x, nyu, nxu);
+      a.uten[pidx] = a.uten[pidx] + tot * sc;
+    }
+    zlo = zhi;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) win[q] = win[q + 1];
+    win[6] = unext;
+    __syncthreads();
+  }
 }
 
-// ------------------------------------------------------------- advv
-GM1_DEV float vfx_base(const MomArgs& a, int k, int g, int p) {
+// v-momentum: faces g0 .. g0+nr-1 (tile rows), columns 0..ni-1 (tile columns)
+template <bool FLAT>
+__global__ __launch_bounds__(MNT) void k_mom_advv_tile(MomArgs a, int g0, int nr, int kchunk, int nchunks) {
+  __shared__ float lp[MTY + 6][MLX];   // v: LDS row r = padded row ga + r, col c = padded x ia + c
+  __shared__ float lfx[MTY][MTX + 1];  // x flux at corner column ia + t of face row ga + r
+  __shared__ float lfy[MTY + 1][MTX];  // y flux at centre ga - 1 + r of column ia + t
+  const int ni = a.ni, nj = a.nj, nk = a.nk;
+  const int tx = threadIdx.x % MTX, ty = threadIdx.x / MTX;
+  const int gx = (ni + MTX - 1) / MTX, gy = (nr + MTY - 1) / MTY;
+  const int tile = xcd_tile((int)blockIdx.x, gx * gy * nchunks);
+  const int ia = (tile % gx) * MTX, ga = g0 + ((tile / gx) % gy) * MTY;
+  const int kb = 1 + (tile / (gx * gy)) * kchunk, ke = min(nk, kb + kchunk - 1);
+  const int gend = g0 + nr;
   const bool ps = per(a.obc, 2), pn = per(a.obc, 3);
-  const int nxu = ux(a.ni), nyu = sy(a.nj), nxv = sx(a.ni), nyv = vy(a.nj);
-  const int y = g + NG, x = p + NG;
-  const float m = 0.5f * (a.rru[ix3(k, wc(g, a.nj, ps, pn) + NG, x, nyu, nxu)] +
-                          a.rru[ix3(k, wc(g - 1, a.nj, ps, pn) + NG, x, nyu, nxu)]);
-  const float* V = a.v + ix3(k, y, 0, nyv, nxv);
-  const float r = (m >= 0.0f) ? flx5(V[x - 3], V[x - 2], V[x - 1], V[x], V[x + 1])
-                              : flx5(V[x + 2], V[x + 1], V[x], V[x - 1], V[x - 2]);
-  return m * r;
-}
-
-GM1_DEV float vfx(const MomArgs& a, int k, int g, int p) {
-  const bool ps = per(a.obc, 2), pn = per(a.obc, 3);
-  const int nxu = ux(a.ni), nyu = sy(a.nj);
-  const int x = p + NG;
-  const float mt = a.rru[ix3(k, wc(g, a.nj, ps, pn) + NG, x, nyu, nxu)] +
-                   a.rru[ix3(k, wc(g - 1, a.nj, ps, pn) + NG, x, nyu, nxu)];
-  if (p == 0 && opn(a.obc, 0) && mt >= 0.0f) return vfx_base(a, k, g, 1);
-  if (p == a.ni && opn(a.obc, 1) && mt <= 0.0f) return vfx_base(a, k, g, a.ni - 1);
-  return vfx_base(a, k, g, p);
-}
-
-GM1_DEV float vfy(const MomArgs& a, int k, int p, int i) {
-  const bool ps = per(a.obc, 2), pn = per(a.obc, 3);
-  const int nxv = sx(a.ni), nyv = vy(a.nj);
-  const int y = p + NG, x = i + NG;
-  const float m = 0.5f * (a.rrv[ix3(k, wf(p, a.nj, ps, pn) + NG, x, nyv, nxv)] +
-                          a.rrv[ix3(k, wf(p + 1, a.nj, ps, pn) + NG, x, nyv, nxv)]);
-  const float* V = a.v + ix3(k, 0, x, nyv, nxv);
-  auto q = [&](int yy) { return V[(int64_t)yy * nxv]; };
-  const float r = (m >= 0.0f) ? flx5(q(y - 2), q(y - 1), q(y), q(y + 1), q(y + 2))
-                              : flx5(q(y + 3), q(y + 2), q(y + 1), q(y), q(y - 1));
-  return m * r;
+  const int nxu = ux(ni), nyu = sy(nj), nxv = sx(ni), nyv = vy(nj);
+  const int64_t vst = (int64_t)nyv * nxv;
+  const int i = ia + tx, g = ga + ty;
+  const bool live = i < ni && g < gend;
+  const int y = g + NG, x = i + NG;
+  const float* col = a.v + (int64_t)y * nxv + x;
+  float win[7];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    const int lev = kb - 3 + q;
+    win[q] = (live && lev >= 0 && lev <= nk + 1) ? col[(int64_t)lev * vst] : 0.0f;
+  }
+  auto zf = [&](int kf, const float* b) {
+    return uvz_g<1>(a, kf, g, i, [&](int lev) { return b[lev - (kf - 3)]; });
+  };
+  float zlo = (live && kb >= 2) ? zf(kb, win) : 0.0f;
+  for (int k = kb; k <= ke; ++k) {
+    const int lnext = k + 4;
+    const float vnext = (live && lnext <= nk + 1) ? col[(int64_t)lnext * vst] : 0.0f;
+    {
+      const float* plane = a.v + (int64_t)k * vst;
+      for (int sl = threadIdx.x; sl < (MTY + 6) * (MTX + 6); sl += MNT) {
+        const int r = sl / (MTX + 6), c = sl % (MTX + 6);
+        const int yy = ga + r, xx = ia + c;
+        if (yy < nyv && xx < nxv) lp[r][c] = plane[(int64_t)yy * nxv + xx];
+      }
+    }
+    __syncthreads();
+    // x fluxes of v at corner columns p = ia .. ia+MTX of the tile's face rows;
+    // on an open west/east side an inflowing edge corner takes the next column's
+    for (int sl = threadIdx.x; sl < MTY * (MTX + 1); sl += MNT) {
+      const int r = sl / (MTX + 1), t = sl % (MTX + 1);
+      const int p = ia + t, gg = ga + r;
+      if (gg < gend && p <= ni) {
+        const int rn = wc(gg, nj, ps, pn) + NG, rs = wc(gg - 1, nj, ps, pn) + NG;
+        const float mt = a.rru[ix3(k, rn, p + NG, nyu, nxu)] + a.rru[ix3(k, rs, p + NG, nyu, nxu)];
+        int pb = p;
+        if (p == 0 && opn(a.obc, 0) && mt >= 0.0f) pb = 1;
+        else if (p == ni && opn(a.obc, 1) && mt <= 0.0f) pb = ni - 1;
+        const float m = 0.5f * (a.rru[ix3(k, rn, pb + NG, nyu, nxu)] + a.rru[ix3(k, rs, pb + NG, nyu, nxu)]);
+        const float* L = lp[r + 3] + (pb - ia);  // L[c] = v at padded x (pb + NG) - 3 + c
+        const float rv = (m >= 0.0f) ? flx5(L[0], L[1], L[2], L[3], L[4]) : flx5(L[5], L[4], L[3], L[2], L[1]);
+        lfx[r][t] = m * rv;
+      }
+    }
+    // y fluxes of v at centres pc = ga-1 .. ga+MTY-1 of the tile's columns
+    for (int sl = threadIdx.x; sl < (MTY + 1) * MTX; sl += MNT) {
+      const int r = sl / MTX, t = sl % MTX;
+      const int pc = ga - 1 + r, ii = ia + t;
+      if (ii < ni && pc < gend) {
+        const float m = 0.5f * (a.rrv[ix3(k, wf(pc, nj, ps, pn) + NG, ii + NG, nyv, nxv)] +
+                                a.rrv[ix3(k, wf(pc + 1, nj, ps, pn) + NG, ii + NG, nyv, nxv)]);
+        const int c = t + 3;
+        // LDS row of v at padded row (pc + NG) - 2 is pc + 1 - ga = r
+        const float rv = (m >= 0.0f) ? flx5(lp[r][c], lp[r + 1][c], lp[r + 2][c], lp[r + 3][c], lp[r + 4][c])
+                                     : flx5(lp[r + 5][c], lp[r + 4][c], lp[r + 3][c], lp[r + 2][c], lp[r + 1][c]);
+        lfy[r][t] = m * rv;
+      }
+    }
+    float zhi = 0.0f;
+    if (live && k + 1 <= nk) zhi = zf(k + 1, win + 1);
+    __syncthreads();
+    if (live) {
+      const int south = wc(g - 1, nj, ps, pn), north = wc(g, nj, ps, pn);
+      float ax = (-(lfx[ty][tx + 1] - lfx[ty][tx])) * a.rdx;
+      float ay = (-(lfy[ty + 1][tx] - lfy[ty][tx])) * a.rdy;
+      const float vc = lp[ty + 3][tx + 3];
+      const float* Rs = a.rru + ix3(k, south + NG, 0, nyu, nxu);
+      const float* Rn = a.rru + ix3(k, north + NG, 0, nyu, nxu);
+      if (i == 0 && opn(a.obc, 0)) {
+        const float m = Rn[NG] + Rs[NG];
+        if (m >= 0.0f) {
+          const float c = (Rs[NG + 1] - Rs[NG]) + (Rn[NG + 1] - Rn[NG]);
+          ax = FLAT ? ax - (((vc * 0.5f) * c) * a.rdx) * a.uh[NG] : ax - (vc * 0.5f) * (c * (a.rdx * a.uh[NG]));
+        }
+      } else if (i == ni - 1 && opn(a.obc, 1)) {
+        const float m = Rn[NG + ni] + Rs[NG + ni];
+        if (m <= 0.0f) {
+          const float c = (Rs[x + 1] - Rs[x]) + (Rn[x + 1] - Rn[x]);
+          ax = FLAT ? ax - (((vc * 0.5f) * c) * a.rdx) * a.uh[x] : ax - (vc * 0.5f) * (c * (a.rdx * a.uh[x]));
+        }
+      }
+      float az;
+      if (k == 1) az = (-zhi) * a.rdz;
+      else if (k == nk) az = zlo * a.rdz;
+      else az = (-(zhi - zlo)) * a.rdz;
+      if constexpr (FLAT) az = az * a.flat[k];
+      float tot = (ax + ay) + az;
+      const int k0 = k - 1;
+      const int64_t plane = (int64_t)k0 * nj * ni;
+      tot = tot + vc * (0.5f * (a.divx[plane + (int64_t)north * ni + i] + a.divx[plane + (int64_t)south * ni + i]));
+      const float sc = FLAT ? a.rr0[plane]
+                            : a.gzv[(int64_t)g * ni + i] /
+                                  (0.5f * (a.rho0[plane + (int64_t)south * ni + i] +
+                                           a.rho0[plane + (int64_t)north * ni + i]));
+      const int64_t pidx = ix3(k, y, x, nyv, nxv);
+      a.vten[pidx] = a.vten[pidx] + tot * sc;
+    }
+    zlo = zhi;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) win[q] = win[q + 1];
+    win[6] = vnext;
+    __syncthreads();
+  }
 }
 
 template <bool FLAT>
-__global__ __launch_bounds__(256) void k_mom_advv(MomArgs a, int g0, int nr) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = blockIdx.y, k0 = blockIdx.z, k = k0 + 1;
-  const int ni = a.ni, nj = a.nj, nk = a.nk;
-  if (i >= ni || r >= nr) return;
-  const int g = g0 + r;
-  const bool ps = per(a.obc, 2), pn = per(a.obc, 3);
-  const int south = wc(g - 1, nj, ps, pn), north = wc(g, nj, ps, pn);
-  const int nxu = ux(ni), nyu = sy(nj), nxv = sx(ni), nyv = vy(nj);
-  const int y = g + NG, x = i + NG;
-  float ax = (-(vfx(a, k, g, i + 1) - vfx(a, k, g, i))) * a.rdx;
-  float ay = (-(vfy(a, k, g, i) - vfy(a, k, g - 1, i))) * a.rdy;
-  const float vc = a.v[ix3(k, y, x, nyv, nxv)];
-  const float* Rs = a.rru + ix3(k, south + NG, 0, nyu, nxu);
-  const float* Rn = a.rru + ix3(k, north + NG, 0, nyu, nxu);
-  if (i == 0 && opn(a.obc, 0)) {
-    const float m = Rn[NG] + Rs[NG];
-    if (m >= 0.0f) {
-      const float c = (Rs[NG + 1] - Rs[NG]) + (Rn[NG + 1] - Rn[NG]);
-      ax = FLAT ? ax - (((vc * 0.5f) * c) * a.rdx) * a.uh[NG] : ax - (vc * 0.5f) * (c * (a.rdx * a.uh[NG]));
-    }
-  } else if (i == ni - 1 && opn(a.obc, 1)) {
-    const float m = Rn[NG + ni] + Rs[NG + ni];
-    if (m <= 0.0f) {
-      const float c = (Rs[x + 1] - Rs[x]) + (Rn[x + 1] - Rn[x]);
-      ax = FLAT ? ax - (((vc * 0.5f) * c) * a.rdx) * a.uh[x] : ax - (vc * 0.5f) * (c * (a.rdx * a.uh[x]));
-    }
-  }
-  float az;
-  if (k == 1) az = (-uvz<1>(a, 2, g, i)) * a.rdz;
-  else if (k == nk) az = uvz<1>(a, nk, g, i) * a.rdz;
-  else az = (-(uvz<1>(a, k + 1, g, i) - uvz<1>(a, k, g, i))) * a.rdz;
-  if constexpr (FLAT) az = az * a.flat[k];
-  float tot = (ax + ay) + az;
-  const int64_t plane = (int64_t)k0 * nj * ni;
-  tot = tot + vc * (0.5f * (a.divx[plane + (int64_t)north * ni + i] + a.divx[plane + (int64_t)south * ni + i]));
-  const float sc = FLAT ? a.rr0[plane]
-                          : a.gzv[(int64_t)g * ni + i] /
-                                (0.5f * (a.rho0[plane + (int64_t)south * ni + i] + a.rho0[plane + (int64_t)north * ni + i]));
-  const int64_t p = ix3(k, y, x, nyv, nxv);
-  a.vten[p] = a.vten[p] + tot * sc;
+void launch_advu_tile(const MomArgs& a, int f0, int nf, hipStream_t s) {
+  int kchunk = 1;
+  const int tiles = ((nf + MTX - 1) / MTX) * ((a.nj + MTY - 1) / MTY);
+  const int chunks = mom_chunks(tiles, a.nk, &kchunk);
+  GM1_TIMED(TK_MOM_ADVU, s, hipLaunchKernelGGL(k_mom_advu_tile<FLAT>, dim3(tiles * chunks), dim3(MNT), 0, s, a, f0,
+                                               nf, kchunk, chunks));
+}
+
+template <bool FLAT>
+void launch_advv_tile(const MomArgs& a, int g0, int nr, hipStream_t s) {
+  int kchunk = 1;
+  const int tiles = ((a.ni + MTX - 1) / MTX) * ((nr + MTY - 1) / MTY);
+  const int chunks = mom_chunks(tiles, a.nk, &kchunk);
+  GM1_TIMED(TK_MOM_ADVV, s, hipLaunchKernelGGL(k_mom_advv_tile<FLAT>, dim3(tiles * chunks), dim3(MNT), 0, s, a, g0,
+                                               nr, kchunk, chunks));
 }
 
 // ------------------------------------------------------------- advw
@@ -564,9 +688,9 @@ int gm1hip_momentum_advection5(
   const bool ps = side_of(obc, 2) == 1u, pn = side_of(obc, 3) == 1u;
   const int f0 = pw ? 0 : 1, nf = ni - 1 + (pw ? 1 : 0) + (pe ? 1 : 0);
   const int g0 = ps ? 0 : 1, nr = nj - 1 + (ps ? 1 : 0) + (pn ? 1 : 0);
-  GM1_TIMED(TK_MOM_ADVU, s, hipLaunchKernelGGL(k_mom_advu<false>, dim3((nf + 63) / 64, nj, nk), blk, 0, s, a, f0, nf));
+  launch_advu_tile<false>(a, f0, nf, s);
   GM1_LAUNCH_CHECK();
-  GM1_TIMED(TK_MOM_ADVV, s, hipLaunchKernelGGL(k_mom_advv<false>, dim3((ni + 63) / 64, nr, nk), blk, 0, s, a, g0, nr));
+  launch_advv_tile<false>(a, g0, nr, s);
   GM1_LAUNCH_CHECK();
   GM1_TIMED(TK_MOM_ADVW, s, hipLaunchKernelGGL(k_mom_advw<false>, dim3((ni + 63) / 64, nj, nk - 1), blk, 0, s, a));
   GM1_LAUNCH_CHECK();
@@ -591,9 +715,9 @@ int gm1hip_momentum_transport5(
   const bool ps = side_of(obc, 2) == 1u, pn = side_of(obc, 3) == 1u;
   const int f0 = pw ? 0 : 1, nf = ni - 1 + (pw ? 1 : 0) + (pe ? 1 : 0);
   const int g0 = ps ? 0 : 1, nr = nj - 1 + (ps ? 1 : 0) + (pn ? 1 : 0);
-  GM1_TIMED(TK_MOM_ADVU, s, hipLaunchKernelGGL(k_mom_advu<false>, dim3((nf + 63) / 64, nj, nk), blk, 0, s, a, f0, nf));
+  launch_advu_tile<false>(a, f0, nf, s);
   GM1_LAUNCH_CHECK();
-  GM1_TIMED(TK_MOM_ADVV, s, hipLaunchKernelGGL(k_mom_advv<false>, dim3((ni + 63) / 64, nr, nk), blk, 0, s, a, g0, nr));
+  launch_advv_tile<false>(a, g0, nr, s);
   GM1_LAUNCH_CHECK();
   GM1_TIMED(TK_MOM_ADVW, s, hipLaunchKernelGGL(k_mom_advw<false>, dim3((ni + 63) / 64, nj, nk - 1), blk, 0, s, a));
   GM1_LAUNCH_CHECK();
@@ -626,9 +750,9 @@ int gm1hip_momentum_advection5_flat(
   const bool ps = side_of(obc, 2) == 1u, pn = side_of(obc, 3) == 1u;
   const int f0 = pw ? 0 : 1, nf = ni - 1 + (pw ? 1 : 0) + (pe ? 1 : 0);
   const int g0 = ps ? 0 : 1, nr = nj - 1 + (ps ? 1 : 0) + (pn ? 1 : 0);
-  GM1_TIMED(TK_MOM_ADVU, s, hipLaunchKernelGGL(k_mom_advu<true>, dim3((nf + 63) / 64, nj, nk), blk, 0, s, a, f0, nf));
+  launch_advu_tile<true>(a, f0, nf, s);
   GM1_LAUNCH_CHECK();
-  GM1_TIMED(TK_MOM_ADVV, s, hipLaunchKernelGGL(k_mom_advv<true>, dim3((ni + 63) / 64, nr, nk), blk, 0, s, a, g0, nr));
+  launch_advv_tile<true>(a, g0, nr, s);
   GM1_LAUNCH_CHECK();
   GM1_TIMED(TK_MOM_ADVW, s, hipLaunchKernelGGL(k_mom_advw<true>, dim3((ni + 63) / 64, nj, nk - 1), blk, 0, s, a));
   GM1_LAUNCH_CHECK();
