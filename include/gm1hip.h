@@ -122,6 +122,23 @@ int gm1hip_psolver3_stage(
 int gm1hip_sound3_periodic_stage(void* const* arrays, int ni, int nj, int nk, int nrk, int nloop, float dts,
                                  float rdx, float rdy, float rdz, float cp, float alph, float kdiv, void* stream);
 
+// Adjoint of one gm1hip_psolver3_stage call on flat terrain metrics, a
+// horizontally uniform base state and doubly periodic sides (no counterpart
+// in the reference, which has no adjoint; the forward it transposes is
+// cuda_psolver3_terrain_open.py:774-893).  For the fixed thv, ppterm and
+// piadv the forward ran with, it maps the adjoints of the stage's outputs to
+// those of its inputs, in binary32.  `arrays` is a host table of 33 device
+// pointers:
+//   thv ppterm piadv | rho0_3d rr0 rf0 c1 c2 (terrain_static_shapes) |
+//   profiles (sigma|sigmaf|rds|rdsf..., nk+2 each) |
+//   qk pk1 pk2 mm fpkb elim_c elim_d (pressure-shaped scratch) |
+//   u3d v3d w3d pp3d ppd ppx rru rrv rrw (in: output adjoints; out: u3d v3d
+//   w3d pp3d ppx hold input adjoints, the rest is consumed) |
+//   uten vten wten ppten ua va wa ppi (input adjoints, accumulated: zero them)
+// with the forward's shapes.  ua, va and ppi receive nothing at nrk = 1.
+int gm1hip_psolver3_stage_adjoint(void* const* arrays, int ni, int nj, int nk, int nrk, int nloop, float dts,
+                                  float dx, float dy, float alph, float kdiv, void* stream);
+
 // ----------------------------------------------------------------- boundaries
 // Family halo fill of one padded field (cuda_boundary_families.py:887-940,
 // cpu_family_we_inplace / cpu_family_sn_inplace); kind 0 scalar, 1 u, 2 v, 3 w.
