@@ -66,6 +66,8 @@ def parse_args(argv=None):
     p.add_argument("--cpu-sample", type=int, default=96, help="horizontal size of the oracle's sub-domain sample")
     p.add_argument("--cpu-steps", type=int, default=2)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing")
+    p.add_argument("--adjoint-repeats", type=int, default=10)
     p.add_argument("--graph", action="store_true",
                    help="time replays of the step captured into a HIP graph (the kernel timer then runs in an "
                         "eager pass of the same length)")
@@ -213,6 +215,62 @@ def run_ensemble(args) -> int:
     return 0
 
 
+def adjoint_ratio(repeats: int):
+    """Device time of the acoustic RK stages (nrk 1, 2, 3) and of their adjoints on the dry periodic
+    192x192x64 case after one forward large step (its thv, ppterm and piadv held fixed)."""
+    import torch
+
+    from gm1_amd.adjoint_sound import STATE, check_flat_periodic, launch_cuda_psolver3_stage_adjoint
+    from gm1_amd.fields import adjoint_c5
+    from gm1_amd.large_step import create_gm1_large_step_state, host_initial_state, launch_cuda_gm1_large_step
+    from gm1_amd.psolver3 import launch_cuda_psolver3_terrain_open_stage
+
+    cfg = adjoint_c5()
+    stream = torch.cuda.Stream()
+    st = create_gm1_large_step_state(host_initial_state(cfg), "cuda:0", stream=stream)
+    launch_cuda_gm1_large_step(st)
+    torch.cuda.synchronize()
+    check_flat_periodic(st.metrics, cfg.lateral_sides)
+    kw = {"ni": cfg.ni, "nj": cfg.nj, "nk": cfg.nk, "dt": cfg.dt, "nsound": cfg.nsound, "dx": cfg.dx,
+          "dy": cfg.dy, "dz": cfg.dz}
+    a = st.a
+    coef = {n: a[n] for n in ("thv", "ppterm", "piadv")}
+    gen = torch.Generator(device="cuda:0").manual_seed(5)
+    bar = {n: torch.randn(a[n].shape, device="cuda:0", generator=gen) for n in STATE}
+
+    def forward():
+        for nrk in (1, 2, 3):
+            launch_cuda_psolver3_terrain_open_stage(a, st.metrics, nrk=nrk, lateral_sides=cfg.lateral_sides,
+                                                    stream=stream, **kw)
+
+    def adjoint():
+        b = bar
+        for nrk in (3, 2, 1):
+            out, _ = launch_cuda_psolver3_stage_adjoint(b, coef, st.metrics, nrk=nrk, check_geometry=False,
+                                                        stream=stream, **kw)
+            b = {**bar, **{n: out[n] for n in ("u3d", "v3d", "w3d", "pp3d", "ppx")}}
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for _ in range(repeats):
+            fn()
+        stop.record(stream)
+        torch.cuda.synchronize()
+        return start.elapsed_time(stop) / repeats
+
+    fwd_ms = timed(forward)
+    adj_ms = timed(adjoint)
+    print(f"[bench] acoustic stages {fwd_ms:.3f} ms, adjoint {adj_ms:.3f} ms", file=sys.stderr, flush=True)
+    return {"scope": "psolver=3 acoustic RK stages nrk 1-3 (forward) vs their adjoints (reverse order), "
+                     "fixed thv/ppterm/piadv; the adjoint of the whole large step is not built",
+            "workload": f"dry periodic {cfg.ni}x{cfg.nj}x{cfg.nk}, dx={cfg.dx:g} m, dz={cfg.dz:g} m, "
+                        f"dt={cfg.dt:g} s, {cfg.nsound} substeps",
+            "forward_ms": fwd_ms, "adjoint_ms": adj_ms, "ratio": adj_ms / fwd_ms}
+
+
 def main(argv=None) -> int:
     args = parse_args(argv)
     if args.gpus != 1:
@@ -343,6 +401,7 @@ def main(argv=None) -> int:
             "kernel_share_of_step": kernel_ms / elapsed_ms,
         },
     }
+    result["adjoint"] = None if args.no_adjoint else adjoint_ratio(args.adjoint_repeats)
     if not args.no_cpu_baseline:
         print("[bench] timing the CPU baseline sample", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(cfg, args.cpu_sample, args.cpu_steps)

@@ -131,6 +131,12 @@ def large_stretched_c3(ni=640, nj=768, nk=80) -> CaseConfig:
                       stretched=(17850.0, 200.0, 2450.0, 20.0, 280.0))
 
 
+def adjoint_c5(ni=192, nj=192, nk=64) -> CaseConfig:
+    """192x192x64, dx 500 m, dz 250 m, periodic, dt 3 s, 6 substeps, dry; the dry case's bubble, noise 0.1 K."""
+    return CaseConfig(f"dry-periodic-adjoint-{ni}x{nj}x{nk}", ni, nj, nk, 500.0, 500.0, 250.0, 3.0, 6, (1, 1, 1, 1),
+                      False, "linear15", ((2.0, ni * 250.0, nj * 250.0, 1400.0),), noise=0.1, seed=5)
+
+
 ENSEMBLE_SIZES = (128, 192, 256, 320)
 ENSEMBLE_LEVELS = (64, 80)
 
