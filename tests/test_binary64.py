@@ -52,7 +52,7 @@ def oracle_binary64():
         sys.modules[name] = pkg
         spec.loader.exec_module(pkg)
         for sub in ("common", "boundary", "stage", "sound", "advection5", "momentum5", "pdef", "kessler",
-                    "large_step", "adjoint_sound"):
+                    "large_step", "adjoint_sound", "adjoint_large_step"):
             importlib.import_module(f"{name}.{sub}")
     finally:
         if saved is None:
