@@ -1,0 +1,190 @@
+"""Dry large step and its adjoint on halo-free periodic fields (gfx950).
+
+GM1 has no adjoint.  This module runs the dry large step of the adjoint
+configuration (flat terrain, horizontally uniform base state, CM1's
+(1, 1, 1, 1) lateral sides) on fields without halos -- u, v and cell fields
+(nk, nj, ni), w (nk+1, nj, ni) for levels 1..nk+1 -- and its adjoint:
+
+* ``DryPeriodicStep.forward_stage`` / ``forward`` restate one RK stage / one
+  large step (cuda_gm1_forecast.py:1241-1345, 1414-1457) in the reference's
+  association order, keeping every acoustic substep's state;
+* ``DryPeriodicStep.adjoint`` is the transpose of one large step at its
+  trajectory (upwind choices, WENO5 weights and limiters frozen at their
+  forward values, smeps clamps as the identity);
+* ``storm_intensity`` is the adjoint configuration's loss (base-density
+  weighted mean of w^2 over the w levels between 2 and 8 km) and
+  ``gradient`` its gradient with respect to the initial state over a window
+  of large steps, recomputing forward segments from binomially placed
+  checkpoints (``revolve_schedule``).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import Any
+
+import numpy as np
+import torch
+
+from . import _native
+from ._device import stream_handle
+from ._native import D as Dbl
+from ._native import I, P, signature
+from .grid import sound3_stage_schedule_f32, terrain_coordinate_profiles
+
+signature("gm1hip_dry_stage_forward", P, P, I, I, I, I, I, Dbl, P)
+
+F32 = np.float32
+STATE = ("u", "v", "w", "th", "pp", "ppx")
+
+# pointer-table slots (the order of adjoint_step.hip's Slot enum)
+SLOTS = (
+    "COL", "XU", "XV", "XW", "XTH", "XPP", "AU", "AV", "AW", "ATH", "API", "PPX",
+    "THV", "PPTERM", "PIADV", "DUM8", "THTERM", "RRU", "RRV", "RRW", "DIVX", "UTEN", "VTEN", "WTEN",
+    "QK", "PK1", "PK2", "MM", "FPK", "EC", "ED", "SU", "SV", "SW", "SPP", "SPPD", "ARU", "ARV", "ARW",
+    "PPXO", "DIV", "TDIV", "DIVX2", "THO",
+    "B_OU", "B_OV", "B_OW", "B_OTH", "B_OPP", "B_PPX", "B_XU", "B_XV", "B_XW", "B_XTH", "B_XPP",
+    "B_AU", "B_AV", "B_AW", "B_ATH", "B_API",
+    "B_THV", "B_PPTERM", "B_PIADV", "B_DUM8", "B_THTERM", "B_RRU", "B_RRV", "B_RRW", "B_DIVX", "B_UTEN",
+    "B_VTEN", "B_WTEN", "B_QK", "B_PK1", "B_PK2", "B_MM", "B_U", "B_V", "B_W", "B_PP", "B_PPD", "B_ARU", "B_ARV",
+    "B_ARW", "B_DIV", "B_XA", "B_TOT", "B_MU", "B_MV", "B_MW", "B_DIVX2", "B_SC", "B_EC", "B_ED", "B_FPKF",
+)
+W_SLOTS = {"XW", "AW", "RRW", "WTEN", "ARW", "B_OW", "B_XW", "B_AW", "B_RRW", "B_WTEN", "B_W", "B_ARW", "B_MW"}
+_RD, _RV, _CP, _CPV, _G = F32(287.04), F32(461.5), F32(1005.7), F32(1870.0), F32(9.81)
+THETA_WEPS = float(F32(10.0) * F32(1.0e-18))
+
+
+def stage_consts(dt: float, nsound: int, nrk: int, dx: float, dy: float, dz: float) -> tuple[np.ndarray, int]:
+    """The binary32 scalars of one stage (adjoint_step.hip's Consts) and its substep count."""
+    nloop, dts = sound3_stage_schedule_f32(dt, nsound, nrk)
+    dts = F32(dts)
+    rdx, rdy, rdz = F32(1.0) / F32(dx), F32(1.0) / F32(dy), F32(1.0) / F32(dz)
+    alph, kdiv = F32(0.60), F32(0.10)
+    rovcp = _RD / _CP
+    cpm = _CP + _CPV * F32(0.0)
+    vals = [rdx, rdy, rdz, dts, F32(F32(dt) / F32(4 - nrk)), alph, F32(1.0) - alph, kdiv, _CP * F32(0.5), _CP,
+            F32(1.0) / F32(nloop), (dts * F32(0.5)) * rdx, (dts * F32(0.5)) * rdy,
+            _G, rovcp, _RV / _RD, _RV / _RD - F32(1.0), cpm, F32(1.0) / ((_CP - _RD) + F32(0.0)),
+            (_RD + _RV * F32(0.0)) - rovcp * cpm, alph * alph, F32(THETA_WEPS)]
+    return np.array(vals, F32), nloop
+
+
+def pack_columns(metrics: dict[str, Any], base: dict[str, Any], nk: int, dz: float) -> np.ndarray:
+    """rho rr0 rf0 rrf0 c1 c2 th0 pi0 qv0 rth rds rdsf of one uniform column, packed for the kernels."""
+    col = lambda n: np.asarray(metrics[n], F32)[:, 0, 0]  # noqa: E731
+    prof = terrain_coordinate_profiles(nk, dz)
+    th0 = np.asarray(base["th0"], F32)
+    parts = [col("rho0_3d"), col("rr0"), col("rf0"), col("rrf0"), col("c1"), col("c2"), th0,
+             np.asarray(base["pi0"], F32), np.asarray(base["qv0"], F32), F32(1.0) / th0, prof.rds, prof.rdsf]
+    out = np.concatenate(parts).astype(F32)
+    assert out.size == 12 * nk + 8
+    return out
+
+
+def _require_flat_periodic(cfg, metrics) -> None:
+    if tuple(cfg.lateral_sides) != (1, 1, 1, 1) or cfg.moist or cfg.stretched is not None:
+        raise ValueError("the halo-free step is the dry, uniform-column, doubly periodic case")
+    for name, val in (("gz", 1.0), ("rgzu", 1.0), ("rgzv", 1.0), ("dzdx", 0.0), ("dzdy", 0.0)):
+        if not np.all(np.asarray(metrics[name]) == val):
+            raise ValueError(f"the halo-free step needs flat metrics ({name} != {val})")
+    rho = np.asarray(metrics["rho0_3d"])
+    if not np.all(rho == rho[:, :1, :1]):
+        raise ValueError("the halo-free step needs a horizontally uniform base state")
+
+
+class DryPeriodicStep:
+    """Workspace and launches of the halo-free dry periodic step (one device, one stream)."""
+
+    def __init__(self, cfg, metrics, base, device="cuda", stream: Any | None = None):
+        _require_flat_periodic(cfg, metrics)
+        self.cfg = cfg
+        self.ni, self.nj, self.nk = cfg.ni, cfg.nj, cfg.nk
+        if min(self.ni, self.nj, self.nk) < 6:
+            raise ValueError("ni, nj and nk must be at least 6")
+        self.dev = torch.device(device)
+        self.stream = stream
+        self.consts = {}
+        self.nloops = {}
+        for nrk in (1, 2, 3):
+            self.consts[nrk], self.nloops[nrk] = stage_consts(cfg.dt, cfg.nsound, nrk, cfg.dx, cfg.dy, cfg.dz)
+        self.nloop_max = max(self.nloops.values())
+        ni, nj, nk = self.ni, self.nj, self.nk
+        self.shape = (nk, nj, ni)
+        self.wshape = (nk + 1, nj, ni)
+        t = {}
+        with self._on_stream():
+            for s in SLOTS:
+                if s == "COL":
+                    t[s] = torch.from_numpy(pack_columns(metrics, base, nk, cfg.dz)).to(self.dev)
+                elif s in ("SU", "SV", "SPP", "SPPD"):
+                    t[s] = torch.zeros((self.nloop_max + 1,) + self.shape, dtype=torch.float32, device=self.dev)
+                elif s == "SW":
+                    t[s] = torch.zeros((self.nloop_max + 1,) + self.wshape, dtype=torch.float32, device=self.dev)
+                else:
+                    t[s] = torch.zeros(self.wshape if s in W_SLOTS else self.shape, dtype=torch.float32,
+                                       device=self.dev)
+        self.t = t
+        self.table = (ctypes.c_void_p * len(SLOTS))(*(t[s].data_ptr() for s in SLOTS))
+
+    def _on_stream(self):
+        return torch.cuda.stream(self.stream) if self.stream is not None else torch.cuda.stream(
+            torch.cuda.current_stream(self.dev))
+
+    def _h(self):
+        return stream_handle(self.stream)
+
+    def _load(self, pairs):
+        with self._on_stream():
+            for slot, x in pairs:
+                self.t[slot].copy_(x)
+
+    # ------------------------------------------------------------ forward
+    def forward_stage(self, nrk: int, X: dict, A: dict, ppx: torch.Tensor) -> tuple[dict, torch.Tensor]:
+        """One RK stage from X (u v w th pp), the step-start state A and ppx; returns new X and ppx.
+
+        At nrk = 3 the returned ppx already holds solve3's pp + ppx.  The
+        workspace keeps the stage's intermediates and substep states.
+        """
+        self._load([("XU", X["u"]), ("XV", X["v"]), ("XW", X["w"]), ("XTH", X["th"]), ("XPP", X["pp"]),
+                    ("AU", A["u"]), ("AV", A["v"]), ("AW", A["w"]), ("ATH", A["th"]), ("API", A["pp"]),
+                    ("PPX", ppx)])
+        self._launch_forward(nrk)
+        n = self.nloops[nrk]
+        t = self.t
+        with self._on_stream():
+            Xn = {"u": t["SU"][n].clone(), "v": t["SV"][n].clone(), "w": t["SW"][n].clone(),
+                  "th": t["THO"].clone(), "pp": t["SPP"][n].clone()}
+            ppx_out = t["PPXO"].clone() if nrk == 3 else ppx.clone()
+        return Xn, ppx_out
+
+    def _launch_forward(self, nrk: int) -> None:
+        c = self.consts[nrk]
+        _native.call("gm1hip_dry_stage_forward", ctypes.cast(self.table, ctypes.c_void_p),
+                     c.ctypes.data_as(ctypes.c_void_p), self.ni, self.nj, self.nk, nrk, self.nloops[nrk],
+                     THETA_WEPS, self._h())
+
+    def forward(self, S: dict) -> dict:
+        """One large step of the state S (u v w th pp ppx); returns the new state."""
+        A = {n: S[n] for n in ("u", "v", "w", "th", "pp")}
+        X, ppx = dict(A), S["ppx"]
+        for nrk in (1, 2, 3):
+            X, ppx = self.forward_stage(nrk, X, A, ppx)
+        return {**X, "ppx": ppx}
+
+    def empty_state(self) -> dict:
+        with self._on_stream():
+            return {n: torch.zeros(self.wshape if n == "w" else self.shape, dtype=torch.float32, device=self.dev)
+                    for n in STATE}
+
+
+def from_padded(a: dict, ni: int, nj: int, nk: int, ng: int = 3) -> dict:
+    """Halo-free state from a padded large-step state (torch tensors or arrays)."""
+    return {"u": a["u3d"][1 : nk + 1, ng : ng + nj, ng : ng + ni],
+            "v": a["v3d"][1 : nk + 1, ng : ng + nj, ng : ng + ni],
+            "w": a["w3d"][1 : nk + 2, ng : ng + nj, ng : ng + ni],
+            "th": a["th3d"],
+            "pp": a["pp3d"][1 : nk + 1, 1 : nj + 1, 1 : ni + 1],
+            "ppx": a["ppx"][1 : nk + 1, 1 : nj + 1, 1 : ni + 1]}
+
+
+__all__ = ["DryPeriodicStep", "STATE", "from_padded", "pack_columns", "stage_consts"]

@@ -1,0 +1,730 @@
+// Dry large step and its adjoint on halo-free periodic fields (gfx950).
+//
+// Geometry: flat terrain, horizontally uniform base state, CM1's (1,1,1,1)
+// lateral sides.  Fields carry no halo: u, v and cell fields are
+// (nk, nj, ni), w is (nk+1, nj, ni) holding levels 1..nk+1; u face f is the
+// west face of cell f, v face g the south face of row g; every lateral
+// neighbour is a periodic index.  The forward stage restates one RK stage
+// of the large step (cuda_gm1_forecast.py:1241-1345 with the acoustic stage
+// cuda_psolver3_terrain_open.py:774-893 and the scalar stage
+// cuda_scalar_moist_rk.py:283-349) in the reference's association order, so
+// it produces the words of the padded device step; it keeps every acoustic
+// substep's state (u, v, w, pp, ppd) as the adjoint's trajectory.  The
+// adjoint stage is its transpose at that trajectory with the upwind choices,
+// WENO5 weights and positive-definite limiters frozen at their forward
+// values and the smeps clamps taken as the identity.
+#include "cm1_common.h"
+
+namespace {
+
+using gm1::flx3;
+using gm1::flx5;
+using gm1::upstrpd;
+using gm1::weno5;
+
+// Pointer-table slots of gm1hip_dry_stage_forward / gm1hip_dry_stage_adjoint.
+enum Slot {
+  COL,                                    // packed columns (see Cols)
+  XU, XV, XW, XTH, XPP,                   // stage input state
+  AU, AV, AW, ATH, API,                   // large-step start state (ua va wa tha ppi)
+  PPX,                                    // ppx entering the stage
+  THV, PPTERM, PIADV, DUM8, THTERM,       // stage terms
+  RRU, RRV, RRW, DIVX, UTEN, VTEN, WTEN,  // momentum mass fluxes, divx, tendencies
+  QK, PK1, PK2, MM, FPK, EC, ED,          // acoustic coefficients and column scratch
+  SU, SV, SW, SPP, SPPD,                  // substep states, (nloop+1) planes each; plane n = after substep n
+  ARU, ARV, ARW,                          // stage-averaged winds
+  PPXO,                                   // ppx leaving the stage (nrk 3: after solve3)
+  DIV, TDIV, DIVX2, THO,                  // post-acoustic divergence, theta_div, divx, theta out
+  // adjoint
+  B_OU, B_OV, B_OW, B_OTH, B_OPP,         // in: adjoints of the stage outputs
+  B_PPX,                                  // in: adjoint of PPXO; out: adjoint of PPX
+  B_XU, B_XV, B_XW, B_XTH, B_XPP,         // out: adjoints of the stage inputs X
+  B_AU, B_AV, B_AW, B_ATH, B_API,         // accumulated adjoints of A
+  B_THV, B_PPTERM, B_PIADV, B_DUM8, B_THTERM,
+  B_RRU, B_RRV, B_RRW, B_DIVX, B_UTEN, B_VTEN, B_WTEN,
+  B_QK, B_PK1, B_PK2, B_MM, B_U, B_V, B_W, B_PP, B_PPD, B_ARU, B_ARV, B_ARW,
+  B_DIV, B_XA, B_TOT, B_MU, B_MV, B_MW, B_DIVX2, B_SC, B_EC, B_ED, B_FPKF,
+  NSLOT
+};
+
+// Scalars of one stage, formed on the host in binary32 as the restatement forms them.
+struct Consts {
+  float rdx, rdy, rdz, dts, dttmp, alph, beta, kdiv, halfcp, cp, tavg, temx, temy;
+  float g, rovcp, reps, repsm1, cpm, cvm, kterm, asq, weps_f;
+};
+constexpr int NCONST = 22;
+
+struct Cols {
+  const float *rho, *rr0, *rf0, *rrf0, *c1, *c2, *th0, *pi0, *qv0, *rth, *rds, *rdsf;
+};
+
+struct W {
+  float* p[NSLOT];
+  int ni, nj, nk, nrk, nloop;
+  double weps;
+  Consts c;
+  Cols col;
+  // index of (level l, row j, column i), levels 1-based
+  __device__ __forceinline__ int64_t at(int l, int j, int i) const {
+    return ((int64_t)(l - 1) * nj + j) * ni + i;
+  }
+  __device__ __forceinline__ int X(int i, int o) const { return (i + o + 4 * ni) % ni; }
+  __device__ __forceinline__ int Y(int j, int o) const { return (j + o + 4 * nj) % nj; }
+  __device__ __forceinline__ int64_t plane() const { return (int64_t)nk * nj * ni; }
+  __device__ __forceinline__ int64_t wplane() const { return (int64_t)(nk + 1) * nj * ni; }
+  // substep snapshot n of a cell/u/v array or of w
+  __device__ __forceinline__ float* snap(int s, int n) const { return p[s] + (int64_t)n * plane(); }
+  __device__ __forceinline__ float* wsnap(int n) const { return p[SW] + (int64_t)n * wplane(); }
+};
+
+Cols make_cols(const float* col, int nk) {
+  Cols c;
+  c.rho = col;
+  c.rr0 = col + nk;
+  c.rf0 = col + 2 * nk;
+  c.rrf0 = col + 3 * nk + 1;
+  c.c1 = col + 4 * nk + 2;
+  c.c2 = col + 5 * nk + 3;
+  c.th0 = col + 6 * nk + 4;
+  c.pi0 = col + 7 * nk + 4;
+  c.qv0 = col + 8 * nk + 4;
+  c.rth = col + 9 * nk + 4;
+  c.rds = col + 10 * nk + 4;
+  c.rdsf = col + 11 * nk + 6;
+  return c;
+}
+
+#define CELL_INDEX                                               \
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;           \
+  const int j = blockIdx.y, l = blockIdx.z + 1;                  \
+  if (i >= w.ni) return;
+
+// ------------------------------------------------------------------ forward
+
+__global__ __launch_bounds__(64) void k_terms(W w) {
+  CELL_INDEX
+  const Consts& c = w.c;
+  const int64_t q = w.at(l, j, i);
+  const float th = w.p[XTH][q], pp = w.p[XPP][q];
+  const float theta = w.col.th0[l - 1] + th;
+  float b = th * w.col.rth[l - 1] + c.repsm1 * (0.0f - w.col.qv0[l - 1]);
+  b = b - ((0.0f - 0.0f) - 0.0f);
+  w.p[THV][q] = (theta * (1.0f + c.reps * 0.0f)) / ((1.0f + 0.0f) + 0.0f);
+  w.p[THTERM][q] = (theta * c.kterm) * c.cvm;
+  const float pi = w.col.pi0[l - 1] + pp;
+  w.p[PPTERM][q] = ((pi * c.rovcp) * c.cpm) * c.cvm;
+  w.p[PIADV][q] = w.col.pi0[l - 1] + pp;
+  w.p[DUM8][q] = c.g * b;
+}
+
+__global__ __launch_bounds__(64) void k_mass(W w) {
+  CELL_INDEX
+  const int64_t q = w.at(l, j, i);
+  const float den = 0.5f * (w.col.rho[l - 1] + w.col.rho[l - 1]);
+  w.p[RRU][q] = den * w.p[XU][q];
+  w.p[RRV][q] = den * w.p[XV][q];
+  w.p[RRW][q] = (l >= 2) ? w.col.rf0[l - 1] * w.p[XW][q] : 0.0f;
+  if (l == w.nk) w.p[RRW][w.at(w.nk + 1, j, i)] = 0.0f;
+}
+
+__global__ __launch_bounds__(64) void k_divx(W w) {
+  CELL_INDEX
+  const Consts& c = w.c;
+  const float* ru = w.p[RRU];
+  const float* rv = w.p[RRV];
+  const float* rw = w.p[RRW];
+  float d = (ru[w.at(l, j, w.X(i, 1))] - ru[w.at(l, j, i)]) * c.rdx + (rv[w.at(l, w.Y(j, 1), i)] - rv[w.at(l, j, i)]) * c.rdy;
+  d = d + (rw[w.at(l + 1, j, i)] - rw[w.at(l, j, i)]) * c.rdz;
+  w.p[DIVX][w.at(l, j, i)] = fabsf(d) < 1.0e-20f ? 0.0f : d;
+}
+
+// Vertical stencil kinds of the u/v and w vertical fluxes and of the scalar z flux.
+enum Kind { LIN = 0, HALF2 = 1, F3 = 2, F5 = 3, UPD = 4, WENO = 5 };
+struct Stencil {
+  Kind kind;
+  int lev[5];
+};
+
+__device__ __forceinline__ void uvz_branch(int kf, int nk, Stencil& pos, Stencil& neg) {
+  if (kf == 2) {
+    pos = {LIN, {1, 2}};
+    neg = {F3, {3, 2, 1}};
+  } else if (kf == nk) {
+    pos = {F3, {nk - 2, nk - 1, nk}};
+    neg = {LIN, {nk - 1, nk}};
+  } else if (kf == 3) {
+    pos = {F3, {1, 2, 3}};
+    neg = {F5, {5, 4, 3, 2, 1}};
+  } else if (kf == nk - 1) {
+    pos = {F5, {kf - 3, kf - 2, kf - 1, kf, kf + 1}};
+    neg = {F3, {kf + 1, kf, kf - 1}};
+  } else {
+    pos = {F5, {kf - 3, kf - 2, kf - 1, kf, kf + 1}};
+    neg = {F5, {kf + 2, kf + 1, kf, kf - 1, kf - 2}};
+  }
+}
+
+__device__ __forceinline__ void wz_branch(int k, int nk, Stencil& pos, Stencil& neg) {
+  if (k == 1) {
+    pos = {HALF2, {1, 2}};
+    neg = {F3, {3, 2, 1}};
+  } else if (k == 2) {
+    pos = {F3, {1, 2, 3}};
+    neg = {F5, {5, 4, 3, 2, 1}};
+  } else if (k == nk - 1) {
+    pos = {F5, {k - 2, k - 1, k, k + 1, k + 2}};
+    neg = {F3, {k + 2, k + 1, k}};
+  } else if (k == nk) {
+    pos = {F3, {k - 1, k, k + 1}};
+    neg = {HALF2, {k, k + 1}};
+  } else {
+    pos = {F5, {k - 2, k - 1, k, k + 1, k + 2}};
+    neg = {F5, {k + 3, k + 2, k + 1, k, k - 1}};
+  }
+}
+
+// scalar z flux at face k: (pos, neg, strict '>' selection)
+__device__ __forceinline__ bool zflux_branch(int k, int nk, bool weno, Stencil& pos, Stencil& neg) {
+  const Kind up3 = weno ? UPD : F3;
+  const Kind r5 = weno ? WENO : F5;
+  if (k == 2) {
+    pos = {LIN, {1, 2}};
+    neg = {up3, {3, 2, 1}};
+    return false;
+  }
+  if (k == nk) {
+    pos = {up3, {nk - 2, nk - 1, nk}};
+    neg = {LIN, {nk - 1, nk}};
+    return weno;
+  }
+  if (k == 3) {
+    pos = {up3, {1, 2, 3}};
+    neg = {r5, {5, 4, 3, 2, 1}};
+    return false;
+  }
+  if (k == nk - 1) {
+    pos = {r5, {k - 3, k - 2, k - 1, k, k + 1}};
+    neg = {up3, {k + 1, k, k - 1}};
+    return false;
+  }
+  pos = {r5, {k - 3, k - 2, k - 1, k, k + 1}};
+  neg = {r5, {k + 2, k + 1, k, k - 1, k - 2}};
+  return false;
+}
+
+template <typename Q>
+__device__ __forceinline__ float eval_branch(const Stencil& s, Q q, float c2h, float c1lin, double weps, float wf) {
+  switch (s.kind) {
+    case LIN: return c1lin * q(s.lev[0]) + c2h * q(s.lev[1]);
+    case HALF2: return 0.5f * (q(s.lev[0]) + q(s.lev[1]));
+    case F3: return flx3(q(s.lev[0]), q(s.lev[1]), q(s.lev[2]));
+    case F5: return flx5(q(s.lev[0]), q(s.lev[1]), q(s.lev[2]), q(s.lev[3]), q(s.lev[4]));
+    case UPD: return upstrpd(q(s.lev[0]), q(s.lev[1]), q(s.lev[2]), wf);
+    default: return weno5(q(s.lev[0]), q(s.lev[1]), q(s.lev[2]), q(s.lev[3]), q(s.lev[4]), weps);
+  }
+}
+
+// Coefficients of the stencil's values (frozen WENO5 weights and limiter) in branch order.
+template <typename Q>
+__device__ __forceinline__ int branch_coefs(const Stencil& s, Q q, float c2h, float c1lin, float wf, float* cf) {
+  switch (s.kind) {
+    case LIN: cf[0] = c1lin; cf[1] = c2h; return 2;
+    case HALF2: cf[0] = 0.5f; cf[1] = 0.5f; return 2;
+    case F3: cf[0] = gm1::CM1_6; cf[1] = gm1::C5_6; cf[2] = gm1::C2_6; return 3;
+    case F5:
+      cf[0] = gm1::C2_60; cf[1] = gm1::CM13_60; cf[2] = gm1::C47_60; cf[3] = gm1::C27_60; cf[4] = gm1::CM3_60;
+      return 5;
+    case UPD: {
+      const float s1 = q(s.lev[0]), s2 = q(s.lev[1]), s3 = q(s.lev[2]);
+      const float dd = s2 - s1;
+      const float se = copysignf(fabsf(wf), dd);
+      const float rr = (s3 - s2) / (se + dd);
+      const float cand = gm1::C1_6 + gm1::C2_6 * rr;
+      const float phi = fmaxf(0.0f, fminf(rr, fminf(cand, 1.0f)));
+      cf[0] = -phi; cf[1] = 1.0f + phi; cf[2] = 0.0f;
+      return 3;
+    }
+    default: return -1;
+  }
+}
+
+// Frozen WENO5 coefficients of s1..s5 (weights as weno5 forms them).
+__device__ __forceinline__ void weno_coefs(float s1, float s2, float s3, float s4, float s5, double weps, float* cf) {
+  const float d1 = (s1 - 2.0f * s2) + s3;
+  const float e1 = (s1 - 4.0f * s2) + 3.0f * s3;
+  const float b1r = gm1::C13_12 * (d1 * d1) + 0.25f * (e1 * e1);
+  const float d2 = (s2 - 2.0f * s3) + s4;
+  const float e2 = s2 - s4;
+  const float b2r = gm1::C13_12 * (d2 * d2) + 0.25f * (e2 * e2);
+  const float d3 = (s3 - 2.0f * s4) + s5;
+  const float e3 = (3.0f * s3 - 4.0f * s4) + s5;
+  const float b3r = gm1::C13_12 * (d3 * d3) + 0.25f * (e3 * e3);
+  const double b1 = b1r, b2 = b2r, b3 = b3r;
+  const double tau = fabs(b1 - b3);
+  const double q1 = fmin(1.0e30, tau / (b1 + weps));
+  const double q2 = fmin(1.0e30, tau / (b2 + weps));
+  const double q3 = fmin(1.0e30, tau / (b3 + weps));
+  const double w1 = (double)0.1f * (1.0 + q1 * q1);
+  const double w2 = (double)0.6f * (1.0 + q2 * q2);
+  const double w3 = (double)0.3f * (1.0 + q3 * q3);
+  const double den = (w1 + w2) + w3;
+  const double o1 = w1 / den, o2 = w2 / den, o3 = w3 / den;
+  cf[0] = (float)(o1 * (double)gm1::C2_6);
+  cf[1] = (float)(o1 * (double)gm1::CM7_6 + o2 * (double)gm1::CM1_6);
+  cf[2] = (float)((o1 * (double)gm1::C11_6 + o2 * (double)gm1::C5_6) + o3 * (double)gm1::C2_6);
+  cf[3] = (float)(o2 * (double)gm1::C2_6 + o3 * (double)gm1::C5_6);
+  cf[4] = (float)(o3 * (double)gm1::CM1_6);
+}
+
+template <typename Q>
+__device__ __forceinline__ int stencil_coefs(const Stencil& s, Q q, float c2h, float c1lin, double weps, float wf,
+                                             float* cf) {
+  if (s.kind == WENO) {
+    weno_coefs(q(s.lev[0]), q(s.lev[1]), q(s.lev[2]), q(s.lev[3]), q(s.lev[4]), weps, cf);
+    return 5;
+  }
+  return branch_coefs(s, q, c2h, c1lin, wf, cf);
+}
+
+// u tendency at (l, j, face i)
+__global__ __launch_bounds__(64) void k_advu(W w) {
+  CELL_INDEX
+  const Consts& c = w.c;
+  const float *u = w.p[XU], *ru = w.p[RRU], *rv = w.p[RRV], *rw = w.p[RRW], *dx = w.p[DIVX];
+  auto U = [&](int ll, int jj, int ii) { return u[w.at(ll, w.Y(jj, 0), w.X(ii, 0))]; };
+  auto fx = [&](int cc) {
+    const float m = 0.5f * (ru[w.at(l, j, w.X(cc, 0))] + ru[w.at(l, j, w.X(cc, 1))]);
+    const float pos = flx5(U(l, j, cc - 2), U(l, j, cc - 1), U(l, j, cc), U(l, j, cc + 1), U(l, j, cc + 2));
+    const float neg = flx5(U(l, j, cc + 3), U(l, j, cc + 2), U(l, j, cc + 1), U(l, j, cc), U(l, j, cc - 1));
+    return m * (m >= 0.0f ? pos : neg);
+  };
+  auto fy = [&](int pp) {
+    const float m = 0.5f * (rv[w.at(l, w.Y(pp, 0), i)] + rv[w.at(l, w.Y(pp, 0), w.X(i, -1))]);
+    const float pos = flx5(U(l, pp - 3, i), U(l, pp - 2, i), U(l, pp - 1, i), U(l, pp, i), U(l, pp + 1, i));
+    const float neg = flx5(U(l, pp + 2, i), U(l, pp + 1, i), U(l, pp, i), U(l, pp - 1, i), U(l, pp - 2, i));
+    return m * (m >= 0.0f ? pos : neg);
+  };
+  auto fz = [&](int kf) {
+    const float pair = rw[w.at(kf, j, i)] + rw[w.at(kf, j, w.X(i, -1))];
+    const float m = 0.5f * pair;
+    const float c2h = 0.5f * (w.col.c2[kf - 1] + w.col.c2[kf - 1]);
+    Stencil ps, ng;
+    uvz_branch(kf, w.nk, ps, ng);
+    auto q = [&](int ll) { return U(ll, j, i); };
+    const float r = m >= 0.0f ? eval_branch(ps, q, c2h, 1.0f - c2h, 0.0, 0.0f)
+                              : eval_branch(ng, q, c2h, 1.0f - c2h, 0.0, 0.0f);
+    return m * r;
+  };
+  const float advx = (-(fx(i) - fx(i - 1))) * c.rdx;
+  const float advy = (-(fy(j + 1) - fy(j))) * c.rdy;
+  float advz;
+  if (l == 1) advz = (-fz(2)) * c.rdz;
+  else if (l == w.nk) advz = fz(w.nk) * c.rdz;
+  else advz = (-(fz(l + 1) - fz(l))) * c.rdz;
+  float total = (advx + advy) + advz;
+  total = total + U(l, j, i) * (0.5f * (dx[w.at(l, j, i)] + dx[w.at(l, j, w.X(i, -1))]));
+  const float rho = w.col.rho[l - 1];
+  w.p[UTEN][w.at(l, j, i)] = total * (1.0f / (0.5f * (rho + rho)));
+}
+
+// v tendency at (l, face j, i)
+__global__ __launch_bounds__(64) void k_advv(W w) {
+  CELL_INDEX
+  const Consts& c = w.c;
+  const float *v = w.p[XV], *ru = w.p[RRU], *rv = w.p[RRV], *rw = w.p[RRW], *dx = w.p[DIVX];
+  auto V = [&](int ll, int jj, int ii) { return v[w.at(ll, w.Y(jj, 0), w.X(ii, 0))]; };
+  auto fx = [&](int pp) {
+    const float m = 0.5f * (ru[w.at(l, j, w.X(pp, 0))] + ru[w.at(l, w.Y(j, -1), w.X(pp, 0))]);
+    const float pos = flx5(V(l, j, pp - 3), V(l, j, pp - 2), V(l, j, pp - 1), V(l, j, pp), V(l, j, pp + 1));
+    const float neg = flx5(V(l, j, pp + 2), V(l, j, pp + 1), V(l, j, pp), V(l, j, pp - 1), V(l, j, pp - 2));
+    return m * (m >= 0.0f ? pos : neg);
+  };
+  auto fy = [&](int cc) {
+    const float m = 0.5f * (rv[w.at(l, w.Y(cc, 0), i)] + rv[w.at(l, w.Y(cc, 1), i)]);
+    const float pos = flx5(V(l, cc - 2, i), V(l, cc - 1, i), V(l, cc, i), V(l, cc + 1, i), V(l, cc + 2, i));
+    const float neg = flx5(V(l, cc + 3, i), V(l, cc + 2, i), V(l, cc + 1, i), V(l, cc, i), V(l, cc - 1, i));
+    return m * (m >= 0.0f ? pos : neg);
+  };
+  auto fz = [&](int kf) {
+    const float pair = rw[w.at(kf, j, i)] + rw[w.at(kf, w.Y(j, -1), i)];
+    const float m = 0.5f * pair;
+    const float c2h = 0.5f * (w.col.c2[kf - 1] + w.col.c2[kf - 1]);
+    Stencil ps, ng;
+    uvz_branch(kf, w.nk, ps, ng);
+    auto q = [&](int ll) { return V(ll, j, i); };
+    const float r = m >= 0.0f ? eval_branch(ps, q, c2h, 1.0f - c2h, 0.0, 0.0f)
+                              : eval_branch(ng, q, c2h, 1.0f - c2h, 0.0, 0.0f);
+    return m * r;
+  };
+  const float advx = (-(fx(i + 1) - fx(i))) * c.rdx;
+  const float advy = (-(fy(j) - fy(j - 1))) * c.rdy;
+  float advz;
+  if (l == 1) advz = (-fz(2)) * c.rdz;
+  else if (l == w.nk) advz = fz(w.nk) * c.rdz;
+  else advz = (-(fz(l + 1) - fz(l))) * c.rdz;
+  float total = (advx + advy) + advz;
+  total = total + V(l, j, i) * (0.5f * (dx[w.at(l, j, i)] + dx[w.at(l, w.Y(j, -1), i)]));
+  const float rho = w.col.rho[l - 1];
+  w.p[VTEN][w.at(l, j, i)] = total * (1.0f / (0.5f * (rho + rho)));
+}
+
+// w tendency (with buoyancy) at levels 2..nk; levels 1 and nk+1 are zero
+__global__ __launch_bounds__(64) void k_advw(W w) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, kf = blockIdx.z + 1;
+  if (i >= w.ni) return;
+  const Consts& c = w.c;
+  const int nk = w.nk;
+  if (kf == 1 || kf == nk + 1) {
+    w.p[WTEN][w.at(kf, j, i)] = 0.0f;
+    return;
+  }
+  const float *ww = w.p[XW], *ru = w.p[RRU], *rv = w.p[RRV], *rw = w.p[RRW], *dx = w.p[DIVX];
+  auto Wv = [&](int ll, int jj, int ii) { return ww[w.at(ll, w.Y(jj, 0), w.X(ii, 0))]; };
+  const float c2h = 0.5f * (w.col.c2[kf - 1] + w.col.c2[kf - 1]);
+  const float c1h = 1.0f - c2h;
+  auto fx = [&](int pp) {
+    const float m = c2h * ru[w.at(kf, j, w.X(pp, 0))] + c1h * ru[w.at(kf - 1, j, w.X(pp, 0))];
+    const float pos = flx5(Wv(kf, j, pp - 3), Wv(kf, j, pp - 2), Wv(kf, j, pp - 1), Wv(kf, j, pp), Wv(kf, j, pp + 1));
+    const float neg = flx5(Wv(kf, j, pp + 2), Wv(kf, j, pp + 1), Wv(kf, j, pp), Wv(kf, j, pp - 1), Wv(kf, j, pp - 2));
+    return m * (m >= 0.0f ? pos : neg);
+  };
+  auto fy = [&](int pp) {
+    const float m = c2h * rv[w.at(kf, w.Y(pp, 0), i)] + c1h * rv[w.at(kf - 1, w.Y(pp, 0), i)];
+    const float pos = flx5(Wv(kf, pp - 3, i), Wv(kf, pp - 2, i), Wv(kf, pp - 1, i), Wv(kf, pp, i), Wv(kf, pp + 1, i));
+    const float neg = flx5(Wv(kf, pp + 2, i), Wv(kf, pp + 1, i), Wv(kf, pp, i), Wv(kf, pp - 1, i), Wv(kf, pp - 2, i));
+    return m * (m >= 0.0f ? pos : neg);
+  };
+  auto wz = [&](int k) {
+    const float m = 0.5f * (rw[w.at(k, j, i)] + rw[w.at(k + 1, j, i)]);
+    Stencil ps, ng;
+    wz_branch(k, nk, ps, ng);
+    auto q = [&](int ll) { return Wv(ll, j, i); };
+    const float r = m >= 0.0f ? eval_branch(ps, q, 0.0f, 0.0f, 0.0, 0.0f) : eval_branch(ng, q, 0.0f, 0.0f, 0.0, 0.0f);
+    return m * r;
+  };
+  const float advx = (-(fx(i + 1) - fx(i))) * (c.rdx * 1.0f);
+  const float advy = (-(fy(j + 1) - fy(j))) * (c.rdy * 1.0f);
+  const float advz = (-(wz(kf) - wz(kf - 1))) * c.rdz;
+  const float cc2 = w.col.c2[kf - 1];
+  const float cc1 = 1.0f - cc2;
+  float total = (advx + advy) + advz;
+  total = total + Wv(kf, j, i) * (cc2 * dx[w.at(kf, j, i)] + cc1 * dx[w.at(kf - 1, j, i)]);
+  const float adv = total * (w.col.rrf0[kf - 1] * 1.0f);
+  const float* d8 = w.p[DUM8];
+  const float add = w.col.c1[kf - 1] * d8[w.at(kf - 1, j, i)] + w.col.c2[kf - 1] * d8[w.at(kf, j, i)];
+  w.p[WTEN][w.at(kf, j, i)] = adv + add;
+}
+
+// ------------------------------------------------------------------ acoustic stage (forward)
+
+__global__ __launch_bounds__(64) void k_coef(W w) {
+  CELL_INDEX
+  const Consts& c = w.c;
+  const int nk = w.nk;
+  const int64_t q = w.at(l, j, i);
+  const float tem = c.dts * w.col.rdsf[l];
+  w.p[QK][q] = (tem * w.p[PPTERM][q]) * 1.0f;
+  const float tem2 = ((c.dts * 1.0f) * w.col.rdsf[l]) * w.col.rr0[l - 1];
+  const float* pia = w.p[PIADV];
+  float pk2 = 0.0f, pk1 = 0.0f, mm = 0.0f;
+  if (l < nk) {
+    const float b2 = (-w.col.c2[l]) * pia[w.at(l + 1, j, i)] + (1.0f - w.col.c1[l]) * pia[q];
+    pk2 = (tem2 * w.col.rf0[l]) * b2;
+  }
+  if (l > 1) {
+    const float b1 = w.col.c1[l - 1] * pia[w.at(l - 1, j, i)] + (w.col.c2[l - 1] - 1.0f) * pia[q];
+    pk1 = (tem2 * w.col.rf0[l - 1]) * b1;
+    const float tem1 = (c.dts * w.col.rds[l]) * c.cp;
+    const float th = w.col.c2[l - 1] * w.p[THV][q] + w.col.c1[l - 1] * w.p[THV][w.at(l - 1, j, i)];
+    mm = (tem1 * th) * 1.0f;
+  }
+  w.p[PK1][q] = pk1;
+  w.p[PK2][q] = pk2;
+  w.p[MM][q] = mm;
+}
+
+__global__ __launch_bounds__(64) void k_ac_start(W w) {
+  CELL_INDEX
+  const int64_t q = w.at(l, j, i);
+  const bool first = w.nrk == 1;
+  w.snap(SPPD, 0)[q] = w.p[PPX][q];
+  w.snap(SPP, 0)[q] = first ? w.p[XPP][q] : w.p[API][q];
+  w.snap(SU, 0)[q] = first ? w.p[XU][q] : w.p[AU][q];
+  w.snap(SV, 0)[q] = first ? w.p[XV][q] : w.p[AV][q];
+  w.wsnap(0)[q] = first ? w.p[XW][q] : w.p[AW][q];
+  w.p[ARU][q] = 0.0f;
+  w.p[ARV][q] = 0.0f;
+  w.p[ARW][q] = w.p[AW][q] * w.c.beta;
+  if (l == w.nk) {
+    const int64_t top = w.at(w.nk + 1, j, i);
+    w.wsnap(0)[top] = w.p[XW][top];
+    w.p[ARW][top] = 0.0f;
+  }
+}
+
+// substep n: pressure gradient on u and v faces, running sums of the winds
+__global__ __launch_bounds__(64) void k_ss_pgrad(W w, int n) {
+  CELL_INDEX
+  const Consts& c = w.c;
+  const bool last = n == w.nloop;
+  const float* thv = w.p[THV];
+  const float* ppd = w.snap(SPPD, n - 1);
+  const int64_t q = w.at(l, j, i);
+  {
+    const int64_t qw = w.at(l, j, w.X(i, -1));
+    const float pg = (c.halfcp * (thv[q] + thv[qw])) * ((ppd[q] - ppd[qw]) * c.rdx);
+    const float u1 = w.snap(SU, n - 1)[q] + c.dts * (w.p[UTEN][q] - pg);
+    w.snap(SU, n)[q] = u1;
+    w.p[ARU][q] = last ? (w.p[ARU][q] + u1) * c.tavg : w.p[ARU][q] + u1;
+  }
+  {
+    const int64_t qs = w.at(l, w.Y(j, -1), i);
+    const float pg = (c.halfcp * (thv[q] + thv[qs])) * ((ppd[q] - ppd[qs]) * c.rdy);
+    const float v1 = w.snap(SV, n - 1)[q] + c.dts * (w.p[VTEN][q] - pg);
+    w.snap(SV, n)[q] = v1;
+    w.p[ARV][q] = last ? (w.p[ARV][q] + v1) * c.tavg : w.p[ARV][q] + v1;
+  }
+}
+
+// fpk of one column at level l from the substep's u1, v1 and starting w (sound.F's fpk and pressure advection)
+__device__ __forceinline__ float column_fpk(const W& w, int n, int l, int j, int i, float* div_out) {
+  const Consts& c = w.c;
+  const int nk = w.nk;
+  const float* u1 = w.snap(SU, n);
+  const float* v1 = w.snap(SV, n);
+  const float* ws = w.wsnap(n - 1);
+  const float* pia = w.p[PIADV];
+  const int64_t q = w.at(l, j, i);
+  const int64_t qe = w.at(l, j, w.X(i, 1)), qn = w.at(l, w.Y(j, 1), i);
+  const float hz = (u1[qe] - u1[q]) * c.rdx + (v1[qn] - v1[q]) * c.rdy;
+  float div = 1.0f * (hz + 0.0f);
+  div = fabsf(div) < 1.0e-20f ? 0.0f : div;
+  if (div_out) *div_out = div;
+  const float wlo = l == 1 ? 0.0f : ws[q];
+  const float whi = l == nk ? 0.0f : ws[w.at(l + 1, j, i)];
+  const float src = c.dts * (0.0f - w.p[PPTERM][q] * div);
+  const float qp = (-w.p[QK][q]) * (whi - wlo);
+  float fpk = src + c.beta * qp;
+  const float rho = w.col.rho[l - 1];
+  const float uu2 = c.temx * (rho * u1[qe]), uu1 = c.temx * (rho * u1[q]);
+  const float vv2 = c.temy * (rho * v1[qn]), vv1 = c.temy * (rho * v1[q]);
+  const float ww2 = 0.0f + c.beta * ws[w.at(l + 1, j, i)];
+  const float ww1 = 0.0f + c.beta * wlo;
+  const float pc = pia[q];
+  const float xa = uu2 * (pia[qe] - pc) + uu1 * (pc - pia[w.at(l, j, w.X(i, -1))]);
+  const float ya = vv2 * (pia[qn] - pc) + vv1 * (pc - pia[w.at(l, w.Y(j, -1), i)]);
+  float hz2 = (-(xa + ya)) * w.col.rr0[l - 1];
+  hz2 = hz2 * 1.0f;
+  const float vt = ww1 * w.p[PK1][q] + ww2 * w.p[PK2][q];
+  return fpk + (hz2 + vt);
+}
+
+// Thomas rows of level k (2..nk) of one column
+struct Row {
+  float aa, bb, cc;
+};
+__device__ __forceinline__ Row thomas_row(const W& w, int k, int j, int i) {
+  const int nk = w.nk;
+  const float asq = w.c.asq;
+  const float *qk = w.p[QK], *pk1 = w.p[PK1], *pk2 = w.p[PK2], *mm = w.p[MM];
+  const int64_t q = w.at(k, j, i), qd = w.at(k - 1, j, i);
+  Row r;
+  r.aa = k >= 3 ? (-asq) * (mm[q] * (pk1[qd] + qk[qd])) : 0.0f;
+  r.cc = k < nk ? (-asq) * ((-mm[q]) * (pk2[q] - qk[q])) : 0.0f;
+  const float diag = (-(pk1[q] + qk[q])) + (pk2[qd] - qk[qd]);
+  r.bb = 1.0f - asq * (mm[q] * diag);
+  return r;
+}
+
+__global__ __launch_bounds__(64) void k_ss_column(W w, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= w.ni) return;
+  const Consts& c = w.c;
+  const int nk = w.nk;
+  const bool last = n == w.nloop;
+  const float* ws = w.wsnap(n - 1);
+  float* wn = w.wsnap(n);
+  const float* pp = w.snap(SPP, n - 1);
+  float *fpk = w.p[FPK], *ec = w.p[EC], *ed = w.p[ED];
+  for (int l = 1; l <= nk; ++l) fpk[w.at(l, j, i)] = column_fpk(w, n, l, j, i, nullptr);
+  // elimination of rows 2..nk (the top row solves w(nk))
+  for (int k = 2; k <= nk; ++k) {
+    const int64_t q = w.at(k, j, i), qd = w.at(k - 1, j, i);
+    const Row r = thomas_row(w, k, j, i);
+    const float mmk = w.p[MM][q];
+    const float pdelta = pp[q] - pp[qd];
+    const float fwk = c.dts * w.p[WTEN][q] + c.beta * ((-mmk) * pdelta);
+    const float udelta = (pp[q] + fpk[q]) - (pp[qd] + fpk[qd]);
+    const float dd = (ws[q] + fwk) + c.alph * ((-mmk) * udelta);
+    const float pivot = k == 2 ? r.bb : r.aa * ec[qd] + r.bb;
+    const float inv = 1.0f / pivot;
+    if (k < nk) {
+      ec[q] = (-r.cc) * inv;
+      ed[q] = (k == 2 ? dd : dd - r.aa * ed[qd]) * inv;
+    } else {
+      wn[q] = (nk == 2 ? dd : dd - r.aa * ed[qd]) * inv;
+    }
+  }
+  for (int k = nk - 1; k >= 2; --k) {
+    const int64_t q = w.at(k, j, i);
+    wn[q] = ec[q] * wn[w.at(k + 1, j, i)] + ed[q];
+  }
+  wn[w.at(1, j, i)] = 0.0f;
+  wn[w.at(nk + 1, j, i)] = ws[w.at(nk + 1, j, i)];
+  float* ppo = w.snap(SPP, n);
+  float* ppdo = w.snap(SPPD, n);
+  for (int l = 1; l <= nk; ++l) {
+    const int64_t q = w.at(l, j, i);
+    const float wlo = l == 1 ? 0.0f : wn[q];
+    const float whi = l == nk ? 0.0f : wn[w.at(l + 1, j, i)];
+    const float vt = wlo * (w.p[PK1][q] + w.p[QK][q]) + whi * (w.p[PK2][q] - w.p[QK][q]);
+    float up = (pp[q] + fpk[q]) + c.alph * vt;
+    up = fabsf(up) < 1.0e-20f ? 0.0f : up;
+    const float tend = c.kdiv * (up - pp[q]);
+    ppo[q] = up;
+    ppdo[q] = up + tend;
+    if (w.nrk == 3 && last) w.p[PPXO][q] = tend;
+    float* arw = w.p[ARW];
+    arw[q] = last ? (arw[q] + wn[q] * c.alph) * c.tavg : arw[q] + wn[q];
+  }
+}
+
+// ------------------------------------------------------------------ post-acoustic and theta (forward)
+
+__device__ __forceinline__ float mw_at(const W& w, int l, int j, int i) {
+  return (l >= 2 && l <= w.nk) ? w.col.rf0[l - 1] * w.p[ARW][w.at(l, j, i)] + 0.0f : 0.0f;
+}
+__device__ __forceinline__ float mu_at(const W& w, int l, int j, int i) {
+  const float rho = w.col.rho[l - 1];
+  return ((0.5f * (rho + rho)) * w.p[ARU][w.at(l, j, i)]) * 1.0f;
+}
+__device__ __forceinline__ float mv_at(const W& w, int l, int j, int i) {
+  const float rho = w.col.rho[l - 1];
+  return ((0.5f * (rho + rho)) * w.p[ARV][w.at(l, j, i)]) * 1.0f;
+}
+
+__global__ __launch_bounds__(64) void k_post(W w) {
+  CELL_INDEX
+  const Consts& c = w.c;
+  const int nk = w.nk;
+  const int64_t q = w.at(l, j, i);
+  const int ie = w.X(i, 1), jn = w.Y(j, 1);
+  const float *aru = w.p[ARU], *arv = w.p[ARV], *arw = w.p[ARW];
+  // getdiv
+  auto wt = [&](int ll) { return (ll >= 2 && ll <= nk) ? arw[w.at(ll, j, i)] + 0.0f : 0.0f; };
+  float d = (aru[w.at(l, j, ie)] * 1.0f - aru[q] * 1.0f) * c.rdx + (arv[w.at(l, jn, i)] * 1.0f - arv[q] * 1.0f) * c.rdy;
+  d = d + (wt(l + 1) - wt(l)) * c.rdz;
+  d = fabsf(d) < 1.0e-20f ? 0.0f : d;
+  w.p[DIV][q] = d;
+  w.p[TDIV][q] = d * w.p[THTERM][q];
+  // getdivx
+  float dx = (mu_at(w, l, j, ie) - mu_at(w, l, j, i)) * c.rdx + (mv_at(w, l, jn, i) - mv_at(w, l, j, i)) * c.rdy;
+  dx = dx + (mw_at(w, l + 1, j, i) - mw_at(w, l, j, i)) * c.rdz;
+  w.p[DIVX2][q] = fabsf(dx) < 1.0e-20f ? 0.0f : dx;
+}
+
+__device__ __forceinline__ float sc_at(const W& w, int l, int j, int i) {
+  return (w.col.th0[l - 1] - 300.0f) + w.p[XTH][w.at(l, w.Y(j, 0), w.X(i, 0))];
+}
+
+__global__ __launch_bounds__(64) void k_theta(W w) {
+  CELL_INDEX
+  const Consts& c = w.c;
+  const int nk = w.nk;
+  const bool weno = w.nrk == 3;
+  const double weps = w.weps;
+  auto S = [&](int ll, int jj, int ii) { return sc_at(w, ll, jj, ii); };
+  auto recon = [&](float a, float b, float cc, float d, float e) {
+    return weno ? weno5(a, b, cc, d, e, weps) : flx5(a, b, cc, d, e);
+  };
+  auto fx = [&](int f) {
+    const float m = mu_at(w, l, j, w.X(f, 0));
+    const float pos = recon(S(l, j, f - 3), S(l, j, f - 2), S(l, j, f - 1), S(l, j, f), S(l, j, f + 1));
+    const float neg = recon(S(l, j, f + 2), S(l, j, f + 1), S(l, j, f), S(l, j, f - 1), S(l, j, f - 2));
+    return m * (m >= 0.0f ? pos : neg);
+  };
+  auto fy = [&](int g) {
+    const float m = mv_at(w, l, w.Y(g, 0), i);
+    const float pos = recon(S(l, g - 3, i), S(l, g - 2, i), S(l, g - 1, i), S(l, g, i), S(l, g + 1, i));
+    const float neg = recon(S(l, g + 2, i), S(l, g + 1, i), S(l, g, i), S(l, g - 1, i), S(l, g - 2, i));
+    return m * (m >= 0.0f ? pos : neg);
+  };
+  auto fz = [&](int k) {
+    const float m = mw_at(w, k, j, i);
+    Stencil ps, ng;
+    const bool strict = zflux_branch(k, nk, weno, ps, ng);
+    auto q = [&](int ll) { return S(ll, j, i); };
+    const float c1 = w.col.c1[k - 1], c2 = w.col.c2[k - 1];
+    const bool up = strict ? (m > 0.0f) : (m >= 0.0f);
+    const float r = up ? eval_branch(ps, q, c2, c1, weps, c.weps_f) : eval_branch(ng, q, c2, c1, weps, c.weps_f);
+    return m * r;
+  };
+  const float ax = (fx(i) - fx(i + 1)) * c.rdx;
+  const float ay = (fy(j) - fy(j + 1)) * c.rdy;
+  float az;
+  if (l == 1) az = (-fz(2)) * c.rdz;
+  else if (l == nk) az = fz(nk) * c.rdz;
+  else az = (fz(l) - fz(l + 1)) * c.rdz;
+  const int64_t q = w.at(l, j, i);
+  float total = (ax + ay) + az;
+  total = total + S(l, j, i) * w.p[DIVX2][q];
+  const float adv = (total * w.col.rr0[l - 1]) * 1.0f;
+  float out = w.p[ATH][q] + c.dttmp * ((0.0f - w.p[TDIV][q]) + adv);
+  w.p[THO][q] = fabsf(out) < 1.0e-30f ? 0.0f : out;
+}
+
+__global__ __launch_bounds__(64) void k_solve3(W w) {
+  CELL_INDEX
+  const int64_t q = w.at(l, j, i);
+  w.p[PPXO][q] = w.snap(SPP, w.nloop)[q] + w.p[PPXO][q];
+}
+
+W make_w(void* const* table, const float* consts, int ni, int nj, int nk, int nrk, int nloop, double weps) {
+  W w;
+  for (int s = 0; s < NSLOT; ++s) w.p[s] = (float*)table[s];
+  w.ni = ni;
+  w.nj = nj;
+  w.nk = nk;
+  w.nrk = nrk;
+  w.nloop = nloop;
+  w.weps = weps;
+  const float* cs = consts;
+  float* dst = (float*)&w.c;
+  for (int n = 0; n < NCONST; ++n) dst[n] = cs[n];
+  w.col = make_cols(w.p[COL], nk);
+  return w;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gm1hip_dry_stage_forward(void* const* table, const float* consts, int ni, int nj, int nk, int nrk, int nloop,
+                             double weps, void* stream) {
+  if (!table || !consts || ni < 6 || nj < 6 || nk < 6 || nrk < 1 || nrk > 3 || nloop < 1)
+    return (int)hipErrorInvalidValue;
+  hipStream_t s = (hipStream_t)stream;
+  const W w = make_w(table, consts, ni, nj, nk, nrk, nloop, weps);
+  const dim3 b(64), g((ni + 63) / 64, nj, nk), gw((ni + 63) / 64, nj, nk + 1), gc((ni + 63) / 64, nj);
+  hipLaunchKernelGGL(k_terms, g, b, 0, s, w);
+  hipLaunchKernelGGL(k_mass, g, b, 0, s, w);
+  hipLaunchKernelGGL(k_divx, g, b, 0, s, w);
+  hipLaunchKernelGGL(k_advu, g, b, 0, s, w);
+  hipLaunchKernelGGL(k_advv, g, b, 0, s, w);
+  hipLaunchKernelGGL(k_advw, gw, b, 0, s, w);
+  hipLaunchKernelGGL(k_coef, g, b, 0, s, w);
+  hipLaunchKernelGGL(k_ac_start, g, b, 0, s, w);
+  GM1_LAUNCH_CHECK();
+  for (int n = 1; n <= nloop; ++n) {
+    hipLaunchKernelGGL(k_ss_pgrad, g, b, 0, s, w, n);
+    hipLaunchKernelGGL(k_ss_column, gc, b, 0, s, w, n);
+  }
+  hipLaunchKernelGGL(k_post, g, b, 0, s, w);
+  hipLaunchKernelGGL(k_theta, g, b, 0, s, w);
+  if (nrk == 3) hipLaunchKernelGGL(k_solve3, g, b, 0, s, w);
+  GM1_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // extern "C"

@@ -139,6 +139,17 @@ int gm1hip_sound3_periodic_stage(void* const* arrays, int ni, int nj, int nk, in
 int gm1hip_psolver3_stage_adjoint(void* const* arrays, int ni, int nj, int nk, int nrk, int nloop, float dts,
                                   float dx, float dy, float alph, float kdiv, void* stream);
 
+// One RK stage of the dry large step on halo-free periodic fields (flat
+// terrain, uniform base column, CM1's (1,1,1,1) sides): no counterpart in the
+// reference's interface (it restates cuda_gm1_forecast.py:1241-1345 for the
+// adjoint's trajectory; the reference has no adjoint).  `table` holds the
+// device pointers of the workspace in the order of gm1_amd/adjoint_step.py
+// SLOTS, `consts` the stage's 22 binary32 scalars (stage_consts).  Fields are
+// (nk, nj, ni), w-type fields (nk+1, nj, ni), substep states (nloop+1)
+// planes.  ni, nj, nk >= 6.
+int gm1hip_dry_stage_forward(void* const* table, const float* consts, int ni, int nj, int nk, int nrk, int nloop,
+                             double weps, void* stream);
+
 // ----------------------------------------------------------------- boundaries
 // Family halo fill of one padded field (cuda_boundary_families.py:887-940,
 // cpu_family_we_inplace / cpu_family_sn_inplace); kind 0 scalar, 1 u, 2 v, 3 w.

@@ -1,0 +1,94 @@
+"""Halo-free dry periodic step on the device (gm1_amd.adjoint_step) and its adjoint.
+
+CPU: the workspace slot list matches the kernels' pointer table.
+GPU: the device forward step equals the binary32 restatement
+(oracle.large_step) word for word on a C5-like state; the device adjoint of
+one large step against the binary64 adjoint (oracle/adjoint_large_step.py)
+at the same trajectory.
+"""
+
+from __future__ import annotations
+
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from gm1_amd.adjoint_step import SLOTS
+from gm1_amd.large_step import host_initial_state
+from test_adjoint_sound import c5_like
+
+ROOT = Path(__file__).resolve().parents[1]
+FIELDS = ("u", "v", "w", "th", "pp", "ppx")
+
+
+def test_slot_table_matches_the_kernels():
+    text = (ROOT / "gm1_amd" / "csrc" / "adjoint_step.hip").read_text()
+    body = re.search(r"enum Slot \{(.*?)NSLOT", text, flags=re.S).group(1)
+    body = re.sub(r"//[^\n]*", "", body)
+    names = [n.strip() for n in body.split(",") if n.strip()]
+    assert tuple(names) == SLOTS
+
+
+def state32(seed=3, ni=14, nj=12, nk=10):
+    """A binary32 C5-like padded state with winds, theta' and pressure everywhere (periodic-consistent)."""
+    from oracle import boundary as bc
+    from oracle import sound as osd
+
+    cfg = c5_like(ni, nj, nk, seed=seed)
+    host = host_initial_state(cfg)
+    a = {k: v.copy() for k, v in host["arrays"].items()}
+    rng = np.random.default_rng(seed)
+    ng = 3
+    f32 = np.float32
+    a["th3d"] = (a["th3d"] + 0.3 * rng.standard_normal(a["th3d"].shape)).astype(f32)
+    for name in ("pp3d", "ppx"):
+        a[name][1 : nk + 1, 1 : nj + 1, 1 : ni + 1] += (1e-4 * rng.standard_normal((nk, nj, ni))).astype(f32)
+    a["w3d"][2 : nk + 1, ng : ng + nj, ng : ng + ni] += (0.5 * rng.standard_normal((nk - 1, nj, ni))).astype(f32)
+    for name in ("u3d", "v3d"):
+        a[name][1 : nk + 1, ng : ng + nj, ng : ng + ni] += (3.0 * rng.standard_normal((nk, nj, ni))).astype(f32)
+    a["u3d"][:, :, ng + ni] = a["u3d"][:, :, ng]
+    a["v3d"][:, ng + nj, :] = a["v3d"][:, ng, :]
+    sides = (1, 1, 1, 1)
+    for name, kind in (("u3d", bc.KIND_U), ("v3d", bc.KIND_V), ("w3d", bc.KIND_W)):
+        bc.fill(a[name], kind, sides, ni=ni, nj=nj, nk=nk)
+    for nm, src in (("ua", "u3d"), ("va", "v3d"), ("wa", "w3d"), ("tha", "th3d")):
+        a[nm] = a[src].copy()
+    for name in ("pp3d", "ppx"):
+        bc.bcp(a[name], sides, ni=ni, nj=nj, nk=nk)
+    a["ppi"] = a["pp3d"].copy()
+    prof = osd.terrain_coordinate_profiles(nk, cfg.dz)
+    c = {"ni": ni, "nj": nj, "nk": nk, "dx": cfg.dx, "dy": cfg.dy, "dz": cfg.dz, "dt": cfg.dt,
+         "nsound": cfg.nsound, "lateral_sides": sides}
+    return cfg, host, a, prof, c
+
+
+def unpad_np(a, ni, nj, nk):
+    from gm1_amd.adjoint_step import from_padded
+
+    return {n: np.ascontiguousarray(v) for n, v in from_padded(a, ni, nj, nk).items()}
+
+
+@pytest.mark.gpu
+def test_gpu_halo_free_step_equals_the_restatement(device):
+    import torch
+
+    from gm1_amd.adjoint_step import DryPeriodicStep
+    from oracle import large_step as ol
+
+    cfg, host, a, prof, c = state32()
+    ni, nj, nk = cfg.ni, cfg.nj, cfg.nk
+    S = unpad_np(a, ni, nj, nk)
+    a1 = {k: v.copy() for k, v in a.items()}
+    ol.large_step(a1, host["metrics"], host["base"], prof, c, zt=np.float32(nk * cfg.dz))
+    ref = unpad_np(a1, ni, nj, nk)
+    step = DryPeriodicStep(cfg, host["metrics"], host["base"], device)
+    out = step.forward({n: torch.from_numpy(v).to(device) for n, v in S.items()})
+    torch.cuda.synchronize()
+    for n in FIELDS:
+        got = out[n].cpu().numpy()
+        diff = np.abs(got.astype(np.float64) - ref[n]).max() / max(np.abs(ref[n]).max(), 1e-30)
+        print(n, f"{diff:.2e}", int(np.sum(got != ref[n])))
+    for n in FIELDS:
+        assert np.array_equal(out[n].cpu().numpy(), ref[n]), n
