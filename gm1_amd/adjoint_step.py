@@ -33,6 +33,7 @@ from ._native import I, P, signature
 from .grid import sound3_stage_schedule_f32, terrain_coordinate_profiles
 
 signature("gm1hip_dry_stage_forward", P, P, I, I, I, I, I, Dbl, P)
+signature("gm1hip_dry_stage_adjoint", P, P, I, I, I, I, I, Dbl, P)
 
 F32 = np.float32
 STATE = ("u", "v", "w", "th", "pp", "ppx")
@@ -170,6 +171,51 @@ class DryPeriodicStep:
         for nrk in (1, 2, 3):
             X, ppx = self.forward_stage(nrk, X, A, ppx)
         return {**X, "ppx": ppx}
+
+    # ------------------------------------------------------------ adjoint
+    def adjoint_stage(self, nrk: int, X: dict, A: dict, ppx: torch.Tensor, Xb: dict, ppxb: torch.Tensor) -> tuple[
+            dict, torch.Tensor]:
+        """Adjoints of one stage's inputs (X, ppx) from those of its outputs; the adjoint of A accumulates
+        in the workspace (``zero_start_adjoint`` / ``start_adjoint``)."""
+        self.forward_stage(nrk, X, A, ppx)
+        self._load([("B_OU", Xb["u"]), ("B_OV", Xb["v"]), ("B_OW", Xb["w"]), ("B_OTH", Xb["th"]),
+                    ("B_OPP", Xb["pp"]), ("B_PPX", ppxb)])
+        c = self.consts[nrk]
+        _native.call("gm1hip_dry_stage_adjoint", ctypes.cast(self.table, ctypes.c_void_p),
+                     c.ctypes.data_as(ctypes.c_void_p), self.ni, self.nj, self.nk, nrk, self.nloops[nrk],
+                     THETA_WEPS, self._h())
+        t = self.t
+        with self._on_stream():
+            out = {"u": t["B_XU"].clone(), "v": t["B_XV"].clone(), "w": t["B_XW"].clone(), "th": t["B_XTH"].clone(),
+                   "pp": t["B_XPP"].clone()}
+            return out, t["B_PPX"].clone()
+
+    def zero_start_adjoint(self) -> None:
+        with self._on_stream():
+            for s in ("B_AU", "B_AV", "B_AW", "B_ATH", "B_API"):
+                self.t[s].zero_()
+
+    def start_adjoint(self) -> dict:
+        t = self.t
+        with self._on_stream():
+            return {"u": t["B_AU"].clone(), "v": t["B_AV"].clone(), "w": t["B_AW"].clone(), "th": t["B_ATH"].clone(),
+                    "pp": t["B_API"].clone()}
+
+    def adjoint(self, S: dict, Sb: dict) -> dict:
+        """Adjoint of the state before one large step (S) from the adjoint Sb of the state after it."""
+        A = {n: S[n] for n in ("u", "v", "w", "th", "pp")}
+        X2, ppx2 = self.forward_stage(1, A, A, S["ppx"])
+        X3, ppx3 = self.forward_stage(2, X2, A, ppx2)
+        self.zero_start_adjoint()
+        Xb = {n: Sb[n] for n in ("u", "v", "w", "th", "pp")}
+        ppxb = Sb["ppx"]
+        for nrk, X, ppx in ((3, X3, ppx3), (2, X2, ppx2), (1, A, S["ppx"])):
+            Xb, ppxb = self.adjoint_stage(nrk, X, A, ppx, Xb, ppxb)
+        Ab = self.start_adjoint()
+        with self._on_stream():
+            out = {n: Xb[n] + Ab[n] for n in Xb}
+        out["ppx"] = ppxb
+        return out
 
     def empty_state(self) -> dict:
         with self._on_stream():

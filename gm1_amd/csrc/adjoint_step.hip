@@ -680,6 +680,628 @@ __global__ __launch_bounds__(64) void k_solve3(W w) {
   w.p[PPXO][q] = w.snap(SPP, w.nloop)[q] + w.p[PPXO][q];
 }
 
+// ------------------------------------------------------------------ adjoint
+
+__device__ __forceinline__ void aadd(float* p, float v) { atomicAdd(p, v); }
+
+__global__ __launch_bounds__(256) void k_b_fill(float* p, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0.0f;
+}
+
+// theta compose and the cell parts of the scalar tendency
+__global__ __launch_bounds__(64) void k_b_theta_cell(W w) {
+  CELL_INDEX
+  const Consts& c = w.c;
+  const int64_t q = w.at(l, j, i);
+  const float thb = w.p[B_OTH][q];
+  w.p[B_ATH][q] += thb;
+  const float tdb = -c.dttmp * thb;
+  const float advb = c.dttmp * thb;
+  const float totb = advb * w.col.rr0[l - 1];
+  w.p[B_TOT][q] = totb;
+  w.p[B_DIV][q] = tdb * w.p[THTERM][q];
+  w.p[B_THTERM][q] = tdb * w.p[DIV][q];
+  w.p[B_SC][q] = totb * w.p[DIVX2][q];
+  w.p[B_DIVX2][q] = totb * sc_at(w, l, j, i);
+}
+
+// scalar x (dir 0) and y (dir 1) face fluxes: face index i (dir 0) or j (dir 1)
+__global__ __launch_bounds__(64) void k_b_theta_xy(W w, int dir) {
+  CELL_INDEX
+  const Consts& c = w.c;
+  const bool weno = w.nrk == 3;
+  const float* tot = w.p[B_TOT];
+  float fb, m;
+  if (dir == 0) {
+    fb = tot[w.at(l, j, i)] * c.rdx - tot[w.at(l, j, w.X(i, -1))] * c.rdx;
+    m = mu_at(w, l, j, i);
+  } else {
+    fb = tot[w.at(l, j, i)] * c.rdy - tot[w.at(l, w.Y(j, -1), i)] * c.rdy;
+    m = mv_at(w, l, j, i);
+  }
+  auto S = [&](int o) { return dir == 0 ? sc_at(w, l, j, i + o) : sc_at(w, l, j + o, i); };
+  const bool pos = m >= 0.0f;
+  const int o0 = pos ? -3 : 2, step = pos ? 1 : -1;
+  float s[5], cf[5];
+  for (int n = 0; n < 5; ++n) s[n] = S(o0 + step * n);
+  float r;
+  if (weno) {
+    r = weno5(s[0], s[1], s[2], s[3], s[4], w.weps);
+    weno_coefs(s[0], s[1], s[2], s[3], s[4], w.weps, cf);
+  } else {
+    r = flx5(s[0], s[1], s[2], s[3], s[4]);
+    cf[0] = gm1::C2_60; cf[1] = gm1::CM13_60; cf[2] = gm1::C47_60; cf[3] = gm1::C27_60; cf[4] = gm1::CM3_60;
+  }
+  w.p[dir == 0 ? B_MU : B_MV][w.at(l, j, i)] = fb * r;
+  const float rb = fb * m;
+  for (int n = 0; n < 5; ++n) {
+    const int o = o0 + step * n;
+    const int64_t t = dir == 0 ? w.at(l, j, w.X(i, o)) : w.at(l, w.Y(j, o), i);
+    aadd(&w.p[B_SC][t], cf[n] * rb);
+  }
+}
+
+// scalar z face fluxes at face levels k = 2..nk (grid level l = k)
+__global__ __launch_bounds__(64) void k_b_theta_z(W w) {
+  CELL_INDEX
+  const Consts& c = w.c;
+  const int nk = w.nk, k = l;
+  if (k < 2) {
+    w.p[B_MW][w.at(1, j, i)] = 0.0f;
+    w.p[B_MW][w.at(nk + 1, j, i)] = 0.0f;
+    return;
+  }
+  const bool weno = w.nrk == 3;
+  const float* tot = w.p[B_TOT];
+  const float fb = tot[w.at(k, j, i)] * c.rdz - tot[w.at(k - 1, j, i)] * c.rdz;
+  const float m = mw_at(w, k, j, i);
+  Stencil ps, ng;
+  const bool strict = zflux_branch(k, nk, weno, ps, ng);
+  const bool up = strict ? (m > 0.0f) : (m >= 0.0f);
+  const Stencil& st = up ? ps : ng;
+  auto q = [&](int ll) { return sc_at(w, ll, j, i); };
+  const float c1 = w.col.c1[k - 1], c2 = w.col.c2[k - 1];
+  const float r = eval_branch(st, q, c2, c1, w.weps, c.weps_f);
+  w.p[B_MW][w.at(k, j, i)] = fb * r;
+  float cf[5];
+  const int nc = stencil_coefs(st, q, c2, c1, w.weps, c.weps_f, cf);
+  const float rb = fb * m;
+  for (int n = 0; n < nc; ++n) aadd(&w.p[B_SC][w.at(st.lev[n], j, i)], cf[n] * rb);
+}
+
+// getdivx and getdiv transposed: adjoints of the stage-averaged winds (w levels on grid level l = 1..nk+1)
+__global__ __launch_bounds__(64) void k_b_post(W w) {
+  CELL_INDEX
+  const Consts& c = w.c;
+  const int nk = w.nk;
+  const float *dx2 = w.p[B_DIVX2], *db = w.p[B_DIV];
+  if (l <= nk) {
+    const float rho = w.col.rho[l - 1];
+    const float den = 0.5f * (rho + rho);
+    const int64_t q = w.at(l, j, i), qw = w.at(l, j, w.X(i, -1)), qs = w.at(l, w.Y(j, -1), i);
+    w.p[B_ARU][q] = den * (w.p[B_MU][q] + c.rdx * (dx2[qw] - dx2[q])) + c.rdx * (db[qw] - db[q]);
+    w.p[B_ARV][q] = den * (w.p[B_MV][q] + c.rdy * (dx2[qs] - dx2[q])) + c.rdy * (db[qs] - db[q]);
+  }
+  const int64_t qw = w.at(l, j, i);
+  if (l >= 2 && l <= nk) {
+    const int64_t lo = w.at(l - 1, j, i);
+    w.p[B_ARW][qw] = w.col.rf0[l - 1] * (w.p[B_MW][qw] + c.rdz * (dx2[lo] - dx2[qw])) + c.rdz * (db[lo] - db[qw]);
+  } else {
+    w.p[B_ARW][qw] = 0.0f;
+  }
+}
+
+// acoustic adjoint state from the stage-output adjoints (grid level l = 1..nk+1)
+__global__ __launch_bounds__(64) void k_b_ac_init(W w) {
+  CELL_INDEX
+  const int64_t q = w.at(l, j, i);
+  w.p[B_W][q] = w.p[B_OW][q];
+  if (l > w.nk) return;
+  w.p[B_U][q] = w.p[B_OU][q];
+  w.p[B_V][q] = w.p[B_OV][q];
+  w.p[B_PP][q] = w.p[B_OPP][q] + (w.nrk == 3 ? w.p[B_PPX][q] : 0.0f);
+  w.p[B_PPD][q] = 0.0f;
+}
+
+__global__ __launch_bounds__(64) void k_b_ss_column(W w, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= w.ni) return;
+  const Consts& c = w.c;
+  const int nk = w.nk;
+  const bool last = n == w.nloop;
+  const bool last3 = last && w.nrk == 3;
+  const float* ws = w.wsnap(n - 1);
+  const float* wn = w.wsnap(n);
+  const float* pp = w.snap(SPP, n - 1);
+  const float *qk = w.p[QK], *pk1 = w.p[PK1], *pk2 = w.p[PK2], *mm = w.p[MM];
+  float *fpk = w.p[FPK], *fpkb = w.p[B_FPKF], *bw = w.p[B_W], *bpp = w.p[B_PP];
+  float *bqk = w.p[B_QK], *bpk1 = w.p[B_PK1], *bpk2 = w.p[B_PK2], *bmm = w.p[B_MM];
+  float *bec = w.p[B_EC], *bed = w.p[B_ED];
+  float* divs = w.p[DIV];  // forward div of this substep, recomputed (DIV is free after the post pass)
+  for (int l = 1; l <= nk; ++l) {
+    float dv;
+    fpk[w.at(l, j, i)] = column_fpk(w, n, l, j, i, &dv);
+    divs[w.at(l, j, i)] = dv;
+  }
+  // averages of w
+  for (int l = 1; l <= nk; ++l) {
+    const int64_t q = w.at(l, j, i);
+    if (last) {
+      bw[q] += (w.p[B_ARW][q] * c.tavg) * c.alph;
+      w.p[B_ARW][q] *= c.tavg;
+    } else {
+      bw[q] += w.p[B_ARW][q];
+    }
+  }
+  // pressure update and filter
+  for (int l = 1; l <= nk; ++l) {
+    const int64_t q = w.at(l, j, i);
+    float upb = bpp[q] + (1.0f + c.kdiv) * w.p[B_PPD][q];
+    float ppb = -c.kdiv * w.p[B_PPD][q];
+    if (last3) {
+      upb = upb + c.kdiv * w.p[B_PPX][q];
+      ppb = ppb - c.kdiv * w.p[B_PPX][q];
+      w.p[B_PPX][q] = 0.0f;
+    }
+    bpp[q] = ppb + upb;
+    fpkb[q] = upb;
+    const float vtb = c.alph * upb;
+    const float wlo = l == 1 ? 0.0f : wn[q];
+    const float whi = l == nk ? 0.0f : wn[w.at(l + 1, j, i)];
+    bpk1[q] += vtb * wlo;
+    bqk[q] += vtb * (wlo - whi);
+    bpk2[q] += vtb * whi;
+    if (l >= 2) bw[q] += vtb * (pk1[q] + qk[q]);
+    if (l <= nk - 1) bw[w.at(l + 1, j, i)] += vtb * (pk2[q] - qk[q]);
+  }
+  // transposed Thomas solve: T^T lam = wbar(2..nk); row k of T^T has cc(k-1) below and aa(k+1) above
+  float cprev = 0.0f, dprev = 0.0f;
+  for (int k = 2; k <= nk; ++k) {
+    const int64_t q = w.at(k, j, i);
+    const float lo = k == 2 ? 0.0f : thomas_row(w, k - 1, j, i).cc;
+    const float up = k < nk ? thomas_row(w, k + 1, j, i).aa : 0.0f;
+    const float piv = thomas_row(w, k, j, i).bb - lo * cprev;
+    const float cpk = up / piv;
+    const float dpk = (bw[q] - lo * dprev) / piv;
+    bec[q] = cpk;
+    bed[q] = dpk;
+    cprev = cpk;
+    dprev = dpk;
+  }
+  // start-w adjoint: level 1 gets nothing, nk+1 passes through, 2..nk from the solve
+  float lamnext = 0.0f;
+  for (int k = nk; k >= 2; --k) {
+    const int64_t q = w.at(k, j, i), qd = w.at(k - 1, j, i);
+    const float lam = k == nk ? bed[q] : bed[q] - bec[q] * lamnext;
+    lamnext = lam;
+    const float mmk = mm[q];
+    const float pdelta = pp[q] - pp[qd];
+    const float udelta = (pp[q] + fpk[q]) - (pp[qd] + fpk[qd]);
+    bw[q] = lam;  // stored after the row's rhs was read in the forward sweep
+    w.p[B_WTEN][q] += c.dts * lam;
+    bmm[q] += (-(c.beta * pdelta + c.alph * udelta)) * lam;
+    const float pdb = c.beta * (-mmk) * lam;
+    const float udb = c.alph * (-mmk) * lam;
+    bpp[q] += pdb + udb;
+    bpp[qd] -= pdb + udb;
+    fpkb[q] += udb;
+    fpkb[qd] -= udb;
+    const float bbb = -lam * wn[q];
+    const float diag = (-(pk1[q] + qk[q])) + (pk2[qd] - qk[qd]);
+    bmm[q] += (-c.asq * diag) * bbb;
+    const float diagb = (-c.asq * mmk) * bbb;
+    bpk1[q] -= diagb;
+    bqk[q] -= diagb;
+    bpk2[qd] += diagb;
+    bqk[qd] -= diagb;
+    if (k >= 3) {
+      const float aab = -lam * wn[qd];
+      bmm[q] += (-c.asq) * (pk1[qd] + qk[qd]) * aab;
+      bpk1[qd] += (-c.asq) * mmk * aab;
+      bqk[qd] += (-c.asq) * mmk * aab;
+    }
+    if (k < nk) {
+      const float ccb = -lam * wn[w.at(k + 1, j, i)];
+      bmm[q] += c.asq * (pk2[q] - qk[q]) * ccb;
+      bpk2[q] += c.asq * mmk * ccb;
+      bqk[q] -= c.asq * mmk * ccb;
+    }
+  }
+  bw[w.at(1, j, i)] = 0.0f;
+  // pressure advection (w and coefficient parts) and fpk's w term
+  for (int l = 1; l <= nk; ++l) {
+    const int64_t q = w.at(l, j, i);
+    const float f = fpkb[q];
+    const float wlo = l == 1 ? 0.0f : ws[q];
+    const float wup = ws[w.at(l + 1, j, i)];
+    const float whi = l == nk ? 0.0f : wup;
+    bpk1[q] += f * (c.beta * wlo);
+    bpk2[q] += f * (c.beta * wup);
+    if (l >= 2) bw[q] += c.beta * (f * pk1[q]);
+    bw[w.at(l + 1, j, i)] += c.beta * (f * pk2[q]);
+    const float qpb = c.beta * f;
+    bqk[q] -= qpb * (whi - wlo);
+    if (l <= nk - 1) bw[w.at(l + 1, j, i)] += -qk[q] * qpb;
+    if (l >= 2) bw[q] += qk[q] * qpb;
+    w.p[B_PPTERM][q] -= c.dts * (divs[q] * f);
+    w.p[B_DIV][q] = -c.dts * (w.p[PPTERM][q] * f);
+    w.p[B_XA][q] = -(f * w.col.rr0[l - 1]);
+  }
+}
+
+// u and v faces of substep n: averages, pressure advection and divergence adjoints, forcing
+__global__ __launch_bounds__(64) void k_b_ss_faces(W w, int n) {
+  CELL_INDEX
+  const Consts& c = w.c;
+  const bool last = n == w.nloop;
+  const float* pia = w.p[PIADV];
+  const float *xa = w.p[B_XA], *db = w.p[B_DIV];
+  const float rho = w.col.rho[l - 1];
+  const int64_t q = w.at(l, j, i);
+  {
+    const int64_t qw = w.at(l, j, w.X(i, -1));
+    float ub = w.p[B_U][q];
+    if (last) {
+      ub += w.p[B_ARU][q] * c.tavg;
+      w.p[B_ARU][q] *= c.tavg;
+    } else {
+      ub += w.p[B_ARU][q];
+    }
+    const float dp = pia[q] - pia[qw];
+    const float xub = c.temx * (xa[qw] * dp) + c.temx * (xa[q] * dp);
+    ub += rho * xub;
+    ub += db[qw] * c.rdx - db[q] * c.rdx;
+    w.p[B_UTEN][q] += c.dts * ub;
+    w.p[B_U][q] = ub;
+  }
+  {
+    const int64_t qs = w.at(l, w.Y(j, -1), i);
+    float vb = w.p[B_V][q];
+    if (last) {
+      vb += w.p[B_ARV][q] * c.tavg;
+      w.p[B_ARV][q] *= c.tavg;
+    } else {
+      vb += w.p[B_ARV][q];
+    }
+    const float dp = pia[q] - pia[qs];
+    const float yvb = c.temy * (xa[qs] * dp) + c.temy * (xa[q] * dp);
+    vb += rho * yvb;
+    vb += db[qs] * c.rdy - db[q] * c.rdy;
+    w.p[B_VTEN][q] += c.dts * vb;
+    w.p[B_V][q] = vb;
+  }
+}
+
+// cells of substep n: ppd (start) and thv adjoints of the pressure gradient, piadv's of the pressure advection
+__global__ __launch_bounds__(64) void k_b_ss_cells(W w, int n) {
+  CELL_INDEX
+  const Consts& c = w.c;
+  const float* thv = w.p[THV];
+  const float* ppd = w.snap(SPPD, n - 1);
+  const float* u1 = w.snap(SU, n);
+  const float* v1 = w.snap(SV, n);
+  const float *bu = w.p[B_U], *bv = w.p[B_V], *xa = w.p[B_XA];
+  const float rho = w.col.rho[l - 1];
+  const int64_t q = w.at(l, j, i);
+  const int64_t qe = w.at(l, j, w.X(i, 1)), qw = w.at(l, j, w.X(i, -1));
+  const int64_t qn = w.at(l, w.Y(j, 1), i), qs = w.at(l, w.Y(j, -1), i);
+  // u faces i (cell i is the right cell) and i+1 (left cell)
+  const float pgx0 = -c.dts * bu[q], pgx1 = -c.dts * bu[qe];
+  const float A0 = c.halfcp * (thv[q] + thv[qw]), A1 = c.halfcp * (thv[qe] + thv[q]);
+  const float B0 = (ppd[q] - ppd[qw]) * c.rdx, B1 = (ppd[qe] - ppd[q]) * c.rdx;
+  const float pgy0 = -c.dts * bv[q], pgy1 = -c.dts * bv[qn];
+  const float Ay0 = c.halfcp * (thv[q] + thv[qs]), Ay1 = c.halfcp * (thv[qn] + thv[q]);
+  const float By0 = (ppd[q] - ppd[qs]) * c.rdy, By1 = (ppd[qn] - ppd[q]) * c.rdy;
+  w.p[B_PPD][q] = (c.rdx * (A0 * pgx0) - c.rdx * (A1 * pgx1)) + (c.rdy * (Ay0 * pgy0) - c.rdy * (Ay1 * pgy1));
+  float tb = c.halfcp * (pgx0 * B0) + c.halfcp * (pgx1 * B1);
+  tb += c.halfcp * (pgy0 * By0) + c.halfcp * (pgy1 * By1);
+  w.p[B_THV][q] += tb;
+  // pressure advection: t2(c) = xa(c)*temx*rho*u1(c+1), t1(c) = xa(c)*temx*rho*u1(c)
+  auto t2x = [&](int64_t cell, int64_t east) { return xa[cell] * (c.temx * (rho * u1[east])); };
+  auto t1x = [&](int64_t cell) { return xa[cell] * (c.temx * (rho * u1[cell])); };
+  auto t2y = [&](int64_t cell, int64_t north) { return xa[cell] * (c.temy * (rho * v1[north])); };
+  auto t1y = [&](int64_t cell) { return xa[cell] * (c.temy * (rho * v1[cell])); };
+  float pb = (t2x(qw, q) - t2x(q, qe)) + (t1x(q) - t1x(qe));
+  pb += (t2y(qs, q) - t2y(q, qn)) + (t1y(q) - t1y(qn));
+  w.p[B_PIADV][q] += pb;
+}
+
+// coefficient adjoints -> thv, ppterm, piadv (one column)
+__global__ __launch_bounds__(64) void k_b_coef(W w) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= w.ni) return;
+  const Consts& c = w.c;
+  const int nk = w.nk;
+  for (int l = 1; l <= nk; ++l) {
+    const int64_t q = w.at(l, j, i);
+    w.p[B_PPTERM][q] += (c.dts * w.col.rdsf[l]) * w.p[B_QK][q];
+    const float tem2 = ((c.dts * 1.0f) * w.col.rdsf[l]) * w.col.rr0[l - 1];
+    if (l < nk) {
+      const float b2b = (tem2 * w.col.rf0[l]) * w.p[B_PK2][q];
+      w.p[B_PIADV][w.at(l + 1, j, i)] += (-w.col.c2[l]) * b2b;
+      w.p[B_PIADV][q] += (1.0f - w.col.c1[l]) * b2b;
+    }
+    if (l > 1) {
+      const int64_t qd = w.at(l - 1, j, i);
+      const float b1b = (tem2 * w.col.rf0[l - 1]) * w.p[B_PK1][q];
+      w.p[B_PIADV][qd] += w.col.c1[l - 1] * b1b;
+      w.p[B_PIADV][q] += (w.col.c2[l - 1] - 1.0f) * b1b;
+      const float thb = ((c.dts * w.col.rds[l]) * c.cp) * w.p[B_MM][q];
+      w.p[B_THV][q] += w.col.c2[l - 1] * thb;
+      w.p[B_THV][qd] += w.col.c1[l - 1] * thb;
+    }
+  }
+}
+
+// stage start transposed (grid level l = 1..nk+1)
+__global__ __launch_bounds__(64) void k_b_ac_end(W w) {
+  CELL_INDEX
+  const int nk = w.nk;
+  const int64_t q = w.at(l, j, i);
+  if (l <= nk) {
+    w.p[B_PPX][q] = w.p[B_PPD][q] + w.p[B_PPX][q];
+    w.p[B_AW][q] += w.c.beta * w.p[B_ARW][q];
+    if (w.nrk != 1) {
+      w.p[B_API][q] += w.p[B_PP][q];
+      w.p[B_AU][q] += w.p[B_U][q];
+      w.p[B_AV][q] += w.p[B_V][q];
+      w.p[B_AW][q] += w.p[B_W][q];
+    } else {
+      w.p[B_XU][q] += w.p[B_U][q];
+      w.p[B_XV][q] += w.p[B_V][q];
+      w.p[B_XW][q] += w.p[B_W][q];
+      w.p[B_XPP][q] += w.p[B_PP][q];
+    }
+  } else {
+    w.p[B_XW][q] += w.p[B_W][q];
+  }
+}
+
+// momentum: local terms (u*divx, w*divx), buoyancy (grid level l = 1..nk+1)
+__global__ __launch_bounds__(64) void k_b_adv_local(W w) {
+  CELL_INDEX
+  const int nk = w.nk;
+  const float *dx = w.p[DIVX], *btu = w.p[B_UTEN], *btv = w.p[B_VTEN], *btw = w.p[B_WTEN];
+  auto totu = [&](int64_t qq, int ll) {
+    const float rho = w.col.rho[ll - 1];
+    return btu[qq] * (1.0f / (0.5f * (rho + rho)));
+  };
+  auto totv = [&](int64_t qq, int ll) {
+    const float rho = w.col.rho[ll - 1];
+    return btv[qq] * (1.0f / (0.5f * (rho + rho)));
+  };
+  auto totw = [&](int kf, int64_t qq) { return btw[qq] * (w.col.rrf0[kf - 1] * 1.0f); };
+  const int64_t q = w.at(l, j, i);
+  if (l <= nk) {
+    const int64_t qw = w.at(l, j, w.X(i, -1)), qe = w.at(l, j, w.X(i, 1));
+    const int64_t qs = w.at(l, w.Y(j, -1), i), qn = w.at(l, w.Y(j, 1), i);
+    const float tu = totu(q, l), tv = totv(q, l);
+    w.p[B_XU][q] += tu * (0.5f * (dx[q] + dx[qw]));
+    w.p[B_XV][q] += tv * (0.5f * (dx[q] + dx[qs]));
+    const float* u = w.p[XU];
+    const float* v = w.p[XV];
+    const float* ww = w.p[XW];
+    float g = 0.5f * (tu * u[q]) + 0.5f * (totu(qe, l) * u[qe]);
+    g += 0.5f * (tv * v[q]) + 0.5f * (totv(qn, l) * v[qn]);
+    if (l >= 2) g += w.col.c2[l - 1] * (totw(l, q) * ww[q]);
+    if (l + 1 <= nk) {
+      const int64_t qu = w.at(l + 1, j, i);
+      g += (1.0f - w.col.c2[l]) * (totw(l + 1, qu) * ww[qu]);
+    }
+    w.p[B_DIVX][q] += g;
+    // buoyancy: dum8 level l from w levels l (c2) and l+1 (c1)
+    float d8 = 0.0f;
+    if (l >= 2) d8 += w.col.c2[l - 1] * btw[q];
+    if (l + 1 <= nk) d8 += w.col.c1[l] * btw[w.at(l + 1, j, i)];
+    w.p[B_DUM8][q] = d8;
+  }
+  if (l >= 2 && l <= nk) {
+    const float cc2 = w.col.c2[l - 1], cc1 = 1.0f - cc2;
+    w.p[B_XW][q] += totw(l, q) * (cc2 * dx[q] + cc1 * dx[w.at(l - 1, j, i)]);
+  }
+}
+
+// scatter of one horizontal flux's adjoint into its mass and stencil
+template <typename Q, typename Add>
+__device__ __forceinline__ void flux5_T(float fb, float m, float pos, float neg, int o_pos, int o_neg, Q idx,
+                                        Add add, float* bq) {
+  const bool up = m >= 0.0f;
+  const float rb = fb * m;
+  const float cf[5] = {gm1::C2_60, gm1::CM13_60, gm1::C47_60, gm1::C27_60, gm1::CM3_60};
+  const int o0 = up ? o_pos : o_neg, st = up ? 1 : -1;
+  for (int n = 0; n < 5; ++n) aadd(&bq[idx(o0 + st * n)], cf[n] * rb);
+  add(fb * (up ? pos : neg));
+}
+
+// u: x fluxes at cell centres (dir 0), y fluxes at row faces (dir 1), z fluxes at w levels (dir 2)
+__global__ __launch_bounds__(64) void k_b_advu(W w, int dir) {
+  CELL_INDEX
+  const Consts& c = w.c;
+  const int nk = w.nk;
+  const float *u = w.p[XU], *ru = w.p[RRU], *rv = w.p[RRV], *rw = w.p[RRW], *bt = w.p[B_UTEN];
+  auto tot = [&](int ll, int jj, int ii) {
+    const float rho = w.col.rho[ll - 1];
+    return bt[w.at(ll, w.Y(jj, 0), w.X(ii, 0))] * (1.0f / (0.5f * (rho + rho)));
+  };
+  auto U = [&](int ll, int jj, int ii) { return u[w.at(ll, w.Y(jj, 0), w.X(ii, 0))]; };
+  if (dir == 0) {  // flux at cell centre i (faces i, i+1)
+    const float fb = c.rdx * tot(l, j, i + 1) - c.rdx * tot(l, j, i);
+    const float m = 0.5f * (ru[w.at(l, j, i)] + ru[w.at(l, j, w.X(i, 1))]);
+    const float pos = flx5(U(l, j, i - 2), U(l, j, i - 1), U(l, j, i), U(l, j, i + 1), U(l, j, i + 2));
+    const float neg = flx5(U(l, j, i + 3), U(l, j, i + 2), U(l, j, i + 1), U(l, j, i), U(l, j, i - 1));
+    flux5_T(fb, m, pos, neg, -2, 3, [&](int o) { return w.at(l, j, w.X(i, o)); },
+            [&](float mb) {
+              aadd(&w.p[B_RRU][w.at(l, j, i)], 0.5f * mb);
+              aadd(&w.p[B_RRU][w.at(l, j, w.X(i, 1))], 0.5f * mb);
+            },
+            w.p[B_XU]);
+  } else if (dir == 1) {  // flux at row face j of face column i
+    const float fb = c.rdy * tot(l, j, i) - c.rdy * tot(l, j - 1, i);
+    const float m = 0.5f * (rv[w.at(l, j, i)] + rv[w.at(l, j, w.X(i, -1))]);
+    const float pos = flx5(U(l, j - 3, i), U(l, j - 2, i), U(l, j - 1, i), U(l, j, i), U(l, j + 1, i));
+    const float neg = flx5(U(l, j + 2, i), U(l, j + 1, i), U(l, j, i), U(l, j - 1, i), U(l, j - 2, i));
+    flux5_T(fb, m, pos, neg, -3, 2, [&](int o) { return w.at(l, w.Y(j, o), i); },
+            [&](float mb) {
+              aadd(&w.p[B_RRV][w.at(l, j, i)], 0.5f * mb);
+              aadd(&w.p[B_RRV][w.at(l, j, w.X(i, -1))], 0.5f * mb);
+            },
+            w.p[B_XU]);
+  } else {  // vertical flux at w level kf = l (2..nk)
+    const int kf = l;
+    if (kf < 2) return;
+    const float fb = c.rdz * tot(kf, j, i) - c.rdz * tot(kf - 1, j, i);
+    const float m = 0.5f * (rw[w.at(kf, j, i)] + rw[w.at(kf, j, w.X(i, -1))]);
+    const float c2h = 0.5f * (w.col.c2[kf - 1] + w.col.c2[kf - 1]);
+    Stencil ps, ng;
+    uvz_branch(kf, nk, ps, ng);
+    const Stencil& st = m >= 0.0f ? ps : ng;
+    auto q = [&](int ll) { return U(ll, j, i); };
+    const float r = eval_branch(st, q, c2h, 1.0f - c2h, 0.0, 0.0f);
+    const float pb = 0.5f * (fb * r);
+    aadd(&w.p[B_RRW][w.at(kf, j, i)], pb);
+    aadd(&w.p[B_RRW][w.at(kf, j, w.X(i, -1))], pb);
+    float cf[5];
+    const int nc = branch_coefs(st, q, c2h, 1.0f - c2h, 0.0f, cf);
+    const float rb = fb * m;
+    for (int n2 = 0; n2 < nc; ++n2) aadd(&w.p[B_XU][w.at(st.lev[n2], j, i)], cf[n2] * rb);
+  }
+}
+
+__global__ __launch_bounds__(64) void k_b_advv(W w, int dir) {
+  CELL_INDEX
+  const Consts& c = w.c;
+  const int nk = w.nk;
+  const float *v = w.p[XV], *ru = w.p[RRU], *rv = w.p[RRV], *rw = w.p[RRW], *bt = w.p[B_VTEN];
+  auto tot = [&](int ll, int jj, int ii) {
+    const float rho = w.col.rho[ll - 1];
+    return bt[w.at(ll, w.Y(jj, 0), w.X(ii, 0))] * (1.0f / (0.5f * (rho + rho)));
+  };
+  auto V = [&](int ll, int jj, int ii) { return v[w.at(ll, w.Y(jj, 0), w.X(ii, 0))]; };
+  if (dir == 0) {  // flux at face column i of v row j
+    const float fb = c.rdx * tot(l, j, i) - c.rdx * tot(l, j, i - 1);
+    const float m = 0.5f * (ru[w.at(l, j, i)] + ru[w.at(l, w.Y(j, -1), i)]);
+    const float pos = flx5(V(l, j, i - 3), V(l, j, i - 2), V(l, j, i - 1), V(l, j, i), V(l, j, i + 1));
+    const float neg = flx5(V(l, j, i + 2), V(l, j, i + 1), V(l, j, i), V(l, j, i - 1), V(l, j, i - 2));
+    flux5_T(fb, m, pos, neg, -3, 2, [&](int o) { return w.at(l, j, w.X(i, o)); },
+            [&](float mb) {
+              aadd(&w.p[B_RRU][w.at(l, j, i)], 0.5f * mb);
+              aadd(&w.p[B_RRU][w.at(l, w.Y(j, -1), i)], 0.5f * mb);
+            },
+            w.p[B_XV]);
+  } else if (dir == 1) {  // flux at cell-centre row j (v faces j, j+1)
+    const float fb = c.rdy * tot(l, j + 1, i) - c.rdy * tot(l, j, i);
+    const float m = 0.5f * (rv[w.at(l, j, i)] + rv[w.at(l, w.Y(j, 1), i)]);
+    const float pos = flx5(V(l, j - 2, i), V(l, j - 1, i), V(l, j, i), V(l, j + 1, i), V(l, j + 2, i));
+    const float neg = flx5(V(l, j + 3, i), V(l, j + 2, i), V(l, j + 1, i), V(l, j, i), V(l, j - 1, i));
+    flux5_T(fb, m, pos, neg, -2, 3, [&](int o) { return w.at(l, w.Y(j, o), i); },
+            [&](float mb) {
+              aadd(&w.p[B_RRV][w.at(l, j, i)], 0.5f * mb);
+              aadd(&w.p[B_RRV][w.at(l, w.Y(j, 1), i)], 0.5f * mb);
+            },
+            w.p[B_XV]);
+  } else {
+    const int kf = l;
+    if (kf < 2) return;
+    const float fb = c.rdz * tot(kf, j, i) - c.rdz * tot(kf - 1, j, i);
+    const float m = 0.5f * (rw[w.at(kf, j, i)] + rw[w.at(kf, w.Y(j, -1), i)]);
+    const float c2h = 0.5f * (w.col.c2[kf - 1] + w.col.c2[kf - 1]);
+    Stencil ps, ng;
+    uvz_branch(kf, nk, ps, ng);
+    const Stencil& st = m >= 0.0f ? ps : ng;
+    auto q = [&](int ll) { return V(ll, j, i); };
+    const float r = eval_branch(st, q, c2h, 1.0f - c2h, 0.0, 0.0f);
+    const float pb = 0.5f * (fb * r);
+    aadd(&w.p[B_RRW][w.at(kf, j, i)], pb);
+    aadd(&w.p[B_RRW][w.at(kf, w.Y(j, -1), i)], pb);
+    float cf[5];
+    const int nc = branch_coefs(st, q, c2h, 1.0f - c2h, 0.0f, cf);
+    const float rb = fb * m;
+    for (int n2 = 0; n2 < nc; ++n2) aadd(&w.p[B_XV][w.at(st.lev[n2], j, i)], cf[n2] * rb);
+  }
+}
+
+// w: x/y fluxes at faces of w level kf = l (dir 0, 1), vertical fluxes at scalar level k = l (dir 2)
+__global__ __launch_bounds__(64) void k_b_advw(W w, int dir) {
+  CELL_INDEX
+  const Consts& c = w.c;
+  const int nk = w.nk;
+  const float *ww = w.p[XW], *ru = w.p[RRU], *rv = w.p[RRV], *rw = w.p[RRW], *bt = w.p[B_WTEN];
+  auto tot = [&](int kf, int jj, int ii) {
+    if (kf < 2 || kf > nk) return 0.0f;
+    return bt[w.at(kf, w.Y(jj, 0), w.X(ii, 0))] * (w.col.rrf0[kf - 1] * 1.0f);
+  };
+  auto Wv = [&](int ll, int jj, int ii) { return ww[w.at(ll, w.Y(jj, 0), w.X(ii, 0))]; };
+  if (dir < 2) {
+    const int kf = l;
+    if (kf < 2 || kf > nk) return;
+    const float c2h = 0.5f * (w.col.c2[kf - 1] + w.col.c2[kf - 1]);
+    const float c1h = 1.0f - c2h;
+    if (dir == 0) {
+      const float fb = c.rdx * tot(kf, j, i) - c.rdx * tot(kf, j, i - 1);
+      const float m = c2h * ru[w.at(kf, j, i)] + c1h * ru[w.at(kf - 1, j, i)];
+      const float pos = flx5(Wv(kf, j, i - 3), Wv(kf, j, i - 2), Wv(kf, j, i - 1), Wv(kf, j, i), Wv(kf, j, i + 1));
+      const float neg = flx5(Wv(kf, j, i + 2), Wv(kf, j, i + 1), Wv(kf, j, i), Wv(kf, j, i - 1), Wv(kf, j, i - 2));
+      flux5_T(fb, m, pos, neg, -3, 2, [&](int o) { return w.at(kf, j, w.X(i, o)); },
+              [&](float mb) {
+                aadd(&w.p[B_RRU][w.at(kf, j, i)], c2h * mb);
+                aadd(&w.p[B_RRU][w.at(kf - 1, j, i)], c1h * mb);
+              },
+              w.p[B_XW]);
+    } else {
+      const float fb = c.rdy * tot(kf, j, i) - c.rdy * tot(kf, j - 1, i);
+      const float m = c2h * rv[w.at(kf, j, i)] + c1h * rv[w.at(kf - 1, j, i)];
+      const float pos = flx5(Wv(kf, j - 3, i), Wv(kf, j - 2, i), Wv(kf, j - 1, i), Wv(kf, j, i), Wv(kf, j + 1, i));
+      const float neg = flx5(Wv(kf, j + 2, i), Wv(kf, j + 1, i), Wv(kf, j, i), Wv(kf, j - 1, i), Wv(kf, j - 2, i));
+      flux5_T(fb, m, pos, neg, -3, 2, [&](int o) { return w.at(kf, w.Y(j, o), i); },
+              [&](float mb) {
+                aadd(&w.p[B_RRV][w.at(kf, j, i)], c2h * mb);
+                aadd(&w.p[B_RRV][w.at(kf - 1, j, i)], c1h * mb);
+              },
+              w.p[B_XW]);
+    }
+    return;
+  }
+  const int k = l;  // scalar level 1..nk
+  const float fb = c.rdz * tot(k + 1, j, i) - c.rdz * tot(k, j, i);
+  const float m = 0.5f * (rw[w.at(k, j, i)] + rw[w.at(k + 1, j, i)]);
+  Stencil ps, ng;
+  wz_branch(k, nk, ps, ng);
+  const Stencil& st = m >= 0.0f ? ps : ng;
+  auto q = [&](int ll) { return Wv(ll, j, i); };
+  const float r = eval_branch(st, q, 0.0f, 0.0f, 0.0, 0.0f);
+  const float mb = 0.5f * (fb * r);
+  aadd(&w.p[B_RRW][w.at(k, j, i)], mb);
+  aadd(&w.p[B_RRW][w.at(k + 1, j, i)], mb);
+  float cf[5];
+  const int nc = branch_coefs(st, q, 0.0f, 0.0f, 0.0f, cf);
+  const float rb = fb * m;
+  for (int n2 = 0; n2 < nc; ++n2) aadd(&w.p[B_XW][w.at(st.lev[n2], j, i)], cf[n2] * rb);
+}
+
+// mass fluxes and divx transposed, then the stage terms (grid level l = 1..nk+1)
+__global__ __launch_bounds__(64) void k_b_mass_terms(W w) {
+  CELL_INDEX
+  const Consts& c = w.c;
+  const int nk = w.nk;
+  const float* dxb = w.p[B_DIVX];
+  const int64_t q = w.at(l, j, i);
+  if (l <= nk) {
+    const float rho = w.col.rho[l - 1];
+    const float den = 0.5f * (rho + rho);
+    const int64_t qw = w.at(l, j, w.X(i, -1)), qs = w.at(l, w.Y(j, -1), i);
+    w.p[B_XU][q] += den * (w.p[B_RRU][q] + (dxb[qw] * c.rdx - dxb[q] * c.rdx));
+    w.p[B_XV][q] += den * (w.p[B_RRV][q] + (dxb[qs] * c.rdy - dxb[q] * c.rdy));
+    w.p[B_XTH][q] += ((w.p[B_THV][q] * 1.0f + w.p[B_DUM8][q] * (c.g * w.col.rth[l - 1])) +
+                      w.p[B_THTERM][q] * (c.kterm * c.cvm)) + w.p[B_SC][q];
+    w.p[B_XPP][q] += w.p[B_PPTERM][q] * ((c.rovcp * c.cpm) * c.cvm) + w.p[B_PIADV][q];
+  }
+  if (l >= 2 && l <= nk) {
+    const int64_t qd = w.at(l - 1, j, i);
+    w.p[B_XW][q] += w.col.rf0[l - 1] * (w.p[B_RRW][q] + (dxb[qd] * c.rdz - dxb[q] * c.rdz));
+  }
+}
+
 W make_w(void* const* table, const float* consts, int ni, int nj, int nk, int nrk, int nloop, double weps) {
   W w;
   for (int s = 0; s < NSLOT; ++s) w.p[s] = (float*)table[s];
@@ -723,6 +1345,48 @@ int gm1hip_dry_stage_forward(void* const* table, const float* consts, int ni, in
   hipLaunchKernelGGL(k_post, g, b, 0, s, w);
   hipLaunchKernelGGL(k_theta, g, b, 0, s, w);
   if (nrk == 3) hipLaunchKernelGGL(k_solve3, g, b, 0, s, w);
+  GM1_LAUNCH_CHECK();
+  return 0;
+}
+
+int gm1hip_dry_stage_adjoint(void* const* table, const float* consts, int ni, int nj, int nk, int nrk, int nloop,
+                             double weps, void* stream) {
+  if (!table || !consts || ni < 6 || nj < 6 || nk < 6 || nrk < 1 || nrk > 3 || nloop < 1)
+    return (int)hipErrorInvalidValue;
+  hipStream_t s = (hipStream_t)stream;
+  const W w = make_w(table, consts, ni, nj, nk, nrk, nloop, weps);
+  const int64_t cells = (int64_t)ni * nj * nk, wcells = (int64_t)ni * nj * (nk + 1);
+  const int zero_cell[] = {B_XU, B_XV, B_XTH, B_XPP, B_THV, B_PPTERM, B_PIADV, B_RRU, B_RRV, B_DIVX, B_UTEN,
+                           B_VTEN, B_QK, B_PK1, B_PK2, B_MM};
+  const int zero_w[] = {B_XW, B_RRW, B_WTEN};
+  for (int z : zero_cell) hipLaunchKernelGGL(k_b_fill, dim3(gm1_blocks(cells, 256)), dim3(256), 0, s, w.p[z], cells);
+  for (int z : zero_w) hipLaunchKernelGGL(k_b_fill, dim3(gm1_blocks(wcells, 256)), dim3(256), 0, s, w.p[z], wcells);
+  GM1_LAUNCH_CHECK();
+  const dim3 b(64), g((ni + 63) / 64, nj, nk), gw((ni + 63) / 64, nj, nk + 1), gc((ni + 63) / 64, nj);
+  // theta and the post-acoustic divergences
+  hipLaunchKernelGGL(k_b_theta_cell, g, b, 0, s, w);
+  hipLaunchKernelGGL(k_b_theta_xy, g, b, 0, s, w, 0);
+  hipLaunchKernelGGL(k_b_theta_xy, g, b, 0, s, w, 1);
+  hipLaunchKernelGGL(k_b_theta_z, g, b, 0, s, w);
+  hipLaunchKernelGGL(k_b_post, gw, b, 0, s, w);
+  // acoustic stage
+  hipLaunchKernelGGL(k_b_ac_init, gw, b, 0, s, w);
+  GM1_LAUNCH_CHECK();
+  for (int n = nloop; n >= 1; --n) {
+    hipLaunchKernelGGL(k_b_ss_column, gc, b, 0, s, w, n);
+    hipLaunchKernelGGL(k_b_ss_faces, g, b, 0, s, w, n);
+    hipLaunchKernelGGL(k_b_ss_cells, g, b, 0, s, w, n);
+  }
+  hipLaunchKernelGGL(k_b_coef, gc, b, 0, s, w);
+  hipLaunchKernelGGL(k_b_ac_end, gw, b, 0, s, w);
+  // momentum advection, buoyancy, mass fluxes, stage terms
+  hipLaunchKernelGGL(k_b_adv_local, gw, b, 0, s, w);
+  for (int d = 0; d < 3; ++d) {
+    hipLaunchKernelGGL(k_b_advu, g, b, 0, s, w, d);
+    hipLaunchKernelGGL(k_b_advv, g, b, 0, s, w, d);
+    hipLaunchKernelGGL(k_b_advw, d < 2 ? gw : g, b, 0, s, w, d);
+  }
+  hipLaunchKernelGGL(k_b_mass_terms, gw, b, 0, s, w);
   GM1_LAUNCH_CHECK();
   return 0;
 }

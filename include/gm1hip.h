@@ -150,6 +150,18 @@ int gm1hip_psolver3_stage_adjoint(void* const* arrays, int ni, int nj, int nk, i
 int gm1hip_dry_stage_forward(void* const* table, const float* consts, int ni, int nj, int nk, int nrk, int nloop,
                              double weps, void* stream);
 
+// Adjoint of one gm1hip_dry_stage_forward call (no counterpart in the
+// reference, which has no adjoint).  Run it right after the forward call of
+// the same stage on the same workspace (it reads that call's intermediates
+// and substep states).  In: B_OU B_OV B_OW B_OTH B_OPP (adjoints of the
+// stage's u v w theta' pp'), B_PPX (adjoint of the stage's ppx; at nrk 3 of
+// solve3's pp + ppx).  Out: B_XU B_XV B_XW B_XTH B_XPP (adjoints of the
+// stage's input state), B_PPX (adjoint of the ppx it read); B_AU B_AV B_AW
+// B_ATH B_API (adjoints of the step-start state) are accumulated.  Upwind
+// choices, WENO5 weights and limiters are frozen at their forward values.
+int gm1hip_dry_stage_adjoint(void* const* table, const float* consts, int ni, int nj, int nk, int nrk, int nloop,
+                             double weps, void* stream);
+
 // ----------------------------------------------------------------- boundaries
 // Family halo fill of one padded field (cuda_boundary_families.py:887-940,
 // cpu_family_we_inplace / cpu_family_sn_inplace); kind 0 scalar, 1 u, 2 v, 3 w.

@@ -92,3 +92,89 @@ def test_gpu_halo_free_step_equals_the_restatement(device):
         print(n, f"{diff:.2e}", int(np.sum(got != ref[n])))
     for n in FIELDS:
         assert np.array_equal(out[n].cpu().numpy(), ref[n]), n
+
+
+def _oracle64_trajectory(a, host, cfg, prof, c):
+    from test_binary64 import oracle_binary64, promote
+
+    o64 = oracle_binary64()
+    als = o64.adjoint_large_step
+    p = als.Params(promote(host["metrics"]), promote(host["base"]), prof, c)
+    return o64, als, p
+
+
+def _rel(g, r):
+    return float(np.abs(g.astype(np.float64) - r).max() / max(np.abs(r).max(), 1e-300))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nrk", [1, 2, 3])
+def test_gpu_stage_adjoint_matches_the_binary64_stage_adjoint(device, nrk):
+    """One RK stage's device adjoint against oracle rk_stage_T at the same (binary32) stage input."""
+    import torch
+
+    from gm1_amd.adjoint_step import DryPeriodicStep
+
+    cfg, host, a, prof, c = state32()
+    ni, nj, nk = cfg.ni, cfg.nj, cfg.nk
+    o64, als, p = _oracle64_trajectory(a, host, cfg, prof, c)
+    S = {n: v.astype(np.float64) for n, v in unpad_np(a, ni, nj, nk).items()}
+    _, T = als.Step(p).forward(S)
+    # the stage input of this stage, rounded to binary32 and used by both sides
+    X = {n: T.traj[f"s{nrk}"]["X"][n].astype(np.float32) for n in ("u", "v", "w", "th", "pp")}
+    ppx = T.traj[f"s{nrk}"]["ppx"].astype(np.float32)
+    A = {n: S[n].astype(np.float32) for n in ("u", "v", "w", "th", "pp")}
+    T2 = als.Tape()
+    X64 = {n: v.astype(np.float64) for n, v in X.items()}
+    A64 = {n: v.astype(np.float64) for n, v in A.items()}
+    T2.traj["s1"] = {"X": A64}
+    Xn, ppxn = als.rk_stage(p, T2, nrk, X64, A64, ppx.astype(np.float64))
+    rng = np.random.default_rng(20 + nrk)
+    Xb = {n: rng.standard_normal(Xn[n].shape) for n in Xn}
+    ppxb = rng.standard_normal(ppx.shape)
+    # the oracle's stage 3 leaves tend in ppx; solve3 (ppx = pp + tend) is folded into the device stage
+    Ab = {n: np.zeros_like(A64[n]) for n in A64}
+    Xb_o = dict(Xb)
+    if nrk == 3:
+        Xb_o["pp"] = Xb["pp"] + ppxb
+    ref_X, ref_ppx = als.rk_stage_T(p, T2, nrk, A64, Xb_o, ppxb, Ab)
+    step = DryPeriodicStep(cfg, host["metrics"], host["base"], device)
+    dev = lambda x: torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).to(device)  # noqa: E731
+    step.zero_start_adjoint()
+    got_X, got_ppx = step.adjoint_stage(nrk, {n: dev(v) for n, v in X.items()}, {n: dev(v) for n, v in A.items()},
+                                        dev(ppx), {n: dev(v) for n, v in Xb.items()}, dev(ppxb))
+    got_A = step.start_adjoint()
+    torch.cuda.synchronize()
+    errs = {f"X.{n}": _rel(got_X[n].cpu().numpy(), ref_X[n]) for n in ref_X}
+    errs["ppx"] = _rel(got_ppx.cpu().numpy(), ref_ppx)
+    for n in Ab:
+        if np.abs(Ab[n]).max() > 0:
+            errs[f"A.{n}"] = _rel(got_A[n].cpu().numpy(), Ab[n])
+    print(f"nrk={nrk}", {k: f"{v:.1e}" for k, v in errs.items()})
+    for k, v in errs.items():
+        assert v <= 1e-4, (k, v)
+
+
+@pytest.mark.gpu
+def test_gpu_step_adjoint_matches_the_binary64_adjoint(device):
+    import torch
+
+    from gm1_amd.adjoint_step import DryPeriodicStep
+
+    cfg, host, a, prof, c = state32()
+    ni, nj, nk = cfg.ni, cfg.nj, cfg.nk
+    o64, als, p = _oracle64_trajectory(a, host, cfg, prof, c)
+    S32 = unpad_np(a, ni, nj, nk)
+    S = {n: v.astype(np.float64) for n, v in S32.items()}
+    out, T = als.Step(p).forward(S)
+    rng = np.random.default_rng(9)
+    Sb = {n: rng.standard_normal(out[n].shape) for n in FIELDS}
+    ref = als.Step(p).adjoint(T, Sb)
+    step = DryPeriodicStep(cfg, host["metrics"], host["base"], device)
+    dev = lambda x: torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).to(device)  # noqa: E731
+    got = step.adjoint({n: dev(v) for n, v in S32.items()}, {n: dev(v) for n, v in Sb.items()})
+    torch.cuda.synchronize()
+    errs = {n: _rel(got[n].cpu().numpy(), ref[n]) for n in FIELDS}
+    print({k: f"{v:.1e}" for k, v in errs.items()})
+    for n, e in errs.items():
+        assert e <= 1e-4, (n, e)
