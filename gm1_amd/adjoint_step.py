@@ -223,6 +223,127 @@ class DryPeriodicStep:
                     for n in STATE}
 
 
+def revolve_schedule(nsteps: int, snapshots: int) -> tuple[dict, int]:
+    """Binomial checkpointing: the split table of the cheapest recursive schedule and its forward-step count.
+
+    A window of n steps starting from a held state with s free checkpoint
+    slots is reversed by advancing m steps, storing that state, reversing the
+    last n - m steps with s - 1 slots and then the first m with s slots; the
+    table gives the m of least recomputation (Griewank's binomial bound is
+    reached), with cost(1, s) = 0 and, without free slots, m = n - 1.
+    """
+    from functools import lru_cache
+
+    @lru_cache(maxsize=None)
+    def cost(n: int, s: int) -> tuple[int, int]:
+        if n <= 1:
+            return 0, 0
+        if s <= 0:
+            return n * (n - 1) // 2, n - 1
+        best, arg = None, 1
+        for m in range(1, n):
+            c = m + cost(n - m, s - 1)[0] + cost(m, s)[0]
+            if best is None or c < best:
+                best, arg = c, m
+        return best, arg
+
+    table = {}
+    for n in range(2, nsteps + 1):
+        for s in range(0, snapshots + 1):
+            table[(n, s)] = cost(n, s)[1]
+    return table, cost(nsteps, snapshots)[0] if nsteps > 1 else 0
+
+
+def storm_intensity_weights(cfg, metrics, zlo: float = 2000.0, zhi: float = 8000.0) -> np.ndarray:
+    """Per-level weights of the storm-intensity loss: rf0 on the w levels with height in [zlo, zhi], normalised
+    so that J = sum(weight * w^2) is the base-density-weighted mean of w^2 over that layer."""
+    rf0 = np.asarray(metrics["rf0"], np.float64)[:, 0, 0]
+    z = np.arange(cfg.nk + 1) * float(cfg.dz)
+    wt = np.where((z >= zlo) & (z <= zhi), rf0, 0.0)
+    return (wt / (wt.sum() * cfg.ni * cfg.nj)).astype(np.float32)
+
+
+class WindowGradient:
+    """Gradient of the storm-intensity loss after ``nsteps`` large steps with respect to the initial state."""
+
+    def __init__(self, step: DryPeriodicStep, nsteps: int, snapshots: int = 8, zlo: float = 2000.0,
+                 zhi: float = 8000.0, metrics=None):
+        self.step = step
+        self.nsteps = nsteps
+        self.snapshots = max(0, min(snapshots, nsteps))
+        self.table, self.recompute_steps = revolve_schedule(nsteps, self.snapshots)
+        m = metrics
+        wt = storm_intensity_weights(step.cfg, m, zlo, zhi)
+        with step._on_stream():
+            self.weights = torch.from_numpy(wt).to(step.dev)[:, None, None]
+        self.forward_steps = 0
+        self.adjoint_steps = 0
+
+    def loss(self, S: dict) -> torch.Tensor:
+        return torch.sum(self.weights * S["w"] * S["w"])
+
+    def loss_adjoint(self, S: dict) -> dict:
+        Sb = self.step.empty_state()
+        with self.step._on_stream():
+            Sb["w"] = 2.0 * self.weights * S["w"]
+        return Sb
+
+    def advance(self, S: dict, n: int) -> dict:
+        for _ in range(n):
+            S = self.step.forward(S)
+            self.forward_steps += 1
+        return S
+
+    def value(self, S0: dict) -> torch.Tensor:
+        return self.loss(self.advance(S0, self.nsteps))
+
+    def _reverse(self, S: dict, n: int, s: int, Sb_end: dict | None):
+        """(adjoint at S, loss) for the n steps from S; ``Sb_end`` None means the loss is taken at the end."""
+        if n == 1:
+            J = None
+            if Sb_end is None:
+                end = self.step.forward(S)
+                self.forward_steps += 1
+                J = self.loss(end)
+                Sb_end = self.loss_adjoint(end)
+            self.adjoint_steps += 1
+            return self.step.adjoint(S, Sb_end), J
+        m = self.table[(n, max(s, 0))]
+        mid = self.advance(S, m)
+        Sb_mid, J = self._reverse(mid, n - m, s - 1, Sb_end)
+        del mid
+        Sb, _ = self._reverse(S, m, s, Sb_mid)
+        return Sb, J
+
+    def gradient(self, S0: dict) -> tuple[torch.Tensor, dict]:
+        """(J, adjoint of the initial state) by binomial checkpointing."""
+        self.forward_steps = 0
+        self.adjoint_steps = 0
+        Sb, J = self._reverse(S0, self.nsteps, self.snapshots, None)
+        return J, Sb
+
+
+class StormIntensity(torch.autograd.Function):
+    """J(theta', u, v) after a window of large steps, differentiable by the device adjoint.
+
+    ``apply(window, base_state, th, u, v)``: ``window`` a WindowGradient,
+    ``base_state`` the initial halo-free state whose w, pp and ppx are held
+    fixed; returns the scalar loss.
+    """
+
+    @staticmethod
+    def forward(ctx, window, base_state, th, u, v):
+        S0 = {**base_state, "th": th.detach(), "u": u.detach(), "v": v.detach()}
+        ctx.window = window
+        ctx.S0 = S0
+        return window.value(S0)
+
+    @staticmethod
+    def backward(ctx, gJ):
+        _, Sb = ctx.window.gradient(ctx.S0)
+        return None, None, gJ * Sb["th"], gJ * Sb["u"], gJ * Sb["v"]
+
+
 def from_padded(a: dict, ni: int, nj: int, nk: int, ng: int = 3) -> dict:
     """Halo-free state from a padded large-step state (torch tensors or arrays)."""
     return {"u": a["u3d"][1 : nk + 1, ng : ng + nj, ng : ng + ni],
@@ -233,4 +354,5 @@ def from_padded(a: dict, ni: int, nj: int, nk: int, ng: int = 3) -> dict:
             "ppx": a["ppx"][1 : nk + 1, 1 : nj + 1, 1 : ni + 1]}
 
 
-__all__ = ["DryPeriodicStep", "STATE", "from_padded", "pack_columns", "stage_consts"]
+__all__ = ["DryPeriodicStep", "STATE", "StormIntensity", "WindowGradient", "from_padded", "pack_columns",
+           "revolve_schedule", "stage_consts", "storm_intensity_weights"]

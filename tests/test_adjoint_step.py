@@ -178,3 +178,116 @@ def test_gpu_step_adjoint_matches_the_binary64_adjoint(device):
     print({k: f"{v:.1e}" for k, v in errs.items()})
     for n, e in errs.items():
         assert e <= 1e-4, (n, e)
+
+
+def test_revolve_schedule_reaches_the_binomial_bound():
+    """The split table's forward-step count equals Griewank's minimum t*n - C(s+t, s+1) (s free slots + the
+    held start = s+1 snapshots, t the smallest repetition number with C(s+1+t, t) >= n)."""
+    from math import comb
+
+    from gm1_amd.adjoint_step import revolve_schedule
+
+    for n, s in ((120, 8), (120, 12), (37, 3), (10, 2), (64, 5)):
+        snaps = s + 1
+        t = 0
+        while comb(snaps + t, t) < n:
+            t += 1
+        bound = t * n - comb(snaps + t, t - 1) if t > 0 else 0
+        _, cost = revolve_schedule(n, s)
+        assert cost == bound, (n, s, cost, bound)
+
+
+def _window_case(device, nsteps, snapshots):
+    import torch
+
+    from gm1_amd.adjoint_step import DryPeriodicStep, WindowGradient
+
+    cfg, host, a, prof, c = state32()
+    ni, nj, nk = cfg.ni, cfg.nj, cfg.nk
+    S32 = unpad_np(a, ni, nj, nk)
+    step = DryPeriodicStep(cfg, host["metrics"], host["base"], device)
+    win = WindowGradient(step, nsteps, snapshots, zlo=500.0, zhi=2000.0, metrics=host["metrics"])
+    dev = lambda x: torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).to(device)  # noqa: E731
+    return cfg, host, a, prof, c, S32, step, win, {n: dev(v) for n, v in S32.items()}
+
+
+@pytest.mark.gpu
+def test_gpu_window_gradient_against_the_binary64_gradient(device):
+    """dJ/d(theta', u, v) after 3 steps: device (binary32, checkpointed) against the binary64 adjoint."""
+    import torch
+
+    nsteps = 3
+    cfg, host, a, prof, c, S32, step, win, S0 = _window_case(device, nsteps, 1)
+    J, g = win.gradient(S0)
+    torch.cuda.synchronize()
+    o64, als, p = _oracle64_trajectory(a, host, cfg, prof, c)
+    S = {n: v.astype(np.float64) for n, v in S32.items()}
+    tapes, states = [], [S]
+    for _ in range(nsteps):
+        out, T = als.Step(p).forward(states[-1])
+        tapes.append(T)
+        states.append(out)
+    J64, wb = als.storm_intensity(p, states[-1]["w"], zlo=500.0, zhi=2000.0)
+    Sb = {n: np.zeros_like(states[-1][n]) for n in FIELDS}
+    Sb["w"] = wb
+    for T in reversed(tapes):
+        Sb = als.Step(p).adjoint(T, Sb)
+    print("J", float(J), J64, "recomputed steps", win.forward_steps)
+    assert abs(float(J) - J64) <= 1e-5 * abs(J64)
+    for n in ("th", "u", "v", "w", "pp"):
+        e = _rel(g[n].cpu().numpy(), Sb[n])
+        print(n, f"{e:.2e}")
+        assert e <= 1e-4, (n, e)
+
+
+@pytest.mark.gpu
+def test_gpu_checkpoint_schedules_agree(device):
+    import torch
+
+    cfg, host, a, prof, c, S32, step, win0, S0 = _window_case(device, 5, 0)
+    J0, g0 = win0.gradient(S0)
+    from gm1_amd.adjoint_step import WindowGradient
+
+    win3 = WindowGradient(step, 5, 3, zlo=500.0, zhi=2000.0, metrics=host["metrics"])
+    J3, g3 = win3.gradient(S0)
+    torch.cuda.synchronize()
+    print("forward steps", win0.forward_steps, win3.forward_steps)
+    assert win0.forward_steps > win3.forward_steps
+    assert float(J0) == float(J3)
+    for n in ("th", "u", "v"):
+        e = _rel(g3[n].cpu().numpy(), g0[n].cpu().numpy().astype(np.float64))
+        assert e <= 1e-5, (n, e)
+
+
+@pytest.mark.gpu
+def test_gpu_gradient_central_differences_and_autograd(device):
+    """Central differences (eps = 1e-3 of each field's scale) in 4 random directions, and torch.autograd."""
+    import torch
+
+    from gm1_amd.adjoint_step import StormIntensity
+
+    cfg, host, a, prof, c, S32, step, win, S0 = _window_case(device, 3, 2)
+    J, g = win.gradient(S0)
+    rng = np.random.default_rng(31)
+    worst = 0.0
+    for trial in range(4):
+        d = {n: torch.from_numpy((rng.standard_normal(S32[n].shape) * np.abs(S32[n]).max()).astype(np.float32)).to(
+            device) for n in ("th", "u", "v")}
+        eps = 1e-3
+        Sp = {**S0, **{n: S0[n] + eps * d[n] for n in d}}
+        Sm = {**S0, **{n: S0[n] - eps * d[n] for n in d}}
+        Jp = torch.sum(win.weights.double() * win.advance(Sp, 3)["w"].double() ** 2)
+        Jm = torch.sum(win.weights.double() * win.advance(Sm, 3)["w"].double() ** 2)
+        fd = float((Jp - Jm) / (2 * eps))
+        ad = sum(float(torch.sum(g[n].double() * d[n].double())) for n in d)
+        rel = abs(fd - ad) / abs(fd)
+        worst = max(worst, rel)
+        print(f"direction {trial}: fd={fd:.8e} adjoint={ad:.8e} rel={rel:.2e}")
+    assert worst <= 1e-2, worst
+    base = {n: S0[n] for n in ("w", "pp", "ppx")}
+    th, u, v = (S0[n].clone().requires_grad_(True) for n in ("th", "u", "v"))
+    Jt = StormIntensity.apply(win, base, th, u, v)
+    gth, gu, gv = torch.autograd.grad(Jt, (th, u, v))
+    assert float(Jt) == float(J)
+    for got, ref in ((gth, g["th"]), (gu, g["u"]), (gv, g["v"])):
+        assert _rel(got.cpu().numpy(), ref.cpu().numpy().astype(np.float64)) <= 1e-5
