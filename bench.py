@@ -68,6 +68,9 @@ def parse_args(argv=None):
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing")
     p.add_argument("--adjoint-repeats", type=int, default=10)
+    p.add_argument("--adjoint-window", type=int, default=120, help="large steps of the gradient window")
+    p.add_argument("--adjoint-snapshots", type=int, default=16, help="free checkpoint slots of the window")
+    p.add_argument("--adjoint-fd", type=int, default=2, help="central-difference directions checked")
     p.add_argument("--graph", action="store_true",
                    help="time replays of the step captured into a HIP graph (the kernel timer then runs in an "
                         "eager pass of the same length)")
@@ -215,19 +218,87 @@ def run_ensemble(args) -> int:
     return 0
 
 
-def adjoint_ratio(repeats: int):
-    """Device time of the acoustic RK stages (nrk 1, 2, 3) and of their adjoints on the dry periodic
-    192x192x64 case after one forward large step (its thv, ppterm and piadv held fixed)."""
+def adjoint_ratio(repeats: int, window: int, snapshots: int, fd_directions: int):
+    """Adjoint/forward device time on the dry periodic 192x192x64 case.
+
+    * large step: the production forward step (padded, eager) against the
+      adjoint of one step (halo-free stages recomputed, then transposed);
+    * window: storm-intensity value over ``window`` steps against its gradient
+      by binomial checkpointing with ``snapshots`` free slots, and central
+      differences (eps = 1e-3 of each field's scale) along ``fd_directions``
+      smooth random directions of (theta', u, v);
+    * acoustic stages nrk 1-3 with fixed coefficients against their adjoints.
+    """
+    import numpy as np
     import torch
 
-    from gm1_amd.adjoint_sound import STATE, check_flat_periodic, launch_cuda_psolver3_stage_adjoint
+    from gm1_amd.adjoint_sound import STATE as AC_STATE
+    from gm1_amd.adjoint_sound import check_flat_periodic, launch_cuda_psolver3_stage_adjoint
+    from gm1_amd.adjoint_step import DryPeriodicStep, WindowGradient, from_padded, smooth_direction
     from gm1_amd.fields import adjoint_c5
     from gm1_amd.large_step import create_gm1_large_step_state, host_initial_state, launch_cuda_gm1_large_step
     from gm1_amd.psolver3 import launch_cuda_psolver3_terrain_open_stage
 
     cfg = adjoint_c5()
+    host = host_initial_state(cfg)
     stream = torch.cuda.Stream()
-    st = create_gm1_large_step_state(host_initial_state(cfg), "cuda:0", stream=stream)
+    st = create_gm1_large_step_state(host, "cuda:0", stream=stream)
+
+    def timed(fn, reps=repeats):
+        fn()
+        torch.cuda.synchronize()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for _ in range(reps):
+            fn()
+        stop.record(stream)
+        torch.cuda.synchronize()
+        return start.elapsed_time(stop) / reps
+
+    step_fwd_ms = timed(lambda: launch_cuda_gm1_large_step(st))
+    step = DryPeriodicStep(cfg, host["metrics"], host["base"], "cuda:0", stream=stream)
+    with torch.cuda.stream(stream):
+        S0 = {n: v.contiguous().clone() for n, v in from_padded(
+            {k: torch.from_numpy(np.ascontiguousarray(host["arrays"][k])).to("cuda:0")
+             for k in ("u3d", "v3d", "w3d", "th3d", "pp3d", "ppx")}, cfg.ni, cfg.nj, cfg.nk).items()}
+    gen = torch.Generator(device="cuda:0").manual_seed(5)
+    with torch.cuda.stream(stream):
+        Sb = {n: torch.randn(S0[n].shape, device="cuda:0", generator=gen) for n in S0}
+    halo_free_fwd_ms = timed(lambda: step.forward(S0))
+    step_adj_ms = timed(lambda: step.adjoint(S0, Sb))
+    print(f"[bench] C5 step: forward {step_fwd_ms:.3f} ms (halo-free {halo_free_fwd_ms:.3f}), adjoint "
+          f"{step_adj_ms:.3f} ms", file=sys.stderr, flush=True)
+    win = WindowGradient(step, window, snapshots, metrics=host["metrics"])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J = win.value(S0)
+    torch.cuda.synchronize()
+    win_fwd_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    Jg, g = win.gradient(S0)
+    torch.cuda.synchronize()
+    win_grad_s = time.perf_counter() - t0
+    grad_forward_steps, grad_adjoint_steps = win.forward_steps, win.adjoint_steps
+    print(f"[bench] window {window}: value {win_fwd_s:.2f} s, gradient {win_grad_s:.2f} s "
+          f"({grad_forward_steps} forward, {grad_adjoint_steps} adjoint steps)", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(11)
+    fd = []
+    side = torch.cuda.stream(stream)  # perturbed states and sums on the stream the steps run on
+    side.__enter__()
+    for _ in range(fd_directions):
+        d = {n: torch.from_numpy((smooth_direction(rng, tuple(S0[n].shape)) * float(S0[n].abs().max())).astype(
+            np.float32)).to("cuda:0") for n in ("th", "u", "v")}
+        eps = 1e-3
+        Jp = torch.sum(win.weights.double() * win.advance({**S0, **{n: S0[n] + eps * d[n] for n in d}},
+                                                          window)["w"].double() ** 2)
+        Jm = torch.sum(win.weights.double() * win.advance({**S0, **{n: S0[n] - eps * d[n] for n in d}},
+                                                          window)["w"].double() ** 2)
+        num = float((Jp - Jm) / (2 * eps))
+        ana = sum(float(torch.sum(g[n].double() * d[n].double())) for n in d)
+        fd.append({"fd": num, "adjoint": ana, "rel": abs(num - ana) / max(abs(num), 1e-300)})
+    side.__exit__(None, None, None)
+    print(f"[bench] central differences: {[round(x['rel'], 6) for x in fd]}", file=sys.stderr, flush=True)
+    # the acoustic stages with their coefficients held
     launch_cuda_gm1_large_step(st)
     torch.cuda.synchronize()
     check_flat_periodic(st.metrics, cfg.lateral_sides)
@@ -235,40 +306,32 @@ def adjoint_ratio(repeats: int):
           "dy": cfg.dy, "dz": cfg.dz}
     a = st.a
     coef = {n: a[n] for n in ("thv", "ppterm", "piadv")}
-    gen = torch.Generator(device="cuda:0").manual_seed(5)
-    bar = {n: torch.randn(a[n].shape, device="cuda:0", generator=gen) for n in STATE}
+    with torch.cuda.stream(stream):
+        bar = {n: torch.randn(a[n].shape, device="cuda:0", generator=gen) for n in AC_STATE}
 
-    def forward():
+    def ac_forward():
         for nrk in (1, 2, 3):
             launch_cuda_psolver3_terrain_open_stage(a, st.metrics, nrk=nrk, lateral_sides=cfg.lateral_sides,
                                                     stream=stream, **kw)
 
-    def adjoint():
+    def ac_adjoint():
         b = bar
         for nrk in (3, 2, 1):
             out, _ = launch_cuda_psolver3_stage_adjoint(b, coef, st.metrics, nrk=nrk, check_geometry=False,
                                                         stream=stream, **kw)
             b = {**bar, **{n: out[n] for n in ("u3d", "v3d", "w3d", "pp3d", "ppx")}}
 
-    def timed(fn):
-        fn()
-        torch.cuda.synchronize()
-        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        start.record(stream)
-        for _ in range(repeats):
-            fn()
-        stop.record(stream)
-        torch.cuda.synchronize()
-        return start.elapsed_time(stop) / repeats
-
-    fwd_ms = timed(forward)
-    adj_ms = timed(adjoint)
-    print(f"[bench] acoustic stages {fwd_ms:.3f} ms, adjoint {adj_ms:.3f} ms", file=sys.stderr, flush=True)
-    return {"scope": "psolver=3 acoustic RK stages nrk 1-3 (forward) vs their adjoints (reverse order), "
-                     "fixed thv/ppterm/piadv; the adjoint of the whole large step is not built",
-            "workload": f"dry periodic {cfg.ni}x{cfg.nj}x{cfg.nk}, dx={cfg.dx:g} m, dz={cfg.dz:g} m, "
+    ac_fwd_ms = timed(ac_forward)
+    ac_adj_ms = timed(ac_adjoint)
+    return {"workload": f"dry periodic {cfg.ni}x{cfg.nj}x{cfg.nk}, dx={cfg.dx:g} m, dz={cfg.dz:g} m, "
                         f"dt={cfg.dt:g} s, {cfg.nsound} substeps",
-            "forward_ms": fwd_ms, "adjoint_ms": adj_ms, "ratio": adj_ms / fwd_ms}
+            "step_forward_ms": step_fwd_ms, "step_forward_halo_free_ms": halo_free_fwd_ms,
+            "step_adjoint_ms": step_adj_ms, "ratio": step_adj_ms / step_fwd_ms,
+            "window": {"steps": window, "snapshots": snapshots, "loss": float(J),
+                       "value_s": win_fwd_s, "gradient_s": win_grad_s, "ratio": win_grad_s / win_fwd_s,
+                       "forward_steps": grad_forward_steps, "adjoint_steps": grad_adjoint_steps,
+                       "central_differences": fd},
+            "acoustic_stages": {"forward_ms": ac_fwd_ms, "adjoint_ms": ac_adj_ms, "ratio": ac_adj_ms / ac_fwd_ms}}
 
 
 def main(argv=None) -> int:
@@ -401,7 +464,8 @@ def main(argv=None) -> int:
             "kernel_share_of_step": kernel_ms / elapsed_ms,
         },
     }
-    result["adjoint"] = None if args.no_adjoint else adjoint_ratio(args.adjoint_repeats)
+    result["adjoint"] = None if args.no_adjoint else adjoint_ratio(args.adjoint_repeats, args.adjoint_window,
+                                                                   args.adjoint_snapshots, args.adjoint_fd)
     if not args.no_cpu_baseline:
         print("[bench] timing the CPU baseline sample", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(cfg, args.cpu_sample, args.cpu_steps)

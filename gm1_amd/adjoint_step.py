@@ -9,8 +9,11 @@ configuration (flat terrain, horizontally uniform base state, CM1's
   large step (cuda_gm1_forecast.py:1241-1345, 1414-1457) in the reference's
   association order, keeping every acoustic substep's state;
 * ``DryPeriodicStep.adjoint`` is the transpose of one large step at its
-  trajectory (upwind choices, WENO5 weights and limiters frozen at their
-  forward values, smeps clamps as the identity);
+  trajectory: upwind choices held at their forward values, the smeps
+  clamps taken as the identity, and the WENO5 weights and positive-definite
+  limiters of the final stage's theta fluxes either differentiated
+  (``weno="exact"``, the default: the gradient of the step itself) or frozen
+  at their forward values (``weno="frozen"``);
 * ``storm_intensity`` is the adjoint configuration's loss (base-density
   weighted mean of w^2 over the w levels between 2 and 8 km) and
   ``gradient`` its gradient with respect to the initial state over a window
@@ -55,8 +58,14 @@ _RD, _RV, _CP, _CPV, _G = F32(287.04), F32(461.5), F32(1005.7), F32(1870.0), F32
 THETA_WEPS = float(F32(10.0) * F32(1.0e-18))
 
 
-def stage_consts(dt: float, nsound: int, nrk: int, dx: float, dy: float, dz: float) -> tuple[np.ndarray, int]:
-    """The binary32 scalars of one stage (adjoint_step.hip's Consts) and its substep count."""
+def stage_consts(dt: float, nsound: int, nrk: int, dx: float, dy: float, dz: float,
+                 weno: str = "exact") -> tuple[np.ndarray, int]:
+    """The binary32 scalars of one stage (adjoint_step.hip's Consts) and its substep count.
+
+    ``weno`` "exact" differentiates the WENO5 weights and limiters in the
+    adjoint; "frozen" holds them at their forward values."""
+    if weno not in ("exact", "frozen"):
+        raise ValueError("weno must be 'exact' or 'frozen'")
     nloop, dts = sound3_stage_schedule_f32(dt, nsound, nrk)
     dts = F32(dts)
     rdx, rdy, rdz = F32(1.0) / F32(dx), F32(1.0) / F32(dy), F32(1.0) / F32(dz)
@@ -66,7 +75,7 @@ def stage_consts(dt: float, nsound: int, nrk: int, dx: float, dy: float, dz: flo
     vals = [rdx, rdy, rdz, dts, F32(F32(dt) / F32(4 - nrk)), alph, F32(1.0) - alph, kdiv, _CP * F32(0.5), _CP,
             F32(1.0) / F32(nloop), (dts * F32(0.5)) * rdx, (dts * F32(0.5)) * rdy,
             _G, rovcp, _RV / _RD, _RV / _RD - F32(1.0), cpm, F32(1.0) / ((_CP - _RD) + F32(0.0)),
-            (_RD + _RV * F32(0.0)) - rovcp * cpm, alph * alph, F32(THETA_WEPS)]
+            (_RD + _RV * F32(0.0)) - rovcp * cpm, alph * alph, F32(THETA_WEPS), F32(1.0 if weno == "exact" else 0.0)]
     return np.array(vals, F32), nloop
 
 
@@ -96,8 +105,9 @@ def _require_flat_periodic(cfg, metrics) -> None:
 class DryPeriodicStep:
     """Workspace and launches of the halo-free dry periodic step (one device, one stream)."""
 
-    def __init__(self, cfg, metrics, base, device="cuda", stream: Any | None = None):
+    def __init__(self, cfg, metrics, base, device="cuda", stream: Any | None = None, weno: str = "exact"):
         _require_flat_periodic(cfg, metrics)
+        self.weno = weno
         self.cfg = cfg
         self.ni, self.nj, self.nk = cfg.ni, cfg.nj, cfg.nk
         if min(self.ni, self.nj, self.nk) < 6:
@@ -107,7 +117,8 @@ class DryPeriodicStep:
         self.consts = {}
         self.nloops = {}
         for nrk in (1, 2, 3):
-            self.consts[nrk], self.nloops[nrk] = stage_consts(cfg.dt, cfg.nsound, nrk, cfg.dx, cfg.dy, cfg.dz)
+            self.consts[nrk], self.nloops[nrk] = stage_consts(cfg.dt, cfg.nsound, nrk, cfg.dx, cfg.dy, cfg.dz,
+                                                              weno)
         self.nloop_max = max(self.nloops.values())
         ni, nj, nk = self.ni, self.nj, self.nk
         self.shape = (nk, nj, ni)
@@ -254,6 +265,22 @@ def revolve_schedule(nsteps: int, snapshots: int) -> tuple[dict, int]:
     return table, cost(nsteps, snapshots)[0] if nsteps > 1 else 0
 
 
+def smooth_direction(rng: np.random.Generator, shape: tuple, modes: int = 3) -> np.ndarray:
+    """A random periodic perturbation of unit max-norm built from the lowest horizontal and vertical modes
+    (wavenumbers 1..modes), the kind of direction a finite-difference check of the gradient resolves."""
+    nk, nj, ni = shape
+    k = np.arange(nk)[:, None, None] + 0.5
+    j = np.arange(nj)[None, :, None]
+    i = np.arange(ni)[None, None, :]
+    out = np.zeros(shape)
+    for _ in range(2 * modes):
+        kx, ky, kz = rng.integers(1, modes + 1, size=3)
+        ph = rng.uniform(0.0, 2.0 * np.pi, size=3)
+        out += rng.standard_normal() * (np.cos(2 * np.pi * kx * i / ni + ph[0]) * np.cos(2 * np.pi * ky * j / nj + ph[1])
+                                        * np.sin(np.pi * kz * k / nk + ph[2]))
+    return out / np.abs(out).max()
+
+
 def storm_intensity_weights(cfg, metrics, zlo: float = 2000.0, zhi: float = 8000.0) -> np.ndarray:
     """Per-level weights of the storm-intensity loss: rf0 on the w levels with height in [zlo, zhi], normalised
     so that J = sum(weight * w^2) is the base-density-weighted mean of w^2 over that layer."""
@@ -280,7 +307,8 @@ class WindowGradient:
         self.adjoint_steps = 0
 
     def loss(self, S: dict) -> torch.Tensor:
-        return torch.sum(self.weights * S["w"] * S["w"])
+        with self.step._on_stream():
+            return torch.sum(self.weights * S["w"] * S["w"])
 
     def loss_adjoint(self, S: dict) -> dict:
         Sb = self.step.empty_state()
@@ -355,4 +383,4 @@ def from_padded(a: dict, ni: int, nj: int, nk: int, ng: int = 3) -> dict:
 
 
 __all__ = ["DryPeriodicStep", "STATE", "StormIntensity", "WindowGradient", "from_padded", "pack_columns",
-           "revolve_schedule", "stage_consts", "storm_intensity_weights"]
+           "revolve_schedule", "smooth_direction", "stage_consts", "storm_intensity_weights"]

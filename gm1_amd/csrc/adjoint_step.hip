@@ -51,8 +51,9 @@ enum Slot {
 struct Consts {
   float rdx, rdy, rdz, dts, dttmp, alph, beta, kdiv, halfcp, cp, tavg, temx, temy;
   float g, rovcp, reps, repsm1, cpm, cvm, kterm, asq, weps_f;
+  float exact;  // 1: WENO5 weights and limiters differentiated; 0: frozen at their forward values
 };
-constexpr int NCONST = 22;
+constexpr int NCONST = 23;
 
 struct Cols {
   const float *rho, *rr0, *rf0, *rrf0, *c1, *c2, *th0, *pi0, *qv0, *rth, *rds, *rdsf;
@@ -226,7 +227,8 @@ __device__ __forceinline__ float eval_branch(const Stencil& s, Q q, float c2h, f
 
 // Coefficients of the stencil's values (frozen WENO5 weights and limiter) in branch order.
 template <typename Q>
-__device__ __forceinline__ int branch_coefs(const Stencil& s, Q q, float c2h, float c1lin, float wf, float* cf) {
+__device__ __forceinline__ int branch_coefs(const Stencil& s, Q q, float c2h, float c1lin, float wf, float* cf,
+                                            bool exact = false) {
   switch (s.kind) {
     case LIN: cf[0] = c1lin; cf[1] = c2h; return 2;
     case HALF2: cf[0] = 0.5f; cf[1] = 0.5f; return 2;
@@ -241,7 +243,20 @@ __device__ __forceinline__ int branch_coefs(const Stencil& s, Q q, float c2h, fl
       const float rr = (s3 - s2) / (se + dd);
       const float cand = gm1::C1_6 + gm1::C2_6 * rr;
       const float phi = fmaxf(0.0f, fminf(rr, fminf(cand, 1.0f)));
-      cf[0] = -phi; cf[1] = 1.0f + phi; cf[2] = 0.0f;
+      if (!exact) {
+        cf[0] = -phi; cf[1] = 1.0f + phi; cf[2] = 0.0f;
+        return 3;
+      }
+      // the limiter's active branch differentiated: phi = rr (scale 1), 1/6 + rr/3 (scale 1/3), 0 or 1 (0)
+      const float inner = fminf(cand, 1.0f);
+      const float mn = fminf(rr, inner);
+      const float scale = !(mn > 0.0f) ? 0.0f : (rr <= inner ? 1.0f : (cand <= 1.0f ? gm1::C2_6 : 0.0f));
+      const double den = (double)se + (double)dd;
+      const double g1 = ((double)s3 - s2) / (den * den), g2 = -1.0 / den - ((double)s3 - s2) / (den * den);
+      const double g3 = 1.0 / den;
+      cf[0] = (float)(-phi + dd * (scale * g1));
+      cf[1] = (float)(1.0f + phi + dd * (scale * g2));
+      cf[2] = (float)(dd * (scale * g3));
       return 3;
     }
     default: return -1;
@@ -276,14 +291,62 @@ __device__ __forceinline__ void weno_coefs(float s1, float s2, float s3, float s
   cf[4] = (float)(o3 * (double)gm1::CM1_6);
 }
 
+// d(WENO5 reconstruction)/d(s1..s5) with the WENO-Z weights differentiated (binary64; zero where q is clamped).
+__device__ __forceinline__ void weno_jacobian(float f1, float f2, float f3, float f4, float f5, double weps,
+                                              float* cf) {
+  const double s1 = f1, s2 = f2, s3 = f3, s4 = f4, s5 = f5;
+  const double c13 = (double)gm1::C13_12;
+  const double d1 = (s1 - 2.0 * s2) + s3, e1 = (s1 - 4.0 * s2) + 3.0 * s3;
+  const double d2 = (s2 - 2.0 * s3) + s4, e2 = s2 - s4;
+  const double d3 = (s3 - 2.0 * s4) + s5, e3 = (3.0 * s3 - 4.0 * s4) + s5;
+  const double b[3] = {c13 * (d1 * d1) + 0.25 * (e1 * e1), c13 * (d2 * d2) + 0.25 * (e2 * e2),
+                       c13 * (d3 * d3) + 0.25 * (e3 * e3)};
+  const double db[3][5] = {
+      {2 * c13 * d1 + 0.5 * e1, -4 * c13 * d1 - 2.0 * e1, 2 * c13 * d1 + 1.5 * e1, 0.0, 0.0},
+      {0.0, 2 * c13 * d2 + 0.5 * e2, -4 * c13 * d2, 2 * c13 * d2 - 0.5 * e2, 0.0},
+      {0.0, 0.0, 2 * c13 * d3 + 1.5 * e3, -4 * c13 * d3 - 2.0 * e3, 2 * c13 * d3 + 0.5 * e3}};
+  const double diff = b[0] - b[2];
+  const double tau = fabs(diff);
+  const double sg = diff > 0.0 ? 1.0 : (diff < 0.0 ? -1.0 : 0.0);
+  const double a[3] = {(double)0.1f, (double)0.6f, (double)0.3f};
+  double wv[3], dw[3][5];
+  for (int m = 0; m < 3; ++m) {
+    const double den = b[m] + weps;
+    double q = tau / den;
+    const bool live = q < 1.0e30;
+    q = fmin(1.0e30, q);
+    wv[m] = a[m] * (1.0 + q * q);
+    for (int n = 0; n < 5; ++n) {
+      const double dtau = sg * (db[0][n] - db[2][n]);
+      const double dq = live ? dtau / den - tau * db[m][n] / (den * den) : 0.0;
+      dw[m][n] = 2.0 * a[m] * q * dq;
+    }
+  }
+  const double W = (wv[0] + wv[1]) + wv[2];
+  const double om[3] = {wv[0] / W, wv[1] / W, wv[2] / W};
+  const double p[3] = {((double)gm1::C2_6 * s1 + (double)gm1::CM7_6 * s2) + (double)gm1::C11_6 * s3,
+                       ((double)gm1::CM1_6 * s2 + (double)gm1::C5_6 * s3) + (double)gm1::C2_6 * s4,
+                       ((double)gm1::C2_6 * s3 + (double)gm1::C5_6 * s4) + (double)gm1::CM1_6 * s5};
+  const double dp[3][5] = {{gm1::C2_6, gm1::CM7_6, gm1::C11_6, 0.0, 0.0},
+                           {0.0, gm1::CM1_6, gm1::C5_6, gm1::C2_6, 0.0},
+                           {0.0, 0.0, gm1::C2_6, gm1::C5_6, gm1::CM1_6}};
+  for (int n = 0; n < 5; ++n) {
+    const double dW = (dw[0][n] + dw[1][n]) + dw[2][n];
+    double g = 0.0;
+    for (int m = 0; m < 3; ++m) g += om[m] * dp[m][n] + p[m] * ((dw[m][n] - om[m] * dW) / W);
+    cf[n] = (float)g;
+  }
+}
+
 template <typename Q>
 __device__ __forceinline__ int stencil_coefs(const Stencil& s, Q q, float c2h, float c1lin, double weps, float wf,
-                                             float* cf) {
+                                             float* cf, bool exact) {
   if (s.kind == WENO) {
-    weno_coefs(q(s.lev[0]), q(s.lev[1]), q(s.lev[2]), q(s.lev[3]), q(s.lev[4]), weps, cf);
+    if (exact) weno_jacobian(q(s.lev[0]), q(s.lev[1]), q(s.lev[2]), q(s.lev[3]), q(s.lev[4]), weps, cf);
+    else weno_coefs(q(s.lev[0]), q(s.lev[1]), q(s.lev[2]), q(s.lev[3]), q(s.lev[4]), weps, cf);
     return 5;
   }
-  return branch_coefs(s, q, c2h, c1lin, wf, cf);
+  return branch_coefs(s, q, c2h, c1lin, wf, cf, exact);
 }
 
 // u tendency at (l, j, face i)
@@ -728,7 +791,8 @@ __global__ __launch_bounds__(64) void k_b_theta_xy(W w, int dir) {
   float r;
   if (weno) {
     r = weno5(s[0], s[1], s[2], s[3], s[4], w.weps);
-    weno_coefs(s[0], s[1], s[2], s[3], s[4], w.weps, cf);
+    if (c.exact != 0.0f) weno_jacobian(s[0], s[1], s[2], s[3], s[4], w.weps, cf);
+    else weno_coefs(s[0], s[1], s[2], s[3], s[4], w.weps, cf);
   } else {
     r = flx5(s[0], s[1], s[2], s[3], s[4]);
     cf[0] = gm1::C2_60; cf[1] = gm1::CM13_60; cf[2] = gm1::C47_60; cf[3] = gm1::C27_60; cf[4] = gm1::CM3_60;
@@ -765,7 +829,7 @@ __global__ __launch_bounds__(64) void k_b_theta_z(W w) {
   const float r = eval_branch(st, q, c2, c1, w.weps, c.weps_f);
   w.p[B_MW][w.at(k, j, i)] = fb * r;
   float cf[5];
-  const int nc = stencil_coefs(st, q, c2, c1, w.weps, c.weps_f, cf);
+  const int nc = stencil_coefs(st, q, c2, c1, w.weps, c.weps_f, cf, c.exact != 0.0f);
   const float rb = fb * m;
   for (int n = 0; n < nc; ++n) aadd(&w.p[B_SC][w.at(st.lev[n], j, i)], cf[n] * rb);
 }

@@ -144,7 +144,7 @@ int gm1hip_psolver3_stage_adjoint(void* const* arrays, int ni, int nj, int nk, i
 // reference's interface (it restates cuda_gm1_forecast.py:1241-1345 for the
 // adjoint's trajectory; the reference has no adjoint).  `table` holds the
 // device pointers of the workspace in the order of gm1_amd/adjoint_step.py
-// SLOTS, `consts` the stage's 22 binary32 scalars (stage_consts).  Fields are
+// SLOTS, `consts` the stage's 23 binary32 scalars (stage_consts).  Fields are
 // (nk, nj, ni), w-type fields (nk+1, nj, ni), substep states (nloop+1)
 // planes.  ni, nj, nk >= 6.
 int gm1hip_dry_stage_forward(void* const* table, const float* consts, int ni, int nj, int nk, int nrk, int nloop,
@@ -158,7 +158,9 @@ int gm1hip_dry_stage_forward(void* const* table, const float* consts, int ni, in
 // solve3's pp + ppx).  Out: B_XU B_XV B_XW B_XTH B_XPP (adjoints of the
 // stage's input state), B_PPX (adjoint of the ppx it read); B_AU B_AV B_AW
 // B_ATH B_API (adjoints of the step-start state) are accumulated.  Upwind
-// choices, WENO5 weights and limiters are frozen at their forward values.
+// choices are held at their forward values; the last of `consts` selects
+// whether the theta WENO5 weights and limiters are differentiated (1) or
+// frozen at their forward values (0).
 int gm1hip_dry_stage_adjoint(void* const* table, const float* consts, int ni, int nj, int nk, int nrk, int nloop,
                              double weps, void* stream);
 

@@ -70,10 +70,13 @@ class Tape:
     into the linear map that the adjoint transposes.
     """
 
-    def __init__(self):
+    def __init__(self, exact: bool = True):
         self.record = True
+        self.exact = exact
         self.sel = {}
         self.coef = {}
+        self.jac = {}
+        self.branch = {}
         self.traj = {}
 
     def replay(self):
@@ -89,8 +92,10 @@ class Tape:
         return np.where(np.abs(x) < eps, 0.0, x) if self.record else x
 
     def weno(self, key, s1, s2, s3, s4, s5, weps):
-        """WENO5 (common.weno5) with its normalised weights frozen on replay."""
-        if self.record:
+        """WENO5 (common.weno5); on replay its normalised weights stay frozen unless ``exact``."""
+        if self.record or self.exact:
+            if self.record:
+                self.jac[key] = _weno_jacobian(s1, s2, s3, s4, s5, weps)
             d1 = (s1 - 2.0 * s2) + s3
             e1 = (s1 - 4.0 * s2) + 3.0 * s3
             b1 = float(C13_12) * (d1 * d1) + 0.25 * (e1 * e1)
@@ -109,7 +114,8 @@ class Tape:
             w2 = float(F(0.6)) * (1.0 + q2 * q2)
             w3 = float(F(0.3)) * (1.0 + q3 * q3)
             den = (w1 + w2) + w3
-            self.coef[key] = (w1 / den, w2 / den, w3 / den)
+            if self.record:
+                self.coef[key] = (w1 / den, w2 / den, w3 / den)
             p1, p2, p3 = self._parts(s1, s2, s3, s4, s5)
             return ((w1 * p1 + w2 * p2) + w3 * p3) / den
         o1, o2, o3 = self.coef[key]
@@ -124,24 +130,97 @@ class Tape:
         return p1, p2, p3
 
     def weno_coefs(self, key):
-        """Coefficients of s1..s5 in the frozen WENO5 reconstruction."""
+        """d(reconstruction)/d(s1..s5): exact, or with the weights frozen."""
+        if self.exact:
+            return self.jac[key]
         o1, o2, o3 = self.coef[key]
         c = lambda x: float(x)  # noqa: E731
         return (o1 * c(C2_6), o1 * c(CM7_6) + o2 * c(CM1_6), (o1 * c(C11_6) + o2 * c(C5_6)) + o3 * c(C2_6),
                 o2 * c(C2_6) + o3 * c(C5_6), o3 * c(CM1_6))
 
     def upstrpd(self, key, s1, s2, s3, weps):
-        """Positive-definite closure (common.upstrpd) with its limiter frozen on replay."""
+        """Positive-definite closure (common.upstrpd); on replay its limiter stays frozen, or with ``exact``
+        keeps its recorded branch (0, 1, rr or 1/6 + rr/3) evaluated at the new values."""
+        dd = s2 - s1
+        se = np.copysign(abs(weps), dd)
+        with np.errstate(all="ignore"):
+            rr = (s3 - s2) / (se + dd)
+        cand = float(C1_6) + float(C2_6) * rr
         if self.record:
-            dd = s2 - s1
-            se = np.copysign(abs(weps), dd)
+            phi = np.maximum(0.0, np.minimum(rr, np.minimum(cand, 1.0)))
+            inner = np.minimum(cand, 1.0)
+            mn = np.minimum(rr, inner)
+            br = np.where(~(mn > 0.0), 0, np.where(rr <= inner, 2, np.where(cand <= 1.0, 3, 1)))
+            self.coef[key] = phi
+            self.branch[key] = br
+            den = se + dd
             with np.errstate(all="ignore"):
-                rr = (s3 - s2) / (se + dd)
-            cand = float(C1_6) + float(C2_6) * rr
-            self.coef[key] = np.maximum(0.0, np.minimum(rr, np.minimum(cand, 1.0)))
-        phi = self.coef[key]
+                g1 = (s3 - s2) / (den * den)
+                g2 = -1.0 / den - (s3 - s2) / (den * den)
+                g3 = 1.0 / den
+            scale = np.where(br == 2, 1.0, np.where(br == 3, float(C2_6), 0.0))
+            dphi = [scale * g for g in (g1, g2, g3)]
+            self.jac[key] = (-phi + dd * dphi[0], 1.0 + phi + dd * dphi[1], dd * dphi[2])
+            return s2 + phi * (s2 - s1)
+        if self.exact:
+            br = self.branch[key]
+            phi = np.where(br == 0, 0.0, np.where(br == 1, 1.0, np.where(br == 2, rr, cand)))
+        else:
+            phi = self.coef[key]
         return s2 + phi * (s2 - s1)
 
+    def upstrpd_coefs(self, key):
+        """d(closure)/d(s1, s2, s3): exact, or with the limiter frozen."""
+        if self.exact:
+            return self.jac[key]
+        phi = self.coef[key]
+        return (-phi, 1.0 + phi, 0.0)
+
+
+
+def _weno_jacobian(s1, s2, s3, s4, s5, weps):
+    """d(WENO5 reconstruction)/d(s1..s5) with the WENO-Z weights differentiated (zero where q is clamped)."""
+    c13 = float(C13_12)
+    d1 = (s1 - 2.0 * s2) + s3
+    e1 = (s1 - 4.0 * s2) + 3.0 * s3
+    d2 = (s2 - 2.0 * s3) + s4
+    e2 = s2 - s4
+    d3 = (s3 - 2.0 * s4) + s5
+    e3 = (3.0 * s3 - 4.0 * s4) + s5
+    b = [c13 * (d1 * d1) + 0.25 * (e1 * e1), c13 * (d2 * d2) + 0.25 * (e2 * e2), c13 * (d3 * d3) + 0.25 * (e3 * e3)]
+    z = np.zeros_like(d1)
+    db = [[2 * c13 * d1 + 0.5 * e1, -4 * c13 * d1 - 2.0 * e1, 2 * c13 * d1 + 1.5 * e1, z, z],
+          [z, 2 * c13 * d2 + 0.5 * e2, -4 * c13 * d2, 2 * c13 * d2 - 0.5 * e2, z],
+          [z, z, 2 * c13 * d3 + 1.5 * e3, -4 * c13 * d3 - 2.0 * e3, 2 * c13 * d3 + 0.5 * e3]]
+    diff = b[0] - b[2]
+    tau = np.abs(diff)
+    sg = np.sign(diff)
+    dtau = [sg * (db[0][n] - db[2][n]) for n in range(5)]
+    a = (float(F(0.1)), float(F(0.6)), float(F(0.3)))
+    wv, dw = [], []
+    for i in range(3):
+        den = b[i] + weps
+        with np.errstate(all="ignore"):
+            q = tau / den
+        live = q < 1.0e30
+        q = np.fmin(1.0e30, q)
+        wv.append(a[i] * (1.0 + q * q))
+        with np.errstate(all="ignore"):
+            dq = [np.where(live, dtau[n] / den - tau * db[i][n] / (den * den), 0.0) for n in range(5)]
+        dw.append([2.0 * a[i] * q * dq[n] for n in range(5)])
+    W = (wv[0] + wv[1]) + wv[2]
+    om = [x / W for x in wv]
+    p = Tape._parts(s1, s2, s3, s4, s5)
+    dp = [(float(C2_6), float(CM7_6), float(C11_6), 0.0, 0.0), (0.0, float(CM1_6), float(C5_6), float(C2_6), 0.0),
+          (0.0, 0.0, float(C2_6), float(C5_6), float(CM1_6))]
+    jac = []
+    for n in range(5):
+        dW = (dw[0][n] + dw[1][n]) + dw[2][n]
+        g = 0.0
+        for i in range(3):
+            g = g + om[i] * dp[i][n] + p[i] * ((dw[i][n] - om[i] * dW) / W)
+        jac.append(g)
+    return tuple(jac)
 
 
 class Params:
@@ -1098,7 +1177,7 @@ def scalar_adv_T(p, T, key, sc, mu, mv, mw, divx, weno, advb):
                 vals.append(p.c1[k - 1] * S(levs[0]) + p.c2[k - 1] * S(levs[1]))
             elif kind == "upd":
                 phi = T.coef[f"{key}.z{k}{tag}"]
-                cs = (-phi, 1.0 + phi, 0.0)
+                cs = T.upstrpd_coefs(f"{key}.z{k}{tag}")
                 vals.append(S(levs[1]) + phi * (S(levs[1]) - S(levs[0])))
             elif kind == "f3":
                 cs = FLX3

@@ -154,7 +154,9 @@ def test_scalar_advection_adjoint(weno):
 
     x = {"s": sc0, "mu": mu0, "mv": mv0, "mw": mw0, "divx": dx0}
     ybar = {"adv": rng.standard_normal(sc0.shape)}
-    _fd_check(fwd, adj, x, ybar, rng, names=["s", "mu", "mv", "divx"])
+    # WENO5's weights are differentiated: the map is smooth but strongly curved, so the step is small
+    _fd_check(fwd, adj, x, ybar, rng, eps=1e-8 if weno else 1e-6, tol=1e-7 if weno else 1e-8,
+              names=["s", "mu", "mv", "divx"])
 
 
 @pytest.mark.parametrize("nrk", [1, 2, 3])
@@ -215,7 +217,7 @@ def test_rk_stage_adjoint():
     x = {**{"X" + n: X[n] for n in X}, **{"A" + n: A[n] for n in A}, "ppx": ppx}
     y0 = fwd(x)
     ybar = {n: rng.standard_normal(y0[n].shape) for n in y0}
-    _fd_check(fwd, adj, x, ybar, rng, tol=1e-7)
+    _fd_check(fwd, adj, x, ybar, rng, eps=1e-8, tol=1e-6)
 
 
 def test_large_step_adjoint_and_storm_intensity_gradient():
@@ -229,7 +231,7 @@ def test_large_step_adjoint_and_storm_intensity_gradient():
         return step.forward(x, T)[0]
 
     ybar = {n: rng.standard_normal(out[n].shape) for n in FIELDS}
-    _fd_check(fwd, lambda yb: step.adjoint(T, yb), S, ybar, rng, tol=1e-7)
+    _fd_check(fwd, lambda yb: step.adjoint(T, yb), S, ybar, rng, eps=1e-8, tol=1e-6)
     # the loss and its gradient through the adjoint: dJ/d(th, u, v) against central differences
     J, wb = als.storm_intensity(p, out["w"])
     Sb = {n: np.zeros_like(out[n]) for n in FIELDS}
