@@ -103,7 +103,17 @@ def _require_flat_periodic(cfg, metrics) -> None:
 
 
 class DryPeriodicStep:
-    """Workspace and launches of the halo-free dry periodic step (one device, one stream)."""
+    """Workspaces and launches of the halo-free dry periodic step (one device, one stream).
+
+    Each RK stage has a workspace of its own, chained without copies: stage
+    2 reads stage 1's final substep planes as its input state, stage 3 reads
+    stage 2's, the three share the step-start state and ppx; on the way
+    back, stage 2 reads stage 3's input adjoints as its output adjoints.  A
+    large step's adjoint therefore reruns its three forward stages once and
+    keeps every stage's intermediates for the transposed pass.
+    """
+
+    SHARED = ("AU", "AV", "AW", "ATH", "API", "PPX", "COL", "B_AU", "B_AV", "B_AW", "B_ATH", "B_API", "B_PPX")
 
     def __init__(self, cfg, metrics, base, device="cuda", stream: Any | None = None, weno: str = "exact"):
         _require_flat_periodic(cfg, metrics)
@@ -119,24 +129,42 @@ class DryPeriodicStep:
         for nrk in (1, 2, 3):
             self.consts[nrk], self.nloops[nrk] = stage_consts(cfg.dt, cfg.nsound, nrk, cfg.dx, cfg.dy, cfg.dz,
                                                               weno)
-        self.nloop_max = max(self.nloops.values())
         ni, nj, nk = self.ni, self.nj, self.nk
         self.shape = (nk, nj, ni)
         self.wshape = (nk + 1, nj, ni)
-        t = {}
+        z = lambda shape: torch.zeros(shape, dtype=torch.float32, device=self.dev)  # noqa: E731
+        ws = {}
         with self._on_stream():
-            for s in SLOTS:
-                if s == "COL":
-                    t[s] = torch.from_numpy(pack_columns(metrics, base, nk, cfg.dz)).to(self.dev)
-                elif s in ("SU", "SV", "SPP", "SPPD"):
-                    t[s] = torch.zeros((self.nloop_max + 1,) + self.shape, dtype=torch.float32, device=self.dev)
-                elif s == "SW":
-                    t[s] = torch.zeros((self.nloop_max + 1,) + self.wshape, dtype=torch.float32, device=self.dev)
-                else:
-                    t[s] = torch.zeros(self.wshape if s in W_SLOTS else self.shape, dtype=torch.float32,
-                                       device=self.dev)
-        self.t = t
-        self.table = (ctypes.c_void_p * len(SLOTS))(*(t[s].data_ptr() for s in SLOTS))
+            shared = {"COL": torch.from_numpy(pack_columns(metrics, base, nk, cfg.dz)).to(self.dev)}
+            for sl in self.SHARED:
+                if sl != "COL":
+                    shared[sl] = z(self.wshape if sl in W_SLOTS else self.shape)
+            for nrk in (1, 2, 3):
+                nl = self.nloops[nrk]
+                t = dict(shared)
+                for sl in SLOTS:
+                    if sl in t:
+                        continue
+                    if nrk == 1 and sl in ("XU", "XV", "XW", "XTH", "XPP"):
+                        t[sl] = shared["A" + sl[1:]] if sl != "XPP" else shared["API"]
+                    elif nrk > 1 and sl in ("XU", "XV", "XW", "XTH", "XPP"):
+                        prev, pn = ws[nrk - 1], self.nloops[nrk - 1]
+                        t[sl] = {"XU": prev["SU"][pn], "XV": prev["SV"][pn], "XW": prev["SW"][pn],
+                                 "XTH": prev["THO"], "XPP": prev["SPP"][pn]}[sl]
+                    elif sl in ("SU", "SV", "SPP", "SPPD"):
+                        t[sl] = z((nl + 1,) + self.shape)
+                    elif sl == "SW":
+                        t[sl] = z((nl + 1,) + self.wshape)
+                    else:
+                        t[sl] = z(self.wshape if sl in W_SLOTS else self.shape)
+                ws[nrk] = t
+            for nrk in (2, 1):  # output adjoints of stage nrk are the input adjoints of stage nrk + 1
+                for o, x in (("B_OU", "B_XU"), ("B_OV", "B_XV"), ("B_OW", "B_XW"), ("B_OTH", "B_XTH"),
+                             ("B_OPP", "B_XPP")):
+                    ws[nrk][o] = ws[nrk + 1][x]
+        self.ws = ws
+        self.tables = {nrk: (ctypes.c_void_p * len(SLOTS))(*(ws[nrk][sl].data_ptr() for sl in SLOTS))
+                       for nrk in (1, 2, 3)}
 
     def _on_stream(self):
         return torch.cuda.stream(self.stream) if self.stream is not None else torch.cuda.stream(
@@ -145,57 +173,62 @@ class DryPeriodicStep:
     def _h(self):
         return stream_handle(self.stream)
 
-    def _load(self, pairs):
+    def _load(self, nrk, pairs):
         with self._on_stream():
             for slot, x in pairs:
-                self.t[slot].copy_(x)
+                dst = self.ws[nrk][slot]
+                if dst.data_ptr() != x.data_ptr():
+                    dst.copy_(x)
+
+    def _launch(self, name: str, nrk: int) -> None:
+        c = self.consts[nrk]
+        _native.call(name, ctypes.cast(self.tables[nrk], ctypes.c_void_p), c.ctypes.data_as(ctypes.c_void_p),
+                     self.ni, self.nj, self.nk, nrk, self.nloops[nrk], THETA_WEPS, self._h())
+
+    def _stage_out(self, nrk: int) -> tuple[dict, torch.Tensor]:
+        t, n = self.ws[nrk], self.nloops[nrk]
+        return ({"u": t["SU"][n], "v": t["SV"][n], "w": t["SW"][n], "th": t["THO"], "pp": t["SPP"][n]},
+                t["PPXO"] if nrk == 3 else t["PPX"])
 
     # ------------------------------------------------------------ forward
+    def load_start(self, S: dict) -> None:
+        """The step-start state (u v w th pp ppx), shared by the three stages."""
+        self._load(1, [("AU", S["u"]), ("AV", S["v"]), ("AW", S["w"]), ("ATH", S["th"]), ("API", S["pp"]),
+                       ("PPX", S["ppx"])])
+
     def forward_stage(self, nrk: int, X: dict, A: dict, ppx: torch.Tensor) -> tuple[dict, torch.Tensor]:
-        """One RK stage from X (u v w th pp), the step-start state A and ppx; returns new X and ppx.
-
-        At nrk = 3 the returned ppx already holds solve3's pp + ppx.  The
-        workspace keeps the stage's intermediates and substep states.
-        """
-        self._load([("XU", X["u"]), ("XV", X["v"]), ("XW", X["w"]), ("XTH", X["th"]), ("XPP", X["pp"]),
-                    ("AU", A["u"]), ("AV", A["v"]), ("AW", A["w"]), ("ATH", A["th"]), ("API", A["pp"]),
-                    ("PPX", ppx)])
-        self._launch_forward(nrk)
-        n = self.nloops[nrk]
-        t = self.t
+        """One RK stage from X (u v w th pp), the step-start state A and ppx; returns copies of the new X and
+        ppx (at nrk = 3 solve3's pp + ppx).  The stage's workspace keeps its intermediates."""
+        self._load(nrk, [("AU", A["u"]), ("AV", A["v"]), ("AW", A["w"]), ("ATH", A["th"]), ("API", A["pp"]),
+                         ("PPX", ppx)])
+        self._load(nrk, [("XU", X["u"]), ("XV", X["v"]), ("XW", X["w"]), ("XTH", X["th"]), ("XPP", X["pp"])])
+        self._launch("gm1hip_dry_stage_forward", nrk)
+        Xn, p = self._stage_out(nrk)
         with self._on_stream():
-            Xn = {"u": t["SU"][n].clone(), "v": t["SV"][n].clone(), "w": t["SW"][n].clone(),
-                  "th": t["THO"].clone(), "pp": t["SPP"][n].clone()}
-            ppx_out = t["PPXO"].clone() if nrk == 3 else ppx.clone()
-        return Xn, ppx_out
+            return {k: v.clone() for k, v in Xn.items()}, p.clone()
 
-    def _launch_forward(self, nrk: int) -> None:
-        c = self.consts[nrk]
-        _native.call("gm1hip_dry_stage_forward", ctypes.cast(self.table, ctypes.c_void_p),
-                     c.ctypes.data_as(ctypes.c_void_p), self.ni, self.nj, self.nk, nrk, self.nloops[nrk],
-                     THETA_WEPS, self._h())
+    def _forward_chain(self, S: dict) -> None:
+        self.load_start(S)
+        for nrk in (1, 2, 3):
+            self._launch("gm1hip_dry_stage_forward", nrk)
 
     def forward(self, S: dict) -> dict:
-        """One large step of the state S (u v w th pp ppx); returns the new state."""
-        A = {n: S[n] for n in ("u", "v", "w", "th", "pp")}
-        X, ppx = dict(A), S["ppx"]
-        for nrk in (1, 2, 3):
-            X, ppx = self.forward_stage(nrk, X, A, ppx)
-        return {**X, "ppx": ppx}
+        """One large step of the state S (u v w th pp ppx); returns the new state (fresh tensors)."""
+        self._forward_chain(S)
+        Xn, p = self._stage_out(3)
+        with self._on_stream():
+            return {**{k: v.clone() for k, v in Xn.items()}, "ppx": p.clone()}
 
     # ------------------------------------------------------------ adjoint
     def adjoint_stage(self, nrk: int, X: dict, A: dict, ppx: torch.Tensor, Xb: dict, ppxb: torch.Tensor) -> tuple[
             dict, torch.Tensor]:
         """Adjoints of one stage's inputs (X, ppx) from those of its outputs; the adjoint of A accumulates
-        in the workspace (``zero_start_adjoint`` / ``start_adjoint``)."""
+        (``zero_start_adjoint`` / ``start_adjoint``)."""
         self.forward_stage(nrk, X, A, ppx)
-        self._load([("B_OU", Xb["u"]), ("B_OV", Xb["v"]), ("B_OW", Xb["w"]), ("B_OTH", Xb["th"]),
-                    ("B_OPP", Xb["pp"]), ("B_PPX", ppxb)])
-        c = self.consts[nrk]
-        _native.call("gm1hip_dry_stage_adjoint", ctypes.cast(self.table, ctypes.c_void_p),
-                     c.ctypes.data_as(ctypes.c_void_p), self.ni, self.nj, self.nk, nrk, self.nloops[nrk],
-                     THETA_WEPS, self._h())
-        t = self.t
+        self._load(nrk, [("B_OU", Xb["u"]), ("B_OV", Xb["v"]), ("B_OW", Xb["w"]), ("B_OTH", Xb["th"]),
+                         ("B_OPP", Xb["pp"]), ("B_PPX", ppxb)])
+        self._launch("gm1hip_dry_stage_adjoint", nrk)
+        t = self.ws[nrk]
         with self._on_stream():
             out = {"u": t["B_XU"].clone(), "v": t["B_XV"].clone(), "w": t["B_XW"].clone(), "th": t["B_XTH"].clone(),
                    "pp": t["B_XPP"].clone()}
@@ -203,29 +236,28 @@ class DryPeriodicStep:
 
     def zero_start_adjoint(self) -> None:
         with self._on_stream():
-            for s in ("B_AU", "B_AV", "B_AW", "B_ATH", "B_API"):
-                self.t[s].zero_()
+            for sl in ("B_AU", "B_AV", "B_AW", "B_ATH", "B_API"):
+                self.ws[1][sl].zero_()
 
     def start_adjoint(self) -> dict:
-        t = self.t
+        t = self.ws[1]
         with self._on_stream():
             return {"u": t["B_AU"].clone(), "v": t["B_AV"].clone(), "w": t["B_AW"].clone(), "th": t["B_ATH"].clone(),
                     "pp": t["B_API"].clone()}
 
     def adjoint(self, S: dict, Sb: dict) -> dict:
         """Adjoint of the state before one large step (S) from the adjoint Sb of the state after it."""
-        A = {n: S[n] for n in ("u", "v", "w", "th", "pp")}
-        X2, ppx2 = self.forward_stage(1, A, A, S["ppx"])
-        X3, ppx3 = self.forward_stage(2, X2, A, ppx2)
+        self._forward_chain(S)
         self.zero_start_adjoint()
-        Xb = {n: Sb[n] for n in ("u", "v", "w", "th", "pp")}
-        ppxb = Sb["ppx"]
-        for nrk, X, ppx in ((3, X3, ppx3), (2, X2, ppx2), (1, A, S["ppx"])):
-            Xb, ppxb = self.adjoint_stage(nrk, X, A, ppx, Xb, ppxb)
-        Ab = self.start_adjoint()
+        self._load(3, [("B_OU", Sb["u"]), ("B_OV", Sb["v"]), ("B_OW", Sb["w"]), ("B_OTH", Sb["th"]),
+                       ("B_OPP", Sb["pp"]), ("B_PPX", Sb["ppx"])])
+        for nrk in (3, 2, 1):
+            self._launch("gm1hip_dry_stage_adjoint", nrk)
+        t = self.ws[1]
         with self._on_stream():
-            out = {n: Xb[n] + Ab[n] for n in Xb}
-        out["ppx"] = ppxb
+            out = {n: t["B_X" + k] + t["B_A" + a] for n, k, a in (("u", "U", "U"), ("v", "V", "V"), ("w", "W", "W"),
+                                                                  ("th", "TH", "TH"), ("pp", "PP", "PI"))}
+            out["ppx"] = t["B_PPX"].clone()
         return out
 
     def empty_state(self) -> dict:

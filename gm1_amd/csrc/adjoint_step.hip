@@ -600,57 +600,76 @@ __device__ __forceinline__ Row thomas_row(const W& w, int k, int j, int i) {
   return r;
 }
 
-__global__ __launch_bounds__(64) void k_ss_column(W w, int n) {
+// substep n, every cell: fpk
+__global__ __launch_bounds__(64) void k_ss_fpk(W w, int n) {
+  CELL_INDEX
+  w.p[FPK][w.at(l, j, i)] = column_fpk(w, n, l, j, i, nullptr);
+}
+
+// substep n, one column: the Thomas elimination of rows 2..nk and back substitution (w at levels 2..nk)
+__global__ __launch_bounds__(64) void k_ss_thomas(W w, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
   if (i >= w.ni) return;
   const Consts& c = w.c;
   const int nk = w.nk;
-  const bool last = n == w.nloop;
   const float* ws = w.wsnap(n - 1);
   float* wn = w.wsnap(n);
   const float* pp = w.snap(SPP, n - 1);
-  float *fpk = w.p[FPK], *ec = w.p[EC], *ed = w.p[ED];
-  for (int l = 1; l <= nk; ++l) fpk[w.at(l, j, i)] = column_fpk(w, n, l, j, i, nullptr);
-  // elimination of rows 2..nk (the top row solves w(nk))
+  const float* fpk = w.p[FPK];
+  float *ec = w.p[EC], *ed = w.p[ED];
+  float ecd = 0.0f, edd = 0.0f, ppd_ = pp[w.at(1, j, i)], fpd = fpk[w.at(1, j, i)];
   for (int k = 2; k <= nk; ++k) {
-    const int64_t q = w.at(k, j, i), qd = w.at(k - 1, j, i);
+    const int64_t q = w.at(k, j, i);
     const Row r = thomas_row(w, k, j, i);
     const float mmk = w.p[MM][q];
-    const float pdelta = pp[q] - pp[qd];
+    const float ppk = pp[q], fpkk = fpk[q];
+    const float pdelta = ppk - ppd_;
     const float fwk = c.dts * w.p[WTEN][q] + c.beta * ((-mmk) * pdelta);
-    const float udelta = (pp[q] + fpk[q]) - (pp[qd] + fpk[qd]);
+    const float udelta = (ppk + fpkk) - (ppd_ + fpd);
     const float dd = (ws[q] + fwk) + c.alph * ((-mmk) * udelta);
-    const float pivot = k == 2 ? r.bb : r.aa * ec[qd] + r.bb;
+    const float pivot = k == 2 ? r.bb : r.aa * ecd + r.bb;
     const float inv = 1.0f / pivot;
     if (k < nk) {
-      ec[q] = (-r.cc) * inv;
-      ed[q] = (k == 2 ? dd : dd - r.aa * ed[qd]) * inv;
+      ecd = (-r.cc) * inv;
+      edd = (k == 2 ? dd : dd - r.aa * edd) * inv;
+      ec[q] = ecd;
+      ed[q] = edd;
     } else {
-      wn[q] = (nk == 2 ? dd : dd - r.aa * ed[qd]) * inv;
+      wn[q] = (dd - r.aa * edd) * inv;
     }
+    ppd_ = ppk;
+    fpd = fpkk;
   }
+  float wnext = wn[w.at(nk, j, i)];
   for (int k = nk - 1; k >= 2; --k) {
     const int64_t q = w.at(k, j, i);
-    wn[q] = ec[q] * wn[w.at(k + 1, j, i)] + ed[q];
+    wnext = ec[q] * wnext + ed[q];
+    wn[q] = wnext;
   }
   wn[w.at(1, j, i)] = 0.0f;
   wn[w.at(nk + 1, j, i)] = ws[w.at(nk + 1, j, i)];
-  float* ppo = w.snap(SPP, n);
-  float* ppdo = w.snap(SPPD, n);
-  for (int l = 1; l <= nk; ++l) {
-    const int64_t q = w.at(l, j, i);
-    const float wlo = l == 1 ? 0.0f : wn[q];
-    const float whi = l == nk ? 0.0f : wn[w.at(l + 1, j, i)];
-    const float vt = wlo * (w.p[PK1][q] + w.p[QK][q]) + whi * (w.p[PK2][q] - w.p[QK][q]);
-    float up = (pp[q] + fpk[q]) + c.alph * vt;
-    up = fabsf(up) < 1.0e-20f ? 0.0f : up;
-    const float tend = c.kdiv * (up - pp[q]);
-    ppo[q] = up;
-    ppdo[q] = up + tend;
-    if (w.nrk == 3 && last) w.p[PPXO][q] = tend;
-    float* arw = w.p[ARW];
-    arw[q] = last ? (arw[q] + wn[q] * c.alph) * c.tavg : arw[q] + wn[q];
-  }
+}
+
+// substep n, every cell: pressure update, filter, ppx, w running sum
+__global__ __launch_bounds__(64) void k_ss_update(W w, int n) {
+  CELL_INDEX
+  const Consts& c = w.c;
+  const int nk = w.nk;
+  const bool last = n == w.nloop;
+  const float* wn = w.wsnap(n);
+  const float* pp = w.snap(SPP, n - 1);
+  const int64_t q = w.at(l, j, i);
+  const float wlo = l == 1 ? 0.0f : wn[q];
+  const float whi = l == nk ? 0.0f : wn[w.at(l + 1, j, i)];
+  const float vt = wlo * (w.p[PK1][q] + w.p[QK][q]) + whi * (w.p[PK2][q] - w.p[QK][q]);
+  float up = (pp[q] + w.p[FPK][q]) + c.alph * vt;
+  up = fabsf(up) < 1.0e-20f ? 0.0f : up;
+  const float tend = c.kdiv * (up - pp[q]);
+  w.snap(SPP, n)[q] = up;
+  w.snap(SPPD, n)[q] = up + tend;
+  if (w.nrk == 3 && last) w.p[PPXO][q] = tend;
+  float* arw = w.p[ARW];
+  arw[q] = last ? (arw[q] + wn[q] * c.alph) * c.tavg : arw[q] + wn[q];
 }
 
 // ------------------------------------------------------------------ post-acoustic and theta (forward)
@@ -868,130 +887,165 @@ __global__ __launch_bounds__(64) void k_b_ac_init(W w) {
   w.p[B_PPD][q] = 0.0f;
 }
 
-__global__ __launch_bounds__(64) void k_b_ss_column(W w, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-  if (i >= w.ni) return;
+// The column adjoint of substep n in three passes.  A (every cell): forward fpk and div recomputed, the w
+// averages, the pressure update's adjoint (fpk's and vt's adjoints kept) and the w adjoint of vt gathered.
+__global__ __launch_bounds__(64) void k_b_ssA(W w, int n) {
+  CELL_INDEX
   const Consts& c = w.c;
   const int nk = w.nk;
   const bool last = n == w.nloop;
   const bool last3 = last && w.nrk == 3;
-  const float* ws = w.wsnap(n - 1);
   const float* wn = w.wsnap(n);
-  const float* pp = w.snap(SPP, n - 1);
-  const float *qk = w.p[QK], *pk1 = w.p[PK1], *pk2 = w.p[PK2], *mm = w.p[MM];
-  float *fpk = w.p[FPK], *fpkb = w.p[B_FPKF], *bw = w.p[B_W], *bpp = w.p[B_PP];
-  float *bqk = w.p[B_QK], *bpk1 = w.p[B_PK1], *bpk2 = w.p[B_PK2], *bmm = w.p[B_MM];
+  const int64_t q = w.at(l, j, i);
+  float dv;
+  w.p[FPK][q] = column_fpk(w, n, l, j, i, &dv);
+  w.p[DIV][q] = dv;
+  float bw = w.p[B_W][q];
+  if (last) {
+    bw += (w.p[B_ARW][q] * c.tavg) * c.alph;
+    w.p[B_ARW][q] *= c.tavg;
+  } else {
+    bw += w.p[B_ARW][q];
+  }
+  w.p[B_W][q] = bw;
+  float upb = w.p[B_PP][q] + (1.0f + c.kdiv) * w.p[B_PPD][q];
+  float ppb = -c.kdiv * w.p[B_PPD][q];
+  if (last3) {
+    upb = upb + c.kdiv * w.p[B_PPX][q];
+    ppb = ppb - c.kdiv * w.p[B_PPX][q];
+    w.p[B_PPX][q] = 0.0f;
+  }
+  w.p[B_PP][q] = ppb + upb;
+  w.p[B_FPKF][q] = upb;
+  const float vtb = c.alph * upb;
+  w.p[B_XA][q] = vtb;
+  const float wlo = l == 1 ? 0.0f : wn[q];
+  const float whi = l == nk ? 0.0f : wn[w.at(l + 1, j, i)];
+  w.p[B_PK1][q] += vtb * wlo;
+  w.p[B_QK][q] += vtb * (wlo - whi);
+  w.p[B_PK2][q] += vtb * whi;
+}
+
+// A2 (levels 2..nk): the w adjoint of vt from levels l (w(k)) and l-1 (w(k+1))
+__global__ __launch_bounds__(64) void k_b_ssA2(W w) {
+  CELL_INDEX
+  if (l < 2) return;
+  const float *qk = w.p[QK], *pk1 = w.p[PK1], *pk2 = w.p[PK2], *vtb = w.p[B_XA];
+  const int64_t q = w.at(l, j, i), qd = w.at(l - 1, j, i);
+  w.p[B_W][q] += vtb[q] * (pk1[q] + qk[q]) + vtb[qd] * (pk2[qd] - qk[qd]);
+}
+
+// B (one column): the transposed Thomas solve T^T lam = wbar(2..nk); lam is left in B_ED
+__global__ __launch_bounds__(64) void k_b_ssB(W w) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= w.ni) return;
+  const int nk = w.nk;
   float *bec = w.p[B_EC], *bed = w.p[B_ED];
-  float* divs = w.p[DIV];  // forward div of this substep, recomputed (DIV is free after the post pass)
-  for (int l = 1; l <= nk; ++l) {
-    float dv;
-    fpk[w.at(l, j, i)] = column_fpk(w, n, l, j, i, &dv);
-    divs[w.at(l, j, i)] = dv;
-  }
-  // averages of w
-  for (int l = 1; l <= nk; ++l) {
-    const int64_t q = w.at(l, j, i);
-    if (last) {
-      bw[q] += (w.p[B_ARW][q] * c.tavg) * c.alph;
-      w.p[B_ARW][q] *= c.tavg;
-    } else {
-      bw[q] += w.p[B_ARW][q];
-    }
-  }
-  // pressure update and filter
-  for (int l = 1; l <= nk; ++l) {
-    const int64_t q = w.at(l, j, i);
-    float upb = bpp[q] + (1.0f + c.kdiv) * w.p[B_PPD][q];
-    float ppb = -c.kdiv * w.p[B_PPD][q];
-    if (last3) {
-      upb = upb + c.kdiv * w.p[B_PPX][q];
-      ppb = ppb - c.kdiv * w.p[B_PPX][q];
-      w.p[B_PPX][q] = 0.0f;
-    }
-    bpp[q] = ppb + upb;
-    fpkb[q] = upb;
-    const float vtb = c.alph * upb;
-    const float wlo = l == 1 ? 0.0f : wn[q];
-    const float whi = l == nk ? 0.0f : wn[w.at(l + 1, j, i)];
-    bpk1[q] += vtb * wlo;
-    bqk[q] += vtb * (wlo - whi);
-    bpk2[q] += vtb * whi;
-    if (l >= 2) bw[q] += vtb * (pk1[q] + qk[q]);
-    if (l <= nk - 1) bw[w.at(l + 1, j, i)] += vtb * (pk2[q] - qk[q]);
-  }
-  // transposed Thomas solve: T^T lam = wbar(2..nk); row k of T^T has cc(k-1) below and aa(k+1) above
+  const float* bw = w.p[B_W];
   float cprev = 0.0f, dprev = 0.0f;
+  Row here = thomas_row(w, 2, j, i);
+  float cc_below = 0.0f;  // cc of row k-1
   for (int k = 2; k <= nk; ++k) {
     const int64_t q = w.at(k, j, i);
-    const float lo = k == 2 ? 0.0f : thomas_row(w, k - 1, j, i).cc;
-    const float up = k < nk ? thomas_row(w, k + 1, j, i).aa : 0.0f;
-    const float piv = thomas_row(w, k, j, i).bb - lo * cprev;
+    const Row next = k < nk ? thomas_row(w, k + 1, j, i) : Row{0.0f, 0.0f, 0.0f};
+    const float lo = k == 2 ? 0.0f : cc_below;
+    const float up = k < nk ? next.aa : 0.0f;
+    const float piv = here.bb - lo * cprev;
     const float cpk = up / piv;
     const float dpk = (bw[q] - lo * dprev) / piv;
     bec[q] = cpk;
     bed[q] = dpk;
     cprev = cpk;
     dprev = dpk;
+    cc_below = here.cc;
+    here = next;
   }
-  // start-w adjoint: level 1 gets nothing, nk+1 passes through, 2..nk from the solve
   float lamnext = 0.0f;
   for (int k = nk; k >= 2; --k) {
-    const int64_t q = w.at(k, j, i), qd = w.at(k - 1, j, i);
+    const int64_t q = w.at(k, j, i);
     const float lam = k == nk ? bed[q] : bed[q] - bec[q] * lamnext;
+    bed[q] = lam;
     lamnext = lam;
-    const float mmk = mm[q];
-    const float pdelta = pp[q] - pp[qd];
-    const float udelta = (pp[q] + fpk[q]) - (pp[qd] + fpk[qd]);
-    bw[q] = lam;  // stored after the row's rhs was read in the forward sweep
-    w.p[B_WTEN][q] += c.dts * lam;
-    bmm[q] += (-(c.beta * pdelta + c.alph * udelta)) * lam;
-    const float pdb = c.beta * (-mmk) * lam;
-    const float udb = c.alph * (-mmk) * lam;
-    bpp[q] += pdb + udb;
-    bpp[qd] -= pdb + udb;
-    fpkb[q] += udb;
-    fpkb[qd] -= udb;
-    const float bbb = -lam * wn[q];
+  }
+}
+
+// C (every cell, w levels 1..nk+1): adjoints of the Thomas rows' right-hand sides and coefficients, of the
+// pressure advection and fpk's w terms; the start-w adjoint, fpk's adjoint as divergence and advection parts
+__global__ __launch_bounds__(64) void k_b_ssC(W w, int n) {
+  CELL_INDEX
+  const Consts& c = w.c;
+  const int nk = w.nk;
+  const int64_t q = w.at(l, j, i);
+  if (l == nk + 1) return;  // the lid level's adjoint passes through (pk2(nk) = 0)
+  const float* ws = w.wsnap(n - 1);
+  const float* wn = w.wsnap(n);
+  const float* pp = w.snap(SPP, n - 1);
+  const float *qk = w.p[QK], *pk1 = w.p[PK1], *pk2 = w.p[PK2], *mm = w.p[MM], *fpk = w.p[FPK];
+  const float* lam = w.p[B_ED];
+  auto LAM = [&](int k) { return (k >= 2 && k <= nk) ? lam[w.at(k, j, i)] : 0.0f; };
+  auto PP = [&](int k) { return pp[w.at(k, j, i)]; };
+  auto FP = [&](int k) { return fpk[w.at(k, j, i)]; };
+  // udb(k), pdb(k) of row k (2..nk)
+  auto udb = [&](int k) { return (k >= 2 && k <= nk) ? c.alph * (-mm[w.at(k, j, i)]) * LAM(k) : 0.0f; };
+  auto pdb = [&](int k) { return (k >= 2 && k <= nk) ? c.beta * (-mm[w.at(k, j, i)]) * LAM(k) : 0.0f; };
+  // fpk's full adjoint at level k (1..nk)
+  auto F = [&](int k) { return w.p[B_FPKF][w.at(k, j, i)] + udb(k) - udb(k + 1); };
+  // bb's and aa's, cc's adjoints of row k
+  auto diagb = [&](int k) {
+    if (k < 2 || k > nk) return 0.0f;
+    const int64_t qq = w.at(k, j, i);
+    return (-c.asq * mm[qq]) * (-LAM(k) * wn[qq]);
+  };
+  auto aab = [&](int k) { return (k >= 3 && k <= nk) ? -LAM(k) * wn[w.at(k - 1, j, i)] : 0.0f; };
+  auto ccb = [&](int k) { return (k >= 2 && k < nk) ? -LAM(k) * wn[w.at(k + 1, j, i)] : 0.0f; };
+  const float f = F(l);
+  // start-w adjoint of level l
+  float bw = LAM(l);
+  if (l >= 2) bw += c.beta * (f * pk1[q]) + qk[q] * (c.beta * f);
+  if (l >= 2) {
+    const int64_t qd = w.at(l - 1, j, i);
+    const float fd = F(l - 1);
+    bw += c.beta * (fd * pk2[qd]);
+    if (l - 1 <= nk - 1) bw += -qk[qd] * (c.beta * fd);
+  }
+  w.p[B_W][q] = l == 1 ? 0.0f : bw;
+  if (l >= 2) {
+    const float lk = LAM(l);
+    w.p[B_WTEN][q] += c.dts * lk;
+    const int64_t qd = w.at(l - 1, j, i);
+    const float pdelta = PP(l) - PP(l - 1);
+    const float udelta = (PP(l) + FP(l)) - (PP(l - 1) + FP(l - 1));
+    float mmb = (-(c.beta * pdelta + c.alph * udelta)) * lk;
     const float diag = (-(pk1[q] + qk[q])) + (pk2[qd] - qk[qd]);
-    bmm[q] += (-c.asq * diag) * bbb;
-    const float diagb = (-c.asq * mmk) * bbb;
-    bpk1[q] -= diagb;
-    bqk[q] -= diagb;
-    bpk2[qd] += diagb;
-    bqk[qd] -= diagb;
-    if (k >= 3) {
-      const float aab = -lam * wn[qd];
-      bmm[q] += (-c.asq) * (pk1[qd] + qk[qd]) * aab;
-      bpk1[qd] += (-c.asq) * mmk * aab;
-      bqk[qd] += (-c.asq) * mmk * aab;
-    }
-    if (k < nk) {
-      const float ccb = -lam * wn[w.at(k + 1, j, i)];
-      bmm[q] += c.asq * (pk2[q] - qk[q]) * ccb;
-      bpk2[q] += c.asq * mmk * ccb;
-      bqk[q] -= c.asq * mmk * ccb;
-    }
+    mmb += (-c.asq * diag) * (-lk * wn[q]);
+    if (l >= 3) mmb += (-c.asq) * (pk1[qd] + qk[qd]) * aab(l);
+    if (l < nk) mmb += c.asq * (pk2[q] - qk[q]) * ccb(l);
+    w.p[B_MM][q] += mmb;
   }
-  bw[w.at(1, j, i)] = 0.0f;
-  // pressure advection (w and coefficient parts) and fpk's w term
-  for (int l = 1; l <= nk; ++l) {
-    const int64_t q = w.at(l, j, i);
-    const float f = fpkb[q];
-    const float wlo = l == 1 ? 0.0f : ws[q];
-    const float wup = ws[w.at(l + 1, j, i)];
-    const float whi = l == nk ? 0.0f : wup;
-    bpk1[q] += f * (c.beta * wlo);
-    bpk2[q] += f * (c.beta * wup);
-    if (l >= 2) bw[q] += c.beta * (f * pk1[q]);
-    bw[w.at(l + 1, j, i)] += c.beta * (f * pk2[q]);
-    const float qpb = c.beta * f;
-    bqk[q] -= qpb * (whi - wlo);
-    if (l <= nk - 1) bw[w.at(l + 1, j, i)] += -qk[q] * qpb;
-    if (l >= 2) bw[q] += qk[q] * qpb;
-    w.p[B_PPTERM][q] -= c.dts * (divs[q] * f);
-    w.p[B_DIV][q] = -c.dts * (w.p[PPTERM][q] * f);
-    w.p[B_XA][q] = -(f * w.col.rr0[l - 1]);
+  // pressure adjoint from rows l and l+1
+  w.p[B_PP][q] += (pdb(l) + udb(l)) - (pdb(l + 1) + udb(l + 1));
+  // coefficient adjoints of level l
+  const float wlo = l == 1 ? 0.0f : ws[q];
+  const float wup = ws[w.at(l + 1, j, i)];
+  const float whi = l == nk ? 0.0f : wup;
+  float bpk1 = -diagb(l) + f * (c.beta * wlo);
+  float bqk = -diagb(l) - diagb(l + 1) - (c.beta * f) * (whi - wlo);
+  float bpk2 = diagb(l + 1) + f * (c.beta * wup);
+  if (l + 1 <= nk && l + 1 >= 3) {
+    const float m1 = mm[w.at(l + 1, j, i)];
+    bpk1 += (-c.asq) * m1 * aab(l + 1);
+    bqk += (-c.asq) * m1 * aab(l + 1);
   }
+  if (l < nk && l >= 2) {
+    bpk2 += c.asq * mm[q] * ccb(l);
+    bqk -= c.asq * mm[q] * ccb(l);
+  }
+  w.p[B_PK1][q] += bpk1;
+  w.p[B_QK][q] += bqk;
+  w.p[B_PK2][q] += bpk2;
+  w.p[B_PPTERM][q] -= c.dts * (w.p[DIV][q] * f);
+  w.p[B_DIV][q] = -c.dts * (w.p[PPTERM][q] * f);
+  w.p[B_XA][q] = -(f * w.col.rr0[l - 1]);
 }
 
 // u and v faces of substep n: averages, pressure advection and divergence adjoints, forcing
@@ -1404,7 +1458,9 @@ int gm1hip_dry_stage_forward(void* const* table, const float* consts, int ni, in
   GM1_LAUNCH_CHECK();
   for (int n = 1; n <= nloop; ++n) {
     hipLaunchKernelGGL(k_ss_pgrad, g, b, 0, s, w, n);
-    hipLaunchKernelGGL(k_ss_column, gc, b, 0, s, w, n);
+    hipLaunchKernelGGL(k_ss_fpk, g, b, 0, s, w, n);
+    hipLaunchKernelGGL(k_ss_thomas, gc, b, 0, s, w, n);
+    hipLaunchKernelGGL(k_ss_update, g, b, 0, s, w, n);
   }
   hipLaunchKernelGGL(k_post, g, b, 0, s, w);
   hipLaunchKernelGGL(k_theta, g, b, 0, s, w);
@@ -1437,7 +1493,10 @@ int gm1hip_dry_stage_adjoint(void* const* table, const float* consts, int ni, in
   hipLaunchKernelGGL(k_b_ac_init, gw, b, 0, s, w);
   GM1_LAUNCH_CHECK();
   for (int n = nloop; n >= 1; --n) {
-    hipLaunchKernelGGL(k_b_ss_column, gc, b, 0, s, w, n);
+    hipLaunchKernelGGL(k_b_ssA, g, b, 0, s, w, n);
+    hipLaunchKernelGGL(k_b_ssA2, g, b, 0, s, w);
+    hipLaunchKernelGGL(k_b_ssB, gc, b, 0, s, w);
+    hipLaunchKernelGGL(k_b_ssC, g, b, 0, s, w, n);
     hipLaunchKernelGGL(k_b_ss_faces, g, b, 0, s, w, n);
     hipLaunchKernelGGL(k_b_ss_cells, g, b, 0, s, w, n);
   }
