@@ -13,6 +13,8 @@
 // adjoint stage is its transpose at that trajectory with the upwind choices,
 // WENO5 weights and positive-definite limiters frozen at their forward
 // values and the smeps clamps taken as the identity.
+#include <algorithm>
+
 #include "cm1_common.h"
 
 namespace {
@@ -44,6 +46,7 @@ enum Slot {
   B_RRU, B_RRV, B_RRW, B_DIVX, B_UTEN, B_VTEN, B_WTEN,
   B_QK, B_PK1, B_PK2, B_MM, B_U, B_V, B_W, B_PP, B_PPD, B_ARU, B_ARV, B_ARW,
   B_DIV, B_XA, B_TOT, B_MU, B_MV, B_MW, B_DIVX2, B_SC, B_EC, B_ED, B_FPKF,
+  RA, RB, RC, RD,                         // tridiagonal rows of the stage (aa, bb, cc) and a substep's right side
   NSLOT
 };
 
@@ -507,6 +510,9 @@ __global__ __launch_bounds__(64) void k_coef(W w) {
   w.p[MM][q] = mm;
 }
 
+// rows k = 2..nk of the stage's tridiagonal systems (they depend on the coefficients only)
+__global__ __launch_bounds__(64) void k_rows(W w);
+
 __global__ __launch_bounds__(64) void k_ac_start(W w) {
   CELL_INDEX
   const int64_t q = w.at(l, j, i);
@@ -601,6 +607,32 @@ __device__ __forceinline__ Row thomas_row(const W& w, int k, int j, int i) {
 }
 
 // substep n, every cell: fpk
+__global__ __launch_bounds__(64) void k_rows(W w) {
+  CELL_INDEX
+  if (l < 2) return;
+  const Row r = thomas_row(w, l, j, i);
+  const int64_t q = w.at(l, j, i);
+  w.p[RA][q] = r.aa;
+  w.p[RB][q] = r.bb;
+  w.p[RC][q] = r.cc;
+}
+
+// substep n, levels 2..nk: the right side of the tridiagonal system (fpk of levels l and l-1 from FPK)
+__global__ __launch_bounds__(64) void k_ss_rhs(W w, int n) {
+  CELL_INDEX
+  if (l < 2) return;
+  const Consts& c = w.c;
+  const float* ws = w.wsnap(n - 1);
+  const float* pp = w.snap(SPP, n - 1);
+  const float* fpk = w.p[FPK];
+  const int64_t q = w.at(l, j, i), qd = w.at(l - 1, j, i);
+  const float mmk = w.p[MM][q];
+  const float pdelta = pp[q] - pp[qd];
+  const float fwk = c.dts * w.p[WTEN][q] + c.beta * ((-mmk) * pdelta);
+  const float udelta = (pp[q] + fpk[q]) - (pp[qd] + fpk[qd]);
+  w.p[RD][q] = (ws[q] + fwk) + c.alph * ((-mmk) * udelta);
+}
+
 __global__ __launch_bounds__(64) void k_ss_fpk(W w, int n) {
   CELL_INDEX
   w.p[FPK][w.at(l, j, i)] = column_fpk(w, n, l, j, i, nullptr);
@@ -610,37 +642,29 @@ __global__ __launch_bounds__(64) void k_ss_fpk(W w, int n) {
 __global__ __launch_bounds__(64) void k_ss_thomas(W w, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
   if (i >= w.ni) return;
-  const Consts& c = w.c;
   const int nk = w.nk;
   const float* ws = w.wsnap(n - 1);
   float* wn = w.wsnap(n);
-  const float* pp = w.snap(SPP, n - 1);
-  const float* fpk = w.p[FPK];
+  const float *ra = w.p[RA], *rb = w.p[RB], *rc = w.p[RC], *rd = w.p[RD];
   float *ec = w.p[EC], *ed = w.p[ED];
-  float ecd = 0.0f, edd = 0.0f, ppd_ = pp[w.at(1, j, i)], fpd = fpk[w.at(1, j, i)];
+  float ecd = 0.0f, edd = 0.0f;
+#pragma unroll 4
   for (int k = 2; k <= nk; ++k) {
     const int64_t q = w.at(k, j, i);
-    const Row r = thomas_row(w, k, j, i);
-    const float mmk = w.p[MM][q];
-    const float ppk = pp[q], fpkk = fpk[q];
-    const float pdelta = ppk - ppd_;
-    const float fwk = c.dts * w.p[WTEN][q] + c.beta * ((-mmk) * pdelta);
-    const float udelta = (ppk + fpkk) - (ppd_ + fpd);
-    const float dd = (ws[q] + fwk) + c.alph * ((-mmk) * udelta);
-    const float pivot = k == 2 ? r.bb : r.aa * ecd + r.bb;
+    const float aa = ra[q], bb = rb[q], cc = rc[q], dd = rd[q];
+    const float pivot = k == 2 ? bb : aa * ecd + bb;
     const float inv = 1.0f / pivot;
     if (k < nk) {
-      ecd = (-r.cc) * inv;
-      edd = (k == 2 ? dd : dd - r.aa * edd) * inv;
+      ecd = (-cc) * inv;
+      edd = (k == 2 ? dd : dd - aa * edd) * inv;
       ec[q] = ecd;
       ed[q] = edd;
     } else {
-      wn[q] = (dd - r.aa * edd) * inv;
+      wn[q] = (dd - aa * edd) * inv;
     }
-    ppd_ = ppk;
-    fpd = fpkk;
   }
   float wnext = wn[w.at(nk, j, i)];
+#pragma unroll 4
   for (int k = nk - 1; k >= 2; --k) {
     const int64_t q = w.at(k, j, i);
     wnext = ec[q] * wnext + ed[q];
@@ -766,9 +790,17 @@ __global__ __launch_bounds__(64) void k_solve3(W w) {
 
 __device__ __forceinline__ void aadd(float* p, float v) { atomicAdd(p, v); }
 
-__global__ __launch_bounds__(256) void k_b_fill(float* p, int64_t n) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) p[t] = 0.0f;
+struct FillList {
+  float* p[24];
+  int64_t n[24];
+  int count;
+};
+// zero every listed array (blockIdx.y selects the array)
+__global__ __launch_bounds__(256) void k_b_fill(FillList f) {
+  const int a = blockIdx.y;
+  if (a >= f.count) return;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < f.n[a]; t += (int64_t)gridDim.x * blockDim.x)
+    f.p[a][t] = 0.0f;
 }
 
 // theta compose and the cell parts of the scalar tendency
@@ -941,26 +973,23 @@ __global__ __launch_bounds__(64) void k_b_ssB(W w) {
   if (i >= w.ni) return;
   const int nk = w.nk;
   float *bec = w.p[B_EC], *bed = w.p[B_ED];
-  const float* bw = w.p[B_W];
+  const float *bw = w.p[B_W], *ra = w.p[RA], *rb = w.p[RB], *rc = w.p[RC];
   float cprev = 0.0f, dprev = 0.0f;
-  Row here = thomas_row(w, 2, j, i);
-  float cc_below = 0.0f;  // cc of row k-1
+#pragma unroll 4
   for (int k = 2; k <= nk; ++k) {
     const int64_t q = w.at(k, j, i);
-    const Row next = k < nk ? thomas_row(w, k + 1, j, i) : Row{0.0f, 0.0f, 0.0f};
-    const float lo = k == 2 ? 0.0f : cc_below;
-    const float up = k < nk ? next.aa : 0.0f;
-    const float piv = here.bb - lo * cprev;
+    const float lo = k == 2 ? 0.0f : rc[w.at(k - 1, j, i)];
+    const float up = k < nk ? ra[w.at(k + 1, j, i)] : 0.0f;
+    const float piv = rb[q] - lo * cprev;
     const float cpk = up / piv;
     const float dpk = (bw[q] - lo * dprev) / piv;
     bec[q] = cpk;
     bed[q] = dpk;
     cprev = cpk;
     dprev = dpk;
-    cc_below = here.cc;
-    here = next;
   }
   float lamnext = 0.0f;
+#pragma unroll 4
   for (int k = nk; k >= 2; --k) {
     const int64_t q = w.at(k, j, i);
     const float lam = k == nk ? bed[q] : bed[q] - bec[q] * lamnext;
@@ -1125,31 +1154,27 @@ __global__ __launch_bounds__(64) void k_b_ss_cells(W w, int n) {
   w.p[B_PIADV][q] += pb;
 }
 
-// coefficient adjoints -> thv, ppterm, piadv (one column)
+// coefficient adjoints -> thv, ppterm, piadv, gathered per cell from the coefficients that read it
 __global__ __launch_bounds__(64) void k_b_coef(W w) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-  if (i >= w.ni) return;
+  CELL_INDEX
   const Consts& c = w.c;
   const int nk = w.nk;
-  for (int l = 1; l <= nk; ++l) {
-    const int64_t q = w.at(l, j, i);
-    w.p[B_PPTERM][q] += (c.dts * w.col.rdsf[l]) * w.p[B_QK][q];
-    const float tem2 = ((c.dts * 1.0f) * w.col.rdsf[l]) * w.col.rr0[l - 1];
-    if (l < nk) {
-      const float b2b = (tem2 * w.col.rf0[l]) * w.p[B_PK2][q];
-      w.p[B_PIADV][w.at(l + 1, j, i)] += (-w.col.c2[l]) * b2b;
-      w.p[B_PIADV][q] += (1.0f - w.col.c1[l]) * b2b;
-    }
-    if (l > 1) {
-      const int64_t qd = w.at(l - 1, j, i);
-      const float b1b = (tem2 * w.col.rf0[l - 1]) * w.p[B_PK1][q];
-      w.p[B_PIADV][qd] += w.col.c1[l - 1] * b1b;
-      w.p[B_PIADV][q] += (w.col.c2[l - 1] - 1.0f) * b1b;
-      const float thb = ((c.dts * w.col.rds[l]) * c.cp) * w.p[B_MM][q];
-      w.p[B_THV][q] += w.col.c2[l - 1] * thb;
-      w.p[B_THV][qd] += w.col.c1[l - 1] * thb;
-    }
-  }
+  const int64_t q = w.at(l, j, i);
+  auto tem2 = [&](int k) { return ((c.dts * 1.0f) * w.col.rdsf[k]) * w.col.rr0[k - 1]; };
+  auto b2b = [&](int k) { return (tem2(k) * w.col.rf0[k]) * w.p[B_PK2][w.at(k, j, i)]; };      // k < nk
+  auto b1b = [&](int k) { return (tem2(k) * w.col.rf0[k - 1]) * w.p[B_PK1][w.at(k, j, i)]; };  // k > 1
+  auto thb = [&](int k) { return ((c.dts * w.col.rds[k]) * c.cp) * w.p[B_MM][w.at(k, j, i)]; };  // k >= 2
+  w.p[B_PPTERM][q] += (c.dts * w.col.rdsf[l]) * w.p[B_QK][q];
+  float pia = 0.0f;
+  if (l < nk) pia += (1.0f - w.col.c1[l]) * b2b(l);
+  if (l >= 2 && l - 1 < nk) pia += (-w.col.c2[l - 1]) * b2b(l - 1);
+  if (l > 1) pia += (w.col.c2[l - 1] - 1.0f) * b1b(l);
+  if (l + 1 <= nk) pia += w.col.c1[l] * b1b(l + 1);
+  w.p[B_PIADV][q] += pia;
+  float tv = 0.0f;
+  if (l >= 2) tv += w.col.c2[l - 1] * thb(l);
+  if (l + 1 <= nk) tv += w.col.c1[l] * thb(l + 1);
+  w.p[B_THV][q] += tv;
 }
 
 // stage start transposed (grid level l = 1..nk+1)
@@ -1454,11 +1479,13 @@ int gm1hip_dry_stage_forward(void* const* table, const float* consts, int ni, in
   hipLaunchKernelGGL(k_advv, g, b, 0, s, w);
   hipLaunchKernelGGL(k_advw, gw, b, 0, s, w);
   hipLaunchKernelGGL(k_coef, g, b, 0, s, w);
+  hipLaunchKernelGGL(k_rows, g, b, 0, s, w);
   hipLaunchKernelGGL(k_ac_start, g, b, 0, s, w);
   GM1_LAUNCH_CHECK();
   for (int n = 1; n <= nloop; ++n) {
     hipLaunchKernelGGL(k_ss_pgrad, g, b, 0, s, w, n);
     hipLaunchKernelGGL(k_ss_fpk, g, b, 0, s, w, n);
+    hipLaunchKernelGGL(k_ss_rhs, g, b, 0, s, w, n);
     hipLaunchKernelGGL(k_ss_thomas, gc, b, 0, s, w, n);
     hipLaunchKernelGGL(k_ss_update, g, b, 0, s, w, n);
   }
@@ -1479,8 +1506,18 @@ int gm1hip_dry_stage_adjoint(void* const* table, const float* consts, int ni, in
   const int zero_cell[] = {B_XU, B_XV, B_XTH, B_XPP, B_THV, B_PPTERM, B_PIADV, B_RRU, B_RRV, B_DIVX, B_UTEN,
                            B_VTEN, B_QK, B_PK1, B_PK2, B_MM};
   const int zero_w[] = {B_XW, B_RRW, B_WTEN};
-  for (int z : zero_cell) hipLaunchKernelGGL(k_b_fill, dim3(gm1_blocks(cells, 256)), dim3(256), 0, s, w.p[z], cells);
-  for (int z : zero_w) hipLaunchKernelGGL(k_b_fill, dim3(gm1_blocks(wcells, 256)), dim3(256), 0, s, w.p[z], wcells);
+  FillList fl;
+  fl.count = 0;
+  for (int z : zero_cell) {
+    fl.p[fl.count] = w.p[z];
+    fl.n[fl.count++] = cells;
+  }
+  for (int z : zero_w) {
+    fl.p[fl.count] = w.p[z];
+    fl.n[fl.count++] = wcells;
+  }
+  const int fill_blocks = (int)std::min<int64_t>(gm1_blocks(wcells, 256), 2048);
+  hipLaunchKernelGGL(k_b_fill, dim3(fill_blocks, fl.count), dim3(256), 0, s, fl);
   GM1_LAUNCH_CHECK();
   const dim3 b(64), g((ni + 63) / 64, nj, nk), gw((ni + 63) / 64, nj, nk + 1), gc((ni + 63) / 64, nj);
   // theta and the post-acoustic divergences
@@ -1500,7 +1537,7 @@ int gm1hip_dry_stage_adjoint(void* const* table, const float* consts, int ni, in
     hipLaunchKernelGGL(k_b_ss_faces, g, b, 0, s, w, n);
     hipLaunchKernelGGL(k_b_ss_cells, g, b, 0, s, w, n);
   }
-  hipLaunchKernelGGL(k_b_coef, gc, b, 0, s, w);
+  hipLaunchKernelGGL(k_b_coef, g, b, 0, s, w);
   hipLaunchKernelGGL(k_b_ac_end, gw, b, 0, s, w);
   // momentum advection, buoyancy, mass fluxes, stage terms
   hipLaunchKernelGGL(k_b_adv_local, gw, b, 0, s, w);
