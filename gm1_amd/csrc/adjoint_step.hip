@@ -638,37 +638,70 @@ __global__ __launch_bounds__(64) void k_ss_fpk(W w, int n) {
   w.p[FPK][w.at(l, j, i)] = column_fpk(w, n, l, j, i, nullptr);
 }
 
-// substep n, one column: the Thomas elimination of rows 2..nk and back substitution (w at levels 2..nk)
+// substep n, one column: the Thomas elimination of rows 2..nk and back substitution (w at levels 2..nk).
+// Rows are fetched TH_CHUNK levels at a time into registers (independent loads in flight together), then the
+// recurrence runs over them.
+constexpr int TH_CHUNK = 8;
+
 __global__ __launch_bounds__(64) void k_ss_thomas(W w, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
   if (i >= w.ni) return;
   const int nk = w.nk;
-  const float* ws = w.wsnap(n - 1);
-  float* wn = w.wsnap(n);
-  const float *ra = w.p[RA], *rb = w.p[RB], *rc = w.p[RC], *rd = w.p[RD];
-  float *ec = w.p[EC], *ed = w.p[ED];
+  const float* __restrict__ ws = w.wsnap(n - 1);
+  float* __restrict__ wn = w.wsnap(n);
+  const float* __restrict__ ra = w.p[RA];
+  const float* __restrict__ rb = w.p[RB];
+  const float* __restrict__ rc = w.p[RC];
+  const float* __restrict__ rd = w.p[RD];
+  float* __restrict__ ec = w.p[EC];
+  float* __restrict__ ed = w.p[ED];
   float ecd = 0.0f, edd = 0.0f;
-#pragma unroll 4
-  for (int k = 2; k <= nk; ++k) {
-    const int64_t q = w.at(k, j, i);
-    const float aa = ra[q], bb = rb[q], cc = rc[q], dd = rd[q];
-    const float pivot = k == 2 ? bb : aa * ecd + bb;
-    const float inv = 1.0f / pivot;
-    if (k < nk) {
-      ecd = (-cc) * inv;
-      edd = (k == 2 ? dd : dd - aa * edd) * inv;
-      ec[q] = ecd;
-      ed[q] = edd;
-    } else {
-      wn[q] = (dd - aa * edd) * inv;
+  for (int k0 = 2; k0 <= nk; k0 += TH_CHUNK) {
+    float a[TH_CHUNK], bb[TH_CHUNK], c[TH_CHUNK], d[TH_CHUNK];
+#pragma unroll
+    for (int u = 0; u < TH_CHUNK; ++u) {
+      const int k = k0 + u;
+      const int64_t q = w.at(k <= nk ? k : nk, j, i);
+      a[u] = ra[q];
+      bb[u] = rb[q];
+      c[u] = rc[q];
+      d[u] = rd[q];
+    }
+#pragma unroll
+    for (int u = 0; u < TH_CHUNK; ++u) {
+      const int k = k0 + u;
+      if (k > nk) break;
+      const int64_t q = w.at(k, j, i);
+      const float pivot = k == 2 ? bb[u] : a[u] * ecd + bb[u];
+      const float inv = 1.0f / pivot;
+      if (k < nk) {
+        ecd = (-c[u]) * inv;
+        edd = (k == 2 ? d[u] : d[u] - a[u] * edd) * inv;
+        ec[q] = ecd;
+        ed[q] = edd;
+      } else {
+        wn[q] = (d[u] - a[u] * edd) * inv;
+        edd = wn[q];
+      }
     }
   }
-  float wnext = wn[w.at(nk, j, i)];
-#pragma unroll 4
-  for (int k = nk - 1; k >= 2; --k) {
-    const int64_t q = w.at(k, j, i);
-    wnext = ec[q] * wnext + ed[q];
-    wn[q] = wnext;
+  float wnext = edd;  // w(nk)
+  for (int k1 = nk - 1; k1 >= 2; k1 -= TH_CHUNK) {
+    float e[TH_CHUNK], f[TH_CHUNK];
+#pragma unroll
+    for (int u = 0; u < TH_CHUNK; ++u) {
+      const int k = k1 - u;
+      const int64_t q = w.at(k >= 2 ? k : 2, j, i);
+      e[u] = ec[q];
+      f[u] = ed[q];
+    }
+#pragma unroll
+    for (int u = 0; u < TH_CHUNK; ++u) {
+      const int k = k1 - u;
+      if (k < 2) break;
+      wnext = e[u] * wnext + f[u];
+      wn[w.at(k, j, i)] = wnext;
+    }
   }
   wn[w.at(1, j, i)] = 0.0f;
   wn[w.at(nk + 1, j, i)] = ws[w.at(nk + 1, j, i)];
@@ -967,34 +1000,60 @@ __global__ __launch_bounds__(64) void k_b_ssA2(W w) {
   w.p[B_W][q] += vtb[q] * (pk1[q] + qk[q]) + vtb[qd] * (pk2[qd] - qk[qd]);
 }
 
-// B (one column): the transposed Thomas solve T^T lam = wbar(2..nk); lam is left in B_ED
+// B (one column): the transposed Thomas solve T^T lam = wbar(2..nk); lam is left in B_ED.  Row k of T^T
+// has cc(k-1) below and aa(k+1) above its diagonal bb(k); rows are fetched TH_CHUNK levels at a time.
 __global__ __launch_bounds__(64) void k_b_ssB(W w) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
   if (i >= w.ni) return;
   const int nk = w.nk;
-  float *bec = w.p[B_EC], *bed = w.p[B_ED];
-  const float *bw = w.p[B_W], *ra = w.p[RA], *rb = w.p[RB], *rc = w.p[RC];
+  float* __restrict__ bec = w.p[B_EC];
+  float* __restrict__ bed = w.p[B_ED];
+  const float* __restrict__ bw = w.p[B_W];
+  const float* __restrict__ ra = w.p[RA];
+  const float* __restrict__ rb = w.p[RB];
+  const float* __restrict__ rc = w.p[RC];
   float cprev = 0.0f, dprev = 0.0f;
-#pragma unroll 4
-  for (int k = 2; k <= nk; ++k) {
-    const int64_t q = w.at(k, j, i);
-    const float lo = k == 2 ? 0.0f : rc[w.at(k - 1, j, i)];
-    const float up = k < nk ? ra[w.at(k + 1, j, i)] : 0.0f;
-    const float piv = rb[q] - lo * cprev;
-    const float cpk = up / piv;
-    const float dpk = (bw[q] - lo * dprev) / piv;
-    bec[q] = cpk;
-    bed[q] = dpk;
-    cprev = cpk;
-    dprev = dpk;
+  for (int k0 = 2; k0 <= nk; k0 += TH_CHUNK) {
+    float lo[TH_CHUNK], up[TH_CHUNK], dg[TH_CHUNK], rh[TH_CHUNK];
+#pragma unroll
+    for (int u = 0; u < TH_CHUNK; ++u) {
+      const int k = k0 + u;
+      const int kk = k <= nk ? k : nk;
+      lo[u] = kk > 2 ? rc[w.at(kk - 1, j, i)] : 0.0f;
+      up[u] = kk < nk ? ra[w.at(kk + 1, j, i)] : 0.0f;
+      dg[u] = rb[w.at(kk, j, i)];
+      rh[u] = bw[w.at(kk, j, i)];
+    }
+#pragma unroll
+    for (int u = 0; u < TH_CHUNK; ++u) {
+      const int k = k0 + u;
+      if (k > nk) break;
+      const int64_t q = w.at(k, j, i);
+      const float piv = dg[u] - lo[u] * cprev;
+      const float cpk = up[u] / piv;
+      const float dpk = (rh[u] - lo[u] * dprev) / piv;
+      bec[q] = cpk;
+      bed[q] = dpk;
+      cprev = cpk;
+      dprev = dpk;
+    }
   }
-  float lamnext = 0.0f;
-#pragma unroll 4
-  for (int k = nk; k >= 2; --k) {
-    const int64_t q = w.at(k, j, i);
-    const float lam = k == nk ? bed[q] : bed[q] - bec[q] * lamnext;
-    bed[q] = lam;
-    lamnext = lam;
+  float lamnext = dprev;  // lam(nk)
+  for (int k1 = nk - 1; k1 >= 2; k1 -= TH_CHUNK) {
+    float e[TH_CHUNK], f[TH_CHUNK];
+#pragma unroll
+    for (int u = 0; u < TH_CHUNK; ++u) {
+      const int64_t q = w.at(k1 - u >= 2 ? k1 - u : 2, j, i);
+      e[u] = bec[q];
+      f[u] = bed[q];
+    }
+#pragma unroll
+    for (int u = 0; u < TH_CHUNK; ++u) {
+      const int k = k1 - u;
+      if (k < 2) break;
+      lamnext = f[u] - e[u] * lamnext;
+      bed[w.at(k, j, i)] = lamnext;
+    }
   }
 }
 
