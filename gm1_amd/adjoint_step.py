@@ -52,9 +52,10 @@ SLOTS = (
     "B_THV", "B_PPTERM", "B_PIADV", "B_DUM8", "B_THTERM", "B_RRU", "B_RRV", "B_RRW", "B_DIVX", "B_UTEN",
     "B_VTEN", "B_WTEN", "B_QK", "B_PK1", "B_PK2", "B_MM", "B_U", "B_V", "B_W", "B_PP", "B_PPD", "B_ARU", "B_ARV",
     "B_ARW", "B_DIV", "B_XA", "B_TOT", "B_MU", "B_MV", "B_MW", "B_DIVX2", "B_SC", "B_EC", "B_ED", "B_FPKF",
-    "RA", "RB", "RC", "RD",
+    "RA", "RB", "RC", "RD", "G0", "G1", "G2", "G3", "G4", "G5",
 )
-W_SLOTS = {"XW", "AW", "RRW", "WTEN", "ARW", "B_OW", "B_XW", "B_AW", "B_RRW", "B_WTEN", "B_W", "B_ARW", "B_MW"}
+W_SLOTS = {"XW", "AW", "RRW", "WTEN", "ARW", "B_OW", "B_XW", "B_AW", "B_RRW", "B_WTEN", "B_W", "B_ARW", "B_MW",
+           "G0", "G1", "G2", "G3", "G4", "G5"}
 _RD, _RV, _CP, _CPV, _G = F32(287.04), F32(461.5), F32(1005.7), F32(1870.0), F32(9.81)
 THETA_WEPS = float(F32(10.0) * F32(1.0e-18))
 

@@ -47,6 +47,7 @@ enum Slot {
   B_QK, B_PK1, B_PK2, B_MM, B_U, B_V, B_W, B_PP, B_PPD, B_ARU, B_ARV, B_ARW,
   B_DIV, B_XA, B_TOT, B_MU, B_MV, B_MW, B_DIVX2, B_SC, B_EC, B_ED, B_FPKF,
   RA, RB, RC, RD,                         // tridiagonal rows of the stage (aa, bb, cc) and a substep's right side
+  G0, G1, G2, G3, G4, G5,                 // momentum transposes: per-flux adjoints (upwind/downwind parts, mass)
   NSLOT
 };
 
@@ -1481,6 +1482,118 @@ __global__ __launch_bounds__(64) void k_b_advw(W w, int dir) {
   for (int n2 = 0; n2 < nc; ++n2) aadd(&w.p[B_XW][w.at(st.lev[n2], j, i)], cf[n2] * rb);
 }
 
+// Horizontal momentum flux transposes without atomics.  Pass 1 stores, per flux, fb*m on the side its
+// upwind choice took (G0/G3 where m >= 0, G1/G4 where m < 0) and fb*r (G2/G5), for the x (G0-G2) and y
+// (G3-G5) fluxes of one component; pass 2 gathers them at each grid point.  comp 0 u, 1 v, 2 w (levels 2..nk).
+__global__ __launch_bounds__(64) void k_b_advh_flux(W w, int comp) {
+  CELL_INDEX
+  const Consts& c = w.c;
+  const int nk = w.nk;
+  const int64_t q = w.at(l, j, i);
+  const float *ru = w.p[RRU], *rv = w.p[RRV];
+  float *g0 = w.p[G0], *g1 = w.p[G1], *g2 = w.p[G2], *g3 = w.p[G3], *g4 = w.p[G4], *g5 = w.p[G5];
+  if (comp == 2 && (l < 2 || l > nk)) {
+    g0[q] = g1[q] = g2[q] = g3[q] = g4[q] = g5[q] = 0.0f;
+    return;
+  }
+  const float* fld = comp == 0 ? w.p[XU] : (comp == 1 ? w.p[XV] : w.p[XW]);
+  const float* bt = comp == 0 ? w.p[B_UTEN] : (comp == 1 ? w.p[B_VTEN] : w.p[B_WTEN]);
+  auto F = [&](int jj, int ii) { return fld[w.at(l, w.Y(jj, 0), w.X(ii, 0))]; };
+  auto tot = [&](int jj, int ii) {
+    const float v = bt[w.at(l, w.Y(jj, 0), w.X(ii, 0))];
+    if (comp == 2) return v * (w.col.rrf0[l - 1] * 1.0f);
+    const float rho = w.col.rho[l - 1];
+    return v * (1.0f / (0.5f * (rho + rho)));
+  };
+  float fbx, mx, posx, negx, fby, my, posy, negy;
+  if (comp == 0) {  // x flux at the cell centre i, y flux at row face j (face column i)
+    fbx = c.rdx * tot(j, i + 1) - c.rdx * tot(j, i);
+    mx = 0.5f * (ru[q] + ru[w.at(l, j, w.X(i, 1))]);
+    posx = flx5(F(j, i - 2), F(j, i - 1), F(j, i), F(j, i + 1), F(j, i + 2));
+    negx = flx5(F(j, i + 3), F(j, i + 2), F(j, i + 1), F(j, i), F(j, i - 1));
+    fby = c.rdy * tot(j, i) - c.rdy * tot(j - 1, i);
+    my = 0.5f * (rv[q] + rv[w.at(l, j, w.X(i, -1))]);
+    posy = flx5(F(j - 3, i), F(j - 2, i), F(j - 1, i), F(j, i), F(j + 1, i));
+    negy = flx5(F(j + 2, i), F(j + 1, i), F(j, i), F(j - 1, i), F(j - 2, i));
+  } else if (comp == 1) {  // x flux at face column i of row j, y flux at the cell-centre row j
+    fbx = c.rdx * tot(j, i) - c.rdx * tot(j, i - 1);
+    mx = 0.5f * (ru[q] + ru[w.at(l, w.Y(j, -1), i)]);
+    posx = flx5(F(j, i - 3), F(j, i - 2), F(j, i - 1), F(j, i), F(j, i + 1));
+    negx = flx5(F(j, i + 2), F(j, i + 1), F(j, i), F(j, i - 1), F(j, i - 2));
+    fby = c.rdy * tot(j + 1, i) - c.rdy * tot(j, i);
+    my = 0.5f * (rv[q] + rv[w.at(l, w.Y(j, 1), i)]);
+    posy = flx5(F(j - 2, i), F(j - 1, i), F(j, i), F(j + 1, i), F(j + 2, i));
+    negy = flx5(F(j + 3, i), F(j + 2, i), F(j + 1, i), F(j, i), F(j - 1, i));
+  } else {  // w at level l: x flux at face i, y flux at face j
+    const float c2h = 0.5f * (w.col.c2[l - 1] + w.col.c2[l - 1]);
+    const float c1h = 1.0f - c2h;
+    fbx = c.rdx * tot(j, i) - c.rdx * tot(j, i - 1);
+    mx = c2h * ru[w.at(l, j, i)] + c1h * ru[w.at(l - 1, j, i)];
+    posx = flx5(F(j, i - 3), F(j, i - 2), F(j, i - 1), F(j, i), F(j, i + 1));
+    negx = flx5(F(j, i + 2), F(j, i + 1), F(j, i), F(j, i - 1), F(j, i - 2));
+    fby = c.rdy * tot(j, i) - c.rdy * tot(j - 1, i);
+    my = c2h * rv[w.at(l, j, i)] + c1h * rv[w.at(l - 1, j, i)];
+    posy = flx5(F(j - 3, i), F(j - 2, i), F(j - 1, i), F(j, i), F(j + 1, i));
+    negy = flx5(F(j + 2, i), F(j + 1, i), F(j, i), F(j - 1, i), F(j - 2, i));
+  }
+  const bool ux = mx >= 0.0f, uy = my >= 0.0f;
+  g0[q] = ux ? fbx * mx : 0.0f;
+  g1[q] = ux ? 0.0f : fbx * mx;
+  g2[q] = fbx * (ux ? posx : negx);
+  g3[q] = uy ? fby * my : 0.0f;
+  g4[q] = uy ? 0.0f : fby * my;
+  g5[q] = fby * (uy ? posy : negy);
+}
+
+// pass 2: the field adjoint gathered from the fluxes whose stencils hold the point, and the mass-flux adjoints
+__global__ __launch_bounds__(64) void k_b_advh_gather(W w, int comp) {
+  CELL_INDEX
+  const int nk = w.nk;
+  const int64_t q = w.at(l, j, i);
+  const float *g0 = w.p[G0], *g1 = w.p[G1], *g2 = w.p[G2], *g3 = w.p[G3], *g4 = w.p[G4], *g5 = w.p[G5];
+  const float cf[5] = {gm1::C2_60, gm1::CM13_60, gm1::C47_60, gm1::C27_60, gm1::CM3_60};
+  auto at = [&](int ll, int jj, int ii) { return w.at(ll, w.Y(jj, 0), w.X(ii, 0)); };
+  // stencil offsets: a flux at index p reads points p + o0 + n (upwind side, o0 = -2 or -3) or p + o1 - n
+  const int xo_pos = comp == 0 ? -2 : -3, xo_neg = comp == 0 ? 3 : 2;
+  const int yo_pos = comp == 1 ? -2 : -3, yo_neg = comp == 1 ? 3 : 2;
+  const bool wlevel = comp != 2 || (l >= 2 && l <= nk);
+  if (wlevel) {
+    float acc = 0.0f;
+    for (int n = 0; n < 5; ++n) {
+      acc += cf[n] * g0[at(l, j, i - xo_pos - n)];
+      acc += cf[n] * g1[at(l, j, i - xo_neg + n)];
+      acc += cf[n] * g3[at(l, j - yo_pos - n, i)];
+      acc += cf[n] * g4[at(l, j - yo_neg + n, i)];
+    }
+    float* bf = comp == 0 ? w.p[B_XU] : (comp == 1 ? w.p[B_XV] : w.p[B_XW]);
+    bf[q] += acc;
+  }
+  if (l > nk) return;
+  // mass-flux adjoints at the rru / rrv point (l, j, i)
+  if (comp == 0) {
+    w.p[B_RRU][q] += 0.5f * g2[q] + 0.5f * g2[at(l, j, i - 1)];
+    w.p[B_RRV][q] += 0.5f * g5[q] + 0.5f * g5[at(l, j, i + 1)];
+  } else if (comp == 1) {
+    w.p[B_RRU][q] += 0.5f * g2[q] + 0.5f * g2[at(l, j + 1, i)];
+    w.p[B_RRV][q] += 0.5f * g5[q] + 0.5f * g5[at(l, j - 1, i)];
+  } else {
+    float gu = 0.0f, gv = 0.0f;
+    if (l >= 2) {
+      const float c2h = 0.5f * (w.col.c2[l - 1] + w.col.c2[l - 1]);
+      gu += c2h * g2[q];
+      gv += c2h * g5[q];
+    }
+    if (l + 1 <= nk) {
+      const float c2h = 0.5f * (w.col.c2[l] + w.col.c2[l]);
+      const int64_t qu = w.at(l + 1, j, i);
+      gu += (1.0f - c2h) * g2[qu];
+      gv += (1.0f - c2h) * g5[qu];
+    }
+    w.p[B_RRU][q] += gu;
+    w.p[B_RRV][q] += gv;
+  }
+}
+
 // mass fluxes and divx transposed, then the stage terms (grid level l = 1..nk+1)
 __global__ __launch_bounds__(64) void k_b_mass_terms(W w) {
   CELL_INDEX
@@ -1600,11 +1713,13 @@ int gm1hip_dry_stage_adjoint(void* const* table, const float* consts, int ni, in
   hipLaunchKernelGGL(k_b_ac_end, gw, b, 0, s, w);
   // momentum advection, buoyancy, mass fluxes, stage terms
   hipLaunchKernelGGL(k_b_adv_local, gw, b, 0, s, w);
-  for (int d = 0; d < 3; ++d) {
-    hipLaunchKernelGGL(k_b_advu, g, b, 0, s, w, d);
-    hipLaunchKernelGGL(k_b_advv, g, b, 0, s, w, d);
-    hipLaunchKernelGGL(k_b_advw, d < 2 ? gw : g, b, 0, s, w, d);
+  for (int comp = 0; comp < 3; ++comp) {
+    hipLaunchKernelGGL(k_b_advh_flux, comp == 2 ? gw : g, b, 0, s, w, comp);
+    hipLaunchKernelGGL(k_b_advh_gather, comp == 2 ? gw : g, b, 0, s, w, comp);
   }
+  hipLaunchKernelGGL(k_b_advu, g, b, 0, s, w, 2);
+  hipLaunchKernelGGL(k_b_advv, g, b, 0, s, w, 2);
+  hipLaunchKernelGGL(k_b_advw, g, b, 0, s, w, 2);
   hipLaunchKernelGGL(k_b_mass_terms, gw, b, 0, s, w);
   GM1_LAUNCH_CHECK();
   return 0;
