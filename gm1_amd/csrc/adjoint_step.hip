@@ -954,7 +954,8 @@ __global__ __launch_bounds__(64) void k_b_ac_init(W w) {
 }
 
 // The column adjoint of substep n in three passes.  A (every cell): forward fpk and div recomputed, the w
-// averages, the pressure update's adjoint (fpk's and vt's adjoints kept) and the w adjoint of vt gathered.
+// averages, the pressure update's adjoint (fpk's part kept in B_FPKF), vt's coefficient adjoints and the w
+// adjoint of vt gathered from levels l (w(k)) and l-1 (w(k+1)).  B_PP, B_PPD and B_PPX are only read here.
 __global__ __launch_bounds__(64) void k_b_ssA(W w, int n) {
   CELL_INDEX
   const Consts& c = w.c;
@@ -966,6 +967,11 @@ __global__ __launch_bounds__(64) void k_b_ssA(W w, int n) {
   float dv;
   w.p[FPK][q] = column_fpk(w, n, l, j, i, &dv);
   w.p[DIV][q] = dv;
+  auto upb_at = [&](int64_t qq) {
+    float u = w.p[B_PP][qq] + (1.0f + c.kdiv) * w.p[B_PPD][qq];
+    if (last3) u = u + c.kdiv * w.p[B_PPX][qq];
+    return u;
+  };
   float bw = w.p[B_W][q];
   if (last) {
     bw += (w.p[B_ARW][q] * c.tavg) * c.alph;
@@ -973,32 +979,20 @@ __global__ __launch_bounds__(64) void k_b_ssA(W w, int n) {
   } else {
     bw += w.p[B_ARW][q];
   }
-  w.p[B_W][q] = bw;
-  float upb = w.p[B_PP][q] + (1.0f + c.kdiv) * w.p[B_PPD][q];
-  float ppb = -c.kdiv * w.p[B_PPD][q];
-  if (last3) {
-    upb = upb + c.kdiv * w.p[B_PPX][q];
-    ppb = ppb - c.kdiv * w.p[B_PPX][q];
-    w.p[B_PPX][q] = 0.0f;
-  }
-  w.p[B_PP][q] = ppb + upb;
+  const float upb = upb_at(q);
   w.p[B_FPKF][q] = upb;
   const float vtb = c.alph * upb;
-  w.p[B_XA][q] = vtb;
+  const float *qk = w.p[QK], *pk1 = w.p[PK1], *pk2 = w.p[PK2];
   const float wlo = l == 1 ? 0.0f : wn[q];
   const float whi = l == nk ? 0.0f : wn[w.at(l + 1, j, i)];
   w.p[B_PK1][q] += vtb * wlo;
   w.p[B_QK][q] += vtb * (wlo - whi);
   w.p[B_PK2][q] += vtb * whi;
-}
-
-// A2 (levels 2..nk): the w adjoint of vt from levels l (w(k)) and l-1 (w(k+1))
-__global__ __launch_bounds__(64) void k_b_ssA2(W w) {
-  CELL_INDEX
-  if (l < 2) return;
-  const float *qk = w.p[QK], *pk1 = w.p[PK1], *pk2 = w.p[PK2], *vtb = w.p[B_XA];
-  const int64_t q = w.at(l, j, i), qd = w.at(l - 1, j, i);
-  w.p[B_W][q] += vtb[q] * (pk1[q] + qk[q]) + vtb[qd] * (pk2[qd] - qk[qd]);
+  if (l >= 2) {
+    const int64_t qd = w.at(l - 1, j, i);
+    bw += vtb * (pk1[q] + qk[q]) + (c.alph * upb_at(qd)) * (pk2[qd] - qk[qd]);
+  }
+  w.p[B_W][q] = bw;
 }
 
 // B (one column): the transposed Thomas solve T^T lam = wbar(2..nk); lam is left in B_ED.  Row k of T^T
@@ -1111,8 +1105,16 @@ __global__ __launch_bounds__(64) void k_b_ssC(W w, int n) {
     if (l < nk) mmb += c.asq * (pk2[q] - qk[q]) * ccb(l);
     w.p[B_MM][q] += mmb;
   }
-  // pressure adjoint from rows l and l+1
-  w.p[B_PP][q] += (pdb(l) + udb(l)) - (pdb(l + 1) + udb(l + 1));
+  // pressure adjoint: the update's (from the step's output adjoints) and rows l and l+1's
+  {
+    const bool last3 = n == w.nloop && w.nrk == 3;
+    float ppb = -c.kdiv * w.p[B_PPD][q];
+    if (last3) {
+      ppb = ppb - c.kdiv * w.p[B_PPX][q];
+      w.p[B_PPX][q] = 0.0f;
+    }
+    w.p[B_PP][q] = (ppb + w.p[B_FPKF][q]) + ((pdb(l) + udb(l)) - (pdb(l + 1) + udb(l + 1)));
+  }
   // coefficient adjoints of level l
   const float wlo = l == 1 ? 0.0f : ws[q];
   const float wup = ws[w.at(l + 1, j, i)];
@@ -1305,183 +1307,6 @@ __global__ __launch_bounds__(64) void k_b_adv_local(W w) {
   }
 }
 
-// scatter of one horizontal flux's adjoint into its mass and stencil
-template <typename Q, typename Add>
-__device__ __forceinline__ void flux5_T(float fb, float m, float pos, float neg, int o_pos, int o_neg, Q idx,
-                                        Add add, float* bq) {
-  const bool up = m >= 0.0f;
-  const float rb = fb * m;
-  const float cf[5] = {gm1::C2_60, gm1::CM13_60, gm1::C47_60, gm1::C27_60, gm1::CM3_60};
-  const int o0 = up ? o_pos : o_neg, st = up ? 1 : -1;
-  for (int n = 0; n < 5; ++n) aadd(&bq[idx(o0 + st * n)], cf[n] * rb);
-  add(fb * (up ? pos : neg));
-}
-
-// u: x fluxes at cell centres (dir 0), y fluxes at row faces (dir 1), z fluxes at w levels (dir 2)
-__global__ __launch_bounds__(64) void k_b_advu(W w, int dir) {
-  CELL_INDEX
-  const Consts& c = w.c;
-  const int nk = w.nk;
-  const float *u = w.p[XU], *ru = w.p[RRU], *rv = w.p[RRV], *rw = w.p[RRW], *bt = w.p[B_UTEN];
-  auto tot = [&](int ll, int jj, int ii) {
-    const float rho = w.col.rho[ll - 1];
-    return bt[w.at(ll, w.Y(jj, 0), w.X(ii, 0))] * (1.0f / (0.5f * (rho + rho)));
-  };
-  auto U = [&](int ll, int jj, int ii) { return u[w.at(ll, w.Y(jj, 0), w.X(ii, 0))]; };
-  if (dir == 0) {  // flux at cell centre i (faces i, i+1)
-    const float fb = c.rdx * tot(l, j, i + 1) - c.rdx * tot(l, j, i);
-    const float m = 0.5f * (ru[w.at(l, j, i)] + ru[w.at(l, j, w.X(i, 1))]);
-    const float pos = flx5(U(l, j, i - 2), U(l, j, i - 1), U(l, j, i), U(l, j, i + 1), U(l, j, i + 2));
-    const float neg = flx5(U(l, j, i + 3), U(l, j, i + 2), U(l, j, i + 1), U(l, j, i), U(l, j, i - 1));
-    flux5_T(fb, m, pos, neg, -2, 3, [&](int o) { return w.at(l, j, w.X(i, o)); },
-            [&](float mb) {
-              aadd(&w.p[B_RRU][w.at(l, j, i)], 0.5f * mb);
-              aadd(&w.p[B_RRU][w.at(l, j, w.X(i, 1))], 0.5f * mb);
-            },
-            w.p[B_XU]);
-  } else if (dir == 1) {  // flux at row face j of face column i
-    const float fb = c.rdy * tot(l, j, i) - c.rdy * tot(l, j - 1, i);
-    const float m = 0.5f * (rv[w.at(l, j, i)] + rv[w.at(l, j, w.X(i, -1))]);
-    const float pos = flx5(U(l, j - 3, i), U(l, j - 2, i), U(l, j - 1, i), U(l, j, i), U(l, j + 1, i));
-    const float neg = flx5(U(l, j + 2, i), U(l, j + 1, i), U(l, j, i), U(l, j - 1, i), U(l, j - 2, i));
-    flux5_T(fb, m, pos, neg, -3, 2, [&](int o) { return w.at(l, w.Y(j, o), i); },
-            [&](float mb) {
-              aadd(&w.p[B_RRV][w.at(l, j, i)], 0.5f * mb);
-              aadd(&w.p[B_RRV][w.at(l, j, w.X(i, -1))], 0.5f * mb);
-            },
-            w.p[B_XU]);
-  } else {  // vertical flux at w level kf = l (2..nk)
-    const int kf = l;
-    if (kf < 2) return;
-    const float fb = c.rdz * tot(kf, j, i) - c.rdz * tot(kf - 1, j, i);
-    const float m = 0.5f * (rw[w.at(kf, j, i)] + rw[w.at(kf, j, w.X(i, -1))]);
-    const float c2h = 0.5f * (w.col.c2[kf - 1] + w.col.c2[kf - 1]);
-    Stencil ps, ng;
-    uvz_branch(kf, nk, ps, ng);
-    const Stencil& st = m >= 0.0f ? ps : ng;
-    auto q = [&](int ll) { return U(ll, j, i); };
-    const float r = eval_branch(st, q, c2h, 1.0f - c2h, 0.0, 0.0f);
-    const float pb = 0.5f * (fb * r);
-    aadd(&w.p[B_RRW][w.at(kf, j, i)], pb);
-    aadd(&w.p[B_RRW][w.at(kf, j, w.X(i, -1))], pb);
-    float cf[5];
-    const int nc = branch_coefs(st, q, c2h, 1.0f - c2h, 0.0f, cf);
-    const float rb = fb * m;
-    for (int n2 = 0; n2 < nc; ++n2) aadd(&w.p[B_XU][w.at(st.lev[n2], j, i)], cf[n2] * rb);
-  }
-}
-
-__global__ __launch_bounds__(64) void k_b_advv(W w, int dir) {
-  CELL_INDEX
-  const Consts& c = w.c;
-  const int nk = w.nk;
-  const float *v = w.p[XV], *ru = w.p[RRU], *rv = w.p[RRV], *rw = w.p[RRW], *bt = w.p[B_VTEN];
-  auto tot = [&](int ll, int jj, int ii) {
-    const float rho = w.col.rho[ll - 1];
-    return bt[w.at(ll, w.Y(jj, 0), w.X(ii, 0))] * (1.0f / (0.5f * (rho + rho)));
-  };
-  auto V = [&](int ll, int jj, int ii) { return v[w.at(ll, w.Y(jj, 0), w.X(ii, 0))]; };
-  if (dir == 0) {  // flux at face column i of v row j
-    const float fb = c.rdx * tot(l, j, i) - c.rdx * tot(l, j, i - 1);
-    const float m = 0.5f * (ru[w.at(l, j, i)] + ru[w.at(l, w.Y(j, -1), i)]);
-    const float pos = flx5(V(l, j, i - 3), V(l, j, i - 2), V(l, j, i - 1), V(l, j, i), V(l, j, i + 1));
-    const float neg = flx5(V(l, j, i + 2), V(l, j, i + 1), V(l, j, i), V(l, j, i - 1), V(l, j, i - 2));
-    flux5_T(fb, m, pos, neg, -3, 2, [&](int o) { return w.at(l, j, w.X(i, o)); },
-            [&](float mb) {
-              aadd(&w.p[B_RRU][w.at(l, j, i)], 0.5f * mb);
-              aadd(&w.p[B_RRU][w.at(l, w.Y(j, -1), i)], 0.5f * mb);
-            },
-            w.p[B_XV]);
-  } else if (dir == 1) {  // flux at cell-centre row j (v faces j, j+1)
-    const float fb = c.rdy * tot(l, j + 1, i) - c.rdy * tot(l, j, i);
-    const float m = 0.5f * (rv[w.at(l, j, i)] + rv[w.at(l, w.Y(j, 1), i)]);
-    const float pos = flx5(V(l, j - 2, i), V(l, j - 1, i), V(l, j, i), V(l, j + 1, i), V(l, j + 2, i));
-    const float neg = flx5(V(l, j + 3, i), V(l, j + 2, i), V(l, j + 1, i), V(l, j, i), V(l, j - 1, i));
-    flux5_T(fb, m, pos, neg, -2, 3, [&](int o) { return w.at(l, w.Y(j, o), i); },
-            [&](float mb) {
-              aadd(&w.p[B_RRV][w.at(l, j, i)], 0.5f * mb);
-              aadd(&w.p[B_RRV][w.at(l, w.Y(j, 1), i)], 0.5f * mb);
-            },
-            w.p[B_XV]);
-  } else {
-    const int kf = l;
-    if (kf < 2) return;
-    const float fb = c.rdz * tot(kf, j, i) - c.rdz * tot(kf - 1, j, i);
-    const float m = 0.5f * (rw[w.at(kf, j, i)] + rw[w.at(kf, w.Y(j, -1), i)]);
-    const float c2h = 0.5f * (w.col.c2[kf - 1] + w.col.c2[kf - 1]);
-    Stencil ps, ng;
-    uvz_branch(kf, nk, ps, ng);
-    const Stencil& st = m >= 0.0f ? ps : ng;
-    auto q = [&](int ll) { return V(ll, j, i); };
-    const float r = eval_branch(st, q, c2h, 1.0f - c2h, 0.0, 0.0f);
-    const float pb = 0.5f * (fb * r);
-    aadd(&w.p[B_RRW][w.at(kf, j, i)], pb);
-    aadd(&w.p[B_RRW][w.at(kf, w.Y(j, -1), i)], pb);
-    float cf[5];
-    const int nc = branch_coefs(st, q, c2h, 1.0f - c2h, 0.0f, cf);
-    const float rb = fb * m;
-    for (int n2 = 0; n2 < nc; ++n2) aadd(&w.p[B_XV][w.at(st.lev[n2], j, i)], cf[n2] * rb);
-  }
-}
-
-// w: x/y fluxes at faces of w level kf = l (dir 0, 1), vertical fluxes at scalar level k = l (dir 2)
-__global__ __launch_bounds__(64) void k_b_advw(W w, int dir) {
-  CELL_INDEX
-  const Consts& c = w.c;
-  const int nk = w.nk;
-  const float *ww = w.p[XW], *ru = w.p[RRU], *rv = w.p[RRV], *rw = w.p[RRW], *bt = w.p[B_WTEN];
-  auto tot = [&](int kf, int jj, int ii) {
-    if (kf < 2 || kf > nk) return 0.0f;
-    return bt[w.at(kf, w.Y(jj, 0), w.X(ii, 0))] * (w.col.rrf0[kf - 1] * 1.0f);
-  };
-  auto Wv = [&](int ll, int jj, int ii) { return ww[w.at(ll, w.Y(jj, 0), w.X(ii, 0))]; };
-  if (dir < 2) {
-    const int kf = l;
-    if (kf < 2 || kf > nk) return;
-    const float c2h = 0.5f * (w.col.c2[kf - 1] + w.col.c2[kf - 1]);
-    const float c1h = 1.0f - c2h;
-    if (dir == 0) {
-      const float fb = c.rdx * tot(kf, j, i) - c.rdx * tot(kf, j, i - 1);
-      const float m = c2h * ru[w.at(kf, j, i)] + c1h * ru[w.at(kf - 1, j, i)];
-      const float pos = flx5(Wv(kf, j, i - 3), Wv(kf, j, i - 2), Wv(kf, j, i - 1), Wv(kf, j, i), Wv(kf, j, i + 1));
-      const float neg = flx5(Wv(kf, j, i + 2), Wv(kf, j, i + 1), Wv(kf, j, i), Wv(kf, j, i - 1), Wv(kf, j, i - 2));
-      flux5_T(fb, m, pos, neg, -3, 2, [&](int o) { return w.at(kf, j, w.X(i, o)); },
-              [&](float mb) {
-                aadd(&w.p[B_RRU][w.at(kf, j, i)], c2h * mb);
-                aadd(&w.p[B_RRU][w.at(kf - 1, j, i)], c1h * mb);
-              },
-              w.p[B_XW]);
-    } else {
-      const float fb = c.rdy * tot(kf, j, i) - c.rdy * tot(kf, j - 1, i);
-      const float m = c2h * rv[w.at(kf, j, i)] + c1h * rv[w.at(kf - 1, j, i)];
-      const float pos = flx5(Wv(kf, j - 3, i), Wv(kf, j - 2, i), Wv(kf, j - 1, i), Wv(kf, j, i), Wv(kf, j + 1, i));
-      const float neg = flx5(Wv(kf, j + 2, i), Wv(kf, j + 1, i), Wv(kf, j, i), Wv(kf, j - 1, i), Wv(kf, j - 2, i));
-      flux5_T(fb, m, pos, neg, -3, 2, [&](int o) { return w.at(kf, w.Y(j, o), i); },
-              [&](float mb) {
-                aadd(&w.p[B_RRV][w.at(kf, j, i)], c2h * mb);
-                aadd(&w.p[B_RRV][w.at(kf - 1, j, i)], c1h * mb);
-              },
-              w.p[B_XW]);
-    }
-    return;
-  }
-  const int k = l;  // scalar level 1..nk
-  const float fb = c.rdz * tot(k + 1, j, i) - c.rdz * tot(k, j, i);
-  const float m = 0.5f * (rw[w.at(k, j, i)] + rw[w.at(k + 1, j, i)]);
-  Stencil ps, ng;
-  wz_branch(k, nk, ps, ng);
-  const Stencil& st = m >= 0.0f ? ps : ng;
-  auto q = [&](int ll) { return Wv(ll, j, i); };
-  const float r = eval_branch(st, q, 0.0f, 0.0f, 0.0, 0.0f);
-  const float mb = 0.5f * (fb * r);
-  aadd(&w.p[B_RRW][w.at(k, j, i)], mb);
-  aadd(&w.p[B_RRW][w.at(k + 1, j, i)], mb);
-  float cf[5];
-  const int nc = branch_coefs(st, q, 0.0f, 0.0f, 0.0f, cf);
-  const float rb = fb * m;
-  for (int n2 = 0; n2 < nc; ++n2) aadd(&w.p[B_XW][w.at(st.lev[n2], j, i)], cf[n2] * rb);
-}
-
 // Horizontal momentum flux transposes without atomics.  Pass 1 stores, per flux, fb*m on the side its
 // upwind choice took (G0/G3 where m >= 0, G1/G4 where m < 0) and fb*r (G2/G5), for the x (G0-G2) and y
 // (G3-G5) fluxes of one component; pass 2 gathers them at each grid point.  comp 0 u, 1 v, 2 w (levels 2..nk).
@@ -1594,6 +1419,130 @@ __global__ __launch_bounds__(64) void k_b_advh_gather(W w, int comp) {
   }
 }
 
+// Vertical momentum flux transposes, the same two passes: pass 1 per flux (u/v: w levels kf = 2..nk,
+// w: scalar levels k = 1..nk) stores G0/G1 (fb*m by upwind side) and G2 (fb*r); pass 2 gathers per point.
+__global__ __launch_bounds__(64) void k_b_advz_flux(W w, int comp) {
+  CELL_INDEX
+  const Consts& c = w.c;
+  const int nk = w.nk;
+  const int64_t q = w.at(l, j, i);
+  float *g0 = w.p[G0], *g1 = w.p[G1], *g2 = w.p[G2];
+  const float* rw = w.p[RRW];
+  const bool live = comp == 2 ? (l <= nk) : (l >= 2 && l <= nk);
+  if (!live) {
+    g0[q] = g1[q] = g2[q] = 0.0f;
+    return;
+  }
+  const float* fld = comp == 0 ? w.p[XU] : (comp == 1 ? w.p[XV] : w.p[XW]);
+  auto Q = [&](int ll) { return fld[w.at(ll, j, i)]; };
+  float fb, m, r;
+  if (comp < 2) {
+    const float* bt = comp == 0 ? w.p[B_UTEN] : w.p[B_VTEN];
+    auto tot = [&](int ll) {
+      const float rho = w.col.rho[ll - 1];
+      return bt[w.at(ll, j, i)] * (1.0f / (0.5f * (rho + rho)));
+    };
+    const int kf = l;
+    fb = c.rdz * tot(kf) - c.rdz * tot(kf - 1);
+    const int64_t qo = comp == 0 ? w.at(kf, j, w.X(i, -1)) : w.at(kf, w.Y(j, -1), i);
+    m = 0.5f * (rw[q] + rw[qo]);
+    const float c2h = 0.5f * (w.col.c2[kf - 1] + w.col.c2[kf - 1]);
+    Stencil ps, ng;
+    uvz_branch(kf, nk, ps, ng);
+    r = m >= 0.0f ? eval_branch(ps, Q, c2h, 1.0f - c2h, 0.0, 0.0f) : eval_branch(ng, Q, c2h, 1.0f - c2h, 0.0, 0.0f);
+  } else {
+    const float* bt = w.p[B_WTEN];
+    auto tot = [&](int kf) {
+      if (kf < 2 || kf > nk) return 0.0f;
+      return bt[w.at(kf, j, i)] * (w.col.rrf0[kf - 1] * 1.0f);
+    };
+    const int k = l;
+    fb = c.rdz * tot(k + 1) - c.rdz * tot(k);
+    m = 0.5f * (rw[w.at(k, j, i)] + rw[w.at(k + 1, j, i)]);
+    Stencil ps, ng;
+    wz_branch(k, nk, ps, ng);
+    r = m >= 0.0f ? eval_branch(ps, Q, 0.0f, 0.0f, 0.0, 0.0f) : eval_branch(ng, Q, 0.0f, 0.0f, 0.0, 0.0f);
+  }
+  const bool up = m >= 0.0f;
+  g0[q] = up ? fb * m : 0.0f;
+  g1[q] = up ? 0.0f : fb * m;
+  g2[q] = fb * r;
+}
+
+__global__ __launch_bounds__(64) void k_b_advz_gather(W w, int comp) {
+  CELL_INDEX
+  const int nk = w.nk;
+  const int64_t q = w.at(l, j, i);
+  const float *g0 = w.p[G0], *g1 = w.p[G1], *g2 = w.p[G2];
+  auto Q0 = [&](int) { return 0.0f; };
+  float acc = 0.0f;
+  const float cf5[5] = {gm1::C2_60, gm1::CM13_60, gm1::C47_60, gm1::C27_60, gm1::CM3_60};
+  if (comp < 2 && l >= 6 && l <= nk - 5) {
+    // every flux kf = l-2..l+3 uses 5th-order stencils: upwind levels kf-3..kf+1, downwind kf+2..kf-2
+#pragma unroll
+    for (int d = -1; d <= 3; ++d) acc += cf5[3 - d] * g0[w.at(l + d, j, i)];
+#pragma unroll
+    for (int d = -2; d <= 2; ++d) acc += cf5[2 + d] * g1[w.at(l + d, j, i)];
+    (comp == 0 ? w.p[B_XU] : w.p[B_XV])[q] += acc;
+    const int64_t qn = comp == 0 ? w.at(l, j, w.X(i, 1)) : w.at(l, w.Y(j, 1), i);
+    w.p[B_RRW][q] += 0.5f * g2[q] + 0.5f * g2[qn];
+    return;
+  }
+  if (comp == 2 && l >= 6 && l <= nk - 4) {
+    // every flux k = l-3..l+2 uses 5th-order stencils: upwind levels k-2..k+2, downwind k+3..k-1
+#pragma unroll
+    for (int d = -2; d <= 2; ++d) acc += cf5[2 - d] * g0[w.at(l + d, j, i)];
+#pragma unroll
+    for (int d = -3; d <= 1; ++d) acc += cf5[3 + d] * g1[w.at(l + d, j, i)];
+    w.p[B_XW][q] += acc;
+    w.p[B_RRW][q] += 0.5f * g2[q] + 0.5f * g2[w.at(l - 1, j, i)];
+    return;
+  }
+  if (comp < 2) {
+    if (l <= nk) {
+      const int lo = l - 2 < 2 ? 2 : l - 2, hi = l + 3 > nk ? nk : l + 3;
+      for (int kf = lo; kf <= hi; ++kf) {
+        const float c2h = 0.5f * (w.col.c2[kf - 1] + w.col.c2[kf - 1]);
+        Stencil ps, ng;
+        uvz_branch(kf, nk, ps, ng);
+        float cf[5];
+        const int64_t qf = w.at(kf, j, i);
+        int nc = branch_coefs(ps, Q0, c2h, 1.0f - c2h, 0.0f, cf);
+        for (int n = 0; n < nc; ++n)
+          if (ps.lev[n] == l) acc += cf[n] * g0[qf];
+        nc = branch_coefs(ng, Q0, c2h, 1.0f - c2h, 0.0f, cf);
+        for (int n = 0; n < nc; ++n)
+          if (ng.lev[n] == l) acc += cf[n] * g1[qf];
+      }
+      (comp == 0 ? w.p[B_XU] : w.p[B_XV])[q] += acc;
+    }
+    // rrw at w level l (2..nk) from the fluxes at this point and its +x / +y neighbour
+    if (l >= 2 && l <= nk) {
+      const int64_t qn = comp == 0 ? w.at(l, j, w.X(i, 1)) : w.at(l, w.Y(j, 1), i);
+      w.p[B_RRW][q] += 0.5f * g2[q] + 0.5f * g2[qn];
+    }
+  } else {
+    const int lo = l - 3 < 1 ? 1 : l - 3, hi = l + 2 > nk ? nk : l + 2;
+    for (int k = lo; k <= hi; ++k) {
+      Stencil ps, ng;
+      wz_branch(k, nk, ps, ng);
+      float cf[5];
+      const int64_t qf = w.at(k, j, i);
+      int nc = branch_coefs(ps, Q0, 0.0f, 0.0f, 0.0f, cf);
+      for (int n = 0; n < nc; ++n)
+        if (ps.lev[n] == l) acc += cf[n] * g0[qf];
+      nc = branch_coefs(ng, Q0, 0.0f, 0.0f, 0.0f, cf);
+      for (int n = 0; n < nc; ++n)
+        if (ng.lev[n] == l) acc += cf[n] * g1[qf];
+    }
+    w.p[B_XW][q] += acc;
+    float mb = 0.0f;
+    if (l <= nk) mb += 0.5f * g2[q];
+    if (l >= 2) mb += 0.5f * g2[w.at(l - 1, j, i)];
+    w.p[B_RRW][q] += mb;
+  }
+}
+
 // mass fluxes and divx transposed, then the stage terms (grid level l = 1..nk+1)
 __global__ __launch_bounds__(64) void k_b_mass_terms(W w) {
   CELL_INDEX
@@ -1703,7 +1652,6 @@ int gm1hip_dry_stage_adjoint(void* const* table, const float* consts, int ni, in
   GM1_LAUNCH_CHECK();
   for (int n = nloop; n >= 1; --n) {
     hipLaunchKernelGGL(k_b_ssA, g, b, 0, s, w, n);
-    hipLaunchKernelGGL(k_b_ssA2, g, b, 0, s, w);
     hipLaunchKernelGGL(k_b_ssB, gc, b, 0, s, w);
     hipLaunchKernelGGL(k_b_ssC, g, b, 0, s, w, n);
     hipLaunchKernelGGL(k_b_ss_faces, g, b, 0, s, w, n);
@@ -1717,9 +1665,10 @@ int gm1hip_dry_stage_adjoint(void* const* table, const float* consts, int ni, in
     hipLaunchKernelGGL(k_b_advh_flux, comp == 2 ? gw : g, b, 0, s, w, comp);
     hipLaunchKernelGGL(k_b_advh_gather, comp == 2 ? gw : g, b, 0, s, w, comp);
   }
-  hipLaunchKernelGGL(k_b_advu, g, b, 0, s, w, 2);
-  hipLaunchKernelGGL(k_b_advv, g, b, 0, s, w, 2);
-  hipLaunchKernelGGL(k_b_advw, g, b, 0, s, w, 2);
+  for (int comp = 0; comp < 3; ++comp) {
+    hipLaunchKernelGGL(k_b_advz_flux, gw, b, 0, s, w, comp);
+    hipLaunchKernelGGL(k_b_advz_gather, gw, b, 0, s, w, comp);
+  }
   hipLaunchKernelGGL(k_b_mass_terms, gw, b, 0, s, w);
   GM1_LAUNCH_CHECK();
   return 0;
