@@ -286,6 +286,10 @@ __global__ __launch_bounds__(SS_T) void k_snd_ssopen2(SoundArgs a) {
 // ------------------------------------------------------------------ pgrad
 constexpr int PG_TX = 64, PG_TY = 4;
 
+// SLOPES0: the caller states gxu = gyv = +0 everywhere, so the terrain term tr = (vs/8)*(gxu+gxu) is a
+// signed zero and h + tr = h whenever h != 0; tr (and the vertical differences it needs) is then formed
+// only where h == 0, which keeps every word of the general text.
+template <bool SLOPES0>
 __global__ __launch_bounds__(PG_TX * PG_TY) void k_snd_pgrad(SoundArgs a) {
   const int ni = a.ni, nj = a.nj, nk = a.nk;
   const int gx = (ni + 1 + PG_TX - 1) / PG_TX, gy = (nj + 1 + PG_TY - 1) / PG_TY;
@@ -298,6 +302,21 @@ __global__ __launch_bounds__(PG_TX * PG_TY) void k_snd_pgrad(SoundArgs a) {
   const float halfcp = a.cp * 0.5f, eighth = 0.125f;
   const float* rds = P_rds(a);
   const float* ppd = a.ppd;
+  // the terrain term of a face whose cells are (pj, pi) and (pjm, pim), slope pair gsum
+  auto terrain = [&](int pj, int pi, int pjm, int pim, float gsum) {
+    // vertical differences, same statement order for face and its west (south) cell
+    float dph = 0.0f, dphl = 0.0f, dpl = 0.0f, dpll = 0.0f;
+    if (k < nk) {
+      dph = (ppd[pix(k + 1, pj, pi, ni, nj)] - ppd[pix(k, pj, pi, ni, nj)]) * rds[k + 1];
+      dphl = (ppd[pix(k + 1, pjm, pim, ni, nj)] - ppd[pix(k, pjm, pim, ni, nj)]) * rds[k + 1];
+    }
+    if (k > 1) {
+      dpl = (ppd[pix(k, pj, pi, ni, nj)] - ppd[pix(k - 1, pj, pi, ni, nj)]) * rds[k];
+      dpll = (ppd[pix(k, pjm, pim, ni, nj)] - ppd[pix(k - 1, pjm, pim, ni, nj)]) * rds[k];
+    }
+    const float vs = (dph + dphl) + (dpl + dpll);
+    return (eighth * vs) * gsum;
+  };
   // u face ff = x+1 (1-based) on row j = y
   if (y < nj) {
     const int ff = x + 1;
@@ -306,20 +325,12 @@ __global__ __launch_bounds__(PG_TX * PG_TY) void k_snd_pgrad(SoundArgs a) {
     if (on) {
       const int pj = y + 1;
       const float h = (ppd[pix(k, pj, ff, ni, nj)] - ppd[pix(k, pj, ff - 1, ni, nj)]) * a.rdx;
-      // vertical differences, same statement order for face and its west cell
-      float dph = 0.0f, dphl = 0.0f, dpl = 0.0f, dpll = 0.0f;
-      if (k < nk) {
-        dph = (ppd[pix(k + 1, pj, ff, ni, nj)] - ppd[pix(k, pj, ff, ni, nj)]) * rds[k + 1];
-        dphl = (ppd[pix(k + 1, pj, ff - 1, ni, nj)] - ppd[pix(k, pj, ff - 1, ni, nj)]) * rds[k + 1];
+      float hs = h;
+      if (!SLOPES0 || h == 0.0f) {
+        const int64_t g = ((int64_t)(k - 1) * nj + y) * (ni + 1) + x;
+        hs = h + terrain(pj, ff, pj, ff - 1, a.gxu[g] + a.gxu[g + (int64_t)nj * (ni + 1)]);
       }
-      if (k > 1) {
-        dpl = (ppd[pix(k, pj, ff, ni, nj)] - ppd[pix(k - 1, pj, ff, ni, nj)]) * rds[k];
-        dpll = (ppd[pix(k, pj, ff - 1, ni, nj)] - ppd[pix(k - 1, pj, ff - 1, ni, nj)]) * rds[k];
-      }
-      const float vs = (dph + dphl) + (dpl + dpll);
-      const int64_t g = ((int64_t)(k - 1) * nj + y) * (ni + 1) + x;
-      const float tr = (eighth * vs) * (a.gxu[g] + a.gxu[g + (int64_t)nj * (ni + 1)]);
-      const float pg = (halfcp * (a.thv[pix(k, pj, ff, ni, nj)] + a.thv[pix(k, pj, ff - 1, ni, nj)])) * (h + tr);
+      const float pg = (halfcp * (a.thv[pix(k, pj, ff, ni, nj)] + a.thv[pix(k, pj, ff - 1, ni, nj)])) * hs;
       const int64_t pu = ix3(k, y + NG, x + NG, sy(nj), ux(ni));
       a.u3d[pu] = a.u3d[pu] + a.dts * (a.uten[pu] - pg);
     }
@@ -332,19 +343,12 @@ __global__ __launch_bounds__(PG_TX * PG_TY) void k_snd_pgrad(SoundArgs a) {
     if (on) {
       const int pi = x + 1;
       const float h = (ppd[pix(k, ff, pi, ni, nj)] - ppd[pix(k, ff - 1, pi, ni, nj)]) * a.rdy;
-      float dph = 0.0f, dphl = 0.0f, dpl = 0.0f, dpll = 0.0f;
-      if (k < nk) {
-        dph = (ppd[pix(k + 1, ff, pi, ni, nj)] - ppd[pix(k, ff, pi, ni, nj)]) * rds[k + 1];
-        dphl = (ppd[pix(k + 1, ff - 1, pi, ni, nj)] - ppd[pix(k, ff - 1, pi, ni, nj)]) * rds[k + 1];
+      float hs = h;
+      if (!SLOPES0 || h == 0.0f) {
+        const int64_t g = ((int64_t)(k - 1) * (nj + 1) + y) * ni + x;
+        hs = h + terrain(ff, pi, ff - 1, pi, a.gyv[g] + a.gyv[g + (int64_t)(nj + 1) * ni]);
       }
-      if (k > 1) {
-        dpl = (ppd[pix(k, ff, pi, ni, nj)] - ppd[pix(k - 1, ff, pi, ni, nj)]) * rds[k];
-        dpll = (ppd[pix(k, ff - 1, pi, ni, nj)] - ppd[pix(k - 1, ff - 1, pi, ni, nj)]) * rds[k];
-      }
-      const float vs = (dph + dphl) + (dpl + dpll);
-      const int64_t g = ((int64_t)(k - 1) * (nj + 1) + y) * ni + x;
-      const float tr = (eighth * vs) * (a.gyv[g] + a.gyv[g + (int64_t)(nj + 1) * ni]);
-      const float pg = (halfcp * (a.thv[pix(k, ff, pi, ni, nj)] + a.thv[pix(k, ff - 1, pi, ni, nj)])) * (h + tr);
+      const float pg = (halfcp * (a.thv[pix(k, ff, pi, ni, nj)] + a.thv[pix(k, ff - 1, pi, ni, nj)])) * hs;
       const int64_t pv = ix3(k, y + NG, x + NG, vy(nj), sx(ni));
       a.v3d[pv] = a.v3d[pv] + a.dts * (a.vten[pv] - pg);
     }
@@ -1287,9 +1291,12 @@ static int psolver3_stage(
       if (nk <= SS_MAXK) hipLaunchKernelGGL(k_snd_ssopen2, dim3(edge), dim3(SS_T), 0, s, a);
       else hipLaunchKernelGGL(k_snd_ssopen, dim3((edge + 63) / 64), dim3(64), 0, s, a);
     }
-    GM1_TIMED(TK_ACOUSTIC_PGRAD, s,
-              hipLaunchKernelGGL(k_snd_pgrad, dim3(((ni + 1 + PG_TX - 1) / PG_TX) * ((nj + 1 + PG_TY - 1) / PG_TY) * nk),
-                                 dim3(PG_TX * PG_TY), 0, s, a));
+    // flags & 32: the caller states the terrain slopes gxu, gyv are +0 everywhere
+    const dim3 pgrid(((ni + 1 + PG_TX - 1) / PG_TX) * ((nj + 1 + PG_TY - 1) / PG_TY) * nk);
+    if (flags & 32)
+      GM1_TIMED(TK_ACOUSTIC_PGRAD, s, hipLaunchKernelGGL(k_snd_pgrad<true>, pgrid, dim3(PG_TX * PG_TY), 0, s, a));
+    else
+      GM1_TIMED(TK_ACOUSTIC_PGRAD, s, hipLaunchKernelGGL(k_snd_pgrad<false>, pgrid, dim3(PG_TX * PG_TY), 0, s, a));
     if (staged)
       GM1_TIMED(TK_ACOUSTIC_COLUMN, s, launch_column2<false>(narrow, pipe, a, n, s));
     else

@@ -146,9 +146,20 @@ class Gm1LargeStepState:
     dry_mass: Any = None
     mass1: float | None = None
     column_pass: str = "pipelined"
+    # gxu and gyv are +0 everywhere (flat terrain), measured once from the host metrics
+    slopes_zero: bool = False
     receipts: list = field(default_factory=list)
     # a stretched (flat) column runs the terrain_flag=.false. family
     flat: Any = None
+
+
+def slopes_are_zero(metrics: dict[str, Any]) -> bool:
+    """True when the terrain slopes gxu and gyv are +0 in every word (flat terrain)."""
+    for name in ("gxu", "gyv"):
+        g = np.asarray(metrics[name])
+        if g.size and (np.any(g != 0) or np.any(np.signbit(g))):
+            return False
+    return True
 
 
 def create_gm1_large_step_state(host: dict[str, Any], device: torch.device | str = "cuda",
@@ -165,6 +176,7 @@ def create_gm1_large_step_state(host: dict[str, Any], device: torch.device | str
         host["cfg"], host["prof"], {k: t(v) for k, v in host["base"].items()},
         {k: t(v) for k, v in host["metrics"].items()}, {k: t(v) for k, v in host["arrays"].items()},
         host["numq"], stream)
+    st.slopes_zero = slopes_are_zero(host["metrics"])
     if host.get("levels") is not None:
         st.flat = flat_stretched_column(host["levels"], dev)
     # device copies of the vertical profiles are made here, never inside a step
@@ -225,7 +237,7 @@ def _rk_stage(st: Gm1LargeStepState, nrk: int) -> None:
     launch_cuda_psolver3_terrain_open_stage(
         a, m, ni=ni, nj=nj, nk=nk, nrk=nrk, dt=cfg.dt, nsound=cfg.nsound, dx=cfg.dx, dy=cfg.dy, dz=cfg.dz,
         stream=st.stream, lateral_sides=sides, profiles=st.prof, ablations=TerrainOpenSoundAblations(),
-        column_pass=st.column_pass)
+        column_pass=st.column_pass, terrain_slopes_zero=st.slopes_zero)
     _native.call("gm1hip_post_acoustic", ptr(a["rru"]), ptr(a["rrv"]), ptr(a["rrw"]), ptr(m["rho0_3d"]),
                  ptr(m["rf0"]), ptr(m["rgzu"]), ptr(m["rgzv"]), ptr(m["dzdx"]), ptr(m["dzdy"]), ptr(m["gz"]),
                  ptr(a["thterm"]), ptr(a["divergence"]), ptr(a["theta_div"]), ptr(a["divx"]), ni, nj, nk,

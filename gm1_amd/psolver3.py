@@ -123,7 +123,7 @@ def launch_cuda_psolver3_terrain_open_three_stage(
     ablations: TerrainOpenSoundAblations | None = None, stream: Any | None = None, compiled: Any | None = None,
     lateral_sides: tuple[int, int, int, int] = (2, 2, 2, 2), alph: float | None = None,
     kdiv: float | None = None, profiles: VerticalProfiles | None = None, _nrk_sequence: tuple[int, ...] = (1, 2, 3),
-    column_pass: str = "pipelined", **_: Any,
+    column_pass: str = "pipelined", terrain_slopes_zero: bool = False, **_: Any,
 ) -> CudaPsolver3TerrainOpenLaunch:
     """Enqueue the acoustic RK stages of ``_nrk_sequence`` on ``stream``.
 
@@ -131,7 +131,10 @@ def launch_cuda_psolver3_terrain_open_three_stage(
     outside the Thomas recurrence, 32 columns x 512 threads per workgroup),
     "staged16" (16 x 256), "pipelined"/"pipelined16" (the same with loads
     issued a level ahead) or "register" (one thread walks each column); all
-    produce the same words.
+    produce the same words.  ``terrain_slopes_zero`` states that the metrics'
+    gxu and gyv are +0 everywhere (checked once by the caller, e.g.
+    ``large_step.slopes_are_zero``); the stage then gives the same words with
+    less work.
     """
     if column_pass not in COLUMN_PASSES:
         raise ValueError(f"column_pass must be one of {COLUMN_PASSES}")
@@ -140,6 +143,10 @@ def launch_cuda_psolver3_terrain_open_three_stage(
             raise ValueError(f"{name}={value} is outside CM1's admitted [0, 1]")
     dt, dx, dy, dz = (check_spacing(n, v) for n, v in (("dt", dt), ("dx", dx), ("dy", dy), ("dz", dz)))
     flags = _check_ablations(ablations or TerrainOpenSoundAblations()) | COLUMN_PASSES[column_pass]
+    if terrain_slopes_zero:
+        # the caller vouches that gxu and gyv are +0 everywhere (flat terrain): the pressure gradient
+        # forms its terrain term only where it can change a word
+        flags |= 32
     sides = lateral_code(lateral_sides)
     shapes = terrain_open_sound_shapes(ni, nj, nk)
     a = {}

@@ -187,15 +187,41 @@ def test_periodic_stage_is_translation_equivariant_and_keeps_duplicate_faces():
 # ------------------------------------------------------------------ GPU parity
 
 def _gpu_stage(device, a, m, *, ni, nj, nk, dt, ns, dx, dy, dz, sides, nrks=(1, 2, 3), flags=None,
-               column_pass="staged"):
+               column_pass="staged", slopes_zero=False):
     from gm1_amd.psolver3 import TerrainOpenSoundAblations, launch_cuda_psolver3_terrain_open_stage
 
     da, dm = to_dev(a, device), to_dev(m, device)
     abl = TerrainOpenSoundAblations() if flags is None else flags
     for nrk in nrks:
         launch_cuda_psolver3_terrain_open_stage(da, dm, ni=ni, nj=nj, nk=nk, nrk=nrk, dt=dt, nsound=ns, dx=dx, dy=dy,
-                                                dz=dz, lateral_sides=sides, ablations=abl, column_pass=column_pass)
+                                                dz=dz, lateral_sides=sides, ablations=abl, column_pass=column_pass,
+                                                terrain_slopes_zero=slopes_zero)
     return to_host(da)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sides", [(2, 2, 2, 2), (1, 1, 1, 1)])
+@pytest.mark.parametrize("rest", [False, True])
+def test_gpu_flat_slope_pressure_gradient_bit_exact(device, sides, rest):
+    """terrain_slopes_zero on flat metrics gives the general text's words, zeros and their signs included
+    (a state at rest has h = 0 on every face, where the terrain term is still formed)."""
+    from gm1_amd.large_step import slopes_are_zero
+
+    ni, nj, nk = 40, 33, 12
+    a, m = _acoustic_case(np.random.default_rng(5), ni, nj, nk, terrain=False)
+    assert slopes_are_zero(m)
+    if rest:
+        for n in ("u3d", "v3d", "w3d", "ua", "va", "wa", "uten", "vten", "wten", "ppten", "ppi", "pp3d", "ppx",
+                  "ppd"):
+            a[n][...] = F(0.0)
+    want = {k: v.copy() for k, v in a.items()}
+    got = _gpu_stage(device, a, m, ni=ni, nj=nj, nk=nk, dt=2.0, ns=8, dx=250.0, dy=200.0, dz=150.0, sides=sides,
+                     slopes_zero=True)
+    for nrk in (1, 2, 3):
+        osd.cpu_psolver3_stage_inplace(want, m, ni=ni, nj=nj, nk=nk, dt=2.0, nsound=8, nrk=nrk, dx=250.0, dy=200.0,
+                                       dz=150.0, lateral_sides=sides)
+    for k in OUT:
+        assert_bits_equal(got[k], want[k], f"{k} sides={sides} rest={rest}")
 
 
 @pytest.mark.gpu
