@@ -57,6 +57,7 @@ def parse_args(argv=None):
                         "members on uneven grids, batched (graph replays over streams) against plain "
                         "(sequential eager steps)")
     p.add_argument("--members", type=int, default=24)
+    p.add_argument("--no-autotune", action="store_true", help="ensemble members keep the default column pass")
     p.add_argument("--ensemble-seed", type=int, default=2024)
     p.add_argument("--streams", type=int, default=4)
     p.add_argument("--ni", type=int, default=None)
@@ -141,7 +142,8 @@ def run_ensemble(args) -> int:
     from gm1_amd.fields import ensemble_members
 
     members = ensemble_members(args.ensemble_seed, args.members)
-    states = create_ensemble_states(members, "cuda:0")
+    states = create_ensemble_states(members, "cuda:0", autotune=not args.no_autotune)
+    tuned = [st.column_pass for st in states]
     cells = sum(m.cells for m in members)
     torch.cuda.synchronize()
     print(f"[bench] {len(members)} members resident, {cells} cells", file=sys.stderr, flush=True)
@@ -198,6 +200,7 @@ def run_ensemble(args) -> int:
             "plain_ms_per_step": plain_s * 1e3 / args.steps,
             "batched_over_plain": plain_s / batched_s,
             "member_steps_per_s": len(members) * args.steps / batched_s,
+            "column_passes": {p: tuned.count(p) for p in sorted(set(tuned))},
         },
         "roofline": {
             "bound": "hbm",

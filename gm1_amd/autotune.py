@@ -1,0 +1,109 @@
+"""Choice of the acoustic column pass per grid shape, by timing it (one GPU).
+
+The column pass has word-identical variants (``psolver3.COLUMN_PASSES``):
+32- or 16-column workgroups (the tile width across i), LDS-staged or
+pipelined (loads issued a level ahead).  Which is fastest depends on the
+column count and height, so ``tune_column_pass`` runs each candidate's
+acoustic RK stage on a scratch copy of a large-step state, times the
+column launches alone with the library's event-pair timer, and returns the
+fastest.  Results are kept per (ni, nj, nk, sides, stretched) for the
+process, and in a JSON file when ``GM1_AUTOTUNE_CACHE`` names one.
+
+Choosing a variant never changes a word of the result, so a tuned run
+equals an untuned one bit for bit.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+from pathlib import Path
+from typing import Any
+
+import torch
+
+from . import timing
+from .psolver3 import COLUMN_PASSES, TerrainOpenSoundAblations, launch_cuda_psolver3_terrain_open_stage
+from .stretched import launch_cuda_gm1_flat_acoustic_stage
+
+CANDIDATES = ("pipelined", "staged", "pipelined16", "staged16")
+_memo: dict[str, dict[str, Any]] = {}
+
+
+def _key(st) -> str:
+    cfg = st.cfg
+    return json.dumps([cfg.ni, cfg.nj, cfg.nk, list(cfg.lateral_sides), st.flat is not None])
+
+
+def _cache_path() -> Path | None:
+    p = os.environ.get("GM1_AUTOTUNE_CACHE")
+    return Path(p) if p else None
+
+
+def _load_cache() -> None:
+    path = _cache_path()
+    if path and path.exists():
+        try:
+            _memo.update(json.loads(path.read_text()))
+        except (OSError, ValueError):
+            pass
+
+
+def _save_cache() -> None:
+    path = _cache_path()
+    if path:
+        path.write_text(json.dumps(_memo, indent=1, sort_keys=True))
+
+
+def _stage(st, a, nrk: int, column_pass: str) -> None:
+    cfg = st.cfg
+    kw = dict(ni=cfg.ni, nj=cfg.nj, nk=cfg.nk, nrk=nrk, dt=cfg.dt, nsound=cfg.nsound, dx=cfg.dx, dy=cfg.dy,
+              stream=st.stream, lateral_sides=cfg.lateral_sides, column_pass=column_pass)
+    if st.flat is not None:
+        launch_cuda_gm1_flat_acoustic_stage(a, st.metrics, vertical=st.flat, **kw)
+    else:
+        launch_cuda_psolver3_terrain_open_stage(a, st.metrics, dz=cfg.dz, profiles=st.prof,
+                                                ablations=TerrainOpenSoundAblations(),
+                                                terrain_slopes_zero=st.slopes_zero, **kw)
+
+
+def tune_column_pass(st, repeats: int = 2, candidates=CANDIDATES) -> dict[str, Any]:
+    """Fastest column pass for ``st``'s grid: {"choice", "us": {variant: mean column launch in us}}."""
+    if not _memo:
+        _load_cache()
+    key = _key(st)
+    if key in _memo:
+        return _memo[key]
+    for c in candidates:
+        if c not in COLUMN_PASSES:
+            raise ValueError(f"unknown column pass {c!r}")
+    if st.flat is not None:  # the flat column runs the 32-column passes only
+        candidates = tuple(c for c in candidates if not c.endswith("16"))
+    with torch.cuda.stream(st.stream) if st.stream is not None else torch.cuda.stream(torch.cuda.current_stream()):
+        scratch = {k: (v.clone() if isinstance(v, torch.Tensor) else v) for k, v in st.a.items()}
+    torch.cuda.synchronize()
+    us = {}
+    for c in candidates:
+        _stage(st, scratch, 3, c)  # first launch of a variant: code object load, caches
+        torch.cuda.synchronize()
+        timing.arm("k_snd_column", 64 * repeats + 8)
+        for _ in range(repeats):
+            _stage(st, scratch, 3, c)
+        torch.cuda.synchronize()
+        total_ms, launches = timing.collect()
+        us[c] = total_ms * 1e3 / max(launches, 1)
+    timing.disarm()
+    del scratch
+    choice = min(us, key=us.get)
+    _memo[key] = {"choice": choice, "us": us}
+    _save_cache()
+    return _memo[key]
+
+
+def autotune_state(st, **kw) -> str:
+    """Set ``st.column_pass`` to the tuned choice for its grid and return it."""
+    st.column_pass = tune_column_pass(st, **kw)["choice"]
+    return st.column_pass
+
+
+__all__ = ["CANDIDATES", "autotune_state", "tune_column_pass"]

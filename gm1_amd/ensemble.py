@@ -52,13 +52,22 @@ class Gm1Ensemble:
             s.synchronize()
 
 
-def create_ensemble_states(members: Sequence[CaseConfig], device: Any = "cuda") -> list[Gm1LargeStepState]:
-    """Initialise and upload every member (host fields built one member at a time)."""
+def create_ensemble_states(members: Sequence[CaseConfig], device: Any = "cuda",
+                           autotune: bool = False) -> list[Gm1LargeStepState]:
+    """Initialise and upload every member (host fields built one member at a time).
+
+    ``autotune`` picks each member's column pass by timing (``autotune.tune_column_pass``, once per grid
+    shape); the variants give the same words."""
     out = []
     for cfg in members:
         host = host_initial_state(cfg)
-        out.append(create_gm1_large_step_state(host, device))
+        st = create_gm1_large_step_state(host, device)
         del host
+        if autotune:
+            from .autotune import autotune_state
+
+            autotune_state(st)
+        out.append(st)
     return out
 
 

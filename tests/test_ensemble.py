@@ -73,3 +73,26 @@ def test_gpu_batched_ensemble_equals_each_members_eager_steps(device):
             for name in FIELDS:
                 assert_bits_equal(to_host(b.a[name]), to_host(e.a[name]), f"{b.cfg.name} step {step + 1} {name}")
     assert all(st.step_index == 2 for st in batched)
+
+
+@pytest.mark.gpu
+def test_gpu_autotuned_members_equal_untuned_members(device):
+    import torch
+
+    from gm1_amd import autotune
+    from gm1_amd.ensemble import create_ensemble_states, sequential_steps
+
+    members = small_members()[:2]
+    print("autotune of", [m.name for m in members], flush=True)
+    plain = create_ensemble_states(members, device)
+    tuned = create_ensemble_states(members, device, autotune=True)
+    for st in tuned:
+        rec = autotune.tune_column_pass(st)
+        assert st.column_pass == rec["choice"] in autotune.CANDIDATES
+        assert set(rec["us"]) == set(autotune.CANDIDATES) and all(v > 0 for v in rec["us"].values())
+    sequential_steps(plain, 2)
+    sequential_steps(tuned, 2)
+    torch.cuda.synchronize()
+    for p, t in zip(plain, tuned):
+        for name in FIELDS:
+            assert_bits_equal(to_host(t.a[name]), to_host(p.a[name]), f"{t.cfg.name} tuned {t.column_pass} {name}")
