@@ -292,26 +292,78 @@ __global__ __launch_bounds__(MNT) void k_mom_advu_tile(MomArgs a, int f0, int nf
     return uvz_g<0>(a, kf, j, f, [&](int lev) { return b[lev - (kf - 3)]; });
   };
   float zlo = (live && kb >= 2) ? zf(kb, win) : 0.0f;
-  for (int k = kb; k <= ke; ++k) {
+  // A level's u plane and the rru / rrv pairs its face fluxes average are
+  // requested one level ahead into registers (Lv), so they arrive while the
+  // level before it is formed.  Slot loops run at most PS / XS / YS times.
+  constexpr int PS = ((MTY + 6) * (MTX + 6) + MNT - 1) / MNT;
+  constexpr int XS = (MTY * (MTX + 1) + MNT - 1) / MNT;
+  constexpr int YS = ((MTY + 1) * MTX + MNT - 1) / MNT;
+  struct Lv {
+    float plane[PS], rx0[XS], rx1[XS], ry0[YS], ry1[YS];
+    float unext;
+  };
+  auto fetch = [&](int k) {
+    Lv v;
     const int lnext = k + 4;
-    const float unext = (live && lnext <= nk + 1) ? col[(int64_t)lnext * ust] : 0.0f;
-    {
-      const float* plane = a.u + (int64_t)k * ust;
-      for (int sl = threadIdx.x; sl < (MTY + 6) * (MTX + 6); sl += MNT) {
-        const int r = sl / (MTX + 6), c = sl % (MTX + 6);
-        const int yy = ja + r, xx = fa + c;
-        if (yy < nyu && xx < nxu) lp[r][c] = plane[(int64_t)yy * nxu + xx];
+    v.unext = (live && lnext <= nk + 1) ? col[(int64_t)lnext * ust] : 0.0f;
+    const float* plane = a.u + (int64_t)k * ust;
+#pragma unroll
+    for (int q = 0; q < PS; ++q) {
+      int sl = threadIdx.x + q * MNT;
+      asm volatile("" : "+v"(sl));  // recomputed per level, not held across the march
+      const int r = sl / (MTX + 6), c = sl % (MTX + 6);
+      const int yy = ja + r, xx = fa + c;
+      v.plane[q] = (sl < (MTY + 6) * (MTX + 6) && yy < nyu && xx < nxu) ? plane[(int64_t)yy * nxu + xx] : 0.0f;
+    }
+#pragma unroll
+    for (int q = 0; q < XS; ++q) {
+      int sl = threadIdx.x + q * MNT;
+      asm volatile("" : "+v"(sl));
+      const int r = sl / (MTX + 1), t = sl % (MTX + 1);
+      const int cc = fa - 1 + t, jj = ja + r;
+      v.rx0[q] = v.rx1[q] = 0.0f;
+      if (sl < MTY * (MTX + 1) && jj < nj && cc < fend) {
+        const float* R = a.rru + ix3(k, jj + NG, 0, nyu, nxu);
+        v.rx0[q] = R[wf(cc, ni, pw, pe) + NG];
+        v.rx1[q] = R[wf(cc + 1, ni, pw, pe) + NG];
       }
+    }
+#pragma unroll
+    for (int q = 0; q < YS; ++q) {
+      int sl = threadIdx.x + q * MNT;
+      asm volatile("" : "+v"(sl));
+      const int r = sl / MTX, t = sl % MTX;
+      const int p = ja + r, ff = fa + t;
+      v.ry0[q] = v.ry1[q] = 0.0f;
+      if (sl < (MTY + 1) * MTX && p <= nj && ff < fend) {
+        const float* R = a.rrv + ix3(k, p + NG, 0, nyv, nxv);
+        v.ry0[q] = R[wc(ff, ni, pw, pe) + NG];
+        v.ry1[q] = R[wc(ff - 1, ni, pw, pe) + NG];
+      }
+    }
+    return v;
+  };
+  Lv nx = fetch(kb);
+  for (int k = kb; k <= ke; ++k) {
+    const Lv cu = nx;
+    if (k < ke) nx = fetch(k + 1);
+    const float unext = cu.unext;
+#pragma unroll
+    for (int q = 0; q < PS; ++q) {
+      const int sl = threadIdx.x + q * MNT;
+      const int r = sl / (MTX + 6), c = sl % (MTX + 6);
+      if (sl < (MTY + 6) * (MTX + 6) && ja + r < nyu && fa + c < nxu) lp[r][c] = cu.plane[q];
     }
     __syncthreads();
     // x fluxes of u at centres cc = fa-1 .. fa+MTX-1 of the tile's rows: the
     // mean of the two rru faces times u's fifth-order upwind value
-    for (int sl = threadIdx.x; sl < MTY * (MTX + 1); sl += MNT) {
+#pragma unroll
+    for (int q = 0; q < XS; ++q) {
+      const int sl = threadIdx.x + q * MNT;
       const int r = sl / (MTX + 1), t = sl % (MTX + 1);
       const int cc = fa - 1 + t, jj = ja + r;
-      if (jj < nj && cc < fend) {
-        const float* R = a.rru + ix3(k, jj + NG, 0, nyu, nxu);
-        const float m = 0.5f * (R[wf(cc, ni, pw, pe) + NG] + R[wf(cc + 1, ni, pw, pe) + NG]);
+      if (sl < MTY * (MTX + 1) && jj < nj && cc < fend) {
+        const float m = 0.5f * (cu.rx0[q] + cu.rx1[q]);
         const float* L = lp[r + 3] + t;  // L[c] = u at padded x (cc + NG) - 2 + c
         const float rv = (m >= 0.0f) ? flx5(L[0], L[1], L[2], L[3], L[4]) : flx5(L[5], L[4], L[3], L[2], L[1]);
         lfx[r][t] = m * rv;
@@ -319,17 +371,23 @@ __global__ __launch_bounds__(MNT) void k_mom_advu_tile(MomArgs a, int f0, int nf
     }
     // y fluxes of u at corner rows p = ja .. ja+MTY of the tile's faces; on an
     // open south/north side an inflowing edge corner takes the next row's flux
-    for (int sl = threadIdx.x; sl < (MTY + 1) * MTX; sl += MNT) {
+#pragma unroll
+    for (int q = 0; q < YS; ++q) {
+      const int sl = threadIdx.x + q * MNT;
       const int r = sl / MTX, t = sl % MTX;
       const int p = ja + r, ff = fa + t;
-      if (p <= nj && ff < fend) {
-        const float* R = a.rrv + ix3(k, p + NG, 0, nyv, nxv);
-        const float mt = R[wc(ff, ni, pw, pe) + NG] + R[wc(ff - 1, ni, pw, pe) + NG];
+      if (sl < (MTY + 1) * MTX && p <= nj && ff < fend) {
+        const float mt = cu.ry0[q] + cu.ry1[q];
         int pb = p;
         if (p == 0 && opn(a.obc, 2) && mt >= 0.0f) pb = 1;
         else if (p == nj && opn(a.obc, 3) && mt <= 0.0f) pb = nj - 1;
-        const float* Rb = a.rrv + ix3(k, pb + NG, 0, nyv, nxv);
-        const float m = 0.5f * (Rb[wc(ff, ni, pw, pe) + NG] + Rb[wc(ff - 1, ni, pw, pe) + NG]);
+        float m;
+        if (pb == p) {
+          m = 0.5f * mt;
+        } else {
+          const float* Rb = a.rrv + ix3(k, pb + NG, 0, nyv, nxv);
+          m = 0.5f * (Rb[wc(ff, ni, pw, pe) + NG] + Rb[wc(ff - 1, ni, pw, pe) + NG]);
+        }
         const int r0 = pb - ja;  // LDS row of u at padded row pb + NG - 3
         const int c = t + 3;
         const float rv = (m >= 0.0f)
@@ -416,42 +474,96 @@ __global__ __launch_bounds__(MNT) void k_mom_advv_tile(MomArgs a, int g0, int nr
     return uvz_g<1>(a, kf, g, i, [&](int lev) { return b[lev - (kf - 3)]; });
   };
   float zlo = (live && kb >= 2) ? zf(kb, win) : 0.0f;
-  for (int k = kb; k <= ke; ++k) {
+  // A level's v plane and the rru / rrv pairs its face fluxes average are
+  // requested one level ahead into registers (Lv), as in the u tile.
+  constexpr int PS = ((MTY + 6) * (MTX + 6) + MNT - 1) / MNT;
+  constexpr int XS = (MTY * (MTX + 1) + MNT - 1) / MNT;
+  constexpr int YS = ((MTY + 1) * MTX + MNT - 1) / MNT;
+  struct Lv {
+    float plane[PS], rx0[XS], rx1[XS], ry0[YS], ry1[YS];
+    float vnext;
+  };
+  auto fetch = [&](int k) {
+    Lv v;
     const int lnext = k + 4;
-    const float vnext = (live && lnext <= nk + 1) ? col[(int64_t)lnext * vst] : 0.0f;
-    {
-      const float* plane = a.v + (int64_t)k * vst;
-      for (int sl = threadIdx.x; sl < (MTY + 6) * (MTX + 6); sl += MNT) {
-        const int r = sl / (MTX + 6), c = sl % (MTX + 6);
-        const int yy = ga + r, xx = ia + c;
-        if (yy < nyv && xx < nxv) lp[r][c] = plane[(int64_t)yy * nxv + xx];
+    v.vnext = (live && lnext <= nk + 1) ? col[(int64_t)lnext * vst] : 0.0f;
+    const float* plane = a.v + (int64_t)k * vst;
+#pragma unroll
+    for (int q = 0; q < PS; ++q) {
+      int sl = threadIdx.x + q * MNT;
+      asm volatile("" : "+v"(sl));  // recomputed per level, not held across the march
+      const int r = sl / (MTX + 6), c = sl % (MTX + 6);
+      const int yy = ga + r, xx = ia + c;
+      v.plane[q] = (sl < (MTY + 6) * (MTX + 6) && yy < nyv && xx < nxv) ? plane[(int64_t)yy * nxv + xx] : 0.0f;
+    }
+#pragma unroll
+    for (int q = 0; q < XS; ++q) {
+      int sl = threadIdx.x + q * MNT;
+      asm volatile("" : "+v"(sl));
+      const int r = sl / (MTX + 1), t = sl % (MTX + 1);
+      const int p = ia + t, gg = ga + r;
+      v.rx0[q] = v.rx1[q] = 0.0f;
+      if (sl < MTY * (MTX + 1) && gg < gend && p <= ni) {
+        const int rn = wc(gg, nj, ps, pn) + NG, rs = wc(gg - 1, nj, ps, pn) + NG;
+        v.rx0[q] = a.rru[ix3(k, rn, p + NG, nyu, nxu)];
+        v.rx1[q] = a.rru[ix3(k, rs, p + NG, nyu, nxu)];
       }
+    }
+#pragma unroll
+    for (int q = 0; q < YS; ++q) {
+      int sl = threadIdx.x + q * MNT;
+      asm volatile("" : "+v"(sl));
+      const int r = sl / MTX, t = sl % MTX;
+      const int pc = ga - 1 + r, ii = ia + t;
+      v.ry0[q] = v.ry1[q] = 0.0f;
+      if (sl < (MTY + 1) * MTX && ii < ni && pc < gend) {
+        v.ry0[q] = a.rrv[ix3(k, wf(pc, nj, ps, pn) + NG, ii + NG, nyv, nxv)];
+        v.ry1[q] = a.rrv[ix3(k, wf(pc + 1, nj, ps, pn) + NG, ii + NG, nyv, nxv)];
+      }
+    }
+    return v;
+  };
+  Lv nx = fetch(kb);
+  for (int k = kb; k <= ke; ++k) {
+    const Lv cu = nx;
+    if (k < ke) nx = fetch(k + 1);
+    const float vnext = cu.vnext;
+#pragma unroll
+    for (int q = 0; q < PS; ++q) {
+      const int sl = threadIdx.x + q * MNT;
+      const int r = sl / (MTX + 6), c = sl % (MTX + 6);
+      if (sl < (MTY + 6) * (MTX + 6) && ga + r < nyv && ia + c < nxv) lp[r][c] = cu.plane[q];
     }
     __syncthreads();
     // x fluxes of v at corner columns p = ia .. ia+MTX of the tile's face rows;
     // on an open west/east side an inflowing edge corner takes the next column's
-    for (int sl = threadIdx.x; sl < MTY * (MTX + 1); sl += MNT) {
+#pragma unroll
+    for (int q = 0; q < XS; ++q) {
+      const int sl = threadIdx.x + q * MNT;
       const int r = sl / (MTX + 1), t = sl % (MTX + 1);
       const int p = ia + t, gg = ga + r;
-      if (gg < gend && p <= ni) {
+      if (sl < MTY * (MTX + 1) && gg < gend && p <= ni) {
         const int rn = wc(gg, nj, ps, pn) + NG, rs = wc(gg - 1, nj, ps, pn) + NG;
-        const float mt = a.rru[ix3(k, rn, p + NG, nyu, nxu)] + a.rru[ix3(k, rs, p + NG, nyu, nxu)];
+        const float mt = cu.rx0[q] + cu.rx1[q];
         int pb = p;
         if (p == 0 && opn(a.obc, 0) && mt >= 0.0f) pb = 1;
         else if (p == ni && opn(a.obc, 1) && mt <= 0.0f) pb = ni - 1;
-        const float m = 0.5f * (a.rru[ix3(k, rn, pb + NG, nyu, nxu)] + a.rru[ix3(k, rs, pb + NG, nyu, nxu)]);
+        const float m =
+            (pb == p) ? 0.5f * mt
+                      : 0.5f * (a.rru[ix3(k, rn, pb + NG, nyu, nxu)] + a.rru[ix3(k, rs, pb + NG, nyu, nxu)]);
         const float* L = lp[r + 3] + (pb - ia);  // L[c] = v at padded x (pb + NG) - 3 + c
         const float rv = (m >= 0.0f) ? flx5(L[0], L[1], L[2], L[3], L[4]) : flx5(L[5], L[4], L[3], L[2], L[1]);
         lfx[r][t] = m * rv;
       }
     }
     // y fluxes of v at centres pc = ga-1 .. ga+MTY-1 of the tile's columns
-    for (int sl = threadIdx.x; sl < (MTY + 1) * MTX; sl += MNT) {
+#pragma unroll
+    for (int q = 0; q < YS; ++q) {
+      const int sl = threadIdx.x + q * MNT;
       const int r = sl / MTX, t = sl % MTX;
       const int pc = ga - 1 + r, ii = ia + t;
-      if (ii < ni && pc < gend) {
-        const float m = 0.5f * (a.rrv[ix3(k, wf(pc, nj, ps, pn) + NG, ii + NG, nyv, nxv)] +
-                                a.rrv[ix3(k, wf(pc + 1, nj, ps, pn) + NG, ii + NG, nyv, nxv)]);
+      if (sl < (MTY + 1) * MTX && ii < ni && pc < gend) {
+        const float m = 0.5f * (cu.ry0[q] + cu.ry1[q]);
         const int c = t + 3;
         // LDS row of v at padded row (pc + NG) - 2 is pc + 1 - ga = r
         const float rv = (m >= 0.0f) ? flx5(lp[r][c], lp[r + 1][c], lp[r + 2][c], lp[r + 3][c], lp[r + 4][c])
