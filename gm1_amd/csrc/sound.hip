@@ -1063,11 +1063,15 @@ __device__ __forceinline__ void column2_body(const SoundArgs& a, int n, int tile
     }
   }
   if constexpr (PIPE) {
-    // while lane group 0 runs the recurrence, the other level slots add the
-    // u and v faces of every column of the tile into their time averages
-    if (inside && slot >= 1) {
-      const int ch2 = (nk + C2S - 2) / (C2S - 1);
-      const int kb2 = 1 + (slot - 1) * ch2, ke2 = min(nk, kb2 + ch2 - 1);
+    // while lane group 0 runs the recurrence, the level slots of the other
+    // wavefronts add the u and v faces of every column of the tile into their
+    // time averages (slots sharing wavefront 0 stay idle: work there would run
+    // after the recurrence, on the workgroup's critical path)
+    constexpr int S0 = 64 / C2W;  // first slot outside wavefront 0
+    static_assert(C2S > S0, "the averages need slots outside wavefront 0");
+    if (inside && slot >= S0) {
+      const int ch2 = (nk + (C2S - S0) - 1) / (C2S - S0);
+      const int kb2 = 1 + (slot - S0) * ch2, ke2 = min(nk, kb2 + ch2 - 1);
       average_uv_faces(a, kb2, ke2, ust, uc, vst, vc, last, tavg);
     }
   }
