@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import json
 import os
+import warnings
 from pathlib import Path
 from typing import Any
 
@@ -45,8 +46,8 @@ def _load_cache() -> None:
     if path and path.exists():
         try:
             _memo.update(json.loads(path.read_text()))
-        except (OSError, ValueError):
-            pass
+        except (OSError, ValueError) as exc:  # an unreadable cache is re-measured, not trusted
+            warnings.warn(f"ignoring autotune cache {path}: {exc}", RuntimeWarning, stacklevel=2)
 
 
 def _save_cache() -> None:
@@ -79,7 +80,8 @@ def tune_column_pass(st, repeats: int = 2, candidates=CANDIDATES) -> dict[str, A
             raise ValueError(f"unknown column pass {c!r}")
     if st.flat is not None:  # the flat column runs the 32-column passes only
         candidates = tuple(c for c in candidates if not c.endswith("16"))
-    with torch.cuda.stream(st.stream) if st.stream is not None else torch.cuda.stream(torch.cuda.current_stream()):
+    stream = st.stream if st.stream is not None else torch.cuda.current_stream()
+    with torch.cuda.stream(stream):  # the clones are ordered after the state's own pending work
         scratch = {k: (v.clone() if isinstance(v, torch.Tensor) else v) for k, v in st.a.items()}
     torch.cuda.synchronize()
     us = {}

@@ -96,3 +96,21 @@ def test_gpu_autotuned_members_equal_untuned_members(device):
     for p, t in zip(plain, tuned):
         for name in FIELDS:
             assert_bits_equal(to_host(t.a[name]), to_host(p.a[name]), f"{t.cfg.name} tuned {t.column_pass} {name}")
+
+
+def test_autotune_choice_is_read_from_the_cache_file(tmp_path, monkeypatch):
+    """A cached choice for a grid shape is returned without timing anything."""
+    import json
+    from types import SimpleNamespace
+
+    from gm1_amd import autotune
+
+    cfg = SimpleNamespace(ni=48, nj=40, nk=20, lateral_sides=(2, 2, 2, 2))
+    st = SimpleNamespace(cfg=cfg, flat=None, column_pass="pipelined")
+    path = tmp_path / "tune.json"
+    path.write_text(json.dumps({autotune._key(st): {"choice": "staged16", "us": {"staged16": 1.0}}}))
+    monkeypatch.setenv("GM1_AUTOTUNE_CACHE", str(path))
+    monkeypatch.setattr(autotune, "_memo", {})
+    assert autotune.autotune_state(st) == "staged16" and st.column_pass == "staged16"
+    flat = SimpleNamespace(cfg=cfg, flat=object(), column_pass="pipelined")
+    assert autotune._key(flat) != autotune._key(st)  # the stretched family is tuned on its own
