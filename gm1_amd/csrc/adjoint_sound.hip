@@ -168,22 +168,40 @@ __global__ __launch_bounds__(64) void k_adj_column(Adj a, int last3) {
   auto P = [&](int k) { return pix(k, j, i, ni, nj); };
   float* wb = a.wb;
   const float alph = a.alph, beta = a.beta, kdiv = a.kdiv, dts = a.dts;
-  // A6: update and filter
-  for (int k = 1; k <= nk; ++k) {
-    const int64_t p = P(k);
-    float upb = a.pb[p] + (1.0f + kdiv) * a.pdb[p];
-    float oldb = -kdiv * a.pdb[p];
-    if (last3) {
-      upb = upb + kdiv * a.ppxb[p];
-      oldb = oldb - kdiv * a.ppxb[p];
-      a.ppxb[p] = 0.0f;
+  // A6: update and filter (each chunk of 8 levels' words requested together)
+  for (int k0 = 1; k0 <= nk; k0 += 8) {
+    float pb_[8], pdb_[8], ppxb_[8], s1_[8], s2_[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int k = k0 + u;
+      if (k <= nk) {
+        const int64_t p = P(k);
+        pb_[u] = a.pb[p];
+        pdb_[u] = a.pdb[p];
+        ppxb_[u] = last3 ? a.ppxb[p] : 0.0f;
+        s1_[u] = a.pk1[p] + a.qk[p];
+        s2_[u] = a.pk2[p] - a.qk[p];
+      }
     }
-    a.pdb[p] = 0.0f;
-    a.pb[p] = oldb + upb;
-    a.fpkb[p] = upb;
-    const float s1 = a.pk1[p] + a.qk[p], s2 = a.pk2[p] - a.qk[p];
-    if (k >= 2) wb[(int64_t)k * wst + wc] += alph * s1 * upb;
-    if (k <= nk - 1) wb[(int64_t)(k + 1) * wst + wc] += alph * s2 * upb;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int k = k0 + u;
+      if (k <= nk) {
+        const int64_t p = P(k);
+        float upb = pb_[u] + (1.0f + kdiv) * pdb_[u];
+        float oldb = -kdiv * pdb_[u];
+        if (last3) {
+          upb = upb + kdiv * ppxb_[u];
+          oldb = oldb - kdiv * ppxb_[u];
+          a.ppxb[p] = 0.0f;
+        }
+        a.pdb[p] = 0.0f;
+        a.pb[p] = oldb + upb;
+        a.fpkb[p] = upb;
+        if (k >= 2) wb[(int64_t)k * wst + wc] += alph * s1_[u] * upb;
+        if (k <= nk - 1) wb[(int64_t)(k + 1) * wst + wc] += alph * s2_[u] * upb;
+      }
+    }
   }
   // A5: T^T x = wb(2..nk); T's row k: aa(k) below, bb(k) on, cc(k) above the diagonal
   const float asq = alph * alph;
@@ -195,44 +213,105 @@ __global__ __launch_bounds__(64) void k_adj_column(Adj a, int last3) {
     const float diag = (-(a.pk1[P(k)] + a.qk[P(k)])) + (a.pk2[P(k - 1)] - a.qk[P(k - 1)]);
     return 1.0f - asq * (a.mm[P(k)] * diag);
   };
-  // forward sweep of T^T: row k has CC(k-1) below and AA(k+1) above its diagonal
+  // forward sweep of T^T: row k has CC(k-1) below and AA(k+1) above its
+  // diagonal.  Each chunk of CH levels has its coefficients and right-hand
+  // sides formed into registers before its recurrence runs, so the chunk's
+  // loads are in flight together.
+  constexpr int CH = 8;
   float cprev = 0.0f, dprev = 0.0f;
-  for (int k = 2; k <= nk; ++k) {
-    const float lo = (k == 2) ? 0.0f : CC(k - 1);
-    const float up = (k < nk) ? AA(k + 1) : 0.0f;
-    const float piv = BB(k) - lo * cprev;
-    const float cpk = up / piv;
-    const float dpk = (wb[(int64_t)k * wst + wc] - lo * dprev) / piv;
-    a.ec[P(k)] = cpk;
-    a.ed[P(k)] = dpk;
-    cprev = cpk;
-    dprev = dpk;
+  for (int k0 = 2; k0 <= nk; k0 += CH) {
+    float lo_[CH], up_[CH], bb_[CH], rhs_[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int k = k0 + u;
+      if (k <= nk) {
+        lo_[u] = (k == 2) ? 0.0f : CC(k - 1);
+        up_[u] = (k < nk) ? AA(k + 1) : 0.0f;
+        bb_[u] = BB(k);
+        rhs_[u] = wb[(int64_t)k * wst + wc];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int k = k0 + u;
+      if (k <= nk) {
+        const float piv = bb_[u] - lo_[u] * cprev;
+        const float cpk = up_[u] / piv;
+        const float dpk = (rhs_[u] - lo_[u] * dprev) / piv;
+        a.ec[P(k)] = cpk;
+        a.ed[P(k)] = dpk;
+        cprev = cpk;
+        dprev = dpk;
+      }
+    }
   }
+  // back substitution, CH levels' ec, ed and mm requested at a time
   float xnext = 0.0f;
-  for (int k = nk; k >= 2; --k) {
-    const float x = (k == nk) ? a.ed[P(k)] : a.ed[P(k)] - a.ec[P(k)] * xnext;
-    xnext = x;
-    // dd(k) = w(k) + dts*wten(k) + beta*(-mm)*(pp(k)-pp(k-1)) + alph*(-mm)*((pp+fpk)(k) - (pp+fpk)(k-1))
-    const float mmk = a.mm[P(k)];
-    wb[(int64_t)k * wst + wc] = x;
-    a.wtenb[(int64_t)k * wst + wc] += dts * x;
-    a.pb[P(k)] += (-mmk) * x;
-    a.pb[P(k - 1)] -= (-mmk) * x;
-    a.fpkb[P(k)] += alph * (-mmk) * x;
-    a.fpkb[P(k - 1)] -= alph * (-mmk) * x;
+  for (int k0 = nk; k0 >= 2; k0 -= CH) {
+    float ec_[CH], ed_[CH], mm_[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int k = k0 - u;
+      if (k >= 2) {
+        ec_[u] = a.ec[P(k)];
+        ed_[u] = a.ed[P(k)];
+        mm_[u] = a.mm[P(k)];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int k = k0 - u;
+      if (k >= 2) {
+        const float x = (k == nk) ? ed_[u] : ed_[u] - ec_[u] * xnext;
+        xnext = x;
+        // dd(k) = w(k) + dts*wten(k) + beta*(-mm)*(pp(k)-pp(k-1)) + alph*(-mm)*((pp+fpk)(k) - (pp+fpk)(k-1))
+        const float mmk = mm_[u];
+        wb[(int64_t)k * wst + wc] = x;
+        a.wtenb[(int64_t)k * wst + wc] += dts * x;
+        a.pb[P(k)] += (-mmk) * x;
+        a.pb[P(k - 1)] -= (-mmk) * x;
+        a.fpkb[P(k)] += alph * (-mmk) * x;
+        a.fpkb[P(k - 1)] -= alph * (-mmk) * x;
+      }
+    }
   }
-  // A4 (w part), A3, A2 (w part, ppten)
-  for (int k = 1; k <= nk; ++k) {
-    const float f = a.fpkb[P(k)];
-    wb[(int64_t)k * wst + wc] += beta * a.pk1[P(k)] * f;
-    wb[(int64_t)(k + 1) * wst + wc] += beta * a.pk2[P(k)] * f;
+  // A4 (w part), A3, A2 (w part, ppten), 8 levels' words requested together
+  for (int k0 = 1; k0 <= nk; k0 += 8) {
+    float f_[8], p1_[8], p2_[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (k0 + u <= nk) {
+        f_[u] = a.fpkb[P(k0 + u)];
+        p1_[u] = a.pk1[P(k0 + u)];
+        p2_[u] = a.pk2[P(k0 + u)];
+      }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (k0 + u <= nk) {
+        const int k = k0 + u;
+        wb[(int64_t)k * wst + wc] += beta * p1_[u] * f_[u];
+        wb[(int64_t)(k + 1) * wst + wc] += beta * p2_[u] * f_[u];
+      }
   }
   wb[wst + wc] = 0.0f;
-  for (int k = 1; k <= nk; ++k) {
-    const float f = a.fpkb[P(k)];
-    a.pptenb[P(k)] += dts * f;
-    if (k < nk) wb[(int64_t)(k + 1) * wst + wc] += -beta * a.qk[P(k)] * f;
-    if (k > 1) wb[(int64_t)k * wst + wc] += beta * a.qk[P(k)] * f;
+  for (int k0 = 1; k0 <= nk; k0 += 8) {
+    float f_[8], q_[8], t_[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (k0 + u <= nk) {
+        f_[u] = a.fpkb[P(k0 + u)];
+        q_[u] = a.qk[P(k0 + u)];
+        t_[u] = a.pptenb[P(k0 + u)];
+      }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (k0 + u <= nk) {
+        const int k = k0 + u;
+        const float f = f_[u];
+        a.pptenb[P(k)] = t_[u] + dts * f;
+        if (k < nk) wb[(int64_t)(k + 1) * wst + wc] += -beta * q_[u] * f;
+        if (k > 1) wb[(int64_t)k * wst + wc] += beta * q_[u] * f;
+      }
   }
 }
 
