@@ -642,6 +642,29 @@ __global__ __launch_bounds__(RT) void k_pdef_repair(PdefArgs a) {
   if (live)
     for (int d = 0; d < 4; ++d) cz[d + 1] = cand_z<FLAT>(a, kb - 2 + d, j, i);
   for (int k = kb; k <= ke; ++k) {
+    // this level's cell words, requested before the candidates are formed
+    // (the face pairs whether or not a flag then needs them)
+    float c_ax = 0.0f, c_ay = 0.0f, c_az = 0.0f, c_qa = 0.0f, c_rho = 0.0f, c_sv = 0.0f, c_divx = 0.0f,
+          c_rr0 = 0.0f, c_held = 0.0f, c_uw = 0.0f, c_ue = 0.0f, c_vs = 0.0f, c_vn = 0.0f, c_wb = 0.0f,
+          c_wt = 0.0f;
+    if (live) {
+      const int64_t q = (int64_t)(k - 1) * mst + mcol;
+      c_ax = a.advx[q];
+      c_ay = a.advy[q];
+      c_az = a.advz[q];
+      c_qa = a.qa[q];
+      c_rho = FLAT ? a.rho0[(int64_t)(k - 1) * mst] : a.rho0[q];
+      c_sv = pdef_word(a, k, y, x);
+      c_divx = a.divx[q];
+      c_rr0 = FLAT ? a.rr0[(int64_t)(k - 1) * mst] : a.rr0[q];
+      c_held = a.held[q];
+      c_uw = a.rru[ix3(k, y, x, nys, nxu)];
+      c_ue = a.rru[ix3(k, y, x + 1, nys, nxu)];
+      c_vs = a.rrv[ix3(k, y, x, nyv, nxs)];
+      c_vn = a.rrv[ix3(k, y + 1, x, nyv, nxs)];
+      c_wb = a.rrw[ix3(k, y, x, nys, nxs)];
+      c_wt = a.rrw[ix3(k + 1, y, x, nys, nxs)];
+    }
     for (int slot = threadIdx.x; slot < RY * (RX + 4); slot += RT) {
       const int r = slot / (RX + 4), t = slot % (RX + 4);
       const int jj = j0 + r, c = i0 - 2 + t;
@@ -657,37 +680,37 @@ __global__ __launch_bounds__(RT) void k_pdef_repair(PdefArgs a) {
     __syncthreads();
     if (live) {
       const int64_t q = (int64_t)(k - 1) * mst + mcol;
-      const float sv = pdef_word(a, k, y, x);
+      const float sv = c_sv;
       constexpr bool flat = FLAT;
-      const float r0a = (flat ? a.rho0[(int64_t)(k - 1) * mst] : a.rho0[q]) * a.qa[q];
+      const float r0a = c_rho * c_qa;
       const float rg = flat ? 1.0f : a.rgz[mcol];
-      float ax = a.advx[q], ay = a.advy[q], az = a.advz[q];
+      float ax = c_ax, ay = c_ay, az = c_az;
       float v[5], mass;
       bool flag;
       for (int d = 0; d < 5; ++d) v[d] = lcx[ty][tx + d];
       row_mass(v, mass, flag);
       if (flag) {
-        const float du = a.rru[ix3(k, y, x + 1, nys, nxu)] - a.rru[ix3(k, y, x, nys, nxu)];
+        const float du = c_ue - c_uw;
         const float first = flat ? ((v[2] + mass) - r0a) * a.rdt : (((v[2] + mass) - r0a) * a.rdt) * rg;
         ax = first - (sv * du) * a.rdx;
       }
       for (int d = 0; d < 5; ++d) v[d] = lcy[ty + d][tx];
       row_mass(v, mass, flag);
       if (flag) {
-        const float dv = a.rrv[ix3(k, y + 1, x, nyv, nxs)] - a.rrv[ix3(k, y, x, nyv, nxs)];
+        const float dv = c_vn - c_vs;
         const float first = flat ? ((v[2] + mass) - r0a) * a.rdt : (((v[2] + mass) - r0a) * a.rdt) * rg;
         ay = first - (sv * dv) * a.rdy;
       }
       row_mass(cz, mass, flag);
       if (flag) {
-        const float dw = a.rrw[ix3(k + 1, y, x, nys, nxs)] - a.rrw[ix3(k, y, x, nys, nxs)];
+        const float dw = c_wt - c_wb;
         if (flat) az = ((cz[2] + mass) - r0a) * a.rdt - ((sv * dw) * a.rdz) * a.mh[k];
         else az = (((cz[2] + mass) - r0a) * a.rdt) * rg - (sv * dw) * a.rdz;
       }
       float t = (ax + ay) + az;
-      t = t + sv * a.divx[q];
-      const float adv = flat ? t * a.rr0[(int64_t)(k - 1) * mst] : (t * a.rr0[q]) * a.gz[mcol];
-      const float o = a.qa[q] + a.dt * (a.held[q] + adv);
+      t = t + sv * c_divx;
+      const float adv = flat ? t * c_rr0 : (t * c_rr0) * a.gz[mcol];
+      const float o = c_qa + a.dt * (c_held + adv);
       a.out[q] = (fabsf(o) < 1.0e-30f) ? 0.0f : o;
     }
     __syncthreads();
