@@ -348,6 +348,22 @@ __global__ __launch_bounds__(MNT) void k_mom_advu_tile(MomArgs a, int f0, int nf
     const Lv cu = nx;
     if (k < ke) nx = fetch(k + 1);
     const float unext = cu.unext;
+    // this level's point words, requested before its fluxes are formed
+    float e_dxe = 0.0f, e_dxw = 0.0f, e_sc0 = 0.0f, e_rw = 0.0f, e_re = 0.0f, e_ten = 0.0f;
+    if (live) {
+      const int west = wc(f - 1, ni, pw, pe), east = wc(f, ni, pw, pe);
+      const int64_t row = ((int64_t)(k - 1) * nj + j) * ni;
+      e_dxe = a.divx[row + east];
+      e_dxw = a.divx[row + west];
+      if (FLAT) {
+        e_sc0 = a.rr0[(int64_t)(k - 1) * nj * ni];
+      } else {
+        e_sc0 = a.gzu[(int64_t)j * (ni + 1) + f];
+        e_rw = a.rho0[row + west];
+        e_re = a.rho0[row + east];
+      }
+      e_ten = a.uten[ix3(k, y, x, nyu, nxu)];
+    }
 #pragma unroll
     for (int q = 0; q < PS; ++q) {
       const int sl = threadIdx.x + q * MNT;
@@ -430,11 +446,10 @@ __global__ __launch_bounds__(MNT) void k_mom_advu_tile(MomArgs a, int f0, int nf
       float tot = (ax + ay) + az;
       const int k0 = k - 1;
       const int64_t row = ((int64_t)k0 * nj + j) * ni;
-      tot = tot + uc * (0.5f * (a.divx[row + east] + a.divx[row + west]));
-      const float sc = FLAT ? a.rr0[(int64_t)k0 * nj * ni]
-                            : a.gzu[(int64_t)j * (ni + 1) + f] / (0.5f * (a.rho0[row + west] + a.rho0[row + east]));
+      tot = tot + uc * (0.5f * (e_dxe + e_dxw));
+      const float sc = FLAT ? e_sc0 : e_sc0 / (0.5f * (e_rw + e_re));
       const int64_t pidx = ix3(k, y, x, nyu, nxu);
-      a.uten[pidx] = a.uten[pidx] + tot * sc;
+      a.uten[pidx] = e_ten + tot * sc;
     }
     zlo = zhi;
 #pragma unroll
@@ -528,6 +543,22 @@ __global__ __launch_bounds__(MNT) void k_mom_advv_tile(MomArgs a, int g0, int nr
     const Lv cu = nx;
     if (k < ke) nx = fetch(k + 1);
     const float vnext = cu.vnext;
+    // this level's point words, requested before its fluxes are formed
+    float e_dxn = 0.0f, e_dxs = 0.0f, e_sc0 = 0.0f, e_rs = 0.0f, e_rn = 0.0f, e_ten = 0.0f;
+    if (live) {
+      const int south = wc(g - 1, nj, ps, pn), north = wc(g, nj, ps, pn);
+      const int64_t plane = (int64_t)(k - 1) * nj * ni;
+      e_dxn = a.divx[plane + (int64_t)north * ni + i];
+      e_dxs = a.divx[plane + (int64_t)south * ni + i];
+      if (FLAT) {
+        e_sc0 = a.rr0[plane];
+      } else {
+        e_sc0 = a.gzv[(int64_t)g * ni + i];
+        e_rs = a.rho0[plane + (int64_t)south * ni + i];
+        e_rn = a.rho0[plane + (int64_t)north * ni + i];
+      }
+      e_ten = a.vten[ix3(k, y, x, nyv, nxv)];
+    }
 #pragma unroll
     for (int q = 0; q < PS; ++q) {
       const int sl = threadIdx.x + q * MNT;
@@ -602,13 +633,10 @@ __global__ __launch_bounds__(MNT) void k_mom_advv_tile(MomArgs a, int g0, int nr
       float tot = (ax + ay) + az;
       const int k0 = k - 1;
       const int64_t plane = (int64_t)k0 * nj * ni;
-      tot = tot + vc * (0.5f * (a.divx[plane + (int64_t)north * ni + i] + a.divx[plane + (int64_t)south * ni + i]));
-      const float sc = FLAT ? a.rr0[plane]
-                            : a.gzv[(int64_t)g * ni + i] /
-                                  (0.5f * (a.rho0[plane + (int64_t)south * ni + i] +
-                                           a.rho0[plane + (int64_t)north * ni + i]));
+      tot = tot + vc * (0.5f * (e_dxn + e_dxs));
+      const float sc = FLAT ? e_sc0 : e_sc0 / (0.5f * (e_rs + e_rn));
       const int64_t pidx = ix3(k, y, x, nyv, nxv);
-      a.vten[pidx] = a.vten[pidx] + tot * sc;
+      a.vten[pidx] = e_ten + tot * sc;
     }
     zlo = zhi;
 #pragma unroll
