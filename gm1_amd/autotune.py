@@ -31,9 +31,27 @@ CANDIDATES = ("pipelined", "staged", "pipelined16", "staged16")
 _memo: dict[str, dict[str, Any]] = {}
 
 
+def _library_tag() -> str:
+    """The built library's source hash and arch (a cache written by another build is not reused)."""
+    from .build import RECEIPT
+
+    try:
+        r = json.loads(RECEIPT.read_text())
+        return f"{r.get('arch', '?')}:{r.get('sources_sha256', '?')[:16]}"
+    except (OSError, ValueError):
+        return "unknown"
+
+
 def _key(st) -> str:
     cfg = st.cfg
-    return json.dumps([cfg.ni, cfg.nj, cfg.nk, list(cfg.lateral_sides), st.flat is not None])
+    device = torch.cuda.get_device_name() if torch.cuda.is_available() else "no-device"
+    return json.dumps([_library_tag(), device, cfg.ni, cfg.nj, cfg.nk,
+                       list(cfg.lateral_sides), st.flat is not None])
+
+
+def _valid(entry: Any, flat: bool) -> bool:
+    allowed = tuple(c for c in CANDIDATES if not (flat and c.endswith("16")))
+    return isinstance(entry, dict) and entry.get("choice") in allowed and entry.get("choice") in COLUMN_PASSES
 
 
 def _cache_path() -> Path | None:
@@ -73,7 +91,7 @@ def tune_column_pass(st, repeats: int = 2, candidates=CANDIDATES) -> dict[str, A
     if not _memo:
         _load_cache()
     key = _key(st)
-    if key in _memo:
+    if key in _memo and _valid(_memo[key], st.flat is not None):
         return _memo[key]
     for c in candidates:
         if c not in COLUMN_PASSES:

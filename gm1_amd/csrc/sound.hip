@@ -1256,8 +1256,6 @@ static int psolver3_stage(
   a.dts = dts; a.rdx = 1.0f / dx; a.rdy = 1.0f / dy; a.alph = alph; a.beta = 1.0f - alph; a.kdiv = kdiv;
   a.zt = zt; a.rzt = 1.0f / zt; a.cp = 1005.7f;
   a.flat = flat; a.rdzf = rdz;
-  hipLaunchKernelGGL(k_snd_prepare, dim3((ni + 2 + 63) / 64, nj + 2, nk), dim3(64), 0, s, a);
-  GM1_LAUNCH_CHECK();
   const bool any_open = side_of(sides, 0) == 2u || side_of(sides, 1) == 2u || side_of(sides, 2) == 2u ||
                         side_of(sides, 3) == 2u;
   const int edge = 2 * (ni + nj);
@@ -1272,9 +1270,14 @@ static int psolver3_stage(
   // (a column too tall for the seventh plane runs the six-plane staged pass)
   const bool pipe = (flags & 16) != 0 && column2_lds_bytes(nk, cw, true) <= lds_max;
   const bool staged = !(flags & 4) && column2_lds_bytes(nk, cw, pipe) <= lds_max;
+  // every refusal comes before the first launch, so a refused call leaves the state untouched:
+  // the flat column runs the staged pass only (k_snd_column is the terrain text), and the
+  // radiation pass keeps a column's phase speeds in LDS
+  if (flat && (!staged || narrow)) return (int)hipErrorInvalidValue;
+  if ((flags & 1) && nk > RAD_MAXK) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_snd_prepare, dim3((ni + 2 + 63) / 64, nj + 2, nk), dim3(64), 0, s, a);
+  GM1_LAUNCH_CHECK();
   if (flat) {
-    // the flat column runs the staged pass only (k_snd_column is the terrain text)
-    if (!staged || narrow) return (int)hipErrorInvalidValue;
     for (int n = 1; n <= nloop; ++n) {
       if (any_open)
         hipLaunchKernelGGL(k_snd_ssopen_flat, dim3(gm1_blocks((int64_t)nk * (ni + nj), 256)), dim3(256), 0, s, a);
@@ -1313,7 +1316,6 @@ static int psolver3_stage(
     }
   }
   if (flags & 1) {
-    if (nk > RAD_MAXK) return (int)hipErrorInvalidValue;
     hipLaunchKernelGGL(k_snd_radiation, dim3(edge), dim3(RAD_T), 0, s, a, dx, dy, dttmp);
     GM1_LAUNCH_CHECK();
   }

@@ -11,6 +11,13 @@ bounds a sequential run of small members.
 
 Every member's words are those of its own eager large step: a replay runs
 the captured launches on the member's buffers, and members share no buffer.
+
+Ordering: ``step`` forks once -- every member stream waits for the caller's
+current stream and for each member state's bound stream -- then replays
+all members, then joins once: the current stream and every member's bound
+stream wait for all member streams.  A perturbation or upload queued on
+either before ``step`` is seen by the replay, and work queued after it sees
+the advanced members, with no host synchronisation.
 """
 
 from __future__ import annotations
@@ -42,10 +49,25 @@ class Gm1Ensemble:
     def cells(self) -> int:
         return sum(st.cfg.cells for st in self.states)
 
+    def _callers(self) -> list:
+        cur = torch.cuda.current_stream()
+        out = [cur]
+        for st in self.states:
+            if st.stream is not None and all(st.stream != s for s in out):
+                out.append(st.stream)
+        return [s for s in out if all(s != m for m in self.streams)]
+
     def step(self) -> None:
-        """Enqueue ``steps_per_graph`` large steps of every member (asynchronous)."""
+        """Enqueue ``steps_per_graph`` large steps of every member (asynchronous, fork/join ordered)."""
+        callers = self._callers()
+        for m in self.streams:
+            for c in callers:
+                m.wait_stream(c)
         for g in self.graphs:
-            g.replay()
+            g.replay(ordered=False)
+        for c in callers:
+            for m in self.streams:
+                c.wait_stream(m)
 
     def synchronize(self) -> None:
         for s in self.streams:

@@ -7,6 +7,12 @@ Here every operator of ``launch_cuda_gm1_large_step`` enqueues on one stream
 with no host synchronisation and no allocation, so ``steps`` consecutive
 large steps are captured once with ``torch.cuda.CUDAGraph`` and replayed;
 the replay reads and writes the same device buffers as the eager step.
+
+Ordering: a replay runs on the graph's own stream.  Before it, that stream
+waits for the caller's current stream and the state's bound stream (so an
+upload or perturbation queued there lands first); after it, both of those
+streams wait for the replay (so whatever the caller queues next sees the
+advanced state).  No host synchronisation is involved.
 """
 
 from __future__ import annotations
@@ -38,10 +44,26 @@ class Gm1LargeStepGraph:
         finally:
             state.stream, state.step_index = saved_stream, saved_index
 
-    def replay(self) -> None:
-        """Enqueue the captured steps (asynchronously) on the stream they were captured on."""
+    def _neighbours(self) -> list:
+        out = [torch.cuda.current_stream()]
+        if self.state.stream is not None and self.state.stream != out[0]:
+            out.append(self.state.stream)
+        return [s for s in out if s != self.stream]
+
+    def replay(self, ordered: bool = True) -> None:
+        """Enqueue the captured steps (asynchronously).
+
+        ``ordered`` (the default) makes the replay wait for the caller's current stream and the state's
+        stream, and makes them wait for the replay.  A caller that orders a batch of replays itself (the
+        ensemble forks once before its members and joins once after) passes ``ordered=False``.
+        """
+        around = self._neighbours() if ordered else []
+        for s in around:
+            self.stream.wait_stream(s)
         with torch.cuda.stream(self.stream):
             self.graph.replay()
+        for s in around:
+            s.wait_stream(self.stream)
         self.state.step_index += self.steps
 
 

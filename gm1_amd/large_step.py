@@ -177,6 +177,10 @@ def create_gm1_large_step_state(host: dict[str, Any], device: torch.device | str
         {k: t(v) for k, v in host["metrics"].items()}, {k: t(v) for k, v in host["arrays"].items()},
         host["numq"], stream)
     st.slopes_zero = slopes_are_zero(host["metrics"])
+    if st.slopes_zero:  # the device copies are checked too, before any step can be captured
+        from .psolver3 import verify_terrain_slopes_zero
+
+        st.slopes_zero = verify_terrain_slopes_zero(st.metrics)
     if host.get("levels") is not None:
         st.flat = flat_stretched_column(host["levels"], dev)
     # device copies of the vertical profiles are made here, never inside a step

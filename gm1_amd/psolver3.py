@@ -98,6 +98,33 @@ class CudaPsolver3TerrainOpenLaunch:
     receipt: dict[str, Any]
 
 
+# (data_ptr, version) of gxu and gyv tensors whose every word has been checked to be +0
+_slopes_checked: set[tuple] = set()
+
+
+def _slopes_key(metrics: Mapping[str, Any]) -> tuple:
+    return tuple((int(metrics[n].data_ptr()), int(metrics[n]._version)) for n in ("gxu", "gyv"))
+
+
+def verify_terrain_slopes_zero(metrics: Mapping[str, Any]) -> bool:
+    """Check once on the device that gxu and gyv are +0 in every word (reads one flag back).
+
+    The result is remembered for these tensors until they are written again, so the check
+    runs outside any step (a step under graph capture may not read back to the host).
+    """
+    key = _slopes_key(metrics)
+    if key in _slopes_checked:
+        return True
+    bad = False
+    for name in ("gxu", "gyv"):
+        g = metrics[name]
+        if g.numel():
+            bad = bad or bool(torch.any((g != 0) | torch.signbit(g)).item())
+    if not bad:
+        _slopes_checked.add(key)
+    return not bad
+
+
 # column pass name -> the stage call's flag bits: "staged" is 32 columns x 512
 # threads per workgroup, "staged16" 16 x 256; the "pipelined" forms issue each
 # level's loads a level ahead and run the u/v time averages beside the
@@ -143,10 +170,6 @@ def launch_cuda_psolver3_terrain_open_three_stage(
             raise ValueError(f"{name}={value} is outside CM1's admitted [0, 1]")
     dt, dx, dy, dz = (check_spacing(n, v) for n, v in (("dt", dt), ("dx", dx), ("dy", dy), ("dz", dz)))
     flags = _check_ablations(ablations or TerrainOpenSoundAblations()) | COLUMN_PASSES[column_pass]
-    if terrain_slopes_zero:
-        # the caller vouches that gxu and gyv are +0 everywhere (flat terrain): the pressure gradient
-        # forms its terrain term only where it can change a word
-        flags |= 32
     sides = lateral_code(lateral_sides)
     shapes = terrain_open_sound_shapes(ni, nj, nk)
     a = {}
@@ -157,6 +180,16 @@ def launch_cuda_psolver3_terrain_open_three_stage(
         scalar = require(scalar, "scalar", shapes["scalar"])
     ms = terrain_static_shapes(ni=ni, nj=nj, nk=nk)
     m = {name: require(metrics[name], name, ms[name]) for name in METRIC_NAMES}
+    if terrain_slopes_zero:
+        # gxu and gyv are +0 everywhere (flat terrain): the pressure gradient then forms its terrain
+        # term only where it can change a word.  The claim is checked on the device (once per
+        # tensor version; a check inside a graph capture is refused rather than trusted).
+        if _slopes_key(m) not in _slopes_checked and torch.cuda.is_current_stream_capturing():
+            raise ValueError("terrain_slopes_zero=True on metrics not yet checked; call "
+                             "verify_terrain_slopes_zero(metrics) before capturing")
+        if not verify_terrain_slopes_zero(m):
+            raise ValueError("terrain_slopes_zero=True but gxu/gyv hold a word other than +0")
+        flags |= 32
     no_alias({**a, "scalar": scalar}, "terrain/open sound arrays must not alias")
     prof = profiles or terrain_coordinate_profiles(nk, dz)
     dprof = device_profiles(prof, a["u3d"].device)
@@ -202,5 +235,5 @@ def launch_cuda_psolver3_terrain_open_stage(
 __all__ = [
     "CM1_ALPH_F32", "CM1_CP_F32", "CM1_KDIV_F32", "CudaPsolver3TerrainOpenLaunch", "TerrainOpenSoundAblations",
     "launch_cuda_psolver3_terrain_open_stage", "launch_cuda_psolver3_terrain_open_three_stage",
-    "terrain_open_sound_shapes",
+    "terrain_open_sound_shapes", "verify_terrain_slopes_zero",
 ]
