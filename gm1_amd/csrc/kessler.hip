@@ -62,6 +62,11 @@ struct KesArgs {
   double* colbud;   // [5][nj][ni]: pdefq qv, qc, qr, fallout, surface energy
   double* wavebud;  // [5][nk][nj][nseg]: autoconversion, accretion, evaporation, condensation, cloud evaporation
   int* wavestat;    // [nk][nj][nseg][2]: max satadj iterations, failures
+  // per-cell branch decisions (null: not written).  Bits 0..6: qc > 0, autoconversion
+  // (qc > 0.001), accretion, evaporation (qr > 0 and qv < qvs), evaporation capped at
+  // saturation, the ar + cr > qc limiter, the theta/pi' update (er > 1e-7); bit 7: the
+  // saturation adjustment ran; bits 8..15: its iteration count; bit 16: it did not converge.
+  unsigned* decisions;
   // held rates of the large-step transition (eqtset 2; null when not recapturing)
   float* qpten; float* qtten; float* qvten; float* qcten;
   int ni, nj, nk, nseg, eqtset;
@@ -270,24 +275,35 @@ __global__ __launch_bounds__(64) void k_kes_cell(KesArgs a) {
     float pisum = pi0 + pp;
     const float tmp = (th0 + th) * pisum;
     float ar = 0.0f, cr = 0.0f, er = 0.0f;
+    unsigned word = 0u;
     if (qc > 0.0f) {
+      word |= 1u;
+      if (qc - 0.001f > 0.0f) word |= 2u;
       ar = fmaxf(0.001f * (qc - 0.001f), 0.0f);
       ar = ar * dt;
     }
-    if (qc > 0.0f && qr > 0.0f) cr = ((2.2f * qc) * pow_r(qr, 0.875f)) * dt;
+    if (qc > 0.0f && qr > 0.0f) {
+      word |= 4u;
+      cr = ((2.2f * qc) * pow_r(qr, 0.875f)) * dt;
+    }
     if (qr > 0.0f) {
       const float qvs = sat_qv(c, tmp, prs, prs);
       if (qv < qvs) {
+        word |= 8u;
         const float rhoqr = rho * qr;
         const float vent = 1.6f + 30.3922f * pow_r(rhoqr, 0.2046f);
         const float deficit = 1.0f - qv / qvs;
         const float num = (vent * deficit) * pow_r(rhoqr, 0.525f);
         const float den = (2.03e4f + 9.584e6f / (qvs * prs)) * rho;
         er = fminf((num / den) * dt, qr);
-        if (qv + er > qvs) er = qvs - qv;
+        if (qv + er > qvs) {
+          word |= 16u;
+          er = qvs - qv;
+        }
       }
     }
     if (ar + cr > qc) {
+      word |= 32u;
       const float term1 = ar + cr;
       ar = (qc * ar) / term1;
       cr = (qc * cr) / term1;
@@ -295,6 +311,7 @@ __global__ __launch_bounds__(64) void k_kes_cell(KesArgs a) {
     qv = qv + er;
     qc = qc - (ar + cr);
     qr = qr + ((ar + cr) - er);
+    if (er > 1.0e-7f) word |= 64u;
     if (er > 1.0e-7f) {
       if (a.eqtset == 2) {
         const float cpml = (CP + CPV * qv) + CPL * (qc + qr);
@@ -389,6 +406,8 @@ __global__ __launch_bounds__(64) void k_kes_cell(KesArgs a) {
     a.qv[p] = qv;
     a.qc[p] = qc;
     a.qr[p] = qr;
+    if (iters > 0) word |= 128u | ((unsigned)(iters & 0xFF) << 8) | (fail ? (1u << 16) : 0u);
+    if (a.decisions) a.decisions[p] = word;
     if (a.qpten) {  // eqtset=2 held recapture (cuda_gm1_kessler_large_step.py:281-285)
       a.qpten[p] = (pp - a.qpten[p]) / dt;
       a.qtten[p] = (th - a.qtten[p]) / dt;
@@ -632,12 +651,13 @@ int64_t gm1hip_kessler_scratch_bytes(int ni, int nj, int nk) {
   return col + wave + stat + rows + 8LL * 2048 + 64;
 }
 
-int gm1hip_kessler_driver(const float* ruh, const float* rvh, const float* rmh, const float* mh, const float* pi0,
-                          const float* th0, float* rho, float* rr, float* pp3d, float* th3d, float* prs, float* qv,
-                          float* qc, float* qr, float* fall_velocity, float* rain, float* prate, double* asq,
-                          double* qbudget, int* status, float* qpten, float* qtten, float* qvten, float* qcten,
-                          void* scratch, int64_t scratch_bytes, int ni, int nj, int nk, float dx, float dy, float dz,
-                          float dt, int eqtset, void* stream) {
+int gm1hip_kessler_driver_decisions(const float* ruh, const float* rvh, const float* rmh, const float* mh,
+                                    const float* pi0, const float* th0, float* rho, float* rr, float* pp3d,
+                                    float* th3d, float* prs, float* qv, float* qc, float* qr, float* fall_velocity,
+                                    float* rain, float* prate, double* asq, double* qbudget, int* status,
+                                    float* qpten, float* qtten, float* qvten, float* qcten, unsigned* decisions,
+                                    void* scratch, int64_t scratch_bytes, int ni, int nj, int nk, float dx, float dy,
+                                    float dz, float dt, int eqtset, void* stream) {
   if (eqtset != 1 && eqtset != 2) return (int)hipErrorInvalidValue;
   if (scratch_bytes < gm1hip_kessler_scratch_bytes(ni, nj, nk)) return (int)hipErrorInvalidValue;
   hipStream_t s = (hipStream_t)stream;
@@ -646,6 +666,7 @@ int gm1hip_kessler_driver(const float* ruh, const float* rvh, const float* rmh, 
   a.rho = rho; a.rr = rr; a.pp3d = pp3d; a.th3d = th3d; a.prs = prs; a.qv = qv; a.qc = qc; a.qr = qr;
   a.vt = fall_velocity; a.rain = rain; a.prate = prate;
   a.qpten = qpten; a.qtten = qtten; a.qvten = qvten; a.qcten = qcten;
+  a.decisions = decisions;
   a.ni = ni; a.nj = nj; a.nk = nk; a.nseg = (ni + 63) / 64; a.eqtset = eqtset;
   a.dx = dx; a.dy = dy; a.dz = dz; a.dt = dt;
   char* base = (char*)scratch;
@@ -666,6 +687,17 @@ int gm1hip_kessler_driver(const float* ruh, const float* rvh, const float* rmh, 
                      dz);
   GM1_LAUNCH_CHECK();
   return 0;
+}
+
+int gm1hip_kessler_driver(const float* ruh, const float* rvh, const float* rmh, const float* mh, const float* pi0,
+                          const float* th0, float* rho, float* rr, float* pp3d, float* th3d, float* prs, float* qv,
+                          float* qc, float* qr, float* fall_velocity, float* rain, float* prate, double* asq,
+                          double* qbudget, int* status, float* qpten, float* qtten, float* qvten, float* qcten,
+                          void* scratch, int64_t scratch_bytes, int ni, int nj, int nk, float dx, float dy, float dz,
+                          float dt, int eqtset, void* stream) {
+  return gm1hip_kessler_driver_decisions(ruh, rvh, rmh, mh, pi0, th0, rho, rr, pp3d, th3d, prs, qv, qc, qr,
+                                         fall_velocity, rain, prate, asq, qbudget, status, qpten, qtten, qvten, qcten,
+                                         nullptr, scratch, scratch_bytes, ni, nj, nk, dx, dy, dz, dt, eqtset, stream);
 }
 
 int gm1hip_kessler_transition_pre(const float* pp_family, float* pp3d, float* th3d, float* qv, float* qc, float* qpten,

@@ -27,6 +27,7 @@ I64 = ctypes.c_int64
 signature("gm1hip_kessler_scratch_bytes", I, I, I, restype=I64)
 signature("gm1hip_dry_mass_scratch_bytes", I, I, I, restype=I64)
 signature("gm1hip_kessler_driver", *([P] * 25), I64, I, I, I, Fl, Fl, Fl, Fl, I, P)
+signature("gm1hip_kessler_driver_decisions", *([P] * 26), I64, I, I, I, Fl, Fl, Fl, Fl, I, P)
 signature("gm1hip_kessler_transition_pre", *([P] * 14), I, I, I, Fl, I, I, P)
 signature("gm1hip_apmasscon", *([P] * 13), D, P, I64, I, I, I, P)
 
@@ -113,25 +114,35 @@ def _check(arrays: Mapping[str, Any], ni: int, nj: int, nk: int) -> dict[str, to
 def launch_cuda_gm1_kessler_driver(
     arrays: Mapping[str, Any], *, ni: int, nj: int, nk: int, dx: float, dy: float, dz: float, dt: float,
     selection: Gm1KesslerDriverSelection = DEFAULT_GM1_KESSLER_DRIVER_SELECTION, stream: Any | None = None,
-    workspace: Gm1KesslerDriverWorkspace | None = None, held: Mapping[str, Any] | None = None, **_: Any,
+    workspace: Gm1KesslerDriverWorkspace | None = None, held: Mapping[str, Any] | None = None,
+    decisions: torch.Tensor | None = None, **_: Any,
 ) -> Gm1KesslerDriverWorkspace:
     """Every ptype=1 phase (pdefq, fall speed, fallout, local, satadj) in place.
 
     With ``held`` (qpten/qtten/qvten/qcten holding the snapshot of the
     eqtset=2 transition) the held rates are recaptured in the same pass.
+    ``decisions`` (int32 (nk, nj, ni), optional) receives each cell's branch
+    word: bits 0..6 the local-conversion branches (qc > 0, autoconversion,
+    accretion, evaporation, evaporation capped at saturation, the ar + cr > qc
+    limiter, the er > 1e-7 update), bit 7 the saturation adjustment ran, bits
+    8..15 its iteration count, bit 16 it did not converge.
     """
     validate_selection(selection)
     a = _check(arrays, ni, nj, nk)
+    if decisions is not None and (not isinstance(decisions, torch.Tensor) or decisions.dtype != torch.int32
+                                  or tuple(decisions.shape) != (nk, nj, ni) or not decisions.is_contiguous()
+                                  or decisions.device != a["rho"].device):
+        raise ValueError(f"decisions must be a contiguous int32 tensor of shape {(nk, nj, ni)} on the device")
     if workspace is None:
         workspace = create_cuda_gm1_kessler_driver_workspace(ni, nj, nk, a["rho"].device)
     h = held or {}
     _native.call(
-        "gm1hip_kessler_driver",
+        "gm1hip_kessler_driver_decisions",
         *(ptr(a[n]) for n in ("ruh", "rvh", "rmh", "mh", "pi0", "th0", "rho", "rr", "pp3d", "th3d", "prs", "qv",
                               "qc", "qr", "fall_velocity", "rain", "prate", "asq", "qbudget")),
         ptr(workspace.status), ptr(h.get("qpten")), ptr(h.get("qtten")), ptr(h.get("qvten")), ptr(h.get("qcten")),
-        ptr(workspace.scratch), workspace.scratch.numel(), ni, nj, nk, float(dx), float(dy), float(dz), float(dt),
-        selection.equation_set, stream_handle(stream))
+        ptr(decisions), ptr(workspace.scratch), workspace.scratch.numel(), ni, nj, nk, float(dx), float(dy),
+        float(dz), float(dt), selection.equation_set, stream_handle(stream))
     return workspace
 
 

@@ -247,6 +247,24 @@ int gm1hip_kessler_driver(const float* ruh, const float* rvh, const float* rmh, 
                           void* scratch, int64_t scratch_bytes, int ni, int nj, int nk, float dx, float dy, float dz,
                           float dt, int eqtset, void* stream);
 
+// gm1hip_kessler_driver, plus one uint32 word per cell (decisions, (nk,nj,ni),
+// may be NULL) recording the branches the cell took: bit 0 qc > 0, bit 1
+// autoconversion (qc > 0.001), bit 2 accretion, bit 3 evaporation (qr > 0 and
+// qv < qvs), bit 4 evaporation capped at saturation, bit 5 the ar + cr > qc
+// limiter, bit 6 the theta/pi' update (er > 1e-7), bit 7 the saturation
+// adjustment ran, bits 8..15 its iteration count, bit 16 it did not converge.
+// The reference's driver takes these branches inside
+// cpu_gm1_kessler_local_inplace (cuda_gm1_kessler_local.py:290-416) and
+// _cpu_satadj (cuda_gm1_kessler_driver.py:337-428); the word lets a caller
+// check them cell by cell.
+int gm1hip_kessler_driver_decisions(const float* ruh, const float* rvh, const float* rmh, const float* mh,
+                                    const float* pi0, const float* th0, float* rho, float* rr, float* pp3d,
+                                    float* th3d, float* prs, float* qv, float* qc, float* qr, float* fall_velocity,
+                                    float* rain, float* prate, double* asq, double* qbudget, int* status,
+                                    float* qpten, float* qtten, float* qvten, float* qcten, unsigned* decisions,
+                                    void* scratch, int64_t scratch_bytes, int ni, int nj, int nk, float dx, float dy,
+                                    float dz, float dt, int eqtset, void* stream);
+
 // First half of cpu_gm1_kessler_large_step_inplace
 // (cuda_gm1_kessler_large_step.py:238-268): eqtset=2 held remove/snapshot
 // (state -= rate*dt, rate <- state) and the pre-physics equation of state.

@@ -191,6 +191,10 @@ def local_conversion(a, *, dt, equation_set, temperature_override=None):
         term1 = ar + cr
         ar2 = np.where(over, (qc * ar) / term1, ar)
         cr2 = np.where(over, (qc * cr) / term1, cr)
+    branches = {"qc_positive": qc > ZERO, "autoconverting": (qc > ZERO) & ((qc - F(0.001)) > ZERO),
+                "accreting": (qc > ZERO) & (qr > ZERO), "evaporating": (qr > ZERO) & (qv < qvs),
+                "evaporation_capped": (qr > ZERO) & (qv < qvs) & (qv + np.minimum((num / den) * dt32, qr) > qvs),
+                "limited": over, "updated": er > F(1.0e-7)}
     ar, cr = ar2, cr2
     qvn = qv + er
     qcn = qc - (ar + cr)
@@ -219,7 +223,7 @@ def local_conversion(a, *, dt, equation_set, temperature_override=None):
         a["th3d"][...] = np.where(upd, th_new, a["th3d"])
         a["rho"][...] = np.where(upd, rho_new, a["rho"])
     a["qv"][...], a["qc"][...], a["qr"][...] = qvn, qcn, qrn
-    return {"tmp": tmp, "autoconversion": ar, "accretion": cr, "evaporation": er}
+    return {"tmp": tmp, "autoconversion": ar, "accretion": cr, "evaporation": er, "branches": branches}
 
 
 def _budget_by_level(values, rr, ruh, rvh, rmh):
@@ -350,7 +354,26 @@ def kessler_driver(a, *, dx, dy, dz, dt, equation_set=2, budgets=True):
             for k in range(len(lv)):
                 a["qbudget"][2 + p] = a["qbudget"][2 + p] + lv[k] * vol
     iters = satadj(a, dx=dx32, dy=dy32, dz=dz32, equation_set=equation_set)
-    return {"temperature_initial": temperature, "local": local, "iterations": iters, "max_fall_substeps": nfall}
+    return {"temperature_initial": temperature, "local": local, "iterations": iters, "max_fall_substeps": nfall,
+            "decisions": decision_words(local["branches"], iters, equation_set)}
+
+
+# Bit layout of a cell's branch decisions (the device driver writes the same word).
+DECISION_BITS = ("qc_positive", "autoconverting", "accreting", "evaporating", "evaporation_capped", "limited",
+                 "updated")
+DECISION_SATADJ = 1 << 7  # the saturation adjustment ran (eqtset 2: qc > 1e-12 or qv > qvs; eqtset 1: always)
+DECISION_ITER_SHIFT = 8  # bits 8..15: its iteration count
+
+
+def decision_words(branches, iterations, equation_set):
+    """uint32 per cell: the local-conversion branches taken, whether satadj ran, and its iteration count."""
+    word = np.zeros(iterations.shape, np.uint32)
+    for b, name in enumerate(DECISION_BITS):
+        word |= branches[name].astype(np.uint32) << np.uint32(b)
+    ran = iterations > 0
+    word |= np.where(ran, np.uint32(DECISION_SATADJ), np.uint32(0))
+    word |= (iterations.astype(np.uint32) & np.uint32(0xFF)) << np.uint32(DECISION_ITER_SHIFT)
+    return word
 
 
 HELD = (("pp3d", "qpten"), ("th3d", "qtten"), ("qv", "qvten"), ("qc", "qcten"))

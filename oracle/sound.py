@@ -462,7 +462,9 @@ def flat_pressure_gradient(a, sides, *, ni, nj, nk, dts, rdx, rdy):
         a["v3d"][sl] = a["v3d"][sl] + (dts * wv) * (a["vten"][sl] - pv)
 
 
-def flat_small_step(a, m, flat, sides, *, ni, nj, nk, dts, rdx, rdy, nrk, n, nloop, alph, beta, kdiv):
+def flat_small_step(a, m, flat, sides, *, ni, nj, nk, dts, rdx, rdy, nrk, n, nloop, alph, beta, kdiv,
+                    tridiagonal=None):
+    """One small step of the flat column; ``tridiagonal`` (a dict) receives the (aa, bb, cc, dd) rows."""
     ng = NG
     ys, xs = slice(ng, ng + nj), slice(ng, ng + ni)
     J, I = slice(1, nj + 1), slice(1, ni + 1)
@@ -499,6 +501,8 @@ def flat_small_step(a, m, flat, sides, *, ni, nj, nk, dts, rdx, rdy, nrk, n, nlo
         fwk = dts * a["wten"][k, ys, xs] + beta * ((-mm[k, J, I]) * pdelta)
         udelta = (pp3d[k, J, I] + fpk[k, J, I]) - (pp3d[k - 1, J, I] + fpk[k - 1, J, I])
         dd = (w[k, ys, xs] + fwk) + alph * ((-mm[k, J, I]) * udelta)
+        if tridiagonal is not None:
+            tridiagonal.setdefault("rows", []).append((k, aa, bb, cc, dd))
         _eliminate_row(k, nk, aa, bb, cc, dd, ec, ed, wt, T)
     _back_substitute(nk, ec, ed, wt, w, ys, xs)
     for k in range(1, nk + 1):

@@ -86,6 +86,82 @@ def test_water_changes_only_by_surface_rain():
     np.testing.assert_allclose(w0 - w1, fell, rtol=0, atol=3e-8)
 
 
+# ---------------------------------------------------------------- branch decisions
+# The north star asks for identical saturation / autoconversion branch decisions.  The
+# decision word (oracle.kessler.decision_words, and the device driver's per-cell output)
+# records every branch of the local conversion (cuda_gm1_kessler_local.py:290-416), whether
+# the saturation adjustment ran, and its iteration count (cuda_gm1_kessler_driver.py:337-428).
+BRANCH_MASK = 0xFF  # local-conversion branches and "satadj ran"
+ITER_SHIFT = ok.DECISION_ITER_SHIFT
+
+
+def c2_like_kessler_fields(ni=64, nj=64, nk=80, seed=7):
+    """Kessler mass arrays on the 320x320x80 case's column (250 m layers, moist WK sounding) with cloud and rain.
+
+    qv is drawn at 0.75..1.06 of saturation, qc and qr are log-normal in 40 % of the cells: the
+    autoconversion, accretion, evaporation and update branches and the saturation adjustment are
+    exercised across the column.  Evaporation capped at saturation and the ar + cr > qc limiter
+    are out of reach at dt = 1.5 s (the capped rate needs er > qvs - qv, about 100 times the
+    evaporation a step allows); both words still have to agree there.
+    """
+    from gm1_amd.fields import moist_c2
+    from gm1_amd.large_step import host_initial_state
+    from oracle import large_step as ol
+
+    cfg = moist_c2(ni, nj, nk)
+    host = host_initial_state(cfg)
+    a = {k: v.copy() for k, v in host["arrays"].items()}
+    a["pp_mass"][...] = a["pp3d"][1 : nk + 1, 1 : nj + 1, 1 : ni + 1]
+    k = {n: (v.copy() if isinstance(v, np.ndarray) else v) for n, v in ol.kessler_arrays(a).items()}
+    ok.eos(k)
+    k["rr"][...] = k["rho"]
+    rng = np.random.default_rng(seed)
+    t = (k["th0"] + k["th3d"]) * (k["pi0"] + k["pp3d"])
+    qvs = ok.saturation_qv(t, k["prs"], k["prs"])
+    k["qv"][...] = (qvs * rng.uniform(0.75, 1.06, qvs.shape)).astype(F)
+    k["qc"][...] = np.where(rng.random(qvs.shape) < 0.4, rng.lognormal(np.log(8e-4), 1.0, qvs.shape), 0).astype(F)
+    k["qr"][...] = np.where(rng.random(qvs.shape) < 0.4, rng.lognormal(np.log(4e-4), 1.2, qvs.shape), 0).astype(F)
+    return cfg, k
+
+
+def _decisions(a, mode, *, dz, dt, eqt, dx=250.0):
+    b = {n: (v.copy() if isinstance(v, np.ndarray) else v) for n, v in a.items()}
+    with ok.transcendentals(mode):
+        res = ok.kessler_driver(b, dx=dx, dy=dx, dz=dz, dt=dt, equation_set=eqt, budgets=False)
+    return res["decisions"], res["iterations"]
+
+
+@pytest.mark.parametrize("case", [0, 1, 2])
+def test_branch_decisions_of_both_transcendental_forms_match_the_reference_on_its_vectors(case):
+    """On the golden cells the reference's float32 transcendentals and the device's rounded ones take every branch
+    alike, and both give the reference's own satadj iteration counts."""
+    g = golden("kessler")
+    a = _inputs(g, f"d{case}_in_")
+    dt, dz, eqt = (float(v) for v in g[f"d{case}_params"])
+    dn, itn = _decisions(a, "numpy", dz=dz, dt=dt, eqt=int(eqt))
+    dr, itr = _decisions(a, "rounded", dz=dz, dt=dt, eqt=int(eqt))
+    np.testing.assert_array_equal(itn, g[f"d{case}_iterations"])
+    np.testing.assert_array_equal(dn, dr)
+    np.testing.assert_array_equal((dn >> ITER_SHIFT) & 0xFF, g[f"d{case}_iterations"])
+
+
+def test_branch_decisions_on_c2_like_fields():
+    """327 680 cells of cloud and rain on the 320x320x80 column: the local-conversion branches and the saturation
+    adjustment's on/off decision are identical for the reference's transcendentals and the device's; the
+    adjustment's iteration count (its |dT| <= 2e-4 test) moves by one in a few cells in 10^4, where the two exp
+    results straddle the threshold."""
+    cfg, k = c2_like_kessler_fields()
+    dn, _ = _decisions(k, "numpy", dz=cfg.dz, dt=cfg.dt, eqt=2)
+    dr, _ = _decisions(k, "rounded", dz=cfg.dz, dt=cfg.dt, eqt=2)
+    for bit in range(8):
+        assert ((dn >> bit) & 1).sum() > 1000 or bit in (4, 5), f"bit {bit} not exercised"
+    np.testing.assert_array_equal(dn & BRANCH_MASK, dr & BRANCH_MASK)
+    it_n = ((dn >> ITER_SHIFT) & 0xFF).astype(np.int64)
+    it_r = ((dr >> ITER_SHIFT) & 0xFF).astype(np.int64)
+    moved = it_n != it_r
+    assert moved.mean() < 5e-4 and np.abs(it_n - it_r).max() <= 1
+
+
 # ---------------------------------------------------------------- GPU parity
 def _device_arrays(a, device):
     import torch
@@ -169,3 +245,51 @@ def test_gpu_transition_matches_oracle(device, hook):
         assert_bits_equal(da[name].cpu().numpy(), a[name], name)
     for name in held:
         assert_bits_equal(dh[name].cpu().numpy(), held[name], name)
+
+
+def _device_decisions(a, device, *, dz, dt, eqt, ni, nj, nk, dx=250.0):
+    import torch
+
+    from gm1_amd.kessler import Gm1KesslerDriverSelection, launch_cuda_gm1_kessler_driver
+
+    da = _device_arrays({n: v for n, v in a.items() if isinstance(v, np.ndarray)}, device)
+    out = torch.zeros((nk, nj, ni), dtype=torch.int32, device=device)
+    launch_cuda_gm1_kessler_driver(da, ni=ni, nj=nj, nk=nk, dx=dx, dy=dx, dz=dz, dt=dt,
+                                   selection=Gm1KesslerDriverSelection(equation_set=eqt), decisions=out)
+    torch.cuda.synchronize()
+    return out.cpu().numpy().view(np.uint32), da
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", [0, 1, 2])
+def test_gpu_branch_decisions_match_the_reference_on_its_vectors(device, case):
+    """Every cell's branch word from the device equals the oracle's in both transcendental forms and carries the
+    reference's satadj iteration counts."""
+    g = golden("kessler")
+    ni, nj, nk = (int(v) for v in g["dims"])
+    a = _inputs(g, f"d{case}_in_")
+    dt, dz, eqt = (float(v) for v in g[f"d{case}_params"])
+    got, _ = _device_decisions(a, device, dz=dz, dt=dt, eqt=int(eqt), ni=ni, nj=nj, nk=nk)
+    dr, _ = _decisions(a, "rounded", dz=dz, dt=dt, eqt=int(eqt))
+    dn, _ = _decisions(a, "numpy", dz=dz, dt=dt, eqt=int(eqt))
+    np.testing.assert_array_equal(got, dr)
+    np.testing.assert_array_equal(got, dn)
+    np.testing.assert_array_equal((got >> ITER_SHIFT) & 0xFF, g[f"d{case}_iterations"])
+
+
+@pytest.mark.gpu
+def test_gpu_branch_decisions_on_c2_like_fields(device):
+    """327 680 cloudy and rainy cells: the device's words equal the rounded oracle's bit for bit (fields too), and
+    the reference-transcendental oracle's in every branch bit."""
+    cfg, k = c2_like_kessler_fields()
+    print(f"kessler decisions {cfg.ni}x{cfg.nj}x{cfg.nk}", flush=True)
+    got, da = _device_decisions(k, device, dz=cfg.dz, dt=cfg.dt, eqt=2, ni=cfg.ni, nj=cfg.nj, nk=cfg.nk)
+    b = {n: (v.copy() if isinstance(v, np.ndarray) else v) for n, v in k.items()}
+    with ok.transcendentals("rounded"):
+        res = ok.kessler_driver(b, dx=250.0, dy=250.0, dz=cfg.dz, dt=cfg.dt, equation_set=2, budgets=False)
+    np.testing.assert_array_equal(got, res["decisions"])
+    for name in FIELDS:
+        assert_bits_equal(da[name].cpu().numpy(), b[name], name)
+    dn, _ = _decisions(k, "numpy", dz=cfg.dz, dt=cfg.dt, eqt=2)
+    np.testing.assert_array_equal(got & BRANCH_MASK, dn & BRANCH_MASK)
+    assert (((got >> ITER_SHIFT) & 0xFF) != ((dn >> ITER_SHIFT) & 0xFF)).mean() < 5e-4

@@ -98,6 +98,97 @@ def test_density_finish_closed_forms():
     np.testing.assert_allclose(rf[1:8], 0.5 * (rho[:-1] + rho[1:]), rtol=1e-7)
 
 
+def _reference_schedule(moist, terrain, numq=3):
+    """The operator order of one large step as the reference's composer runs it.
+
+    Phases and their order: ``gm1_runtime.large_step_schedule`` (gm1_runtime.py:1969-2007);
+    inside them ``CartesianDynamics.finalize_held_tendencies`` (cuda_cartesian_dynamics.py:1014),
+    ``pre_acoustic`` (:1047-1110: stage terms, held momentum, bcwsfc + bc2d from nrk 2 on a
+    terrain column, momentum advection, buoyancy), ``acoustic`` (:1127), ``post_acoustic``
+    (:1204: the flux bridge's getdiv then getdivx, cuda_moist_stage_bridge.py:456-520),
+    ``scalar_moist`` (:1235: theta, then each moisture row; the final stage's rows through the
+    positive-definite chain, cuda_scalar_moist_rk.py:1041-1499), then Kessler's large-step
+    transition and solve3 (cuda_gm1_forecast.py:1414-1457).
+    """
+    out = []
+    if moist:
+        out.append(("held", None))
+    for nrk in (1, 2, 3):
+        out += [("stage_terms", nrk), ("seed_held", nrk)]
+        if nrk >= 2 and terrain:
+            out += [("bcwsfc", nrk), ("bc2d", nrk)]
+        out += [("momentum", nrk), ("buoyancy", nrk), ("acoustic", nrk)]
+        out += [("getdiv", nrk), ("getdivx", nrk)]
+        out += [("pack_theta", nrk), ("scalar_advection", nrk), ("compose_theta", nrk)]
+        if moist:
+            for _ in range(numq):
+                out += [("pack_scalar", nrk), ("pdef" if nrk == 3 else "scalar_advection", nrk),
+                        ("compose_moisture", nrk)]
+    if moist:
+        out.append(("kessler_transition", None))
+    out.append(("solve3_pressure", None))
+    if moist:
+        out.append(("solve3_density", None))
+    return out
+
+
+@pytest.mark.parametrize("moist", [False, True])
+def test_composition_follows_the_reference_schedule(monkeypatch, moist):
+    """The oracle's large step calls its pinned operators in the reference composer's order (outermost calls)."""
+    from oracle import boundary as obc
+    from oracle import kessler as okes
+
+    trace = []
+    depth = [0]
+    stage = [None]
+
+    def wrap(owner, attr, label):
+        fn = getattr(owner, attr)
+
+        def recorder(*args, **kwargs):
+            if depth[0] == 0:
+                trace.append((label, stage[0]))
+            depth[0] += 1
+            try:
+                return fn(*args, **kwargs)
+            finally:
+                depth[0] -= 1
+
+        monkeypatch.setattr(owner, attr, recorder)
+
+    rk_stage = ol.rk_stage
+
+    def staged(a, m, base, prof, cfg, nrk, **kw):
+        stage[0] = nrk
+        try:
+            return rk_stage(a, m, base, prof, cfg, nrk, **kw)
+        finally:
+            stage[0] = None
+
+    monkeypatch.setattr(ol, "rk_stage", staged)
+    for owner, attr, label in (
+            (ost, "predict_and_compose_held", "held"), (ost, "stage_terms", "stage_terms"),
+            (ost, "seed_held", "seed_held"), (obc, "bcwsfc", "bcwsfc"), (obc, "bc2d_open_w_surface", "bc2d"),
+            (ol, "cpu_terrain_open_momentum_advection5_inplace", "momentum"), (ost, "buoyancy", "buoyancy"),
+            (ol, "cpu_psolver3_stage_inplace", "acoustic"), (ost, "getdiv", "getdiv"), (ost, "getdivx", "getdivx"),
+            (ost, "pack_theta", "pack_theta"), (ol, "cpu_scalar_advection5_inplace", "scalar_advection"),
+            (ost, "compose_theta", "compose_theta"), (ost, "pack_scalar", "pack_scalar"),
+            (ol, "pdef_advection", "pdef"), (ost, "compose_moisture", "compose_moisture"),
+            (okes, "large_step_transition", "kessler_transition"), (ost, "solve3_pressure", "solve3_pressure"),
+            (ost, "solve3_density", "solve3_density")):
+        wrap(owner, attr, label)
+    cfg = moist_case(12, 10, 10, (2, 2, 2, 2)) if moist else dry_case(12, 10, 10, (2, 2, 2, 2))
+    host = host_initial_state(cfg)
+    a = {k: v.copy() for k, v in host["arrays"].items()}
+    prof = osound.terrain_coordinate_profiles(cfg.nk, cfg.dz)
+    c = {"ni": cfg.ni, "nj": cfg.nj, "nk": cfg.nk, "dx": cfg.dx, "dy": cfg.dy, "dz": cfg.dz, "dt": cfg.dt,
+         "nsound": cfg.nsound, "lateral_sides": cfg.lateral_sides}
+    ol.large_step(a, host["metrics"], host["base"], prof, c, zt=F(cfg.nk * cfg.dz),
+                  mass_correction=0.0 if moist else None, budgets=False)
+    # bcwsfc also runs inside the acoustic stage's fills; only outermost calls are recorded
+    assert trace == _reference_schedule(moist, terrain=True)
+
+
 def _stage_inputs(rng, ni, nj, nk):
     sh = shapes(ni, nj, nk)
     m = seeded_metrics(rng, ni, nj, nk)

@@ -219,6 +219,105 @@ def test_periodic_stretched_step_is_translation_equivariant():
     assert np.array_equal(np.roll(w0, (dj, di), axis=(1, 2)).view(np.uint32), w1.view(np.uint32))
 
 
+ACOUSTIC_OUTPUTS = ("u3d", "v3d", "w3d", "pp3d", "ppd", "rru", "rrv", "rrw")
+
+
+def _flat_and_terrain_stages(sides, uniform, *, flat_levels=None, flat_metrics=None):
+    """Three acoustic stages of the flat column and of the terrain-coordinate stage on the same flat grid.
+
+    The terrain-coordinate stage (``cpu_psolver3_stage_inplace``, pinned to the
+    reference's golden three-stage vectors) runs on sigma = zh, sigmaf = zf with
+    gz = 1 and zero slopes; there rds = rdz*mf and rdsf = rdz*mh, so the two
+    stages are the same equations in different association.  Returns the
+    worst relative max-norm difference per output over the three stages.
+    """
+    host, a = evolved(sides=sides, uniform=uniform)
+    cfg, m, lv = host["cfg"], host["metrics"], host["levels"]
+    ni, nj, nk = cfg.ni, cfg.nj, cfg.nk
+    assert not np.any(m["dzdx"]) and not np.any(m["gxu"]) and np.all(m["gz"] == 1)
+    ost.stage_terms(a, host["base"], a["th3d"], ol._species6(a["q3d"]), ni=ni, nj=nj, nk=nk, sides=sides)
+    ost.buoyancy_flat(a, m, ni=ni, nj=nj, nk=nk)
+    b = {k: v.copy() for k, v in a.items()}
+    prof = osound.terrain_coordinate_profiles(nk, cfg.dz, sigma=lv.sigma, sigmaf=lv.sigmaf)
+    worst = dict.fromkeys(ACOUSTIC_OUTPUTS, 0.0)
+    for nrk in (1, 2, 3):
+        osound.cpu_flat_stage_inplace(a, flat_metrics or m, flat_levels or lv, ni=ni, nj=nj, nk=nk, dt=cfg.dt,
+                                      nsound=cfg.nsound, nrk=nrk, dx=cfg.dx, dy=cfg.dy, lateral_sides=sides)
+        osound.cpu_psolver3_stage_inplace(b, m, ni=ni, nj=nj, nk=nk, dt=cfg.dt, nsound=cfg.nsound, nrk=nrk,
+                                          dx=cfg.dx, dy=cfg.dy, dz=cfg.dz, lateral_sides=sides, prof=prof,
+                                          zt=lv.ztop)
+        for n in ACOUSTIC_OUTPUTS:
+            x, y = a[n].astype(np.float64), b[n].astype(np.float64)
+            worst[n] = max(worst[n], float(np.abs(x - y).max() / max(np.abs(y).max(), 1e-30)))
+    return worst
+
+
+@pytest.mark.parametrize("uniform", [True, False])
+@pytest.mark.parametrize("sides", [(2, 2, 2, 2), (1, 1, 1, 1)])
+def test_flat_acoustic_stage_equals_the_terrain_coordinate_stage_on_a_flat_grid(sides, uniform):
+    """The flat stage (cuda_gm1_child_psolver3_open.py:340-535) against the golden-pinned terrain stage.
+
+    Same equations, different association: within the 1e-5 relative max-norm of
+    the task's tolerance for every output of three stages (measured <= 1.5e-6),
+    on a uniform and on a stretch_z=1 column.
+    """
+    worst = _flat_and_terrain_stages(sides, uniform)
+    assert max(worst.values()) < 1e-5, worst
+
+
+def test_flat_acoustic_stage_comparison_sees_exchanged_column_factors():
+    """mh and mf exchanged, or c1 and c2, move the stretched flat stage far from the terrain stage."""
+    from dataclasses import replace
+
+    lv = host_initial_state(stretched_case(16, 12, 16, (2, 2, 2, 2)))["levels"]
+    swapped = replace(lv, mh=lv.mf[: lv.nk + 2].copy(), mf=np.concatenate([lv.mh, lv.mh[-1:]]).astype(F))
+    worst = _flat_and_terrain_stages((2, 2, 2, 2), False, flat_levels=swapped)
+    assert max(worst.values()) > 1e-3, worst
+    host = host_initial_state(stretched_case(16, 12, 16, (2, 2, 2, 2)))
+    m = dict(host["metrics"])
+    m["c1"], m["c2"] = m["c2"], m["c1"]
+    worst = _flat_and_terrain_stages((2, 2, 2, 2), False, flat_metrics=m)
+    assert max(worst.values()) > 1e-3, worst
+
+
+def test_flat_tridiagonal_rows_solve_like_a_dense_binary64_solve():
+    """On the stretched column the flat stage's Thomas sweep gives a dense binary64 solve of its rows."""
+    host, a = evolved(sides=(2, 2, 2, 2))
+    cfg, m, lv = host["cfg"], host["metrics"], host["levels"]
+    ni, nj, nk = cfg.ni, cfg.nj, cfg.nk
+    ost.stage_terms(a, host["base"], a["th3d"], ol._species6(a["q3d"]), ni=ni, nj=nj, nk=nk, sides=cfg.lateral_sides)
+    nloop, dts = osound.sound3_stage_schedule_f32(cfg.dt, cfg.nsound, 3)
+    dts = F(dts)
+    osound.flat_prepare_coefficients(a, m, lv, ni=ni, nj=nj, nk=nk, dts=dts)
+    a["ppd"][1 : nk + 1] = a["ppx"][1 : nk + 1]
+    rec = {}
+    rdx, rdy = F(1.0) / F(cfg.dx), F(1.0) / F(cfg.dy)
+    osound.flat_small_step(a, m, lv, cfg.lateral_sides, ni=ni, nj=nj, nk=nk, dts=dts, rdx=rdx, rdy=rdy, nrk=3, n=1,
+                           nloop=nloop, alph=F(0.6), beta=F(0.4), kdiv=F(0.1), tridiagonal=rec)
+    rows = {k: (aa, bb, cc, dd) for k, aa, bb, cc, dd in rec["rows"]}
+    spacing = lv.spacings
+    assert spacing.max() / spacing.min() > 5  # a genuinely stretched column
+    n = nk - 1
+    worst = 0.0
+    for j in range(nj):
+        for i in range(ni):
+            A = np.zeros((n, n))
+            rhs = np.zeros(n)
+            for r, k in enumerate(range(2, nk + 1)):
+                aa, bb, cc, dd = (np.asarray(x, np.float64)[j, i] for x in rows[k])
+                A[r, r] = bb
+                if r > 0:
+                    A[r, r - 1] = aa
+                if r < n - 1:
+                    A[r, r + 1] = cc
+                rhs[r] = dd
+            exact = np.linalg.solve(A, rhs)
+            got = a["w3d"][2 : nk + 1, j + NG, i + NG].astype(np.float64)
+            worst = max(worst, float(np.max(np.abs(got - exact)) / (np.max(np.abs(exact)) + 1e-30)))
+            assert np.all(np.diag(A) > 0.9)
+    assert worst < 1e-5
+
+
 # ----------------------------------------------------------------------------- GPU
 
 
