@@ -19,6 +19,8 @@
 // status words stay on the device; nothing synchronises with the host.
 #include "cm1_common.h"
 
+#include <mutex>
+
 using namespace gm1;
 
 namespace {
@@ -639,6 +641,40 @@ __global__ __launch_bounds__(256) void k_mass_apply(const float* __restrict__ co
 
 }  // namespace
 
+// ------------------------------------------------------------------ the no-allocation driver
+// File-scope device workspace of gm1hip_kessler_driver_noalloc, for callers that
+// cannot allocate (code under HIP graph capture, tight serving loops).  Each GPU
+// holds its own copy for the life of the process, at its own address: host code
+// uses the per-device address from hipGetSymbolAddress, never this name.  Every
+// call on one GPU shares it; each call writes every word it reads, so nothing
+// needs clearing, and calls that overlap on one GPU race.
+constexpr int64_t KESSLER_NOALLOC_MAX_BYTES = 64LL << 20;  // 640x768x80 needs 49.6 MB
+constexpr int KESSLER_NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[KESSLER_NOALLOC_MAX_BYTES];
+
+namespace {
+std::mutex g_scratch_mutex;
+void* g_scratch_address[KESSLER_NOALLOC_MAX_DEVICES] = {};
+
+// The current device's address of `scratch`, looked up once per device.
+int scratch_address(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return (int)e;
+  if (dev < 0 || dev >= KESSLER_NOALLOC_MAX_DEVICES) return (int)hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_address[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return (int)e;
+    if (p == nullptr) return (int)hipErrorInvalidSymbol;
+    g_scratch_address[dev] = p;
+  }
+  *out = g_scratch_address[dev];
+  return 0;
+}
+}  // namespace
+
 extern "C" {
 
 // Scratch the driver needs (bytes), for ni x nj x nk.
@@ -656,10 +692,10 @@ int gm1hip_kessler_driver_decisions(const float* ruh, const float* rvh, const fl
                                     float* th3d, float* prs, float* qv, float* qc, float* qr, float* fall_velocity,
                                     float* rain, float* prate, double* asq, double* qbudget, int* status,
                                     float* qpten, float* qtten, float* qvten, float* qcten, unsigned* decisions,
-                                    void* scratch, int64_t scratch_bytes, int ni, int nj, int nk, float dx, float dy,
+                                    void* work, int64_t work_bytes, int ni, int nj, int nk, float dx, float dy,
                                     float dz, float dt, int eqtset, void* stream) {
   if (eqtset != 1 && eqtset != 2) return (int)hipErrorInvalidValue;
-  if (scratch_bytes < gm1hip_kessler_scratch_bytes(ni, nj, nk)) return (int)hipErrorInvalidValue;
+  if (work_bytes < gm1hip_kessler_scratch_bytes(ni, nj, nk)) return (int)hipErrorInvalidValue;
   hipStream_t s = (hipStream_t)stream;
   KesArgs a{};
   a.pi0 = pi0; a.th0 = th0; a.ruh = ruh; a.rvh = rvh; a.rmh = rmh; a.mh = mh;
@@ -669,7 +705,7 @@ int gm1hip_kessler_driver_decisions(const float* ruh, const float* rvh, const fl
   a.decisions = decisions;
   a.ni = ni; a.nj = nj; a.nk = nk; a.nseg = (ni + 63) / 64; a.eqtset = eqtset;
   a.dx = dx; a.dy = dy; a.dz = dz; a.dt = dt;
-  char* base = (char*)scratch;
+  char* base = (char*)work;
   a.colbud = (double*)base;
   base += 5LL * nj * ni * 8;
   a.wavebud = (double*)base;
@@ -693,11 +729,28 @@ int gm1hip_kessler_driver(const float* ruh, const float* rvh, const float* rmh, 
                           const float* th0, float* rho, float* rr, float* pp3d, float* th3d, float* prs, float* qv,
                           float* qc, float* qr, float* fall_velocity, float* rain, float* prate, double* asq,
                           double* qbudget, int* status, float* qpten, float* qtten, float* qvten, float* qcten,
-                          void* scratch, int64_t scratch_bytes, int ni, int nj, int nk, float dx, float dy, float dz,
+                          void* work, int64_t work_bytes, int ni, int nj, int nk, float dx, float dy, float dz,
                           float dt, int eqtset, void* stream) {
   return gm1hip_kessler_driver_decisions(ruh, rvh, rmh, mh, pi0, th0, rho, rr, pp3d, th3d, prs, qv, qc, qr,
                                          fall_velocity, rain, prate, asq, qbudget, status, qpten, qtten, qvten, qcten,
-                                         nullptr, scratch, scratch_bytes, ni, nj, nk, dx, dy, dz, dt, eqtset, stream);
+                                         nullptr, work, work_bytes, ni, nj, nk, dx, dy, dz, dt, eqtset, stream);
+}
+
+int gm1hip_kessler_driver_noalloc(const float* ruh, const float* rvh, const float* rmh, const float* mh,
+                                  const float* pi0, const float* th0, float* rho, float* rr, float* pp3d, float* th3d,
+                                  float* prs, float* qv, float* qc, float* qr, float* fall_velocity, float* rain,
+                                  float* prate, double* asq, double* qbudget, int* status, float* qpten,
+                                  float* qtten, float* qvten, float* qcten, int ni, int nj, int nk, float dx,
+                                  float dy, float dz, float dt, int eqtset, void* stream) {
+  if (ni <= 0 || nj <= 0 || nk <= 0) return (int)hipErrorInvalidValue;
+  if (gm1hip_kessler_scratch_bytes(ni, nj, nk) > KESSLER_NOALLOC_MAX_BYTES) return (int)hipErrorInvalidValue;
+  void* work = nullptr;
+  const int rc = scratch_address(&work);
+  if (rc != 0) return rc;
+  return gm1hip_kessler_driver_decisions(ruh, rvh, rmh, mh, pi0, th0, rho, rr, pp3d, th3d, prs, qv, qc, qr,
+                                         fall_velocity, rain, prate, asq, qbudget, status, qpten, qtten, qvten, qcten,
+                                         nullptr, work, KESSLER_NOALLOC_MAX_BYTES, ni, nj, nk, dx, dy, dz, dt, eqtset,
+                                         stream);
 }
 
 int gm1hip_kessler_transition_pre(const float* pp_family, float* pp3d, float* th3d, float* qv, float* qc, float* qpten,
@@ -721,12 +774,12 @@ int64_t gm1hip_dry_mass_scratch_bytes(int ni, int nj, int nk) {
 // sum(pi0+pp3d) before, corr[0] = the f32 correction.
 int gm1hip_apmasscon(const float* ruh, const float* rvh, const float* rmh, const float* pi0, const float* th0,
                      const float* th3d, const float* qv, float* pp3d, float* prs, float* rho, float* rr,
-                     double* ledger, float* corr, double mass1, void* scratch, int64_t scratch_bytes, int ni, int nj,
+                     double* ledger, float* corr, double mass1, void* work, int64_t work_bytes, int ni, int nj,
                      int nk, void* stream) {
-  if (scratch_bytes < gm1hip_dry_mass_scratch_bytes(ni, nj, nk)) return (int)hipErrorInvalidValue;
+  if (work_bytes < gm1hip_dry_mass_scratch_bytes(ni, nj, nk)) return (int)hipErrorInvalidValue;
   hipStream_t s = (hipStream_t)stream;
   const int nseg = (ni + 63) / 64;
-  double* part = (double*)scratch;
+  double* part = (double*)work;
   double* levels = part + 2LL * nk * nj * nseg;
   hipLaunchKernelGGL(k_mass_partials, dim3(nseg, nj, nk), dim3(64), 0, s, rho, ruh, rvh, rmh, pi0, pp3d, part, ni,
                      nj, nk, nseg, mass1 > 0.0 ? 1 : 0);

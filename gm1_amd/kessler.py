@@ -28,6 +28,10 @@ signature("gm1hip_kessler_scratch_bytes", I, I, I, restype=I64)
 signature("gm1hip_dry_mass_scratch_bytes", I, I, I, restype=I64)
 signature("gm1hip_kessler_driver", *([P] * 25), I64, I, I, I, Fl, Fl, Fl, Fl, I, P)
 signature("gm1hip_kessler_driver_decisions", *([P] * 26), I64, I, I, I, Fl, Fl, Fl, Fl, I, P)
+signature("gm1hip_kessler_driver_noalloc", *([P] * 24), I, I, I, Fl, Fl, Fl, Fl, I, P)
+
+# Size of the no-allocation driver's device workspace (a file-scope array in the library).
+KESSLER_NOALLOC_MAX_BYTES = 64 << 20
 signature("gm1hip_kessler_transition_pre", *([P] * 14), I, I, I, Fl, I, I, P)
 signature("gm1hip_apmasscon", *([P] * 13), D, P, I64, I, I, I, P)
 
@@ -144,6 +148,35 @@ def launch_cuda_gm1_kessler_driver(
         ptr(decisions), ptr(workspace.scratch), workspace.scratch.numel(), ni, nj, nk, float(dx), float(dy),
         float(dz), float(dt), selection.equation_set, stream_handle(stream))
     return workspace
+
+
+def launch_cuda_gm1_kessler_driver_noalloc(
+    arrays: Mapping[str, Any], status: torch.Tensor, *, ni: int, nj: int, nk: int, dx: float, dy: float, dz: float,
+    dt: float, selection: Gm1KesslerDriverSelection = DEFAULT_GM1_KESSLER_DRIVER_SELECTION, stream: Any | None = None,
+    held: Mapping[str, Any] | None = None,
+) -> None:
+    """``launch_cuda_gm1_kessler_driver`` with no workspace: nothing is allocated.
+
+    The driver's scratch is the library's file-scope device array
+    (``KESSLER_NOALLOC_MAX_BYTES``); ``status`` (int32 [2]) is the caller's.
+    Calls that overlap on one GPU share that array and race.
+    """
+    validate_selection(selection)
+    a = _check(arrays, ni, nj, nk)
+    need = int(_native.query("gm1hip_kessler_scratch_bytes", ni, nj, nk))
+    if need > KESSLER_NOALLOC_MAX_BYTES:
+        raise ValueError(f"a {ni}x{nj}x{nk} Kessler driver needs {need} bytes of scratch; the no-allocation "
+                         f"driver holds {KESSLER_NOALLOC_MAX_BYTES} (use launch_cuda_gm1_kessler_driver)")
+    if (not isinstance(status, torch.Tensor) or status.dtype != torch.int32 or tuple(status.shape) != (2,)
+            or status.device != a["rho"].device):
+        raise ValueError("status must be an int32 tensor of shape (2,) on the device")
+    h = held or {}
+    _native.call(
+        "gm1hip_kessler_driver_noalloc",
+        *(ptr(a[n]) for n in ("ruh", "rvh", "rmh", "mh", "pi0", "th0", "rho", "rr", "pp3d", "th3d", "prs", "qv",
+                              "qc", "qr", "fall_velocity", "rain", "prate", "asq", "qbudget")),
+        ptr(status), ptr(h.get("qpten")), ptr(h.get("qtten")), ptr(h.get("qvten")), ptr(h.get("qcten")),
+        ni, nj, nk, float(dx), float(dy), float(dz), float(dt), selection.equation_set, stream_handle(stream))
 
 
 @dataclass

@@ -265,6 +265,21 @@ int gm1hip_kessler_driver_decisions(const float* ruh, const float* rvh, const fl
                                     void* scratch, int64_t scratch_bytes, int ni, int nj, int nk, float dx, float dy,
                                     float dz, float dt, int eqtset, void* stream);
 
+// No counterpart in the reference: gm1hip_kessler_driver for callers that
+// cannot allocate (code under HIP graph capture, tight serving loops, harnesses
+// that forbid hipMalloc).  The same kernels and results, with no workspace
+// argument: the driver's scratch is a file-scope device array of 64 MiB
+// (enough for 640x768x80; gm1hip_kessler_scratch_bytes gives the need), which
+// each GPU holds until the process ends.  Inputs whose need exceeds it return
+// hipErrorInvalidValue before any launch.  The call allocates nothing.  Every
+// call on one GPU shares the array: calls that overlap on one GPU race.
+int gm1hip_kessler_driver_noalloc(const float* ruh, const float* rvh, const float* rmh, const float* mh,
+                                  const float* pi0, const float* th0, float* rho, float* rr, float* pp3d, float* th3d,
+                                  float* prs, float* qv, float* qc, float* qr, float* fall_velocity, float* rain,
+                                  float* prate, double* asq, double* qbudget, int* status, float* qpten,
+                                  float* qtten, float* qvten, float* qcten, int ni, int nj, int nk, float dx,
+                                  float dy, float dz, float dt, int eqtset, void* stream);
+
 // First half of cpu_gm1_kessler_large_step_inplace
 // (cuda_gm1_kessler_large_step.py:238-268): eqtset=2 held remove/snapshot
 // (state -= rate*dt, rate <- state) and the pre-physics equation of state.
