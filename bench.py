@@ -45,9 +45,10 @@ KERNEL_WORDS = {
     # reads u, v, w, wten, pp3d, ppten, ppterm, piadv, qk, pk1, pk2, mm, rr0,
     # rrf0, rru, rrv, rrw; writes w, pp3d, ppd, rru, rrv, rrw
     ("k_snd_column", False): 23,
-    # flat column (1-D base state): reads u, v, w, wten, pp3d, ppten, ppterm,
-    # piadv, qk, pk1, pk2, mm, rru, rrv, rrw; writes w, pp3d, ppd, rru, rrv, rrw
-    ("k_snd_column", True): 21,
+    # flat column (1-D base state; qk, pk1, pk2, mm formed in registers from
+    # ppterm, piadv, thv and per-level factors): reads u, v, w, wten, pp3d,
+    # ppten, ppterm, piadv, thv, rru, rrv, rrw; writes w, pp3d, ppd, rru, rrv, rrw
+    ("k_snd_column", True): 18,
     # reads u, v, uten, vten, ppd, thv, gxu, gyv; writes u, v
     ("k_snd_pgrad", False): 10,
     # reads u, v, uten, vten, ppd, thv; writes u, v

@@ -448,25 +448,46 @@ GM1_DEV double ordered_sum(const double* __restrict__ v, int64_t n, int64_t stri
   return s;
 }
 
-// rows: colbud rows (5 x nj, i order), level sums of the wave partials
-// (5 x nk, (j, segment) order) and STAT_PARTS strided partial maxima of the
-// satadj status words, one thread each.
+// Sum of n doubles by one workgroup of BS threads in a fixed order: thread t
+// adds elements t, t + BS, ... in index order (four loads requested ahead),
+// then a fixed shared-memory tree.  Deterministic; every thread returns it.
+constexpr int BS = 256;
+GM1_DEV double block_sum(const double* __restrict__ v, int64_t n, int64_t stride) {
+  __shared__ double sh[BS];
+  const int t = threadIdx.x;
+  double acc = 0.0;
+  int64_t q = t;
+  for (; q + 3 * BS < n; q += 4 * BS) {
+    double b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) b[u] = v[(q + u * BS) * stride];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc += b[u];
+  }
+  for (; q < n; q += BS) acc += v[q * stride];
+  sh[t] = acc;
+  __syncthreads();
+  for (int w = BS / 2; w > 0; w >>= 1) {
+    if (t < w) sh[t] += sh[t + w];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// rows: colbud rows (5 x nj, i order) and STAT_PARTS strided partial maxima of
+// the satadj status words, one thread each.
 constexpr int STAT_PARTS = 2048;
 
-__global__ __launch_bounds__(256) void k_kes_rows(const double* __restrict__ colbud, const double* __restrict__ wavebud,
-                                                  const int* __restrict__ wavestat, double* __restrict__ rowscr,
-                                                  int* __restrict__ statscr, int ni, int nj, int nk, int nseg) {
+__global__ __launch_bounds__(256) void k_kes_rows(const double* __restrict__ colbud, const int* __restrict__ wavestat,
+                                                  double* __restrict__ rowscr, int* __restrict__ statscr, int ni,
+                                                  int nj, int nk, int nseg) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nrow = 5 * nj, nlev = 5 * nk;
+  const int nrow = 5 * nj;
   if (r < nrow) {
     const int b = r / nj, j = r % nj;
     rowscr[r] = ordered_sum(colbud + (int64_t)b * nj * ni + (int64_t)j * ni, ni, 1);
-  } else if (r < nrow + nlev) {
-    const int q = r - nrow, b = q / nk, k = q % nk;
-    const int64_t lstride = (int64_t)nj * nseg;
-    rowscr[r] = ordered_sum(wavebud + (int64_t)b * nk * lstride + (int64_t)k * lstride, lstride, 1);
-  } else if (r < nrow + nlev + STAT_PARTS) {
-    const int part = r - nrow - nlev;
+  } else if (r < nrow + STAT_PARTS) {
+    const int part = r - nrow;
     const int64_t n = (int64_t)nk * nj * nseg;
     int mi = 0, mf = 0;
     for (int64_t q = part; q < n; q += STAT_PARTS) {
@@ -476,6 +497,16 @@ __global__ __launch_bounds__(256) void k_kes_rows(const double* __restrict__ col
     statscr[2 * part] = mi;
     statscr[2 * part + 1] = mf;
   }
+}
+
+// Level sums of the wave partials (5 x nk), one workgroup each, into rowscr
+// after the 5 x nj row sums.
+__global__ __launch_bounds__(BS) void k_kes_levels(const double* __restrict__ wavebud, double* __restrict__ rowscr,
+                                                   int nj, int nk, int nseg) {
+  const int q = blockIdx.x, b = q / nk, k = q % nk;
+  const int64_t lstride = (int64_t)nj * nseg;
+  const double v = block_sum(wavebud + (int64_t)b * nk * lstride + (int64_t)k * lstride, lstride, 1);
+  if (threadIdx.x == 0) rowscr[5 * nj + q] = v;
 }
 
 // The folds into asq/qbudget in the reference's order (j rows, k levels) and
@@ -592,13 +623,14 @@ __global__ __launch_bounds__(64) void k_mass_partials(const float* __restrict__ 
 
 // ledger[0] = dry mass (levels in k order); with mass1 > 0 also the f32
 // Exner correction into corr[0].
-__global__ __launch_bounds__(256) void k_mass_levels(const double* __restrict__ part, double* __restrict__ levels,
-                                                     int nk, int nj, int nseg) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= 2 * nk) return;
+// One workgroup per level sum (dry mass, then the pressure sum).
+__global__ __launch_bounds__(BS) void k_mass_levels(const double* __restrict__ part, double* __restrict__ levels,
+                                                    int nk, int nj, int nseg) {
+  const int r = blockIdx.x;
   const int which = r / nk, k = r % nk;
   const int64_t lstride = (int64_t)nj * nseg;
-  levels[r] = ordered_sum(part + 2 * (int64_t)k * lstride + which, lstride, 2);
+  const double v = block_sum(part + 2 * (int64_t)k * lstride + which, lstride, 2);
+  if (threadIdx.x == 0) levels[r] = v;
 }
 
 __global__ __launch_bounds__(64) void k_mass_fold(const double* __restrict__ levels, double* __restrict__ ledger,
@@ -716,9 +748,10 @@ int gm1hip_kessler_driver_decisions(const float* ruh, const float* rvh, const fl
   hipLaunchKernelGGL(k_kes_column, dim3((ni + 63) / 64, nj), dim3(64), 0, s, a);
   hipLaunchKernelGGL(k_kes_cell, dim3(a.nseg, nj, nk), dim3(64), 0, s, a);
   int* statscr = a.wavestat + 2LL * nk * nj * a.nseg;
-  const int nsum = 5 * nj + 5 * nk + STAT_PARTS;
-  hipLaunchKernelGGL(k_kes_rows, dim3((nsum + 255) / 256), dim3(256), 0, s, a.colbud, a.wavebud, a.wavestat, rows,
-                     statscr, ni, nj, nk, a.nseg);
+  const int nsum = 5 * nj + STAT_PARTS;
+  hipLaunchKernelGGL(k_kes_rows, dim3((nsum + 255) / 256), dim3(256), 0, s, a.colbud, a.wavestat, rows, statscr, ni,
+                     nj, nk, a.nseg);
+  hipLaunchKernelGGL(k_kes_levels, dim3(5 * nk), dim3(BS), 0, s, a.wavebud, rows, nj, nk, a.nseg);
   hipLaunchKernelGGL(k_kes_budgets, dim3(1), dim3(256), 0, s, rows, statscr, asq, qbudget, status, nj, nk, dx, dy,
                      dz);
   GM1_LAUNCH_CHECK();
@@ -783,7 +816,7 @@ int gm1hip_apmasscon(const float* ruh, const float* rvh, const float* rmh, const
   double* levels = part + 2LL * nk * nj * nseg;
   hipLaunchKernelGGL(k_mass_partials, dim3(nseg, nj, nk), dim3(64), 0, s, rho, ruh, rvh, rmh, pi0, pp3d, part, ni,
                      nj, nk, nseg, mass1 > 0.0 ? 1 : 0);
-  hipLaunchKernelGGL(k_mass_levels, dim3((2 * nk + 255) / 256), dim3(256), 0, s, part, levels, nk, nj, nseg);
+  hipLaunchKernelGGL(k_mass_levels, dim3(2 * nk), dim3(BS), 0, s, part, levels, nk, nj, nseg);
   hipLaunchKernelGGL(k_mass_fold, dim3(1), dim3(64), 0, s, levels, ledger, corr, nk, mass1, ni * nj * nk);
   if (mass1 > 0.0) {
     const int64_t n = (int64_t)ni * nj * nk;

@@ -50,6 +50,41 @@ GM1_DEV const float* P_rdsf(const SoundArgs& a) { return a.prof + 3 * (a.nk + 2)
 GM1_DEV const float* P_cgs(const SoundArgs& a) { return a.prof + 4 * (a.nk + 2); }
 GM1_DEV const float* P_cgt(const SoundArgs& a) { return a.prof + 4 * (a.nk + 2) + 3; }
 
+// sound.F:119-143 on a flat column: qk, pk1, pk2, mm of level k from ppterm,
+// piadv at k-1, k, k+1 and thv at k-1, k, with ((dts*rdz)*mh(k)) and the base
+// state at CM1's (1,1,k).  The level factors (FLAT_COEF words a level) are
+// formed once per stage by the prepare pass into the qk scratch.
+constexpr int FLAT_COEF = 8;
+struct FlatCoef { float tem, r1, r2, tem1, c1a, c1b, c2a, c2b; };
+
+GM1_DEV FlatCoef flat_level(const SoundArgs& a, int k) {
+  const int64_t plane = (int64_t)a.nj * a.ni, c0 = (int64_t)(k - 1) * plane, c1i = (int64_t)k * plane;
+  const float mhk = a.flat[k], mfk = a.flat[(a.nk + 2) + k];
+  const float dr = a.dts * a.rdzf;
+  FlatCoef f;
+  f.tem = dr * mhk;
+  f.r2 = ((dr * mhk) * a.rr0[c0]) * a.rf0[c1i];
+  f.r1 = ((dr * mhk) * a.rr0[c0]) * a.rf0[c0];
+  f.tem1 = (dr * a.cp) * mfk;
+  f.c1a = a.c1[c0];
+  f.c1b = a.c1[c1i];
+  f.c2a = a.c2[c0];
+  f.c2b = a.c2[c1i];
+  return f;
+}
+
+struct FlatTerms { float qk, pk1, pk2, mm; };
+
+GM1_DEV FlatTerms flat_terms(const FlatCoef& f, int k, int nk, float ppterm, float pd, float pc, float pu, float thv,
+                             float thvm) {
+  FlatTerms t;
+  t.qk = f.tem * ppterm;
+  t.pk2 = (k == nk) ? 0.0f : f.r2 * ((-f.c2b) * pu + (1.0f - f.c1b) * pc);
+  t.pk1 = (k == 1) ? 0.0f : f.r1 * (f.c1a * pd + (f.c2a - 1.0f) * pc);
+  t.mm = f.tem1 * (f.c2a * thv + f.c1a * thvm);
+  return t;
+}
+
 // ------------------------------------------------------------------ prepare
 // Threads over pressure-family columns (i, j) in 0..n+1 and levels 1..nk:
 // stage coefficients on the interior (zero on halos), ppd = ppx, pp3d = ppi,
@@ -75,24 +110,19 @@ __global__ __launch_bounds__(256) void k_snd_prepare(SoundArgs a) {
     a.v3d[pv] = a.va[pv];
     a.w3d[pw] = a.wa[pw];
   }
+  // the flat column forms qk, pk1, pk2 and mm inside its column pass (flat_terms) from
+  // the level factors one thread per level writes here into the qk scratch
+  if (a.flat) {
+    if (i == 0 && j == 0) {
+      const FlatCoef f = flat_level(a, k);
+      float4* d = reinterpret_cast<float4*>(a.qk + (int64_t)k * FLAT_COEF);
+      d[0] = make_float4(f.tem, f.r1, f.r2, f.tem1);
+      d[1] = make_float4(f.c1a, f.c1b, f.c2a, f.c2b);
+    }
+    return;
+  }
   float qk = 0.0f, pk1 = 0.0f, pk2 = 0.0f, mm = 0.0f;
-  if (a.flat && i >= 1 && i <= ni && j >= 1 && j <= nj) {
-    // sound.F:119-143: ((dts*rdz)*mh(k)) and the base state at CM1's (1,1,k)
-    const int64_t plane = (int64_t)nj * ni, c0 = (int64_t)(k - 1) * plane, c1i = (int64_t)k * plane;
-    const float mhk = a.flat[k], mfk = a.flat[(nk + 2) + k];
-    const float dr = a.dts * a.rdzf;
-    const float tem = dr * mhk;
-    const float r2 = ((dr * mhk) * a.rr0[c0]) * a.rf0[c1i];
-    const float r1 = ((dr * mhk) * a.rr0[c0]) * a.rf0[c0];
-    const float tem1 = (dr * a.cp) * mfk;
-    const float c1a = a.c1[c0], c1b = a.c1[c1i], c2a = a.c2[c0], c2b = a.c2[c1i];
-    qk = tem * a.ppterm[p];
-    pk2 = r2 * ((-c2b) * a.piadv[pix(k + 1, j, i, ni, nj)] + (1.0f - c1b) * a.piadv[p]);
-    pk1 = r1 * (c1a * a.piadv[pix(k - 1, j, i, ni, nj)] + (c2a - 1.0f) * a.piadv[p]);
-    mm = tem1 * (c2a * a.thv[p] + c1a * a.thv[pix(k - 1, j, i, ni, nj)]);
-    if (k == 1) pk1 = 0.0f;
-    if (k == nk) pk2 = 0.0f;
-  } else if (i >= 1 && i <= ni && j >= 1 && j <= nj) {
+  if (i >= 1 && i <= ni && j >= 1 && j <= nj) {
     const int i0 = i - 1, j0 = j - 1;
     const int64_t h = (int64_t)j0 * ni + i0, plane = (int64_t)nj * ni;
     const float g = a.gz[h];
@@ -699,6 +729,14 @@ __device__ __forceinline__ void column2_body(const SoundArgs& a, int n, int tile
   float* Ldd = lds + 5 * L;
   float* Lwt = lds + 6 * L;  // wten (pipelined pass only)
   auto at = [&](int k) { return k * C2W + c; };
+  auto coef = [&](int k) {  // the flat column's level factors, from the prepare pass
+    const float4* d = reinterpret_cast<const float4*>(a.qk + (int64_t)k * FLAT_COEF);
+    const float4 x = d[0], y = d[1];
+    FlatCoef f;
+    f.tem = x.x; f.r1 = x.y; f.r2 = x.z; f.tem1 = x.w;
+    f.c1a = y.x; f.c1b = y.y; f.c2a = y.z; f.c2b = y.w;
+    return f;
+  };
   const int chunk = (nk + C2S - 1) / C2S;
   const int kb = 1 + slot * chunk, ke = min(nk, kb + chunk - 1);
   const float asq = a.alph * a.alph;
@@ -712,7 +750,13 @@ __device__ __forceinline__ void column2_body(const SoundArgs& a, int n, int tile
     if (kb == 1) lw1[c] = w[wst + wc];
     if constexpr (PIPE) {
       // level k+1's words are requested before level k is formed
-      struct In { float uw, ue, vs, vn, pc, pe, pw, pn, ps, wup, wt, qk, pk1, pk2, ppten, ppterm, pp, mm; };
+      // piadv at k+1 is the next level's centre word and piadv, thv at k-1
+      // the previous level's, so each level requests only its own words
+      struct In {
+        float uw, ue, vs, vn, pc, pe, pw, pn, ps, wup, wt, ppten, ppterm, pp, th;
+        FlatCoef f;
+      };
+      const int64_t pst = (int64_t)(nj + 2) * (ni + 2);
       auto load = [&](int k) {
         const int64_t p = pix(k, pj, pi, ni, nj);
         In v;
@@ -727,20 +771,20 @@ __device__ __forceinline__ void column2_body(const SoundArgs& a, int n, int tile
         v.ps = a.piadv[p - (ni + 2)];
         v.wup = w[(int64_t)(k + 1) * wst + wc];
         v.wt = a.wten[(int64_t)k * wst + wc];
-        v.qk = a.qk[p];
-        v.pk1 = a.pk1[p];
-        v.pk2 = a.pk2[p];
         v.ppten = a.ppten[p];
         v.ppterm = a.ppterm[p];
         v.pp = a.pp3d[p];
-        v.mm = a.mm[p];
+        v.th = a.thv[p];
+        v.f = coef(k);
         return v;
       };
       float wraw = w[(int64_t)kb * wst + wc];
+      const int64_t pb = pix(kb - 1, pj, pi, ni, nj);
+      float pd = a.piadv[pb], thm = a.thv[pb];
       In nx = load(kb);
       for (int k = kb; k <= ke; ++k) {
         const In cu = nx;
-        nx = load(min(k + 1, ke));
+        nx = load(k + 1);  // level nk+1 exists in every array read here
         float div = (cu.ue - cu.uw) * a.rdx + (cu.vn - cu.vs) * a.rdy;
         if (fabsf(div) < smeps) div = 0.0f;
         float hx = cu.uw * (cu.pc - cu.pw) + cu.ue * (cu.pe - cu.pc);
@@ -748,13 +792,16 @@ __device__ __forceinline__ void column2_body(const SoundArgs& a, int n, int tile
         float hy = cu.vs * (cu.pc - cu.ps) + cu.vn * (cu.pn - cu.pc);
         hy = (-hy) * temy;
         const float hp = hx + hy;
-        const float s1 = cu.pk1 + cu.qk, s2 = cu.pk2 - cu.qk;
+        const FlatTerms t = flat_terms(cu.f, k, nk, cu.ppterm, pd, cu.pc, nx.pc, cu.th, thm);
+        pd = cu.pc;
+        thm = cu.th;
+        const float s1 = t.pk1 + t.qk, s2 = t.pk2 - t.qk;
         const float vert = wraw * s1 + cu.wup * s2;
         Lfpk[at(k)] = (a.dts * (cu.ppten - cu.ppterm * div) + hp) + a.beta * vert;
         Lpp[at(k)] = cu.pp;
         Ls1[at(k)] = s1;
         Ls2[at(k)] = s2;
-        Lmm[at(k)] = cu.mm;
+        Lmm[at(k)] = t.mm;
         Ldd[at(k)] = wraw;
         Lwt[at(k)] = cu.wt;
         wraw = cu.wup;
@@ -773,14 +820,17 @@ __device__ __forceinline__ void column2_body(const SoundArgs& a, int n, int tile
         hy = (-hy) * temy;
         const float hp = hx + hy;
         const float wraw = w[(int64_t)k * wst + wc];
-        const float qkk = a.qk[p], pk1k = a.pk1[p], pk2k = a.pk2[p];
-        const float s1 = pk1k + qkk, s2 = pk2k - qkk;
+        const int64_t pst = (int64_t)(nj + 2) * (ni + 2);
+        const float ppterm = a.ppterm[p];
+        const FlatTerms t = flat_terms(coef(k), k, nk, ppterm, a.piadv[p - pst], pc, a.piadv[p + pst], a.thv[p],
+                                       a.thv[p - pst]);
+        const float s1 = t.pk1 + t.qk, s2 = t.pk2 - t.qk;
         const float vert = wraw * s1 + w[(int64_t)(k + 1) * wst + wc] * s2;
-        Lfpk[at(k)] = (a.dts * (a.ppten[p] - a.ppterm[p] * div) + hp) + a.beta * vert;
+        Lfpk[at(k)] = (a.dts * (a.ppten[p] - ppterm * div) + hp) + a.beta * vert;
         Lpp[at(k)] = a.pp3d[p];
         Ls1[at(k)] = s1;
         Ls2[at(k)] = s2;
-        Lmm[at(k)] = a.mm[p];
+        Lmm[at(k)] = t.mm;
         Ldd[at(k)] = wraw;
         {
           const int64_t pu = (int64_t)k * ust + uc, pv = (int64_t)k * vst + vc;
