@@ -189,19 +189,25 @@ def base_state(cfg: CaseConfig, prof: VerticalProfiles) -> dict[str, np.ndarray]
     zf = prof.sigmaf[1 : nk + 2].astype(np.float64)
     th = _theta_env(zs)
     qv = np.zeros(nk)
-    # integrate Exner from the surface (pi=1 at 1000 hPa) at scalar levels
+    # integrate Exner from the surface (pi=1 at 1000 hPa) at scalar levels; the first half
+    # step averages theta_v with the surface's (300 K, saturated at 1000 hPa), as CM1's
+    # base.F does (storm_initialization.rs:62-79, 281-286)
+    t_sfc = 300.0
+    es_sfc = min(611.2 * math.exp(17.67 * (t_sfc - 273.15) / (t_sfc - 29.65)), 0.5 * P00)
+    qv_sfc = EPS * es_sfc / (P00 - es_sfc)
+    thv_sfc = 300.0 * (1.0 + qv_sfc / EPS) / (1.0 + qv_sfc)
     pi = np.empty(nk)
     for it in range(20):
         thv = th * (1.0 + qv / EPS) / (1.0 + qv)
-        pi[0] = 1.0 - G / CP * zs[0] / thv[0]
+        pi[0] = 1.0 - G * zs[0] / (CP * 0.5 * (thv_sfc + thv[0]))
         for k in range(1, nk):
             pi[k] = pi[k - 1] - G / CP * (zs[k] - zs[k - 1]) * 2.0 / (thv[k - 1] + thv[k])
         if not cfg.moist:
             break
         t = th * pi
         p = P00 * pi ** (CP / RD)
-        es = 611.2 * np.exp(17.67 * (t - 273.15) / (t - 29.65))
-        qvs = EPS * es / np.maximum(p - es, 1.0)
+        es = np.minimum(611.2 * np.exp(17.67 * (t - 273.15) / (t - 29.65)), 0.5 * p)
+        qvs = EPS * es / (p - es)
         qv = np.minimum(_rh_env(zs) * qvs, 0.014) * float(cfg.extra.get("qv_scale", 1.0))
     thv = th * (1.0 + qv / EPS) / (1.0 + qv)
     p = P00 * pi ** (CP / RD)
@@ -214,7 +220,8 @@ def base_state(cfg: CaseConfig, prof: VerticalProfiles) -> dict[str, np.ndarray]
     c2[1:nk] = (zf[1:nk] - zs[:-1]) / (zs[1:] - zs[:-1])
     c1 = 1.0 - c2
     return {"th0": th.astype(F), "pi0": pi.astype(F), "qv0": qv.astype(F), "rho0": rho.astype(F),
-            "rf0": rhof.astype(F), "c1": c1.astype(F), "c2": c2.astype(F), "zs": zs, "zf": zf}
+            "rf0": rhof.astype(F), "c1": c1.astype(F), "c2": c2.astype(F), "zs": zs, "zf": zf,
+            "binary64": {"th": th, "pi": pi, "qv": qv, "thv_sfc": thv_sfc}}
 
 
 def metrics_for(cfg: CaseConfig, base: dict[str, np.ndarray]) -> dict[str, np.ndarray]:
