@@ -235,7 +235,177 @@ def launch_cuda_gm1_kessler_transition(
                                    stream=stream, workspace=workspace, held=held if eqt == 2 else None)
 
 
+# ----------------------------------------------------------------------------- the persistent transition
+# CudaGm1KesslerLargeStepTransition (cuda_gm1_kessler_large_step.py:295-560): one object per device
+# state carries the eqtset=2 held rates across large steps and applies, after the final RK stage,
+# the held remove/snapshot, the pre-physics equation of state, an optional hook (the dry-mass
+# correction), rr <- rho, the ptype=1 driver and the held recapture.
+
+GM1_KESSLER_LARGE_STEP_RESTART_SCHEMA = "gm1_amd.kessler-large-step-restart/v1"
+_RESTART_NAMES = ("held.qpten", "held.qtten", "held.qvten", "held.qcten", "persistent.rain", "persistent.prate",
+                  "persistent.asq", "persistent.qbudget")
+
+
+@dataclass
+class Gm1KesslerHeldCarriers:
+    qpten: torch.Tensor
+    qtten: torch.Tensor
+    qvten: torch.Tensor
+    qcten: torch.Tensor
+
+
+@dataclass(frozen=True)
+class Gm1KesslerLargeStepRestart:
+    schema: str
+    completed_transitions: int
+    arrays: Mapping[str, Any]
+
+
+@dataclass(frozen=True)
+class Gm1KesslerLargeStepResult:
+    ready_event: Any
+    completed_transitions: int
+    receipt: Mapping[str, Any]
+    restart: Gm1KesslerLargeStepRestart
+    driver_launch: Any
+
+
+class CudaGm1KesslerLargeStepTransition:
+    """Persistent source-ordered Kessler transition for one device state."""
+
+    def __init__(self, *, arrays: dict[str, Any], carriers: Gm1KesslerHeldCarriers, ni: int, nj: int, nk: int,
+                 dx: float, dy: float, dz: float, dt: float, selection: Gm1KesslerDriverSelection, stream: Any,
+                 workspace: Gm1KesslerDriverWorkspace, completed_transitions: int = 0, **_: Any) -> None:
+        self.arrays = arrays
+        self.carriers = carriers
+        self.ni, self.nj, self.nk = ni, nj, nk
+        self.dx, self.dy, self.dz, self.dt = dx, dy, dz, dt
+        self.selection = selection
+        self.stream = stream
+        self.workspace = workspace
+        self.completed_transitions = completed_transitions
+        self._last_receipt: dict[str, Any] | None = None
+
+    def _restart_arrays(self) -> dict[str, Any]:
+        c, a = self.carriers, self.arrays
+        return {"held.qpten": c.qpten, "held.qtten": c.qtten, "held.qvten": c.qvten, "held.qcten": c.qcten,
+                "persistent.rain": a["rain"], "persistent.prate": a["prate"], "persistent.asq": a["asq"],
+                "persistent.qbudget": a["qbudget"]}
+
+    def capture_restart(self) -> Gm1KesslerLargeStepRestart:
+        values = self._restart_arrays()
+        if tuple(values) != _RESTART_NAMES:
+            raise RuntimeError("GM1 Kessler restart inventory drifted")
+        return Gm1KesslerLargeStepRestart(GM1_KESSLER_LARGE_STEP_RESTART_SCHEMA, self.completed_transitions,
+                                          dict(values))
+
+    def clone_restart(self) -> Gm1KesslerLargeStepRestart:
+        with torch.cuda.stream(self.stream) if self.stream is not None else _NullContext():
+            values = {k: v.clone() for k, v in self._restart_arrays().items()}
+        return Gm1KesslerLargeStepRestart(GM1_KESSLER_LARGE_STEP_RESTART_SCHEMA, self.completed_transitions, values)
+
+    def receipt_dict(self) -> dict[str, Any]:
+        if self._last_receipt is None:
+            raise RuntimeError("GM1 Kessler large-step transition has not completed")
+        return dict(self._last_receipt)
+
+    def apply(self, *, final_rk: bool = True, refl_10cm_due: bool = False,
+              pre_component_hook: Any | None = None) -> Gm1KesslerLargeStepResult:
+        """cuda_gm1_kessler_large_step.py:380-555: held remove/snapshot, EOS, hook, rr <- rho, driver, recapture."""
+        if type(final_rk) is not bool or not final_rk:
+            refuse(code="GM1-KESSLER-LARGE-STEP-CADENCE", gate="physics-seam", option="final_rk", value=final_rk,
+                   reason="CM1 r21.1 calls mp_driver once after the completed RK solve",
+                   remedy="invoke the transition only after final RK stage 3")
+        if type(refl_10cm_due) is not bool:
+            raise TypeError("refl_10cm_due must be boolean")
+        if refl_10cm_due:
+            refuse(code="GM1-KESSLER-REFLECTIVITY-UNPORTED", gate="physics-seam", option="refl_10cm_due",
+                   value=True, reason="CM1 does not admit the selected reflectivity consumer for ptype=1",
+                   remedy="disable reflectivity or select an admitted microphysics consumer")
+        if pre_component_hook is not None and not callable(pre_component_hook):
+            raise TypeError("pre_component_hook must be callable or None")
+        a = _check(self.arrays, self.ni, self.nj, self.nk)
+        c = self.carriers
+        eqt = self.selection.equation_set
+        h = stream_handle(self.stream)
+        _native.call("gm1hip_kessler_transition_pre", None, ptr(a["pp3d"]), ptr(a["th3d"]), ptr(a["qv"]),
+                     ptr(a["qc"]), ptr(c.qpten), ptr(c.qtten), ptr(c.qvten), ptr(c.qcten), ptr(a["pi0"]),
+                     ptr(a["th0"]), ptr(a["prs"]), ptr(a["rho"]), ptr(a["rr"]), self.ni, self.nj, self.nk,
+                     float(self.dt), eqt, int(pre_component_hook is None), h)
+        hook_result = None
+        if pre_component_hook is not None:
+            hook_result = pre_component_hook(self.carriers)
+            with torch.cuda.stream(self.stream) if self.stream is not None else _NullContext():
+                a["rr"].copy_(a["rho"])
+        held = {"qpten": c.qpten, "qtten": c.qtten, "qvten": c.qvten, "qcten": c.qcten} if eqt == 2 else None
+        driver = launch_cuda_gm1_kessler_driver(a, ni=self.ni, nj=self.nj, nk=self.nk, dx=self.dx, dy=self.dy,
+                                                dz=self.dz, dt=self.dt, selection=self.selection,
+                                                stream=self.stream, workspace=self.workspace, held=held)
+        self.completed_transitions += 1
+        restart = self.capture_restart()
+        self._last_receipt = {
+            "completed_transitions": self.completed_transitions,
+            "configuration": {"ni": self.ni, "nj": self.nj, "nk": self.nk, "dx_m": self.dx, "dy_m": self.dy,
+                              "dz_m": self.dz, "dt_s": self.dt, "ptype": 1, "equation_set": eqt,
+                              "efall": self.selection.efall, "final_rk": True},
+            "source_order": ["held-rate remove/snapshot" if eqt == 2 else "held-rate lifecycle inactive",
+                             "pre-physics EOS",
+                             "dry-mass correction hook" if pre_component_hook is not None else "dry-mass hook absent",
+                             "mp_driver ptype=1", "held-rate recapture" if eqt == 2 else "held-rate recapture inactive"],
+            "restart": {"inventory": list(restart.arrays), "persistent": ["rain", "prate", "asq", "qbudget"]},
+            "hook_result": None if hook_result is None else repr(hook_result),
+            "target_arch": "gfx950",
+        }
+        event = None
+        if not torch.cuda.is_current_stream_capturing():
+            event = torch.cuda.Event()
+            event.record(self.stream if self.stream is not None else torch.cuda.current_stream())
+        return Gm1KesslerLargeStepResult(event, self.completed_transitions, self._last_receipt, restart, driver)
+
+
+def create_cuda_gm1_kessler_large_step_transition(
+    arrays: Mapping[str, Any], *, ni: int, nj: int, nk: int, dx: float, dy: float, dz: float, dt: float,
+    selection: Gm1KesslerDriverSelection, stream: Any | None = None, workspace: Gm1KesslerDriverWorkspace | None = None,
+    restart: Gm1KesslerLargeStepRestart | None = None, **_: Any,
+) -> CudaGm1KesslerLargeStepTransition:
+    """Bind a Kessler-initialised device state (cuda_gm1_kessler_large_step.py:558-640).
+
+    The held rates are allocated zero, or taken from ``restart`` (its arrays
+    copied in place, its transition count carried on).
+    """
+    validate_selection(selection)
+    for name, value, minimum in (("ni", ni, 1), ("nj", nj, 1), ("nk", nk, 2)):
+        if type(value) is not int or value < minimum:
+            raise ValueError(f"{name} must be an integer >= {minimum}")
+    a = _check(arrays, ni, nj, nk)
+    dev = a["rho"].device
+    held = Gm1KesslerHeldCarriers(*(torch.zeros((nk, nj, ni), dtype=torch.float32, device=dev) for _ in range(4)))
+    done = 0
+    if restart is not None:
+        if restart.schema != GM1_KESSLER_LARGE_STEP_RESTART_SCHEMA or tuple(restart.arrays) != _RESTART_NAMES:
+            raise ValueError("the restart is not a GM1 Kessler large-step restart of this inventory")
+        for name, dst in (("held.qpten", held.qpten), ("held.qtten", held.qtten), ("held.qvten", held.qvten),
+                          ("held.qcten", held.qcten), ("persistent.rain", a["rain"]), ("persistent.prate", a["prate"]),
+                          ("persistent.asq", a["asq"]), ("persistent.qbudget", a["qbudget"])):
+            dst.copy_(restart.arrays[name])
+        done = int(restart.completed_transitions)
+    workspace = workspace or create_cuda_gm1_kessler_driver_workspace(ni, nj, nk, dev)
+    return CudaGm1KesslerLargeStepTransition(arrays=dict(arrays), carriers=held, ni=ni, nj=nj, nk=nk, dx=float(dx),
+                                             dy=float(dy), dz=float(dz), dt=float(dt), selection=selection,
+                                             stream=stream, workspace=workspace, completed_transitions=done)
+
+
+class _NullContext:
+    def __enter__(self):
+        return None
+
+    def __exit__(self, *exc):
+        return False
+
+
 __all__ = [
+    "CudaGm1KesslerLargeStepTransition", "GM1_KESSLER_LARGE_STEP_RESTART_SCHEMA", "Gm1KesslerHeldCarriers",
+    "Gm1KesslerLargeStepRestart", "Gm1KesslerLargeStepResult", "create_cuda_gm1_kessler_large_step_transition",
     "DEFAULT_GM1_KESSLER_DRIVER_SELECTION", "DryMassWorkspace", "Gm1KesslerDriverSelection",
     "Gm1KesslerDriverWorkspace", "create_cuda_gm1_kessler_driver_workspace", "create_dry_mass_workspace",
     "gm1_kessler_driver_shapes", "launch_cuda_apmasscon", "launch_cuda_gm1_kessler_driver",

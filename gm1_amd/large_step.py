@@ -24,7 +24,7 @@ from __future__ import annotations
 
 import ctypes
 from dataclasses import dataclass, field
-from typing import Any
+from typing import Any, Callable
 
 import numpy as np
 import torch
@@ -45,6 +45,7 @@ from .kessler import (
     launch_cuda_gm1_kessler_transition,
 )
 from .momentum_advection5 import launch_cuda_terrain_open_momentum_advection5
+from .errors import refuse
 from .psolver3 import TerrainOpenSoundAblations, device_profiles, launch_cuda_psolver3_terrain_open_stage
 from .scalar_moist_rk import launch_scalar_stage_from_carrier, moisture_weps_f64, theta_weps_f64
 
@@ -296,8 +297,51 @@ def _scalar_stages(st: Gm1LargeStepState, nrk: int, mass: dict[str, Any], *, fla
                                                            "advz": a["advz"]}, **grid)
 
 
-def launch_cuda_gm1_large_step(st: Gm1LargeStepState) -> dict[str, Any]:
-    """Enqueue one complete large step on ``st.stream`` (no host synchronisation)."""
+@dataclass(frozen=True)
+class Gm1Solve1Carriers:
+    """The held (solve1) tendencies a solve1 hook may add into (cuda_gm1_forecast.py Gm1Solve1Carriers)."""
+
+    uten1: torch.Tensor
+    vten1: torch.Tensor
+    wten1: torch.Tensor
+    thten1: torch.Tensor
+    qten: torch.Tensor | None
+    ppten: torch.Tensor
+
+
+# the reference's name for the composer's state
+CudaGm1ForecastState = Gm1LargeStepState
+
+
+def launch_cuda_gm1_large_step(
+    st: Gm1LargeStepState, *, solve1_hook: Callable[[Gm1LargeStepState, Gm1Solve1Carriers], object] | None = None,
+    post_stage_hook: Callable[[Gm1LargeStepState, int], object] | None = None, lateral_forcing: Any | None = None,
+    reflectivity_due: bool = False, allow_incomplete_validation: bool = False,
+) -> dict[str, Any]:
+    """Enqueue one complete large step on ``st.stream`` (no host synchronisation).
+
+    The reference composer's keywords (cuda_gm1_forecast.py:1569-1627):
+    ``solve1_hook(st, carriers)`` runs after the held carriers are cleared
+    (this path has no built-in closure) and before the held compose, and may
+    add tendencies into them; ``post_stage_hook(st, nrk)`` runs after each RK
+    stage.  Both enqueue on ``st.stream``.  ``lateral_forcing`` (a parent
+    model's boundary nudging) and ``reflectivity_due`` (no reflectivity
+    consumer for ptype=1) are refused before any launch.
+    """
+    if solve1_hook is not None and not callable(solve1_hook):
+        raise TypeError("solve1_hook must be callable or None")
+    if post_stage_hook is not None and not callable(post_stage_hook):
+        raise TypeError("post_stage_hook must be callable or None")
+    if lateral_forcing is not None:
+        refuse(code="GM1AMD-LATERAL-FORCING", gate="large-step", option="lateral_forcing", value="given",
+               reason="parent-model lateral forcing is not part of the gfx950 large step",
+               remedy="run with the open-radiative or periodic lateral families and lateral_forcing=None")
+    if type(reflectivity_due) is not bool:
+        raise TypeError("reflectivity_due must be boolean")
+    if reflectivity_due:
+        refuse(code="GM1-KESSLER-REFLECTIVITY-UNPORTED", gate="physics-seam", option="reflectivity_due", value=True,
+               reason="CM1 does not admit the selected reflectivity consumer for ptype=1",
+               remedy="disable reflectivity or select an admitted microphysics consumer")
     cfg, a = st.cfg, st.a
     ni, nj, nk = cfg.ni, cfg.nj, cfg.nk
     h = stream_handle(st.stream)
@@ -307,13 +351,20 @@ def launch_cuda_gm1_large_step(st: Gm1LargeStepState) -> dict[str, Any]:
             a[name].zero_()
         if st.numq:
             a["qten"].zero_()
+    solve1_result = None
+    if solve1_hook is not None:
+        carriers = Gm1Solve1Carriers(a["uten1"], a["vten1"], a["wten1"], a["thten1"], a.get("qten"), a["ppten"])
+        solve1_result = solve1_hook(st, carriers)
     if st.numq:
         b = st.base
         _native.call("gm1hip_held_compose", ptr(b["th0"]), ptr(b["pi0"]), ptr(a["rho"]), ptr(a["ppi"]),
                      ptr(a["tha"]), ptr(a["qa"]), ptr(a["thten1"]), ptr(a["qten"]), ptr(a["ppten"]),
                      ptr(a["qpten"]), ptr(a["qtten"]), ptr(a["qvten"]), ptr(a["qcten"]), ni, nj, nk, cfg.dt, h)
+    stage_hooks = []
     for nrk in (1, 2, 3):
         _rk_stage(st, nrk)
+        if post_stage_hook is not None:
+            stage_hooks.append(post_stage_hook(st, nrk))
     if st.numq:
         launch_cuda_gm1_kessler_transition(
             _kessler_arrays(st), {n: a[n] for n in KESSLER_HELD}, ni=ni, nj=nj, nk=nk, dx=cfg.dx, dy=cfg.dy,
@@ -334,7 +385,10 @@ def launch_cuda_gm1_large_step(st: Gm1LargeStepState) -> dict[str, Any]:
                      ni, nj, nk, h)
     st.step_index += 1
     return {"step": st.step_index, "cells": cfg.cells, "lateral_sides": list(cfg.lateral_sides),
-            "moist": bool(st.numq)}
+            "moist": bool(st.numq), "solve1_hook": solve1_hook is not None,
+            "post_stage_hook": post_stage_hook is not None, "stages": [1, 2, 3],
+            "hook_results": {"solve1": solve1_result, "post_stage": stage_hooks},
+            "allow_incomplete_validation": bool(allow_incomplete_validation)}
 
 
 def kessler_status(st: Gm1LargeStepState) -> dict[str, int]:

@@ -13,13 +13,14 @@ total) fused with its compose (``gm1hip_pdef_moisture_stage``).
 from __future__ import annotations
 
 import ctypes
+from dataclasses import dataclass, field
 from typing import Any, Mapping
 
 import numpy as np
 import torch
 
 from . import _native
-from ._device import NG, lateral_code, ptr, require, shapes, stream_handle, terrain_static_shapes
+from ._device import NG, lateral_code, no_alias, ptr, record, require, shapes, stream_handle, terrain_static_shapes
 from ._native import D, Fl, I, P, U, signature
 
 signature("gm1hip_scalar_rk_stage", *([P] * 13), I, I, I, I, D, Fl, Fl, Fl, U, Fl, P)
@@ -179,8 +180,155 @@ class _NoStream:
         return False
 
 
+# ----------------------------------------------------------------------------- the stateful stage API
+# create_/begin_/launch_cuda_scalar_moist_rk_stage (cuda_scalar_moist_rk.py:823-1499): a state binds
+# the live theta and moisture carriers (A level, current, held) and the stage scratch; stages run
+# in the order 1, 2, 3 and a new large step begins only after stage 3.
+
+
+@dataclass
+class CudaScalarMoistRKState:
+    ni: int
+    nj: int
+    nk: int
+    th0: torch.Tensor
+    theta_a: torch.Tensor
+    theta_current: torch.Tensor
+    moisture_a: torch.Tensor
+    moisture_current: torch.Tensor
+    theta_held: torch.Tensor
+    moisture_held: torch.Tensor
+    scratch: torch.Tensor
+    advx: torch.Tensor
+    advy: torch.Tensor
+    advz: torch.Tensor
+    stream: Any
+    device_id: int
+    next_nrk: int = 1
+    completed: bool = False
+    qmag: tuple[float, ...] = CM1_QMAG_F32
+    names: tuple[str, ...] = CM1_SCALAR_MOIST_SPECIES
+    lateral_sides: tuple[int, int, int, int] = (2, 2, 2, 2)
+
+
+@dataclass(frozen=True)
+class CudaScalarMoistRKStageLaunch:
+    state: CudaScalarMoistRKState
+    ready_event: Any
+    receipt: dict[str, Any] = field(default_factory=dict)
+
+
+def create_cuda_scalar_moist_rk_state(
+    carriers: Mapping[str, Any], held: Mapping[str, Any], th0: Any, *, ni: int, nj: int, nk: int,
+    stream: Any | None = None, qmag: tuple[float, ...] = CM1_QMAG_F32,
+    names: tuple[str, ...] = CM1_SCALAR_MOIST_SPECIES, lateral_sides: Any = (2, 2, 2, 2),
+) -> CudaScalarMoistRKState:
+    """Bind the live carriers (tha, th3d, qa, q3d; thten1, qten) and allocate the stage scratch (:856-966).
+
+    ``lateral_sides`` is the family the scalars' halos follow (the reference
+    binds it through its compiled open-boundary image).
+    """
+    for name, value in (("ni", ni), ("nj", nj), ("nk", nk)):
+        if isinstance(value, bool) or not isinstance(value, int) or value < 10:
+            raise ValueError(f"{name} must be an integer >= 10")
+    if not qmag or any(isinstance(v, bool) or not isinstance(v, (int, float)) or v <= 0.0 for v in qmag):
+        raise ValueError("qmag must be CM1's positive per-scalar magnitude table")
+    if len(names) != len(qmag):
+        raise ValueError(f"names {tuple(names)!r} must give CM1's qname for each of the {len(qmag)} rows qmag "
+                         "describes")
+    physical = (nk, nj, ni)
+    moisture = (len(qmag), nk, nj, ni)
+    bound = {
+        "th0": require(th0, "th0", (nk,)), "tha": require(carriers["tha"], "tha", physical),
+        "th3d": require(carriers["th3d"], "th3d", physical), "qa": require(carriers["qa"], "qa", moisture),
+        "q3d": require(carriers["q3d"], "q3d", moisture), "thten1": require(held["thten1"], "thten1", physical),
+        "qten": require(held["qten"], "qten", moisture),
+    }
+    no_alias(bound, "live scalar/moisture RK carriers must not alias")
+    dev = bound["th3d"].device
+    z = lambda: torch.zeros(physical, dtype=torch.float32, device=dev)  # noqa: E731
+    return CudaScalarMoistRKState(
+        ni, nj, nk, bound["th0"], bound["tha"], bound["th3d"], bound["qa"], bound["q3d"], bound["thten1"],
+        bound["qten"], z(), z(), z(), z(), stream, dev.index if dev.index is not None else torch.cuda.current_device(),
+        qmag=tuple(float(v) for v in qmag), names=tuple(names), lateral_sides=tuple(int(v) for v in lateral_sides))
+
+
+def begin_cuda_scalar_moist_rk_step(state: CudaScalarMoistRKState) -> None:
+    """Reset the cadence guard after the caller advances the A and held fields (:969-978)."""
+    if not isinstance(state, CudaScalarMoistRKState):
+        raise TypeError("state must be a CudaScalarMoistRKState")
+    if not state.completed or state.next_nrk != 4:
+        raise ValueError("cannot begin a new large step before stage 3 completes")
+    state.next_nrk = 1
+    state.completed = False
+
+
+def launch_cuda_scalar_moist_rk_stage(
+    state: CudaScalarMoistRKState, mass_flux: Mapping[str, Any], metrics: Mapping[str, Any], divx: Any,
+    theta_divergence_term: Any, *, nrk: int, dt: float, dx: float, dy: float, dz: float | None = None,
+    vertical: Any | None = None, stream: Any | None = None, compiled: Any = None, advection_compiled: Any = None,
+    open_compiled: Any = None,
+) -> CudaScalarMoistRKStageLaunch:
+    """One theta + moisture RK stage (:1041-1499): 5th order below nrk=3, WENO5 theta and the
+    positive-definite moisture chain at nrk=3, each composed A-anchored into the current carrier.
+
+    Exactly one vertical coordinate: ``dz`` (the uniform terrain-coordinate
+    column) or ``vertical`` (a flat stretched column,
+    ``gm1_amd.stretched.FlatStretchedColumn``).
+    """
+    if (dz is None) == (vertical is None):
+        raise ValueError("the scalar/moist stage takes exactly one vertical coordinate: dz for the uniform "
+                         "column, or vertical for a stretched one")
+    if not isinstance(state, CudaScalarMoistRKState):
+        raise TypeError("state must be a CudaScalarMoistRKState")
+    if isinstance(nrk, bool) or nrk != state.next_nrk or state.completed:
+        raise ValueError(f"scalar/moist RK cadence requires stage {state.next_nrk}, received {nrk}")
+    for name, value in (("dx", dx), ("dy", dy)) + ((("dz", dz),) if dz is not None else ()):
+        if isinstance(value, bool) or not isinstance(value, (int, float)):
+            raise TypeError(f"{name} must be numeric")
+        if not np.isfinite(float(value)) or float(value) <= 0.0:
+            raise ValueError(f"{name} must be finite and > 0")
+    if torch.cuda.current_device() != state.device_id:
+        raise ValueError("scalar/moist RK state is bound to another HIP device")
+    stream = stream if stream is not None else state.stream
+    if stream_handle(stream) != stream_handle(state.stream):
+        raise ValueError("scalar/moist RK state must remain on its bound stream")
+    physical = (state.nk, state.nj, state.ni)
+    require(theta_divergence_term, "theta_divergence_term", physical)
+    require(divx, "divx", physical)
+    if vertical is None:
+        flat_column, rdz = None, float(np.float32(1.0) / np.float32(dz))
+    else:
+        flat_column, rdz = vertical.column, float(vertical.rdz)
+    grid = dict(ni=state.ni, nj=state.nj, nk=state.nk, nrk=nrk, dt=dt, dx=dx, dy=dy, rdz=rdz,
+                lateral_sides=state.lateral_sides, flat_column=flat_column, stream=stream)
+    launch_scalar_stage_from_carrier(state.theta_current, state.th0, mass_flux, metrics, divx, state.theta_a,
+                                     state.theta_held, theta_divergence_term, state.scratch, weps=theta_weps_f64(),
+                                     **grid)
+    for sp in range(len(state.qmag)):
+        if nrk < CM1_NRKMAX:
+            launch_scalar_stage_from_carrier(state.moisture_current[sp], None, mass_flux, metrics, divx,
+                                             state.moisture_a[sp], state.moisture_held[sp], None, state.scratch,
+                                             weps=moisture_weps_f64(sp, state.qmag), **grid)
+        else:
+            if state.qmag[sp] != CM1_QMAG_F32[0]:
+                raise ValueError("the positive-definite chain's weps is CM1's 1e-2 qmag")
+            launch_scalar_stage_from_carrier(state.moisture_current[sp], None, mass_flux, metrics, divx,
+                                             state.moisture_a[sp], state.moisture_held[sp], None, state.scratch,
+                                             pdef_species=sp, pdef_scratch={"advx": state.advx, "advy": state.advy,
+                                                                            "advz": state.advz}, **grid)
+    state.next_nrk = nrk + 1
+    state.completed = nrk == CM1_NRKMAX
+    receipt = {"nrk": nrk, "dttmp": scalar_moist_rk_stage_timestep_f32(dt, nrk), "species": list(state.names),
+               "theta": "weno5" if nrk == CM1_NRKMAX else "flux5",
+               "moisture": "positive-definite" if nrk == CM1_NRKMAX else "flux5",
+               "vertical": "uniform" if vertical is None else "stretched", "target_arch": "gfx950"}
+    return CudaScalarMoistRKStageLaunch(state, record(stream), receipt)
+
+
 __all__ = [
-    "CM1_QMAG_F32", "CM1_SCALAR_MOIST_SPECIES", "launch_pdef_moisture", "launch_scalar_rk_compose",
-    "launch_scalar_stage_from_carrier",
+    "CM1_QMAG_F32", "CM1_SCALAR_MOIST_SPECIES", "CudaScalarMoistRKStageLaunch", "CudaScalarMoistRKState",
+    "begin_cuda_scalar_moist_rk_step", "create_cuda_scalar_moist_rk_state", "launch_cuda_scalar_moist_rk_stage",
+    "launch_pdef_moisture", "launch_scalar_rk_compose", "launch_scalar_stage_from_carrier",
     "moisture_weps_f64", "pack_scalar", "scalar_moist_rk_stage_timestep_f32", "theta_weps_f64",
 ]
