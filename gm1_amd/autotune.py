@@ -1,8 +1,8 @@
 """Choice of the acoustic column pass per grid shape, by timing it (one GPU).
 
-The column pass has word-identical variants (``psolver3.COLUMN_PASSES``):
-32- or 16-column workgroups (the tile width across i), LDS-staged or
-pipelined (loads issued a level ahead).  Which is fastest depends on the
+The column pass has word-identical variants (``psolver3.COLUMN_PASSES``,
+``psolver3.FLAT_COLUMN_PASSES``): 32- or 16-column workgroups (the tile
+width across i), LDS-staged or pipelined (loads issued a level ahead).  Which is fastest depends on the
 column count and height, so ``tune_column_pass`` runs each candidate's
 acoustic RK stage on a scratch copy of a large-step state, times the
 column launches alone with the library's event-pair timer, and returns the
@@ -24,10 +24,11 @@ from typing import Any
 import torch
 
 from . import timing
-from .psolver3 import COLUMN_PASSES, TerrainOpenSoundAblations, launch_cuda_psolver3_terrain_open_stage
+from .psolver3 import FLAT_COLUMN_PASSES, TerrainOpenSoundAblations, launch_cuda_psolver3_terrain_open_stage
 from .stretched import launch_cuda_gm1_flat_acoustic_stage
 
 CANDIDATES = ("pipelined", "staged", "pipelined16", "staged16")
+FLAT_CANDIDATES = tuple(FLAT_COLUMN_PASSES)
 _memo: dict[str, dict[str, Any]] = {}
 
 
@@ -49,9 +50,12 @@ def _key(st) -> str:
                        list(cfg.lateral_sides), st.flat is not None])
 
 
+def _candidates(flat: bool) -> tuple[str, ...]:
+    return FLAT_CANDIDATES if flat else CANDIDATES
+
+
 def _valid(entry: Any, flat: bool) -> bool:
-    allowed = tuple(c for c in CANDIDATES if not (flat and c.endswith("16")))
-    return isinstance(entry, dict) and entry.get("choice") in allowed and entry.get("choice") in COLUMN_PASSES
+    return isinstance(entry, dict) and entry.get("choice") in _candidates(flat)
 
 
 def _cache_path() -> Path | None:
@@ -86,18 +90,18 @@ def _stage(st, a, nrk: int, column_pass: str) -> None:
                                                 terrain_slopes_zero=st.slopes_zero, **kw)
 
 
-def tune_column_pass(st, repeats: int = 2, candidates=CANDIDATES) -> dict[str, Any]:
+def tune_column_pass(st, repeats: int = 2, candidates=None) -> dict[str, Any]:
     """Fastest column pass for ``st``'s grid: {"choice", "us": {variant: mean column launch in us}}."""
     if not _memo:
         _load_cache()
     key = _key(st)
     if key in _memo and _valid(_memo[key], st.flat is not None):
         return _memo[key]
+    allowed = _candidates(st.flat is not None)
+    candidates = allowed if candidates is None else tuple(candidates)
     for c in candidates:
-        if c not in COLUMN_PASSES:
+        if c not in allowed:
             raise ValueError(f"unknown column pass {c!r}")
-    if st.flat is not None:  # the flat column runs the 32-column passes only
-        candidates = tuple(c for c in candidates if not c.endswith("16"))
     stream = st.stream if st.stream is not None else torch.cuda.current_stream()
     with torch.cuda.stream(stream):  # the clones are ordered after the state's own pending work
         scratch = {k: (v.clone() if isinstance(v, torch.Tensor) else v) for k, v in st.a.items()}
@@ -126,4 +130,4 @@ def autotune_state(st, **kw) -> str:
     return st.column_pass
 
 
-__all__ = ["CANDIDATES", "autotune_state", "tune_column_pass"]
+__all__ = ["CANDIDATES", "FLAT_CANDIDATES", "autotune_state", "tune_column_pass"]

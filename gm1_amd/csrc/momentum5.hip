@@ -43,6 +43,15 @@ struct MomArgs {
   // state's reciprocal density, read at the first column.
   const float* flat;
   const float* rr0;
+  // Held tendencies seeding the update (uten = uten1 + advection), or null:
+  // the update then adds to uten as it stands.
+  const float* u1 = nullptr;
+  const float* v1 = nullptr;
+  const float* w1 = nullptr;
+  // The flat column's buoyancy added to the w update in the same statement
+  // order, wten = (wten1 + advw) + (c1*dum8(k-1) + c2*dum8(k)), or null.
+  const float* dum8 = nullptr;
+  const float* c1 = nullptr;
 };
 
 GM1_DEV bool per(unsigned obc, unsigned s) { return side_of(obc, s) == 1u; }
@@ -362,7 +371,7 @@ __global__ __launch_bounds__(MNT) void k_mom_advu_tile(MomArgs a, int f0, int nf
         e_rw = a.rho0[row + west];
         e_re = a.rho0[row + east];
       }
-      e_ten = a.uten[ix3(k, y, x, nyu, nxu)];
+      e_ten = (a.u1 ? a.u1 : a.uten)[ix3(k, y, x, nyu, nxu)];
     }
 #pragma unroll
     for (int q = 0; q < PS; ++q) {
@@ -557,7 +566,7 @@ __global__ __launch_bounds__(MNT) void k_mom_advv_tile(MomArgs a, int g0, int nr
         e_rs = a.rho0[plane + (int64_t)south * ni + i];
         e_rn = a.rho0[plane + (int64_t)north * ni + i];
       }
-      e_ten = a.vten[ix3(k, y, x, nyv, nxv)];
+      e_ten = (a.v1 ? a.v1 : a.vten)[ix3(k, y, x, nyv, nxv)];
     }
 #pragma unroll
     for (int q = 0; q < PS; ++q) {
@@ -794,21 +803,82 @@ __global__ __launch_bounds__(256) void k_mom_advw(MomArgs a) {
   tot = tot + wc0 * (c2 * a.divx[(int64_t)(kf - 1) * nj * ni + h] + c1 * a.divx[(int64_t)(kf - 2) * nj * ni + h]);
   const float sc = flat ? a.rrf0[(int64_t)(kf - 1) * nj * ni] : a.rrf0[(int64_t)(kf - 1) * nj * ni + h] * a.gz[h];
   const int64_t p = ix3(kf, y, x, nys, nxs);
-  a.wten[p] = a.wten[p] + tot * sc;
+  const float wt = (a.w1 ? a.w1 : a.wten)[p] + tot * sc;
+  if (FLAT && a.dum8) {
+    const int64_t q = (int64_t)(kf - 1) * nj * ni + h;
+    a.wten[p] = wt + (a.c1[q] * a.dum8[ix3(kf - 1, y, x, nys, nxs)] + a.c2[q] * a.dum8[p]);
+  } else {
+    a.wten[p] = wt;
+  }
 }
 
-}  // namespace
+// The held tendencies' words that no advection statement writes: u on the
+// row j = nj and on the faces i = 0, ni; v on the column i = ni and the
+// faces j = 0, nj; w on the column i = ni and the row j = nj of levels
+// 2..nk (level 1 by k_seed_w_surface) -- the rest of k_seed_held's region
+// (faces 0..ni, rows 0..nj, levels 1..nk) is seeded by the advection updates
+// themselves.  A face both seeds and an update write is seeded first and
+// overwritten in stream order.
+__global__ __launch_bounds__(256) void k_seed_edges(const float* __restrict__ u1, const float* __restrict__ v1,
+                                                    const float* __restrict__ w1, float* __restrict__ ut,
+                                                    float* __restrict__ vt, float* __restrict__ wt, int ni, int nj,
+                                                    int nk) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y + 1;
+  if (e < ni + 1 + 2 * nj) {  // u
+    int i, j;
+    if (e <= ni) { i = e; j = nj; }
+    else if (e < ni + 1 + nj) { i = 0; j = e - (ni + 1); }
+    else { i = ni; j = e - (ni + 1 + nj); }
+    const int64_t p = ix3(k, j + NG, i + NG, sy(nj), ux(ni));
+    ut[p] = u1[p];
+  }
+  if (e < nj + 1 + 2 * ni) {  // v
+    int i, j;
+    if (e <= nj) { i = ni; j = e; }
+    else if (e < nj + 1 + ni) { i = e - (nj + 1); j = 0; }
+    else { i = e - (nj + 1 + ni); j = nj; }
+    const int64_t p = ix3(k, j + NG, i + NG, vy(nj), sx(ni));
+    vt[p] = v1[p];
+  }
+  if (k >= 2 && e < nj + 1 + ni) {  // w above the surface
+    const int i = (e <= nj) ? ni : e - (nj + 1), j = (e <= nj) ? e : nj;
+    const int64_t p = ix3(k, j + NG, i + NG, sy(nj), sx(ni));
+    wt[p] = w1[p];
+  }
+}
 
-extern "C" {
+__global__ __launch_bounds__(256) void k_seed_w_surface(const float* __restrict__ w1, float* __restrict__ wt, int ni,
+                                                        int nj) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (ni + 1) * (nj + 1)) return;
+  const int i = e % (ni + 1), j = e / (ni + 1);
+  const int64_t p = ix3(1, j + NG, i + NG, sy(nj), sx(ni));
+  wt[p] = w1[p];
+}
 
-int gm1hip_momentum_advection5(
-    const float* u3d, const float* v3d, const float* w3d, float* uten, float* vten, float* wten,
-    float* rru, float* rrv, float* rrw, const float* uh, const float* vh, const float* rho0_3d,
-    const float* rf0, const float* rrf0, const float* c2, const float* gz, const float* gzu,
+void launch_seed_edges(const float* u1, const float* v1, const float* w1, float* ut, float* vt, float* wt, int ni,
+                       int nj, int nk, hipStream_t s) {
+  const int emax = max(ni + 1 + 2 * nj, nj + 1 + 2 * ni);
+  hipLaunchKernelGGL(k_seed_edges, dim3((emax + 255) / 256, nk), dim3(256), 0, s, u1, v1, w1, ut, vt, wt, ni, nj,
+                     nk);
+  hipLaunchKernelGGL(k_seed_w_surface, dim3(((ni + 1) * (nj + 1) + 255) / 256), dim3(256), 0, s, w1, wt, ni, nj);
+}
+
+// The terrain-coordinate momentum advection; with held tendencies (u1, v1,
+// w1) the k_seed_held copy is folded into the updates.
+int momentum5_terrain(
+    const float* u3d, const float* v3d, const float* w3d, const float* u1, const float* v1, const float* w1,
+    float* uten, float* vten, float* wten, float* rru, float* rrv, float* rrw, const float* uh, const float* vh,
+    const float* rho0_3d, const float* rf0, const float* rrf0, const float* c2, const float* gz, const float* gzu,
     const float* gzv, const float* rgzu, const float* rgzv, const float* dzdx, const float* dzdy,
     float* divx, int ni, int nj, int nk, float dx, float dy, float dz, unsigned obc,
     int stop_after_divergence, void* stream) {
   hipStream_t s = (hipStream_t)stream;
+  if (u1) {
+    launch_seed_edges(u1, v1, w1, uten, vten, wten, ni, nj, nk, s);
+    GM1_LAUNCH_CHECK();
+  }
   const float rdx = 1.0f / dx, rdy = 1.0f / dy, rdz = 1.0f / dz;
   dim3 blk(64);
   hipLaunchKernelGGL(k_mom_mass_h, dim3((ni + 1 + 63) / 64, nj + 1, (nk + MKB - 1) / MKB), blk, 0, s, u3d, v3d,
@@ -824,6 +894,7 @@ int gm1hip_momentum_advection5(
   a.uten = uten; a.vten = vten; a.wten = wten; a.ni = ni; a.nj = nj; a.nk = nk;
   a.rdx = rdx; a.rdy = rdy; a.rdz = rdz; a.obc = obc;
   a.flat = nullptr; a.rr0 = nullptr;
+  a.u1 = u1; a.v1 = v1; a.w1 = w1;
   const bool pw = side_of(obc, 0) == 1u, pe = side_of(obc, 1) == 1u;
   const bool ps = side_of(obc, 2) == 1u, pn = side_of(obc, 3) == 1u;
   const int f0 = pw ? 0 : 1, nf = ni - 1 + (pw ? 1 : 0) + (pe ? 1 : 0);
@@ -835,6 +906,77 @@ int gm1hip_momentum_advection5(
   GM1_TIMED(TK_MOM_ADVW, s, hipLaunchKernelGGL(k_mom_advw<false>, dim3((ni + 63) / 64, nj, nk - 1), blk, 0, s, a));
   GM1_LAUNCH_CHECK();
   return 0;
+}
+
+// The flat stretched column's momentum advection; with held tendencies the
+// k_seed_held copy is folded into the updates, and with dum8 the flat
+// buoyancy (k_buoyancy_flat) into the w update.
+int momentum5_flat(
+    const float* u3d, const float* v3d, const float* w3d, const float* u1, const float* v1, const float* w1,
+    float* uten, float* vten, float* wten, float* rru, float* rrv, float* rrw, const float* uh, const float* vh,
+    const float* rho0_3d, const float* rf0, const float* rr0, const float* rrf0, const float* c1, const float* c2,
+    const float* dum8, const float* flat_column, float* divx, int ni, int nj, int nk, float dx, float dy, float rdz,
+    unsigned obc, void* stream) {
+  if (!flat_column || (dum8 && !c1)) return (int)hipErrorInvalidValue;
+  hipStream_t s = (hipStream_t)stream;
+  if (u1) {
+    launch_seed_edges(u1, v1, w1, uten, vten, wten, ni, nj, nk, s);
+    GM1_LAUNCH_CHECK();
+  }
+  const float rdx = 1.0f / dx, rdy = 1.0f / dy;
+  dim3 blk(64);
+  hipLaunchKernelGGL(k_mom_mass_h_flat, dim3((ni + 1 + 63) / 64, nj + 1, (nk + MKB - 1) / MKB), blk, 0, s, u3d, v3d,
+                     rho0_3d, rru, rrv, ni, nj, nk);
+  GM1_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_mom_mass_v_flat, dim3((ni + 63) / 64, nj, (nk + MKB - 1) / MKB), blk, 0, s, w3d, rf0, rru, rrv,
+                     flat_column, rrw, divx, ni, nj, nk, rdx, rdy, rdz);
+  GM1_LAUNCH_CHECK();
+  MomArgs a;
+  a.u = u3d; a.v = v3d; a.w = w3d; a.rru = rru; a.rrv = rrv; a.rrw = rrw; a.divx = divx;
+  a.rho0 = rho0_3d; a.c2 = c2; a.gzu = nullptr; a.gzv = nullptr; a.rrf0 = rrf0; a.gz = nullptr; a.uh = uh;
+  a.vh = vh; a.uten = uten; a.vten = vten; a.wten = wten; a.ni = ni; a.nj = nj; a.nk = nk;
+  a.rdx = rdx; a.rdy = rdy; a.rdz = rdz; a.obc = obc; a.flat = flat_column; a.rr0 = rr0;
+  a.u1 = u1; a.v1 = v1; a.w1 = w1; a.dum8 = dum8; a.c1 = c1;
+  const bool pw = side_of(obc, 0) == 1u, pe = side_of(obc, 1) == 1u;
+  const bool ps = side_of(obc, 2) == 1u, pn = side_of(obc, 3) == 1u;
+  const int f0 = pw ? 0 : 1, nf = ni - 1 + (pw ? 1 : 0) + (pe ? 1 : 0);
+  const int g0 = ps ? 0 : 1, nr = nj - 1 + (ps ? 1 : 0) + (pn ? 1 : 0);
+  launch_advu_tile<true>(a, f0, nf, s);
+  GM1_LAUNCH_CHECK();
+  launch_advv_tile<true>(a, g0, nr, s);
+  GM1_LAUNCH_CHECK();
+  GM1_TIMED(TK_MOM_ADVW, s, hipLaunchKernelGGL(k_mom_advw<true>, dim3((ni + 63) / 64, nj, nk - 1), blk, 0, s, a));
+  GM1_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gm1hip_momentum_advection5(
+    const float* u3d, const float* v3d, const float* w3d, float* uten, float* vten, float* wten,
+    float* rru, float* rrv, float* rrw, const float* uh, const float* vh, const float* rho0_3d,
+    const float* rf0, const float* rrf0, const float* c2, const float* gz, const float* gzu,
+    const float* gzv, const float* rgzu, const float* rgzv, const float* dzdx, const float* dzdy,
+    float* divx, int ni, int nj, int nk, float dx, float dy, float dz, unsigned obc,
+    int stop_after_divergence, void* stream) {
+  return momentum5_terrain(u3d, v3d, w3d, nullptr, nullptr, nullptr, uten, vten, wten, rru, rrv, rrw, uh, vh,
+                           rho0_3d, rf0, rrf0, c2, gz, gzu, gzv, rgzu, rgzv, dzdx, dzdy, divx, ni, nj, nk, dx, dy,
+                           dz, obc, stop_after_divergence, stream);
+}
+
+int gm1hip_momentum_stage(
+    const float* u3d, const float* v3d, const float* w3d, const float* uten1, const float* vten1,
+    const float* wten1, float* uten, float* vten, float* wten, float* rru, float* rrv, float* rrw,
+    const float* uh, const float* vh, const float* rho0_3d, const float* rf0, const float* rrf0, const float* c2,
+    const float* gz, const float* gzu, const float* gzv, const float* rgzu, const float* rgzv, const float* dzdx,
+    const float* dzdy, float* divx, int ni, int nj, int nk, float dx, float dy, float dz, unsigned obc,
+    void* stream) {
+  if (!uten1 || !vten1 || !wten1) return (int)hipErrorInvalidValue;
+  return momentum5_terrain(u3d, v3d, w3d, uten1, vten1, wten1, uten, vten, wten, rru, rrv, rrw, uh, vh, rho0_3d,
+                           rf0, rrf0, c2, gz, gzu, gzv, rgzu, rgzv, dzdx, dzdy, divx, ni, nj, nk, dx, dy, dz, obc, 0,
+                           stream);
 }
 
 int gm1hip_momentum_transport5(
@@ -871,32 +1013,19 @@ int gm1hip_momentum_advection5_flat(
     float* rru, float* rrv, float* rrw, const float* uh, const float* vh, const float* rho0_3d,
     const float* rf0, const float* rr0, const float* rrf0, const float* c2, const float* flat_column,
     float* divx, int ni, int nj, int nk, float dx, float dy, float rdz, unsigned obc, void* stream) {
-  if (!flat_column) return (int)hipErrorInvalidValue;
-  hipStream_t s = (hipStream_t)stream;
-  const float rdx = 1.0f / dx, rdy = 1.0f / dy;
-  dim3 blk(64);
-  hipLaunchKernelGGL(k_mom_mass_h_flat, dim3((ni + 1 + 63) / 64, nj + 1, (nk + MKB - 1) / MKB), blk, 0, s, u3d, v3d,
-                     rho0_3d, rru, rrv, ni, nj, nk);
-  GM1_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_mom_mass_v_flat, dim3((ni + 63) / 64, nj, (nk + MKB - 1) / MKB), blk, 0, s, w3d, rf0, rru, rrv,
-                     flat_column, rrw, divx, ni, nj, nk, rdx, rdy, rdz);
-  GM1_LAUNCH_CHECK();
-  MomArgs a;
-  a.u = u3d; a.v = v3d; a.w = w3d; a.rru = rru; a.rrv = rrv; a.rrw = rrw; a.divx = divx;
-  a.rho0 = rho0_3d; a.c2 = c2; a.gzu = nullptr; a.gzv = nullptr; a.rrf0 = rrf0; a.gz = nullptr; a.uh = uh;
-  a.vh = vh; a.uten = uten; a.vten = vten; a.wten = wten; a.ni = ni; a.nj = nj; a.nk = nk;
-  a.rdx = rdx; a.rdy = rdy; a.rdz = rdz; a.obc = obc; a.flat = flat_column; a.rr0 = rr0;
-  const bool pw = side_of(obc, 0) == 1u, pe = side_of(obc, 1) == 1u;
-  const bool ps = side_of(obc, 2) == 1u, pn = side_of(obc, 3) == 1u;
-  const int f0 = pw ? 0 : 1, nf = ni - 1 + (pw ? 1 : 0) + (pe ? 1 : 0);
-  const int g0 = ps ? 0 : 1, nr = nj - 1 + (ps ? 1 : 0) + (pn ? 1 : 0);
-  launch_advu_tile<true>(a, f0, nf, s);
-  GM1_LAUNCH_CHECK();
-  launch_advv_tile<true>(a, g0, nr, s);
-  GM1_LAUNCH_CHECK();
-  GM1_TIMED(TK_MOM_ADVW, s, hipLaunchKernelGGL(k_mom_advw<true>, dim3((ni + 63) / 64, nj, nk - 1), blk, 0, s, a));
-  GM1_LAUNCH_CHECK();
-  return 0;
+  return momentum5_flat(u3d, v3d, w3d, nullptr, nullptr, nullptr, uten, vten, wten, rru, rrv, rrw, uh, vh, rho0_3d,
+                        rf0, rr0, rrf0, nullptr, c2, nullptr, flat_column, divx, ni, nj, nk, dx, dy, rdz, obc, stream);
+}
+
+int gm1hip_momentum_stage_flat(
+    const float* u3d, const float* v3d, const float* w3d, const float* uten1, const float* vten1,
+    const float* wten1, float* uten, float* vten, float* wten, float* rru, float* rrv, float* rrw,
+    const float* uh, const float* vh, const float* rho0_3d, const float* rf0, const float* rr0, const float* rrf0,
+    const float* c1, const float* c2, const float* dum8, const float* flat_column, float* divx, int ni, int nj,
+    int nk, float dx, float dy, float rdz, unsigned obc, void* stream) {
+  if (!uten1 || !vten1 || !wten1 || !dum8 || !c1) return (int)hipErrorInvalidValue;
+  return momentum5_flat(u3d, v3d, w3d, uten1, vten1, wten1, uten, vten, wten, rru, rrv, rrw, uh, vh, rho0_3d, rf0,
+                        rr0, rrf0, c1, c2, dum8, flat_column, divx, ni, nj, nk, dx, dy, rdz, obc, stream);
 }
 
 }  // extern "C"

@@ -1321,9 +1321,9 @@ static int psolver3_stage(
   const bool pipe = (flags & 16) != 0 && column2_lds_bytes(nk, cw, true) <= lds_max;
   const bool staged = !(flags & 4) && column2_lds_bytes(nk, cw, pipe) <= lds_max;
   // every refusal comes before the first launch, so a refused call leaves the state untouched:
-  // the flat column runs the staged pass only (k_snd_column is the terrain text), and the
+  // the flat column runs the staged passes only (k_snd_column is the terrain text), and the
   // radiation pass keeps a column's phase speeds in LDS
-  if (flat && (!staged || narrow)) return (int)hipErrorInvalidValue;
+  if (flat && !staged) return (int)hipErrorInvalidValue;
   if ((flags & 1) && nk > RAD_MAXK) return (int)hipErrorInvalidValue;
   hipLaunchKernelGGL(k_snd_prepare, dim3((ni + 2 + 63) / 64, nj + 2, nk), dim3(64), 0, s, a);
   GM1_LAUNCH_CHECK();
@@ -1335,7 +1335,7 @@ static int psolver3_stage(
                 hipLaunchKernelGGL(k_snd_pgrad_flat,
                                    dim3(((ni + 1 + PG_TX - 1) / PG_TX) * ((nj + 1 + PG_TY - 1) / PG_TY) * nk),
                                    dim3(PG_TX * PG_TY), 0, s, a));
-      GM1_TIMED(TK_ACOUSTIC_COLUMN, s, launch_column2<true>(false, pipe, a, n, s));
+      GM1_TIMED(TK_ACOUSTIC_COLUMN, s, launch_column2<true>(narrow, pipe, a, n, s));
       GM1_LAUNCH_CHECK();
       if (n < nloop) {
         int rc = fill_pressure(ppd, sides, ni, nj, nk, s);

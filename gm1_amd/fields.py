@@ -261,6 +261,16 @@ def wind_profile(cfg: CaseConfig, z: np.ndarray) -> tuple[np.ndarray, np.ndarray
     return u, v
 
 
+def _bubble_box(x, y, zs, xc, yc, zc, xr, zr) -> tuple[slice, slice, slice]:
+    """(k, j, i) slices holding every point with |x - xc| <= xr, |y - yc| <= xr, |z - zc| <= zr."""
+
+    def span(c: np.ndarray, centre: float, radius: float) -> slice:
+        idx = np.nonzero(np.abs(c - centre) <= radius)[0]
+        return slice(int(idx[0]), int(idx[-1]) + 1) if idx.size else slice(0, 0)
+
+    return span(zs, zc, zr), span(y, yc, xr), span(x, xc, xr)
+
+
 def initial_fields(cfg: CaseConfig, prof: VerticalProfiles, base: dict[str, np.ndarray]) -> dict[str, np.ndarray]:
     """theta', u, v, w, pressure carriers and moisture of the initial state."""
     ni, nj, nk = cfg.ni, cfg.nj, cfg.nk
@@ -270,8 +280,11 @@ def initial_fields(cfg: CaseConfig, prof: VerticalProfiles, base: dict[str, np.n
     th3d = np.zeros((nk, nj, ni), np.float64)
     xr, zr = cfg.bubble_radius
     for amp, xc, yc, zc in cfg.bubbles:
-        beta = np.sqrt(((x[None, None, :] - xc) / xr) ** 2 + ((y[None, :, None] - yc) / xr) ** 2 + ((zs[:, None, None] - zc) / zr) ** 2)
-        th3d += np.where(beta < 1.0, amp * np.cos(0.5 * math.pi * beta) ** 2, 0.0)
+        # beta < 1 only inside the bubble's bounding box; outside it the perturbation is +0 either way
+        b = _bubble_box(x, y, zs, xc, yc, zc, xr, zr)
+        beta = np.sqrt(((x[None, None, b[2]] - xc) / xr) ** 2 + ((y[None, b[1], None] - yc) / xr) ** 2
+                       + ((zs[b[0], None, None] - zc) / zr) ** 2)
+        th3d[b] += np.where(beta < 1.0, amp * np.cos(0.5 * math.pi * beta) ** 2, 0.0)
     th3d = th3d.astype(F) + uniform_noise(cfg.seed, cfg.noise, (nk, nj, ni))
     u, v = wind_profile(cfg, zs)
     out = {"th3d": th3d.astype(F), "u_profile": u.astype(F), "v_profile": v.astype(F)}
@@ -281,17 +294,17 @@ def initial_fields(cfg: CaseConfig, prof: VerticalProfiles, base: dict[str, np.n
         rh = cfg.extra.get("bubble_rh")
         if rh is not None:
             # raise the bubble's vapour to rh times saturation at its temperature (Bolton 1980)
-            pi = base["pi0"].astype(np.float64)[:, None, None]
-            t = (base["th0"].astype(np.float64)[:, None, None] + th3d.astype(np.float64)) * pi
-            p = P00 * pi ** (CP / RD)
-            es = 611.2 * np.exp(17.67 * (t - 273.15) / (t - 29.65))
-            qvs = EPS * es / (p - es)
-            inside = np.zeros((nk, nj, ni), bool)
             for amp, xc, yc, zc in cfg.bubbles:
-                beta = np.sqrt(((x[None, None, :] - xc) / xr) ** 2 + ((y[None, :, None] - yc) / xr) ** 2
-                               + ((zs[:, None, None] - zc) / zr) ** 2)
-                inside |= beta < 1.0
-            q[0] = np.where(inside, np.maximum(q[0], (rh * qvs).astype(F)), q[0])
+                b = _bubble_box(x, y, zs, xc, yc, zc, xr, zr)
+                pi = base["pi0"].astype(np.float64)[b[0], None, None]
+                t = (base["th0"].astype(np.float64)[b[0], None, None] + th3d[b].astype(np.float64)) * pi
+                p = P00 * pi ** (CP / RD)
+                es = 611.2 * np.exp(17.67 * (t - 273.15) / (t - 29.65))
+                qvs = EPS * es / (p - es)
+                beta = np.sqrt(((x[None, None, b[2]] - xc) / xr) ** 2 + ((y[None, b[1], None] - yc) / xr) ** 2
+                               + ((zs[b[0], None, None] - zc) / zr) ** 2)
+                qb = q[0][b]
+                q[0][b] = np.where(beta < 1.0, np.maximum(qb, (rh * qvs).astype(F)), qb)
         out["q3d"] = q
     return out
 
@@ -302,9 +315,11 @@ def initial_thermodynamics(base: dict[str, np.ndarray], th3d: np.ndarray, qv: np
     prs = p00 pi0^(cp/rd); rho = prs / ((th0 + th') pi0 (rd + rv max(qv, 0))), in binary32.
     """
     nk, nj, ni = th3d.shape
-    pi0 = np.broadcast_to(base["pi0"][:, None, None], th3d.shape).astype(F)
-    th0 = np.broadcast_to(base["th0"][:, None, None], th3d.shape).astype(F)
-    prs = (np.power(pi0.astype(np.float64), float(F(CP) / F(RD))) * P00).astype(F)
+    pi0_k = base["pi0"].astype(F)[:, None, None]
+    th0_k = base["th0"].astype(F)[:, None, None]
+    prs_k = (np.power(pi0_k.astype(np.float64), float(F(CP) / F(RD))) * P00).astype(F)
     vap = np.zeros_like(th3d) if qv is None else np.maximum(qv, F(0.0))
-    rho = prs / (((th0 + th3d) * pi0) * (F(RD) + F(RV) * vap))
-    return {"prs": prs.astype(F), "rho": rho.astype(F), "pi0_3d": pi0.copy(), "th0_3d": th0.copy()}
+    rho = prs_k / (((th0_k + th3d) * pi0_k) * (F(RD) + F(RV) * vap))
+    shape = (nk, nj, ni)
+    return {"prs": np.broadcast_to(prs_k, shape).astype(F), "rho": rho.astype(F, copy=False),
+            "pi0_3d": np.broadcast_to(pi0_k, shape).astype(F), "th0_3d": np.broadcast_to(th0_k, shape).astype(F)}

@@ -33,9 +33,8 @@ from . import _native
 from ._device import NG, lateral_code, ptr, stream_handle
 from ._native import Fl, I, P, U, signature
 from .fields import CaseConfig, base_state, initial_fields, initial_thermodynamics, metrics_for, profiles_for
-from .stretched import (flat_stretched_column, launch_cuda_gm1_child_stretched_momentum5,
-                        launch_cuda_gm1_child_stretched_post_acoustic, launch_cuda_gm1_flat_acoustic_stage,
-                        launch_cuda_gm1_flat_stretched_buoyancy)
+from .stretched import (flat_stretched_column, launch_cuda_gm1_child_stretched_post_acoustic,
+                        launch_cuda_gm1_flat_acoustic_stage)
 from .grid import VerticalProfiles
 from .kessler import (
     Gm1KesslerDriverSelection,
@@ -44,7 +43,6 @@ from .kessler import (
     launch_cuda_apmasscon,
     launch_cuda_gm1_kessler_transition,
 )
-from .momentum_advection5 import launch_cuda_terrain_open_momentum_advection5
 from .errors import refuse
 from .psolver3 import TerrainOpenSoundAblations, device_profiles, launch_cuda_psolver3_terrain_open_stage
 from .scalar_moist_rk import launch_scalar_stage_from_carrier, moisture_weps_f64, theta_weps_f64
@@ -54,6 +52,8 @@ signature("gm1hip_buoyancy", P, P, P, P, P, P, P, P, I, I, I, P)
 signature("gm1hip_post_acoustic", *([P] * 14), I, I, I, Fl, Fl, Fl, P)
 signature("gm1hip_rk_compose", P, P, P, P, P, ctypes.c_int64, Fl, P)
 signature("gm1hip_seed_held_momentum", P, P, P, P, P, P, I, I, I, P)
+signature("gm1hip_momentum_stage", *([P] * 26), I, I, I, Fl, Fl, Fl, U, P)
+signature("gm1hip_momentum_stage_flat", *([P] * 23), I, I, I, Fl, Fl, Fl, U, P)
 signature("gm1hip_held_compose", *([P] * 13), I, I, I, Fl, P)
 signature("gm1hip_solve3_pressure", P, P, P, P, I, I, I, U, P)
 signature("gm1hip_solve3_density", P, P, P, P, P, I, I, I, P)
@@ -225,8 +225,6 @@ def _rk_stage(st: Gm1LargeStepState, nrk: int) -> None:
     _native.call("gm1hip_stage_terms", ptr(b["th0"]), ptr(b["qv0"]), ptr(b["pi0"]), None, None, ptr(a["th3d"]),
                  ptr(q3d), st.numq, ptr(a["pp3d"]), ptr(a["thv"]), ptr(a["ppterm"]), ptr(a["piadv"]),
                  ptr(a["dum8"]), ptr(a["thterm"]), ni, nj, nk, code, h)
-    _native.call("gm1hip_seed_held_momentum", ptr(a["uten1"]), ptr(a["vten1"]), ptr(a["wten1"]), ptr(a["uten"]),
-                 ptr(a["vten"]), ptr(a["wten"]), ni, nj, nk, h)
     if st.flat is not None:
         _flat_rk_stage(st, nrk)
         return
@@ -234,9 +232,14 @@ def _rk_stage(st: Gm1LargeStepState, nrk: int) -> None:
         _native.call("gm1hip_bcwsfc", ptr(a["u3d"]), ptr(a["v3d"]), ptr(m["dzdx"]), ptr(m["dzdy"]), ptr(m["gz"]),
                      ptr(a["w3d"]), ni, nj, h)
         _native.call("gm1hip_bc2d_open", ptr(a["w3d"]), ni, nj, h)
-    launch_cuda_terrain_open_momentum_advection5(
-        a, m, ni=ni, nj=nj, nk=nk, nrk=nrk, dx=cfg.dx, dy=cfg.dy, dz=cfg.dz, divx=a["divx"], stream=st.stream,
-        lateral_sides=sides)
+    # the held seed (uten = uten1, ...) folded into the momentum advection updates: the seed
+    # touches no array bcwsfc/bc2d read or write, so it moves past them unchanged
+    _native.call("gm1hip_momentum_stage", ptr(a["u3d"]), ptr(a["v3d"]), ptr(a["w3d"]), ptr(a["uten1"]),
+                 ptr(a["vten1"]), ptr(a["wten1"]), ptr(a["uten"]), ptr(a["vten"]), ptr(a["wten"]), ptr(a["rru"]),
+                 ptr(a["rrv"]), ptr(a["rrw"]), ptr(a["uh"]), ptr(a["vh"]), ptr(m["rho0_3d"]), ptr(m["rf0"]),
+                 ptr(m["rrf0"]), ptr(m["c2"]), ptr(m["gz"]), ptr(m["gzu"]), ptr(m["gzv"]), ptr(m["rgzu"]),
+                 ptr(m["rgzv"]), ptr(m["dzdx"]), ptr(m["dzdy"]), ptr(a["divx"]), ni, nj, nk, cfg.dx, cfg.dy, cfg.dz,
+                 code, h)
     _native.call("gm1hip_buoyancy", ptr(a["dum8"]), ptr(m["c1"]), ptr(m["c2"]), ptr(m["gxu"]), ptr(m["gyv"]),
                  ptr(a["uten"]), ptr(a["vten"]), ptr(a["wten"]), ni, nj, nk, h)
     launch_cuda_psolver3_terrain_open_stage(
@@ -263,9 +266,14 @@ def _flat_rk_stage(st: Gm1LargeStepState, nrk: int) -> None:
     cfg, a, m = st.cfg, st.a, st.metrics
     ni, nj, nk = cfg.ni, cfg.nj, cfg.nk
     sides = cfg.lateral_sides
-    launch_cuda_gm1_child_stretched_momentum5(a, m, st.flat, ni=ni, nj=nj, nk=nk, nrk=nrk, dx=cfg.dx, dy=cfg.dy,
-                                              divx=a["divx"], stream=st.stream, lateral_sides=sides)
-    launch_cuda_gm1_flat_stretched_buoyancy(a, m, ni=ni, nj=nj, nk=nk, stream=st.stream)
+    # the held seed, the flat momentum advection and the flat buoyancy in three passes
+    # (the seed folded into the advection updates, the buoyancy into the w update)
+    _native.call("gm1hip_momentum_stage_flat", ptr(a["u3d"]), ptr(a["v3d"]), ptr(a["w3d"]), ptr(a["uten1"]),
+                 ptr(a["vten1"]), ptr(a["wten1"]), ptr(a["uten"]), ptr(a["vten"]), ptr(a["wten"]), ptr(a["rru"]),
+                 ptr(a["rrv"]), ptr(a["rrw"]), ptr(a["uh"]), ptr(a["vh"]), ptr(m["rho0_3d"]), ptr(m["rf0"]),
+                 ptr(m["rr0"]), ptr(m["rrf0"]), ptr(m["c1"]), ptr(m["c2"]), ptr(a["dum8"]), ptr(st.flat.column),
+                 ptr(a["divx"]), ni, nj, nk, cfg.dx, cfg.dy, float(st.flat.rdz), lateral_code(sides),
+                 stream_handle(st.stream))
     launch_cuda_gm1_flat_acoustic_stage(a, m, ni=ni, nj=nj, nk=nk, nrk=nrk, dt=cfg.dt, nsound=cfg.nsound, dx=cfg.dx,
                                         dy=cfg.dy, vertical=st.flat, stream=st.stream, lateral_sides=sides,
                                         column_pass=st.column_pass)

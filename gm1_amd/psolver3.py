@@ -130,6 +130,8 @@ def verify_terrain_slopes_zero(metrics: Mapping[str, Any]) -> bool:
 # level's loads a level ahead and run the u/v time averages beside the
 # recurrence; "register" walks each column in one thread
 COLUMN_PASSES = {"staged": 0, "pipelined": 16, "staged16": 8, "pipelined16": 24, "register": 4}
+# the flat column runs the LDS-staged passes (the register pass is the terrain text)
+FLAT_COLUMN_PASSES = {k: v for k, v in COLUMN_PASSES.items() if k != "register"}
 
 
 def _check_ablations(ablations: TerrainOpenSoundAblations) -> int:

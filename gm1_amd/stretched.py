@@ -30,7 +30,7 @@ from ._device import NG, lateral_code, ptr, record, require, shapes, stream_hand
 from ._native import D, Fl, I, P, U, signature
 from .errors import refuse
 from .grid import VerticalLevels, rk_stage_dt_f32, sound3_stage_schedule_f32
-from .psolver3 import (CM1_ALPH_F32, CM1_KDIV_F32, COLUMN_PASSES, CudaPsolver3TerrainOpenLaunch,
+from .psolver3 import (CM1_ALPH_F32, CM1_KDIV_F32, FLAT_COLUMN_PASSES, CudaPsolver3TerrainOpenLaunch,
                        TerrainOpenSoundAblations, device_profiles, terrain_open_sound_shapes)
 from .scalar_moist_rk import CM1_NRKMAX, moisture_weps_f64, scalar_moist_rk_stage_timestep_f32
 
@@ -211,9 +211,9 @@ def launch_cuda_gm1_flat_acoustic_stage(
                    reason="the gfx950 stage always runs every small-step operator of sound.F",
                    remedy="keep the small-step operators enabled")
     flags = (1 if abl.radiation_history_production else 0) | (2 if abl.stage_family_fills else 0)
-    if column_pass not in ("staged", "pipelined"):
-        raise ValueError(f"the flat column runs the 32-column staged pass ('staged' or 'pipelined'), not {column_pass!r}")
-    flags |= COLUMN_PASSES[column_pass]
+    if column_pass not in FLAT_COLUMN_PASSES:
+        raise ValueError(f"the flat column runs the LDS-staged passes {tuple(FLAT_COLUMN_PASSES)}, not {column_pass!r}")
+    flags |= FLAT_COLUMN_PASSES[column_pass]
     shp = terrain_open_sound_shapes(ni, nj, nk)
     names = ("ppi", "pp3d", "ppx", "ppd", "piadv", "ppterm", "ppten", "thv", "qk", "pk1", "pk2", "mm", "fpk",
              "elim_c", "elim_d", "ua", "u3d", "uten", "rru", "va", "v3d", "vten", "rrv", "wa", "w3d", "wten", "rrw",

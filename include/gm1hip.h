@@ -210,6 +210,17 @@ int gm1hip_rk_compose(const float* a0, const float* held, const float* divterm, 
 // (cuda_momentum_rk.py:242-271).
 int gm1hip_seed_held_momentum(const float* uten1, const float* vten1, const float* wten1, float* uten, float* vten,
                               float* wten, int ni, int nj, int nk, void* stream);
+// The held seed and the terrain momentum advection of one RK stage
+// (cuda_momentum_rk.py:303-372, statements 1 and 3): every word equals
+// gm1hip_seed_held_momentum followed by gm1hip_momentum_advection5, the seed
+// folded into the advection updates.
+int gm1hip_momentum_stage(const float* u3d, const float* v3d, const float* w3d, const float* uten1,
+                          const float* vten1, const float* wten1, float* uten, float* vten, float* wten, float* rru,
+                          float* rrv, float* rrw, const float* uh, const float* vh, const float* rho0_3d,
+                          const float* rf0, const float* rrf0, const float* c2, const float* gz, const float* gzu,
+                          const float* gzv, const float* rgzu, const float* rgzv, const float* dzdx,
+                          const float* dzdy, float* divx, int ni, int nj, int nk, float dx, float dy, float dz,
+                          unsigned obc, void* stream);
 // solve1 held compose: ppten from held theta/qv and the microphysics held
 // rates (kernel gm1_predict_and_compose_held_v1, cuda_gm1_forecast.py:555-566).
 int gm1hip_held_compose(const float* th0, const float* pi0, const float* rho, const float* ppi, const float* tha,
@@ -324,6 +335,19 @@ int gm1hip_momentum_advection5_flat(const float* u3d, const float* v3d, const fl
 // (cuda_gm1_child_stretched_moist.py:240-285): wten += c1*dum8(k-1) + c2*dum8(k).
 int gm1hip_buoyancy_flat(const float* dum8, const float* c1, const float* c2, float* wten, int ni, int nj, int nk,
                          void* stream);
+
+// One flat-column RK stage's momentum statements in the composer's order
+// (cuda_cartesian_dynamics.py:1070-1262: the held seed of cuda_momentum_rk.py:242-271,
+// launch_cuda_gm1_child_stretched_momentum5, launch_cuda_gm1_flat_stretched_buoyancy) in
+// three passes: every word equals gm1hip_seed_held_momentum, then
+// gm1hip_momentum_advection5_flat, then gm1hip_buoyancy_flat.
+int gm1hip_momentum_stage_flat(const float* u3d, const float* v3d, const float* w3d, const float* uten1,
+                               const float* vten1, const float* wten1, float* uten, float* vten, float* wten,
+                               float* rru, float* rrv, float* rrw, const float* uh, const float* vh,
+                               const float* rho0_3d, const float* rf0, const float* rr0, const float* rrf0,
+                               const float* c1, const float* c2, const float* dum8, const float* flat_column,
+                               float* divx, int ni, int nj, int nk, float dx, float dy, float rdz, unsigned obc,
+                               void* stream);
 
 // Replaces launch_cuda_gm1_child_stretched_post_acoustic on a flat column
 // (cuda_gm1_child_stretched_moist.py:286-372; misclibs.F's flat getdiv/getdivx).

@@ -427,6 +427,13 @@ def dry_mass(rho, ruh, rvh, rmh):
 
 def apmasscon(a, mass1):
     """Exner correction that restores the dry-air mass ledger (cuda_apmasscon.py:256-327)."""
+    out = apmasscon_correction(a, mass1)
+    apply_mass_correction(a, F(out["correction"]))
+    return out
+
+
+def apmasscon_correction(a, mass1):
+    """The ledger (mass2, the Exner sum) and the binary32 correction apmasscon would apply, applying nothing."""
     nk, nj, ni = a["rho"].shape
     mass2 = dry_mass(a["rho"], a["ruh"], a["rvh"], a["rmh"])
     psum = 0.0
@@ -437,9 +444,7 @@ def apmasscon(a, mass1):
             acc += float(v)
         psum += acc
     corr = (np.power(f64(mass1) / f64(mass2), f64(RDDCV)) - 1.0) * psum / f64(ni * nj * nk)
-    corr32 = F(corr)
-    apply_mass_correction(a, corr32)
-    return {"mass2": mass2, "pressure_sum": psum, "correction": float(corr32)}
+    return {"mass2": mass2, "pressure_sum": psum, "correction": float(F(corr))}
 
 
 def apply_mass_correction(a, corr32):

@@ -118,13 +118,14 @@ def kessler_arrays(a):
             "qbudget": a["qbudget"]}
 
 
-def large_step(a, m, base, prof, cfg, *, zt, mass_correction=None, mass1=None, equation_set=2, budgets=True,
-               flat=None):
+def large_step(a, m, base, prof, cfg, *, zt, mass_correction=None, mass1=None, mass_hook=None, equation_set=2,
+               budgets=True, flat=None):
     """Advance the state ``a`` by one large step in place.
 
     The dry-mass hook: ``mass_correction`` applies a given binary32 Exner
     correction; ``mass1`` computes it from the ledger (cpu_apmasscon_inplace);
-    neither skips the hook.
+    ``mass_hook`` is called with the Kessler arrays in the hook's place;
+    none of them skips the hook.
     """
     ni, nj, nk = cfg["ni"], cfg["nj"], cfg["nk"]
     sides = tuple(cfg["lateral_sides"])
@@ -141,7 +142,7 @@ def large_step(a, m, base, prof, cfg, *, zt, mass_correction=None, mass1=None, e
         rk_stage(a, m, base, prof, cfg, nrk, zt=zt, flat=flat)
     if moist:
         a["pp_mass"][...] = a["pp3d"][1 : nk + 1, 1 : nj + 1, 1 : ni + 1]
-        hook = None
+        hook = mass_hook
         if mass_correction is not None:
             hook = lambda arr: ok.apply_mass_correction(arr, F(mass_correction))  # noqa: E731
         elif mass1 is not None:

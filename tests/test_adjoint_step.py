@@ -9,6 +9,7 @@ at the same trajectory.
 
 from __future__ import annotations
 
+import math
 import re
 from pathlib import Path
 
@@ -261,29 +262,47 @@ def test_gpu_checkpoint_schedules_agree(device):
 
 @pytest.mark.gpu
 def test_gpu_gradient_central_differences_and_autograd(device):
-    """Central differences (eps = 1e-3 of each field's scale) along 4 smooth random directions, and autograd."""
+    """Central differences (eps = 1e-3 of each field's scale) along 8 smooth random directions, and autograd.
+
+    This small state carries white noise of 0.5 m/s in every wind, so many
+    faces sit within eps of an upwind switch of the 5th-order fluxes, where
+    J is not differentiable; the binary32 central differences agree to 2e-3
+    here (the 192x192x64 window below is held to 1e-3).  A direction nearly
+    orthogonal to the gradient (cosine below 1e-2) has a directional
+    derivative made of cancelling terms, which binary32 central differences
+    cannot resolve; such a draw is replaced by the next one (at most 8
+    replacements), and the count is printed.
+    """
     import torch
 
     from gm1_amd.adjoint_step import StormIntensity, smooth_direction
 
     cfg, host, a, prof, c, S32, step, win, S0 = _window_case(device, 3, 2)
     J, g = win.gradient(S0)
+    gnorm = math.sqrt(sum(float(torch.sum(g[n].double() ** 2)) for n in ("th", "u", "v")))
     rng = np.random.default_rng(31)
-    worst = 0.0
-    for trial in range(4):
+    worst, accepted, drawn = 0.0, 0, 0
+    while accepted < 8:
+        assert drawn < 16, "too many directions nearly orthogonal to the gradient"
+        drawn += 1
         d = {n: torch.from_numpy((smooth_direction(rng, S32[n].shape) * np.abs(S32[n]).max()).astype(
             np.float32)).to(device) for n in ("th", "u", "v")}
+        ad = sum(float(torch.sum(g[n].double() * d[n].double())) for n in d)
+        dnorm = math.sqrt(sum(float(torch.sum(d[n].double() ** 2)) for n in d))
+        if abs(ad) < 1e-2 * gnorm * dnorm:
+            print(f"draw {drawn}: cosine {ad / (gnorm * dnorm):.1e}, replaced")
+            continue
         eps = 1e-3
         Sp = {**S0, **{n: S0[n] + eps * d[n] for n in d}}
         Sm = {**S0, **{n: S0[n] - eps * d[n] for n in d}}
         Jp = torch.sum(win.weights.double() * win.advance(Sp, 3)["w"].double() ** 2)
         Jm = torch.sum(win.weights.double() * win.advance(Sm, 3)["w"].double() ** 2)
         fd = float((Jp - Jm) / (2 * eps))
-        ad = sum(float(torch.sum(g[n].double() * d[n].double())) for n in d)
         rel = abs(fd - ad) / abs(fd)
         worst = max(worst, rel)
-        print(f"direction {trial}: fd={fd:.8e} adjoint={ad:.8e} rel={rel:.2e}")
-    assert worst <= 1e-2, worst
+        accepted += 1
+        print(f"direction {accepted}: fd={fd:.8e} adjoint={ad:.8e} rel={rel:.2e}")
+    assert worst <= 2e-3, worst
     base = {n: S0[n] for n in ("w", "pp", "ppx")}
     th, u, v = (S0[n].clone().requires_grad_(True) for n in ("th", "u", "v"))
     Jt = StormIntensity.apply(win, base, th, u, v)
@@ -291,3 +310,42 @@ def test_gpu_gradient_central_differences_and_autograd(device):
     assert float(Jt.detach()) == float(J)
     for got, ref in ((gth, g["th"]), (gu, g["u"]), (gv, g["v"])):
         assert _rel(got.cpu().numpy(), ref.cpu().numpy().astype(np.float64)) <= 1e-5
+
+
+@pytest.mark.gpu
+def test_gpu_storm_intensity_gradient_over_120_steps_central_differences(device):
+    """The 192x192x64 dry periodic case: the 120-step window's gradient (16 free checkpoint slots) against
+    central differences of the device forward, eps = 1e-3 of each field's scale, along 8 smooth random
+    directions of (theta', u, v), each within 1e-3."""
+    import torch
+
+    from gm1_amd.adjoint_step import DryPeriodicStep, WindowGradient, from_padded, smooth_direction
+    from gm1_amd.fields import adjoint_c5
+
+    cfg = adjoint_c5()
+    print("case", cfg.name, "window 120", flush=True)
+    host = host_initial_state(cfg)
+    S0 = {n: v.contiguous().clone() for n, v in from_padded(
+        {k: torch.from_numpy(np.ascontiguousarray(host["arrays"][k])).to(device)
+         for k in ("u3d", "v3d", "w3d", "th3d", "pp3d", "ppx")}, cfg.ni, cfg.nj, cfg.nk).items()}
+    step = DryPeriodicStep(cfg, host["metrics"], host["base"], device)
+    win = WindowGradient(step, 120, 16, metrics=host["metrics"])
+    J, g = win.gradient(S0)
+    torch.cuda.synchronize()
+    print("J", float(J), "forward steps", win.forward_steps, flush=True)
+    rng = np.random.default_rng(11)
+    worst = 0.0
+    for trial in range(8):
+        d = {n: torch.from_numpy((smooth_direction(rng, tuple(S0[n].shape)) * float(S0[n].abs().max())).astype(
+            np.float32)).to(device) for n in ("th", "u", "v")}
+        eps = 1e-3
+        Jp = torch.sum(win.weights.double() * win.advance({**S0, **{n: S0[n] + eps * d[n] for n in d}},
+                                                          120)["w"].double() ** 2)
+        Jm = torch.sum(win.weights.double() * win.advance({**S0, **{n: S0[n] - eps * d[n] for n in d}},
+                                                          120)["w"].double() ** 2)
+        fd = float((Jp - Jm) / (2 * eps))
+        ad = sum(float(torch.sum(g[n].double() * d[n].double())) for n in d)
+        rel = abs(fd - ad) / abs(fd)
+        worst = max(worst, rel)
+        print(f"direction {trial + 1}: fd={fd:.8e} adjoint={ad:.8e} rel={rel:.2e}", flush=True)
+    assert worst <= 1e-3, worst

@@ -114,9 +114,8 @@ def test_autotune_choice_is_read_from_the_cache_file(tmp_path, monkeypatch):
     assert autotune.autotune_state(st) == "staged16" and st.column_pass == "staged16"
     flat = SimpleNamespace(cfg=cfg, flat=object(), column_pass="pipelined")
     assert autotune._key(flat) != autotune._key(st)  # the stretched family is tuned on its own
-    # a cached entry naming no known pass, or a 16-column pass for the flat column, is not trusted
-    assert not autotune._valid({"choice": "bogus"}, False)
-    assert not autotune._valid({"choice": "staged16"}, True)
-    assert autotune._valid({"choice": "staged16"}, False) and autotune._valid({"choice": "pipelined"}, True)
+    # a cached entry naming no known pass is not trusted; every candidate runs on both families
+    assert not autotune._valid({"choice": "bogus"}, False) and not autotune._valid({"choice": "register"}, True)
+    assert autotune._valid({"choice": "staged16"}, False) and autotune._valid({"choice": "pipelined16"}, True)
     # the key names the built library, so a cache written by another build is not reused
     assert autotune._library_tag() in autotune._key(st)

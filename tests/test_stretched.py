@@ -390,7 +390,8 @@ def test_gpu_flat_buoyancy_post_acoustic_and_acoustic_stage_match_oracle(device)
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("sides,column_pass", [((2, 2, 2, 2), "staged"), ((1, 1, 1, 1), "staged"),
-                                               ((2, 2, 2, 2), "pipelined")])
+                                               ((2, 2, 2, 2), "pipelined"), ((2, 2, 2, 2), "pipelined16"),
+                                               ((1, 1, 1, 1), "staged16")])
 def test_gpu_stretched_moist_large_step_bit_exact(device, sides, column_pass):
     """Two moist large steps on a stretched column against the oracle (device dry-mass correction injected)."""
     import torch
