@@ -1,4 +1,4 @@
-// Kessler warm rain (CM1 ptype=1, efall=0) and its large-step transition on gfx950.
+// Kessler warm rain (CM1 ptype=1, efall=0 or 1) and its large-step transition on gfx950.
 //
 // Replaces cuda_gm1_kessler_driver.py / cuda_gm1_kessler_local.py /
 // cuda_gm1_kessler_large_step.py / cuda_apmasscon.py.  Design:
@@ -6,7 +6,11 @@
 //     k: pdefq of qv, qc, qr (binary64 column sums in k order, so the column
 //     rescale and its budget are the reference's words), the fall speed, and
 //     the flux-form fallout in nfall substeps with a carried flux.  (The
-//     reference runs pre-pdefq and fallout one thread per j-ROW.)
+//     reference runs pre-pdefq and fallout one thread per j-ROW.)  With
+//     efall=1 the energy fallout (getcvm, getefall, the guarded temperature /
+//     pressure / Exner / theta update, cuda_gm1_kessler_efall.py:201-256) runs
+//     in the same column pass between the surface-energy budget and the fallout,
+//     and the held temperature it leaves is the local conversion's.
 //   * k_kes_cell: ONE thread per cell for autoconversion, accretion, rain
 //     evaporation and the saturation adjustment (eqtset 2 iterates to
 //     |dT| <= 2e-4 K, at most 100 times); the eqtset=2 held recapture of the
@@ -71,6 +75,9 @@ struct KesArgs {
   unsigned* decisions;
   // held rates of the large-step transition (eqtset 2; null when not recapturing)
   float* qpten; float* qtten; float* qvten; float* qcten;
+  // efall=1: mf (nk+1, nj, ni) and the held temperature (nk, nj, ni) the column
+  // pass writes and the local conversion reads; both null for efall=0
+  const float* mf; float* temp;
   int ni, nj, nk, nseg, eqtset;
   float dx, dy, dz, dt;
 };
@@ -170,6 +177,36 @@ __global__ __launch_bounds__(64) void k_kes_column(KesArgs a) {
     term = term * ru;
     term = term * rv;
     a.colbud[4 * plane + h] = (double)term;
+  }
+  if (a.temp) {  // efall=1: energy fallout of every level below the top, bottom up
+    // level k's tendency reads level k+1's temperature before level k+1 is updated
+    const float rdzf = 1.0f / a.dz;
+    float tk = t0;
+    for (int k = 0; k < nk; ++k) {
+      const int64_t p = (int64_t)k * plane + h;
+      float tnew = tk, tup = 0.0f;
+      if (k + 1 < nk) {
+        const int64_t pu = p + plane;
+        tup = (a.th0[pu] + a.th3d[pu]) * (a.pi0[pu] + a.pp3d[pu]);
+        const float qvk = a.qv[p], qrk = qr[p];
+        const float cvm = (a.eqtset <= 1) ? c.cv : ((c.cv + c.cvv * qvk) + CPL * a.qc[p]) + CPL * qrk;
+        float vertical = CPL * (tup - tk);
+        vertical = vertical * rdzf;
+        vertical = vertical * a.mf[(int64_t)(k + 1) * plane + h];
+        const float tend = ((qrk * vt[p]) * (vertical + 9.81f)) / cvm;
+        const float inc = a.dt * tend;
+        if (fabsf(inc) >= 0.0001f) {
+          tnew = tk + inc;
+          const float pr = ((rho[p] * RD) * tnew) * (1.0f + qvk * c.reps);
+          const float pp = pow_r(pr * c.rp00, c.rovcp) - a.pi0[p];
+          a.prs[p] = pr;
+          a.pp3d[p] = pp;
+          a.th3d[p] = tnew / (a.pi0[p] + pp) - a.th0[p];
+        }
+      }
+      a.temp[p] = tnew;
+      tk = tup;
+    }
   }
   // fallout
   const float rdz = 1.0f / a.dz;
@@ -275,7 +312,7 @@ __global__ __launch_bounds__(64) void k_kes_cell(KesArgs a) {
     const float dt = a.dt;
     // ---- local conversion (cuda_gm1_kessler_local.py:290-416)
     float pisum = pi0 + pp;
-    const float tmp = (th0 + th) * pisum;
+    const float tmp = a.temp ? a.temp[p] : (th0 + th) * pisum;  // efall=1: the held temperature
     float ar = 0.0f, cr = 0.0f, er = 0.0f;
     unsigned word = 0u;
     if (qc > 0.0f) {
@@ -719,17 +756,21 @@ int64_t gm1hip_kessler_scratch_bytes(int ni, int nj, int nk) {
   return col + wave + stat + rows + 8LL * 2048 + 64;
 }
 
-int gm1hip_kessler_driver_decisions(const float* ruh, const float* rvh, const float* rmh, const float* mh,
-                                    const float* pi0, const float* th0, float* rho, float* rr, float* pp3d,
-                                    float* th3d, float* prs, float* qv, float* qc, float* qr, float* fall_velocity,
-                                    float* rain, float* prate, double* asq, double* qbudget, int* status,
-                                    float* qpten, float* qtten, float* qvten, float* qcten, unsigned* decisions,
-                                    void* work, int64_t work_bytes, int ni, int nj, int nk, float dx, float dy,
-                                    float dz, float dt, int eqtset, void* stream) {
+}  // extern "C"
+
+namespace {
+int kessler_driver(const float* ruh, const float* rvh, const float* rmh, const float* mh, const float* mf,
+                   const float* pi0, const float* th0, float* rho, float* rr, float* pp3d, float* th3d, float* prs,
+                   float* qv, float* qc, float* qr, float* fall_velocity, float* rain, float* prate,
+                   float* temperature, double* asq, double* qbudget, int* status, float* qpten, float* qtten,
+                   float* qvten, float* qcten, unsigned* decisions, void* work, int64_t work_bytes, int ni, int nj,
+                   int nk, float dx, float dy, float dz, float dt, int eqtset, void* stream) {
   if (eqtset != 1 && eqtset != 2) return (int)hipErrorInvalidValue;
   if (work_bytes < gm1hip_kessler_scratch_bytes(ni, nj, nk)) return (int)hipErrorInvalidValue;
+  if ((mf == nullptr) != (temperature == nullptr) || (mf && nk < 2)) return (int)hipErrorInvalidValue;
   hipStream_t s = (hipStream_t)stream;
   KesArgs a{};
+  a.mf = mf; a.temp = temperature;
   a.pi0 = pi0; a.th0 = th0; a.ruh = ruh; a.rvh = rvh; a.rmh = rmh; a.mh = mh;
   a.rho = rho; a.rr = rr; a.pp3d = pp3d; a.th3d = th3d; a.prs = prs; a.qv = qv; a.qc = qc; a.qr = qr;
   a.vt = fall_velocity; a.rain = rain; a.prate = prate;
@@ -756,6 +797,34 @@ int gm1hip_kessler_driver_decisions(const float* ruh, const float* rvh, const fl
                      dz);
   GM1_LAUNCH_CHECK();
   return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int gm1hip_kessler_driver_decisions(const float* ruh, const float* rvh, const float* rmh, const float* mh,
+                                    const float* pi0, const float* th0, float* rho, float* rr, float* pp3d,
+                                    float* th3d, float* prs, float* qv, float* qc, float* qr, float* fall_velocity,
+                                    float* rain, float* prate, double* asq, double* qbudget, int* status,
+                                    float* qpten, float* qtten, float* qvten, float* qcten, unsigned* decisions,
+                                    void* work, int64_t work_bytes, int ni, int nj, int nk, float dx, float dy,
+                                    float dz, float dt, int eqtset, void* stream) {
+  return kessler_driver(ruh, rvh, rmh, mh, nullptr, pi0, th0, rho, rr, pp3d, th3d, prs, qv, qc, qr, fall_velocity,
+                        rain, prate, nullptr, asq, qbudget, status, qpten, qtten, qvten, qcten, decisions, work,
+                        work_bytes, ni, nj, nk, dx, dy, dz, dt, eqtset, stream);
+}
+
+int gm1hip_kessler_driver_efall(const float* ruh, const float* rvh, const float* rmh, const float* mh,
+                                const float* mf, const float* pi0, const float* th0, float* rho, float* rr,
+                                float* pp3d, float* th3d, float* prs, float* qv, float* qc, float* qr,
+                                float* fall_velocity, float* rain, float* prate, float* temperature, double* asq,
+                                double* qbudget, int* status, float* qpten, float* qtten, float* qvten, float* qcten,
+                                unsigned* decisions, void* work, int64_t work_bytes, int ni, int nj, int nk, float dx,
+                                float dy, float dz, float dt, int eqtset, void* stream) {
+  if (!mf || !temperature) return (int)hipErrorInvalidValue;
+  return kessler_driver(ruh, rvh, rmh, mh, mf, pi0, th0, rho, rr, pp3d, th3d, prs, qv, qc, qr, fall_velocity, rain,
+                        prate, temperature, asq, qbudget, status, qpten, qtten, qvten, qcten, decisions, work,
+                        work_bytes, ni, nj, nk, dx, dy, dz, dt, eqtset, stream);
 }
 
 int gm1hip_kessler_driver(const float* ruh, const float* rvh, const float* rmh, const float* mh, const float* pi0,

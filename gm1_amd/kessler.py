@@ -19,7 +19,7 @@ from typing import Any, Mapping
 import torch
 
 from . import _native
-from ._device import ptr, stream_handle
+from ._device import no_alias, ptr, stream_handle
 from ._native import D, Fl, I, P, signature
 from .errors import refuse
 
@@ -29,6 +29,7 @@ signature("gm1hip_dry_mass_scratch_bytes", I, I, I, restype=I64)
 signature("gm1hip_kessler_driver", *([P] * 25), I64, I, I, I, Fl, Fl, Fl, Fl, I, P)
 signature("gm1hip_kessler_driver_decisions", *([P] * 26), I64, I, I, I, Fl, Fl, Fl, Fl, I, P)
 signature("gm1hip_kessler_driver_noalloc", *([P] * 24), I, I, I, Fl, Fl, Fl, Fl, I, P)
+signature("gm1hip_kessler_driver_efall", *([P] * 28), I64, I, I, I, Fl, Fl, Fl, Fl, I, P)
 
 # Size of the no-allocation driver's device workspace (a file-scope array in the library).
 KESSLER_NOALLOC_MAX_BYTES = 64 << 20
@@ -38,7 +39,7 @@ signature("gm1hip_apmasscon", *([P] * 13), D, P, I64, I, I, I, P)
 
 @dataclass(frozen=True)
 class Gm1KesslerDriverSelection:
-    """The sealed native branch (cuda_gm1_kessler_driver.py:100-160); efall=1 is not built."""
+    """The sealed native branch (cuda_gm1_kessler_driver.py:100-160), efall 0 or 1."""
 
     ptype: int = 1
     equation_set: int = 1
@@ -67,10 +68,10 @@ def validate_selection(selection: Gm1KesslerDriverSelection) -> None:
             refuse(code="GM1-KESSLER-DRIVER-SELECTION", gate="physics-seam", option=name, value=value,
                    reason="the complete Kessler operator is sealed to one exact native branch",
                    remedy=f"use {name}={wanted!r}")
-    if selection.efall != 0:
+    if type(selection.efall) is not int or selection.efall not in (0, 1):
         refuse(code="GM1-KESSLER-DRIVER-EFALL", gate="physics-seam", option="efall", value=selection.efall,
-               reason="the gfx950 driver implements the efall=0 branch only",
-               remedy="use efall=0")
+               reason="the authority admits exactly the efall=0 and efall=1 branches",
+               remedy="use efall=0 or efall=1")
     if type(selection.equation_set) is not int or selection.equation_set not in (1, 2):
         refuse(code="GM1-KESSLER-DRIVER-EQTSET", gate="physics-seam", option="equation_set",
                value=selection.equation_set,
@@ -93,6 +94,7 @@ class Gm1KesslerDriverWorkspace:
     ni: int
     nj: int
     nk: int
+    temperature: torch.Tensor | None = None  # efall=1: the held temperature of the last call
 
 
 def create_cuda_gm1_kessler_driver_workspace(ni: int, nj: int, nk: int, device: Any = "cuda") -> Gm1KesslerDriverWorkspace:
@@ -102,9 +104,16 @@ def create_cuda_gm1_kessler_driver_workspace(ni: int, nj: int, nk: int, device: 
     return Gm1KesslerDriverWorkspace(scratch, status, ni, nj, nk)
 
 
-def _check(arrays: Mapping[str, Any], ni: int, nj: int, nk: int) -> dict[str, torch.Tensor]:
+def _check(arrays: Mapping[str, Any], ni: int, nj: int, nk: int, efall: int = 0) -> dict[str, torch.Tensor]:
     out = {}
-    for name, shape in gm1_kessler_driver_shapes(ni, nj, nk).items():
+    shapes = gm1_kessler_driver_shapes(ni, nj, nk)
+    if efall:
+        if nk < 2:
+            raise ValueError("efall=1 requires nk >= 2")
+        shapes["mf"] = (nk + 1, nj, ni)
+    for name, shape in shapes.items():
+        if name not in arrays:
+            raise ValueError(f"missing GM1 Kessler array: {name}")
         t = arrays[name]
         dtype = torch.float64 if name in ("asq", "qbudget") else torch.float32
         if not isinstance(t, torch.Tensor) or t.dtype != dtype or tuple(t.shape) != shape or not t.is_contiguous():
@@ -119,9 +128,14 @@ def launch_cuda_gm1_kessler_driver(
     arrays: Mapping[str, Any], *, ni: int, nj: int, nk: int, dx: float, dy: float, dz: float, dt: float,
     selection: Gm1KesslerDriverSelection = DEFAULT_GM1_KESSLER_DRIVER_SELECTION, stream: Any | None = None,
     workspace: Gm1KesslerDriverWorkspace | None = None, held: Mapping[str, Any] | None = None,
-    decisions: torch.Tensor | None = None, **_: Any,
+    decisions: torch.Tensor | None = None, temperature: torch.Tensor | None = None, **_: Any,
 ) -> Gm1KesslerDriverWorkspace:
-    """Every ptype=1 phase (pdefq, fall speed, fallout, local, satadj) in place.
+    """Every ptype=1 phase (pdefq, fall speed, [efall,] fallout, local, satadj) in place.
+
+    ``selection.efall == 1`` needs ``arrays["mf"]`` (nk+1, nj, ni) and runs the
+    energy fallout before the mass fallout (cuda_gm1_kessler_driver.py:482-551);
+    the held temperature it leaves lands in ``temperature`` (float32 (nk, nj,
+    ni), allocated when not given) and is kept on the workspace.
 
     With ``held`` (qpten/qtten/qvten/qcten holding the snapshot of the
     eqtset=2 transition) the held rates are recaptured in the same pass.
@@ -132,7 +146,7 @@ def launch_cuda_gm1_kessler_driver(
     8..15 its iteration count, bit 16 it did not converge.
     """
     validate_selection(selection)
-    a = _check(arrays, ni, nj, nk)
+    a = _check(arrays, ni, nj, nk, selection.efall)
     if decisions is not None and (not isinstance(decisions, torch.Tensor) or decisions.dtype != torch.int32
                                   or tuple(decisions.shape) != (nk, nj, ni) or not decisions.is_contiguous()
                                   or decisions.device != a["rho"].device):
@@ -140,6 +154,25 @@ def launch_cuda_gm1_kessler_driver(
     if workspace is None:
         workspace = create_cuda_gm1_kessler_driver_workspace(ni, nj, nk, a["rho"].device)
     h = held or {}
+    if selection.efall == 1:
+        if temperature is None:
+            temperature = workspace.temperature
+        if temperature is None or tuple(temperature.shape) != (nk, nj, ni):
+            temperature = torch.empty((nk, nj, ni), dtype=torch.float32, device=a["rho"].device)
+        if (temperature.dtype != torch.float32 or not temperature.is_contiguous()
+                or temperature.device != a["rho"].device):
+            raise ValueError(f"temperature must be a contiguous float32 tensor of shape {(nk, nj, ni)}")
+        no_alias({**a, "temperature": temperature}, "Kessler arrays and the held temperature must not alias")
+        workspace.temperature = temperature
+        _native.call(
+            "gm1hip_kessler_driver_efall",
+            *(ptr(a[n]) for n in ("ruh", "rvh", "rmh", "mh", "mf", "pi0", "th0", "rho", "rr", "pp3d", "th3d", "prs",
+                                  "qv", "qc", "qr", "fall_velocity", "rain", "prate")),
+            ptr(temperature), ptr(a["asq"]), ptr(a["qbudget"]),
+            ptr(workspace.status), ptr(h.get("qpten")), ptr(h.get("qtten")), ptr(h.get("qvten")),
+            ptr(h.get("qcten")), ptr(decisions), ptr(workspace.scratch), workspace.scratch.numel(), ni, nj, nk,
+            float(dx), float(dy), float(dz), float(dt), selection.equation_set, stream_handle(stream))
+        return workspace
     _native.call(
         "gm1hip_kessler_driver_decisions",
         *(ptr(a[n]) for n in ("ruh", "rvh", "rmh", "mh", "pi0", "th0", "rho", "rr", "pp3d", "th3d", "prs", "qv",
@@ -162,6 +195,10 @@ def launch_cuda_gm1_kessler_driver_noalloc(
     Calls that overlap on one GPU share that array and race.
     """
     validate_selection(selection)
+    if selection.efall != 0:
+        refuse(code="GM1-KESSLER-NOALLOC-EFALL", gate="physics-seam", option="efall", value=selection.efall,
+               reason="the no-allocation driver runs the efall=0 branch (efall=1 needs a held-temperature buffer)",
+               remedy="use launch_cuda_gm1_kessler_driver for efall=1")
     a = _check(arrays, ni, nj, nk)
     need = int(_native.query("gm1hip_kessler_scratch_bytes", ni, nj, nk))
     if need > KESSLER_NOALLOC_MAX_BYTES:
@@ -220,7 +257,7 @@ def launch_cuda_gm1_kessler_transition(
     given, ``rr <- rho``, the driver, and the held recapture.
     """
     validate_selection(selection)
-    a = _check(arrays, ni, nj, nk)
+    a = _check(arrays, ni, nj, nk, selection.efall)
     eqt = selection.equation_set
     hook = mass1 is not None
     _native.call("gm1hip_kessler_transition_pre", ptr(pressure_family), ptr(a["pp3d"]), ptr(a["th3d"]),
@@ -377,7 +414,7 @@ def create_cuda_gm1_kessler_large_step_transition(
     for name, value, minimum in (("ni", ni, 1), ("nj", nj, 1), ("nk", nk, 2)):
         if type(value) is not int or value < minimum:
             raise ValueError(f"{name} must be an integer >= {minimum}")
-    a = _check(arrays, ni, nj, nk)
+    a = _check(arrays, ni, nj, nk, selection.efall)
     dev = a["rho"].device
     held = Gm1KesslerHeldCarriers(*(torch.zeros((nk, nj, ni), dtype=torch.float32, device=dev) for _ in range(4)))
     done = 0

@@ -276,6 +276,21 @@ int gm1hip_kessler_driver_decisions(const float* ruh, const float* rvh, const fl
                                     void* scratch, int64_t scratch_bytes, int ni, int nj, int nk, float dx, float dy,
                                     float dz, float dt, int eqtset, void* stream);
 
+// The driver with efall=1 (cuda_gm1_kessler_driver.py:482-551 with
+// launch_cuda_gm1_kessler_efall, cuda_gm1_kessler_efall.py:380-431): the
+// energy fallout (getcvm, getefall, the temperature / prs / pp3d / th3d
+// update where |dt * tendency| >= 1e-4, every level below the top) between
+// the surface-energy budget and the mass fallout, and the local conversion
+// reading the held temperature.  mf is (nk+1, nj, ni); temperature (nk, nj,
+// ni) receives the held temperature; nk >= 2.  decisions may be NULL.
+int gm1hip_kessler_driver_efall(const float* ruh, const float* rvh, const float* rmh, const float* mh,
+                                const float* mf, const float* pi0, const float* th0, float* rho, float* rr,
+                                float* pp3d, float* th3d, float* prs, float* qv, float* qc, float* qr,
+                                float* fall_velocity, float* rain, float* prate, float* temperature, double* asq,
+                                double* qbudget, int* status, float* qpten, float* qtten, float* qvten, float* qcten,
+                                unsigned* decisions, void* scratch, int64_t scratch_bytes, int ni, int nj, int nk,
+                                float dx, float dy, float dz, float dt, int eqtset, void* stream);
+
 // No counterpart in the reference: gm1hip_kessler_driver for callers that
 // cannot allocate (code under HIP graph capture, tight serving loops, harnesses
 // that forbid hipMalloc).  The same kernels and results, with no workspace

@@ -9,7 +9,9 @@ reference's statement order:
                           budget, fallout in ``nfall`` substeps, the local
                           conversion (``cpu_gm1_kessler_local_inplace``,
                           cuda_gm1_kessler_local.py:251-426) and satadj
-                          (:337-428), ptype 1, efall 0, eqtset 1 or 2;
+                          (:337-428), ptype 1, efall 0 or 1 (``efall``:
+                          ``cpu_gm1_kessler_efall_inplace``,
+                          cuda_gm1_kessler_efall.py:201-256), eqtset 1 or 2;
 * ``large_step_transition`` -> ``cpu_gm1_kessler_large_step_inplace``
                           (cuda_gm1_kessler_large_step.py:183-292): eqtset=2
                           held remove/snapshot, pre-physics EOS, the dry-mass
@@ -318,13 +320,48 @@ def satadj(a, *, dx, dy, dz, equation_set):
     return iterations
 
 
-def kessler_driver(a, *, dx, dy, dz, dt, equation_set=2, budgets=True):
-    """Every phase of the ptype=1 driver (efall=0) in place on the mass arrays of ``a``.
+GRAVITY = F(9.81)
+EFALL_MIN = F(0.0001)
+
+
+def efall(a, temperature, *, dz, dt, equation_set):
+    """cpu_gm1_kessler_efall_inplace (cuda_gm1_kessler_efall.py:201-256; kessler.F:485-522,
+    mp_driver.F:200-225): the mixture heat capacity, the energy-fallout tendency of every level below
+    the top, and where |dt * tendency| >= 1e-4 the updated temperature, pressure, Exner perturbation and
+    theta'.  In place on ``temperature`` (the driver's held temperature) and a's pp3d, th3d, prs; ``a``
+    also holds mf (nk+1, nj, ni) and the fall speed.  Returns (cvm, tendency)."""
+    qv, qc, qr, vt = a["qv"], a["qc"], a["qr"], a["fall_velocity"]
+    if equation_set <= 1:
+        cvm = np.full(qv.shape, CV, F)
+    else:
+        cvm = ((CV + CVV * qv) + CPL * qc) + CPL * qr
+    rdz = F(1.0) / F(dz)
+    vertical = CPL * (temperature[1:] - temperature[:-1])
+    vertical = vertical * rdz
+    vertical = vertical * a["mf"][1:-1]
+    tendency = np.zeros(qv.shape, F)
+    tendency[:-1] = ((qr[:-1] * vt[:-1]) * (vertical + GRAVITY)) / cvm[:-1]
+    inc = F(dt) * tendency[:-1]
+    hit = np.abs(inc) >= EFALL_MIN
+    t_new = temperature[:-1] + inc
+    prs = ((a["rho"][:-1] * RD) * t_new) * (F(1.0) + qv[:-1] * REPS)
+    pp = _pow(prs * RP00, ROVCP) - a["pi0"][:-1]
+    th = t_new / (a["pi0"][:-1] + pp) - a["th0"][:-1]
+    for name, new in ((temperature, t_new), (a["prs"], prs), (a["pp3d"], pp), (a["th3d"], th)):
+        name[:-1] = np.where(hit, new, name[:-1])
+    return cvm, tendency
+
+
+def kessler_driver(a, *, dx, dy, dz, dt, equation_set=2, budgets=True, efall_on=0):
+    """Every phase of the ptype=1 driver in place on the mass arrays of ``a``.
 
     ``a`` holds (nk, nj, ni) float32 ruh/rvh/rmh/mh/pi0/th0/rho/rr/pp3d/th3d/
     prs/qv/qc/qr/fall_velocity, rain (1, nj, ni), prate (nj, ni) and float64
     asq (3,), qbudget (10,).  With ``budgets=False`` the binary64 diagnostics
-    are skipped (they never feed back into the fields).
+    are skipped (they never feed back into the fields).  ``efall_on=1`` runs
+    the energy fallout (``efall``, which needs a["mf"]) between the
+    surface-energy budget and the mass fallout, and the local conversion then
+    reads its held temperature (cuda_gm1_kessler_driver.py:482-551).
     """
     dx32, dy32, dz32, dt32 = F(dx), F(dy), F(dz), F(dt)
     a["dx"], a["dy"] = dx32, dy32
@@ -346,15 +383,19 @@ def kessler_driver(a, *, dx, dy, dz, dt, equation_set=2, budgets=True):
             for i in range(ni):
                 bud += float(term[j, i])
             a["qbudget"][6] += bud * float(dx32) * float(dy32)
+    held = temperature.copy()
+    energy = efall(a, held, dz=dz32, dt=dt32, equation_set=equation_set) if efall_on else None
     nfall = fallout(a, dt=dt32, dz=dz32)
-    local = local_conversion(a, dt=dt32, equation_set=equation_set)
+    local = local_conversion(a, dt=dt32, equation_set=equation_set,
+                             temperature_override=held if efall_on else None)
     if budgets:
         for p, name in enumerate(("autoconversion", "accretion", "evaporation")):
             lv = _budget_by_level(local[name], a["rr"], a["ruh"], a["rvh"], a["rmh"])
             for k in range(len(lv)):
                 a["qbudget"][2 + p] = a["qbudget"][2 + p] + lv[k] * vol
     iters = satadj(a, dx=dx32, dy=dy32, dz=dz32, equation_set=equation_set)
-    return {"temperature_initial": temperature, "local": local, "iterations": iters, "max_fall_substeps": nfall,
+    return {"temperature_initial": temperature, "temperature_held": held, "efall": energy, "local": local,
+            "iterations": iters, "max_fall_substeps": nfall,
             "decisions": decision_words(local["branches"], iters, equation_set)}
 
 
@@ -388,7 +429,7 @@ def eos(a):
     a["rho"][...] = a["prs"] / (rho * gas)
 
 
-def large_step_transition(a, held, *, dx, dy, dz, dt, equation_set=2, hook=None, budgets=True):
+def large_step_transition(a, held, *, dx, dy, dz, dt, equation_set=2, hook=None, budgets=True, efall_on=0):
     """cpu_gm1_kessler_large_step_inplace: held remove/snapshot, EOS, hook, rr<-rho, driver, recapture."""
     dt32 = F(dt)
     if equation_set == 2:
@@ -399,7 +440,8 @@ def large_step_transition(a, held, *, dx, dy, dz, dt, equation_set=2, hook=None,
     if hook is not None:
         hook(a)
     a["rr"][...] = a["rho"]
-    out = kessler_driver(a, dx=dx, dy=dy, dz=dz, dt=dt, equation_set=equation_set, budgets=budgets)
+    out = kessler_driver(a, dx=dx, dy=dy, dz=dz, dt=dt, equation_set=equation_set, budgets=budgets,
+                         efall_on=efall_on)
     if equation_set == 2:
         for s, h in HELD:
             held[h][...] = (a[s] - held[h]) / dt32

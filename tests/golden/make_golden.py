@@ -401,6 +401,66 @@ def kessler():
     _save("kessler", **out)
 
 
+def kessler_efall():
+    """cpu_gm1_kessler_efall_inplace (the isolated getcvm/getefall/update, eqtset 1 and 2),
+    cpu_gm1_kessler_driver_inplace with efall=1 (eqtset 1 and 2, one and several fall substeps) and
+    cpu_gm1_kessler_large_step_inplace with efall=1 (eqtset 2 with held rates)."""
+    _ref()
+    from cm1core.cuda_gm1_kessler_driver import Gm1KesslerDriverSelection, cpu_gm1_kessler_driver_inplace
+    from cm1core.cuda_gm1_kessler_efall import Gm1KesslerEfallSelection, cpu_gm1_kessler_efall_inplace
+    from cm1core.cuda_gm1_kessler_large_step import cpu_gm1_kessler_large_step_inplace
+
+    out = {}
+    ni, nj, nk = 7, 5, 12
+    out["dims"] = np.array([ni, nj, nk], np.int64)
+
+    def with_mf(rng, a):
+        a["mf"] = (1.0 + 0.1 * rng.uniform(-1.0, 1.0, (nk + 1, nj, ni))).astype(F)
+        return a
+
+    # the isolated operator on a post-geterain state (fall speed formed, temperature held)
+    for case, (dt, dz, eqt) in enumerate(((1.5, 250.0, 2), (6.0, 100.0, 1))):
+        rng = np.random.default_rng(90 + case)
+        k = with_mf(rng, kessler_state(rng, ni, nj, nk, dz=dz))
+        rho, qr = k["rho"].astype(np.float64), k["qr"].astype(np.float64)
+        vt = np.where(qr > 1e-12, 14.34 * np.abs(rho * qr) ** 0.1346 * np.sqrt(1.15 / rho), 0.0)
+        e = {n: k[n] for n in ("mf", "pi0", "th0", "rho", "qv", "qc", "qr", "pp3d", "th3d", "prs")}
+        e["fall_velocity"] = vt.astype(F)
+        e["temperature"] = ((k["th0"] + k["th3d"]) * (k["pi0"] + k["pp3d"])).astype(F)
+        out.update({f"e{case}_in_{n}": v.copy() for n, v in e.items()})
+        res = cpu_gm1_kessler_efall_inplace(e, ni=ni, nj=nj, nk=nk, dz=dz, dt=dt,
+                                            selection=Gm1KesslerEfallSelection(equation_set=eqt))
+        out.update({f"e{case}_out_{n}": v for n, v in e.items()})
+        out[f"e{case}_cvm"] = res["mixture_cv"]
+        out[f"e{case}_tendency"] = res["energy_tendency"]
+        out[f"e{case}_params"] = np.array([dt, dz, eqt], np.float64)
+    # the whole driver with efall=1
+    cases = ((0, 1.5, 250.0, 2), (1, 20.0, 100.0, 2), (2, 6.0, 250.0, 1))
+    for case, dt, dz, eqt in cases:
+        rng = np.random.default_rng(95 + case)
+        a = with_mf(rng, kessler_state(rng, ni, nj, nk, dz=dz))
+        out.update({f"d{case}_in_{k}": v.copy() for k, v in a.items()})
+        res = cpu_gm1_kessler_driver_inplace(a, ni=ni, nj=nj, nk=nk, dx=250.0, dy=250.0, dz=dz, dt=dt,
+                                             selection=Gm1KesslerDriverSelection(equation_set=eqt, efall=1))
+        out.update({f"d{case}_out_{k}": v for k, v in a.items()})
+        out[f"d{case}_iterations"] = res["satadj"]["iterations"]
+        out[f"d{case}_nfall"] = np.array(res["max_fall_substeps"])
+        out[f"d{case}_temperature"] = res["temperature_held"]
+        out[f"d{case}_params"] = np.array([dt, dz, eqt], np.float64)
+    # the large-step transition with efall=1 and held rates from a previous step
+    rng = np.random.default_rng(99)
+    a = with_mf(rng, kessler_state(rng, ni, nj, nk, dz=250.0))
+    held = {n: rng.normal(0.0, s, (nk, nj, ni)).astype(F)
+            for n, s in (("qpten", 1e-6), ("qtten", 1e-3), ("qvten", 1e-7), ("qcten", 1e-7))}
+    out.update({f"t_in_{k}": v.copy() for k, v in a.items()})
+    out.update({f"t_in_{k}": v.copy() for k, v in held.items()})
+    cpu_gm1_kessler_large_step_inplace(a, held, ni=ni, nj=nj, nk=nk, dx=250.0, dy=250.0, dz=250.0, dt=1.5,
+                                       selection=Gm1KesslerDriverSelection(equation_set=2, efall=1))
+    out.update({f"t_out_{k}": v for k, v in a.items()})
+    out.update({f"t_out_{k}": v for k, v in held.items()})
+    _save("kessler_efall", **out)
+
+
 def periodic_inputs(rng, ni, nj, nk, *, smooth):
     """Inputs of the periodic acoustic stage (our own construction), halos periodic."""
     _ref()
@@ -489,7 +549,7 @@ def sound3_periodic():
 
 
 GENERATORS = {"advection5": advection5, "momentum5": momentum5, "psolver3": psolver3, "stage": stage, "pdef": pdef,
-              "kessler": kessler, "sound3_periodic": sound3_periodic}
+              "kessler": kessler, "kessler_efall": kessler_efall, "sound3_periodic": sound3_periodic}
 
 
 if __name__ == "__main__":
