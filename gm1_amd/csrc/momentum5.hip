@@ -812,6 +812,261 @@ __global__ __launch_bounds__(256) void k_mom_advw(MomArgs a) {
   }
 }
 
+// vertical flux of w at scalar level k (wz) with the column read through Q
+template <class Get>
+GM1_DEV float wz_g(float m, int k, int nk, Get Q) {
+  float r;
+  if (k == 1) r = (m >= 0.0f) ? 0.5f * (Q(1) + Q(2)) : flx3(Q(3), Q(2), Q(1));
+  else if (k == 2) r = (m >= 0.0f) ? flx3(Q(1), Q(2), Q(3)) : flx5(Q(5), Q(4), Q(3), Q(2), Q(1));
+  else if (k == nk - 1) r = (m >= 0.0f) ? flx5(Q(k - 2), Q(k - 1), Q(k), Q(k + 1), Q(k + 2)) : flx3(Q(k + 2), Q(k + 1), Q(k));
+  else if (k == nk) r = (m >= 0.0f) ? flx3(Q(k - 1), Q(k), Q(k + 1)) : 0.5f * (Q(k) + Q(k + 1));
+  else r = (m >= 0.0f) ? flx5(Q(k - 2), Q(k - 1), Q(k), Q(k + 1), Q(k + 2)) : flx5(Q(k + 3), Q(k + 2), Q(k + 1), Q(k), Q(k - 1));
+  return m * r;
+}
+
+// advw (the statements of k_mom_advw) as a MTX x MTY tile of cells marching a
+// chunk of w levels 2..nk: per level the tile's w plane (3-cell halo) and its
+// c2 plane (1-cell halo, clamped as the face weights clamp) are staged in LDS,
+// every x and y face flux of the tile is formed once into LDS together with
+// its mass flux (the open sides' fallback to the neighbouring face reads it),
+// and the vertical flux comes from a 7-level window of the thread's own
+// column, the lower flux, rru/rrv of the level below, divx and dum8 carried.
+constexpr int WLX = MTX + 6, WLY = MTY + 6;
+template <bool FLAT>
+// (registers held to four waves per SIMD: two 512-lane workgroups per CU)
+__global__ __launch_bounds__(MNT) __attribute__((amdgpu_waves_per_eu(4)))
+void k_mom_advw_tile(MomArgs a, int kchunk, int nchunks) {
+  __shared__ float lw[WLY][WLX];            // w: LDS row r = padded row ja + r, col c = padded x ia + c
+  __shared__ float lc[MTY + 2][MTX + 2];    // c2 of cells (clamp(ja-1+r), clamp(ia-1+c)) at level kf-1
+  __shared__ float lfx[MTY][MTX + 1], lmx[MTY][MTX + 1];  // x flux and mass flux at face ia + t
+  __shared__ float lfy[MTY + 1][MTX], lmy[MTY + 1][MTX];  // y flux and mass flux at face ja + r
+  const int ni = a.ni, nj = a.nj, nk = a.nk;
+  const int tx = threadIdx.x % MTX, ty = threadIdx.x / MTX;
+  const int gx = (ni + MTX - 1) / MTX, gy = (nj + MTY - 1) / MTY;
+  const int tile = xcd_tile((int)blockIdx.x, gx * gy * nchunks);
+  const int ia = (tile % gx) * MTX, ja = ((tile / gx) % gy) * MTY;
+  const int kb = 2 + (tile / (gx * gy)) * kchunk, ke = min(nk, kb + kchunk - 1);
+  const int i = ia + tx, j = ja + ty;
+  const bool live = i < ni && j < nj;
+  const int nxu = ux(ni), nyu = sy(nj), nxs = sx(ni), nys = sy(nj), nyv = vy(nj);
+  const int64_t wst = (int64_t)nys * nxs, plane = (int64_t)nj * ni;
+  const int y = j + NG, x = i + NG;
+  const int64_t h = (int64_t)j * ni + i;
+  const int64_t wcol = (int64_t)y * nxs + x;
+  constexpr int PS = (WLY * WLX + MNT - 1) / MNT;
+  constexpr int CS = ((MTY + 2) * (MTX + 2) + MNT - 1) / MNT;
+  constexpr int XS = (MTY * (MTX + 1) + MNT - 1) / MNT;
+  constexpr int YS = ((MTY + 1) * MTX + MNT - 1) / MNT;
+  // the faces a slot forms (fixed across levels, so rru/rrv of the level below ride in registers)
+  auto rx_at = [&](int k, int q) {
+    const int sl = threadIdx.x + q * MNT;
+    const int r = sl / (MTX + 1), t = sl % (MTX + 1);
+    const int pf = ia + t, jj = ja + r;
+    return (sl < MTY * (MTX + 1) && jj < nj && pf <= ni) ? a.rru[ix3(k, jj + NG, pf + NG, nyu, nxu)] : 0.0f;
+  };
+  auto ry_at = [&](int k, int q) {
+    const int sl = threadIdx.x + q * MNT;
+    const int r = sl / MTX, t = sl % MTX;
+    const int pf = ja + r, ii = ia + t;
+    return (sl < (MTY + 1) * MTX && ii < ni && pf <= nj) ? a.rrv[ix3(k, pf + NG, ii + NG, nyv, nxs)] : 0.0f;
+  };
+  const float* colw = a.w + wcol;
+  float win[7];  // w(kf-3 .. kf+3) of this cell's column
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    const int lev = kb - 3 + q;
+    win[q] = (live && lev >= 0 && lev <= nk + 2) ? colw[(int64_t)lev * wst] : 0.0f;
+  }
+  float rxp[XS], ryp[YS];
+#pragma unroll
+  for (int q = 0; q < XS; ++q) rxp[q] = rx_at(kb - 1, q);
+#pragma unroll
+  for (int q = 0; q < YS; ++q) ryp[q] = ry_at(kb - 1, q);
+  float rrw_lo = 0.0f, rrw_hi = 0.0f, zlo = 0.0f, dxlo = 0.0f, d8lo = 0.0f;
+  if (live) {
+    rrw_lo = a.rrw[(int64_t)(kb - 1) * wst + wcol];
+    rrw_hi = a.rrw[(int64_t)kb * wst + wcol];
+    zlo = wz_g(0.5f * (rrw_lo + rrw_hi), kb - 1, nk, [&](int lev) { return win[lev - (kb - 3)]; });
+    dxlo = a.divx[(int64_t)(kb - 2) * plane + h];
+    if (FLAT && a.dum8) d8lo = a.dum8[(int64_t)(kb - 1) * wst + wcol];
+  }
+  struct Lv {
+    float plane[PS], c2[CS], rx[XS], ry[YS];
+    float wnext, rrwn, divx, base, d8, c1;
+  };
+  auto fetch = [&](int kf) {
+    Lv v;
+    const float* wp = a.w + (int64_t)kf * wst;
+#pragma unroll
+    for (int q = 0; q < PS; ++q) {
+      int sl = threadIdx.x + q * MNT;
+      asm volatile("" : "+v"(sl));  // recomputed per level, not held across the march
+      const int r = sl / WLX, c = sl % WLX;
+      const int yy = ja + r, xx = ia + c;
+      v.plane[q] = (sl < WLY * WLX && yy < nys && xx < nxs) ? wp[(int64_t)yy * nxs + xx] : 0.0f;
+    }
+    const float* c2k = a.c2 + (int64_t)(kf - 1) * plane;
+#pragma unroll
+    for (int q = 0; q < CS; ++q) {
+      int sl = threadIdx.x + q * MNT;
+      asm volatile("" : "+v"(sl));
+      const int r = sl / (MTX + 2), c = sl % (MTX + 2);
+      v.c2[q] = (sl < (MTY + 2) * (MTX + 2)) ? c2k[(int64_t)clampi(ja - 1 + r, nj) * ni + clampi(ia - 1 + c, ni)]
+                                              : 0.0f;
+    }
+#pragma unroll
+    for (int q = 0; q < XS; ++q) v.rx[q] = rx_at(kf, q);
+#pragma unroll
+    for (int q = 0; q < YS; ++q) v.ry[q] = ry_at(kf, q);
+    v.wnext = v.rrwn = v.divx = v.base = v.d8 = v.c1 = 0.0f;
+    if (live) {
+      const int lnext = kf + 4;
+      v.wnext = (lnext <= nk + 2) ? colw[(int64_t)lnext * wst] : 0.0f;
+      v.rrwn = a.rrw[(int64_t)(kf + 1) * wst + wcol];
+      v.divx = a.divx[(int64_t)(kf - 1) * plane + h];
+      v.base = (a.w1 ? a.w1 : a.wten)[(int64_t)kf * wst + wcol];
+      if (FLAT && a.dum8) {
+        v.d8 = a.dum8[(int64_t)kf * wst + wcol];
+        v.c1 = a.c1[(int64_t)(kf - 1) * plane + h];
+      }
+    }
+    return v;
+  };
+  const bool pw = opn(a.obc, 0), pe = opn(a.obc, 1), ps = opn(a.obc, 2), pn = opn(a.obc, 3);
+  Lv nx = fetch(kb);
+  for (int kf = kb; kf <= ke; ++kf) {
+    const Lv cu = nx;
+    if (kf < ke) nx = fetch(kf + 1);
+#pragma unroll
+    for (int q = 0; q < PS; ++q) {
+      const int sl = threadIdx.x + q * MNT;
+      if (sl < WLY * WLX) lw[sl / WLX][sl % WLX] = cu.plane[q];
+    }
+#pragma unroll
+    for (int q = 0; q < CS; ++q) {
+      const int sl = threadIdx.x + q * MNT;
+      if (sl < (MTY + 2) * (MTX + 2)) lc[sl / (MTX + 2)][sl % (MTX + 2)] = cu.c2[q];
+    }
+    __syncthreads();
+    // x faces ia + t of the tile's rows (wfx_base)
+#pragma unroll
+    for (int q = 0; q < XS; ++q) {
+      const int sl = threadIdx.x + q * MNT;
+      const int r = sl / (MTX + 1), t = sl % (MTX + 1);
+      if (sl < MTY * (MTX + 1) && ja + r < nj && ia + t <= ni) {
+        const float c2h = 0.5f * (lc[r + 1][t] + lc[r + 1][t + 1]);
+        const float c1h = 1.0f - c2h;
+        const float m = c2h * cu.rx[q] + c1h * rxp[q];
+        const float* L = lw[r + 3] + t;  // L[c] = w at padded x (ia + t + NG) - 3 + c
+        const float rv = (m >= 0.0f) ? flx5(L[0], L[1], L[2], L[3], L[4]) : flx5(L[5], L[4], L[3], L[2], L[1]);
+        lfx[r][t] = m * rv;
+        lmx[r][t] = m;
+      }
+    }
+    // y faces ja + r of the tile's columns (wfy_base)
+#pragma unroll
+    for (int q = 0; q < YS; ++q) {
+      const int sl = threadIdx.x + q * MNT;
+      const int r = sl / MTX, t = sl % MTX;
+      if (sl < (MTY + 1) * MTX && ja + r <= nj && ia + t < ni) {
+        const float c2h = 0.5f * (lc[r][t + 1] + lc[r + 1][t + 1]);
+        const float c1h = 1.0f - c2h;
+        const float m = c2h * cu.ry[q] + c1h * ryp[q];
+        const int c = t + 3;
+        const float rv = (m >= 0.0f) ? flx5(lw[r][c], lw[r + 1][c], lw[r + 2][c], lw[r + 3][c], lw[r + 4][c])
+                                     : flx5(lw[r + 5][c], lw[r + 4][c], lw[r + 3][c], lw[r + 2][c], lw[r + 1][c]);
+        lfy[r][t] = m * rv;
+        lmy[r][t] = m;
+      }
+    }
+    float zhi = 0.0f;
+    if (live) zhi = wz_g(0.5f * (rrw_hi + cu.rrwn), kf, nk, [&](int lev) { return win[lev - (kf - 3)]; });
+    __syncthreads();
+    if (live) {
+      // the open sides take the neighbouring face's flux where the mass flux points inward
+      float fw = lfx[ty][tx], fe = lfx[ty][tx + 1];
+      if (i == 0 && pw && lmx[ty][0] >= 0.0f) fw = lfx[ty][1];
+      if (i == ni - 1 && pe && lmx[ty][tx + 1] <= 0.0f) fe = lfx[ty][tx];
+      float fs = lfy[ty][tx], fn = lfy[ty + 1][tx];
+      if (j == 0 && ps && lmy[0][tx] >= 0.0f) fs = lfy[1][tx];
+      if (j == nj - 1 && pn && lmy[ty + 1][tx] <= 0.0f) fn = lfy[ty][tx];
+      float ax, ay;
+      if (FLAT) {
+        ax = ((-(fe - fw)) * a.rdx) * a.uh[x];
+        ay = ((-(fn - fs)) * a.rdy) * a.vh[y];
+      } else {
+        ax = (-(fe - fw)) * (a.rdx * a.uh[x]);
+        ay = (-(fn - fs)) * (a.rdy * a.vh[y]);
+      }
+      const float c2 = lc[ty + 1][tx + 1];
+      const float c1 = 1.0f - c2;
+      const float wc0 = win[3];
+      if (i == 0 && pw) {
+        const float* Rt = a.rru + ix3(kf, y, 0, nyu, nxu);
+        const float* Rb = a.rru + ix3(kf - 1, y, 0, nyu, nxu);
+        const float m = c2 * Rt[NG] + c1 * Rb[NG];
+        if (m >= 0.0f) {
+          const float c = c1 * (Rb[NG + 1] - Rb[NG]) + c2 * (Rt[NG + 1] - Rt[NG]);
+          ax = FLAT ? ax - ((wc0 * c) * a.rdx) * a.uh[x] : ax - wc0 * (c * (a.rdx * a.uh[x]));
+        }
+      } else if (i == ni - 1 && pe) {
+        const float* Rt = a.rru + ix3(kf, y, 0, nyu, nxu);
+        const float* Rb = a.rru + ix3(kf - 1, y, 0, nyu, nxu);
+        const float m = c2 * Rt[NG + ni] + c1 * Rb[NG + ni];
+        if (m <= 0.0f) {
+          const float c = c1 * (Rb[x + 1] - Rb[x]) + c2 * (Rt[x + 1] - Rt[x]);
+          ax = FLAT ? ax - ((wc0 * c) * a.rdx) * a.uh[x] : ax - wc0 * (c * (a.rdx * a.uh[x]));
+        }
+      }
+      if (j == 0 && ps) {
+        const float m = c2 * a.rrv[ix3(kf, NG, x, nyv, nxs)] + c1 * a.rrv[ix3(kf - 1, NG, x, nyv, nxs)];
+        if (m >= 0.0f) {
+          const float c = c1 * (a.rrv[ix3(kf - 1, NG + 1, x, nyv, nxs)] - a.rrv[ix3(kf - 1, NG, x, nyv, nxs)]) +
+                          c2 * (a.rrv[ix3(kf, NG + 1, x, nyv, nxs)] - a.rrv[ix3(kf, NG, x, nyv, nxs)]);
+          ay = FLAT ? ay - ((wc0 * c) * a.rdy) * a.vh[y] : ay - wc0 * (c * (a.rdy * a.vh[y]));
+        }
+      } else if (j == nj - 1 && pn) {
+        const float m = c2 * a.rrv[ix3(kf, NG + nj, x, nyv, nxs)] + c1 * a.rrv[ix3(kf - 1, NG + nj, x, nyv, nxs)];
+        if (m <= 0.0f) {
+          const float c = c1 * (a.rrv[ix3(kf - 1, y + 1, x, nyv, nxs)] - a.rrv[ix3(kf - 1, y, x, nyv, nxs)]) +
+                          c2 * (a.rrv[ix3(kf, y + 1, x, nyv, nxs)] - a.rrv[ix3(kf, y, x, nyv, nxs)]);
+          ay = FLAT ? ay - ((wc0 * c) * a.rdy) * a.vh[y] : ay - wc0 * (c * (a.rdy * a.vh[y]));
+        }
+      }
+      float az = (-(zhi - zlo)) * a.rdz;
+      if (FLAT) az = az * a.flat[(nk + 2) + kf];
+      float tot = (ax + ay) + az;
+      tot = tot + wc0 * (c2 * cu.divx + c1 * dxlo);
+      const float sc = FLAT ? a.rrf0[(int64_t)(kf - 1) * plane] : a.rrf0[(int64_t)(kf - 1) * plane + h] * a.gz[h];
+      const int64_t p = (int64_t)kf * wst + wcol;
+      const float wt = cu.base + tot * sc;
+      if (FLAT && a.dum8) a.wten[p] = wt + (cu.c1 * d8lo + c2 * cu.d8);
+      else a.wten[p] = wt;
+      d8lo = cu.d8;
+    }
+    zlo = zhi;
+    rrw_hi = cu.rrwn;
+    dxlo = cu.divx;
+#pragma unroll
+    for (int q = 0; q < XS; ++q) rxp[q] = cu.rx[q];
+#pragma unroll
+    for (int q = 0; q < YS; ++q) ryp[q] = cu.ry[q];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) win[q] = win[q + 1];
+    win[6] = cu.wnext;
+    __syncthreads();
+  }
+}
+
+template <bool FLAT>
+void launch_advw_tile(const MomArgs& a, hipStream_t s) {
+  int kchunk = 1;
+  const int tiles = ((a.ni + MTX - 1) / MTX) * ((a.nj + MTY - 1) / MTY);
+  const int chunks = mom_chunks(tiles, a.nk - 1, &kchunk);
+  GM1_TIMED(TK_MOM_ADVW, s, hipLaunchKernelGGL(k_mom_advw_tile<FLAT>, dim3(tiles * chunks), dim3(MNT), 0, s, a,
+                                               kchunk, chunks));
+}
+
 // The held tendencies' words that no advection statement writes: u on the
 // row j = nj and on the faces i = 0, ni; v on the column i = ni and the
 // faces j = 0, nj; w on the column i = ni and the row j = nj of levels
@@ -903,7 +1158,7 @@ int momentum5_terrain(
   GM1_LAUNCH_CHECK();
   launch_advv_tile<false>(a, g0, nr, s);
   GM1_LAUNCH_CHECK();
-  GM1_TIMED(TK_MOM_ADVW, s, hipLaunchKernelGGL(k_mom_advw<false>, dim3((ni + 63) / 64, nj, nk - 1), blk, 0, s, a));
+  launch_advw_tile<false>(a, s);
   GM1_LAUNCH_CHECK();
   return 0;
 }
@@ -945,7 +1200,7 @@ int momentum5_flat(
   GM1_LAUNCH_CHECK();
   launch_advv_tile<true>(a, g0, nr, s);
   GM1_LAUNCH_CHECK();
-  GM1_TIMED(TK_MOM_ADVW, s, hipLaunchKernelGGL(k_mom_advw<true>, dim3((ni + 63) / 64, nj, nk - 1), blk, 0, s, a));
+  launch_advw_tile<true>(a, s);
   GM1_LAUNCH_CHECK();
   return 0;
 }
@@ -1001,7 +1256,7 @@ int gm1hip_momentum_transport5(
   GM1_LAUNCH_CHECK();
   launch_advv_tile<false>(a, g0, nr, s);
   GM1_LAUNCH_CHECK();
-  GM1_TIMED(TK_MOM_ADVW, s, hipLaunchKernelGGL(k_mom_advw<false>, dim3((ni + 63) / 64, nj, nk - 1), blk, 0, s, a));
+  launch_advw_tile<false>(a, s);
   GM1_LAUNCH_CHECK();
   return 0;
 }
