@@ -358,6 +358,7 @@ def adjoint_ratio(repeats: int, window: int, snapshots: int, fd_directions: int)
     fd = []
     side = torch.cuda.stream(stream)  # perturbed states and sums on the stream the steps run on
     side.__enter__()
+    gnorm = float(sum(torch.sum(g[n].double() ** 2) for n in ("th", "u", "v"))) ** 0.5
     for _ in range(fd_directions):
         d = {n: torch.from_numpy((smooth_direction(rng, tuple(S0[n].shape)) * float(S0[n].abs().max())).astype(
             np.float32)).to("cuda:0") for n in ("th", "u", "v")}
@@ -368,7 +369,11 @@ def adjoint_ratio(repeats: int, window: int, snapshots: int, fd_directions: int)
                                                           window)["w"].double() ** 2)
         num = float((Jp - Jm) / (2 * eps))
         ana = sum(float(torch.sum(g[n].double() * d[n].double())) for n in d)
-        fd.append({"fd": num, "adjoint": ana, "rel": abs(num - ana) / max(abs(num), 1e-300)})
+        dnorm = float(sum(torch.sum(d[n].double() ** 2) for n in d)) ** 0.5
+        # cosine: how far the direction is from orthogonal to the gradient (below ~1e-2 the
+        # directional derivative is a sum of cancelling terms that binary32 differences cannot resolve)
+        fd.append({"fd": num, "adjoint": ana, "rel": abs(num - ana) / max(abs(num), 1e-300),
+                   "cosine": ana / max(gnorm * dnorm, 1e-300)})
     side.__exit__(None, None, None)
     print(f"[bench] central differences: {[round(x['rel'], 6) for x in fd]}", file=sys.stderr, flush=True)
     # the acoustic stages with their coefficients held
