@@ -267,7 +267,18 @@ struct ScalarAdvArgs {
   // null; th0 (nk) turns it into the packed theta (th0 - 300) + theta'.
   const float* src;
   const float* th0;
+  // the caller states c1 and c2 hold one word per level (every column's the
+  // same): the z faces read the first column's
+  int c12col;
 };
+
+// Lateral-family word with its flag bits split off: bit 16 of a caller's obc
+// states that c1 and c2 are horizontally uniform.
+constexpr unsigned OBC_C12_COLUMN = 1u << 16;
+inline void set_obc(ScalarAdvArgs& a, unsigned obc) {
+  a.obc = obc & 0xFFFFu;
+  a.c12col = (obc & OBC_C12_COLUMN) ? 1 : 0;
+}
 
 // bc.F's halo cell c (-3..n+2) of a row of n cells as a physical index.
 GM1_DEV int family_cell(int c, int n, unsigned lo, unsigned hi) {
@@ -330,7 +341,8 @@ __global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
   // z-face flux at face kf from the window b, b[0] = S(kf-3).
   auto zf = [&](int kf, const float* b) {
     const int64_t wp = (int64_t)kf * st + wcol;
-    const float cc1 = a.c1[(int64_t)(kf - 1) * mst + mcol], cc2 = a.c2[(int64_t)(kf - 1) * mst + mcol];
+    const int64_t zq = (int64_t)(kf - 1) * mst + (a.c12col ? 0 : mcol);
+    const float cc1 = a.c1[zq], cc2 = a.c2[zq];
     auto S = [&](int level) { return b[level - (kf - 3)]; };
     if (MODE == MODE_PDEF) return zflux_pdef_g(S, a.rrw[wp], cc1, cc2, kf, nk, a.weps, a.pdefweno != 0);
     return zflux_g<WENO>(S, a.rrw[wp], cc1, cc2, kf, nk, a.weps);
@@ -794,7 +806,7 @@ int gm1hip_scalar_advection5(const float* scalar, const float* rru, const float*
   a.fx = flux_x; a.fy = flux_y; a.fz = flux_z;
   a.ni = ni; a.nj = nj; a.nk = nk;
   a.rdx = 1.0f / dx; a.rdy = 1.0f / dy; a.rdsf = 1.0f / dz;
-  a.weps = weps; a.obc = obc;
+  a.weps = weps; set_obc(a, obc);
   return launch_scalar_tile(a, use_weno, MODE_TENDENCY, (hipStream_t)stream);
 }
 
@@ -803,14 +815,14 @@ int gm1hip_scalar_rk_stage(const float* scalar, const float* rru, const float* r
                            const float* a0, const float* held, const float* divterm, float* out, int ni, int nj,
                            int nk, int use_weno, double weps, float dx, float dy, float dz, unsigned obc,
                            float dttmp, void* stream) {
-  if (obc == 0u) return (int)hipErrorInvalidValue;
+  if ((obc & 0xFFFFu) == 0u) return (int)hipErrorInvalidValue;
   ScalarAdvArgs a{};
   a.s = scalar; a.rru = rru; a.rrv = rrv; a.rrw = rrw; a.c1 = c1; a.c2 = c2;
   a.divx = divx; a.rr0 = rr0; a.gz = gz;
   a.a0 = a0; a.held = held; a.divterm = divterm; a.out = out; a.dttmp = dttmp;
   a.ni = ni; a.nj = nj; a.nk = nk;
   a.rdx = 1.0f / dx; a.rdy = 1.0f / dy; a.rdsf = 1.0f / dz;
-  a.weps = weps; a.obc = obc;
+  a.weps = weps; set_obc(a, obc);
   return launch_scalar_tile(a, use_weno, MODE_COMPOSE, (hipStream_t)stream);
 }
 
@@ -819,14 +831,14 @@ int gm1hip_scalar_rk_stage_flat(const float* scalar, const float* rru, const flo
                                 const float* divx, const float* a0, const float* held, const float* divterm, float* out,
                                 int ni, int nj, int nk, int use_weno, double weps, float dx, float dy, float rdz,
                                 unsigned obc, float dttmp, void* stream) {
-  if (obc == 0u || !flat_column) return (int)hipErrorInvalidValue;
+  if ((obc & 0xFFFFu) == 0u || !flat_column) return (int)hipErrorInvalidValue;
   ScalarAdvArgs a{};
   a.s = scalar; a.rru = rru; a.rrv = rrv; a.rrw = rrw; a.c1 = c1; a.c2 = c2;
   a.divx = divx; a.rr0 = rr0; a.gz = nullptr; a.mh = flat_column;
   a.a0 = a0; a.held = held; a.divterm = divterm; a.out = out; a.dttmp = dttmp;
   a.ni = ni; a.nj = nj; a.nk = nk;
   a.rdx = 1.0f / dx; a.rdy = 1.0f / dy; a.rdsf = rdz;
-  a.weps = weps; a.obc = obc;
+  a.weps = weps; set_obc(a, obc);
   return launch_scalar_tile(a, use_weno, MODE_COMPOSE, (hipStream_t)stream);
 }
 
@@ -835,7 +847,7 @@ static int pdef_stage(const float* scalar, const float* src, const float* rru, c
                       const float* rgz, const float* mh, const float* divx, const float* qa, const float* held, float* out,
                       float* advx, float* advy, float* advz, int ni, int nj, int nk, int pdefweno, double weps,
                       float dx, float dy, float rdzv, unsigned obc, float dttmp, void* stream) {
-  if (obc == 0u || nk < 4) return (int)hipErrorInvalidValue;
+  if ((obc & 0xFFFFu) == 0u || nk < 4) return (int)hipErrorInvalidValue;
   hipStream_t s = (hipStream_t)stream;
   ScalarAdvArgs a{};
   a.s = scalar; a.src = src; a.rru = rru; a.rrv = rrv; a.rrw = rrw; a.c1 = c1; a.c2 = c2;
@@ -843,7 +855,7 @@ static int pdef_stage(const float* scalar, const float* src, const float* rru, c
   a.advx = advx; a.advy = advy; a.advz = advz; a.pdefweno = pdefweno;
   a.ni = ni; a.nj = nj; a.nk = nk;
   a.rdx = 1.0f / dx; a.rdy = 1.0f / dy; a.rdsf = rdzv;
-  a.weps = weps; a.obc = obc;
+  a.weps = weps; set_obc(a, obc);
   int rc = launch_scalar_tile(a, 1, MODE_PDEF, s);
   if (rc) return rc;
   PdefArgs p{};
@@ -852,7 +864,7 @@ static int pdef_stage(const float* scalar, const float* src, const float* rru, c
   p.qa = qa; p.held = held; p.advx = advx; p.advy = advy; p.advz = advz; p.out = out;
   p.ni = ni; p.nj = nj; p.nk = nk;
   p.dt = dttmp; p.rdt = 1.0f / dttmp; p.rdx = 1.0f / dx; p.rdy = 1.0f / dy; p.rdz = rdzv;
-  p.obc = obc;
+  p.obc = obc & 0xFFFFu;
   const int tiles = ((ni + RX - 1) / RX) * ((nj + RY - 1) / RY);
   const int chunks = chunks_for(tiles, nk, &p.kchunk);
   p.nchunks = chunks;
@@ -893,14 +905,14 @@ int gm1hip_scalar_rk_stage_src(const float* src, const float* th0, const float* 
                                const float* held, const float* divterm, float* out, int ni, int nj, int nk,
                                int use_weno, double weps, float dx, float dy, float rdz, unsigned obc, float dttmp,
                                void* stream) {
-  if (obc == 0u || !src || (gz == nullptr) == (flat_column == nullptr)) return (int)hipErrorInvalidValue;
+  if ((obc & 0xFFFFu) == 0u || !src || (gz == nullptr) == (flat_column == nullptr)) return (int)hipErrorInvalidValue;
   ScalarAdvArgs a{};
   a.src = src; a.th0 = th0; a.rru = rru; a.rrv = rrv; a.rrw = rrw; a.c1 = c1; a.c2 = c2;
   a.divx = divx; a.rr0 = rr0; a.gz = gz; a.mh = flat_column;
   a.a0 = a0; a.held = held; a.divterm = divterm; a.out = out; a.dttmp = dttmp;
   a.ni = ni; a.nj = nj; a.nk = nk;
   a.rdx = 1.0f / dx; a.rdy = 1.0f / dy; a.rdsf = rdz;
-  a.weps = weps; a.obc = obc;
+  a.weps = weps; set_obc(a, obc);
   return launch_scalar_tile(a, use_weno, MODE_COMPOSE, (hipStream_t)stream);
 }
 

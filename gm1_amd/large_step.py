@@ -45,7 +45,8 @@ from .kessler import (
 )
 from .errors import refuse
 from .psolver3 import TerrainOpenSoundAblations, device_profiles, launch_cuda_psolver3_terrain_open_stage
-from .scalar_moist_rk import launch_scalar_stage_from_carrier, moisture_weps_f64, theta_weps_f64
+from .scalar_moist_rk import (launch_scalar_stage_from_carrier, moisture_weps_f64, theta_weps_f64,
+                              verify_horizontally_uniform)
 
 signature("gm1hip_stage_terms", P, P, P, P, P, P, P, I, P, P, P, P, P, P, I, I, I, U, P)
 signature("gm1hip_buoyancy", P, P, P, P, P, P, P, P, I, I, I, P)
@@ -149,6 +150,8 @@ class Gm1LargeStepState:
     column_pass: str = "pipelined"
     # gxu and gyv are +0 everywhere (flat terrain), measured once from the host metrics
     slopes_zero: bool = False
+    # c1 and c2 hold one word per level at every column (flat terrain): the scalar advection reads one column
+    c12_columns: bool = False
     receipts: list = field(default_factory=list)
     # a stretched (flat) column runs the terrain_flag=.false. family
     flat: Any = None
@@ -182,6 +185,7 @@ def create_gm1_large_step_state(host: dict[str, Any], device: torch.device | str
         from .psolver3 import verify_terrain_slopes_zero
 
         st.slopes_zero = verify_terrain_slopes_zero(st.metrics)
+    st.c12_columns = verify_horizontally_uniform(st.metrics)
     if host.get("levels") is not None:
         st.flat = flat_stretched_column(host["levels"], dev)
     # device copies of the vertical profiles are made here, never inside a step
@@ -291,7 +295,8 @@ def _scalar_stages(st: Gm1LargeStepState, nrk: int, mass: dict[str, Any], *, fla
     cfg, a, m, b = st.cfg, st.a, st.metrics, st.base
     q3d = a.get("q3d")
     grid = dict(ni=cfg.ni, nj=cfg.nj, nk=cfg.nk, nrk=nrk, dt=cfg.dt, dx=cfg.dx, dy=cfg.dy, rdz=rdz,
-                lateral_sides=cfg.lateral_sides, flat_column=flat_column, stream=st.stream)
+                lateral_sides=cfg.lateral_sides, flat_column=flat_column, stream=st.stream,
+                c12_columns=st.c12_columns)
     launch_scalar_stage_from_carrier(a["th3d"], b["th0"], mass, m, a["divx"], a["tha"], a["thten1"],
                                      a["theta_div"], a["scalar_next"], weps=THETA_WEPS, **grid)
     for sp in range(st.numq):

@@ -122,6 +122,7 @@ def launch_scalar_stage_from_carrier(
     held: Any, divterm: Any | None, scratch: Any, *, ni: int, nj: int, nk: int, nrk: int, dt: float, dx: float,
     dy: float, rdz: float, lateral_sides, flat_column: Any | None = None, pdef_species: int | None = None,
     pdef_scratch: Mapping[str, Any] | None = None, weps: float | None = None, stream: Any | None = None,
+    c12_columns: bool = False,
 ) -> None:
     """One scalar's RK stage read straight from its unpadded carrier (nk, nj, ni), result back in it.
 
@@ -133,7 +134,9 @@ def launch_scalar_stage_from_carrier(
     halo words of the carrier, so the carrier is updated by a copy after it.
     Terrain columns take ``metrics["gz"]``/``["rgz"]`` and ``rdz = 1/dz``; flat
     stretched columns ``flat_column`` and the metric rdz.  ``pdef_species``
-    selects the final-stage positive-definite moisture chain.
+    selects the final-stage positive-definite moisture chain.  ``c12_columns``
+    states that c1 and c2 hold one word per level at every column (checked by
+    ``verify_horizontally_uniform``); the kernels then read the first column's.
     """
     cell = (nk, nj, ni)
     sh = shapes(ni, nj, nk)
@@ -147,7 +150,7 @@ def launch_scalar_stage_from_carrier(
                      require(mass_flux["rrw"], "rrw", sh["w"]))
     terrain = flat_column is None
     gz = metrics["gz"] if terrain else None
-    obc = lateral_code(lateral_sides)
+    obc = lateral_code(lateral_sides) | (OBC_C12_COLUMN if c12_columns else 0)
     dttmp = scalar_moist_rk_stage_timestep_f32(dt, nrk)
     h = stream_handle(stream)
     if pdef_species is None:
@@ -170,6 +173,29 @@ def launch_scalar_stage_from_carrier(
                      moisture_weps_f64(pdef_species), float(dx), float(dy), float(rdz), obc, dttmp, h)
     with torch.cuda.stream(stream) if stream is not None else _NoStream():
         carrier.copy_(scratch)
+
+
+# bit 16 of the lateral-family word: c1 and c2 are horizontally uniform
+OBC_C12_COLUMN = 1 << 16
+_uniform_checked: set[tuple] = set()
+
+
+def verify_horizontally_uniform(metrics: Mapping[str, Any], names: tuple[str, ...] = ("c1", "c2")) -> bool:
+    """Check once on the device that every level of each named (levels, nj, ni) metric holds one word at every
+    column (the first column's, sign and NaN payloads included).  Remembered for these tensors until they are
+    written again, so the check runs outside any step."""
+    key = tuple((int(metrics[n].data_ptr()), int(metrics[n]._version)) for n in names)
+    if key in _uniform_checked:
+        return True
+    for n in names:
+        t = metrics[n]
+        if t.dim() != 3 or t.dtype != torch.float32:
+            return False
+        w = t.contiguous().view(torch.int32)
+        if not bool(torch.equal(w, w[:, :1, :1].expand_as(w))):
+            return False
+    _uniform_checked.add(key)
+    return True
 
 
 class _NoStream:

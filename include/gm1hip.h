@@ -13,7 +13,11 @@
 // Padded index i (0-based) holds CM1's Fortran column i-2 (ngxy = 3).
 // `obc`/`sides` packs CM1's lateral families (wbc, ebc, sbc, nbc) as
 // wbc | ebc<<4 | sbc<<8 | nbc<<12 with 1 periodic, 2 open-radiative,
-// 3/4 rigid; 0 (scalar advection only) is a tile-local launch.
+// 3/4 rigid; 0 (scalar advection only) is a tile-local launch.  The scalar
+// advection and RK stage entry points also read bit 16 of obc: set, the
+// caller states that c1 and c2 hold one word per level at every column (flat
+// terrain), and the vertical fluxes read the first column's words -- the same
+// words, with one read per level in place of one per cell.
 //
 // Arithmetic: IEEE binary32 with no contraction, IEEE denormals and
 // correctly rounded division and square root; WENO weights in binary64.
