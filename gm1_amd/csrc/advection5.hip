@@ -300,9 +300,12 @@ GM1_DEV float packed_word(const float* __restrict__ src, const float* __restrict
 }
 
 // FLAT: the flat stretched column's text (a.mh set), fixed at compile time so
-// the terrain instance carries none of its branches.
-template <bool WENO, int MODE, bool FLAT>
-__global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
+// the terrain instance carries none of its branches.  SRC: the scalar is read
+// from its unpadded carrier (a.src) rather than a padded array (a.s); flux
+// outputs exist only in the tendency mode -- each instance keeps only the
+// pointers it reads live in scalar registers.
+template <bool WENO, int MODE, bool FLAT, bool SRC>
+__device__ __forceinline__ void scalar_adv5_body(const ScalarAdvArgs& a) {
   constexpr bool GATE = (MODE == MODE_PDEF);
   __shared__ float lfx[TY][TX + 1];
   __shared__ float lfy[TY + 1][TX];
@@ -323,7 +326,7 @@ __global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
   const bool owned = live && (obc != 0u || (i >= 3 && i + 3 < ni && j >= 3 && j + 3 < nj));
   const int x = i + NG, y = j + NG;
   const int64_t st = (int64_t)nys * nxs;  // level stride of scalar and w arrays
-  const float* col = a.s + (int64_t)y * nxs + x;
+  const float* col = SRC ? nullptr : a.s + (int64_t)y * nxs + x;
   const int64_t wcol = (int64_t)y * nxs + x;
   const int64_t mcol = (int64_t)j * ni + i;
   const int64_t mst = (int64_t)nj * ni;
@@ -332,10 +335,20 @@ __global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
   // holds level k-3+q (0 outside the padded column; those words are never
   // read by a face's stencil, which stays within levels 1..nk).
   float win[7];
+  // the own column is a physical cell, so its unpadded word needs no lateral image
+  const float* scol = SRC ? a.src + (int64_t)j * ni + i : nullptr;
   auto colword = [&](int lev) {  // the column's word at a level, 0 where no stencil reads
-    if (a.src) return (lev >= 1 && lev <= nk) ? packed_word(a.src, a.th0, lev, y, x, ni, nj, obc) : 0.0f;
+    if constexpr (SRC) {
+      if (lev < 1 || lev > nk) return 0.0f;
+      const float v = scol[(int64_t)(lev - 1) * mst];
+      return a.th0 ? (a.th0[lev - 1] - 300.0f) + v : v;
+    }
     return (lev >= 0 && lev <= nk + 1) ? col[(int64_t)lev * st] : 0.0f;
   };
+  // a tile whose plane (3-cell halo included) lies inside the domain reads the
+  // unpadded source directly; only edge tiles form the lateral family's image
+  const bool inner = SRC && j0 >= 3 && j0 + TY + 3 <= nj && i0 >= 3 && i0 + TX + 3 <= ni;
+  const float* sbase = SRC ? a.src + (int64_t)(j0 - NG) * ni + (i0 - NG) : nullptr;
 #pragma unroll
   for (int q = 0; q < 7; ++q) win[q] = live ? colword(kb - 3 + q) : 0.0f;
   // z-face flux at face kf from the window b, b[0] = S(kf-3).
@@ -368,7 +381,9 @@ __global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
     v.snext = live ? colword(k + 4) : 0.0f;  // the level entering the window after level k
     // the level plane: rows j0-3..j0+TY+2, columns i0-3..i0+TX+2 (padded
     // indices j0..j0+TY+5, i0..i0+TX+5), clipped to the padded array
-    const float* plane = a.s + (int64_t)k * st;
+    const float* plane = SRC ? nullptr : a.s + (int64_t)k * st;
+    const float pack = (SRC && a.th0) ? a.th0[k - 1] - 300.0f : 0.0f;
+    const float* splane = inner ? sbase + (int64_t)(k - 1) * mst : nullptr;
 #pragma unroll
     for (int u = 0; u < PS; ++u) {
       int slot = threadIdx.x + u * NT;
@@ -376,8 +391,18 @@ __global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
       const int r = slot / (TX + 6), c = slot % (TX + 6);
       const int yy = j0 + r, xx = i0 + c;
       v.plane[u] = 0.0f;
-      if (slot < (TY + 6) * (TX + 6) && yy < nys && xx < nxs)
-        v.plane[u] = a.src ? packed_word(a.src, a.th0, k, yy, xx, ni, nj, obc) : plane[(int64_t)yy * nxs + xx];
+      if (slot < (TY + 6) * (TX + 6) && yy < nys && xx < nxs) {
+        if constexpr (SRC) {
+          if (inner) {
+            const float w = splane[r * ni + c];
+            v.plane[u] = a.th0 ? pack + w : w;
+          } else {
+            v.plane[u] = packed_word(a.src, a.th0, k, yy, xx, ni, nj, obc);
+          }
+        } else {
+          v.plane[u] = plane[(int64_t)yy * nxs + xx];
+        }
+      }
     }
 #pragma unroll
     for (int u = 0; u < XS; ++u) {
@@ -440,7 +465,7 @@ __global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
         const float m = cu.mu[u];
         float v = xflux_lds<WENO, GATE>(lp, m, r, t, a.weps);
         lfx[r][t] = v;
-        if (a.fx && (t < TX || f == ni)) a.fx[ix3(k, jj + NG, f + NG, nys, nxu)] = v;
+        if (MODE == MODE_TENDENCY && a.fx && (t < TX || f == ni)) a.fx[ix3(k, jj + NG, f + NG, nys, nxu)] = v;
       }
     }
     // y-face fluxes: rows j0..j0+TY of the TX columns.
@@ -453,13 +478,13 @@ __global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
         const float m = cu.mv[u];
         float v = yflux_lds<WENO, GATE>(lp, m, r, t, a.weps);
         lfy[r][t] = v;
-        if (a.fy && (r < TY || g == nj)) a.fy[ix3(k, g + NG, ii + NG, nyv, nxs)] = v;
+        if (MODE == MODE_TENDENCY && a.fy && (r < TY || g == nj)) a.fy[ix3(k, g + NG, ii + NG, nyv, nxs)] = v;
       }
     }
     float fzt = 0.0f;
     if (live && k + 1 <= nk) fzt = zf(k + 1, win + 1);  // S(k-2..)
     __syncthreads();
-    if (live && a.fz) {
+    if (MODE == MODE_TENDENCY && live && a.fz) {
       if (k == 1) a.fz[1 * st + wcol] = 0.0f;
       a.fz[(int64_t)(k + 1) * st + wcol] = (k + 1 <= nk) ? fzt : 0.0f;
       if (k == kb && kb >= 2) a.fz[(int64_t)kb * st + wcol] = fzb;
@@ -524,6 +549,19 @@ __global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
   }
 }
 
+
+template <bool WENO, int MODE, bool FLAT, bool SRC>
+__global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
+  scalar_adv5_body<WENO, MODE, FLAT, SRC>(a);
+}
+
+// The same body held to six waves per SIMD (three 512-lane workgroups per CU):
+// the 5th-order compose fits that register budget, and more waves in flight
+// hide more of each level's load latency.
+template <bool WENO, int MODE, bool FLAT, bool SRC>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_scalar_adv5_occ(ScalarAdvArgs a) {
+  scalar_adv5_body<WENO, MODE, FLAT, SRC>(a);
+}
 
 // ---------------------------------------------------------------------------
 // positive-definite repair of the final-stage moisture advection
@@ -738,24 +776,25 @@ int chunks_for(int tiles, int nk, int* kchunk) {
   return (nk + *kchunk - 1) / *kchunk;
 }
 
-template <bool FLAT>
+template <bool FLAT, bool SRC>
 void launch_scalar_variant(const ScalarAdvArgs& a, int use_weno, int mode, dim3 grid, hipStream_t s) {
   if (mode == MODE_TENDENCY) {
     if (use_weno)
       GM1_TIMED(TK_SCALAR_ADV5, s,
-                hipLaunchKernelGGL((k_scalar_adv5<true, MODE_TENDENCY, FLAT>), grid, dim3(NT), 0, s, a));
+                hipLaunchKernelGGL((k_scalar_adv5<true, MODE_TENDENCY, FLAT, SRC>), grid, dim3(NT), 0, s, a));
     else
       GM1_TIMED(TK_SCALAR_ADV5, s,
-                hipLaunchKernelGGL((k_scalar_adv5<false, MODE_TENDENCY, FLAT>), grid, dim3(NT), 0, s, a));
+                hipLaunchKernelGGL((k_scalar_adv5<false, MODE_TENDENCY, FLAT, SRC>), grid, dim3(NT), 0, s, a));
   } else if (mode == MODE_COMPOSE) {
     if (use_weno)
       GM1_TIMED(TK_SCALAR_ADV5, s,
-                hipLaunchKernelGGL((k_scalar_adv5<true, MODE_COMPOSE, FLAT>), grid, dim3(NT), 0, s, a));
+                hipLaunchKernelGGL((k_scalar_adv5<true, MODE_COMPOSE, FLAT, SRC>), grid, dim3(NT), 0, s, a));
     else
       GM1_TIMED(TK_SCALAR_ADV5, s,
-                hipLaunchKernelGGL((k_scalar_adv5<false, MODE_COMPOSE, FLAT>), grid, dim3(NT), 0, s, a));
+                hipLaunchKernelGGL((k_scalar_adv5_occ<false, MODE_COMPOSE, FLAT, SRC>), grid, dim3(NT), 0, s, a));
   } else {
-    GM1_TIMED(TK_SCALAR_ADV5, s, hipLaunchKernelGGL((k_scalar_adv5<true, MODE_PDEF, FLAT>), grid, dim3(NT), 0, s, a));
+    GM1_TIMED(TK_SCALAR_ADV5, s,
+              hipLaunchKernelGGL((k_scalar_adv5<true, MODE_PDEF, FLAT, SRC>), grid, dim3(NT), 0, s, a));
   }
 }
 
@@ -764,8 +803,13 @@ int launch_scalar_tile(ScalarAdvArgs& a, int use_weno, int mode, hipStream_t s) 
   const int chunks = chunks_for(tiles, a.nk, &a.kchunk);
   a.nchunks = chunks;
   dim3 grid(tiles * chunks);
-  if (a.mh) launch_scalar_variant<true>(a, use_weno, mode, grid, s);
-  else launch_scalar_variant<false>(a, use_weno, mode, grid, s);
+  if (a.mh) {
+    if (a.src) launch_scalar_variant<true, true>(a, use_weno, mode, grid, s);
+    else launch_scalar_variant<true, false>(a, use_weno, mode, grid, s);
+  } else {
+    if (a.src) launch_scalar_variant<false, true>(a, use_weno, mode, grid, s);
+    else launch_scalar_variant<false, false>(a, use_weno, mode, grid, s);
+  }
   GM1_LAUNCH_CHECK();
   return 0;
 }
