@@ -291,14 +291,14 @@ def run_ensemble(args) -> int:
     return 0
 
 
-def adjoint_ratio(repeats: int, window: int, snapshots: int, fd_directions: int):
+def adjoint_ratio(repeats: int, window: int, snapshots: int, fd_directions_n: int):
     """Adjoint/forward device time on the dry periodic 192x192x64 case.
 
     * large step: the production forward step (padded, eager) against the
       adjoint of one step (halo-free stages recomputed, then transposed);
     * window: storm-intensity value over ``window`` steps against its gradient
       by binomial checkpointing with ``snapshots`` free slots, and central
-      differences (eps = 1e-3 of each field's scale) along ``fd_directions``
+      differences (eps = 1e-3 of each field's scale) along ``fd_directions_n``
       smooth random directions of (theta', u, v);
     * acoustic stages nrk 1-3 with fixed coefficients against their adjoints.
     """
@@ -307,7 +307,7 @@ def adjoint_ratio(repeats: int, window: int, snapshots: int, fd_directions: int)
 
     from gm1_amd.adjoint_sound import STATE as AC_STATE
     from gm1_amd.adjoint_sound import check_flat_periodic, launch_cuda_psolver3_stage_adjoint
-    from gm1_amd.adjoint_step import DryPeriodicStep, WindowGradient, from_padded, smooth_direction
+    from gm1_amd.adjoint_step import DryPeriodicStep, WindowGradient, fd_directions, from_padded
     from gm1_amd.fields import adjoint_c5
     from gm1_amd.large_step import create_gm1_large_step_state, host_initial_state, launch_cuda_gm1_large_step
     from gm1_amd.psolver3 import launch_cuda_psolver3_terrain_open_stage
@@ -359,19 +359,16 @@ def adjoint_ratio(repeats: int, window: int, snapshots: int, fd_directions: int)
     side = torch.cuda.stream(stream)  # perturbed states and sums on the stream the steps run on
     side.__enter__()
     gnorm = float(sum(torch.sum(g[n].double() ** 2) for n in ("th", "u", "v"))) ** 0.5
-    for _ in range(fd_directions):
-        d = {n: torch.from_numpy((smooth_direction(rng, tuple(S0[n].shape)) * float(S0[n].abs().max())).astype(
-            np.float32)).to("cuda:0") for n in ("th", "u", "v")}
+    kept, replaced = fd_directions(rng, {n: S0[n].cpu().numpy() for n in ("th", "u", "v")}, g, "cuda:0",
+                                   count=fd_directions_n)
+    for d, ana in kept:
         eps = 1e-3
         Jp = torch.sum(win.weights.double() * win.advance({**S0, **{n: S0[n] + eps * d[n] for n in d}},
                                                           window)["w"].double() ** 2)
         Jm = torch.sum(win.weights.double() * win.advance({**S0, **{n: S0[n] - eps * d[n] for n in d}},
                                                           window)["w"].double() ** 2)
         num = float((Jp - Jm) / (2 * eps))
-        ana = sum(float(torch.sum(g[n].double() * d[n].double())) for n in d)
         dnorm = float(sum(torch.sum(d[n].double() ** 2) for n in d)) ** 0.5
-        # cosine: how far the direction is from orthogonal to the gradient (below ~1e-2 the
-        # directional derivative is a sum of cancelling terms that binary32 differences cannot resolve)
         fd.append({"fd": num, "adjoint": ana, "rel": abs(num - ana) / max(abs(num), 1e-300),
                    "cosine": ana / max(gnorm * dnorm, 1e-300)})
     side.__exit__(None, None, None)
@@ -409,7 +406,8 @@ def adjoint_ratio(repeats: int, window: int, snapshots: int, fd_directions: int)
             "window": {"steps": window, "snapshots": snapshots, "loss": float(J),
                        "value_s": win_fwd_s, "gradient_s": win_grad_s, "ratio": win_grad_s / win_fwd_s,
                        "forward_steps": grad_forward_steps, "adjoint_steps": grad_adjoint_steps,
-                       "central_differences": fd},
+                       "central_differences": fd,
+                       "replaced_directions": [{"draw": k + 1, "adjoint": ad} for k, ad in replaced]},
             "acoustic_stages": {"forward_ms": ac_fwd_ms, "adjoint_ms": ac_adj_ms, "ratio": ac_adj_ms / ac_fwd_ms}}
 
 

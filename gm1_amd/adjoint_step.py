@@ -24,7 +24,7 @@ configuration (flat terrain, horizontally uniform base state, CM1's
 from __future__ import annotations
 
 import ctypes
-from typing import Any
+from typing import Any, Mapping
 
 import numpy as np
 import torch
@@ -313,6 +313,33 @@ def smooth_direction(rng: np.random.Generator, shape: tuple, modes: int = 3) -> 
         out += rng.standard_normal() * (np.cos(2 * np.pi * kx * i / ni + ph[0]) * np.cos(2 * np.pi * ky * j / nj + ph[1])
                                         * np.sin(np.pi * kz * k / nk + ph[2]))
     return out / np.abs(out).max()
+
+
+def fd_directions(rng: np.random.Generator, fields: Mapping[str, Any], gradient: Mapping[str, Any], device: Any,
+                  count: int = 8, pool: int = 16) -> tuple[list, list]:
+    """Directions for a central-difference check of ``gradient``: ``count`` smooth random directions of
+    (theta', u, v), each field's scaled by its max-norm in ``fields``, with the gradient's directional derivative
+    g.d.  Of ``pool`` draws, one whose |g.d| is below 5 % of the pool's median is nearly orthogonal to the
+    gradient -- a derivative of cancelling terms that binary32 central differences cannot resolve -- and gives
+    way to the next draw.  Returns ([(d, g.d)] accepted, [(draw index, g.d)] replaced)."""
+    draws = []
+    for _ in range(pool):
+        d = {n: torch.from_numpy((smooth_direction(rng, tuple(fields[n].shape))
+                                  * float(np.abs(np.asarray(fields[n])).max())).astype(np.float32)).to(device)
+             for n in ("th", "u", "v")}
+        draws.append((d, sum(float(torch.sum(gradient[n].double() * d[n].double())) for n in d)))
+    floor = 0.05 * float(np.median([abs(ad) for _, ad in draws]))
+    kept, replaced = [], []
+    for k, (d, ad) in enumerate(draws):
+        if len(kept) == count:
+            break
+        if abs(ad) < floor:
+            replaced.append((k, ad))
+        else:
+            kept.append((d, ad))
+    if len(kept) < count:
+        raise ValueError("too many directions nearly orthogonal to the gradient")
+    return kept, replaced
 
 
 def storm_intensity_weights(cfg, metrics, zlo: float = 2000.0, zhi: float = 8000.0) -> np.ndarray:

@@ -212,28 +212,13 @@ def _window_case(device, nsteps, snapshots):
 
 
 def _fd_directions(rng, S32, g, device, count=8, pool=16):
-    """``count`` smooth random directions of (theta', u, v) scaled by each field's max, with the adjoint's
-    directional derivative g.d.  Of ``pool`` draws, one whose |g.d| is below 5 % of the pool's median is
-    nearly orthogonal to the gradient -- a derivative of cancelling terms that binary32 central differences
-    cannot resolve -- and gives way to the next draw; the rejections are printed."""
-    import torch
+    """``gm1_amd.adjoint_step.fd_directions``, the replaced draws printed."""
+    from gm1_amd.adjoint_step import fd_directions
 
-    from gm1_amd.adjoint_step import smooth_direction
-
-    draws = []
-    for _ in range(pool):
-        d = {n: torch.from_numpy((smooth_direction(rng, tuple(S32[n].shape)) * float(np.abs(S32[n]).max()))
-                                 .astype(np.float32)).to(device) for n in ("th", "u", "v")}
-        draws.append((d, sum(float(torch.sum(g[n].double() * d[n].double())) for n in d)))
-    floor = 0.05 * float(np.median([abs(ad) for _, ad in draws]))
-    out = []
-    for k, (d, ad) in enumerate(draws):
-        if abs(ad) < floor:
-            print(f"draw {k + 1}: g.d = {ad:.3e} below 5 % of the median {floor / 0.05:.3e}, replaced")
-            continue
-        out.append((d, ad))
-    assert len(out) >= count, "too many directions nearly orthogonal to the gradient"
-    return out[:count]
+    kept, replaced = fd_directions(rng, S32, g, device, count, pool)
+    for k, ad in replaced:
+        print(f"draw {k + 1}: g.d = {ad:.3e} nearly orthogonal to the gradient, replaced")
+    return kept
 
 
 @pytest.mark.gpu
