@@ -152,6 +152,9 @@ class Gm1LargeStepState:
     slopes_zero: bool = False
     # c1 and c2 hold one word per level at every column (flat terrain): the scalar advection reads one column
     c12_columns: bool = False
+    # the scalars' second carriers: stage 1 writes them, stage 2 reads them back into th3d/q3d,
+    # stage 3 writes them again and one copy returns the result (none between stages 1 and 2)
+    alt: dict = field(default_factory=dict)
     receipts: list = field(default_factory=list)
     # a stretched (flat) column runs the terrain_flag=.false. family
     flat: Any = None
@@ -193,7 +196,9 @@ def create_gm1_large_step_state(host: dict[str, Any], device: torch.device | str
     device_profiles(st.prof, dev)
     if st.flat is not None:
         device_profiles(st.flat.levels.profiles(), dev)
+    st.alt = {"th3d": torch.empty_like(st.a["th3d"])}
     if st.numq:
+        st.alt["q3d"] = torch.empty_like(st.a["q3d"])
         cfg = st.cfg
         st.selection = Gm1KesslerDriverSelection(equation_set=equation_set)
         st.kessler_workspace = create_cuda_gm1_kessler_driver_workspace(cfg.ni, cfg.nj, cfg.nk, dev)
@@ -219,18 +224,26 @@ def _kessler_arrays(st: Gm1LargeStepState) -> dict[str, torch.Tensor]:
             "qbudget": a["qbudget"]}
 
 
-def _rk_stage(st: Gm1LargeStepState, nrk: int) -> None:
+def _scalar_carriers(st: Gm1LargeStepState, nrk: int, alternate: bool) -> tuple[Any, Any]:
+    """Where this stage's theta' and moisture live: the second carriers during stage 2 when alternating."""
+    a = st.a
+    if alternate and nrk == 2:
+        return st.alt["th3d"], st.alt.get("q3d")
+    return a["th3d"], a.get("q3d")
+
+
+def _rk_stage(st: Gm1LargeStepState, nrk: int, alternate: bool = False) -> None:
     cfg, a, m, b = st.cfg, st.a, st.metrics, st.base
     ni, nj, nk = cfg.ni, cfg.nj, cfg.nk
     sides = cfg.lateral_sides
     code = lateral_code(sides)
     h = stream_handle(st.stream)
-    q3d = a.get("q3d")
-    _native.call("gm1hip_stage_terms", ptr(b["th0"]), ptr(b["qv0"]), ptr(b["pi0"]), None, None, ptr(a["th3d"]),
+    th3d, q3d = _scalar_carriers(st, nrk, alternate)
+    _native.call("gm1hip_stage_terms", ptr(b["th0"]), ptr(b["qv0"]), ptr(b["pi0"]), None, None, ptr(th3d),
                  ptr(q3d), st.numq, ptr(a["pp3d"]), ptr(a["thv"]), ptr(a["ppterm"]), ptr(a["piadv"]),
                  ptr(a["dum8"]), ptr(a["thterm"]), ni, nj, nk, code, h)
     if st.flat is not None:
-        _flat_rk_stage(st, nrk)
+        _flat_rk_stage(st, nrk, alternate)
         return
     if nrk >= 2:
         _native.call("gm1hip_bcwsfc", ptr(a["u3d"]), ptr(a["v3d"]), ptr(m["dzdx"]), ptr(m["dzdy"]), ptr(m["gz"]),
@@ -258,10 +271,11 @@ def _rk_stage(st: Gm1LargeStepState, nrk: int) -> None:
     # theta' and each moisture species advected straight from their carriers (the packed
     # (th0 - 300) + theta' and the lateral-family halos formed in the tiles' loads) and
     # composed A-anchored in one pass
-    _scalar_stages(st, nrk, mass, flat_column=None, rdz=float(np.float32(1.0) / np.float32(cfg.dz)))
+    _scalar_stages(st, nrk, mass, flat_column=None, rdz=float(np.float32(1.0) / np.float32(cfg.dz)),
+                   alternate=alternate)
 
 
-def _flat_rk_stage(st: Gm1LargeStepState, nrk: int) -> None:
+def _flat_rk_stage(st: Gm1LargeStepState, nrk: int, alternate: bool = False) -> None:
     """The rest of one RK stage on a flat stretched column (cuda_cartesian_dynamics.py:1070-1262).
 
     No bcwsfc/bc2d (terrain_flag statements), then the flat momentum,
@@ -286,28 +300,47 @@ def _flat_rk_stage(st: Gm1LargeStepState, nrk: int) -> None:
                                                   theta_divergence_term=a["theta_div"], divx=a["divx"],
                                                   stream=st.stream)
     mass = {"rru": a["rru"], "rrv": a["rrv"], "rrw": a["rrw"]}
-    _scalar_stages(st, nrk, mass, flat_column=st.flat.column, rdz=float(st.flat.rdz))
+    _scalar_stages(st, nrk, mass, flat_column=st.flat.column, rdz=float(st.flat.rdz), alternate=alternate)
 
 
 def _scalar_stages(st: Gm1LargeStepState, nrk: int, mass: dict[str, Any], *, flat_column: Any | None,
-                   rdz: float) -> None:
-    """Theta' then each moisture species (the final stage's rows through the pdef chain)."""
+                   rdz: float, alternate: bool = False) -> None:
+    """Theta' then each moisture species (the final stage's rows through the pdef chain).
+
+    ``alternate``: stage 1 writes the second carriers (st.alt), stage 2 reads them and writes th3d/q3d, stage 3
+    writes the second carriers and copies them back once -- the same words as a copy-back after every scalar.
+    """
     cfg, a, m, b = st.cfg, st.a, st.metrics, st.base
-    q3d = a.get("q3d")
     grid = dict(ni=cfg.ni, nj=cfg.nj, nk=cfg.nk, nrk=nrk, dt=cfg.dt, dx=cfg.dx, dy=cfg.dy, rdz=rdz,
                 lateral_sides=cfg.lateral_sides, flat_column=flat_column, stream=st.stream,
                 c12_columns=st.c12_columns)
-    launch_scalar_stage_from_carrier(a["th3d"], b["th0"], mass, m, a["divx"], a["tha"], a["thten1"],
-                                     a["theta_div"], a["scalar_next"], weps=THETA_WEPS, **grid)
-    for sp in range(st.numq):
-        if nrk < 3:
-            launch_scalar_stage_from_carrier(q3d[sp], None, mass, m, a["divx"], a["qa"][sp], a["qten"][sp], None,
-                                             a["scalar_next"], weps=moisture_weps_f64(sp), **grid)
+    if alternate:
+        src_th, src_q = _scalar_carriers(st, nrk, True)
+        if nrk == 2:
+            out_th, out_q = a["th3d"], a.get("q3d")
         else:
-            launch_scalar_stage_from_carrier(q3d[sp], None, mass, m, a["divx"], a["qa"][sp], a["qten"][sp], None,
-                                             a["scalar_next"], pdef_species=sp,
+            out_th, out_q = st.alt["th3d"], st.alt.get("q3d")
+        grid["copy_back"] = False
+    else:
+        src_th, src_q = a["th3d"], a.get("q3d")
+        out_th, out_q = a["scalar_next"], None
+    launch_scalar_stage_from_carrier(src_th, b["th0"], mass, m, a["divx"], a["tha"], a["thten1"],
+                                     a["theta_div"], out_th, weps=THETA_WEPS, **grid)
+    for sp in range(st.numq):
+        scratch = out_q[sp] if out_q is not None else a["scalar_next"]
+        if nrk < 3:
+            launch_scalar_stage_from_carrier(src_q[sp], None, mass, m, a["divx"], a["qa"][sp], a["qten"][sp], None,
+                                             scratch, weps=moisture_weps_f64(sp), **grid)
+        else:
+            launch_scalar_stage_from_carrier(src_q[sp], None, mass, m, a["divx"], a["qa"][sp], a["qten"][sp], None,
+                                             scratch, pdef_species=sp,
                                              pdef_scratch={"advx": a["advx"], "advy": a["advy"],
                                                            "advz": a["advz"]}, **grid)
+    if alternate and nrk == 3:
+        with torch.cuda.stream(st.stream) if st.stream is not None else _nullcontext():
+            a["th3d"].copy_(st.alt["th3d"])
+            if st.numq:
+                a["q3d"].copy_(st.alt["q3d"])
 
 
 @dataclass(frozen=True)
@@ -374,8 +407,10 @@ def launch_cuda_gm1_large_step(
                      ptr(a["tha"]), ptr(a["qa"]), ptr(a["thten1"]), ptr(a["qten"]), ptr(a["ppten"]),
                      ptr(a["qpten"]), ptr(a["qtten"]), ptr(a["qvten"]), ptr(a["qcten"]), ni, nj, nk, cfg.dt, h)
     stage_hooks = []
+    # with no hook between stages, the scalars alternate two carriers (one copy-back per step)
+    alternate = post_stage_hook is None
     for nrk in (1, 2, 3):
-        _rk_stage(st, nrk)
+        _rk_stage(st, nrk, alternate)
         if post_stage_hook is not None:
             stage_hooks.append(post_stage_hook(st, nrk))
     if st.numq:

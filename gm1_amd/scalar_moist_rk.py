@@ -122,7 +122,7 @@ def launch_scalar_stage_from_carrier(
     held: Any, divterm: Any | None, scratch: Any, *, ni: int, nj: int, nk: int, nrk: int, dt: float, dx: float,
     dy: float, rdz: float, lateral_sides, flat_column: Any | None = None, pdef_species: int | None = None,
     pdef_scratch: Mapping[str, Any] | None = None, weps: float | None = None, stream: Any | None = None,
-    c12_columns: bool = False,
+    c12_columns: bool = False, copy_back: bool = True,
 ) -> None:
     """One scalar's RK stage read straight from its unpadded carrier (nk, nj, ni), result back in it.
 
@@ -137,6 +137,8 @@ def launch_scalar_stage_from_carrier(
     selects the final-stage positive-definite moisture chain.  ``c12_columns``
     states that c1 and c2 hold one word per level at every column (checked by
     ``verify_horizontally_uniform``); the kernels then read the first column's.
+    ``copy_back=False`` leaves the result in ``scratch`` (a caller alternating
+    two carriers across stages).
     """
     cell = (nk, nj, ni)
     sh = shapes(ni, nj, nk)
@@ -171,8 +173,9 @@ def launch_scalar_stage_from_carrier(
                      ptr(rgz), ptr(flat_column), ptr(divx), ptr(a0), ptr(held), ptr(scratch), ptr(ps["advx"]),
                      ptr(ps["advy"]), ptr(ps["advz"]), ni, nj, nk, int(pdef_species != 0),
                      moisture_weps_f64(pdef_species), float(dx), float(dy), float(rdz), obc, dttmp, h)
-    with torch.cuda.stream(stream) if stream is not None else _NoStream():
-        carrier.copy_(scratch)
+    if copy_back:
+        with torch.cuda.stream(stream) if stream is not None else _NoStream():
+            carrier.copy_(scratch)
 
 
 # bit 16 of the lateral-family word: c1 and c2 are horizontally uniform
