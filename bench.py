@@ -58,6 +58,8 @@ KERNEL_WORDS = {
     ("k_scalar_adv5", True): 7,
 }
 KERNELS = sorted({k for k, _ in KERNEL_WORDS})
+# the fused column passes also read uten, vten, ppd and write u, v (no pressure-gradient launch)
+FUSED_COLUMN_EXTRA_WORDS = 5
 
 
 def step_words(numq: int, small_steps: int) -> int:
@@ -103,9 +105,13 @@ def parse_args(argv=None):
     p.add_argument("--graph", action="store_true",
                    help="time replays of the step captured into a HIP graph (the kernel timer then runs in an "
                         "eager pass of the same length)")
-    p.add_argument("--column-pass", choices=("staged", "pipelined", "staged16", "pipelined16", "register"),
-                   default="pipelined",
-                   help="the acoustic column pass (all give the same words)")
+    p.add_argument("--column-pass",
+                   choices=("staged", "pipelined", "staged16", "pipelined16", "register", "fused", "fused16",
+                            "resident", "resident16"),
+                   default=None,
+                   help="the acoustic column pass (all give the same words; default: the state's own, resident on "
+                        "a flat column and pipelined otherwise; the fused and resident forms run on the flat "
+                        "column only)")
     p.add_argument("--calibration-copy-mib", type=int, default=0,
                    help="issue one plain device copy of this size before the steps (HBM counter calibration)")
     return p.parse_args(argv)
@@ -442,7 +448,8 @@ def main(argv=None) -> int:
     host = host_initial_state(cfg)
     stream = torch.cuda.Stream()
     st = create_gm1_large_step_state(host, "cuda:0", stream=stream)
-    st.column_pass = args.column_pass
+    if args.column_pass is not None:
+        st.column_pass = args.column_pass
     del host
     torch.cuda.synchronize()
     print(f"[bench] state resident: {cfg.name}, {cfg.cells} cells", file=sys.stderr, flush=True)
@@ -517,6 +524,8 @@ def main(argv=None) -> int:
     value = cfg.cells * args.steps / (elapsed_ms * 1e-3)
     avg_launch_s = kernel_ms * 1e-3 / max(launches, 1)
     words = KERNEL_WORDS[(args.kernel, flat)]
+    if args.kernel == "k_snd_column" and st.column_pass.startswith("fused"):
+        words += FUSED_COLUMN_EXTRA_WORDS
     algo_bytes = words * WORD * cfg.cells
     achieved = algo_bytes / avg_launch_s / 1e9
     step_w = step_words(st.numq, small_steps)
@@ -546,7 +555,7 @@ def main(argv=None) -> int:
                         f"{cfg.nsound} substeps ({small_steps} small steps/step)",
             "cells": cfg.cells,
             "launch": mode,
-            "column_pass": args.column_pass,
+            "column_pass": st.column_pass,
         },
         "roofline": {
             "bound": "hbm",

@@ -2,11 +2,12 @@
 
 The column pass has word-identical variants (``psolver3.COLUMN_PASSES``,
 ``psolver3.FLAT_COLUMN_PASSES``): 32- or 16-column workgroups (the tile
-width across i), LDS-staged or pipelined (loads issued a level ahead).  Which is fastest depends on the
-column count and height, so ``tune_column_pass`` runs each candidate's
-acoustic RK stage on a scratch copy of a large-step state, times the
-column launches alone with the library's event-pair timer, and returns the
-fastest.  Results are kept per (ni, nj, nk, sides, stretched) for the
+width across i), LDS-staged or pipelined (loads issued a level ahead), and on
+the flat column the fused forms that step u and v inside the pipelined pass.
+Which is fastest depends on the column count and height, so
+``tune_column_pass`` runs each candidate's acoustic RK stage on a scratch
+copy of a large-step state, times its small steps (pressure gradient and
+column pass) with the library's event-pair timer, and returns the fastest.  Results are kept per (ni, nj, nk, sides, stretched) for the
 process, and in a JSON file when ``GM1_AUTOTUNE_CACHE`` names one.
 
 Choosing a variant never changes a word of the result, so a tuned run
@@ -83,7 +84,7 @@ def _stage(st, a, nrk: int, column_pass: str) -> None:
     kw = dict(ni=cfg.ni, nj=cfg.nj, nk=cfg.nk, nrk=nrk, dt=cfg.dt, nsound=cfg.nsound, dx=cfg.dx, dy=cfg.dy,
               stream=st.stream, lateral_sides=cfg.lateral_sides, column_pass=column_pass)
     if st.flat is not None:
-        launch_cuda_gm1_flat_acoustic_stage(a, st.metrics, vertical=st.flat, **kw)
+        launch_cuda_gm1_flat_acoustic_stage(a, st.metrics, vertical=st.flat, alternates=st.alt, **kw)
     else:
         launch_cuda_psolver3_terrain_open_stage(a, st.metrics, dz=cfg.dz, profiles=st.prof,
                                                 ablations=TerrainOpenSoundAblations(),
@@ -91,7 +92,12 @@ def _stage(st, a, nrk: int, column_pass: str) -> None:
 
 
 def tune_column_pass(st, repeats: int = 2, candidates=None) -> dict[str, Any]:
-    """Fastest column pass for ``st``'s grid: {"choice", "us": {variant: mean column launch in us}}."""
+    """Fastest column pass for ``st``'s grid: {"choice", "us": {variant: mean small step in us}}.
+
+    A small step is timed from its pressure-gradient launch to the end of its
+    column pass (one launch for the fused forms), so every form is charged the
+    same work.
+    """
     if not _memo:
         _load_cache()
     key = _key(st)
@@ -110,7 +116,8 @@ def tune_column_pass(st, repeats: int = 2, candidates=None) -> dict[str, Any]:
     for c in candidates:
         _stage(st, scratch, 3, c)  # first launch of a variant: code object load, caches
         torch.cuda.synchronize()
-        timing.arm("k_snd_column", 64 * repeats + 8)
+        # a small step's pressure-gradient and column work, one pass in the fused forms and two in the others
+        timing.arm("k_snd_small_step", 64 * repeats + 8)
         for _ in range(repeats):
             _stage(st, scratch, 3, c)
         torch.cuda.synchronize()

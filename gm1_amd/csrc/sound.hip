@@ -41,6 +41,12 @@ struct SoundArgs {
   // null on the terrain-coordinate family.
   const float* flat;
   float rdzf;
+  // Fused flat pass (pressure gradient inside the column pass): the small step
+  // reads u3d, v3d, ppd and writes u_out, v_out, ppd_out.  The prepare pass
+  // writes ppx into ppd_twin as well, and the start winds into u_start, v_start
+  // when the stage starts from the alternate buffers.  Null otherwise.
+  float* u_out; float* v_out; float* ppd_out;
+  float* ppd_twin; float* u_start; float* v_start;
 };
 
 GM1_DEV const float* P_sigma(const SoundArgs& a) { return a.prof; }
@@ -96,7 +102,9 @@ __global__ __launch_bounds__(256) void k_snd_prepare(SoundArgs a) {
   const int ni = a.ni, nj = a.nj, nk = a.nk;
   if (i >= ni + 2) return;
   const int64_t p = pix(k, j, i, ni, nj);
-  a.ppd[p] = a.ppx[p];
+  const float px = a.ppx[p];
+  a.ppd[p] = px;
+  if (a.ppd_twin) a.ppd_twin[p] = px;
   if (a.nrk != 1) a.pp3d[p] = a.ppi[p];
   const int y = j + NG - 1, x = i + NG - 1;
   const int64_t pu = ix3(k, y, x, sy(nj), ux(ni));
@@ -105,9 +113,18 @@ __global__ __launch_bounds__(256) void k_snd_prepare(SoundArgs a) {
   a.rru[pu] = 0.0f;
   a.rrv[pv] = 0.0f;
   a.rrw[pw] = a.wa[pw] * a.beta;
+  if (a.u_start) {
+    // the small steps start from the alternate winds: they take the start
+    // values on every face a small step writes (i, j >= 1 here); u3d and v3d
+    // take ua, va on the row and column outside those faces as below
+    a.u_start[pu] = (a.nrk != 1) ? a.ua[pu] : a.u3d[pu];
+    a.v_start[pv] = (a.nrk != 1) ? a.va[pv] : a.v3d[pv];
+  }
   if (a.nrk != 1) {
-    a.u3d[pu] = a.ua[pu];
-    a.v3d[pv] = a.va[pv];
+    if (!a.u_start || i == 0 || j == 0) {
+      a.u3d[pu] = a.ua[pu];
+      a.v3d[pv] = a.va[pv];
+    }
     a.w3d[pw] = a.wa[pw];
   }
   // the flat column forms qk, pk1, pk2 and mm inside its column pass (flat_terms) from
@@ -661,19 +678,25 @@ __global__ __launch_bounds__(64) void k_snd_column(SoundArgs a, int n) {
 constexpr int C2PLANES = 6;
 
 // the pipelined pass keeps wten in a seventh plane
-int column2_lds_bytes(int nk, int cw, bool pipe = false) {
-  return (C2PLANES + (pipe ? 1 : 0)) * (nk + 1) * cw * (int)sizeof(float);
+// (and the flat pipelined pass stages the level factors, FLAT_COEF words a level)
+int column2_lds_bytes(int nk, int cw, bool pipe = false, bool flat_pipe = false) {
+  return (C2PLANES + (pipe ? 1 : 0)) * (nk + 1) * cw * (int)sizeof(float) +
+         (flat_pipe ? (nk + 1) * FLAT_COEF * (int)sizeof(float) : 0);
+}
+
+// The word at a 32-bit byte offset from an array base: the load takes the
+// base in scalar registers and the offset in one vector register.
+GM1_DEV float ld32(const float* base, uint32_t byte_offset) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + byte_offset);
 }
 
 // rru(k) += u(k) and rrv(k) += v(k) on one column's west u face and south v
 // face for levels kb..ke (divided by nloop on the last small step); the words
 // of four levels are requested before any is written.
-GM1_DEV void average_uv_faces(const SoundArgs& a, int kb, int ke, int64_t ust, int64_t uc, int64_t vst, int64_t vc,
-                              bool last, float tavg) {
+GM1_DEV void average_uv_faces(const SoundArgs& a, const float* __restrict__ u3, const float* __restrict__ v3, int kb,
+                              int ke, int64_t ust, int64_t uc, int64_t vst, int64_t vc, bool last, float tavg) {
   float* __restrict__ rru = a.rru;
   float* __restrict__ rrv = a.rrv;
-  const float* __restrict__ u3 = a.u3d;
-  const float* __restrict__ v3 = a.v3d;
   constexpr int B = 4;
   for (int k0 = kb; k0 <= ke; k0 += B) {
     float ru[B], uu[B], rv[B], vv[B];
@@ -698,8 +721,61 @@ GM1_DEV void average_uv_faces(const SoundArgs& a, int kb, int ke, int64_t ust, i
   }
 }
 
-template <int C2W, int C2T, bool FLAT, bool PIPE>
+// sound.F's tridiagonal solve of one column for w(2..nk) over LDS planes of
+// C2W columns: forward elimination into ec (over mm) and ed (over dd), then
+// back substitution into dd.  The next level's LDS words are fetched while
+// the current level's division runs; s1/s2 of the level below are carried.
+template <int C2W>
+GM1_DEV void thomas_column(const float* Ls1, const float* Ls2, float* Lmm, float* Ldd, int c, int nk, float asq) {
+  auto at = [&](int k) { return k * C2W + c; };
+  float ec_prev = 0.0f, ed_prev = 0.0f, wtop = 0.0f;
+  float s1m = Ls1[at(1)], s2m = Ls2[at(1)];
+  float mmn = Lmm[at(2)], s1n = Ls1[at(2)], s2n = Ls2[at(2)], ddn = Ldd[at(2)];
+  for (int k = 2; k <= nk; ++k) {
+    const float mmk = mmn, s1k = s1n, s2k = s2n, dd = ddn;
+    if (k < nk) {
+      mmn = Lmm[at(k + 1)];
+      s1n = Ls1[at(k + 1)];
+      s2n = Ls2[at(k + 1)];
+      ddn = Ldd[at(k + 1)];
+    }
+    const float aa = (k >= 3) ? (-asq) * (mmk * s1m) : 0.0f;
+    const float cc = (k < nk) ? (-asq) * ((-mmk) * s2k) : 0.0f;
+    const float diag = (-s1k) + s2m;
+    const float bb = 1.0f - asq * (mmk * diag);
+    const float pivot = (k == 2) ? bb : aa * ec_prev + bb;
+    const float inv = 1.0f / pivot;
+    if (k < nk) {
+      const float eck = (-cc) * inv;
+      const float edk = ((k == 2) ? dd : dd - aa * ed_prev) * inv;
+      Lmm[at(k)] = eck;
+      Ldd[at(k)] = edk;
+      ec_prev = eck;
+      ed_prev = edk;
+    } else {
+      wtop = (dd - aa * ed_prev) * inv;
+    }
+    s1m = s1k;
+    s2m = s2k;
+  }
+  float wup = wtop;
+  Ldd[at(nk)] = wtop;
+  float ecn = Lmm[at(nk - 1)], edn = Ldd[at(nk - 1)];
+  for (int k = nk - 1; k >= 2; --k) {
+    const float eck = ecn, edk = edn;
+    if (k > 2) {
+      ecn = Lmm[at(k - 1)];
+      edn = Ldd[at(k - 1)];
+    }
+    const float wk = eck * wup + edk;
+    Ldd[at(k)] = wk;
+    wup = wk;
+  }
+}
+
+template <int C2W, int C2T, bool FLAT, bool PIPE, bool FUSE>
 __device__ __forceinline__ void column2_body(const SoundArgs& a, int n, int tile) {
+  static_assert(!FUSE || (FLAT && PIPE), "the pressure gradient is fused into the pipelined flat pass only");
   constexpr int C2S = C2T / C2W;
   extern __shared__ float lds[];
   __shared__ float lw1[C2W];
@@ -728,6 +804,21 @@ __device__ __forceinline__ void column2_body(const SoundArgs& a, int n, int tile
   float* Lmm = lds + 4 * L;
   float* Ldd = lds + 5 * L;
   float* Lwt = lds + 6 * L;  // wten (pipelined pass only)
+  float* Lcf = lds + 7 * L;  // the flat level factors, FLAT_COEF words a level (flat pipelined pass only)
+  if constexpr (FLAT && PIPE) {
+    const float4* src = reinterpret_cast<const float4*>(a.qk);
+    float4* dst = reinterpret_cast<float4*>(Lcf);
+    for (int t = (int)threadIdx.x; t < (nk + 1) * (FLAT_COEF / 4); t += C2T) dst[t] = src[t];
+    __syncthreads();
+  }
+  auto lcoef = [&](int k) {  // the flat level factors from LDS
+    const float4* lcf = reinterpret_cast<const float4*>(Lcf);
+    const float4 x = lcf[2 * k], y = lcf[2 * k + 1];
+    FlatCoef f;
+    f.tem = x.x; f.r1 = x.y; f.r2 = x.z; f.tem1 = x.w;
+    f.c1a = y.x; f.c1b = y.y; f.c2a = y.z; f.c2b = y.w;
+    return f;
+  };
   auto at = [&](int k) { return k * C2W + c; };
   auto coef = [&](int k) {  // the flat column's level factors, from the prepare pass
     const float4* d = reinterpret_cast<const float4*>(a.qk + (int64_t)k * FLAT_COEF);
@@ -744,47 +835,200 @@ __device__ __forceinline__ void column2_body(const SoundArgs& a, int n, int tile
 
   // ---------------- phase A (flat column, sound.F:491-509): fpk from the
   // horizontal divergence, the flat pressure advection and w as it is
-  if (FLAT && phys && kb <= ke) {
+  if constexpr (FUSE) {
+    // sound.F:318-342 inside the pass: every thread steps its own u face
+    // (i, j) and v face (i, j), i = 0..ni, j = 0..nj (the faces the pressure
+    // gradient pass writes, with its weights), from u3d, v3d and ppd into
+    // u_out and v_out.  A column's lane takes its east u face from lane c+1
+    // at the same level (the tile's last lane and the lane west of the east
+    // column form it themselves) and forms its north v face itself; every
+    // word these read is left unwritten by the pass.
+    if (inside && kb <= ke) {
+      const int pj = j + 1, pi = i + 1, np = ni + 2, fi = i + 1, fj = j + 1;
+      const float tem1 = (a.rdx * a.cp) * 0.5f, tem2 = (a.rdy * a.cp) * 0.5f;
+      const bool pw = side_of(a.sides, 0) == 1u, pe = side_of(a.sides, 1) == 1u;
+      const bool ps = side_of(a.sides, 2) == 1u, pn = side_of(a.sides, 3) == 1u;
+      const float dtwu = a.dts * ((fj <= nj && (fi > 1 || pw) && (fi < ni + 1 || pe)) ? 1.0f : 0.0f);
+      const float dtwv = a.dts * ((fi <= ni && (fj > 1 || ps) && (fj < nj + 1 || pn)) ? 1.0f : 0.0f);
+      const float* __restrict__ ppd = a.ppd;
+      const float* __restrict__ thv = a.thv;
+      if (phys) {
+        const float dtwue = a.dts * ((fj <= nj && (fi + 1 > 1 || pw) && (fi + 1 < ni + 1 || pe)) ? 1.0f : 0.0f);
+        const float dtwvn = a.dts * ((fi <= ni && (fj + 1 > 1 || ps) && (fj + 1 < nj + 1 || pn)) ? 1.0f : 0.0f);
+        const bool efb = (c == C2W - 1) || (i + 1 == ni);
+        const float temx = (a.dts * 0.5f) * a.rdx, temy = (a.dts * 0.5f) * a.rdy;
+        if (kb == 1) lw1[c] = w[wst + wc];
+        struct In {
+          float uo, ut, vo, vt, von, vtn, dc, dw, ds, dn, thw, ths, thn, uoe, ute, de, the;
+          float pc, pe, pw, pn, ps, wup, wt, ppten, ppterm, pp, th;
+        };
+
+        // 32-bit byte offsets (every array of the pass is under 4 GiB): one
+        // offset register serves every array of a family
+        const uint32_t onp = (uint32_t)np * 4u, onx = (uint32_t)nxs * 4u, ows = (uint32_t)wst * 4u;
+        auto load = [&](int k) {
+          const uint32_t op = (uint32_t)pix(k, pj, pi, ni, nj) * 4u;
+          const uint32_t ou = (uint32_t)((int64_t)k * ust + uc) * 4u;
+          const uint32_t ov = (uint32_t)((int64_t)k * vst + vc) * 4u;
+          const uint32_t ow = (uint32_t)((int64_t)k * wst + wc) * 4u;
+          In v;
+          v.uo = ld32(a.u3d, ou);
+          v.ut = ld32(a.uten, ou);
+          v.vo = ld32(a.v3d, ov);
+          v.vt = ld32(a.vten, ov);
+          v.von = ld32(a.v3d, ov + onx);
+          v.vtn = ld32(a.vten, ov + onx);
+          v.dc = ld32(ppd, op);
+          v.dw = ld32(ppd, op - 4u);
+          v.ds = ld32(ppd, op - onp);
+          v.dn = ld32(ppd, op + onp);
+          v.thw = ld32(thv, op - 4u);
+          v.ths = ld32(thv, op - onp);
+          v.thn = ld32(thv, op + onp);
+          v.uoe = 0.0f; v.ute = 0.0f; v.de = 0.0f; v.the = 0.0f;
+          if (efb) {
+            v.uoe = ld32(a.u3d, ou + 4u);
+            v.ute = ld32(a.uten, ou + 4u);
+            v.de = ld32(ppd, op + 4u);
+            v.the = ld32(thv, op + 4u);
+          }
+          v.pc = ld32(a.piadv, op);
+          v.pe = ld32(a.piadv, op + 4u);
+          v.pw = ld32(a.piadv, op - 4u);
+          v.pn = ld32(a.piadv, op + onp);
+          v.ps = ld32(a.piadv, op - onp);
+          v.wup = ld32(w, ow + ows);
+          v.wt = ld32(a.wten, ow);
+          v.ppten = ld32(a.ppten, op);
+          v.ppterm = ld32(a.ppterm, op);
+          v.pp = ld32(a.pp3d, op);
+          v.th = ld32(thv, op);
+          return v;
+        };
+        float wraw = w[(int64_t)kb * wst + wc];
+        const int64_t pb = pix(kb - 1, pj, pi, ni, nj);
+        float pd = a.piadv[pb], thm = thv[pb];
+        In nx = load(kb);
+        for (int k = kb; k <= ke; ++k) {
+          const In cu = nx;
+          nx = load(k + 1);  // level nk+1 exists in every array read here
+          float fu = tem1 * (cu.dc - cu.dw);
+          fu = fu * (cu.th + cu.thw);
+          const float uw = cu.uo + dtwu * (cu.ut - fu);
+          float fv = tem2 * (cu.dc - cu.ds);
+          fv = fv * (cu.th + cu.ths);
+          const float vs = cu.vo + dtwv * (cu.vt - fv);
+          float fn = tem2 * (cu.dn - cu.dc);
+          fn = fn * (cu.thn + cu.th);
+          const float vn = cu.von + dtwvn * (cu.vtn - fn);
+          float fe = tem1 * (cu.de - cu.dc);
+          fe = fe * (cu.the + cu.th);
+          const float uef = cu.uoe + dtwue * (cu.ute - fe);
+          const float ush = __shfl_down(uw, 1, C2W);
+          const float ue = efb ? uef : ush;
+          a.u_out[(int64_t)k * ust + uc] = uw;
+          a.v_out[(int64_t)k * vst + vc] = vs;
+          float div = (ue - uw) * a.rdx + (vn - vs) * a.rdy;
+          if (fabsf(div) < smeps) div = 0.0f;
+          float hx = uw * (cu.pc - cu.pw) + ue * (cu.pe - cu.pc);
+          hx = (-hx) * temx;
+          float hy = vs * (cu.pc - cu.ps) + vn * (cu.pn - cu.pc);
+          hy = (-hy) * temy;
+          const float hp = hx + hy;
+          const FlatTerms t = flat_terms(lcoef(k), k, nk, cu.ppterm, pd, cu.pc, nx.pc, cu.th, thm);
+          pd = cu.pc;
+          thm = cu.th;
+          const float s1 = t.pk1 + t.qk, s2 = t.pk2 - t.qk;
+          const float vert = wraw * s1 + cu.wup * s2;
+          Lfpk[at(k)] = (a.dts * (cu.ppten - cu.ppterm * div) + hp) + a.beta * vert;
+          Lpp[at(k)] = cu.pp;
+          Ls1[at(k)] = s1;
+          Ls2[at(k)] = s2;
+          Lmm[at(k)] = t.mm;
+          Ldd[at(k)] = wraw;
+          Lwt[at(k)] = cu.wt;
+          wraw = cu.wup;
+        }
+      } else {  // the east column's and the north row's faces
+        constexpr int B = 4;
+        for (int k0 = kb; k0 <= ke; k0 += B) {
+          float uo[B], ut[B], vo[B], vt[B], dc[B], dw[B], ds[B], tc[B], tw[B], ts[B];
+#pragma unroll
+          for (int u = 0; u < B; ++u)
+            if (k0 + u <= ke) {
+              const int64_t p = pix(k0 + u, pj, pi, ni, nj);
+              const int64_t pu = (int64_t)(k0 + u) * ust + uc, pv = (int64_t)(k0 + u) * vst + vc;
+              uo[u] = a.u3d[pu];
+              ut[u] = a.uten[pu];
+              vo[u] = a.v3d[pv];
+              vt[u] = a.vten[pv];
+              dc[u] = ppd[p];
+              dw[u] = ppd[p - 1];
+              ds[u] = ppd[p - np];
+              tc[u] = thv[p];
+              tw[u] = thv[p - 1];
+              ts[u] = thv[p - np];
+            }
+#pragma unroll
+          for (int u = 0; u < B; ++u)
+            if (k0 + u <= ke) {
+              const int64_t pu = (int64_t)(k0 + u) * ust + uc, pv = (int64_t)(k0 + u) * vst + vc;
+              float fu = tem1 * (dc[u] - dw[u]);
+              fu = fu * (tc[u] + tw[u]);
+              a.u_out[pu] = uo[u] + dtwu * (ut[u] - fu);
+              float fv = tem2 * (dc[u] - ds[u]);
+              fv = fv * (tc[u] + ts[u]);
+              a.v_out[pv] = vo[u] + dtwv * (vt[u] - fv);
+            }
+        }
+      }
+    }
+  } else if (FLAT && phys && kb <= ke) {
     const int pj = j + 1, pi = i + 1;
     const float temx = (a.dts * 0.5f) * a.rdx, temy = (a.dts * 0.5f) * a.rdy;
     if (kb == 1) lw1[c] = w[wst + wc];
     if constexpr (PIPE) {
-      // level k+1's words are requested before level k is formed
-      // piadv at k+1 is the next level's centre word and piadv, thv at k-1
-      // the previous level's, so each level requests only its own words
+      // the words of levels k+1 and k+2 are requested before level k is
+      // formed; piadv at k+1 is the next level's centre word and piadv, thv at
+      // k-1 the previous level's, so each level requests only its own words.
+      // 32-bit byte offsets (every array is under 4 GiB): one offset register
+      // serves every array of a family; the level factors come from LDS.
       struct In {
         float uw, ue, vs, vn, pc, pe, pw, pn, ps, wup, wt, ppten, ppterm, pp, th;
-        FlatCoef f;
       };
-      const int64_t pst = (int64_t)(nj + 2) * (ni + 2);
+      const uint32_t onp = (uint32_t)(ni + 2) * 4u, onx = (uint32_t)nxs * 4u, ows = (uint32_t)wst * 4u;
       auto load = [&](int k) {
-        const int64_t p = pix(k, pj, pi, ni, nj);
+        const uint32_t op = (uint32_t)pix(k, pj, pi, ni, nj) * 4u;
+        const uint32_t ou = (uint32_t)((int64_t)k * ust + uc) * 4u;
+        const uint32_t ov = (uint32_t)((int64_t)k * vst + vc) * 4u;
+        const uint32_t ow = (uint32_t)((int64_t)k * wst + wc) * 4u;
         In v;
-        v.uw = a.u3d[(int64_t)k * ust + uc];
-        v.ue = a.u3d[(int64_t)k * ust + uc + 1];
-        v.vs = a.v3d[(int64_t)k * vst + vc];
-        v.vn = a.v3d[(int64_t)k * vst + vc + nxs];
-        v.pc = a.piadv[p];
-        v.pe = a.piadv[p + 1];
-        v.pw = a.piadv[p - 1];
-        v.pn = a.piadv[p + (ni + 2)];
-        v.ps = a.piadv[p - (ni + 2)];
-        v.wup = w[(int64_t)(k + 1) * wst + wc];
-        v.wt = a.wten[(int64_t)k * wst + wc];
-        v.ppten = a.ppten[p];
-        v.ppterm = a.ppterm[p];
-        v.pp = a.pp3d[p];
-        v.th = a.thv[p];
-        v.f = coef(k);
+        v.uw = ld32(a.u3d, ou);
+        v.ue = ld32(a.u3d, ou + 4u);
+        v.vs = ld32(a.v3d, ov);
+        v.vn = ld32(a.v3d, ov + onx);
+        v.pc = ld32(a.piadv, op);
+        v.pe = ld32(a.piadv, op + 4u);
+        v.pw = ld32(a.piadv, op - 4u);
+        v.pn = ld32(a.piadv, op + onp);
+        v.ps = ld32(a.piadv, op - onp);
+        v.wup = ld32(w, ow + ows);
+        v.wt = ld32(a.wten, ow);
+        v.ppten = ld32(a.ppten, op);
+        v.ppterm = ld32(a.ppterm, op);
+        v.pp = ld32(a.pp3d, op);
+        v.th = ld32(a.thv, op);
         return v;
       };
       float wraw = w[(int64_t)kb * wst + wc];
       const int64_t pb = pix(kb - 1, pj, pi, ni, nj);
       float pd = a.piadv[pb], thm = a.thv[pb];
-      In nx = load(kb);
+      In n1 = load(kb);
+      In n2 = load(kb + 1);  // level nk+1 exists in every array read here
       for (int k = kb; k <= ke; ++k) {
-        const In cu = nx;
-        nx = load(k + 1);  // level nk+1 exists in every array read here
+        const In cu = n1;
+        n1 = n2;
+        if (k + 2 <= ke + 1) n2 = load(k + 2);
         float div = (cu.ue - cu.uw) * a.rdx + (cu.vn - cu.vs) * a.rdy;
         if (fabsf(div) < smeps) div = 0.0f;
         float hx = cu.uw * (cu.pc - cu.pw) + cu.ue * (cu.pe - cu.pc);
@@ -792,7 +1036,7 @@ __device__ __forceinline__ void column2_body(const SoundArgs& a, int n, int tile
         float hy = cu.vs * (cu.pc - cu.ps) + cu.vn * (cu.pn - cu.pc);
         hy = (-hy) * temy;
         const float hp = hx + hy;
-        const FlatTerms t = flat_terms(cu.f, k, nk, cu.ppterm, pd, cu.pc, nx.pc, cu.th, thm);
+        const FlatTerms t = flat_terms(lcoef(k), k, nk, cu.ppterm, pd, cu.pc, n1.pc, cu.th, thm);
         pd = cu.pc;
         thm = cu.th;
         const float s1 = t.pk1 + t.qk, s2 = t.pk2 - t.qk;
@@ -1065,53 +1309,7 @@ __device__ __forceinline__ void column2_body(const SoundArgs& a, int n, int tile
   }
   __syncthreads();
   // ---------------- phase B: Thomas recurrence, one lane per column
-  // (the next level's LDS words are fetched while the current level's
-  // division runs; s1/s2 of the level below are carried)
-  if (phys && slot == 0) {
-    float ec_prev = 0.0f, ed_prev = 0.0f, wtop = 0.0f;
-    float s1m = Ls1[at(1)], s2m = Ls2[at(1)];
-    float mmn = Lmm[at(2)], s1n = Ls1[at(2)], s2n = Ls2[at(2)], ddn = Ldd[at(2)];
-    for (int k = 2; k <= nk; ++k) {
-      const float mmk = mmn, s1k = s1n, s2k = s2n, dd = ddn;
-      if (k < nk) {
-        mmn = Lmm[at(k + 1)];
-        s1n = Ls1[at(k + 1)];
-        s2n = Ls2[at(k + 1)];
-        ddn = Ldd[at(k + 1)];
-      }
-      const float aa = (k >= 3) ? (-asq) * (mmk * s1m) : 0.0f;
-      const float cc = (k < nk) ? (-asq) * ((-mmk) * s2k) : 0.0f;
-      const float diag = (-s1k) + s2m;
-      const float bb = 1.0f - asq * (mmk * diag);
-      const float pivot = (k == 2) ? bb : aa * ec_prev + bb;
-      const float inv = 1.0f / pivot;
-      if (k < nk) {
-        const float eck = (-cc) * inv;
-        const float edk = ((k == 2) ? dd : dd - aa * ed_prev) * inv;
-        Lmm[at(k)] = eck;
-        Ldd[at(k)] = edk;
-        ec_prev = eck;
-        ed_prev = edk;
-      } else {
-        wtop = (dd - aa * ed_prev) * inv;
-      }
-      s1m = s1k;
-      s2m = s2k;
-    }
-    float wup = wtop;
-    Ldd[at(nk)] = wtop;
-    float ecn = Lmm[at(nk - 1)], edn = Ldd[at(nk - 1)];
-    for (int k = nk - 1; k >= 2; --k) {
-      const float eck = ecn, edk = edn;
-      if (k > 2) {
-        ecn = Lmm[at(k - 1)];
-        edn = Ldd[at(k - 1)];
-      }
-      const float wk = eck * wup + edk;
-      Ldd[at(k)] = wk;
-      wup = wk;
-    }
-  }
+  if (phys && slot == 0) thomas_column<C2W>(Ls1, Ls2, Lmm, Ldd, c, nk, asq);
   if constexpr (PIPE) {
     // while lane group 0 runs the recurrence, the level slots of the other
     // wavefronts add the u and v faces of every column of the tile into their
@@ -1122,7 +1320,8 @@ __device__ __forceinline__ void column2_body(const SoundArgs& a, int n, int tile
     if (inside && slot >= S0) {
       const int ch2 = (nk + (C2S - S0) - 1) / (C2S - S0);
       const int kb2 = 1 + (slot - S0) * ch2, ke2 = min(nk, kb2 + ch2 - 1);
-      average_uv_faces(a, kb2, ke2, ust, uc, vst, vc, last, tavg);
+      average_uv_faces(a, FUSE ? a.u_out : a.u3d, FUSE ? a.v_out : a.v3d, kb2, ke2, ust, uc, vst, vc, last,
+                       tavg);
     }
   }
   __syncthreads();
@@ -1154,7 +1353,7 @@ __device__ __forceinline__ void column2_body(const SoundArgs& a, int n, int tile
           if (fabsf(up) < smeps) up = 0.0f;
           a.pp3d[p] = up;
           const float tend = a.kdiv * (up - old);
-          a.ppd[p] = up + tend;
+          (FUSE ? a.ppd_out : a.ppd)[p] = up + tend;
           if (a.nrk == 3 && last) a.ppx[p] = tend;
           const float vw = rw[u] + (last ? wk * a.alph : wk);
           a.rrw[(int64_t)k * wst + wc] = last ? vw * tavg : vw;
@@ -1190,13 +1389,227 @@ __device__ __forceinline__ void column2_body(const SoundArgs& a, int n, int tile
     }
   }
   if (!PIPE && inside && !phys)  // faces of the east column and north row
-    average_uv_faces(a, kb, ke, ust, uc, vst, vc, last, tavg);
+    average_uv_faces(a, a.u3d, a.v3d, kb, ke, ust, uc, vst, vc, last, tavg);
 }
 
 template <int C2W, int C2T, bool FLAT, bool PIPE>
 __global__ __launch_bounds__(C2T) void k_snd_column2(SoundArgs a, int n) {
   const int ntiles = ((a.ni + 1 + C2W - 1) / C2W) * (a.nj + 1);
-  column2_body<C2W, C2T, FLAT, PIPE>(a, n, xcd_tile((int)blockIdx.x, ntiles));
+  column2_body<C2W, C2T, FLAT, PIPE, false>(a, n, xcd_tile((int)blockIdx.x, ntiles));
+}
+
+// the fused pass held to the four waves per SIMD the pass's LDS allows
+template <int C2W, int C2T>
+__global__ __launch_bounds__(C2T) __attribute__((amdgpu_waves_per_eu(4))) void k_snd_column2_fused(SoundArgs a,
+                                                                                                   int n) {
+  const int ntiles = ((a.ni + 1 + C2W - 1) / C2W) * (a.nj + 1);
+  column2_body<C2W, C2T, true, true, true>(a, n, xcd_tile((int)blockIdx.x, ntiles));
+}
+
+// ------------------------------------------------------------------ resident flat pass
+// The flat pipelined pass with pp and fpk of a thread's own levels (at most CH
+// of them) in registers: the right-hand side dd(k) is formed in phase A from
+// the level below's words (a slot's first level after one barrier, from the
+// previous slot's last-level words in LDS), so LDS holds only the planes the
+// recurrence reads (s1, s2, mm -> ec, dd -> ed -> w), the level factors and
+// the slots' last-level words: about 48 KiB at 32 columns x 81 levels, three
+// workgroups a CU where the seven-plane pass fits two.
+constexpr int C3PLANES = 4;
+
+int column3_lds_bytes(int nk, int cw, int slots) {
+  return (C3PLANES * (nk + 1) * cw + (nk + 1) * FLAT_COEF + 2 * slots * cw) * (int)sizeof(float);
+}
+
+template <int C2W, int C2T, int CH>
+__global__ __launch_bounds__(C2T) __attribute__((amdgpu_waves_per_eu(6))) void k_snd_column3(SoundArgs a, int n) {
+  constexpr int C2S = C2T / C2W;
+  extern __shared__ float lds[];
+  __shared__ float lw1[C2W];
+  const int ni = a.ni, nj = a.nj, nk = a.nk;
+  const int gx = (ni + 1 + C2W - 1) / C2W;
+  const int tile = xcd_tile((int)blockIdx.x, gx * (nj + 1));
+  const int j = tile / gx;
+  const int c = threadIdx.x % C2W, slot = threadIdx.x / C2W;
+  const int i = (tile % gx) * C2W + c;
+  const bool inside = (i <= ni);
+  const bool phys = (i < ni) && (j < nj);
+  const bool last = (n == a.nloop);
+  const float tavg = 1.0f / (float)a.nloop;
+  const int nxs = sx(ni), nys = sy(nj), nxu = ux(ni), nyv = vy(nj);
+  const int64_t wst = (int64_t)nys * nxs;
+  const int64_t wc = (int64_t)(j + NG) * nxs + (i + NG);
+  const int64_t uc = (int64_t)(j + NG) * nxu + (i + NG);
+  const int64_t ust = (int64_t)nys * nxu;
+  const int64_t vc = (int64_t)(j + NG) * nxs + (i + NG);
+  const int64_t vst = (int64_t)nyv * nxs;
+  float* w = a.w3d;
+  const int L = (nk + 1) * C2W;
+  float* Ls1 = lds;
+  float* Ls2 = lds + L;
+  float* Lmm = lds + 2 * L;
+  float* Ldd = lds + 3 * L;
+  float* Lcf = lds + 4 * L;  // the level factors, FLAT_COEF words a level
+  float* Lbp = Lcf + (nk + 1) * FLAT_COEF;  // pp of each slot's last level
+  float* Lbf = Lbp + C2S * C2W;             // fpk of each slot's last level
+  auto at = [&](int k) { return k * C2W + c; };
+  {
+    const float4* src = reinterpret_cast<const float4*>(a.qk);
+    float4* dst = reinterpret_cast<float4*>(Lcf);
+    for (int t = (int)threadIdx.x; t < (nk + 1) * (FLAT_COEF / 4); t += C2T) dst[t] = src[t];
+  }
+  __syncthreads();
+  auto lcoef = [&](int k) {
+    const float4* lcf = reinterpret_cast<const float4*>(Lcf);
+    const float4 x = lcf[2 * k], y = lcf[2 * k + 1];
+    FlatCoef f;
+    f.tem = x.x; f.r1 = x.y; f.r2 = x.z; f.tem1 = x.w;
+    f.c1a = y.x; f.c1b = y.y; f.c2a = y.z; f.c2b = y.w;
+    return f;
+  };
+  const int chunk = (nk + C2S - 1) / C2S;  // at most CH (checked by the launcher)
+  const int kb = 1 + slot * chunk, ke = min(nk, kb + chunk - 1);
+  const float asq = a.alph * a.alph;
+  const float smeps = 1.0e-30f;
+  const int pj = j + 1, pi = i + 1;
+  float rpp[CH], rfpk[CH];
+  float wt0 = 0.0f, wo0 = 0.0f;  // level kb's wten and w, for its right-hand side after the barrier
+
+  // ---------------- phase A (sound.F:491-509 and the right-hand sides)
+  if (phys && kb <= ke) {
+    const float temx = (a.dts * 0.5f) * a.rdx, temy = (a.dts * 0.5f) * a.rdy;
+    if (kb == 1) lw1[c] = w[wst + wc];
+    struct In {
+      float uw, ue, vs, vn, pc, pe, pw, pn, ps, wup, wt, ppten, ppterm, pp, th;
+    };
+    const uint32_t onp = (uint32_t)(ni + 2) * 4u, onx = (uint32_t)nxs * 4u, ows = (uint32_t)wst * 4u;
+    auto load = [&](int k) {
+      const uint32_t op = (uint32_t)pix(k, pj, pi, ni, nj) * 4u;
+      const uint32_t ou = (uint32_t)((int64_t)k * ust + uc) * 4u;
+      const uint32_t ov = (uint32_t)((int64_t)k * vst + vc) * 4u;
+      const uint32_t ow = (uint32_t)((int64_t)k * wst + wc) * 4u;
+      In v;
+      v.uw = ld32(a.u3d, ou);
+      v.ue = ld32(a.u3d, ou + 4u);
+      v.vs = ld32(a.v3d, ov);
+      v.vn = ld32(a.v3d, ov + onx);
+      v.pc = ld32(a.piadv, op);
+      v.pe = ld32(a.piadv, op + 4u);
+      v.pw = ld32(a.piadv, op - 4u);
+      v.pn = ld32(a.piadv, op + onp);
+      v.ps = ld32(a.piadv, op - onp);
+      v.wup = ld32(w, ow + ows);
+      v.wt = ld32(a.wten, ow);
+      v.ppten = ld32(a.ppten, op);
+      v.ppterm = ld32(a.ppterm, op);
+      v.pp = ld32(a.pp3d, op);
+      v.th = ld32(a.thv, op);
+      return v;
+    };
+    float wraw = w[(int64_t)kb * wst + wc];
+    const int64_t pb = pix(kb - 1, pj, pi, ni, nj);
+    float pd = a.piadv[pb], thm = a.thv[pb];
+    In nx = load(kb);
+#pragma unroll
+    for (int kk = 0; kk < CH; ++kk) {
+      const int k = kb + kk;
+      if (k <= ke) {
+        const In cu = nx;
+        nx = load(k + 1);  // level nk+1 exists in every array read here
+        float div = (cu.ue - cu.uw) * a.rdx + (cu.vn - cu.vs) * a.rdy;
+        if (fabsf(div) < smeps) div = 0.0f;
+        float hx = cu.uw * (cu.pc - cu.pw) + cu.ue * (cu.pe - cu.pc);
+        hx = (-hx) * temx;
+        float hy = cu.vs * (cu.pc - cu.ps) + cu.vn * (cu.pn - cu.pc);
+        hy = (-hy) * temy;
+        const float hp = hx + hy;
+        const FlatTerms t = flat_terms(lcoef(k), k, nk, cu.ppterm, pd, cu.pc, nx.pc, cu.th, thm);
+        pd = cu.pc;
+        thm = cu.th;
+        const float s1 = t.pk1 + t.qk, s2 = t.pk2 - t.qk;
+        const float vert = wraw * s1 + cu.wup * s2;
+        const float fpk = (a.dts * (cu.ppten - cu.ppterm * div) + hp) + a.beta * vert;
+        Ls1[at(k)] = s1;
+        Ls2[at(k)] = s2;
+        Lmm[at(k)] = t.mm;
+        rpp[kk] = cu.pp;
+        rfpk[kk] = fpk;
+        if (kk == 0) {
+          wt0 = cu.wt;
+          wo0 = wraw;
+        } else {  // the right-hand side of level k from level k-1's words
+          const float mmk = t.mm;
+          const float pdelta = rpp[kk] - rpp[kk - 1];
+          const float fwk = a.dts * cu.wt + a.beta * ((-mmk) * pdelta);
+          const float udelta = (rpp[kk] + rfpk[kk]) - (rpp[kk - 1] + rfpk[kk - 1]);
+          Ldd[at(k)] = (wraw + fwk) + a.alph * ((-mmk) * udelta);
+        }
+        if (k == ke) {
+          Lbp[slot * C2W + c] = cu.pp;
+          Lbf[slot * C2W + c] = fpk;
+        }
+        wraw = cu.wup;
+      }
+    }
+  }
+  __syncthreads();
+  // ---------------- a slot's first level: its right-hand side from the previous slot's last level
+  if (phys && kb <= ke && kb >= 2) {
+    const float ppm = Lbp[(slot - 1) * C2W + c], fm = Lbf[(slot - 1) * C2W + c];
+    const float mmk = Lmm[at(kb)];
+    const float pdelta = rpp[0] - ppm;
+    const float fwk = a.dts * wt0 + a.beta * ((-mmk) * pdelta);
+    const float udelta = (rpp[0] + rfpk[0]) - (ppm + fm);
+    Ldd[at(kb)] = (wo0 + fwk) + a.alph * ((-mmk) * udelta);
+  }
+  __syncthreads();
+  // ---------------- phase B: Thomas recurrence, one lane per column, while
+  // the slots outside wavefront 0 add the u and v faces into their averages
+  if (phys && slot == 0) thomas_column<C2W>(Ls1, Ls2, Lmm, Ldd, c, nk, asq);
+  {
+    constexpr int S0 = 64 / C2W;
+    static_assert(C2S > S0, "the averages need slots outside wavefront 0");
+    if (inside && slot >= S0) {
+      const int ch2 = (nk + (C2S - S0) - 1) / (C2S - S0);
+      const int kb2 = 1 + (slot - S0) * ch2, ke2 = min(nk, kb2 + ch2 - 1);
+      average_uv_faces(a, a.u3d, a.v3d, kb2, ke2, ust, uc, vst, vc, last, tavg);
+    }
+  }
+  __syncthreads();
+  // ---------------- phase C: pressure update, filter, time averages
+  if (phys && kb <= ke) {
+    const float wlid = w[(int64_t)(nk + 1) * wst + wc];
+    float rw[CH];
+#pragma unroll
+    for (int kk = 0; kk < CH; ++kk)
+      if (kb + kk <= ke) rw[kk] = a.rrw[(int64_t)(kb + kk) * wst + wc];
+#pragma unroll
+    for (int kk = 0; kk < CH; ++kk) {
+      const int k = kb + kk;
+      if (k <= ke) {
+        const int64_t p = pix(k, pj, pi, ni, nj);
+        const float wk = (k == 1) ? lw1[c] : Ldd[at(k)];
+        const float wup = (k == nk) ? wlid : Ldd[at(k + 1)];
+        if (k >= 2) w[(int64_t)k * wst + wc] = wk;
+        const float old = rpp[kk];
+        const float vt = wk * Ls1[at(k)] + wup * Ls2[at(k)];
+        float up = (old + rfpk[kk]) + a.alph * vt;
+        if (fabsf(up) < smeps) up = 0.0f;
+        a.pp3d[p] = up;
+        const float tend = a.kdiv * (up - old);
+        a.ppd[p] = up + tend;
+        if (a.nrk == 3 && last) a.ppx[p] = tend;
+        const float vw = rw[kk] + (last ? wk * a.alph : wk);
+        a.rrw[(int64_t)k * wst + wc] = last ? vw * tavg : vw;
+      }
+    }
+  } else if (inside) {
+    for (int k = kb; k <= ke; ++k) {
+      const int64_t pw = (int64_t)k * wst + wc;
+      const float wk = w[pw];
+      const float vw = a.rrw[pw] + (last ? wk * a.alph : wk);
+      a.rrw[pw] = last ? vw * tavg : vw;
+    }
+  }
 }
 
 // ------------------------------------------------------------------ radiation
@@ -1253,6 +1666,21 @@ __global__ __launch_bounds__(RAD_T) void k_snd_radiation(SoundArgs a, float dx, 
   for (int k = 1 + (int)threadIdx.x; k <= nk; k += RAD_T) out[(int64_t)k * rs + idx + NG] = avg;
 }
 
+// The halo ring (i = 0, ni+1 or j = 0, nj+1) of levels 1..nk copied from one
+// pressure-family array into another.
+__global__ __launch_bounds__(256) void k_pressure_ring_copy(const float* __restrict__ src, float* __restrict__ dst,
+                                                            int ni, int nj) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y + 1;
+  if (q >= 2 * (ni + 2) + 2 * nj) return;
+  int i, j;
+  if (q < ni + 2) { j = 0; i = q; }
+  else if (q < 2 * (ni + 2)) { j = nj + 1; i = q - (ni + 2); }
+  else { const int r = q - 2 * (ni + 2); j = 1 + r / 2; i = (r & 1) ? ni + 1 : 0; }
+  const int64_t p = pix(k, j, i, ni, nj);
+  dst[p] = src[p];
+}
+
 }  // namespace
 
 // LDS a workgroup may allocate on the current device (160 KiB on gfx950).
@@ -1269,15 +1697,39 @@ static int lds_limit() {
 }
 
 template <bool FLAT>
-static void launch_column2(bool narrow, bool pipe, const SoundArgs& a, int n, hipStream_t s) {
+static void launch_column2(bool narrow, bool pipe, const SoundArgs& a, int n, hipStream_t s, bool fuse = false) {
   const int ni = a.ni, nj = a.nj, nk = a.nk;
   const int cw = narrow ? 16 : 32;
   const dim3 grid(((ni + cw - 1 + 1) / cw) * (nj + 1));
-  const int lds = column2_lds_bytes(nk, cw, pipe);
+  const int lds = column2_lds_bytes(nk, cw, pipe, FLAT && pipe);
+  if constexpr (FLAT) {
+    if (fuse) {  // pipe holds
+      if (narrow) hipLaunchKernelGGL((k_snd_column2_fused<16, 256>), grid, dim3(256), lds, s, a, n);
+      else hipLaunchKernelGGL((k_snd_column2_fused<32, 512>), grid, dim3(512), lds, s, a, n);
+      return;
+    }
+  }
   if (narrow && pipe) hipLaunchKernelGGL((k_snd_column2<16, 256, FLAT, true>), grid, dim3(256), lds, s, a, n);
   else if (narrow) hipLaunchKernelGGL((k_snd_column2<16, 256, FLAT, false>), grid, dim3(256), lds, s, a, n);
   else if (pipe) hipLaunchKernelGGL((k_snd_column2<32, 512, FLAT, true>), grid, dim3(512), lds, s, a, n);
   else hipLaunchKernelGGL((k_snd_column2<32, 512, FLAT, false>), grid, dim3(512), lds, s, a, n);
+}
+
+// the resident flat pass with the smallest level budget CH >= its chunk
+static void launch_column3(bool narrow, const SoundArgs& a, int n, hipStream_t s) {
+  const int cw = narrow ? 16 : 32, slots = 16;
+  const dim3 grid(((a.ni + 1 + cw - 1) / cw) * (a.nj + 1));
+  const int lds = column3_lds_bytes(a.nk, cw, slots);
+  const int chunk = (a.nk + slots - 1) / slots;
+  if (narrow) {
+    if (chunk <= 4) hipLaunchKernelGGL((k_snd_column3<16, 256, 4>), grid, dim3(256), lds, s, a, n);
+    else if (chunk <= 6) hipLaunchKernelGGL((k_snd_column3<16, 256, 6>), grid, dim3(256), lds, s, a, n);
+    else hipLaunchKernelGGL((k_snd_column3<16, 256, 8>), grid, dim3(256), lds, s, a, n);
+  } else {
+    if (chunk <= 4) hipLaunchKernelGGL((k_snd_column3<32, 512, 4>), grid, dim3(512), lds, s, a, n);
+    else if (chunk <= 6) hipLaunchKernelGGL((k_snd_column3<32, 512, 6>), grid, dim3(512), lds, s, a, n);
+    else hipLaunchKernelGGL((k_snd_column3<32, 512, 8>), grid, dim3(512), lds, s, a, n);
+  }
 }
 
 static int psolver3_stage(
@@ -1290,9 +1742,10 @@ static int psolver3_stage(
     const float* c2, const float* gz, const float* rgzu, const float* rgzv, const float* dzdx, const float* dzdy,
     const float* gx, const float* gy, const float* gxu, const float* gyv, const float* profiles, float zt,
     const float* flat, float rdz, int ni, int nj, int nk, int nrk, int nloop, float dts, float dttmp, float dx,
-    float dy, float alph, float kdiv, unsigned sides, int flags, void* stream) {
+    float dy, float alph, float kdiv, unsigned sides, int flags, void* stream, float* u_alt = nullptr,
+    float* v_alt = nullptr, float* ppd_alt = nullptr) {
   hipStream_t s = (hipStream_t)stream;
-  SoundArgs a;
+  SoundArgs a = {};
   a.ppi = ppi; a.pp3d = pp3d; a.ppx = ppx; a.ppd = ppd; a.piadv = piadv; a.ppterm = ppterm; a.ppten = ppten;
   a.thv = thv; a.qk = qk; a.pk1 = pk1; a.pk2 = pk2; a.mm = mm; a.fpk = fpk; a.ec = elim_c; a.ed = elim_d;
   a.ua = ua; a.u3d = u3d; a.uten = uten; a.rru = rru; a.va = va; a.v3d = v3d; a.vten = vten; a.rrv = rrv;
@@ -1318,24 +1771,76 @@ static int psolver3_stage(
   const int cw = narrow ? 16 : 32;
   const int lds_max = lds_limit() - (int)(cw * sizeof(float));  // less the static w(1) row
   // (a column too tall for the seventh plane runs the six-plane staged pass)
-  const bool pipe = (flags & 16) != 0 && column2_lds_bytes(nk, cw, true) <= lds_max;
+  // (the flat pipelined passes address every array by 32-bit byte offsets: a
+  // flat column with an array of 4 GiB or more runs the staged pass)
+  const int64_t span = (int64_t)(nk + 3) * (nj + 1 + 2 * NG) * (ni + 1 + 2 * NG) * 4;  // bounds every array's bytes
+  const bool pipe = (flags & 16) != 0 && column2_lds_bytes(nk, cw, true, flat != nullptr) <= lds_max &&
+                    (!flat || span < ((int64_t)1 << 32));
   const bool staged = !(flags & 4) && column2_lds_bytes(nk, cw, pipe) <= lds_max;
   // every refusal comes before the first launch, so a refused call leaves the state untouched:
   // the flat column runs the staged passes only (k_snd_column is the terrain text), and the
   // radiation pass keeps a column's phase speeds in LDS
   if (flat && !staged) return (int)hipErrorInvalidValue;
   if ((flags & 1) && nk > RAD_MAXK) return (int)hipErrorInvalidValue;
+  // flags & 64: the pressure gradient runs inside the pipelined flat pass, the
+  // small steps alternating the winds and ppd between the caller's arrays and
+  // u_alt, v_alt, ppd_alt (so that no pass reads a word another workgroup of
+  // the same pass writes); the last small step writes the caller's arrays
+  const bool fuse = (flags & 64) != 0;
+  if (fuse && (!flat || !pipe || !u_alt || !v_alt || !ppd_alt))
+    return (int)hipErrorInvalidValue;
+  // flags & 128: the flat pass with its own levels' pressure words in registers
+  // (16 level slots of at most 8 levels: nk <= 128)
+  const bool resident = (flags & 128) != 0;
+  if (resident && (!flat || fuse || nk > 128 || column3_lds_bytes(nk, cw, 16) > lds_max ||
+                   span >= ((int64_t)1 << 32)))
+    return (int)hipErrorInvalidValue;
+  // with an odd number of small steps the first one reads the alternates
+  const bool start_alt = fuse && (nloop & 1);
+  if (fuse) a.ppd_twin = ppd_alt;
+  if (start_alt) { a.u_start = u_alt; a.v_start = v_alt; }
   hipLaunchKernelGGL(k_snd_prepare, dim3((ni + 2 + 63) / 64, nj + 2, nk), dim3(64), 0, s, a);
   GM1_LAUNCH_CHECK();
-  if (flat) {
+  a.ppd_twin = nullptr; a.u_start = nullptr; a.v_start = nullptr;
+  if (fuse) {
+    float* ub[2] = {u3d, u_alt};
+    float* vb[2] = {v3d, v_alt};
+    float* pb[2] = {ppd, ppd_alt};
+    for (int n = 1; n <= nloop; ++n) {
+      const int in = (n - 1 + (start_alt ? 1 : 0)) & 1, out = in ^ 1;
+      SoundArgs b = a;
+      b.u3d = ub[in]; b.v3d = vb[in]; b.ppd = pb[in];
+      b.u_out = ub[out]; b.v_out = vb[out]; b.ppd_out = pb[out];
+      if (any_open)
+        hipLaunchKernelGGL(k_snd_ssopen_flat, dim3(gm1_blocks((int64_t)nk * (ni + nj), 256)), dim3(256), 0, s, b);
+      GM1_TIMED(TK_ACOUSTIC_SMALL_STEP, s,
+                GM1_TIMED(TK_ACOUSTIC_COLUMN, s, launch_column2<true>(narrow, pipe, b, n, s, true)));
+      GM1_LAUNCH_CHECK();
+      if (n < nloop) {
+        int rc = fill_pressure(pb[out], sides, ni, nj, nk, s);
+        if (rc) return rc;
+      }
+    }
+    // ppd's halo ring takes the filled ring of the last small step's input,
+    // as the ring of the unfused pass holds the fill after step nloop-1 (both
+    // arrays took ppx from the prepare pass where no fill writes)
+    hipLaunchKernelGGL(k_pressure_ring_copy, dim3((2 * (ni + 2) + 2 * nj + 255) / 256, nk), dim3(256), 0, s,
+                       (const float*)ppd_alt, ppd, ni, nj);
+    GM1_LAUNCH_CHECK();
+  } else if (flat) {
     for (int n = 1; n <= nloop; ++n) {
       if (any_open)
         hipLaunchKernelGGL(k_snd_ssopen_flat, dim3(gm1_blocks((int64_t)nk * (ni + nj), 256)), dim3(256), 0, s, a);
-      GM1_TIMED(TK_ACOUSTIC_PGRAD, s,
-                hipLaunchKernelGGL(k_snd_pgrad_flat,
-                                   dim3(((ni + 1 + PG_TX - 1) / PG_TX) * ((nj + 1 + PG_TY - 1) / PG_TY) * nk),
-                                   dim3(PG_TX * PG_TY), 0, s, a));
-      GM1_TIMED(TK_ACOUSTIC_COLUMN, s, launch_column2<true>(narrow, pipe, a, n, s));
+      GM1_TIMED(TK_ACOUSTIC_SMALL_STEP, s, {
+        GM1_TIMED(TK_ACOUSTIC_PGRAD, s,
+                  hipLaunchKernelGGL(k_snd_pgrad_flat,
+                                     dim3(((ni + 1 + PG_TX - 1) / PG_TX) * ((nj + 1 + PG_TY - 1) / PG_TY) * nk),
+                                     dim3(PG_TX * PG_TY), 0, s, a));
+        if (resident)
+          GM1_TIMED(TK_ACOUSTIC_COLUMN, s, launch_column3(narrow, a, n, s));
+        else
+          GM1_TIMED(TK_ACOUSTIC_COLUMN, s, launch_column2<true>(narrow, pipe, a, n, s));
+      });
       GM1_LAUNCH_CHECK();
       if (n < nloop) {
         int rc = fill_pressure(ppd, sides, ni, nj, nk, s);
@@ -1350,15 +1855,17 @@ static int psolver3_stage(
     }
     // flags & 32: the caller states the terrain slopes gxu, gyv are +0 everywhere
     const dim3 pgrid(((ni + 1 + PG_TX - 1) / PG_TX) * ((nj + 1 + PG_TY - 1) / PG_TY) * nk);
-    if (flags & 32)
-      GM1_TIMED(TK_ACOUSTIC_PGRAD, s, hipLaunchKernelGGL(k_snd_pgrad<true>, pgrid, dim3(PG_TX * PG_TY), 0, s, a));
-    else
-      GM1_TIMED(TK_ACOUSTIC_PGRAD, s, hipLaunchKernelGGL(k_snd_pgrad<false>, pgrid, dim3(PG_TX * PG_TY), 0, s, a));
-    if (staged)
-      GM1_TIMED(TK_ACOUSTIC_COLUMN, s, launch_column2<false>(narrow, pipe, a, n, s));
-    else
-      GM1_TIMED(TK_ACOUSTIC_COLUMN, s,
-                hipLaunchKernelGGL(k_snd_column, dim3((ni + 1 + 63) / 64, nj + 1), dim3(64), 0, s, a, n));
+    GM1_TIMED(TK_ACOUSTIC_SMALL_STEP, s, {
+      if (flags & 32)
+        GM1_TIMED(TK_ACOUSTIC_PGRAD, s, hipLaunchKernelGGL(k_snd_pgrad<true>, pgrid, dim3(PG_TX * PG_TY), 0, s, a));
+      else
+        GM1_TIMED(TK_ACOUSTIC_PGRAD, s, hipLaunchKernelGGL(k_snd_pgrad<false>, pgrid, dim3(PG_TX * PG_TY), 0, s, a));
+      if (staged)
+        GM1_TIMED(TK_ACOUSTIC_COLUMN, s, launch_column2<false>(narrow, pipe, a, n, s));
+      else
+        GM1_TIMED(TK_ACOUSTIC_COLUMN, s,
+                  hipLaunchKernelGGL(k_snd_column, dim3((ni + 1 + 63) / 64, nj + 1), dim3(64), 0, s, a, n));
+    });
     GM1_LAUNCH_CHECK();
     if (n < nloop) {
       int rc = fill_pressure(ppd, sides, ni, nj, nk, s);
@@ -1417,4 +1924,27 @@ extern "C" int gm1hip_psolver3_stage_flat(
                         rvf, scalar, rho0_3d, rr0, rf0, nullptr, c1, c2, nullptr, nullptr, nullptr, nullptr, nullptr,
                         nullptr, nullptr, nullptr, nullptr, profiles, 1.0f, flat_column, rdz, ni, nj, nk, nrk, nloop,
                         dts, dttmp, dx, dy, alph, kdiv, sides, flags, stream);
+}
+
+// The flat stretched column's acoustic stage with the pressure-gradient step inside the column pass
+// (no counterpart of its own in the reference: the same stage as gm1hip_psolver3_stage_flat, word for word,
+// with one pass per small step instead of two).  u_alt, v_alt (the wind shapes) and ppd_alt (the pressure
+// shape) are workspace the small steps alternate with u3d, v3d and ppd; the stage ends in the caller's arrays.
+extern "C" int gm1hip_psolver3_stage_flat_fused(
+    float* ppi, float* pp3d, float* ppx, float* ppd, const float* piadv, const float* ppterm, const float* ppten,
+    const float* thv, float* qk, float* pk1, float* pk2, float* mm, float* fpk, float* elim_c, float* elim_d,
+    const float* ua, float* u3d, const float* uten, float* rru, const float* va, float* v3d, const float* vten,
+    float* rrv, const float* wa, float* w3d, const float* wten, float* rrw, float* radbcw, float* radbce,
+    float* radbcs, float* radbcn, const float* uh, const float* vh, const float* ruf, const float* rvf,
+    float* scalar, const float* rho0_3d, const float* rr0, const float* rf0, const float* c1, const float* c2,
+    const float* flat_column, float rdz, const float* profiles, float* u_alt, float* v_alt, float* ppd_alt, int ni,
+    int nj, int nk, int nrk, int nloop, float dts, float dttmp, float dx, float dy, float alph, float kdiv,
+    unsigned sides, int flags, void* stream) {
+  if (!flat_column || !profiles || !u_alt || !v_alt || !ppd_alt) return (int)hipErrorInvalidValue;
+  if (u_alt == u3d || v_alt == v3d || ppd_alt == ppd) return (int)hipErrorInvalidValue;
+  return psolver3_stage(ppi, pp3d, ppx, ppd, piadv, ppterm, ppten, thv, qk, pk1, pk2, mm, fpk, elim_c, elim_d, ua, u3d,
+                        uten, rru, va, v3d, vten, rrv, wa, w3d, wten, rrw, radbcw, radbce, radbcs, radbcn, uh, vh, ruf,
+                        rvf, scalar, rho0_3d, rr0, rf0, nullptr, c1, c2, nullptr, nullptr, nullptr, nullptr, nullptr,
+                        nullptr, nullptr, nullptr, nullptr, profiles, 1.0f, flat_column, rdz, ni, nj, nk, nrk, nloop,
+                        dts, dttmp, dx, dy, alph, kdiv, sides, flags | 16 | 64, stream, u_alt, v_alt, ppd_alt);
 }

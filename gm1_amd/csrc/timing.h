@@ -18,6 +18,7 @@ enum TimedKernel : int {
   TK_MOM_ADVU = 4,
   TK_MOM_ADVV = 5,
   TK_MOM_ADVW = 6,
+  TK_ACOUSTIC_SMALL_STEP = 7,  // one small step's pressure-gradient and column work (one or two passes)
 };
 
 extern int g_timed_kernel;

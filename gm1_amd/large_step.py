@@ -191,12 +191,17 @@ def create_gm1_large_step_state(host: dict[str, Any], device: torch.device | str
     st.c12_columns = verify_horizontally_uniform(st.metrics)
     if host.get("levels") is not None:
         st.flat = flat_stretched_column(host["levels"], dev)
+        if st.cfg.nk <= 128:  # the flat pass with its own levels' pressure words in registers
+            st.column_pass = "resident"
     # device copies of the vertical profiles are made here, never inside a step
     # (a step may be captured into a graph, where host-to-device copies are not allowed)
     device_profiles(st.prof, dev)
     if st.flat is not None:
         device_profiles(st.flat.levels.profiles(), dev)
     st.alt = {"th3d": torch.empty_like(st.a["th3d"])}
+    if st.flat is not None:  # the fused column passes' wind and ppd workspace
+        for name in ("u3d", "v3d", "ppd"):
+            st.alt[name] = torch.empty_like(st.a[name])
     if st.numq:
         st.alt["q3d"] = torch.empty_like(st.a["q3d"])
         cfg = st.cfg
@@ -294,7 +299,7 @@ def _flat_rk_stage(st: Gm1LargeStepState, nrk: int, alternate: bool = False) -> 
                  stream_handle(st.stream))
     launch_cuda_gm1_flat_acoustic_stage(a, m, ni=ni, nj=nj, nk=nk, nrk=nrk, dt=cfg.dt, nsound=cfg.nsound, dx=cfg.dx,
                                         dy=cfg.dy, vertical=st.flat, stream=st.stream, lateral_sides=sides,
-                                        column_pass=st.column_pass)
+                                        column_pass=st.column_pass, alternates=st.alt)
     launch_cuda_gm1_child_stretched_post_acoustic(a, m, st.flat, ni=ni, nj=nj, nk=nk, dx=cfg.dx, dy=cfg.dy,
                                                   thterm=a["thterm"], divergence=a["divergence"],
                                                   theta_divergence_term=a["theta_div"], divx=a["divx"],

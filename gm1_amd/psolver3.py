@@ -130,8 +130,16 @@ def verify_terrain_slopes_zero(metrics: Mapping[str, Any]) -> bool:
 # level's loads a level ahead and run the u/v time averages beside the
 # recurrence; "register" walks each column in one thread
 COLUMN_PASSES = {"staged": 0, "pipelined": 16, "staged16": 8, "pipelined16": 24, "register": 4}
-# the flat column runs the LDS-staged passes (the register pass is the terrain text)
-FLAT_COLUMN_PASSES = {k: v for k, v in COLUMN_PASSES.items() if k != "register"}
+# the flat column runs the LDS-staged passes (the register pass is the terrain text); its
+# "fused" forms also step u and v inside the pipelined pass (one pass per small step instead
+# of two), alternating the winds and ppd with a workspace of the same shapes; its "resident"
+# forms keep pp and fpk of a thread's own levels in registers (four LDS planes instead of
+# seven: three workgroups a CU where the pipelined pass fits two; nk <= 128)
+FUSED_PASS_FLAG = 64
+RESIDENT_PASS_FLAG = 128
+FLAT_COLUMN_PASSES = {**{k: v for k, v in COLUMN_PASSES.items() if k != "register"},
+                      "fused": 16 | FUSED_PASS_FLAG, "fused16": 24 | FUSED_PASS_FLAG,
+                      "resident": RESIDENT_PASS_FLAG, "resident16": 8 | RESIDENT_PASS_FLAG}
 
 
 def _check_ablations(ablations: TerrainOpenSoundAblations) -> int:

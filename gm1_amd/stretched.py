@@ -30,7 +30,7 @@ from ._device import NG, lateral_code, ptr, record, require, shapes, stream_hand
 from ._native import D, Fl, I, P, U, signature
 from .errors import refuse
 from .grid import VerticalLevels, rk_stage_dt_f32, sound3_stage_schedule_f32
-from .psolver3 import (CM1_ALPH_F32, CM1_KDIV_F32, FLAT_COLUMN_PASSES, CudaPsolver3TerrainOpenLaunch,
+from .psolver3 import (CM1_ALPH_F32, CM1_KDIV_F32, FLAT_COLUMN_PASSES, FUSED_PASS_FLAG, CudaPsolver3TerrainOpenLaunch,
                        TerrainOpenSoundAblations, device_profiles, terrain_open_sound_shapes)
 from .scalar_moist_rk import CM1_NRKMAX, moisture_weps_f64, scalar_moist_rk_stage_timestep_f32
 
@@ -40,6 +40,8 @@ signature("gm1hip_post_acoustic_flat", *([P] * 10), I, I, I, Fl, Fl, Fl, P)
 signature("gm1hip_scalar_rk_stage_flat", *([P] * 13), I, I, I, I, D, Fl, Fl, Fl, U, Fl, P)
 signature("gm1hip_pdef_moisture_stage_flat", *([P] * 16), I, I, I, I, D, Fl, Fl, Fl, U, Fl, P)
 signature("gm1hip_psolver3_stage_flat", *([P] * 42), Fl, P, I, I, I, I, I, Fl, Fl, Fl, Fl, Fl, Fl, U, I, P)
+signature("gm1hip_psolver3_stage_flat_fused", *([P] * 42), Fl, P, P, P, P, I, I, I, I, I, Fl, Fl, Fl, Fl, Fl, Fl, U, I,
+          P)
 
 
 @dataclass(frozen=True)
@@ -189,12 +191,16 @@ def launch_cuda_gm1_flat_acoustic_stage(
     arrays: Mapping[str, Any], metrics: Mapping[str, Any], *, ni: int, nj: int, nk: int, nrk: int, dt: float,
     nsound: int, dx: float, dy: float, vertical: FlatStretchedColumn, ablations: TerrainOpenSoundAblations | None = None,
     stream: Any | None = None, lateral_sides: tuple[int, int, int, int] = (2, 2, 2, 2), alph: float | None = None,
-    kdiv: float | None = None, lateral: Any | None = None, column_pass: str = "pipelined", **_: Any,
+    kdiv: float | None = None, lateral: Any | None = None, column_pass: str = "pipelined",
+    alternates: Mapping[str, Any] | None = None, **_: Any,
 ) -> CudaPsolver3TerrainOpenLaunch:
     """One acoustic RK stage on a flat stretched column (sound.F with terrain_flag=.false.).
 
     Only the all-open Durran-Klemp lateral family is built (``lateral=None``,
     the reference's sealed path); irbc=2 phase speeds and roflux are refused.
+    The fused column passes need ``alternates``: workspace arrays ``u3d``,
+    ``v3d`` and ``ppd`` of those arrays' shapes, distinct from them, which the
+    small steps alternate with the caller's arrays (no state is kept there).
     """
     if lateral is not None:
         refuse(code="GM1-FLAT-ACOUSTIC-LATERAL", gate="stretched", option="lateral", value=type(lateral).__name__,
@@ -228,10 +234,21 @@ def launch_cuda_gm1_flat_acoustic_stage(
     nloop, dts = sound3_stage_schedule_f32(dt, nsound, nrk)
     alph32 = float(np.float32(CM1_ALPH_F32 if alph is None else alph))
     kdiv32 = float(np.float32(CM1_KDIV_F32 if kdiv is None else kdiv))
-    _native.call("gm1hip_psolver3_stage_flat", *(ptr(a[n]) for n in names), ptr(scalar),
-                 *(ptr(m[n]) for n in ("rho0_3d", "rr0", "rf0", "c1", "c2")), ptr(v.column), float(v.rdz),
-                 ptr(prof), ni, nj, nk, nrk, nloop, dts, rk_stage_dt_f32(dt, nrk), float(dx), float(dy), alph32,
-                 kdiv32, lateral_code(lateral_sides), flags, stream_handle(stream))
+    common = (*(ptr(a[n]) for n in names), ptr(scalar), *(ptr(m[n]) for n in ("rho0_3d", "rr0", "rf0", "c1", "c2")),
+              ptr(v.column), float(v.rdz), ptr(prof))
+    tail = (ni, nj, nk, nrk, nloop, dts, rk_stage_dt_f32(dt, nrk), float(dx), float(dy), alph32, kdiv32,
+            lateral_code(lateral_sides), flags, stream_handle(stream))
+    if flags & FUSED_PASS_FLAG:
+        if alternates is None:
+            raise ValueError(f"the {column_pass!r} column pass needs alternates u3d, v3d and ppd")
+        alt = {n: require(alternates[n], f"alternates[{n!r}]", shp[n]) for n in ("u3d", "v3d", "ppd")}
+        for n in alt:
+            if alt[n].data_ptr() == a[n].data_ptr():
+                raise ValueError(f"alternates[{n!r}] must be a different array from {n}")
+        _native.call("gm1hip_psolver3_stage_flat_fused", *common, ptr(alt["u3d"]), ptr(alt["v3d"]), ptr(alt["ppd"]),
+                     *tail)
+    else:
+        _native.call("gm1hip_psolver3_stage_flat", *common, *tail)
     cadence = [{"nrk": nrk, "nloop": nloop, "dts_f32": dts}]
     return CudaPsolver3TerrainOpenLaunch(record(stream), None, tuple(cadence), {
         "domain": {"ni": ni, "nj": nj, "nk": nk, "ngxy": NG}, "cadence": cadence, "family": "flat-stretched",

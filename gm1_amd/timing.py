@@ -22,6 +22,7 @@ KERNELS = {
     "k_mom_advu": 4,
     "k_mom_advv": 5,
     "k_mom_advw": 6,
+    "k_snd_small_step": 7,
 }
 
 

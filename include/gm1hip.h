@@ -434,6 +434,30 @@ int gm1hip_psolver3_stage_flat(
     const float* flat_column, float rdz, const float* profiles, int ni, int nj, int nk, int nrk, int nloop,
     float dts, float dttmp, float dx, float dy, float alph, float kdiv, unsigned sides, int flags, void* stream);
 
+// The same stage as gm1hip_psolver3_stage_flat, word for word, with the
+// pressure-gradient step of each small step (cuda_gm1_child_psolver3_open.py
+// :387-399, launched at :1701-1711) inside the pipelined column pass: one pass per
+// small step instead of two.  The reference has no fused form; it replaces the
+// same launch_cuda_gm1_flat_acoustic_stage.  u_alt and v_alt (the u3d and v3d
+// shapes) and ppd_alt (the ppd shape) are workspace distinct from those
+// arrays: the small steps alternate between them so that no pass reads a word
+// another workgroup of the pass writes, and the stage ends in the caller's
+// arrays (nothing is kept in the workspace between calls).  flags bit 8
+// selects 16-column workgroups; the pipelined and fused bits are implied.
+// Refused (hipErrorInvalidValue, nothing launched): a missing or aliased
+// alternate, a column too tall for the fused pass's LDS, or an array of
+// 4 GiB or more (the pass addresses by 32-bit byte offsets).
+int gm1hip_psolver3_stage_flat_fused(
+    float* ppi, float* pp3d, float* ppx, float* ppd, const float* piadv, const float* ppterm, const float* ppten,
+    const float* thv, float* qk, float* pk1, float* pk2, float* mm, float* fpk, float* elim_c, float* elim_d,
+    const float* ua, float* u3d, const float* uten, float* rru, const float* va, float* v3d, const float* vten,
+    float* rrv, const float* wa, float* w3d, const float* wten, float* rrw, float* radbcw, float* radbce,
+    float* radbcs, float* radbcn, const float* uh, const float* vh, const float* ruf, const float* rvf,
+    float* scalar, const float* rho0_3d, const float* rr0, const float* rf0, const float* c1, const float* c2,
+    const float* flat_column, float rdz, const float* profiles, float* u_alt, float* v_alt, float* ppd_alt, int ni,
+    int nj, int nk, int nrk, int nloop, float dts, float dttmp, float dx, float dy, float alph, float kdiv,
+    unsigned sides, int flags, void* stream);
+
 // ----------------------------------------------------------------- measurement
 // No counterpart in the reference: live per-kernel GPU time for throughput
 // reports.  arm() creates `capacity` event pairs (before a timed region) and

@@ -389,16 +389,23 @@ def test_gpu_flat_buoyancy_post_acoustic_and_acoustic_stage_match_oracle(device)
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("sides,column_pass", [((2, 2, 2, 2), "staged"), ((1, 1, 1, 1), "staged"),
-                                               ((2, 2, 2, 2), "pipelined"), ((2, 2, 2, 2), "pipelined16"),
-                                               ((1, 1, 1, 1), "staged16")])
-def test_gpu_stretched_moist_large_step_bit_exact(device, sides, column_pass):
-    """Two moist large steps on a stretched column against the oracle (device dry-mass correction injected)."""
+@pytest.mark.parametrize("sides,column_pass,ni", [((2, 2, 2, 2), "staged", 24), ((1, 1, 1, 1), "staged", 24),
+                                                  ((2, 2, 2, 2), "pipelined", 24), ((2, 2, 2, 2), "pipelined16", 24),
+                                                  ((1, 1, 1, 1), "staged16", 24), ((2, 2, 2, 2), "fused", 70),
+                                                  ((1, 1, 1, 1), "fused", 70), ((2, 2, 2, 2), "fused16", 24),
+                                                  ((2, 2, 2, 2), "resident", 70), ((1, 1, 1, 1), "resident16", 24)])
+def test_gpu_stretched_moist_large_step_bit_exact(device, sides, column_pass, ni):
+    """Two moist large steps on a stretched column against the oracle (device dry-mass correction injected).
+
+    At ni = 70 the 32-column fused pass has two full tiles and a ragged one, so
+    a tile's last lane and the lane west of the east column both form their own
+    east u face; the 16-column one does at ni = 24.
+    """
     import torch
 
     from gm1_amd.large_step import create_gm1_large_step_state, launch_cuda_gm1_large_step
 
-    cfg = stretched_case(24, 20, 16, sides)
+    cfg = stretched_case(ni, 20, 16, sides)
     host = host_initial_state(cfg)
     st = create_gm1_large_step_state(host, device)
     st.column_pass = column_pass
@@ -416,3 +423,53 @@ def test_gpu_stretched_moist_large_step_bit_exact(device, sides, column_pass):
             assert_bits_equal(to_host(st.a[name]), a[name], f"step {step + 1} {name}")
         np.testing.assert_allclose(to_host(st.a["qbudget"]), a["qbudget"], rtol=1e-11, atol=1e-25)
     assert np.abs(a["w3d"]).max() > 0.01
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sides", [(2, 2, 2, 2), (1, 1, 1, 1)])
+def test_gpu_fused_column_pass_equals_the_two_pass_stage(device, sides):
+    """The fused and resident passes against the pipelined stage, word for word, on the 80-level stretched
+    column at 130x90 (about 5x91 and 9x91 workgroups a pass): every output of the three RK stages (3, 5 and
+    10 small steps: both parities of the alternation), including ppd's halo ring and the radiation phase
+    speeds."""
+    from dataclasses import replace
+
+    import torch
+
+    from gm1_amd.fields import large_stretched_c3
+    from gm1_amd.large_step import create_gm1_large_step_state, launch_cuda_gm1_large_step
+    from gm1_amd.stretched import launch_cuda_gm1_flat_acoustic_stage
+
+    cfg = replace(large_stretched_c3(130, 90, 80), lateral_sides=tuple(sides))
+    print(f"fused column pass {cfg.ni}x{cfg.nj}x{cfg.nk} sides {sides}", flush=True)
+    st = create_gm1_large_step_state(host_initial_state(cfg), device)
+    launch_cuda_gm1_large_step(st)  # a stirred state: winds, pressure and tendencies all nonzero
+    torch.cuda.synchronize()
+    names = ("u3d", "v3d", "w3d", "pp3d", "ppd", "ppx", "ppi", "rru", "rrv", "rrw", "radbcw", "radbce", "radbcs",
+             "radbcn")
+    runs = {}
+    for column_pass in ("pipelined", "fused", "fused16", "resident", "resident16"):
+        a = {k: v.clone() for k, v in st.a.items()}
+        alt = {k: torch.full_like(st.alt[k], float("nan")) for k in ("u3d", "v3d", "ppd")}
+        out = []
+        for nrk in (1, 2, 3):
+            launch_cuda_gm1_flat_acoustic_stage(a, st.metrics, ni=cfg.ni, nj=cfg.nj, nk=cfg.nk, nrk=nrk, dt=cfg.dt,
+                                                nsound=cfg.nsound, dx=cfg.dx, dy=cfg.dy, vertical=st.flat,
+                                                lateral_sides=sides, column_pass=column_pass, alternates=alt)
+            torch.cuda.synchronize()
+            out.append({n: to_host(a[n]) for n in names})
+        runs[column_pass] = out
+    for column_pass in ("fused", "fused16", "resident", "resident16"):
+        for nrk in (1, 2, 3):
+            for n in names:
+                assert_bits_equal(runs[column_pass][nrk - 1][n], runs["pipelined"][nrk - 1][n],
+                                  f"{column_pass} stage {nrk} {n}")
+    assert np.abs(runs["pipelined"][2]["w3d"]).max() > 0.0
+
+
+def test_fused_column_pass_flags():
+    """Only the fused forms carry the fused-pass flag (the flag that makes the stage take alternates)."""
+    from gm1_amd.psolver3 import FLAT_COLUMN_PASSES, FUSED_PASS_FLAG
+
+    assert FLAT_COLUMN_PASSES["fused"] & FUSED_PASS_FLAG and FLAT_COLUMN_PASSES["fused16"] & FUSED_PASS_FLAG
+    assert not any(v & FUSED_PASS_FLAG for k, v in FLAT_COLUMN_PASSES.items() if not k.startswith("fused"))
