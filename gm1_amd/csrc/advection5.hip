@@ -550,9 +550,280 @@ __device__ __forceinline__ void scalar_adv5_body(const ScalarAdvArgs& a) {
 }
 
 
+// The word at a byte offset from a base the whole workgroup shares: the load
+// takes the base in scalar registers and the offset in one vector register.
+GM1_DEV float ldo(const float* base, uint32_t byte_offset) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + byte_offset);
+}
+GM1_DEV void sto(float* base, uint32_t byte_offset, float v) {
+  *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + byte_offset) = v;
+}
+constexpr uint32_t NO_WORD = 0xFFFFFFFFu;  // a slot with no word to read
+
+// The same statements as scalar_adv5_body with another addressing, for the
+// instances that run at four waves per SIMD (the WENO forms and the 5th-order
+// tendency).  Addressing: every word a thread reads or writes at level k sits at a byte
+// offset, fixed for the thread, from a base that depends on k alone (the
+// level's plane of that array).  The offsets -- the tile's level-plane
+// slots, face mass-flux slots, the own cell and faces -- are formed once
+// before the level march (the lateral family's image of a halo cell
+// included), so a level's addresses cost one scalar base each.
+template <bool WENO, int MODE, bool FLAT, bool SRC>
+__device__ __forceinline__ void scalar_adv5_body_held(const ScalarAdvArgs& a) {
+  constexpr bool GATE = (MODE == MODE_PDEF);
+  __shared__ float lfx[TY][TX + 1];
+  __shared__ float lfy[TY + 1][TX];
+  __shared__ float lp[TY + 6][LPX];  // the tile's level plane with a 3-cell halo
+  const int ni = a.ni, nj = a.nj, nk = a.nk;
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  // tiles ordered i fastest, then j, then column chunk; each XCD sweeps a
+  // contiguous range so the y-halo rows a tile shares stay in one L2
+  const int gxt = (ni + TX - 1) / TX, gyt = (nj + TY - 1) / TY;
+  const int tile = xcd_tile((int)blockIdx.x, gxt * gyt * a.nchunks);
+  const int i0 = (tile % gxt) * TX, j0 = ((tile / gxt) % gyt) * TY;
+  const int i = i0 + tx, j = j0 + ty;
+  const int kb = 1 + (tile / (gxt * gyt)) * a.kchunk;
+  const int ke = min(nk, kb + a.kchunk - 1);
+  const int nxs = sx(ni), nys = sy(nj), nxu = ux(ni), nyv = vy(nj);
+  const bool live = (i < ni) && (j < nj);
+  const unsigned obc = a.obc;
+  const bool owned = live && (obc != 0u || (i >= 3 && i + 3 < ni && j >= 3 && j + 3 < nj));
+  const int x = i + NG, y = j + NG;
+  // level strides (words) of the padded scalar / w, u, v and unpadded families
+  const int64_t st = (int64_t)nys * nxs, ust = (int64_t)nys * nxu, vst = (int64_t)nyv * nxs;
+  const int64_t mst = (int64_t)nj * ni;
+  // the thread's own cell and faces within a level plane (bytes)
+  const uint32_t o_s = (uint32_t)(y * nxs + x) * 4u;  // padded scalar, w and v (south face) families
+  const uint32_t o_u = (uint32_t)(y * nxu + x) * 4u;  // u family (west face)
+  const uint32_t o_m = (uint32_t)(j * ni + i) * 4u;   // unpadded cells
+  const uint32_t o_z = a.c12col ? 0u : o_m;           // c1, c2 words of the z faces
+
+  // Slot offsets: the level plane's rows j0-3..j0+TY+2 and columns
+  // i0-3..i0+TX+2 (padded indices j0..j0+TY+5, i0..i0+TX+5, clipped to the
+  // padded array; from the unpadded carrier, the lateral family's image of
+  // each halo cell), the x faces i0..i0+TX of the TY rows and the y faces
+  // j0..j0+TY of the TX columns.
+  constexpr int PS = ((TY + 6) * (TX + 6) + NT - 1) / NT;
+  constexpr int XS = (TY * (TX + 1) + NT - 1) / NT;
+  constexpr int YS = ((TY + 1) * TX + NT - 1) / NT;
+  uint32_t poff[PS], muoff[XS], mvoff[YS];
+#pragma unroll
+  for (int u = 0; u < PS; ++u) {
+    const int slot = threadIdx.x + u * NT;
+    const int r = slot / (TX + 6), c = slot % (TX + 6);
+    const int yy = j0 + r, xx = i0 + c;
+    poff[u] = NO_WORD;
+    if (slot < (TY + 6) * (TX + 6) && yy < nys && xx < nxs) {
+      if constexpr (SRC) {
+        const int pj = family_cell(yy - NG, nj, side_of(obc, 2), side_of(obc, 3));
+        const int pi = family_cell(xx - NG, ni, side_of(obc, 0), side_of(obc, 1));
+        poff[u] = (uint32_t)(pj * ni + pi) * 4u;
+      } else {
+        poff[u] = (uint32_t)(yy * nxs + xx) * 4u;
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < XS; ++u) {
+    const int slot = threadIdx.x + u * NT;
+    const int r = slot / (TX + 1), t = slot % (TX + 1);
+    const int f = i0 + t, jj = j0 + r;
+    muoff[u] = (slot < TY * (TX + 1) && f <= ni && jj < nj) ? (uint32_t)((jj + NG) * nxu + f + NG) * 4u : NO_WORD;
+  }
+#pragma unroll
+  for (int u = 0; u < YS; ++u) {
+    const int slot = threadIdx.x + u * NT;
+    const int r = slot / TX, t = slot % TX;
+    const int ii = i0 + t, g = j0 + r;
+    mvoff[u] = (slot < (TY + 1) * TX && ii < ni && g <= nj) ? (uint32_t)((g + NG) * nxs + ii + NG) * 4u : NO_WORD;
+  }
+
+  // The column's scalar S(k-3..k+3) at level k rides in registers: win[q]
+  // holds level k-3+q (0 outside the padded column; those words are never
+  // read by a face's stencil, which stays within levels 1..nk).
+  float win[7];
+  auto colword = [&](int lev) {  // the column's word at a level, 0 where no stencil reads
+    if constexpr (SRC) {
+      // the own column is a physical cell, so its unpadded word needs no lateral image
+      if (lev < 1 || lev > nk) return 0.0f;
+      const float v = ldo(a.src + (int64_t)(lev - 1) * mst, o_m);
+      return a.th0 ? (a.th0[lev - 1] - 300.0f) + v : v;
+    }
+    return (lev >= 0 && lev <= nk + 1) ? ldo(a.s + (int64_t)lev * st, o_s) : 0.0f;
+  };
+#pragma unroll
+  for (int q = 0; q < 7; ++q) win[q] = live ? colword(kb - 3 + q) : 0.0f;
+  // z-face flux at face kf from the window b, b[0] = S(kf-3).
+  auto zf = [&](int kf, const float* b) {
+    const float m = ldo(a.rrw + (int64_t)kf * st, o_s);
+    const float cc1 = ldo(a.c1 + (int64_t)(kf - 1) * mst, o_z), cc2 = ldo(a.c2 + (int64_t)(kf - 1) * mst, o_z);
+    auto S = [&](int level) { return b[level - (kf - 3)]; };
+    if (MODE == MODE_PDEF) return zflux_pdef_g(S, m, cc1, cc2, kf, nk, a.weps, a.pdefweno != 0);
+    return zflux_g<WENO>(S, m, cc1, cc2, kf, nk, a.weps);
+  };
+  float fzb = 0.0f;
+  if (live && kb >= 2) fzb = zf(kb, win);  // S(kb-3..)
+  const float gzc = (!FLAT && owned && MODE != MODE_PDEF) ? ldo(a.gz, o_m) : 0.0f;
+  // A level's plane and face mass fluxes are requested one level ahead into
+  // registers (Lv), and its cell words at the top of the level, so they
+  // arrive while the level before it (or its fluxes) is formed.
+  struct Lv {
+    float plane[PS], mu[XS], mv[YS];
+    float snext;
+  };
+  auto fetch = [&](int k) {
+    Lv v;
+    v.snext = live ? colword(k + 4) : 0.0f;  // the level entering the window after level k
+    const float* pl = SRC ? a.src + (int64_t)(k - 1) * mst : a.s + (int64_t)k * st;
+    const float pack = (SRC && a.th0) ? a.th0[k - 1] - 300.0f : 0.0f;
+#pragma unroll
+    for (int u = 0; u < PS; ++u) {
+      v.plane[u] = 0.0f;
+      if (poff[u] != NO_WORD) {
+        const float w = ldo(pl, poff[u]);
+        v.plane[u] = (SRC && a.th0) ? pack + w : w;
+      }
+    }
+    const float* pu = a.rru + (int64_t)k * ust;
+#pragma unroll
+    for (int u = 0; u < XS; ++u) v.mu[u] = (muoff[u] != NO_WORD) ? ldo(pu, muoff[u]) : 0.0f;
+    const float* pv = a.rrv + (int64_t)k * vst;
+#pragma unroll
+    for (int u = 0; u < YS; ++u) v.mv[u] = (mvoff[u] != NO_WORD) ? ldo(pv, mvoff[u]) : 0.0f;
+    return v;
+  };
+  Lv nx = fetch(kb);
+  for (int k = kb; k <= ke; ++k) {
+    const Lv cu = nx;
+    if (k < ke) nx = fetch(k + 1);
+    const float snext = cu.snext;
+    const int64_t lm = (int64_t)(k - 1) * mst;  // the unpadded families' level k
+    // this level's cell words, consumed after its fluxes
+    float c_uw = 0.0f, c_ue = 0.0f, c_vs = 0.0f, c_vn = 0.0f, c_divx = 0.0f, c_rr0 = 0.0f, c_a0 = 0.0f,
+          c_held = 0.0f, c_divterm = 0.0f;
+    if (owned) {
+      if (obc != 0u) {
+        const float* pu = a.rru + (int64_t)k * ust;
+        const float* pv = a.rrv + (int64_t)k * vst;
+        c_uw = ldo(pu, o_u);
+        c_ue = ldo(pu, o_u + 4u);
+        c_vs = ldo(pv, o_s);
+        c_vn = ldo(pv, o_s + (uint32_t)nxs * 4u);
+      }
+      if (MODE != MODE_PDEF) {
+        c_divx = ldo(a.divx + lm, o_m);
+        c_rr0 = FLAT ? a.rr0[lm] : ldo(a.rr0 + lm, o_m);
+        if (MODE == MODE_COMPOSE) {
+          c_a0 = ldo(a.a0 + lm, o_m);
+          c_held = ldo(a.held + lm, o_m);
+          if (a.divterm) c_divterm = ldo(a.divterm + lm, o_m);
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < PS; ++u) {
+      if (poff[u] != NO_WORD) {
+        const int slot = threadIdx.x + u * NT;
+        const int r = slot / (TX + 6), c = slot % (TX + 6);
+        lp[r][c] = cu.plane[u];
+      }
+    }
+    __syncthreads();
+    // x-face fluxes: faces i0..i0+TX of the TY rows.
+#pragma unroll
+    for (int u = 0; u < XS; ++u) {
+      if (muoff[u] != NO_WORD) {
+        const int slot = threadIdx.x + u * NT;
+        const int r = slot / (TX + 1), t = slot % (TX + 1);
+        const float m = cu.mu[u];
+        float v = xflux_lds<WENO, GATE>(lp, m, r, t, a.weps);
+        lfx[r][t] = v;
+        if (MODE == MODE_TENDENCY && a.fx && (t < TX || i0 + t == ni)) sto(a.fx + (int64_t)k * ust, muoff[u], v);
+      }
+    }
+    // y-face fluxes: rows j0..j0+TY of the TX columns.
+#pragma unroll
+    for (int u = 0; u < YS; ++u) {
+      if (mvoff[u] != NO_WORD) {
+        const int slot = threadIdx.x + u * NT;
+        const int r = slot / TX, t = slot % TX;
+        const float m = cu.mv[u];
+        float v = yflux_lds<WENO, GATE>(lp, m, r, t, a.weps);
+        lfy[r][t] = v;
+        if (MODE == MODE_TENDENCY && a.fy && (r < TY || j0 + r == nj)) sto(a.fy + (int64_t)k * vst, mvoff[u], v);
+      }
+    }
+    float fzt = 0.0f;
+    if (live && k + 1 <= nk) fzt = zf(k + 1, win + 1);  // S(k-2..)
+    __syncthreads();
+    if (MODE == MODE_TENDENCY && live && a.fz) {
+      if (k == 1) sto(a.fz + st, o_s, 0.0f);
+      sto(a.fz + (int64_t)(k + 1) * st, o_s, (k + 1 <= nk) ? fzt : 0.0f);
+      if (k == kb && kb >= 2) sto(a.fz + (int64_t)kb * st, o_s, fzb);
+    }
+    if (owned) {
+      float fw = lfx[ty][tx], fe = lfx[ty][tx + 1];
+      float fs = lfy[ty][tx], fn = lfy[ty + 1][tx];
+      const float sv = lp[ty + 3][tx + 3];
+      float ax, ay, az;
+      if (obc != 0u) {
+        const float uw = c_uw, ue = c_ue, vs = c_vs, vn = c_vn;
+        const bool iw = (i == 0) && side_of(obc, 0) == 2u && uw >= 0.0f;
+        const bool ie = (i + 1 == ni) && side_of(obc, 1) == 2u && ue <= 0.0f;
+        const bool js = (j == 0) && side_of(obc, 2) == 2u && vs >= 0.0f;
+        const bool jn = (j + 1 == nj) && side_of(obc, 3) == 2u && vn <= 0.0f;
+        if (iw) fw = fe;
+        if (ie) fe = fw;
+        if (js) fs = fn;
+        if (jn) fn = fs;
+        ax = FLAT ? (-(fe - fw)) * a.rdx : (fw - fe) * a.rdx;
+        ay = FLAT ? (-(fn - fs)) * a.rdy : (fs - fn) * a.rdy;
+        if (iw) ax = ax - (sv * (ue - uw)) * a.rdx;
+        if (ie) ax = ax - (sv * (ue - uw)) * a.rdx;
+        if (js) ay = ay - (sv * (vn - vs)) * a.rdy;
+        if (jn) ay = ay - (sv * (vn - vs)) * a.rdy;
+      } else {
+        ax = FLAT ? (-(fe - fw)) * a.rdx : (fw - fe) * a.rdx;
+        ay = FLAT ? (-(fn - fs)) * a.rdy : (fs - fn) * a.rdy;
+      }
+      if constexpr (FLAT) {
+        if (k == 1) az = ((-fzt) * a.rdsf) * a.mh[k];
+        else if (k == nk) az = (fzb * a.rdsf) * a.mh[k];
+        else az = ((-(fzt - fzb)) * a.rdsf) * a.mh[k];
+      } else {
+        if (k == 1) az = (-fzt) * a.rdsf;
+        else if (k == nk) az = fzb * a.rdsf;
+        else az = (fzb - fzt) * a.rdsf;
+      }
+      if (MODE == MODE_PDEF) {
+        sto(a.advx + lm, o_m, ax);
+        sto(a.advy + lm, o_m, ay);
+        sto(a.advz + lm, o_m, az);
+      } else {
+        float total = (ax + ay) + az;
+        total = total + sv * c_divx;
+        const float adv = FLAT ? total * c_rr0 : (total * c_rr0) * gzc;
+        if (MODE == MODE_TENDENCY) {
+          sto(a.tend + lm, o_m, adv);
+        } else {
+          const float seed = a.divterm ? c_held - c_divterm : c_held;
+          const float v = c_a0 + a.dttmp * (seed + adv);
+          sto(a.out + lm, o_m, (fabsf(v) < 1.0e-30f) ? 0.0f : v);
+        }
+      }
+    }
+    fzb = fzt;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) win[q] = win[q + 1];
+    win[6] = snext;
+    __syncthreads();
+  }
+}
+
+
 template <bool WENO, int MODE, bool FLAT, bool SRC>
 __global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
-  scalar_adv5_body<WENO, MODE, FLAT, SRC>(a);
+  scalar_adv5_body_held<WENO, MODE, FLAT, SRC>(a);
 }
 
 // The same body held to six waves per SIMD (three 512-lane workgroups per CU):
