@@ -299,6 +299,16 @@ GM1_DEV float packed_word(const float* __restrict__ src, const float* __restrict
   return th0 ? (th0[k - 1] - 300.0f) + v : v;
 }
 
+// The word at a byte offset from a base the whole workgroup shares: the load
+// takes the base in scalar registers and the offset in one vector register.
+GM1_DEV float ldo(const float* base, uint32_t byte_offset) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + byte_offset);
+}
+GM1_DEV void sto(float* base, uint32_t byte_offset, float v) {
+  *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + byte_offset) = v;
+}
+constexpr uint32_t NO_WORD = 0xFFFFFFFFu;  // a slot with no word to read
+
 // FLAT: the flat stretched column's text (a.mh set), fixed at compile time so
 // the terrain instance carries none of its branches.  SRC: the scalar is read
 // from its unpadded carrier (a.src) rather than a padded array (a.s); flux
@@ -326,9 +336,10 @@ __device__ __forceinline__ void scalar_adv5_body(const ScalarAdvArgs& a) {
   const bool owned = live && (obc != 0u || (i >= 3 && i + 3 < ni && j >= 3 && j + 3 < nj));
   const int x = i + NG, y = j + NG;
   const int64_t st = (int64_t)nys * nxs;  // level stride of scalar and w arrays
-  const float* col = SRC ? nullptr : a.s + (int64_t)y * nxs + x;
-  const int64_t wcol = (int64_t)y * nxs + x;
-  const int64_t mcol = (int64_t)j * ni + i;
+  // a level's words sit at 32-bit byte offsets from the level's base: the
+  // own column in the padded (wcol) and unpadded (mcol) families
+  const uint32_t wcol = (uint32_t)(y * nxs + x) * 4u;
+  const uint32_t mcol = (uint32_t)(j * ni + i) * 4u;
   const int64_t mst = (int64_t)nj * ni;
 
   // The column's scalar S(k-3..k+3) at level k rides in registers: win[q]
@@ -336,14 +347,13 @@ __device__ __forceinline__ void scalar_adv5_body(const ScalarAdvArgs& a) {
   // read by a face's stencil, which stays within levels 1..nk).
   float win[7];
   // the own column is a physical cell, so its unpadded word needs no lateral image
-  const float* scol = SRC ? a.src + (int64_t)j * ni + i : nullptr;
   auto colword = [&](int lev) {  // the column's word at a level, 0 where no stencil reads
     if constexpr (SRC) {
       if (lev < 1 || lev > nk) return 0.0f;
-      const float v = scol[(int64_t)(lev - 1) * mst];
+      const float v = ldo(a.src + (int64_t)(lev - 1) * mst, mcol);
       return a.th0 ? (a.th0[lev - 1] - 300.0f) + v : v;
     }
-    return (lev >= 0 && lev <= nk + 1) ? col[(int64_t)lev * st] : 0.0f;
+    return (lev >= 0 && lev <= nk + 1) ? ldo(a.s + (int64_t)lev * st, wcol) : 0.0f;
   };
   // a tile whose plane (3-cell halo included) lies inside the domain reads the
   // unpadded source directly; only edge tiles form the lateral family's image
@@ -353,12 +363,12 @@ __device__ __forceinline__ void scalar_adv5_body(const ScalarAdvArgs& a) {
   for (int q = 0; q < 7; ++q) win[q] = live ? colword(kb - 3 + q) : 0.0f;
   // z-face flux at face kf from the window b, b[0] = S(kf-3).
   auto zf = [&](int kf, const float* b) {
-    const int64_t wp = (int64_t)kf * st + wcol;
-    const int64_t zq = (int64_t)(kf - 1) * mst + (a.c12col ? 0 : mcol);
-    const float cc1 = a.c1[zq], cc2 = a.c2[zq];
+    const float m = ldo(a.rrw + (int64_t)kf * st, wcol);
+    const uint32_t zq = a.c12col ? 0u : mcol;
+    const float cc1 = ldo(a.c1 + (int64_t)(kf - 1) * mst, zq), cc2 = ldo(a.c2 + (int64_t)(kf - 1) * mst, zq);
     auto S = [&](int level) { return b[level - (kf - 3)]; };
-    if (MODE == MODE_PDEF) return zflux_pdef_g(S, a.rrw[wp], cc1, cc2, kf, nk, a.weps, a.pdefweno != 0);
-    return zflux_g<WENO>(S, a.rrw[wp], cc1, cc2, kf, nk, a.weps);
+    if (MODE == MODE_PDEF) return zflux_pdef_g(S, m, cc1, cc2, kf, nk, a.weps, a.pdefweno != 0);
+    return zflux_g<WENO>(S, m, cc1, cc2, kf, nk, a.weps);
   };
   float fzb = 0.0f;
   if (live && kb >= 2) fzb = zf(kb, win);  // S(kb-3..)
@@ -394,13 +404,13 @@ __device__ __forceinline__ void scalar_adv5_body(const ScalarAdvArgs& a) {
       if (slot < (TY + 6) * (TX + 6) && yy < nys && xx < nxs) {
         if constexpr (SRC) {
           if (inner) {
-            const float w = splane[r * ni + c];
+            const float w = ldo(splane, (uint32_t)(r * ni + c) * 4u);
             v.plane[u] = a.th0 ? pack + w : w;
           } else {
             v.plane[u] = packed_word(a.src, a.th0, k, yy, xx, ni, nj, obc);
           }
         } else {
-          v.plane[u] = plane[(int64_t)yy * nxs + xx];
+          v.plane[u] = ldo(plane, (uint32_t)(yy * nxs + xx) * 4u);
         }
       }
     }
@@ -410,7 +420,9 @@ __device__ __forceinline__ void scalar_adv5_body(const ScalarAdvArgs& a) {
       asm volatile("" : "+v"(slot));  // recomputed per level, not held across the march
       const int r = slot / (TX + 1), t = slot % (TX + 1);
       const int f = i0 + t, jj = j0 + r;
-      v.mu[u] = (slot < TY * (TX + 1) && f <= ni && jj < nj) ? a.rru[ix3(k, jj + NG, f + NG, nys, nxu)] : 0.0f;
+      v.mu[u] = (slot < TY * (TX + 1) && f <= ni && jj < nj)
+                    ? ldo(a.rru + (int64_t)k * nys * nxu, (uint32_t)((jj + NG) * nxu + f + NG) * 4u)
+                    : 0.0f;
     }
 #pragma unroll
     for (int u = 0; u < YS; ++u) {
@@ -418,7 +430,9 @@ __device__ __forceinline__ void scalar_adv5_body(const ScalarAdvArgs& a) {
       asm volatile("" : "+v"(slot));  // recomputed per level, not held across the march
       const int r = slot / TX, t = slot % TX;
       const int ii = i0 + t, g = j0 + r;
-      v.mv[u] = (slot < (TY + 1) * TX && ii < ni && g <= nj) ? a.rrv[ix3(k, g + NG, ii + NG, nyv, nxs)] : 0.0f;
+      v.mv[u] = (slot < (TY + 1) * TX && ii < ni && g <= nj)
+                    ? ldo(a.rrv + (int64_t)k * nyv * nxs, (uint32_t)((g + NG) * nxs + ii + NG) * 4u)
+                    : 0.0f;
     }
     return v;
   };
@@ -432,19 +446,22 @@ __device__ __forceinline__ void scalar_adv5_body(const ScalarAdvArgs& a) {
           c_held = 0.0f, c_divterm = 0.0f;
     if (owned) {
       if (obc != 0u) {
-        c_uw = a.rru[ix3(k, y, x, nys, nxu)];
-        c_ue = a.rru[ix3(k, y, x + 1, nys, nxu)];
-        c_vs = a.rrv[ix3(k, y, x, nyv, nxs)];
-        c_vn = a.rrv[ix3(k, y + 1, x, nyv, nxs)];
+        const float* pu = a.rru + (int64_t)k * nys * nxu;
+        const float* pv = a.rrv + (int64_t)k * nyv * nxs;
+        const uint32_t ou = (uint32_t)(y * nxu + x) * 4u;
+        c_uw = ldo(pu, ou);
+        c_ue = ldo(pu, ou + 4u);
+        c_vs = ldo(pv, wcol);
+        c_vn = ldo(pv, wcol + (uint32_t)nxs * 4u);
       }
       if (MODE != MODE_PDEF) {
-        const int64_t q = (int64_t)(k - 1) * mst + mcol;
-        c_divx = a.divx[q];
-        c_rr0 = FLAT ? a.rr0[(int64_t)(k - 1) * mst] : a.rr0[q];
+        const int64_t lm = (int64_t)(k - 1) * mst;
+        c_divx = ldo(a.divx + lm, mcol);
+        c_rr0 = FLAT ? a.rr0[lm] : ldo(a.rr0 + lm, mcol);
         if (MODE == MODE_COMPOSE) {
-          c_a0 = a.a0[q];
-          c_held = a.held[q];
-          if (a.divterm) c_divterm = a.divterm[q];
+          c_a0 = ldo(a.a0 + lm, mcol);
+          c_held = ldo(a.held + lm, mcol);
+          if (a.divterm) c_divterm = ldo(a.divterm + lm, mcol);
         }
       }
     }
@@ -465,7 +482,8 @@ __device__ __forceinline__ void scalar_adv5_body(const ScalarAdvArgs& a) {
         const float m = cu.mu[u];
         float v = xflux_lds<WENO, GATE>(lp, m, r, t, a.weps);
         lfx[r][t] = v;
-        if (MODE == MODE_TENDENCY && a.fx && (t < TX || f == ni)) a.fx[ix3(k, jj + NG, f + NG, nys, nxu)] = v;
+        if (MODE == MODE_TENDENCY && a.fx && (t < TX || f == ni))
+          sto(a.fx + (int64_t)k * nys * nxu, (uint32_t)((jj + NG) * nxu + f + NG) * 4u, v);
       }
     }
     // y-face fluxes: rows j0..j0+TY of the TX columns.
@@ -478,16 +496,17 @@ __device__ __forceinline__ void scalar_adv5_body(const ScalarAdvArgs& a) {
         const float m = cu.mv[u];
         float v = yflux_lds<WENO, GATE>(lp, m, r, t, a.weps);
         lfy[r][t] = v;
-        if (MODE == MODE_TENDENCY && a.fy && (r < TY || g == nj)) a.fy[ix3(k, g + NG, ii + NG, nyv, nxs)] = v;
+        if (MODE == MODE_TENDENCY && a.fy && (r < TY || g == nj))
+          sto(a.fy + (int64_t)k * nyv * nxs, (uint32_t)((g + NG) * nxs + ii + NG) * 4u, v);
       }
     }
     float fzt = 0.0f;
     if (live && k + 1 <= nk) fzt = zf(k + 1, win + 1);  // S(k-2..)
     __syncthreads();
     if (MODE == MODE_TENDENCY && live && a.fz) {
-      if (k == 1) a.fz[1 * st + wcol] = 0.0f;
-      a.fz[(int64_t)(k + 1) * st + wcol] = (k + 1 <= nk) ? fzt : 0.0f;
-      if (k == kb && kb >= 2) a.fz[(int64_t)kb * st + wcol] = fzb;
+      if (k == 1) sto(a.fz + st, wcol, 0.0f);
+      sto(a.fz + (int64_t)(k + 1) * st, wcol, (k + 1 <= nk) ? fzt : 0.0f);
+      if (k == kb && kb >= 2) sto(a.fz + (int64_t)kb * st, wcol, fzb);
     }
     if (owned) {
       float fw = lfx[ty][tx], fe = lfx[ty][tx + 1];
@@ -523,21 +542,21 @@ __device__ __forceinline__ void scalar_adv5_body(const ScalarAdvArgs& a) {
         else if (k == nk) az = fzb * a.rdsf;
         else az = (fzb - fzt) * a.rdsf;
       }
-      const int64_t q = (int64_t)(k - 1) * mst + mcol;
+      const int64_t lm = (int64_t)(k - 1) * mst;
       if (MODE == MODE_PDEF) {
-        a.advx[q] = ax;
-        a.advy[q] = ay;
-        a.advz[q] = az;
+        sto(a.advx + lm, mcol, ax);
+        sto(a.advy + lm, mcol, ay);
+        sto(a.advz + lm, mcol, az);
       } else {
         float total = (ax + ay) + az;
         total = total + sv * c_divx;
-        const float adv = FLAT ? total * c_rr0 : (total * c_rr0) * a.gz[mcol];
+        const float adv = FLAT ? total * c_rr0 : (total * c_rr0) * ldo(a.gz, mcol);
         if (MODE == MODE_TENDENCY) {
-          a.tend[q] = adv;
+          sto(a.tend + lm, mcol, adv);
         } else {
           const float seed = a.divterm ? c_held - c_divterm : c_held;
           const float v = c_a0 + a.dttmp * (seed + adv);
-          a.out[q] = (fabsf(v) < 1.0e-30f) ? 0.0f : v;
+          sto(a.out + lm, mcol, (fabsf(v) < 1.0e-30f) ? 0.0f : v);
         }
       }
     }
@@ -550,19 +569,12 @@ __device__ __forceinline__ void scalar_adv5_body(const ScalarAdvArgs& a) {
 }
 
 
-// The word at a byte offset from a base the whole workgroup shares: the load
-// takes the base in scalar registers and the offset in one vector register.
-GM1_DEV float ldo(const float* base, uint32_t byte_offset) {
-  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + byte_offset);
-}
-GM1_DEV void sto(float* base, uint32_t byte_offset, float v) {
-  *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + byte_offset) = v;
-}
-constexpr uint32_t NO_WORD = 0xFFFFFFFFu;  // a slot with no word to read
-
 // The same statements as scalar_adv5_body with another addressing, for the
 // instances that run at four waves per SIMD (the WENO forms and the 5th-order
-// tendency).  Addressing: every word a thread reads or writes at level k sits at a byte
+// tendency; measured at 640x768x80, the WENO compose 1200 -> 1054 us and the
+// pdef tendencies 773 -> 716 us against the recomputed offsets, while the
+// six-wave compose, whose registers these offsets would crowd, is faster with
+// the recomputed ones).  Addressing: every word a thread reads or writes at level k sits at a byte
 // offset, fixed for the thread, from a base that depends on k alone (the
 // level's plane of that array).  The offsets -- the tile's level-plane
 // slots, face mass-flux slots, the own cell and faces -- are formed once
