@@ -299,14 +299,6 @@ GM1_DEV float packed_word(const float* __restrict__ src, const float* __restrict
   return th0 ? (th0[k - 1] - 300.0f) + v : v;
 }
 
-// The word at a byte offset from a base the whole workgroup shares: the load
-// takes the base in scalar registers and the offset in one vector register.
-GM1_DEV float ldo(const float* base, uint32_t byte_offset) {
-  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + byte_offset);
-}
-GM1_DEV void sto(float* base, uint32_t byte_offset, float v) {
-  *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + byte_offset) = v;
-}
 constexpr uint32_t NO_WORD = 0xFFFFFFFFu;  // a slot with no word to read
 
 // FLAT: the flat stretched column's text (a.mh set), fixed at compile time so

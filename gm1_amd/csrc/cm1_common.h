@@ -46,6 +46,17 @@ GM1_DEV int xcd_tile(int linear, int n) {
   return (x < rem) ? x * (per + 1) + r : rem * (per + 1) + (x - rem) * per + r;
 }
 
+// The word at a 32-bit byte offset from a base the whole workgroup shares:
+// the load takes the base in scalar registers and the offset in one vector
+// register (the hardware adds them), so a thread's level-k address costs no
+// 64-bit vector arithmetic.  The offset must stay below 4 GiB.
+GM1_DEV float ldo(const float* base, uint32_t byte_offset) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + byte_offset);
+}
+GM1_DEV void sto(float* base, uint32_t byte_offset, float v) {
+  *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + byte_offset) = v;
+}
+
 // Pressure family (nk+2, nj+2, ni+2).
 GM1_DEV int64_t pix(int k, int j, int i, int ni, int nj) {
   return ((int64_t)k * (nj + 2) + j) * (int64_t)(ni + 2) + i;

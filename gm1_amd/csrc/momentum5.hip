@@ -208,19 +208,23 @@ GM1_DEV float uvz_g(const MomArgs& a, int kf, int j, int i, Get Q) {
   const int ni = a.ni, nj = a.nj, nk = a.nk;
   const int nxs = sx(ni), nys = sy(nj);
   float m, c2h;
+  // the level's planes, each word at a 32-bit byte offset
   const float* c2k = a.c2 + (int64_t)(kf - 1) * nj * ni;
+  const float* R = a.rrw + (int64_t)kf * nys * nxs;
   if (STAG == 0) {
     const bool pw = per(a.obc, 0), pe = per(a.obc, 1);
-    const float* R = a.rrw + ix3(kf, j + NG, 0, nys, nxs);
-    m = 0.5f * (R[wc(i, ni, pw, pe) + NG] + R[wc(i - 1, ni, pw, pe) + NG]);
+    const int rb = (j + NG) * nxs + NG;
+    m = 0.5f * (ldo(R, (uint32_t)(rb + wc(i, ni, pw, pe)) * 4u) + ldo(R, (uint32_t)(rb + wc(i - 1, ni, pw, pe)) * 4u));
     const int jc = clampi(j, nj);
-    c2h = 0.5f * (c2k[(int64_t)jc * ni + clampi(i - 1, ni)] + c2k[(int64_t)jc * ni + clampi(i, ni)]);
+    c2h = 0.5f * (ldo(c2k, (uint32_t)(jc * ni + clampi(i - 1, ni)) * 4u) +
+                  ldo(c2k, (uint32_t)(jc * ni + clampi(i, ni)) * 4u));
   } else {
     const bool ps = per(a.obc, 2), pn = per(a.obc, 3);
-    m = 0.5f * (a.rrw[ix3(kf, wc(j, nj, ps, pn) + NG, i + NG, nys, nxs)] +
-                a.rrw[ix3(kf, wc(j - 1, nj, ps, pn) + NG, i + NG, nys, nxs)]);
+    m = 0.5f * (ldo(R, (uint32_t)((wc(j, nj, ps, pn) + NG) * nxs + i + NG) * 4u) +
+                ldo(R, (uint32_t)((wc(j - 1, nj, ps, pn) + NG) * nxs + i + NG) * 4u));
     const int ic = clampi(i, ni);
-    c2h = 0.5f * (c2k[(int64_t)clampi(j - 1, nj) * ni + ic] + c2k[(int64_t)clampi(j, nj) * ni + ic]);
+    c2h = 0.5f * (ldo(c2k, (uint32_t)(clampi(j - 1, nj) * ni + ic) * 4u) +
+                  ldo(c2k, (uint32_t)(clampi(j, nj) * ni + ic) * 4u));
   }
   const float c1h = 1.0f - c2h;
   float r;
@@ -286,16 +290,17 @@ __global__ __launch_bounds__(MNT) void k_mom_advu_tile(MomArgs a, int f0, int nf
   const int fend = f0 + nf;  // one past the last face
   const bool pw = per(a.obc, 0), pe = per(a.obc, 1);
   const int nxu = ux(ni), nyu = sy(nj), nxv = sx(ni), nyv = vy(nj);
-  const int64_t ust = (int64_t)nyu * nxu;
+  const int64_t ust = (int64_t)nyu * nxu, vst = (int64_t)nyv * nxv, mst = (int64_t)nj * ni;
   const int f = fa + tx, j = ja + ty;
   const bool live = f < fend && j < nj;
   const int y = j + NG, x = f + NG;
-  const float* col = a.u + (int64_t)y * nxu + x;
+  // a level's words sit at 32-bit byte offsets from the level's base
+  const uint32_t ocol = (uint32_t)(y * nxu + x) * 4u;  // this face in the u family
   float win[7];  // u(k-3 .. k+3) of this face's column
 #pragma unroll
   for (int q = 0; q < 7; ++q) {
     const int lev = kb - 3 + q;
-    win[q] = (live && lev >= 0 && lev <= nk + 1) ? col[(int64_t)lev * ust] : 0.0f;
+    win[q] = (live && lev >= 0 && lev <= nk + 1) ? ldo(a.u + (int64_t)lev * ust, ocol) : 0.0f;
   }
   auto zf = [&](int kf, const float* b) {  // b[0] = u(kf - 3)
     return uvz_g<0>(a, kf, j, f, [&](int lev) { return b[lev - (kf - 3)]; });
@@ -314,7 +319,7 @@ __global__ __launch_bounds__(MNT) void k_mom_advu_tile(MomArgs a, int f0, int nf
   auto fetch = [&](int k) {
     Lv v;
     const int lnext = k + 4;
-    v.unext = (live && lnext <= nk + 1) ? col[(int64_t)lnext * ust] : 0.0f;
+    v.unext = (live && lnext <= nk + 1) ? ldo(a.u + (int64_t)lnext * ust, ocol) : 0.0f;
     const float* plane = a.u + (int64_t)k * ust;
 #pragma unroll
     for (int q = 0; q < PS; ++q) {
@@ -322,7 +327,8 @@ __global__ __launch_bounds__(MNT) void k_mom_advu_tile(MomArgs a, int f0, int nf
       asm volatile("" : "+v"(sl));  // recomputed per level, not held across the march
       const int r = sl / (MTX + 6), c = sl % (MTX + 6);
       const int yy = ja + r, xx = fa + c;
-      v.plane[q] = (sl < (MTY + 6) * (MTX + 6) && yy < nyu && xx < nxu) ? plane[(int64_t)yy * nxu + xx] : 0.0f;
+      v.plane[q] = (sl < (MTY + 6) * (MTX + 6) && yy < nyu && xx < nxu) ? ldo(plane, (uint32_t)(yy * nxu + xx) * 4u)
+                                                                          : 0.0f;
     }
 #pragma unroll
     for (int q = 0; q < XS; ++q) {
@@ -332,9 +338,10 @@ __global__ __launch_bounds__(MNT) void k_mom_advu_tile(MomArgs a, int f0, int nf
       const int cc = fa - 1 + t, jj = ja + r;
       v.rx0[q] = v.rx1[q] = 0.0f;
       if (sl < MTY * (MTX + 1) && jj < nj && cc < fend) {
-        const float* R = a.rru + ix3(k, jj + NG, 0, nyu, nxu);
-        v.rx0[q] = R[wf(cc, ni, pw, pe) + NG];
-        v.rx1[q] = R[wf(cc + 1, ni, pw, pe) + NG];
+        const float* R = a.rru + (int64_t)k * ust;
+        const int rb = (jj + NG) * nxu + NG;
+        v.rx0[q] = ldo(R, (uint32_t)(rb + wf(cc, ni, pw, pe)) * 4u);
+        v.rx1[q] = ldo(R, (uint32_t)(rb + wf(cc + 1, ni, pw, pe)) * 4u);
       }
     }
 #pragma unroll
@@ -345,9 +352,10 @@ __global__ __launch_bounds__(MNT) void k_mom_advu_tile(MomArgs a, int f0, int nf
       const int p = ja + r, ff = fa + t;
       v.ry0[q] = v.ry1[q] = 0.0f;
       if (sl < (MTY + 1) * MTX && p <= nj && ff < fend) {
-        const float* R = a.rrv + ix3(k, p + NG, 0, nyv, nxv);
-        v.ry0[q] = R[wc(ff, ni, pw, pe) + NG];
-        v.ry1[q] = R[wc(ff - 1, ni, pw, pe) + NG];
+        const float* R = a.rrv + (int64_t)k * vst;
+        const int rb = (p + NG) * nxv + NG;
+        v.ry0[q] = ldo(R, (uint32_t)(rb + wc(ff, ni, pw, pe)) * 4u);
+        v.ry1[q] = ldo(R, (uint32_t)(rb + wc(ff - 1, ni, pw, pe)) * 4u);
       }
     }
     return v;
@@ -361,17 +369,18 @@ __global__ __launch_bounds__(MNT) void k_mom_advu_tile(MomArgs a, int f0, int nf
     float e_dxe = 0.0f, e_dxw = 0.0f, e_sc0 = 0.0f, e_rw = 0.0f, e_re = 0.0f, e_ten = 0.0f;
     if (live) {
       const int west = wc(f - 1, ni, pw, pe), east = wc(f, ni, pw, pe);
-      const int64_t row = ((int64_t)(k - 1) * nj + j) * ni;
-      e_dxe = a.divx[row + east];
-      e_dxw = a.divx[row + west];
+      const int64_t lm = (int64_t)(k - 1) * mst;
+      const uint32_t ow = (uint32_t)(j * ni + west) * 4u, oe = (uint32_t)(j * ni + east) * 4u;
+      e_dxe = ldo(a.divx + lm, oe);
+      e_dxw = ldo(a.divx + lm, ow);
       if (FLAT) {
-        e_sc0 = a.rr0[(int64_t)(k - 1) * nj * ni];
+        e_sc0 = a.rr0[lm];
       } else {
         e_sc0 = a.gzu[(int64_t)j * (ni + 1) + f];
-        e_rw = a.rho0[row + west];
-        e_re = a.rho0[row + east];
+        e_rw = ldo(a.rho0 + lm, ow);
+        e_re = ldo(a.rho0 + lm, oe);
       }
-      e_ten = (a.u1 ? a.u1 : a.uten)[ix3(k, y, x, nyu, nxu)];
+      e_ten = ldo((a.u1 ? a.u1 : a.uten) + (int64_t)k * ust, ocol);
     }
 #pragma unroll
     for (int q = 0; q < PS; ++q) {
@@ -410,8 +419,10 @@ __global__ __launch_bounds__(MNT) void k_mom_advu_tile(MomArgs a, int f0, int nf
         if (pb == p) {
           m = 0.5f * mt;
         } else {
-          const float* Rb = a.rrv + ix3(k, pb + NG, 0, nyv, nxv);
-          m = 0.5f * (Rb[wc(ff, ni, pw, pe) + NG] + Rb[wc(ff - 1, ni, pw, pe) + NG]);
+          const float* Rb = a.rrv + (int64_t)k * vst;
+          const int rb = (pb + NG) * nxv + NG;
+          m = 0.5f * (ldo(Rb, (uint32_t)(rb + wc(ff, ni, pw, pe)) * 4u) +
+                      ldo(Rb, (uint32_t)(rb + wc(ff - 1, ni, pw, pe)) * 4u));
         }
         const int r0 = pb - ja;  // LDS row of u at padded row pb + NG - 3
         const int c = t + 3;
@@ -453,12 +464,9 @@ __global__ __launch_bounds__(MNT) void k_mom_advu_tile(MomArgs a, int f0, int nf
       else az = (-(zhi - zlo)) * a.rdz;
       if constexpr (FLAT) az = az * a.flat[k];
       float tot = (ax + ay) + az;
-      const int k0 = k - 1;
-      const int64_t row = ((int64_t)k0 * nj + j) * ni;
       tot = tot + uc * (0.5f * (e_dxe + e_dxw));
       const float sc = FLAT ? e_sc0 : e_sc0 / (0.5f * (e_rw + e_re));
-      const int64_t pidx = ix3(k, y, x, nyu, nxu);
-      a.uten[pidx] = e_ten + tot * sc;
+      sto(a.uten + (int64_t)k * ust, ocol, e_ten + tot * sc);
     }
     zlo = zhi;
 #pragma unroll
@@ -483,16 +491,17 @@ __global__ __launch_bounds__(MNT) void k_mom_advv_tile(MomArgs a, int g0, int nr
   const int gend = g0 + nr;
   const bool ps = per(a.obc, 2), pn = per(a.obc, 3);
   const int nxu = ux(ni), nyu = sy(nj), nxv = sx(ni), nyv = vy(nj);
-  const int64_t vst = (int64_t)nyv * nxv;
+  const int64_t vst = (int64_t)nyv * nxv, ust = (int64_t)nyu * nxu, mst = (int64_t)nj * ni;
   const int i = ia + tx, g = ga + ty;
   const bool live = i < ni && g < gend;
   const int y = g + NG, x = i + NG;
-  const float* col = a.v + (int64_t)y * nxv + x;
+  // a level's words sit at 32-bit byte offsets from the level's base
+  const uint32_t ocol = (uint32_t)(y * nxv + x) * 4u;  // this face in the v family
   float win[7];
 #pragma unroll
   for (int q = 0; q < 7; ++q) {
     const int lev = kb - 3 + q;
-    win[q] = (live && lev >= 0 && lev <= nk + 1) ? col[(int64_t)lev * vst] : 0.0f;
+    win[q] = (live && lev >= 0 && lev <= nk + 1) ? ldo(a.v + (int64_t)lev * vst, ocol) : 0.0f;
   }
   auto zf = [&](int kf, const float* b) {
     return uvz_g<1>(a, kf, g, i, [&](int lev) { return b[lev - (kf - 3)]; });
@@ -510,7 +519,7 @@ __global__ __launch_bounds__(MNT) void k_mom_advv_tile(MomArgs a, int g0, int nr
   auto fetch = [&](int k) {
     Lv v;
     const int lnext = k + 4;
-    v.vnext = (live && lnext <= nk + 1) ? col[(int64_t)lnext * vst] : 0.0f;
+    v.vnext = (live && lnext <= nk + 1) ? ldo(a.v + (int64_t)lnext * vst, ocol) : 0.0f;
     const float* plane = a.v + (int64_t)k * vst;
 #pragma unroll
     for (int q = 0; q < PS; ++q) {
@@ -518,7 +527,8 @@ __global__ __launch_bounds__(MNT) void k_mom_advv_tile(MomArgs a, int g0, int nr
       asm volatile("" : "+v"(sl));  // recomputed per level, not held across the march
       const int r = sl / (MTX + 6), c = sl % (MTX + 6);
       const int yy = ga + r, xx = ia + c;
-      v.plane[q] = (sl < (MTY + 6) * (MTX + 6) && yy < nyv && xx < nxv) ? plane[(int64_t)yy * nxv + xx] : 0.0f;
+      v.plane[q] = (sl < (MTY + 6) * (MTX + 6) && yy < nyv && xx < nxv) ? ldo(plane, (uint32_t)(yy * nxv + xx) * 4u)
+                                                                          : 0.0f;
     }
 #pragma unroll
     for (int q = 0; q < XS; ++q) {
@@ -529,8 +539,9 @@ __global__ __launch_bounds__(MNT) void k_mom_advv_tile(MomArgs a, int g0, int nr
       v.rx0[q] = v.rx1[q] = 0.0f;
       if (sl < MTY * (MTX + 1) && gg < gend && p <= ni) {
         const int rn = wc(gg, nj, ps, pn) + NG, rs = wc(gg - 1, nj, ps, pn) + NG;
-        v.rx0[q] = a.rru[ix3(k, rn, p + NG, nyu, nxu)];
-        v.rx1[q] = a.rru[ix3(k, rs, p + NG, nyu, nxu)];
+        const float* R = a.rru + (int64_t)k * ust;
+        v.rx0[q] = ldo(R, (uint32_t)(rn * nxu + p + NG) * 4u);
+        v.rx1[q] = ldo(R, (uint32_t)(rs * nxu + p + NG) * 4u);
       }
     }
 #pragma unroll
@@ -541,8 +552,9 @@ __global__ __launch_bounds__(MNT) void k_mom_advv_tile(MomArgs a, int g0, int nr
       const int pc = ga - 1 + r, ii = ia + t;
       v.ry0[q] = v.ry1[q] = 0.0f;
       if (sl < (MTY + 1) * MTX && ii < ni && pc < gend) {
-        v.ry0[q] = a.rrv[ix3(k, wf(pc, nj, ps, pn) + NG, ii + NG, nyv, nxv)];
-        v.ry1[q] = a.rrv[ix3(k, wf(pc + 1, nj, ps, pn) + NG, ii + NG, nyv, nxv)];
+        const float* R = a.rrv + (int64_t)k * vst;
+        v.ry0[q] = ldo(R, (uint32_t)((wf(pc, nj, ps, pn) + NG) * nxv + ii + NG) * 4u);
+        v.ry1[q] = ldo(R, (uint32_t)((wf(pc + 1, nj, ps, pn) + NG) * nxv + ii + NG) * 4u);
       }
     }
     return v;
@@ -556,17 +568,18 @@ __global__ __launch_bounds__(MNT) void k_mom_advv_tile(MomArgs a, int g0, int nr
     float e_dxn = 0.0f, e_dxs = 0.0f, e_sc0 = 0.0f, e_rs = 0.0f, e_rn = 0.0f, e_ten = 0.0f;
     if (live) {
       const int south = wc(g - 1, nj, ps, pn), north = wc(g, nj, ps, pn);
-      const int64_t plane = (int64_t)(k - 1) * nj * ni;
-      e_dxn = a.divx[plane + (int64_t)north * ni + i];
-      e_dxs = a.divx[plane + (int64_t)south * ni + i];
+      const int64_t lm = (int64_t)(k - 1) * mst;
+      const uint32_t on = (uint32_t)(north * ni + i) * 4u, os = (uint32_t)(south * ni + i) * 4u;
+      e_dxn = ldo(a.divx + lm, on);
+      e_dxs = ldo(a.divx + lm, os);
       if (FLAT) {
-        e_sc0 = a.rr0[plane];
+        e_sc0 = a.rr0[lm];
       } else {
         e_sc0 = a.gzv[(int64_t)g * ni + i];
-        e_rs = a.rho0[plane + (int64_t)south * ni + i];
-        e_rn = a.rho0[plane + (int64_t)north * ni + i];
+        e_rs = ldo(a.rho0 + lm, os);
+        e_rn = ldo(a.rho0 + lm, on);
       }
-      e_ten = (a.v1 ? a.v1 : a.vten)[ix3(k, y, x, nyv, nxv)];
+      e_ten = ldo((a.v1 ? a.v1 : a.vten) + (int64_t)k * vst, ocol);
     }
 #pragma unroll
     for (int q = 0; q < PS; ++q) {
@@ -588,9 +601,10 @@ __global__ __launch_bounds__(MNT) void k_mom_advv_tile(MomArgs a, int g0, int nr
         int pb = p;
         if (p == 0 && opn(a.obc, 0) && mt >= 0.0f) pb = 1;
         else if (p == ni && opn(a.obc, 1) && mt <= 0.0f) pb = ni - 1;
-        const float m =
-            (pb == p) ? 0.5f * mt
-                      : 0.5f * (a.rru[ix3(k, rn, pb + NG, nyu, nxu)] + a.rru[ix3(k, rs, pb + NG, nyu, nxu)]);
+        const float* R = a.rru + (int64_t)k * ust;
+        const float m = (pb == p) ? 0.5f * mt
+                                  : 0.5f * (ldo(R, (uint32_t)(rn * nxu + pb + NG) * 4u) +
+                                            ldo(R, (uint32_t)(rs * nxu + pb + NG) * 4u));
         const float* L = lp[r + 3] + (pb - ia);  // L[c] = v at padded x (pb + NG) - 3 + c
         const float rv = (m >= 0.0f) ? flx5(L[0], L[1], L[2], L[3], L[4]) : flx5(L[5], L[4], L[3], L[2], L[1]);
         lfx[r][t] = m * rv;
@@ -640,12 +654,9 @@ __global__ __launch_bounds__(MNT) void k_mom_advv_tile(MomArgs a, int g0, int nr
       else az = (-(zhi - zlo)) * a.rdz;
       if constexpr (FLAT) az = az * a.flat[k];
       float tot = (ax + ay) + az;
-      const int k0 = k - 1;
-      const int64_t plane = (int64_t)k0 * nj * ni;
       tot = tot + vc * (0.5f * (e_dxn + e_dxs));
       const float sc = FLAT ? e_sc0 : e_sc0 / (0.5f * (e_rs + e_rn));
-      const int64_t pidx = ix3(k, y, x, nyv, nxv);
-      a.vten[pidx] = e_ten + tot * sc;
+      sto(a.vten + (int64_t)k * vst, ocol, e_ten + tot * sc);
     }
     zlo = zhi;
 #pragma unroll
@@ -850,9 +861,11 @@ void k_mom_advw_tile(MomArgs a, int kchunk, int nchunks) {
   const bool live = i < ni && j < nj;
   const int nxu = ux(ni), nyu = sy(nj), nxs = sx(ni), nys = sy(nj), nyv = vy(nj);
   const int64_t wst = (int64_t)nys * nxs, plane = (int64_t)nj * ni;
+  const int64_t ust = (int64_t)nyu * nxu, vst = (int64_t)nyv * nxs;
   const int y = j + NG, x = i + NG;
   const int64_t h = (int64_t)j * ni + i;
-  const int64_t wcol = (int64_t)y * nxs + x;
+  // a level's words sit at 32-bit byte offsets from the level's base
+  const uint32_t oh = (uint32_t)(j * ni + i) * 4u, ow = (uint32_t)(y * nxs + x) * 4u;
   constexpr int PS = (WLY * WLX + MNT - 1) / MNT;
   constexpr int CS = ((MTY + 2) * (MTX + 2) + MNT - 1) / MNT;
   constexpr int XS = (MTY * (MTX + 1) + MNT - 1) / MNT;
@@ -862,20 +875,23 @@ void k_mom_advw_tile(MomArgs a, int kchunk, int nchunks) {
     const int sl = threadIdx.x + q * MNT;
     const int r = sl / (MTX + 1), t = sl % (MTX + 1);
     const int pf = ia + t, jj = ja + r;
-    return (sl < MTY * (MTX + 1) && jj < nj && pf <= ni) ? a.rru[ix3(k, jj + NG, pf + NG, nyu, nxu)] : 0.0f;
+    return (sl < MTY * (MTX + 1) && jj < nj && pf <= ni)
+               ? ldo(a.rru + (int64_t)k * ust, (uint32_t)((jj + NG) * nxu + pf + NG) * 4u)
+               : 0.0f;
   };
   auto ry_at = [&](int k, int q) {
     const int sl = threadIdx.x + q * MNT;
     const int r = sl / MTX, t = sl % MTX;
     const int pf = ja + r, ii = ia + t;
-    return (sl < (MTY + 1) * MTX && ii < ni && pf <= nj) ? a.rrv[ix3(k, pf + NG, ii + NG, nyv, nxs)] : 0.0f;
+    return (sl < (MTY + 1) * MTX && ii < ni && pf <= nj)
+               ? ldo(a.rrv + (int64_t)k * vst, (uint32_t)((pf + NG) * nxs + ii + NG) * 4u)
+               : 0.0f;
   };
-  const float* colw = a.w + wcol;
   float win[7];  // w(kf-3 .. kf+3) of this cell's column
 #pragma unroll
   for (int q = 0; q < 7; ++q) {
     const int lev = kb - 3 + q;
-    win[q] = (live && lev >= 0 && lev <= nk + 2) ? colw[(int64_t)lev * wst] : 0.0f;
+    win[q] = (live && lev >= 0 && lev <= nk + 2) ? ldo(a.w + (int64_t)lev * wst, ow) : 0.0f;
   }
   float rxp[XS], ryp[YS];
 #pragma unroll
@@ -884,11 +900,11 @@ void k_mom_advw_tile(MomArgs a, int kchunk, int nchunks) {
   for (int q = 0; q < YS; ++q) ryp[q] = ry_at(kb - 1, q);
   float rrw_lo = 0.0f, rrw_hi = 0.0f, zlo = 0.0f, dxlo = 0.0f, d8lo = 0.0f;
   if (live) {
-    rrw_lo = a.rrw[(int64_t)(kb - 1) * wst + wcol];
-    rrw_hi = a.rrw[(int64_t)kb * wst + wcol];
+    rrw_lo = ldo(a.rrw + (int64_t)(kb - 1) * wst, ow);
+    rrw_hi = ldo(a.rrw + (int64_t)kb * wst, ow);
     zlo = wz_g(0.5f * (rrw_lo + rrw_hi), kb - 1, nk, [&](int lev) { return win[lev - (kb - 3)]; });
-    dxlo = a.divx[(int64_t)(kb - 2) * plane + h];
-    if (FLAT && a.dum8) d8lo = a.dum8[(int64_t)(kb - 1) * wst + wcol];
+    dxlo = ldo(a.divx + (int64_t)(kb - 2) * plane, oh);
+    if (FLAT && a.dum8) d8lo = ldo(a.dum8 + (int64_t)(kb - 1) * wst, ow);
   }
   struct Lv {
     float plane[PS], c2[CS], rx[XS], ry[YS];
@@ -903,7 +919,7 @@ void k_mom_advw_tile(MomArgs a, int kchunk, int nchunks) {
       asm volatile("" : "+v"(sl));  // recomputed per level, not held across the march
       const int r = sl / WLX, c = sl % WLX;
       const int yy = ja + r, xx = ia + c;
-      v.plane[q] = (sl < WLY * WLX && yy < nys && xx < nxs) ? wp[(int64_t)yy * nxs + xx] : 0.0f;
+      v.plane[q] = (sl < WLY * WLX && yy < nys && xx < nxs) ? ldo(wp, (uint32_t)(yy * nxs + xx) * 4u) : 0.0f;
     }
     const float* c2k = a.c2 + (int64_t)(kf - 1) * plane;
 #pragma unroll
@@ -911,8 +927,9 @@ void k_mom_advw_tile(MomArgs a, int kchunk, int nchunks) {
       int sl = threadIdx.x + q * MNT;
       asm volatile("" : "+v"(sl));
       const int r = sl / (MTX + 2), c = sl % (MTX + 2);
-      v.c2[q] = (sl < (MTY + 2) * (MTX + 2)) ? c2k[(int64_t)clampi(ja - 1 + r, nj) * ni + clampi(ia - 1 + c, ni)]
-                                              : 0.0f;
+      v.c2[q] = (sl < (MTY + 2) * (MTX + 2))
+                    ? ldo(c2k, (uint32_t)(clampi(ja - 1 + r, nj) * ni + clampi(ia - 1 + c, ni)) * 4u)
+                    : 0.0f;
     }
 #pragma unroll
     for (int q = 0; q < XS; ++q) v.rx[q] = rx_at(kf, q);
@@ -921,13 +938,13 @@ void k_mom_advw_tile(MomArgs a, int kchunk, int nchunks) {
     v.wnext = v.rrwn = v.divx = v.base = v.d8 = v.c1 = 0.0f;
     if (live) {
       const int lnext = kf + 4;
-      v.wnext = (lnext <= nk + 2) ? colw[(int64_t)lnext * wst] : 0.0f;
-      v.rrwn = a.rrw[(int64_t)(kf + 1) * wst + wcol];
-      v.divx = a.divx[(int64_t)(kf - 1) * plane + h];
-      v.base = (a.w1 ? a.w1 : a.wten)[(int64_t)kf * wst + wcol];
+      v.wnext = (lnext <= nk + 2) ? ldo(a.w + (int64_t)lnext * wst, ow) : 0.0f;
+      v.rrwn = ldo(a.rrw + (int64_t)(kf + 1) * wst, ow);
+      v.divx = ldo(a.divx + (int64_t)(kf - 1) * plane, oh);
+      v.base = ldo((a.w1 ? a.w1 : a.wten) + (int64_t)kf * wst, ow);
       if (FLAT && a.dum8) {
-        v.d8 = a.dum8[(int64_t)kf * wst + wcol];
-        v.c1 = a.c1[(int64_t)(kf - 1) * plane + h];
+        v.d8 = ldo(a.dum8 + (int64_t)kf * wst, ow);
+        v.c1 = ldo(a.c1 + (int64_t)(kf - 1) * plane, oh);
       }
     }
     return v;
@@ -1037,11 +1054,12 @@ void k_mom_advw_tile(MomArgs a, int kchunk, int nchunks) {
       if (FLAT) az = az * a.flat[(nk + 2) + kf];
       float tot = (ax + ay) + az;
       tot = tot + wc0 * (c2 * cu.divx + c1 * dxlo);
-      const float sc = FLAT ? a.rrf0[(int64_t)(kf - 1) * plane] : a.rrf0[(int64_t)(kf - 1) * plane + h] * a.gz[h];
-      const int64_t p = (int64_t)kf * wst + wcol;
+      const float sc = FLAT ? a.rrf0[(int64_t)(kf - 1) * plane]
+                            : ldo(a.rrf0 + (int64_t)(kf - 1) * plane, oh) * a.gz[h];
+      float* wt_k = a.wten + (int64_t)kf * wst;
       const float wt = cu.base + tot * sc;
-      if (FLAT && a.dum8) a.wten[p] = wt + (cu.c1 * d8lo + c2 * cu.d8);
-      else a.wten[p] = wt;
+      if (FLAT && a.dum8) sto(wt_k, ow, wt + (cu.c1 * d8lo + c2 * cu.d8));
+      else sto(wt_k, ow, wt);
       d8lo = cu.d8;
     }
     zlo = zhi;

@@ -684,12 +684,6 @@ int column2_lds_bytes(int nk, int cw, bool pipe = false, bool flat_pipe = false)
          (flat_pipe ? (nk + 1) * FLAT_COEF * (int)sizeof(float) : 0);
 }
 
-// The word at a 32-bit byte offset from an array base: the load takes the
-// base in scalar registers and the offset in one vector register.
-GM1_DEV float ld32(const float* base, uint32_t byte_offset) {
-  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + byte_offset);
-}
-
 // rru(k) += u(k) and rrv(k) += v(k) on one column's west u face and south v
 // face for levels kb..ke (divided by nloop on the last small step); the words
 // of four levels are requested before any is written.
@@ -872,37 +866,37 @@ __device__ __forceinline__ void column2_body(const SoundArgs& a, int n, int tile
           const uint32_t ov = (uint32_t)((int64_t)k * vst + vc) * 4u;
           const uint32_t ow = (uint32_t)((int64_t)k * wst + wc) * 4u;
           In v;
-          v.uo = ld32(a.u3d, ou);
-          v.ut = ld32(a.uten, ou);
-          v.vo = ld32(a.v3d, ov);
-          v.vt = ld32(a.vten, ov);
-          v.von = ld32(a.v3d, ov + onx);
-          v.vtn = ld32(a.vten, ov + onx);
-          v.dc = ld32(ppd, op);
-          v.dw = ld32(ppd, op - 4u);
-          v.ds = ld32(ppd, op - onp);
-          v.dn = ld32(ppd, op + onp);
-          v.thw = ld32(thv, op - 4u);
-          v.ths = ld32(thv, op - onp);
-          v.thn = ld32(thv, op + onp);
+          v.uo = ldo(a.u3d, ou);
+          v.ut = ldo(a.uten, ou);
+          v.vo = ldo(a.v3d, ov);
+          v.vt = ldo(a.vten, ov);
+          v.von = ldo(a.v3d, ov + onx);
+          v.vtn = ldo(a.vten, ov + onx);
+          v.dc = ldo(ppd, op);
+          v.dw = ldo(ppd, op - 4u);
+          v.ds = ldo(ppd, op - onp);
+          v.dn = ldo(ppd, op + onp);
+          v.thw = ldo(thv, op - 4u);
+          v.ths = ldo(thv, op - onp);
+          v.thn = ldo(thv, op + onp);
           v.uoe = 0.0f; v.ute = 0.0f; v.de = 0.0f; v.the = 0.0f;
           if (efb) {
-            v.uoe = ld32(a.u3d, ou + 4u);
-            v.ute = ld32(a.uten, ou + 4u);
-            v.de = ld32(ppd, op + 4u);
-            v.the = ld32(thv, op + 4u);
+            v.uoe = ldo(a.u3d, ou + 4u);
+            v.ute = ldo(a.uten, ou + 4u);
+            v.de = ldo(ppd, op + 4u);
+            v.the = ldo(thv, op + 4u);
           }
-          v.pc = ld32(a.piadv, op);
-          v.pe = ld32(a.piadv, op + 4u);
-          v.pw = ld32(a.piadv, op - 4u);
-          v.pn = ld32(a.piadv, op + onp);
-          v.ps = ld32(a.piadv, op - onp);
-          v.wup = ld32(w, ow + ows);
-          v.wt = ld32(a.wten, ow);
-          v.ppten = ld32(a.ppten, op);
-          v.ppterm = ld32(a.ppterm, op);
-          v.pp = ld32(a.pp3d, op);
-          v.th = ld32(thv, op);
+          v.pc = ldo(a.piadv, op);
+          v.pe = ldo(a.piadv, op + 4u);
+          v.pw = ldo(a.piadv, op - 4u);
+          v.pn = ldo(a.piadv, op + onp);
+          v.ps = ldo(a.piadv, op - onp);
+          v.wup = ldo(w, ow + ows);
+          v.wt = ldo(a.wten, ow);
+          v.ppten = ldo(a.ppten, op);
+          v.ppterm = ldo(a.ppterm, op);
+          v.pp = ldo(a.pp3d, op);
+          v.th = ldo(thv, op);
           return v;
         };
         float wraw = w[(int64_t)kb * wst + wc];
@@ -1003,21 +997,21 @@ __device__ __forceinline__ void column2_body(const SoundArgs& a, int n, int tile
         const uint32_t ov = (uint32_t)((int64_t)k * vst + vc) * 4u;
         const uint32_t ow = (uint32_t)((int64_t)k * wst + wc) * 4u;
         In v;
-        v.uw = ld32(a.u3d, ou);
-        v.ue = ld32(a.u3d, ou + 4u);
-        v.vs = ld32(a.v3d, ov);
-        v.vn = ld32(a.v3d, ov + onx);
-        v.pc = ld32(a.piadv, op);
-        v.pe = ld32(a.piadv, op + 4u);
-        v.pw = ld32(a.piadv, op - 4u);
-        v.pn = ld32(a.piadv, op + onp);
-        v.ps = ld32(a.piadv, op - onp);
-        v.wup = ld32(w, ow + ows);
-        v.wt = ld32(a.wten, ow);
-        v.ppten = ld32(a.ppten, op);
-        v.ppterm = ld32(a.ppterm, op);
-        v.pp = ld32(a.pp3d, op);
-        v.th = ld32(a.thv, op);
+        v.uw = ldo(a.u3d, ou);
+        v.ue = ldo(a.u3d, ou + 4u);
+        v.vs = ldo(a.v3d, ov);
+        v.vn = ldo(a.v3d, ov + onx);
+        v.pc = ldo(a.piadv, op);
+        v.pe = ldo(a.piadv, op + 4u);
+        v.pw = ldo(a.piadv, op - 4u);
+        v.pn = ldo(a.piadv, op + onp);
+        v.ps = ldo(a.piadv, op - onp);
+        v.wup = ldo(w, ow + ows);
+        v.wt = ldo(a.wten, ow);
+        v.ppten = ldo(a.ppten, op);
+        v.ppterm = ldo(a.ppterm, op);
+        v.pp = ldo(a.pp3d, op);
+        v.th = ldo(a.thv, op);
         return v;
       };
       float wraw = w[(int64_t)kb * wst + wc];
@@ -1488,21 +1482,21 @@ __global__ __launch_bounds__(C2T) __attribute__((amdgpu_waves_per_eu(6))) void k
       const uint32_t ov = (uint32_t)((int64_t)k * vst + vc) * 4u;
       const uint32_t ow = (uint32_t)((int64_t)k * wst + wc) * 4u;
       In v;
-      v.uw = ld32(a.u3d, ou);
-      v.ue = ld32(a.u3d, ou + 4u);
-      v.vs = ld32(a.v3d, ov);
-      v.vn = ld32(a.v3d, ov + onx);
-      v.pc = ld32(a.piadv, op);
-      v.pe = ld32(a.piadv, op + 4u);
-      v.pw = ld32(a.piadv, op - 4u);
-      v.pn = ld32(a.piadv, op + onp);
-      v.ps = ld32(a.piadv, op - onp);
-      v.wup = ld32(w, ow + ows);
-      v.wt = ld32(a.wten, ow);
-      v.ppten = ld32(a.ppten, op);
-      v.ppterm = ld32(a.ppterm, op);
-      v.pp = ld32(a.pp3d, op);
-      v.th = ld32(a.thv, op);
+      v.uw = ldo(a.u3d, ou);
+      v.ue = ldo(a.u3d, ou + 4u);
+      v.vs = ldo(a.v3d, ov);
+      v.vn = ldo(a.v3d, ov + onx);
+      v.pc = ldo(a.piadv, op);
+      v.pe = ldo(a.piadv, op + 4u);
+      v.pw = ldo(a.piadv, op - 4u);
+      v.pn = ldo(a.piadv, op + onp);
+      v.ps = ldo(a.piadv, op - onp);
+      v.wup = ldo(w, ow + ows);
+      v.wt = ldo(a.wten, ow);
+      v.ppten = ldo(a.ppten, op);
+      v.ppterm = ldo(a.ppterm, op);
+      v.pp = ldo(a.pp3d, op);
+      v.th = ldo(a.thv, op);
       return v;
     };
     float wraw = w[(int64_t)kb * wst + wc];
