@@ -452,11 +452,16 @@ __global__ __launch_bounds__(256) void k_snd_ssopen_flat(SoundArgs a) {
 // sound.F:318-342 on a flat column: every face i = 1..ni+1, j = 1..nj+1 takes
 // u += (dts*dtu)*(uten - ((tem1*dpp)*(thv(i)+thv(i-1)))), dtu = 0 on an open
 // boundary face and on the row outside the grid (param.F:9182-9219).
+// Each thread steps its face pair at PG_KB consecutive levels, every level's
+// words requested before any level is formed.
+constexpr int PG_KB = 4;
+
 __global__ __launch_bounds__(PG_TX * PG_TY) void k_snd_pgrad_flat(SoundArgs a) {
   const int ni = a.ni, nj = a.nj, nk = a.nk;
   const int gx = (ni + 1 + PG_TX - 1) / PG_TX, gy = (nj + 1 + PG_TY - 1) / PG_TY;
-  const int tile = xcd_tile((int)blockIdx.x, gx * gy * nk);
-  const int k = tile / (gx * gy) + 1;
+  const int nkb = (nk + PG_KB - 1) / PG_KB;
+  const int tile = xcd_tile((int)blockIdx.x, gx * gy * nkb);
+  const int kb = (tile / (gx * gy)) * PG_KB + 1;
   const int rem = tile % (gx * gy);
   const int x = (rem % gx) * PG_TX + (int)threadIdx.x % PG_TX;  // i - 1, 0..ni
   const int y = (rem / gx) * PG_TY + (int)threadIdx.x / PG_TX;  // j - 1, 0..nj
@@ -467,17 +472,41 @@ __global__ __launch_bounds__(PG_TX * PG_TY) void k_snd_pgrad_flat(SoundArgs a) {
   const bool ps = side_of(a.sides, 2) == 1u, pn = side_of(a.sides, 3) == 1u;
   const float wu = (j <= nj && (i > 1 || pw) && (i < ni + 1 || pe)) ? 1.0f : 0.0f;
   const float wv = (i <= ni && (j > 1 || ps) && (j < nj + 1 || pn)) ? 1.0f : 0.0f;
-  const float* ppd = a.ppd;
-  const float* thv = a.thv;
-  const int64_t pc = pix(k, j, i, ni, nj);
-  float pu = tem1 * (ppd[pc] - ppd[pc - 1]);
-  pu = pu * (thv[pc] + thv[pc - 1]);
-  const int64_t xu0 = ix3(k, y + NG, x + NG, sy(nj), ux(ni));
-  a.u3d[xu0] = a.u3d[xu0] + (a.dts * wu) * (a.uten[xu0] - pu);
-  float pv = tem2 * (ppd[pc] - ppd[pc - (ni + 2)]);
-  pv = pv * (thv[pc] + thv[pc - (ni + 2)]);
-  const int64_t xv0 = ix3(k, y + NG, x + NG, vy(nj), sx(ni));
-  a.v3d[xv0] = a.v3d[xv0] + (a.dts * wv) * (a.vten[xv0] - pv);
+  // a level's words at 32-bit byte offsets from the level's base
+  const uint32_t op = (uint32_t)((j * (ni + 2) + i) * 4), onp = (uint32_t)(ni + 2) * 4u;
+  const uint32_t ou = (uint32_t)(((y + NG) * ux(ni) + x + NG) * 4), ov = (uint32_t)(((y + NG) * sx(ni) + x + NG) * 4);
+  const int64_t pst = (int64_t)(nj + 2) * (ni + 2), ust = (int64_t)sy(nj) * ux(ni), vst = (int64_t)vy(nj) * sx(ni);
+  float u[PG_KB], ut[PG_KB], v[PG_KB], vt[PG_KB], dc[PG_KB], dw[PG_KB], ds[PG_KB], tc[PG_KB], tw[PG_KB], ts[PG_KB];
+#pragma unroll
+  for (int q = 0; q < PG_KB; ++q) {
+    const int k = kb + q;
+    if (k <= nk) {
+      const float* P = a.ppd + (int64_t)k * pst;
+      const float* T = a.thv + (int64_t)k * pst;
+      u[q] = ldo(a.u3d + (int64_t)k * ust, ou);
+      ut[q] = ldo(a.uten + (int64_t)k * ust, ou);
+      v[q] = ldo(a.v3d + (int64_t)k * vst, ov);
+      vt[q] = ldo(a.vten + (int64_t)k * vst, ov);
+      dc[q] = ldo(P, op);
+      dw[q] = ldo(P, op - 4u);
+      ds[q] = ldo(P, op - onp);
+      tc[q] = ldo(T, op);
+      tw[q] = ldo(T, op - 4u);
+      ts[q] = ldo(T, op - onp);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < PG_KB; ++q) {
+    const int k = kb + q;
+    if (k <= nk) {
+      float pu = tem1 * (dc[q] - dw[q]);
+      pu = pu * (tc[q] + tw[q]);
+      sto(a.u3d + (int64_t)k * ust, ou, u[q] + (a.dts * wu) * (ut[q] - pu));
+      float pv = tem2 * (dc[q] - ds[q]);
+      pv = pv * (tc[q] + ts[q]);
+      sto(a.v3d + (int64_t)k * vst, ov, v[q] + (a.dts * wv) * (vt[q] - pv));
+    }
+  }
 }
 
 // ------------------------------------------------------------------ column
@@ -1828,7 +1857,8 @@ static int psolver3_stage(
       GM1_TIMED(TK_ACOUSTIC_SMALL_STEP, s, {
         GM1_TIMED(TK_ACOUSTIC_PGRAD, s,
                   hipLaunchKernelGGL(k_snd_pgrad_flat,
-                                     dim3(((ni + 1 + PG_TX - 1) / PG_TX) * ((nj + 1 + PG_TY - 1) / PG_TY) * nk),
+                                     dim3(((ni + 1 + PG_TX - 1) / PG_TX) * ((nj + 1 + PG_TY - 1) / PG_TY) *
+                                          ((nk + PG_KB - 1) / PG_KB)),
                                      dim3(PG_TX * PG_TY), 0, s, a));
         if (resident)
           GM1_TIMED(TK_ACOUSTIC_COLUMN, s, launch_column3(narrow, a, n, s));
