@@ -47,6 +47,10 @@ struct SoundArgs {
   // when the stage starts from the alternate buffers.  Null otherwise.
   float* u_out; float* v_out; float* ppd_out;
   float* ppd_twin; float* u_start; float* v_start;
+  // resident pass: the prepare pass writes the time-average starts (rru = rrv
+  // = 0, rrw = beta*wa) on the halo ring only, and the first small step forms
+  // them where it reads them
+  int fresh_averages;
 };
 
 GM1_DEV const float* P_sigma(const SoundArgs& a) { return a.prof; }
@@ -110,9 +114,11 @@ __global__ __launch_bounds__(256) void k_snd_prepare(SoundArgs a) {
   const int64_t pu = ix3(k, y, x, sy(nj), ux(ni));
   const int64_t pv = ix3(k, y, x, vy(nj), sx(ni));
   const int64_t pw = ix3(k, y, x, sy(nj), sx(ni));
-  a.rru[pu] = 0.0f;
-  a.rrv[pv] = 0.0f;
-  a.rrw[pw] = a.wa[pw] * a.beta;
+  if (!a.fresh_averages || i == 0 || j == 0) {
+    a.rru[pu] = 0.0f;
+    a.rrv[pv] = 0.0f;
+    a.rrw[pw] = a.wa[pw] * a.beta;
+  }
   if (a.u_start) {
     // the small steps start from the alternate winds: they take the start
     // values on every face a small step writes (i, j >= 1 here); u3d and v3d
@@ -716,8 +722,10 @@ int column2_lds_bytes(int nk, int cw, bool pipe = false, bool flat_pipe = false)
 // rru(k) += u(k) and rrv(k) += v(k) on one column's west u face and south v
 // face for levels kb..ke (divided by nloop on the last small step); the words
 // of four levels are requested before any is written.
+// (fresh: the averages start here, from 0 rather than the words in rru, rrv)
 GM1_DEV void average_uv_faces(const SoundArgs& a, const float* __restrict__ u3, const float* __restrict__ v3, int kb,
-                              int ke, int64_t ust, int64_t uc, int64_t vst, int64_t vc, bool last, float tavg) {
+                              int ke, int64_t ust, int64_t uc, int64_t vst, int64_t vc, bool last, float tavg,
+                              bool fresh = false) {
   float* __restrict__ rru = a.rru;
   float* __restrict__ rrv = a.rrv;
   constexpr int B = 4;
@@ -727,9 +735,9 @@ GM1_DEV void average_uv_faces(const SoundArgs& a, const float* __restrict__ u3, 
     for (int u = 0; u < B; ++u)
       if (k0 + u <= ke) {
         const int64_t pu = (int64_t)(k0 + u) * ust + uc, pv = (int64_t)(k0 + u) * vst + vc;
-        ru[u] = rru[pu];
+        ru[u] = fresh ? 0.0f : rru[pu];
         uu[u] = u3[pu];
-        rv[u] = rrv[pv];
+        rv[u] = fresh ? 0.0f : rrv[pv];
         vv[u] = v3[pv];
       }
 #pragma unroll
@@ -1457,6 +1465,7 @@ __global__ __launch_bounds__(C2T) __attribute__((amdgpu_waves_per_eu(6))) void k
   const bool inside = (i <= ni);
   const bool phys = (i < ni) && (j < nj);
   const bool last = (n == a.nloop);
+  const bool fresh = a.fresh_averages && n == 1;  // the first small step starts the time averages
   const float tavg = 1.0f / (float)a.nloop;
   const int nxs = sx(ni), nys = sy(nj), nxu = ux(ni), nyv = vy(nj);
   const int64_t wst = (int64_t)nys * nxs;
@@ -1594,7 +1603,7 @@ __global__ __launch_bounds__(C2T) __attribute__((amdgpu_waves_per_eu(6))) void k
     if (inside && slot >= S0) {
       const int ch2 = (nk + (C2S - S0) - 1) / (C2S - S0);
       const int kb2 = 1 + (slot - S0) * ch2, ke2 = min(nk, kb2 + ch2 - 1);
-      average_uv_faces(a, a.u3d, a.v3d, kb2, ke2, ust, uc, vst, vc, last, tavg);
+      average_uv_faces(a, a.u3d, a.v3d, kb2, ke2, ust, uc, vst, vc, last, tavg, fresh);
     }
   }
   __syncthreads();
@@ -1604,7 +1613,8 @@ __global__ __launch_bounds__(C2T) __attribute__((amdgpu_waves_per_eu(6))) void k
     float rw[CH];
 #pragma unroll
     for (int kk = 0; kk < CH; ++kk)
-      if (kb + kk <= ke) rw[kk] = a.rrw[(int64_t)(kb + kk) * wst + wc];
+      if (kb + kk <= ke)
+        rw[kk] = fresh ? a.wa[(int64_t)(kb + kk) * wst + wc] * a.beta : a.rrw[(int64_t)(kb + kk) * wst + wc];
 #pragma unroll
     for (int kk = 0; kk < CH; ++kk) {
       const int k = kb + kk;
@@ -1629,7 +1639,7 @@ __global__ __launch_bounds__(C2T) __attribute__((amdgpu_waves_per_eu(6))) void k
     for (int k = kb; k <= ke; ++k) {
       const int64_t pw = (int64_t)k * wst + wc;
       const float wk = w[pw];
-      const float vw = a.rrw[pw] + (last ? wk * a.alph : wk);
+      const float vw = (fresh ? a.wa[pw] * a.beta : a.rrw[pw]) + (last ? wk * a.alph : wk);
       a.rrw[pw] = last ? vw * tavg : vw;
     }
   }
@@ -1818,6 +1828,7 @@ static int psolver3_stage(
   if (resident && (!flat || fuse || nk > 128 || column3_lds_bytes(nk, cw, 16) > lds_max ||
                    span >= ((int64_t)1 << 32)))
     return (int)hipErrorInvalidValue;
+  a.fresh_averages = resident ? 1 : 0;
   // with an odd number of small steps the first one reads the alternates
   const bool start_alt = fuse && (nloop & 1);
   if (fuse) a.ppd_twin = ppd_alt;
