@@ -51,6 +51,10 @@ struct SoundArgs {
   // = 0, rrw = beta*wa) on the halo ring only, and the first small step forms
   // them where it reads them
   int fresh_averages;
+  // resident pass: the first small step reads ppi for pp3d (the prepare pass
+  // copies it on the halo ring only) and, with skip_ppd (two or more small
+  // steps, whose fills rewrite ppd's ring), ppx for ppd (not copied at all)
+  int skip_ppd;
 };
 
 GM1_DEV const float* P_sigma(const SoundArgs& a) { return a.prof; }
@@ -106,10 +110,13 @@ __global__ __launch_bounds__(256) void k_snd_prepare(SoundArgs a) {
   const int ni = a.ni, nj = a.nj, nk = a.nk;
   if (i >= ni + 2) return;
   const int64_t p = pix(k, j, i, ni, nj);
-  const float px = a.ppx[p];
-  a.ppd[p] = px;
-  if (a.ppd_twin) a.ppd_twin[p] = px;
-  if (a.nrk != 1) a.pp3d[p] = a.ppi[p];
+  if (!a.skip_ppd || a.ppd_twin) {
+    const float px = a.ppx[p];
+    if (!a.skip_ppd) a.ppd[p] = px;
+    if (a.ppd_twin) a.ppd_twin[p] = px;
+  }
+  const bool ring = i == 0 || j == 0 || i == ni + 1 || j == nj + 1;
+  if (a.nrk != 1 && (!a.fresh_averages || ring)) a.pp3d[p] = a.ppi[p];
   const int y = j + NG - 1, x = i + NG - 1;
   const int64_t pu = ix3(k, y, x, sy(nj), ux(ni));
   const int64_t pv = ix3(k, y, x, vy(nj), sx(ni));
@@ -1514,6 +1521,7 @@ __global__ __launch_bounds__(C2T) __attribute__((amdgpu_waves_per_eu(6))) void k
       float uw, ue, vs, vn, pc, pe, pw, pn, ps, wup, wt, ppten, ppterm, pp, th;
     };
     const uint32_t onp = (uint32_t)(ni + 2) * 4u, onx = (uint32_t)nxs * 4u, ows = (uint32_t)wst * 4u;
+    const float* pp_in = (fresh && a.nrk != 1) ? a.ppi : a.pp3d;  // the first small step reads ppi
     auto load = [&](int k) {
       const uint32_t op = (uint32_t)pix(k, pj, pi, ni, nj) * 4u;
       const uint32_t ou = (uint32_t)((int64_t)k * ust + uc) * 4u;
@@ -1533,7 +1541,7 @@ __global__ __launch_bounds__(C2T) __attribute__((amdgpu_waves_per_eu(6))) void k
       v.wt = ldo(a.wten, ow);
       v.ppten = ldo(a.ppten, op);
       v.ppterm = ldo(a.ppterm, op);
-      v.pp = ldo(a.pp3d, op);
+      v.pp = ldo(pp_in, op);
       v.th = ldo(a.thv, op);
       return v;
     };
@@ -1829,6 +1837,9 @@ static int psolver3_stage(
                    span >= ((int64_t)1 << 32)))
     return (int)hipErrorInvalidValue;
   a.fresh_averages = resident ? 1 : 0;
+  // (the fills after small steps 1..nloop-1 rewrite ppd's whole halo ring when every side is a family)
+  const bool all_sides = side_of(sides, 0) && side_of(sides, 1) && side_of(sides, 2) && side_of(sides, 3);
+  a.skip_ppd = (resident && nloop >= 2 && all_sides) ? 1 : 0;
   // with an odd number of small steps the first one reads the alternates
   const bool start_alt = fuse && (nloop & 1);
   if (fuse) a.ppd_twin = ppd_alt;
@@ -1865,12 +1876,14 @@ static int psolver3_stage(
     for (int n = 1; n <= nloop; ++n) {
       if (any_open)
         hipLaunchKernelGGL(k_snd_ssopen_flat, dim3(gm1_blocks((int64_t)nk * (ni + nj), 256)), dim3(256), 0, s, a);
+      SoundArgs pg = a;
+      if (a.skip_ppd && n == 1) pg.ppd = ppx;  // ppd = ppx until the first small step writes it
       GM1_TIMED(TK_ACOUSTIC_SMALL_STEP, s, {
         GM1_TIMED(TK_ACOUSTIC_PGRAD, s,
                   hipLaunchKernelGGL(k_snd_pgrad_flat,
                                      dim3(((ni + 1 + PG_TX - 1) / PG_TX) * ((nj + 1 + PG_TY - 1) / PG_TY) *
                                           ((nk + PG_KB - 1) / PG_KB)),
-                                     dim3(PG_TX * PG_TY), 0, s, a));
+                                     dim3(PG_TX * PG_TY), 0, s, pg));
         if (resident)
           GM1_TIMED(TK_ACOUSTIC_COLUMN, s, launch_column3(narrow, a, n, s));
         else
