@@ -55,6 +55,11 @@ struct SoundArgs {
   // copies it on the halo ring only) and, with skip_ppd (two or more small
   // steps, whose fills rewrite ppd's ring), ppx for ppd (not copied at all)
   int skip_ppd;
+  // resident pass, nrk != 1: the first small step reads the winds from ua,
+  // va, wa (u_in, v_in, w_in, set for that step only) where the prepare pass
+  // copies them into u3d, v3d, w3d on the halo ring only; skip_winds marks it
+  int skip_winds;
+  const float* u_in; const float* v_in; const float* w_in;
 };
 
 GM1_DEV const float* P_sigma(const SoundArgs& a) { return a.prof; }
@@ -103,12 +108,8 @@ GM1_DEV FlatTerms flat_terms(const FlatCoef& f, int k, int nk, float ppterm, flo
 // Threads over pressure-family columns (i, j) in 0..n+1 and levels 1..nk:
 // stage coefficients on the interior (zero on halos), ppd = ppx, pp3d = ppi,
 // and the winds' average carriers over padded rows/cols NG-1..NG+n.
-__global__ __launch_bounds__(256) void k_snd_prepare(SoundArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // 0..ni+1
-  const int j = blockIdx.y;                             // 0..nj+1
-  const int k = blockIdx.z + 1;                         // 1..nk
+GM1_DEV void prepare_cell(const SoundArgs& a, int i, int j, int k) {
   const int ni = a.ni, nj = a.nj, nk = a.nk;
-  if (i >= ni + 2) return;
   const int64_t p = pix(k, j, i, ni, nj);
   if (!a.skip_ppd || a.ppd_twin) {
     const float px = a.ppx[p];
@@ -134,11 +135,11 @@ __global__ __launch_bounds__(256) void k_snd_prepare(SoundArgs a) {
     a.v_start[pv] = (a.nrk != 1) ? a.va[pv] : a.v3d[pv];
   }
   if (a.nrk != 1) {
-    if (!a.u_start || i == 0 || j == 0) {
+    if ((!a.u_start && !a.skip_winds) || i == 0 || j == 0) {
       a.u3d[pu] = a.ua[pu];
       a.v3d[pv] = a.va[pv];
     }
-    a.w3d[pw] = a.wa[pw];
+    if (!a.skip_winds || i == 0 || j == 0) a.w3d[pw] = a.wa[pw];
   }
   // the flat column forms qk, pk1, pk2 and mm inside its column pass (flat_terms) from
   // the level factors one thread per level writes here into the qk scratch
@@ -177,6 +178,30 @@ __global__ __launch_bounds__(256) void k_snd_prepare(SoundArgs a) {
   a.pk1[p] = pk1;
   a.pk2[p] = pk2;
   a.mm[p] = mm;
+}
+
+__global__ __launch_bounds__(256) void k_snd_prepare(SoundArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // 0..ni+1
+  const int j = blockIdx.y;                             // 0..nj+1
+  const int k = blockIdx.z + 1;                         // 1..nk
+  if (i >= a.ni + 2) return;
+  prepare_cell(a, i, j, k);
+}
+
+// The same statements on the halo ring (i = 0, ni+1 or j = 0, nj+1) of each
+// level only: when the resident pass forms every other start word in its
+// first small step, these are all the prepare pass writes (the level factors
+// included, at i = j = 0).
+__global__ __launch_bounds__(256) void k_snd_prepare_ring(SoundArgs a) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y + 1;
+  const int ni = a.ni, nj = a.nj;
+  if (q >= 2 * (ni + 2) + 2 * nj) return;
+  int i, j;
+  if (q < ni + 2) { j = 0; i = q; }
+  else if (q < 2 * (ni + 2)) { j = nj + 1; i = q - (ni + 2); }
+  else { const int r = q - 2 * (ni + 2); j = 1 + r / 2; i = (r & 1) ? ni + 1 : 0; }
+  prepare_cell(a, i, j, k);
 }
 
 // ------------------------------------------------------------------ ssopen
@@ -424,21 +449,23 @@ __global__ __launch_bounds__(256) void k_snd_ssopen_flat(SoundArgs a) {
   const int ni = a.ni, nj = a.nj, nk = a.nk;
   const int64_t we = (int64_t)nk * nj, sn = (int64_t)nk * ni;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const float* ui = a.u_in ? a.u_in : a.u3d;  // the words before this small step
+  const float* vi = a.v_in ? a.v_in : a.v3d;
   if (q < we) {
     const int j0 = (int)(q % nj), k = (int)(q / nj) + 1, y = j0 + NG;
     const int nxu = ux(ni), nyu = sy(nj);
     if (side_of(a.sides, 0) == 2u) {
       const int64_t b = ix3(k, y, NG, nyu, nxu);
-      const float t = ((((-a.radbcw[(int64_t)k * (nj + 2 * NG) + y]) * (a.u3d[b + 1] - a.u3d[b])) * a.rdx) * a.uh[NG]) +
+      const float t = ((((-a.radbcw[(int64_t)k * (nj + 2 * NG) + y]) * (ui[b + 1] - ui[b])) * a.rdx) * a.uh[NG]) +
                       a.uten[b];
-      a.u3d[b] = a.u3d[b] + a.dts * t;
+      a.u3d[b] = ui[b] + a.dts * t;
     }
     if (side_of(a.sides, 1) == 2u) {
       const int64_t b = ix3(k, y, ni + NG, nyu, nxu);
       const float t =
-          ((((-a.radbce[(int64_t)k * (nj + 2 * NG) + y]) * (a.u3d[b] - a.u3d[b - 1])) * a.rdx) * a.uh[ni + NG - 1]) +
+          ((((-a.radbce[(int64_t)k * (nj + 2 * NG) + y]) * (ui[b] - ui[b - 1])) * a.rdx) * a.uh[ni + NG - 1]) +
           a.uten[b];
-      a.u3d[b] = a.u3d[b] + a.dts * t;
+      a.u3d[b] = ui[b] + a.dts * t;
     }
     return;
   }
@@ -448,16 +475,16 @@ __global__ __launch_bounds__(256) void k_snd_ssopen_flat(SoundArgs a) {
     const int nxv = sx(ni), nyv = vy(nj);
     if (side_of(a.sides, 2) == 2u) {
       const int64_t b = ix3(k, NG, x, nyv, nxv);
-      const float t = ((((-a.radbcs[(int64_t)k * (ni + 2 * NG) + x]) * (a.v3d[b + nxv] - a.v3d[b])) * a.rdy) * a.vh[NG]) +
+      const float t = ((((-a.radbcs[(int64_t)k * (ni + 2 * NG) + x]) * (vi[b + nxv] - vi[b])) * a.rdy) * a.vh[NG]) +
                       a.vten[b];
-      a.v3d[b] = a.v3d[b] + a.dts * t;
+      a.v3d[b] = vi[b] + a.dts * t;
     }
     if (side_of(a.sides, 3) == 2u) {
       const int64_t b = ix3(k, nj + NG, x, nyv, nxv);
       const float t =
-          ((((-a.radbcn[(int64_t)k * (ni + 2 * NG) + x]) * (a.v3d[b] - a.v3d[b - nxv])) * a.rdy) * a.vh[nj + NG - 1]) +
+          ((((-a.radbcn[(int64_t)k * (ni + 2 * NG) + x]) * (vi[b] - vi[b - nxv])) * a.rdy) * a.vh[nj + NG - 1]) +
           a.vten[b];
-      a.v3d[b] = a.v3d[b] + a.dts * t;
+      a.v3d[b] = vi[b] + a.dts * t;
     }
   }
 }
@@ -489,6 +516,12 @@ __global__ __launch_bounds__(PG_TX * PG_TY) void k_snd_pgrad_flat(SoundArgs a) {
   const uint32_t op = (uint32_t)((j * (ni + 2) + i) * 4), onp = (uint32_t)(ni + 2) * 4u;
   const uint32_t ou = (uint32_t)(((y + NG) * ux(ni) + x + NG) * 4), ov = (uint32_t)(((y + NG) * sx(ni) + x + NG) * 4);
   const int64_t pst = (int64_t)(nj + 2) * (ni + 2), ust = (int64_t)sy(nj) * ux(ni), vst = (int64_t)vy(nj) * sx(ni);
+  // the words before this small step: u_in / v_in when set, except on the open
+  // boundary faces this step's open-side pass has already stepped
+  const bool ow_face = (x == 0 && side_of(a.sides, 0) == 2u) || (x == ni && side_of(a.sides, 1) == 2u);
+  const bool os_face = (y == 0 && side_of(a.sides, 2) == 2u) || (y == nj && side_of(a.sides, 3) == 2u);
+  const float* usrc = (a.u_in && !(ow_face && y < nj)) ? a.u_in : a.u3d;
+  const float* vsrc = (a.v_in && !(os_face && x < ni)) ? a.v_in : a.v3d;
   float u[PG_KB], ut[PG_KB], v[PG_KB], vt[PG_KB], dc[PG_KB], dw[PG_KB], ds[PG_KB], tc[PG_KB], tw[PG_KB], ts[PG_KB];
 #pragma unroll
   for (int q = 0; q < PG_KB; ++q) {
@@ -496,9 +529,9 @@ __global__ __launch_bounds__(PG_TX * PG_TY) void k_snd_pgrad_flat(SoundArgs a) {
     if (k <= nk) {
       const float* P = a.ppd + (int64_t)k * pst;
       const float* T = a.thv + (int64_t)k * pst;
-      u[q] = ldo(a.u3d + (int64_t)k * ust, ou);
+      u[q] = ldo(usrc + (int64_t)k * ust, ou);
       ut[q] = ldo(a.uten + (int64_t)k * ust, ou);
-      v[q] = ldo(a.v3d + (int64_t)k * vst, ov);
+      v[q] = ldo(vsrc + (int64_t)k * vst, ov);
       vt[q] = ldo(a.vten + (int64_t)k * vst, ov);
       dc[q] = ldo(P, op);
       dw[q] = ldo(P, op - 4u);
@@ -1516,7 +1549,10 @@ __global__ __launch_bounds__(C2T) __attribute__((amdgpu_waves_per_eu(6))) void k
   // ---------------- phase A (sound.F:491-509 and the right-hand sides)
   if (phys && kb <= ke) {
     const float temx = (a.dts * 0.5f) * a.rdx, temy = (a.dts * 0.5f) * a.rdy;
-    if (kb == 1) lw1[c] = w[wst + wc];
+    // the small step's w before it (w_in, set on the first small step: levels
+    // 1..nk, the lid stays in w3d)
+    const float* wi = a.w_in ? a.w_in : w;
+    if (kb == 1) lw1[c] = wi[wst + wc];
     struct In {
       float uw, ue, vs, vn, pc, pe, pw, pn, ps, wup, wt, ppten, ppterm, pp, th;
     };
@@ -1537,7 +1573,7 @@ __global__ __launch_bounds__(C2T) __attribute__((amdgpu_waves_per_eu(6))) void k
       v.pw = ldo(a.piadv, op - 4u);
       v.pn = ldo(a.piadv, op + onp);
       v.ps = ldo(a.piadv, op - onp);
-      v.wup = ldo(w, ow + ows);
+      v.wup = ldo(k + 1 <= nk ? wi : w, ow + ows);
       v.wt = ldo(a.wten, ow);
       v.ppten = ldo(a.ppten, op);
       v.ppterm = ldo(a.ppterm, op);
@@ -1545,7 +1581,7 @@ __global__ __launch_bounds__(C2T) __attribute__((amdgpu_waves_per_eu(6))) void k
       v.th = ldo(a.thv, op);
       return v;
     };
-    float wraw = w[(int64_t)kb * wst + wc];
+    float wraw = wi[(int64_t)kb * wst + wc];
     const int64_t pb = pix(kb - 1, pj, pi, ni, nj);
     float pd = a.piadv[pb], thm = a.thv[pb];
     In nx = load(kb);
@@ -1630,7 +1666,7 @@ __global__ __launch_bounds__(C2T) __attribute__((amdgpu_waves_per_eu(6))) void k
         const int64_t p = pix(k, pj, pi, ni, nj);
         const float wk = (k == 1) ? lw1[c] : Ldd[at(k)];
         const float wup = (k == nk) ? wlid : Ldd[at(k + 1)];
-        if (k >= 2) w[(int64_t)k * wst + wc] = wk;
+        if (k >= 2 || a.w_in) w[(int64_t)k * wst + wc] = wk;  // (level 1 copied from w_in on its first step)
         const float old = rpp[kk];
         const float vt = wk * Ls1[at(k)] + wup * Ls2[at(k)];
         float up = (old + rfpk[kk]) + a.alph * vt;
@@ -1646,7 +1682,8 @@ __global__ __launch_bounds__(C2T) __attribute__((amdgpu_waves_per_eu(6))) void k
   } else if (inside) {
     for (int k = kb; k <= ke; ++k) {
       const int64_t pw = (int64_t)k * wst + wc;
-      const float wk = w[pw];
+      const float wk = a.w_in ? a.w_in[pw] : w[pw];
+      if (a.w_in) w[pw] = wk;  // the east column's and north row's w, copied on the first step
       const float vw = (fresh ? a.wa[pw] * a.beta : a.rrw[pw]) + (last ? wk * a.alph : wk);
       a.rrw[pw] = last ? vw * tavg : vw;
     }
@@ -1840,13 +1877,21 @@ static int psolver3_stage(
   // (the fills after small steps 1..nloop-1 rewrite ppd's whole halo ring when every side is a family)
   const bool all_sides = side_of(sides, 0) && side_of(sides, 1) && side_of(sides, 2) && side_of(sides, 3);
   a.skip_ppd = (resident && nloop >= 2 && all_sides) ? 1 : 0;
+  a.skip_winds = (resident && nrk != 1) ? 1 : 0;
   // with an odd number of small steps the first one reads the alternates
   const bool start_alt = fuse && (nloop & 1);
   if (fuse) a.ppd_twin = ppd_alt;
   if (start_alt) { a.u_start = u_alt; a.v_start = v_alt; }
-  hipLaunchKernelGGL(k_snd_prepare, dim3((ni + 2 + 63) / 64, nj + 2, nk), dim3(64), 0, s, a);
+  if (resident && a.skip_ppd)
+    hipLaunchKernelGGL(k_snd_prepare_ring, dim3((2 * (ni + 2) + 2 * nj + 255) / 256, nk), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_snd_prepare, dim3((ni + 2 + 63) / 64, nj + 2, nk), dim3(64), 0, s, a);
   GM1_LAUNCH_CHECK();
   a.ppd_twin = nullptr; a.u_start = nullptr; a.v_start = nullptr;
+  // the first small step's readers of the winds before it (ua, va, wa where the prepare pass left them)
+  auto first_step = [&](SoundArgs& b, int n) {
+    if (a.skip_winds && n == 1) { b.u_in = ua; b.v_in = va; b.w_in = wa; }
+  };
   if (fuse) {
     float* ub[2] = {u3d, u_alt};
     float* vb[2] = {v3d, v_alt};
@@ -1874,9 +1919,11 @@ static int psolver3_stage(
     GM1_LAUNCH_CHECK();
   } else if (flat) {
     for (int n = 1; n <= nloop; ++n) {
+      SoundArgs b = a;
+      first_step(b, n);
       if (any_open)
-        hipLaunchKernelGGL(k_snd_ssopen_flat, dim3(gm1_blocks((int64_t)nk * (ni + nj), 256)), dim3(256), 0, s, a);
-      SoundArgs pg = a;
+        hipLaunchKernelGGL(k_snd_ssopen_flat, dim3(gm1_blocks((int64_t)nk * (ni + nj), 256)), dim3(256), 0, s, b);
+      SoundArgs pg = b;
       if (a.skip_ppd && n == 1) pg.ppd = ppx;  // ppd = ppx until the first small step writes it
       GM1_TIMED(TK_ACOUSTIC_SMALL_STEP, s, {
         GM1_TIMED(TK_ACOUSTIC_PGRAD, s,
@@ -1885,7 +1932,7 @@ static int psolver3_stage(
                                           ((nk + PG_KB - 1) / PG_KB)),
                                      dim3(PG_TX * PG_TY), 0, s, pg));
         if (resident)
-          GM1_TIMED(TK_ACOUSTIC_COLUMN, s, launch_column3(narrow, a, n, s));
+          GM1_TIMED(TK_ACOUSTIC_COLUMN, s, launch_column3(narrow, b, n, s));
         else
           GM1_TIMED(TK_ACOUSTIC_COLUMN, s, launch_column2<true>(narrow, pipe, a, n, s));
       });
