@@ -105,70 +105,77 @@ __global__ __launch_bounds__(64) void k_kes_column(KesArgs a) {
   const float rmax[3] = {0.0f, 1.0e-14f, 1.0e-14f};
   // temperature at the lowest level before any phase (pdefq leaves theta, pi alone)
   const float t0 = (a.th0[h] + a.th3d[h]) * (a.pi0[h] + a.pp3d[h]);
-  for (int sp = 0; sp < 3; ++sp) {
-    float* __restrict__ q = qs[sp];
-    double t1 = 0.0, t2 = 0.0, a1 = 0.0, a2 = 0.0;
-    for (int k0 = 0; k0 < nk; k0 += KB) {
-      float r[KB], v[KB], m[KB];
-#pragma unroll
-      for (int u = 0; u < KB; ++u)
-        if (k0 + u < nk) {
-          const int64_t p = (k0 + u) * plane + h;
-          r[u] = rho[p];
-          v[u] = q[p];
-          m[u] = rmh[p];
-        }
-#pragma unroll
-      for (int u = 0; u < KB; ++u)
-        if (k0 + u < nk) {
-          t1 += (double)(r[u] * v[u]);
-          a1 += (double)((((r[u] * v[u]) * ru) * rv) * m[u]);
-          if (v[u] < rmax[sp]) {
-            v[u] = 0.0f;
-            q[(k0 + u) * plane + h] = 0.0f;
-          }
-          t2 += (double)(r[u] * v[u]);
-        }
-    }
-    double t3 = (t1 + 1.0e-20) / (t2 + 1.0e-20);
-    if (t3 < 0.0) t3 = 1.0;
-    for (int k0 = 0; k0 < nk; k0 += KB) {
-      float r[KB], v[KB], m[KB];
-#pragma unroll
-      for (int u = 0; u < KB; ++u)
-        if (k0 + u < nk) {
-          const int64_t p = (k0 + u) * plane + h;
-          r[u] = rho[p];
-          v[u] = q[p];
-          m[u] = rmh[p];
-        }
-#pragma unroll
-      for (int u = 0; u < KB; ++u)
-        if (k0 + u < nk) {
-          const float nv = (float)(t3 * (double)v[u]);
-          q[(k0 + u) * plane + h] = nv;
-          a2 += (double)((((r[u] * nv) * ru) * rv) * m[u]);
-        }
-    }
-    a.colbud[(int64_t)sp * plane + h] = a2 - a1;
-  }
-  // fall speed
+  const float rdz = 1.0f / a.dz;
+  // pdefq of the three species in two sweeps up the column (each species' sums
+  // in level order, as species by species): the first forms the masses before
+  // and after the small values are cleared, the second scales each species by
+  // its mass ratio -- re-clearing the same small values rather than reading
+  // back cleared words -- and forms the fall speed and its Courant maximum
+  double t1[3] = {0.0, 0.0, 0.0}, t2[3] = {0.0, 0.0, 0.0}, a1[3] = {0.0, 0.0, 0.0}, a2[3] = {0.0, 0.0, 0.0};
   for (int k0 = 0; k0 < nk; k0 += KB) {
-    float r[KB], q[KB];
+    float r[KB], m[KB], v[3][KB];
 #pragma unroll
     for (int u = 0; u < KB; ++u)
       if (k0 + u < nk) {
-        r[u] = rho[(k0 + u) * plane + h];
-        q[u] = qr[(k0 + u) * plane + h];
+        const int64_t p = (k0 + u) * plane + h;
+        r[u] = rho[p];
+        m[u] = rmh[p];
+#pragma unroll
+        for (int sp = 0; sp < 3; ++sp) v[sp][u] = qs[sp][p];
       }
 #pragma unroll
     for (int u = 0; u < KB; ++u)
       if (k0 + u < nk) {
-        float v = 0.0f;
-        if (q[u] > 1.0e-12f) v = (14.34f * pow_r(r[u] * q[u], 0.1346f)) * sqrtf(1.15f / r[u]);
-        vt[(k0 + u) * plane + h] = v;
+#pragma unroll
+        for (int sp = 0; sp < 3; ++sp) {
+          const float x = v[sp][u];
+          t1[sp] += (double)(r[u] * x);
+          a1[sp] += (double)((((r[u] * x) * ru) * rv) * m[u]);
+          const float xz = (x < rmax[sp]) ? 0.0f : x;
+          t2[sp] += (double)(r[u] * xz);
+        }
       }
   }
+  double t3[3];
+#pragma unroll
+  for (int sp = 0; sp < 3; ++sp) {
+    t3[sp] = (t1[sp] + 1.0e-20) / (t2[sp] + 1.0e-20);
+    if (t3[sp] < 0.0) t3[sp] = 1.0;
+  }
+  float crmax = 0.0f;
+  for (int k0 = 0; k0 < nk; k0 += KB) {
+    float r[KB], m[KB], mf[KB], v[3][KB];
+#pragma unroll
+    for (int u = 0; u < KB; ++u)
+      if (k0 + u < nk) {
+        const int64_t p = (k0 + u) * plane + h;
+        r[u] = rho[p];
+        m[u] = rmh[p];
+        mf[u] = mh[p];
+#pragma unroll
+        for (int sp = 0; sp < 3; ++sp) v[sp][u] = qs[sp][p];
+      }
+#pragma unroll
+    for (int u = 0; u < KB; ++u)
+      if (k0 + u < nk) {
+        const int64_t p = (k0 + u) * plane + h;
+        float qrn = 0.0f;
+#pragma unroll
+        for (int sp = 0; sp < 3; ++sp) {
+          const float x = (v[sp][u] < rmax[sp]) ? 0.0f : v[sp][u];
+          const float nv = (float)(t3[sp] * (double)x);
+          qs[sp][p] = nv;
+          a2[sp] += (double)((((r[u] * nv) * ru) * rv) * m[u]);
+          qrn = nv;  // the last species is rain
+        }
+        float vf = 0.0f;
+        if (qrn > 1.0e-12f) vf = (14.34f * pow_r(r[u] * qrn, 0.1346f)) * sqrtf(1.15f / r[u]);
+        vt[p] = vf;
+        crmax = fmaxf(crmax, ((vf * a.dt) * rdz) * mf[u]);
+      }
+  }
+#pragma unroll
+  for (int sp = 0; sp < 3; ++sp) a.colbud[(int64_t)sp * plane + h] = a2[sp] - a1[sp];
   {  // surface energy flux of falling rain
     float term = a.dt * vt[h];
     term = term * a.rr[h];
@@ -208,21 +215,7 @@ __global__ __launch_bounds__(64) void k_kes_column(KesArgs a) {
       tk = tup;
     }
   }
-  // fallout
-  const float rdz = 1.0f / a.dz;
-  float crmax = 0.0f;
-  for (int k0 = 0; k0 < nk; k0 += KB) {
-    float v[KB], m[KB];
-#pragma unroll
-    for (int u = 0; u < KB; ++u)
-      if (k0 + u < nk) {
-        v[u] = vt[(k0 + u) * plane + h];
-        m[u] = mh[(k0 + u) * plane + h];
-      }
-#pragma unroll
-    for (int u = 0; u < KB; ++u)
-      if (k0 + u < nk) crmax = fmaxf(crmax, ((v[u] * a.dt) * rdz) * m[u]);
-  }
+  // fallout (crmax from the second pdefq sweep)
   int nfall = (int)(crmax + 1.0f);
   if (nfall < 1) nfall = 1;
   {  // levels with little rain take the speed of the level above, top down
