@@ -1554,7 +1554,7 @@ __global__ __launch_bounds__(C2T) __attribute__((amdgpu_waves_per_eu(6))) void k
     const float* wi = a.w_in ? a.w_in : w;
     if (kb == 1) lw1[c] = wi[wst + wc];
     struct In {
-      float uw, ue, vs, vn, pc, pe, pw, pn, ps, wup, wt, ppten, ppterm, pp, th;
+      float uw, ue, vs, vn, pc, pe, pw, pn, ps, wup, wt, ppten, ppterm, pp, th, ru, rv;
     };
     const uint32_t onp = (uint32_t)(ni + 2) * 4u, onx = (uint32_t)nxs * 4u, ows = (uint32_t)wst * 4u;
     const float* pp_in = (fresh && a.nrk != 1) ? a.ppi : a.pp3d;  // the first small step reads ppi
@@ -1579,6 +1579,8 @@ __global__ __launch_bounds__(C2T) __attribute__((amdgpu_waves_per_eu(6))) void k
       v.ppterm = ldo(a.ppterm, op);
       v.pp = ldo(pp_in, op);
       v.th = ldo(a.thv, op);
+      v.ru = fresh ? 0.0f : ldo(a.rru, ou);  // the own faces' time averages so far
+      v.rv = fresh ? 0.0f : ldo(a.rrv, ov);
       return v;
     };
     float wraw = wi[(int64_t)kb * wst + wc];
@@ -1604,6 +1606,11 @@ __global__ __launch_bounds__(C2T) __attribute__((amdgpu_waves_per_eu(6))) void k
         const float s1 = t.pk1 + t.qk, s2 = t.pk2 - t.qk;
         const float vert = wraw * s1 + cu.wup * s2;
         const float fpk = (a.dts * (cu.ppten - cu.ppterm * div) + hp) + a.beta * vert;
+        {  // the own u and v faces into their time averages, from the words read above
+          const float au = cu.ru + cu.uw, av = cu.rv + cu.vs;
+          a.rru[(int64_t)k * ust + uc] = last ? au * tavg : au;
+          a.rrv[(int64_t)k * vst + vc] = last ? av * tavg : av;
+        }
         Ls1[at(k)] = s1;
         Ls2[at(k)] = s2;
         Lmm[at(k)] = t.mm;
@@ -1644,7 +1651,7 @@ __global__ __launch_bounds__(C2T) __attribute__((amdgpu_waves_per_eu(6))) void k
   {
     constexpr int S0 = 64 / C2W;
     static_assert(C2S > S0, "the averages need slots outside wavefront 0");
-    if (inside && slot >= S0) {
+    if (inside && !phys && slot >= S0) {  // the east column's and north row's faces
       const int ch2 = (nk + (C2S - S0) - 1) / (C2S - S0);
       const int kb2 = 1 + (slot - S0) * ch2, ke2 = min(nk, kb2 + ch2 - 1);
       average_uv_faces(a, a.u3d, a.v3d, kb2, ke2, ust, uc, vst, vc, last, tavg, fresh);
