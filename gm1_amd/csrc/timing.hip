@@ -46,7 +46,7 @@ int gm1hip_kernel_timer_arm(int kernel, int capacity) {
   release();
   g_overflow = false;
   if (kernel == TK_NONE) return 0;
-  if (kernel < TK_ACOUSTIC_COLUMN || kernel > TK_MOM_ADVW || capacity < 1) return (int)hipErrorInvalidValue;
+  if (kernel < TK_ACOUSTIC_COLUMN || kernel > TK_ACOUSTIC_SMALL_STEP || capacity < 1) return (int)hipErrorInvalidValue;
   g_pairs.resize((size_t)capacity);
   for (Pair& p : g_pairs) {
     hipError_t e = hipEventCreate(&p.start);

@@ -462,8 +462,10 @@ int gm1hip_psolver3_stage_flat_fused(
 // No counterpart in the reference: live per-kernel GPU time for throughput
 // reports.  arm() creates `capacity` event pairs (before a timed region) and
 // brackets every launch of kernel id 1 k_snd_column, 2 k_snd_pgrad,
-// 3 k_scalar_adv5, 4/5/6 k_mom_advu/advv/advw; 0 disarms.  collect() waits,
-// sums and disarms; it fails if more launches happened than were armed.
+// 3 k_scalar_adv5, 4/5/6 k_mom_advu/advv/advw, or 7 one acoustic small step's
+// pressure-gradient and column work (one bracket per small step, around one
+// pass in the fused forms and two in the others); 0 disarms.  collect()
+// waits, sums and disarms; it fails if more launches happened than were armed.
 int gm1hip_kernel_timer_arm(int kernel, int capacity);
 int gm1hip_kernel_timer_collect(double* total_ms, int* launches);
 
