@@ -44,10 +44,13 @@ struct MomArgs {
   const float* flat;
   const float* rr0;
   // Held tendencies seeding the update (uten = uten1 + advection), or null:
-  // the update then adds to uten as it stands.
+  // the update then adds to uten as it stands -- unless zero_held, where the
+  // held tendencies are +0 everywhere (no solve1 contribution) and the update
+  // forms 0 + advection without reading them.
   const float* u1 = nullptr;
   const float* v1 = nullptr;
   const float* w1 = nullptr;
+  int zero_held = 0;
   // The flat column's buoyancy added to the w update in the same statement
   // order, wten = (wten1 + advw) + (c1*dum8(k-1) + c2*dum8(k)), or null.
   const float* dum8 = nullptr;
@@ -380,7 +383,7 @@ __global__ __launch_bounds__(MNT) void k_mom_advu_tile(MomArgs a, int f0, int nf
         e_rw = ldo(a.rho0 + lm, ow);
         e_re = ldo(a.rho0 + lm, oe);
       }
-      e_ten = ldo((a.u1 ? a.u1 : a.uten) + (int64_t)k * ust, ocol);
+      e_ten = a.zero_held ? 0.0f : ldo((a.u1 ? a.u1 : a.uten) + (int64_t)k * ust, ocol);
     }
 #pragma unroll
     for (int q = 0; q < PS; ++q) {
@@ -579,7 +582,7 @@ __global__ __launch_bounds__(MNT) void k_mom_advv_tile(MomArgs a, int g0, int nr
         e_rs = ldo(a.rho0 + lm, os);
         e_rn = ldo(a.rho0 + lm, on);
       }
-      e_ten = ldo((a.v1 ? a.v1 : a.vten) + (int64_t)k * vst, ocol);
+      e_ten = a.zero_held ? 0.0f : ldo((a.v1 ? a.v1 : a.vten) + (int64_t)k * vst, ocol);
     }
 #pragma unroll
     for (int q = 0; q < PS; ++q) {
@@ -685,144 +688,6 @@ void launch_advv_tile(const MomArgs& a, int g0, int nr, hipStream_t s) {
 }
 
 // ------------------------------------------------------------- advw
-GM1_DEV float wfx_base(const MomArgs& a, int kf, int j, int p) {
-  const int ni = a.ni, nj = a.nj;
-  const float* c2k = a.c2 + (int64_t)(kf - 1) * nj * ni + (int64_t)j * ni;
-  const float c2h = 0.5f * (c2k[clampi(p - 1, ni)] + c2k[clampi(p, ni)]);
-  const float c1h = 1.0f - c2h;
-  const int nxu = ux(ni), nyu = sy(nj), nxs = sx(ni), nys = sy(nj);
-  const int y = j + NG, x = p + NG;
-  const float m = c2h * a.rru[ix3(kf, y, x, nyu, nxu)] + c1h * a.rru[ix3(kf - 1, y, x, nyu, nxu)];
-  const float* W = a.w + ix3(kf, y, 0, nys, nxs);
-  const float r = (m >= 0.0f) ? flx5(W[x - 3], W[x - 2], W[x - 1], W[x], W[x + 1])
-                              : flx5(W[x + 2], W[x + 1], W[x], W[x - 1], W[x - 2]);
-  return m * r;
-}
-
-GM1_DEV float wfx(const MomArgs& a, int kf, int j, int p) {
-  const int ni = a.ni, nj = a.nj;
-  const float* c2k = a.c2 + (int64_t)(kf - 1) * nj * ni + (int64_t)j * ni;
-  const float c2h = 0.5f * (c2k[clampi(p - 1, ni)] + c2k[clampi(p, ni)]);
-  const float c1h = 1.0f - c2h;
-  const int nxu = ux(ni), nyu = sy(nj);
-  const int y = j + NG, x = p + NG;
-  const float m = c2h * a.rru[ix3(kf, y, x, nyu, nxu)] + c1h * a.rru[ix3(kf - 1, y, x, nyu, nxu)];
-  if (p == 0 && opn(a.obc, 0) && m >= 0.0f) return wfx_base(a, kf, j, 1);
-  if (p == ni && opn(a.obc, 1) && m <= 0.0f) return wfx_base(a, kf, j, ni - 1);
-  return wfx_base(a, kf, j, p);
-}
-
-GM1_DEV float wfy_base(const MomArgs& a, int kf, int p, int i) {
-  const int ni = a.ni, nj = a.nj;
-  const float* c2k = a.c2 + (int64_t)(kf - 1) * nj * ni;
-  const float c2h = 0.5f * (c2k[(int64_t)clampi(p - 1, nj) * ni + i] + c2k[(int64_t)clampi(p, nj) * ni + i]);
-  const float c1h = 1.0f - c2h;
-  const int nxs = sx(ni), nyv = vy(nj), nys = sy(nj);
-  const int y = p + NG, x = i + NG;
-  const float m = c2h * a.rrv[ix3(kf, y, x, nyv, nxs)] + c1h * a.rrv[ix3(kf - 1, y, x, nyv, nxs)];
-  const float* W = a.w + ix3(kf, 0, x, nys, nxs);
-  auto q = [&](int yy) { return W[(int64_t)yy * nxs]; };
-  const float r = (m >= 0.0f) ? flx5(q(y - 3), q(y - 2), q(y - 1), q(y), q(y + 1))
-                              : flx5(q(y + 2), q(y + 1), q(y), q(y - 1), q(y - 2));
-  return m * r;
-}
-
-GM1_DEV float wfy(const MomArgs& a, int kf, int p, int i) {
-  const int ni = a.ni, nj = a.nj;
-  const float* c2k = a.c2 + (int64_t)(kf - 1) * nj * ni;
-  const float c2h = 0.5f * (c2k[(int64_t)clampi(p - 1, nj) * ni + i] + c2k[(int64_t)clampi(p, nj) * ni + i]);
-  const float c1h = 1.0f - c2h;
-  const int nxs = sx(ni), nyv = vy(nj);
-  const int y = p + NG, x = i + NG;
-  const float m = c2h * a.rrv[ix3(kf, y, x, nyv, nxs)] + c1h * a.rrv[ix3(kf - 1, y, x, nyv, nxs)];
-  if (p == 0 && opn(a.obc, 2) && m >= 0.0f) return wfy_base(a, kf, 1, i);
-  if (p == nj && opn(a.obc, 3) && m <= 0.0f) return wfy_base(a, kf, nj - 1, i);
-  return wfy_base(a, kf, p, i);
-}
-
-GM1_DEV float wz(const MomArgs& a, int k, int j, int i) {
-  const int nxs = sx(a.ni), nys = sy(a.nj), nk = a.nk;
-  const int64_t st = (int64_t)nys * nxs;
-  const int64_t c = (int64_t)(j + NG) * nxs + (i + NG);
-  const float m = 0.5f * (a.rrw[(int64_t)k * st + c] + a.rrw[(int64_t)(k + 1) * st + c]);
-  auto Q = [&](int lev) { return a.w[(int64_t)lev * st + c]; };
-  float r;
-  if (k == 1) r = (m >= 0.0f) ? 0.5f * (Q(1) + Q(2)) : flx3(Q(3), Q(2), Q(1));
-  else if (k == 2) r = (m >= 0.0f) ? flx3(Q(1), Q(2), Q(3)) : flx5(Q(5), Q(4), Q(3), Q(2), Q(1));
-  else if (k == nk - 1) r = (m >= 0.0f) ? flx5(Q(k - 2), Q(k - 1), Q(k), Q(k + 1), Q(k + 2)) : flx3(Q(k + 2), Q(k + 1), Q(k));
-  else if (k == nk) r = (m >= 0.0f) ? flx3(Q(k - 1), Q(k), Q(k + 1)) : 0.5f * (Q(k) + Q(k + 1));
-  else r = (m >= 0.0f) ? flx5(Q(k - 2), Q(k - 1), Q(k), Q(k + 1), Q(k + 2)) : flx5(Q(k + 3), Q(k + 2), Q(k + 1), Q(k), Q(k - 1));
-  return m * r;
-}
-
-template <bool FLAT>
-__global__ __launch_bounds__(256) void k_mom_advw(MomArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, kf = blockIdx.z + 2;
-  const int ni = a.ni, nj = a.nj;
-  if (i >= ni) return;
-  const int nxu = ux(ni), nyu = sy(nj), nxs = sx(ni), nys = sy(nj), nyv = vy(nj);
-  const int y = j + NG, x = i + NG;
-  constexpr bool flat = FLAT;
-  float ax, ay;
-  if (flat) {
-    ax = ((-(wfx(a, kf, j, i + 1) - wfx(a, kf, j, i))) * a.rdx) * a.uh[x];
-    ay = ((-(wfy(a, kf, j + 1, i) - wfy(a, kf, j, i))) * a.rdy) * a.vh[y];
-  } else {
-    ax = (-(wfx(a, kf, j, i + 1) - wfx(a, kf, j, i))) * (a.rdx * a.uh[x]);
-    ay = (-(wfy(a, kf, j + 1, i) - wfy(a, kf, j, i))) * (a.rdy * a.vh[y]);
-  }
-  const int64_t h = (int64_t)j * ni + i;
-  const float c2 = a.c2[(int64_t)(kf - 1) * nj * ni + h];
-  const float c1 = 1.0f - c2;
-  const float wc0 = a.w[ix3(kf, y, x, nys, nxs)];
-  if (i == 0 && opn(a.obc, 0)) {
-    const float* Rt = a.rru + ix3(kf, y, 0, nyu, nxu);
-    const float* Rb = a.rru + ix3(kf - 1, y, 0, nyu, nxu);
-    const float m = c2 * Rt[NG] + c1 * Rb[NG];
-    if (m >= 0.0f) {
-      const float c = c1 * (Rb[NG + 1] - Rb[NG]) + c2 * (Rt[NG + 1] - Rt[NG]);
-      ax = flat ? ax - ((wc0 * c) * a.rdx) * a.uh[x] : ax - wc0 * (c * (a.rdx * a.uh[x]));
-    }
-  } else if (i == ni - 1 && opn(a.obc, 1)) {
-    const float* Rt = a.rru + ix3(kf, y, 0, nyu, nxu);
-    const float* Rb = a.rru + ix3(kf - 1, y, 0, nyu, nxu);
-    const float m = c2 * Rt[NG + ni] + c1 * Rb[NG + ni];
-    if (m <= 0.0f) {
-      const float c = c1 * (Rb[x + 1] - Rb[x]) + c2 * (Rt[x + 1] - Rt[x]);
-      ax = flat ? ax - ((wc0 * c) * a.rdx) * a.uh[x] : ax - wc0 * (c * (a.rdx * a.uh[x]));
-    }
-  }
-  if (j == 0 && opn(a.obc, 2)) {
-    const float m = c2 * a.rrv[ix3(kf, NG, x, nyv, nxs)] + c1 * a.rrv[ix3(kf - 1, NG, x, nyv, nxs)];
-    if (m >= 0.0f) {
-      const float c = c1 * (a.rrv[ix3(kf - 1, NG + 1, x, nyv, nxs)] - a.rrv[ix3(kf - 1, NG, x, nyv, nxs)]) +
-                      c2 * (a.rrv[ix3(kf, NG + 1, x, nyv, nxs)] - a.rrv[ix3(kf, NG, x, nyv, nxs)]);
-      ay = flat ? ay - ((wc0 * c) * a.rdy) * a.vh[y] : ay - wc0 * (c * (a.rdy * a.vh[y]));
-    }
-  } else if (j == nj - 1 && opn(a.obc, 3)) {
-    const float m = c2 * a.rrv[ix3(kf, NG + nj, x, nyv, nxs)] + c1 * a.rrv[ix3(kf - 1, NG + nj, x, nyv, nxs)];
-    if (m <= 0.0f) {
-      const float c = c1 * (a.rrv[ix3(kf - 1, y + 1, x, nyv, nxs)] - a.rrv[ix3(kf - 1, y, x, nyv, nxs)]) +
-                      c2 * (a.rrv[ix3(kf, y + 1, x, nyv, nxs)] - a.rrv[ix3(kf, y, x, nyv, nxs)]);
-      ay = flat ? ay - ((wc0 * c) * a.rdy) * a.vh[y] : ay - wc0 * (c * (a.rdy * a.vh[y]));
-    }
-  }
-  float az = (-(wz(a, kf, j, i) - wz(a, kf - 1, j, i))) * a.rdz;
-  if (flat) az = az * a.flat[(a.nk + 2) + kf];
-  float tot = (ax + ay) + az;
-  tot = tot + wc0 * (c2 * a.divx[(int64_t)(kf - 1) * nj * ni + h] + c1 * a.divx[(int64_t)(kf - 2) * nj * ni + h]);
-  const float sc = flat ? a.rrf0[(int64_t)(kf - 1) * nj * ni] : a.rrf0[(int64_t)(kf - 1) * nj * ni + h] * a.gz[h];
-  const int64_t p = ix3(kf, y, x, nys, nxs);
-  const float wt = (a.w1 ? a.w1 : a.wten)[p] + tot * sc;
-  if (FLAT && a.dum8) {
-    const int64_t q = (int64_t)(kf - 1) * nj * ni + h;
-    a.wten[p] = wt + (a.c1[q] * a.dum8[ix3(kf - 1, y, x, nys, nxs)] + a.c2[q] * a.dum8[p]);
-  } else {
-    a.wten[p] = wt;
-  }
-}
-
 // vertical flux of w at scalar level k (wz) with the column read through Q
 template <class Get>
 GM1_DEV float wz_g(float m, int k, int nk, Get Q) {
@@ -835,7 +700,8 @@ GM1_DEV float wz_g(float m, int k, int nk, Get Q) {
   return m * r;
 }
 
-// advw (the statements of k_mom_advw) as a MTX x MTY tile of cells marching a
+// advw (cuda_momentum_advection5.py:1064, cm1_mom5_advw_v1; the transport of
+// _cpu_momentum_transport5, :871-1012) as a MTX x MTY tile of cells marching a
 // chunk of w levels 2..nk: per level the tile's w plane (3-cell halo) and its
 // c2 plane (1-cell halo, clamped as the face weights clamp) are staged in LDS,
 // every x and y face flux of the tile is formed once into LDS together with
@@ -941,7 +807,7 @@ void k_mom_advw_tile(MomArgs a, int kchunk, int nchunks) {
       v.wnext = (lnext <= nk + 2) ? ldo(a.w + (int64_t)lnext * wst, ow) : 0.0f;
       v.rrwn = ldo(a.rrw + (int64_t)(kf + 1) * wst, ow);
       v.divx = ldo(a.divx + (int64_t)(kf - 1) * plane, oh);
-      v.base = ldo((a.w1 ? a.w1 : a.wten) + (int64_t)kf * wst, ow);
+      v.base = a.zero_held ? 0.0f : ldo((a.w1 ? a.w1 : a.wten) + (int64_t)kf * wst, ow);
       if (FLAT && a.dum8) {
         v.d8 = ldo(a.dum8 + (int64_t)kf * wst, ow);
         v.c1 = ldo(a.c1 + (int64_t)(kf - 1) * plane, oh);
@@ -1091,7 +957,8 @@ void launch_advw_tile(const MomArgs& a, hipStream_t s) {
 // 2..nk (level 1 by k_seed_w_surface) -- the rest of k_seed_held's region
 // (faces 0..ni, rows 0..nj, levels 1..nk) is seeded by the advection updates
 // themselves.  A face both seeds and an update write is seeded first and
-// overwritten in stream order.
+// overwritten in stream order.  Null held tendencies seed +0 (the zeroed
+// held fields of a step without a solve1 contribution).
 __global__ __launch_bounds__(256) void k_seed_edges(const float* __restrict__ u1, const float* __restrict__ v1,
                                                     const float* __restrict__ w1, float* __restrict__ ut,
                                                     float* __restrict__ vt, float* __restrict__ wt, int ni, int nj,
@@ -1104,7 +971,7 @@ __global__ __launch_bounds__(256) void k_seed_edges(const float* __restrict__ u1
     else if (e < ni + 1 + nj) { i = 0; j = e - (ni + 1); }
     else { i = ni; j = e - (ni + 1 + nj); }
     const int64_t p = ix3(k, j + NG, i + NG, sy(nj), ux(ni));
-    ut[p] = u1[p];
+    ut[p] = u1 ? u1[p] : 0.0f;
   }
   if (e < nj + 1 + 2 * ni) {  // v
     int i, j;
@@ -1112,12 +979,12 @@ __global__ __launch_bounds__(256) void k_seed_edges(const float* __restrict__ u1
     else if (e < nj + 1 + ni) { i = e - (nj + 1); j = 0; }
     else { i = e - (nj + 1 + ni); j = nj; }
     const int64_t p = ix3(k, j + NG, i + NG, vy(nj), sx(ni));
-    vt[p] = v1[p];
+    vt[p] = v1 ? v1[p] : 0.0f;
   }
   if (k >= 2 && e < nj + 1 + ni) {  // w above the surface
     const int i = (e <= nj) ? ni : e - (nj + 1), j = (e <= nj) ? e : nj;
     const int64_t p = ix3(k, j + NG, i + NG, sy(nj), sx(ni));
-    wt[p] = w1[p];
+    wt[p] = w1 ? w1[p] : 0.0f;
   }
 }
 
@@ -1127,7 +994,7 @@ __global__ __launch_bounds__(256) void k_seed_w_surface(const float* __restrict_
   if (e >= (ni + 1) * (nj + 1)) return;
   const int i = e % (ni + 1), j = e / (ni + 1);
   const int64_t p = ix3(1, j + NG, i + NG, sy(nj), sx(ni));
-  wt[p] = w1[p];
+  wt[p] = w1 ? w1[p] : 0.0f;
 }
 
 void launch_seed_edges(const float* u1, const float* v1, const float* w1, float* ut, float* vt, float* wt, int ni,
@@ -1146,9 +1013,9 @@ int momentum5_terrain(
     const float* rho0_3d, const float* rf0, const float* rrf0, const float* c2, const float* gz, const float* gzu,
     const float* gzv, const float* rgzu, const float* rgzv, const float* dzdx, const float* dzdy,
     float* divx, int ni, int nj, int nk, float dx, float dy, float dz, unsigned obc,
-    int stop_after_divergence, void* stream) {
+    int stop_after_divergence, void* stream, bool zero_held = false) {
   hipStream_t s = (hipStream_t)stream;
-  if (u1) {
+  if (u1 || zero_held) {
     launch_seed_edges(u1, v1, w1, uten, vten, wten, ni, nj, nk, s);
     GM1_LAUNCH_CHECK();
   }
@@ -1167,7 +1034,7 @@ int momentum5_terrain(
   a.uten = uten; a.vten = vten; a.wten = wten; a.ni = ni; a.nj = nj; a.nk = nk;
   a.rdx = rdx; a.rdy = rdy; a.rdz = rdz; a.obc = obc;
   a.flat = nullptr; a.rr0 = nullptr;
-  a.u1 = u1; a.v1 = v1; a.w1 = w1;
+  a.u1 = u1; a.v1 = v1; a.w1 = w1; a.zero_held = zero_held ? 1 : 0;
   const bool pw = side_of(obc, 0) == 1u, pe = side_of(obc, 1) == 1u;
   const bool ps = side_of(obc, 2) == 1u, pn = side_of(obc, 3) == 1u;
   const int f0 = pw ? 0 : 1, nf = ni - 1 + (pw ? 1 : 0) + (pe ? 1 : 0);
@@ -1189,10 +1056,10 @@ int momentum5_flat(
     float* uten, float* vten, float* wten, float* rru, float* rrv, float* rrw, const float* uh, const float* vh,
     const float* rho0_3d, const float* rf0, const float* rr0, const float* rrf0, const float* c1, const float* c2,
     const float* dum8, const float* flat_column, float* divx, int ni, int nj, int nk, float dx, float dy, float rdz,
-    unsigned obc, void* stream) {
+    unsigned obc, void* stream, bool zero_held = false) {
   if (!flat_column || (dum8 && !c1)) return (int)hipErrorInvalidValue;
   hipStream_t s = (hipStream_t)stream;
-  if (u1) {
+  if (u1 || zero_held) {
     launch_seed_edges(u1, v1, w1, uten, vten, wten, ni, nj, nk, s);
     GM1_LAUNCH_CHECK();
   }
@@ -1209,7 +1076,7 @@ int momentum5_flat(
   a.rho0 = rho0_3d; a.c2 = c2; a.gzu = nullptr; a.gzv = nullptr; a.rrf0 = rrf0; a.gz = nullptr; a.uh = uh;
   a.vh = vh; a.uten = uten; a.vten = vten; a.wten = wten; a.ni = ni; a.nj = nj; a.nk = nk;
   a.rdx = rdx; a.rdy = rdy; a.rdz = rdz; a.obc = obc; a.flat = flat_column; a.rr0 = rr0;
-  a.u1 = u1; a.v1 = v1; a.w1 = w1; a.dum8 = dum8; a.c1 = c1;
+  a.u1 = u1; a.v1 = v1; a.w1 = w1; a.dum8 = dum8; a.c1 = c1; a.zero_held = zero_held ? 1 : 0;
   const bool pw = side_of(obc, 0) == 1u, pe = side_of(obc, 1) == 1u;
   const bool ps = side_of(obc, 2) == 1u, pn = side_of(obc, 3) == 1u;
   const int f0 = pw ? 0 : 1, nf = ni - 1 + (pw ? 1 : 0) + (pe ? 1 : 0);
@@ -1246,10 +1113,12 @@ int gm1hip_momentum_stage(
     const float* gz, const float* gzu, const float* gzv, const float* rgzu, const float* rgzv, const float* dzdx,
     const float* dzdy, float* divx, int ni, int nj, int nk, float dx, float dy, float dz, unsigned obc,
     void* stream) {
-  if (!uten1 || !vten1 || !wten1) return (int)hipErrorInvalidValue;
+  // all three held tendencies null: they are +0 (no solve1 contribution) and are not read
+  const bool zero_held = !uten1 && !vten1 && !wten1;
+  if (!zero_held && (!uten1 || !vten1 || !wten1)) return (int)hipErrorInvalidValue;
   return momentum5_terrain(u3d, v3d, w3d, uten1, vten1, wten1, uten, vten, wten, rru, rrv, rrw, uh, vh, rho0_3d,
                            rf0, rrf0, c2, gz, gzu, gzv, rgzu, rgzv, dzdx, dzdy, divx, ni, nj, nk, dx, dy, dz, obc, 0,
-                           stream);
+                           stream, zero_held);
 }
 
 int gm1hip_momentum_transport5(
@@ -1296,9 +1165,11 @@ int gm1hip_momentum_stage_flat(
     const float* uh, const float* vh, const float* rho0_3d, const float* rf0, const float* rr0, const float* rrf0,
     const float* c1, const float* c2, const float* dum8, const float* flat_column, float* divx, int ni, int nj,
     int nk, float dx, float dy, float rdz, unsigned obc, void* stream) {
-  if (!uten1 || !vten1 || !wten1 || !dum8 || !c1) return (int)hipErrorInvalidValue;
+  // all three held tendencies null: they are +0 (no solve1 contribution) and are not read
+  const bool zero_held = !uten1 && !vten1 && !wten1;
+  if ((!zero_held && (!uten1 || !vten1 || !wten1)) || !dum8 || !c1) return (int)hipErrorInvalidValue;
   return momentum5_flat(u3d, v3d, w3d, uten1, vten1, wten1, uten, vten, wten, rru, rrv, rrw, uh, vh, rho0_3d, rf0,
-                        rr0, rrf0, c1, c2, dum8, flat_column, divx, ni, nj, nk, dx, dy, rdz, obc, stream);
+                        rr0, rrf0, c1, c2, dum8, flat_column, divx, ni, nj, nk, dx, dy, rdz, obc, stream, zero_held);
 }
 
 }  // extern "C"

@@ -555,14 +555,15 @@ __global__ __launch_bounds__(256) void k_held_compose(
     const float* __restrict__ ppi, const float* __restrict__ tha, const float* __restrict__ qa,
     float* __restrict__ thheld, float* __restrict__ qheld, float* __restrict__ ppten,
     const float* __restrict__ qpten, const float* __restrict__ qtten, const float* __restrict__ qvten,
-    const float* __restrict__ qcten, int ni, int nj, int nk, float dt) {
+    const float* __restrict__ qcten, int ni, int nj, int nk, float dt, int zero_held) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y, k = blockIdx.z;
   if (i >= ni) return;
   const float rp00 = 1.0f / 100000.0f, rddcv = RD / (CP - RD), tem = 0.0001f * 0.0001f, qsmall = 1.0e-14f;
   const int64_t n = (int64_t)ni * nj * nk, z = ((int64_t)k * nj + j) * ni + i;
   const int64_t p = pix(k + 1, j + 1, i + 1, ni, nj);
-  const float ht = thheld[z], qvt = qheld[z];
+  // zero_held: the held fields are +0 (no solve1 contribution) and are not read
+  const float ht = zero_held ? 0.0f : thheld[z], qvt = zero_held ? 0.0f : qheld[z];
   if (fabsf(dt * ht) > tem || fabsf(dt * qvt) > qsmall) {
     const float thnew = tha[z] + dt * ht, qvnew = qa[z] + dt * qvt;
     float base = (rho[z] * (th0[k] + thnew)) * (RD + RV * qvnew);
@@ -574,7 +575,7 @@ __global__ __launch_bounds__(256) void k_held_compose(
   }
   thheld[z] = ht + qtten[z];
   qheld[z] = qvt + qvten[z];
-  qheld[n + z] = qheld[n + z] + qcten[z];
+  qheld[n + z] = (zero_held ? 0.0f : qheld[n + z]) + qcten[z];
 }
 
 // ----------------------------------------------------------- solve3
@@ -712,9 +713,9 @@ int gm1hip_seed_held_momentum(const float* uten1, const float* vten1, const floa
 int gm1hip_held_compose(const float* th0, const float* pi0, const float* rho, const float* ppi, const float* tha,
                         const float* qa, float* thheld, float* qheld, float* ppten, const float* qpten,
                         const float* qtten, const float* qvten, const float* qcten, int ni, int nj, int nk, float dt,
-                        void* stream) {
+                        int zero_held, void* stream) {
   hipLaunchKernelGGL(k_held_compose, dim3((ni + 63) / 64, nj, nk), dim3(64), 0, (hipStream_t)stream, th0, pi0, rho,
-                     ppi, tha, qa, thheld, qheld, ppten, qpten, qtten, qvten, qcten, ni, nj, nk, dt);
+                     ppi, tha, qa, thheld, qheld, ppten, qpten, qtten, qvten, qcten, ni, nj, nk, dt, zero_held ? 1 : 0);
   GM1_LAUNCH_CHECK();
   return 0;
 }

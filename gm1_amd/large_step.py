@@ -55,7 +55,7 @@ signature("gm1hip_rk_compose", P, P, P, P, P, ctypes.c_int64, Fl, P)
 signature("gm1hip_seed_held_momentum", P, P, P, P, P, P, I, I, I, P)
 signature("gm1hip_momentum_stage", *([P] * 26), I, I, I, Fl, Fl, Fl, U, P)
 signature("gm1hip_momentum_stage_flat", *([P] * 23), I, I, I, Fl, Fl, Fl, U, P)
-signature("gm1hip_held_compose", *([P] * 13), I, I, I, Fl, P)
+signature("gm1hip_held_compose", *([P] * 13), I, I, I, Fl, I, P)
 signature("gm1hip_solve3_pressure", P, P, P, P, I, I, I, U, P)
 signature("gm1hip_solve3_density", P, P, P, P, P, I, I, I, P)
 
@@ -158,6 +158,10 @@ class Gm1LargeStepState:
     receipts: list = field(default_factory=list)
     # a stretched (flat) column runs the terrain_flag=.false. family
     flat: Any = None
+    # uten1, vten1, wten1 and qten's species beyond qc hold +0 since their last fill and no solve1
+    # hook has run: a step without a hook then skips the fills and its passes form 0 + x themselves
+    held_clean: bool = False
+    zero_held: bool = False
 
 
 def slopes_are_zero(metrics: dict[str, Any]) -> bool:
@@ -256,8 +260,9 @@ def _rk_stage(st: Gm1LargeStepState, nrk: int, alternate: bool = False) -> None:
         _native.call("gm1hip_bc2d_open", ptr(a["w3d"]), ni, nj, h)
     # the held seed (uten = uten1, ...) folded into the momentum advection updates: the seed
     # touches no array bcwsfc/bc2d read or write, so it moves past them unchanged
-    _native.call("gm1hip_momentum_stage", ptr(a["u3d"]), ptr(a["v3d"]), ptr(a["w3d"]), ptr(a["uten1"]),
-                 ptr(a["vten1"]), ptr(a["wten1"]), ptr(a["uten"]), ptr(a["vten"]), ptr(a["wten"]), ptr(a["rru"]),
+    held = (None, None, None) if st.zero_held else (a["uten1"], a["vten1"], a["wten1"])
+    _native.call("gm1hip_momentum_stage", ptr(a["u3d"]), ptr(a["v3d"]), ptr(a["w3d"]), *(ptr(x) for x in held),
+                 ptr(a["uten"]), ptr(a["vten"]), ptr(a["wten"]), ptr(a["rru"]),
                  ptr(a["rrv"]), ptr(a["rrw"]), ptr(a["uh"]), ptr(a["vh"]), ptr(m["rho0_3d"]), ptr(m["rf0"]),
                  ptr(m["rrf0"]), ptr(m["c2"]), ptr(m["gz"]), ptr(m["gzu"]), ptr(m["gzv"]), ptr(m["rgzu"]),
                  ptr(m["rgzv"]), ptr(m["dzdx"]), ptr(m["dzdy"]), ptr(a["divx"]), ni, nj, nk, cfg.dx, cfg.dy, cfg.dz,
@@ -291,8 +296,9 @@ def _flat_rk_stage(st: Gm1LargeStepState, nrk: int, alternate: bool = False) -> 
     sides = cfg.lateral_sides
     # the held seed, the flat momentum advection and the flat buoyancy in three passes
     # (the seed folded into the advection updates, the buoyancy into the w update)
-    _native.call("gm1hip_momentum_stage_flat", ptr(a["u3d"]), ptr(a["v3d"]), ptr(a["w3d"]), ptr(a["uten1"]),
-                 ptr(a["vten1"]), ptr(a["wten1"]), ptr(a["uten"]), ptr(a["vten"]), ptr(a["wten"]), ptr(a["rru"]),
+    held = (None, None, None) if st.zero_held else (a["uten1"], a["vten1"], a["wten1"])
+    _native.call("gm1hip_momentum_stage_flat", ptr(a["u3d"]), ptr(a["v3d"]), ptr(a["w3d"]), *(ptr(x) for x in held),
+                 ptr(a["uten"]), ptr(a["vten"]), ptr(a["wten"]), ptr(a["rru"]),
                  ptr(a["rrv"]), ptr(a["rrw"]), ptr(a["uh"]), ptr(a["vh"]), ptr(m["rho0_3d"]), ptr(m["rf0"]),
                  ptr(m["rr0"]), ptr(m["rrf0"]), ptr(m["c1"]), ptr(m["c2"]), ptr(a["dum8"]), ptr(st.flat.column),
                  ptr(a["divx"]), ni, nj, nk, cfg.dx, cfg.dy, float(st.flat.rdz), lateral_code(sides),
@@ -397,11 +403,16 @@ def launch_cuda_gm1_large_step(
     ni, nj, nk = cfg.ni, cfg.nj, cfg.nk
     h = stream_handle(st.stream)
     code = lateral_code(cfg.lateral_sides)
-    with torch.cuda.stream(st.stream) if st.stream is not None else _nullcontext():
-        for name in ("uten1", "vten1", "wten1", "thten1", "ppten"):
-            a[name].zero_()
-        if st.numq:
-            a["qten"].zero_()
+    # the held tendencies start each step at +0; with no solve1 hook and the fields still +0 from
+    # their last fill, the passes that read them form 0 + x without the fills (the same words)
+    st.zero_held = solve1_hook is None and st.held_clean
+    if not st.zero_held:
+        with torch.cuda.stream(st.stream) if st.stream is not None else _nullcontext():
+            for name in ("uten1", "vten1", "wten1", "thten1", "ppten"):
+                a[name].zero_()
+            if st.numq:
+                a["qten"].zero_()
+    st.held_clean = solve1_hook is None and post_stage_hook is None  # (a hook may write the held fields)
     solve1_result = None
     if solve1_hook is not None:
         carriers = Gm1Solve1Carriers(a["uten1"], a["vten1"], a["wten1"], a["thten1"], a.get("qten"), a["ppten"])
@@ -410,7 +421,8 @@ def launch_cuda_gm1_large_step(
         b = st.base
         _native.call("gm1hip_held_compose", ptr(b["th0"]), ptr(b["pi0"]), ptr(a["rho"]), ptr(a["ppi"]),
                      ptr(a["tha"]), ptr(a["qa"]), ptr(a["thten1"]), ptr(a["qten"]), ptr(a["ppten"]),
-                     ptr(a["qpten"]), ptr(a["qtten"]), ptr(a["qvten"]), ptr(a["qcten"]), ni, nj, nk, cfg.dt, h)
+                     ptr(a["qpten"]), ptr(a["qtten"]), ptr(a["qvten"]), ptr(a["qcten"]), ni, nj, nk, cfg.dt,
+                     int(st.zero_held), h)
     stage_hooks = []
     # with no hook between stages, the scalars alternate two carriers (one copy-back per step)
     alternate = post_stage_hook is None

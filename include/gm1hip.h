@@ -217,7 +217,9 @@ int gm1hip_seed_held_momentum(const float* uten1, const float* vten1, const floa
 // The held seed and the terrain momentum advection of one RK stage
 // (cuda_momentum_rk.py:303-372, statements 1 and 3): every word equals
 // gm1hip_seed_held_momentum followed by gm1hip_momentum_advection5, the seed
-// folded into the advection updates.
+// folded into the advection updates.  All three held tendencies null: they
+// are +0 (a step without a solve1 contribution) and are not read; every word
+// equals the seed of zeroed fields followed by the advection.
 int gm1hip_momentum_stage(const float* u3d, const float* v3d, const float* w3d, const float* uten1,
                           const float* vten1, const float* wten1, float* uten, float* vten, float* wten, float* rru,
                           float* rrv, float* rrw, const float* uh, const float* vh, const float* rho0_3d,
@@ -227,10 +229,14 @@ int gm1hip_momentum_stage(const float* u3d, const float* v3d, const float* w3d, 
                           unsigned obc, void* stream);
 // solve1 held compose: ppten from held theta/qv and the microphysics held
 // rates (kernel gm1_predict_and_compose_held_v1, cuda_gm1_forecast.py:555-566).
+// zero_held != 0 states that thheld and qheld hold +0 (a step without a solve1
+// contribution): they are not read, every word written is the same as with
+// zeroed fields, and qheld's species beyond qc are left as they are (the
+// caller keeps them +0).
 int gm1hip_held_compose(const float* th0, const float* pi0, const float* rho, const float* ppi, const float* tha,
                         const float* qa, float* thheld, float* qheld, float* ppten, const float* qpten,
                         const float* qtten, const float* qvten, const float* qcten, int ni, int nj, int nk, float dt,
-                        void* stream);
+                        int zero_held, void* stream);
 // solve3 pressure: ppx = pp3d + ppx, ppi = pp3d and their halos
 // (cuda_gm1_forecast.py:1414-1435, _finish_solve3).  With pp3d_mass (the
 // microphysics' (nk,nj,ni) pp3d, may be NULL) the new pp3d comes from it and
@@ -359,7 +365,8 @@ int gm1hip_buoyancy_flat(const float* dum8, const float* c1, const float* c2, fl
 // (cuda_cartesian_dynamics.py:1070-1262: the held seed of cuda_momentum_rk.py:242-271,
 // launch_cuda_gm1_child_stretched_momentum5, launch_cuda_gm1_flat_stretched_buoyancy) in
 // three passes: every word equals gm1hip_seed_held_momentum, then
-// gm1hip_momentum_advection5_flat, then gm1hip_buoyancy_flat.
+// gm1hip_momentum_advection5_flat, then gm1hip_buoyancy_flat.  Null uten1,
+// vten1 and wten1 as in gm1hip_momentum_stage.
 int gm1hip_momentum_stage_flat(const float* u3d, const float* v3d, const float* w3d, const float* uten1,
                                const float* vten1, const float* wten1, float* uten, float* vten, float* wten,
                                float* rru, float* rrv, float* rrw, const float* uh, const float* vh,
