@@ -40,6 +40,78 @@ def cfgd(cfg):
             "nsound": cfg.nsound, "lateral_sides": cfg.lateral_sides}
 
 
+# ----------------------------------------------------------------------------- CPU: the boundary mirror's checks
+def test_boundary_mirror_shapes_and_refusals():
+    """The family extents (cuda_boundary_families.py:844-853 for ngxy = 3) and the grids, kinds and side words
+    the fills refuse before anything is enqueued."""
+    from gm1_amd import boundary
+
+    assert boundary.family_array_shape(0, ni=10, nj=8, nk=5) == (7, 14, 16)
+    assert boundary.family_array_shape(1, ni=10, nj=8, nk=5) == (7, 14, 17)
+    assert boundary.family_array_shape(2, ni=10, nj=8, nk=5) == (7, 15, 16)
+    assert boundary.family_array_shape(3, ni=10, nj=8, nk=5) == (8, 14, 16)
+    assert boundary.pressure_array_shape(ni=10, nj=8, nk=5) == (7, 10, 12)
+    for bad in ({"ni": 3, "nj": 8, "nk": 5}, {"ni": 10, "nj": 8, "nk": 2}, {"ni": True, "nj": 8, "nk": 5}):
+        with pytest.raises(ValueError):
+            boundary.family_array_shape(0, **bad)
+    with pytest.raises(ValueError):
+        boundary.family_array_shape(4, ni=10, nj=8, nk=5)
+    with pytest.raises(ValueError):  # a side word outside 1..4
+        boundary.fill_pressure(None, (2, 2, 5, 2), ni=10, nj=8, nk=5)
+    with pytest.raises(ValueError):  # the tile-local word has no family to fill
+        boundary.fill_family(None, 0, None, ni=10, nj=8, nk=5)
+    with pytest.raises(TypeError):  # not a device tensor: refused before any launch
+        boundary.fill_family(np.zeros((7, 14, 16), np.float32), 0, (1, 1, 2, 3), ni=10, nj=8, nk=5)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sides", [(1, 1, 1, 1), (2, 2, 2, 2), (3, 4, 2, 2), (2, 2, 4, 3)])
+def test_gpu_boundary_mirror_fills_match_the_oracle(device, sides):
+    """Every fill through the mirror on random carriers equals the oracle's fill word for word (mixed families
+    included: periodic, open-radiative, rigid free-slip and no-slip on different sides)."""
+    import torch
+
+    from gm1_amd import boundary
+
+    ni, nj, nk = 13, 10, 6
+    rng = np.random.default_rng(11)
+    print("boundary mirror fills", sides, flush=True)
+    for kind in (0, 1, 2, 3):
+        h = rng.standard_normal(boundary.family_array_shape(kind, ni=ni, nj=nj, nk=nk)).astype(np.float32)
+        d = torch.from_numpy(h.copy()).to(device)
+        boundary.fill_family(d, kind, sides, ni=ni, nj=nj, nk=nk)
+        obc.fill(h, kind, sides, ni=ni, nj=nj, nk=nk)
+        torch.cuda.synchronize()
+        assert_bits_equal(to_host(d), h, f"kind {kind}")
+    p = rng.standard_normal(boundary.pressure_array_shape(ni=ni, nj=nj, nk=nk)).astype(np.float32)
+    dp = torch.from_numpy(p.copy()).to(device)
+    boundary.fill_pressure(dp, sides, ni=ni, nj=nj, nk=nk)
+    obc.bcp(p, sides, ni=ni, nj=nj, nk=nk)
+    w = rng.standard_normal(boundary.family_array_shape(3, ni=ni, nj=nj, nk=nk)).astype(np.float32)
+    dw = torch.from_numpy(w.copy()).to(device)
+    boundary.w_rigid(dw, ni=ni, nj=nj, nk=nk)
+    obc.bcw_rigid(w, ni=ni, nj=nj, nk=nk)
+    ha = {n: rng.standard_normal(boundary.family_array_shape(kd, ni=ni, nj=nj, nk=nk)).astype(np.float32)
+          for n, kd in (("u3d", 1), ("v3d", 2))}
+    ha["w3d"] = w
+    hm = {n: rng.standard_normal((nj, ni)).astype(np.float32) for n in ("dzdx", "dzdy", "gz")}
+    da = {n: torch.from_numpy(x.copy()).to(device) for n, x in ha.items() if n != "w3d"}
+    da["w3d"] = dw
+    dm = {n: torch.from_numpy(x).to(device) for n, x in hm.items()}
+    boundary.bcwsfc(da, dm, ni=ni, nj=nj)
+    obc.bcwsfc(ha, hm, ni=ni, nj=nj, nk=nk)
+    boundary.bc2d_open(dw, ni=ni, nj=nj)
+    obc.bc2d_open_w_surface(w, ni=ni, nj=nj)
+    torch.cuda.synchronize()
+    assert_bits_equal(to_host(dp), p, "pressure")
+    assert_bits_equal(to_host(dw), w, "w rigid + bcwsfc + bc2d")
+    su, sw = tuple(da["u3d"].shape), tuple(dw.shape)
+    shared = dw.new_zeros(max(int(np.prod(su)), int(np.prod(sw))))
+    with pytest.raises(ValueError):  # w aliasing u is refused before the launch
+        boundary.bcwsfc({"u3d": shared[: int(np.prod(su))].view(su), "v3d": da["v3d"],
+                         "w3d": shared[: int(np.prod(sw))].view(sw)}, dm, ni=ni, nj=nj)
+
+
 # ----------------------------------------------------------------------------- CPU: refusals before any launch
 
 
