@@ -116,47 +116,6 @@ __device__ __forceinline__ float recon_gated(float m, float s1, float s2, float 
   return m * recon5(s1, s2, s3, s4, s5, WENO, weps);
 }
 
-template <bool WENO, bool GATE>
-__device__ __forceinline__ float xflux(const float* __restrict__ s, const float* __restrict__ rru,
-                                       int k, int y, int x, int nys, int nxs, int nxu, double weps) {
-  float m = rru[ix3(k, y, x, nys, nxu)];
-  const float* row = s + ix3(k, y, 0, nys, nxs);
-  return (m >= 0.0f) ? recon_gated<WENO, GATE>(m, row[x - 3], row[x - 2], row[x - 1], row[x], row[x + 1], weps)
-                     : recon_gated<WENO, GATE>(m, row[x + 2], row[x + 1], row[x], row[x - 1], row[x - 2], weps);
-}
-
-template <bool WENO, bool GATE>
-__device__ __forceinline__ float yflux(const float* __restrict__ s, const float* __restrict__ rrv,
-                                       int k, int y, int x, int nys, int nxs, int nyv, double weps) {
-  float m = rrv[ix3(k, y, x, nyv, nxs)];
-  const float* col = s + ix3(k, 0, x, nys, nxs);
-  const int64_t st = nxs;
-  return (m >= 0.0f)
-             ? recon_gated<WENO, GATE>(m, col[(y - 3) * st], col[(y - 2) * st], col[(y - 1) * st], col[y * st],
-                                       col[(y + 1) * st], weps)
-             : recon_gated<WENO, GATE>(m, col[(y + 2) * st], col[(y + 1) * st], col[y * st], col[(y - 1) * st],
-                                       col[(y - 2) * st], weps);
-}
-
-// x- and y-face fluxes from a level plane staged in LDS: lp[r][c] holds the
-// padded scalar at row j0+NG-3+r, column i0+NG-3+c of the tile's level.
-constexpr int LPX = TX + 7;  // TX + 6 halo columns, +1 against bank aliasing
-template <bool WENO, bool GATE>
-__device__ __forceinline__ float xflux_lds(const float (*lp)[LPX], float m, int r, int t, double weps) {
-  const float* row = lp[r + 3] + t;  // row[3] is the cell east of face t
-  return (m >= 0.0f) ? recon_gated<WENO, GATE>(m, row[0], row[1], row[2], row[3], row[4], weps)
-                     : recon_gated<WENO, GATE>(m, row[5], row[4], row[3], row[2], row[1], weps);
-}
-
-template <bool WENO, bool GATE>
-__device__ __forceinline__ float yflux_lds(const float (*lp)[LPX], float m, int r, int t, double weps) {
-  const int c = t + 3;  // lp[r + 3][c] is the cell north of face r
-  return (m >= 0.0f) ? recon_gated<WENO, GATE>(m, lp[r][c], lp[r + 1][c], lp[r + 2][c], lp[r + 3][c], lp[r + 4][c],
-                                               weps)
-                     : recon_gated<WENO, GATE>(m, lp[r + 5][c], lp[r + 4][c], lp[r + 3][c], lp[r + 2][c],
-                                               lp[r + 1][c], weps);
-}
-
 // z-face flux at face k (2..nk) of one column with the lower-order closures.
 // S(level) reads the column's scalar at a level.
 template <bool WENO, class Get>
@@ -181,13 +140,6 @@ __device__ __forceinline__ float zflux_g(Get S, float m, float cc1, float cc2, i
     else r = recon5(S(k + 2), S(k + 1), S(k), S(k - 1), S(k - 2), WENO, weps);
   }
   return m * r;
-}
-
-template <bool WENO>
-__device__ __forceinline__ float zflux(const float* __restrict__ col, int64_t st,
-                                       const float* __restrict__ rrw, int64_t wp,
-                                       float cc1, float cc2, int k, int nk, double weps) {
-  return zflux_g<WENO>([&](int level) { return col[(int64_t)level * st]; }, rrw[wp], cc1, cc2, k, nk, weps);
 }
 
 // z-face flux of the positive-definite chain (cm1_rk_pdef_vertical_flux_v1):
@@ -215,13 +167,6 @@ __device__ __forceinline__ float zflux_pdef_g(Get S, float m, float cc1, float c
     }
   }
   return m * r;
-}
-
-__device__ __forceinline__ float zflux_pdef(const float* __restrict__ col, int64_t st,
-                                            const float* __restrict__ rrw, int64_t wp, float cc1, float cc2,
-                                            int k, int nk, double weps, bool pdefweno) {
-  return zflux_pdef_g([&](int level) { return col[(int64_t)level * st]; }, rrw[wp], cc1, cc2, k, nk, weps,
-                      pdefweno);
 }
 
 // Output modes of the fused tile kernel.
@@ -287,31 +232,34 @@ GM1_DEV int family_cell(int c, int n, unsigned lo, unsigned hi) {
   return c;
 }
 
-// The padded scalar's word (k, yy, xx) formed from an unpadded source: the
-// lateral family's image of a halo cell (periodic wrap, open clamp, rigid
-// mirror -- the cells bcs writes and the face stencils read) and, for theta,
-// the pack (th0 - 300) + theta' rounded once.
-GM1_DEV float packed_word(const float* __restrict__ src, const float* __restrict__ th0, int k, int yy, int xx,
-                          int ni, int nj, unsigned obc) {
-  const int j = family_cell(yy - NG, nj, side_of(obc, 2), side_of(obc, 3));
-  const int i = family_cell(xx - NG, ni, side_of(obc, 0), side_of(obc, 1));
-  const float v = src[((int64_t)(k - 1) * nj + j) * ni + i];
-  return th0 ? (th0[k - 1] - 300.0f) + v : v;
+
+// The fused tile kernel, one lane per face.  Each lane forms its own cell's
+// west x face and south y face from its own mass-flux words (wavefront TY-1
+// also the tile's east faces, wavefront TY-2 its north faces), so no slot
+// index is formed per level; the level plane's slots keep their byte offsets within
+// a level (from an unpadded carrier, the lateral family's image of each halo
+// cell) in LDS, formed once per workgroup.  A lane whose face lies outside the
+// domain reads word 0 and its flux is never consumed.  FLAT: the flat
+// stretched column's text (a.mh set); SRC: the scalar read from its unpadded
+// carrier (a.src) rather than a padded array (a.s).
+constexpr int LQ = TX + 6;  // the level plane's row pitch
+static_assert(TX == 64, "a wavefront is one tile row");
+
+template <bool WENO, bool GATE>
+__device__ __forceinline__ float face_flux(float m, float q0, float q1, float q2, float q3, float q4, float q5,
+                                           double weps) {
+  const bool up = m >= 0.0f;  // the five cells upwind of the face: q0..q4, or q5..q1 against the flow
+  return recon_gated<WENO, GATE>(m, up ? q0 : q5, up ? q1 : q4, up ? q2 : q3, up ? q3 : q2, up ? q4 : q1, weps);
 }
 
-constexpr uint32_t NO_WORD = 0xFFFFFFFFu;  // a slot with no word to read
-
-// FLAT: the flat stretched column's text (a.mh set), fixed at compile time so
-// the terrain instance carries none of its branches.  SRC: the scalar is read
-// from its unpadded carrier (a.src) rather than a padded array (a.s); flux
-// outputs exist only in the tendency mode -- each instance keeps only the
-// pointers it reads live in scalar registers.
 template <bool WENO, int MODE, bool FLAT, bool SRC>
-__device__ __forceinline__ void scalar_adv5_body(const ScalarAdvArgs& a) {
+__device__ __forceinline__ void scalar_adv5_body_lanes(const ScalarAdvArgs& a) {
   constexpr bool GATE = (MODE == MODE_PDEF);
+  constexpr int PS = ((TY + 6) * LQ + NT - 1) / NT;
   __shared__ float lfx[TY][TX + 1];
   __shared__ float lfy[TY + 1][TX];
-  __shared__ float lp[TY + 6][LPX];  // the tile's level plane with a 3-cell halo
+  __shared__ float lp[PS * NT];       // the level plane, TY+6 rows of LQ (slots past them unread)
+  __shared__ uint32_t lpo[PS * NT];   // each plane slot's byte offset within a level
   const int ni = a.ni, nj = a.nj, nk = a.nk;
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   // tiles ordered i fastest, then j, then column chunk; each XCD sweeps a
@@ -326,320 +274,38 @@ __device__ __forceinline__ void scalar_adv5_body(const ScalarAdvArgs& a) {
   const bool live = (i < ni) && (j < nj);
   const unsigned obc = a.obc;
   const bool owned = live && (obc != 0u || (i >= 3 && i + 3 < ni && j >= 3 && j + 3 < nj));
+  // an owned cell on an open side reads its four faces' mass fluxes for the outflow closure
+  const bool edge = owned && obc != 0u && (i == 0 || i + 1 == ni || j == 0 || j + 1 == nj);
   const int x = i + NG, y = j + NG;
-  const int64_t st = (int64_t)nys * nxs;  // level stride of scalar and w arrays
-  // a level's words sit at 32-bit byte offsets from the level's base: the
-  // own column in the padded (wcol) and unpadded (mcol) families
-  const uint32_t wcol = (uint32_t)(y * nxs + x) * 4u;
-  const uint32_t mcol = (uint32_t)(j * ni + i) * 4u;
-  const int64_t mst = (int64_t)nj * ni;
-
-  // The column's scalar S(k-3..k+3) at level k rides in registers: win[q]
-  // holds level k-3+q (0 outside the padded column; those words are never
-  // read by a face's stencil, which stays within levels 1..nk).
-  float win[7];
-  // the own column is a physical cell, so its unpadded word needs no lateral image
-  auto colword = [&](int lev) {  // the column's word at a level, 0 where no stencil reads
-    if constexpr (SRC) {
-      if (lev < 1 || lev > nk) return 0.0f;
-      const float v = ldo(a.src + (int64_t)(lev - 1) * mst, mcol);
-      return a.th0 ? (a.th0[lev - 1] - 300.0f) + v : v;
-    }
-    return (lev >= 0 && lev <= nk + 1) ? ldo(a.s + (int64_t)lev * st, wcol) : 0.0f;
-  };
-  // a tile whose plane (3-cell halo included) lies inside the domain reads the
-  // unpadded source directly; only edge tiles form the lateral family's image
-  const bool inner = SRC && j0 >= 3 && j0 + TY + 3 <= nj && i0 >= 3 && i0 + TX + 3 <= ni;
-  const float* sbase = SRC ? a.src + (int64_t)(j0 - NG) * ni + (i0 - NG) : nullptr;
-#pragma unroll
-  for (int q = 0; q < 7; ++q) win[q] = live ? colword(kb - 3 + q) : 0.0f;
-  // z-face flux at face kf from the window b, b[0] = S(kf-3).
-  auto zf = [&](int kf, const float* b) {
-    const float m = ldo(a.rrw + (int64_t)kf * st, wcol);
-    const uint32_t zq = a.c12col ? 0u : mcol;
-    const float cc1 = ldo(a.c1 + (int64_t)(kf - 1) * mst, zq), cc2 = ldo(a.c2 + (int64_t)(kf - 1) * mst, zq);
-    auto S = [&](int level) { return b[level - (kf - 3)]; };
-    if (MODE == MODE_PDEF) return zflux_pdef_g(S, m, cc1, cc2, kf, nk, a.weps, a.pdefweno != 0);
-    return zflux_g<WENO>(S, m, cc1, cc2, kf, nk, a.weps);
-  };
-  float fzb = 0.0f;
-  if (live && kb >= 2) fzb = zf(kb, win);  // S(kb-3..)
-  // A level's plane and face mass fluxes are requested one level ahead into
-  // registers (Lv), and its cell words at the top of the level, so they
-  // arrive while the level before it (or its fluxes) is formed.  Measured at
-  // 320x320x80: 5th-order compose 168 -> 151 us, pdef tendencies 176 -> 172
-  // us; the WENO compose (weights in binary64, register-bound) 255 -> 295 us,
-  // and no arrangement of its loads tried gave its lost occupancy back.
-  // Slot loops below run at most PS / XS / YS times per thread.
-  constexpr int PS = ((TY + 6) * (TX + 6) + NT - 1) / NT;
-  constexpr int XS = (TY * (TX + 1) + NT - 1) / NT;
-  constexpr int YS = ((TY + 1) * TX + NT - 1) / NT;
-  struct Lv {
-    float plane[PS], mu[XS], mv[YS];
-    float snext;
-  };
-  auto fetch = [&](int k) {
-    Lv v;
-    v.snext = live ? colword(k + 4) : 0.0f;  // the level entering the window after level k
-    // the level plane: rows j0-3..j0+TY+2, columns i0-3..i0+TX+2 (padded
-    // indices j0..j0+TY+5, i0..i0+TX+5), clipped to the padded array
-    const float* plane = SRC ? nullptr : a.s + (int64_t)k * st;
-    const float pack = (SRC && a.th0) ? a.th0[k - 1] - 300.0f : 0.0f;
-    const float* splane = inner ? sbase + (int64_t)(k - 1) * mst : nullptr;
-#pragma unroll
-    for (int u = 0; u < PS; ++u) {
-      int slot = threadIdx.x + u * NT;
-      asm volatile("" : "+v"(slot));  // recomputed per level, not held across the march
-      const int r = slot / (TX + 6), c = slot % (TX + 6);
-      const int yy = j0 + r, xx = i0 + c;
-      v.plane[u] = 0.0f;
-      if (slot < (TY + 6) * (TX + 6) && yy < nys && xx < nxs) {
-        if constexpr (SRC) {
-          if (inner) {
-            const float w = ldo(splane, (uint32_t)(r * ni + c) * 4u);
-            v.plane[u] = a.th0 ? pack + w : w;
-          } else {
-            v.plane[u] = packed_word(a.src, a.th0, k, yy, xx, ni, nj, obc);
-          }
-        } else {
-          v.plane[u] = ldo(plane, (uint32_t)(yy * nxs + xx) * 4u);
-        }
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < XS; ++u) {
-      int slot = threadIdx.x + u * NT;
-      asm volatile("" : "+v"(slot));  // recomputed per level, not held across the march
-      const int r = slot / (TX + 1), t = slot % (TX + 1);
-      const int f = i0 + t, jj = j0 + r;
-      v.mu[u] = (slot < TY * (TX + 1) && f <= ni && jj < nj)
-                    ? ldo(a.rru + (int64_t)k * nys * nxu, (uint32_t)((jj + NG) * nxu + f + NG) * 4u)
-                    : 0.0f;
-    }
-#pragma unroll
-    for (int u = 0; u < YS; ++u) {
-      int slot = threadIdx.x + u * NT;
-      asm volatile("" : "+v"(slot));  // recomputed per level, not held across the march
-      const int r = slot / TX, t = slot % TX;
-      const int ii = i0 + t, g = j0 + r;
-      v.mv[u] = (slot < (TY + 1) * TX && ii < ni && g <= nj)
-                    ? ldo(a.rrv + (int64_t)k * nyv * nxs, (uint32_t)((g + NG) * nxs + ii + NG) * 4u)
-                    : 0.0f;
-    }
-    return v;
-  };
-  Lv nx = fetch(kb);
-  for (int k = kb; k <= ke; ++k) {
-    const Lv cu = nx;
-    if (k < ke) nx = fetch(k + 1);
-    const float snext = cu.snext;
-    // this level's cell words, consumed after its fluxes
-    float c_uw = 0.0f, c_ue = 0.0f, c_vs = 0.0f, c_vn = 0.0f, c_divx = 0.0f, c_rr0 = 0.0f, c_a0 = 0.0f,
-          c_held = 0.0f, c_divterm = 0.0f;
-    if (owned) {
-      if (obc != 0u) {
-        const float* pu = a.rru + (int64_t)k * nys * nxu;
-        const float* pv = a.rrv + (int64_t)k * nyv * nxs;
-        const uint32_t ou = (uint32_t)(y * nxu + x) * 4u;
-        c_uw = ldo(pu, ou);
-        c_ue = ldo(pu, ou + 4u);
-        c_vs = ldo(pv, wcol);
-        c_vn = ldo(pv, wcol + (uint32_t)nxs * 4u);
-      }
-      if (MODE != MODE_PDEF) {
-        const int64_t lm = (int64_t)(k - 1) * mst;
-        c_divx = ldo(a.divx + lm, mcol);
-        c_rr0 = FLAT ? a.rr0[lm] : ldo(a.rr0 + lm, mcol);
-        if (MODE == MODE_COMPOSE) {
-          c_a0 = ldo(a.a0 + lm, mcol);
-          c_held = ldo(a.held + lm, mcol);
-          if (a.divterm) c_divterm = ldo(a.divterm + lm, mcol);
-        }
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < PS; ++u) {
-      const int slot = threadIdx.x + u * NT;
-      const int r = slot / (TX + 6), c = slot % (TX + 6);
-      if (slot < (TY + 6) * (TX + 6) && j0 + r < nys && i0 + c < nxs) lp[r][c] = cu.plane[u];
-    }
-    __syncthreads();
-    // x-face fluxes: faces i0..i0+TX of the TY rows.
-#pragma unroll
-    for (int u = 0; u < XS; ++u) {
-      const int slot = threadIdx.x + u * NT;
-      const int r = slot / (TX + 1), t = slot % (TX + 1);
-      const int f = i0 + t, jj = j0 + r;
-      if (slot < TY * (TX + 1) && f <= ni && jj < nj) {
-        const float m = cu.mu[u];
-        float v = xflux_lds<WENO, GATE>(lp, m, r, t, a.weps);
-        lfx[r][t] = v;
-        if (MODE == MODE_TENDENCY && a.fx && (t < TX || f == ni))
-          sto(a.fx + (int64_t)k * nys * nxu, (uint32_t)((jj + NG) * nxu + f + NG) * 4u, v);
-      }
-    }
-    // y-face fluxes: rows j0..j0+TY of the TX columns.
-#pragma unroll
-    for (int u = 0; u < YS; ++u) {
-      const int slot = threadIdx.x + u * NT;
-      const int r = slot / TX, t = slot % TX;
-      const int ii = i0 + t, g = j0 + r;
-      if (slot < (TY + 1) * TX && ii < ni && g <= nj) {
-        const float m = cu.mv[u];
-        float v = yflux_lds<WENO, GATE>(lp, m, r, t, a.weps);
-        lfy[r][t] = v;
-        if (MODE == MODE_TENDENCY && a.fy && (r < TY || g == nj))
-          sto(a.fy + (int64_t)k * nyv * nxs, (uint32_t)((g + NG) * nxs + ii + NG) * 4u, v);
-      }
-    }
-    float fzt = 0.0f;
-    if (live && k + 1 <= nk) fzt = zf(k + 1, win + 1);  // S(k-2..)
-    __syncthreads();
-    if (MODE == MODE_TENDENCY && live && a.fz) {
-      if (k == 1) sto(a.fz + st, wcol, 0.0f);
-      sto(a.fz + (int64_t)(k + 1) * st, wcol, (k + 1 <= nk) ? fzt : 0.0f);
-      if (k == kb && kb >= 2) sto(a.fz + (int64_t)kb * st, wcol, fzb);
-    }
-    if (owned) {
-      float fw = lfx[ty][tx], fe = lfx[ty][tx + 1];
-      float fs = lfy[ty][tx], fn = lfy[ty + 1][tx];
-      const float sv = lp[ty + 3][tx + 3];
-      float ax, ay, az;
-      if (obc != 0u) {
-        const float uw = c_uw, ue = c_ue, vs = c_vs, vn = c_vn;
-        const bool iw = (i == 0) && side_of(obc, 0) == 2u && uw >= 0.0f;
-        const bool ie = (i + 1 == ni) && side_of(obc, 1) == 2u && ue <= 0.0f;
-        const bool js = (j == 0) && side_of(obc, 2) == 2u && vs >= 0.0f;
-        const bool jn = (j + 1 == nj) && side_of(obc, 3) == 2u && vn <= 0.0f;
-        if (iw) fw = fe;
-        if (ie) fe = fw;
-        if (js) fs = fn;
-        if (jn) fn = fs;
-        ax = FLAT ? (-(fe - fw)) * a.rdx : (fw - fe) * a.rdx;
-        ay = FLAT ? (-(fn - fs)) * a.rdy : (fs - fn) * a.rdy;
-        if (iw) ax = ax - (sv * (ue - uw)) * a.rdx;
-        if (ie) ax = ax - (sv * (ue - uw)) * a.rdx;
-        if (js) ay = ay - (sv * (vn - vs)) * a.rdy;
-        if (jn) ay = ay - (sv * (vn - vs)) * a.rdy;
-      } else {
-        ax = FLAT ? (-(fe - fw)) * a.rdx : (fw - fe) * a.rdx;
-        ay = FLAT ? (-(fn - fs)) * a.rdy : (fs - fn) * a.rdy;
-      }
-      if constexpr (FLAT) {
-        if (k == 1) az = ((-fzt) * a.rdsf) * a.mh[k];
-        else if (k == nk) az = (fzb * a.rdsf) * a.mh[k];
-        else az = ((-(fzt - fzb)) * a.rdsf) * a.mh[k];
-      } else {
-        if (k == 1) az = (-fzt) * a.rdsf;
-        else if (k == nk) az = fzb * a.rdsf;
-        else az = (fzb - fzt) * a.rdsf;
-      }
-      const int64_t lm = (int64_t)(k - 1) * mst;
-      if (MODE == MODE_PDEF) {
-        sto(a.advx + lm, mcol, ax);
-        sto(a.advy + lm, mcol, ay);
-        sto(a.advz + lm, mcol, az);
-      } else {
-        float total = (ax + ay) + az;
-        total = total + sv * c_divx;
-        const float adv = FLAT ? total * c_rr0 : (total * c_rr0) * ldo(a.gz, mcol);
-        if (MODE == MODE_TENDENCY) {
-          sto(a.tend + lm, mcol, adv);
-        } else {
-          const float seed = a.divterm ? c_held - c_divterm : c_held;
-          const float v = c_a0 + a.dttmp * (seed + adv);
-          sto(a.out + lm, mcol, (fabsf(v) < 1.0e-30f) ? 0.0f : v);
-        }
-      }
-    }
-    fzb = fzt;
-#pragma unroll
-    for (int q = 0; q < 6; ++q) win[q] = win[q + 1];
-    win[6] = snext;
-    __syncthreads();
-  }
-}
-
-
-// The same statements as scalar_adv5_body with another addressing, for the
-// instances that run at four waves per SIMD (the WENO forms and the 5th-order
-// tendency; measured at 640x768x80, the WENO compose 1200 -> 1054 us and the
-// pdef tendencies 773 -> 716 us against the recomputed offsets, while the
-// six-wave compose, whose registers these offsets would crowd, is faster with
-// the recomputed ones).  Addressing: every word a thread reads or writes at level k sits at a byte
-// offset, fixed for the thread, from a base that depends on k alone (the
-// level's plane of that array).  The offsets -- the tile's level-plane
-// slots, face mass-flux slots, the own cell and faces -- are formed once
-// before the level march (the lateral family's image of a halo cell
-// included), so a level's addresses cost one scalar base each.
-template <bool WENO, int MODE, bool FLAT, bool SRC>
-__device__ __forceinline__ void scalar_adv5_body_held(const ScalarAdvArgs& a) {
-  constexpr bool GATE = (MODE == MODE_PDEF);
-  __shared__ float lfx[TY][TX + 1];
-  __shared__ float lfy[TY + 1][TX];
-  __shared__ float lp[TY + 6][LPX];  // the tile's level plane with a 3-cell halo
-  const int ni = a.ni, nj = a.nj, nk = a.nk;
-  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-  // tiles ordered i fastest, then j, then column chunk; each XCD sweeps a
-  // contiguous range so the y-halo rows a tile shares stay in one L2
-  const int gxt = (ni + TX - 1) / TX, gyt = (nj + TY - 1) / TY;
-  const int tile = xcd_tile((int)blockIdx.x, gxt * gyt * a.nchunks);
-  const int i0 = (tile % gxt) * TX, j0 = ((tile / gxt) % gyt) * TY;
-  const int i = i0 + tx, j = j0 + ty;
-  const int kb = 1 + (tile / (gxt * gyt)) * a.kchunk;
-  const int ke = min(nk, kb + a.kchunk - 1);
-  const int nxs = sx(ni), nys = sy(nj), nxu = ux(ni), nyv = vy(nj);
-  const bool live = (i < ni) && (j < nj);
-  const unsigned obc = a.obc;
-  const bool owned = live && (obc != 0u || (i >= 3 && i + 3 < ni && j >= 3 && j + 3 < nj));
-  const int x = i + NG, y = j + NG;
-  // level strides (words) of the padded scalar / w, u, v and unpadded families
   const int64_t st = (int64_t)nys * nxs, ust = (int64_t)nys * nxu, vst = (int64_t)nyv * nxs;
   const int64_t mst = (int64_t)nj * ni;
-  // the thread's own cell and faces within a level plane (bytes)
   const uint32_t o_s = (uint32_t)(y * nxs + x) * 4u;  // padded scalar, w and v (south face) families
   const uint32_t o_u = (uint32_t)(y * nxu + x) * 4u;  // u family (west face)
   const uint32_t o_m = (uint32_t)(j * ni + i) * 4u;   // unpadded cells
-  const uint32_t o_z = a.c12col ? 0u : o_m;           // c1, c2 words of the z faces
-
-  // Slot offsets: the level plane's rows j0-3..j0+TY+2 and columns
-  // i0-3..i0+TX+2 (padded indices j0..j0+TY+5, i0..i0+TX+5, clipped to the
-  // padded array; from the unpadded carrier, the lateral family's image of
-  // each halo cell), the x faces i0..i0+TX of the TY rows and the y faces
-  // j0..j0+TY of the TX columns.
-  constexpr int PS = ((TY + 6) * (TX + 6) + NT - 1) / NT;
-  constexpr int XS = (TY * (TX + 1) + NT - 1) / NT;
-  constexpr int YS = ((TY + 1) * TX + NT - 1) / NT;
-  uint32_t poff[PS], muoff[XS], mvoff[YS];
+  const uint32_t o_fu = (i <= ni && j < nj) ? o_u : 0u;  // the own west x face
+  const uint32_t o_fv = (i < ni && j <= nj) ? o_s : 0u;  // the own south y face
+  const bool xe = (ty == TY - 1), yn = (ty == TY - 2);   // the wavefronts forming the east / north faces
+  uint32_t o_ex = 0u;
+  if (xe && tx < TY && i0 + TX <= ni && j0 + tx < nj) o_ex = (uint32_t)((j0 + tx + NG) * nxu + i0 + TX + NG) * 4u;
+  if (yn && j0 + TY <= nj && i0 + tx < ni) o_ex = (uint32_t)((j0 + TY + NG) * nxs + i0 + tx + NG) * 4u;
+  // the plane's rows j0-3..j0+TY+2, columns i0-3..i0+TX+2 (padded indices
+  // j0..j0+TY+5, i0..i0+TX+5, clipped to the padded array)
 #pragma unroll
   for (int u = 0; u < PS; ++u) {
     const int slot = threadIdx.x + u * NT;
-    const int r = slot / (TX + 6), c = slot % (TX + 6);
+    const int r = slot / LQ, c = slot % LQ;
     const int yy = j0 + r, xx = i0 + c;
-    poff[u] = NO_WORD;
-    if (slot < (TY + 6) * (TX + 6) && yy < nys && xx < nxs) {
+    uint32_t off = 0u;
+    if (slot < (TY + 6) * LQ && yy < nys && xx < nxs) {
       if constexpr (SRC) {
         const int pj = family_cell(yy - NG, nj, side_of(obc, 2), side_of(obc, 3));
         const int pi = family_cell(xx - NG, ni, side_of(obc, 0), side_of(obc, 1));
-        poff[u] = (uint32_t)(pj * ni + pi) * 4u;
+        off = (uint32_t)(pj * ni + pi) * 4u;
       } else {
-        poff[u] = (uint32_t)(yy * nxs + xx) * 4u;
+        off = (uint32_t)(yy * nxs + xx) * 4u;
       }
     }
-  }
-#pragma unroll
-  for (int u = 0; u < XS; ++u) {
-    const int slot = threadIdx.x + u * NT;
-    const int r = slot / (TX + 1), t = slot % (TX + 1);
-    const int f = i0 + t, jj = j0 + r;
-    muoff[u] = (slot < TY * (TX + 1) && f <= ni && jj < nj) ? (uint32_t)((jj + NG) * nxu + f + NG) * 4u : NO_WORD;
-  }
-#pragma unroll
-  for (int u = 0; u < YS; ++u) {
-    const int slot = threadIdx.x + u * NT;
-    const int r = slot / TX, t = slot % TX;
-    const int ii = i0 + t, g = j0 + r;
-    mvoff[u] = (slot < (TY + 1) * TX && ii < ni && g <= nj) ? (uint32_t)((g + NG) * nxs + ii + NG) * 4u : NO_WORD;
+    lpo[slot] = off;  // read back by this lane alone
   }
 
   // The column's scalar S(k-3..k+3) at level k rides in registers: win[q]
@@ -660,7 +326,9 @@ __device__ __forceinline__ void scalar_adv5_body_held(const ScalarAdvArgs& a) {
   // z-face flux at face kf from the window b, b[0] = S(kf-3).
   auto zf = [&](int kf, const float* b) {
     const float m = ldo(a.rrw + (int64_t)kf * st, o_s);
-    const float cc1 = ldo(a.c1 + (int64_t)(kf - 1) * mst, o_z), cc2 = ldo(a.c2 + (int64_t)(kf - 1) * mst, o_z);
+    const float* p1 = a.c1 + (int64_t)(kf - 1) * mst;
+    const float* p2 = a.c2 + (int64_t)(kf - 1) * mst;
+    const float cc1 = a.c12col ? p1[0] : ldo(p1, o_m), cc2 = a.c12col ? p2[0] : ldo(p2, o_m);
     auto S = [&](int level) { return b[level - (kf - 3)]; };
     if (MODE == MODE_PDEF) return zflux_pdef_g(S, m, cc1, cc2, kf, nk, a.weps, a.pdefweno != 0);
     return zflux_g<WENO>(S, m, cc1, cc2, kf, nk, a.weps);
@@ -668,11 +336,12 @@ __device__ __forceinline__ void scalar_adv5_body_held(const ScalarAdvArgs& a) {
   float fzb = 0.0f;
   if (live && kb >= 2) fzb = zf(kb, win);  // S(kb-3..)
   const float gzc = (!FLAT && owned && MODE != MODE_PDEF) ? ldo(a.gz, o_m) : 0.0f;
-  // A level's plane and face mass fluxes are requested one level ahead into
-  // registers (Lv), and its cell words at the top of the level, so they
+  // A level's plane slots and own faces' mass fluxes are requested one level
+  // ahead into registers, and its cell words at the top of the level, so they
   // arrive while the level before it (or its fluxes) is formed.
   struct Lv {
-    float plane[PS], mu[XS], mv[YS];
+    float plane[PS];
+    float mu, mv, mx;
     float snext;
   };
   auto fetch = [&](int k) {
@@ -682,18 +351,14 @@ __device__ __forceinline__ void scalar_adv5_body_held(const ScalarAdvArgs& a) {
     const float pack = (SRC && a.th0) ? a.th0[k - 1] - 300.0f : 0.0f;
 #pragma unroll
     for (int u = 0; u < PS; ++u) {
-      v.plane[u] = 0.0f;
-      if (poff[u] != NO_WORD) {
-        const float w = ldo(pl, poff[u]);
-        v.plane[u] = (SRC && a.th0) ? pack + w : w;
-      }
+      const float w = ldo(pl, lpo[threadIdx.x + u * NT]);
+      v.plane[u] = (SRC && a.th0) ? pack + w : w;
     }
     const float* pu = a.rru + (int64_t)k * ust;
-#pragma unroll
-    for (int u = 0; u < XS; ++u) v.mu[u] = (muoff[u] != NO_WORD) ? ldo(pu, muoff[u]) : 0.0f;
     const float* pv = a.rrv + (int64_t)k * vst;
-#pragma unroll
-    for (int u = 0; u < YS; ++u) v.mv[u] = (mvoff[u] != NO_WORD) ? ldo(pv, mvoff[u]) : 0.0f;
+    v.mu = ldo(pu, o_fu);
+    v.mv = ldo(pv, o_fv);
+    v.mx = xe ? ldo(pu, o_ex) : (yn ? ldo(pv, o_ex) : 0.0f);
     return v;
   };
   Lv nx = fetch(kb);
@@ -706,14 +371,6 @@ __device__ __forceinline__ void scalar_adv5_body_held(const ScalarAdvArgs& a) {
     float c_uw = 0.0f, c_ue = 0.0f, c_vs = 0.0f, c_vn = 0.0f, c_divx = 0.0f, c_rr0 = 0.0f, c_a0 = 0.0f,
           c_held = 0.0f, c_divterm = 0.0f;
     if (owned) {
-      if (obc != 0u) {
-        const float* pu = a.rru + (int64_t)k * ust;
-        const float* pv = a.rrv + (int64_t)k * vst;
-        c_uw = ldo(pu, o_u);
-        c_ue = ldo(pu, o_u + 4u);
-        c_vs = ldo(pv, o_s);
-        c_vn = ldo(pv, o_s + (uint32_t)nxs * 4u);
-      }
       if (MODE != MODE_PDEF) {
         c_divx = ldo(a.divx + lm, o_m);
         c_rr0 = FLAT ? a.rr0[lm] : ldo(a.rr0 + lm, o_m);
@@ -723,39 +380,43 @@ __device__ __forceinline__ void scalar_adv5_body_held(const ScalarAdvArgs& a) {
           if (a.divterm) c_divterm = ldo(a.divterm + lm, o_m);
         }
       }
-    }
-#pragma unroll
-    for (int u = 0; u < PS; ++u) {
-      if (poff[u] != NO_WORD) {
-        const int slot = threadIdx.x + u * NT;
-        const int r = slot / (TX + 6), c = slot % (TX + 6);
-        lp[r][c] = cu.plane[u];
+      if (edge) {
+        const float* pu = a.rru + (int64_t)k * ust;
+        const float* pv = a.rrv + (int64_t)k * vst;
+        c_uw = ldo(pu, o_u);
+        c_ue = ldo(pu, o_u + 4u);
+        c_vs = ldo(pv, o_s);
+        c_vn = ldo(pv, o_s + (uint32_t)nxs * 4u);
       }
     }
+#pragma unroll
+    for (int u = 0; u < PS; ++u) lp[threadIdx.x + u * NT] = cu.plane[u];
     __syncthreads();
-    // x-face fluxes: faces i0..i0+TX of the TY rows.
-#pragma unroll
-    for (int u = 0; u < XS; ++u) {
-      if (muoff[u] != NO_WORD) {
-        const int slot = threadIdx.x + u * NT;
-        const int r = slot / (TX + 1), t = slot % (TX + 1);
-        const float m = cu.mu[u];
-        float v = xflux_lds<WENO, GATE>(lp, m, r, t, a.weps);
-        lfx[r][t] = v;
-        if (MODE == MODE_TENDENCY && a.fx && (t < TX || i0 + t == ni)) sto(a.fx + (int64_t)k * ust, muoff[u], v);
+    {  // the own faces: x face i0+tx of row j0+ty, y face j0+ty of column i0+tx
+      const float* q = lp + (ty + 3) * LQ + tx;
+      const float fx = face_flux<WENO, GATE>(cu.mu, q[0], q[1], q[2], q[3], q[4], q[5], a.weps);
+      lfx[ty][tx] = fx;
+      const float* p = lp + ty * LQ + tx + 3;
+      const float fy = face_flux<WENO, GATE>(cu.mv, p[0], p[LQ], p[2 * LQ], p[3 * LQ], p[4 * LQ], p[5 * LQ], a.weps);
+      lfy[ty][tx] = fy;
+      if (MODE == MODE_TENDENCY) {
+        if (a.fx && i <= ni && j < nj) sto(a.fx + (int64_t)k * ust, o_u, fx);
+        if (a.fy && i < ni && j <= nj) sto(a.fy + (int64_t)k * vst, o_s, fy);
       }
     }
-    // y-face fluxes: rows j0..j0+TY of the TX columns.
-#pragma unroll
-    for (int u = 0; u < YS; ++u) {
-      if (mvoff[u] != NO_WORD) {
-        const int slot = threadIdx.x + u * NT;
-        const int r = slot / TX, t = slot % TX;
-        const float m = cu.mv[u];
-        float v = yflux_lds<WENO, GATE>(lp, m, r, t, a.weps);
-        lfy[r][t] = v;
-        if (MODE == MODE_TENDENCY && a.fy && (r < TY || j0 + r == nj)) sto(a.fy + (int64_t)k * vst, mvoff[u], v);
+    if (xe) {  // face i0+TX of row j0+tx
+      if (tx < TY) {
+        const float* q = lp + (tx + 3) * LQ + TX;
+        const float fx = face_flux<WENO, GATE>(cu.mx, q[0], q[1], q[2], q[3], q[4], q[5], a.weps);
+        lfx[tx][TX] = fx;
+        // the domain's east face, which no tile forms as its own
+        if (MODE == MODE_TENDENCY && a.fx && i0 + TX == ni && j0 + tx < nj) sto(a.fx + (int64_t)k * ust, o_ex, fx);
       }
+    } else if (yn) {  // face j0+TY of column i0+tx
+      const float* p = lp + TY * LQ + tx + 3;
+      const float fy = face_flux<WENO, GATE>(cu.mx, p[0], p[LQ], p[2 * LQ], p[3 * LQ], p[4 * LQ], p[5 * LQ], a.weps);
+      lfy[TY][tx] = fy;
+      if (MODE == MODE_TENDENCY && a.fy && j0 + TY == nj && i0 + tx < ni) sto(a.fy + (int64_t)k * vst, o_ex, fy);
     }
     float fzt = 0.0f;
     if (live && k + 1 <= nk) fzt = zf(k + 1, win + 1);  // S(k-2..)
@@ -768,10 +429,10 @@ __device__ __forceinline__ void scalar_adv5_body_held(const ScalarAdvArgs& a) {
     if (owned) {
       float fw = lfx[ty][tx], fe = lfx[ty][tx + 1];
       float fs = lfy[ty][tx], fn = lfy[ty + 1][tx];
-      const float sv = lp[ty + 3][tx + 3];
+      const float sv = lp[(ty + 3) * LQ + tx + 3];
       float ax, ay, az;
       if (obc != 0u) {
-        const float uw = c_uw, ue = c_ue, vs = c_vs, vn = c_vn;
+        const float uw = c_uw, ue = c_ue, vs = c_vs, vn = c_vn;  // zero off the edges, where no closure applies
         const bool iw = (i == 0) && side_of(obc, 0) == 2u && uw >= 0.0f;
         const bool ie = (i + 1 == ni) && side_of(obc, 1) == 2u && ue <= 0.0f;
         const bool js = (j == 0) && side_of(obc, 2) == 2u && vs >= 0.0f;
@@ -825,17 +486,14 @@ __device__ __forceinline__ void scalar_adv5_body_held(const ScalarAdvArgs& a) {
 }
 
 
+// Held to six waves per SIMD (three 512-lane workgroups per CU): more
+// waves in flight hide more of each level's load latency.  Measured at
+// 640x768x80 against the unconstrained register budget (four waves): the
+// pdef tendencies 641 -> 585 us and the WENO compose 1025 -> 964 us, though
+// both spill a few registers.
 template <bool WENO, int MODE, bool FLAT, bool SRC>
-__global__ __launch_bounds__(NT) void k_scalar_adv5(ScalarAdvArgs a) {
-  scalar_adv5_body_held<WENO, MODE, FLAT, SRC>(a);
-}
-
-// The same body held to six waves per SIMD (three 512-lane workgroups per CU):
-// the 5th-order compose fits that register budget, and more waves in flight
-// hide more of each level's load latency.
-template <bool WENO, int MODE, bool FLAT, bool SRC>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_scalar_adv5_occ(ScalarAdvArgs a) {
-  scalar_adv5_body<WENO, MODE, FLAT, SRC>(a);
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_scalar_adv5(ScalarAdvArgs a) {
+  scalar_adv5_body_lanes<WENO, MODE, FLAT, SRC>(a);
 }
 
 // ---------------------------------------------------------------------------
@@ -1066,7 +724,7 @@ void launch_scalar_variant(const ScalarAdvArgs& a, int use_weno, int mode, dim3 
                 hipLaunchKernelGGL((k_scalar_adv5<true, MODE_COMPOSE, FLAT, SRC>), grid, dim3(NT), 0, s, a));
     else
       GM1_TIMED(TK_SCALAR_ADV5, s,
-                hipLaunchKernelGGL((k_scalar_adv5_occ<false, MODE_COMPOSE, FLAT, SRC>), grid, dim3(NT), 0, s, a));
+                hipLaunchKernelGGL((k_scalar_adv5<false, MODE_COMPOSE, FLAT, SRC>), grid, dim3(NT), 0, s, a));
   } else {
     GM1_TIMED(TK_SCALAR_ADV5, s,
               hipLaunchKernelGGL((k_scalar_adv5<true, MODE_PDEF, FLAT, SRC>), grid, dim3(NT), 0, s, a));
