@@ -8,7 +8,8 @@
 //   * k_mom_mass_v   : one thread per column marches k, forms rrw at every
 //                      w level and divx from the rrw pair it holds in
 //                      registers (one launch instead of two).
-//   * k_mom_adv{u,v,w}: one thread per output word, i fastest.
+//   * k_mom_adv{u,v,w}_tile: an LDS-staged tile marching a chunk of levels,
+//                      one lane per flux point.
 #include "cm1_common.h"
 #include "timing.h"
 
@@ -267,8 +268,15 @@ GM1_DEV float uvz(const MomArgs& a, int kf, int j, int i) {
 // every x and y flux point of the tile is formed once from it into LDS (the
 // open fallback rows included), and the vertical flux comes from a 7-level
 // register window of the thread's own column, the lower face's flux carried
-// from the level below.
-constexpr int MTX = 64, MTY = 8, MNT = MTX * MTY, MLX = MTX + 7;
+// from the level below.  One lane per flux point: each lane forms the x and
+// y points west and south of its own face (wavefront MTY-1 also the tile's
+// east points, wavefront MTY-2 its north points); the byte offsets of the
+// plane slots and of each point's mass-flux pair (periodic seams wrapped) are
+// formed once into LDS.  A lane whose point lies outside reads word 0 and its
+// flux is never consumed.
+constexpr int MTX = 64, MTY = 8, MNT = MTX * MTY, MPX = MTX + 6;
+constexpr int MPS = ((MTY + 6) * MPX + MNT - 1) / MNT;  // plane slots a lane
+static_assert(MTX == 64, "a wavefront is one tile row");
 
 int mom_chunks(int tiles, int nk, int* kchunk) {
   int chunks = (2048 + tiles - 1) / tiles;  // >= 2048 tiles in flight
@@ -278,12 +286,25 @@ int mom_chunks(int tiles, int nk, int* kchunk) {
   return (nk + *kchunk - 1) / *kchunk;
 }
 
-// u-momentum: faces f0 .. f0+nf-1 (tile columns), rows 0..nj-1 (tile rows)
+// fifth-order upwind value of the six words q0..q5 around a flux point (q2|q3
+// straddle it), times the mass flux m
+GM1_DEV float point_flux(float m, float q0, float q1, float q2, float q3, float q4, float q5) {
+  const bool up = m >= 0.0f;
+  return m * flx5(up ? q0 : q5, up ? q1 : q4, up ? q2 : q3, up ? q3 : q2, up ? q4 : q1);
+}
+
+// u-momentum: faces f0 .. f0+nf-1 (tile columns), rows 0..nj-1 (tile rows).
+// The u and v tiles are held to six waves per SIMD (three workgroups a CU):
+// measured at 640x768x80, u 472 -> 427 us and v 492 -> 416 us against four
+// waves, though both spill a few registers; the w tile, which spills far
+// more at six, stays at four.
 template <bool FLAT>
-__global__ __launch_bounds__(MNT) void k_mom_advu_tile(MomArgs a, int f0, int nf, int kchunk, int nchunks) {
-  __shared__ float lp[MTY + 6][MLX];   // u: LDS row r = padded row ja + r, col c = padded x fa + c
+__global__ __launch_bounds__(MNT) __attribute__((amdgpu_waves_per_eu(6))) void k_mom_advu_tile(MomArgs a, int f0, int nf, int kchunk, int nchunks) {
+  __shared__ float lp[MPS * MNT];      // u: row r = padded row ja + r, col c = padded x fa + c (pitch MPX)
   __shared__ float lfx[MTY][MTX + 1];  // x flux at centre fa - 1 + t
   __shared__ float lfy[MTY + 1][MTX];  // y flux at corner row ja + r of face fa + t
+  __shared__ uint32_t lpo[MPS * MNT];  // plane slot offsets within a level (read back by their own lane)
+  __shared__ uint32_t lmo[6][MNT];     // mass-flux pairs: own x centre, own y corner, extra point
   const int ni = a.ni, nj = a.nj, nk = a.nk;
   const int tx = threadIdx.x % MTX, ty = threadIdx.x / MTX;
   const int gx = (nf + MTX - 1) / MTX, gy = (nj + MTY - 1) / MTY;
@@ -297,6 +318,35 @@ __global__ __launch_bounds__(MNT) void k_mom_advu_tile(MomArgs a, int f0, int nf
   const int f = fa + tx, j = ja + ty;
   const bool live = f < fend && j < nj;
   const int y = j + NG, x = f + NG;
+  const bool xe = (ty == MTY - 1), yn = (ty == MTY - 2);  // the wavefronts forming the extra points
+  {
+#pragma unroll
+    for (int q = 0; q < MPS; ++q) {
+      const int sl = threadIdx.x + q * MNT;
+      const int r = sl / MPX, c = sl % MPX;
+      const int yy = ja + r, xx = fa + c;
+      lpo[sl] = (sl < (MTY + 6) * MPX && yy < nyu && xx < nxu) ? (uint32_t)(yy * nxu + xx) * 4u : 0u;
+    }
+    // x centre cc of row jj: the rru faces wf(cc), wf(cc+1)
+    auto xpair = [&](int slot, int cc, int jj) {
+      const bool ok = jj < nj && cc < fend;
+      const int rb = (jj + NG) * nxu + NG;
+      lmo[slot][threadIdx.x] = ok ? (uint32_t)(rb + wf(cc, ni, pw, pe)) * 4u : 0u;
+      lmo[slot + 1][threadIdx.x] = ok ? (uint32_t)(rb + wf(cc + 1, ni, pw, pe)) * 4u : 0u;
+    };
+    // y corner p of face ff: the rrv faces wc(ff), wc(ff-1)
+    auto ypair = [&](int slot, int p, int ff) {
+      const bool ok = p <= nj && ff < fend;
+      const int rb = (p + NG) * nxv + NG;
+      lmo[slot][threadIdx.x] = ok ? (uint32_t)(rb + wc(ff, ni, pw, pe)) * 4u : 0u;
+      lmo[slot + 1][threadIdx.x] = ok ? (uint32_t)(rb + wc(ff - 1, ni, pw, pe)) * 4u : 0u;
+    };
+    xpair(0, f - 1, j);
+    ypair(2, j, f);
+    if (xe) xpair(4, fa - 1 + MTX, ja + tx);  // lanes tx < MTY are read
+    else if (yn) ypair(4, ja + MTY, f);
+    else lmo[4][threadIdx.x] = lmo[5][threadIdx.x] = 0u;
+  }
   // a level's words sit at 32-bit byte offsets from the level's base
   const uint32_t ocol = (uint32_t)(y * nxu + x) * 4u;  // this face in the u family
   float win[7];  // u(k-3 .. k+3) of this face's column
@@ -309,14 +359,11 @@ __global__ __launch_bounds__(MNT) void k_mom_advu_tile(MomArgs a, int f0, int nf
     return uvz_g<0>(a, kf, j, f, [&](int lev) { return b[lev - (kf - 3)]; });
   };
   float zlo = (live && kb >= 2) ? zf(kb, win) : 0.0f;
-  // A level's u plane and the rru / rrv pairs its face fluxes average are
+  // A level's u plane and the rru / rrv pairs its points average are
   // requested one level ahead into registers (Lv), so they arrive while the
-  // level before it is formed.  Slot loops run at most PS / XS / YS times.
-  constexpr int PS = ((MTY + 6) * (MTX + 6) + MNT - 1) / MNT;
-  constexpr int XS = (MTY * (MTX + 1) + MNT - 1) / MNT;
-  constexpr int YS = ((MTY + 1) * MTX + MNT - 1) / MNT;
+  // level before it is formed.
   struct Lv {
-    float plane[PS], rx0[XS], rx1[XS], ry0[YS], ry1[YS];
+    float plane[MPS], m[6];
     float unext;
   };
   auto fetch = [&](int k) {
@@ -325,43 +372,48 @@ __global__ __launch_bounds__(MNT) void k_mom_advu_tile(MomArgs a, int f0, int nf
     v.unext = (live && lnext <= nk + 1) ? ldo(a.u + (int64_t)lnext * ust, ocol) : 0.0f;
     const float* plane = a.u + (int64_t)k * ust;
 #pragma unroll
-    for (int q = 0; q < PS; ++q) {
-      int sl = threadIdx.x + q * MNT;
-      asm volatile("" : "+v"(sl));  // recomputed per level, not held across the march
-      const int r = sl / (MTX + 6), c = sl % (MTX + 6);
-      const int yy = ja + r, xx = fa + c;
-      v.plane[q] = (sl < (MTY + 6) * (MTX + 6) && yy < nyu && xx < nxu) ? ldo(plane, (uint32_t)(yy * nxu + xx) * 4u)
-                                                                          : 0.0f;
-    }
-#pragma unroll
-    for (int q = 0; q < XS; ++q) {
-      int sl = threadIdx.x + q * MNT;
-      asm volatile("" : "+v"(sl));
-      const int r = sl / (MTX + 1), t = sl % (MTX + 1);
-      const int cc = fa - 1 + t, jj = ja + r;
-      v.rx0[q] = v.rx1[q] = 0.0f;
-      if (sl < MTY * (MTX + 1) && jj < nj && cc < fend) {
-        const float* R = a.rru + (int64_t)k * ust;
-        const int rb = (jj + NG) * nxu + NG;
-        v.rx0[q] = ldo(R, (uint32_t)(rb + wf(cc, ni, pw, pe)) * 4u);
-        v.rx1[q] = ldo(R, (uint32_t)(rb + wf(cc + 1, ni, pw, pe)) * 4u);
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < YS; ++q) {
-      int sl = threadIdx.x + q * MNT;
-      asm volatile("" : "+v"(sl));
-      const int r = sl / MTX, t = sl % MTX;
-      const int p = ja + r, ff = fa + t;
-      v.ry0[q] = v.ry1[q] = 0.0f;
-      if (sl < (MTY + 1) * MTX && p <= nj && ff < fend) {
-        const float* R = a.rrv + (int64_t)k * vst;
-        const int rb = (p + NG) * nxv + NG;
-        v.ry0[q] = ldo(R, (uint32_t)(rb + wc(ff, ni, pw, pe)) * 4u);
-        v.ry1[q] = ldo(R, (uint32_t)(rb + wc(ff - 1, ni, pw, pe)) * 4u);
-      }
+    for (int q = 0; q < MPS; ++q) v.plane[q] = ldo(plane, lpo[threadIdx.x + q * MNT]);
+    const float* Ru = a.rru + (int64_t)k * ust;
+    const float* Rv = a.rrv + (int64_t)k * vst;
+    v.m[0] = ldo(Ru, lmo[0][threadIdx.x]);
+    v.m[1] = ldo(Ru, lmo[1][threadIdx.x]);
+    v.m[2] = ldo(Rv, lmo[2][threadIdx.x]);
+    v.m[3] = ldo(Rv, lmo[3][threadIdx.x]);
+    v.m[4] = v.m[5] = 0.0f;
+    if (xe || yn) {
+      const float* R = xe ? Ru : Rv;
+      v.m[4] = ldo(R, lmo[4][threadIdx.x]);
+      v.m[5] = ldo(R, lmo[5][threadIdx.x]);
     }
     return v;
+  };
+  // x point (row r, centre fa-1+t): the mean of the two rru faces times u's
+  // fifth-order upwind value
+  auto xflux = [&](int r, int t, float m0, float m1) {
+    const float m = 0.5f * (m0 + m1);
+    const float* L = lp + (r + 3) * MPX + t;  // L[c] = u at padded x (cc + NG) - 2 + c
+    lfx[r][t] = point_flux(m, L[0], L[1], L[2], L[3], L[4], L[5]);
+  };
+  // y point (corner row p = ja+r of face ff = fa+t); on an open south/north
+  // side an inflowing edge corner takes the next row's flux
+  auto yflux = [&](int k, int r, int t, float m0, float m1) {
+    const int p = ja + r, ff = fa + t;
+    const float mt = m0 + m1;
+    int pb = p;
+    if (p == 0 && opn(a.obc, 2) && mt >= 0.0f) pb = 1;
+    else if (p == nj && opn(a.obc, 3) && mt <= 0.0f) pb = nj - 1;
+    float m;
+    if (pb == p) {
+      m = 0.5f * mt;
+    } else {  // (p is 0 or nj, so the point lies inside when its face does)
+      const float* Rb = a.rrv + (int64_t)k * vst;
+      const int rb = (pb + NG) * nxv + NG;
+      m = (ff < fend) ? 0.5f * (ldo(Rb, (uint32_t)(rb + wc(ff, ni, pw, pe)) * 4u) +
+                                ldo(Rb, (uint32_t)(rb + wc(ff - 1, ni, pw, pe)) * 4u))
+                      : 0.0f;
+    }
+    const float* L = lp + (pb - ja) * MPX + t + 3;  // u at padded row pb + NG - 3 + q
+    lfy[r][t] = point_flux(m, L[0], L[MPX], L[2 * MPX], L[3 * MPX], L[4 * MPX], L[5 * MPX]);
   };
   Lv nx = fetch(kb);
   for (int k = kb; k <= ke; ++k) {
@@ -386,54 +438,14 @@ __global__ __launch_bounds__(MNT) void k_mom_advu_tile(MomArgs a, int f0, int nf
       e_ten = a.zero_held ? 0.0f : ldo((a.u1 ? a.u1 : a.uten) + (int64_t)k * ust, ocol);
     }
 #pragma unroll
-    for (int q = 0; q < PS; ++q) {
-      const int sl = threadIdx.x + q * MNT;
-      const int r = sl / (MTX + 6), c = sl % (MTX + 6);
-      if (sl < (MTY + 6) * (MTX + 6) && ja + r < nyu && fa + c < nxu) lp[r][c] = cu.plane[q];
-    }
+    for (int q = 0; q < MPS; ++q) lp[threadIdx.x + q * MNT] = cu.plane[q];
     __syncthreads();
-    // x fluxes of u at centres cc = fa-1 .. fa+MTX-1 of the tile's rows: the
-    // mean of the two rru faces times u's fifth-order upwind value
-#pragma unroll
-    for (int q = 0; q < XS; ++q) {
-      const int sl = threadIdx.x + q * MNT;
-      const int r = sl / (MTX + 1), t = sl % (MTX + 1);
-      const int cc = fa - 1 + t, jj = ja + r;
-      if (sl < MTY * (MTX + 1) && jj < nj && cc < fend) {
-        const float m = 0.5f * (cu.rx0[q] + cu.rx1[q]);
-        const float* L = lp[r + 3] + t;  // L[c] = u at padded x (cc + NG) - 2 + c
-        const float rv = (m >= 0.0f) ? flx5(L[0], L[1], L[2], L[3], L[4]) : flx5(L[5], L[4], L[3], L[2], L[1]);
-        lfx[r][t] = m * rv;
-      }
-    }
-    // y fluxes of u at corner rows p = ja .. ja+MTY of the tile's faces; on an
-    // open south/north side an inflowing edge corner takes the next row's flux
-#pragma unroll
-    for (int q = 0; q < YS; ++q) {
-      const int sl = threadIdx.x + q * MNT;
-      const int r = sl / MTX, t = sl % MTX;
-      const int p = ja + r, ff = fa + t;
-      if (sl < (MTY + 1) * MTX && p <= nj && ff < fend) {
-        const float mt = cu.ry0[q] + cu.ry1[q];
-        int pb = p;
-        if (p == 0 && opn(a.obc, 2) && mt >= 0.0f) pb = 1;
-        else if (p == nj && opn(a.obc, 3) && mt <= 0.0f) pb = nj - 1;
-        float m;
-        if (pb == p) {
-          m = 0.5f * mt;
-        } else {
-          const float* Rb = a.rrv + (int64_t)k * vst;
-          const int rb = (pb + NG) * nxv + NG;
-          m = 0.5f * (ldo(Rb, (uint32_t)(rb + wc(ff, ni, pw, pe)) * 4u) +
-                      ldo(Rb, (uint32_t)(rb + wc(ff - 1, ni, pw, pe)) * 4u));
-        }
-        const int r0 = pb - ja;  // LDS row of u at padded row pb + NG - 3
-        const int c = t + 3;
-        const float rv = (m >= 0.0f)
-                             ? flx5(lp[r0][c], lp[r0 + 1][c], lp[r0 + 2][c], lp[r0 + 3][c], lp[r0 + 4][c])
-                             : flx5(lp[r0 + 5][c], lp[r0 + 4][c], lp[r0 + 3][c], lp[r0 + 2][c], lp[r0 + 1][c]);
-        lfy[r][t] = m * rv;
-      }
+    xflux(ty, tx, cu.m[0], cu.m[1]);
+    yflux(k, ty, tx, cu.m[2], cu.m[3]);
+    if (xe) {
+      if (tx < MTY) xflux(tx, MTX, cu.m[4], cu.m[5]);
+    } else if (yn) {
+      yflux(k, MTY, tx, cu.m[4], cu.m[5]);
     }
     float zhi = 0.0f;
     if (live && k + 1 <= nk) zhi = zf(k + 1, win + 1);
@@ -442,7 +454,7 @@ __global__ __launch_bounds__(MNT) void k_mom_advu_tile(MomArgs a, int f0, int nf
       const int west = wc(f - 1, ni, pw, pe), east = wc(f, ni, pw, pe);
       float ax = (-(lfx[ty][tx + 1] - lfx[ty][tx])) * a.rdx;
       float ay = (-(lfy[ty + 1][tx] - lfy[ty][tx])) * a.rdy;
-      const float uc = lp[ty + 3][tx + 3];
+      const float uc = lp[(ty + 3) * MPX + tx + 3];
       if (j == 0 && opn(a.obc, 2)) {
         const float* R0 = a.rrv + ix3(k, NG, 0, nyv, nxv);
         const float* R1 = a.rrv + ix3(k, NG + 1, 0, nyv, nxv);
@@ -481,10 +493,12 @@ __global__ __launch_bounds__(MNT) void k_mom_advu_tile(MomArgs a, int f0, int nf
 
 // v-momentum: faces g0 .. g0+nr-1 (tile rows), columns 0..ni-1 (tile columns)
 template <bool FLAT>
-__global__ __launch_bounds__(MNT) void k_mom_advv_tile(MomArgs a, int g0, int nr, int kchunk, int nchunks) {
-  __shared__ float lp[MTY + 6][MLX];   // v: LDS row r = padded row ga + r, col c = padded x ia + c
+__global__ __launch_bounds__(MNT) __attribute__((amdgpu_waves_per_eu(6))) void k_mom_advv_tile(MomArgs a, int g0, int nr, int kchunk, int nchunks) {
+  __shared__ float lp[MPS * MNT];      // v: row r = padded row ga + r, col c = padded x ia + c (pitch MPX)
   __shared__ float lfx[MTY][MTX + 1];  // x flux at corner column ia + t of face row ga + r
   __shared__ float lfy[MTY + 1][MTX];  // y flux at centre ga - 1 + r of column ia + t
+  __shared__ uint32_t lpo[MPS * MNT];  // plane slot offsets within a level (read back by their own lane)
+  __shared__ uint32_t lmo[6][MNT];     // mass-flux pairs: own x corner, own y centre, extra point
   const int ni = a.ni, nj = a.nj, nk = a.nk;
   const int tx = threadIdx.x % MTX, ty = threadIdx.x / MTX;
   const int gx = (ni + MTX - 1) / MTX, gy = (nr + MTY - 1) / MTY;
@@ -498,6 +512,34 @@ __global__ __launch_bounds__(MNT) void k_mom_advv_tile(MomArgs a, int g0, int nr
   const int i = ia + tx, g = ga + ty;
   const bool live = i < ni && g < gend;
   const int y = g + NG, x = i + NG;
+  const bool xe = (ty == MTY - 1), yn = (ty == MTY - 2);  // the wavefronts forming the extra points
+  {
+#pragma unroll
+    for (int q = 0; q < MPS; ++q) {
+      const int sl = threadIdx.x + q * MNT;
+      const int r = sl / MPX, c = sl % MPX;
+      const int yy = ga + r, xx = ia + c;
+      lpo[sl] = (sl < (MTY + 6) * MPX && yy < nyv && xx < nxv) ? (uint32_t)(yy * nxv + xx) * 4u : 0u;
+    }
+    // x corner p of face row gg: the rru faces of rows wc(gg), wc(gg-1)
+    auto xpair = [&](int slot, int p, int gg) {
+      const bool ok = gg < gend && p <= ni;
+      const int rn = wc(gg, nj, ps, pn) + NG, rs = wc(gg - 1, nj, ps, pn) + NG;
+      lmo[slot][threadIdx.x] = ok ? (uint32_t)(rn * nxu + p + NG) * 4u : 0u;
+      lmo[slot + 1][threadIdx.x] = ok ? (uint32_t)(rs * nxu + p + NG) * 4u : 0u;
+    };
+    // y centre pc of column ii: the rrv faces of rows wf(pc), wf(pc+1)
+    auto ypair = [&](int slot, int pc, int ii) {
+      const bool ok = ii < ni && pc < gend;
+      lmo[slot][threadIdx.x] = ok ? (uint32_t)((wf(pc, nj, ps, pn) + NG) * nxv + ii + NG) * 4u : 0u;
+      lmo[slot + 1][threadIdx.x] = ok ? (uint32_t)((wf(pc + 1, nj, ps, pn) + NG) * nxv + ii + NG) * 4u : 0u;
+    };
+    xpair(0, i, g);
+    ypair(2, g - 1, i);
+    if (xe) xpair(4, ia + MTX, ga + tx);  // lanes tx < MTY are read
+    else if (yn) ypair(4, ga - 1 + MTY, i);
+    else lmo[4][threadIdx.x] = lmo[5][threadIdx.x] = 0u;
+  }
   // a level's words sit at 32-bit byte offsets from the level's base
   const uint32_t ocol = (uint32_t)(y * nxv + x) * 4u;  // this face in the v family
   float win[7];
@@ -510,13 +552,10 @@ __global__ __launch_bounds__(MNT) void k_mom_advv_tile(MomArgs a, int g0, int nr
     return uvz_g<1>(a, kf, g, i, [&](int lev) { return b[lev - (kf - 3)]; });
   };
   float zlo = (live && kb >= 2) ? zf(kb, win) : 0.0f;
-  // A level's v plane and the rru / rrv pairs its face fluxes average are
+  // A level's v plane and the rru / rrv pairs its points average are
   // requested one level ahead into registers (Lv), as in the u tile.
-  constexpr int PS = ((MTY + 6) * (MTX + 6) + MNT - 1) / MNT;
-  constexpr int XS = (MTY * (MTX + 1) + MNT - 1) / MNT;
-  constexpr int YS = ((MTY + 1) * MTX + MNT - 1) / MNT;
   struct Lv {
-    float plane[PS], rx0[XS], rx1[XS], ry0[YS], ry1[YS];
+    float plane[MPS], m[6];
     float vnext;
   };
   auto fetch = [&](int k) {
@@ -525,42 +564,48 @@ __global__ __launch_bounds__(MNT) void k_mom_advv_tile(MomArgs a, int g0, int nr
     v.vnext = (live && lnext <= nk + 1) ? ldo(a.v + (int64_t)lnext * vst, ocol) : 0.0f;
     const float* plane = a.v + (int64_t)k * vst;
 #pragma unroll
-    for (int q = 0; q < PS; ++q) {
-      int sl = threadIdx.x + q * MNT;
-      asm volatile("" : "+v"(sl));  // recomputed per level, not held across the march
-      const int r = sl / (MTX + 6), c = sl % (MTX + 6);
-      const int yy = ga + r, xx = ia + c;
-      v.plane[q] = (sl < (MTY + 6) * (MTX + 6) && yy < nyv && xx < nxv) ? ldo(plane, (uint32_t)(yy * nxv + xx) * 4u)
-                                                                          : 0.0f;
-    }
-#pragma unroll
-    for (int q = 0; q < XS; ++q) {
-      int sl = threadIdx.x + q * MNT;
-      asm volatile("" : "+v"(sl));
-      const int r = sl / (MTX + 1), t = sl % (MTX + 1);
-      const int p = ia + t, gg = ga + r;
-      v.rx0[q] = v.rx1[q] = 0.0f;
-      if (sl < MTY * (MTX + 1) && gg < gend && p <= ni) {
-        const int rn = wc(gg, nj, ps, pn) + NG, rs = wc(gg - 1, nj, ps, pn) + NG;
-        const float* R = a.rru + (int64_t)k * ust;
-        v.rx0[q] = ldo(R, (uint32_t)(rn * nxu + p + NG) * 4u);
-        v.rx1[q] = ldo(R, (uint32_t)(rs * nxu + p + NG) * 4u);
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < YS; ++q) {
-      int sl = threadIdx.x + q * MNT;
-      asm volatile("" : "+v"(sl));
-      const int r = sl / MTX, t = sl % MTX;
-      const int pc = ga - 1 + r, ii = ia + t;
-      v.ry0[q] = v.ry1[q] = 0.0f;
-      if (sl < (MTY + 1) * MTX && ii < ni && pc < gend) {
-        const float* R = a.rrv + (int64_t)k * vst;
-        v.ry0[q] = ldo(R, (uint32_t)((wf(pc, nj, ps, pn) + NG) * nxv + ii + NG) * 4u);
-        v.ry1[q] = ldo(R, (uint32_t)((wf(pc + 1, nj, ps, pn) + NG) * nxv + ii + NG) * 4u);
-      }
+    for (int q = 0; q < MPS; ++q) v.plane[q] = ldo(plane, lpo[threadIdx.x + q * MNT]);
+    const float* Ru = a.rru + (int64_t)k * ust;
+    const float* Rv = a.rrv + (int64_t)k * vst;
+    v.m[0] = ldo(Ru, lmo[0][threadIdx.x]);
+    v.m[1] = ldo(Ru, lmo[1][threadIdx.x]);
+    v.m[2] = ldo(Rv, lmo[2][threadIdx.x]);
+    v.m[3] = ldo(Rv, lmo[3][threadIdx.x]);
+    v.m[4] = v.m[5] = 0.0f;
+    if (xe || yn) {
+      const float* R = xe ? Ru : Rv;
+      v.m[4] = ldo(R, lmo[4][threadIdx.x]);
+      v.m[5] = ldo(R, lmo[5][threadIdx.x]);
     }
     return v;
+  };
+  // x point (face row gg = ga+r, corner column p = ia+t); on an open
+  // west/east side an inflowing edge corner takes the next column's
+  auto xflux = [&](int k, int r, int t, float m0, float m1) {
+    const int p = ia + t, gg = ga + r;
+    const float mt = m0 + m1;
+    int pb = p;
+    if (p == 0 && opn(a.obc, 0) && mt >= 0.0f) pb = 1;
+    else if (p == ni && opn(a.obc, 1) && mt <= 0.0f) pb = ni - 1;
+    float m;
+    if (pb == p) {
+      m = 0.5f * mt;
+    } else {  // (p is 0 or ni, so the point lies inside when its row does)
+      const float* R = a.rru + (int64_t)k * ust;
+      const int rn = wc(gg, nj, ps, pn) + NG, rs = wc(gg - 1, nj, ps, pn) + NG;
+      m = (gg < gend) ? 0.5f * (ldo(R, (uint32_t)(rn * nxu + pb + NG) * 4u) +
+                                ldo(R, (uint32_t)(rs * nxu + pb + NG) * 4u))
+                      : 0.0f;
+    }
+    const float* L = lp + (r + 3) * MPX + (pb - ia);  // L[c] = v at padded x (pb + NG) - 3 + c
+    lfx[r][t] = point_flux(m, L[0], L[1], L[2], L[3], L[4], L[5]);
+  };
+  // y point (centre pc = ga-1+r of column ia+t)
+  auto yflux = [&](int r, int t, float m0, float m1) {
+    const float m = 0.5f * (m0 + m1);
+    // LDS row of v at padded row (pc + NG) - 2 is pc + 1 - ga = r
+    const float* L = lp + r * MPX + t + 3;
+    lfy[r][t] = point_flux(m, L[0], L[MPX], L[2 * MPX], L[3 * MPX], L[4 * MPX], L[5 * MPX]);
   };
   Lv nx = fetch(kb);
   for (int k = kb; k <= ke; ++k) {
@@ -585,48 +630,14 @@ __global__ __launch_bounds__(MNT) void k_mom_advv_tile(MomArgs a, int g0, int nr
       e_ten = a.zero_held ? 0.0f : ldo((a.v1 ? a.v1 : a.vten) + (int64_t)k * vst, ocol);
     }
 #pragma unroll
-    for (int q = 0; q < PS; ++q) {
-      const int sl = threadIdx.x + q * MNT;
-      const int r = sl / (MTX + 6), c = sl % (MTX + 6);
-      if (sl < (MTY + 6) * (MTX + 6) && ga + r < nyv && ia + c < nxv) lp[r][c] = cu.plane[q];
-    }
+    for (int q = 0; q < MPS; ++q) lp[threadIdx.x + q * MNT] = cu.plane[q];
     __syncthreads();
-    // x fluxes of v at corner columns p = ia .. ia+MTX of the tile's face rows;
-    // on an open west/east side an inflowing edge corner takes the next column's
-#pragma unroll
-    for (int q = 0; q < XS; ++q) {
-      const int sl = threadIdx.x + q * MNT;
-      const int r = sl / (MTX + 1), t = sl % (MTX + 1);
-      const int p = ia + t, gg = ga + r;
-      if (sl < MTY * (MTX + 1) && gg < gend && p <= ni) {
-        const int rn = wc(gg, nj, ps, pn) + NG, rs = wc(gg - 1, nj, ps, pn) + NG;
-        const float mt = cu.rx0[q] + cu.rx1[q];
-        int pb = p;
-        if (p == 0 && opn(a.obc, 0) && mt >= 0.0f) pb = 1;
-        else if (p == ni && opn(a.obc, 1) && mt <= 0.0f) pb = ni - 1;
-        const float* R = a.rru + (int64_t)k * ust;
-        const float m = (pb == p) ? 0.5f * mt
-                                  : 0.5f * (ldo(R, (uint32_t)(rn * nxu + pb + NG) * 4u) +
-                                            ldo(R, (uint32_t)(rs * nxu + pb + NG) * 4u));
-        const float* L = lp[r + 3] + (pb - ia);  // L[c] = v at padded x (pb + NG) - 3 + c
-        const float rv = (m >= 0.0f) ? flx5(L[0], L[1], L[2], L[3], L[4]) : flx5(L[5], L[4], L[3], L[2], L[1]);
-        lfx[r][t] = m * rv;
-      }
-    }
-    // y fluxes of v at centres pc = ga-1 .. ga+MTY-1 of the tile's columns
-#pragma unroll
-    for (int q = 0; q < YS; ++q) {
-      const int sl = threadIdx.x + q * MNT;
-      const int r = sl / MTX, t = sl % MTX;
-      const int pc = ga - 1 + r, ii = ia + t;
-      if (sl < (MTY + 1) * MTX && ii < ni && pc < gend) {
-        const float m = 0.5f * (cu.ry0[q] + cu.ry1[q]);
-        const int c = t + 3;
-        // LDS row of v at padded row (pc + NG) - 2 is pc + 1 - ga = r
-        const float rv = (m >= 0.0f) ? flx5(lp[r][c], lp[r + 1][c], lp[r + 2][c], lp[r + 3][c], lp[r + 4][c])
-                                     : flx5(lp[r + 5][c], lp[r + 4][c], lp[r + 3][c], lp[r + 2][c], lp[r + 1][c]);
-        lfy[r][t] = m * rv;
-      }
+    xflux(k, ty, tx, cu.m[0], cu.m[1]);
+    yflux(ty, tx, cu.m[2], cu.m[3]);
+    if (xe) {
+      if (tx < MTY) xflux(k, tx, MTX, cu.m[4], cu.m[5]);
+    } else if (yn) {
+      yflux(MTY, tx, cu.m[4], cu.m[5]);
     }
     float zhi = 0.0f;
     if (live && k + 1 <= nk) zhi = zf(k + 1, win + 1);
@@ -635,7 +646,7 @@ __global__ __launch_bounds__(MNT) void k_mom_advv_tile(MomArgs a, int g0, int nr
       const int south = wc(g - 1, nj, ps, pn), north = wc(g, nj, ps, pn);
       float ax = (-(lfx[ty][tx + 1] - lfx[ty][tx])) * a.rdx;
       float ay = (-(lfy[ty + 1][tx] - lfy[ty][tx])) * a.rdy;
-      const float vc = lp[ty + 3][tx + 3];
+      const float vc = lp[(ty + 3) * MPX + tx + 3];
       const float* Rs = a.rru + ix3(k, south + NG, 0, nyu, nxu);
       const float* Rn = a.rru + ix3(k, north + NG, 0, nyu, nxu);
       if (i == 0 && opn(a.obc, 0)) {
@@ -708,15 +719,21 @@ GM1_DEV float wz_g(float m, int k, int nk, Get Q) {
 // its mass flux (the open sides' fallback to the neighbouring face reads it),
 // and the vertical flux comes from a 7-level window of the thread's own
 // column, the lower flux, rru/rrv of the level below, divx and dum8 carried.
-constexpr int WLX = MTX + 6, WLY = MTY + 6;
+// One lane per face, as in the u and v tiles: each lane forms its cell's west
+// x face and south y face (wavefront MTY-1 also the tile's east faces,
+// wavefront MTY-2 its north faces); the plane slots' byte offsets are formed
+// once into LDS.
+constexpr int WLX = MTX + 6, WLY = MTY + 6, WCX = MTX + 2;
+constexpr int WPS = (WLY * WLX + MNT - 1) / MNT, WCS = ((MTY + 2) * WCX + MNT - 1) / MNT;
 template <bool FLAT>
 // (registers held to four waves per SIMD: two 512-lane workgroups per CU)
 __global__ __launch_bounds__(MNT) __attribute__((amdgpu_waves_per_eu(4)))
 void k_mom_advw_tile(MomArgs a, int kchunk, int nchunks) {
-  __shared__ float lw[WLY][WLX];            // w: LDS row r = padded row ja + r, col c = padded x ia + c
-  __shared__ float lc[MTY + 2][MTX + 2];    // c2 of cells (clamp(ja-1+r), clamp(ia-1+c)) at level kf-1
+  __shared__ float lw[WPS * MNT];           // w: row r = padded row ja + r, col c = padded x ia + c (pitch WLX)
+  __shared__ float lc[WCS * MNT];           // c2 of cells (clamp(ja-1+r), clamp(ia-1+c)) at level kf-1 (pitch WCX)
   __shared__ float lfx[MTY][MTX + 1], lmx[MTY][MTX + 1];  // x flux and mass flux at face ia + t
   __shared__ float lfy[MTY + 1][MTX], lmy[MTY + 1][MTX];  // y flux and mass flux at face ja + r
+  __shared__ uint32_t lpo[WPS * MNT], lco[WCS * MNT];     // slot offsets (read back by their own lane)
   const int ni = a.ni, nj = a.nj, nk = a.nk;
   const int tx = threadIdx.x % MTX, ty = threadIdx.x / MTX;
   const int gx = (ni + MTX - 1) / MTX, gy = (nj + MTY - 1) / MTY;
@@ -730,28 +747,30 @@ void k_mom_advw_tile(MomArgs a, int kchunk, int nchunks) {
   const int64_t ust = (int64_t)nyu * nxu, vst = (int64_t)nyv * nxs;
   const int y = j + NG, x = i + NG;
   const int64_t h = (int64_t)j * ni + i;
+  const bool xe = (ty == MTY - 1), yn = (ty == MTY - 2);  // the wavefronts forming the extra faces
   // a level's words sit at 32-bit byte offsets from the level's base
   const uint32_t oh = (uint32_t)(j * ni + i) * 4u, ow = (uint32_t)(y * nxs + x) * 4u;
-  constexpr int PS = (WLY * WLX + MNT - 1) / MNT;
-  constexpr int CS = ((MTY + 2) * (MTX + 2) + MNT - 1) / MNT;
-  constexpr int XS = (MTY * (MTX + 1) + MNT - 1) / MNT;
-  constexpr int YS = ((MTY + 1) * MTX + MNT - 1) / MNT;
-  // the faces a slot forms (fixed across levels, so rru/rrv of the level below ride in registers)
-  auto rx_at = [&](int k, int q) {
+  // the lane's faces: west x face (i, j), south y face (i, j), and the extra face
+  const uint32_t orx = (j < nj && i <= ni) ? (uint32_t)(y * nxu + x) * 4u : 0u;
+  const uint32_t ory = (i < ni && j <= nj) ? ow : 0u;
+  uint32_t ore = 0u;
+  if (xe && tx < MTY && ja + tx < nj && ia + MTX <= ni) ore = (uint32_t)((ja + tx + NG) * nxu + ia + MTX + NG) * 4u;
+  if (yn && ja + MTY <= nj && i < ni) ore = (uint32_t)((ja + MTY + NG) * nxs + x) * 4u;
+#pragma unroll
+  for (int q = 0; q < WPS; ++q) {
     const int sl = threadIdx.x + q * MNT;
-    const int r = sl / (MTX + 1), t = sl % (MTX + 1);
-    const int pf = ia + t, jj = ja + r;
-    return (sl < MTY * (MTX + 1) && jj < nj && pf <= ni)
-               ? ldo(a.rru + (int64_t)k * ust, (uint32_t)((jj + NG) * nxu + pf + NG) * 4u)
-               : 0.0f;
-  };
-  auto ry_at = [&](int k, int q) {
+    const int r = sl / WLX, c = sl % WLX;
+    const int yy = ja + r, xx = ia + c;
+    lpo[sl] = (sl < WLY * WLX && yy < nys && xx < nxs) ? (uint32_t)(yy * nxs + xx) * 4u : 0u;
+  }
+#pragma unroll
+  for (int q = 0; q < WCS; ++q) {
     const int sl = threadIdx.x + q * MNT;
-    const int r = sl / MTX, t = sl % MTX;
-    const int pf = ja + r, ii = ia + t;
-    return (sl < (MTY + 1) * MTX && ii < ni && pf <= nj)
-               ? ldo(a.rrv + (int64_t)k * vst, (uint32_t)((pf + NG) * nxs + ii + NG) * 4u)
-               : 0.0f;
+    const int r = sl / WCX, c = sl % WCX;
+    lco[sl] = (sl < (MTY + 2) * WCX) ? (uint32_t)(clampi(ja - 1 + r, nj) * ni + clampi(ia - 1 + c, ni)) * 4u : 0u;
+  }
+  auto rface = [&](int k, uint32_t off, bool xface) {  // rru (x face) or rrv (y face) at level k
+    return xface ? ldo(a.rru + (int64_t)k * ust, off) : ldo(a.rrv + (int64_t)k * vst, off);
   };
   float win[7];  // w(kf-3 .. kf+3) of this cell's column
 #pragma unroll
@@ -759,11 +778,9 @@ void k_mom_advw_tile(MomArgs a, int kchunk, int nchunks) {
     const int lev = kb - 3 + q;
     win[q] = (live && lev >= 0 && lev <= nk + 2) ? ldo(a.w + (int64_t)lev * wst, ow) : 0.0f;
   }
-  float rxp[XS], ryp[YS];
-#pragma unroll
-  for (int q = 0; q < XS; ++q) rxp[q] = rx_at(kb - 1, q);
-#pragma unroll
-  for (int q = 0; q < YS; ++q) ryp[q] = ry_at(kb - 1, q);
+  // the faces' mass fluxes of the level below
+  float rxp = rface(kb - 1, orx, true), ryp = rface(kb - 1, ory, false);
+  float rep = (xe || yn) ? rface(kb - 1, ore, xe) : 0.0f;
   float rrw_lo = 0.0f, rrw_hi = 0.0f, zlo = 0.0f, dxlo = 0.0f, d8lo = 0.0f;
   if (live) {
     rrw_lo = ldo(a.rrw + (int64_t)(kb - 1) * wst, ow);
@@ -773,34 +790,21 @@ void k_mom_advw_tile(MomArgs a, int kchunk, int nchunks) {
     if (FLAT && a.dum8) d8lo = ldo(a.dum8 + (int64_t)(kb - 1) * wst, ow);
   }
   struct Lv {
-    float plane[PS], c2[CS], rx[XS], ry[YS];
+    float plane[WPS], c2[WCS];
+    float rx, ry, re;
     float wnext, rrwn, divx, base, d8, c1;
   };
   auto fetch = [&](int kf) {
     Lv v;
     const float* wp = a.w + (int64_t)kf * wst;
 #pragma unroll
-    for (int q = 0; q < PS; ++q) {
-      int sl = threadIdx.x + q * MNT;
-      asm volatile("" : "+v"(sl));  // recomputed per level, not held across the march
-      const int r = sl / WLX, c = sl % WLX;
-      const int yy = ja + r, xx = ia + c;
-      v.plane[q] = (sl < WLY * WLX && yy < nys && xx < nxs) ? ldo(wp, (uint32_t)(yy * nxs + xx) * 4u) : 0.0f;
-    }
+    for (int q = 0; q < WPS; ++q) v.plane[q] = ldo(wp, lpo[threadIdx.x + q * MNT]);
     const float* c2k = a.c2 + (int64_t)(kf - 1) * plane;
 #pragma unroll
-    for (int q = 0; q < CS; ++q) {
-      int sl = threadIdx.x + q * MNT;
-      asm volatile("" : "+v"(sl));
-      const int r = sl / (MTX + 2), c = sl % (MTX + 2);
-      v.c2[q] = (sl < (MTY + 2) * (MTX + 2))
-                    ? ldo(c2k, (uint32_t)(clampi(ja - 1 + r, nj) * ni + clampi(ia - 1 + c, ni)) * 4u)
-                    : 0.0f;
-    }
-#pragma unroll
-    for (int q = 0; q < XS; ++q) v.rx[q] = rx_at(kf, q);
-#pragma unroll
-    for (int q = 0; q < YS; ++q) v.ry[q] = ry_at(kf, q);
+    for (int q = 0; q < WCS; ++q) v.c2[q] = ldo(c2k, lco[threadIdx.x + q * MNT]);
+    v.rx = rface(kf, orx, true);
+    v.ry = rface(kf, ory, false);
+    v.re = (xe || yn) ? rface(kf, ore, xe) : 0.0f;
     v.wnext = v.rrwn = v.divx = v.base = v.d8 = v.c1 = 0.0f;
     if (live) {
       const int lnext = kf + 4;
@@ -815,52 +819,40 @@ void k_mom_advw_tile(MomArgs a, int kchunk, int nchunks) {
     }
     return v;
   };
+  // x face ia + t of row ja + r (wfx_base)
+  auto xface = [&](int r, int t, float mt, float mb) {
+    const float c2h = 0.5f * (lc[(r + 1) * WCX + t] + lc[(r + 1) * WCX + t + 1]);
+    const float c1h = 1.0f - c2h;
+    const float m = c2h * mt + c1h * mb;
+    const float* L = lw + (r + 3) * WLX + t;  // L[c] = w at padded x (ia + t + NG) - 3 + c
+    lfx[r][t] = point_flux(m, L[0], L[1], L[2], L[3], L[4], L[5]);
+    lmx[r][t] = m;
+  };
+  // y face ja + r of column ia + t (wfy_base)
+  auto yface = [&](int r, int t, float mt, float mb) {
+    const float c2h = 0.5f * (lc[r * WCX + t + 1] + lc[(r + 1) * WCX + t + 1]);
+    const float c1h = 1.0f - c2h;
+    const float m = c2h * mt + c1h * mb;
+    const float* L = lw + r * WLX + t + 3;
+    lfy[r][t] = point_flux(m, L[0], L[WLX], L[2 * WLX], L[3 * WLX], L[4 * WLX], L[5 * WLX]);
+    lmy[r][t] = m;
+  };
   const bool pw = opn(a.obc, 0), pe = opn(a.obc, 1), ps = opn(a.obc, 2), pn = opn(a.obc, 3);
   Lv nx = fetch(kb);
   for (int kf = kb; kf <= ke; ++kf) {
     const Lv cu = nx;
     if (kf < ke) nx = fetch(kf + 1);
 #pragma unroll
-    for (int q = 0; q < PS; ++q) {
-      const int sl = threadIdx.x + q * MNT;
-      if (sl < WLY * WLX) lw[sl / WLX][sl % WLX] = cu.plane[q];
-    }
+    for (int q = 0; q < WPS; ++q) lw[threadIdx.x + q * MNT] = cu.plane[q];
 #pragma unroll
-    for (int q = 0; q < CS; ++q) {
-      const int sl = threadIdx.x + q * MNT;
-      if (sl < (MTY + 2) * (MTX + 2)) lc[sl / (MTX + 2)][sl % (MTX + 2)] = cu.c2[q];
-    }
+    for (int q = 0; q < WCS; ++q) lc[threadIdx.x + q * MNT] = cu.c2[q];
     __syncthreads();
-    // x faces ia + t of the tile's rows (wfx_base)
-#pragma unroll
-    for (int q = 0; q < XS; ++q) {
-      const int sl = threadIdx.x + q * MNT;
-      const int r = sl / (MTX + 1), t = sl % (MTX + 1);
-      if (sl < MTY * (MTX + 1) && ja + r < nj && ia + t <= ni) {
-        const float c2h = 0.5f * (lc[r + 1][t] + lc[r + 1][t + 1]);
-        const float c1h = 1.0f - c2h;
-        const float m = c2h * cu.rx[q] + c1h * rxp[q];
-        const float* L = lw[r + 3] + t;  // L[c] = w at padded x (ia + t + NG) - 3 + c
-        const float rv = (m >= 0.0f) ? flx5(L[0], L[1], L[2], L[3], L[4]) : flx5(L[5], L[4], L[3], L[2], L[1]);
-        lfx[r][t] = m * rv;
-        lmx[r][t] = m;
-      }
-    }
-    // y faces ja + r of the tile's columns (wfy_base)
-#pragma unroll
-    for (int q = 0; q < YS; ++q) {
-      const int sl = threadIdx.x + q * MNT;
-      const int r = sl / MTX, t = sl % MTX;
-      if (sl < (MTY + 1) * MTX && ja + r <= nj && ia + t < ni) {
-        const float c2h = 0.5f * (lc[r][t + 1] + lc[r + 1][t + 1]);
-        const float c1h = 1.0f - c2h;
-        const float m = c2h * cu.ry[q] + c1h * ryp[q];
-        const int c = t + 3;
-        const float rv = (m >= 0.0f) ? flx5(lw[r][c], lw[r + 1][c], lw[r + 2][c], lw[r + 3][c], lw[r + 4][c])
-                                     : flx5(lw[r + 5][c], lw[r + 4][c], lw[r + 3][c], lw[r + 2][c], lw[r + 1][c]);
-        lfy[r][t] = m * rv;
-        lmy[r][t] = m;
-      }
+    xface(ty, tx, cu.rx, rxp);
+    yface(ty, tx, cu.ry, ryp);
+    if (xe) {
+      if (tx < MTY) xface(tx, MTX, cu.re, rep);
+    } else if (yn) {
+      yface(MTY, tx, cu.re, rep);
     }
     float zhi = 0.0f;
     if (live) zhi = wz_g(0.5f * (rrw_hi + cu.rrwn), kf, nk, [&](int lev) { return win[lev - (kf - 3)]; });
@@ -881,7 +873,7 @@ void k_mom_advw_tile(MomArgs a, int kchunk, int nchunks) {
         ax = (-(fe - fw)) * (a.rdx * a.uh[x]);
         ay = (-(fn - fs)) * (a.rdy * a.vh[y]);
       }
-      const float c2 = lc[ty + 1][tx + 1];
+      const float c2 = lc[(ty + 1) * WCX + tx + 1];
       const float c1 = 1.0f - c2;
       const float wc0 = win[3];
       if (i == 0 && pw) {
@@ -931,10 +923,9 @@ void k_mom_advw_tile(MomArgs a, int kchunk, int nchunks) {
     zlo = zhi;
     rrw_hi = cu.rrwn;
     dxlo = cu.divx;
-#pragma unroll
-    for (int q = 0; q < XS; ++q) rxp[q] = cu.rx[q];
-#pragma unroll
-    for (int q = 0; q < YS; ++q) ryp[q] = cu.ry[q];
+    rxp = cu.rx;
+    ryp = cu.ry;
+    rep = cu.re;
 #pragma unroll
     for (int q = 0; q < 6; ++q) win[q] = win[q + 1];
     win[6] = cu.wnext;
