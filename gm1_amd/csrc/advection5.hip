@@ -528,54 +528,16 @@ struct PdefArgs {
   const float* src;  // unpadded source (nk, nj, ni) in place of s, or null
 };
 
-// The scalar at an interior cell (k, y, x padded indices).
-GM1_DEV float pdef_word(const PdefArgs& a, int k, int y, int x) {
-  if (a.src) return a.src[((int64_t)(k - 1) * a.nj + (y - NG)) * a.ni + (x - NG)];
-  return a.s[ix3(k, y, x, sy(a.nj), sx(a.ni))];
-}
-
-
+// A candidate of the directional repair from its cell's words (pdef.F's
+// x/y candidate): rho0*qa, and unless an open edge's face pair points
+// inward, + dt*(adv + s*(m_hi - m_lo)*rd) (times gz on the terrain family).
 template <bool FLAT>
-GM1_DEV float cand_x(const PdefArgs& a, int k, int j, int i) {
-  const int nxs = sx(a.ni), nys = sy(a.nj), nxu = ux(a.ni);
-  const int64_t q = ((int64_t)(k - 1) * a.nj + j) * a.ni + i;
-  const int x = i + NG, y = j + NG;
-  const float uw = a.rru[ix3(k, y, x, nys, nxu)], ue = a.rru[ix3(k, y, x + 1, nys, nxu)];
-  const float r0a = (FLAT ? a.rho0[(int64_t)(k - 1) * a.nj * a.ni] : a.rho0[q]) * a.qa[q];
-  if ((i == 0 && side_of(a.obc, 0) == 2u && uw >= 0.0f) || (i + 1 == a.ni && side_of(a.obc, 1) == 2u && ue <= 0.0f))
-    return r0a;
-  const float d = ue - uw;
-  if constexpr (FLAT) return r0a + a.dt * (a.advx[q] + (pdef_word(a, k, y, x) * d) * a.rdx);
-  return r0a + (a.dt * (a.advx[q] + (pdef_word(a, k, y, x) * d) * a.rdx)) * a.gz[(int64_t)j * a.ni + i];
-}
-
-template <bool FLAT>
-GM1_DEV float cand_y(const PdefArgs& a, int k, int j, int i) {
-  const int nxs = sx(a.ni), nys = sy(a.nj), nyv = vy(a.nj);
-  const int64_t q = ((int64_t)(k - 1) * a.nj + j) * a.ni + i;
-  const int x = i + NG, y = j + NG;
-  const float vs = a.rrv[ix3(k, y, x, nyv, nxs)], vn = a.rrv[ix3(k, y + 1, x, nyv, nxs)];
-  const float r0a = (FLAT ? a.rho0[(int64_t)(k - 1) * a.nj * a.ni] : a.rho0[q]) * a.qa[q];
-  if ((j == 0 && side_of(a.obc, 2) == 2u && vs >= 0.0f) || (j + 1 == a.nj && side_of(a.obc, 3) == 2u && vn <= 0.0f))
-    return r0a;
-  const float d = vn - vs;
-  if constexpr (FLAT) return r0a + a.dt * (a.advy[q] + (pdef_word(a, k, y, x) * d) * a.rdy);
-  return r0a + (a.dt * (a.advy[q] + (pdef_word(a, k, y, x) * d) * a.rdy)) * a.gz[(int64_t)j * a.ni + i];
-}
-
-template <bool FLAT>
-GM1_DEV float cand_z(const PdefArgs& a, int k, int j, int i) {
-  if (k < 1 || k > a.nk) return 0.0f;
-  const int nxs = sx(a.ni), nys = sy(a.nj);
-  const int64_t q = ((int64_t)(k - 1) * a.nj + j) * a.ni + i;
-  const int x = i + NG, y = j + NG;
-  const float d = a.rrw[ix3(k + 1, y, x, nys, nxs)] - a.rrw[ix3(k, y, x, nys, nxs)];
-  if constexpr (FLAT) {
-    const float r0c = a.rho0[(int64_t)(k - 1) * a.nj * a.ni] * a.qa[q];
-    return r0c + a.dt * (a.advz[q] + ((pdef_word(a, k, y, x) * d) * a.rdz) * a.mh[k]);
-  }
-  const float r0a = a.rho0[q] * a.qa[q];
-  return r0a + (a.dt * (a.advz[q] + (pdef_word(a, k, y, x) * d) * a.rdz)) * a.gz[(int64_t)j * a.ni + i];
+GM1_DEV float pdef_cand(float r0a, float adv, float sv, float mlo, float mhi, float rd, float dt, float gzv,
+                        bool lo_edge, bool hi_edge) {
+  if ((lo_edge && mlo >= 0.0f) || (hi_edge && mhi <= 0.0f)) return r0a;
+  const float d = mhi - mlo;
+  if constexpr (FLAT) return r0a + dt * (adv + (sv * d) * rd);
+  return r0a + (dt * (adv + (sv * d) * rd)) * gzv;
 }
 
 // ratio of the donor repair at the middle of three consecutive values
@@ -604,6 +566,12 @@ GM1_DEV void row_mass(const float v[5], float& mass, bool& flag) {
   }
 }
 
+// The repair pass: each lane forms the x and y candidates of its own cell
+// (a lane in the two columns east of the domain or the two rows north of it,
+// those of the lateral family's image), wavefronts 0-3 the tile's south and
+// north halo rows' y candidates and wavefront 4 the west and east halo
+// columns' x candidates; the z candidates ride in a five-level register
+// window.  Every word a level needs is requested one level ahead.
 template <bool FLAT>
 __global__ __launch_bounds__(RT) void k_pdef_repair(PdefArgs a) {
   __shared__ float lcx[RY][RX + 4];
@@ -619,82 +587,179 @@ __global__ __launch_bounds__(RT) void k_pdef_repair(PdefArgs a) {
   const int ke = min(nk, kb + a.kchunk - 1);
   const unsigned wbc = side_of(a.obc, 0), ebc = side_of(a.obc, 1), sbc = side_of(a.obc, 2), nbc = side_of(a.obc, 3);
   const int nxs = sx(ni), nys = sy(nj), nxu = ux(ni), nyv = vy(nj);
-  const int x = i + NG, y = j + NG;
-  const int64_t mcol = (int64_t)j * ni + i, mst = (int64_t)nj * ni;
+  const int64_t st = (int64_t)nys * nxs, ust = (int64_t)nys * nxu, vst = (int64_t)nyv * nxs, mst = (int64_t)nj * ni;
+  // a cell's words: unpadded (m), padded scalar / v / w families (s), u family (u)
+  struct Cell {
+    uint32_t m, s, u;
+  };
+  auto cell = [&](int ci, int cj) {
+    Cell c;
+    c.m = (uint32_t)(cj * ni + ci) * 4u;
+    c.s = (uint32_t)((cj + NG) * nxs + ci + NG) * 4u;
+    c.u = (uint32_t)((cj + NG) * nxu + ci + NG) * 4u;
+    return c;
+  };
+  auto word = [&](int k, const Cell& c) {  // the scalar's current value
+    return a.src ? ldo(a.src + (int64_t)(k - 1) * mst, c.m) : ldo(a.s + (int64_t)k * st, c.s);
+  };
+  auto rho_at = [&](int k, const Cell& c) {
+    return FLAT ? a.rho0[(int64_t)(k - 1) * mst] : ldo(a.rho0 + (int64_t)(k - 1) * mst, c.m);
+  };
+  // the lane's candidate cell
+  const bool xc = (j < nj) && (i <= ni + 1);  // x candidate at lcx[ty][tx + 2]
+  const bool yc = (i < ni) && (j <= nj + 1);  // y candidate at lcy[ty + 2][tx]
+  const int ci = (i < ni) ? i : family_cell(i, ni, wbc, ebc);
+  const int cj = (j < nj) ? j : family_cell(j, nj, sbc, nbc);
+  const Cell oc = cell((xc || yc) ? ci : 0, (xc || yc) ? cj : 0);
+  const bool x_lo = ci == 0 && wbc == 2u, x_hi = ci + 1 == ni && ebc == 2u;
+  const bool y_lo = cj == 0 && sbc == 2u, y_hi = cj + 1 == nj && nbc == 2u;
+  const float gzc = (!FLAT && (xc || yc)) ? ldo(a.gz, oc.m) : 1.0f;
+  // the lane's halo candidate: hk 1 (x, lcx[hr][ht]) or 2 (y, lcy[hr][ht])
+  int hk = 0, hr = 0, ht = 0, hi = 0, hj = 0;
+  if (ty == 4 && tx < 4 * RY) {
+    hr = tx >> 2;
+    ht = (tx & 3) < 2 ? (tx & 3) : RX + (tx & 3);
+    const int c = i0 - 2 + ht, jj = j0 + hr;
+    if (jj < nj && c >= -2 && c <= ni + 1) {
+      hk = 1;
+      hi = family_cell(c, ni, wbc, ebc);
+      hj = jj;
+    }
+  } else if (ty < 4) {
+    hr = ty < 2 ? ty : RY + ty;
+    ht = tx;
+    const int c = j0 - 2 + hr, ii = i0 + tx;
+    if (ii < ni && c >= -2 && c <= nj + 1) {
+      hk = 2;
+      hi = ii;
+      hj = family_cell(c, nj, sbc, nbc);
+    }
+  }
+  const Cell hc = cell(hi, hj);
+  const bool h_lo = hk == 1 ? (hi == 0 && wbc == 2u) : (hj == 0 && sbc == 2u);
+  const bool h_hi = hk == 1 ? (hi + 1 == ni && ebc == 2u) : (hj + 1 == nj && nbc == 2u);
+  const float hgz = (!FLAT && hk) ? ldo(a.gz, hc.m) : 1.0f;
+  // z candidate of the own column at level k (0 outside 1..nk) from its words
+  auto zcand = [&](int k, float rho, float qa, float az, float sv, float wb, float wt) {
+    if (k < 1 || k > nk) return 0.0f;
+    const float d = wt - wb;
+    if constexpr (FLAT) return rho * qa + a.dt * (az + ((sv * d) * a.rdz) * a.mh[k]);
+    return rho * qa + (a.dt * (az + (sv * d) * a.rdz)) * gzc;
+  };
+  auto zwords = [&](int k) {  // the own column's z candidate at level k from memory
+    if (!live || k < 1 || k > nk) return 0.0f;
+    const int64_t lm = (int64_t)(k - 1) * mst;
+    return zcand(k, rho_at(k, oc), ldo(a.qa + lm, oc.m), ldo(a.advz + lm, oc.m), word(k, oc),
+                 ldo(a.rrw + (int64_t)k * st, oc.s), ldo(a.rrw + (int64_t)(k + 1) * st, oc.s));
+  };
   float cz[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};  // candidates at k-2..k+2
-  if (live)
-    for (int d = 0; d < 4; ++d) cz[d + 1] = cand_z<FLAT>(a, kb - 2 + d, j, i);
-  for (int k = kb; k <= ke; ++k) {
-    // this level's cell words, requested before the candidates are formed
-    // (the face pairs whether or not a flag then needs them)
-    float c_ax = 0.0f, c_ay = 0.0f, c_az = 0.0f, c_qa = 0.0f, c_rho = 0.0f, c_sv = 0.0f, c_divx = 0.0f,
-          c_rr0 = 0.0f, c_held = 0.0f, c_uw = 0.0f, c_ue = 0.0f, c_vs = 0.0f, c_vn = 0.0f, c_wb = 0.0f,
-          c_wt = 0.0f;
+  for (int d = 0; d < 4; ++d) cz[d + 1] = zwords(kb - 2 + d);
+  struct Lv {
+    float qa, ax, ay, az, sv, divx, held, uw, ue, vs, vn, wb, wt, rho, rr0;  // the own cell at level k
+    float zqa, zaz, zsv, zwb, zwt, zrho;                                      // the own column at level k+2
+    float hqa, hadv, hsv, hm0, hm1, hrho;                                     // the halo cell at level k
+  };
+  auto fetch = [&](int k) {
+    Lv v{};
+    const int64_t lm = (int64_t)(k - 1) * mst;
+    if (xc || yc) {
+      v.qa = ldo(a.qa + lm, oc.m);
+      v.sv = word(k, oc);
+      v.rho = rho_at(k, oc);
+      if (xc) {
+        v.ax = ldo(a.advx + lm, oc.m);
+        v.uw = ldo(a.rru + (int64_t)k * ust, oc.u);
+        v.ue = ldo(a.rru + (int64_t)k * ust, oc.u + 4u);
+      }
+      if (yc) {
+        v.ay = ldo(a.advy + lm, oc.m);
+        v.vs = ldo(a.rrv + (int64_t)k * vst, oc.s);
+        v.vn = ldo(a.rrv + (int64_t)k * vst, oc.s + (uint32_t)nxs * 4u);
+      }
+    }
     if (live) {
-      const int64_t q = (int64_t)(k - 1) * mst + mcol;
-      c_ax = a.advx[q];
-      c_ay = a.advy[q];
-      c_az = a.advz[q];
-      c_qa = a.qa[q];
-      c_rho = FLAT ? a.rho0[(int64_t)(k - 1) * mst] : a.rho0[q];
-      c_sv = pdef_word(a, k, y, x);
-      c_divx = a.divx[q];
-      c_rr0 = FLAT ? a.rr0[(int64_t)(k - 1) * mst] : a.rr0[q];
-      c_held = a.held[q];
-      c_uw = a.rru[ix3(k, y, x, nys, nxu)];
-      c_ue = a.rru[ix3(k, y, x + 1, nys, nxu)];
-      c_vs = a.rrv[ix3(k, y, x, nyv, nxs)];
-      c_vn = a.rrv[ix3(k, y + 1, x, nyv, nxs)];
-      c_wb = a.rrw[ix3(k, y, x, nys, nxs)];
-      c_wt = a.rrw[ix3(k + 1, y, x, nys, nxs)];
+      v.az = ldo(a.advz + lm, oc.m);
+      v.divx = ldo(a.divx + lm, oc.m);
+      v.held = ldo(a.held + lm, oc.m);
+      v.wb = ldo(a.rrw + (int64_t)k * st, oc.s);
+      v.wt = ldo(a.rrw + (int64_t)(k + 1) * st, oc.s);
+      v.rr0 = FLAT ? a.rr0[lm] : ldo(a.rr0 + lm, oc.m);
+      const int kz = k + 2;
+      if (kz <= nk) {
+        const int64_t lz = (int64_t)(kz - 1) * mst;
+        v.zqa = ldo(a.qa + lz, oc.m);
+        v.zaz = ldo(a.advz + lz, oc.m);
+        v.zsv = word(kz, oc);
+        v.zwb = ldo(a.rrw + (int64_t)kz * st, oc.s);
+        v.zwt = ldo(a.rrw + (int64_t)(kz + 1) * st, oc.s);
+        v.zrho = rho_at(kz, oc);
+      }
     }
-    for (int slot = threadIdx.x; slot < RY * (RX + 4); slot += RT) {
-      const int r = slot / (RX + 4), t = slot % (RX + 4);
-      const int jj = j0 + r, c = i0 - 2 + t;
-      if (jj < nj && c <= ni + 1 && c >= -2 && (c < i0 + RX + 2)) lcx[r][t] = cand_x<FLAT>(a, k, jj, family_cell(c, ni, wbc, ebc));
+    if (hk) {
+      v.hqa = ldo(a.qa + lm, hc.m);
+      v.hsv = word(k, hc);
+      v.hrho = rho_at(k, hc);
+      if (hk == 1) {
+        v.hadv = ldo(a.advx + lm, hc.m);
+        v.hm0 = ldo(a.rru + (int64_t)k * ust, hc.u);
+        v.hm1 = ldo(a.rru + (int64_t)k * ust, hc.u + 4u);
+      } else {
+        v.hadv = ldo(a.advy + lm, hc.m);
+        v.hm0 = ldo(a.rrv + (int64_t)k * vst, hc.s);
+        v.hm1 = ldo(a.rrv + (int64_t)k * vst, hc.s + (uint32_t)nxs * 4u);
+      }
     }
-    for (int slot = threadIdx.x; slot < (RY + 4) * RX; slot += RT) {
-      const int r = slot / RX, t = slot % RX;
-      const int ii = i0 + t, c = j0 - 2 + r;
-      if (ii < ni && c <= nj + 1 && c >= -2) lcy[r][t] = cand_y<FLAT>(a, k, family_cell(c, nj, sbc, nbc), ii);
+    return v;
+  };
+  Lv nx = fetch(kb);
+  for (int k = kb; k <= ke; ++k) {
+    const Lv cu = nx;
+    if (k < ke) nx = fetch(k + 1);
+    if (xc) lcx[ty][tx + 2] = pdef_cand<FLAT>(cu.rho * cu.qa, cu.ax, cu.sv, cu.uw, cu.ue, a.rdx, a.dt, gzc, x_lo, x_hi);
+    if (yc) lcy[ty + 2][tx] = pdef_cand<FLAT>(cu.rho * cu.qa, cu.ay, cu.sv, cu.vs, cu.vn, a.rdy, a.dt, gzc, y_lo, y_hi);
+    if (hk) {
+      const float c = pdef_cand<FLAT>(cu.hrho * cu.hqa, cu.hadv, cu.hsv, cu.hm0, cu.hm1, hk == 1 ? a.rdx : a.rdy, a.dt,
+                                      hgz, h_lo, h_hi);
+      if (hk == 1) lcx[hr][ht] = c;
+      else lcy[hr][ht] = c;
     }
     for (int d = 0; d < 4; ++d) cz[d] = cz[d + 1];
-    cz[4] = live ? cand_z<FLAT>(a, k + 2, j, i) : 0.0f;
+    cz[4] = live ? zcand(k + 2, cu.zrho, cu.zqa, cu.zaz, cu.zsv, cu.zwb, cu.zwt) : 0.0f;
     __syncthreads();
     if (live) {
-      const int64_t q = (int64_t)(k - 1) * mst + mcol;
-      const float sv = c_sv;
+      const int64_t lm = (int64_t)(k - 1) * mst;
+      const float sv = cu.sv;
       constexpr bool flat = FLAT;
-      const float r0a = c_rho * c_qa;
-      const float rg = flat ? 1.0f : a.rgz[mcol];
-      float ax = c_ax, ay = c_ay, az = c_az;
+      const float r0a = cu.rho * cu.qa;
+      const float rg = flat ? 1.0f : ldo(a.rgz, oc.m);
+      float ax = cu.ax, ay = cu.ay, az = cu.az;
       float v[5], mass;
       bool flag;
       for (int d = 0; d < 5; ++d) v[d] = lcx[ty][tx + d];
       row_mass(v, mass, flag);
       if (flag) {
-        const float du = c_ue - c_uw;
+        const float du = cu.ue - cu.uw;
         const float first = flat ? ((v[2] + mass) - r0a) * a.rdt : (((v[2] + mass) - r0a) * a.rdt) * rg;
         ax = first - (sv * du) * a.rdx;
       }
       for (int d = 0; d < 5; ++d) v[d] = lcy[ty + d][tx];
       row_mass(v, mass, flag);
       if (flag) {
-        const float dv = c_vn - c_vs;
+        const float dv = cu.vn - cu.vs;
         const float first = flat ? ((v[2] + mass) - r0a) * a.rdt : (((v[2] + mass) - r0a) * a.rdt) * rg;
         ay = first - (sv * dv) * a.rdy;
       }
       row_mass(cz, mass, flag);
       if (flag) {
-        const float dw = c_wt - c_wb;
+        const float dw = cu.wt - cu.wb;
         if (flat) az = ((cz[2] + mass) - r0a) * a.rdt - ((sv * dw) * a.rdz) * a.mh[k];
         else az = (((cz[2] + mass) - r0a) * a.rdt) * rg - (sv * dw) * a.rdz;
       }
       float t = (ax + ay) + az;
-      t = t + sv * c_divx;
-      const float adv = flat ? t * c_rr0 : (t * c_rr0) * a.gz[mcol];
-      const float o = c_qa + a.dt * (c_held + adv);
-      a.out[q] = (fabsf(o) < 1.0e-30f) ? 0.0f : o;
+      t = t + sv * cu.divx;
+      const float adv = flat ? t * cu.rr0 : (t * cu.rr0) * gzc;
+      const float o = cu.qa + a.dt * (cu.held + adv);
+      sto(a.out + lm, oc.m, (fabsf(o) < 1.0e-30f) ? 0.0f : o);
     }
     __syncthreads();
   }
