@@ -496,6 +496,241 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
   scalar_adv5_body_lanes<WENO, MODE, FLAT, SRC>(a);
 }
 
+// The fifth-order compose of NS scalars read from their unpadded carriers in
+// one pass: the mass fluxes, divx and the addressing are shared, each scalar
+// keeps its own level plane, flux planes and vertical window.  Every scalar's
+// words are those of its own single-scalar compose (the same statements, in
+// the same order).
+constexpr int NSMAX = 4;
+struct ScalarBatch {
+  const float* src[NSMAX];
+  const float* th0[NSMAX];      // packs theta' as (th0 - 300) + theta', or null
+  const float* a0[NSMAX];
+  const float* held[NSMAX];
+  const float* divterm[NSMAX];  // may be null
+  float* out[NSMAX];
+};
+
+template <int NS, bool FLAT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_scalar_compose_batch(ScalarAdvArgs a, ScalarBatch b) {
+  constexpr int PS = ((TY + 6) * LQ + NT - 1) / NT;
+  __shared__ float lfx[NS][TY][TX + 1];
+  __shared__ float lfy[NS][TY + 1][TX];
+  __shared__ float lp[NS][PS * NT];
+  __shared__ uint32_t lpo[PS * NT];
+  const int ni = a.ni, nj = a.nj, nk = a.nk;
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const int gxt = (ni + TX - 1) / TX, gyt = (nj + TY - 1) / TY;
+  const int tile = xcd_tile((int)blockIdx.x, gxt * gyt * a.nchunks);
+  const int i0 = (tile % gxt) * TX, j0 = ((tile / gxt) % gyt) * TY;
+  const int i = i0 + tx, j = j0 + ty;
+  const int kb = 1 + (tile / (gxt * gyt)) * a.kchunk;
+  const int ke = min(nk, kb + a.kchunk - 1);
+  const int nxs = sx(ni), nys = sy(nj), nxu = ux(ni), nyv = vy(nj);
+  const bool live = (i < ni) && (j < nj);
+  const unsigned obc = a.obc;
+  const bool owned = live && (obc != 0u || (i >= 3 && i + 3 < ni && j >= 3 && j + 3 < nj));
+  const bool edge = owned && obc != 0u && (i == 0 || i + 1 == ni || j == 0 || j + 1 == nj);
+  const int x = i + NG, y = j + NG;
+  const int64_t st = (int64_t)nys * nxs, ust = (int64_t)nys * nxu, vst = (int64_t)nyv * nxs;
+  const int64_t mst = (int64_t)nj * ni;
+  const uint32_t o_s = (uint32_t)(y * nxs + x) * 4u;
+  const uint32_t o_u = (uint32_t)(y * nxu + x) * 4u;
+  const uint32_t o_m = (uint32_t)(j * ni + i) * 4u;
+  const uint32_t o_fu = (i <= ni && j < nj) ? o_u : 0u;
+  const uint32_t o_fv = (i < ni && j <= nj) ? o_s : 0u;
+  const bool xe = (ty == TY - 1), yn = (ty == TY - 2);
+  uint32_t o_ex = 0u;
+  if (xe && tx < TY && i0 + TX <= ni && j0 + tx < nj) o_ex = (uint32_t)((j0 + tx + NG) * nxu + i0 + TX + NG) * 4u;
+  if (yn && j0 + TY <= nj && i0 + tx < ni) o_ex = (uint32_t)((j0 + TY + NG) * nxs + i0 + tx + NG) * 4u;
+#pragma unroll
+  for (int u = 0; u < PS; ++u) {
+    const int slot = threadIdx.x + u * NT;
+    const int r = slot / LQ, c = slot % LQ;
+    const int yy = j0 + r, xx = i0 + c;
+    uint32_t off = 0u;
+    if (slot < (TY + 6) * LQ && yy < nys && xx < nxs) {
+      const int pj = family_cell(yy - NG, nj, side_of(obc, 2), side_of(obc, 3));
+      const int pi = family_cell(xx - NG, ni, side_of(obc, 0), side_of(obc, 1));
+      off = (uint32_t)(pj * ni + pi) * 4u;
+    }
+    lpo[slot] = off;
+  }
+  auto colword = [&](int q, int lev) {  // scalar q's own column at a level, 0 where no stencil reads
+    if (lev < 1 || lev > nk) return 0.0f;
+    const float v = ldo(b.src[q] + (int64_t)(lev - 1) * mst, o_m);
+    return b.th0[q] ? (b.th0[q][lev - 1] - 300.0f) + v : v;
+  };
+  float win[NS][7];
+#pragma unroll
+  for (int q = 0; q < NS; ++q)
+#pragma unroll
+    for (int l = 0; l < 7; ++l) win[q][l] = live ? colword(q, kb - 3 + l) : 0.0f;
+  // the z face kf's mass flux and weights, shared by the scalars
+  auto zmass = [&](int kf, float& m, float& cc1, float& cc2) {
+    m = ldo(a.rrw + (int64_t)kf * st, o_s);
+    const float* p1 = a.c1 + (int64_t)(kf - 1) * mst;
+    const float* p2 = a.c2 + (int64_t)(kf - 1) * mst;
+    cc1 = a.c12col ? p1[0] : ldo(p1, o_m);
+    cc2 = a.c12col ? p2[0] : ldo(p2, o_m);
+  };
+  float fzb[NS];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) fzb[q] = 0.0f;
+  if (live && kb >= 2) {
+    float m, cc1, cc2;
+    zmass(kb, m, cc1, cc2);
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      const float* w = win[q];
+      fzb[q] = zflux_g<false>([&](int level) { return w[level - (kb - 3)]; }, m, cc1, cc2, kb, nk, a.weps);
+    }
+  }
+  const float gzc = (!FLAT && owned) ? ldo(a.gz, o_m) : 0.0f;
+  struct Lv {
+    float plane[NS][PS];
+    float snext[NS];
+    float mu, mv, mx;
+  };
+  auto fetch = [&](int k) {
+    Lv v;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      v.snext[q] = live ? colword(q, k + 4) : 0.0f;
+      const float* pl = b.src[q] + (int64_t)(k - 1) * mst;
+      const float pack = b.th0[q] ? b.th0[q][k - 1] - 300.0f : 0.0f;
+#pragma unroll
+      for (int u = 0; u < PS; ++u) {
+        const float w = ldo(pl, lpo[threadIdx.x + u * NT]);
+        v.plane[q][u] = b.th0[q] ? pack + w : w;
+      }
+    }
+    const float* pu = a.rru + (int64_t)k * ust;
+    const float* pv = a.rrv + (int64_t)k * vst;
+    v.mu = ldo(pu, o_fu);
+    v.mv = ldo(pv, o_fv);
+    v.mx = xe ? ldo(pu, o_ex) : (yn ? ldo(pv, o_ex) : 0.0f);
+    return v;
+  };
+  Lv nx = fetch(kb);
+  for (int k = kb; k <= ke; ++k) {
+    const Lv cu = nx;
+    if (k < ke) nx = fetch(k + 1);
+    const int64_t lm = (int64_t)(k - 1) * mst;
+    float c_uw = 0.0f, c_ue = 0.0f, c_vs = 0.0f, c_vn = 0.0f, c_divx = 0.0f, c_rr0 = 0.0f;
+    float c_a0[NS], c_held[NS], c_divterm[NS];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) c_a0[q] = c_held[q] = c_divterm[q] = 0.0f;
+    if (owned) {
+      c_divx = ldo(a.divx + lm, o_m);
+      c_rr0 = FLAT ? a.rr0[lm] : ldo(a.rr0 + lm, o_m);
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        c_a0[q] = ldo(b.a0[q] + lm, o_m);
+        c_held[q] = ldo(b.held[q] + lm, o_m);
+        if (b.divterm[q]) c_divterm[q] = ldo(b.divterm[q] + lm, o_m);
+      }
+      if (edge) {
+        const float* pu = a.rru + (int64_t)k * ust;
+        const float* pv = a.rrv + (int64_t)k * vst;
+        c_uw = ldo(pu, o_u);
+        c_ue = ldo(pu, o_u + 4u);
+        c_vs = ldo(pv, o_s);
+        c_vn = ldo(pv, o_s + (uint32_t)nxs * 4u);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NS; ++q)
+#pragma unroll
+      for (int u = 0; u < PS; ++u) lp[q][threadIdx.x + u * NT] = cu.plane[q][u];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      const float* P = lp[q];
+      const float* xq = P + (ty + 3) * LQ + tx;
+      lfx[q][ty][tx] = face_flux<false, false>(cu.mu, xq[0], xq[1], xq[2], xq[3], xq[4], xq[5], a.weps);
+      const float* yq = P + ty * LQ + tx + 3;
+      lfy[q][ty][tx] =
+          face_flux<false, false>(cu.mv, yq[0], yq[LQ], yq[2 * LQ], yq[3 * LQ], yq[4 * LQ], yq[5 * LQ], a.weps);
+      if (xe) {
+        if (tx < TY) {
+          const float* e = P + (tx + 3) * LQ + TX;
+          lfx[q][tx][TX] = face_flux<false, false>(cu.mx, e[0], e[1], e[2], e[3], e[4], e[5], a.weps);
+        }
+      } else if (yn) {
+        const float* e = P + TY * LQ + tx + 3;
+        lfy[q][TY][tx] =
+            face_flux<false, false>(cu.mx, e[0], e[LQ], e[2 * LQ], e[3 * LQ], e[4 * LQ], e[5 * LQ], a.weps);
+      }
+    }
+    float fzt[NS];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) fzt[q] = 0.0f;
+    if (live && k + 1 <= nk) {
+      float m, cc1, cc2;
+      zmass(k + 1, m, cc1, cc2);
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        const float* w = win[q] + 1;  // S(k-2..)
+        fzt[q] = zflux_g<false>([&](int level) { return w[level - (k - 2)]; }, m, cc1, cc2, k + 1, nk, a.weps);
+      }
+    }
+    __syncthreads();
+    if (owned) {
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        float fw = lfx[q][ty][tx], fe = lfx[q][ty][tx + 1];
+        float fs = lfy[q][ty][tx], fn = lfy[q][ty + 1][tx];
+        const float sv = lp[q][(ty + 3) * LQ + tx + 3];
+        float ax, ay, az;
+        if (obc != 0u) {
+          const float uw = c_uw, ue = c_ue, vs = c_vs, vn = c_vn;  // zero off the edges, where no closure applies
+          const bool iw = (i == 0) && side_of(obc, 0) == 2u && uw >= 0.0f;
+          const bool ie = (i + 1 == ni) && side_of(obc, 1) == 2u && ue <= 0.0f;
+          const bool js = (j == 0) && side_of(obc, 2) == 2u && vs >= 0.0f;
+          const bool jn = (j + 1 == nj) && side_of(obc, 3) == 2u && vn <= 0.0f;
+          if (iw) fw = fe;
+          if (ie) fe = fw;
+          if (js) fs = fn;
+          if (jn) fn = fs;
+          ax = FLAT ? (-(fe - fw)) * a.rdx : (fw - fe) * a.rdx;
+          ay = FLAT ? (-(fn - fs)) * a.rdy : (fs - fn) * a.rdy;
+          if (iw) ax = ax - (sv * (ue - uw)) * a.rdx;
+          if (ie) ax = ax - (sv * (ue - uw)) * a.rdx;
+          if (js) ay = ay - (sv * (vn - vs)) * a.rdy;
+          if (jn) ay = ay - (sv * (vn - vs)) * a.rdy;
+        } else {
+          ax = FLAT ? (-(fe - fw)) * a.rdx : (fw - fe) * a.rdx;
+          ay = FLAT ? (-(fn - fs)) * a.rdy : (fs - fn) * a.rdy;
+        }
+        if constexpr (FLAT) {
+          if (k == 1) az = ((-fzt[q]) * a.rdsf) * a.mh[k];
+          else if (k == nk) az = (fzb[q] * a.rdsf) * a.mh[k];
+          else az = ((-(fzt[q] - fzb[q])) * a.rdsf) * a.mh[k];
+        } else {
+          if (k == 1) az = (-fzt[q]) * a.rdsf;
+          else if (k == nk) az = fzb[q] * a.rdsf;
+          else az = (fzb[q] - fzt[q]) * a.rdsf;
+        }
+        float total = (ax + ay) + az;
+        total = total + sv * c_divx;
+        const float adv = FLAT ? total * c_rr0 : (total * c_rr0) * gzc;
+        const float seed = b.divterm[q] ? c_held[q] - c_divterm[q] : c_held[q];
+        const float v = c_a0[q] + a.dttmp * (seed + adv);
+        sto(b.out[q] + lm, o_m, (fabsf(v) < 1.0e-30f) ? 0.0f : v);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      fzb[q] = fzt[q];
+#pragma unroll
+      for (int l = 0; l < 6; ++l) win[q][l] = win[q][l + 1];
+      win[q][6] = cu.snext[q];
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------------------
 // positive-definite repair of the final-stage moisture advection
 // (cm1_rk_pdef_candidate_{x,y,z}_v1 / correct_{x,y,z}_v1 / total_v1, pdef.F)
@@ -956,6 +1191,50 @@ int gm1hip_scalar_rk_stage_src(const float* src, const float* th0, const float* 
   a.rdx = 1.0f / dx; a.rdy = 1.0f / dy; a.rdsf = rdz;
   a.weps = weps; set_obc(a, obc);
   return launch_scalar_tile(a, use_weno, MODE_COMPOSE, (hipStream_t)stream);
+}
+
+int gm1hip_scalar_rk_stage_src_batch(int ns, const float* const* srcs, const float* const* th0s, const float* rru,
+                                     const float* rrv, const float* rrw, const float* c1, const float* c2,
+                                     const float* rr0, const float* gz, const float* flat_column, const float* divx,
+                                     const float* const* a0s, const float* const* helds,
+                                     const float* const* divterms, float* const* outs, int ni, int nj, int nk,
+                                     float dx, float dy, float rdz, unsigned obc, float dttmp, void* stream) {
+  if (ns < 1 || ns > NSMAX || !srcs || !a0s || !helds || !outs || (obc & 0xFFFFu) == 0u ||
+      (gz == nullptr) == (flat_column == nullptr))
+    return (int)hipErrorInvalidValue;
+  ScalarAdvArgs a{};
+  a.rru = rru; a.rrv = rrv; a.rrw = rrw; a.c1 = c1; a.c2 = c2;
+  a.divx = divx; a.rr0 = rr0; a.gz = gz; a.mh = flat_column; a.dttmp = dttmp;
+  a.ni = ni; a.nj = nj; a.nk = nk;
+  a.rdx = 1.0f / dx; a.rdy = 1.0f / dy; a.rdsf = rdz;
+  a.weps = 0.0; set_obc(a, obc);
+  ScalarBatch b{};
+  for (int q = 0; q < ns; ++q) {
+    if (!srcs[q] || !a0s[q] || !helds[q] || !outs[q]) return (int)hipErrorInvalidValue;
+    b.src[q] = srcs[q]; b.th0[q] = th0s ? th0s[q] : nullptr; b.a0[q] = a0s[q]; b.held[q] = helds[q];
+    b.divterm[q] = divterms ? divterms[q] : nullptr; b.out[q] = outs[q];
+  }
+  const int tiles = ((ni + TX - 1) / TX) * ((nj + TY - 1) / TY);
+  a.nchunks = chunks_for(tiles, nk, &a.kchunk);
+  const dim3 grid(tiles * a.nchunks);
+  hipStream_t s = (hipStream_t)stream;
+  const bool flat = flat_column != nullptr;
+#define GM1_BATCH(N)                                                                                           \
+  do {                                                                                                         \
+    if (flat)                                                                                                  \
+      GM1_TIMED(TK_SCALAR_ADV5, s, hipLaunchKernelGGL((k_scalar_compose_batch<N, true>), grid, dim3(NT), 0, s, a, b)); \
+    else                                                                                                       \
+      GM1_TIMED(TK_SCALAR_ADV5, s, hipLaunchKernelGGL((k_scalar_compose_batch<N, false>), grid, dim3(NT), 0, s, a, b)); \
+  } while (0)
+  switch (ns) {
+    case 1: GM1_BATCH(1); break;
+    case 2: GM1_BATCH(2); break;
+    case 3: GM1_BATCH(3); break;
+    default: GM1_BATCH(4); break;
+  }
+#undef GM1_BATCH
+  GM1_LAUNCH_CHECK();
+  return 0;
 }
 
 int gm1hip_pdef_moisture_stage_src(const float* src, const float* rru, const float* rrv, const float* rrw,

@@ -45,8 +45,8 @@ from .kessler import (
 )
 from .errors import refuse
 from .psolver3 import TerrainOpenSoundAblations, device_profiles, launch_cuda_psolver3_terrain_open_stage
-from .scalar_moist_rk import (launch_scalar_stage_from_carrier, moisture_weps_f64, theta_weps_f64,
-                              verify_horizontally_uniform)
+from .scalar_moist_rk import (BATCH_MAX, launch_scalar_stage_from_carrier, launch_scalar_stages_from_carriers,
+                              moisture_weps_f64, theta_weps_f64, verify_horizontally_uniform)
 
 signature("gm1hip_stage_terms", P, P, P, P, P, P, P, I, P, P, P, P, P, P, I, I, I, U, P)
 signature("gm1hip_buoyancy", P, P, P, P, P, P, P, P, I, I, I, P)
@@ -162,6 +162,8 @@ class Gm1LargeStepState:
     # hook has run: a step without a hook then skips the fills and its passes form 0 + x themselves
     held_clean: bool = False
     zero_held: bool = False
+    # scalars composed together in one pass at the stages before the last (1: one pass each)
+    scalar_batch: int = 2
 
 
 def slopes_are_zero(metrics: dict[str, Any]) -> bool:
@@ -335,6 +337,17 @@ def _scalar_stages(st: Gm1LargeStepState, nrk: int, mass: dict[str, Any], *, fla
     else:
         src_th, src_q = a["th3d"], a.get("q3d")
         out_th, out_q = a["scalar_next"], None
+    if alternate and nrk < 3 and st.scalar_batch > 1:
+        # theta' and the species in batches sharing the mass fluxes: the same words, fewer passes
+        rows = [(src_th, b["th0"], a["tha"], a["thten1"], a["theta_div"], out_th)]
+        rows += [(src_q[sp], None, a["qa"][sp], a["qten"][sp], None, out_q[sp]) for sp in range(st.numq)]
+        grid.pop("copy_back")
+        n = min(st.scalar_batch, BATCH_MAX)
+        for lo in range(0, len(rows), n):
+            cols = list(zip(*rows[lo:lo + n]))
+            launch_scalar_stages_from_carriers(list(cols[0]), list(cols[1]), mass, m, a["divx"], list(cols[2]),
+                                               list(cols[3]), list(cols[4]), list(cols[5]), **grid)
+        return
     launch_scalar_stage_from_carrier(src_th, b["th0"], mass, m, a["divx"], a["tha"], a["thten1"],
                                      a["theta_div"], out_th, weps=THETA_WEPS, **grid)
     for sp in range(st.numq):

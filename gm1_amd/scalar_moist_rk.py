@@ -178,6 +178,56 @@ def launch_scalar_stage_from_carrier(
             carrier.copy_(scratch)
 
 
+signature("gm1hip_scalar_rk_stage_src_batch", I, P, P, *([P] * 9), P, P, P, P, I, I, I, Fl, Fl, Fl, U, Fl, P)
+BATCH_MAX = 4
+
+
+def launch_scalar_stages_from_carriers(
+    carriers: list, th0s: list, mass_flux: Mapping[str, Any], metrics: Mapping[str, Any], divx: Any, a0s: list,
+    helds: list, divterms: list, outs: list, *, ni: int, nj: int, nk: int, nrk: int, dt: float, dx: float, dy: float,
+    rdz: float, lateral_sides, flat_column: Any | None = None, stream: Any | None = None, c12_columns: bool = False,
+) -> None:
+    """Several scalars' stage before the last (fifth-order fluxes) in one pass, each into its ``outs`` entry.
+
+    Each output holds the words :func:`launch_scalar_stage_from_carrier` with
+    ``copy_back=False`` leaves for that scalar; the mass fluxes, divx and
+    the tile addressing are shared.  ``th0s`` / ``divterms`` entries may be
+    None.  The reference advects theta and each species in passes of their
+    own (cuda_scalar_moist_rk.py:1296, 1314-1340).
+    """
+    ns = len(carriers)
+    if not 1 <= ns <= BATCH_MAX or not all(len(x) == ns for x in (th0s, a0s, helds, divterms, outs)):
+        raise ValueError(f"a batch holds 1..{BATCH_MAX} scalars, with one entry per scalar in every list")
+    if nrk >= CM1_NRKMAX:
+        raise ValueError("the batched compose is the fifth-order form of the stages before the last")
+    cell = (nk, nj, ni)
+    sh = shapes(ni, nj, nk)
+    require(divx, "divx", cell)
+    for q in range(ns):
+        for name, t in (("carrier", carriers[q]), ("a0", a0s[q]), ("held", helds[q]), ("out", outs[q])):
+            require(t, f"{name}[{q}]", cell)
+        if divterms[q] is not None:
+            require(divterms[q], f"divterm[{q}]", cell)
+        if th0s[q] is not None:
+            require(th0s[q], f"th0[{q}]", (nk,))
+    written = {int(t.data_ptr()) for t in outs}
+    if len(written) != ns or written & {int(t.data_ptr()) for t in carriers}:
+        raise ValueError("each output must be its own array, apart from every carrier")
+    rru, rrv, rrw = (require(mass_flux["rru"], "rru", sh["u"]), require(mass_flux["rrv"], "rrv", sh["v"]),
+                     require(mass_flux["rrw"], "rrw", sh["w"]))
+    terrain = flat_column is None
+    obc = lateral_code(lateral_sides) | (OBC_C12_COLUMN if c12_columns else 0)
+    dttmp = scalar_moist_rk_stage_timestep_f32(dt, nrk)
+
+    def arr(ts):
+        return (ctypes.c_void_p * ns)(*[ptr(t) for t in ts])
+
+    _native.call("gm1hip_scalar_rk_stage_src_batch", ns, arr(carriers), arr(th0s), ptr(rru), ptr(rrv), ptr(rrw),
+                 ptr(metrics["c1"]), ptr(metrics["c2"]), ptr(metrics["rr0"]), ptr(metrics["gz"] if terrain else None),
+                 ptr(flat_column), ptr(divx), arr(a0s), arr(helds), arr(divterms), arr(outs), ni, nj, nk,
+                 float(dx), float(dy), float(rdz), obc, dttmp, stream_handle(stream))
+
+
 # bit 16 of the lateral-family word: c1 and c2 are horizontally uniform
 OBC_C12_COLUMN = 1 << 16
 _uniform_checked: set[tuple] = set()

@@ -415,6 +415,20 @@ int gm1hip_scalar_rk_stage_src(const float* src, const float* th0, const float* 
                                int use_weno, double weps, float dx, float dy, float rdz, unsigned obc, float dttmp,
                                void* stream);
 
+// gm1hip_scalar_rk_stage_src with use_weno = 0 for ns (1..4) scalars in one
+// pass: host arrays of ns device pointers (th0s and divterms may be NULL, or
+// hold NULL entries), every other argument shared.  Each out[q] receives the
+// words gm1hip_scalar_rk_stage_src writes for scalar q; the reference advects
+// theta and then each species in passes of their own within one stage
+// (cuda_scalar_moist_rk.py:1296 and :1314-1340), this form shares the mass
+// fluxes, divx and addressing among them.
+int gm1hip_scalar_rk_stage_src_batch(int ns, const float* const* srcs, const float* const* th0s, const float* rru,
+                                     const float* rrv, const float* rrw, const float* c1, const float* c2,
+                                     const float* rr0, const float* gz, const float* flat_column, const float* divx,
+                                     const float* const* a0s, const float* const* helds,
+                                     const float* const* divterms, float* const* outs, int ni, int nj, int nk,
+                                     float dx, float dy, float rdz, unsigned obc, float dttmp, void* stream);
+
 // gm1hip_pdef_moisture_stage / _flat on an unpadded moisture carrier `src`,
 // as gm1hip_scalar_rk_stage_src (terrain: gz and rgz; flat: flat_column).
 int gm1hip_pdef_moisture_stage_src(const float* src, const float* rru, const float* rrv, const float* rrw,

@@ -152,6 +152,60 @@ def test_gpu_fused_scalar_compose_bit_exact(device, nrk):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("sides", [(3, 4, 2, 2), (1, 1, 1, 1), (2, 2, 4, 3)])
+@pytest.mark.parametrize("ns", [1, 2, 3, 4])
+def test_gpu_batched_compose_equals_single_passes(device, sides, ns):
+    """ns scalars composed in one pass (theta' with its pack and divergence term first, then species) write
+    the words of their own single-scalar passes, at stages 1 and 2."""
+    import torch
+
+    from gm1_amd.scalar_moist_rk import launch_scalar_stage_from_carrier, launch_scalar_stages_from_carriers
+
+    ni, nj, nk = 70, 21, 12
+    rng = np.random.default_rng(500 + ns + sum(sides))
+    a, m, divx = _moist_inputs(rng, ni, nj, nk)
+    cell = (nk, nj, ni)
+    d = lambda x: to_dev(x, device)  # noqa: E731
+    mass = {k: d(a[k]) for k in ("rru", "rrv", "rrw")}
+    grid = dict(ni=ni, nj=nj, nk=nk, dt=1.5, dx=250.0, dy=300.0, rdz=float(F(1.0) / F(200.0)), lateral_sides=sides)
+    th0 = d((F(300.0) + np.linspace(0.0, 40.0, nk)).astype(F))
+    rows = []
+    for q in range(ns):
+        theta = q == 0
+        x = rng.normal(0.0, 2.0, cell) if theta else np.abs(rng.normal(0.0, 1e-3, cell))
+        rows.append((d(x.astype(F)), th0 if theta else None, d(rng.normal(0.0, 1.0, cell).astype(F)),
+                     d(rng.normal(0.0, 1e-3, cell).astype(F)),
+                     d(rng.normal(0.0, 1e-3, cell).astype(F)) if theta else None))
+    for nrk in (1, 2):
+        print("batched compose", ns, sides, nrk, flush=True)
+        want = []
+        for src, t0, a0, held, divterm in rows:
+            out = torch.zeros(cell, dtype=torch.float32, device=device)
+            launch_scalar_stage_from_carrier(src, t0, mass, d(m), d(divx), a0, held, divterm, out, nrk=nrk,
+                                             weps=1e-17, copy_back=False, **grid)
+            want.append(out)
+        got = [torch.full(cell, float("nan"), dtype=torch.float32, device=device) for _ in rows]
+        cols = list(zip(*rows))
+        launch_scalar_stages_from_carriers(list(cols[0]), list(cols[1]), mass, d(m), d(divx), list(cols[2]),
+                                           list(cols[3]), list(cols[4]), got, nrk=nrk, **grid)
+        torch.cuda.synchronize()
+        for q in range(ns):
+            assert_bits_equal(to_host(got[q]), to_host(want[q]), f"scalar {q} of {ns}, nrk={nrk}")
+
+
+def test_batched_compose_refusals():
+    """The batch launcher refuses the final stage, more than four scalars and outputs that alias."""
+    from gm1_amd.scalar_moist_rk import launch_scalar_stages_from_carriers
+
+    kw = dict(ni=8, nj=8, nk=4, dt=1.0, dx=1.0, dy=1.0, rdz=1.0, lateral_sides=(1, 1, 1, 1))
+    with pytest.raises(ValueError):
+        launch_scalar_stages_from_carriers([None] * 5, [None] * 5, {}, {}, None, [None] * 5, [None] * 5, [None] * 5,
+                                           [None] * 5, nrk=1, **kw)
+    with pytest.raises(ValueError):
+        launch_scalar_stages_from_carriers([None], [None], {}, {}, None, [None], [None], [None], [None], nrk=3, **kw)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sides", [(3, 4, 2, 2), (1, 1, 1, 1), (2, 2, 4, 3)])
 @pytest.mark.parametrize("row", ["theta-nrk1", "theta-nrk3", "qv-pdef", "qc-pdef"])
 def test_gpu_stage_from_unpadded_carrier_bit_exact(device, sides, row):
     """The carrier entry points (pack and lateral fill formed in the tiles' loads) against the oracle's
