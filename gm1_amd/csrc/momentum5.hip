@@ -8,6 +8,7 @@
 //   * k_mom_mass_v   : one thread per column marches k, forms rrw at every
 //                      w level and divx from the rrw pair it holds in
 //                      registers (one launch instead of two).
+//   * k_mom_mass_flat: the flat column's rru, rrv, rrw and divx in one pass.
 //   * k_mom_adv{u,v,w}_tile: an LDS-staged tile marching a chunk of levels,
 //                      one lane per flux point.
 #include "cm1_common.h"
@@ -153,56 +154,63 @@ __global__ __launch_bounds__(256) void k_mom_mass_v(
 // Flat stretched column (solve2.F:493-520, 605-617 with terrain_flag=.false.):
 // rru = rho0(1,1,k)*u, rrv = rho0(1,1,k)*v, rrw = rf0(1,1,k)*w (zero at both
 // lids), divx's vertical term ((d*rdz)*mh(k)) and smeps 1e-30.
-__global__ __launch_bounds__(256) void k_mom_mass_h_flat(
-    const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ rho0,
-    float* __restrict__ rru, float* __restrict__ rrv, int ni, int nj, int nk) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y;
-#pragma unroll
-  for (int uu = 0; uu < MKB; ++uu) {
-    const int k0 = min((int)blockIdx.z * MKB + uu, nk - 1), k = k0 + 1;  // clamped: the top level again
-    const float rc = rho0[(int64_t)k0 * nj * ni];
-    if (i <= ni && j < nj) {
-      const int64_t p = ix3(k, j + NG, i + NG, sy(nj), ux(ni));
-      rru[p] = rc * u[p];
-    }
-    if (i < ni && j <= nj) {
-      const int64_t p = ix3(k, j + NG, i + NG, vy(nj), sx(ni));
-      rrv[p] = rc * v[p];
-    }
-  }
-}
-
-__global__ __launch_bounds__(256) void k_mom_mass_v_flat(
-    const float* __restrict__ w, const float* __restrict__ rf0, const float* __restrict__ rru,
-    const float* __restrict__ rrv, const float* __restrict__ mh, float* __restrict__ rrw,
-    float* __restrict__ divx, int ni, int nj, int nk, float rdx, float rdy, float rdz) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y;
-  if (i >= ni) return;
+// The flat column's mass fluxes in one pass: rru = rho0(k)*u on faces
+// 0..ni, rrv = rho0(k)*v on faces 0..nj, rrw = rf0(kf)*w on w levels 2..nk
+// (0 at 1 and nk+1), and divx from them.  A cell forms its east and north
+// faces' products itself (the same product its neighbour writes), so no
+// thread reads a word another writes.
+__global__ __launch_bounds__(256) void k_mom_mass_flat(
+    const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+    const float* __restrict__ rho0, const float* __restrict__ rf0, const float* __restrict__ mh,
+    float* __restrict__ rru, float* __restrict__ rrv, float* __restrict__ rrw, float* __restrict__ divx, int ni,
+    int nj, int nk, float rdx, float rdy, float rdz) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // 0..ni
+  const int j = blockIdx.y;                             // 0..nj
+  if (i > ni) return;
   const int nxu = ux(ni), nyu = sy(nj), nxs = sx(ni), nyv = vy(nj);
   const int x = i + NG, y = j + NG;
-  const int64_t h = (int64_t)j * ni + i;
-  const int64_t wst = (int64_t)nyu * nxs;
-  const int64_t wcol = (int64_t)y * nxs + x;
+  const int64_t plane = (int64_t)nj * ni;
+  const int64_t ust = (int64_t)nyu * nxu, vst = (int64_t)nyv * nxs, wst = (int64_t)nyu * nxs;
+  const int64_t uc = (int64_t)y * nxu + x, vc = (int64_t)y * nxs + x, wc = vc;
+  const bool uface = j < nj, vface = i < ni, cell = i < ni && j < nj;
   auto rrw_at = [&](int kf) {
-    return (kf <= 1 || kf > nk) ? 0.0f : rf0[(int64_t)(kf - 1) * nj * ni] * w[(int64_t)kf * wst + wcol];
+    return (kf <= 1 || kf > nk) ? 0.0f : rf0[(int64_t)(kf - 1) * plane] * w[(int64_t)kf * wst + wc];
   };
-  // this thread's chunk: w levels kf0 .. kf0+MKB-1 of 2..nk+1 (rrw(1) = 0 by the first chunk)
-  const int kf0 = 2 + (int)blockIdx.z * MKB;
-  if (kf0 == 2) rrw[wst + wcol] = 0.0f;
-  float below = rrw_at(kf0 - 1);
+  const int kb = 1 + (int)blockIdx.z * MKB;
+  // every word of the chunk requested first (levels past the top clamped to it), then formed
+  float u0[MKB], u1[MKB], v0[MKB], v1[MKB], wk[MKB], rc[MKB], rf[MKB];
 #pragma unroll
   for (int q = 0; q < MKB; ++q) {
-    const int kf = min(kf0 + q, nk + 1);  // clamped past w level nk+1: the top pair again
-    const float val = rrw_at(kf);
-    rrw[(int64_t)kf * wst + wcol] = val;
-    const int k = kf - 1;
-    float d = (rru[ix3(k, y, x + 1, nyu, nxu)] - rru[ix3(k, y, x, nyu, nxu)]) * rdx +
-              (rrv[ix3(k, y + 1, x, nyv, nxs)] - rrv[ix3(k, y, x, nyv, nxs)]) * rdy;
-    d = d + ((val - below) * rdz) * mh[k];
-    divx[(int64_t)(k - 1) * nj * ni + h] = (fabsf(d) < 1.0e-30f) ? 0.0f : d;
-    below = (kf <= nk) ? val : below;  // the top pair keeps its lower word for the clamped repeats
+    const int k = min(kb + q, nk), kf = min(k + 1, nk);
+    rc[q] = rho0[(int64_t)(k - 1) * plane];
+    rf[q] = rf0[(int64_t)(kf - 1) * plane];
+    const float* uk = u + (int64_t)k * ust;
+    const float* vk = v + (int64_t)k * vst;
+    u0[q] = uface ? uk[uc] : 0.0f;
+    v0[q] = vface ? vk[vc] : 0.0f;
+    u1[q] = cell ? uk[uc + 1] : 0.0f;
+    v1[q] = cell ? vk[vc + nxs] : 0.0f;
+    wk[q] = cell ? w[(int64_t)kf * wst + wc] : 0.0f;
+  }
+  if (cell && kb == 1) rrw[wst + wc] = 0.0f;
+  float below = cell ? rrw_at(kb) : 0.0f;
+#pragma unroll
+  for (int q = 0; q < MKB; ++q) {
+    const int k = kb + q;
+    if (k > nk) break;
+    const float ru0 = rc[q] * u0[q], rv0 = rc[q] * v0[q];
+    if (uface) rru[(int64_t)k * ust + uc] = ru0;
+    if (vface) rrv[(int64_t)k * vst + vc] = rv0;
+    if (cell) {
+      const float ru1 = rc[q] * u1[q], rv1 = rc[q] * v1[q];
+      const int kf = k + 1;
+      const float val = (kf > nk) ? 0.0f : rf[q] * wk[q];
+      rrw[(int64_t)kf * wst + wc] = val;
+      float d = (ru1 - ru0) * rdx + (rv1 - rv0) * rdy;
+      d = d + ((val - below) * rdz) * mh[k];
+      divx[(int64_t)(k - 1) * plane + (int64_t)j * ni + i] = (fabsf(d) < 1.0e-30f) ? 0.0f : d;
+      below = val;
+    }
   }
 }
 
@@ -1056,11 +1064,8 @@ int momentum5_flat(
   }
   const float rdx = 1.0f / dx, rdy = 1.0f / dy;
   dim3 blk(64);
-  hipLaunchKernelGGL(k_mom_mass_h_flat, dim3((ni + 1 + 63) / 64, nj + 1, (nk + MKB - 1) / MKB), blk, 0, s, u3d, v3d,
-                     rho0_3d, rru, rrv, ni, nj, nk);
-  GM1_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_mom_mass_v_flat, dim3((ni + 63) / 64, nj, (nk + MKB - 1) / MKB), blk, 0, s, w3d, rf0, rru, rrv,
-                     flat_column, rrw, divx, ni, nj, nk, rdx, rdy, rdz);
+  hipLaunchKernelGGL(k_mom_mass_flat, dim3((ni + 1 + 63) / 64, nj + 1, (nk + MKB - 1) / MKB), blk, 0, s, u3d, v3d,
+                     w3d, rho0_3d, rf0, flat_column, rru, rrv, rrw, divx, ni, nj, nk, rdx, rdy, rdz);
   GM1_LAUNCH_CHECK();
   MomArgs a;
   a.u = u3d; a.v = v3d; a.w = w3d; a.rru = rru; a.rrv = rrv; a.rrw = rrw; a.divx = divx;
