@@ -472,42 +472,53 @@ __global__ __launch_bounds__(64) void k_getdivx_v_flat(FlatDivArgs a) {
   }
 }
 
-// k_getdivx_v_flat over GX_SLOTS level slots of one workgroup (raw words read
-// before the barrier, rewritten after it), as k_getdivx_v_slots.
-__global__ __launch_bounds__(64 * GX_SLOTS) void k_getdivx_v_flat_slots(FlatDivArgs a) {
+// getdiv and getdivx's vertical pass of the flat column in one pass over
+// GX_SLOTS level slots of a workgroup: each slot reads the raw averaged w of
+// its levels (and the level above them) before the barrier, then forms div and
+// theta_div from the raw averages, rewrites rrw = rf0*w in place and forms
+// divx from the rescaled faces -- its own rho0(k)*rru / rho0(k)*rrv products,
+// the words k_getdivx_h_flat writes after it.
+__global__ __launch_bounds__(64 * GX_SLOTS) void k_post_acoustic_flat(FlatDivArgs a) {
   const int c = threadIdx.x % 64, slot = threadIdx.x / 64;
   const int i = blockIdx.x * 64 + c;
   const int j = blockIdx.y;
   const int ni = a.ni, nj = a.nj, nk = a.nk;
   const bool live = i < ni;
   const int L = (nk + GX_SLOTS - 1) / GX_SLOTS;
-  const int kf0 = 2 + slot * L, kf1 = min(kf0 + L - 1, nk + 1);
+  const int k0 = 1 + slot * L, k1 = min(k0 + L - 1, nk);  // this slot's levels
   const int nxu = ux(ni), nyu = sy(nj), nxs = sx(ni), nyv = vy(nj);
   const int x = i + NG, y = j + NG;
-  const int64_t h = (int64_t)j * ni + i;
+  const int64_t h = (int64_t)j * ni + i, plane = (int64_t)nj * ni;
   const int64_t wst = (int64_t)nyu * nxs, wc = (int64_t)y * nxs + x;
-  float raw[GX_MAXL + 1];  // raw[q]: averaged w at level kf0 - 1 + q
+  float raw[GX_MAXL + 1];  // raw[q]: averaged w at level k0 + q
 #pragma unroll
   for (int q = 0; q <= GX_MAXL; ++q) {
-    const int kf = kf0 - 1 + q;
-    raw[q] = (live && kf <= kf1 && kf >= 2 && kf <= nk) ? a.rrw[(int64_t)kf * wst + wc] : 0.0f;
+    const int kf = k0 + q;
+    raw[q] = (live && kf <= k1 + 1) ? a.rrw[(int64_t)kf * wst + wc] : 0.0f;
   }
   __syncthreads();
-  if (!live || kf0 > kf1) return;
-  auto val_at = [&](int kf, float w) { return (kf >= 2 && kf <= nk) ? w * a.rf0[(int64_t)(kf - 1) * nj * ni] : 0.0f; };
-  if (kf0 == 2) a.rrw[wst + wc] = 0.0f;
-  float below = val_at(kf0 - 1, raw[0]);
+  if (!live || k0 > k1) return;
+  auto val_at = [&](int kf, float w) { return (kf >= 2 && kf <= nk) ? w * a.rf0[(int64_t)(kf - 1) * plane] : 0.0f; };
+  if (k0 == 1) a.rrw[wst + wc] = 0.0f;
+  float below = val_at(k0, raw[0]);
 #pragma unroll
-  for (int q = 1; q <= GX_MAXL; ++q) {
-    const int kf = kf0 - 1 + q;
-    if (kf > kf1) break;
-    const float val = val_at(kf, raw[q]);
-    a.rrw[(int64_t)kf * wst + wc] = val;
-    const int k = kf - 1;
-    float d = (a.rru[ix3(k, y, x + 1, nyu, nxu)] - a.rru[ix3(k, y, x, nyu, nxu)]) * a.rdx +
-              (a.rrv[ix3(k, y + 1, x, nyv, nxs)] - a.rrv[ix3(k, y, x, nyv, nxs)]) * a.rdy;
-    d = d + ((val - below) * a.rdz) * a.mh[k];
-    a.divx[(int64_t)(k - 1) * nj * ni + h] = (fabsf(d) < 1.0e-30f) ? 0.0f : d;
+  for (int q = 0; q < GX_MAXL; ++q) {
+    const int k = k0 + q;
+    if (k > k1) break;
+    const float ru0 = a.rru[ix3(k, y, x, nyu, nxu)], ru1 = a.rru[ix3(k, y, x + 1, nyu, nxu)];
+    const float rv0 = a.rrv[ix3(k, y, x, nyv, nxs)], rv1 = a.rrv[ix3(k, y + 1, x, nyv, nxs)];
+    const int64_t z = (int64_t)(k - 1) * plane + h;
+    float d = (ru1 - ru0) * a.rdx + (rv1 - rv0) * a.rdy;
+    d = d + ((raw[q + 1] - raw[q]) * a.rdz) * a.mh[k];
+    if (fabsf(d) < 1.0e-30f) d = 0.0f;
+    a.div[z] = d;
+    a.theta_div[z] = d * a.thterm[z];
+    const float rc = a.rho0[(int64_t)(k - 1) * plane];
+    const float val = val_at(k + 1, raw[q + 1]);
+    a.rrw[(int64_t)(k + 1) * wst + wc] = val;
+    float dx = (ru1 * rc - ru0 * rc) * a.rdx + (rv1 * rc - rv0 * rc) * a.rdy;
+    dx = dx + ((val - below) * a.rdz) * a.mh[k];
+    a.divx[z] = (fabsf(dx) < 1.0e-30f) ? 0.0f : dx;
     below = val;
   }
 }
@@ -676,12 +687,14 @@ int gm1hip_post_acoustic_flat(float* rru, float* rrv, float* rrw, const float* r
   hipStream_t s = (hipStream_t)stream;
   FlatDivArgs a{rru, rrv, rrw, rho0_3d, rf0, flat_column, thterm, divergence, theta_div, divx,
                 ni, nj, nk, 1.0f / dx, 1.0f / dy, rdz};
-  hipLaunchKernelGGL(k_getdiv_flat, dim3((ni + 63) / 64, nj, (nk + KB - 1) / KB), dim3(64), 0, s, a);
-  hipLaunchKernelGGL(k_getdivx_h_flat, dim3((ni + 1 + 63) / 64, nj + 1, (nk + KB - 1) / KB), dim3(64), 0, s, a);
-  if (slotted_columns(ni, nj, nk))
-    hipLaunchKernelGGL(k_getdivx_v_flat_slots, dim3((ni + 63) / 64, nj), dim3(64 * GX_SLOTS), 0, s, a);
-  else
+  if (nk <= GX_SLOTS * GX_MAXL) {  // getdiv and getdivx's vertical pass together, then the faces rescaled
+    hipLaunchKernelGGL(k_post_acoustic_flat, dim3((ni + 63) / 64, nj), dim3(64 * GX_SLOTS), 0, s, a);
+    hipLaunchKernelGGL(k_getdivx_h_flat, dim3((ni + 1 + 63) / 64, nj + 1, (nk + KB - 1) / KB), dim3(64), 0, s, a);
+  } else {
+    hipLaunchKernelGGL(k_getdiv_flat, dim3((ni + 63) / 64, nj, (nk + KB - 1) / KB), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_getdivx_h_flat, dim3((ni + 1 + 63) / 64, nj + 1, (nk + KB - 1) / KB), dim3(64), 0, s, a);
     hipLaunchKernelGGL(k_getdivx_v_flat, dim3((ni + 63) / 64, nj), dim3(64), 0, s, a);
+  }
   GM1_LAUNCH_CHECK();
   return 0;
 }
