@@ -152,8 +152,8 @@ class Gm1LargeStepState:
     slopes_zero: bool = False
     # c1 and c2 hold one word per level at every column (flat terrain): the scalar advection reads one column
     c12_columns: bool = False
-    # the scalars' second carriers: stage 1 writes them, stage 2 reads them back into th3d/q3d,
-    # stage 3 writes them again and one copy returns the result (none between stages 1 and 2)
+    # the scalars' second and third carriers: stage 1 writes the second (th3d, q3d), stage 2 the third
+    # (th3d2, q3d2), stage 3 reads those and writes th3d/q3d; also the fused column passes' workspace
     alt: dict = field(default_factory=dict)
     receipts: list = field(default_factory=list)
     # a stretched (flat) column runs the terrain_flag=.false. family
@@ -204,12 +204,13 @@ def create_gm1_large_step_state(host: dict[str, Any], device: torch.device | str
     device_profiles(st.prof, dev)
     if st.flat is not None:
         device_profiles(st.flat.levels.profiles(), dev)
-    st.alt = {"th3d": torch.empty_like(st.a["th3d"])}
+    st.alt = {"th3d": torch.empty_like(st.a["th3d"]), "th3d2": torch.empty_like(st.a["th3d"])}
     if st.flat is not None:  # the fused column passes' wind and ppd workspace
         for name in ("u3d", "v3d", "ppd"):
             st.alt[name] = torch.empty_like(st.a[name])
     if st.numq:
         st.alt["q3d"] = torch.empty_like(st.a["q3d"])
+        st.alt["q3d2"] = torch.empty_like(st.a["q3d"])
         cfg = st.cfg
         st.selection = Gm1KesslerDriverSelection(equation_set=equation_set)
         st.kessler_workspace = create_cuda_gm1_kessler_driver_workspace(cfg.ni, cfg.nj, cfg.nk, dev)
@@ -236,10 +237,13 @@ def _kessler_arrays(st: Gm1LargeStepState) -> dict[str, torch.Tensor]:
 
 
 def _scalar_carriers(st: Gm1LargeStepState, nrk: int, alternate: bool) -> tuple[Any, Any]:
-    """Where this stage's theta' and moisture live: the second carriers during stage 2 when alternating."""
+    """Where this stage's theta' and moisture live: when alternating, the second carriers during stage 2 and
+    the third during stage 3."""
     a = st.a
     if alternate and nrk == 2:
         return st.alt["th3d"], st.alt.get("q3d")
+    if alternate and nrk == 3:
+        return st.alt["th3d2"], st.alt.get("q3d2")
     return a["th3d"], a.get("q3d")
 
 
@@ -320,8 +324,8 @@ def _scalar_stages(st: Gm1LargeStepState, nrk: int, mass: dict[str, Any], *, fla
                    rdz: float, alternate: bool = False) -> None:
     """Theta' then each moisture species (the final stage's rows through the pdef chain).
 
-    ``alternate``: stage 1 writes the second carriers (st.alt), stage 2 reads them and writes th3d/q3d, stage 3
-    writes the second carriers and copies them back once -- the same words as a copy-back after every scalar.
+    ``alternate``: stage 1 writes the second carriers (st.alt), stage 2 reads them and writes the third, stage 3
+    reads those and writes th3d/q3d -- the same words as a copy-back after every scalar, with no copy.
     """
     cfg, a, m, b = st.cfg, st.a, st.metrics, st.base
     grid = dict(ni=cfg.ni, nj=cfg.nj, nk=cfg.nk, nrk=nrk, dt=cfg.dt, dx=cfg.dx, dy=cfg.dy, rdz=rdz,
@@ -329,10 +333,12 @@ def _scalar_stages(st: Gm1LargeStepState, nrk: int, mass: dict[str, Any], *, fla
                 c12_columns=st.c12_columns)
     if alternate:
         src_th, src_q = _scalar_carriers(st, nrk, True)
-        if nrk == 2:
-            out_th, out_q = a["th3d"], a.get("q3d")
-        else:
+        if nrk == 1:
             out_th, out_q = st.alt["th3d"], st.alt.get("q3d")
+        elif nrk == 2:
+            out_th, out_q = st.alt["th3d2"], st.alt.get("q3d2")
+        else:
+            out_th, out_q = a["th3d"], a.get("q3d")
         grid["copy_back"] = False
     else:
         src_th, src_q = a["th3d"], a.get("q3d")
@@ -360,11 +366,6 @@ def _scalar_stages(st: Gm1LargeStepState, nrk: int, mass: dict[str, Any], *, fla
                                              scratch, pdef_species=sp,
                                              pdef_scratch={"advx": a["advx"], "advy": a["advy"],
                                                            "advz": a["advz"]}, **grid)
-    if alternate and nrk == 3:
-        with torch.cuda.stream(st.stream) if st.stream is not None else _nullcontext():
-            a["th3d"].copy_(st.alt["th3d"])
-            if st.numq:
-                a["q3d"].copy_(st.alt["q3d"])
 
 
 @dataclass(frozen=True)
@@ -437,7 +438,7 @@ def launch_cuda_gm1_large_step(
                      ptr(a["qpten"]), ptr(a["qtten"]), ptr(a["qvten"]), ptr(a["qcten"]), ni, nj, nk, cfg.dt,
                      int(st.zero_held), h)
     stage_hooks = []
-    # with no hook between stages, the scalars alternate two carriers (one copy-back per step)
+    # with no hook between stages, the scalars rotate through three carriers (no copy-back)
     alternate = post_stage_hook is None
     for nrk in (1, 2, 3):
         _rk_stage(st, nrk, alternate)
