@@ -592,16 +592,16 @@ __global__ __launch_bounds__(256) void k_kes_budgets(const double* __restrict__ 
 // ------------------------------------------------------------------ transition
 // eqtset=2 held remove/snapshot, then the pre-physics EOS; pp3d is taken from
 // the dynamics' pressure family (one-cell halo) into the mass array.
-__global__ __launch_bounds__(256) void k_kes_transition_pre(
-    const float* __restrict__ pp_family, float* __restrict__ pp3d, float* __restrict__ th3d, float* __restrict__ qv,
-    float* __restrict__ qc, float* __restrict__ qpten, float* __restrict__ qtten, float* __restrict__ qvten,
-    float* __restrict__ qcten, const float* __restrict__ pi0, const float* __restrict__ th0, float* __restrict__ prs,
-    float* __restrict__ rho, float* __restrict__ rr, int ni, int nj, int nk, float dt, int eqtset, int copy_rr) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z;
-  if (i >= ni) return;
-  const int64_t p = ((int64_t)k * nj + j) * ni + i;
-  float pp = pp_family ? pp_family[pix(k + 1, j + 1, i + 1, ni, nj)] : pp3d[p];
+// One cell of the pre-physics transition: the eqtset=2 held remove/snapshot,
+// pp3d/th3d/qv/qc written back, and the equation of state (prs, rho, and rr
+// when copy_rr); returns pp and rho.
+GM1_DEV void kes_pre_cell(int64_t p, const float* __restrict__ pp_family, int64_t pf, float* __restrict__ pp3d,
+                          float* __restrict__ th3d, float* __restrict__ qv, float* __restrict__ qc,
+                          float* __restrict__ qpten, float* __restrict__ qtten, float* __restrict__ qvten,
+                          float* __restrict__ qcten, const float* __restrict__ pi0, const float* __restrict__ th0,
+                          float* __restrict__ prs, float* __restrict__ rho, float* __restrict__ rr, float dt,
+                          int eqtset, int copy_rr, float& pp_out, float& rho_out) {
+  float pp = pp_family ? pp_family[pf] : pp3d[p];
   float th = th3d[p], v = qv[p], cc = qc[p];
   if (eqtset == 2) {
     pp = pp - qpten[p] * dt;
@@ -625,6 +625,22 @@ __global__ __launch_bounds__(256) void k_kes_transition_pre(
   r = pr / (r * gas);
   rho[p] = r;
   if (copy_rr) rr[p] = r;
+  pp_out = pp;
+  rho_out = r;
+}
+
+__global__ __launch_bounds__(256) void k_kes_transition_pre(
+    const float* __restrict__ pp_family, float* __restrict__ pp3d, float* __restrict__ th3d, float* __restrict__ qv,
+    float* __restrict__ qc, float* __restrict__ qpten, float* __restrict__ qtten, float* __restrict__ qvten,
+    float* __restrict__ qcten, const float* __restrict__ pi0, const float* __restrict__ th0, float* __restrict__ prs,
+    float* __restrict__ rho, float* __restrict__ rr, int ni, int nj, int nk, float dt, int eqtset, int copy_rr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i >= ni) return;
+  const int64_t p = ((int64_t)k * nj + j) * ni + i;
+  float pp, r;
+  kes_pre_cell(p, pp_family, pix(k + 1, j + 1, i + 1, ni, nj), pp3d, th3d, qv, qc, qpten, qtten, qvten, qcten, pi0,
+               th0, prs, rho, rr, dt, eqtset, copy_rr, pp, r);
 }
 
 // dry-air mass ledger (cuda_apmasscon.py): per-wave partials of
@@ -641,6 +657,36 @@ __global__ __launch_bounds__(64) void k_mass_partials(const float* __restrict__ 
     const int64_t p = ((int64_t)k * nj + j) * ni + i;
     d = (double)(((rho[p] * ruh[i]) * rvh[j]) * rmh[p]);
     if (with_pressure) pr = (double)(pi0[p] + pp3d[p]);
+  }
+  d = wave_sum(d);
+  pr = wave_sum(pr);
+  const int64_t slot = ((int64_t)k * nj + j) * nseg + blockIdx.x;
+  if (threadIdx.x == 0) {
+    part[2 * slot] = d;
+    part[2 * slot + 1] = pr;
+  }
+}
+
+// The transition's cells with the dry-air mass partials of the rho and pp3d
+// they write (the words k_mass_partials would read back), one wave per
+// 64-cell row segment in k_mass_partials' layout.
+__global__ __launch_bounds__(64) void k_kes_transition_pre_mass(
+    const float* __restrict__ pp_family, float* __restrict__ pp3d, float* __restrict__ th3d, float* __restrict__ qv,
+    float* __restrict__ qc, float* __restrict__ qpten, float* __restrict__ qtten, float* __restrict__ qvten,
+    float* __restrict__ qcten, const float* __restrict__ pi0, const float* __restrict__ th0, float* __restrict__ prs,
+    float* __restrict__ rho, const float* __restrict__ ruh, const float* __restrict__ rvh,
+    const float* __restrict__ rmh, double* __restrict__ part, int ni, int nj, int nk, int nseg, float dt,
+    int eqtset) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  double d = 0.0, pr = 0.0;
+  if (i < ni) {
+    const int64_t p = ((int64_t)k * nj + j) * ni + i;
+    float pp, r;
+    kes_pre_cell(p, pp_family, pix(k + 1, j + 1, i + 1, ni, nj), pp3d, th3d, qv, qc, qpten, qtten, qvten, qcten,
+                 pi0, th0, prs, rho, nullptr, dt, eqtset, 0, pp, r);
+    d = (double)(((r * ruh[i]) * rvh[j]) * rmh[p]);
+    pr = (double)(pi0[p] + pp);
   }
   d = wave_sum(d);
   pr = wave_sum(pr);
@@ -885,6 +931,29 @@ int gm1hip_apmasscon(const float* ruh, const float* rvh, const float* rmh, const
     hipLaunchKernelGGL(k_mass_apply, dim3(gm1_blocks(n, 256)), dim3(256), 0, s, corr, pp3d, pi0, th0, th3d, qv, prs,
                        rho, rr, n);
   }
+  GM1_LAUNCH_CHECK();
+  return 0;
+}
+
+int gm1hip_kessler_transition_pre_apmasscon(const float* pp_family, float* pp3d, float* th3d, float* qv, float* qc,
+                                            float* qpten, float* qtten, float* qvten, float* qcten, const float* pi0,
+                                            const float* th0, float* prs, float* rho, float* rr, const float* ruh,
+                                            const float* rvh, const float* rmh, double* ledger, float* corr,
+                                            double mass1, void* work, int64_t work_bytes, int ni, int nj, int nk,
+                                            float dt, int eqtset, void* stream) {
+  if ((eqtset != 1 && eqtset != 2) || !(mass1 > 0.0) || work_bytes < gm1hip_dry_mass_scratch_bytes(ni, nj, nk))
+    return (int)hipErrorInvalidValue;
+  hipStream_t s = (hipStream_t)stream;
+  const int nseg = (ni + 63) / 64;
+  double* part = (double*)work;
+  double* levels = part + 2LL * nk * nj * nseg;
+  hipLaunchKernelGGL(k_kes_transition_pre_mass, dim3(nseg, nj, nk), dim3(64), 0, s, pp_family, pp3d, th3d, qv, qc,
+                     qpten, qtten, qvten, qcten, pi0, th0, prs, rho, ruh, rvh, rmh, part, ni, nj, nk, nseg, dt, eqtset);
+  hipLaunchKernelGGL(k_mass_levels, dim3(2 * nk), dim3(BS), 0, s, part, levels, nk, nj, nseg);
+  hipLaunchKernelGGL(k_mass_fold, dim3(1), dim3(64), 0, s, levels, ledger, corr, nk, mass1, ni * nj * nk);
+  const int64_t n = (int64_t)ni * nj * nk;
+  hipLaunchKernelGGL(k_mass_apply, dim3(gm1_blocks(n, 256)), dim3(256), 0, s, corr, pp3d, pi0, th0, th3d, qv, prs, rho,
+                     rr, n);
   GM1_LAUNCH_CHECK();
   return 0;
 }

@@ -28,8 +28,8 @@ __global__ __launch_bounds__(256) void k_stage_terms(
     const float* __restrict__ th0, const float* __restrict__ qv0, const float* __restrict__ pi0,
     const float* __restrict__ qc0, const float* __restrict__ qi0, const float* __restrict__ th3d,
     const float* __restrict__ q3d, const float* __restrict__ pp3d, float* __restrict__ thv,
-    float* __restrict__ ppterm, float* __restrict__ dum8, float* __restrict__ thterm, int ni, int nj, int nk,
-    int numq) {
+    float* __restrict__ ppterm, float* __restrict__ piadv, float* __restrict__ dum8, float* __restrict__ thterm,
+    int ni, int nj, int nk, int numq) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y, k = blockIdx.z;
   if (i >= ni) return;
@@ -58,20 +58,23 @@ __global__ __launch_bounds__(256) void k_stage_terms(
   const float cvm = 1.0f / ((cv + cvv * qv) + cpli);
   thterm[z] = (theta * ((RD + RV * qv) - rovcp * cpm)) * cvm;
   const float pi = bp + pp3d[p];
+  piadv[p] = pi;
   ppterm[p] = ((pi * rovcp) * cpm) * cvm;
 }
 
-__global__ __launch_bounds__(256) void k_piadv(const float* __restrict__ pi0, const float* __restrict__ pp3d,
-                                               float* __restrict__ piadv, int ni, int nj, int nk) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y;
-  if (i >= ni + 2) return;
-#pragma unroll
-  for (int u = 0; u < KB; ++u) {
-    const int k = min((int)blockIdx.z * KB + u, nk - 1) + 1;  // clamped: the top level again
-    const int64_t p = pix(k, j, i, ni, nj);
-    piadv[p] = pi0[k - 1] + pp3d[p];
-  }
+// piadv = pi0 + pp3d on the halo ring (i = 0, ni+1 or j = 0, nj+1) of levels
+// 1..nk; the stage terms write the interior.
+__global__ __launch_bounds__(256) void k_piadv_ring(const float* __restrict__ pi0, const float* __restrict__ pp3d,
+                                                    float* __restrict__ piadv, int ni, int nj) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y + 1;
+  if (q >= 2 * (ni + 2) + 2 * nj) return;
+  int i, j;
+  if (q < ni + 2) { j = 0; i = q; }
+  else if (q < 2 * (ni + 2)) { j = nj + 1; i = q - (ni + 2); }
+  else { const int r = q - 2 * (ni + 2); j = 1 + r / 2; i = (r & 1) ? ni + 1 : 0; }
+  const int64_t p = pix(k, j, i, ni, nj);
+  piadv[p] = pi0[k - 1] + pp3d[p];
 }
 
 // ----------------------------------------------------------- buoyancy
@@ -636,9 +639,9 @@ int gm1hip_stage_terms(const float* th0, const float* qv0, const float* pi0, con
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_stage_terms, dim3((ni + 63) / 64, nj, nk), dim3(64), 0, s, th0, qv0, pi0,
                      qc0, qi0, th3d, q3d,
-                     pp3d, thv, ppterm, dum8, thterm, ni, nj, nk, numq);
-  hipLaunchKernelGGL(k_piadv, dim3((ni + 2 + 63) / 64, nj + 2, (nk + KB - 1) / KB), dim3(64), 0, s, pi0, pp3d,
-                     piadv, ni, nj, nk);
+                     pp3d, thv, ppterm, piadv, dum8, thterm, ni, nj, nk, numq);
+  hipLaunchKernelGGL(k_piadv_ring, dim3((2 * (ni + 2) + 2 * nj + 255) / 256, nk), dim3(256), 0, s, pi0, pp3d,
+                     piadv, ni, nj);
   GM1_LAUNCH_CHECK();
   int rc = fill_pressure(thv, sides, ni, nj, nk, s);
   if (!rc) rc = fill_family(dum8, 0, sides, ni, nj, nk, s);

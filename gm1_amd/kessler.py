@@ -35,6 +35,7 @@ signature("gm1hip_kessler_driver_efall", *([P] * 28), I64, I, I, I, Fl, Fl, Fl, 
 KESSLER_NOALLOC_MAX_BYTES = 64 << 20
 signature("gm1hip_kessler_transition_pre", *([P] * 14), I, I, I, Fl, I, I, P)
 signature("gm1hip_apmasscon", *([P] * 13), D, P, I64, I, I, I, P)
+signature("gm1hip_kessler_transition_pre_apmasscon", *([P] * 19), D, P, I64, I, I, I, Fl, I, P)
 
 
 @dataclass(frozen=True)
@@ -260,14 +261,21 @@ def launch_cuda_gm1_kessler_transition(
     a = _check(arrays, ni, nj, nk, selection.efall)
     eqt = selection.equation_set
     hook = mass1 is not None
-    _native.call("gm1hip_kessler_transition_pre", ptr(pressure_family), ptr(a["pp3d"]), ptr(a["th3d"]),
-                 ptr(a["qv"]), ptr(a["qc"]), ptr(held["qpten"]), ptr(held["qtten"]), ptr(held["qvten"]),
-                 ptr(held["qcten"]), ptr(a["pi0"]), ptr(a["th0"]), ptr(a["prs"]), ptr(a["rho"]), ptr(a["rr"]), ni,
-                 nj, nk, float(dt), eqt, int(not hook), stream_handle(stream))
-    if hook:
-        if dry_mass is None:
-            raise ValueError("the dry-mass hook needs a DryMassWorkspace")
-        launch_cuda_apmasscon(a, ni=ni, nj=nj, nk=nk, mass1=float(mass1), workspace=dry_mass, stream=stream)
+    if hook and dry_mass is None:
+        raise ValueError("the dry-mass hook needs a DryMassWorkspace")
+    pre = (ptr(pressure_family), ptr(a["pp3d"]), ptr(a["th3d"]), ptr(a["qv"]), ptr(a["qc"]), ptr(held["qpten"]),
+           ptr(held["qtten"]), ptr(held["qvten"]), ptr(held["qcten"]), ptr(a["pi0"]), ptr(a["th0"]), ptr(a["prs"]),
+           ptr(a["rho"]), ptr(a["rr"]))
+    if hook and float(mass1) > 0.0:
+        # the transition's cells and the ledger's partials of the rho and pp3d they write in one pass
+        _native.call("gm1hip_kessler_transition_pre_apmasscon", *pre, ptr(a["ruh"]), ptr(a["rvh"]), ptr(a["rmh"]),
+                     ptr(dry_mass.ledger), ptr(dry_mass.correction), float(mass1), ptr(dry_mass.scratch),
+                     dry_mass.scratch.numel(), ni, nj, nk, float(dt), eqt, stream_handle(stream))
+    else:
+        _native.call("gm1hip_kessler_transition_pre", *pre, ni, nj, nk, float(dt), eqt, int(not hook),
+                     stream_handle(stream))
+        if hook:
+            launch_cuda_apmasscon(a, ni=ni, nj=nj, nk=nk, mass1=float(mass1), workspace=dry_mass, stream=stream)
     launch_cuda_gm1_kessler_driver(a, ni=ni, nj=nj, nk=nk, dx=dx, dy=dy, dz=dz, dt=dt, selection=selection,
                                    stream=stream, workspace=workspace, held=held if eqt == 2 else None)
 

@@ -339,6 +339,18 @@ int gm1hip_apmasscon(const float* ruh, const float* rvh, const float* rmh, const
                      double* ledger, float* corr, double mass1, void* scratch, int64_t scratch_bytes, int ni, int nj,
                      int nk, void* stream);
 
+// gm1hip_kessler_transition_pre (copy_rr = 0) followed by gm1hip_apmasscon
+// with mass1 > 0, the ledger's partials formed in the transition's own pass
+// from the rho and pp3d it writes: the transition's dry-mass hook
+// (cuda_gm1_kessler_large_step.py:183-292, apmasscon between the EOS and
+// the driver) with the same words as the two calls.
+int gm1hip_kessler_transition_pre_apmasscon(const float* pp_family, float* pp3d, float* th3d, float* qv, float* qc,
+                                            float* qpten, float* qtten, float* qvten, float* qcten, const float* pi0,
+                                            const float* th0, float* prs, float* rho, float* rr, const float* ruh,
+                                            const float* rvh, const float* rmh, double* ledger, float* corr,
+                                            double mass1, void* scratch, int64_t scratch_bytes, int ni, int nj,
+                                            int nk, float dt, int eqtset, void* stream);
+
 // ----------------------------------------------------------------- flat stretched column
 // The stretched Cartesian lane (cuda_cartesian_dynamics.py:1070-1262): CM1's
 // terrain_flag=.false. statements on a stretch_z column.  `flat_column` is a

@@ -10,7 +10,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from helpers import F, assert_bits_equal, golden
+from helpers import F, assert_bits_equal, golden, to_host
 from oracle import kessler as ok
 
 FIELDS = ("rho", "rr", "pp3d", "th3d", "prs", "qv", "qc", "qr", "fall_velocity", "rain", "prate")
@@ -245,6 +245,44 @@ def test_gpu_transition_matches_oracle(device, hook):
         assert_bits_equal(da[name].cpu().numpy(), a[name], name)
     for name in held:
         assert_bits_equal(dh[name].cpu().numpy(), held[name], name)
+
+
+@pytest.mark.gpu
+def test_gpu_fused_transition_and_dry_mass_equal_the_two_calls(device):
+    """gm1hip_kessler_transition_pre_apmasscon writes the words of gm1hip_kessler_transition_pre (copy_rr = 0)
+    then gm1hip_apmasscon: every field, the held rates, the binary64 ledger and the correction, bit for bit."""
+    import torch
+
+    from gm1_amd import _native
+    from gm1_amd._device import ptr
+    from gm1_amd.kessler import create_dry_mass_workspace, launch_cuda_apmasscon
+
+    g = golden("kessler")
+    ni, nj, nk = (int(v) for v in g["dims"])
+    a = _inputs(g, "t_in_")
+    held = {k: a.pop(k) for k in ("qpten", "qtten", "qvten", "qcten")}
+    mass1 = ok.dry_mass(a["rho"], a["ruh"], a["rvh"], a["rmh"]) * (1.0 + 3e-6)
+    runs = []
+    for fused in (False, True):
+        da, dh = _device_arrays(a, device), _device_arrays(held, device)
+        dm = create_dry_mass_workspace(ni, nj, nk, device)
+        pre = (None, ptr(da["pp3d"]), ptr(da["th3d"]), ptr(da["qv"]), ptr(da["qc"]), ptr(dh["qpten"]),
+               ptr(dh["qtten"]), ptr(dh["qvten"]), ptr(dh["qcten"]), ptr(da["pi0"]), ptr(da["th0"]), ptr(da["prs"]),
+               ptr(da["rho"]), ptr(da["rr"]))
+        if fused:
+            _native.call("gm1hip_kessler_transition_pre_apmasscon", *pre, ptr(da["ruh"]), ptr(da["rvh"]),
+                         ptr(da["rmh"]), ptr(dm.ledger), ptr(dm.correction), float(mass1), ptr(dm.scratch),
+                         dm.scratch.numel(), ni, nj, nk, 1.5, 2, 0)
+        else:
+            _native.call("gm1hip_kessler_transition_pre", *pre, ni, nj, nk, 1.5, 2, 0, 0)
+            launch_cuda_apmasscon(da, ni=ni, nj=nj, nk=nk, mass1=float(mass1), workspace=dm)
+        torch.cuda.synchronize()
+        runs.append(({**{n: to_host(t) for n, t in da.items()}, **{n: to_host(t) for n, t in dh.items()}},
+                     to_host(dm.ledger)[:2].copy(), to_host(dm.correction)[:1].copy()))
+    (f0, l0, c0), (f1, l1, c1) = runs
+    for n in f0:
+        assert_bits_equal(f1[n], f0[n], n)
+    assert l1.tobytes() == l0.tobytes() and c1.tobytes() == c0.tobytes()
 
 
 def _device_decisions(a, device, *, dz, dt, eqt, ni, nj, nk, dx=250.0):
