@@ -93,24 +93,29 @@ __global__ __launch_bounds__(256) void k_family_sn(float* __restrict__ a, int ni
 }
 
 // Pressure family (nk+2, nj+2, ni+2): west/east columns of every row.
-__global__ __launch_bounds__(256) void k_pressure_we(float* __restrict__ p, int ni, int nj, int nk, unsigned w,
-                                                     unsigned e) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+// bcp on the one-cell halo ring of levels 1..nk in one pass: bc.F fills the
+// west/east columns then the south/north rows (corners from the filled
+// columns), so ring cell (x, y) takes cell (xs(x), ys(y)), where xs maps
+// column 0 to ni (periodic west) or 1, column ni+1 to 1 (periodic east) or
+// ni, and leaves the others (a side with code 0 keeps its ring); ys likewise.
+// No source is a ring cell this pass writes.
+__global__ __launch_bounds__(256) void k_pressure_ring(float* __restrict__ p, int ni, int nj, unsigned w, unsigned e,
+                                                       unsigned so, unsigned n) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
   const int k = blockIdx.y + 1;
-  if (j >= nj + 2) return;
-  float* row = p + ((int64_t)k * (nj + 2) + j) * (ni + 2);
-  if (w) row[0] = row[w == 1u ? ni : 1];
-  if (e) row[ni + 1] = row[e == 1u ? 1 : ni];
-}
-
-__global__ __launch_bounds__(256) void k_pressure_sn(float* __restrict__ p, int ni, int nj, int nk, unsigned s,
-                                                     unsigned n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int k = blockIdx.y + 1;
-  if (i >= ni + 2) return;
+  if (q >= 2 * (ni + 2) + 2 * nj) return;
+  int x, y;
+  if (q < ni + 2) { y = 0; x = q; }
+  else if (q < 2 * (ni + 2)) { y = nj + 1; x = q - (ni + 2); }
+  else { const int r = q - 2 * (ni + 2); y = 1 + r / 2; x = (r & 1) ? ni + 1 : 0; }
+  int xs = x, ys = y;
+  if (x == 0 && w) xs = (w == 1u) ? ni : 1;
+  else if (x == ni + 1 && e) xs = (e == 1u) ? 1 : ni;
+  if (y == 0 && so) ys = (so == 1u) ? nj : 1;
+  else if (y == nj + 1 && n) ys = (n == 1u) ? 1 : nj;
+  if (xs == x && ys == y) return;
   float* plane = p + (int64_t)k * (nj + 2) * (ni + 2);
-  if (s) plane[i] = plane[(int64_t)(s == 1u ? nj : 1) * (ni + 2) + i];
-  if (n) plane[(int64_t)(nj + 1) * (ni + 2) + i] = plane[(int64_t)(n == 1u ? 1 : nj) * (ni + 2) + i];
+  plane[(int64_t)y * (ni + 2) + x] = plane[(int64_t)ys * (ni + 2) + xs];
 }
 
 __global__ __launch_bounds__(256) void k_w_rigid(float* __restrict__ w, int ni, int nj, int nk) {
@@ -175,10 +180,8 @@ int fill_family(float* a, int kind, unsigned sides, int ni, int nj, int nk, hipS
 }
 
 int fill_pressure(float* p, unsigned sides, int ni, int nj, int nk, hipStream_t s) {
-  hipLaunchKernelGGL(k_pressure_we, dim3((nj + 2 + 255) / 256, nk), dim3(256), 0, s, p, ni, nj, nk,
-                     side_of(sides, 0), side_of(sides, 1));
-  hipLaunchKernelGGL(k_pressure_sn, dim3((ni + 2 + 255) / 256, nk), dim3(256), 0, s, p, ni, nj, nk,
-                     side_of(sides, 2), side_of(sides, 3));
+  hipLaunchKernelGGL(k_pressure_ring, dim3((2 * (ni + 2) + 2 * nj + 255) / 256, nk), dim3(256), 0, s, p, ni, nj,
+                     side_of(sides, 0), side_of(sides, 1), side_of(sides, 2), side_of(sides, 3));
   GM1_LAUNCH_CHECK();
   return 0;
 }
