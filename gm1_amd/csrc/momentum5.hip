@@ -1130,7 +1130,6 @@ int gm1hip_momentum_transport5(
   a.uten = uten; a.vten = vten; a.wten = wten; a.ni = ni; a.nj = nj; a.nk = nk;
   a.rdx = 1.0f / dx; a.rdy = 1.0f / dy; a.rdz = 1.0f / dz; a.obc = obc;
   a.flat = nullptr; a.rr0 = nullptr;
-  dim3 blk(64);
   const bool pw = side_of(obc, 0) == 1u, pe = side_of(obc, 1) == 1u;
   const bool ps = side_of(obc, 2) == 1u, pn = side_of(obc, 3) == 1u;
   const int f0 = pw ? 0 : 1, nf = ni - 1 + (pw ? 1 : 0) + (pe ? 1 : 0);

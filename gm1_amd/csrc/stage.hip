@@ -83,12 +83,6 @@ struct BuoyArgs {
   float* uten; float* vten; float* wten; int ni, nj, nk;
 };
 
-GM1_DEV float d8(const BuoyArgs& a, int k, int j, int i) { return a.dum8[ix3(k, j + NG, i + NG, sy(a.nj), sx(a.ni))]; }
-GM1_DEV float bfw(const BuoyArgs& a, int kf, int j, int i) {
-  if (kf == 1 || kf == a.nk + 1) return 0.0f;
-  const int64_t p = ((int64_t)(kf - 1) * a.nj + j) * a.ni + i;
-  return a.c1[p] * d8(a, kf - 1, j, i) + a.c2[p] * d8(a, kf, j, i);
-}
 
 // wten(k) += b(k) at w levels 2..nk, and the terrain-horizontal parts on the
 // interior u and v faces.  Each thread marches one column through a chunk of
