@@ -504,25 +504,64 @@ GM1_DEV double block_sum(const double* __restrict__ v, int64_t n, int64_t stride
   return sh[0];
 }
 
-// rows: colbud rows (5 x nj, i order) and STAT_PARTS strided partial maxima of
-// the satadj status words, one thread each.
+// rows: colbud rows (5 x nj, i order), one wavefront each: the wave stages a
+// chunk of the row in LDS with coalesced loads and its first lane adds the
+// chunk in index order (the sequential sum); then STAT_PARTS strided partial
+// maxima of the satadj status words, one thread each, in the blocks after.
 constexpr int STAT_PARTS = 2048;
+constexpr int ROW_WAVES = 4, ROW_CHUNK = 512;
 
-__global__ __launch_bounds__(256) void k_kes_rows(const double* __restrict__ colbud, const int* __restrict__ wavestat,
-                                                  double* __restrict__ rowscr, int* __restrict__ statscr, int ni,
-                                                  int nj, int nk, int nseg) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64 * ROW_WAVES) void k_kes_rows(const double* __restrict__ colbud,
+                                                             const int* __restrict__ wavestat,
+                                                             double* __restrict__ rowscr, int* __restrict__ statscr,
+                                                             int ni, int nj, int nk, int nseg, int row_blocks) {
+  __shared__ double chunk[ROW_WAVES][ROW_CHUNK];
   const int nrow = 5 * nj;
-  if (r < nrow) {
-    const int b = r / nj, j = r % nj;
-    rowscr[r] = ordered_sum(colbud + (int64_t)b * nj * ni + (int64_t)j * ni, ni, 1);
-  } else if (r < nrow + STAT_PARTS) {
-    const int part = r - nrow;
+  if ((int)blockIdx.x < row_blocks) {
+    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+    const int r = blockIdx.x * ROW_WAVES + wv;
+    const double* row = colbud + (int64_t)(r < nrow ? r / nj : 0) * nj * ni + (int64_t)(r < nrow ? r % nj : 0) * ni;
+    double acc = 0.0;
+    for (int c0 = 0; c0 < ni; c0 += ROW_CHUNK) {  // the same trip count in every wave of the block
+      const int len = min(ROW_CHUNK, ni - c0);
+#pragma unroll
+      for (int u = 0; u < ROW_CHUNK / 64; ++u) {
+        const int q = u * 64 + lane;
+        if (r < nrow && q < len) chunk[wv][q] = row[c0 + q];
+      }
+      __syncthreads();
+      if (r < nrow && lane == 0) {
+        int q = 0;
+        for (; q + 8 <= len; q += 8) {  // eight words read ahead, added in index order
+          double b[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) b[u] = chunk[wv][q + u];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) acc += b[u];
+        }
+        for (; q < len; ++q) acc += chunk[wv][q];
+      }
+      __syncthreads();
+    }
+    if (r < nrow && lane == 0) rowscr[r] = acc;
+    return;
+  }
+  const int part = (blockIdx.x - row_blocks) * blockDim.x + threadIdx.x;
+  if (part < STAT_PARTS) {
     const int64_t n = (int64_t)nk * nj * nseg;
-    int mi = 0, mf = 0;
-    for (int64_t q = part; q < n; q += STAT_PARTS) {
-      mi = max(mi, wavestat[2 * q]);
-      mf += wavestat[2 * q + 1];
+    int mi = 0, mf = 0;  // (an integer maximum and count: any order gives the same words)
+    for (int64_t q0 = part; q0 < n; q0 += 8 * (int64_t)STAT_PARTS) {
+      int2 w[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int64_t q = q0 + (int64_t)u * STAT_PARTS;
+        w[u] = (q < n) ? reinterpret_cast<const int2*>(wavestat)[q] : make_int2(0, 0);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        mi = max(mi, w[u].x);
+        mf += w[u].y;
+      }
     }
     statscr[2 * part] = mi;
     statscr[2 * part + 1] = mf;
@@ -541,13 +580,17 @@ __global__ __launch_bounds__(BS) void k_kes_levels(const double* __restrict__ wa
 
 // The folds into asq/qbudget in the reference's order (j rows, k levels) and
 // the satadj status maxima.
-__global__ __launch_bounds__(256) void k_kes_budgets(const double* __restrict__ rowscr,
+__global__ __launch_bounds__(256) void k_kes_budgets(const double* __restrict__ rowscr_g,
                                                      const int* __restrict__ statscr, double* __restrict__ asq,
                                                      double* __restrict__ qbudget, int* __restrict__ status, int nj,
-                                                     int nk, float dx, float dy, float dz) {
+                                                     int nk, float dx, float dy, float dz, int staged) {
   __shared__ int smax[256];
   __shared__ int sfail[256];
+  extern __shared__ double lrows[];  // the row and level sums, when staged
   const int t = threadIdx.x;
+  if (staged)
+    for (int q = t; q < 5 * nj + 5 * nk; q += blockDim.x) lrows[q] = rowscr_g[q];
+  const double* __restrict__ rowscr = staged ? lrows : rowscr_g;
   {
     int mi = 0, mf = 0;
     for (int q = t; q < STAT_PARTS; q += blockDim.x) {
@@ -560,25 +603,31 @@ __global__ __launch_bounds__(256) void k_kes_budgets(const double* __restrict__ 
   __syncthreads();
   const double volume = (double)((dx * dy) * dz);
   const double area = (double)(dx * dy);
+  // each fold on a wavefront of its own (lanes of one wave running the same
+  // fold's code in parallel), its words read eight ahead and added in order
+  auto fold = [&](double acc, const double* v, int n, double f1, double f2, bool two) {
+    int q = 0;
+    for (; q + 8 <= n; q += 8) {
+      double b[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) b[u] = v[q + u];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc = two ? acc + (b[u] * f1) * f2 : acc + b[u] * f1;
+    }
+    for (; q < n; ++q) acc = two ? acc + (v[q] * f1) * f2 : acc + v[q] * f1;
+    return acc;
+  };
   if (t < 3) {  // pdefq asq[t]
-    double acc = asq[t];
-    for (int j = 0; j < nj; ++j) acc += rowscr[t * nj + j] * volume;
-    asq[t] = acc;
-  } else if (t == 3) {  // fallout -> qbudget[5]
-    double acc = qbudget[5];
-    for (int j = 0; j < nj; ++j) acc += rowscr[3 * nj + j] * area;
-    qbudget[5] = acc;
-  } else if (t == 4) {  // surface energy -> qbudget[6]
-    double acc = qbudget[6];
-    for (int j = 0; j < nj; ++j) acc += (rowscr[4 * nj + j] * (double)dx) * (double)dy;
-    qbudget[6] = acc;
-  } else if (t >= 8 && t < 13) {  // level budgets -> qbudget[2,3,4,0,1]
-    const int b = t - 8;
+    asq[t] = fold(asq[t], rowscr + t * nj, nj, volume, 1.0, false);
+  } else if (t == 64) {  // fallout -> qbudget[5]
+    qbudget[5] = fold(qbudget[5], rowscr + 3 * nj, nj, area, 1.0, false);
+  } else if (t == 128) {  // surface energy -> qbudget[6]
+    qbudget[6] = fold(qbudget[6], rowscr + 4 * nj, nj, (double)dx, (double)dy, true);
+  } else if (t >= 192 && t < 197) {  // level budgets -> qbudget[2,3,4,0,1]
+    const int b = t - 192;
     const int slot = (b < 3) ? 2 + b : b - 3;
-    double acc = qbudget[slot];
-    for (int k = 0; k < nk; ++k) acc = acc + rowscr[5 * nj + b * nk + k] * volume;
-    qbudget[slot] = acc;
-  } else if (t == 16) {
+    qbudget[slot] = fold(qbudget[slot], rowscr + 5 * nj + b * nk, nk, volume, 1.0, false);
+  } else if (t == 200) {
     int mi = 0, mf = 0;
     for (int q = 0; q < (int)blockDim.x; ++q) {
       mi = max(mi, smax[q]);
@@ -828,12 +877,16 @@ int kessler_driver(const float* ruh, const float* rvh, const float* rmh, const f
   hipLaunchKernelGGL(k_kes_column, dim3((ni + 63) / 64, nj), dim3(64), 0, s, a);
   hipLaunchKernelGGL(k_kes_cell, dim3(a.nseg, nj, nk), dim3(64), 0, s, a);
   int* statscr = a.wavestat + 2LL * nk * nj * a.nseg;
-  const int nsum = 5 * nj + STAT_PARTS;
-  hipLaunchKernelGGL(k_kes_rows, dim3((nsum + 255) / 256), dim3(256), 0, s, a.colbud, a.wavestat, rows, statscr, ni,
-                     nj, nk, a.nseg);
+  const int row_blocks = (5 * nj + ROW_WAVES - 1) / ROW_WAVES;
+  const int stat_blocks = (STAT_PARTS + 64 * ROW_WAVES - 1) / (64 * ROW_WAVES);
+  hipLaunchKernelGGL(k_kes_rows, dim3(row_blocks + stat_blocks), dim3(64 * ROW_WAVES), 0, s, a.colbud, a.wavestat,
+                     rows, statscr, ni, nj, nk, a.nseg, row_blocks);
   hipLaunchKernelGGL(k_kes_levels, dim3(5 * nk), dim3(BS), 0, s, a.wavebud, rows, nj, nk, a.nseg);
-  hipLaunchKernelGGL(k_kes_budgets, dim3(1), dim3(256), 0, s, rows, statscr, asq, qbudget, status, nj, nk, dx, dy,
-                     dz);
+  // the folds read the row and level sums from LDS when they fit in 48 KiB
+  const int64_t stage_bytes = (5LL * nj + 5LL * nk) * 8;
+  const int staged = stage_bytes <= 48 * 1024 ? 1 : 0;
+  hipLaunchKernelGGL(k_kes_budgets, dim3(1), dim3(256), staged ? (size_t)stage_bytes : 0, s, rows, statscr, asq,
+                     qbudget, status, nj, nk, dx, dy, dz, staged);
   GM1_LAUNCH_CHECK();
   return 0;
 }
